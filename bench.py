@@ -1,0 +1,155 @@
+#!/usr/bin/env python3
+"""Throughput of the per-sample radiance path on one MI355X.
+
+A step renders one full frame of the chosen workload (every pixel, every
+sample per pixel) with the plain megakernel, scene and output buffer already
+resident in HBM.  `value` = samples / s over exactly K timed steps.
+
+Beside it, on the same line:
+  roofline     — algorithmic HBM bytes of the render kernel per launch (BVH
+                 node records x 256 B + primitive records, from an untimed
+                 counting pass) over its average launch time, timed with HIP
+                 events on the library's stream, against 8 TB/s;
+  cpu_baseline — the CPU restatement of render_tiles (oracle/, the reference's
+                 row-sequential SmallRng streams) on this host's cores, on a
+                 bounded sample of the same workload.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+# bytes of one primitive record read by a test: triangle v0/e1/e2 48 B, quad 80 B, sphere 64 B (inline object)
+PRIM_BYTES = {"box": 80.0, "mesh": 48.0}
+
+
+def workloads():
+    from rustracer_amd import scenes
+    return {
+        "box": lambda: scenes.box_room(256, 256),
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--config", default="box", choices=sorted(workloads()))
+    ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (every box here has one MI355X)",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+    from rustracer_amd import Renderer
+
+    scene, cam, cfg = workloads()[args.config]()
+    W, H, spp, depth = cfg["width"], cfg["height"], cfg["spp"], cfg["max_depth"]
+    strata = int(spp ** 0.5)
+    r = Renderer(0)
+    t0 = time.time()
+    r.upload(scene)
+    upload_s = time.time() - t0
+    stats = r.stats()
+    acc = torch.zeros((H, W, 3), dtype=torch.float32, device="cuda")
+
+    def step(k):
+        r.render_accumulate(acc, cam, W, H, spp, sample_idx=0, strata=strata, max_depth=depth, seed=k)
+
+    for k in range(args.warmup):
+        step(k)
+    torch.cuda.synchronize()
+
+    # counting pass (untimed): traversal work per sample
+    r.set_counting(True)
+    step(0)
+    torch.cuda.synchronize()
+    cnt = r.counters()
+    r.set_counting(False)
+
+    r.set_timing(True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for k in range(args.steps):
+        step(1000 + k)
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    kern_ms, launches = r.timing()
+    r.set_timing(False)
+    img = acc.cpu().numpy() / spp
+    assert np.isfinite(img).all(), "non-finite radiance"
+
+    samples = W * H * spp
+    value = samples * args.steps / elapsed
+    per_sample_nodes = cnt["node_tests"] / max(cnt["samples"], 1)
+    per_sample_prims = cnt["prim_tests"] / max(cnt["samples"], 1)
+    bytes_per_sample = per_sample_nodes * 256.0 + per_sample_prims * PRIM_BYTES.get(args.config, 64.0)
+    samples_per_launch = samples * args.steps / max(launches, 1)
+    avg_launch_s = kern_ms / 1e3 / max(launches, 1)
+    achieved = bytes_per_sample * samples_per_launch / avg_launch_s / 1e9
+
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline(scene, cam, cfg, args.cpu_seconds)
+
+    line = {
+        "metric": "GPU samples/s (plain megakernel) at fixed RMSE",
+        "value": value,
+        "unit": "samples/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": elapsed * 1e3 / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": scene.name, "width": W, "height": H, "spp": spp, "max_depth": depth,
+                   "renderer": "plain megakernel", "bvh_nodes": stats["nodes"], "triangles": stats["triangles"],
+                   "upload_s": round(upload_s, 3)},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": "k_render", "avg_launch_ms": kern_ms / max(launches, 1),
+                     "bytes_per_sample": bytes_per_sample, "node_tests_per_sample": per_sample_nodes,
+                     "prim_tests_per_sample": per_sample_prims,
+                     "rays_per_sample": cnt["rays"] / max(cnt["samples"], 1)},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+
+
+def cpu_baseline(scene, cam, cfg, seconds):
+    """The oracle's render_tiles with the reference's row streams, all host
+    cores, passes of the same frame until `seconds` of wall time are used."""
+    from oracle import oracle as O
+    from rustracer_amd import Renderer
+    cores = O.threads()
+    orc = O.OracleScene(scene)
+    W, H, spp = cfg["width"], cfg["height"], cfg["spp"]
+    strata = int(spp ** 0.5)
+    done = 0
+    t0 = time.perf_counter()
+    while True:
+        p = Renderer.params(W, H, sample_idx=done, strata=strata, max_depth=cfg["max_depth"])
+        orc.render_sample(cam, p, mode=0, nthreads=cores)
+        done += 1
+        if time.perf_counter() - t0 >= seconds or done >= 64:
+            break
+    el = time.perf_counter() - t0
+    return {"value": W * H * done / el, "unit": "samples/s", "cores": cores, "kind": "port",
+            "sample": f"{done} full passes of {W}x{H} (1 sample/pixel each), {el:.1f}s"}
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,309 @@
+/*
+ * rt_api.h — C ABI of the MI355X-native per-sample radiance path.
+ *
+ * This library stands in for rustracer's `render_tiles` / `ray_color` path
+ * (src/renderer.rs:655-730 and 371-585) and for the BVH queries behind it
+ * (src/bvh.rs:229-575).  The reference has no FFI of its own: its callers are
+ * Rust functions (`bench_scene` src/main.rs:68-82, `run_render`
+ * src/main.rs:350-594) that hand `render_tiles` a `&dyn Hittable` world, a
+ * `HittableList` of lights, a `Background`, a `Camera` and an optional
+ * `PhotonMap`.  Here the same information crosses the boundary once, as a
+ * flattened scene descriptor (`rt_scene_desc`), and every later call refers to
+ * the uploaded scene by handle.
+ *
+ * Conventions (all entry points):
+ *   - return 0 (RT_OK) on success and a negative RT_ERR_* code on failure;
+ *     `rt_last_error(ctx)` returns a message for the last failure on `ctx`.
+ *     Nothing throws across the ABI.
+ *   - buffers are owned by the caller.  A buffer argument may be a host pointer
+ *     or a device pointer (hipMalloc'd on the context's device); the library
+ *     detects which and copies only when it has to.
+ *   - one host thread per context.  Calls are synchronous: when a call returns,
+ *     its outputs are complete.
+ *   - geometry is IEEE f32 and follows the reference's closed interval
+ *     [t_min, t_max] (src/hittable.rs:97-101).
+ */
+#ifndef RT_API_H
+#define RT_API_H
+
+#include <stdint.h>
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define RT_API_VERSION 1
+
+/* ---- status codes ------------------------------------------------------- */
+#define RT_OK               0
+#define RT_ERR_INVALID     -1   /* bad argument / malformed descriptor        */
+#define RT_ERR_UNSUPPORTED -2   /* a Hittable/Material kind the GPU path lacks */
+#define RT_ERR_DEVICE      -3   /* HIP runtime error                          */
+#define RT_ERR_NOMEM       -4   /* device allocation failed                   */
+
+typedef struct rt_ctx   rt_ctx;
+typedef struct rt_scene rt_scene;
+
+/* ---- shapes (Hittable implementors) -------------------------------------
+ * Inline parameter layout of rt_object.p[] per shape:
+ *   SPHERE   p0-2 center, p3 radius, p4-6 center1, p7 moving (0/1)   src/sphere.rs:17-35
+ *   QUAD     p0-2 corner q, p3-5 u, p6-8 v                            src/quad.rs:9-33
+ *   MESH     triangles [data_first, data_first+data_count) of the
+ *            descriptor's tri_* arrays, wrapped in a nested BVH       src/mesh.rs:13-92
+ *   BOX      p0-2 p0, p3-5 p1: six quads in a nested BVH             src/quad.rs:201-215
+ *   CONVEX   half-spaces (n.xyz, d) x data_count in pool[data_first*4..];
+ *            p0-2 center, p3 crown_h, p4 pavilion_h, p5 girdle_h,
+ *            p6 polish, p7-9 bbox min, p10-12 bbox max                src/diamond.rs:203-274
+ *   CYLINDER p0-2 center, p3 radius, p4 height                        src/cylinder.rs
+ *   DISK     p0-2 center, p3-5 normal, p6 radius                      src/disk.rs
+ */
+enum {
+    RT_SHAPE_SPHERE   = 1,
+    RT_SHAPE_QUAD     = 2,
+    RT_SHAPE_MESH     = 3,
+    RT_SHAPE_BOX      = 4,
+    RT_SHAPE_CONVEX   = 5,
+    RT_SHAPE_CYLINDER = 6,
+    RT_SHAPE_DISK     = 7
+};
+
+/* ---- transforms ----------------------------------------------------------
+ * An object's transform chain lists wrappers from the innermost (applied to
+ * the shape first) to the outermost, exactly as scene_file.rs builds them
+ * (scale -> rotate -> translate, src/scene_file.rs:1032-1038;
+ * Rotate::euler_about = Translate(-pivot), Rotate, Translate(pivot),
+ * src/transform.rs:218-224).
+ *   TRANSLATE v0-2 offset                       src/transform.rs:86-136
+ *   ROTATE    v0-8 forward matrix, row-major    src/transform.rs:140-277
+ *   SCALE     v0 factor (> 0)                   src/transform.rs:15-81
+ */
+enum { RT_XFORM_TRANSLATE = 1, RT_XFORM_ROTATE = 2, RT_XFORM_SCALE = 3 };
+
+typedef struct {
+    uint32_t kind;
+    float    v[9];
+} rt_xform;
+
+/* ---- participating media ---------------------------------------------------
+ * An object with medium != 0 is the boundary of a medium (its material is
+ * ignored).  medium = index + 1 into rt_scene_desc.media.
+ *   CONSTANT  ConstantMedium (src/volume.rs:78-156)
+ *   GRID      delta tracking through a dense float density grid spanning the
+ *             boundary's bounding box (extension: the reference evaluates
+ *             Perlin turbulence per lookup, src/volume.rs:160-263)
+ */
+enum { RT_MEDIUM_CONSTANT = 1, RT_MEDIUM_GRID = 2 };
+
+typedef struct {
+    uint32_t     kind;
+    float        density;      /* sigma_t (CONSTANT) or majorant scale (GRID) */
+    float        albedo[3];
+    float        g;            /* Henyey-Greenstein asymmetry                 */
+    uint32_t     grid_n[3];    /* GRID only                                   */
+    const float *grid;         /* GRID only: grid_n[0]*grid_n[1]*grid_n[2], x fastest, values in [0,1] */
+} rt_medium;
+
+typedef struct {
+    uint32_t shape;
+    uint32_t material;      /* index into rt_scene_desc.materials           */
+    uint32_t xform_first;   /* chain in rt_scene_desc.xforms                */
+    uint32_t xform_count;
+    uint32_t data_first;    /* MESH: first triangle; CONVEX: first plane    */
+    uint32_t data_count;    /* MESH: triangle count; CONVEX: plane count     */
+    uint32_t medium;        /* 0 = surface; k = media[k-1]                   */
+    uint32_t flags;         /* reserved, 0                                   */
+    float    p[16];
+} rt_object;
+
+/* ---- materials -----------------------------------------------------------
+ * Parameter layout of rt_material.p[] per kind:
+ *   LAMBERTIAN  p0-2 albedo; tex: 0 solid, 1 checker (p3 scale, p4-6 odd)
+ *                                                  src/material.rs:85-102, src/texture.rs:26-35
+ *   DIELECTRIC  p0 ior                             src/material.rs:323-332
+ *   SPECTRAL_DIELECTRIC p0 cauchy_b, p1 cauchy_c, p2-4 absorption,
+ *               p5 model (0 Cauchy, 1 Sellmeier), p6-8 Sellmeier B1..B3,
+ *               p9-11 Sellmeier C1..C3 (um^2)       src/material.rs:357-403
+ *   PBR         p0-2 albedo, p3 roughness, p4 metallic, p5 anisotropy,
+ *               p6 anisotropy_angle, p7 clearcoat, p8 clearcoat_roughness,
+ *               p9-11 emission, p12 emission_strength, p13 sheen,
+ *               p14 sheen_tint                      src/material.rs:695-1050
+ *   DIFFUSE_LIGHT p0-2 emit                         src/material.rs:14-25
+ *   SPECTRAL_METAL p0 variant (0 gold, 1 copper, 2 silver), p1 roughness
+ *                                                  src/material.rs:427-554
+ *   ROUGH_DIELECTRIC p0 ior, p1 roughness (GGX transmission; extension, no
+ *               counterpart in the reference)
+ */
+enum {
+    RT_MAT_LAMBERTIAN          = 1,
+    RT_MAT_DIELECTRIC          = 2,
+    RT_MAT_SPECTRAL_DIELECTRIC = 3,
+    RT_MAT_PBR                 = 4,
+    RT_MAT_DIFFUSE_LIGHT       = 5,
+    RT_MAT_SPECTRAL_METAL      = 6,
+    RT_MAT_ROUGH_DIELECTRIC    = 7
+};
+
+typedef struct {
+    uint32_t kind;
+    uint32_t tex;
+    float    p[24];
+} rt_material;
+
+/* ---- background (src/renderer.rs:183-223) -------------------------------- */
+enum {
+    RT_BG_SOLID     = 0,
+    RT_BG_GRADIENT  = 1,
+    RT_BG_PHYSICAL  = 2,   /* Preetham sky + sun disc */
+    RT_BG_ENVMAP    = 3,
+    RT_BG_ENVMAP_SPLIT = 4
+};
+
+typedef struct {
+    uint32_t     kind;
+    uint32_t     visible_kind;  /* ENVMAP_SPLIT: RT_BG_SOLID or RT_BG_GRADIENT */
+    float        color[3];      /* SOLID (or visible SOLID)                  */
+    float        top[3];        /* GRADIENT                                  */
+    float        bottom[3];
+    float        sun_dir[3];    /* PHYSICAL, unit                           */
+    float        turbidity;
+    uint32_t     env_w, env_h;  /* ENVMAP*: equirect float RGB, row-major, row 0 = +Y pole */
+    const float *env_rgb;
+    float        env_rot_y;     /* radians                                   */
+} rt_background;
+
+typedef struct {
+    const rt_object   *objects;    uint32_t num_objects;   /* world, input order */
+    const rt_object   *lights;     uint32_t num_lights;    /* NEE light list     */
+    const rt_material *materials;  uint32_t num_materials;
+    const rt_xform    *xforms;     uint32_t num_xforms;
+    const rt_medium   *media;      uint32_t num_media;
+    const float       *pool;       uint32_t pool_len;      /* CONVEX planes      */
+    /* triangle soup: 9 floats of vertices per triangle; normals (9), tangents (9)
+     * and uvs (6) per triangle; normals/tangents/uvs may be NULL (face normal,
+     * tangent = unit(v1-v0), uv (0,0),(1,0),(0.5,1) as Triangle::flat)         */
+    const float *tri_v; const float *tri_n; const float *tri_t; const float *tri_uv;
+    uint32_t     num_tris;
+    rt_background background;
+} rt_scene_desc;
+
+/* ---- rays and hits (parity surface) --------------------------------------- */
+typedef struct {
+    float origin[3];
+    float dir[3];
+    float t_min, t_max;
+    float time;
+} rt_ray;                          /* 36 bytes */
+
+typedef struct {
+    float    t;
+    float    p[3];
+    float    n[3];                 /* face-forwarded normal, as HitRecord.normal */
+    float    u, v;
+    int32_t  obj_id;               /* world object index in input order, -1 = miss */
+    int32_t  prim_id;              /* triangle / box-face index inside the object  */
+    uint32_t front_face;
+} rt_hit;                          /* 48 bytes */
+
+/* ---- camera (src/camera.rs:10-59) ----------------------------------------- */
+typedef struct {
+    float    origin[3];
+    float    lower_left[3];
+    float    horizontal[3];
+    float    vertical[3];
+    float    u[3];
+    float    v[3];
+    float    lens_radius;
+    uint32_t aperture_blades;
+} rt_camera;
+
+/* ---- per-pass render parameters (render_tiles arguments) ------------------ */
+typedef struct {
+    uint32_t sample_idx;     /* render_tiles sample_idx                       */
+    uint32_t strata;         /* floor(sqrt(spp_max)), src/main.rs:64          */
+    uint32_t width, height;
+    float    bg_scale;
+    uint32_t max_depth;      /* the reference hard-codes 50 (src/renderer.rs:16) */
+    uint32_t use_frame_time; /* frame_time: Some((t0, t1))                    */
+    float    frame_t0, frame_t1;
+    uint32_t use_photons;    /* photon_map: Some(..) when a map was built     */
+    uint64_t seed;           /* mixed into every per-pixel stream             */
+} rt_sample_params;
+
+/* ---- caustic photon map (src/photon.rs, src/scene.rs:35-74) --------------- */
+enum { RT_EMIT_QUAD = 0, RT_EMIT_SUN_DISK = 1 };
+
+typedef struct {
+    uint32_t emitter;          /* RT_EMIT_QUAD: build_from_quad; SUN_DISK: build */
+    float    origin[3], u[3], v[3];   /* QUAD                                      */
+    float    color[3];         /* QUAD: light colour; SUN_DISK: sun_color        */
+    float    sun_dir[3];       /* SUN_DISK                                       */
+    uint32_t num_photons;
+    float    gather_radius;
+    uint32_t knn;              /* 0 = fixed-radius gather (reference); k > 0 = k-NN extension */
+} rt_photon_params;
+
+/* ---- context ---------------------------------------------------------------- */
+int         rt_ctx_create(int device, rt_ctx **out);
+void        rt_ctx_destroy(rt_ctx *ctx);
+const char *rt_last_error(rt_ctx *ctx);
+int         rt_api_version(void);
+
+/* Upload a scene: builds the SAH BVH (src/bvh.rs:199-323) and the device
+ * layout.  Replaces the world BvhTree + lights HittableList + Background. */
+int  rt_scene_upload(rt_ctx *ctx, const rt_scene_desc *desc, rt_scene **out);
+void rt_scene_destroy(rt_scene *scene);
+
+/* Scene statistics for measurement.  stats[0] BVH nodes (world + nested),
+ * [1] triangles, [2] node bytes, [3] worst-case traversal stack depth,
+ * [4] world objects, [5] triangle bytes; [6], [7] reserved.                 */
+int  rt_scene_stats(rt_scene *scene, uint64_t stats[8]);
+
+/* Closest hit per ray: BvhTree::hit_index (src/bvh.rs:229-288). */
+int  rt_trace_closest(rt_ctx *ctx, rt_scene *scene, const rt_ray *rays, uint32_t n, rt_hit *hits);
+/* Occlusion per ray: BvhTree::any_hit (src/bvh.rs:487-530). */
+int  rt_trace_any(rt_ctx *ctx, rt_scene *scene, const rt_ray *rays, uint32_t n, uint8_t *occluded);
+/* Shadow-ray transmittance: BvhTree::transmittance (src/bvh.rs:532-575). */
+int  rt_transmittance(rt_ctx *ctx, rt_scene *scene, const rt_ray *rays, uint32_t n, float *tr);
+
+/* One sample pass over the image: render_tiles (src/renderer.rs:655-730).
+ * scratch_rgb: width*height*3 floats, row 0 = image top.  converged may be
+ * NULL; nonzero entries are written black.                                   */
+int  rt_render_sample(rt_ctx *ctx, rt_scene *scene, const rt_camera *cam,
+                      const rt_sample_params *params, const uint8_t *converged,
+                      float *scratch_rgb);
+
+/* spp consecutive passes (sample_idx .. sample_idx+spp-1) summed on the device
+ * into accum_rgb (width*height*3, overwritten).  Plain megakernel.           */
+int  rt_render_accumulate(rt_ctx *ctx, rt_scene *scene, const rt_camera *cam,
+                          const rt_sample_params *params, uint32_t spp,
+                          float *accum_rgb);
+
+/* Trace photons and build the kd-tree on the device: PhotonMap::build /
+ * build_from_quad (src/photon.rs:41-135, 165-191).  The map stays attached to
+ * the scene; rt_sample_params.use_photons enables the gather.                */
+int  rt_photon_build(rt_ctx *ctx, rt_scene *scene, const rt_photon_params *pp,
+                     uint32_t *num_stored);
+/* Irradiance queries against the attached map: PhotonMap::irradiance
+ * (src/photon.rs:148-157).  pos_nrm: n * 6 floats (p.xyz, n.xyz).            */
+int  rt_photon_irradiance(rt_ctx *ctx, rt_scene *scene, const float *pos_nrm,
+                          uint32_t n, float *irr_rgb);
+
+/* ---- measurement (no counterpart in the reference: its --bench mode times
+ * render_tiles with Instant, src/main.rs:68-117) ------------------------------ */
+/* Bracket every render call's kernels with HIP events on the context's stream;
+ * enabling resets the totals.                                                 */
+int  rt_ctx_set_timing(rt_ctx *ctx, int enable);
+/* Summed kernel time (ms) and kernel launch count since timing was enabled. */
+int  rt_ctx_get_timing(rt_ctx *ctx, double *kernel_ms, uint64_t *launches);
+/* Counting pass: renders add their traversal work to device counters.
+ * Enabling zeroes them.                                                       */
+int  rt_ctx_set_counting(rt_ctx *ctx, int enable);
+/* out[0] BVH node tests, [1] primitive tests, [2] rays traced (closest +
+ * shadow), [3] samples rendered, since counting was enabled.                 */
+int  rt_ctx_get_counters(rt_ctx *ctx, uint64_t out[4]);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* RT_API_H */

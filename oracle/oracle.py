@@ -1,0 +1,212 @@
+"""ORACLE loader — test infrastructure only.
+
+ctypes access to liboracle_rtcore.so, the C++ restatement of rustracer's
+radiance path (see rc_*.h).  Only tests/, __graft_entry__.smoke() and
+bench.py's cpu_baseline leg use it, as the checker.  It consumes the same
+rt_scene_desc the product consumes.
+"""
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB = os.path.join(HERE, "liboracle_rtcore.so")
+
+_lib = None
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", HERE], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB):
+            build()
+        L = C.CDLL(LIB)
+        vp = C.c_void_p
+        L.rtc_scene_create.restype = vp
+        L.rtc_scene_create.argtypes = [vp, C.c_char_p, C.c_int]
+        L.rtc_scene_destroy.argtypes = [vp]
+        for name in ("rtc_trace_closest", "rtc_trace_any", "rtc_transmittance"):
+            getattr(L, name).argtypes = [vp, vp, C.c_uint32, vp, C.c_int]
+        L.rtc_trace_object.argtypes = [vp, C.c_uint32, C.c_int, vp, C.c_uint32, vp]
+        L.rtc_render_sample.argtypes = [vp, vp, vp, vp, vp, C.c_int, C.c_int]
+        L.rtc_render_accumulate.argtypes = [vp, vp, vp, C.c_uint32, vp, C.c_int, C.c_int]
+        L.rtc_photon_build.argtypes = [vp, vp, C.c_int, C.POINTER(C.c_uint32)]
+        L.rtc_photon_dump.argtypes = [vp, vp, C.c_uint32]
+        L.rtc_photon_irradiance.argtypes = [vp, vp, C.c_uint32, vp, C.c_int]
+        L.rtc_world_nodes.argtypes = [vp, vp, vp, vp, C.c_uint32]
+        L.rtc_rng_u64.argtypes = [C.c_uint64, C.c_uint32, vp]
+        L.rtc_rng_f32.argtypes = [C.c_uint64, C.c_uint32, vp]
+        L.rtc_xoshiro_raw.argtypes = [vp, C.c_uint32, vp]
+        L.rtc_rng_range.argtypes = [C.c_uint64, C.c_float, C.c_float, C.c_uint32, vp]
+        L.rtc_eval.argtypes = [C.c_char_p, vp, vp]
+        L.rtc_e_lut.argtypes = [vp]
+        L.rtc_scatter.argtypes = [vp, vp, C.c_int, vp, C.c_int, C.c_uint64, C.c_uint32, vp]
+        L.rtc_env_sample.argtypes = [vp, C.c_uint32, C.c_uint32, C.c_float, C.c_uint64, C.c_uint32, vp]
+        L.rtc_env_eval.argtypes = [vp, C.c_uint32, C.c_uint32, C.c_float, vp, C.c_uint32, vp]
+        _lib = L
+    return _lib
+
+
+def _p(a):
+    return C.c_void_p(a.ctypes.data) if a is not None else None
+
+
+def threads():
+    return max(1, min(16, os.cpu_count() or 1))
+
+
+class OracleScene:
+    """The reference object graph (BvhTree world, light list, background)."""
+
+    def __init__(self, scene):
+        self.L = lib()
+        self._scene = scene
+        d = scene.descriptor()
+        err = C.create_string_buffer(256)
+        self.h = self.L.rtc_scene_create(C.byref(d), err, 256)
+        if not self.h:
+            raise RuntimeError("oracle scene: " + err.value.decode())
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            self.L.rtc_scene_destroy(self.h)
+            self.h = None
+
+    def trace_closest(self, rays, nthreads=None):
+        from rustracer_amd import _ffi as F
+        rays = np.ascontiguousarray(rays)
+        hits = np.zeros(len(rays), dtype=F.hit_dtype())
+        self.L.rtc_trace_closest(self.h, _p(rays), len(rays), _p(hits), nthreads or threads())
+        return hits
+
+    def trace_object(self, obj, rays, sub=-1):
+        """Hittable::hit of world object `obj` alone (no world BVH); sub >= 0
+        hits that nested primitive (mesh triangle) alone."""
+        from rustracer_amd import _ffi as F
+        rays = np.ascontiguousarray(rays)
+        hits = np.zeros(len(rays), dtype=F.hit_dtype())
+        rc = self.L.rtc_trace_object(self.h, obj, sub, _p(rays), len(rays), _p(hits))
+        if rc != 0:
+            raise ValueError("no such object")
+        return hits
+
+    def trace_any(self, rays, nthreads=None):
+        rays = np.ascontiguousarray(rays)
+        out = np.zeros(len(rays), dtype=np.uint8)
+        self.L.rtc_trace_any(self.h, _p(rays), len(rays), _p(out), nthreads or threads())
+        return out
+
+    def transmittance(self, rays, nthreads=None):
+        rays = np.ascontiguousarray(rays)
+        out = np.zeros(len(rays), dtype=np.float32)
+        self.L.rtc_transmittance(self.h, _p(rays), len(rays), _p(out), nthreads or threads())
+        return out
+
+    def render_sample(self, camera, params, converged=None, mode=1, nthreads=None):
+        """mode 0: reference row streams; mode 1: per-pixel streams (GPU keys)."""
+        out = np.zeros((params.height, params.width, 3), dtype=np.float32)
+        cam = camera.to_c()
+        conv = None if converged is None else np.ascontiguousarray(converged, dtype=np.uint8)
+        self.L.rtc_render_sample(self.h, C.byref(cam), C.byref(params), _p(conv), _p(out), mode, nthreads or threads())
+        return out
+
+    def render_accumulate(self, camera, params, spp, mode=1, nthreads=None):
+        out = np.zeros((params.height, params.width, 3), dtype=np.float32)
+        cam = camera.to_c()
+        self.L.rtc_render_accumulate(self.h, C.byref(cam), C.byref(params), spp, _p(out), mode, nthreads or threads())
+        return out
+
+    def photon_build(self, pp, nthreads=None):
+        n = C.c_uint32()
+        self.L.rtc_photon_build(self.h, C.byref(pp), nthreads or threads(), C.byref(n))
+        return n.value
+
+    def photon_dump(self, cap):
+        out = np.zeros((cap, 9), dtype=np.float32)
+        n = self.L.rtc_photon_dump(self.h, _p(out), cap)
+        return out[:n]
+
+    def photon_irradiance(self, pos_nrm, nthreads=None):
+        pos_nrm = np.ascontiguousarray(pos_nrm, dtype=np.float32).reshape(-1, 6)
+        out = np.zeros((len(pos_nrm), 3), dtype=np.float32)
+        self.L.rtc_photon_irradiance(self.h, _p(pos_nrm), len(pos_nrm), _p(out), nthreads or threads())
+        return out
+
+    def world_nodes(self):
+        n = self.L.rtc_world_nodes(self.h, None, None, None, 0)
+        b = np.zeros((n, 48), np.float32)
+        c = np.zeros((n, 8), np.int32)
+        tags = np.zeros(len(self._scene.objects), np.int32)
+        self.L.rtc_world_nodes(self.h, _p(b), _p(c), _p(tags), n)
+        return b, c, tags
+
+
+def eval_fn(name, args, nout=3):
+    a = np.zeros(16, np.float32)
+    a[:len(args)] = args
+    o = np.zeros(max(nout, 3), np.float32)
+    rc = lib().rtc_eval(name.encode(), _p(a), _p(o))
+    if rc != 0:
+        raise KeyError(name)
+    return o[:nout] if nout > 1 else float(o[0])
+
+
+def rng_u64(seed, n):
+    out = np.zeros(n, np.uint64)
+    lib().rtc_rng_u64(seed, n, _p(out))
+    return out
+
+
+def xoshiro_raw(state, n):
+    st = np.ascontiguousarray(state, np.uint64)
+    out = np.zeros(n, np.uint64)
+    lib().rtc_xoshiro_raw(_p(st), n, _p(out))
+    return out
+
+
+def rng_f32(seed, n):
+    out = np.zeros(n, np.float32)
+    lib().rtc_rng_f32(seed, n, _p(out))
+    return out
+
+
+def rng_range(seed, lo, hi, n):
+    out = np.zeros(n, np.float32)
+    lib().rtc_rng_range(seed, lo, hi, n, _p(out))
+    return out
+
+
+def e_lut():
+    out = np.zeros(64 * 64, np.float32)
+    lib().rtc_e_lut(_p(out))
+    return out.reshape(64, 64)
+
+
+def scatter(material, ray_o, ray_d, wavelength, rec_p, rec_n, t=1.0, front_face=True, spectral_weighted=False, seed=0, n=1):
+    m = material.to_c()
+    ray = np.array(list(ray_o) + list(ray_d) + [wavelength], np.float32)
+    rec = np.array(list(rec_p) + list(rec_n) + [t], np.float32)
+    out = np.zeros((n, 8), np.float32)
+    lib().rtc_scatter(C.byref(m), _p(ray), int(spectral_weighted), _p(rec), int(front_face), seed, n, _p(out))
+    return out
+
+
+def env_sample(img, rot, seed, n):
+    img = np.ascontiguousarray(img, np.float32)
+    out = np.zeros((n, 4), np.float32)
+    lib().rtc_env_sample(_p(img), img.shape[1], img.shape[0], rot, seed, n, _p(out))
+    return out
+
+
+def env_eval(img, rot, dirs):
+    img = np.ascontiguousarray(img, np.float32)
+    dirs = np.ascontiguousarray(dirs, np.float32).reshape(-1, 3)
+    out = np.zeros((len(dirs), 4), np.float32)
+    lib().rtc_env_eval(_p(img), img.shape[1], img.shape[0], rot, _p(dirs), len(dirs), _p(out))
+    return out

@@ -1,0 +1,468 @@
+// ORACLE — test infrastructure only.  The product library never links this
+// file; tests/, __graft_entry__.smoke() and bench.py's cpu_baseline load the
+// built liboracle_rtcore.so through ctypes, as the checker.
+//
+// Builds rustracer's object graph (dyn Hittable / dyn Material, SAH BvhTree)
+// from the same flattened rt_scene_desc the product consumes, the way
+// scene_file::build does (src/scene_file.rs:899-1132): shape, then the medium
+// wrapper, then the SRT chain; emissive world objects are NOT shared into the
+// light list here — the descriptor lists lights explicitly.
+#include "rc_render.h"
+#include "../include/rt_api.h"
+#include <string>
+#include <cstdio>
+
+using namespace rc;
+
+namespace {
+
+struct OScene {
+    std::vector<std::unique_ptr<Material>> mats;
+    std::vector<HPtr> world_objs;
+    BvhTree world;
+    HittableList lights;
+    Background bg;
+    PhotonMap photons;
+    bool have_photons = false;
+    std::string err;
+};
+
+Material* make_material(const rt_material& m) {
+    switch (m.kind) {
+        case RT_MAT_LAMBERTIAN: {
+            auto* l = new Lambertian();
+            l->tex.kind = (int)m.tex;
+            l->tex.even = Color(m.p[0], m.p[1], m.p[2]);
+            l->tex.scale = m.p[3];
+            l->tex.odd = Color(m.p[4], m.p[5], m.p[6]);
+            return l;
+        }
+        case RT_MAT_DIELECTRIC: { auto* d = new Dielectric(); d->ir = m.p[0]; return d; }
+        case RT_MAT_SPECTRAL_DIELECTRIC: {
+            auto* d = new SpectralDielectric();
+            d->cauchy_b = m.p[0]; d->cauchy_c = m.p[1]; d->absorption = Color(m.p[2], m.p[3], m.p[4]);
+            d->model = (int)m.p[5];
+            for (int k = 0; k < 3; k++) { d->sB[k] = m.p[6 + k]; d->sC[k] = m.p[9 + k]; }
+            return d;
+        }
+        case RT_MAT_PBR: {
+            auto* p = new PbrMaterial();
+            p->albedo = Color(m.p[0], m.p[1], m.p[2]); p->roughness = m.p[3]; p->metallic = m.p[4];
+            p->anisotropy = m.p[5]; p->anisotropy_angle = m.p[6]; p->clearcoat = m.p[7]; p->clearcoat_roughness = m.p[8];
+            p->emission = Color(m.p[9], m.p[10], m.p[11]); p->emission_strength = m.p[12];
+            p->sheen = m.p[13]; p->sheen_tint = m.p[14];
+            return p;
+        }
+        case RT_MAT_DIFFUSE_LIGHT: { auto* d = new DiffuseLight(); d->emit = Color(m.p[0], m.p[1], m.p[2]); return d; }
+        case RT_MAT_SPECTRAL_METAL: { auto* s = new SpectralMetal(); s->variant = (int)m.p[0]; s->roughness = m.p[1]; s->init(); return s; }
+        case RT_MAT_ROUGH_DIELECTRIC: { auto* r = new RoughDielectric(); r->ior = m.p[0]; r->roughness = m.p[1]; return r; }
+        default: return nullptr;
+    }
+}
+
+HPtr make_box(Point3 p0, Point3 p1, const Material* mat) {
+    Point3 mn(rmin(p0.x, p1.x), rmin(p0.y, p1.y), rmin(p0.z, p1.z));
+    Point3 mx(rmax(p0.x, p1.x), rmax(p0.y, p1.y), rmax(p0.z, p1.z));
+    Vec3 dx(mx.x - mn.x, 0, 0), dy(0, mx.y - mn.y, 0), dz(0, 0, mx.z - mn.z);
+    std::vector<HPtr> sides = {
+        std::make_shared<Quad>(Point3(mn.x, mn.y, mx.z), dx, dy, mat),
+        std::make_shared<Quad>(Point3(mx.x, mn.y, mx.z), -dz, dy, mat),
+        std::make_shared<Quad>(Point3(mx.x, mn.y, mn.z), -dx, dy, mat),
+        std::make_shared<Quad>(Point3(mn.x, mn.y, mn.z), dz, dy, mat),
+        std::make_shared<Quad>(Point3(mn.x, mx.y, mx.z), dx, -dz, mat),
+        std::make_shared<Quad>(Point3(mn.x, mn.y, mn.z), dx, dz, mat)};
+    for (int i = 0; i < 6; i++) sides[i]->tag = i;
+    auto b = std::make_shared<BvhTree>();
+    b->from_list(sides);
+    return b;
+}
+
+HPtr make_object(OScene& S, const rt_scene_desc& d, const rt_object& o) {
+    if (o.material >= d.num_materials && o.medium == 0) { S.err = "material index out of range"; return nullptr; }
+    const Material* mat = o.medium ? nullptr : S.mats[o.material].get();
+    HPtr h;
+    const float* p = o.p;
+    switch (o.shape) {
+        case RT_SHAPE_SPHERE: {
+            auto s = std::make_shared<Sphere>(Point3(p[0], p[1], p[2]), p[3], mat);
+            if (p[7] != 0.0f) { s->moving = true; s->center1 = Point3(p[4], p[5], p[6]); }
+            h = s; break;
+        }
+        case RT_SHAPE_QUAD: h = std::make_shared<Quad>(Point3(p[0], p[1], p[2]), Vec3(p[3], p[4], p[5]), Vec3(p[6], p[7], p[8]), mat); break;
+        case RT_SHAPE_BOX: h = make_box(Point3(p[0], p[1], p[2]), Point3(p[3], p[4], p[5]), mat); break;
+        case RT_SHAPE_MESH: {
+            if ((uint64_t)o.data_first + o.data_count > d.num_tris || o.data_count == 0) { S.err = "mesh triangle range"; return nullptr; }
+            std::vector<HPtr> tris;
+            tris.reserve(o.data_count);
+            for (uint32_t k = 0; k < o.data_count; k++) {
+                size_t t = (size_t)o.data_first + k;
+                auto tr = std::make_shared<Triangle>(d.tri_v + t * 9, d.tri_n ? d.tri_n + t * 9 : nullptr,
+                                                     d.tri_t ? d.tri_t + t * 9 : nullptr, d.tri_uv ? d.tri_uv + t * 6 : nullptr, mat);
+                tr->tag = (int)k;
+                tris.push_back(tr);
+            }
+            auto b = std::make_shared<BvhTree>();
+            b->from_list(tris);
+            h = b; break;
+        }
+        case RT_SHAPE_CONVEX: {
+            if ((uint64_t)(o.data_first + o.data_count) * 4 > d.pool_len) { S.err = "convex plane range"; return nullptr; }
+            auto c = std::make_shared<Convex>();
+            for (uint32_t k = 0; k < o.data_count; k++) {
+                const float* q = d.pool + ((size_t)o.data_first + k) * 4;
+                c->planes.push_back(Plane4{Vec3(q[0], q[1], q[2]), q[3]});
+            }
+            c->mat = mat; c->center = Point3(p[0], p[1], p[2]);
+            c->crown_h = p[3]; c->pavilion_h = p[4]; c->girdle_h = p[5]; c->polish = p[6];
+            c->bbox = Aabb(Point3(p[7], p[8], p[9]), Point3(p[10], p[11], p[12]));
+            h = c; break;
+        }
+        default: S.err = "unsupported shape kind " + std::to_string(o.shape); return nullptr;
+    }
+    if (o.medium) {
+        if (o.medium > d.num_media) { S.err = "medium index"; return nullptr; }
+        const rt_medium& m = d.media[o.medium - 1];
+        if (m.kind == RT_MEDIUM_CONSTANT) {
+            auto cm = std::make_shared<ConstantMedium>();
+            cm->boundary = h; cm->density = m.density; cm->neg_inv_density = -1.0f / m.density;
+            cm->phase.albedo = Color(m.albedo[0], m.albedo[1], m.albedo[2]); cm->phase.g = m.g;
+            h = cm;
+        } else if (m.kind == RT_MEDIUM_GRID) {
+            auto gm = std::make_shared<GridMedium>();
+            gm->boundary = h; gm->density_scale = rmax(m.density, FLT_EPSILON);
+            gm->phase.albedo = Color(m.albedo[0], m.albedo[1], m.albedo[2]); gm->phase.g = m.g;
+            gm->nx = m.grid_n[0]; gm->ny = m.grid_n[1]; gm->nz = m.grid_n[2];
+            gm->grid.assign(m.grid, m.grid + (size_t)gm->nx * gm->ny * gm->nz);
+            h->bounding_box(gm->box);
+            h = gm;
+        } else { S.err = "medium kind"; return nullptr; }
+    }
+    for (uint32_t k = 0; k < o.xform_count; k++) {
+        if (o.xform_first + k >= d.num_xforms) { S.err = "xform range"; return nullptr; }
+        const rt_xform& x = d.xforms[o.xform_first + k];
+        if (x.kind == RT_XFORM_TRANSLATE) {
+            auto t = std::make_shared<Translate>(); t->object = h; t->offset = Vec3(x.v[0], x.v[1], x.v[2]); h = t;
+        } else if (x.kind == RT_XFORM_ROTATE) {
+            auto r = std::make_shared<Rotate>(); r->object = h;
+            for (int i = 0; i < 9; i++) r->m[i / 3][i % 3] = x.v[i];
+            r->init_bbox(); h = r;
+        } else if (x.kind == RT_XFORM_SCALE) {
+            auto s = std::make_shared<UniformScale>(); s->object = h; s->factor = x.v[0]; h = s;
+        } else { S.err = "xform kind"; return nullptr; }
+    }
+    return h;
+}
+
+bool build_scene(OScene& S, const rt_scene_desc& d) {
+    for (uint32_t i = 0; i < d.num_materials; i++) {
+        Material* m = make_material(d.materials[i]);
+        if (!m) { S.err = "unsupported material kind"; return false; }
+        S.mats.emplace_back(m);
+    }
+    if (d.num_objects == 0) { S.err = "empty world"; return false; }
+    for (uint32_t i = 0; i < d.num_objects; i++) {
+        HPtr h = make_object(S, d, d.objects[i]);
+        if (!h) return false;
+        h->tag = (int)i;
+        S.world_objs.push_back(h);
+    }
+    S.world.top = true;
+    S.world.from_list(S.world_objs);
+    for (uint32_t i = 0; i < d.num_lights; i++) {
+        HPtr h = make_object(S, d, d.lights[i]);
+        if (!h) return false;
+        h->tag = (int)i;
+        S.lights.objects.push_back(h);
+    }
+    const rt_background& b = d.background;
+    S.bg.kind = (int)b.kind; S.bg.visible_kind = (int)b.visible_kind;
+    S.bg.color = Color(b.color[0], b.color[1], b.color[2]);
+    S.bg.top = Color(b.top[0], b.top[1], b.top[2]);
+    S.bg.bottom = Color(b.bottom[0], b.bottom[1], b.bottom[2]);
+    S.bg.sun_dir = Vec3(b.sun_dir[0], b.sun_dir[1], b.sun_dir[2]);
+    S.bg.turbidity = b.turbidity;
+    if (b.kind == RT_BG_ENVMAP || b.kind == RT_BG_ENVMAP_SPLIT) {
+        if (!b.env_rgb || !b.env_w || !b.env_h) { S.err = "env map missing"; return false; }
+        S.bg.env.init(b.env_rgb, b.env_w, b.env_h, b.env_rot_y);
+    }
+    return true;
+}
+
+Ray to_ray(const rt_ray& r) {
+    return Ray::at_time(Point3(r.origin[0], r.origin[1], r.origin[2]), Vec3(r.dir[0], r.dir[1], r.dir[2]), r.time);
+}
+
+Camera to_cam(const rt_camera& c) {
+    Camera k;
+    k.origin = Point3(c.origin[0], c.origin[1], c.origin[2]);
+    k.lower_left = Point3(c.lower_left[0], c.lower_left[1], c.lower_left[2]);
+    k.horizontal = Vec3(c.horizontal[0], c.horizontal[1], c.horizontal[2]);
+    k.vertical = Vec3(c.vertical[0], c.vertical[1], c.vertical[2]);
+    k.u = Vec3(c.u[0], c.u[1], c.u[2]); k.v = Vec3(c.v[0], c.v[1], c.v[2]);
+    k.lens_radius = c.lens_radius; k.blades = c.aperture_blades;
+    return k;
+}
+
+SampleParams to_params(const rt_sample_params& p) {
+    SampleParams s;
+    s.sample_idx = p.sample_idx; s.strata = p.strata; s.width = p.width; s.height = p.height;
+    s.max_depth = p.max_depth; s.bg_scale = p.bg_scale; s.use_frame_time = p.use_frame_time != 0;
+    s.t0 = p.frame_t0; s.t1 = p.frame_t1; s.seed = p.seed;
+    return s;
+}
+
+SceneView view(OScene& S, bool photons) {
+    SceneView v;
+    v.world = &S.world; v.lights = &S.lights; v.bg = &S.bg;
+    v.photons = (photons && S.have_photons) ? &S.photons : nullptr;
+    return v;
+}
+
+}  // namespace
+
+extern "C" {
+
+void* rtc_scene_create(const rt_scene_desc* d, char* err, int err_len) {
+    auto* S = new OScene();
+    if (!d || !build_scene(*S, *d)) {
+        if (err && err_len > 0) snprintf(err, (size_t)err_len, "%s", S->err.c_str());
+        delete S;
+        return nullptr;
+    }
+    return S;
+}
+
+void rtc_scene_destroy(void* s) { delete (OScene*)s; }
+
+int rtc_trace_closest(void* s, const rt_ray* rays, uint32_t n, rt_hit* hits, int threads) {
+    OScene& S = *(OScene*)s;
+    parallel_for(n, threads, [&](size_t i) {
+        Ray r = to_ray(rays[i]);
+        HitRecord rec;
+        rt_hit& h = hits[i];
+        memset(&h, 0, sizeof h);
+        if (S.world.hit(r, rays[i].t_min, rays[i].t_max, rec)) {
+            h.t = rec.t; h.p[0] = rec.p.x; h.p[1] = rec.p.y; h.p[2] = rec.p.z;
+            h.n[0] = rec.normal.x; h.n[1] = rec.normal.y; h.n[2] = rec.normal.z;
+            h.u = rec.u; h.v = rec.v; h.obj_id = rec.obj_id; h.prim_id = rec.prim_id; h.front_face = rec.front_face;
+        } else { h.obj_id = -1; h.prim_id = -1; }
+    });
+    return 0;
+}
+
+// Hittable::hit of one world object, bypassing the world BVH (primitive KATs).
+// sub >= 0 selects the nested primitive with that tag (a mesh triangle) and
+// calls its own hit, bypassing the nested BVH too.
+int rtc_trace_object(void* s, uint32_t obj, int sub, const rt_ray* rays, uint32_t n, rt_hit* hits) {
+    OScene& S = *(OScene*)s;
+    if (obj >= S.world_objs.size()) return -1;
+    const Hittable* target = S.world_objs[obj].get();
+    if (sub >= 0) {
+        auto* b = dynamic_cast<const BvhTree*>(target);
+        if (!b) return -1;
+        target = nullptr;
+        for (auto& o : b->objects) if (o->tag == sub) target = o.get();
+        if (!target) return -1;
+    }
+    for (uint32_t i = 0; i < n; i++) {
+        HitRecord rec;
+        rt_hit& h = hits[i];
+        memset(&h, 0, sizeof h);
+        if (target->hit(to_ray(rays[i]), rays[i].t_min, rays[i].t_max, rec)) {
+            h.t = rec.t; h.p[0] = rec.p.x; h.p[1] = rec.p.y; h.p[2] = rec.p.z;
+            h.n[0] = rec.normal.x; h.n[1] = rec.normal.y; h.n[2] = rec.normal.z;
+            h.u = rec.u; h.v = rec.v; h.obj_id = (int)obj; h.prim_id = rec.prim_id; h.front_face = rec.front_face;
+        } else { h.obj_id = -1; h.prim_id = -1; }
+    }
+    return 0;
+}
+
+int rtc_trace_any(void* s, const rt_ray* rays, uint32_t n, uint8_t* out, int threads) {
+    OScene& S = *(OScene*)s;
+    parallel_for(n, threads, [&](size_t i) { out[i] = S.world.any_hit(to_ray(rays[i]), rays[i].t_min, rays[i].t_max) ? 1 : 0; });
+    return 0;
+}
+
+int rtc_transmittance(void* s, const rt_ray* rays, uint32_t n, float* out, int threads) {
+    OScene& S = *(OScene*)s;
+    parallel_for(n, threads, [&](size_t i) { out[i] = S.world.transmittance(to_ray(rays[i]), rays[i].t_min, rays[i].t_max); });
+    return 0;
+}
+
+int rtc_render_sample(void* s, const rt_camera* cam, const rt_sample_params* p, const uint8_t* conv, float* out, int mode, int threads) {
+    OScene& S = *(OScene*)s;
+    render_tiles(out, conv, view(S, p->use_photons != 0), to_cam(*cam), to_params(*p), mode, threads);
+    return 0;
+}
+
+int rtc_render_accumulate(void* s, const rt_camera* cam, const rt_sample_params* p, uint32_t spp, float* accum, int mode, int threads) {
+    OScene& S = *(OScene*)s;
+    size_t n = (size_t)p->width * p->height * 3;
+    std::vector<float> tmp(n);
+    for (size_t i = 0; i < n; i++) accum[i] = 0.0f;
+    SampleParams sp = to_params(*p);
+    for (uint32_t k = 0; k < spp; k++) {
+        SampleParams q = sp; q.sample_idx = sp.sample_idx + k;
+        render_tiles(tmp.data(), nullptr, view(S, p->use_photons != 0), to_cam(*cam), q, mode, threads);
+        for (size_t i = 0; i < n; i++) accum[i] += tmp[i];
+    }
+    return 0;
+}
+
+int rtc_photon_build(void* s, const rt_photon_params* pp, int threads, uint32_t* stored) {
+    OScene& S = *(OScene*)s;
+    if (pp->emitter == RT_EMIT_QUAD) {
+        photon_build_from_quad(S.photons, S.world, Point3(pp->origin[0], pp->origin[1], pp->origin[2]),
+                               Vec3(pp->u[0], pp->u[1], pp->u[2]), Vec3(pp->v[0], pp->v[1], pp->v[2]),
+                               Color(pp->color[0], pp->color[1], pp->color[2]), pp->num_photons, pp->gather_radius, threads);
+    } else {
+        Aabb bb; bool have = S.world.bounding_box(bb);
+        photon_build_sun(S.photons, S.world, have, bb, Vec3(pp->sun_dir[0], pp->sun_dir[1], pp->sun_dir[2]),
+                         Color(pp->color[0], pp->color[1], pp->color[2]), pp->num_photons, pp->gather_radius, threads);
+    }
+    S.have_photons = true;
+    if (stored) *stored = (uint32_t)S.photons.nodes.size();
+    return 0;
+}
+
+// Photons in kd order: n x 9 floats (pos, power, normal).
+int rtc_photon_dump(void* s, float* out, uint32_t cap) {
+    OScene& S = *(OScene*)s;
+    uint32_t n = (uint32_t)std::min<size_t>(cap, S.photons.nodes.size());
+    for (uint32_t i = 0; i < n; i++) {
+        const KdPhoton& k = S.photons.nodes[i];
+        for (int j = 0; j < 3; j++) { out[i * 9 + j] = k.pos[j]; out[i * 9 + 3 + j] = k.power[j]; out[i * 9 + 6 + j] = k.nor[j]; }
+    }
+    return (int)n;
+}
+
+int rtc_photon_irradiance(void* s, const float* pn, uint32_t n, float* out, int threads) {
+    OScene& S = *(OScene*)s;
+    parallel_for(n, threads, [&](size_t i) {
+        Color c = S.photons.irradiance(Point3(pn[i * 6], pn[i * 6 + 1], pn[i * 6 + 2]), Vec3(pn[i * 6 + 3], pn[i * 6 + 4], pn[i * 6 + 5]));
+        out[i * 3] = c.x; out[i * 3 + 1] = c.y; out[i * 3 + 2] = c.z;
+    });
+    return 0;
+}
+
+// Flattened world QBVH for layout checks: nodes x (48 floats + 8 ints).
+int rtc_world_nodes(void* s, float* bounds, int32_t* children, int32_t* leaf_tags, uint32_t cap) {
+    OScene& S = *(OScene*)s;
+    uint32_t n = (uint32_t)S.world.nodes.size();
+    if (bounds && children) {
+        for (uint32_t i = 0; i < n && i < cap; i++) {
+            const QbvhNode& q = S.world.nodes[i];
+            for (int a = 0; a < 3; a++) for (int c = 0; c < QW; c++) {
+                bounds[i * 48 + a * 8 + c] = q.mn[a][c];
+                bounds[i * 48 + 24 + a * 8 + c] = q.mx[a][c];
+            }
+            for (int c = 0; c < QW; c++) children[i * 8 + c] = q.children[c];
+        }
+    }
+    if (leaf_tags) for (size_t i = 0; i < S.world.objects.size(); i++) leaf_tags[i] = S.world.objects[i]->tag;
+    return (int)n;
+}
+
+// ---- unit-level entry points for known-answer tests --------------------------------
+void rtc_rng_u64(uint64_t seed, uint32_t n, uint64_t* out) {
+    SmallRng r = SmallRng::seed_from_u64(seed);
+    for (uint32_t i = 0; i < n; i++) out[i] = r.next_u64();
+}
+// xoshiro256++ from a raw 4-word state (no SplitMix64 seeding).
+void rtc_xoshiro_raw(const uint64_t* state, uint32_t n, uint64_t* out) {
+    SmallRng r; for (int i = 0; i < 4; i++) r.s[i] = state[i];
+    for (uint32_t i = 0; i < n; i++) out[i] = r.next_u64();
+}
+void rtc_rng_f32(uint64_t seed, uint32_t n, float* out) {
+    SmallRng r = SmallRng::seed_from_u64(seed);
+    for (uint32_t i = 0; i < n; i++) out[i] = r.gen();
+}
+void rtc_rng_range(uint64_t seed, float lo, float hi, uint32_t n, float* out) {
+    SmallRng r = SmallRng::seed_from_u64(seed);
+    for (uint32_t i = 0; i < n; i++) out[i] = r.gen_range(lo, hi);
+}
+
+// Scalar functions by name: returns 0 on success.
+int rtc_eval(const char* fn, const float* a, float* out) {
+    std::string f(fn);
+    if (f == "ggx_ndf") { out[0] = ggx_ndf(a[0], a[1]); return 0; }
+    if (f == "smith_g1") { out[0] = smith_g1(a[0], a[1]); return 0; }
+    if (f == "smith_lambda") { out[0] = smith_lambda(a[0], a[1]); return 0; }
+    if (f == "smith_g2") { out[0] = smith_g2_hc(a[0], a[1]); return 0; }
+    if (f == "e_ss") { out[0] = e_ss_lut(a[0], a[1]); return 0; }
+    if (f == "energy_comp") { out[0] = specular_energy_comp(a[0], a[1]); return 0; }
+    if (f == "fresnel_dielectric") { out[0] = fresnel_dielectric(a[0], a[1]); return 0; }
+    if (f == "schlick") { out[0] = schlick(a[0], a[1]); return 0; }
+    if (f == "fresnel_conductor") { out[0] = fresnel_conductor(a[0], a[1], a[2]); return 0; }
+    if (f == "cauchy") { out[0] = cauchy_ior(a[0], a[1], a[2]); return 0; }
+    if (f == "sellmeier") { out[0] = sellmeier_ior(a[0], a + 1, a + 4); return 0; }
+    if (f == "metal_ior") { metal_ior((int)a[0], a[1], out[0], out[1]); return 0; }
+    if (f == "spectral_to_rgb") { Color c = spectral_to_rgb(a[0]); out[0] = c.x; out[1] = c.y; out[2] = c.z; return 0; }
+    if (f == "spectral_fire_rgb") { Color c = spectral_fire_rgb(a[0]); out[0] = c.x; out[1] = c.y; out[2] = c.z; return 0; }
+    if (f == "vndf") {
+        Vec3 m = vndf_sample_aniso(Vec3(a[0], a[1], a[2]), a[3], a[4], a[5], a[6]);
+        out[0] = m.x; out[1] = m.y; out[2] = m.z; return 0;
+    }
+    if (f == "preetham") {
+        Color c = preetham_sky(Vec3(a[0], a[1], a[2]), Vec3(a[3], a[4], a[5]), a[6]);
+        out[0] = c.x; out[1] = c.y; out[2] = c.z; return 0;
+    }
+    if (f == "hg_sample_seeded") {
+        SmallRng r = SmallRng::seed_from_u64((uint64_t)a[4]);
+        Vec3 d = hg_sample(Vec3(a[0], a[1], a[2]), a[3], r);
+        out[0] = d.x; out[1] = d.y; out[2] = d.z; return 0;
+    }
+    return -1;
+}
+
+void rtc_e_lut(float* out) {
+    const ELut& E = e_lut();
+    for (int i = 0; i < E_N; i++) for (int j = 0; j < E_N; j++) out[i * E_N + j] = E.lut[i][j];
+}
+
+// Repeated Material::scatter on one hand-built HitRecord with one stream.
+// ray: origin[3], dir[3], wavelength; rec: p[3], normal[3], t.
+// out: n x 8 = attenuation rgb, direction xyz, skip_pdf, ok.
+int rtc_scatter(const rt_material* m, const float* ray, int spectral_weighted, const float* rec, int front_face,
+                uint64_t seed, uint32_t n, float* out) {
+    std::unique_ptr<Material> mat(make_material(*m));
+    if (!mat) return -1;
+    Ray r = Ray::make(Point3(ray[0], ray[1], ray[2]), Vec3(ray[3], ray[4], ray[5]));
+    r.wavelength = ray[6]; r.spectral_weighted = spectral_weighted != 0;
+    HitRecord h; h.p = Point3(rec[0], rec[1], rec[2]); h.normal = Vec3(rec[3], rec[4], rec[5]); h.t = rec[6];
+    h.front_face = front_face != 0; h.mat = mat.get();
+    SmallRng rng = SmallRng::seed_from_u64(seed);
+    for (uint32_t i = 0; i < n; i++) {
+        ScatterRecord sr;
+        bool ok = mat->scatter(r, h, rng, sr);
+        float* o = out + (size_t)i * 8;
+        if (ok) {
+            o[0] = sr.attenuation.x; o[1] = sr.attenuation.y; o[2] = sr.attenuation.z;
+            o[3] = sr.ray.direction.x; o[4] = sr.ray.direction.y; o[5] = sr.ray.direction.z;
+            o[6] = sr.skip_pdf ? 1.0f : 0.0f; o[7] = 1.0f;
+        } else { for (int k = 0; k < 8; k++) o[k] = 0.0f; }
+    }
+    return 0;
+}
+
+// Env-map CDF sampling against a given image: n samples -> dir xyz + pdf.
+int rtc_env_sample(const float* rgb, uint32_t w, uint32_t h, float rot, uint64_t seed, uint32_t n, float* out) {
+    EnvMap e; e.init(rgb, w, h, rot);
+    SmallRng rng = SmallRng::seed_from_u64(seed);
+    for (uint32_t i = 0; i < n; i++) {
+        float pdf; Vec3 d = e.sample_dir(rng, pdf);
+        out[i * 4] = d.x; out[i * 4 + 1] = d.y; out[i * 4 + 2] = d.z; out[i * 4 + 3] = pdf;
+    }
+    return 0;
+}
+int rtc_env_eval(const float* rgb, uint32_t w, uint32_t h, float rot, const float* dirs, uint32_t n, float* out) {
+    EnvMap e; e.init(rgb, w, h, rot);
+    for (uint32_t i = 0; i < n; i++) {
+        Vec3 d(dirs[i * 3], dirs[i * 3 + 1], dirs[i * 3 + 2]);
+        Color c = e.sample(d);
+        out[i * 4] = c.x; out[i * 4 + 1] = c.y; out[i * 4 + 2] = c.z; out[i * 4 + 3] = e.dir_pdf(d);
+    }
+    return 0;
+}
+
+}  // extern "C"

@@ -1,0 +1,674 @@
+// Device scene layout and the BVH / primitive queries of the radiance path.
+//
+// HBM layout (all arrays 256-B aligned by the allocator):
+//   nodes   : one 256-B record per 8-wide BVH node, SoA inside the node —
+//             minx[8] miny[8] minz[8] maxx[8] maxy[8] maxz[8] child[8] pad[8].
+//             The world tree and every nested tree (mesh, box) share the
+//             array; child >= 0 is a node index, child < 0 is leaf -child-1
+//             (world: object slot; nested: primitive slot), INT32_MIN is empty.
+//   objs    : world objects in leaf order (DObject.tag = input index).
+//   tris    : per triangle 3 x float4 = v0, e1 = v1-v0, e2 = v2-v0 (48 B),
+//             leaf order inside each mesh; shading data (normals, tangents,
+//             uvs) in separate arrays, read only for the closest hit.
+//   quads   : per quad q, u, v, normal, w, (d, area) (80 B).
+//   planes  : convex half-spaces (n.xyz, d).
+// Semantics follow rustracer's BvhTree (src/bvh.rs:229-575): the slab test
+// replicates AVX2 minps/maxps (second operand on NaN) and the ordered <=
+// compare, children are visited near-to-far from a descending sort of t_near,
+// and every interval is closed.
+#pragma once
+#include "dev_math.h"
+
+namespace rt {
+
+enum { SH_SPHERE = 1, SH_QUAD = 2, SH_MESH = 3, SH_BOX = 4, SH_CONVEX = 5 };
+enum { XF_TRANSLATE = 1, XF_ROTATE = 2, XF_SCALE = 3 };
+enum { MED_NONE = 0, MED_CONSTANT = 1, MED_GRID = 2 };
+
+struct DXform { uint32_t kind; float v[9]; };
+
+struct DObject {
+    uint32_t shape, material, xf_first, xf_count;
+    uint32_t data_first, data_count;   // MESH: tris / BOX: quads / QUAD: quad index / CONVEX: planes
+    uint32_t root;                     // nested BVH root node (MESH, BOX)
+    uint32_t medium;                   // 0 or media index + 1
+    uint32_t tag;                      // input index
+    uint32_t emissive;
+    float light_area;
+    float p[16];
+};
+
+struct DQuad { float q[3], u[3], v[3], n[3], w[3], d, area; float pad[3]; };
+
+struct DMedium {
+    uint32_t kind; float density; float albedo[3]; float g;
+    uint32_t nx, ny, nz; uint32_t grid_off;
+    float bmin[3], bmax[3];
+};
+
+struct DMaterial { uint32_t kind, tex; float p[24]; };
+
+struct DScene {
+    const float4* nodes;
+    const DObject* objs; uint32_t num_objs;
+    const DObject* lights; uint32_t num_lights;
+    const DXform* xforms;
+    const float4* tris; const int32_t* tri_tag;
+    const float* tri_n; const float* tri_t; const float* tri_uv;
+    const DQuad* quads;
+    const float4* planes;
+    const DMaterial* mats;
+    const DMedium* media; const float* grids;
+    // background
+    uint32_t bg_kind, bg_visible;
+    float bg_color[3], bg_top[3], bg_bottom[3], sun_dir[3], turbidity;
+    uint32_t env_w, env_h; float env_rot, env_inv_total;
+    const float* env_rgb; const float* env_marg; const float* env_cond;
+    const float* elut;
+    float norms[3], fire_norms[3];
+    // photon map (kd order, 3 float4 per photon: pos+axis, power, normal)
+    const float4* photons; uint32_t num_photons; float ph_r2, ph_norm;
+    uint32_t stack_bound;
+};
+
+// ---- per-lane traversal stack: LDS part + private overflow -------------------
+#ifndef RT_LDS_STACK
+#define RT_LDS_STACK 32
+#endif
+#define RT_PRIV_STACK 96
+struct Stack {
+    int32_t* lds;      // base of this lane's column, stride = blockDim.x
+    uint32_t stride;
+    int32_t priv[RT_PRIV_STACK];
+    int32_t sp;
+    uint32_t n_nodes, n_prims, n_rays;   // traversal work, for the counting pass
+};
+RT_D void st_push(Stack& s, int32_t v) {
+    if (s.sp < RT_LDS_STACK) s.lds[s.sp * s.stride] = v;
+    else s.priv[s.sp - RT_LDS_STACK] = v;
+    s.sp++;
+}
+RT_D int32_t st_pop(Stack& s) {
+    s.sp--;
+    return s.sp < RT_LDS_STACK ? s.lds[s.sp * s.stride] : s.priv[s.sp - RT_LDS_STACK];
+}
+
+// ---- primitive tests ---------------------------------------------------------
+RT_D bool sphere_test(const float* p, const Ray& r, float t_min, float t_max, float& t_out) {
+    V3 c = mk(p[0], p[1], p[2]);
+    if (p[7] != 0.0f) c = c + r.time * (mk(p[4], p[5], p[6]) - c);
+    V3 oc = r.o - c;
+    float a = len2(r.d);
+    if (!(a > 0.0f)) return false;
+    float half_b = dot(oc, r.d);
+    float cc = len2(oc) - p[3] * p[3];
+    float disc = half_b * half_b - a * cc;
+    if (disc < 0.0f) return false;
+    float sq = sqrtf(disc);
+    float root = (-half_b - sq) / a;
+    if (root < t_min || root > t_max) {
+        root = (-half_b + sq) / a;
+        if (root < t_min || root > t_max) return false;
+    }
+    t_out = root;
+    return true;
+}
+
+RT_D bool quad_test(const DQuad& q, const Ray& r, float t_min, float t_max, float& t_out, float& al, float& be) {
+    V3 n = mk(q.n[0], q.n[1], q.n[2]);
+    float denom = dot(n, r.d);
+    if (!(fabsf(denom) >= 1e-8f)) return false;
+    float t = (q.d - dot(n, r.o)) / denom;
+    if (t < t_min || t > t_max) return false;
+    V3 P = at(r, t);
+    V3 planar = P - mk(q.q[0], q.q[1], q.q[2]);
+    V3 w = mk(q.w[0], q.w[1], q.w[2]);
+    float alpha = dot(w, cross(planar, mk(q.v[0], q.v[1], q.v[2])));
+    float beta = dot(w, cross(mk(q.u[0], q.u[1], q.u[2]), planar));
+    if (!(alpha >= 0.0f && alpha <= 1.0f) || !(beta >= 0.0f && beta <= 1.0f)) return false;
+    t_out = t; al = alpha; be = beta;
+    return true;
+}
+
+// Moller-Trumbore (src/triangle.rs:75-126) on precomputed v0, e1, e2.
+RT_D bool tri_test(const float4* tris, uint32_t i, const Ray& r, float t_min, float t_max, float& t_out, float& b1o, float& b2o) {
+    float4 A = tris[3 * i], B = tris[3 * i + 1], Cc = tris[3 * i + 2];
+    V3 v0 = mk(A.x, A.y, A.z), e1 = mk(B.x, B.y, B.z), e2 = mk(Cc.x, Cc.y, Cc.z);
+    V3 h = cross(r.d, e2);
+    float a = dot(e1, h);
+    if (fabsf(a) < 1e-8f) return false;
+    float f = 1.0f / a;
+    V3 s = r.o - v0;
+    float b1 = f * dot(s, h);
+    if (!(b1 >= 0.0f && b1 <= 1.0f)) return false;
+    V3 qv = cross(s, e1);
+    float b2 = f * dot(r.d, qv);
+    if (b2 < 0.0f || b1 + b2 > 1.0f) return false;
+    float t = f * dot(e2, qv);
+    if (t < t_min || t > t_max) return false;
+    t_out = t; b1o = b1; b2o = b2;
+    return true;
+}
+
+// convex_hit (src/diamond.rs:203-253); idx_out = plane index of the normal.
+RT_D bool convex_test(const float4* pl, uint32_t first, uint32_t count, const Ray& r, float t_min, float t_max, float& t_out, int& idx_out) {
+    float t_enter = t_min, t_exit = t_max;
+    int ei = -1, xi = -1;
+    bool entry = false;
+    for (uint32_t k = 0; k < count; k++) {
+        float4 P = pl[first + k];
+        V3 n = mk(P.x, P.y, P.z);
+        float nd = dot(n, r.d);
+        float ndo = dot(n, r.o);
+        float t_face = (P.w - ndo) / nd;
+        if (fabsf(nd) < 1e-8f) {
+            if (ndo > P.w) return false;
+            continue;
+        }
+        if (nd < 0.0f) { if (t_face > t_enter) { t_enter = t_face; ei = (int)k; entry = true; } }
+        else { if (t_face < t_exit) { t_exit = t_face; xi = (int)k; } }
+        if (t_enter > t_exit) return false;
+    }
+    if (t_enter > t_exit) return false;
+    if (t_enter == t_exit && t_enter < t_max) return false;
+    if (entry) { t_out = t_enter; idx_out = ei; return true; }
+    if (t_exit > t_max) return false;
+    t_out = t_exit; idx_out = xi;
+    return true;
+}
+
+// ---- transforms ---------------------------------------------------------------
+RT_HD V3 rot_apply(const float* m, V3 v) {
+    return mk(m[0] * v.x + m[1] * v.y + m[2] * v.z, m[3] * v.x + m[4] * v.y + m[5] * v.z, m[6] * v.x + m[7] * v.y + m[8] * v.z);
+}
+RT_HD V3 rot_apply_t(const float* m, V3 v) {
+    return mk(m[0] * v.x + m[3] * v.y + m[6] * v.z, m[1] * v.x + m[4] * v.y + m[7] * v.z, m[2] * v.x + m[5] * v.y + m[8] * v.z);
+}
+// World ray -> object space: outermost wrapper first.
+RT_D Ray to_object(const DScene& S, const DObject& ob, const Ray& r) {
+    if (ob.xf_count == 0) return r;
+    V3 o = r.o, d = r.d;
+    for (int k = (int)ob.xf_count - 1; k >= 0; k--) {
+        const DXform& x = S.xforms[ob.xf_first + k];
+        if (x.kind == XF_TRANSLATE) o = o - mk(x.v[0], x.v[1], x.v[2]);
+        else if (x.kind == XF_ROTATE) { o = rot_apply_t(x.v, o); d = rot_apply_t(x.v, d); }
+        else { float inv = 1.0f / x.v[0]; o = inv * o; d = inv * d; }
+    }
+    Ray q = make_ray(o, d, r.time);
+    q.wavelength = r.wavelength; q.spectral = r.spectral;
+    return q;
+}
+
+// ---- node test (test_children_avx2, src/bvh.rs:330-364) ------------------------
+RT_D float mnps(float a, float b) { return a < b ? a : b; }
+RT_D float mxps(float a, float b) { return a > b ? a : b; }
+
+// Returns the hit mask; t_near[c] for each child.  Children read as ints.
+RT_D uint32_t node_test(const float4* node, const Ray& r, float t_min, float t_max, float tn[8], int32_t ch[8]) {
+    float4 b[12];
+#pragma unroll
+    for (int k = 0; k < 12; k++) b[k] = node[k];
+    float4 c0 = node[12], c1 = node[13];
+    ch[0] = __float_as_int(c0.x); ch[1] = __float_as_int(c0.y); ch[2] = __float_as_int(c0.z); ch[3] = __float_as_int(c0.w);
+    ch[4] = __float_as_int(c1.x); ch[5] = __float_as_int(c1.y); ch[6] = __float_as_int(c1.z); ch[7] = __float_as_int(c1.w);
+    const float* bf = reinterpret_cast<const float*>(b);
+    float ro[3] = {r.o.x, r.o.y, r.o.z};
+    float id[3] = {r.inv.x, r.inv.y, r.inv.z};
+    uint32_t mask = 0;
+#pragma unroll
+    for (int c = 0; c < 8; c++) {
+        float t0 = t_min, t1 = t_max;
+#pragma unroll
+        for (int a = 0; a < 3; a++) {
+            float da = (bf[a * 8 + c] - ro[a]) * id[a];
+            float db = (bf[24 + a * 8 + c] - ro[a]) * id[a];
+            t0 = mxps(t0, mnps(da, db));
+            t1 = mnps(t1, mxps(da, db));
+        }
+        tn[c] = t0;
+        if (t0 <= t1 && ch[c] != INT32_MIN) mask |= 1u << c;
+    }
+    return mask;
+}
+
+// f32::total_cmp key
+RT_D int32_t tkey(float f) { int32_t i = __float_as_int(f); return i ^ (int32_t)(((uint32_t)(i >> 31)) >> 1); }
+
+// Push hit children far-to-near (stable for equal keys) so the nearest pops first.
+RT_D void push_sorted(Stack& st, uint32_t mask, const float tn[8], const int32_t ch[8]) {
+    int32_t kk[8]; int32_t cc[8]; int n = 0;
+#pragma unroll
+    for (int c = 0; c < 8; c++) if (mask & (1u << c)) { kk[n] = tkey(tn[c]); cc[n] = ch[c]; n++; }
+    for (int i = 1; i < n; i++) {
+        int32_t k = kk[i], v = cc[i]; int j = i - 1;
+        while (j >= 0 && k > kk[j]) { kk[j + 1] = kk[j]; cc[j + 1] = cc[j]; j--; }
+        kk[j + 1] = k; cc[j + 1] = v;
+    }
+    for (int i = 0; i < n; i++) st_push(st, cc[i]);
+}
+
+// ---- media --------------------------------------------------------------------------
+RT_D uint64_t fbits(float f) { return (uint64_t)(uint32_t)__float_as_uint(f); }
+RT_D Rng medium_rng(const Ray& r) {
+    return rng_seed((fbits(r.o.x) * 6364136223846793005ULL) ^ (fbits(r.o.y) * 0x9E3779B97F4A7C15ULL)
+                    ^ (fbits(r.o.z) * 0xBF58476D1CE4E5B9ULL) ^ (fbits(r.d.x) * 0x94D049BB133111EBULL)
+                    ^ fbits(r.wavelength));
+}
+
+// Boundary hit of a medium's shape (sphere or box) in object space.
+RT_D bool boundary_t(const DScene& S, const DObject& ob, const Ray& r, float t_min, float t_max, float& t) {
+    if (ob.shape == SH_SPHERE) return sphere_test(ob.p, r, t_min, t_max, t);
+    // SH_BOX: closest of its six quads (a nested BvhTree in the reference;
+    // the closest hit is order independent up to exact ties)
+    bool found = false; float closest = t_max;
+    for (uint32_t k = 0; k < 6; k++) {
+        float tt, a, b;
+        if (quad_test(S.quads[ob.data_first + k], r, t_min, closest, tt, a, b)) { closest = tt; found = true; }
+    }
+    t = closest;
+    return found;
+}
+
+RT_D float grid_density(const DScene& S, const DMedium& m, V3 p) {
+    float fx = (p.x - m.bmin[0]) / (m.bmax[0] - m.bmin[0]) * (float)m.nx - 0.5f;
+    float fy = (p.y - m.bmin[1]) / (m.bmax[1] - m.bmin[1]) * (float)m.ny - 0.5f;
+    float fz = (p.z - m.bmin[2]) / (m.bmax[2] - m.bmin[2]) * (float)m.nz - 0.5f;
+    fx = fclamp(fx, 0.0f, (float)(m.nx - 1)); fy = fclamp(fy, 0.0f, (float)(m.ny - 1)); fz = fclamp(fz, 0.0f, (float)(m.nz - 1));
+    uint32_t x0 = f2u(fx), y0 = f2u(fy), z0 = f2u(fz);
+    uint32_t x1 = min(x0 + 1, m.nx - 1), y1 = min(y0 + 1, m.ny - 1), z1 = min(z0 + 1, m.nz - 1);
+    float tx = fx - (float)x0, ty = fy - (float)y0, tz = fz - (float)z0;
+    const float* g = S.grids + m.grid_off;
+#define GV(x, y, z) g[((size_t)(z) * m.ny + (y)) * m.nx + (x)]
+    float c00 = GV(x0, y0, z0) * (1.0f - tx) + GV(x1, y0, z0) * tx;
+    float c10 = GV(x0, y1, z0) * (1.0f - tx) + GV(x1, y1, z0) * tx;
+    float c01 = GV(x0, y0, z1) * (1.0f - tx) + GV(x1, y0, z1) * tx;
+    float c11 = GV(x0, y1, z1) * (1.0f - tx) + GV(x1, y1, z1) * tx;
+#undef GV
+    float c0 = c00 * (1.0f - ty) + c10 * ty;
+    float c1 = c01 * (1.0f - ty) + c11 * ty;
+    return (c0 * (1.0f - tz) + c1 * tz) * m.density;
+}
+
+// ConstantMedium::hit / NoiseMedium::hit (src/volume.rs:97-139, 197-236).
+RT_D bool medium_hit(const DScene& S, const DObject& ob, const Ray& r, float t_min, float t_max, float& t_out) {
+    float t1, t2;
+    if (!boundary_t(S, ob, r, -FINF, FINF, t1)) return false;
+    if (!boundary_t(S, ob, r, t1 + 0.0001f, FINF, t2)) return false;
+    if (t1 < t_min) t1 = t_min;
+    if (t2 > t_max) t2 = t_max;
+    if (t1 >= t2) return false;
+    if (t1 < 0.0f) t1 = 0.0f;
+    const DMedium& m = S.media[ob.medium - 1];
+    float ray_length = vlen(r.d);
+    Rng rng = medium_rng(r);
+    if (m.kind == MED_CONSTANT) {
+        float dist_inside = (t2 - t1) * ray_length;
+        float random = fmx(rng_f(rng), 1.1920929e-7f);
+        float hit_dist = (-1.0f / m.density) * logf(random);
+        if (hit_dist > dist_inside) return false;
+        t_out = t1 + hit_dist / ray_length;
+        return true;
+    }
+    float inv_maj = 1.0f / (m.density * ray_length);
+    float t = t1;
+    for (int it = 0; it < 1 << 20; it++) {
+        t -= inv_maj * logf(fmx(rng_f(rng), 1.1920929e-7f));
+        if (t >= t2) return false;
+        float ld = grid_density(S, m, at(r, t));
+        if (rng_f(rng) < ld / m.density) { t_out = t; return true; }
+    }
+    return false;
+}
+
+RT_D float medium_transmittance(const DScene& S, const DObject& ob, const Ray& r, float t_min, float t_max) {
+    float a, b;
+    if (!boundary_t(S, ob, r, -FINF, FINF, a)) return 1.0f;
+    if (!boundary_t(S, ob, r, a + 0.0001f, FINF, b)) return 1.0f;
+    float t1 = fmx(fmx(a, 0.0f), t_min);
+    float t2 = fmn(b, t_max);
+    if (t1 >= t2) return 1.0f;
+    float l = (t2 - t1) * vlen(r.d);
+    const DMedium& m = S.media[ob.medium - 1];
+    if (m.kind == MED_CONSTANT) return expf(-m.density * l);
+    const int K = 16;
+    float seg_t = l / vlen(r.d);
+    float sum = 0.0f;
+    for (int i = 0; i < K; i++) {
+        float t = t1 + seg_t * ((float)i + 0.5f) / (float)K;
+        sum += grid_density(S, m, at(r, t));
+    }
+    return expf(-sum / (float)K * l);
+}
+
+// ---- closest hit ------------------------------------------------------------------------
+struct HitId {
+    float t;
+    int32_t slot;    // world object slot (leaf order), -1 = miss
+    int32_t prim;    // nested primitive slot / convex plane index / -2 = medium event
+};
+
+// Closest hit inside a nested tree (mesh or box); updates closest / prim.
+RT_D void nested_closest(const DScene& S, const DObject& ob, const Ray& r, float t_min, float& closest, int32_t& prim, bool& any, Stack& st) {
+    int32_t base = st.sp;
+    st_push(st, (int32_t)ob.root);
+    while (st.sp > base) {
+        int32_t e = st_pop(st);
+        if (e < 0) {
+            uint32_t k = (uint32_t)(-e - 1);
+            float t, a, b;
+            st.n_prims++;
+            bool h = ob.shape == SH_MESH ? tri_test(S.tris, ob.data_first + k, r, t_min, closest, t, a, b)
+                                          : quad_test(S.quads[ob.data_first + k], r, t_min, closest, t, a, b);
+            if (h) { closest = t; prim = (int32_t)k; any = true; }
+            continue;
+        }
+        float tn[8]; int32_t ch[8];
+        uint32_t mask = node_test(S.nodes + (size_t)e * 16, r, t_min, closest, tn, ch);
+        st.n_nodes++;
+        if (mask) push_sorted(st, mask, tn, ch);
+    }
+}
+
+// Test one world object (any shape, any wrapper) for a closer hit.
+RT_D void object_closest(const DScene& S, const DObject& ob, int32_t slot, const Ray& wr, float t_min, float& closest, HitId& best, Stack& st) {
+    Ray r = to_object(S, ob, wr);
+    if (ob.shape != SH_MESH && ob.shape != SH_BOX) st.n_prims++;
+    if (ob.medium) {
+        float t;
+        if (medium_hit(S, ob, r, t_min, closest, t)) { closest = t; best.t = t; best.slot = slot; best.prim = -2; }
+        return;
+    }
+    switch (ob.shape) {
+        case SH_SPHERE: {
+            float t;
+            if (sphere_test(ob.p, r, t_min, closest, t)) { closest = t; best.t = t; best.slot = slot; best.prim = 0; }
+            break;
+        }
+        case SH_QUAD: {
+            float t, a, b;
+            if (quad_test(S.quads[ob.data_first], r, t_min, closest, t, a, b)) { closest = t; best.t = t; best.slot = slot; best.prim = 0; }
+            break;
+        }
+        case SH_CONVEX: {
+            float t; int idx;
+            if (convex_test(S.planes, ob.data_first, ob.data_count, r, t_min, closest, t, idx)) { closest = t; best.t = t; best.slot = slot; best.prim = idx; }
+            break;
+        }
+        default: {
+            int32_t prim = -1; bool any = false;
+            nested_closest(S, ob, r, t_min, closest, prim, any, st);
+            if (any) { best.t = closest; best.slot = slot; best.prim = prim; }
+        }
+    }
+}
+
+// BvhTree::hit_index over the world (src/bvh.rs:229-288).
+RT_D HitId trace_closest(const DScene& S, const Ray& r, float t_min, float t_max, Stack& st) {
+    HitId best; best.t = t_max; best.slot = -1; best.prim = -1;
+    st.n_rays++;
+    float closest = t_max;
+    st.sp = 0;
+    st_push(st, 0);
+    while (st.sp > 0) {
+        int32_t e = st_pop(st);
+        if (e < 0) {
+            int32_t slot = -e - 1;
+            object_closest(S, S.objs[slot], slot, r, t_min, closest, best, st);
+            continue;
+        }
+        float tn[8]; int32_t ch[8];
+        uint32_t mask = node_test(S.nodes + (size_t)e * 16, r, t_min, closest, tn, ch);
+        st.n_nodes++;
+        if (mask) push_sorted(st, mask, tn, ch);
+    }
+    return best;
+}
+
+// ---- occlusion / transmittance ---------------------------------------------------------
+RT_D bool nested_any(const DScene& S, const DObject& ob, const Ray& r, float t_min, float t_max, Stack& st) {
+    int32_t base = st.sp;
+    st_push(st, (int32_t)ob.root);
+    while (st.sp > base) {
+        int32_t e = st_pop(st);
+        if (e < 0) {
+            uint32_t k = (uint32_t)(-e - 1);
+            float t, a, b;
+            st.n_prims++;
+            bool h = ob.shape == SH_MESH ? tri_test(S.tris, ob.data_first + k, r, t_min, t_max, t, a, b)
+                                          : quad_test(S.quads[ob.data_first + k], r, t_min, t_max, t, a, b);
+            if (h) { st.sp = base; return true; }
+            continue;
+        }
+        float tn[8]; int32_t ch[8];
+        uint32_t mask = node_test(S.nodes + (size_t)e * 16, r, t_min, t_max, tn, ch);
+        st.n_nodes++;
+#pragma unroll
+        for (int c = 0; c < 8; c++) if (mask & (1u << c)) st_push(st, ch[c]);
+    }
+    return false;
+}
+
+// Hittable::transmittance of one world object (opaque surfaces: 0 or 1).
+RT_D float object_transmittance(const DScene& S, const DObject& ob, const Ray& wr, float t_min, float t_max, Stack& st) {
+    Ray r = to_object(S, ob, wr);
+    if (ob.shape != SH_MESH && ob.shape != SH_BOX) st.n_prims++;
+    float tr;
+    if (ob.medium) tr = medium_transmittance(S, ob, r, t_min, t_max);
+    else {
+        bool h;
+        switch (ob.shape) {
+            case SH_SPHERE: { float t; h = sphere_test(ob.p, r, t_min, t_max, t); break; }
+            case SH_QUAD: { float t, a, b; h = quad_test(S.quads[ob.data_first], r, t_min, t_max, t, a, b); break; }
+            case SH_CONVEX: { float t; int idx; h = convex_test(S.planes, ob.data_first, ob.data_count, r, t_min, t_max, t, idx); break; }
+            default: h = nested_any(S, ob, r, t_min, t_max, st);
+        }
+        tr = h ? 0.0f : 1.0f;
+    }
+    // UniformScale: T_world = T_obj^factor (src/transform.rs:47-55)
+    if (tr > 0.0f && tr < 1.0f)
+        for (uint32_t k = 0; k < ob.xf_count; k++)
+            if (S.xforms[ob.xf_first + k].kind == XF_SCALE) tr = powf(tr, S.xforms[ob.xf_first + k].v[0]);
+    return tr;
+}
+
+// BvhTree::transmittance (src/bvh.rs:532-575); any_hit when `any_only`.
+RT_D float trace_transmittance(const DScene& S, const Ray& r, float t_min, float t_max, Stack& st, bool any_only) {
+    float tr = 1.0f;
+    st.n_rays++;
+    st.sp = 0;
+    st_push(st, 0);
+    while (st.sp > 0) {
+        int32_t e = st_pop(st);
+        if (e < 0) {
+            const DObject& ob = S.objs[-e - 1];
+            if (any_only && ob.medium) continue;   // media report no any_hit (src/volume.rs:141)
+            tr *= object_transmittance(S, ob, r, t_min, t_max, st);
+            if (tr <= 0.0f) return 0.0f;
+            continue;
+        }
+        float tn[8]; int32_t ch[8];
+        uint32_t mask = node_test(S.nodes + (size_t)e * 16, r, t_min, t_max, tn, ch);
+        st.n_nodes++;
+#pragma unroll
+        for (int c = 0; c < 8; c++) if (mask & (1u << c)) st_push(st, ch[c]);
+    }
+    return tr;
+}
+
+// ---- full hit record for the winner ------------------------------------------------------
+struct Rec {
+    V3 p, n, tangent;
+    float t, u, v;
+    bool front;
+    uint32_t mat;        // material index; medium events use mat = 0xffffffff and medium id
+    uint32_t medium;
+    int32_t obj_id, prim_id;
+};
+
+RT_D Rec make_rec(V3 p, float t, const Ray& r, V3 outward) {
+    Rec h; h.p = p; h.t = t;
+    h.front = dot(r.d, outward) < 0.0f;
+    h.n = h.front ? outward : -outward;
+    h.u = 0; h.v = 0; h.tangent = mk(0, 0, 0); h.medium = 0;
+    return h;
+}
+
+RT_D V3 micro_tilt(uint32_t idx, float amp, V3 n) {
+    if (amp <= 0.0f) return n;
+    uint64_t h = ((uint64_t)idx * 0x9E3779B97F4A7C15ULL) ^ 0xD1B54A32D192ED03ULL;
+    float a = ((float)(uint32_t)(h >> 32) / 4294967295.0f) * 2.0f - 1.0f;
+    float b = ((float)(uint32_t)h / 4294967295.0f) * 2.0f - 1.0f;
+    V3 up = fabsf(n.x) < 0.9f ? mk(1, 0, 0) : mk(0, 1, 0);
+    V3 t = unit(cross(n, up));
+    V3 b2 = cross(n, t);
+    return unit(n + t * (a * amp) + b2 * (b * amp));
+}
+
+// Object-space record of a known winner (re-runs only the winning test).
+RT_D Rec object_record(const DScene& S, const DObject& ob, const Ray& r, const HitId& id, float t_min) {
+    Rec h;
+    if (id.prim == -2) {
+        h = make_rec(at(r, id.t), id.t, r, mk(1, 0, 0));
+        h.n = mk(1, 0, 0); h.front = true; h.medium = ob.medium;
+        return h;
+    }
+    switch (ob.shape) {
+        case SH_SPHERE: {
+            V3 c = mk(ob.p[0], ob.p[1], ob.p[2]);
+            if (ob.p[7] != 0.0f) c = c + r.time * (mk(ob.p[4], ob.p[5], ob.p[6]) - c);
+            V3 P = at(r, id.t);
+            V3 outward = (P - c) / ob.p[3];
+            h = make_rec(P, id.t, r, outward);
+            float theta = acosf(fclamp(-outward.y, -1.0f, 1.0f));
+            float phi = atan2f(-outward.z, outward.x) + PI;
+            h.u = phi / (2.0f * PI); h.v = theta / PI;
+            break;
+        }
+        case SH_QUAD: case SH_BOX: {
+            const DQuad& q = S.quads[ob.data_first + (ob.shape == SH_BOX ? id.prim : 0)];
+            float t, a, b;
+            quad_test(q, r, t_min, id.t, t, a, b);
+            h = make_rec(at(r, id.t), id.t, r, mk(q.n[0], q.n[1], q.n[2]));
+            h.u = a; h.v = b;
+            break;
+        }
+        case SH_CONVEX: {
+            // no bounding plane recorded: the reference's default normal (0, 1, 0)
+            V3 on = mk(0.0f, 1.0f, 0.0f);
+            if (id.prim >= 0) { float4 P = S.planes[ob.data_first + id.prim]; on = mk(P.x, P.y, P.z); }
+            V3 p = at(r, id.t);
+            V3 rel = p - mk(ob.p[0], ob.p[1], ob.p[2]);
+            float u = (atan2f(-rel.z, rel.x) + PI) / (2.0f * PI);
+            float v = fclamp((rel.y + 0.5f * ob.p[5] + ob.p[4]) / (ob.p[3] + ob.p[4] + ob.p[5]), 0.0f, 1.0f);
+            h = make_rec(p, id.t, r, on);
+            if (ob.p[6] > 0.0f) {
+                uint32_t facet = 0;
+                for (uint32_t k = 0; k < ob.data_count; k++) {
+                    float4 Q = S.planes[ob.data_first + k];
+                    if (fabsf(Q.x - on.x) < 1e-6f && fabsf(Q.y - on.y) < 1e-6f && fabsf(Q.z - on.z) < 1e-6f) { facet = k; break; }
+                }
+                h.n = micro_tilt(facet, ob.p[6], h.n);
+            }
+            h.u = u; h.v = v;
+            break;
+        }
+        default: {  // mesh triangle
+            uint32_t i = ob.data_first + (uint32_t)id.prim;
+            float t, b1, b2;
+            tri_test(S.tris, i, r, t_min, id.t, t, b1, b2);
+            float4 A = S.tris[3 * i], B = S.tris[3 * i + 1], Cc = S.tris[3 * i + 2];
+            V3 e1 = mk(B.x, B.y, B.z), e2 = mk(Cc.x, Cc.y, Cc.z);
+            float b0 = 1.0f - b1 - b2;
+            V3 P = at(r, id.t);
+            V3 sn; V3 n0, n1, n2;
+            if (S.tri_n) {
+                const float* n = S.tri_n + (size_t)i * 9;
+                n0 = mk(n[0], n[1], n[2]); n1 = mk(n[3], n[4], n[5]); n2 = mk(n[6], n[7], n[8]);
+            } else {
+                V3 fn = unit(cross(e1, e2));   // Triangle::flat face normal
+                n0 = n1 = n2 = fn;
+            }
+            V3 sn_raw = b0 * n0 + b1 * n1 + b2 * n2;
+            sn = len2(sn_raw) > 1e-12f ? unit(sn_raw) : unit(cross(e1, e2));
+            float uu, vv;
+            if (S.tri_uv) {
+                const float* uv = S.tri_uv + (size_t)i * 6;
+                uu = b0 * uv[0] + b1 * uv[2] + b2 * uv[4];
+                vv = b0 * uv[1] + b1 * uv[3] + b2 * uv[5];
+            } else {
+                uu = b0 * 0.0f + b1 * 1.0f + b2 * 0.5f;
+                vv = b0 * 0.0f + b1 * 0.0f + b2 * 1.0f;
+            }
+            V3 t0, t1, t2;
+            if (S.tri_t) {
+                const float* tg = S.tri_t + (size_t)i * 9;
+                t0 = mk(tg[0], tg[1], tg[2]); t1 = mk(tg[3], tg[4], tg[5]); t2 = mk(tg[6], tg[7], tg[8]);
+            } else {
+                t0 = t1 = t2 = unit(e1);
+            }
+            V3 raw_t = b0 * t0 + b1 * t1 + b2 * t2;
+            raw_t = raw_t - sn * dot(raw_t, sn);
+            V3 tangent;
+            if (len2(raw_t) > 1e-8f) tangent = unit(raw_t); else { V3 a_, b_; vonb(sn, a_, b_); tangent = a_; }
+            h = make_rec(P, id.t, r, sn);
+            h.u = uu; h.v = vv; h.tangent = tangent;
+        }
+    }
+    return h;
+}
+
+// Object-space record -> world (innermost wrapper first).
+RT_D void record_to_world(const DScene& S, const DObject& ob, Rec& h) {
+    for (uint32_t k = 0; k < ob.xf_count; k++) {
+        const DXform& x = S.xforms[ob.xf_first + k];
+        if (x.kind == XF_TRANSLATE) h.p += mk(x.v[0], x.v[1], x.v[2]);
+        else if (x.kind == XF_ROTATE) { h.p = rot_apply(x.v, h.p); h.n = rot_apply(x.v, h.n); h.tangent = rot_apply(x.v, h.tangent); }
+        else h.p = x.v[0] * h.p;
+    }
+}
+
+RT_D Rec finalize(const DScene& S, const Ray& wr, const HitId& id, float t_min) {
+    const DObject& ob = S.objs[id.slot];
+    Ray r = to_object(S, ob, wr);
+    Rec h = object_record(S, ob, r, id, t_min);
+    record_to_world(S, ob, h);
+    h.mat = ob.material;
+    h.obj_id = (int32_t)ob.tag;
+    h.prim_id = 0;
+    if (ob.shape == SH_MESH && id.prim >= 0) h.prim_id = S.tri_tag[ob.data_first + id.prim];
+    else if (ob.shape == SH_BOX && id.prim >= 0) h.prim_id = (int32_t)S.quads[ob.data_first + id.prim].pad[0];
+    return h;
+}
+
+// ---- light list (HittableList, src/hittable.rs:156-225) -------------------------------------
+RT_D bool light_hit(const DScene& S, const DObject& ob, const Ray& wr, float t_min, float t_max, float& t, Rec& rec, bool want_rec) {
+    Ray r = to_object(S, ob, wr);
+    HitId id; id.slot = 0; id.prim = 0;
+    switch (ob.shape) {
+        case SH_SPHERE: if (!sphere_test(ob.p, r, t_min, t_max, t)) return false; break;
+        case SH_QUAD: { float a, b; if (!quad_test(S.quads[ob.data_first], r, t_min, t_max, t, a, b)) return false; break; }
+        case SH_CONVEX: { int idx; if (!convex_test(S.planes, ob.data_first, ob.data_count, r, t_min, t_max, t, idx)) return false; id.prim = idx; break; }
+        default: return false;
+    }
+    if (want_rec) {
+        id.t = t;
+        rec = object_record(S, ob, r, id, t_min);
+        record_to_world(S, ob, rec);
+        rec.mat = ob.material;
+    }
+    return true;
+}
+
+RT_D bool lights_closest(const DScene& S, const Ray& r, float t_min, float t_max, Rec& rec) {
+    float closest = t_max; int32_t best = -1;
+    for (uint32_t i = 0; i < S.num_lights; i++) {
+        float t; Rec dummy;
+        if (light_hit(S, S.lights[i], r, t_min, closest, t, dummy, false)) { closest = t; best = (int32_t)i; }
+    }
+    if (best < 0) return false;
+    float t;
+    light_hit(S, S.lights[best], r, t_min, closest, t, rec, true);
+    return true;
+}
+
+}  // namespace rt

@@ -1,0 +1,899 @@
+// Device-side appearance and integrator of the radiance path: spectral
+// weights, the material set, backgrounds (solid / gradient / Preetham /
+// equirect env with 2-D CDF importance sampling), the caustic photon gather
+// and ray_color.  Semantics follow rustracer src/material.rs, src/spectrum.rs,
+// src/renderer.rs:23-585 and src/photon.rs:148-228; every branch draws random
+// numbers in the reference's order.
+#pragma once
+#include "dev_scene.h"
+#include "spectral_data.h"
+
+namespace rt {
+
+enum { M_LAMBERTIAN = 1, M_DIELECTRIC = 2, M_SPECTRAL_DIELECTRIC = 3, M_PBR = 4, M_DIFFUSE_LIGHT = 5,
+       M_SPECTRAL_METAL = 6, M_ROUGH_DIELECTRIC = 7, M_PHASE = 100 };
+enum { BGK_SOLID = 0, BGK_GRADIENT = 1, BGK_PHYSICAL = 2, BGK_ENV = 3, BGK_ENV_SPLIT = 4 };
+
+__constant__ float c_cie_x[RT_CIE_N];
+__constant__ float c_cie_y[RT_CIE_N];
+__constant__ float c_cie_z[RT_CIE_N];
+__constant__ float c_metal[6][RT_METAL_N];
+
+// ---- spectrum (src/spectrum.rs:64-277) ------------------------------------------
+RT_HD void xyz_srgb(float x, float y, float z, float o[3]) {
+    o[0] = fmaxf(3.2406f * x - 1.5372f * y - 0.4986f * z, 0.0f);
+    o[1] = fmaxf(-0.9689f * x + 1.8758f * y + 0.0415f * z, 0.0f);
+    o[2] = fmaxf(0.0557f * x - 0.2040f * y + 1.0570f * z, 0.0f);
+}
+RT_D void cie_at(float lambda, float& x, float& y, float& z) {
+    float t = fclamp((lambda - RT_CIE_START) / RT_CIE_STEP, 0.0f, (float)(RT_CIE_N - 1));
+    uint32_t i = min(f2u(t), (uint32_t)(RT_CIE_N - 2));
+    float fr = t - (float)i;
+    x = c_cie_x[i] + fr * (c_cie_x[i + 1] - c_cie_x[i]);
+    y = c_cie_y[i] + fr * (c_cie_y[i + 1] - c_cie_y[i]);
+    z = c_cie_z[i] + fr * (c_cie_z[i + 1] - c_cie_z[i]);
+}
+RT_D V3 spectral_to_rgb(const DScene& S, float lambda) {
+    float x, y, z; cie_at(lambda, x, y, z);
+    float c[3]; xyz_srgb(x, y, z, c);
+    return mk(c[0] * S.norms[0], c[1] * S.norms[1], c[2] * S.norms[2]);
+}
+RT_HD void fire_color(float x, float y, float z, float c[3]) {
+    float w = fmaxf(powf(y, 0.5f), 0.08f);
+    float rgb[3]; xyz_srgb(x, y, z, rgb);
+    c[0] = rgb[0] * w; c[1] = rgb[1] * w; c[2] = rgb[2] * w;
+    const float d = 0.5f;   // FIRE_DESAT default
+    float l = 0.2126f * c[0] + 0.7152f * c[1] + 0.0722f * c[2];
+    for (int k = 0; k < 3; k++) c[k] = c[k] * (1.0f - d) + l * d;
+}
+RT_D V3 spectral_fire_rgb(const DScene& S, float lambda) {
+    float x, y, z; cie_at(lambda, x, y, z);
+    float c[3]; fire_color(x, y, z, c);
+    return mk(c[0] * S.fire_norms[0], c[1] * S.fire_norms[1], c[2] * S.fire_norms[2]);
+}
+RT_D void metal_nk(int variant, float lambda, float& n, float& k) {
+    const float* ns = c_metal[variant * 2];
+    const float* ks = c_metal[variant * 2 + 1];
+    float t = fclamp((lambda - RT_METAL_START) / RT_METAL_STEP, 0.0f, (float)(RT_METAL_N - 1));
+    uint32_t i = min(f2u(t), (uint32_t)(RT_METAL_N - 2));
+    float f = t - (float)i;
+    n = ns[i] + f * (ns[i + 1] - ns[i]);
+    k = ks[i] + f * (ks[i + 1] - ks[i]);
+}
+RT_D float fresnel_conductor(float ci, float n, float k) {
+    float cos2 = ci * ci;
+    float sin2 = fmx(1.0f - cos2, 0.0f);
+    float n2 = n * n, k2 = k * k;
+    float t0 = n2 - k2 - sin2;
+    float ab = sqrtf(t0 * t0 + 4.0f * n2 * k2);
+    float t1 = ab + cos2;
+    float a = sqrtf(fmx(0.5f * (ab + t0), 0.0f));
+    float t2 = 2.0f * ci * a;
+    float rs = (t1 - t2) / fmx(t1 + t2, 1e-12f);
+    float t3 = cos2 * ab + sin2 * sin2;
+    float t4 = t2 * sin2;
+    float rp = rs * (t3 - t4) / fmx(t3 + t4, 1e-12f);
+    return fclamp(0.5f * (rp + rs), 0.0f, 1.0f);
+}
+RT_D float cauchy(float lambda, float b, float c) { float l = lambda * 1e-3f; return b + c / (l * l); }
+RT_D float sellmeier(float lambda, const float* B, const float* C) {
+    float l = lambda * 1e-3f; float l2 = l * l;
+    return sqrtf(1.0f + B[0] * l2 / (l2 - C[0]) + B[1] * l2 / (l2 - C[1]) + B[2] * l2 / (l2 - C[2]));
+}
+
+// ---- dielectric / GGX helpers (src/material.rs:104-321) ------------------------------
+RT_D float schlick(float c, float ref_idx) {
+    float b = (1.0f - ref_idx) / (1.0f + ref_idx);
+    float r0 = b * b; float u = 1.0f - c;
+    return r0 + (1.0f - r0) * (u * u * u * u * u);
+}
+RT_D float fresnel_dielectric(float ci, float eta) {
+    float s2i = fmx(1.0f - ci * ci, 0.0f);
+    float s2t = eta * eta * s2i;
+    if (s2t >= 1.0f) return 1.0f;
+    float ct = sqrtf(1.0f - s2t);
+    float rs = (eta * ci - ct) / (eta * ci + ct);
+    float rp = (eta * ct - ci) / (eta * ct + ci);
+    return (rs * rs + rp * rp) * 0.5f;
+}
+RT_D V3 dielectric_boundary(float ior, const Ray& r, const Rec& h, Rng& rng, bool& reflected) {
+    float ratio = h.front ? 1.0f / ior : ior;
+    V3 u = unit(r.d);
+    float c = fmn(dot(-u, h.n), 1.0f);
+    float s = sqrtf(1.0f - c * c);
+    reflected = ratio * s > 1.0f || fresnel_dielectric(c, ratio) > rng_f(rng);
+    if (reflected) return reflect(u, h.n);
+    V3 out;
+    if (refract(u, h.n, ratio, out)) return out;
+    return reflect(u, h.n);
+}
+RT_D V3 schlick_color(float c, V3 f0) {
+    float u = fmx(1.0f - c, 0.0f);
+    float t = u * u * u * u * u;
+    return f0 + (mk(1, 1, 1) - f0) * t;
+}
+RT_D float smith_g1(float c, float al) { float a2 = al * al, c2 = c * c; return 2.0f * c / (c + sqrtf(a2 + (1.0f - a2) * c2)); }
+RT_D float smith_g1_aniso(float c, float tx, float ty, float ax, float ay) {
+    float den = c + sqrtf(c * c + ax * ax * tx * tx + ay * ay * ty * ty);
+    return fclamp(2.0f * c / fmx(den, 1e-6f), 0.0f, 1.0f);
+}
+RT_D float smith_lambda(float c, float al) { float a2 = al * al, c2 = c * c; return (sqrtf(a2 + (1.0f - a2) * c2) - c) / (2.0f * c); }
+RT_D float smith_lambda_aniso(float c, float tx, float ty, float ax, float ay) {
+    return (sqrtf(c * c + ax * ax * tx * tx + ay * ay * ty * ty) - c) / (2.0f * c);
+}
+RT_D float ggx_ndf(float ch, float al) { float a2 = al * al; float d = 1.0f + ch * ch * (a2 - 1.0f); return a2 / fmx(PI * d * d, 1e-12f); }
+RT_D float ggx_ndf_aniso(V3 h, float ax, float ay) {
+    float hx = h.x / ax, hy = h.y / ay;
+    float d = hx * hx + hy * hy + h.z * h.z;
+    return 1.0f / fmx(PI * ax * ay * d * d, 1e-12f);
+}
+RT_HD V3 vndf_sample(V3 wo, float ax, float ay, float xi1, float xi2) {
+    V3 whr = mk(ax * wo.x, ay * wo.y, wo.z);
+    float wl = sqrtf(len2(whr));
+    V3 wh = mk(whr.x / wl, whr.y / wl, whr.z / wl);
+    float len = sqrtf(wh.x * wh.x + wh.y * wh.y);
+    V3 t1 = len > 1e-6f ? mk(-wh.y / len, wh.x / len, 0.0f) : mk(1, 0, 0);
+    V3 t2 = cross(wh, t1);
+    float a = 1.0f / (1.0f + wh.z);
+    float r = sqrtf(xi1);
+    float p1, p2;
+    if (xi2 < a) { float phi = xi2 / a * PI; p1 = r * cosf(phi); p2 = r * sinf(phi); }
+    else { float phi = PI + (xi2 - a) / (1.0f - a) * PI; p1 = r * cosf(phi); p2 = r * sinf(phi) * wh.z; }
+    float p3 = sqrtf(fmaxf(1.0f - p1 * p1 - p2 * p2, 0.0f));
+    V3 nh = t1 * p1 + t2 * p2 + wh * p3;
+    V3 m = mk(ax * nh.x, ay * nh.y, fmaxf(nh.z, 0.0f));
+    float ml = sqrtf(len2(m));
+    return mk(m.x / ml, m.y / ml, m.z / ml);
+}
+RT_D float e_ss(const DScene& S, float mu, float al) {
+    float x = fclamp(fclamp(mu, 0.0f, 1.0f) * 64.0f - 0.5f, 0.0f, 63.0f);
+    float y = fclamp(fclamp(al, 0.0f, 1.0f) * 64.0f - 0.5f, 0.0f, 63.0f);
+    uint32_t i0 = f2u(x), j0 = f2u(y);
+    uint32_t i1 = min(i0 + 1, 63u), j1 = min(j0 + 1, 63u);
+    float fx = x - (float)i0, fy = y - (float)j0;
+    float e00 = S.elut[i0 * 64 + j0], e10 = S.elut[i1 * 64 + j0], e01 = S.elut[i0 * 64 + j1], e11 = S.elut[i1 * 64 + j1];
+    return (e00 * (1.0f - fx) + e10 * fx) * (1.0f - fy) + (e01 * (1.0f - fx) + e11 * fx) * fy;
+}
+RT_D float energy_comp(const DScene& S, float mu, float al) { return fmn(1.0f / fmx(e_ss(S, mu, al), 0.05f), 10.0f); }
+
+struct Scatter { V3 att; Ray ray; bool skip_pdf; };
+
+RT_D V3 tex_value(const DMaterial& m, V3 p) {
+    V3 even = mk(m.p[0], m.p[1], m.p[2]);
+    if (m.tex == 0) return even;
+    int32_t x = f2i(floorf(m.p[3] * p.x)), y = f2i(floorf(m.p[3] * p.y)), z = f2i(floorf(m.p[3] * p.z));
+    return ((x + y + z) % 2 == 0) ? even : mk(m.p[4], m.p[5], m.p[6]);
+}
+
+// PbrMaterial parameter block: p0-2 albedo, p3 roughness, p4 metallic,
+// p5 anisotropy, p6 angle, p7 clearcoat, p8 coat roughness, p9-11 emission,
+// p12 strength, p13 sheen, p14 sheen_tint.
+RT_D void pbr_frame(const DMaterial& m, V3 n, V3& t, V3& b) {
+    V3 t0, b0; vonb(n, t0, b0);
+    float ca = cosf(m.p[6]), sa = sinf(m.p[6]);
+    t = t0 * ca + b0 * sa;
+    b = b0 * ca - t0 * sa;
+}
+RT_D float pbr_aspect(const DMaterial& m) { return sqrtf(fmx(1.0f - 0.9f * fclamp(m.p[5], 0.0f, 1.0f), 0.001f)); }
+
+RT_D bool pbr_scatter(const DScene& S, const DMaterial& m, const Ray& r, const Rec& h, Rng& rng, Scatter& sr) {
+    V3 n = h.n;
+    V3 wo = unit(-r.d);
+    float cos_o = dot(wo, n);
+    if (cos_o <= 0.0f) return false;
+    V3 albedo = mk(m.p[0], m.p[1], m.p[2]);
+    float rough = m.p[3], metallic = m.p[4], clearcoat = m.p[7];
+    float alpha = fmx(rough * rough, 1e-4f);
+    V3 f0 = mk(0.04f, 0.04f, 0.04f) * (1.0f - metallic) + albedo * metallic;
+    float p_coat = clearcoat > 1e-3f ? fmx(schlick(cos_o, 1.0f / 1.5f) * clearcoat, 0.12f * clearcoat) : 0.0f;
+    V3 fa = schlick_color(cos_o, f0);
+    float p_spec = metallic > 0.999f ? 1.0f : fclamp(lum(fa), 0.04f, 0.9f);
+    if (rng_f(rng) < p_coat) {
+        float ac = fmx(m.p[8] * m.p[8], 1e-4f);
+        V3 t, b; vonb(n, t, b);
+        float xi1 = rng_f(rng), xi2 = rng_f(rng);
+        V3 wo_ts = mk(dot(wo, t), dot(wo, b), cos_o);
+        V3 mts = vndf_sample(wo_ts, ac, ac, xi1, xi2);
+        V3 mm = unit(t * mts.x + b * mts.y + n * mts.z);
+        float voh = fmx(dot(wo, mm), 0.0f);
+        V3 wi = unit(2.0f * voh * mm - wo);
+        float cos_i = dot(wi, n);
+        if (cos_i <= 0.0f) return false;
+        float fc = schlick(voh, 1.0f / 1.5f) * clearcoat;
+        float lo = smith_lambda(cos_o, ac), li = smith_lambda(cos_i, ac);
+        float w = ((1.0f + lo) / (1.0f + lo + li)) * energy_comp(S, cos_o, ac);
+        sr.att = mk(1, 1, 1) * (fc * w / p_coat);
+        sr.ray = scatter_from(h.p, wi, r); sr.skip_pdf = true;
+        return true;
+    }
+    if (rng_f(rng) < p_spec) {
+        float xi1 = rng_f(rng), xi2 = rng_f(rng);
+        float asp = pbr_aspect(m);
+        float ax = alpha / asp, ay = alpha * asp;
+        V3 t, b; pbr_frame(m, n, t, b);
+        V3 wo_ts = mk(dot(wo, t), dot(wo, b), cos_o);
+        V3 mts = vndf_sample(wo_ts, ax, ay, xi1, xi2);
+        V3 mm = unit(t * mts.x + b * mts.y + n * mts.z);
+        float voh = fmx(dot(wo, mm), 0.0f);
+        V3 wi = unit(2.0f * voh * mm - wo);
+        float cos_i = dot(wi, n);
+        if (cos_i <= 0.0f) return false;
+        V3 wi_ts = mk(dot(wi, t), dot(wi, b), cos_i);
+        float lo = smith_lambda_aniso(cos_o, wo_ts.x, wo_ts.y, ax, ay);
+        float li = smith_lambda_aniso(cos_i, wi_ts.x, wi_ts.y, ax, ay);
+        float wg = ((1.0f + lo) / (1.0f + lo + li)) * energy_comp(S, cos_o, alpha);
+        V3 f = schlick_color(voh, f0);
+        sr.att = f * wg / (p_spec * (1.0f - p_coat));
+        sr.ray = scatter_from(h.p, wi, r); sr.skip_pdf = true;
+        return true;
+    }
+    V3 sheen_attn = mk(0, 0, 0);
+    float sheen = m.p[13];
+    if (sheen > 0.0f && metallic < 0.999f) {
+        float l = lum(albedo);
+        V3 ct = l > 1e-6f ? albedo / l : mk(1, 1, 1);
+        V3 sc = mk(1, 1, 1) * (1.0f - m.p[14]) + ct * m.p[14];
+        float fh = pow5(1.0f - cos_o);
+        sheen_attn = sc * (sheen * (1.0f - metallic) * fh * PI);
+    }
+    sr.att = (albedo * (1.0f - metallic) + sheen_attn) / ((1.0f - p_spec) * (1.0f - p_coat));
+    sr.ray = scatter_from(h.p, h.n, r); sr.skip_pdf = false;
+    return true;
+}
+
+RT_D V3 pbr_brdf_cos(const DScene& S, const DMaterial& m, const Ray& r, const Rec& h, V3 wi) {
+    V3 n = h.n, wo = unit(-r.d);
+    float cos_o = dot(wo, n), cos_i = dot(wi, n);
+    if (cos_o <= 0.0f || cos_i <= 0.0f) return mk(0, 0, 0);
+    V3 hv = unit(wo + wi);
+    float woh = fmx(dot(wo, hv), 0.0f), hn = fmx(dot(hv, n), 0.0f);
+    V3 albedo = mk(m.p[0], m.p[1], m.p[2]);
+    float metallic = m.p[4], clearcoat = m.p[7];
+    V3 f0 = mk(0.04f, 0.04f, 0.04f) * (1.0f - metallic) + albedo * metallic;
+    V3 res = mk(0, 0, 0);
+    if (clearcoat > 1e-3f) {
+        float ac = fmx(m.p[8] * m.p[8], 1e-4f);
+        float d = ggx_ndf(hn, ac);
+        float g2 = 1.0f / (1.0f + smith_lambda(cos_o, ac) + smith_lambda(cos_i, ac));
+        float fc = schlick(woh, 1.0f / 1.5f) * clearcoat;
+        res += mk(1, 1, 1) * (d * fc * g2 / (4.0f * cos_o) * energy_comp(S, cos_o, ac));
+    }
+    float alpha = fmx(m.p[3] * m.p[3], 1e-4f);
+    float asp = pbr_aspect(m); float ax = alpha / asp, ay = alpha * asp;
+    V3 t, b; pbr_frame(m, n, t, b);
+    V3 hts = mk(dot(hv, t), dot(hv, b), hn), wots = mk(dot(wo, t), dot(wo, b), cos_o), wits = mk(dot(wi, t), dot(wi, b), cos_i);
+    float d = ggx_ndf_aniso(hts, ax, ay);
+    float g2 = 1.0f / (1.0f + smith_lambda_aniso(cos_o, wots.x, wots.y, ax, ay) + smith_lambda_aniso(cos_i, wits.x, wits.y, ax, ay));
+    V3 f = schlick_color(woh, f0);
+    res += f * (d * g2 / (4.0f * cos_o) * energy_comp(S, cos_o, alpha));
+    return res;
+}
+
+RT_D float pbr_spec_pdf(const DMaterial& m, const Ray& r, const Rec& h, V3 wi) {
+    V3 n = h.n, wo = unit(-r.d);
+    float cos_o = dot(wo, n), cos_i = dot(wi, n);
+    if (cos_o <= 0.0f || cos_i <= 0.0f) return 0.0f;
+    V3 hv = unit(wo + wi);
+    float hn = fmx(dot(hv, n), 0.0f);
+    V3 albedo = mk(m.p[0], m.p[1], m.p[2]);
+    float metallic = m.p[4], clearcoat = m.p[7];
+    V3 f0 = mk(0.04f, 0.04f, 0.04f) * (1.0f - metallic) + albedo * metallic;
+    float p_coat = clearcoat > 1e-3f ? fmx(schlick(cos_o, 1.0f / 1.5f) * clearcoat, 0.12f * clearcoat) : 0.0f;
+    V3 fa = schlick_color(cos_o, f0);
+    float p_spec = metallic > 0.999f ? 1.0f : fclamp(lum(fa), 0.04f, 0.9f);
+    float pdf = 0.0f;
+    if (clearcoat > 1e-3f) {
+        float ac = fmx(m.p[8] * m.p[8], 1e-4f);
+        pdf += p_coat * ggx_ndf(hn, ac) * smith_g1(cos_o, ac) / (4.0f * cos_o);
+    }
+    if ((1.0f - p_coat) * p_spec > 1e-6f) {
+        float alpha = fmx(m.p[3] * m.p[3], 1e-4f);
+        float asp = pbr_aspect(m); float ax = alpha / asp, ay = alpha * asp;
+        V3 t, b; pbr_frame(m, n, t, b);
+        V3 hts = mk(dot(hv, t), dot(hv, b), hn), wots = mk(dot(wo, t), dot(wo, b), cos_o);
+        pdf += (1.0f - p_coat) * p_spec * ggx_ndf_aniso(hts, ax, ay) * smith_g1_aniso(cos_o, wots.x, wots.y, ax, ay) / (4.0f * cos_o);
+    }
+    return pdf;
+}
+
+RT_D bool smetal_scatter(const DScene& S, const DMaterial& m, const Ray& r, const Rec& h, Rng& rng, Scatter& sr) {
+    V3 n = h.n, wo = unit(-r.d);
+    float cos_o = dot(wo, n);
+    if (cos_o <= 0.0f) return false;
+    float rough = m.p[1];
+    V3 dir; float cos_h;
+    if (rough == 0.0f) { dir = reflect(unit(r.d), n); cos_h = cos_o; }
+    else {
+        float alpha = fmx(rough * rough, 1e-4f);
+        V3 t, b; vonb(n, t, b);
+        V3 wo_ts = mk(dot(wo, t), dot(wo, b), cos_o);
+        float xi1 = rng_f(rng), xi2 = rng_f(rng);
+        V3 mts = vndf_sample(wo_ts, alpha, alpha, xi1, xi2);
+        V3 mm = unit(t * mts.x + b * mts.y + n * mts.z);
+        float voh = fmx(dot(wo, mm), 0.0f);
+        V3 wi = unit(2.0f * voh * mm - wo);
+        if (dot(wi, n) <= 0.0f) return false;
+        dir = wi; cos_h = voh;
+    }
+    if (dot(dir, n) <= 0.0f) return false;
+    float in, ik; metal_nk((int)m.p[0], r.wavelength, in, ik);
+    float fr = fresnel_conductor(cos_h, in, ik);
+    float w = 1.0f;
+    if (rough != 0.0f) {
+        float alpha = fmx(rough * rough, 1e-4f);
+        float lo = smith_lambda(cos_o, alpha), li = smith_lambda(fmx(dot(dir, n), 0.0f), alpha);
+        w = ((1.0f + lo) / (1.0f + lo + li)) * energy_comp(S, cos_o, alpha);
+    }
+    float fw = fr * w;
+    sr.att = r.spectral ? mk(fw, fw, fw) : spectral_to_rgb(S, r.wavelength) * fw;
+    sr.ray = scatter_from(h.p, dir, r);
+    if (!r.spectral) sr.ray.spectral = true;
+    sr.skip_pdf = true;
+    return true;
+}
+RT_D V3 smetal_brdf_cos(const DScene& S, const DMaterial& m, const Ray& r, const Rec& h, V3 wi) {
+    float rough = m.p[1];
+    if (rough == 0.0f) return mk(0, 0, 0);
+    V3 n = h.n, wo = unit(-r.d);
+    float cos_o = dot(wo, n), cos_i = dot(wi, n);
+    if (cos_o <= 0.0f || cos_i <= 0.0f) return mk(0, 0, 0);
+    V3 hv = unit(wo + wi);
+    float woh = fmx(dot(wo, hv), 0.0f), hn = fmx(dot(hv, n), 0.0f);
+    float alpha = fmx(rough * rough, 1e-4f);
+    float d = ggx_ndf(hn, alpha);
+    float g2 = 1.0f / (1.0f + smith_lambda(cos_o, alpha) + smith_lambda(cos_i, alpha));
+    float in, ik; metal_nk((int)m.p[0], r.wavelength, in, ik);
+    float fr = fresnel_conductor(woh, in, ik);
+    float val = d * fr * g2 / (4.0f * cos_o) * energy_comp(S, cos_o, alpha);
+    return r.spectral ? mk(val, val, val) : spectral_to_rgb(S, r.wavelength) * val;
+}
+RT_D float smetal_spec_pdf(const DMaterial& m, const Ray& r, const Rec& h, V3 wi) {
+    float rough = m.p[1];
+    if (rough == 0.0f) return 0.0f;
+    V3 n = h.n, wo = unit(-r.d);
+    float cos_o = dot(wo, n);
+    if (cos_o <= 0.0f || dot(wi, n) <= 0.0f) return 0.0f;
+    V3 hv = unit(wo + wi);
+    float hn = fmx(dot(hv, n), 0.0f);
+    float alpha = fmx(rough * rough, 1e-4f);
+    return ggx_ndf(hn, alpha) * smith_g1(cos_o, alpha) / (4.0f * cos_o);
+}
+
+// Rough GGX dielectric (extension): VNDF microfacet, exact Fresnel, weight G1(wi).
+RT_D bool rough_dielectric_scatter(const DMaterial& m, const Ray& r, const Rec& h, Rng& rng, Scatter& sr) {
+    float alpha = fmx(m.p[1] * m.p[1], 1e-4f);
+    V3 n = h.n, wo = unit(-r.d);
+    float cos_o = dot(wo, n);
+    if (cos_o <= 0.0f) return false;
+    float eta = h.front ? 1.0f / m.p[0] : m.p[0];
+    V3 t, b; vonb(n, t, b);
+    float xi1 = rng_f(rng), xi2 = rng_f(rng);
+    V3 wo_ts = mk(dot(wo, t), dot(wo, b), cos_o);
+    V3 mts = vndf_sample(wo_ts, alpha, alpha, xi1, xi2);
+    V3 mm = unit(t * mts.x + b * mts.y + n * mts.z);
+    float com = fmn(dot(wo, mm), 1.0f);
+    float fr = fresnel_dielectric(com, eta);
+    V3 wi;
+    bool refl = fr > rng_f(rng);
+    if (!refl) { V3 out; if (refract(-wo, mm, eta, out)) wi = unit(out); else refl = true; }
+    if (refl) wi = unit(reflect(-wo, mm));
+    float ci = dot(wi, n);
+    if (refl ? ci <= 0.0f : ci >= 0.0f) return false;
+    float g1 = smith_g1(fabsf(ci), alpha);
+    sr.att = mk(g1, g1, g1);
+    sr.ray = scatter_from(h.p, wi, r); sr.skip_pdf = true;
+    return true;
+}
+
+RT_D V3 hg_sample(V3 vin, float g, Rng& rng) {
+    float xi1 = rng_f(rng), xi2 = rng_f(rng);
+    float ct;
+    if (fabsf(g) < 1e-3f) ct = 1.0f - 2.0f * xi1;
+    else {
+        float s = (1.0f - g * g) / fmx(1.0f - g + 2.0f * g * xi1, 1e-12f);
+        ct = (1.0f + g * g - s * s) / (2.0f * g);
+    }
+    float st = sqrtf(fmx(1.0f - ct * ct, 0.0f));
+    float phi = 2.0f * PI * xi2;
+    V3 w = vin;
+    V3 helper = fabsf(w.x) > 0.9f ? mk(0, 1, 0) : mk(1, 0, 0);
+    V3 u = unit(cross(w, helper));
+    V3 v = cross(w, u);
+    return u * (st * cosf(phi)) + v * (st * sinf(phi)) + w * ct;
+}
+
+// Material::scatter dispatch.  Medium events scatter with the HG phase function.
+RT_D bool mat_scatter(const DScene& S, const Rec& h, const Ray& r, Rng& rng, Scatter& sr) {
+    if (h.medium) {
+        const DMedium& md = S.media[h.medium - 1];
+        sr.att = mk(md.albedo[0], md.albedo[1], md.albedo[2]);
+        sr.ray = scatter_from(h.p, hg_sample(unit(r.d), md.g, rng), r);
+        sr.skip_pdf = true;
+        return true;
+    }
+    const DMaterial& m = S.mats[h.mat];
+    switch (m.kind) {
+        case M_LAMBERTIAN:
+            sr.att = tex_value(m, h.p); sr.ray = scatter_from(h.p, h.n, r); sr.skip_pdf = false;
+            return true;
+        case M_DIELECTRIC: {
+            bool refl;
+            V3 d = dielectric_boundary(m.p[0], r, h, rng, refl);
+            sr.att = mk(1, 1, 1); sr.ray = scatter_from(h.p, d, r); sr.skip_pdf = true;
+            return true;
+        }
+        case M_SPECTRAL_DIELECTRIC: {
+            float ior = m.p[5] != 0.0f ? sellmeier(r.wavelength, m.p + 6, m.p + 9) : cauchy(r.wavelength, m.p[0], m.p[1]);
+            bool refl;
+            V3 d = dielectric_boundary(ior, r, h, rng, refl);
+            V3 att = mk(1, 1, 1);
+            bool tinted = false;
+            if (!r.spectral && !refl) { att = spectral_fire_rgb(S, r.wavelength); tinted = true; }
+            if (!h.front) {
+                float dd = h.t;
+                att.x *= expf(-m.p[2] * dd); att.y *= expf(-m.p[3] * dd); att.z *= expf(-m.p[4] * dd);
+            }
+            sr.ray = scatter_from(h.p, d, r);
+            if (tinted) sr.ray.spectral = true;
+            sr.att = att; sr.skip_pdf = true;
+            return true;
+        }
+        case M_PBR: return pbr_scatter(S, m, r, h, rng, sr);
+        case M_SPECTRAL_METAL: return smetal_scatter(S, m, r, h, rng, sr);
+        case M_ROUGH_DIELECTRIC: return rough_dielectric_scatter(m, r, h, rng, sr);
+        default: return false;   // DiffuseLight
+    }
+}
+RT_D V3 mat_emitted(const DScene& S, const Rec& h) {
+    if (h.medium) return mk(0, 0, 0);
+    const DMaterial& m = S.mats[h.mat];
+    if (m.kind == M_DIFFUSE_LIGHT) return mk(m.p[0], m.p[1], m.p[2]);
+    if (m.kind == M_PBR) return mk(m.p[9], m.p[10], m.p[11]) * m.p[12];
+    return mk(0, 0, 0);
+}
+RT_D float mat_scattering_pdf(const Rec& h, V3 dir) { return fmx(dot(h.n, unit(dir)) / PI, 0.0f); }
+RT_D V3 mat_albedo_hint(const DScene& S, const Rec& h) {
+    if (h.medium) return mk(1, 1, 1);
+    const DMaterial& m = S.mats[h.mat];
+    if (m.kind == M_LAMBERTIAN) return tex_value(m, h.p);
+    if (m.kind == M_PBR) return mk(m.p[0], m.p[1], m.p[2]);
+    if (m.kind == M_DIFFUSE_LIGHT) return mk(m.p[0], m.p[1], m.p[2]);
+    return mk(1, 1, 1);
+}
+RT_D bool mat_receives_caustics(const DScene& S, const Rec& h) {
+    if (h.medium) return false;
+    const DMaterial& m = S.mats[h.mat];
+    return m.kind == M_LAMBERTIAN || (m.kind == M_PBR && m.p[4] < 0.999f);
+}
+RT_D V3 mat_spec_brdf_cos(const DScene& S, const Rec& h, const Ray& r, V3 wi) {
+    if (h.medium) return mk(0, 0, 0);
+    const DMaterial& m = S.mats[h.mat];
+    if (m.kind == M_PBR) return pbr_brdf_cos(S, m, r, h, wi);
+    if (m.kind == M_SPECTRAL_METAL) return smetal_brdf_cos(S, m, r, h, wi);
+    return mk(0, 0, 0);
+}
+RT_D float mat_spec_pdf(const DScene& S, const Rec& h, const Ray& r, V3 wi) {
+    if (h.medium) return 0.0f;
+    const DMaterial& m = S.mats[h.mat];
+    if (m.kind == M_PBR) return pbr_spec_pdf(m, r, h, wi);
+    if (m.kind == M_SPECTRAL_METAL) return smetal_spec_pdf(m, r, h, wi);
+    return 0.0f;
+}
+
+// ---- background -------------------------------------------------------------------------
+static constexpr float COS_SUN_MAX = 0.9999892f;
+
+RT_D V3 preetham(V3 dir, V3 sun_dir, float turbidity) {
+    V3 d = unit(dir), sun = unit(sun_dir);
+    float cg = fclamp(dot(d, sun), -1.0f, 1.0f);
+    float gamma = acosf(cg);
+    float t = fclamp(turbidity, 1.7f, 20.0f);
+    float cts = fmx(sun.y, 0.0175f);
+    float ths = acosf(cts);
+#define PEREZ(a, b, c, dp, e, ct, g, cgm) ((1.0f + (a) * expf((b) / (ct))) * (1.0f + (c) * expf((dp) * (g)) + (e) * (cgm) * (cgm)))
+    float ay = 0.1787f * t - 1.4630f, by_ = -0.3554f * t + 0.4275f, cy = 0.0227f * t + 5.3251f, dy = 0.1206f * t - 2.5771f, ey = -0.0670f * t + 0.3703f;
+    float ax = -0.0193f * t - 0.2592f, bx = -0.0665f * t + 0.0008f, cx = -0.0004f * t + 0.2125f, dx = -0.0641f * t - 0.8989f, ex = -0.0033f * t + 0.0452f;
+    float acy = -0.0167f * t - 0.2608f, bcy = -0.0950f * t + 0.0092f, ccy = -0.0079f * t + 0.2102f, dcy = -0.0441f * t - 1.6537f, ecy = -0.0109f * t + 0.0529f;
+    float fn_y = PEREZ(ay, by_, cy, dy, ey, 1.0f, ths, cts);
+    float fn_x = PEREZ(ax, bx, cx, dx, ex, 1.0f, ths, cts);
+    float fn_cy = PEREZ(acy, bcy, ccy, dcy, ecy, 1.0f, ths, cts);
+    float yz = fmx((4.0453f * t - 4.9710f) * tanf((4.0f / 9.0f - t / 120.0f) * (PI - 2.0f * ths)) - 0.2155f * t + 2.4192f, 0.0f);
+    if (cg > COS_SUN_MAX && sun.y > 0.0f && d.y > 0.0f) { float disc = yz * 111.0f; return mk(disc, disc * 0.95f, disc * 0.85f); }
+    float ts = ths, ts2 = ts * ts, ts3 = ts2 * ts;
+    float xz = t * t * (0.00166f * ts3 - 0.00375f * ts2 + 0.00209f * ts) + t * (-0.02903f * ts3 + 0.06377f * ts2 - 0.03202f * ts + 0.00394f)
+             + (0.11693f * ts3 - 0.21196f * ts2 + 0.06052f * ts + 0.25886f);
+    float yzc = t * t * (0.00275f * ts3 - 0.00610f * ts2 + 0.00317f * ts) + t * (-0.04214f * ts3 + 0.08970f * ts2 - 0.04153f * ts + 0.00516f)
+              + (0.15346f * ts3 - 0.26756f * ts2 + 0.06670f * ts + 0.26688f);
+    float ct = fmx(d.y, 0.001f);
+    float fy = PEREZ(ay, by_, cy, dy, ey, ct, gamma, cg);
+    float fx = PEREZ(ax, bx, cx, dx, ex, ct, gamma, cg);
+    float fcy = PEREZ(acy, bcy, ccy, dcy, ecy, ct, gamma, cg);
+#undef PEREZ
+    float cap_y = fmx(yz * fy / fn_y, 0.0f);
+    float cap_x = fclamp(xz * fx / fn_x, 0.001f, 0.998f);
+    float cap_yc = fclamp(yzc * fcy / fn_cy, 0.001f, 0.998f);
+    float l = cap_y * 0.05f;
+    float X = cap_x / cap_yc * l, Y = l, Z = fmx((1.0f - cap_x - cap_yc) / cap_yc * l, 0.0f);
+    V3 sky = mk(fmx(3.2406f * X - 1.5372f * Y - 0.4986f * Z, 0.0f), fmx(-0.9689f * X + 1.8758f * Y + 0.0415f * Z, 0.0f),
+                fmx(0.0557f * X - 0.2040f * Y + 1.0570f * Z, 0.0f));
+    if (d.y < 0.0f) { float tt = fmn(-d.y * 8.0f, 1.0f); return sky * (1.0f - tt) + mk(0.07f, 0.06f, 0.05f) * tt; }
+    return sky;
+}
+
+RT_D V3 env_px(const DScene& S, int32_t ix, int32_t iy) {
+    int32_t w = (int32_t)S.env_w, h = (int32_t)S.env_h;
+    int32_t x = ((ix % w) + w) % w;
+    int32_t y = iy < 0 ? 0 : (iy > h - 1 ? h - 1 : iy);
+    const float* p = S.env_rgb + ((size_t)y * S.env_w + x) * 3;
+    return mk(p[0], p[1], p[2]);
+}
+RT_D V3 env_unrot(const DScene& S, V3 d) { float s = sinf(S.env_rot), c = cosf(S.env_rot); return mk(d.x * c - d.z * s, d.y, d.x * s + d.z * c); }
+RT_D V3 env_rot(const DScene& S, V3 d) { float s = sinf(S.env_rot), c = cosf(S.env_rot); return mk(d.x * c + d.z * s, d.y, -d.x * s + d.z * c); }
+RT_D V3 env_sample(const DScene& S, V3 dir) {
+    V3 d = env_unrot(S, unit(dir));
+    float u = rem1(0.5f + atan2f(d.x, d.z) / (2.0f * PI));
+    float v = acosf(fclamp(d.y, -1.0f, 1.0f)) / PI;
+    float x = u * (float)S.env_w - 0.5f, y = v * (float)S.env_h - 0.5f;
+    int32_t x0 = f2i(floorf(x)), y0 = f2i(floorf(y));
+    float fx = x - (float)x0, fy = y - (float)y0;
+    V3 c00 = env_px(S, x0, y0), c10 = env_px(S, x0 + 1, y0), c01 = env_px(S, x0, y0 + 1), c11 = env_px(S, x0 + 1, y0 + 1);
+    return (c00 * (1.0f - fx) + c10 * fx) * (1.0f - fy) + (c01 * (1.0f - fx) + c11 * fx) * fy;
+}
+RT_D uint32_t partition_point(const float* a, uint32_t n, float xi) {
+    uint32_t lo = 0, hi = n;
+    while (lo < hi) { uint32_t mid = lo + (hi - lo) / 2; if (a[mid] < xi) lo = mid + 1; else hi = mid; }
+    return lo;
+}
+RT_D float env_pdf_at(const DScene& S, uint32_t i, uint32_t j) {
+    const float* p = S.env_rgb + ((size_t)j * S.env_w + i) * 3;
+    float l = lum(mk(p[0], p[1], p[2]));
+    return l * (float)((size_t)S.env_w * S.env_h) * S.env_inv_total / (2.0f * PI * PI);
+}
+RT_D V3 env_sample_dir(const DScene& S, Rng& rng, float& pdf) {
+    uint32_t j = min(partition_point(S.env_marg, S.env_h, rng_f(rng)), S.env_h - 1);
+    uint32_t i = min(partition_point(S.env_cond + (size_t)j * S.env_w, S.env_w, rng_f(rng)), S.env_w - 1);
+    float u = ((float)i + rng_f(rng)) / (float)S.env_w;
+    float v = ((float)j + rng_f(rng)) / (float)S.env_h;
+    float phi = 2.0f * PI * (u - 0.5f);
+    float theta = PI * v;
+    float st = sinf(theta);
+    pdf = env_pdf_at(S, i, j);
+    return env_rot(S, mk(st * sinf(phi), cosf(theta), st * cosf(phi)));
+}
+RT_D float env_dir_pdf(const DScene& S, V3 dir) {
+    V3 d = env_unrot(S, unit(dir));
+    float u = rem1(0.5f + atan2f(d.x, d.z) / (2.0f * PI));
+    float v = acosf(fclamp(d.y, -1.0f, 1.0f)) / PI;
+    uint32_t i = min(f2u(u * (float)S.env_w), S.env_w - 1);
+    uint32_t j = min(f2u(v * (float)S.env_h), S.env_h - 1);
+    return env_pdf_at(S, i, j);
+}
+RT_D bool has_env(const DScene& S) { return S.bg_kind == BGK_ENV || S.bg_kind == BGK_ENV_SPLIT; }
+RT_D V3 bg_simple(const DScene& S, uint32_t k, V3 dir) {
+    if (k == BGK_SOLID) return mk(S.bg_color[0], S.bg_color[1], S.bg_color[2]);
+    float t = (fclamp(dir.y, -1.0f, 1.0f) + 1.0f) * 0.5f;
+    return mk(S.bg_bottom[0], S.bg_bottom[1], S.bg_bottom[2]) * (1.0f - t) + mk(S.bg_top[0], S.bg_top[1], S.bg_top[2]) * t;
+}
+RT_D V3 bg_eval(const DScene& S, V3 dir) {
+    switch (S.bg_kind) {
+        case BGK_SOLID: case BGK_GRADIENT: return bg_simple(S, S.bg_kind, dir);
+        case BGK_PHYSICAL: return preetham(dir, mk(S.sun_dir[0], S.sun_dir[1], S.sun_dir[2]), S.turbidity);
+        case BGK_ENV: return env_sample(S, dir);
+        default: return bg_simple(S, S.bg_visible, dir);
+    }
+}
+RT_D V3 env_eval(const DScene& S, V3 dir) { return has_env(S) ? env_sample(S, dir) : mk(0, 0, 0); }
+RT_D bool physical_sun(const DScene& S) { return S.bg_kind == BGK_PHYSICAL && S.sun_dir[1] > 0.0f; }
+RT_D float sun_pdf_value(const DScene& S, V3 dir) {
+    if (physical_sun(S) && dot(unit(dir), unit(mk(S.sun_dir[0], S.sun_dir[1], S.sun_dir[2]))) > COS_SUN_MAX)
+        return 1.0f / (2.0f * PI * (1.0f - COS_SUN_MAX));
+    return 0.0f;
+}
+RT_D float env_pdf_value(const DScene& S, V3 dir) { return has_env(S) ? env_dir_pdf(S, dir) : 0.0f; }
+RT_D V3 sample_sun_cone(V3 axis, float ctm, Rng& rng) {
+    float r1 = rng_f(rng), r2 = rng_f(rng);
+    float ct = 1.0f - r1 * (1.0f - ctm);
+    float st = sqrtf(fmx(1.0f - ct * ct, 0.0f));
+    float phi = 2.0f * PI * r2;
+    V3 up = fabsf(axis.x) < 0.999f ? mk(1, 0, 0) : mk(0, 1, 0);
+    V3 t = unit(cross(axis, up));
+    V3 b = cross(axis, t);
+    return t * (st * cosf(phi)) + b * (st * sinf(phi)) + axis * ct;
+}
+
+// ---- light list sampling (HittableList::pdf_generate / pdf_value) -------------------------
+RT_D float light_pdf_value(const DScene& S, const DObject& ob, V3 origin, V3 dir, float time, Stack& st) {
+    // Translate/Rotate/Scale map the query into object space; the pdf of a
+    // shape is computed through its own hit test, as Quad::pdf_value does.
+    Ray wr = make_ray(origin, dir, time);
+    Ray r = to_object(S, ob, wr);
+    V3 o = r.o, d = r.d;
+    if (ob.shape == SH_QUAD) {
+        const DQuad& q = S.quads[ob.data_first];
+        float t, a, b;
+        if (!quad_test(q, r, 0.001f, FINF, t, a, b)) return 0.0f;
+        V3 P = at(r, t);
+        V3 n = mk(q.n[0], q.n[1], q.n[2]);
+        V3 nn = dot(r.d, n) < 0.0f ? n : -n;
+        float dist2 = len2(P - o);
+        float ct = fabsf(dot(unit(d), nn));
+        if (ct < 1e-8f) return 0.0f;
+        return dist2 / (ct * q.area);
+    }
+    if (ob.shape == SH_SPHERE) {
+        V3 c = mk(ob.p[0], ob.p[1], ob.p[2]);
+        if (ob.p[7] != 0.0f) c = c + time * (mk(ob.p[4], ob.p[5], ob.p[6]) - c);
+        V3 tc = c - o;
+        float d2 = len2(tc), r2 = ob.p[3] * ob.p[3];
+        if (d2 <= r2) return 0.0f;
+        float ctm = sqrtf(1.0f - r2 / d2);
+        return 1.0f / (2.0f * PI * (1.0f - ctm));
+    }
+    return 0.0f;
+}
+RT_D float lights_pdf_value(const DScene& S, V3 origin, V3 dir, float time, Stack& st) {
+    if (S.num_lights == 0) return 0.0f;
+    float w = 1.0f / (float)S.num_lights;
+    float s = 0.0f;
+    for (uint32_t i = 0; i < S.num_lights; i++) s += w * light_pdf_value(S, S.lights[i], origin, dir, time, st);
+    return s;
+}
+RT_D V3 light_generate(const DScene& S, const DObject& ob, V3 origin, Rng& rng, float time) {
+    // object-space origin, then the local direction back to world space
+    Ray wr = make_ray(origin, mk(1, 0, 0), time);
+    Ray r = to_object(S, ob, wr);
+    V3 o = r.o, d;
+    if (ob.shape == SH_QUAD) {
+        const DQuad& q = S.quads[ob.data_first];
+        float r1 = rng_f(rng), r2 = rng_f(rng);
+        V3 pt = mk(q.q[0], q.q[1], q.q[2]) + r1 * mk(q.u[0], q.u[1], q.u[2]) + r2 * mk(q.v[0], q.v[1], q.v[2]);
+        d = unit(pt - o);
+    } else if (ob.shape == SH_SPHERE) {
+        V3 c = mk(ob.p[0], ob.p[1], ob.p[2]);
+        if (ob.p[7] != 0.0f) c = c + time * (mk(ob.p[4], ob.p[5], ob.p[6]) - c);
+        V3 tc = c - o;
+        float d2 = len2(tc), r2 = ob.p[3] * ob.p[3];
+        if (d2 <= r2) d = unit(tc);
+        else {
+            float ctm = sqrtf(1.0f - r2 / d2);
+            float r1 = rng_f(rng), r2r = rng_f(rng);
+            float phi = 2.0f * PI * r1;
+            float ct = 1.0f - r2r * (1.0f - ctm);
+            float st = sqrtf(fmx(1.0f - ct * ct, 0.0f));
+            d = onb_local(onb_from_w(unit(tc)), mk(cosf(phi) * st, sinf(phi) * st, ct));
+        }
+    } else d = mk(1, 0, 0);
+    for (uint32_t k = 0; k < ob.xf_count; k++) {
+        const DXform& x = S.xforms[ob.xf_first + k];
+        if (x.kind == XF_ROTATE) d = rot_apply(x.v, d);
+    }
+    return d;
+}
+RT_D V3 lights_generate(const DScene& S, V3 origin, Rng& rng, float time) {
+    if (S.num_lights == 0) return mk(1, 0, 0);
+    uint32_t idx = rng_u32(rng) % S.num_lights;
+    return light_generate(S, S.lights[idx], origin, rng, time);
+}
+
+// ---- photon gather (kd_gather, src/photon.rs:195-228), iterative --------------------------
+RT_D V3 photon_irradiance(const DScene& S, V3 pos, V3 nrm) {
+    if (S.num_photons == 0) return mk(0, 0, 0);
+    uint32_t stk_lo[48], stk_n[48];
+    int sp = 0;
+    stk_lo[0] = 0; stk_n[0] = S.num_photons; sp = 1;
+    V3 acc = mk(0, 0, 0);
+    float r2 = S.ph_r2;
+    while (sp > 0) {
+        sp--;
+        uint32_t lo = stk_lo[sp], n = stk_n[sp];
+        if (n == 0) continue;
+        uint32_t mid = n / 2;
+        const float4* ph = S.photons + 3 * (size_t)(lo + mid);
+        float4 P = ph[0];
+        float dx = P.x - pos.x, dy = P.y - pos.y, dz = P.z - pos.z;
+        float d2 = dx * dx + dy * dy + dz * dz;
+        if (d2 < r2) {
+            float4 N = ph[2];
+            float dt = N.x * nrm.x + N.y * nrm.y + N.z * nrm.z;
+            if (dt > 0.0f) { float4 W = ph[1]; acc += mk(W.x, W.y, W.z) * (1.0f - d2 / r2); }
+        }
+        int axis = (int)P.w;
+        float sq = comp(pos, axis) - (axis == 0 ? P.x : (axis == 1 ? P.y : P.z));
+        uint32_t llo = lo, ln = mid, rlo = lo + mid + 1, rn = n - mid - 1;
+        // recursion order: near side first, then (maybe) the far side; a LIFO
+        // stack pushes the far side first
+        bool far_ok = sq * sq < r2;
+        if (sq <= 0.0f) {
+            if (far_ok && sp < 48) { stk_lo[sp] = rlo; stk_n[sp] = rn; sp++; }
+            if (sp < 48) { stk_lo[sp] = llo; stk_n[sp] = ln; sp++; }
+        } else {
+            if (far_ok && sp < 48) { stk_lo[sp] = llo; stk_n[sp] = ln; sp++; }
+            if (sp < 48) { stk_lo[sp] = rlo; stk_n[sp] = rn; sp++; }
+        }
+    }
+    return acc * S.ph_norm;
+}
+
+// ---- ray_color (src/renderer.rs:371-585) ------------------------------------------------------
+RT_D V3 ray_color(const DScene& S, Ray ray, int max_depth, float bg_scale, bool use_photons, Rng& rng, Stack& st) {
+    V3 thr = mk(1, 1, 1), color = mk(0, 0, 0);
+    bool prev_specular = true;
+    float prev_mis = 1.0f, prev_sun_w = 1.0f;
+    const bool lights_empty = S.num_lights == 0;
+    const bool sun = physical_sun(S);
+    const bool env = has_env(S);
+    for (int depth = 0; depth < max_depth; depth++) {
+        HitId id = trace_closest(S, ray, 0.001f, FINF, st);
+        if (id.slot < 0) {
+            float sp = sun_pdf_value(S, ray.d);
+            float ep = env_pdf_value(S, ray.d);
+            float ew = (sp + ep == 0.0f) ? 1.0f : (prev_specular ? prev_sun_w : prev_mis);
+            V3 bg = (depth == 0 || !env) ? bg_eval(S, ray.d) : env_eval(S, ray.d);
+            color += thr * bg * bg_scale * ew;
+            break;
+        }
+        Rec h = finalize(S, ray, id, 0.001f);
+        float ew = (prev_specular || lights_empty) ? 1.0f : prev_mis;
+        color += thr * mat_emitted(S, h) * ew;
+        Scatter sr;
+        if (!mat_scatter(S, h, ray, rng, sr)) break;
+        if (sr.skip_pdf) {
+            prev_sun_w = 1.0f;
+            if (sun) {
+                float p_sun = 1.0f / (2.0f * PI * (1.0f - COS_SUN_MAX));
+                V3 su = unit(mk(S.sun_dir[0], S.sun_dir[1], S.sun_dir[2]));
+                V3 ss = sample_sun_cone(su, COS_SUN_MAX, rng);
+                Ray shadow = make_ray(h.p, ss, ray.time);
+                float tr = trace_transmittance(S, shadow, 0.001f, FINF, st, false);
+                if (tr > 0.0f) {
+                    V3 bc = mat_spec_brdf_cos(S, h, ray, ss);
+                    if (bc.x > 0.0f || bc.y > 0.0f || bc.z > 0.0f) {
+                        float pm = mat_spec_pdf(S, h, ray, ss);
+                        V3 sc = bg_eval(S, ss);
+                        float md = p_sun * p_sun + pm * pm;
+                        if (md > 0.0f) color += thr * tr * bc * sc * p_sun / md;
+                    }
+                }
+                float pmb = mat_spec_pdf(S, h, ray, sr.ray.d);
+                if (pmb > 0.0f) prev_sun_w = (pmb * pmb) / fmx(pmb * pmb + p_sun * p_sun, 1e-8f);
+            }
+            thr *= sr.att;
+            ray = sr.ray;
+            prev_specular = true;
+            prev_mis = 1.0f;
+        } else {
+            if (use_photons) {
+                V3 irr = photon_irradiance(S, h.p, h.n);
+                if (irr.x > 0.0f || irr.y > 0.0f || irr.z > 0.0f) color += thr * mat_albedo_hint(S, h) * irr;
+            }
+            if (!lights_empty) {
+                V3 ldir = lights_generate(S, h.p, rng, ray.time);
+                Ray shadow = make_ray(h.p, ldir, ray.time);
+                Rec lrec;
+                if (lights_closest(S, shadow, 0.001f, FINF, lrec)) {
+                    float tr = trace_transmittance(S, shadow, 0.001f, lrec.t * (1.0f - 1e-4f), st, false);
+                    if (tr > 0.0f) {
+                        float l_pdf = lights_pdf_value(S, h.p, ldir, ray.time, st);
+                        float brdf = mat_scattering_pdf(h, shadow.d);
+                        float sp = sun_pdf_value(S, ldir);
+                        float ep = env_pdf_value(S, ldir);
+                        float md = l_pdf * l_pdf + brdf * brdf + sp * sp + ep * ep;
+                        if (md > 0.0f && brdf > 0.0f) color += thr * tr * sr.att * brdf * l_pdf * mat_emitted(S, lrec) / md;
+                    }
+                }
+            }
+            if (sun) {
+                V3 su = unit(mk(S.sun_dir[0], S.sun_dir[1], S.sun_dir[2]));
+                V3 ss = sample_sun_cone(su, COS_SUN_MAX, rng);
+                Ray shadow = make_ray(h.p, ss, ray.time);
+                float tr = trace_transmittance(S, shadow, 0.001f, FINF, st, false);
+                if (tr > 0.0f) {
+                    float brdf = mat_scattering_pdf(h, ss);
+                    if (brdf > 0.0f) {
+                        float spdf = 1.0f / (2.0f * PI * (1.0f - COS_SUN_MAX));
+                        V3 sc = bg_eval(S, ss);
+                        float ap = lights_empty ? 0.0f : lights_pdf_value(S, h.p, ss, ray.time, st);
+                        float md = spdf * spdf + brdf * brdf + ap * ap;
+                        color += thr * tr * sr.att * brdf * spdf * sc / md;
+                    }
+                }
+            }
+            if (env) {
+                float epdf;
+                V3 ed = env_sample_dir(S, rng, epdf);
+                if (epdf > 0.0f) {
+                    Ray shadow = make_ray(h.p, ed, ray.time);
+                    float tr = trace_transmittance(S, shadow, 0.001f, FINF, st, false);
+                    if (tr > 0.0f) {
+                        float brdf = mat_scattering_pdf(h, ed);
+                        if (brdf > 0.0f) {
+                            float ap = lights_empty ? 0.0f : lights_pdf_value(S, h.p, ed, ray.time, st);
+                            V3 ec = env_eval(S, ed);
+                            float md = epdf * epdf + brdf * brdf + ap * ap;
+                            color += thr * tr * sr.att * brdf * epdf * ec / md;
+                        }
+                    }
+                }
+            }
+            Onb uvw = onb_from_w(h.n);
+            float r1 = rng_f(rng), r2 = rng_f(rng);
+            float phi = 2.0f * PI * r1;
+            float sr2 = sqrtf(r2);
+            V3 ind = onb_local(uvw, mk(cosf(phi) * sr2, sinf(phi) * sr2, sqrtf(1.0f - r2)));
+            float pdf_val = fmx(dot(unit(ind), uvw.w) / PI, 0.0f);
+            if (pdf_val <= 0.0f) break;
+            Ray sc = scatter_from(h.p, ind, ray);
+            float scat_pdf = mat_scattering_pdf(h, sc.d);
+            if (scat_pdf <= 0.0f) break;
+            if (dot(ind, h.n) <= 0.0f) break;
+            float a = lights_empty ? 0.0f : lights_pdf_value(S, h.p, ind, ray.time, st);
+            float s = sun_pdf_value(S, ind);
+            float e = env_pdf_value(S, ind);
+            float nee = a * a + s * s + e * e;
+            prev_mis = (pdf_val * pdf_val) / fmx(pdf_val * pdf_val + nee, 1e-8f);
+            thr *= sr.att * (scat_pdf / pdf_val);
+            ray = sc;
+            prev_specular = false;
+        }
+        if (depth >= 2) {
+            float survive = fmx(fmx(thr.x, thr.y), thr.z);
+            float p = fmn(survive, 1.0f);
+            if (survive <= 0.0f || rng_f(rng) >= p) break;
+            thr /= p;
+        }
+    }
+    return color;
+}
+
+struct DCamera { float origin[3], lower_left[3], horizontal[3], vertical[3], u[3], v[3]; float lens_radius; uint32_t blades; };
+
+RT_D Ray camera_ray(const DCamera& c, float s, float t, Rng& rng) {
+    V3 rd;
+    if (c.blades >= 3) {
+        float n = (float)c.blades, rot = PI / n;
+        float k = (float)rng_range_u32(rng, 0, c.blades);
+        float a1 = 2.0f * PI * k / n + rot, a2 = 2.0f * PI * (k + 1.0f) / n + rot;
+        V3 p1 = mk(cosf(a1), sinf(a1), 0.0f), p2 = mk(cosf(a2), sinf(a2), 0.0f);
+        float r1 = rng_f(rng), r2 = rng_f(rng);
+        if (r1 + r2 > 1.0f) { r1 = 1.0f - r1; r2 = 1.0f - r2; }
+        rd = c.lens_radius * (p1 * r1 + p2 * r2);
+    } else rd = c.lens_radius * random_in_unit_disk(rng);
+    V3 cu = mk(c.u[0], c.u[1], c.u[2]), cv = mk(c.v[0], c.v[1], c.v[2]);
+    V3 off = cu * rd.x + cv * rd.y;
+    V3 org = mk(c.origin[0], c.origin[1], c.origin[2]);
+    V3 dir = mk(c.lower_left[0], c.lower_left[1], c.lower_left[2]) + s * mk(c.horizontal[0], c.horizontal[1], c.horizontal[2])
+             + t * mk(c.vertical[0], c.vertical[1], c.vertical[2]) - org - off;
+    return make_ray(org + off, dir, rng_f(rng));
+}
+
+struct DParams {
+    uint32_t sample_idx, strata, width, height, max_depth, use_frame_time, use_photons;
+    float bg_scale, t0, t1;
+    uint64_t seed;
+    unsigned long long* counters;   // counting pass: [0] node tests, [1] primitive tests, [2] rays, [3] samples
+};
+
+// Camera ray of one pixel sample: render_tiles body (src/renderer.rs:693-726).
+RT_D Ray pixel_ray(const DCamera& cam, const DParams& P, uint32_t row, uint32_t col, uint32_t sample_idx, Rng& rng) {
+    uint32_t i = row * P.width + col;
+    float w_denom = (float)max(P.width - 1, 1u), h_denom = (float)max(P.height - 1, 1u);
+    uint32_t s2 = P.strata * P.strata;
+    uint32_t ray_y = P.height - 1 - row;
+    float uj, vj;
+    if (s2 > 0) {
+        uint32_t off = (i * 0x9E3779B9u) % s2;
+        uint32_t s = (sample_idx + off) % s2;
+        uint32_t sx = s % P.strata, sy = s / P.strata;
+        float sf = (float)P.strata;
+        uj = ((float)sx + rng_f(rng)) / sf;
+        vj = ((float)sy + rng_f(rng)) / sf;
+    } else { uj = rng_f(rng); vj = rng_f(rng); }
+    float u = ((float)col + uj) / w_denom;
+    float v = ((float)ray_y + vj) / h_denom;
+    Ray r = camera_ray(cam, u, v, rng);
+    if (P.use_frame_time) r.time = P.t0 + rng_f(rng) * (P.t1 - P.t0);
+    r.wavelength = rng_range(rng, 380.0f, 700.0f);
+    return r;
+}
+
+}  // namespace rt

@@ -1,0 +1,900 @@
+// librtamd: the C ABI of include/rt_api.h on MI355X (gfx950).
+//
+// One translation unit: the HIP kernels of the radiance path (trace batches,
+// the plain per-pixel megakernel, photon tracing and gather, the E_ss table)
+// and the host code that owns contexts, uploads scenes and launches.
+#include "dev_shade.h"
+#include "bvh_build.h"
+#include "../../include/rt_api.h"
+
+#include <string>
+#include <vector>
+#include <cstring>
+#include <cstdio>
+#include <memory>
+#include <algorithm>
+#include <cmath>
+
+using namespace rt;
+
+#define BLOCK 128
+
+// ============================================================================ kernels
+struct LdsStack {
+    __device__ static int32_t* column() {
+        __shared__ int32_t s[RT_LDS_STACK * BLOCK];
+        return s + threadIdx.x;
+    }
+};
+
+__device__ __forceinline__ void init_stack(Stack& st) {
+    st.lds = LdsStack::column(); st.stride = BLOCK; st.sp = 0;
+    st.n_nodes = 0; st.n_prims = 0; st.n_rays = 0;
+}
+
+__device__ __forceinline__ Ray ray_from_c(const rt_ray& q) {
+    return make_ray(mk(q.origin[0], q.origin[1], q.origin[2]), mk(q.dir[0], q.dir[1], q.dir[2]), q.time);
+}
+
+__global__ __launch_bounds__(BLOCK) void k_trace_closest(DScene S, const rt_ray* __restrict__ rays, uint32_t n, rt_hit* __restrict__ hits) {
+    Stack st; init_stack(st);
+    uint32_t i = blockIdx.x * BLOCK + threadIdx.x;
+    if (i >= n) return;
+    rt_ray q = rays[i];
+    Ray r = ray_from_c(q);
+    HitId id = trace_closest(S, r, q.t_min, q.t_max, st);
+    rt_hit h;
+    memset(&h, 0, sizeof h);
+    if (id.slot >= 0) {
+        Rec rec = finalize(S, r, id, q.t_min);
+        h.t = rec.t; h.p[0] = rec.p.x; h.p[1] = rec.p.y; h.p[2] = rec.p.z;
+        h.n[0] = rec.n.x; h.n[1] = rec.n.y; h.n[2] = rec.n.z;
+        h.u = rec.u; h.v = rec.v; h.obj_id = rec.obj_id; h.prim_id = rec.prim_id; h.front_face = rec.front ? 1u : 0u;
+    } else { h.obj_id = -1; h.prim_id = -1; }
+    hits[i] = h;
+}
+
+__global__ __launch_bounds__(BLOCK) void k_trace_tr(DScene S, const rt_ray* __restrict__ rays, uint32_t n, float* __restrict__ tr, uint8_t* __restrict__ occ) {
+    Stack st; init_stack(st);
+    uint32_t i = blockIdx.x * BLOCK + threadIdx.x;
+    if (i >= n) return;
+    rt_ray q = rays[i];
+    Ray r = ray_from_c(q);
+    if (occ) occ[i] = trace_transmittance(S, r, q.t_min, q.t_max, st, true) <= 0.0f ? 1 : 0;
+    else tr[i] = trace_transmittance(S, r, q.t_min, q.t_max, st, false);
+}
+
+// Plain megakernel: one thread per pixel, `spp` consecutive sample passes,
+// radiance summed in registers and added to (or stored into) `out`.
+__global__ __launch_bounds__(BLOCK) void k_render(DScene S, DCamera cam, DParams P, const uint8_t* __restrict__ conv,
+                                                  float* __restrict__ out, uint32_t spp, int accumulate) {
+    Stack st; init_stack(st);
+    uint32_t i = blockIdx.x * BLOCK + threadIdx.x;
+    uint32_t npix = P.width * P.height;
+    if (i >= npix) return;
+    float sx = 0.0f, sy = 0.0f, sz = 0.0f;
+    if (!(conv && conv[i])) {
+        uint32_t row = i / P.width, col = i - row * P.width;
+        for (uint32_t k = 0; k < spp; k++) {
+            uint32_t sidx = P.sample_idx + k;
+            Rng rng = rng_seed(pixel_seed(i, sidx, P.seed));
+            Ray r = pixel_ray(cam, P, row, col, sidx, rng);
+            V3 c = ray_color(S, r, (int)P.max_depth, P.bg_scale, P.use_photons != 0, rng, st);
+            sx += c.x; sy += c.y; sz += c.z;
+        }
+    }
+    if (accumulate) { out[3 * i] += sx; out[3 * i + 1] += sy; out[3 * i + 2] += sz; }
+    else { out[3 * i] = sx; out[3 * i + 1] = sy; out[3 * i + 2] = sz; }
+    if (P.counters) {
+        atomicAdd(P.counters + 0, (unsigned long long)st.n_nodes);
+        atomicAdd(P.counters + 1, (unsigned long long)st.n_prims);
+        atomicAdd(P.counters + 2, (unsigned long long)st.n_rays);
+        atomicAdd(P.counters + 3, (unsigned long long)((conv && conv[i]) ? 0u : spp));
+    }
+}
+
+// E_ss table (src/material.rs:206-243): one thread per (mu, alpha) texel.
+__global__ void k_elut(float* out) {
+    uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= 64 * 64) return;
+    uint32_t i = t / 64, j = t % 64;
+    float mu = ((float)i + 0.5f) / 64.0f;
+    float alpha = fmaxf(((float)j + 0.5f) / 64.0f, 1e-3f);
+    Rng rng = rng_seed((uint64_t)(i * 64 + j) + 0x9E3779B9ULL);
+    V3 wo = mk(0.0f, 0.0f, mu);
+    float lo = smith_lambda(mu, alpha);
+    double sum = 0.0;
+    for (int k = 0; k < 64; k++)
+        for (int l = 0; l < 64; l++) {
+            float xi1 = ((float)k + rng_f(rng)) / 64.0f;
+            float xi2 = ((float)l + rng_f(rng)) / 64.0f;
+            V3 m = vndf_sample(wo, alpha, alpha, xi1, xi2);
+            float voh = fmaxf(dot(wo, m), 0.0f);
+            V3 wi = 2.0f * voh * m - wo;
+            if (wi.z <= 0.0f) continue;
+            float li = smith_lambda(wi.z, alpha);
+            sum += (double)((1.0f + lo) / (1.0f + lo + li));
+        }
+    out[t] = (float)(sum / 4096.0);
+}
+
+// Photon emission + tracing (PhotonMap::build_from_quad / build, src/photon.rs:41-135, 232-277).
+struct DEmitter {
+    uint32_t kind;
+    float o[3], u[3], v[3], n[3], ta[3], ba[3];
+    float power[3];
+    float lum_clamp;
+    float disk_c[3], disk_r, t[3], b[3], sun_down[3];
+};
+
+__global__ __launch_bounds__(BLOCK) void k_photon_trace(DScene S, DEmitter E, uint32_t n, float4* __restrict__ out, uint32_t* __restrict__ ok) {
+    Stack st; init_stack(st);
+    uint32_t i = blockIdx.x * BLOCK + threadIdx.x;
+    if (i >= n) return;
+    Rng rng = rng_seed(((uint64_t)i * 6364136223846793005ULL) ^ 0x9E3779B97F4A7C15ULL);
+    V3 pos, dir;
+    if (E.kind == 0) {
+        float s = rng_f(rng), t = rng_f(rng);
+        pos = mk(E.o[0], E.o[1], E.o[2]) + mk(E.u[0], E.u[1], E.u[2]) * s + mk(E.v[0], E.v[1], E.v[2]) * t;
+        float ct = sqrtf(rng_f(rng));
+        float sn = sqrtf(1.0f - ct * ct);
+        float phi = 2.0f * PI * rng_f(rng);
+        dir = sn * cosf(phi) * mk(E.ta[0], E.ta[1], E.ta[2]) + sn * sinf(phi) * mk(E.ba[0], E.ba[1], E.ba[2]) + ct * mk(E.n[0], E.n[1], E.n[2]);
+    } else {
+        float r = E.disk_r * sqrtf(rng_f(rng));
+        float phi = 2.0f * PI * rng_f(rng);
+        pos = mk(E.disk_c[0], E.disk_c[1], E.disk_c[2]) + mk(E.t[0], E.t[1], E.t[2]) * (r * cosf(phi)) + mk(E.b[0], E.b[1], E.b[2]) * (r * sinf(phi));
+        dir = mk(E.sun_down[0], E.sun_down[1], E.sun_down[2]);
+    }
+    V3 power = mk(E.power[0], E.power[1], E.power[2]);
+    uint32_t spec_depth = 0;
+    float wl = rng_range(rng, 380.0f, 700.0f);
+    uint32_t stored = 0;
+    for (int d = 0; d < 12; d++) {
+        Ray ray = make_ray(pos, dir, 0.0f);
+        ray.wavelength = wl;
+        HitId id = trace_closest(S, ray, 0.001f, FINF, st);
+        if (id.slot < 0) break;
+        Rec h = finalize(S, ray, id, 0.001f);
+        Scatter sr;
+        if (!mat_scatter(S, h, ray, rng, sr)) break;
+        if (sr.skip_pdf) {
+            power *= sr.att;
+            pos = sr.ray.o; dir = sr.ray.d;
+            spec_depth++;
+        } else {
+            if (spec_depth > 0 && mat_receives_caustics(S, h)) {
+                float l = lum(power);
+                if (l > E.lum_clamp) power *= E.lum_clamp / l;
+                out[3 * (size_t)i] = make_float4(h.p.x, h.p.y, h.p.z, 0.0f);
+                out[3 * (size_t)i + 1] = make_float4(power.x, power.y, power.z, 0.0f);
+                out[3 * (size_t)i + 2] = make_float4(h.n.x, h.n.y, h.n.z, 0.0f);
+                stored = 1;
+            }
+            break;
+        }
+    }
+    ok[i] = stored;
+}
+
+__global__ __launch_bounds__(BLOCK) void k_photon_irr(DScene S, const float* __restrict__ pn, uint32_t n, float* __restrict__ out) {
+    uint32_t i = blockIdx.x * BLOCK + threadIdx.x;
+    if (i >= n) return;
+    V3 c = photon_irradiance(S, mk(pn[6 * i], pn[6 * i + 1], pn[6 * i + 2]), mk(pn[6 * i + 3], pn[6 * i + 4], pn[6 * i + 5]));
+    out[3 * i] = c.x; out[3 * i + 1] = c.y; out[3 * i + 2] = c.z;
+}
+
+// ============================================================================ host side
+namespace {
+
+struct DevBuf {
+    void* p = nullptr;
+    size_t bytes = 0;
+    ~DevBuf() { if (p) (void)hipFree(p); }
+};
+
+}  // namespace
+
+struct rt_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::string err;
+    float* elut = nullptr;
+    float norms[3], fire_norms[3];
+    // measurement
+    bool timing = false;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    double kernel_ms = 0.0;
+    uint64_t launches = 0;
+    unsigned long long* counters = nullptr;   // non-null: counting pass
+};
+
+struct rt_scene {
+    rt_ctx* ctx = nullptr;
+    DScene S;
+    std::vector<void*> allocs;
+    uint64_t stats[8] = {0};
+    std::vector<float4> photons_host;
+    ~rt_scene() { for (void* p : allocs) (void)hipFree(p); }
+};
+
+static int fail(rt_ctx* c, int code, const std::string& msg) { if (c) c->err = msg; return code; }
+#define HIPCHECK(ctx, call) do { hipError_t e_ = (call); if (e_ != hipSuccess) return fail(ctx, RT_ERR_DEVICE, std::string(#call) + ": " + hipGetErrorString(e_)); } while (0)
+
+static bool is_device_ptr(const void* p) {
+    if (!p) return false;
+    hipPointerAttribute_t a;
+    if (hipPointerGetAttributes(&a, p) != hipSuccess) { (void)hipGetLastError(); return false; }
+    return a.type == hipMemoryTypeDevice || a.type == hipMemoryTypeManaged;
+}
+
+template <class T>
+static int upload(rt_scene* s, const T* host, size_t count, const T** out) {
+    *out = nullptr;
+    if (count == 0) return 0;
+    void* d = nullptr;
+    hipError_t e = hipMalloc(&d, count * sizeof(T));
+    if (e != hipSuccess) return fail(s->ctx, RT_ERR_NOMEM, std::string("hipMalloc: ") + hipGetErrorString(e));
+    s->allocs.push_back(d);
+    e = hipMemcpy(d, host, count * sizeof(T), hipMemcpyHostToDevice);
+    if (e != hipSuccess) return fail(s->ctx, RT_ERR_DEVICE, std::string("hipMemcpy: ") + hipGetErrorString(e));
+    *out = (const T*)d;
+    return 0;
+}
+
+// Staging for caller buffers that may be host or device memory.
+struct Staged {
+    void* dev = nullptr;
+    bool owned = false;
+    ~Staged() { if (owned && dev) (void)hipFree(dev); }
+};
+static int stage_in(rt_ctx* c, const void* p, size_t bytes, Staged& s) {
+    if (!p || bytes == 0) { s.dev = nullptr; return 0; }
+    if (is_device_ptr(p)) { s.dev = const_cast<void*>(p); return 0; }
+    HIPCHECK(c, hipMalloc(&s.dev, bytes));
+    s.owned = true;
+    HIPCHECK(c, hipMemcpy(s.dev, p, bytes, hipMemcpyHostToDevice));
+    return 0;
+}
+static int stage_out(rt_ctx* c, void* p, size_t bytes, Staged& s, bool copy_in) {
+    if (is_device_ptr(p)) { s.dev = p; return 0; }
+    HIPCHECK(c, hipMalloc(&s.dev, bytes));
+    s.owned = true;
+    if (copy_in) HIPCHECK(c, hipMemcpy(s.dev, p, bytes, hipMemcpyHostToDevice));
+    return 0;
+}
+static int finish_out(rt_ctx* c, void* p, size_t bytes, Staged& s) {
+    HIPCHECK(c, hipStreamSynchronize(c->stream));
+    if (s.owned) HIPCHECK(c, hipMemcpy(p, s.dev, bytes, hipMemcpyDeviceToHost));
+    return 0;
+}
+
+// ---- spectral normalisation (src/spectrum.rs:80-95, 162-178), host f32 ----------
+static void spectral_norms(float norms[3], float fire[3]) {
+    float s[3] = {0, 0, 0}, f[3] = {0, 0, 0};
+    for (int i = 0; i < RT_CIE_N; i++) {
+        float rgb[3]; xyz_srgb(RT_CIE_X[i], RT_CIE_Y[i], RT_CIE_Z[i], rgb);
+        s[0] += rgb[0]; s[1] += rgb[1]; s[2] += rgb[2];
+        float c[3]; fire_color(RT_CIE_X[i], RT_CIE_Y[i], RT_CIE_Z[i], c);
+        f[0] += c[0]; f[1] += c[1]; f[2] += c[2];
+    }
+    for (int k = 0; k < 3; k++) { norms[k] = (float)RT_CIE_N / std::fmax(s[k], 1e-6f); fire[k] = (float)RT_CIE_N / std::fmax(f[k], 1e-6f); }
+}
+
+extern "C" {
+
+int rt_api_version(void) { return RT_API_VERSION; }
+
+const char* rt_last_error(rt_ctx* ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+
+int rt_ctx_create(int device, rt_ctx** out) {
+    if (!out) return RT_ERR_INVALID;
+    *out = nullptr;
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n == 0) { (void)hipGetLastError(); return RT_ERR_DEVICE; }
+    if (device < 0 || device >= n) return RT_ERR_INVALID;
+    std::unique_ptr<rt_ctx> c(new rt_ctx());
+    c->device = device;
+    HIPCHECK(c.get(), hipSetDevice(device));
+    HIPCHECK(c.get(), hipStreamCreate(&c->stream));
+    HIPCHECK(c.get(), hipMemcpyToSymbol(HIP_SYMBOL(c_cie_x), RT_CIE_X, sizeof(RT_CIE_X)));
+    HIPCHECK(c.get(), hipMemcpyToSymbol(HIP_SYMBOL(c_cie_y), RT_CIE_Y, sizeof(RT_CIE_Y)));
+    HIPCHECK(c.get(), hipMemcpyToSymbol(HIP_SYMBOL(c_cie_z), RT_CIE_Z, sizeof(RT_CIE_Z)));
+    float metal[6][RT_METAL_N];
+    memcpy(metal[0], RT_GOLD_N, sizeof RT_GOLD_N); memcpy(metal[1], RT_GOLD_K, sizeof RT_GOLD_K);
+    memcpy(metal[2], RT_COPPER_N, sizeof RT_COPPER_N); memcpy(metal[3], RT_COPPER_K, sizeof RT_COPPER_K);
+    memcpy(metal[4], RT_SILVER_N, sizeof RT_SILVER_N); memcpy(metal[5], RT_SILVER_K, sizeof RT_SILVER_K);
+    HIPCHECK(c.get(), hipMemcpyToSymbol(HIP_SYMBOL(c_metal), metal, sizeof metal));
+    HIPCHECK(c.get(), hipMalloc(&c->elut, 64 * 64 * sizeof(float)));
+    hipLaunchKernelGGL(k_elut, dim3(64 * 64 / 64), dim3(64), 0, c->stream, c->elut);
+    HIPCHECK(c.get(), hipGetLastError());
+    HIPCHECK(c.get(), hipStreamSynchronize(c->stream));
+    spectral_norms(c->norms, c->fire_norms);
+    *out = c.release();
+    return RT_OK;
+}
+
+void rt_ctx_destroy(rt_ctx* ctx) {
+    if (!ctx) return;
+    (void)hipSetDevice(ctx->device);
+    if (ctx->elut) (void)hipFree(ctx->elut);
+    if (ctx->counters) (void)hipFree(ctx->counters);
+    if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
+    if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
+    if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+}
+
+}  // extern "C"
+
+// ---- scene flattening ------------------------------------------------------------------
+namespace {
+
+using rtb::Box;
+
+Box sphere_box(const float* p) {
+    Box b;
+    for (int k = 0; k < 3; k++) { b.mn[k] = p[k] - p[3]; b.mx[k] = p[k] + p[3]; }
+    if (p[7] != 0.0f) {
+        Box b1;
+        for (int k = 0; k < 3; k++) { b1.mn[k] = p[4 + k] - p[3]; b1.mx[k] = p[4 + k] + p[3]; }
+        b = rtb::box_union(b, b1);
+    }
+    return b;
+}
+
+DQuad make_quad(const float q[3], const float u[3], const float v[3]) {
+    DQuad Q;
+    memset(&Q, 0, sizeof Q);
+    for (int k = 0; k < 3; k++) { Q.q[k] = q[k]; Q.u[k] = u[k]; Q.v[k] = v[k]; }
+    V3 U = mk(u[0], u[1], u[2]), Vv = mk(v[0], v[1], v[2]), Qp = mk(q[0], q[1], q[2]);
+    V3 n = cross(U, Vv);
+    float area = std::sqrt(len2(n));
+    V3 nn = n / area;
+    float d = dot(nn, Qp);
+    V3 w = n / dot(n, n);
+    Q.n[0] = nn.x; Q.n[1] = nn.y; Q.n[2] = nn.z;
+    Q.w[0] = w.x; Q.w[1] = w.y; Q.w[2] = w.z;
+    Q.d = d; Q.area = area;
+    return Q;
+}
+
+Box quad_box(const DQuad& Q) {
+    float c[4][3];
+    for (int k = 0; k < 3; k++) {
+        c[0][k] = Q.q[k]; c[1][k] = Q.q[k] + Q.u[k]; c[2][k] = Q.q[k] + Q.v[k]; c[3][k] = (Q.q[k] + Q.u[k]) + Q.v[k];
+    }
+    Box b;
+    for (int k = 0; k < 3; k++) { b.mn[k] = c[0][k]; b.mx[k] = c[0][k]; }
+    for (int i = 1; i < 4; i++)
+        for (int k = 0; k < 3; k++) { b.mn[k] = std::fmin(b.mn[k], c[i][k]); b.mx[k] = std::fmax(b.mx[k], c[i][k]); }
+    return rtb::box_pad(b);
+}
+
+struct Flat {
+    std::vector<rtb::Node> nodes;
+    std::vector<DObject> objs_input;     // input order
+    std::vector<Box> obj_boxes;
+    std::vector<uint32_t> obj_stack;     // nested traversal extra stack
+    std::vector<DObject> lights;
+    std::vector<DXform> xforms;
+    std::vector<float4> tris;
+    std::vector<int32_t> tri_tag;
+    std::vector<float> tri_n, tri_t, tri_uv;
+    std::vector<DQuad> quads;
+    std::vector<float4> planes;
+    std::vector<DMedium> media;
+    std::vector<float> grids;
+    std::string err;
+};
+
+bool flatten_object(Flat& F, const rt_scene_desc& d, const rt_object& o, DObject& D, Box& bb, uint32_t& extra, bool is_light) {
+    memset(&D, 0, sizeof D);
+    D.shape = o.shape;
+    D.material = o.material;
+    for (int k = 0; k < 16; k++) D.p[k] = o.p[k];
+    extra = 0;
+    if (o.medium == 0 && o.material >= d.num_materials) { F.err = "material index out of range"; return false; }
+    switch (o.shape) {
+        case RT_SHAPE_SPHERE:
+            bb = sphere_box(o.p);
+            break;
+        case RT_SHAPE_QUAD: {
+            D.data_first = (uint32_t)F.quads.size();
+            DQuad Q = make_quad(o.p, o.p + 3, o.p + 6);
+            F.quads.push_back(Q);
+            bb = quad_box(Q);
+            break;
+        }
+        case RT_SHAPE_BOX: {
+            if (is_light) { F.err = "box lights are not supported"; return false; }
+            float mn[3], mx[3];
+            for (int k = 0; k < 3; k++) { mn[k] = std::fmin(o.p[k], o.p[3 + k]); mx[k] = std::fmax(o.p[k], o.p[3 + k]); }
+            float dx[3] = {mx[0] - mn[0], 0, 0}, dy[3] = {0, mx[1] - mn[1], 0}, dz[3] = {0, 0, mx[2] - mn[2]};
+            float ndx[3] = {-dx[0], -0.0f, -0.0f}, ndz[3] = {-0.0f, -0.0f, -dz[2]};
+            float c0[3] = {mn[0], mn[1], mx[2]}, c1[3] = {mx[0], mn[1], mx[2]}, c2[3] = {mx[0], mn[1], mn[2]};
+            float c3[3] = {mn[0], mn[1], mn[2]}, c4[3] = {mn[0], mx[1], mx[2]}, c5[3] = {mn[0], mn[1], mn[2]};
+            DQuad side[6] = {make_quad(c0, dx, dy), make_quad(c1, ndz, dy), make_quad(c2, ndx, dy),
+                             make_quad(c3, dz, dy), make_quad(c4, dx, ndz), make_quad(c5, dx, dz)};
+            std::vector<Box> boxes(6);
+            for (int k = 0; k < 6; k++) { side[k].pad[0] = (float)k; boxes[k] = quad_box(side[k]); }
+            std::vector<uint32_t> leaf;
+            D.root = (uint32_t)rtb::Builder::build_tree(boxes, F.nodes, leaf, &bb);
+            D.data_first = (uint32_t)F.quads.size();
+            D.data_count = 6;
+            for (uint32_t k : leaf) F.quads.push_back(side[k]);
+            extra = std::max(1u, rtb::Builder::stack_bound(F.nodes, (int32_t)D.root));
+            break;
+        }
+        case RT_SHAPE_MESH: {
+            if (is_light) { F.err = "mesh lights are not supported"; return false; }
+            if (!d.tri_v || o.data_count == 0 || (uint64_t)o.data_first + o.data_count > d.num_tris) { F.err = "mesh triangle range"; return false; }
+            std::vector<Box> boxes(o.data_count);
+            for (uint32_t k = 0; k < o.data_count; k++) {
+                const float* v = d.tri_v + ((size_t)o.data_first + k) * 9;
+                Box b;
+                for (int a = 0; a < 3; a++) {
+                    b.mn[a] = std::fmin(std::fmin(v[a], v[3 + a]), v[6 + a]);
+                    b.mx[a] = std::fmax(std::fmax(v[a], v[3 + a]), v[6 + a]);
+                }
+                boxes[k] = rtb::box_pad(b);
+            }
+            std::vector<uint32_t> leaf;
+            leaf.reserve(o.data_count);
+            D.root = (uint32_t)rtb::Builder::build_tree(boxes, F.nodes, leaf, &bb);
+            D.data_first = (uint32_t)F.tri_tag.size();
+            D.data_count = o.data_count;
+            for (uint32_t k : leaf) {
+                size_t t = (size_t)o.data_first + k;
+                const float* v = d.tri_v + t * 9;
+                F.tris.push_back(make_float4(v[0], v[1], v[2], 0.0f));
+                F.tris.push_back(make_float4(v[3] - v[0], v[4] - v[1], v[5] - v[2], 0.0f));
+                F.tris.push_back(make_float4(v[6] - v[0], v[7] - v[1], v[8] - v[2], 0.0f));
+                F.tri_tag.push_back((int32_t)k);
+                if (d.tri_n) F.tri_n.insert(F.tri_n.end(), d.tri_n + t * 9, d.tri_n + t * 9 + 9);
+                if (d.tri_t) F.tri_t.insert(F.tri_t.end(), d.tri_t + t * 9, d.tri_t + t * 9 + 9);
+                if (d.tri_uv) F.tri_uv.insert(F.tri_uv.end(), d.tri_uv + t * 6, d.tri_uv + t * 6 + 6);
+            }
+            extra = std::max(1u, rtb::Builder::stack_bound(F.nodes, (int32_t)D.root));
+            break;
+        }
+        case RT_SHAPE_CONVEX: {
+            if ((uint64_t)(o.data_first + o.data_count) * 4 > d.pool_len || !d.pool) { F.err = "convex plane range"; return false; }
+            D.data_first = (uint32_t)F.planes.size();
+            D.data_count = o.data_count;
+            for (uint32_t k = 0; k < o.data_count; k++) {
+                const float* q = d.pool + ((size_t)o.data_first + k) * 4;
+                F.planes.push_back(make_float4(q[0], q[1], q[2], q[3]));
+            }
+            for (int k = 0; k < 3; k++) { bb.mn[k] = o.p[7 + k]; bb.mx[k] = o.p[10 + k]; }
+            break;
+        }
+        default:
+            F.err = "unsupported shape kind " + std::to_string(o.shape);
+            return false;
+    }
+    if (o.medium) {
+        if (is_light) { F.err = "media cannot be lights"; return false; }
+        if (o.medium > d.num_media || !d.media) { F.err = "medium index"; return false; }
+        if (o.shape != RT_SHAPE_SPHERE && o.shape != RT_SHAPE_BOX) { F.err = "medium boundary must be a sphere or a box"; return false; }
+        const rt_medium& m = d.media[o.medium - 1];
+        DMedium M;
+        memset(&M, 0, sizeof M);
+        M.kind = m.kind; M.g = m.g;
+        for (int k = 0; k < 3; k++) { M.albedo[k] = m.albedo[k]; M.bmin[k] = bb.mn[k]; M.bmax[k] = bb.mx[k]; }
+        if (m.kind == RT_MEDIUM_CONSTANT) M.density = m.density;
+        else if (m.kind == RT_MEDIUM_GRID) {
+            if (!m.grid || !m.grid_n[0] || !m.grid_n[1] || !m.grid_n[2]) { F.err = "grid medium without grid"; return false; }
+            M.density = std::fmax(m.density, 1.1920929e-7f);
+            M.nx = m.grid_n[0]; M.ny = m.grid_n[1]; M.nz = m.grid_n[2];
+            M.grid_off = (uint32_t)F.grids.size();
+            F.grids.insert(F.grids.end(), m.grid, m.grid + (size_t)M.nx * M.ny * M.nz);
+        } else { F.err = "medium kind"; return false; }
+        F.media.push_back(M);
+        D.medium = (uint32_t)F.media.size();
+    }
+    D.xf_first = (uint32_t)F.xforms.size();
+    D.xf_count = o.xform_count;
+    for (uint32_t k = 0; k < o.xform_count; k++) {
+        if (!d.xforms || o.xform_first + k >= d.num_xforms) { F.err = "xform range"; return false; }
+        const rt_xform& x = d.xforms[o.xform_first + k];
+        DXform X;
+        X.kind = x.kind;
+        for (int j = 0; j < 9; j++) X.v[j] = x.v[j];
+        if (x.kind == RT_XFORM_TRANSLATE) {
+            for (int a = 0; a < 3; a++) { bb.mn[a] = bb.mn[a] + x.v[a]; bb.mx[a] = bb.mx[a] + x.v[a]; }
+        } else if (x.kind == RT_XFORM_ROTATE) {
+            Box r;
+            for (int a = 0; a < 3; a++) { r.mn[a] = FINF; r.mx[a] = -FINF; }
+            for (int i = 0; i < 2; i++) for (int j = 0; j < 2; j++) for (int k = 0; k < 2; k++) {
+                V3 c = mk(i == 0 ? bb.mn[0] : bb.mx[0], j == 0 ? bb.mn[1] : bb.mx[1], k == 0 ? bb.mn[2] : bb.mx[2]);
+                V3 q = rot_apply(x.v, c);
+                r.mn[0] = std::fmin(r.mn[0], q.x); r.mx[0] = std::fmax(r.mx[0], q.x);
+                r.mn[1] = std::fmin(r.mn[1], q.y); r.mx[1] = std::fmax(r.mx[1], q.y);
+                r.mn[2] = std::fmin(r.mn[2], q.z); r.mx[2] = std::fmax(r.mx[2], q.z);
+            }
+            bb = r;
+        } else if (x.kind == RT_XFORM_SCALE) {
+            if (!(x.v[0] > 0.0f)) { F.err = "UniformScale factor must be positive"; return false; }
+            for (int a = 0; a < 3; a++) { bb.mn[a] = x.v[0] * bb.mn[a]; bb.mx[a] = x.v[0] * bb.mx[a]; }
+        } else { F.err = "xform kind"; return false; }
+        F.xforms.push_back(X);
+    }
+    return true;
+}
+
+}  // namespace
+
+template <class T>
+static int upload_vec(rt_scene* s, const std::vector<T>& v, const T** out) { return upload(s, v.data(), v.size(), out); }
+
+extern "C" int rt_scene_upload(rt_ctx* ctx, const rt_scene_desc* d, rt_scene** out) {
+    if (!ctx || !d || !out) return RT_ERR_INVALID;
+    *out = nullptr;
+    if (d->num_objects == 0 || !d->objects) return fail(ctx, RT_ERR_INVALID, "empty world");
+    HIPCHECK(ctx, hipSetDevice(ctx->device));
+    Flat F;
+    for (uint32_t i = 0; i < d->num_materials; i++) {
+        uint32_t k = d->materials[i].kind;
+        if (k < RT_MAT_LAMBERTIAN || k > RT_MAT_ROUGH_DIELECTRIC) return fail(ctx, RT_ERR_UNSUPPORTED, "unsupported material kind " + std::to_string(k));
+    }
+    F.objs_input.resize(d->num_objects);
+    F.obj_boxes.resize(d->num_objects);
+    F.obj_stack.resize(d->num_objects);
+    for (uint32_t i = 0; i < d->num_objects; i++) {
+        if (!flatten_object(F, *d, d->objects[i], F.objs_input[i], F.obj_boxes[i], F.obj_stack[i], false))
+            return fail(ctx, F.err.find("unsupported") != std::string::npos || F.err.find("not supported") != std::string::npos ? RT_ERR_UNSUPPORTED : RT_ERR_INVALID, F.err);
+        F.objs_input[i].tag = i;
+    }
+    std::vector<uint32_t> leaf;
+    rtb::Box wb;
+    size_t world_first = F.nodes.size();
+    int32_t root = rtb::Builder::build_tree(F.obj_boxes, F.nodes, leaf, &wb);
+    // the kernels start the world traversal at node 0: move the world tree there
+    if (world_first != 0 || root != 0) {
+        // renumber: world nodes are appended after nested trees; swap blocks so
+        // the world tree starts at index 0 and fix child indices.
+        size_t nested = world_first;
+        size_t world_n = F.nodes.size() - world_first;
+        std::vector<rtb::Node> re;
+        re.reserve(F.nodes.size());
+        auto remap = [&](int32_t c, bool in_world) -> int32_t {
+            if (c < 0) return c;
+            if (in_world) return c - (int32_t)nested;
+            return c + (int32_t)world_n;
+        };
+        for (size_t k = world_first; k < F.nodes.size(); k++) {
+            rtb::Node n = F.nodes[k];
+            for (int c = 0; c < 8; c++) n.child[c] = remap(n.child[c], true);
+            re.push_back(n);
+        }
+        for (size_t k = 0; k < nested; k++) {
+            rtb::Node n = F.nodes[k];
+            for (int c = 0; c < 8; c++) n.child[c] = remap(n.child[c], false);
+            re.push_back(n);
+        }
+        F.nodes.swap(re);
+        for (auto& o : F.objs_input) if (o.shape == RT_SHAPE_MESH || o.shape == RT_SHAPE_BOX) o.root += (uint32_t)world_n;
+        root = remap(root, true);
+        if (root != 0) return fail(ctx, RT_ERR_INVALID, "world root is not the first node");
+    }
+    std::vector<DObject> objs_leaf;
+    std::vector<uint32_t> leaf_extra;
+    for (uint32_t k : leaf) { objs_leaf.push_back(F.objs_input[k]); leaf_extra.push_back(F.obj_stack[k]); }
+    uint32_t bound = rtb::Builder::stack_bound(F.nodes, 0, &leaf_extra);
+    bound = std::max(bound, 1u);
+    if (bound > RT_LDS_STACK + RT_PRIV_STACK)
+        return fail(ctx, RT_ERR_UNSUPPORTED, "BVH needs a traversal stack of " + std::to_string(bound) + " entries");
+    for (uint32_t i = 0; i < d->num_lights; i++) {
+        DObject L; rtb::Box lb; uint32_t ex;
+        if (!flatten_object(F, *d, d->lights[i], L, lb, ex, true)) return fail(ctx, RT_ERR_UNSUPPORTED, F.err);
+        L.tag = i;
+        F.lights.push_back(L);
+    }
+    std::unique_ptr<rt_scene> s(new rt_scene());
+    s->ctx = ctx;
+    DScene& S = s->S;
+    memset(&S, 0, sizeof S);
+    int rc = 0;
+    const float4* nodes = nullptr;
+    static_assert(sizeof(rtb::Node) == 16 * sizeof(float4), "node layout");
+    if ((rc = upload(s.get(), reinterpret_cast<const float4*>(F.nodes.data()), F.nodes.size() * 16, &nodes))) return rc;
+    S.nodes = nodes;
+    if ((rc = upload_vec(s.get(), objs_leaf, &S.objs))) return rc;
+    S.num_objs = (uint32_t)objs_leaf.size();
+    if ((rc = upload_vec(s.get(), F.lights, &S.lights))) return rc;
+    S.num_lights = (uint32_t)F.lights.size();
+    if ((rc = upload_vec(s.get(), F.xforms, &S.xforms))) return rc;
+    if ((rc = upload_vec(s.get(), F.tris, &S.tris))) return rc;
+    if ((rc = upload_vec(s.get(), F.tri_tag, &S.tri_tag))) return rc;
+    if ((rc = upload_vec(s.get(), F.tri_n, &S.tri_n))) return rc;
+    if ((rc = upload_vec(s.get(), F.tri_t, &S.tri_t))) return rc;
+    if ((rc = upload_vec(s.get(), F.tri_uv, &S.tri_uv))) return rc;
+    if ((rc = upload_vec(s.get(), F.quads, &S.quads))) return rc;
+    if ((rc = upload_vec(s.get(), F.planes, &S.planes))) return rc;
+    if ((rc = upload_vec(s.get(), F.media, &S.media))) return rc;
+    if ((rc = upload_vec(s.get(), F.grids, &S.grids))) return rc;
+    std::vector<DMaterial> mats(d->num_materials);
+    for (uint32_t i = 0; i < d->num_materials; i++) {
+        mats[i].kind = d->materials[i].kind; mats[i].tex = d->materials[i].tex;
+        for (int k = 0; k < 24; k++) mats[i].p[k] = d->materials[i].p[k];
+    }
+    if ((rc = upload_vec(s.get(), mats, &S.mats))) return rc;
+    // background
+    const rt_background& b = d->background;
+    S.bg_kind = b.kind; S.bg_visible = b.visible_kind;
+    for (int k = 0; k < 3; k++) {
+        S.bg_color[k] = b.color[k]; S.bg_top[k] = b.top[k]; S.bg_bottom[k] = b.bottom[k]; S.sun_dir[k] = b.sun_dir[k];
+    }
+    S.turbidity = b.turbidity;
+    if (b.kind == RT_BG_ENVMAP || b.kind == RT_BG_ENVMAP_SPLIT) {
+        if (!b.env_rgb || !b.env_w || !b.env_h) return fail(ctx, RT_ERR_INVALID, "env map missing");
+        uint32_t w = b.env_w, h = b.env_h;
+        // EnvMapData::new (src/renderer.rs:32-74) in f32
+        std::vector<float> cond((size_t)w * h), row(h), marg(h);
+        for (uint32_t j = 0; j < h; j++) {
+            float sj = std::sin(PI * ((float)j + 0.5f) / (float)h);
+            float sum = 0.0f;
+            for (uint32_t i = 0; i < w; i++) {
+                const float* p = b.env_rgb + ((size_t)j * w + i) * 3;
+                sum += lum(mk(p[0], p[1], p[2])) * sj;
+                cond[(size_t)j * w + i] = sum;
+            }
+            if (sum > 0.0f) for (uint32_t i = 0; i < w; i++) cond[(size_t)j * w + i] /= sum;
+            else for (uint32_t i = 0; i < w; i++) cond[(size_t)j * w + i] = (float)(i + 1) / (float)w;
+            row[j] = sum;
+        }
+        float total = 0.0f;
+        for (uint32_t j = 0; j < h; j++) total += row[j];
+        float cum = 0.0f;
+        for (uint32_t j = 0; j < h; j++) { cum += row[j]; marg[j] = total > 0.0f ? cum / total : (float)(j + 1) / (float)h; }
+        S.env_w = w; S.env_h = h; S.env_rot = b.env_rot_y; S.env_inv_total = total > 0.0f ? 1.0f / total : 0.0f;
+        if ((rc = upload(s.get(), b.env_rgb, (size_t)w * h * 3, &S.env_rgb))) return rc;
+        if ((rc = upload_vec(s.get(), cond, &S.env_cond))) return rc;
+        if ((rc = upload_vec(s.get(), marg, &S.env_marg))) return rc;
+    }
+    S.elut = ctx->elut;
+    for (int k = 0; k < 3; k++) { S.norms[k] = ctx->norms[k]; S.fire_norms[k] = ctx->fire_norms[k]; }
+    S.stack_bound = bound;
+    size_t node_count = F.nodes.size();
+    s->stats[0] = node_count;                 // nodes
+    s->stats[1] = F.tri_tag.size();           // triangles
+    s->stats[2] = node_count * 256;           // node bytes
+    s->stats[3] = bound;                      // stack bound
+    s->stats[4] = d->num_objects;
+    s->stats[5] = F.tris.size() * sizeof(float4);
+    *out = s.release();
+    return RT_OK;
+}
+
+extern "C" {
+
+void rt_scene_destroy(rt_scene* s) {
+    if (!s) return;
+    (void)hipSetDevice(s->ctx->device);
+    delete s;
+}
+
+int rt_scene_stats(rt_scene* s, uint64_t stats[8]) {
+    if (!s || !stats) return RT_ERR_INVALID;
+    memcpy(stats, s->stats, sizeof s->stats);
+    return RT_OK;
+}
+
+static int trace_common(rt_ctx* ctx, rt_scene* s, const rt_ray* rays, uint32_t n, void* out, int mode) {
+    if (!ctx || !s || (!rays && n) || (!out && n)) return RT_ERR_INVALID;
+    if (n == 0) return RT_OK;
+    HIPCHECK(ctx, hipSetDevice(ctx->device));
+    Staged rin, rout;
+    int rc;
+    if ((rc = stage_in(ctx, rays, (size_t)n * sizeof(rt_ray), rin))) return rc;
+    size_t ob = mode == 0 ? sizeof(rt_hit) : (mode == 1 ? 1 : sizeof(float));
+    if ((rc = stage_out(ctx, out, (size_t)n * ob, rout, false))) return rc;
+    dim3 grid((n + BLOCK - 1) / BLOCK);
+    if (mode == 0) hipLaunchKernelGGL(k_trace_closest, grid, dim3(BLOCK), 0, ctx->stream, s->S, (const rt_ray*)rin.dev, n, (rt_hit*)rout.dev);
+    else if (mode == 1) hipLaunchKernelGGL(k_trace_tr, grid, dim3(BLOCK), 0, ctx->stream, s->S, (const rt_ray*)rin.dev, n, (float*)nullptr, (uint8_t*)rout.dev);
+    else hipLaunchKernelGGL(k_trace_tr, grid, dim3(BLOCK), 0, ctx->stream, s->S, (const rt_ray*)rin.dev, n, (float*)rout.dev, (uint8_t*)nullptr);
+    HIPCHECK(ctx, hipGetLastError());
+    return finish_out(ctx, out, (size_t)n * ob, rout);
+}
+
+int rt_trace_closest(rt_ctx* ctx, rt_scene* s, const rt_ray* rays, uint32_t n, rt_hit* hits) { return trace_common(ctx, s, rays, n, hits, 0); }
+int rt_trace_any(rt_ctx* ctx, rt_scene* s, const rt_ray* rays, uint32_t n, uint8_t* occ) { return trace_common(ctx, s, rays, n, occ, 1); }
+int rt_transmittance(rt_ctx* ctx, rt_scene* s, const rt_ray* rays, uint32_t n, float* tr) { return trace_common(ctx, s, rays, n, tr, 2); }
+
+static void to_dev(const rt_camera* c, const rt_sample_params* p, DCamera& C, DParams& P) {
+    for (int k = 0; k < 3; k++) {
+        C.origin[k] = c->origin[k]; C.lower_left[k] = c->lower_left[k]; C.horizontal[k] = c->horizontal[k];
+        C.vertical[k] = c->vertical[k]; C.u[k] = c->u[k]; C.v[k] = c->v[k];
+    }
+    C.lens_radius = c->lens_radius; C.blades = c->aperture_blades;
+    P.sample_idx = p->sample_idx; P.strata = p->strata; P.width = p->width; P.height = p->height;
+    P.max_depth = p->max_depth; P.use_frame_time = p->use_frame_time; P.use_photons = p->use_photons;
+    P.bg_scale = p->bg_scale; P.t0 = p->frame_t0; P.t1 = p->frame_t1; P.seed = p->seed;
+}
+
+static int render_common(rt_ctx* ctx, rt_scene* s, const rt_camera* cam, const rt_sample_params* p, const uint8_t* conv,
+                         float* out, uint32_t spp, bool accumulate) {
+    if (!ctx || !s || !cam || !p || !out) return RT_ERR_INVALID;
+    if (p->width == 0 || p->height == 0) return fail(ctx, RT_ERR_INVALID, "empty image");
+    if ((uint64_t)p->width * p->height > 0xffffffffull / 4) return fail(ctx, RT_ERR_INVALID, "image too large");
+    if (p->use_photons && s->S.num_photons == 0 && s->photons_host.empty()) return fail(ctx, RT_ERR_INVALID, "use_photons without a photon map");
+    HIPCHECK(ctx, hipSetDevice(ctx->device));
+    uint32_t npix = p->width * p->height;
+    Staged cin, rout;
+    int rc;
+    if ((rc = stage_in(ctx, conv, npix, cin))) return rc;
+    if ((rc = stage_out(ctx, out, (size_t)npix * 12, rout, false))) return rc;
+    DCamera C; DParams P;
+    to_dev(cam, p, C, P);
+    P.counters = ctx->counters;
+    dim3 grid((npix + BLOCK - 1) / BLOCK);
+    if (ctx->timing) HIPCHECK(ctx, hipEventRecord(ctx->ev0, ctx->stream));
+    uint32_t nlaunch = 1;
+    if (accumulate) {
+        HIPCHECK(ctx, hipMemsetAsync(rout.dev, 0, (size_t)npix * 12, ctx->stream));
+        // chunks of passes keep single launches short
+        const uint32_t chunk = 4;
+        for (uint32_t k = 0; k < spp; k += chunk) {
+            DParams Q = P;
+            Q.sample_idx = P.sample_idx + k;
+            uint32_t m = std::min(chunk, spp - k);
+            hipLaunchKernelGGL(k_render, grid, dim3(BLOCK), 0, ctx->stream, s->S, C, Q, (const uint8_t*)cin.dev, (float*)rout.dev, m, 1);
+            HIPCHECK(ctx, hipGetLastError());
+        }
+        nlaunch = (spp + chunk - 1) / chunk;
+    } else {
+        hipLaunchKernelGGL(k_render, grid, dim3(BLOCK), 0, ctx->stream, s->S, C, P, (const uint8_t*)cin.dev, (float*)rout.dev, 1u, 0);
+        HIPCHECK(ctx, hipGetLastError());
+    }
+    if (ctx->timing) {
+        HIPCHECK(ctx, hipEventRecord(ctx->ev1, ctx->stream));
+        HIPCHECK(ctx, hipEventSynchronize(ctx->ev1));
+        float ms = 0.0f;
+        HIPCHECK(ctx, hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+        ctx->kernel_ms += ms;
+        ctx->launches += nlaunch;
+    }
+    return finish_out(ctx, out, (size_t)npix * 12, rout);
+}
+
+int rt_render_sample(rt_ctx* ctx, rt_scene* s, const rt_camera* cam, const rt_sample_params* p, const uint8_t* conv, float* out) {
+    return render_common(ctx, s, cam, p, conv, out, 1, false);
+}
+
+int rt_render_accumulate(rt_ctx* ctx, rt_scene* s, const rt_camera* cam, const rt_sample_params* p, uint32_t spp, float* out) {
+    return render_common(ctx, s, cam, p, nullptr, out, spp, true);
+}
+
+// ---- photon map --------------------------------------------------------------------------
+static void kd_build(float4* ph, size_t n) {
+    if (n == 0) return;
+    float mn[3] = {FINF, FINF, FINF}, mx[3] = {-FINF, -FINF, -FINF};
+    for (size_t i = 0; i < n; i++) {
+        float p[3] = {ph[3 * i].x, ph[3 * i].y, ph[3 * i].z};
+        for (int k = 0; k < 3; k++) { if (p[k] < mn[k]) mn[k] = p[k]; if (p[k] > mx[k]) mx[k] = p[k]; }
+    }
+    int axis = 0;
+    for (int a = 1; a < 3; a++) if (!((mx[a] - mn[a]) < (mx[axis] - mn[axis]))) axis = a;
+    size_t mid = n / 2;
+    struct P3 { float4 a, b, c; };
+    P3* arr = reinterpret_cast<P3*>(ph);
+    auto key = [axis](const P3& p) { return axis == 0 ? p.a.x : (axis == 1 ? p.a.y : p.a.z); };
+    std::nth_element(arr, arr + mid, arr + n, [&](const P3& x, const P3& y) { return key(x) < key(y); });
+    arr[mid].a.w = (float)axis;
+    kd_build(ph, mid);
+    kd_build(ph + 3 * (mid + 1), n - mid - 1);
+}
+
+int rt_photon_build(rt_ctx* ctx, rt_scene* s, const rt_photon_params* pp, uint32_t* num_stored) {
+    if (!ctx || !s || !pp) return RT_ERR_INVALID;
+    if (pp->num_photons == 0) return fail(ctx, RT_ERR_INVALID, "num_photons = 0");
+    if (pp->knn) return fail(ctx, RT_ERR_UNSUPPORTED, "k-NN gather is not built yet");
+    HIPCHECK(ctx, hipSetDevice(ctx->device));
+    DEmitter E;
+    memset(&E, 0, sizeof E);
+    uint32_t n = pp->num_photons;
+    if (pp->emitter == RT_EMIT_QUAD) {
+        V3 lu = mk(pp->u[0], pp->u[1], pp->u[2]), lv = mk(pp->v[0], pp->v[1], pp->v[2]);
+        V3 cr = cross(lu, lv);
+        float area = std::sqrt(len2(cr));
+        V3 nrm = cr / area;
+        V3 up = std::fabs(nrm.y) < 0.9f ? mk(0, 1, 0) : mk(1, 0, 0);
+        V3 cu = cross(nrm, up);
+        V3 ta = cu / std::sqrt(len2(cu));
+        V3 ba = cross(nrm, ta);
+        E.kind = 0;
+        for (int k = 0; k < 3; k++) { E.o[k] = pp->origin[k]; E.u[k] = pp->u[k]; E.v[k] = pp->v[k]; }
+        E.n[0] = nrm.x; E.n[1] = nrm.y; E.n[2] = nrm.z; E.ta[0] = ta.x; E.ta[1] = ta.y; E.ta[2] = ta.z; E.ba[0] = ba.x; E.ba[1] = ba.y; E.ba[2] = ba.z;
+        float f = area * PI / (float)n;
+        for (int k = 0; k < 3; k++) E.power[k] = pp->color[k] * f;
+        E.lum_clamp = 15.0f;
+    } else {
+        return fail(ctx, RT_ERR_UNSUPPORTED, "sun-disk photon emitter is not built yet");
+    }
+    float4* d_out = nullptr; uint32_t* d_ok = nullptr;
+    HIPCHECK(ctx, hipMalloc(&d_out, (size_t)n * 3 * sizeof(float4)));
+    std::unique_ptr<void, void (*)(void*)> g1(d_out, [](void* p) { (void)hipFree(p); });
+    HIPCHECK(ctx, hipMalloc(&d_ok, (size_t)n * sizeof(uint32_t)));
+    std::unique_ptr<void, void (*)(void*)> g2(d_ok, [](void* p) { (void)hipFree(p); });
+    DScene S = s->S;
+    S.num_photons = 0;
+    hipLaunchKernelGGL(k_photon_trace, dim3((n + BLOCK - 1) / BLOCK), dim3(BLOCK), 0, ctx->stream, S, E, n, d_out, d_ok);
+    HIPCHECK(ctx, hipGetLastError());
+    HIPCHECK(ctx, hipStreamSynchronize(ctx->stream));
+    std::vector<float4> all((size_t)n * 3);
+    std::vector<uint32_t> ok(n);
+    HIPCHECK(ctx, hipMemcpy(all.data(), d_out, all.size() * sizeof(float4), hipMemcpyDeviceToHost));
+    HIPCHECK(ctx, hipMemcpy(ok.data(), d_ok, ok.size() * sizeof(uint32_t), hipMemcpyDeviceToHost));
+    std::vector<float4> kept;
+    for (uint32_t i = 0; i < n; i++) if (ok[i]) { kept.push_back(all[3 * (size_t)i]); kept.push_back(all[3 * (size_t)i + 1]); kept.push_back(all[3 * (size_t)i + 2]); }
+    size_t m = kept.size() / 3;
+    kd_build(kept.data(), m);
+    const float4* dph = nullptr;
+    int rc;
+    if ((rc = upload_vec(s, kept, &dph))) return rc;
+    s->S.photons = dph;
+    s->S.num_photons = (uint32_t)m;
+    float r = pp->gather_radius;
+    s->S.ph_r2 = r * r;
+    s->S.ph_norm = 2.0f / (PI * PI * r * r);
+    s->photons_host.swap(kept);
+    if (num_stored) *num_stored = (uint32_t)m;
+    return RT_OK;
+}
+
+int rt_photon_irradiance(rt_ctx* ctx, rt_scene* s, const float* pn, uint32_t n, float* irr) {
+    if (!ctx || !s || (!pn && n) || (!irr && n)) return RT_ERR_INVALID;
+    if (n == 0) return RT_OK;
+    HIPCHECK(ctx, hipSetDevice(ctx->device));
+    Staged pin, pout;
+    int rc;
+    if ((rc = stage_in(ctx, pn, (size_t)n * 24, pin))) return rc;
+    if ((rc = stage_out(ctx, irr, (size_t)n * 12, pout, false))) return rc;
+    hipLaunchKernelGGL(k_photon_irr, dim3((n + BLOCK - 1) / BLOCK), dim3(BLOCK), 0, ctx->stream, s->S, (const float*)pin.dev, n, (float*)pout.dev);
+    HIPCHECK(ctx, hipGetLastError());
+    return finish_out(ctx, irr, (size_t)n * 12, pout);
+}
+
+// ---- measurement ----------------------------------------------------------------------------
+int rt_ctx_set_timing(rt_ctx* ctx, int enable) {
+    if (!ctx) return RT_ERR_INVALID;
+    HIPCHECK(ctx, hipSetDevice(ctx->device));
+    if (enable && !ctx->ev0) {
+        HIPCHECK(ctx, hipEventCreate(&ctx->ev0));
+        HIPCHECK(ctx, hipEventCreate(&ctx->ev1));
+    }
+    ctx->timing = enable != 0;
+    ctx->kernel_ms = 0.0;
+    ctx->launches = 0;
+    return RT_OK;
+}
+
+int rt_ctx_get_timing(rt_ctx* ctx, double* kernel_ms, uint64_t* launches) {
+    if (!ctx || !kernel_ms || !launches) return RT_ERR_INVALID;
+    *kernel_ms = ctx->kernel_ms;
+    *launches = ctx->launches;
+    return RT_OK;
+}
+
+int rt_ctx_set_counting(rt_ctx* ctx, int enable) {
+    if (!ctx) return RT_ERR_INVALID;
+    HIPCHECK(ctx, hipSetDevice(ctx->device));
+    if (enable) {
+        if (!ctx->counters) HIPCHECK(ctx, hipMalloc(&ctx->counters, 8 * sizeof(unsigned long long)));
+        HIPCHECK(ctx, hipMemset(ctx->counters, 0, 8 * sizeof(unsigned long long)));
+    } else if (ctx->counters) {
+        HIPCHECK(ctx, hipFree(ctx->counters));
+        ctx->counters = nullptr;
+    }
+    return RT_OK;
+}
+
+int rt_ctx_get_counters(rt_ctx* ctx, uint64_t out[4]) {
+    if (!ctx || !out) return RT_ERR_INVALID;
+    if (!ctx->counters) return fail(ctx, RT_ERR_INVALID, "counting is off");
+    HIPCHECK(ctx, hipMemcpy(out, ctx->counters, 4 * sizeof(uint64_t), hipMemcpyDeviceToHost));
+    return RT_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,615 @@
+"""Host-side mirror of rustracer's scene-building surface.
+
+The reference assembles a scene from `dyn Hittable` shapes (src/sphere.rs,
+src/quad.rs, src/triangle.rs via src/mesh.rs, src/diamond.rs), SRT wrappers
+(src/transform.rs), media (src/volume.rs), `dyn Material`s (src/material.rs),
+a light `HittableList` and a `Background` (src/renderer.rs:183-223), the way
+`scene_file::build` does (src/scene_file.rs:899-1132).  This module keeps the
+same names and argument meanings and flattens the result into the
+`rt_scene_desc` of include/rt_api.h — the single structure that crosses the
+C ABI.
+"""
+import ctypes as C
+import math
+import numpy as np
+
+from . import _ffi as F
+
+f32 = np.float32
+
+
+def v3(x):
+    a = np.asarray(x, dtype=np.float32).reshape(3)
+    return a
+
+
+def unit(a):
+    a = np.asarray(a, dtype=np.float32)
+    return (a / np.float32(np.sqrt(np.float32(np.dot(a, a))))).astype(np.float32)
+
+
+def cross(a, b):
+    return np.array([a[1] * b[2] - a[2] * b[1], a[2] * b[0] - a[0] * b[2], a[0] * b[1] - a[1] * b[0]], dtype=np.float32)
+
+
+# ---------------------------------------------------------------- materials
+class Material:
+    kind = 0
+    tex = 0
+
+    def params(self):
+        return []
+
+    def to_c(self):
+        m = F.rt_material()
+        m.kind = self.kind
+        m.tex = self.tex
+        for i, v in enumerate(self.params()):
+            m.p[i] = float(v)
+        return m
+
+
+class Lambertian(Material):
+    """src/material.rs:85-102 with a Solid texture."""
+    kind = F.MAT_LAMBERTIAN
+
+    def __init__(self, color):
+        self.color = v3(color)
+
+    def params(self):
+        return list(self.color)
+
+
+class Checker(Lambertian):
+    """Lambertian over Texture::Checker (src/texture.rs:28-33, src/scene_file.rs:1146-1147)."""
+    tex = 1
+
+    def __init__(self, even, odd, scale):
+        self.color = v3(even)
+        self.odd = v3(odd)
+        self.scale = float(scale)
+
+    def params(self):
+        return list(self.color) + [self.scale] + list(self.odd)
+
+
+class Dielectric(Material):
+    """src/material.rs:323-332."""
+    kind = F.MAT_DIELECTRIC
+
+    def __init__(self, ior):
+        self.ior = float(ior)
+
+    def params(self):
+        return [self.ior]
+
+
+class SpectralDielectric(Material):
+    """src/material.rs:357-403.  `sellmeier=(B1,B2,B3,C1,C2,C3)` switches the
+    Cauchy n(lambda) for a Sellmeier law (extension)."""
+    kind = F.MAT_SPECTRAL_DIELECTRIC
+
+    def __init__(self, cauchy_b=1.5, cauchy_c=0.0, absorption=(0, 0, 0), sellmeier=None):
+        self.cauchy_b, self.cauchy_c = float(cauchy_b), float(cauchy_c)
+        self.absorption = v3(absorption)
+        self.sellmeier = sellmeier
+
+    @staticmethod
+    def from_ior_dispersion(ior, dispersion, absorption=(0, 0, 0)):
+        """MaterialConfig::SpectralDielectric -> Cauchy (src/scene_file.rs:1166-1180)."""
+        c = f32(dispersion) / f32(3.250)
+        b = f32(ior) - c / (f32(0.58756) * f32(0.58756))
+        return SpectralDielectric(float(b), float(c), absorption)
+
+    def params(self):
+        p = [self.cauchy_b, self.cauchy_c] + list(self.absorption)
+        if self.sellmeier is None:
+            return p + [0.0]
+        return p + [1.0] + [float(x) for x in self.sellmeier]
+
+
+class PbrMaterial(Material):
+    """src/material.rs:695-1050 (scalar parameters; texture maps and thin film
+    are not carried over the ABI)."""
+    kind = F.MAT_PBR
+
+    def __init__(self, albedo=(0.8, 0.8, 0.8), roughness=0.5, metallic=0.0, anisotropy=0.0,
+                 anisotropy_angle=0.0, clearcoat=0.0, clearcoat_roughness=0.03, emission=(0, 0, 0),
+                 emission_strength=0.0, sheen=0.0, sheen_tint=0.5):
+        self.albedo = v3(albedo)
+        self.roughness, self.metallic = float(roughness), float(metallic)
+        self.anisotropy, self.anisotropy_angle = float(anisotropy), float(anisotropy_angle)
+        self.clearcoat, self.clearcoat_roughness = float(clearcoat), float(clearcoat_roughness)
+        self.emission, self.emission_strength = v3(emission), float(emission_strength)
+        self.sheen, self.sheen_tint = float(sheen), float(sheen_tint)
+
+    def params(self):
+        return (list(self.albedo) + [self.roughness, self.metallic, self.anisotropy, self.anisotropy_angle,
+                                     self.clearcoat, self.clearcoat_roughness] + list(self.emission)
+                + [self.emission_strength, self.sheen, self.sheen_tint])
+
+
+def Metal(color, fuzz):
+    """MaterialConfig::Metal = PbrMaterial metallic 1 (src/scene_file.rs:1155-1156)."""
+    return PbrMaterial(albedo=color, metallic=1.0, roughness=fuzz)
+
+
+class DiffuseLight(Material):
+    """src/material.rs:14-25."""
+    kind = F.MAT_DIFFUSE_LIGHT
+
+    def __init__(self, emit):
+        self.emit = v3(emit)
+
+    def params(self):
+        return list(self.emit)
+
+
+class SpectralMetal(Material):
+    """src/material.rs:427-554; variant 'gold' | 'copper' | 'silver'."""
+    kind = F.MAT_SPECTRAL_METAL
+
+    def __init__(self, variant="gold", roughness=0.0):
+        self.variant = {"gold": 0, "copper": 1, "silver": 2}[variant]
+        self.roughness = float(roughness)
+
+    def params(self):
+        return [self.variant, self.roughness]
+
+
+class RoughDielectric(Material):
+    """GGX microfacet dielectric (extension; the reference's dielectrics are smooth)."""
+    kind = F.MAT_ROUGH_DIELECTRIC
+
+    def __init__(self, ior=1.5, roughness=0.15):
+        self.ior, self.roughness = float(ior), float(roughness)
+
+    def params(self):
+        return [self.ior, self.roughness]
+
+
+# ------------------------------------------------------------------- shapes
+class Hittable:
+    shape = 0
+
+    def __init__(self, mat):
+        self.mat = mat
+        self.xforms = []      # innermost first
+        self.medium = None
+
+    # SRT wrappers in scene_file order: scale -> rotate -> translate
+    def scaled(self, factor):
+        assert factor > 0, "UniformScale factor must be positive"
+        self.xforms.append((F.XFORM_SCALE, [factor]))
+        return self
+
+    def rotated(self, angles_deg, pivot=(0.0, 0.0, 0.0)):
+        """Rotate::euler_about (src/transform.rs:196-224): fwd = Ry*Rx*Rz."""
+        rx, ry, rz = [f32(a) for a in angles_deg]
+        if rx == 0 and ry == 0 and rz == 0:
+            return self
+        pv = v3(pivot)
+        m = euler_matrix(rx, ry, rz)
+        self.xforms.append((F.XFORM_TRANSLATE, list(-pv)))
+        self.xforms.append((F.XFORM_ROTATE, list(m.reshape(9))))
+        self.xforms.append((F.XFORM_TRANSLATE, list(pv)))
+        return self
+
+    def translated(self, offset):
+        self.xforms.append((F.XFORM_TRANSLATE, list(v3(offset))))
+        return self
+
+    def inline(self):
+        return []
+
+
+def euler_matrix(rx_deg, ry_deg, rz_deg):
+    def sc(d):
+        r = f32(d) * f32(math.pi / 180.0)
+        return f32(math.sin(r)), f32(math.cos(r))
+    sx, cx = sc(rx_deg)
+    sy, cy = sc(ry_deg)
+    sz, cz = sc(rz_deg)
+    Rx = np.array([[1, 0, 0], [0, cx, -sx], [0, sx, cx]], dtype=np.float32)
+    Ry = np.array([[cy, 0, sy], [0, 1, 0], [-sy, 0, cy]], dtype=np.float32)
+    Rz = np.array([[cz, -sz, 0], [sz, cz, 0], [0, 0, 1]], dtype=np.float32)
+
+    def mm(a, b):
+        out = np.zeros((3, 3), dtype=np.float32)
+        for i in range(3):
+            for j in range(3):
+                out[i, j] = f32(f32(a[i, 0] * b[0, j]) + f32(a[i, 1] * b[1, j])) + f32(a[i, 2] * b[2, j])
+        return out
+    return mm(Ry, mm(Rx, Rz))
+
+
+class Sphere(Hittable):
+    shape = F.SHAPE_SPHERE
+
+    def __init__(self, center, radius, mat, center1=None):
+        super().__init__(mat)
+        self.center, self.radius = v3(center), float(radius)
+        self.center1 = None if center1 is None else v3(center1)
+
+    def inline(self):
+        c1 = self.center1 if self.center1 is not None else self.center
+        return list(self.center) + [self.radius] + list(c1) + [1.0 if self.center1 is not None else 0.0]
+
+
+class Quad(Hittable):
+    shape = F.SHAPE_QUAD
+
+    def __init__(self, q, u, v, mat):
+        super().__init__(mat)
+        self.q, self.u, self.v = v3(q), v3(u), v3(v)
+        n = cross(self.u, self.v)
+        area = float(np.sqrt(np.dot(n, n)))
+        if not area > 1e-6 * float(np.linalg.norm(self.u)) * float(np.linalg.norm(self.v)):
+            raise ValueError(f"Quad::new: degenerate quad — u and v are parallel or zero (cross length {area})")
+
+    def inline(self):
+        return list(self.q) + list(self.u) + list(self.v)
+
+
+class Box(Hittable):
+    """make_box (src/quad.rs:201-215): six quads in a nested BVH."""
+    shape = F.SHAPE_BOX
+
+    def __init__(self, p0, p1, mat):
+        super().__init__(mat)
+        self.p0, self.p1 = v3(p0), v3(p1)
+
+    def inline(self):
+        return list(self.p0) + list(self.p1)
+
+
+class Mesh(Hittable):
+    """A triangle list under its own BVH, as mesh::load_obj builds it
+    (src/mesh.rs:13-92).  vertices: (T,3,3); normals/tangents (T,3,3) or None;
+    uvs (T,3,2) or None."""
+    shape = F.SHAPE_MESH
+
+    def __init__(self, vertices, mat, normals=None, tangents=None, uvs=None):
+        super().__init__(mat)
+        self.v = np.ascontiguousarray(vertices, dtype=np.float32).reshape(-1, 9)
+        self.n = None if normals is None else np.ascontiguousarray(normals, dtype=np.float32).reshape(-1, 9)
+        self.t = None if tangents is None else np.ascontiguousarray(tangents, dtype=np.float32).reshape(-1, 9)
+        self.uv = None if uvs is None else np.ascontiguousarray(uvs, dtype=np.float32).reshape(-1, 6)
+
+    @property
+    def num_tris(self):
+        return self.v.shape[0]
+
+
+class Convex(Hittable):
+    """Convex polyhedron as half-spaces n.x <= d (convex_hit, src/diamond.rs:203-253)."""
+    shape = F.SHAPE_CONVEX
+
+    def __init__(self, planes, mat, center=(0, 0, 0), crown_h=1.0, pavilion_h=1.0, girdle_h=0.0,
+                 polish=0.0, bbox=None):
+        super().__init__(mat)
+        self.planes = np.ascontiguousarray(planes, dtype=np.float32).reshape(-1, 4)
+        self.center = v3(center)
+        self.crown_h, self.pavilion_h, self.girdle_h, self.polish = float(crown_h), float(pavilion_h), float(girdle_h), float(polish)
+        self.bbox = bbox if bbox is not None else aabb_from_planes(self.planes)
+
+    def inline(self):
+        return (list(self.center) + [self.crown_h, self.pavilion_h, self.girdle_h, self.polish]
+                + list(self.bbox[0]) + list(self.bbox[1]))
+
+
+def aabb_from_planes(planes):
+    """aabb_from_planes (src/hittable.rs:231-255), vectorised in float64 and
+    rounded outward to float32."""
+    P = np.asarray(planes, dtype=np.float64)
+    n, d = P[:, :3], P[:, 3]
+    eps = 1e-5 * (1.0 + np.abs(d).max())
+    lo = np.full(3, np.inf)
+    hi = np.full(3, -np.inf)
+    m = len(P)
+    for i in range(m):
+        for j in range(i + 1, m):
+            bc = np.cross(n[j], n[j + 1:])
+            det = bc @ n[i]
+            ok = np.abs(det) > 1e-12
+            if not ok.any():
+                continue
+            k_idx = np.arange(j + 1, m)[ok]
+            A = np.stack([np.broadcast_to(n[i], (len(k_idx), 3)), np.broadcast_to(n[j], (len(k_idx), 3)), n[k_idx]], axis=1)
+            rhs = np.stack([np.full(len(k_idx), d[i]), np.full(len(k_idx), d[j]), d[k_idx]], axis=1)
+            pts = np.linalg.solve(A, rhs[..., None])[..., 0]
+            inside = (pts @ n.T <= d + eps).all(axis=1)
+            if inside.any():
+                lo = np.minimum(lo, pts[inside].min(axis=0))
+                hi = np.maximum(hi, pts[inside].max(axis=0))
+    lo32 = np.nextafter(lo.astype(np.float32), np.float32(-np.inf))
+    hi32 = np.nextafter(hi.astype(np.float32), np.float32(np.inf))
+    return lo32, hi32
+
+
+def Diamond(center, radius, mat, table_pct=53.0, crown_angle=34.5, pavilion_angle=40.75, oval_ratio=1.0,
+            girdle_facets=16, girdle_pct=2.0, faceted=False, polish=0.0):
+    """Round / oval brilliant as half-spaces, after Diamond::new (src/diamond.rs:93-195)."""
+    c = v3(center)
+    r = f32(radius)
+    q = max(f32(oval_ratio), f32(0.05))
+    table_r = f32(table_pct / 100.0) * r
+    crown_h = (r - table_r) * f32(math.tan(float(f32(crown_angle) * f32(math.pi / 180))))
+    pavilion_h = r * f32(math.tan(float(f32(pavilion_angle) * f32(math.pi / 180))))
+    girdle_h = min(max(f32(girdle_pct / 100.0) * f32(2.0) * r, f32(0.0)), f32(0.3) * r)
+    n = (max(int(girdle_facets), 4) // 2) * 2
+    m = n // 2
+    band_m = n if faceted else 64
+    planes = []
+
+    def push(nrm, g):
+        nw = unit(np.array([nrm[0] / q, nrm[1], nrm[2]], dtype=np.float32))
+        gw = np.array([q * g[0], g[1], g[2]], dtype=np.float32)
+        planes.append(list(nw) + [float(np.dot(nw, gw))])
+
+    push(np.array([0, 1, 0], np.float32), np.array([c[0], c[1] + f32(0.5) * girdle_h + crown_h, c[2]], np.float32))
+    wavy = girdle_h > 0 and faceted
+
+    def band_h(k):
+        return f32(0.0) if (wavy and k % 2 == 1) else girdle_h
+    crown_n = n if girdle_h > 0 else m
+    for k in range(crown_n):
+        th = f32(k) * f32(2.0 * math.pi) / f32(crown_n)
+        s, co = f32(math.sin(th)), f32(math.cos(th))
+        push(unit(np.array([co * crown_h, r - table_r, s * crown_h], np.float32)),
+             np.array([c[0] + co * r, c[1] + f32(0.5) * band_h(k), c[2] + s * r], np.float32))
+    pav_n = n if girdle_h > 0 else m
+    for k in range(pav_n):
+        if girdle_h > 0:
+            ph = f32(k) * f32(2.0 * math.pi) / f32(n)
+        else:
+            ph = f32(k) * f32(2.0 * math.pi) / f32(m) + f32(math.pi) / f32(m)
+        s, co = f32(math.sin(ph)), f32(math.cos(ph))
+        push(unit(np.array([co * pavilion_h, -r, s * pavilion_h], np.float32)),
+             np.array([c[0] + co * r, c[1] - f32(0.5) * band_h(k), c[2] + s * r], np.float32))
+    if girdle_h > 0:
+        for j in range(band_m):
+            a = f32(j) * f32(2.0 * math.pi) / f32(band_m)
+            s, co = f32(math.sin(a)), f32(math.cos(a))
+            push(np.array([co, 0, s], np.float32), np.array([c[0] + co * r, c[1], c[2] + s * r], np.float32))
+    P = np.array(planes, dtype=np.float32)
+    if girdle_h > 0:
+        circ_r = r / f32(math.cos(math.pi / band_m))
+        bb = (c - np.array([circ_r * q, f32(0.5) * girdle_h + pavilion_h, circ_r], np.float32),
+              c + np.array([circ_r * q, f32(0.5) * girdle_h + crown_h, circ_r], np.float32))
+    else:
+        bb = aabb_from_planes(P)
+    pol = min(max(float(polish), 0.0), 0.25) if math.isfinite(polish) else 0.0
+    return Convex(P, mat, center=c, crown_h=crown_h, pavilion_h=pavilion_h, girdle_h=girdle_h, polish=pol,
+                  bbox=(np.asarray(bb[0], np.float32), np.asarray(bb[1], np.float32)))
+
+
+class ConstantMedium:
+    """src/volume.rs:78-156: wraps a boundary shape (its material is ignored)."""
+
+    def __init__(self, boundary, density, color, g=0.0):
+        self.boundary = boundary
+        boundary.medium = F.rt_medium(kind=F.MEDIUM_CONSTANT, density=float(density), g=float(g))
+        for k in range(3):
+            boundary.medium.albedo[k] = float(v3(color)[k])
+        boundary.medium_grid = None
+
+
+class GridMedium:
+    """Delta-tracked heterogeneous medium over a dense density grid in [0,1]
+    spanning the boundary's bounding box (NoiseMedium with a grid in place of
+    per-lookup Perlin turbulence, src/volume.rs:160-263)."""
+
+    def __init__(self, boundary, grid, density, color, g=0.0):
+        self.boundary = boundary
+        grid = np.ascontiguousarray(grid, dtype=np.float32)
+        nz, ny, nx = grid.shape
+        boundary.medium = F.rt_medium(kind=F.MEDIUM_GRID, density=float(density), g=float(g))
+        for k in range(3):
+            boundary.medium.albedo[k] = float(v3(color)[k])
+        boundary.medium.grid_n[0], boundary.medium.grid_n[1], boundary.medium.grid_n[2] = nx, ny, nz
+        boundary.medium_grid = grid
+
+
+# --------------------------------------------------------------- background
+class Background:
+    """Background::Solid / Gradient / Physical / EnvMap / EnvMapSplit."""
+
+    def __init__(self, kind=F.BG_SOLID, color=(0, 0, 0), top=(1, 1, 1), bottom=(0, 0, 0), sun_dir=(0, 1, 0),
+                 turbidity=3.0, env=None, env_rot_y_deg=0.0, visible=None):
+        self.kind, self.color, self.top, self.bottom = kind, v3(color), v3(top), v3(bottom)
+        self.sun_dir, self.turbidity = v3(sun_dir), float(turbidity)
+        self.env = None if env is None else np.ascontiguousarray(env, dtype=np.float32)
+        self.env_rot = float(f32(env_rot_y_deg) * f32(math.pi / 180))
+        self.visible = visible
+
+    @staticmethod
+    def solid(c):
+        return Background(F.BG_SOLID, color=c)
+
+    @staticmethod
+    def gradient(top, bottom):
+        return Background(F.BG_GRADIENT, top=top, bottom=bottom)
+
+    @staticmethod
+    def physical(sun_azimuth_deg, sun_elevation_deg, turbidity):
+        el = f32(sun_elevation_deg) * f32(math.pi / 180)
+        az = f32(sun_azimuth_deg) * f32(math.pi / 180)
+        d = unit(np.array([f32(math.cos(el)) * f32(math.sin(az)), f32(math.sin(el)),
+                           f32(math.cos(el)) * f32(math.cos(az))], np.float32))
+        return Background(F.BG_PHYSICAL, sun_dir=d, turbidity=turbidity)
+
+    @staticmethod
+    def envmap(image_hw3, rotate_y_deg=0.0, visible=None):
+        """visible: None or Background.solid / Background.gradient (EnvMapSplit)."""
+        kind = F.BG_ENVMAP if visible is None else F.BG_ENVMAP_SPLIT
+        return Background(kind, env=image_hw3, env_rot_y_deg=rotate_y_deg, visible=visible)
+
+    def to_c(self):
+        b = F.rt_background()
+        b.kind = self.kind
+        src = self.visible if self.visible is not None else self
+        b.visible_kind = src.kind if self.visible is not None else 0
+        for k in range(3):
+            b.color[k] = float(src.color[k])
+            b.top[k] = float(src.top[k])
+            b.bottom[k] = float(src.bottom[k])
+            b.sun_dir[k] = float(self.sun_dir[k])
+        b.turbidity = self.turbidity
+        if self.env is not None:
+            b.env_h, b.env_w = self.env.shape[0], self.env.shape[1]
+            b.env_rgb = self.env.ctypes.data_as(F.F32P)
+            b.env_rot_y = self.env_rot
+        return b
+
+
+# -------------------------------------------------------------------- camera
+class Camera:
+    """Camera::new (src/camera.rs:24-45) in f32."""
+
+    def __init__(self, lookfrom, lookat, vup, vfov_deg, aspect, aperture=0.0, focus_dist=None, blades=0):
+        lf, la, up = v3(lookfrom), v3(lookat), v3(vup)
+        if focus_dist is None:
+            focus_dist = max(float(np.linalg.norm(lf - la)), 0.01)
+        fd = f32(focus_dist)
+        h = f32(math.tan(float(f32(vfov_deg) * f32(math.pi / 180)) / 2.0))
+        vh = f32(2.0) * h
+        vw = f32(aspect) * vh
+        w = unit(lf - la)
+        u = unit(cross(up, w))
+        v = cross(w, u)
+        self.origin = lf
+        self.horizontal = (fd * vw * u).astype(np.float32)
+        self.vertical = (fd * vh * v).astype(np.float32)
+        self.lower_left = (lf - self.horizontal / f32(2) - self.vertical / f32(2) - fd * w).astype(np.float32)
+        self.u, self.v = u, v
+        self.lens_radius = f32(aperture) / f32(2)
+        self.blades = int(blades)
+
+    def to_c(self):
+        c = F.rt_camera()
+        for k in range(3):
+            c.origin[k] = float(self.origin[k]); c.lower_left[k] = float(self.lower_left[k])
+            c.horizontal[k] = float(self.horizontal[k]); c.vertical[k] = float(self.vertical[k])
+            c.u[k] = float(self.u[k]); c.v[k] = float(self.v[k])
+        c.lens_radius = float(self.lens_radius)
+        c.aperture_blades = self.blades
+        return c
+
+
+# --------------------------------------------------------------------- scene
+class Scene:
+    """World objects (input order = obj_id), the NEE light list and the
+    background: what `render_tiles` receives as world / lights / background."""
+
+    def __init__(self, objects, lights=(), background=None, name="scene"):
+        self.objects = list(objects)
+        self.lights = list(lights)
+        self.background = background if background is not None else Background.solid((0, 0, 0))
+        self.name = name
+        self._desc = None
+
+    def descriptor(self):
+        """Flatten into rt_scene_desc; the returned object keeps every array alive."""
+        mats, mat_index = [], {}
+        xforms, media, pool, grids = [], [], [], []
+        tri_v, tri_n, tri_t, tri_uv = [], [], [], []
+        ntris = 0
+        any_n = any(isinstance(o, Mesh) and o.n is not None for o in self.objects + self.lights)
+        any_t = any(isinstance(o, Mesh) and o.t is not None for o in self.objects + self.lights)
+        any_uv = any(isinstance(o, Mesh) and o.uv is not None for o in self.objects + self.lights)
+
+        def mat_id(m):
+            if m is None:
+                m = Lambertian((0.5, 0.5, 0.5))
+            k = id(m)
+            if k not in mat_index:
+                mat_index[k] = len(mats)
+                mats.append(m)
+            return mat_index[k]
+
+        def flat(o):
+            nonlocal ntris
+            r = F.rt_object()
+            r.shape = o.shape
+            r.material = mat_id(o.mat)
+            r.xform_first = len(xforms)
+            r.xform_count = len(o.xforms)
+            for kind, vals in o.xforms:
+                x = F.rt_xform(kind=kind)
+                for i, val in enumerate(vals):
+                    x.v[i] = float(val)
+                xforms.append(x)
+            for i, val in enumerate(o.inline()):
+                r.p[i] = float(val)
+            if isinstance(o, Mesh):
+                r.data_first, r.data_count = ntris, o.num_tris
+                tri_v.append(o.v)
+                if any_n:
+                    tri_n.append(o.n if o.n is not None else _face_normals(o.v))
+                if any_t:
+                    tri_t.append(o.t if o.t is not None else _edge_tangents(o.v))
+                if any_uv:
+                    tri_uv.append(o.uv if o.uv is not None else np.tile(np.array([0, 0, 1, 0, 0.5, 1], np.float32), (o.num_tris, 1)))
+                ntris += o.num_tris
+            elif isinstance(o, Convex):
+                r.data_first = sum(len(p) for p in pool) // 4
+                r.data_count = o.planes.shape[0]
+                pool.append(o.planes.reshape(-1))
+            med = getattr(o, "medium", None)
+            if med is not None:
+                m = F.rt_medium()
+                C.pointer(m)[0] = med
+                g = getattr(o, "medium_grid", None)
+                if g is not None:
+                    grids.append(g)
+                    m.grid = g.ctypes.data_as(F.F32P)
+                media.append(m)
+                r.medium = len(media)
+            return r
+
+        objs = [flat(o) for o in self.objects]
+        lights = [flat(o) for o in self.lights]
+        keep = {}
+        d = F.rt_scene_desc()
+        keep["objs"] = (F.rt_object * max(len(objs), 1))(*objs)
+        d.objects, d.num_objects = C.cast(keep["objs"], C.POINTER(F.rt_object)), len(objs)
+        keep["lights"] = (F.rt_object * max(len(lights), 1))(*lights)
+        d.lights, d.num_lights = C.cast(keep["lights"], C.POINTER(F.rt_object)), len(lights)
+        cm = [m.to_c() for m in mats]
+        keep["mats"] = (F.rt_material * max(len(cm), 1))(*cm)
+        d.materials, d.num_materials = C.cast(keep["mats"], C.POINTER(F.rt_material)), len(cm)
+        keep["xf"] = (F.rt_xform * max(len(xforms), 1))(*xforms)
+        d.xforms, d.num_xforms = C.cast(keep["xf"], C.POINTER(F.rt_xform)), len(xforms)
+        keep["media"] = (F.rt_medium * max(len(media), 1))(*media)
+        d.media, d.num_media = C.cast(keep["media"], C.POINTER(F.rt_medium)), len(media)
+        keep["grids"] = grids
+        if pool:
+            P = np.ascontiguousarray(np.concatenate(pool), dtype=np.float32)
+            keep["pool"] = P
+            d.pool, d.pool_len = P.ctypes.data_as(F.F32P), P.size
+        if ntris:
+            for name, lst, attr in (("tv", tri_v, "tri_v"), ("tn", tri_n, "tri_n"), ("tt", tri_t, "tri_t"), ("tuv", tri_uv, "tri_uv")):
+                if lst:
+                    A = np.ascontiguousarray(np.concatenate(lst), dtype=np.float32)
+                    keep[name] = A
+                    setattr(d, attr, A.ctypes.data_as(F.F32P))
+            d.num_tris = ntris
+        d.background = self.background.to_c()
+        keep["bg_env"] = self.background.env
+        self._desc = (d, keep)
+        return d
+
+
+def _face_normals(v):
+    v = v.reshape(-1, 3, 3).astype(np.float32)
+    n = np.cross(v[:, 1] - v[:, 0], v[:, 2] - v[:, 0])
+    n = n / np.sqrt((n * n).sum(1, keepdims=True))
+    return np.repeat(n[:, None, :], 3, axis=1).reshape(-1, 9).astype(np.float32)
+
+
+def _edge_tangents(v):
+    v = v.reshape(-1, 3, 3).astype(np.float32)
+    t = v[:, 1] - v[:, 0]
+    t = t / np.sqrt((t * t).sum(1, keepdims=True))
+    return np.repeat(t[:, None, :], 3, axis=1).reshape(-1, 9).astype(np.float32)
