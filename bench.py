@@ -25,13 +25,15 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
 # bytes of one primitive record read by a test: triangle v0/e1/e2 48 B, quad 80 B, sphere 64 B (inline object)
-PRIM_BYTES = {"box": 80.0, "mesh": 48.0}
+PRIM_BYTES = {"box": 80.0, "mesh": 48.0, "gem": 16.0 * 57}
 
 
 def workloads():
     from rustracer_amd import scenes
     return {
         "box": lambda: scenes.box_room(256, 256),
+        "mesh": lambda: scenes.mesh_envmap(1280, 720, level=8),
+        "gem": lambda: scenes.gem_caustics(1280, 720),
     }
 
 
@@ -61,10 +63,19 @@ def main():
     r.upload(scene)
     upload_s = time.time() - t0
     stats = r.stats()
+    photons = "num_photons" in cfg
+    photon_s = None
+    if photons:
+        lq = cfg["light_quad"]
+        t0 = time.time()
+        stored = r.photon_build_from_quad(lq["q"], lq["u"], lq["v"], cfg["light_color"], cfg["num_photons"],
+                                          cfg["gather_radius"])
+        photon_s = time.time() - t0
     acc = torch.zeros((H, W, 3), dtype=torch.float32, device="cuda")
 
     def step(k):
-        r.render_accumulate(acc, cam, W, H, spp, sample_idx=0, strata=strata, max_depth=depth, seed=k)
+        r.render_accumulate(acc, cam, W, H, spp, sample_idx=0, strata=strata, max_depth=depth, seed=k,
+                            photon_map=photons)
 
     for k in range(args.warmup):
         step(k)
@@ -100,7 +111,7 @@ def main():
 
     cpu = None
     if not args.no_cpu:
-        cpu = cpu_baseline(scene, cam, cfg, args.cpu_seconds)
+        cpu = cpu_baseline(scene, cam, cfg, args.cpu_seconds, photons)
 
     line = {
         "metric": "GPU samples/s (plain megakernel) at fixed RMSE",
@@ -117,7 +128,9 @@ def main():
         "data": "synthetic",
         "config": {"workload": scene.name, "width": W, "height": H, "spp": spp, "max_depth": depth,
                    "renderer": "plain megakernel", "bvh_nodes": stats["nodes"], "triangles": stats["triangles"],
-                   "upload_s": round(upload_s, 3)},
+                   "upload_s": round(upload_s, 3),
+                   "photons": None if not photons else {"traced": cfg["num_photons"], "stored": stored,
+                                                        "build_s": round(photon_s, 3)}},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None,
                      "kernel": "k_render", "avg_launch_ms": kern_ms / max(launches, 1),
@@ -129,22 +142,32 @@ def main():
     print(json.dumps(line), flush=True)
 
 
-def cpu_baseline(scene, cam, cfg, seconds):
+def cpu_baseline(scene, cam, cfg, seconds, photons=False):
     """The oracle's render_tiles with the reference's row streams, all host
-    cores, passes of the same frame until `seconds` of wall time are used."""
+    cores, passes of the same frame until `seconds` of wall time are used
+    (scene build and photon tracing excluded, as the reference's --bench)."""
     from oracle import oracle as O
     from rustracer_amd import Renderer
+    from rustracer_amd import _ffi as F
     cores = O.threads()
     orc = O.OracleScene(scene)
+    if photons:
+        lq = cfg["light_quad"]
+        pp = F.rt_photon_params()
+        pp.emitter = F.EMIT_QUAD
+        for k in range(3):
+            pp.origin[k], pp.u[k], pp.v[k], pp.color[k] = lq["q"][k], lq["u"][k], lq["v"][k], cfg["light_color"][k]
+        pp.num_photons, pp.gather_radius = cfg["num_photons"], cfg["gather_radius"]
+        orc.photon_build(pp, nthreads=cores)
     W, H, spp = cfg["width"], cfg["height"], cfg["spp"]
     strata = int(spp ** 0.5)
     done = 0
     t0 = time.perf_counter()
     while True:
-        p = Renderer.params(W, H, sample_idx=done, strata=strata, max_depth=cfg["max_depth"])
+        p = Renderer.params(W, H, sample_idx=done, strata=strata, max_depth=cfg["max_depth"], use_photons=photons)
         orc.render_sample(cam, p, mode=0, nthreads=cores)
         done += 1
-        if time.perf_counter() - t0 >= seconds or done >= 64:
+        if time.perf_counter() - t0 >= seconds or done >= 100000:
             break
     el = time.perf_counter() - t0
     return {"value": W * H * done / el, "unit": "samples/s", "cores": cores, "kind": "port",

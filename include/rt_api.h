@@ -288,6 +288,10 @@ int  rt_photon_build(rt_ctx *ctx, rt_scene *scene, const rt_photon_params *pp,
  * (src/photon.rs:148-157).  pos_nrm: n * 6 floats (p.xyz, n.xyz).            */
 int  rt_photon_irradiance(rt_ctx *ctx, rt_scene *scene, const float *pos_nrm,
                           uint32_t n, float *irr_rgb);
+/* Copy the attached map's photons (kd order) out: up to cap x 9 floats
+ * (position, power, normal); *count receives the map size.                   */
+int  rt_photon_export(rt_ctx *ctx, rt_scene *scene, float *out, uint32_t cap,
+                      uint32_t *count);
 
 /* ---- measurement (no counterpart in the reference: its --bench mode times
  * render_tiles with Instant, src/main.rs:68-117) ------------------------------ */

@@ -121,6 +121,7 @@ SIGNATURES = {
     "rt_render_accumulate": (C.c_int, [VP, VP, C.POINTER(rt_camera), C.POINTER(rt_sample_params), C.c_uint32, VP]),
     "rt_photon_build": (C.c_int, [VP, VP, C.POINTER(rt_photon_params), C.POINTER(C.c_uint32)]),
     "rt_photon_irradiance": (C.c_int, [VP, VP, VP, C.c_uint32, VP]),
+    "rt_photon_export": (C.c_int, [VP, VP, VP, C.c_uint32, C.POINTER(C.c_uint32)]),
     "rt_ctx_set_timing": (C.c_int, [VP, C.c_int]),
     "rt_ctx_get_timing": (C.c_int, [VP, C.POINTER(C.c_double), C.POINTER(C.c_uint64)]),
     "rt_ctx_set_counting": (C.c_int, [VP, C.c_int]),
@@ -138,6 +139,13 @@ def lib():
         if not os.path.exists(LIB_PATH):
             raise ImportError(f"{LIB_PATH} is not built; run __graft_entry__.build() "
                               f"(hipcc --offload-arch=gfx950)")
+        # torch ships its own libamdhip64 under the same soname; load it first
+        # so the process holds one HIP runtime that both torch and this library
+        # use (the reverse order leaves torch without a device).
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         L = C.CDLL(LIB_PATH)
         for name, (res, args) in SIGNATURES.items():
             fn = getattr(L, name)

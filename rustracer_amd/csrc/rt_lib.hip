@@ -215,6 +215,7 @@ struct rt_scene {
     std::vector<void*> allocs;
     uint64_t stats[8] = {0};
     std::vector<float4> photons_host;
+    float world_min[3] = {0, 0, 0}, world_max[3] = {0, 0, 0};
     ~rt_scene() { for (void* p : allocs) (void)hipFree(p); }
 };
 
@@ -656,6 +657,7 @@ extern "C" int rt_scene_upload(rt_ctx* ctx, const rt_scene_desc* d, rt_scene** o
     S.elut = ctx->elut;
     for (int k = 0; k < 3; k++) { S.norms[k] = ctx->norms[k]; S.fire_norms[k] = ctx->fire_norms[k]; }
     S.stack_bound = bound;
+    for (int k = 0; k < 3; k++) { s->world_min[k] = wb.mn[k]; s->world_max[k] = wb.mx[k]; }
     size_t node_count = F.nodes.size();
     s->stats[0] = node_count;                 // nodes
     s->stats[1] = F.tri_tag.size();           // triangles
@@ -810,7 +812,32 @@ int rt_photon_build(rt_ctx* ctx, rt_scene* s, const rt_photon_params* pp, uint32
         for (int k = 0; k < 3; k++) E.power[k] = pp->color[k] * f;
         E.lum_clamp = 15.0f;
     } else {
-        return fail(ctx, RT_ERR_UNSUPPORTED, "sun-disk photon emitter is not built yet");
+        // PhotonMap::build (src/photon.rs:41-94): a disk facing the sun, sized
+        // to the bounding sphere of the scene bounds and pushed out along the
+        // sun direction.
+        V3 sd = mk(pp->sun_dir[0], pp->sun_dir[1], pp->sun_dir[2]);
+        V3 sdu = sd / std::sqrt(len2(sd));
+        V3 down = -sd;
+        down = down / std::sqrt(len2(down));
+        V3 up = std::fabs(down.x) < 0.999f ? mk(1, 0, 0) : mk(0, 1, 0);
+        V3 tt = cross(down, up);
+        tt = tt / std::sqrt(len2(tt));
+        V3 bb = cross(down, tt);
+        const float CLIP = 1.0e3f, DISK_R = 20.0f;
+        auto clip = [&](float v) { return v < -CLIP ? -CLIP : (v > CLIP ? CLIP : v); };
+        V3 mn = mk(clip(s->world_min[0]), clip(s->world_min[1]), clip(s->world_min[2]));
+        V3 mx = mk(clip(s->world_max[0]), clip(s->world_max[1]), clip(s->world_max[2]));
+        V3 c = (mn + mx) * 0.5f;
+        float r = std::fmax(std::sqrt(len2(mx - mn)) * 0.5f, 1.0e-3f);
+        V3 dc = c + sdu * r;
+        float disk_area = PI * r * r;
+        float f = disk_area / ((float)n * PI);
+        E.kind = 1;
+        for (int k = 0; k < 3; k++) E.power[k] = pp->color[k] * f;
+        E.lum_clamp = 15.0f * disk_area / (PI * DISK_R * DISK_R);
+        E.disk_c[0] = dc.x; E.disk_c[1] = dc.y; E.disk_c[2] = dc.z; E.disk_r = r;
+        E.t[0] = tt.x; E.t[1] = tt.y; E.t[2] = tt.z; E.b[0] = bb.x; E.b[1] = bb.y; E.b[2] = bb.z;
+        E.sun_down[0] = down.x; E.sun_down[1] = down.y; E.sun_down[2] = down.z;
     }
     float4* d_out = nullptr; uint32_t* d_ok = nullptr;
     HIPCHECK(ctx, hipMalloc(&d_out, (size_t)n * 3 * sizeof(float4)));
@@ -854,6 +881,21 @@ int rt_photon_irradiance(rt_ctx* ctx, rt_scene* s, const float* pn, uint32_t n, 
     hipLaunchKernelGGL(k_photon_irr, dim3((n + BLOCK - 1) / BLOCK), dim3(BLOCK), 0, ctx->stream, s->S, (const float*)pin.dev, n, (float*)pout.dev);
     HIPCHECK(ctx, hipGetLastError());
     return finish_out(ctx, irr, (size_t)n * 12, pout);
+}
+
+int rt_photon_export(rt_ctx* ctx, rt_scene* s, float* out, uint32_t cap, uint32_t* count) {
+    if (!ctx || !s || !count) return RT_ERR_INVALID;
+    size_t m = s->photons_host.size() / 3;
+    *count = (uint32_t)m;
+    if (!out) return RT_OK;
+    for (size_t i = 0; i < m && i < cap; i++) {
+        const float4* p = &s->photons_host[3 * i];
+        float* o = out + 9 * i;
+        o[0] = p[0].x; o[1] = p[0].y; o[2] = p[0].z;
+        o[3] = p[1].x; o[4] = p[1].y; o[5] = p[1].z;
+        o[6] = p[2].x; o[7] = p[2].y; o[8] = p[2].z;
+    }
+    return RT_OK;
 }
 
 // ---- measurement ----------------------------------------------------------------------------

@@ -143,6 +143,25 @@ class Renderer:
         self.num_photons = n.value
         return n.value
 
+    def photon_build_sun(self, sun_dir, sun_color, num_photons, gather_radius):
+        """PhotonMap::build (src/photon.rs:41-94): directional disk toward `sun_dir`."""
+        pp = F.rt_photon_params()
+        pp.emitter = F.EMIT_SUN_DISK
+        for k in range(3):
+            pp.sun_dir[k], pp.color[k] = float(sun_dir[k]), float(sun_color[k])
+        pp.num_photons, pp.gather_radius = num_photons, gather_radius
+        n = C.c_uint32()
+        self._check(self.L.rt_photon_build(self.ctx, self.scene, C.byref(pp), C.byref(n)), "rt_photon_build")
+        self.num_photons = n.value
+        return n.value
+
+    def photon_export(self):
+        n = C.c_uint32()
+        self._check(self.L.rt_photon_export(self.ctx, self.scene, None, 0, C.byref(n)), "rt_photon_export")
+        out = np.zeros((n.value, 9), np.float32)
+        self._check(self.L.rt_photon_export(self.ctx, self.scene, _ptr(out), n.value, C.byref(n)), "rt_photon_export")
+        return out
+
     def photon_irradiance(self, pos_nrm):
         pos_nrm = np.ascontiguousarray(pos_nrm, dtype=np.float32).reshape(-1, 6)
         out = np.zeros((len(pos_nrm), 3), dtype=np.float32)

@@ -44,3 +44,172 @@ def box_room(width=256, height=256):
     scene = Scene(objs, lights, Background.solid((0, 0, 0)), name="box_room")
     cam = Camera((0.5, 0.5, -1.4), (0.5, 0.5, 0.5), (0, 1, 0), 40.0, width / height)
     return scene, cam, dict(width=width, height=height, spp=16, max_depth=8, light_quad=lq)
+
+
+# --------------------------------------------------------------------- helpers
+class Perlin:
+    """Gradient noise with the structure of rustracer's Perlin (src/perlin.rs):
+    256 random unit vectors, three permutation tables, Hermite-smoothed
+    trilinear blend of the eight corner dot products; turbulence sums octaves
+    at doubling frequency and halving weight and takes |.|.  Tables come from
+    numpy's seeded generator (the reference seeds from thread_rng)."""
+
+    def __init__(self, seed):
+        g = np.random.default_rng(seed)
+        v = g.uniform(-1, 1, size=(256, 3))
+        self.ranvec = (v / np.linalg.norm(v, axis=1, keepdims=True)).astype(np.float32)
+        self.perm = [g.permutation(256) for _ in range(3)]
+
+    def noise(self, p):
+        p = np.asarray(p, dtype=np.float32)
+        f = np.floor(p)
+        uvw = p - f
+        s = uvw * uvw * (3.0 - 2.0 * uvw)
+        ijk = f.astype(np.int64)
+        acc = np.zeros(len(p), np.float32)
+        for di in (0, 1):
+            for dj in (0, 1):
+                for dk in (0, 1):
+                    idx = (self.perm[0][(ijk[:, 0] + di) & 255] ^ self.perm[1][(ijk[:, 1] + dj) & 255]
+                           ^ self.perm[2][(ijk[:, 2] + dk) & 255])
+                    c = self.ranvec[idx]
+                    w = uvw - np.array([di, dj, dk], np.float32)
+                    wt = ((di * s[:, 0] + (1 - di) * (1 - s[:, 0])) * (dj * s[:, 1] + (1 - dj) * (1 - s[:, 1]))
+                          * (dk * s[:, 2] + (1 - dk) * (1 - s[:, 2])))
+                    acc += wt * (c * w).sum(1)
+        return acc
+
+    def turb(self, p, octaves):
+        p = np.asarray(p, dtype=np.float32)
+        acc = np.zeros(len(p), np.float32)
+        w = 1.0
+        for _ in range(octaves):
+            acc += w * self.noise(p)
+            w *= 0.5
+            p = p * 2.0
+        return acc
+
+
+def icosphere(level, radius=1.0):
+    """Subdivided icosahedron: 20 * 4**level faces.  Returns (verts, faces)."""
+    t = (1.0 + 5 ** 0.5) / 2.0
+    V = np.array([(-1, t, 0), (1, t, 0), (-1, -t, 0), (1, -t, 0), (0, -1, t), (0, 1, t), (0, -1, -t), (0, 1, -t),
+                  (t, 0, -1), (t, 0, 1), (-t, 0, -1), (-t, 0, 1)], dtype=np.float64)
+    Fc = np.array([(0, 11, 5), (0, 5, 1), (0, 1, 7), (0, 7, 10), (0, 10, 11), (1, 5, 9), (5, 11, 4), (11, 10, 2),
+                   (10, 7, 6), (7, 1, 8), (3, 9, 4), (3, 4, 2), (3, 2, 6), (3, 6, 8), (3, 8, 9), (4, 9, 5),
+                   (2, 4, 11), (6, 2, 10), (8, 6, 7), (9, 8, 1)], dtype=np.int64)
+    verts = V / np.linalg.norm(V, axis=1, keepdims=True)
+    faces = Fc
+    for _ in range(level):
+        a, b, c = faces[:, 0], faces[:, 1], faces[:, 2]
+        e = np.concatenate([np.stack([a, b], 1), np.stack([b, c], 1), np.stack([c, a], 1)])
+        e.sort(axis=1)
+        key = e[:, 0] * (len(verts) + 1) + e[:, 1]
+        uniq, inv = np.unique(key, return_inverse=True)
+        ua, ub = uniq // (len(verts) + 1), uniq % (len(verts) + 1)
+        mids = verts[ua] + verts[ub]
+        mids /= np.linalg.norm(mids, axis=1, keepdims=True)
+        base = len(verts)
+        verts = np.concatenate([verts, mids])
+        m = inv.reshape(3, -1) + base
+        ab, bc, ca = m[0], m[1], m[2]
+        faces = np.concatenate([np.stack([a, ab, ca], 1), np.stack([b, bc, ab], 1), np.stack([c, ca, bc], 1),
+                                np.stack([ab, bc, ca], 1)])
+    return verts * radius, faces
+
+
+def displaced_icosphere(level, radius=1.0, amplitude=0.05, octaves=3, frequency=4.0, seed=1):
+    """Icosphere displaced along its normals by `amplitude` x Perlin turbulence
+    (signed: 2*turb - mean) and smooth per-vertex normals.  Returns triangle
+    soup arrays (T,3,3) vertices and normals."""
+    verts, faces = icosphere(level, radius)
+    n = verts / np.linalg.norm(verts, axis=1, keepdims=True)
+    noise = Perlin(seed).turb((verts * frequency).astype(np.float32), octaves).astype(np.float64)
+    noise = noise - noise.mean()
+    pv = verts + n * (amplitude * noise / max(np.abs(noise).max(), 1e-9))[:, None]
+    f = pv[faces]
+    fn = np.cross(f[:, 1] - f[:, 0], f[:, 2] - f[:, 0])
+    vn = np.zeros_like(pv)
+    for k in range(3):
+        np.add.at(vn, faces[:, k], fn)
+    vn /= np.linalg.norm(vn, axis=1, keepdims=True)
+    return pv[faces].astype(np.float32), vn[faces].astype(np.float32)
+
+
+def sky_envmap(width=2048, height=1024, sun_elev_deg=35.0, sun_azim_deg=30.0, sun_sigma_deg=1.0, sun_peak=5.0e4):
+    """Equirect float32 HDR (row 0 = +Y): a vertical sky gradient (deep blue
+    zenith to pale horizon, dim brown below) plus a Gaussian sun lobe of
+    angular sigma `sun_sigma_deg` and peak radiance `sun_peak`.  Texel
+    directions follow EnvMapData's mapping (u = 0.5 + atan2(x,z)/2pi,
+    v = acos(y)/pi, src/renderer.rs:97-121)."""
+    j = (np.arange(height) + 0.5) / height
+    i = (np.arange(width) + 0.5) / width
+    theta = np.pi * j[:, None]
+    phi = 2 * np.pi * (i[None, :] - 0.5)
+    d = np.stack([np.sin(theta) * np.sin(phi), np.cos(theta) * np.ones_like(phi), np.sin(theta) * np.cos(phi)], -1)
+    y = d[..., 1]
+    zen = np.array([0.10, 0.25, 0.65])
+    hor = np.array([0.85, 0.90, 1.00])
+    gnd = np.array([0.12, 0.10, 0.08])
+    up = np.clip(y, 0, 1)[..., None]
+    img = np.where(y[..., None] >= 0, hor * (1 - up) + zen * up, gnd * np.ones_like(up))
+    el, az = np.radians(sun_elev_deg), np.radians(sun_azim_deg)
+    s = np.array([np.cos(el) * np.sin(az), np.sin(el), np.cos(el) * np.cos(az)])
+    ang = np.arccos(np.clip(d @ s, -1, 1))
+    lobe = sun_peak * np.exp(-0.5 * (ang / np.radians(sun_sigma_deg)) ** 2)
+    img = img + lobe[..., None] * np.array([1.0, 0.95, 0.88])
+    return img.astype(np.float32)
+
+
+def mesh_envmap(width=1280, height=720, level=8):
+    """QBVH-bound triangle mesh under an HDR environment.
+
+    An icosphere of 20*4**level triangles (level 8: 1,310,720), radius 1 at the
+    origin, displaced along its normals by 0.05 x 3-octave Perlin turbulence
+    (frequency 4, permutation seed 1), smooth vertex normals, rough GGX
+    dielectric (IOR 1.5, roughness 0.15).  A 20x20 diffuse 0.8 ground quad at
+    y = -1.1.  Lighting: sky_envmap() 2048x1024 (sun lobe sigma 1 deg, peak
+    5e4, elevation 35 deg, azimuth 30 deg), importance sampled.  Camera at
+    (0, 0.6, -3.6) looking at (0, -0.1, 0), vfov 40.  Render: 64 spp, max
+    depth 12.
+    """
+    from .scene import Background, Camera, Lambertian, Mesh, Quad, RoughDielectric, Scene
+    v, n = displaced_icosphere(level)
+    objs = [Mesh(v, RoughDielectric(1.5, 0.15), normals=n),
+            Quad((-10, -1.1, -10), (20, 0, 0), (0, 0, 20), Lambertian((0.8, 0.8, 0.8)))]
+    scene = Scene(objs, [], Background.envmap(sky_envmap()), name=f"mesh_envmap_L{level}")
+    cam = Camera((0.0, 0.6, -3.6), (0.0, -0.1, 0.0), (0, 1, 0), 40.0, width / height)
+    return scene, cam, dict(width=width, height=height, spp=64, max_depth=12)
+
+
+def gem_caustics(width=1280, height=720, num_photons=2_000_000, dispersion="cauchy"):
+    """Dispersive gem with caustics.
+
+    A round brilliant (Diamond, table 57 %, crown 34.5 deg, pavilion 40.75 deg,
+    girdle_facets 56 with no girdle band: 1 + 28 + 28 = 57 half-spaces) of
+    radius 0.5, girdle at y = 0.55, rotated 12 deg about y.  SpectralDielectric
+    diamond: the reference preset Cauchy B = 2.351, C = 0.0230
+    (src/scene_file.rs:686; n(589) ~ 2.417) or, with dispersion="sellmeier",
+    Peter's Sellmeier law (B 0.3306, 4.3356; C 0.175^2, 0.106^2 um^2).  Floor:
+    diffuse 0.8 quad 8x8 at y = 0.  Light: a 0.3x0.3 DiffuseLight quad of
+    radiance 50 at y = 2.2 above the stone, facing down, in world and light
+    list.  Caustic photon map from that quad (build_from_quad), gather radius
+    0.05.  Camera at (0, 1.6, -2.6) looking at (0, 0.35, 0), vfov 35, black
+    background.  Render: 256 spp, max depth 24.
+    """
+    from .scene import Background, Camera, Diamond, DiffuseLight, Lambertian, Quad, Scene, SpectralDielectric
+    if dispersion == "sellmeier":
+        mat = SpectralDielectric(sellmeier=(0.3306, 4.3356, 0.0, 0.175 ** 2, 0.106 ** 2, 0.0))
+    else:
+        mat = SpectralDielectric(2.351, 0.0230)
+    gem = Diamond((0, 0, 0), 0.5, mat, table_pct=57.0, crown_angle=34.5, pavilion_angle=40.75,
+                  girdle_facets=56, girdle_pct=0.0).rotated((0, 12, 0)).translated((0, 0.55, 0))
+    light = DiffuseLight((50.0, 50.0, 50.0))
+    lq = dict(q=(-0.15, 2.2, -0.15), u=(0.3, 0, 0), v=(0, 0, 0.3))
+    objs = [Quad((-4, 0, -4), (8, 0, 0), (0, 0, 8), Lambertian((0.8, 0.8, 0.8))), gem,
+            Quad(lq["q"], lq["u"], lq["v"], light)]
+    lights = [Quad(lq["q"], lq["u"], lq["v"], light)]
+    scene = Scene(objs, lights, Background.solid((0, 0, 0)), name="gem_caustics")
+    cam = Camera((0.0, 1.6, -2.6), (0.0, 0.35, 0.0), (0, 1, 0), 35.0, width / height)
+    return scene, cam, dict(width=width, height=height, spp=256, max_depth=24, light_quad=lq, light_color=(50.0, 50.0, 50.0),
+                            num_photons=num_photons, gather_radius=0.05)

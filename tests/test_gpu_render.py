@@ -95,3 +95,90 @@ def test_device_buffer_output_matches_host(gpu_renderer):
     dev = torch.zeros((32, 32, 3), dtype=torch.float32, device="cuda")
     gpu_renderer.render_tiles(dev, None, 3, 0, 32, 32, cam, max_depth=cfg["max_depth"])
     assert np.array_equal(dev.cpu().numpy(), host)
+
+
+def small_env_scene(W=48, H=48):
+    from rustracer_amd import scene as S
+    v, n = scenes.displaced_icosphere(3, amplitude=0.05)
+    env = scenes.sky_envmap(128, 64, sun_sigma_deg=4.0, sun_peak=200.0)
+    sc = S.Scene([S.Mesh(v, S.RoughDielectric(1.5, 0.15), normals=n),
+                  S.Quad((-10, -1.1, -10), (20, 0, 0), (0, 0, 20), S.Lambertian((0.8, 0.8, 0.8)))],
+                 [], S.Background.envmap(env), name="small_env")
+    cam = S.Camera((0.0, 0.6, -3.6), (0.0, -0.1, 0.0), (0, 1, 0), 40.0, W / H)
+    return sc, cam
+
+
+def test_env_mesh_rough_dielectric_same_streams_and_three_sigma(gpu_renderer):
+    sc, cam = small_env_scene()
+    gpu_renderer.upload(sc)
+    orc = O.OracleScene(sc)
+    g = passes_gpu(gpu_renderer, cam, 48, 48, 4, 12)
+    c = passes_cpu(orc, cam, 48, 48, 4, 12, mode=1)
+    close = np.isclose(g, c, rtol=1e-3, atol=1e-4).all(axis=-1)
+    assert close.mean() > 0.9, close.mean()
+    K = 48
+    g = passes_gpu(gpu_renderer, cam, 32, 32, K, 12, seed=2)
+    c = passes_cpu(orc, cam, 32, 32, K, 12, mode=0)
+    z = three_sigma(g, c)
+    assert (z > 3).mean() <= 0.02 and (z > 6).mean() <= 0.002, ((z > 3).mean(), (z > 6).mean())
+
+
+def gem_small(W, H, photons):
+    sc, cam, cfg = scenes.gem_caustics(W, H, num_photons=photons)
+    return sc, cam, cfg
+
+
+def test_photon_map_matches_checker_and_brute_force(gpu_renderer):
+    from rustracer_amd import _ffi as F
+    sc, cam, cfg = gem_small(32, 32, 200000)
+    gpu_renderer.upload(sc)
+    lq = cfg["light_quad"]
+    ng = gpu_renderer.photon_build_from_quad(lq["q"], lq["u"], lq["v"], cfg["light_color"], 200000, cfg["gather_radius"])
+    orc = O.OracleScene(sc)
+    pp = F.rt_photon_params()
+    pp.emitter = F.EMIT_QUAD
+    for k in range(3):
+        pp.origin[k], pp.u[k], pp.v[k], pp.color[k] = lq["q"][k], lq["u"][k], lq["v"][k], cfg["light_color"][k]
+    pp.num_photons, pp.gather_radius = 200000, cfg["gather_radius"]
+    nc = orc.photon_build(pp)
+    assert ng > 500 and abs(ng - nc) <= 0.02 * nc + 5, (ng, nc)
+    ph = gpu_renderer.photon_export()
+    # same photon positions as the checker (same per-photon streams), up to ulps
+    pc = orc.photon_dump(nc)
+    a = np.sort(ph[:, 0] * 1e3 + ph[:, 2])
+    b = np.sort(pc[:, 0] * 1e3 + pc[:, 2])
+    m = min(len(a), len(b))
+    assert np.mean(np.isclose(a[:m], b[:m], atol=1e-3)) > 0.95
+    # kd gather equals the O(N) brute-force Epanechnikov sum over the same photons
+    g = np.random.default_rng(4)
+    q = np.zeros((400, 6), np.float32)
+    q[:, [0, 2]] = g.uniform(-0.6, 0.6, size=(400, 2))
+    q[:, 4] = 1.0
+    irr = gpu_renderer.photon_irradiance(q)
+    r2 = cfg["gather_radius"] ** 2
+    d2 = ((ph[None, :, :3] - q[:, None, :3]) ** 2).sum(-1)
+    w = np.where((d2 < r2) & ((ph[None, :, 6:9] * q[:, None, 3:6]).sum(-1) > 0), 1 - d2 / r2, 0)
+    brute = (w @ ph[:, 3:6]) * (2.0 / (np.pi ** 2 * r2))
+    assert np.allclose(irr, brute, rtol=1e-4, atol=1e-6)
+    assert (irr.sum(1) > 0).mean() > 0.2
+
+
+def test_gem_with_photons_three_sigma(gpu_renderer):
+    from rustracer_amd import _ffi as F
+    sc, cam, cfg = gem_small(32, 32, 100000)
+    gpu_renderer.upload(sc)
+    lq = cfg["light_quad"]
+    gpu_renderer.photon_build_from_quad(lq["q"], lq["u"], lq["v"], cfg["light_color"], 100000, cfg["gather_radius"])
+    orc = O.OracleScene(sc)
+    pp = F.rt_photon_params()
+    pp.emitter = F.EMIT_QUAD
+    for k in range(3):
+        pp.origin[k], pp.u[k], pp.v[k], pp.color[k] = lq["q"][k], lq["u"][k], lq["v"][k], cfg["light_color"][k]
+    pp.num_photons, pp.gather_radius = 100000, cfg["gather_radius"]
+    orc.photon_build(pp)
+    K = 64
+    g = passes_gpu(gpu_renderer, cam, 32, 32, K, 24, seed=3, photons=True)
+    c = passes_cpu(orc, cam, 32, 32, K, 24, mode=0, photons=True)
+    assert np.isfinite(g).all()
+    z = three_sigma(g, c)
+    assert (z > 3).mean() <= 0.02 and (z > 6).mean() <= 0.002, ((z > 3).mean(), (z > 6).mean())
