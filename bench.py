@@ -45,6 +45,11 @@ def main():
     ap.add_argument("--config", default="box", choices=sorted(workloads()))
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--renderer", default="both", choices=["both", "wavefront", "plain"])
+    ap.add_argument("--wf-capacity", type=int, default=0)
+    ap.add_argument("--wf-grid", type=int, default=0)
+    ap.add_argument("--wf-sort", type=int, default=1)
+    ap.add_argument("--wf-graph", type=int, default=1)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (every box here has one MI355X)",
@@ -73,48 +78,82 @@ def main():
         photon_s = time.time() - t0
     acc = torch.zeros((H, W, 3), dtype=torch.float32, device="cuda")
 
-    def step(k):
+    def step_plain(k):
         r.render_accumulate(acc, cam, W, H, spp, sample_idx=0, strata=strata, max_depth=depth, seed=k,
                             photon_map=photons)
 
-    for k in range(args.warmup):
-        step(k)
-    torch.cuda.synchronize()
+    def step_wf(k):
+        r.render_wavefront(acc, cam, W, H, spp, sample_idx=0, strata=strata, max_depth=depth, seed=k,
+                           photon_map=photons, capacity=args.wf_capacity, grid=args.wf_grid, sort=bool(args.wf_sort),
+                           use_graph=bool(args.wf_graph))
 
-    # counting pass (untimed): traversal work per sample
+    def timed(step):
+        for k in range(args.warmup):
+            step(k)
+        torch.cuda.synchronize()
+        r.set_timing(True)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for k in range(args.steps):
+            step(1000 + k)
+        torch.cuda.synchronize()
+        el = time.perf_counter() - t0
+        kms, nl = r.timing()
+        r.set_timing(False)
+        img = acc.cpu().numpy() / spp
+        assert np.isfinite(img).all(), "non-finite radiance"
+        return el, kms, nl
+
+    # counting pass (untimed): traversal work per sample, plain megakernel
     r.set_counting(True)
-    step(0)
+    step_plain(0)
     torch.cuda.synchronize()
     cnt = r.counters()
     r.set_counting(False)
 
-    r.set_timing(True)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for k in range(args.steps):
-        step(1000 + k)
-    torch.cuda.synchronize()
-    elapsed = time.perf_counter() - t0
-    kern_ms, launches = r.timing()
-    r.set_timing(False)
-    img = acc.cpu().numpy() / spp
-    assert np.isfinite(img).all(), "non-finite radiance"
-
     samples = W * H * spp
+    res = {}
+    if args.renderer in ("both", "plain"):
+        res["plain"] = timed(step_plain)
+    if args.renderer in ("both", "wavefront"):
+        res["wavefront"] = timed(step_wf)
+    head = "wavefront" if "wavefront" in res else "plain"
+    elapsed, kern_ms, launches = res[head]
     value = samples * args.steps / elapsed
-    per_sample_nodes = cnt["node_tests"] / max(cnt["samples"], 1)
-    per_sample_prims = cnt["prim_tests"] / max(cnt["samples"], 1)
-    bytes_per_sample = per_sample_nodes * 256.0 + per_sample_prims * PRIM_BYTES.get(args.config, 64.0)
-    samples_per_launch = samples * args.steps / max(launches, 1)
-    avg_launch_s = kern_ms / 1e3 / max(launches, 1)
-    achieved = bytes_per_sample * samples_per_launch / avg_launch_s / 1e9
-
+    ns = max(cnt["samples"], 1)
+    pb = PRIM_BYTES.get(args.config, 64.0)
+    closest_bps = (cnt["node_tests"] * 256.0 + cnt["prim_tests"] * pb) / ns
+    shadow_bps = (cnt["shadow_node_tests"] * 256.0 + cnt["shadow_prim_tests"] * pb) / ns
+    roof = {}
+    if "plain" in res:
+        el, kms, nl = res["plain"]
+        spl = samples * args.steps / max(nl, 1)
+        a = (closest_bps + shadow_bps) * spl / (kms / 1e3 / max(nl, 1)) / 1e9
+        roof["plain"] = {"kernel": "k_render", "avg_launch_ms": kms / max(nl, 1), "bytes_per_launch": (closest_bps + shadow_bps) * spl,
+                         "achieved": a}
+    if "wavefront" in res:
+        # per-launch HIP events around k_wf_extend need the non-graph launch path
+        r.set_timing(True)
+        for k in range(max(1, args.steps)):
+            r.render_wavefront(acc, cam, W, H, spp, sample_idx=0, strata=strata, max_depth=depth, seed=2000 + k,
+                               photon_map=photons, capacity=args.wf_capacity, grid=args.wf_grid,
+                               sort=bool(args.wf_sort), use_graph=False)
+        torch.cuda.synchronize()
+        ems, en = r.stage_timing()
+        r.set_timing(False)
+        frames = max(1, args.steps)
+        bpl = closest_bps * samples * frames / max(en, 1)
+        roof["wavefront"] = {"kernel": "k_wf_extend", "avg_launch_ms": ems / max(en, 1), "bytes_per_launch": bpl,
+                             "achieved": bpl / (ems / 1e3 / max(en, 1)) / 1e9, "launches": en}
+    rh = roof[head]
+    achieved = rh["achieved"]
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(scene, cam, cfg, args.cpu_seconds, photons)
 
+    rates = {k: samples * args.steps / v[0] for k, v in res.items()}
     line = {
-        "metric": "GPU samples/s (plain megakernel) at fixed RMSE",
+        "metric": "GPU samples/s (wavefront vs plain) at fixed RMSE",
         "value": value,
         "unit": "samples/s",
         "n_gpus": 1,
@@ -127,16 +166,19 @@ def main():
         "dtype": "f32",
         "data": "synthetic",
         "config": {"workload": scene.name, "width": W, "height": H, "spp": spp, "max_depth": depth,
-                   "renderer": "plain megakernel", "bvh_nodes": stats["nodes"], "triangles": stats["triangles"],
+                   "renderer": head, "samples_per_s": rates,
+                   "wavefront_over_plain": (rates["wavefront"] / rates["plain"]) if len(rates) == 2 else None,
+                   "bvh_nodes": stats["nodes"], "triangles": stats["triangles"],
                    "upload_s": round(upload_s, 3),
                    "photons": None if not photons else {"traced": cfg["num_photons"], "stored": stored,
                                                         "build_s": round(photon_s, 3)}},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": "k_render", "avg_launch_ms": kern_ms / max(launches, 1),
-                     "bytes_per_sample": bytes_per_sample, "node_tests_per_sample": per_sample_nodes,
-                     "prim_tests_per_sample": per_sample_prims,
-                     "rays_per_sample": cnt["rays"] / max(cnt["samples"], 1)},
+                     "kernel": rh["kernel"], "avg_launch_ms": rh["avg_launch_ms"], "bytes_per_launch": rh["bytes_per_launch"],
+                     "closest_bytes_per_sample": closest_bps, "shadow_bytes_per_sample": shadow_bps,
+                     "node_tests_per_sample": cnt["node_tests"] / ns, "prim_tests_per_sample": cnt["prim_tests"] / ns,
+                     "rays_per_sample": cnt["rays"] / ns, "shadow_rays_per_sample": cnt["shadow_rays"] / ns,
+                     "per_renderer": roof},
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)

@@ -230,6 +230,17 @@ typedef struct {
     uint64_t seed;           /* mixed into every per-pixel stream             */
 } rt_sample_params;
 
+/* ---- wavefront renderer options ----------------------------------------------
+ * capacity: paths in flight per chunk (0: 4M); grid: blocks of the persistent
+ * stage kernels (0: 2048); sort: 1 files paths by material before shading;
+ * use_graph: 1 captures one chunk's launches in a hipGraph and replays it.   */
+typedef struct {
+    uint32_t capacity;
+    uint32_t grid;
+    uint32_t sort;
+    uint32_t use_graph;
+} rt_wavefront_opts;
+
 /* ---- caustic photon map (src/photon.rs, src/scene.rs:35-74) --------------- */
 enum { RT_EMIT_QUAD = 0, RT_EMIT_SUN_DISK = 1 };
 
@@ -279,6 +290,15 @@ int  rt_render_accumulate(rt_ctx *ctx, rt_scene *scene, const rt_camera *cam,
                           const rt_sample_params *params, uint32_t spp,
                           float *accum_rgb);
 
+/* The same frame as rt_render_accumulate (same per-pixel streams, same sums up
+ * to float atomics' order), rendered as a wavefront: persistent stage kernels
+ * (extend / shade / shadow) over device-resident path queues with compaction
+ * and material sorting.  opts may be NULL.  No counterpart in the reference,
+ * which renders each pixel's path in one loop (src/renderer.rs:379-582).     */
+int  rt_render_wavefront(rt_ctx *ctx, rt_scene *scene, const rt_camera *cam,
+                         const rt_sample_params *params, uint32_t spp,
+                         float *accum_rgb, const rt_wavefront_opts *opts);
+
 /* Trace photons and build the kd-tree on the device: PhotonMap::build /
  * build_from_quad (src/photon.rs:41-135, 165-191).  The map stays attached to
  * the scene; rt_sample_params.use_photons enables the gather.                */
@@ -303,9 +323,13 @@ int  rt_ctx_get_timing(rt_ctx *ctx, double *kernel_ms, uint64_t *launches);
 /* Counting pass: renders add their traversal work to device counters.
  * Enabling zeroes them.                                                       */
 int  rt_ctx_set_counting(rt_ctx *ctx, int enable);
-/* out[0] BVH node tests, [1] primitive tests, [2] rays traced (closest +
- * shadow), [3] samples rendered, since counting was enabled.                 */
-int  rt_ctx_get_counters(rt_ctx *ctx, uint64_t out[4]);
+/* Megakernel renders since counting was enabled: out[0] BVH node tests,
+ * [1] primitive tests, [2] rays of closest-hit queries, [3] samples,
+ * [4] node tests, [5] primitive tests, [6] rays of shadow queries, [7] 0.    */
+int  rt_ctx_get_counters(rt_ctx *ctx, uint64_t out[8]);
+/* Wavefront renders with timing on and graphs off: summed duration (ms) and
+ * count of the k_wf_extend launches (closest-hit traversal stage).           */
+int  rt_ctx_get_stage_timing(rt_ctx *ctx, double *extend_ms, uint64_t *extend_launches);
 
 #ifdef __cplusplus
 }

@@ -86,6 +86,10 @@ class rt_sample_params(C.Structure):
                 ("use_photons", C.c_uint32), ("seed", C.c_uint64)]
 
 
+class rt_wavefront_opts(C.Structure):
+    _fields_ = [("capacity", C.c_uint32), ("grid", C.c_uint32), ("sort", C.c_uint32), ("use_graph", C.c_uint32)]
+
+
 class rt_photon_params(C.Structure):
     _fields_ = [("emitter", C.c_uint32), ("origin", C.c_float * 3), ("u", C.c_float * 3),
                 ("v", C.c_float * 3), ("color", C.c_float * 3), ("sun_dir", C.c_float * 3),
@@ -119,6 +123,8 @@ SIGNATURES = {
     "rt_transmittance": (C.c_int, [VP, VP, VP, C.c_uint32, VP]),
     "rt_render_sample": (C.c_int, [VP, VP, C.POINTER(rt_camera), C.POINTER(rt_sample_params), VP, VP]),
     "rt_render_accumulate": (C.c_int, [VP, VP, C.POINTER(rt_camera), C.POINTER(rt_sample_params), C.c_uint32, VP]),
+    "rt_render_wavefront": (C.c_int, [VP, VP, C.POINTER(rt_camera), C.POINTER(rt_sample_params), C.c_uint32, VP,
+                                      C.POINTER(rt_wavefront_opts)]),
     "rt_photon_build": (C.c_int, [VP, VP, C.POINTER(rt_photon_params), C.POINTER(C.c_uint32)]),
     "rt_photon_irradiance": (C.c_int, [VP, VP, VP, C.c_uint32, VP]),
     "rt_photon_export": (C.c_int, [VP, VP, VP, C.c_uint32, C.POINTER(C.c_uint32)]),
@@ -126,6 +132,7 @@ SIGNATURES = {
     "rt_ctx_get_timing": (C.c_int, [VP, C.POINTER(C.c_double), C.POINTER(C.c_uint64)]),
     "rt_ctx_set_counting": (C.c_int, [VP, C.c_int]),
     "rt_ctx_get_counters": (C.c_int, [VP, C.POINTER(C.c_uint64)]),
+    "rt_ctx_get_stage_timing": (C.c_int, [VP, C.POINTER(C.c_double), C.POINTER(C.c_uint64)]),
 }
 
 _lib = None

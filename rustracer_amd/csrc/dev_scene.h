@@ -82,6 +82,7 @@ struct Stack {
     int32_t priv[RT_PRIV_STACK];
     int32_t sp;
     uint32_t n_nodes, n_prims, n_rays;   // traversal work, for the counting pass
+    uint32_t s_nodes, s_prims, s_rays;   // the same for shadow (transmittance) queries
 };
 RT_D void st_push(Stack& s, int32_t v) {
     if (s.sp < RT_LDS_STACK) s.lds[s.sp * s.stride] = v;
@@ -472,9 +473,17 @@ RT_D float object_transmittance(const DScene& S, const DObject& ob, const Ray& w
 }
 
 // BvhTree::transmittance (src/bvh.rs:532-575); any_hit when `any_only`.
+RT_D float trace_transmittance_impl(const DScene& S, const Ray& r, float t_min, float t_max, Stack& st, bool any_only);
+// Shadow queries count their work separately from closest-hit queries.
 RT_D float trace_transmittance(const DScene& S, const Ray& r, float t_min, float t_max, Stack& st, bool any_only) {
+    uint32_t n0 = st.n_nodes, p0 = st.n_prims;
+    float tr = trace_transmittance_impl(S, r, t_min, t_max, st, any_only);
+    st.s_nodes += st.n_nodes - n0; st.s_prims += st.n_prims - p0; st.s_rays++;
+    st.n_nodes = n0; st.n_prims = p0;
+    return tr;
+}
+RT_D float trace_transmittance_impl(const DScene& S, const Ray& r, float t_min, float t_max, Stack& st, bool any_only) {
     float tr = 1.0f;
-    st.n_rays++;
     st.sp = 0;
     st_push(st, 0);
     while (st.sp > 0) {

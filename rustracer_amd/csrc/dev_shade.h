@@ -714,133 +714,169 @@ RT_D V3 photon_irradiance(const DScene& S, V3 pos, V3 nrm) {
     return acc * S.ph_norm;
 }
 
-// ---- ray_color (src/renderer.rs:371-585) ------------------------------------------------------
-RT_D V3 ray_color(const DScene& S, Ray ray, int max_depth, float bg_scale, bool use_photons, Rng& rng, Stack& st) {
-    V3 thr = mk(1, 1, 1), color = mk(0, 0, 0);
-    bool prev_specular = true;
-    float prev_mis = 1.0f, prev_sun_w = 1.0f;
+// ---- one bounce of ray_color (src/renderer.rs:371-585) ---------------------------------------
+// The path loop is split at its trace calls so the same code serves the
+// megakernel (which resolves shadow rays immediately) and the wavefront stages
+// (which queue them).  Random numbers are drawn in the reference's order: a
+// shadow ray's transmittance never draws, so deferring it leaves the stream
+// unchanged.
+struct PathState {
+    Ray ray;
+    V3 thr;
+    float prev_mis, prev_sun_w;
+    bool prev_specular;
+};
+
+// A deferred shadow query: `contrib` is added (times the transmittance) when
+// the segment [0.001, t_max] toward `dir` is not blocked.
+struct ShadowReq {
+    V3 o, d;
+    float t_max, time;
+    V3 contrib;
+};
+
+RT_D void path_init(PathState& ps, const Ray& r) {
+    ps.ray = r; ps.thr = mk(1, 1, 1); ps.prev_specular = true; ps.prev_mis = 1.0f; ps.prev_sun_w = 1.0f;
+}
+
+// Shade the hit (or miss) `id` of ps.ray at bounce `depth`.  Adds direct
+// emission / background to `color`, appends up to three shadow requests and
+// advances the path.  Returns false when the path ends.
+RT_D bool bounce(const DScene& S, PathState& ps, const HitId& id, int depth, float bg_scale, bool use_photons, Rng& rng,
+                 V3& color, ShadowReq* sh, int& nsh, Stack& st) {
+    Ray& ray = ps.ray;
     const bool lights_empty = S.num_lights == 0;
     const bool sun = physical_sun(S);
     const bool env = has_env(S);
+    nsh = 0;
+    if (id.slot < 0) {
+        float sp = sun_pdf_value(S, ray.d);
+        float ep = env_pdf_value(S, ray.d);
+        float ew = (sp + ep == 0.0f) ? 1.0f : (ps.prev_specular ? ps.prev_sun_w : ps.prev_mis);
+        V3 bg = (depth == 0 || !env) ? bg_eval(S, ray.d) : env_eval(S, ray.d);
+        color += ps.thr * bg * bg_scale * ew;
+        return false;
+    }
+    Rec h = finalize(S, ray, id, 0.001f);
+    float ew = (ps.prev_specular || lights_empty) ? 1.0f : ps.prev_mis;
+    color += ps.thr * mat_emitted(S, h) * ew;
+    Scatter sr;
+    if (!mat_scatter(S, h, ray, rng, sr)) return false;
+    if (sr.skip_pdf) {
+        ps.prev_sun_w = 1.0f;
+        if (sun) {
+            float p_sun = 1.0f / (2.0f * PI * (1.0f - COS_SUN_MAX));
+            V3 su = unit(mk(S.sun_dir[0], S.sun_dir[1], S.sun_dir[2]));
+            V3 ss = sample_sun_cone(su, COS_SUN_MAX, rng);
+            V3 bc = mat_spec_brdf_cos(S, h, ray, ss);
+            if (bc.x > 0.0f || bc.y > 0.0f || bc.z > 0.0f) {
+                float pm = mat_spec_pdf(S, h, ray, ss);
+                V3 sc = bg_eval(S, ss);
+                float md = p_sun * p_sun + pm * pm;
+                if (md > 0.0f) { sh[nsh].o = h.p; sh[nsh].d = ss; sh[nsh].t_max = FINF; sh[nsh].time = ray.time; sh[nsh].contrib = ps.thr * bc * sc * p_sun / md; nsh++; }
+            }
+            float pmb = mat_spec_pdf(S, h, ray, sr.ray.d);
+            if (pmb > 0.0f) ps.prev_sun_w = (pmb * pmb) / fmx(pmb * pmb + p_sun * p_sun, 1e-8f);
+        }
+        ps.thr *= sr.att;
+        ray = sr.ray;
+        ps.prev_specular = true;
+        ps.prev_mis = 1.0f;
+    } else {
+        if (use_photons) {
+            V3 irr = photon_irradiance(S, h.p, h.n);
+            if (irr.x > 0.0f || irr.y > 0.0f || irr.z > 0.0f) color += ps.thr * mat_albedo_hint(S, h) * irr;
+        }
+        if (!lights_empty) {
+            V3 ldir = lights_generate(S, h.p, rng, ray.time);
+            Ray shadow = make_ray(h.p, ldir, ray.time);
+            Rec lrec;
+            if (lights_closest(S, shadow, 0.001f, FINF, lrec)) {
+                float l_pdf = lights_pdf_value(S, h.p, ldir, ray.time, st);
+                float brdf = mat_scattering_pdf(h, shadow.d);
+                float sp = sun_pdf_value(S, ldir);
+                float ep = env_pdf_value(S, ldir);
+                float md = l_pdf * l_pdf + brdf * brdf + sp * sp + ep * ep;
+                if (md > 0.0f && brdf > 0.0f) {
+                    sh[nsh].o = h.p; sh[nsh].d = ldir; sh[nsh].t_max = lrec.t * (1.0f - 1e-4f); sh[nsh].time = ray.time;
+                    sh[nsh].contrib = ps.thr * sr.att * brdf * l_pdf * mat_emitted(S, lrec) / md;
+                    nsh++;
+                }
+            }
+        }
+        if (sun) {
+            V3 su = unit(mk(S.sun_dir[0], S.sun_dir[1], S.sun_dir[2]));
+            V3 ss = sample_sun_cone(su, COS_SUN_MAX, rng);
+            float brdf = mat_scattering_pdf(h, ss);
+            if (brdf > 0.0f) {
+                float spdf = 1.0f / (2.0f * PI * (1.0f - COS_SUN_MAX));
+                V3 sc = bg_eval(S, ss);
+                float ap = lights_empty ? 0.0f : lights_pdf_value(S, h.p, ss, ray.time, st);
+                float md = spdf * spdf + brdf * brdf + ap * ap;
+                sh[nsh].o = h.p; sh[nsh].d = ss; sh[nsh].t_max = FINF; sh[nsh].time = ray.time;
+                sh[nsh].contrib = ps.thr * sr.att * brdf * spdf * sc / md;
+                nsh++;
+            }
+        }
+        if (env) {
+            float epdf;
+            V3 ed = env_sample_dir(S, rng, epdf);
+            if (epdf > 0.0f) {
+                float brdf = mat_scattering_pdf(h, ed);
+                if (brdf > 0.0f) {
+                    float ap = lights_empty ? 0.0f : lights_pdf_value(S, h.p, ed, ray.time, st);
+                    V3 ec = env_eval(S, ed);
+                    float md = epdf * epdf + brdf * brdf + ap * ap;
+                    sh[nsh].o = h.p; sh[nsh].d = ed; sh[nsh].t_max = FINF; sh[nsh].time = ray.time;
+                    sh[nsh].contrib = ps.thr * sr.att * brdf * epdf * ec / md;
+                    nsh++;
+                }
+            }
+        }
+        Onb uvw = onb_from_w(h.n);
+        float r1 = rng_f(rng), r2 = rng_f(rng);
+        float phi = 2.0f * PI * r1;
+        float sr2 = sqrtf(r2);
+        V3 ind = onb_local(uvw, mk(cosf(phi) * sr2, sinf(phi) * sr2, sqrtf(1.0f - r2)));
+        float pdf_val = fmx(dot(unit(ind), uvw.w) / PI, 0.0f);
+        if (pdf_val <= 0.0f) return false;
+        Ray sc = scatter_from(h.p, ind, ray);
+        float scat_pdf = mat_scattering_pdf(h, sc.d);
+        if (scat_pdf <= 0.0f) return false;
+        if (dot(ind, h.n) <= 0.0f) return false;
+        float a = lights_empty ? 0.0f : lights_pdf_value(S, h.p, ind, ray.time, st);
+        float s = sun_pdf_value(S, ind);
+        float e = env_pdf_value(S, ind);
+        float nee = a * a + s * s + e * e;
+        ps.prev_mis = (pdf_val * pdf_val) / fmx(pdf_val * pdf_val + nee, 1e-8f);
+        ps.thr *= sr.att * (scat_pdf / pdf_val);
+        ray = sc;
+        ps.prev_specular = false;
+    }
+    if (depth >= 2) {
+        float survive = fmx(fmx(ps.thr.x, ps.thr.y), ps.thr.z);
+        float p = fmn(survive, 1.0f);
+        if (survive <= 0.0f || rng_f(rng) >= p) return false;
+        ps.thr /= p;
+    }
+    return true;
+}
+
+// Megakernel form of ray_color: trace, shade, resolve shadow rays inline.
+RT_D V3 ray_color(const DScene& S, Ray ray, int max_depth, float bg_scale, bool use_photons, Rng& rng, Stack& st) {
+    PathState ps;
+    path_init(ps, ray);
+    V3 color = mk(0, 0, 0);
+    ShadowReq sh[3];
     for (int depth = 0; depth < max_depth; depth++) {
-        HitId id = trace_closest(S, ray, 0.001f, FINF, st);
-        if (id.slot < 0) {
-            float sp = sun_pdf_value(S, ray.d);
-            float ep = env_pdf_value(S, ray.d);
-            float ew = (sp + ep == 0.0f) ? 1.0f : (prev_specular ? prev_sun_w : prev_mis);
-            V3 bg = (depth == 0 || !env) ? bg_eval(S, ray.d) : env_eval(S, ray.d);
-            color += thr * bg * bg_scale * ew;
-            break;
+        HitId id = trace_closest(S, ps.ray, 0.001f, FINF, st);
+        int nsh = 0;
+        bool cont = bounce(S, ps, id, depth, bg_scale, use_photons, rng, color, sh, nsh, st);
+        for (int k = 0; k < nsh; k++) {
+            float tr = trace_transmittance(S, make_ray(sh[k].o, sh[k].d, sh[k].time), 0.001f, sh[k].t_max, st, false);
+            if (tr > 0.0f) color += sh[k].contrib * tr;
         }
-        Rec h = finalize(S, ray, id, 0.001f);
-        float ew = (prev_specular || lights_empty) ? 1.0f : prev_mis;
-        color += thr * mat_emitted(S, h) * ew;
-        Scatter sr;
-        if (!mat_scatter(S, h, ray, rng, sr)) break;
-        if (sr.skip_pdf) {
-            prev_sun_w = 1.0f;
-            if (sun) {
-                float p_sun = 1.0f / (2.0f * PI * (1.0f - COS_SUN_MAX));
-                V3 su = unit(mk(S.sun_dir[0], S.sun_dir[1], S.sun_dir[2]));
-                V3 ss = sample_sun_cone(su, COS_SUN_MAX, rng);
-                Ray shadow = make_ray(h.p, ss, ray.time);
-                float tr = trace_transmittance(S, shadow, 0.001f, FINF, st, false);
-                if (tr > 0.0f) {
-                    V3 bc = mat_spec_brdf_cos(S, h, ray, ss);
-                    if (bc.x > 0.0f || bc.y > 0.0f || bc.z > 0.0f) {
-                        float pm = mat_spec_pdf(S, h, ray, ss);
-                        V3 sc = bg_eval(S, ss);
-                        float md = p_sun * p_sun + pm * pm;
-                        if (md > 0.0f) color += thr * tr * bc * sc * p_sun / md;
-                    }
-                }
-                float pmb = mat_spec_pdf(S, h, ray, sr.ray.d);
-                if (pmb > 0.0f) prev_sun_w = (pmb * pmb) / fmx(pmb * pmb + p_sun * p_sun, 1e-8f);
-            }
-            thr *= sr.att;
-            ray = sr.ray;
-            prev_specular = true;
-            prev_mis = 1.0f;
-        } else {
-            if (use_photons) {
-                V3 irr = photon_irradiance(S, h.p, h.n);
-                if (irr.x > 0.0f || irr.y > 0.0f || irr.z > 0.0f) color += thr * mat_albedo_hint(S, h) * irr;
-            }
-            if (!lights_empty) {
-                V3 ldir = lights_generate(S, h.p, rng, ray.time);
-                Ray shadow = make_ray(h.p, ldir, ray.time);
-                Rec lrec;
-                if (lights_closest(S, shadow, 0.001f, FINF, lrec)) {
-                    float tr = trace_transmittance(S, shadow, 0.001f, lrec.t * (1.0f - 1e-4f), st, false);
-                    if (tr > 0.0f) {
-                        float l_pdf = lights_pdf_value(S, h.p, ldir, ray.time, st);
-                        float brdf = mat_scattering_pdf(h, shadow.d);
-                        float sp = sun_pdf_value(S, ldir);
-                        float ep = env_pdf_value(S, ldir);
-                        float md = l_pdf * l_pdf + brdf * brdf + sp * sp + ep * ep;
-                        if (md > 0.0f && brdf > 0.0f) color += thr * tr * sr.att * brdf * l_pdf * mat_emitted(S, lrec) / md;
-                    }
-                }
-            }
-            if (sun) {
-                V3 su = unit(mk(S.sun_dir[0], S.sun_dir[1], S.sun_dir[2]));
-                V3 ss = sample_sun_cone(su, COS_SUN_MAX, rng);
-                Ray shadow = make_ray(h.p, ss, ray.time);
-                float tr = trace_transmittance(S, shadow, 0.001f, FINF, st, false);
-                if (tr > 0.0f) {
-                    float brdf = mat_scattering_pdf(h, ss);
-                    if (brdf > 0.0f) {
-                        float spdf = 1.0f / (2.0f * PI * (1.0f - COS_SUN_MAX));
-                        V3 sc = bg_eval(S, ss);
-                        float ap = lights_empty ? 0.0f : lights_pdf_value(S, h.p, ss, ray.time, st);
-                        float md = spdf * spdf + brdf * brdf + ap * ap;
-                        color += thr * tr * sr.att * brdf * spdf * sc / md;
-                    }
-                }
-            }
-            if (env) {
-                float epdf;
-                V3 ed = env_sample_dir(S, rng, epdf);
-                if (epdf > 0.0f) {
-                    Ray shadow = make_ray(h.p, ed, ray.time);
-                    float tr = trace_transmittance(S, shadow, 0.001f, FINF, st, false);
-                    if (tr > 0.0f) {
-                        float brdf = mat_scattering_pdf(h, ed);
-                        if (brdf > 0.0f) {
-                            float ap = lights_empty ? 0.0f : lights_pdf_value(S, h.p, ed, ray.time, st);
-                            V3 ec = env_eval(S, ed);
-                            float md = epdf * epdf + brdf * brdf + ap * ap;
-                            color += thr * tr * sr.att * brdf * epdf * ec / md;
-                        }
-                    }
-                }
-            }
-            Onb uvw = onb_from_w(h.n);
-            float r1 = rng_f(rng), r2 = rng_f(rng);
-            float phi = 2.0f * PI * r1;
-            float sr2 = sqrtf(r2);
-            V3 ind = onb_local(uvw, mk(cosf(phi) * sr2, sinf(phi) * sr2, sqrtf(1.0f - r2)));
-            float pdf_val = fmx(dot(unit(ind), uvw.w) / PI, 0.0f);
-            if (pdf_val <= 0.0f) break;
-            Ray sc = scatter_from(h.p, ind, ray);
-            float scat_pdf = mat_scattering_pdf(h, sc.d);
-            if (scat_pdf <= 0.0f) break;
-            if (dot(ind, h.n) <= 0.0f) break;
-            float a = lights_empty ? 0.0f : lights_pdf_value(S, h.p, ind, ray.time, st);
-            float s = sun_pdf_value(S, ind);
-            float e = env_pdf_value(S, ind);
-            float nee = a * a + s * s + e * e;
-            prev_mis = (pdf_val * pdf_val) / fmx(pdf_val * pdf_val + nee, 1e-8f);
-            thr *= sr.att * (scat_pdf / pdf_val);
-            ray = sc;
-            prev_specular = false;
-        }
-        if (depth >= 2) {
-            float survive = fmx(fmx(thr.x, thr.y), thr.z);
-            float p = fmn(survive, 1.0f);
-            if (survive <= 0.0f || rng_f(rng) >= p) break;
-            thr /= p;
-        }
+        if (!cont) break;
     }
     return color;
 }

@@ -30,6 +30,7 @@ struct LdsStack {
 __device__ __forceinline__ void init_stack(Stack& st) {
     st.lds = LdsStack::column(); st.stride = BLOCK; st.sp = 0;
     st.n_nodes = 0; st.n_prims = 0; st.n_rays = 0;
+    st.s_nodes = 0; st.s_prims = 0; st.s_rays = 0;
 }
 
 __device__ __forceinline__ Ray ray_from_c(const rt_ray& q) {
@@ -90,6 +91,9 @@ __global__ __launch_bounds__(BLOCK) void k_render(DScene S, DCamera cam, DParams
         atomicAdd(P.counters + 1, (unsigned long long)st.n_prims);
         atomicAdd(P.counters + 2, (unsigned long long)st.n_rays);
         atomicAdd(P.counters + 3, (unsigned long long)((conv && conv[i]) ? 0u : spp));
+        atomicAdd(P.counters + 4, (unsigned long long)st.s_nodes);
+        atomicAdd(P.counters + 5, (unsigned long long)st.s_prims);
+        atomicAdd(P.counters + 6, (unsigned long long)st.s_rays);
     }
 }
 
@@ -184,6 +188,227 @@ __global__ __launch_bounds__(BLOCK) void k_photon_irr(DScene S, const float* __r
     out[3 * i] = c.x; out[3 * i + 1] = c.y; out[3 * i + 2] = c.z;
 }
 
+// ============================================================================ wavefront
+// Path records live in HBM (128 B each, one cache line); queues hold path ids.
+// Per bounce: k_wf_extend traces every active path and files its id under the
+// material key of the hit (the sort), k_wf_shade runs one bounce in key order
+// and appends survivors and shadow requests, k_wf_shadow resolves the shadow
+// requests.  All three are persistent grid-stride kernels that read their
+// queue length from device memory, so the host never synchronises inside a
+// frame.  Contributions go straight into the frame accumulator with float
+// atomics.
+#define WF_NB 10          // sort keys: 0 miss, 1..7 material kind, 8 medium event, 9 spare
+#define WF_CNT_ACTIVE 0   // counters[0]: paths to extend this bounce
+#define WF_CNT_NEXT 1     // counters[1]: survivors written by shade
+#define WF_CNT_SHADOW 2   // counters[2]: shadow requests
+#define WF_CNT_BUCKET 8   // counters[8..8+WF_NB): bucket sizes
+
+struct __align__(16) WfPath {
+    float4 o_time;       // ray origin, time
+    float4 d_wl;         // ray direction, wavelength
+    float4 thr_mis;      // throughput, prev_mis
+    float sun_w; uint32_t flags; uint32_t pixel; uint32_t pad0;   // flags: 1 spectral, 2 prev_specular
+    uint64_t rng[4];
+    float hit_t; int32_t hit_slot; int32_t hit_prim; uint32_t key;
+    uint32_t pad1[4];
+};
+static_assert(sizeof(WfPath) == 128, "path record is one 128-B line");
+
+struct __align__(16) WfShadow { float4 o_tmax; float4 d_time; float4 c_pixel; };
+
+struct WfQueues {
+    WfPath* paths;
+    uint32_t cap;
+    uint32_t* active;     // cap
+    uint32_t* next;       // cap
+    uint32_t* buckets;    // WF_NB * cap
+    WfShadow* shadow;     // 3 * cap
+    uint32_t* counters;   // 32
+    float* fb;
+    uint32_t sort;        // 0: one bucket (no material sort)
+};
+
+__device__ __forceinline__ uint32_t lane_id() { return __lane_id(); }
+
+// Wave-aggregated append: one atomic per distinct key in the wave.
+__device__ __forceinline__ uint32_t wave_append(uint32_t* counters, uint32_t key, bool valid) {
+    uint32_t slot = 0;
+    uint32_t lane = lane_id();
+    unsigned long long pending = __ballot(valid);
+    while (pending) {
+        int leader = __ffsll(pending) - 1;
+        uint32_t k = __shfl(key, leader);
+        unsigned long long m = __ballot(valid && key == k);
+        uint32_t base = 0;
+        if ((int)lane == leader) base = atomicAdd(&counters[k], (uint32_t)__popcll(m));
+        base = __shfl(base, leader);
+        if (valid && key == k) slot = base + (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
+        pending &= ~m;
+    }
+    return slot;
+}
+
+__device__ __forceinline__ void wf_load(const WfPath& w, PathState& ps, Rng& rng) {
+    ps.ray = make_ray(mk(w.o_time.x, w.o_time.y, w.o_time.z), mk(w.d_wl.x, w.d_wl.y, w.d_wl.z), w.o_time.w);
+    ps.ray.wavelength = w.d_wl.w;
+    ps.ray.spectral = (w.flags & 1u) != 0;
+    ps.thr = mk(w.thr_mis.x, w.thr_mis.y, w.thr_mis.z);
+    ps.prev_mis = w.thr_mis.w;
+    ps.prev_sun_w = w.sun_w;
+    ps.prev_specular = (w.flags & 2u) != 0;
+    rng.s0 = w.rng[0]; rng.s1 = w.rng[1]; rng.s2 = w.rng[2]; rng.s3 = w.rng[3];
+}
+__device__ __forceinline__ void wf_store(WfPath& w, const PathState& ps, const Rng& rng) {
+    w.o_time = make_float4(ps.ray.o.x, ps.ray.o.y, ps.ray.o.z, ps.ray.time);
+    w.d_wl = make_float4(ps.ray.d.x, ps.ray.d.y, ps.ray.d.z, ps.ray.wavelength);
+    w.thr_mis = make_float4(ps.thr.x, ps.thr.y, ps.thr.z, ps.prev_mis);
+    w.sun_w = ps.prev_sun_w;
+    w.flags = (ps.ray.spectral ? 1u : 0u) | (ps.prev_specular ? 2u : 0u);
+    w.rng[0] = rng.s0; w.rng[1] = rng.s1; w.rng[2] = rng.s2; w.rng[3] = rng.s3;
+}
+
+// Zero the per-bounce counters (a kernel, not a memset, so graph replays are safe).
+__global__ void k_wf_reset(uint32_t* counters, uint32_t active) {
+    uint32_t i = threadIdx.x;
+    if (i < 32) counters[i] = (i == WF_CNT_ACTIVE) ? active : 0u;
+}
+// After shade: survivors become the active set, everything else restarts at 0.
+__global__ void k_wf_advance(uint32_t* counters) {
+    uint32_t i = threadIdx.x;
+    __shared__ uint32_t nxt;
+    if (i == 0) nxt = counters[WF_CNT_NEXT];
+    __syncthreads();
+    if (i < 32) counters[i] = (i == WF_CNT_ACTIVE) ? nxt : 0u;
+}
+
+// counters[16..17] hold the frame position of a graph-replayed chunk.
+__global__ void k_wf_set_first(uint32_t* counters, uint64_t first) {
+    if (threadIdx.x == 0) { counters[16] = (uint32_t)first; counters[17] = (uint32_t)(first >> 32); }
+}
+__global__ void k_wf_bump_first(uint32_t* counters, uint32_t n) {
+    if (threadIdx.x == 0) {
+        uint64_t f = ((uint64_t)counters[17] << 32) | counters[16];
+        f += n;
+        counters[16] = (uint32_t)f; counters[17] = (uint32_t)(f >> 32);
+    }
+}
+
+__global__ __launch_bounds__(BLOCK) void k_wf_generate(DCamera cam, DParams P, WfQueues Q, uint64_t first, uint32_t n) {
+    uint32_t i = blockIdx.x * BLOCK + threadIdx.x;
+    if (i >= n) return;
+    if (first == ~0ull) first = ((uint64_t)Q.counters[17] << 32) | Q.counters[16];
+    uint32_t npix = P.width * P.height;
+    uint64_t g = first + i;
+    uint32_t pixel = (uint32_t)(g % npix);
+    uint32_t sidx = P.sample_idx + (uint32_t)(g / npix);
+    uint32_t row = pixel / P.width, col = pixel - row * P.width;
+    Rng rng = rng_seed(pixel_seed(pixel, sidx, P.seed));
+    Ray r = pixel_ray(cam, P, row, col, sidx, rng);
+    PathState ps;
+    path_init(ps, r);
+    WfPath w;
+    wf_store(w, ps, rng);
+    w.pixel = pixel; w.pad0 = 0; w.hit_t = 0.0f; w.hit_slot = -1; w.hit_prim = -1; w.key = 0;
+    Q.paths[i] = w;
+    Q.active[i] = i;
+}
+
+__device__ __forceinline__ uint32_t sort_key(const DScene& S, const HitId& id) {
+    if (id.slot < 0) return 0u;
+    if (id.prim == -2) return 8u;
+    uint32_t k = S.mats[S.objs[id.slot].material].kind;
+    return k < 8u ? k : 9u;
+}
+
+__global__ __launch_bounds__(BLOCK) void k_wf_extend(DScene S, WfQueues Q) {
+    Stack st; init_stack(st);
+    uint32_t count = Q.counters[WF_CNT_ACTIVE];
+    uint32_t stride = gridDim.x * BLOCK;
+    for (uint32_t base = blockIdx.x * BLOCK; base < count; base += stride) {
+        uint32_t i = base + threadIdx.x;
+        bool valid = i < count;
+        uint32_t pid = 0, key = 0;
+        if (valid) {
+            pid = Q.active[i];
+            WfPath& w = Q.paths[pid];
+            float4 o = w.o_time, d = w.d_wl;
+            Ray r = make_ray(mk(o.x, o.y, o.z), mk(d.x, d.y, d.z), o.w);
+            HitId id = trace_closest(S, r, 0.001f, FINF, st);
+            w.hit_t = id.t; w.hit_slot = id.slot; w.hit_prim = id.prim;
+            key = Q.sort ? sort_key(S, id) : 0u;
+        }
+        uint32_t slot = wave_append(Q.counters + WF_CNT_BUCKET, key, valid);
+        if (valid) Q.buckets[(size_t)key * Q.cap + slot] = pid;
+    }
+}
+
+__global__ __launch_bounds__(BLOCK) void k_wf_shade(DScene S, DParams P, WfQueues Q, int depth) {
+    Stack st; init_stack(st);
+    uint32_t off[WF_NB + 1];
+    off[0] = 0;
+    for (int b = 0; b < WF_NB; b++) off[b + 1] = off[b] + Q.counters[WF_CNT_BUCKET + b];
+    uint32_t total = off[WF_NB];
+    uint32_t stride = gridDim.x * BLOCK;
+    const bool last = depth + 1 >= (int)P.max_depth;
+    for (uint32_t base = blockIdx.x * BLOCK; base < total; base += stride) {
+        uint32_t i = base + threadIdx.x;
+        bool valid = i < total;
+        bool cont = false;
+        int nsh = 0;
+        ShadowReq sh[3];
+        uint32_t pid = 0, pixel = 0;
+        if (valid) {
+            int b = 0;
+            while (i >= off[b + 1]) b++;
+            pid = Q.buckets[(size_t)b * Q.cap + (i - off[b])];
+            WfPath& w = Q.paths[pid];
+            PathState ps; Rng rng;
+            wf_load(w, ps, rng);
+            HitId id; id.t = w.hit_t; id.slot = w.hit_slot; id.prim = w.hit_prim;
+            pixel = w.pixel;
+            V3 color = mk(0, 0, 0);
+            cont = bounce(S, ps, id, depth, P.bg_scale, P.use_photons != 0, rng, color, sh, nsh, st) && !last;
+            if (color.x != 0.0f || color.y != 0.0f || color.z != 0.0f) {
+                atomicAdd(Q.fb + 3 * (size_t)pixel, color.x);
+                atomicAdd(Q.fb + 3 * (size_t)pixel + 1, color.y);
+                atomicAdd(Q.fb + 3 * (size_t)pixel + 2, color.z);
+            }
+            if (cont) wf_store(w, ps, rng);
+        }
+        uint32_t nslot = wave_append(Q.counters + WF_CNT_NEXT, 0u, cont);
+        if (cont) Q.next[nslot] = pid;
+        for (int k = 0; k < 3; k++) {
+            bool has = valid && k < nsh;
+            uint32_t s = wave_append(Q.counters + WF_CNT_SHADOW, 0u, has);
+            if (has) {
+                WfShadow q;
+                q.o_tmax = make_float4(sh[k].o.x, sh[k].o.y, sh[k].o.z, sh[k].t_max);
+                q.d_time = make_float4(sh[k].d.x, sh[k].d.y, sh[k].d.z, sh[k].time);
+                q.c_pixel = make_float4(sh[k].contrib.x, sh[k].contrib.y, sh[k].contrib.z, __uint_as_float(pixel));
+                Q.shadow[s] = q;
+            }
+        }
+    }
+}
+
+__global__ __launch_bounds__(BLOCK) void k_wf_shadow(DScene S, WfQueues Q) {
+    Stack st; init_stack(st);
+    uint32_t count = Q.counters[WF_CNT_SHADOW];
+    uint32_t stride = gridDim.x * BLOCK;
+    for (uint32_t i = blockIdx.x * BLOCK + threadIdx.x; i < count; i += stride) {
+        WfShadow q = Q.shadow[i];
+        Ray r = make_ray(mk(q.o_tmax.x, q.o_tmax.y, q.o_tmax.z), mk(q.d_time.x, q.d_time.y, q.d_time.z), q.d_time.w);
+        float tr = trace_transmittance(S, r, 0.001f, q.o_tmax.w, st, false);
+        if (tr > 0.0f) {
+            size_t px = 3 * (size_t)__float_as_uint(q.c_pixel.w);
+            atomicAdd(Q.fb + px, q.c_pixel.x * tr);
+            atomicAdd(Q.fb + px + 1, q.c_pixel.y * tr);
+            atomicAdd(Q.fb + px + 2, q.c_pixel.z * tr);
+        }
+    }
+}
+
+
 // ============================================================================ host side
 namespace {
 
@@ -207,6 +432,15 @@ struct rt_ctx {
     double kernel_ms = 0.0;
     uint64_t launches = 0;
     unsigned long long* counters = nullptr;   // non-null: counting pass
+    // wavefront buffers (grown on demand, kept across frames)
+    void* wf_mem = nullptr;
+    size_t wf_bytes = 0;
+    uint32_t wf_cap = 0;
+    // per-launch events around k_wf_extend (timing on, graphs off)
+    std::vector<hipEvent_t> ext_ev;
+    size_t ext_used = 0;
+    double extend_ms = 0.0;
+    uint64_t extend_launches = 0;
 };
 
 struct rt_scene {
@@ -320,6 +554,8 @@ void rt_ctx_destroy(rt_ctx* ctx) {
     (void)hipSetDevice(ctx->device);
     if (ctx->elut) (void)hipFree(ctx->elut);
     if (ctx->counters) (void)hipFree(ctx->counters);
+    if (ctx->wf_mem) (void)hipFree(ctx->wf_mem);
+    for (hipEvent_t e : ctx->ext_ev) (void)hipEventDestroy(e);
     if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
     if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
@@ -768,6 +1004,125 @@ int rt_render_accumulate(rt_ctx* ctx, rt_scene* s, const rt_camera* cam, const r
     return render_common(ctx, s, cam, p, nullptr, out, spp, true);
 }
 
+
+}  // extern "C"
+
+// ---- wavefront renderer --------------------------------------------------------------------------
+static int wf_buffers(rt_ctx* ctx, uint32_t cap, WfQueues& Q) {
+    auto al = [](size_t x) { return (x + 255) & ~(size_t)255; };
+    size_t b_paths = al((size_t)cap * sizeof(WfPath));
+    size_t b_q = al((size_t)cap * 4);
+    size_t b_buckets = al((size_t)WF_NB * cap * 4);
+    size_t b_shadow = al((size_t)3 * cap * sizeof(WfShadow));
+    size_t b_cnt = 256;
+    size_t total = b_paths + 2 * b_q + b_buckets + b_shadow + b_cnt;
+    if (ctx->wf_bytes < total) {
+        if (ctx->wf_mem) HIPCHECK(ctx, hipFree(ctx->wf_mem));
+        ctx->wf_mem = nullptr; ctx->wf_bytes = 0;
+        hipError_t e = hipMalloc(&ctx->wf_mem, total);
+        if (e != hipSuccess) return fail(ctx, RT_ERR_NOMEM, std::string("wavefront buffers: ") + hipGetErrorString(e));
+        ctx->wf_bytes = total;
+    }
+    char* m = (char*)ctx->wf_mem;
+    Q.paths = (WfPath*)m; m += b_paths;
+    Q.active = (uint32_t*)m; m += b_q;
+    Q.next = (uint32_t*)m; m += b_q;
+    Q.buckets = (uint32_t*)m; m += b_buckets;
+    Q.shadow = (WfShadow*)m; m += b_shadow;
+    Q.counters = (uint32_t*)m;
+    Q.cap = cap;
+    ctx->wf_cap = cap;
+    return RT_OK;
+}
+
+// Launch one chunk: generate `n` paths at frame position `first` (~0: read
+// from the counters, for graph replays) and run every bounce.
+static void wf_chunk(rt_ctx* ctx, rt_scene* s, const DCamera& C, const DParams& P, const WfQueues& Q0,
+                     uint64_t first, uint32_t n, uint32_t grid) {
+    hipStream_t st = ctx->stream;
+    hipLaunchKernelGGL(k_wf_generate, dim3((n + BLOCK - 1) / BLOCK), dim3(BLOCK), 0, st, C, P, Q0, first, n);
+    hipLaunchKernelGGL(k_wf_reset, dim3(1), dim3(32), 0, st, Q0.counters, n);
+    WfQueues A = Q0, B = Q0;
+    B.active = Q0.next; B.next = Q0.active;
+    for (uint32_t d = 0; d < P.max_depth; d++) {
+        const WfQueues& Q = (d & 1) ? B : A;
+        bool ev = ctx->timing && first != ~0ull;
+        if (ev) {
+            while (ctx->ext_ev.size() < ctx->ext_used + 2) { hipEvent_t e; if (hipEventCreate(&e) != hipSuccess) { ev = false; break; } ctx->ext_ev.push_back(e); }
+        }
+        if (ev) (void)hipEventRecord(ctx->ext_ev[ctx->ext_used], st);
+        hipLaunchKernelGGL(k_wf_extend, dim3(grid), dim3(BLOCK), 0, st, s->S, Q);
+        if (ev) { (void)hipEventRecord(ctx->ext_ev[ctx->ext_used + 1], st); ctx->ext_used += 2; }
+        hipLaunchKernelGGL(k_wf_shade, dim3(grid), dim3(BLOCK), 0, st, s->S, P, Q, (int)d);
+        hipLaunchKernelGGL(k_wf_shadow, dim3(grid), dim3(BLOCK), 0, st, s->S, Q);
+        hipLaunchKernelGGL(k_wf_advance, dim3(1), dim3(32), 0, st, Q.counters);
+    }
+}
+
+extern "C" int rt_render_wavefront(rt_ctx* ctx, rt_scene* s, const rt_camera* cam, const rt_sample_params* p, uint32_t spp,
+                                   float* out, const rt_wavefront_opts* opts) {
+    if (!ctx || !s || !cam || !p || !out) return RT_ERR_INVALID;
+    if (p->width == 0 || p->height == 0 || spp == 0) return fail(ctx, RT_ERR_INVALID, "empty frame");
+    if (p->max_depth == 0) return fail(ctx, RT_ERR_INVALID, "max_depth = 0");
+    if (p->use_photons && s->S.num_photons == 0) return fail(ctx, RT_ERR_INVALID, "use_photons without a photon map");
+    uint32_t cap = opts && opts->capacity ? opts->capacity : (1u << 22);
+    uint32_t grid = opts && opts->grid ? opts->grid : 2048u;
+    uint32_t sort = opts ? opts->sort : 1u;
+    uint32_t use_graph = opts ? opts->use_graph : 0u;
+    if (cap < BLOCK || cap > (1u << 26)) return fail(ctx, RT_ERR_INVALID, "wavefront capacity out of range");
+    HIPCHECK(ctx, hipSetDevice(ctx->device));
+    uint32_t npix = p->width * p->height;
+    uint64_t total = (uint64_t)npix * spp;
+    if (cap > total) cap = (uint32_t)((total + BLOCK - 1) / BLOCK * BLOCK);
+    WfQueues Q;
+    int rc;
+    if ((rc = wf_buffers(ctx, cap, Q))) return rc;
+    Q.sort = sort;
+    Staged rout;
+    if ((rc = stage_out(ctx, out, (size_t)npix * 12, rout, false))) return rc;
+    Q.fb = (float*)rout.dev;
+    DCamera C; DParams P;
+    to_dev(cam, p, C, P);
+    P.counters = nullptr;
+    if (ctx->timing) HIPCHECK(ctx, hipEventRecord(ctx->ev0, ctx->stream));
+    HIPCHECK(ctx, hipMemsetAsync(rout.dev, 0, (size_t)npix * 12, ctx->stream));
+    uint64_t full = total / cap, rem = total % cap;
+    uint64_t launches = 0;
+    if (use_graph && full > 0) {
+        hipGraph_t graph = nullptr;
+        hipGraphExec_t exec = nullptr;
+        hipLaunchKernelGGL(k_wf_set_first, dim3(1), dim3(32), 0, ctx->stream, Q.counters, (uint64_t)0);
+        HIPCHECK(ctx, hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+        wf_chunk(ctx, s, C, P, Q, ~0ull, cap, grid);
+        hipLaunchKernelGGL(k_wf_bump_first, dim3(1), dim3(32), 0, ctx->stream, Q.counters, cap);
+        HIPCHECK(ctx, hipStreamEndCapture(ctx->stream, &graph));
+        HIPCHECK(ctx, hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+        for (uint64_t c = 0; c < full; c++) HIPCHECK(ctx, hipGraphLaunch(exec, ctx->stream));
+        HIPCHECK(ctx, hipGraphExecDestroy(exec));
+        HIPCHECK(ctx, hipGraphDestroy(graph));
+    } else {
+        for (uint64_t c = 0; c < full; c++) wf_chunk(ctx, s, C, P, Q, c * (uint64_t)cap, cap, grid);
+    }
+    launches += full * (2 + 4ull * P.max_depth);
+    if (rem) { wf_chunk(ctx, s, C, P, Q, full * (uint64_t)cap, (uint32_t)rem, grid); launches += 2 + 4ull * P.max_depth; }
+    HIPCHECK(ctx, hipGetLastError());
+    if (ctx->timing) {
+        HIPCHECK(ctx, hipEventRecord(ctx->ev1, ctx->stream));
+        HIPCHECK(ctx, hipEventSynchronize(ctx->ev1));
+        float ms = 0.0f;
+        HIPCHECK(ctx, hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+        ctx->kernel_ms += ms;
+        ctx->launches += full + (rem ? 1 : 0);   // one "launch" = one chunk of paths through every bounce
+        for (size_t k = 0; k + 1 < ctx->ext_used; k += 2) {
+            float e = 0.0f;
+            if (hipEventElapsedTime(&e, ctx->ext_ev[k], ctx->ext_ev[k + 1]) == hipSuccess) { ctx->extend_ms += e; ctx->extend_launches++; }
+        }
+    }
+    ctx->ext_used = 0;
+    return finish_out(ctx, out, (size_t)npix * 12, rout);
+}
+
+extern "C" {
 // ---- photon map --------------------------------------------------------------------------
 static void kd_build(float4* ph, size_t n) {
     if (n == 0) return;
@@ -909,6 +1264,15 @@ int rt_ctx_set_timing(rt_ctx* ctx, int enable) {
     ctx->timing = enable != 0;
     ctx->kernel_ms = 0.0;
     ctx->launches = 0;
+    ctx->extend_ms = 0.0;
+    ctx->extend_launches = 0;
+    return RT_OK;
+}
+
+int rt_ctx_get_stage_timing(rt_ctx* ctx, double* extend_ms, uint64_t* extend_launches) {
+    if (!ctx || !extend_ms || !extend_launches) return RT_ERR_INVALID;
+    *extend_ms = ctx->extend_ms;
+    *extend_launches = ctx->extend_launches;
     return RT_OK;
 }
 
@@ -932,10 +1296,10 @@ int rt_ctx_set_counting(rt_ctx* ctx, int enable) {
     return RT_OK;
 }
 
-int rt_ctx_get_counters(rt_ctx* ctx, uint64_t out[4]) {
+int rt_ctx_get_counters(rt_ctx* ctx, uint64_t out[8]) {
     if (!ctx || !out) return RT_ERR_INVALID;
     if (!ctx->counters) return fail(ctx, RT_ERR_INVALID, "counting is off");
-    HIPCHECK(ctx, hipMemcpy(out, ctx->counters, 4 * sizeof(uint64_t), hipMemcpyDeviceToHost));
+    HIPCHECK(ctx, hipMemcpy(out, ctx->counters, 8 * sizeof(uint64_t), hipMemcpyDeviceToHost));
     return RT_OK;
 }
 

@@ -130,6 +130,16 @@ class Renderer:
                     "rt_render_accumulate")
         return out
 
+    def render_wavefront(self, out, camera, width, height, spp, sample_idx=0, strata=0, bg_scale=1.0,
+                         photon_map=False, max_depth=50, seed=0, capacity=0, grid=0, sort=True, use_graph=False):
+        """Same frame as render_accumulate, through the wavefront stage kernels."""
+        p = self.params(width, height, sample_idx, strata, bg_scale, max_depth, None, photon_map, seed)
+        cam = camera.to_c()
+        o = F.rt_wavefront_opts(capacity=capacity, grid=grid, sort=1 if sort else 0, use_graph=1 if use_graph else 0)
+        self._check(self.L.rt_render_wavefront(self.ctx, self.scene, C.byref(cam), C.byref(p), spp, _ptr(out), C.byref(o)),
+                    "rt_render_wavefront")
+        return out
+
     # ---------------------------------------------------------------- photons
     def photon_build_from_quad(self, origin, u, v, color, num_photons, gather_radius):
         """PhotonMap::build_from_quad (src/photon.rs:97-135)."""
@@ -182,6 +192,12 @@ class Renderer:
         self._check(self.L.rt_ctx_set_counting(self.ctx, 1 if enable else 0), "rt_ctx_set_counting")
 
     def counters(self):
-        out = (C.c_uint64 * 4)()
+        out = (C.c_uint64 * 8)()
         self._check(self.L.rt_ctx_get_counters(self.ctx, out), "rt_ctx_get_counters")
-        return {"node_tests": out[0], "prim_tests": out[1], "rays": out[2], "samples": out[3]}
+        return {"node_tests": out[0], "prim_tests": out[1], "rays": out[2], "samples": out[3],
+                "shadow_node_tests": out[4], "shadow_prim_tests": out[5], "shadow_rays": out[6]}
+
+    def stage_timing(self):
+        ms, n = C.c_double(), C.c_uint64()
+        self._check(self.L.rt_ctx_get_stage_timing(self.ctx, C.byref(ms), C.byref(n)), "rt_ctx_get_stage_timing")
+        return ms.value, n.value

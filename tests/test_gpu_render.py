@@ -182,3 +182,29 @@ def test_gem_with_photons_three_sigma(gpu_renderer):
     assert np.isfinite(g).all()
     z = three_sigma(g, c)
     assert (z > 3).mean() <= 0.02 and (z > 6).mean() <= 0.002, ((z > 3).mean(), (z > 6).mean())
+
+
+@pytest.mark.parametrize("sort,graph,cap", [(True, False, 0), (False, False, 4096), (True, True, 8192)])
+def test_wavefront_equals_megakernel(gpu_renderer, sort, graph, cap):
+    """Same per-pixel streams, same bounce code: the wavefront frame equals the
+    megakernel frame up to the order of float atomic additions."""
+    scene, cam, cfg = scenes.box_room(48, 40)
+    gpu_renderer.upload(scene)
+    a = np.zeros((40, 48, 3), np.float32)
+    b = np.zeros((40, 48, 3), np.float32)
+    gpu_renderer.render_accumulate(a, cam, 48, 40, 8, max_depth=cfg["max_depth"], seed=5)
+    gpu_renderer.render_wavefront(b, cam, 48, 40, 8, max_depth=cfg["max_depth"], seed=5, sort=sort, use_graph=graph,
+                                  capacity=cap)
+    assert np.allclose(a, b, rtol=1e-4, atol=1e-5), np.abs(a - b).max()
+
+
+def test_wavefront_gem_with_photons_equals_megakernel(gpu_renderer):
+    sc, cam, cfg = gem_small(40, 32, 50000)
+    gpu_renderer.upload(sc)
+    lq = cfg["light_quad"]
+    gpu_renderer.photon_build_from_quad(lq["q"], lq["u"], lq["v"], cfg["light_color"], 50000, cfg["gather_radius"])
+    a = np.zeros((32, 40, 3), np.float32)
+    b = np.zeros((32, 40, 3), np.float32)
+    gpu_renderer.render_accumulate(a, cam, 40, 32, 16, max_depth=24, seed=6, photon_map=True)
+    gpu_renderer.render_wavefront(b, cam, 40, 32, 16, max_depth=24, seed=6, photon_map=True, capacity=8192)
+    assert np.allclose(a, b, rtol=1e-4, atol=1e-5), np.abs(a - b).max()
