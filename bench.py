@@ -1,18 +1,25 @@
 #!/usr/bin/env python3
 """Throughput of the per-sample radiance path on one MI355X.
 
-A step renders one full frame of the chosen workload (every pixel, every
-sample per pixel) with the plain megakernel, scene and output buffer already
-resident in HBM.  `value` = samples / s over exactly K timed steps.
+A step renders one full frame of the workload (every pixel, every sample per
+pixel) with scene, photon map and frame accumulator resident in HBM.
+`value` = samples / s of the wavefront renderer over exactly K timed steps;
+the plain megakernel is timed on the same frames beside it.
 
-Beside it, on the same line:
-  roofline     — algorithmic HBM bytes of the render kernel per launch (BVH
-                 node records x 256 B + primitive records, from an untimed
-                 counting pass) over its average launch time, timed with HIP
-                 events on the library's stream, against 8 TB/s;
-  cpu_baseline — the CPU restatement of render_tiles (oracle/, the reference's
-                 row-sequential SmallRng streams) on this host's cores, on a
-                 bounded sample of the same workload.
+Default workload: the QBVH-bound mesh scene (a 1,310,720-triangle displaced
+icosphere in rough GGX glass under a 2048x1024 HDR environment, 1280x720,
+64 spp, depth 12).  The gem-with-caustics scene (1280x720, 256 spp, depth 24,
+2M photons) is measured after it and reported under "also".
+
+Beside the throughput, on the same line:
+  roofline     — algorithmic HBM bytes of the closest-hit traversal kernel
+                 (k_wf_extend) per launch — BVH node records x 256 B +
+                 primitive records, counted by an untimed instrumented pass —
+                 over its average launch time from HIP events on the library's
+                 stream, against 8 TB/s;
+  cpu_baseline — the CPU restatement of render_tiles (oracle/, the
+                 reference's row-sequential SmallRng streams) on this host's
+                 cores, on a bounded sample of the same frame.
 """
 import argparse
 import json
@@ -24,7 +31,8 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
-# bytes of one primitive record read by a test: triangle v0/e1/e2 48 B, quad 80 B, sphere 64 B (inline object)
+# bytes of one primitive record read by a test: triangle v0/e1/e2 48 B,
+# quad 80 B, convex gem 57 half-spaces x 16 B
 PRIM_BYTES = {"box": 80.0, "mesh": 48.0, "gem": 16.0 * 57}
 
 
@@ -37,30 +45,19 @@ def workloads():
     }
 
 
-def main():
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=5)
-    ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--config", default="box", choices=sorted(workloads()))
-    ap.add_argument("--cpu-seconds", type=float, default=12.0)
-    ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--renderer", default="both", choices=["both", "wavefront", "plain"])
-    ap.add_argument("--wf-capacity", type=int, default=0)
-    ap.add_argument("--wf-grid", type=int, default=0)
-    ap.add_argument("--wf-sort", type=int, default=1)
-    ap.add_argument("--wf-graph", type=int, default=1)
-    args = ap.parse_args()
-    if args.gpus != 1:
-        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (every box here has one MI355X)",
-              file=sys.stderr)
-        sys.exit(2)
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
 
+
+def measure(name, args, with_cpu):
     import numpy as np
     import torch
     from rustracer_amd import Renderer
+    from rustracer_amd.autotune import tune_wavefront
 
-    scene, cam, cfg = workloads()[args.config]()
+    t0 = time.time()
+    scene, cam, cfg = workloads()[name]()
+    build_s = time.time() - t0
     W, H, spp, depth = cfg["width"], cfg["height"], cfg["spp"], cfg["max_depth"]
     strata = int(spp ** 0.5)
     r = Renderer(0)
@@ -69,23 +66,29 @@ def main():
     upload_s = time.time() - t0
     stats = r.stats()
     photons = "num_photons" in cfg
-    photon_s = None
+    ph = None
     if photons:
         lq = cfg["light_quad"]
         t0 = time.time()
         stored = r.photon_build_from_quad(lq["q"], lq["u"], lq["v"], cfg["light_color"], cfg["num_photons"],
                                           cfg["gather_radius"])
-        photon_s = time.time() - t0
+        ph = {"traced": cfg["num_photons"], "stored": stored, "build_s": round(time.time() - t0, 3)}
     acc = torch.zeros((H, W, 3), dtype=torch.float32, device="cuda")
+    log(f"[{name}] scene {build_s:.1f}s upload {upload_s:.1f}s, {stats['nodes']} nodes, {stats['triangles']} tris")
+
+    tuned, tune_info = ({}, None)
+    if not args.no_tune:
+        tuned, tune_info = tune_wavefront(r, acc, cam, W, H, spp, depth, photons)
+        log(f"[{name}] autotuned wavefront: {tuned} ({tune_info['seconds']}s)")
 
     def step_plain(k):
-        r.render_accumulate(acc, cam, W, H, spp, sample_idx=0, strata=strata, max_depth=depth, seed=k,
-                            photon_map=photons)
+        r.render_accumulate(acc, cam, W, H, spp, strata=strata, max_depth=depth, seed=k, photon_map=photons)
 
-    def step_wf(k):
-        r.render_wavefront(acc, cam, W, H, spp, sample_idx=0, strata=strata, max_depth=depth, seed=k,
-                           photon_map=photons, capacity=args.wf_capacity, grid=args.wf_grid, sort=bool(args.wf_sort),
-                           use_graph=bool(args.wf_graph))
+    def step_wf(k, graph=None):
+        o = dict(tuned) if tuned else {}
+        if graph is not None:
+            o["use_graph"] = graph
+        r.render_wavefront(acc, cam, W, H, spp, strata=strata, max_depth=depth, seed=k, photon_map=photons, **o)
 
     def timed(step):
         for k in range(args.warmup):
@@ -104,7 +107,7 @@ def main():
         assert np.isfinite(img).all(), "non-finite radiance"
         return el, kms, nl
 
-    # counting pass (untimed): traversal work per sample, plain megakernel
+    # counting pass (untimed): traversal work per sample (plain megakernel, same streams)
     r.set_counting(True)
     step_plain(0)
     torch.cuda.synchronize()
@@ -115,65 +118,52 @@ def main():
     res = {}
     if args.renderer in ("both", "plain"):
         res["plain"] = timed(step_plain)
+        log(f"[{name}] plain {samples * args.steps / res['plain'][0] / 1e6:.1f} M samples/s")
     if args.renderer in ("both", "wavefront"):
         res["wavefront"] = timed(step_wf)
+        log(f"[{name}] wavefront {samples * args.steps / res['wavefront'][0] / 1e6:.1f} M samples/s")
     head = "wavefront" if "wavefront" in res else "plain"
-    elapsed, kern_ms, launches = res[head]
-    value = samples * args.steps / elapsed
+    rates = {k: samples * args.steps / v[0] for k, v in res.items()}
+
     ns = max(cnt["samples"], 1)
-    pb = PRIM_BYTES.get(args.config, 64.0)
+    pb = PRIM_BYTES.get(name, 64.0)
     closest_bps = (cnt["node_tests"] * 256.0 + cnt["prim_tests"] * pb) / ns
     shadow_bps = (cnt["shadow_node_tests"] * 256.0 + cnt["shadow_prim_tests"] * pb) / ns
     roof = {}
     if "plain" in res:
         el, kms, nl = res["plain"]
         spl = samples * args.steps / max(nl, 1)
-        a = (closest_bps + shadow_bps) * spl / (kms / 1e3 / max(nl, 1)) / 1e9
-        roof["plain"] = {"kernel": "k_render", "avg_launch_ms": kms / max(nl, 1), "bytes_per_launch": (closest_bps + shadow_bps) * spl,
-                         "achieved": a}
+        roof["plain"] = {"kernel": "k_render", "avg_launch_ms": kms / max(nl, 1),
+                         "bytes_per_launch": (closest_bps + shadow_bps) * spl,
+                         "achieved": (closest_bps + shadow_bps) * spl / (kms / 1e3 / max(nl, 1)) / 1e9}
     if "wavefront" in res:
-        # per-launch HIP events around k_wf_extend need the non-graph launch path
+        # per-launch HIP events around k_wf_extend: the non-graph launch path
         r.set_timing(True)
-        for k in range(max(1, args.steps)):
-            r.render_wavefront(acc, cam, W, H, spp, sample_idx=0, strata=strata, max_depth=depth, seed=2000 + k,
-                               photon_map=photons, capacity=args.wf_capacity, grid=args.wf_grid,
-                               sort=bool(args.wf_sort), use_graph=False)
+        frames = max(1, args.steps)
+        for k in range(frames):
+            step_wf(2000 + k, graph=False)
         torch.cuda.synchronize()
         ems, en = r.stage_timing()
         r.set_timing(False)
-        frames = max(1, args.steps)
         bpl = closest_bps * samples * frames / max(en, 1)
         roof["wavefront"] = {"kernel": "k_wf_extend", "avg_launch_ms": ems / max(en, 1), "bytes_per_launch": bpl,
                              "achieved": bpl / (ems / 1e3 / max(en, 1)) / 1e9, "launches": en}
     rh = roof[head]
-    achieved = rh["achieved"]
-    cpu = None
-    if not args.no_cpu:
-        cpu = cpu_baseline(scene, cam, cfg, args.cpu_seconds, photons)
 
-    rates = {k: samples * args.steps / v[0] for k, v in res.items()}
-    line = {
-        "metric": "GPU samples/s (wavefront vs plain) at fixed RMSE",
-        "value": value,
-        "unit": "samples/s",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": elapsed * 1e3 / args.steps,
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "f32",
-        "data": "synthetic",
+    cpu = cpu_baseline(scene, cam, cfg, args.cpu_seconds, photons) if with_cpu else None
+    r.close()
+    el = res[head][0]
+    return {
+        "value": rates[head], "ms_per_step": el * 1e3 / args.steps,
         "config": {"workload": scene.name, "width": W, "height": H, "spp": spp, "max_depth": depth,
                    "renderer": head, "samples_per_s": rates,
                    "wavefront_over_plain": (rates["wavefront"] / rates["plain"]) if len(rates) == 2 else None,
-                   "bvh_nodes": stats["nodes"], "triangles": stats["triangles"],
-                   "upload_s": round(upload_s, 3),
-                   "photons": None if not photons else {"traced": cfg["num_photons"], "stored": stored,
-                                                        "build_s": round(photon_s, 3)}},
-        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                   "wavefront_opts": tuned,
+                   "autotune": tune_info and {k: tune_info[k] for k in ("probe_spp", "probes", "seconds")},
+                   "bvh_nodes": stats["nodes"], "triangles": stats["triangles"], "upload_s": round(upload_s, 3),
+                   "photons": ph},
+        "roofline": {"bound": "hbm", "achieved": rh["achieved"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": rh["achieved"] / HBM_PEAK_GBS, "traffic": None,
                      "kernel": rh["kernel"], "avg_launch_ms": rh["avg_launch_ms"], "bytes_per_launch": rh["bytes_per_launch"],
                      "closest_bytes_per_sample": closest_bps, "shadow_bytes_per_sample": shadow_bps,
                      "node_tests_per_sample": cnt["node_tests"] / ns, "prim_tests_per_sample": cnt["prim_tests"] / ns,
@@ -181,13 +171,12 @@ def main():
                      "per_renderer": roof},
         "cpu_baseline": cpu,
     }
-    print(json.dumps(line), flush=True)
 
 
 def cpu_baseline(scene, cam, cfg, seconds, photons=False):
     """The oracle's render_tiles with the reference's row streams, all host
     cores, passes of the same frame until `seconds` of wall time are used
-    (scene build and photon tracing excluded, as the reference's --bench)."""
+    (scene build and photon tracing excluded, as in the reference's --bench)."""
     from oracle import oracle as O
     from rustracer_amd import Renderer
     from rustracer_amd import _ffi as F
@@ -213,7 +202,51 @@ def cpu_baseline(scene, cam, cfg, seconds, photons=False):
             break
     el = time.perf_counter() - t0
     return {"value": W * H * done / el, "unit": "samples/s", "cores": cores, "kind": "port",
-            "sample": f"{done} full passes of {W}x{H} (1 sample/pixel each), {el:.1f}s"}
+            "sample": f"{done} full passes of {W}x{H} at 1 sample/pixel (of {spp}), {el:.1f}s"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--config", default="mesh", choices=sorted(workloads()))
+    ap.add_argument("--also", default="gem", help="comma-separated extra workloads ('' for none)")
+    ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-tune", action="store_true")
+    ap.add_argument("--renderer", default="both", choices=["both", "wavefront", "plain"])
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (every box here has one MI355X)",
+              file=sys.stderr)
+        sys.exit(2)
+    main_res = measure(args.config, args, with_cpu=not args.no_cpu)
+    also = {}
+    for name in [a for a in args.also.split(",") if a and a != args.config]:
+        m = measure(name, args, with_cpu=not args.no_cpu)
+        also[m["config"]["workload"]] = {"value": m["value"], "unit": "samples/s", "config": m["config"],
+                                          "roofline": {k: m["roofline"][k] for k in ("achieved", "frac", "kernel", "avg_launch_ms")},
+                                          "cpu_baseline": m["cpu_baseline"]}
+    line = {
+        "metric": "GPU samples/s (wavefront vs plain) at fixed RMSE",
+        "value": main_res["value"],
+        "unit": "samples/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": main_res["ms_per_step"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": main_res["config"],
+        "roofline": main_res["roofline"],
+        "cpu_baseline": main_res["cpu_baseline"],
+        "also": also,
+    }
+    print(json.dumps(line), flush=True)
 
 
 if __name__ == "__main__":

@@ -233,12 +233,15 @@ typedef struct {
 /* ---- wavefront renderer options ----------------------------------------------
  * capacity: paths in flight per chunk (0: 4M); grid: blocks of the persistent
  * stage kernels (0: 2048); sort: 1 files paths by material before shading;
- * use_graph: 1 captures one chunk's launches in a hipGraph and replays it.   */
+ * use_graph: 1 captures one chunk's launches in a hipGraph and replays it;
+ * occupancy: register budget of the traversal stages — 0 compiler's choice,
+ * 4 or 5: at least that many waves per SIMD (fewer VGPRs, some spills).      */
 typedef struct {
     uint32_t capacity;
     uint32_t grid;
     uint32_t sort;
     uint32_t use_graph;
+    uint32_t occupancy;
 } rt_wavefront_opts;
 
 /* ---- caustic photon map (src/photon.rs, src/scene.rs:35-74) --------------- */

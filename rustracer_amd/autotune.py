@@ -1,0 +1,57 @@
+"""Autotuner for the wavefront renderer's launch parameters.
+
+Picks the register budget (occupancy target) of the traversal stages, the
+path-queue capacity, the persistent grid of the stage kernels, the material
+sort and hipGraph replay by coordinate descent on a short frame of
+the actual workload (same scene, camera, depth and photon setting), timing
+each candidate with HIP events on the library's stream.  The search is cheap
+(each probe renders a few passes) and the result is returned as the kwargs of
+`Renderer.render_wavefront`.
+"""
+import time
+
+CAPACITIES = [1 << 20, 1 << 21, 1 << 22, 1 << 23]
+GRIDS = [1024, 2048, 4096, 8192]
+
+
+def _probe(r, out, cam, W, H, spp, depth, photons, opts, repeats=2):
+    r.render_wavefront(out, cam, W, H, spp, max_depth=depth, photon_map=photons, seed=99, **opts)   # warm / allocate
+    r.set_timing(True)
+    for k in range(repeats):
+        r.render_wavefront(out, cam, W, H, spp, max_depth=depth, photon_map=photons, seed=100 + k, **opts)
+    ms, _ = r.timing()
+    r.set_timing(False)
+    return ms / repeats
+
+
+def tune_wavefront(r, out, cam, W, H, spp, depth, photons=False, probe_spp=None, log=None):
+    """Returns (best_opts, table) where table lists every probe as
+    (opts, ms per probe frame)."""
+    npix = W * H
+    if probe_spp is None:
+        # enough paths for two chunks of the largest capacity, at most the real spp
+        probe_spp = max(1, min(spp, (2 * CAPACITIES[-1] + npix - 1) // npix))
+    best = dict(capacity=1 << 22, grid=2048, sort=True, use_graph=True, occupancy=0)
+    table = []
+
+    def measure(o):
+        ms = _probe(r, out, cam, W, H, probe_spp, depth, photons, o)
+        table.append((dict(o), ms))
+        if log:
+            log(f"  autotune {o} -> {ms:.2f} ms")
+        return ms
+
+    t0 = time.time()
+    best_ms = measure(best)
+    for key, values in (("occupancy", [0, 4, 5]), ("capacity", CAPACITIES), ("grid", GRIDS), ("sort", [True, False]),
+                        ("use_graph", [True, False])):
+        for v in values:
+            if v == best[key]:
+                continue
+            o = dict(best)
+            o[key] = v
+            ms = measure(o)
+            if ms < best_ms:
+                best, best_ms = o, ms
+    return best, {"probe_spp": probe_spp, "probes": len(table), "seconds": round(time.time() - t0, 2),
+                  "best_ms": best_ms, "table": [(o, round(ms, 3)) for o, ms in table]}

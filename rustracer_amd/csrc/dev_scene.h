@@ -205,29 +205,35 @@ RT_D float mnps(float a, float b) { return a < b ? a : b; }
 RT_D float mxps(float a, float b) { return a > b ? a : b; }
 
 // Returns the hit mask; t_near[c] for each child.  Children read as ints.
+// Bounds are loaded and tested one axis at a time (4 x float4 live) so the
+// traversal kernels keep fewer VGPRs than with the whole 192-B box block live.
 RT_D uint32_t node_test(const float4* node, const Ray& r, float t_min, float t_max, float tn[8], int32_t ch[8]) {
-    float4 b[12];
+    float t0[8], t1[8];
 #pragma unroll
-    for (int k = 0; k < 12; k++) b[k] = node[k];
+    for (int c = 0; c < 8; c++) { t0[c] = t_min; t1[c] = t_max; }
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+        float4 l0 = node[2 * a], l1 = node[2 * a + 1], h0 = node[6 + 2 * a], h1 = node[7 + 2 * a];
+        float lo[8] = {l0.x, l0.y, l0.z, l0.w, l1.x, l1.y, l1.z, l1.w};
+        float hi[8] = {h0.x, h0.y, h0.z, h0.w, h1.x, h1.y, h1.z, h1.w};
+        float o = a == 0 ? r.o.x : (a == 1 ? r.o.y : r.o.z);
+        float id = a == 0 ? r.inv.x : (a == 1 ? r.inv.y : r.inv.z);
+#pragma unroll
+        for (int c = 0; c < 8; c++) {
+            float da = (lo[c] - o) * id;
+            float db = (hi[c] - o) * id;
+            t0[c] = mxps(t0[c], mnps(da, db));
+            t1[c] = mnps(t1[c], mxps(da, db));
+        }
+    }
     float4 c0 = node[12], c1 = node[13];
     ch[0] = __float_as_int(c0.x); ch[1] = __float_as_int(c0.y); ch[2] = __float_as_int(c0.z); ch[3] = __float_as_int(c0.w);
     ch[4] = __float_as_int(c1.x); ch[5] = __float_as_int(c1.y); ch[6] = __float_as_int(c1.z); ch[7] = __float_as_int(c1.w);
-    const float* bf = reinterpret_cast<const float*>(b);
-    float ro[3] = {r.o.x, r.o.y, r.o.z};
-    float id[3] = {r.inv.x, r.inv.y, r.inv.z};
     uint32_t mask = 0;
 #pragma unroll
     for (int c = 0; c < 8; c++) {
-        float t0 = t_min, t1 = t_max;
-#pragma unroll
-        for (int a = 0; a < 3; a++) {
-            float da = (bf[a * 8 + c] - ro[a]) * id[a];
-            float db = (bf[24 + a * 8 + c] - ro[a]) * id[a];
-            t0 = mxps(t0, mnps(da, db));
-            t1 = mnps(t1, mxps(da, db));
-        }
-        tn[c] = t0;
-        if (t0 <= t1 && ch[c] != INT32_MIN) mask |= 1u << c;
+        tn[c] = t0[c];
+        if (t0[c] <= t1[c] && ch[c] != INT32_MIN) mask |= 1u << c;
     }
     return mask;
 }

@@ -320,7 +320,9 @@ __device__ __forceinline__ uint32_t sort_key(const DScene& S, const HitId& id) {
     return k < 8u ? k : 9u;
 }
 
-__global__ __launch_bounds__(BLOCK) void k_wf_extend(DScene S, WfQueues Q) {
+// OCC = minimum waves per SIMD the register allocator must allow (0: its own choice).
+template <int OCC>
+__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(OCC > 0 ? OCC : 1, 8))) void k_wf_extend(DScene S, WfQueues Q) {
     Stack st; init_stack(st);
     uint32_t count = Q.counters[WF_CNT_ACTIVE];
     uint32_t stride = gridDim.x * BLOCK;
@@ -391,7 +393,8 @@ __global__ __launch_bounds__(BLOCK) void k_wf_shade(DScene S, DParams P, WfQueue
     }
 }
 
-__global__ __launch_bounds__(BLOCK) void k_wf_shadow(DScene S, WfQueues Q) {
+template <int OCC>
+__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(OCC > 0 ? OCC : 1, 8))) void k_wf_shadow(DScene S, WfQueues Q) {
     Stack st; init_stack(st);
     uint32_t count = Q.counters[WF_CNT_SHADOW];
     uint32_t stride = gridDim.x * BLOCK;
@@ -1037,8 +1040,19 @@ static int wf_buffers(rt_ctx* ctx, uint32_t cap, WfQueues& Q) {
 
 // Launch one chunk: generate `n` paths at frame position `first` (~0: read
 // from the counters, for graph replays) and run every bounce.
+static void launch_extend(int occ, uint32_t grid, hipStream_t st, const DScene& S, const WfQueues& Q) {
+    if (occ >= 5) hipLaunchKernelGGL(k_wf_extend<5>, dim3(grid), dim3(BLOCK), 0, st, S, Q);
+    else if (occ == 4) hipLaunchKernelGGL(k_wf_extend<4>, dim3(grid), dim3(BLOCK), 0, st, S, Q);
+    else hipLaunchKernelGGL(k_wf_extend<0>, dim3(grid), dim3(BLOCK), 0, st, S, Q);
+}
+static void launch_shadow(int occ, uint32_t grid, hipStream_t st, const DScene& S, const WfQueues& Q) {
+    if (occ >= 5) hipLaunchKernelGGL(k_wf_shadow<6>, dim3(grid), dim3(BLOCK), 0, st, S, Q);
+    else if (occ == 4) hipLaunchKernelGGL(k_wf_shadow<5>, dim3(grid), dim3(BLOCK), 0, st, S, Q);
+    else hipLaunchKernelGGL(k_wf_shadow<0>, dim3(grid), dim3(BLOCK), 0, st, S, Q);
+}
+
 static void wf_chunk(rt_ctx* ctx, rt_scene* s, const DCamera& C, const DParams& P, const WfQueues& Q0,
-                     uint64_t first, uint32_t n, uint32_t grid) {
+                     uint64_t first, uint32_t n, uint32_t grid, int occ) {
     hipStream_t st = ctx->stream;
     hipLaunchKernelGGL(k_wf_generate, dim3((n + BLOCK - 1) / BLOCK), dim3(BLOCK), 0, st, C, P, Q0, first, n);
     hipLaunchKernelGGL(k_wf_reset, dim3(1), dim3(32), 0, st, Q0.counters, n);
@@ -1051,10 +1065,10 @@ static void wf_chunk(rt_ctx* ctx, rt_scene* s, const DCamera& C, const DParams& 
             while (ctx->ext_ev.size() < ctx->ext_used + 2) { hipEvent_t e; if (hipEventCreate(&e) != hipSuccess) { ev = false; break; } ctx->ext_ev.push_back(e); }
         }
         if (ev) (void)hipEventRecord(ctx->ext_ev[ctx->ext_used], st);
-        hipLaunchKernelGGL(k_wf_extend, dim3(grid), dim3(BLOCK), 0, st, s->S, Q);
+        launch_extend(occ, grid, st, s->S, Q);
         if (ev) { (void)hipEventRecord(ctx->ext_ev[ctx->ext_used + 1], st); ctx->ext_used += 2; }
         hipLaunchKernelGGL(k_wf_shade, dim3(grid), dim3(BLOCK), 0, st, s->S, P, Q, (int)d);
-        hipLaunchKernelGGL(k_wf_shadow, dim3(grid), dim3(BLOCK), 0, st, s->S, Q);
+        launch_shadow(occ, grid, st, s->S, Q);
         hipLaunchKernelGGL(k_wf_advance, dim3(1), dim3(32), 0, st, Q.counters);
     }
 }
@@ -1069,6 +1083,7 @@ extern "C" int rt_render_wavefront(rt_ctx* ctx, rt_scene* s, const rt_camera* ca
     uint32_t grid = opts && opts->grid ? opts->grid : 2048u;
     uint32_t sort = opts ? opts->sort : 1u;
     uint32_t use_graph = opts ? opts->use_graph : 0u;
+    int occ = opts ? (int)opts->occupancy : 0;
     if (cap < BLOCK || cap > (1u << 26)) return fail(ctx, RT_ERR_INVALID, "wavefront capacity out of range");
     HIPCHECK(ctx, hipSetDevice(ctx->device));
     uint32_t npix = p->width * p->height;
@@ -1093,7 +1108,7 @@ extern "C" int rt_render_wavefront(rt_ctx* ctx, rt_scene* s, const rt_camera* ca
         hipGraphExec_t exec = nullptr;
         hipLaunchKernelGGL(k_wf_set_first, dim3(1), dim3(32), 0, ctx->stream, Q.counters, (uint64_t)0);
         HIPCHECK(ctx, hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
-        wf_chunk(ctx, s, C, P, Q, ~0ull, cap, grid);
+        wf_chunk(ctx, s, C, P, Q, ~0ull, cap, grid, occ);
         hipLaunchKernelGGL(k_wf_bump_first, dim3(1), dim3(32), 0, ctx->stream, Q.counters, cap);
         HIPCHECK(ctx, hipStreamEndCapture(ctx->stream, &graph));
         HIPCHECK(ctx, hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
@@ -1101,10 +1116,10 @@ extern "C" int rt_render_wavefront(rt_ctx* ctx, rt_scene* s, const rt_camera* ca
         HIPCHECK(ctx, hipGraphExecDestroy(exec));
         HIPCHECK(ctx, hipGraphDestroy(graph));
     } else {
-        for (uint64_t c = 0; c < full; c++) wf_chunk(ctx, s, C, P, Q, c * (uint64_t)cap, cap, grid);
+        for (uint64_t c = 0; c < full; c++) wf_chunk(ctx, s, C, P, Q, c * (uint64_t)cap, cap, grid, occ);
     }
     launches += full * (2 + 4ull * P.max_depth);
-    if (rem) { wf_chunk(ctx, s, C, P, Q, full * (uint64_t)cap, (uint32_t)rem, grid); launches += 2 + 4ull * P.max_depth; }
+    if (rem) { wf_chunk(ctx, s, C, P, Q, full * (uint64_t)cap, (uint32_t)rem, grid, occ); launches += 2 + 4ull * P.max_depth; }
     HIPCHECK(ctx, hipGetLastError());
     if (ctx->timing) {
         HIPCHECK(ctx, hipEventRecord(ctx->ev1, ctx->stream));

@@ -131,11 +131,13 @@ class Renderer:
         return out
 
     def render_wavefront(self, out, camera, width, height, spp, sample_idx=0, strata=0, bg_scale=1.0,
-                         photon_map=False, max_depth=50, seed=0, capacity=0, grid=0, sort=True, use_graph=False):
+                         photon_map=False, max_depth=50, seed=0, capacity=0, grid=0, sort=True, use_graph=False,
+                         occupancy=0):
         """Same frame as render_accumulate, through the wavefront stage kernels."""
         p = self.params(width, height, sample_idx, strata, bg_scale, max_depth, None, photon_map, seed)
         cam = camera.to_c()
-        o = F.rt_wavefront_opts(capacity=capacity, grid=grid, sort=1 if sort else 0, use_graph=1 if use_graph else 0)
+        o = F.rt_wavefront_opts(capacity=capacity, grid=grid, sort=1 if sort else 0, use_graph=1 if use_graph else 0,
+                                occupancy=occupancy)
         self._check(self.L.rt_render_wavefront(self.ctx, self.scene, C.byref(cam), C.byref(p), spp, _ptr(out), C.byref(o)),
                     "rt_render_wavefront")
         return out

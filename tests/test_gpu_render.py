@@ -184,8 +184,8 @@ def test_gem_with_photons_three_sigma(gpu_renderer):
     assert (z > 3).mean() <= 0.02 and (z > 6).mean() <= 0.002, ((z > 3).mean(), (z > 6).mean())
 
 
-@pytest.mark.parametrize("sort,graph,cap", [(True, False, 0), (False, False, 4096), (True, True, 8192)])
-def test_wavefront_equals_megakernel(gpu_renderer, sort, graph, cap):
+@pytest.mark.parametrize("sort,graph,cap,occ", [(True, False, 0, 0), (False, False, 4096, 4), (True, True, 8192, 5)])
+def test_wavefront_equals_megakernel(gpu_renderer, sort, graph, cap, occ):
     """Same per-pixel streams, same bounce code: the wavefront frame equals the
     megakernel frame up to the order of float atomic additions."""
     scene, cam, cfg = scenes.box_room(48, 40)
@@ -194,7 +194,7 @@ def test_wavefront_equals_megakernel(gpu_renderer, sort, graph, cap):
     b = np.zeros((40, 48, 3), np.float32)
     gpu_renderer.render_accumulate(a, cam, 48, 40, 8, max_depth=cfg["max_depth"], seed=5)
     gpu_renderer.render_wavefront(b, cam, 48, 40, 8, max_depth=cfg["max_depth"], seed=5, sort=sort, use_graph=graph,
-                                  capacity=cap)
+                                  capacity=cap, occupancy=occ)
     assert np.allclose(a, b, rtol=1e-4, atol=1e-5), np.abs(a - b).max()
 
 
