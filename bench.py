@@ -216,11 +216,21 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-tune", action="store_true")
     ap.add_argument("--renderer", default="both", choices=["both", "wavefront", "plain"])
+    ap.add_argument("--calib-copy-mb", type=int, default=0,
+                    help="first run a device-to-device copy of this many MiB (counter calibration for --pmc runs)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (every box here has one MI355X)",
               file=sys.stderr)
         sys.exit(2)
+    if args.calib_copy_mb:
+        import torch
+        n = args.calib_copy_mb * (1 << 20) // 4
+        a = torch.ones(n, dtype=torch.float32, device="cuda")
+        b = torch.empty_like(a)
+        b.copy_(a)
+        torch.cuda.synchronize()
+        log(f"calibration copy: {args.calib_copy_mb} MiB read + {args.calib_copy_mb} MiB written")
     main_res = measure(args.config, args, with_cpu=not args.no_cpu)
     also = {}
     for name in [a for a in args.also.split(",") if a and a != args.config]:
