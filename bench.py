@@ -9,7 +9,9 @@ the plain megakernel is timed on the same frames beside it.
 Default workload: the QBVH-bound mesh scene (a 1,310,720-triangle displaced
 icosphere in rough GGX glass under a 2048x1024 HDR environment, 1280x720,
 64 spp, depth 12).  The gem-with-caustics scene (1280x720, 256 spp, depth 24,
-2M photons) is measured after it and reported under "also".
+2M photons) and the uneven batch (36 gem spin stills of mixed resolution plus
+a heterogeneous-volume glass box, one launch) are measured after it and
+reported under "also".
 
 Beside the throughput, on the same line:
   roofline     — algorithmic HBM bytes of the closest-hit traversal kernel
@@ -76,8 +78,8 @@ def measure(name, args, with_cpu):
     acc = torch.zeros((H, W, 3), dtype=torch.float32, device="cuda")
     log(f"[{name}] scene {build_s:.1f}s upload {upload_s:.1f}s, {stats['nodes']} nodes, {stats['triangles']} tris")
 
-    tuned, tune_info = ({}, None)
-    if not args.no_tune:
+    tuned, tune_info = (json.loads(args.wf_opts) if args.wf_opts else {}), None
+    if not args.no_tune and not args.wf_opts:
         tuned, tune_info = tune_wavefront(r, acc, cam, W, H, spp, depth, photons)
         log(f"[{name}] autotuned wavefront: {tuned} ({tune_info['seconds']}s)")
 
@@ -173,6 +175,119 @@ def measure(name, args, with_cpu):
     }
 
 
+def measure_batch(args, with_cpu):
+    """Uneven batch: 36 gem spin stills (10 degree steps, resolutions drawn
+    from 256x256 / 512x384 / 960x540, 64 spp, depth 24, 2M-photon caustic map
+    each) plus the 640x480 volume glass box (64 spp, depth 24), all in one
+    rt_render_batch launch per step.  The same views rendered one
+    rt_render_wavefront call after another are timed beside it."""
+    import numpy as np
+    import torch
+    from rustracer_amd import Renderer, scenes
+    from rustracer_amd.autotune import tune_batch
+
+    t0 = time.time()
+    items = scenes.spin_batch()
+    r = Renderer(0)
+    views, photons = [], 0
+    for sc, cam, cfg in items:
+        h = r.upload_extra(sc)
+        use_ph = "num_photons" in cfg
+        if use_ph:
+            lq = cfg["light_quad"]
+            photons += r.photon_build_from_quad(lq["q"], lq["u"], lq["v"], cfg["light_color"], cfg["num_photons"],
+                                                cfg["gather_radius"], scene=h)
+        W, H = cfg["width"], cfg["height"]
+        views.append(dict(scene=h, camera=cam, width=W, height=H, spp=cfg["spp"], max_depth=cfg["max_depth"],
+                          photon_map=use_ph, strata=int(cfg["spp"] ** 0.5),
+                          out=torch.zeros((H, W, 3), dtype=torch.float32, device="cuda")))
+    setup_s = time.time() - t0
+    samples = sum(v["width"] * v["height"] * v["spp"] for v in views)
+    log(f"[batch] {len(views)} views, {samples / 1e6:.1f} M samples, setup {setup_s:.1f}s")
+    tuned, info = (json.loads(args.wf_opts) if args.wf_opts else {}), None
+    if not args.no_tune and not args.wf_opts:
+        tuned, info = tune_batch(r, views)
+        log(f"[batch] autotuned: {tuned} ({info['seconds']}s)")
+
+    def step_batch(k):
+        for v in views:
+            v["seed"] = k
+        r.render_batch(views, **tuned)
+
+    def step_seq(k):
+        o = {key: val for key, val in tuned.items()}
+        for v in views:
+            r.render_wavefront(v["out"], v["camera"], v["width"], v["height"], v["spp"], strata=v["strata"],
+                               max_depth=v["max_depth"], photon_map=v["photon_map"], seed=k, scene=v["scene"], **o)
+
+    def timed(step):
+        for k in range(args.warmup):
+            step(k)
+        torch.cuda.synchronize()
+        r.set_timing(True)
+        t0 = time.perf_counter()
+        for k in range(args.steps):
+            step(1000 + k)
+        torch.cuda.synchronize()
+        el = time.perf_counter() - t0
+        r.set_timing(False)
+        for v in views:
+            assert torch.isfinite(v["out"]).all().item(), "non-finite radiance"
+        return el
+
+    el_b = timed(step_batch)
+    el_s = timed(step_seq)
+    rate_b, rate_s = samples * args.steps / el_b, samples * args.steps / el_s
+    log(f"[batch] one launch {rate_b / 1e6:.1f} M samples/s, view by view {rate_s / 1e6:.1f} M samples/s")
+    # k_wf_extend launch times on the non-graph path for the roofline
+    r.set_timing(True)
+    tb = dict(tuned, use_graph=False)
+    for v in views:
+        v["seed"] = 2000
+    r.render_batch(views, **tb)
+    torch.cuda.synchronize()
+    ems, en = r.stage_timing()
+    r.set_timing(False)
+    # traversal work per sample from a counting pass of the plain megakernel
+    # on every view at 2 spp
+    r.set_counting(True)
+    cs = 0
+    for v in views:
+        r.render_accumulate(v["out"], v["camera"], v["width"], v["height"], 2, max_depth=v["max_depth"],
+                            photon_map=v["photon_map"], scene=v["scene"])
+        cs += v["width"] * v["height"] * 2
+    torch.cuda.synchronize()
+    cnt = r.counters()
+    r.set_counting(False)
+    gem_views = sum(1 for v in views if v["photon_map"])
+    pb = (PRIM_BYTES["gem"] * gem_views + PRIM_BYTES["box"] * (len(views) - gem_views)) / len(views)
+    closest_bps = (cnt["node_tests"] * 256.0 + cnt["prim_tests"] * pb) / max(cnt["samples"], 1)
+    bpl = closest_bps * samples / max(en, 1)
+    achieved = bpl / (ems / 1e3 / max(en, 1)) / 1e9
+    cpu = None
+    if with_cpu:
+        sc, cam, cfg = items[-1]
+        cpu = cpu_baseline(sc, cam, cfg, args.cpu_seconds / 2)
+        sc, cam, cfg = items[0]
+        g = cpu_baseline(sc, cam, cfg, args.cpu_seconds / 2, photons=True)
+        cpu["sample"] = "volume: " + cpu["sample"] + "; gem still: " + g["sample"]
+        # whole-batch CPU rate: time-weighted over the two scene kinds
+        gem_s = sum(v["width"] * v["height"] * v["spp"] for v in views if v["photon_map"]) / g["value"]
+        vol_s = sum(v["width"] * v["height"] * v["spp"] for v in views if not v["photon_map"]) / cpu["value"]
+        cpu["value"] = samples / (gem_s + vol_s)
+    r.close()
+    return {"value": rate_b, "ms_per_step": el_b * 1e3 / args.steps,
+            "config": {"workload": "spin_batch_36_gems_plus_volume_box", "views": len(views), "samples": samples,
+                       "samples_per_s": {"batch": rate_b, "view_by_view": rate_s},
+                       "batch_over_view_by_view": rate_b / rate_s, "wavefront_opts": tuned,
+                       "autotune": info and {k: info[k] for k in ("probe_spp", "probes", "seconds")},
+                       "photons_stored": photons, "setup_s": round(setup_s, 2)},
+            "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                         "frac": achieved / HBM_PEAK_GBS, "kernel": "k_wf_extend", "avg_launch_ms": ems / max(en, 1),
+                         "bytes_per_launch": bpl, "launches": en},
+            "cpu_baseline": cpu}
+
+
 def cpu_baseline(scene, cam, cfg, seconds, photons=False):
     """The oracle's render_tiles with the reference's row streams, all host
     cores, passes of the same frame until `seconds` of wall time are used
@@ -211,10 +326,13 @@ def main():
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--config", default="mesh", choices=sorted(workloads()))
-    ap.add_argument("--also", default="gem", help="comma-separated extra workloads ('' for none)")
+    ap.add_argument("--also", default="gem,batch",
+                    help="comma-separated extra workloads ('' for none): " + ", ".join(sorted(workloads())) + ", batch")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-tune", action="store_true")
+    ap.add_argument("--wf-opts", default="",
+                    help='fixed wavefront options as JSON (e.g. \'{"occupancy": 4, "capacity": 4194304}\'), no autotune')
     ap.add_argument("--renderer", default="both", choices=["both", "wavefront", "plain"])
     ap.add_argument("--calib-copy-mb", type=int, default=0,
                     help="first run a device-to-device copy of this many MiB (counter calibration for --pmc runs)")
@@ -234,7 +352,7 @@ def main():
     main_res = measure(args.config, args, with_cpu=not args.no_cpu)
     also = {}
     for name in [a for a in args.also.split(",") if a and a != args.config]:
-        m = measure(name, args, with_cpu=not args.no_cpu)
+        m = measure_batch(args, not args.no_cpu) if name == "batch" else measure(name, args, with_cpu=not args.no_cpu)
         also[m["config"]["workload"]] = {"value": m["value"], "unit": "samples/s", "config": m["config"],
                                           "roofline": {k: m["roofline"][k] for k in ("achieved", "frac", "kernel", "avg_launch_ms")},
                                           "cpu_baseline": m["cpu_baseline"]}

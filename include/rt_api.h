@@ -302,6 +302,25 @@ int  rt_render_wavefront(rt_ctx *ctx, rt_scene *scene, const rt_camera *cam,
                          const rt_sample_params *params, uint32_t spp,
                          float *accum_rgb, const rt_wavefront_opts *opts);
 
+/* One render of a batch: the scene it draws (each view may differ, e.g. the
+ * same gem spun to another angle), its camera and pass parameters (width,
+ * height, max_depth, bg_scale, use_photons, seed, sample_idx, strata), its
+ * samples per pixel and its accumulator (width*height*3 floats, host or
+ * device, overwritten with the sum of its spp passes).                       */
+typedef struct {
+    rt_scene        *scene;
+    rt_camera        camera;
+    rt_sample_params params;
+    uint32_t         spp;
+    float           *accum_rgb;
+} rt_batch_view;
+
+/* Every view of a batch in one wavefront: renders of uneven resolution, depth
+ * and scene share the path queues, so short views never leave the device
+ * idle.  Replaces one process per still (spin_stills.py:64-101).           */
+int  rt_render_batch(rt_ctx *ctx, const rt_batch_view *views, uint32_t nviews,
+                     const rt_wavefront_opts *opts);
+
 /* Trace photons and build the kd-tree on the device: PhotonMap::build /
  * build_from_quad (src/photon.rs:41-135, 165-191).  The map stays attached to
  * the scene; rt_sample_params.use_photons enables the gather.                */

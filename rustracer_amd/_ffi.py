@@ -91,6 +91,11 @@ class rt_wavefront_opts(C.Structure):
                 ("occupancy", C.c_uint32)]
 
 
+class rt_batch_view(C.Structure):
+    _fields_ = [("scene", C.c_void_p), ("camera", rt_camera), ("params", rt_sample_params), ("spp", C.c_uint32),
+                ("accum_rgb", C.c_void_p)]
+
+
 class rt_photon_params(C.Structure):
     _fields_ = [("emitter", C.c_uint32), ("origin", C.c_float * 3), ("u", C.c_float * 3),
                 ("v", C.c_float * 3), ("color", C.c_float * 3), ("sun_dir", C.c_float * 3),
@@ -126,6 +131,7 @@ SIGNATURES = {
     "rt_render_accumulate": (C.c_int, [VP, VP, C.POINTER(rt_camera), C.POINTER(rt_sample_params), C.c_uint32, VP]),
     "rt_render_wavefront": (C.c_int, [VP, VP, C.POINTER(rt_camera), C.POINTER(rt_sample_params), C.c_uint32, VP,
                                       C.POINTER(rt_wavefront_opts)]),
+    "rt_render_batch": (C.c_int, [VP, C.POINTER(rt_batch_view), C.c_uint32, C.POINTER(rt_wavefront_opts)]),
     "rt_photon_build": (C.c_int, [VP, VP, C.POINTER(rt_photon_params), C.POINTER(C.c_uint32)]),
     "rt_photon_irradiance": (C.c_int, [VP, VP, VP, C.c_uint32, VP]),
     "rt_photon_export": (C.c_int, [VP, VP, VP, C.c_uint32, C.POINTER(C.c_uint32)]),

@@ -14,28 +14,53 @@ CAPACITIES = [1 << 20, 1 << 21, 1 << 22, 1 << 23]
 GRIDS = [1024, 2048, 4096, 8192]
 
 
-def _probe(r, out, cam, W, H, spp, depth, photons, opts, repeats=2):
-    r.render_wavefront(out, cam, W, H, spp, max_depth=depth, photon_map=photons, seed=99, **opts)   # warm / allocate
+def _probe(r, run, opts, repeats=2):
+    run(opts, 99)                       # warm / allocate
     r.set_timing(True)
     for k in range(repeats):
-        r.render_wavefront(out, cam, W, H, spp, max_depth=depth, photon_map=photons, seed=100 + k, **opts)
+        run(opts, 100 + k)
     ms, _ = r.timing()
     r.set_timing(False)
     return ms / repeats
 
 
 def tune_wavefront(r, out, cam, W, H, spp, depth, photons=False, probe_spp=None, log=None):
-    """Returns (best_opts, table) where table lists every probe as
-    (opts, ms per probe frame)."""
+    """Tunes `Renderer.render_wavefront` on one frame.  Returns (best_opts,
+    info) where info["table"] lists every probe as (opts, ms per probe frame)."""
     npix = W * H
     if probe_spp is None:
         # enough paths for two chunks of the largest capacity, at most the real spp
         probe_spp = max(1, min(spp, (2 * CAPACITIES[-1] + npix - 1) // npix))
+
+    def run(o, seed):
+        r.render_wavefront(out, cam, W, H, probe_spp, max_depth=depth, photon_map=photons, seed=seed, **o)
+    best, info = tune(r, run, log)
+    info["probe_spp"] = probe_spp
+    return best, info
+
+
+def tune_batch(r, views, probe_spp=4, log=None):
+    """Tunes `Renderer.render_batch` on copies of the views with at most
+    `probe_spp` samples per pixel each."""
+    probe = [dict(v, spp=min(v["spp"], probe_spp)) for v in views]
+
+    def run(o, seed):
+        for v in probe:
+            v["seed"] = seed
+        r.render_batch(probe, **o)
+    best, info = tune(r, run, log)
+    info["probe_spp"] = probe_spp
+    return best, info
+
+
+def tune(r, run, log=None):
+    """Coordinate descent over the launch options; `run(opts, seed)` renders
+    one probe frame on the renderer's stream."""
     best = dict(capacity=1 << 22, grid=2048, sort=True, use_graph=True, occupancy=0)
     table = []
 
     def measure(o):
-        ms = _probe(r, out, cam, W, H, probe_spp, depth, photons, o)
+        ms = _probe(r, run, o)
         table.append((dict(o), ms))
         if log:
             log(f"  autotune {o} -> {ms:.2f} ms")
@@ -53,5 +78,5 @@ def tune_wavefront(r, out, cam, W, H, spp, depth, photons=False, probe_spp=None,
             ms = measure(o)
             if ms < best_ms:
                 best, best_ms = o, ms
-    return best, {"probe_spp": probe_spp, "probes": len(table), "seconds": round(time.time() - t0, 2),
+    return best, {"probes": len(table), "seconds": round(time.time() - t0, 2),
                   "best_ms": best_ms, "table": [(o, round(ms, 3)) for o, ms in table]}

@@ -214,7 +214,7 @@ struct __align__(16) WfPath {
 };
 static_assert(sizeof(WfPath) == 128, "path record is one 128-B line");
 
-struct __align__(16) WfShadow { float4 o_tmax; float4 d_time; float4 c_pixel; };
+struct __align__(16) WfShadow { float4 o_tmax; float4 d_time; float4 c_pixel; uint4 meta; };   // meta.x = view
 
 struct WfQueues {
     WfPath* paths;
@@ -293,12 +293,38 @@ __global__ void k_wf_bump_first(uint32_t* counters, uint32_t n) {
     }
 }
 
-__global__ __launch_bounds__(BLOCK) void k_wf_generate(DCamera cam, DParams P, WfQueues Q, uint64_t first, uint32_t n) {
+// One render of a batch: its camera and pass parameters, its slice of the
+// global path index space and of the concatenated frame accumulator.
+struct DView {
+    DCamera cam;
+    DParams P;
+    uint64_t first, count;    // global path range [first, first + count)
+    uint32_t fb_off, npix;
+};
+struct WfBatch {
+    const DScene* scenes;     // per view
+    const DView* views;
+    uint32_t nviews;
+};
+
+template <bool BATCH>
+__global__ __launch_bounds__(BLOCK) void k_wf_generate(DCamera cam0, DParams P0, WfQueues Q, uint64_t first, uint32_t n, WfBatch B) {
     uint32_t i = blockIdx.x * BLOCK + threadIdx.x;
     if (i >= n) return;
     if (first == ~0ull) first = ((uint64_t)Q.counters[17] << 32) | Q.counters[16];
-    uint32_t npix = P.width * P.height;
     uint64_t g = first + i;
+    uint32_t view = 0, fb_off = 0;
+    DCamera cam = cam0;
+    DParams P = P0;
+    if (BATCH) {
+        uint32_t lo = 0, hi = B.nviews;            // last view with views[v].first <= g
+        while (hi - lo > 1) { uint32_t mid = (lo + hi) / 2; if (B.views[mid].first <= g) lo = mid; else hi = mid; }
+        view = lo;
+        const DView& V = B.views[view];
+        cam = V.cam; P = V.P; fb_off = V.fb_off;
+        g -= V.first;
+    }
+    uint32_t npix = P.width * P.height;
     uint32_t pixel = (uint32_t)(g % npix);
     uint32_t sidx = P.sample_idx + (uint32_t)(g / npix);
     uint32_t row = pixel / P.width, col = pixel - row * P.width;
@@ -308,7 +334,7 @@ __global__ __launch_bounds__(BLOCK) void k_wf_generate(DCamera cam, DParams P, W
     path_init(ps, r);
     WfPath w;
     wf_store(w, ps, rng);
-    w.pixel = pixel; w.pad0 = 0; w.hit_t = 0.0f; w.hit_slot = -1; w.hit_prim = -1; w.key = 0;
+    w.pixel = fb_off + pixel; w.pad0 = view; w.hit_t = 0.0f; w.hit_slot = -1; w.hit_prim = -1; w.key = 0;
     Q.paths[i] = w;
     Q.active[i] = i;
 }
@@ -320,9 +346,17 @@ __device__ __forceinline__ uint32_t sort_key(const DScene& S, const HitId& id) {
     return k < 8u ? k : 9u;
 }
 
+__device__ __forceinline__ uint32_t extend_one(const DScene& S, WfPath& w, bool sort, Stack& st) {
+    float4 o = w.o_time, d = w.d_wl;
+    Ray r = make_ray(mk(o.x, o.y, o.z), mk(d.x, d.y, d.z), o.w);
+    HitId id = trace_closest(S, r, 0.001f, FINF, st);
+    w.hit_t = id.t; w.hit_slot = id.slot; w.hit_prim = id.prim;
+    return sort ? sort_key(S, id) : 0u;
+}
+
 // OCC = minimum waves per SIMD the register allocator must allow (0: its own choice).
-template <int OCC>
-__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(OCC > 0 ? OCC : 1, 8))) void k_wf_extend(DScene S, WfQueues Q) {
+template <int OCC, bool BATCH>
+__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(OCC > 0 ? OCC : 1, 8))) void k_wf_extend(DScene S, WfQueues Q, WfBatch B) {
     Stack st; init_stack(st);
     uint32_t count = Q.counters[WF_CNT_ACTIVE];
     uint32_t stride = gridDim.x * BLOCK;
@@ -333,49 +367,55 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(OCC > 0 ?
         if (valid) {
             pid = Q.active[i];
             WfPath& w = Q.paths[pid];
-            float4 o = w.o_time, d = w.d_wl;
-            Ray r = make_ray(mk(o.x, o.y, o.z), mk(d.x, d.y, d.z), o.w);
-            HitId id = trace_closest(S, r, 0.001f, FINF, st);
-            w.hit_t = id.t; w.hit_slot = id.slot; w.hit_prim = id.prim;
-            key = Q.sort ? sort_key(S, id) : 0u;
+            if (BATCH) key = extend_one(B.scenes[w.pad0], w, Q.sort != 0, st);
+            else key = extend_one(S, w, Q.sort != 0, st);
         }
         uint32_t slot = wave_append(Q.counters + WF_CNT_BUCKET, key, valid);
         if (valid) Q.buckets[(size_t)key * Q.cap + slot] = pid;
     }
 }
 
-__global__ __launch_bounds__(BLOCK) void k_wf_shade(DScene S, DParams P, WfQueues Q, int depth) {
+__device__ __forceinline__ bool shade_one(const DScene& S, const DParams& P, WfQueues& Q, WfPath& w, int depth,
+                                          ShadowReq* sh, int& nsh, Stack& st) {
+    if (depth >= (int)P.max_depth) return false;
+    PathState ps; Rng rng;
+    wf_load(w, ps, rng);
+    HitId id; id.t = w.hit_t; id.slot = w.hit_slot; id.prim = w.hit_prim;
+    V3 color = mk(0, 0, 0);
+    bool cont = bounce(S, ps, id, depth, P.bg_scale, P.use_photons != 0, rng, color, sh, nsh, st) && depth + 1 < (int)P.max_depth;
+    if (color.x != 0.0f || color.y != 0.0f || color.z != 0.0f) {
+        atomicAdd(Q.fb + 3 * (size_t)w.pixel, color.x);
+        atomicAdd(Q.fb + 3 * (size_t)w.pixel + 1, color.y);
+        atomicAdd(Q.fb + 3 * (size_t)w.pixel + 2, color.z);
+    }
+    if (cont) wf_store(w, ps, rng);
+    return cont;
+}
+
+template <bool BATCH>
+__global__ __launch_bounds__(BLOCK) void k_wf_shade(DScene S, DParams P, WfQueues Q, int depth, WfBatch B) {
     Stack st; init_stack(st);
     uint32_t off[WF_NB + 1];
     off[0] = 0;
     for (int b = 0; b < WF_NB; b++) off[b + 1] = off[b] + Q.counters[WF_CNT_BUCKET + b];
     uint32_t total = off[WF_NB];
     uint32_t stride = gridDim.x * BLOCK;
-    const bool last = depth + 1 >= (int)P.max_depth;
     for (uint32_t base = blockIdx.x * BLOCK; base < total; base += stride) {
         uint32_t i = base + threadIdx.x;
         bool valid = i < total;
         bool cont = false;
         int nsh = 0;
         ShadowReq sh[3];
-        uint32_t pid = 0, pixel = 0;
+        uint32_t pid = 0, pixel = 0, view = 0;
         if (valid) {
             int b = 0;
             while (i >= off[b + 1]) b++;
             pid = Q.buckets[(size_t)b * Q.cap + (i - off[b])];
             WfPath& w = Q.paths[pid];
-            PathState ps; Rng rng;
-            wf_load(w, ps, rng);
-            HitId id; id.t = w.hit_t; id.slot = w.hit_slot; id.prim = w.hit_prim;
             pixel = w.pixel;
-            V3 color = mk(0, 0, 0);
-            cont = bounce(S, ps, id, depth, P.bg_scale, P.use_photons != 0, rng, color, sh, nsh, st) && !last;
-            if (color.x != 0.0f || color.y != 0.0f || color.z != 0.0f) {
-                atomicAdd(Q.fb + 3 * (size_t)pixel, color.x);
-                atomicAdd(Q.fb + 3 * (size_t)pixel + 1, color.y);
-                atomicAdd(Q.fb + 3 * (size_t)pixel + 2, color.z);
-            }
-            if (cont) wf_store(w, ps, rng);
+            view = w.pad0;
+            if (BATCH) cont = shade_one(B.scenes[view], B.views[view].P, Q, w, depth, sh, nsh, st);
+            else cont = shade_one(S, P, Q, w, depth, sh, nsh, st);
         }
         uint32_t nslot = wave_append(Q.counters + WF_CNT_NEXT, 0u, cont);
         if (cont) Q.next[nslot] = pid;
@@ -387,21 +427,23 @@ __global__ __launch_bounds__(BLOCK) void k_wf_shade(DScene S, DParams P, WfQueue
                 q.o_tmax = make_float4(sh[k].o.x, sh[k].o.y, sh[k].o.z, sh[k].t_max);
                 q.d_time = make_float4(sh[k].d.x, sh[k].d.y, sh[k].d.z, sh[k].time);
                 q.c_pixel = make_float4(sh[k].contrib.x, sh[k].contrib.y, sh[k].contrib.z, __uint_as_float(pixel));
+                q.meta = make_uint4(view, 0u, 0u, 0u);
                 Q.shadow[s] = q;
             }
         }
     }
 }
 
-template <int OCC>
-__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(OCC > 0 ? OCC : 1, 8))) void k_wf_shadow(DScene S, WfQueues Q) {
+template <int OCC, bool BATCH>
+__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(OCC > 0 ? OCC : 1, 8))) void k_wf_shadow(DScene S, WfQueues Q, WfBatch B) {
     Stack st; init_stack(st);
     uint32_t count = Q.counters[WF_CNT_SHADOW];
     uint32_t stride = gridDim.x * BLOCK;
     for (uint32_t i = blockIdx.x * BLOCK + threadIdx.x; i < count; i += stride) {
         WfShadow q = Q.shadow[i];
         Ray r = make_ray(mk(q.o_tmax.x, q.o_tmax.y, q.o_tmax.z), mk(q.d_time.x, q.d_time.y, q.d_time.z), q.d_time.w);
-        float tr = trace_transmittance(S, r, 0.001f, q.o_tmax.w, st, false);
+        float tr = BATCH ? trace_transmittance(B.scenes[q.meta.x], r, 0.001f, q.o_tmax.w, st, false)
+                         : trace_transmittance(S, r, 0.001f, q.o_tmax.w, st, false);
         if (tr > 0.0f) {
             size_t px = 3 * (size_t)__float_as_uint(q.c_pixel.w);
             atomicAdd(Q.fb + px, q.c_pixel.x * tr);
@@ -1040,75 +1082,71 @@ static int wf_buffers(rt_ctx* ctx, uint32_t cap, WfQueues& Q) {
 
 // Launch one chunk: generate `n` paths at frame position `first` (~0: read
 // from the counters, for graph replays) and run every bounce.
-static void launch_extend(int occ, uint32_t grid, hipStream_t st, const DScene& S, const WfQueues& Q) {
-    if (occ >= 5) hipLaunchKernelGGL(k_wf_extend<5>, dim3(grid), dim3(BLOCK), 0, st, S, Q);
-    else if (occ == 4) hipLaunchKernelGGL(k_wf_extend<4>, dim3(grid), dim3(BLOCK), 0, st, S, Q);
-    else hipLaunchKernelGGL(k_wf_extend<0>, dim3(grid), dim3(BLOCK), 0, st, S, Q);
+template <bool BT>
+static void launch_extend(int occ, uint32_t grid, hipStream_t st, const DScene& S, const WfQueues& Q, const WfBatch& B) {
+    if (occ >= 5) hipLaunchKernelGGL((k_wf_extend<5, BT>), dim3(grid), dim3(BLOCK), 0, st, S, Q, B);
+    else if (occ == 4) hipLaunchKernelGGL((k_wf_extend<4, BT>), dim3(grid), dim3(BLOCK), 0, st, S, Q, B);
+    else hipLaunchKernelGGL((k_wf_extend<0, BT>), dim3(grid), dim3(BLOCK), 0, st, S, Q, B);
 }
-static void launch_shadow(int occ, uint32_t grid, hipStream_t st, const DScene& S, const WfQueues& Q) {
-    if (occ >= 5) hipLaunchKernelGGL(k_wf_shadow<6>, dim3(grid), dim3(BLOCK), 0, st, S, Q);
-    else if (occ == 4) hipLaunchKernelGGL(k_wf_shadow<5>, dim3(grid), dim3(BLOCK), 0, st, S, Q);
-    else hipLaunchKernelGGL(k_wf_shadow<0>, dim3(grid), dim3(BLOCK), 0, st, S, Q);
+template <bool BT>
+static void launch_shadow(int occ, uint32_t grid, hipStream_t st, const DScene& S, const WfQueues& Q, const WfBatch& B) {
+    if (occ >= 5) hipLaunchKernelGGL((k_wf_shadow<6, BT>), dim3(grid), dim3(BLOCK), 0, st, S, Q, B);
+    else if (occ == 4) hipLaunchKernelGGL((k_wf_shadow<5, BT>), dim3(grid), dim3(BLOCK), 0, st, S, Q, B);
+    else hipLaunchKernelGGL((k_wf_shadow<0, BT>), dim3(grid), dim3(BLOCK), 0, st, S, Q, B);
 }
 
-static void wf_chunk(rt_ctx* ctx, rt_scene* s, const DCamera& C, const DParams& P, const WfQueues& Q0,
-                     uint64_t first, uint32_t n, uint32_t grid, int occ) {
+// Launch one chunk: generate `n` paths at global position `first` (~0: read
+// from the counters, for graph replays) and run `depth` bounces.
+template <bool BT>
+static void wf_chunk(rt_ctx* ctx, const DScene& S, const DCamera& C, const DParams& P, const WfQueues& Q0,
+                     uint64_t first, uint32_t n, uint32_t grid, int occ, uint32_t depth, const WfBatch& Bt) {
     hipStream_t st = ctx->stream;
-    hipLaunchKernelGGL(k_wf_generate, dim3((n + BLOCK - 1) / BLOCK), dim3(BLOCK), 0, st, C, P, Q0, first, n);
+    hipLaunchKernelGGL((k_wf_generate<BT>), dim3((n + BLOCK - 1) / BLOCK), dim3(BLOCK), 0, st, C, P, Q0, first, n, Bt);
     hipLaunchKernelGGL(k_wf_reset, dim3(1), dim3(32), 0, st, Q0.counters, n);
     WfQueues A = Q0, B = Q0;
     B.active = Q0.next; B.next = Q0.active;
-    for (uint32_t d = 0; d < P.max_depth; d++) {
+    for (uint32_t d = 0; d < depth; d++) {
         const WfQueues& Q = (d & 1) ? B : A;
         bool ev = ctx->timing && first != ~0ull;
         if (ev) {
             while (ctx->ext_ev.size() < ctx->ext_used + 2) { hipEvent_t e; if (hipEventCreate(&e) != hipSuccess) { ev = false; break; } ctx->ext_ev.push_back(e); }
         }
         if (ev) (void)hipEventRecord(ctx->ext_ev[ctx->ext_used], st);
-        launch_extend(occ, grid, st, s->S, Q);
+        launch_extend<BT>(occ, grid, st, S, Q, Bt);
         if (ev) { (void)hipEventRecord(ctx->ext_ev[ctx->ext_used + 1], st); ctx->ext_used += 2; }
-        hipLaunchKernelGGL(k_wf_shade, dim3(grid), dim3(BLOCK), 0, st, s->S, P, Q, (int)d);
-        launch_shadow(occ, grid, st, s->S, Q);
+        hipLaunchKernelGGL((k_wf_shade<BT>), dim3(grid), dim3(BLOCK), 0, st, S, P, Q, (int)d, Bt);
+        launch_shadow<BT>(occ, grid, st, S, Q, Bt);
         hipLaunchKernelGGL(k_wf_advance, dim3(1), dim3(32), 0, st, Q.counters);
     }
 }
 
-extern "C" int rt_render_wavefront(rt_ctx* ctx, rt_scene* s, const rt_camera* cam, const rt_sample_params* p, uint32_t spp,
-                                   float* out, const rt_wavefront_opts* opts) {
-    if (!ctx || !s || !cam || !p || !out) return RT_ERR_INVALID;
-    if (p->width == 0 || p->height == 0 || spp == 0) return fail(ctx, RT_ERR_INVALID, "empty frame");
-    if (p->max_depth == 0) return fail(ctx, RT_ERR_INVALID, "max_depth = 0");
-    if (p->use_photons && s->S.num_photons == 0) return fail(ctx, RT_ERR_INVALID, "use_photons without a photon map");
-    uint32_t cap = opts && opts->capacity ? opts->capacity : (1u << 22);
-    uint32_t grid = opts && opts->grid ? opts->grid : 2048u;
-    uint32_t sort = opts ? opts->sort : 1u;
-    uint32_t use_graph = opts ? opts->use_graph : 0u;
-    int occ = opts ? (int)opts->occupancy : 0;
-    if (cap < BLOCK || cap > (1u << 26)) return fail(ctx, RT_ERR_INVALID, "wavefront capacity out of range");
-    HIPCHECK(ctx, hipSetDevice(ctx->device));
-    uint32_t npix = p->width * p->height;
-    uint64_t total = (uint64_t)npix * spp;
-    if (cap > total) cap = (uint32_t)((total + BLOCK - 1) / BLOCK * BLOCK);
-    WfQueues Q;
-    int rc;
-    if ((rc = wf_buffers(ctx, cap, Q))) return rc;
-    Q.sort = sort;
-    Staged rout;
-    if ((rc = stage_out(ctx, out, (size_t)npix * 12, rout, false))) return rc;
-    Q.fb = (float*)rout.dev;
-    DCamera C; DParams P;
-    to_dev(cam, p, C, P);
-    P.counters = nullptr;
-    if (ctx->timing) HIPCHECK(ctx, hipEventRecord(ctx->ev0, ctx->stream));
-    HIPCHECK(ctx, hipMemsetAsync(rout.dev, 0, (size_t)npix * 12, ctx->stream));
+struct WfOpts { uint32_t cap, grid, sort, use_graph; int occ; };
+
+static int wf_opts(rt_ctx* ctx, const rt_wavefront_opts* opts, uint64_t total, WfOpts& o) {
+    o.cap = opts && opts->capacity ? opts->capacity : (1u << 22);
+    o.grid = opts && opts->grid ? opts->grid : 2048u;
+    o.sort = opts ? opts->sort : 1u;
+    o.use_graph = opts ? opts->use_graph : 0u;
+    o.occ = opts ? (int)opts->occupancy : 0;
+    if (o.cap < BLOCK || o.cap > (1u << 26)) return fail(ctx, RT_ERR_INVALID, "wavefront capacity out of range");
+    if (o.grid == 0 || o.grid > (1u << 20)) return fail(ctx, RT_ERR_INVALID, "wavefront grid out of range");
+    if (o.cap > total) o.cap = (uint32_t)((total + BLOCK - 1) / BLOCK * BLOCK);
+    return RT_OK;
+}
+
+// Every chunk of the global path range through every bounce.  fb must be zeroed.
+template <bool BT>
+static int wf_run(rt_ctx* ctx, const DScene& S, const DCamera& C, const DParams& P, WfQueues& Q, const WfOpts& o,
+                  uint64_t total, uint32_t depth, const WfBatch& Bt) {
+    uint32_t cap = o.cap;
     uint64_t full = total / cap, rem = total % cap;
-    uint64_t launches = 0;
-    if (use_graph && full > 0) {
+    if (ctx->timing) HIPCHECK(ctx, hipEventRecord(ctx->ev0, ctx->stream));
+    if (o.use_graph && full > 0) {
         hipGraph_t graph = nullptr;
         hipGraphExec_t exec = nullptr;
         hipLaunchKernelGGL(k_wf_set_first, dim3(1), dim3(32), 0, ctx->stream, Q.counters, (uint64_t)0);
         HIPCHECK(ctx, hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
-        wf_chunk(ctx, s, C, P, Q, ~0ull, cap, grid, occ);
+        wf_chunk<BT>(ctx, S, C, P, Q, ~0ull, cap, o.grid, o.occ, depth, Bt);
         hipLaunchKernelGGL(k_wf_bump_first, dim3(1), dim3(32), 0, ctx->stream, Q.counters, cap);
         HIPCHECK(ctx, hipStreamEndCapture(ctx->stream, &graph));
         HIPCHECK(ctx, hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
@@ -1116,10 +1154,9 @@ extern "C" int rt_render_wavefront(rt_ctx* ctx, rt_scene* s, const rt_camera* ca
         HIPCHECK(ctx, hipGraphExecDestroy(exec));
         HIPCHECK(ctx, hipGraphDestroy(graph));
     } else {
-        for (uint64_t c = 0; c < full; c++) wf_chunk(ctx, s, C, P, Q, c * (uint64_t)cap, cap, grid, occ);
+        for (uint64_t c = 0; c < full; c++) wf_chunk<BT>(ctx, S, C, P, Q, c * (uint64_t)cap, cap, o.grid, o.occ, depth, Bt);
     }
-    launches += full * (2 + 4ull * P.max_depth);
-    if (rem) { wf_chunk(ctx, s, C, P, Q, full * (uint64_t)cap, (uint32_t)rem, grid, occ); launches += 2 + 4ull * P.max_depth; }
+    if (rem) wf_chunk<BT>(ctx, S, C, P, Q, full * (uint64_t)cap, (uint32_t)rem, o.grid, o.occ, depth, Bt);
     HIPCHECK(ctx, hipGetLastError());
     if (ctx->timing) {
         HIPCHECK(ctx, hipEventRecord(ctx->ev1, ctx->stream));
@@ -1134,7 +1171,91 @@ extern "C" int rt_render_wavefront(rt_ctx* ctx, rt_scene* s, const rt_camera* ca
         }
     }
     ctx->ext_used = 0;
+    return RT_OK;
+}
+
+extern "C" int rt_render_wavefront(rt_ctx* ctx, rt_scene* s, const rt_camera* cam, const rt_sample_params* p, uint32_t spp,
+                                   float* out, const rt_wavefront_opts* opts) {
+    if (!ctx || !s || !cam || !p || !out) return RT_ERR_INVALID;
+    if (p->width == 0 || p->height == 0 || spp == 0) return fail(ctx, RT_ERR_INVALID, "empty frame");
+    if (p->max_depth == 0) return fail(ctx, RT_ERR_INVALID, "max_depth = 0");
+    if (p->use_photons && s->S.num_photons == 0) return fail(ctx, RT_ERR_INVALID, "use_photons without a photon map");
+    if ((uint64_t)p->width * p->height > 0xffffffffull / 4) return fail(ctx, RT_ERR_INVALID, "image too large");
+    HIPCHECK(ctx, hipSetDevice(ctx->device));
+    uint32_t npix = p->width * p->height;
+    uint64_t total = (uint64_t)npix * spp;
+    WfOpts o;
+    int rc;
+    if ((rc = wf_opts(ctx, opts, total, o))) return rc;
+    WfQueues Q;
+    if ((rc = wf_buffers(ctx, o.cap, Q))) return rc;
+    Q.sort = o.sort;
+    Staged rout;
+    if ((rc = stage_out(ctx, out, (size_t)npix * 12, rout, false))) return rc;
+    Q.fb = (float*)rout.dev;
+    DCamera C; DParams P;
+    to_dev(cam, p, C, P);
+    P.counters = nullptr;
+    HIPCHECK(ctx, hipMemsetAsync(rout.dev, 0, (size_t)npix * 12, ctx->stream));
+    WfBatch none = {nullptr, nullptr, 0};
+    if ((rc = wf_run<false>(ctx, s->S, C, P, Q, o, total, P.max_depth, none))) return rc;
     return finish_out(ctx, out, (size_t)npix * 12, rout);
+}
+
+extern "C" int rt_render_batch(rt_ctx* ctx, const rt_batch_view* views, uint32_t nviews, const rt_wavefront_opts* opts) {
+    if (!ctx || !views || nviews == 0) return RT_ERR_INVALID;
+    HIPCHECK(ctx, hipSetDevice(ctx->device));
+    std::vector<DView> dv(nviews);
+    std::vector<DScene> ds(nviews);
+    uint64_t total = 0, fb_pix = 0;
+    uint32_t depth = 0;
+    for (uint32_t v = 0; v < nviews; v++) {
+        const rt_batch_view& b = views[v];
+        if (!b.scene || !b.accum_rgb || b.spp == 0) return fail(ctx, RT_ERR_INVALID, "batch view " + std::to_string(v) + ": missing scene/output/spp");
+        if (b.scene->ctx != ctx) return fail(ctx, RT_ERR_INVALID, "batch view " + std::to_string(v) + ": scene of another context");
+        const rt_sample_params& p = b.params;
+        if (p.width == 0 || p.height == 0 || p.max_depth == 0) return fail(ctx, RT_ERR_INVALID, "batch view " + std::to_string(v) + ": empty frame");
+        if (p.use_photons && b.scene->S.num_photons == 0) return fail(ctx, RT_ERR_INVALID, "batch view " + std::to_string(v) + ": no photon map");
+        to_dev(&b.camera, &p, dv[v].cam, dv[v].P);
+        dv[v].P.counters = nullptr;
+        dv[v].npix = p.width * p.height;
+        dv[v].first = total;
+        dv[v].count = (uint64_t)dv[v].npix * b.spp;
+        dv[v].fb_off = (uint32_t)fb_pix;
+        total += dv[v].count;
+        fb_pix += dv[v].npix;
+        if (fb_pix > 0xffffffffull / 4) return fail(ctx, RT_ERR_INVALID, "batch frames too large");
+        depth = std::max(depth, p.max_depth);
+        ds[v] = b.scene->S;
+    }
+    WfOpts o;
+    int rc;
+    if ((rc = wf_opts(ctx, opts, total, o))) return rc;
+    WfQueues Q;
+    if ((rc = wf_buffers(ctx, o.cap, Q))) return rc;
+    Q.sort = o.sort;
+    // batch side data: views, scenes, concatenated accumulator
+    size_t bv = (sizeof(DView) * nviews + 255) & ~(size_t)255, bs = (sizeof(DScene) * nviews + 255) & ~(size_t)255;
+    size_t bf = (size_t)fb_pix * 12;
+    char* side = nullptr;
+    HIPCHECK(ctx, hipMalloc(&side, bv + bs + bf));
+    std::unique_ptr<void, void (*)(void*)> guard(side, [](void* p) { (void)hipFree(p); });
+    HIPCHECK(ctx, hipMemcpyAsync(side, dv.data(), sizeof(DView) * nviews, hipMemcpyHostToDevice, ctx->stream));
+    HIPCHECK(ctx, hipMemcpyAsync(side + bv, ds.data(), sizeof(DScene) * nviews, hipMemcpyHostToDevice, ctx->stream));
+    HIPCHECK(ctx, hipMemsetAsync(side + bv + bs, 0, bf, ctx->stream));
+    Q.fb = (float*)(side + bv + bs);
+    WfBatch Bt = {(const DScene*)(side + bv), (const DView*)side, nviews};
+    DCamera C = dv[0].cam;
+    DParams P = dv[0].P;
+    if ((rc = wf_run<true>(ctx, ds[0], C, P, Q, o, total, depth, Bt))) return rc;
+    HIPCHECK(ctx, hipStreamSynchronize(ctx->stream));
+    for (uint32_t v = 0; v < nviews; v++) {
+        const float* src = Q.fb + 3 * (size_t)dv[v].fb_off;
+        size_t bytes = (size_t)dv[v].npix * 12;
+        hipMemcpyKind kind = is_device_ptr(views[v].accum_rgb) ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost;
+        HIPCHECK(ctx, hipMemcpy(views[v].accum_rgb, src, bytes, kind));
+    }
+    return RT_OK;
 }
 
 extern "C" {

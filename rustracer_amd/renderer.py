@@ -46,6 +46,9 @@ class Renderer:
             raise RtError(f"{what} failed ({rc}): {msg.decode() if msg else ''}")
 
     def close(self):
+        for h in getattr(self, "_extra", []):
+            self.L.rt_scene_destroy(h)
+        self._extra = []
         if self.scene is not None:
             self.L.rt_scene_destroy(self.scene)
             self.scene = None
@@ -61,16 +64,30 @@ class Renderer:
 
     # ---------------------------------------------------------------- scene
     def upload(self, scene):
-        """scene: rustracer_amd.scene.Scene"""
+        """scene: rustracer_amd.scene.Scene; becomes the current scene."""
         if self.scene is not None:
             self.L.rt_scene_destroy(self.scene)
             self.scene = None
+        self.scene = self.upload_extra(scene)
+        self._extra.remove(self.scene)
+        self.num_photons = 0
+        return self
+
+    def upload_extra(self, scene):
+        """Upload another scene (for batches) without replacing the current
+        one; the returned handle lives until close() or destroy(handle)."""
         d = scene.descriptor()
         s = C.c_void_p()
         self._check(self.L.rt_scene_upload(self.ctx, C.byref(d), C.byref(s)), "rt_scene_upload")
-        self.scene = s
-        self.num_photons = 0
-        return self
+        self._extra = getattr(self, "_extra", [])
+        self._extra.append(s)
+        return s
+
+    def destroy(self, handle):
+        if handle is not None:
+            self.L.rt_scene_destroy(handle)
+            if handle in getattr(self, "_extra", []):
+                self._extra.remove(handle)
 
     def stats(self):
         st = (C.c_uint64 * 8)()
@@ -122,28 +139,45 @@ class Renderer:
         return scratch
 
     def render_accumulate(self, out, camera, width, height, spp, sample_idx=0, strata=0, bg_scale=1.0,
-                          photon_map=False, max_depth=50, seed=0):
-        """Sum of `spp` passes (sample_idx .. sample_idx+spp-1) into `out`."""
+                          photon_map=False, max_depth=50, seed=0, scene=None):
+        """Sum of `spp` passes (sample_idx .. sample_idx+spp-1) into `out`;
+        `scene` is a handle from upload_extra (default: the current scene)."""
         p = self.params(width, height, sample_idx, strata, bg_scale, max_depth, None, photon_map, seed)
         cam = camera.to_c()
-        self._check(self.L.rt_render_accumulate(self.ctx, self.scene, C.byref(cam), C.byref(p), spp, _ptr(out)),
+        self._check(self.L.rt_render_accumulate(self.ctx, self.scene if scene is None else scene, C.byref(cam), C.byref(p), spp, _ptr(out)),
                     "rt_render_accumulate")
         return out
 
     def render_wavefront(self, out, camera, width, height, spp, sample_idx=0, strata=0, bg_scale=1.0,
                          photon_map=False, max_depth=50, seed=0, capacity=0, grid=0, sort=True, use_graph=False,
-                         occupancy=0):
+                         occupancy=0, scene=None):
         """Same frame as render_accumulate, through the wavefront stage kernels."""
         p = self.params(width, height, sample_idx, strata, bg_scale, max_depth, None, photon_map, seed)
         cam = camera.to_c()
         o = F.rt_wavefront_opts(capacity=capacity, grid=grid, sort=1 if sort else 0, use_graph=1 if use_graph else 0,
                                 occupancy=occupancy)
-        self._check(self.L.rt_render_wavefront(self.ctx, self.scene, C.byref(cam), C.byref(p), spp, _ptr(out), C.byref(o)),
+        self._check(self.L.rt_render_wavefront(self.ctx, self.scene if scene is None else scene, C.byref(cam), C.byref(p), spp, _ptr(out), C.byref(o)),
                     "rt_render_wavefront")
         return out
 
+    def render_batch(self, views, capacity=0, grid=0, sort=True, use_graph=False, occupancy=0):
+        """views: list of dicts with keys scene (handle), camera, width, height,
+        spp, out, and optionally max_depth, photon_map, seed, bg_scale."""
+        arr = (F.rt_batch_view * len(views))()
+        for i, v in enumerate(views):
+            arr[i].scene = v["scene"]
+            arr[i].camera = v["camera"].to_c()
+            arr[i].params = self.params(v["width"], v["height"], 0, v.get("strata", 0), v.get("bg_scale", 1.0),
+                                        v.get("max_depth", 50), None, v.get("photon_map", False), v.get("seed", 0))
+            arr[i].spp = v["spp"]
+            arr[i].accum_rgb = _ptr(v["out"])
+        o = F.rt_wavefront_opts(capacity=capacity, grid=grid, sort=1 if sort else 0, use_graph=1 if use_graph else 0,
+                                occupancy=occupancy)
+        self._check(self.L.rt_render_batch(self.ctx, arr, len(views), C.byref(o)), "rt_render_batch")
+        return [v["out"] for v in views]
+
     # ---------------------------------------------------------------- photons
-    def photon_build_from_quad(self, origin, u, v, color, num_photons, gather_radius):
+    def photon_build_from_quad(self, origin, u, v, color, num_photons, gather_radius, scene=None):
         """PhotonMap::build_from_quad (src/photon.rs:97-135)."""
         pp = F.rt_photon_params()
         pp.emitter = F.EMIT_QUAD
@@ -151,7 +185,8 @@ class Renderer:
             pp.origin[k], pp.u[k], pp.v[k], pp.color[k] = float(origin[k]), float(u[k]), float(v[k]), float(color[k])
         pp.num_photons, pp.gather_radius = num_photons, gather_radius
         n = C.c_uint32()
-        self._check(self.L.rt_photon_build(self.ctx, self.scene, C.byref(pp), C.byref(n)), "rt_photon_build")
+        self._check(self.L.rt_photon_build(self.ctx, scene if scene is not None else self.scene, C.byref(pp), C.byref(n)),
+                    "rt_photon_build")
         self.num_photons = n.value
         return n.value
 

@@ -87,7 +87,7 @@ class Perlin:
             acc += w * self.noise(p)
             w *= 0.5
             p = p * 2.0
-        return acc
+        return np.abs(acc)
 
 
 def icosphere(level, radius=1.0):
@@ -182,7 +182,7 @@ def mesh_envmap(width=1280, height=720, level=8):
     return scene, cam, dict(width=width, height=height, spp=64, max_depth=12)
 
 
-def gem_caustics(width=1280, height=720, num_photons=2_000_000, dispersion="cauchy"):
+def gem_caustics(width=1280, height=720, num_photons=2_000_000, dispersion="cauchy", spin_deg=0.0):
     """Dispersive gem with caustics.
 
     A round brilliant (Diamond, table 57 %, crown 34.5 deg, pavilion 40.75 deg,
@@ -203,13 +203,64 @@ def gem_caustics(width=1280, height=720, num_photons=2_000_000, dispersion="cauc
     else:
         mat = SpectralDielectric(2.351, 0.0230)
     gem = Diamond((0, 0, 0), 0.5, mat, table_pct=57.0, crown_angle=34.5, pavilion_angle=40.75,
-                  girdle_facets=56, girdle_pct=0.0).rotated((0, 12, 0)).translated((0, 0.55, 0))
+                  girdle_facets=56, girdle_pct=0.0).rotated((0, 12 + spin_deg, 0)).translated((0, 0.55, 0))
     light = DiffuseLight((50.0, 50.0, 50.0))
     lq = dict(q=(-0.15, 2.2, -0.15), u=(0.3, 0, 0), v=(0, 0, 0.3))
     objs = [Quad((-4, 0, -4), (8, 0, 0), (0, 0, 8), Lambertian((0.8, 0.8, 0.8))), gem,
             Quad(lq["q"], lq["u"], lq["v"], light)]
     lights = [Quad(lq["q"], lq["u"], lq["v"], light)]
-    scene = Scene(objs, lights, Background.solid((0, 0, 0)), name="gem_caustics")
+    scene = Scene(objs, lights, Background.solid((0, 0, 0)), name="gem_caustics" if not spin_deg else f"gem_spin{spin_deg:g}")
     cam = Camera((0.0, 1.6, -2.6), (0.0, 0.35, 0.0), (0, 1, 0), 35.0, width / height)
     return scene, cam, dict(width=width, height=height, spp=256, max_depth=24, light_quad=lq, light_color=(50.0, 50.0, 50.0),
                             num_photons=num_photons, gather_radius=0.05)
+
+
+def perlin_grid(n=128, octaves=4, frequency=3.0, seed=2):
+    """n^3 float32 density grid in [0,1]: `octaves`-octave Perlin turbulence
+    over the unit cube, normalised by its maximum (x fastest, then y, z)."""
+    ax = (np.arange(n, dtype=np.float32) + 0.5) / n
+    z, y, x = np.meshgrid(ax, ax, ax, indexing="ij")
+    p = np.stack([x.ravel(), y.ravel(), z.ravel()], 1) * np.float32(frequency)
+    t = Perlin(seed).turb(p, octaves).reshape(n, n, n)
+    return (t / max(float(t.max()), 1e-9)).astype(np.float32)
+
+
+def volume_glass_box(width=640, height=480, grid_n=128):
+    """Heterogeneous medium in a glass box under the HDR environment.
+
+    A Dielectric box (IOR 1.33) spanning [-0.6,0.6]x[0,1.2]x[-0.6,0.6] on a
+    diffuse 0.8 ground, filled (inset 0.02) with a delta-tracked GridMedium:
+    density = perlin_grid(128, 4 octaves, frequency 3, seed 2) x sigma_t 8,
+    albedo 0.9, Henyey-Greenstein g 0.6.  Lit by sky_envmap().  Camera at
+    (0, 1.3, -3.0) looking at (0, 0.55, 0), vfov 35.  64 spp, depth 24.
+    """
+    from .scene import Background, Box, Camera, Dielectric, GridMedium, Lambertian, Quad, Scene
+    inner = Box((-0.58, 0.02, -0.58), (0.58, 1.18, 0.58), None)
+    GridMedium(inner, perlin_grid(grid_n), 8.0, (0.9, 0.9, 0.9), 0.6)
+    objs = [Quad((-10, -0.001, -10), (20, 0, 0), (0, 0, 20), Lambertian((0.8, 0.8, 0.8))),
+            Box((-0.6, 0.0, -0.6), (0.6, 1.2, 0.6), Dielectric(1.33)), inner]
+    scene = Scene(objs, [], Background.envmap(sky_envmap()), name="volume_glass_box")
+    cam = Camera((0.0, 1.3, -3.0), (0.0, 0.55, 0.0), (0, 1, 0), 35.0, width / height)
+    return scene, cam, dict(width=width, height=height, spp=64, max_depth=24)
+
+
+SPIN_RESOLUTIONS = [(256, 256), (512, 384), (960, 540)]
+
+
+def spin_batch(n_stills=36, step_deg=10.0, spp=64, seed=3, num_photons=2_000_000, with_volume=True):
+    """Batched renders of uneven resolution: `n_stills` spin stills of the gem
+    scene (spin k * step_deg about the stone's centre, as scene_file's `spin`
+    key, src/scene_file.rs:1032-1038) at resolutions drawn uniformly with
+    `seed` from SPIN_RESOLUTIONS, each with its own caustic photon map,
+    `spp` spp and depth 24; plus, when `with_volume`, volume_glass_box()."""
+    g = np.random.default_rng(seed)
+    picks = g.integers(0, len(SPIN_RESOLUTIONS), size=n_stills)
+    views = []
+    for k in range(n_stills):
+        W, H = SPIN_RESOLUTIONS[picks[k]]
+        sc, cam, cfg = gem_caustics(W, H, num_photons=num_photons, spin_deg=k * step_deg)
+        cfg["spp"] = spp
+        views.append((sc, cam, cfg))
+    if with_volume:
+        views.append(volume_glass_box())
+    return views

@@ -208,3 +208,63 @@ def test_wavefront_gem_with_photons_equals_megakernel(gpu_renderer):
     gpu_renderer.render_accumulate(a, cam, 40, 32, 16, max_depth=24, seed=6, photon_map=True)
     gpu_renderer.render_wavefront(b, cam, 40, 32, 16, max_depth=24, seed=6, photon_map=True, capacity=8192)
     assert np.allclose(a, b, rtol=1e-4, atol=1e-5), np.abs(a - b).max()
+
+
+def test_volume_glass_box_same_streams_and_three_sigma(gpu_renderer):
+    """Dielectric box around a delta-tracked grid medium under the HDR sky."""
+    sc, cam, cfg = scenes.volume_glass_box(32, 24, grid_n=16)
+    gpu_renderer.upload(sc)
+    orc = O.OracleScene(sc)
+    g = passes_gpu(gpu_renderer, cam, 32, 24, 4, 24, seed=1)
+    c = passes_cpu(orc, cam, 32, 24, 4, 24, mode=1, seed=1)
+    close = np.isclose(g, c, rtol=1e-3, atol=1e-4).all(axis=-1)
+    assert close.mean() > 0.9, close.mean()
+    K = 256
+    g = passes_gpu(gpu_renderer, cam, 32, 24, K, 24, seed=2)
+    c = passes_cpu(orc, cam, 32, 24, K, 24, mode=0, seed=2)
+    assert np.isfinite(g).all()
+    z = three_sigma(g, c)
+    assert (z > 3).mean() <= 0.02 and (z > 6).mean() <= 0.002, ((z > 3).mean(), (z > 6).mean())
+
+
+@pytest.mark.parametrize("graph,cap", [(False, 4096), (True, 2048)])
+def test_batch_equals_individual_wavefront_renders(gpu_renderer, graph, cap):
+    """One batched launch over views of uneven resolution, spp, depth, seed and
+    scene (gem spin stills with their own photon maps, plus the volume box)
+    gives each view the frame its own wavefront render gives."""
+    r = gpu_renderer
+    views = []
+    try:
+        for k, (W, H) in enumerate([(24, 16), (40, 32), (16, 16)]):
+            sc, cam, cfg = scenes.gem_caustics(W, H, num_photons=20000, spin_deg=30.0 * k)
+            h = r.upload_extra(sc)
+            lq = cfg["light_quad"]
+            r.photon_build_from_quad(lq["q"], lq["u"], lq["v"], cfg["light_color"], 20000, cfg["gather_radius"], scene=h)
+            views.append(dict(scene=h, camera=cam, width=W, height=H, spp=3 + k, max_depth=24, photon_map=True,
+                              seed=7 + k))
+        sc, cam, cfg = scenes.volume_glass_box(32, 24, grid_n=16)
+        views.append(dict(scene=r.upload_extra(sc), camera=cam, width=32, height=24, spp=5, max_depth=12, seed=11))
+        for v in views:
+            v["out"] = np.zeros((v["height"], v["width"], 3), np.float32)
+        r.render_batch(views, capacity=cap, use_graph=graph)
+        for v in views:
+            ref = np.zeros_like(v["out"])
+            r.render_wavefront(ref, v["camera"], v["width"], v["height"], v["spp"], max_depth=v["max_depth"],
+                               photon_map=v.get("photon_map", False), seed=v["seed"], capacity=cap, scene=v["scene"])
+            assert np.isfinite(v["out"]).all()
+            assert np.allclose(v["out"], ref, rtol=1e-4, atol=1e-4), np.abs(v["out"] - ref).max()
+            assert (v["out"].sum(-1) > 0).mean() > 0.5
+    finally:
+        for v in views:
+            r.destroy(v["scene"])
+
+
+def test_batch_rejects_bad_views(gpu_renderer):
+    from rustracer_amd.renderer import RtError
+    sc, cam, cfg = scenes.box_room(16, 16)
+    r = gpu_renderer.upload(sc)
+    out = np.zeros((16, 16, 3), np.float32)
+    with pytest.raises(RtError):
+        r.render_batch([dict(scene=r.scene, camera=cam, width=16, height=16, spp=0, out=out)])
+    with pytest.raises(RtError):
+        r.render_batch([dict(scene=r.scene, camera=cam, width=16, height=16, spp=2, out=out, photon_map=True)])
