@@ -107,7 +107,10 @@ def measure(name, args, with_cpu):
         r.set_timing(False)
         img = acc.cpu().numpy() / spp
         assert np.isfinite(img).all(), "non-finite radiance"
+        frames.append(img)
         return el, kms, nl
+
+    frames = []
 
     # counting pass (untimed): traversal work per sample (plain megakernel, same streams)
     r.set_counting(True)
@@ -126,6 +129,12 @@ def measure(name, args, with_cpu):
         log(f"[{name}] wavefront {samples * args.steps / res['wavefront'][0] / 1e6:.1f} M samples/s")
     head = "wavefront" if "wavefront" in res else "plain"
     rates = {k: samples * args.steps / v[0] for k, v in res.items()}
+    same_frame = None
+    if len(frames) == 2:
+        # both renderers drew the same per-pixel streams for the last timed
+        # step: the frames agree up to the order of float additions
+        same_frame = float(np.abs(frames[0] - frames[1]).max() / max(np.abs(frames[0]).max(), 1e-30))
+        assert same_frame < 1e-3, f"wavefront frame differs from the megakernel frame ({same_frame})"
 
     ns = max(cnt["samples"], 1)
     pb = PRIM_BYTES.get(name, 64.0)
@@ -141,31 +150,35 @@ def measure(name, args, with_cpu):
     if "wavefront" in res:
         # per-launch HIP events around k_wf_extend: the non-graph launch path
         r.set_timing(True)
-        frames = max(1, args.steps)
-        for k in range(frames):
+        nfr = max(1, args.steps)
+        for k in range(nfr):
             step_wf(2000 + k, graph=False)
         torch.cuda.synchronize()
         ems, en = r.stage_timing()
         r.set_timing(False)
-        bpl = closest_bps * samples * frames / max(en, 1)
+        bpl = closest_bps * samples * nfr / max(en, 1)
         roof["wavefront"] = {"kernel": "k_wf_extend", "avg_launch_ms": ems / max(en, 1), "bytes_per_launch": bpl,
                              "achieved": bpl / (ems / 1e3 / max(en, 1)) / 1e9, "launches": en}
     rh = roof[head]
 
     cpu = cpu_baseline(scene, cam, cfg, args.cpu_seconds, photons) if with_cpu else None
     r.close()
+    traffic, traffic_note = None, None
+    if head == "wavefront":
+        traffic, traffic_note = measured_traffic(scene.name, tuned)
     el = res[head][0]
     return {
         "value": rates[head], "ms_per_step": el * 1e3 / args.steps,
         "config": {"workload": scene.name, "width": W, "height": H, "spp": spp, "max_depth": depth,
                    "renderer": head, "samples_per_s": rates,
                    "wavefront_over_plain": (rates["wavefront"] / rates["plain"]) if len(rates) == 2 else None,
+                   "frame_max_rel_diff_wavefront_vs_plain": same_frame,
                    "wavefront_opts": tuned,
                    "autotune": tune_info and {k: tune_info[k] for k in ("probe_spp", "probes", "seconds")},
                    "bvh_nodes": stats["nodes"], "triangles": stats["triangles"], "upload_s": round(upload_s, 3),
                    "photons": ph},
         "roofline": {"bound": "hbm", "achieved": rh["achieved"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": rh["achieved"] / HBM_PEAK_GBS, "traffic": None,
+                     "frac": rh["achieved"] / HBM_PEAK_GBS, "traffic": traffic, "traffic_source": traffic_note,
                      "kernel": rh["kernel"], "avg_launch_ms": rh["avg_launch_ms"], "bytes_per_launch": rh["bytes_per_launch"],
                      "closest_bytes_per_sample": closest_bps, "shadow_bytes_per_sample": shadow_bps,
                      "node_tests_per_sample": cnt["node_tests"] / ns, "prim_tests_per_sample": cnt["prim_tests"] / ns,
@@ -288,6 +301,23 @@ def measure_batch(args, with_cpu):
             "cpu_baseline": cpu}
 
 
+def measured_traffic(workload, opts):
+    """HBM bytes per k_wf_extend dispatch from the committed rocprofv3 PMC
+    summary (FETCH_SIZE + WRITE_SIZE, copy-calibrated) when it was measured
+    with the same launch options; otherwise None and the reason."""
+    path = os.path.join(ROOT, "profiles", "traffic.json")
+    try:
+        with open(path) as f:
+            rec = json.load(f).get(workload)
+    except (OSError, ValueError):
+        return None, "no PMC summary"
+    if not rec:
+        return None, "no PMC summary for this workload"
+    if any(opts.get(k) != v for k, v in rec["wavefront_opts"].items()):
+        return None, f"PMC summary measured with {rec['wavefront_opts']}, not {opts}"
+    return rec["traffic_bytes_per_dispatch"], rec["source"]
+
+
 def cpu_baseline(scene, cam, cfg, seconds, photons=False):
     """The oracle's render_tiles with the reference's row streams, all host
     cores, passes of the same frame until `seconds` of wall time are used
@@ -325,7 +355,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--config", default="mesh", choices=sorted(workloads()))
+    ap.add_argument("--config", default="mesh", choices=sorted(workloads()) + ["batch"])
     ap.add_argument("--also", default="gem,batch",
                     help="comma-separated extra workloads ('' for none): " + ", ".join(sorted(workloads())) + ", batch")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
@@ -349,7 +379,10 @@ def main():
         b.copy_(a)
         torch.cuda.synchronize()
         log(f"calibration copy: {args.calib_copy_mb} MiB read + {args.calib_copy_mb} MiB written")
-    main_res = measure(args.config, args, with_cpu=not args.no_cpu)
+    if args.config == "batch":
+        main_res = measure_batch(args, not args.no_cpu)
+    else:
+        main_res = measure(args.config, args, with_cpu=not args.no_cpu)
     also = {}
     for name in [a for a in args.also.split(",") if a and a != args.config]:
         m = measure_batch(args, not args.no_cpu) if name == "batch" else measure(name, args, with_cpu=not args.no_cpu)

@@ -202,6 +202,7 @@ __global__ __launch_bounds__(BLOCK) void k_photon_irr(DScene S, const float* __r
 #define WF_CNT_NEXT 1     // counters[1]: survivors written by shade
 #define WF_CNT_SHADOW 2   // counters[2]: shadow requests
 #define WF_CNT_BUCKET 8   // counters[8..8+WF_NB): bucket sizes
+#define WF_CNT_FIRST 32   // counters[32..33]: 64-bit frame position of a graph-replayed chunk
 
 struct __align__(16) WfPath {
     float4 o_time;       // ray origin, time
@@ -223,7 +224,7 @@ struct WfQueues {
     uint32_t* next;       // cap
     uint32_t* buckets;    // WF_NB * cap
     WfShadow* shadow;     // 3 * cap
-    uint32_t* counters;   // 32
+    uint32_t* counters;   // 64: 32 per-bounce counters, then the chunk position
     float* fb;
     uint32_t sort;        // 0: one bucket (no material sort)
 };
@@ -281,15 +282,16 @@ __global__ void k_wf_advance(uint32_t* counters) {
     if (i < 32) counters[i] = (i == WF_CNT_ACTIVE) ? nxt : 0u;
 }
 
-// counters[16..17] hold the frame position of a graph-replayed chunk.
+// counters[WF_CNT_FIRST..+1] hold the frame position of a graph-replayed
+// chunk; they lie past the 32 per-bounce counters that reset/advance clear.
 __global__ void k_wf_set_first(uint32_t* counters, uint64_t first) {
-    if (threadIdx.x == 0) { counters[16] = (uint32_t)first; counters[17] = (uint32_t)(first >> 32); }
+    if (threadIdx.x == 0) { counters[WF_CNT_FIRST] = (uint32_t)first; counters[WF_CNT_FIRST + 1] = (uint32_t)(first >> 32); }
 }
 __global__ void k_wf_bump_first(uint32_t* counters, uint32_t n) {
     if (threadIdx.x == 0) {
-        uint64_t f = ((uint64_t)counters[17] << 32) | counters[16];
+        uint64_t f = ((uint64_t)counters[WF_CNT_FIRST + 1] << 32) | counters[WF_CNT_FIRST];
         f += n;
-        counters[16] = (uint32_t)f; counters[17] = (uint32_t)(f >> 32);
+        counters[WF_CNT_FIRST] = (uint32_t)f; counters[WF_CNT_FIRST + 1] = (uint32_t)(f >> 32);
     }
 }
 
@@ -311,7 +313,7 @@ template <bool BATCH>
 __global__ __launch_bounds__(BLOCK) void k_wf_generate(DCamera cam0, DParams P0, WfQueues Q, uint64_t first, uint32_t n, WfBatch B) {
     uint32_t i = blockIdx.x * BLOCK + threadIdx.x;
     if (i >= n) return;
-    if (first == ~0ull) first = ((uint64_t)Q.counters[17] << 32) | Q.counters[16];
+    if (first == ~0ull) first = ((uint64_t)Q.counters[WF_CNT_FIRST + 1] << 32) | Q.counters[WF_CNT_FIRST];
     uint64_t g = first + i;
     uint32_t view = 0, fb_off = 0;
     DCamera cam = cam0;
