@@ -133,6 +133,13 @@ typedef struct {
  *                                                  src/material.rs:427-554
  *   ROUGH_DIELECTRIC p0 ior, p1 roughness (GGX transmission; extension, no
  *               counterpart in the reference)
+ *   ROUGH_SPECTRAL p0 n_d (IOR at 589.3 nm), p1-3 Sellmeier B1..B3,
+ *               p4-6 Sellmeier C1..C3 (um^2): n(l) = n_d + s(l) - s(589.3 nm);
+ *               p7 GGX alpha of primitive 0 (the table facet of a Diamond),
+ *               p8-23 GGX alpha of 16 facet groups (primitive k >= 1 uses
+ *               p[8 + (k-1) % 16]).  Rough dispersive dielectric with the
+ *               fire tint of SPECTRAL_DIELECTRIC on the first refraction;
+ *               extension (the differentiable gem of the adjoint path).
  */
 enum {
     RT_MAT_LAMBERTIAN          = 1,
@@ -141,7 +148,8 @@ enum {
     RT_MAT_PBR                 = 4,
     RT_MAT_DIFFUSE_LIGHT       = 5,
     RT_MAT_SPECTRAL_METAL      = 6,
-    RT_MAT_ROUGH_DIELECTRIC    = 7
+    RT_MAT_ROUGH_DIELECTRIC    = 7,
+    RT_MAT_ROUGH_SPECTRAL      = 8
 };
 
 typedef struct {
@@ -320,6 +328,48 @@ typedef struct {
  * idle.  Replaces one process per still (spin_stills.py:64-101).           */
 int  rt_render_batch(rt_ctx *ctx, const rt_batch_view *views, uint32_t nviews,
                      const rt_wavefront_opts *opts);
+
+/* ---- differentiable material parameters (extension: no counterpart in the
+ * reference, which has no derivatives) ----------------------------------------
+ * Parameter i of a vector theta sets rt_material.p[field .. field+count-1] of
+ * material `material` to theta[i] (count 3 on a LAMBERTIAN albedo makes one
+ * grey albedo parameter).  Differentiable fields: LAMBERTIAN (solid, tex 0)
+ * p0-2; ROUGH_SPECTRAL p0 (n_d), p1-3 (Sellmeier B), p7-23 (GGX alpha).
+ * Others return RT_ERR_UNSUPPORTED.                                           */
+typedef struct {
+    uint32_t material;
+    uint32_t field;
+    uint32_t count;
+} rt_param;
+
+/* Write theta into the scene's materials (device and host copies); the
+ * geometry, BVH and photon map are untouched.                                */
+int  rt_scene_set_params(rt_ctx *ctx, rt_scene *scene, const rt_param *params, uint32_t n,
+                         const float *values);
+int  rt_scene_get_params(rt_ctx *ctx, rt_scene *scene, const rt_param *params, uint32_t n,
+                         float *values);
+
+/* The frame of rt_render_accumulate with every sampling decision (directions,
+ * lobe and Fresnel choices, Russian roulette) drawn with the scene's current
+ * parameters and every BSDF value evaluated at eval_values: the estimator the
+ * path-replay adjoint differentiates.  With eval_values equal to the current
+ * parameters it is rt_render_accumulate's frame; at theta +- h with the same
+ * seed it gives central finite differences with common random numbers.       */
+int  rt_render_detached(rt_ctx *ctx, rt_scene *scene, const rt_camera *cam,
+                        const rt_sample_params *params, uint32_t spp,
+                        const rt_param *dparams, uint32_t n, const float *eval_values,
+                        float *accum_rgb);
+
+/* Path-replay adjoint.  For a loss of the mean image I = accum / spp, with
+ * dloss_dimage = dLoss/dI (width*height*3 floats, host or device), writes
+ * grad[n] = dLoss/dtheta at the scene's current parameters (host or device,
+ * overwritten).  Each sample is traced once for its radiance and once more
+ * from its stored stream seed to distribute that radiance over the path's
+ * vertices; no tape is kept.                                                  */
+int  rt_backward(rt_ctx *ctx, rt_scene *scene, const rt_camera *cam,
+                 const rt_sample_params *params, uint32_t spp,
+                 const rt_param *dparams, uint32_t n, const float *dloss_dimage,
+                 float *grad);
 
 /* Trace photons and build the kd-tree on the device: PhotonMap::build /
  * build_from_quad (src/photon.rs:41-135, 165-191).  The map stays attached to

@@ -49,6 +49,10 @@ def lib():
         L.rtc_scatter.argtypes = [vp, vp, C.c_int, vp, C.c_int, C.c_uint64, C.c_uint32, vp]
         L.rtc_env_sample.argtypes = [vp, C.c_uint32, C.c_uint32, C.c_float, C.c_uint64, C.c_uint32, vp]
         L.rtc_env_eval.argtypes = [vp, C.c_uint32, C.c_uint32, C.c_float, vp, C.c_uint32, vp]
+        L.rtc_set_params.argtypes = [vp, vp, C.c_uint32, vp]
+        L.rtc_render_detached.argtypes = [vp, vp, vp, C.c_uint32, vp, C.c_uint32, vp, vp, C.c_int, C.c_int]
+        L.rtc_backward.argtypes = [vp, vp, vp, C.c_uint32, vp, C.c_uint32, vp, vp, C.c_int, C.c_int]
+        L.rtc_rsd_logf.argtypes = [vp, vp, vp, C.c_double, C.c_double, C.c_int, C.POINTER(C.c_double)]
         _lib = L
     return _lib
 
@@ -121,6 +125,38 @@ class OracleScene:
         cam = camera.to_c()
         self.L.rtc_render_accumulate(self.h, C.byref(cam), C.byref(params), spp, _p(out), mode, nthreads or threads())
         return out
+
+    # ---- adjoint checker: forward score accumulation (see rc_adjoint.h) ----
+    @staticmethod
+    def _params(params):
+        from rustracer_amd import _ffi as F
+        arr = (F.rt_param * len(params))()
+        for i, (m, f, c) in enumerate(params):
+            arr[i].material, arr[i].field, arr[i].count = int(m), int(f), int(c)
+        return arr
+
+    def set_params(self, params, values):
+        v = np.ascontiguousarray(values, dtype=np.float32)
+        if self.L.rtc_set_params(self.h, self._params(params), len(params), _p(v)) != 0:
+            raise ValueError("invalid parameter map")
+
+    def render_detached(self, camera, sp, spp, params, eval_values, mode=1, nthreads=None):
+        out = np.zeros((sp.height, sp.width, 3), dtype=np.float32)
+        v = np.ascontiguousarray(eval_values, dtype=np.float32)
+        cam = camera.to_c()
+        if self.L.rtc_render_detached(self.h, C.byref(cam), C.byref(sp), spp, self._params(params), len(params), _p(v),
+                                      _p(out), mode, nthreads or threads()) != 0:
+            raise ValueError("invalid parameter map")
+        return out
+
+    def backward(self, camera, sp, spp, params, dloss_dimage, mode=1, nthreads=None):
+        d = np.ascontiguousarray(dloss_dimage, dtype=np.float32)
+        g = np.zeros(len(params), dtype=np.float64)
+        cam = camera.to_c()
+        if self.L.rtc_backward(self.h, C.byref(cam), C.byref(sp), spp, self._params(params), len(params), _p(d), _p(g),
+                               mode, nthreads or threads()) != 0:
+            raise ValueError("invalid parameter map")
+        return g
 
     def photon_build(self, pp, nthreads=None):
         n = C.c_uint32()
@@ -210,3 +246,13 @@ def env_eval(img, rot, dirs):
     out = np.zeros((len(dirs), 4), np.float32)
     lib().rtc_env_eval(_p(img), img.shape[1], img.shape[0], rot, _p(dirs), len(dirs), _p(out))
     return out
+
+
+def rsd_logf(i, o, n, k, alpha, refl):
+    """log f of the rough spectral dielectric (Walter et al. 2007) for unit
+    directions; None where f = 0."""
+    L = lib()
+    a = [np.ascontiguousarray(x, dtype=np.float32) for x in (i, o, n)]
+    out = C.c_double()
+    ok = L.rtc_rsd_logf(_p(a[0]), _p(a[1]), _p(a[2]), float(k), float(alpha), 1 if refl else 0, C.byref(out))
+    return out.value if ok else None

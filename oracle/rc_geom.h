@@ -362,6 +362,7 @@ struct Convex : Hittable {
             rec.normal = micro_tilt(facet, polish, rec.normal);
         }
         rec.u = u; rec.v = v;
+        rec.prim_id = idx >= 0 ? idx : 0;     // bounding half-space (facet) index
         return true;
     }
     bool any_hit(const Ray& r, float t_min, float t_max) const override {

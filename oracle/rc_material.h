@@ -242,6 +242,7 @@ static inline float specular_energy_comp(float mu_o, float alpha) {
 struct ScatterRecord { Color attenuation; Ray ray; bool skip_pdf; };
 
 struct Material {
+    int id = -1;    // index in the scene descriptor's material table (the adjoint checker's parameter map)
     virtual ~Material() {}
     virtual bool scatter(const Ray& r_in, const HitRecord& rec, SmallRng& rng, ScatterRecord& sr) const = 0;
     virtual Color emitted(float, float, const Point3&) const { return Color(); }
@@ -575,6 +576,51 @@ struct RoughDielectric : Material {
         float g1 = smith_g1(std::fabs(cos_i), alpha);
         sr.attenuation = Color(g1, g1, g1);
         sr.ray = Ray::scatter_from(rec.p, wi, r_in);
+        sr.skip_pdf = true;
+        return true;
+    }
+};
+
+// Rough dispersive dielectric (extension; no reference counterpart):
+// n(l) = n_d + s(l) - s(589.3 nm), s the Sellmeier law; GGX alpha of
+// primitive 0 in alpha[0], of primitive k >= 1 in alpha[1 + (k-1) % 16]; the
+// RoughDielectric sampler above with that n and alpha; the fire tint of
+// SpectralDielectric on the first refraction.
+struct RoughSpectralDielectric : Material {
+    float nd = 2.417f, sB[3] = {0, 0, 0}, sC[3] = {0, 0, 0};
+    float alpha[17];
+    float ior_at(float lambda) const { return nd + (sellmeier_ior(lambda, sB, sC) - sellmeier_ior(589.3f, sB, sC)); }
+    float alpha_of(int prim) const { float a = prim <= 0 ? alpha[0] : alpha[1 + (prim - 1) % 16]; return rclamp(a, 1e-4f, 1.0f); }
+    bool scatter(const Ray& r_in, const HitRecord& rec, SmallRng& rng, ScatterRecord& sr) const override {
+        float al = alpha_of(rec.prim_id);
+        float ior = ior_at(r_in.wavelength);
+        Vec3 n = rec.normal;
+        Vec3 wo = (-r_in.direction).unit();
+        float cos_o = wo.dot(n);
+        if (cos_o <= 0.0f) return false;
+        float eta = rec.front_face ? 1.0f / ior : ior;
+        Vec3 t, b; vonb(n, t, b);
+        float xi1 = rng.gen(), xi2 = rng.gen();
+        Vec3 wo_ts(wo.dot(t), wo.dot(b), cos_o);
+        Vec3 m_ts = vndf_sample_aniso(wo_ts, al, al, xi1, xi2);
+        Vec3 m = (t * m_ts.x + b * m_ts.y + n * m_ts.z).unit();
+        float fr = fresnel_dielectric(rmin(wo.dot(m), 1.0f), eta);
+        Vec3 wi;
+        bool refl = fr > rng.gen();
+        if (!refl) {
+            Vec3 out;
+            if (refract(-wo, m, eta, out)) wi = out.unit(); else refl = true;
+        }
+        if (refl) wi = reflect(-wo, m).unit();
+        float cos_i = wi.dot(n);
+        if (refl ? cos_i <= 0.0f : cos_i >= 0.0f) return false;
+        float g1 = smith_g1(std::fabs(cos_i), al);
+        Color att(g1, g1, g1);
+        bool tinted = false;
+        if (!r_in.spectral_weighted && !refl) { att = att * spectral_fire_rgb(r_in.wavelength); tinted = true; }
+        sr.ray = Ray::scatter_from(rec.p, wi, r_in);
+        if (tinted) sr.ray.spectral_weighted = true;
+        sr.attenuation = att;
         sr.skip_pdf = true;
         return true;
     }

@@ -485,7 +485,18 @@ struct SceneView {
     const PhotonMap* photons;  // may be null
 };
 
-static inline Color ray_color(const Ray& r0, const SceneView& S, float bg_scale, int max_depth, SmallRng& rng) {
+// Radiance hooks: every term added to the pixel passes through H::add (terms
+// fixed at this vertex) or H::add_v (terms that carry this vertex's BSDF
+// value), every sampled continuation through H::scatter.  The plain render
+// uses NoHook; the adjoint checker's hooks live in rc_adjoint.h.
+struct NoHook {
+    Color add(const Color& c) { return c; }
+    Color add_v(const Color& c, const HitRecord&, const Ray&, const Vec3&) { return c; }
+    void scatter(const HitRecord&, const Ray&, const Vec3&) {}
+};
+
+template <class H>
+static inline Color ray_color_h(const Ray& r0, const SceneView& S, float bg_scale, int max_depth, SmallRng& rng, H& hk) {
     const Hittable& world = *S.world;
     const HittableList& lights = *S.lights;
     const Background& background = *S.bg;
@@ -500,11 +511,11 @@ static inline Color ray_color(const Ray& r0, const SceneView& S, float bg_scale,
             float env_pdf = env_pdf_value(ray.direction, background);
             float emit_w = (sun_pdf + env_pdf == 0.0f) ? 1.0f : (prev_specular ? prev_spec_sun_weight : prev_mis_w_brdf);
             Color bg = (depth == 0 || !background.has_env()) ? background.eval(ray.direction) : background.env_eval(ray.direction);
-            color += throughput * bg * bg_scale * emit_w;
+            color += hk.add(throughput * bg * bg_scale * emit_w);
             break;
         }
         float emit_w = (prev_specular || lights.objects.empty()) ? 1.0f : prev_mis_w_brdf;
-        color += throughput * rec.mat->emitted_at(rec.u, rec.v, rec.p, ray.wavelength, ray.spectral_weighted) * emit_w;
+        color += hk.add(throughput * rec.mat->emitted_at(rec.u, rec.v, rec.p, ray.wavelength, ray.spectral_weighted) * emit_w);
         ScatterRecord sr;
         if (!rec.mat->scatter(ray, rec, rng, sr)) break;
         if (sr.skip_pdf) {
@@ -521,13 +532,14 @@ static inline Color ray_color(const Ray& r0, const SceneView& S, float bg_scale,
                         float p_mat = rec.mat->specular_sampling_pdf(ray, rec, sun_sample);
                         Color sun_color = background.eval(sun_sample);
                         float mis_d = p_sun * p_sun + p_mat * p_mat;
-                        if (mis_d > 0.0f) color += throughput * tr * brdf_cos * sun_color * p_sun / mis_d;
+                        if (mis_d > 0.0f) color += hk.add(throughput * tr * brdf_cos * sun_color * p_sun / mis_d);
                     }
                 }
                 float p_mat_brdf = rec.mat->specular_sampling_pdf(ray, rec, sr.ray.direction);
                 if (p_mat_brdf > 0.0f)
                     prev_spec_sun_weight = (p_mat_brdf * p_mat_brdf) / rmax(p_mat_brdf * p_mat_brdf + p_sun * p_sun, 1e-8f);
             }
+            hk.scatter(rec, ray, sr.ray.direction);
             throughput *= sr.attenuation;
             ray = sr.ray;
             prev_specular = true;
@@ -537,7 +549,7 @@ static inline Color ray_color(const Ray& r0, const SceneView& S, float bg_scale,
                 Color irr = S.photons->irradiance(rec.p, rec.normal);
                 if (irr.x > 0.0f || irr.y > 0.0f || irr.z > 0.0f) {
                     Color alb = rec.mat->albedo_hint(rec.u, rec.v, rec.p);
-                    color += throughput * alb * irr;
+                    color += hk.add_v(throughput * alb * irr, rec, ray, -ray.direction);
                 }
             }
             if (!lights.objects.empty()) {
@@ -554,7 +566,7 @@ static inline Color ray_color(const Ray& r0, const SceneView& S, float bg_scale,
                         float mis_d = l_pdf * l_pdf + brdf * brdf + sun_p * sun_p + env_p * env_p;
                         if (mis_d > 0.0f && brdf > 0.0f) {
                             Color nee_emit = lrec.mat->emitted_at(lrec.u, lrec.v, lrec.p, ray.wavelength, ray.spectral_weighted);
-                            color += throughput * tr * sr.attenuation * brdf * l_pdf * nee_emit / mis_d;
+                            color += hk.add_v(throughput * tr * sr.attenuation * brdf * l_pdf * nee_emit / mis_d, rec, ray, light_dir);
                         }
                     }
                 }
@@ -571,7 +583,7 @@ static inline Color ray_color(const Ray& r0, const SceneView& S, float bg_scale,
                         Color sun_color = background.eval(sun_sample);
                         float area_pdf = lights.objects.empty() ? 0.0f : lights.pdf_value(rec.p, sun_sample, ray.time);
                         float mis_d = sun_pdf * sun_pdf + brdf * brdf + area_pdf * area_pdf;
-                        color += throughput * tr * sr.attenuation * brdf * sun_pdf * sun_color / mis_d;
+                        color += hk.add_v(throughput * tr * sr.attenuation * brdf * sun_pdf * sun_color / mis_d, rec, ray, sun_sample);
                     }
                 }
             }
@@ -587,7 +599,7 @@ static inline Color ray_color(const Ray& r0, const SceneView& S, float bg_scale,
                             float area_pdf = lights.objects.empty() ? 0.0f : lights.pdf_value(rec.p, env_dir, ray.time);
                             Color env_color = background.env_eval(env_dir);
                             float mis_d = env_pdf * env_pdf + brdf * brdf + area_pdf * area_pdf;
-                            color += throughput * tr * sr.attenuation * brdf * env_pdf * env_color / mis_d;
+                            color += hk.add_v(throughput * tr * sr.attenuation * brdf * env_pdf * env_color / mis_d, rec, ray, env_dir);
                         }
                     }
                 }
@@ -608,6 +620,7 @@ static inline Color ray_color(const Ray& r0, const SceneView& S, float bg_scale,
             float env = env_pdf_value(ind_dir, background);
             float nee_pdf_sq = area * area + sun * sun + env * env;
             prev_mis_w_brdf = (pdf_val * pdf_val) / rmax(pdf_val * pdf_val + nee_pdf_sq, 1e-8f);
+            hk.scatter(rec, ray, ind_dir);
             throughput *= sr.attenuation * (scat_pdf / pdf_val);
             ray = scattered;
             prev_specular = false;
@@ -622,6 +635,11 @@ static inline Color ray_color(const Ray& r0, const SceneView& S, float bg_scale,
     return color;
 }
 
+static inline Color ray_color(const Ray& r0, const SceneView& S, float bg_scale, int max_depth, SmallRng& rng) {
+    NoHook nh;
+    return ray_color_h(r0, S, bg_scale, max_depth, rng, nh);
+}
+
 struct SampleParams {
     uint32_t sample_idx, strata, width, height, max_depth;
     float bg_scale;
@@ -630,7 +648,8 @@ struct SampleParams {
 };
 
 // Per-pixel body of render_tiles (src/renderer.rs:693-727).
-static inline Color render_pixel(const SceneView& S, const Camera& cam, const SampleParams& P, uint32_t row, uint32_t col, SmallRng& rng) {
+template <class H>
+static inline Color render_pixel_h(const SceneView& S, const Camera& cam, const SampleParams& P, uint32_t row, uint32_t col, SmallRng& rng, H& hk) {
     uint32_t i = row * P.width + col;
     float w_denom = (float)std::max<uint32_t>(P.width - 1, 1);
     float h_denom = (float)std::max<uint32_t>(P.height - 1, 1);
@@ -652,7 +671,11 @@ static inline Color render_pixel(const SceneView& S, const Camera& cam, const Sa
     Ray cam_ray = cam.get_ray(u, v, rng);
     if (P.use_frame_time) cam_ray.time = P.t0 + rng.gen() * (P.t1 - P.t0);
     cam_ray.wavelength = rng.gen_range(380.0f, 700.0f);
-    return ray_color(cam_ray, S, P.bg_scale, (int)P.max_depth, rng);
+    return ray_color_h(cam_ray, S, P.bg_scale, (int)P.max_depth, rng, hk);
+}
+static inline Color render_pixel(const SceneView& S, const Camera& cam, const SampleParams& P, uint32_t row, uint32_t col, SmallRng& rng) {
+    NoHook nh;
+    return render_pixel_h(S, cam, P, row, col, rng, nh);
 }
 
 // The GPU path seeds one stream per pixel instead of one per row; the oracle

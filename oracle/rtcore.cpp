@@ -8,6 +8,7 @@
 // wrapper, then the SRT chain; emissive world objects are NOT shared into the
 // light list here — the descriptor lists lights explicitly.
 #include "rc_render.h"
+#include "rc_adjoint.h"
 #include "../include/rt_api.h"
 #include <string>
 #include <cstdio>
@@ -18,6 +19,7 @@ namespace {
 
 struct OScene {
     std::vector<std::unique_ptr<Material>> mats;
+    std::vector<rt_material> desc_mats;     // the descriptor's table, kept for the adjoint checker
     std::vector<HPtr> world_objs;
     BvhTree world;
     HittableList lights;
@@ -56,6 +58,13 @@ Material* make_material(const rt_material& m) {
         case RT_MAT_DIFFUSE_LIGHT: { auto* d = new DiffuseLight(); d->emit = Color(m.p[0], m.p[1], m.p[2]); return d; }
         case RT_MAT_SPECTRAL_METAL: { auto* s = new SpectralMetal(); s->variant = (int)m.p[0]; s->roughness = m.p[1]; s->init(); return s; }
         case RT_MAT_ROUGH_DIELECTRIC: { auto* r = new RoughDielectric(); r->ior = m.p[0]; r->roughness = m.p[1]; return r; }
+        case RT_MAT_ROUGH_SPECTRAL: {
+            auto* r = new RoughSpectralDielectric();
+            r->nd = m.p[0];
+            for (int k = 0; k < 3; k++) { r->sB[k] = m.p[1 + k]; r->sC[k] = m.p[4 + k]; }
+            for (int k = 0; k < 17; k++) r->alpha[k] = m.p[7 + k];
+            return r;
+        }
         default: return nullptr;
     }
 }
@@ -157,7 +166,9 @@ bool build_scene(OScene& S, const rt_scene_desc& d) {
     for (uint32_t i = 0; i < d.num_materials; i++) {
         Material* m = make_material(d.materials[i]);
         if (!m) { S.err = "unsupported material kind"; return false; }
+        m->id = (int)i;
         S.mats.emplace_back(m);
+        S.desc_mats.push_back(d.materials[i]);
     }
     if (d.num_objects == 0) { S.err = "empty world"; return false; }
     for (uint32_t i = 0; i < d.num_objects; i++) {
@@ -463,6 +474,101 @@ int rtc_env_eval(const float* rgb, uint32_t w, uint32_t h, float rot, const floa
         out[i * 4] = c.x; out[i * 4 + 1] = c.y; out[i * 4 + 2] = c.z; out[i * 4 + 3] = e.dir_pdf(d);
     }
     return 0;
+}
+
+// ---- adjoint checker (extension; see rc_adjoint.h) ------------------------------------------
+// Writes theta into the descriptor table and the live materials in place
+// (the object graph keeps Material pointers).
+int rtc_set_params(void* s, const rt_param* prm, uint32_t n, const float* vals) {
+    OScene& S = *(OScene*)s;
+    ParamMap pm;
+    if (!pm.build(S.desc_mats, prm, n)) return -1;
+    for (uint32_t i = 0; i < n; i++)
+        for (uint32_t f = prm[i].field; f < prm[i].field + prm[i].count; f++) S.desc_mats[prm[i].material].p[f] = vals[i];
+    for (size_t i = 0; i < S.mats.size(); i++) {
+        const rt_material& m = S.desc_mats[i];
+        if (auto* l = dynamic_cast<Lambertian*>(S.mats[i].get())) l->tex.even = Color(m.p[0], m.p[1], m.p[2]);
+        if (auto* r = dynamic_cast<RoughSpectralDielectric*>(S.mats[i].get())) {
+            r->nd = m.p[0];
+            for (int k = 0; k < 3; k++) { r->sB[k] = m.p[1 + k]; r->sC[k] = m.p[4 + k]; }
+            for (int k = 0; k < 17; k++) r->alpha[k] = m.p[7 + k];
+        }
+    }
+    return 0;
+}
+
+static uint64_t row_or_pixel_seed(int mode, uint32_t row, uint32_t i, const SampleParams& P) {
+    return mode == 1 ? pixel_seed(i, P.sample_idx, P.seed)
+                     : (((uint64_t)row * 0x9E3779B97F4A7C15ULL) ^ ((uint64_t)P.sample_idx * 6364136223846793005ULL));
+}
+
+// Detached estimator (decisions at the scene's parameters, BSDF values at
+// eval_vals), summed over spp passes into accum.
+int rtc_render_detached(void* s, const rt_camera* cam, const rt_sample_params* p, uint32_t spp, const rt_param* prm,
+                        uint32_t n, const float* eval_vals, float* accum, int mode, int threads) {
+    OScene& S = *(OScene*)s;
+    ParamMap pm;
+    if (!pm.build(S.desc_mats, prm, n)) return -1;
+    std::vector<rt_material> me = S.desc_mats;
+    for (uint32_t i = 0; i < n; i++)
+        for (uint32_t f = prm[i].field; f < prm[i].field + prm[i].count; f++) me[prm[i].material].p[f] = eval_vals[i];
+    SceneView V = view(S, p->use_photons != 0);
+    Camera C = to_cam(*cam);
+    SampleParams base = to_params(*p);
+    size_t npix = (size_t)p->width * p->height;
+    for (size_t i = 0; i < npix * 3; i++) accum[i] = 0.0f;
+    for (uint32_t k = 0; k < spp; k++) {
+        SampleParams P = base; P.sample_idx = base.sample_idx + k;
+        parallel_for(P.height, threads, [&](size_t row) {
+            SmallRng rng = SmallRng::seed_from_u64(row_or_pixel_seed(0, (uint32_t)row, 0, P));
+            for (uint32_t col = 0; col < P.width; col++) {
+                size_t i = row * P.width + col;
+                if (mode == 1) rng = SmallRng::seed_from_u64(row_or_pixel_seed(1, (uint32_t)row, (uint32_t)i, P));
+                RatioHook hk; hk.ms = &S.desc_mats; hk.me = &me;
+                Color c = render_pixel_h(V, C, P, (uint32_t)row, col, rng, hk);
+                accum[3 * i] += c.x; accum[3 * i + 1] += c.y; accum[3 * i + 2] += c.z;
+            }
+        });
+    }
+    return 0;
+}
+
+// grad[j] = sum over pixels and samples of dloss_dimage / spp . dC/dtheta_j,
+// by forward score accumulation (double accumulators).
+int rtc_backward(void* s, const rt_camera* cam, const rt_sample_params* p, uint32_t spp, const rt_param* prm, uint32_t n,
+                 const float* dimg, double* grad, int mode, int threads) {
+    OScene& S = *(OScene*)s;
+    ParamMap pm;
+    if (!pm.build(S.desc_mats, prm, n)) return -1;
+    SceneView V = view(S, p->use_photons != 0);
+    Camera C = to_cam(*cam);
+    SampleParams base = to_params(*p);
+    std::vector<std::vector<double>> rows(p->height, std::vector<double>(n, 0.0));
+    for (uint32_t k = 0; k < spp; k++) {
+        SampleParams P = base; P.sample_idx = base.sample_idx + k;
+        parallel_for(P.height, threads, [&](size_t row) {
+            SmallRng rng = SmallRng::seed_from_u64(row_or_pixel_seed(0, (uint32_t)row, 0, P));
+            ScoreHook hk;
+            for (uint32_t col = 0; col < P.width; col++) {
+                size_t i = row * P.width + col;
+                if (mode == 1) rng = SmallRng::seed_from_u64(row_or_pixel_seed(1, (uint32_t)row, (uint32_t)i, P));
+                Color d(dimg[3 * i] / (float)spp, dimg[3 * i + 1] / (float)spp, dimg[3 * i + 2] / (float)spp);
+                hk.init(&S.desc_mats, &pm, d, rows[row].data());
+                render_pixel_h(V, C, P, (uint32_t)row, col, rng, hk);
+            }
+        });
+    }
+    for (uint32_t j = 0; j < n; j++) {
+        double t = 0.0;
+        for (auto& r : rows) t += r[j];
+        grad[j] = t;
+    }
+    return 0;
+}
+
+// log f of the rough spectral dielectric for explicit directions (tests).
+int rtc_rsd_logf(const float* i, const float* o, const float* nrm, double k, double alpha, int refl, double* lf) {
+    return logf_rsd(D3{i[0], i[1], i[2]}, D3{o[0], o[1], o[2]}, D3{nrm[0], nrm[1], nrm[2]}, k, alpha, refl != 0, *lf) ? 1 : 0;
 }
 
 }  // extern "C"

@@ -19,7 +19,7 @@ SHAPE_SPHERE, SHAPE_QUAD, SHAPE_MESH, SHAPE_BOX, SHAPE_CONVEX, SHAPE_CYLINDER, S
 XFORM_TRANSLATE, XFORM_ROTATE, XFORM_SCALE = 1, 2, 3
 MEDIUM_CONSTANT, MEDIUM_GRID = 1, 2
 (MAT_LAMBERTIAN, MAT_DIELECTRIC, MAT_SPECTRAL_DIELECTRIC, MAT_PBR, MAT_DIFFUSE_LIGHT,
- MAT_SPECTRAL_METAL, MAT_ROUGH_DIELECTRIC) = 1, 2, 3, 4, 5, 6, 7
+ MAT_SPECTRAL_METAL, MAT_ROUGH_DIELECTRIC, MAT_ROUGH_SPECTRAL) = 1, 2, 3, 4, 5, 6, 7, 8
 BG_SOLID, BG_GRADIENT, BG_PHYSICAL, BG_ENVMAP, BG_ENVMAP_SPLIT = 0, 1, 2, 3, 4
 EMIT_QUAD, EMIT_SUN_DISK = 0, 1
 
@@ -96,6 +96,10 @@ class rt_batch_view(C.Structure):
                 ("accum_rgb", C.c_void_p)]
 
 
+class rt_param(C.Structure):
+    _fields_ = [("material", C.c_uint32), ("field", C.c_uint32), ("count", C.c_uint32)]
+
+
 class rt_photon_params(C.Structure):
     _fields_ = [("emitter", C.c_uint32), ("origin", C.c_float * 3), ("u", C.c_float * 3),
                 ("v", C.c_float * 3), ("color", C.c_float * 3), ("sun_dir", C.c_float * 3),
@@ -132,6 +136,12 @@ SIGNATURES = {
     "rt_render_wavefront": (C.c_int, [VP, VP, C.POINTER(rt_camera), C.POINTER(rt_sample_params), C.c_uint32, VP,
                                       C.POINTER(rt_wavefront_opts)]),
     "rt_render_batch": (C.c_int, [VP, C.POINTER(rt_batch_view), C.c_uint32, C.POINTER(rt_wavefront_opts)]),
+    "rt_scene_set_params": (C.c_int, [VP, VP, C.POINTER(rt_param), C.c_uint32, VP]),
+    "rt_scene_get_params": (C.c_int, [VP, VP, C.POINTER(rt_param), C.c_uint32, VP]),
+    "rt_render_detached": (C.c_int, [VP, VP, C.POINTER(rt_camera), C.POINTER(rt_sample_params), C.c_uint32,
+                                     C.POINTER(rt_param), C.c_uint32, VP, VP]),
+    "rt_backward": (C.c_int, [VP, VP, C.POINTER(rt_camera), C.POINTER(rt_sample_params), C.c_uint32,
+                              C.POINTER(rt_param), C.c_uint32, VP, VP]),
     "rt_photon_build": (C.c_int, [VP, VP, C.POINTER(rt_photon_params), C.POINTER(C.c_uint32)]),
     "rt_photon_irradiance": (C.c_int, [VP, VP, VP, C.c_uint32, VP]),
     "rt_photon_export": (C.c_int, [VP, VP, VP, C.c_uint32, C.POINTER(C.c_uint32)]),

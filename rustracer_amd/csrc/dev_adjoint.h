@@ -1,0 +1,301 @@
+// Derivatives of the radiance estimator with respect to material parameters
+// (extension: the reference renders only; it has no derivatives).
+//
+// Estimator.  Every sampling decision of a path (directions, Fresnel and lobe
+// choices, Russian roulette) is drawn with the scene's current parameters
+// theta_s; every BSDF value is evaluated at theta.  A path's contribution is
+// then  C(theta) = sum_j e_j * prod_{k<j} f_theta(v_k) |cos| / p_theta_s(v_k),
+// which equals the plain renderer's at theta = theta_s and is a smooth
+// function of theta for fixed random numbers: material parameters move no
+// geometry and no decision.  Its derivative at theta = theta_s is
+//   dC/dtheta = sum_k dlog f(v_k)/dtheta * (radiance reaching the camera
+//               through vertex k's outgoing direction)
+//             + sum_j e_j-local terms (NEE / photon gather at a vertex whose
+//               own BSDF value depends on theta).
+//
+// Two hook sets plug into bounce_t:
+//   DetachedHook  forward render at theta != theta_s (finite differences with
+//                 common random numbers);
+//   ReplayHook    path-replay backward: the sample's radiance L is computed
+//                 first, then the path is traced again from the same stream
+//                 and each vertex's dlog f is weighted by the radiance still to
+//                 come (L minus everything collected before that vertex).
+//
+// Differentiable fields: Lambertian (solid) albedo p0-2 (dlog f_c/da_c =
+// 1/a_c); rough spectral dielectric n_d, Sellmeier B1-3 and GGX alpha, with
+// dlog f/dn and dlog f/dalpha from forward-mode dual numbers through the
+// Walter et al. (2007) microfacet BSDF.
+#pragma once
+
+namespace rt {
+
+// ---- forward-mode dual numbers -------------------------------------------------------------
+struct Dl {
+    float v, d;
+    RT_HD Dl() : v(0.0f), d(0.0f) {}
+    RT_HD Dl(float a, float b = 0.0f) : v(a), d(b) {}
+};
+RT_HD Dl operator+(Dl a, Dl b) { return Dl(a.v + b.v, a.d + b.d); }
+RT_HD Dl operator-(Dl a, Dl b) { return Dl(a.v - b.v, a.d - b.d); }
+RT_HD Dl operator*(Dl a, Dl b) { return Dl(a.v * b.v, a.d * b.v + a.v * b.d); }
+RT_HD Dl operator/(Dl a, Dl b) { return Dl(a.v / b.v, (a.d * b.v - a.v * b.d) / (b.v * b.v)); }
+RT_HD Dl operator-(Dl a) { return Dl(-a.v, -a.d); }
+RT_HD Dl operator+(Dl a, float b) { return Dl(a.v + b, a.d); }
+RT_HD Dl operator+(float a, Dl b) { return Dl(a + b.v, b.d); }
+RT_HD Dl operator-(Dl a, float b) { return Dl(a.v - b, a.d); }
+RT_HD Dl operator-(float a, Dl b) { return Dl(a - b.v, -b.d); }
+RT_HD Dl operator*(Dl a, float b) { return Dl(a.v * b, a.d * b); }
+RT_HD Dl operator*(float a, Dl b) { return Dl(a * b.v, a * b.d); }
+RT_HD Dl operator/(Dl a, float b) { return Dl(a.v / b, a.d / b); }
+RT_HD Dl operator/(float a, Dl b) { return Dl(a / b.v, -a * b.d / (b.v * b.v)); }
+RT_HD float val(float a) { return a; }
+RT_HD float val(Dl a) { return a.v; }
+RT_HD float xsqrt(float a) { return sqrtf(a); }
+RT_HD Dl xsqrt(Dl a) { float s = sqrtf(a.v); return Dl(s, s > 0.0f ? a.d * 0.5f / s : 0.0f); }
+RT_HD float xlog(float a) { return logf(a); }
+RT_HD Dl xlog(Dl a) { return Dl(logf(a.v), a.d / a.v); }
+RT_HD float xabs(float a) { return fabsf(a); }
+RT_HD Dl xabs(Dl a) { return a.v < 0.0f ? -a : a; }
+RT_HD double val(double a) { return a; }
+RT_HD double xsqrt(double a) { return sqrt(a); }
+RT_HD double xlog(double a) { return log(a); }
+RT_HD double xabs(double a) { return fabs(a); }
+
+// ---- Walter et al. 2007 rough dielectric, log of the BSDF value ---------------------------
+// The same formulas as the sampler (fresnel_dielectric, smith_g1, ggx_ndf),
+// over T = float or Dl.
+template <class T> RT_HD T t_fresnel(T ci, T eta) {
+    T s2i = T(1.0f) - ci * ci;
+    if (val(s2i) < 0.0f) s2i = T(0.0f);
+    T s2t = eta * eta * s2i;
+    if (val(s2t) >= 1.0f) return T(1.0f);
+    T ct = xsqrt(T(1.0f) - s2t);
+    T rs = (eta * ci - ct) / (eta * ci + ct);
+    T rp = (eta * ct - ci) / (eta * ct + ci);
+    return (rs * rs + rp * rp) * 0.5f;
+}
+template <class T> RT_HD T t_g1(float c, T al) { T a2 = al * al; return (2.0f * c) / (c + xsqrt(a2 + (T(1.0f) - a2) * (c * c))); }
+template <class T> RT_HD T t_ggx(T ch, T al) { T a2 = al * al; T d = T(1.0f) + ch * ch * (a2 - 1.0f); return a2 / (PI * d * d); }
+
+// log f(i, o) for unit i (toward the previous vertex), o (the new direction),
+// shading normal n; k = IOR on o's side over IOR on i's side for a
+// transmission (the Fresnel eta is 1/k in both cases).  False where f = 0.
+template <class T>
+RT_HD bool rsd_logf(V3 i, V3 o, V3 n, T k, T al, bool refl, T& lf) {
+    float cin = dot(i, n), con = dot(o, n);
+    float ain = fabsf(cin), aon = fabsf(con);
+    if (!(ain > 0.0f) || !(aon > 0.0f)) return false;
+    T eta = T(1.0f) / k;
+    T G = t_g1(ain, al) * t_g1(aon, al);
+    if (refl) {
+        V3 h = unit(i + o);
+        if (!(dot(i, h) * cin > 0.0f)) return false;
+        T F = t_fresnel(T(fminf(dot(i, h), 1.0f)), eta);
+        T D = t_ggx(T(dot(h, n)), al);
+        if (!(val(F) > 0.0f) || !(val(D) > 0.0f) || !(val(G) > 0.0f)) return false;
+        lf = xlog(F) + xlog(G) + xlog(D) - logf(4.0f * ain * aon);
+        return true;
+    }
+    T hx = i.x + k * o.x, hy = i.y + k * o.y, hz = i.z + k * o.z;
+    T len = xsqrt(hx * hx + hy * hy + hz * hz);
+    if (!(val(len) > 0.0f)) return false;
+    hx = hx / len; hy = hy / len; hz = hz / len;
+    T hn = hx * n.x + hy * n.y + hz * n.z;
+    if (val(hn) < 0.0f) { hx = -hx; hy = -hy; hz = -hz; hn = -hn; }
+    T ih = hx * i.x + hy * i.y + hz * i.z;
+    T oh = hx * o.x + hy * o.y + hz * o.z;
+    // the microfacet must face both directions from their own sides
+    if (!(val(ih) * cin > 0.0f) || !(val(oh) * con > 0.0f)) return false;
+    T aih = xabs(ih);
+    if (val(aih) > 1.0f) aih = T(1.0f);
+    T tr = T(1.0f) - t_fresnel(aih, eta);
+    T den = ih + k * oh;
+    T D = t_ggx(hn, al);
+    if (!(val(tr) > 0.0f) || !(val(D) > 0.0f) || !(val(G) > 0.0f) || val(ih) == 0.0f || val(oh) == 0.0f || val(den) == 0.0f)
+        return false;
+    lf = xlog(tr) + xlog(G) + xlog(D) + xlog(aih) + xlog(xabs(oh)) + 2.0f * xlog(k) - 2.0f * xlog(xabs(den)) - logf(ain * aon);
+    return true;
+}
+
+// d n(lambda) / d B_j of the rough spectral dielectric's dispersion law.
+RT_HD float rsd_dn_dB(const float* p, float lambda, int j) {
+    auto term = [&](float lam) {
+        float l = lam * 1e-3f, l2 = l * l;
+        float s = sqrtf(1.0f + p[1] * l2 / (l2 - p[4]) + p[2] * l2 / (l2 - p[5]) + p[3] * l2 / (l2 - p[6]));
+        return (l2 / (l2 - p[4 + j])) / (2.0f * s);
+    };
+    return term(lambda) - term(RSD_LAMBDA_D);
+}
+
+// ---- per-vertex scores ---------------------------------------------------------------------
+#define RT_SCORE_MAX 6
+struct Score {
+    int n;
+    int p[RT_SCORE_MAX];
+    V3 v[RT_SCORE_MAX];
+};
+RT_D void score_add(Score& sc, int param, V3 v) {
+    for (int k = 0; k < sc.n; k++) if (sc.p[k] == param) { sc.v[k] += v; return; }
+    if (sc.n < RT_SCORE_MAX) { sc.p[sc.n] = param; sc.v[sc.n] = v; sc.n++; }
+}
+
+// dlog f / dtheta at a vertex (wo toward the previous vertex, wi the new or
+// light direction, both unit).  fmap[24 * mat + field] = parameter or -1.
+RT_D void vertex_score(const DScene& S, const int8_t* fmap, const Rec& h, V3 wo, V3 wi, float lambda, Score& sc) {
+    sc.n = 0;
+    if (h.medium) return;
+    const DMaterial& m = S.mats[h.mat];
+    const int8_t* fm = fmap + 24 * (size_t)h.mat;
+    if (m.kind == M_LAMBERTIAN) {
+        if (m.tex != 0) return;
+        if (fm[0] >= 0) score_add(sc, fm[0], mk(1.0f / m.p[0], 0.0f, 0.0f));
+        if (fm[1] >= 0) score_add(sc, fm[1], mk(0.0f, 1.0f / m.p[1], 0.0f));
+        if (fm[2] >= 0) score_add(sc, fm[2], mk(0.0f, 0.0f, 1.0f / m.p[2]));
+    } else if (m.kind == M_ROUGH_SPECTRAL) {
+        int fa = rsd_alpha_field(h.prim_id);
+        float nl = rsd_ior(m, lambda);
+        float k = h.front ? nl : 1.0f / nl, dk = h.front ? 1.0f : -1.0f / (nl * nl);
+        float al = fclamp(m.p[fa], 1e-4f, 1.0f);
+        bool refl = dot(wi, h.n) > 0.0f;
+        if (fm[0] >= 0 || fm[1] >= 0 || fm[2] >= 0 || fm[3] >= 0) {
+            Dl lf;
+            if (rsd_logf<Dl>(wo, wi, h.n, Dl(k, dk), Dl(al, 0.0f), refl, lf)) {
+                float gn = lf.d;
+                if (fm[0] >= 0) score_add(sc, fm[0], mk(gn, gn, gn));
+                for (int j = 0; j < 3; j++)
+                    if (fm[1 + j] >= 0) { float g = gn * rsd_dn_dB(m.p, lambda, j); score_add(sc, fm[1 + j], mk(g, g, g)); }
+            }
+        }
+        if (fm[fa] >= 0 && m.p[fa] > 1e-4f && m.p[fa] < 1.0f) {
+            Dl lf;
+            if (rsd_logf<Dl>(wo, wi, h.n, Dl(k, 0.0f), Dl(al, 1.0f), refl, lf)) score_add(sc, fm[fa], mk(lf.d, lf.d, lf.d));
+        }
+    }
+}
+
+// n(lambda) in double: the detached ratio compares two parameter sets whose
+// n differ by a few float ulps for a 1e-3 step of a Sellmeier coefficient.
+RT_D double rsd_ior_d(const DMaterial& m, float lambda) {
+    auto sell = [&](double lam) {
+        double l = lam * 1e-3, l2 = l * l;
+        return sqrt(1.0 + m.p[1] * l2 / (l2 - m.p[4]) + m.p[2] * l2 / (l2 - m.p[5]) + m.p[3] * l2 / (l2 - m.p[6]));
+    };
+    return (double)m.p[0] + (sell(lambda) - sell(RSD_LAMBDA_D));
+}
+
+// f_theta / f_theta_s at a vertex: `me` holds the evaluation materials.  The
+// rough spectral dielectric's logs are taken in double: at alpha ~ 0.005 the
+// GGX denominator 1 + c^2 (alpha^2 - 1) cancels to a few float ulps, which
+// would swamp the ratio of two nearby parameter sets.
+RT_D V3 vertex_ratio(const DScene& S, const DMaterial* me, const Rec& h, V3 wo, V3 wi, float lambda) {
+    if (h.medium) return mk(1, 1, 1);
+    const DMaterial& ms = S.mats[h.mat];
+    const DMaterial& mv = me[h.mat];
+    if (ms.kind == M_LAMBERTIAN) {
+        V3 a = tex_value(ms, h.p), b = tex_value(mv, h.p);
+        return mk(a.x != 0.0f ? b.x / a.x : 1.0f, a.y != 0.0f ? b.y / a.y : 1.0f, a.z != 0.0f ? b.z / a.z : 1.0f);
+    }
+    if (ms.kind == M_ROUGH_SPECTRAL) {
+        bool refl = dot(wi, h.n) > 0.0f;
+        double ns = rsd_ior_d(ms, lambda), ne = rsd_ior_d(mv, lambda);
+        double ls, le;
+        bool vs = rsd_logf<double>(wo, wi, h.n, h.front ? ns : 1.0 / ns, (double)rsd_alpha(ms, h), refl, ls);
+        bool ve = rsd_logf<double>(wo, wi, h.n, h.front ? ne : 1.0 / ne, (double)rsd_alpha(mv, h), refl, le);
+        if (!vs) return mk(1, 1, 1);
+        if (!ve) return mk(0, 0, 0);
+        float r = (float)exp(le - ls);
+        return mk(r, r, r);
+    }
+    return mk(1, 1, 1);
+}
+
+// ---- hooks -----------------------------------------------------------------------------------
+struct DetachedHook {
+    const DMaterial* me;
+    V3 ratio;
+    RT_D V3 emit(V3 c) { return c * ratio; }
+    RT_D V3 emit_vertex(V3 c, const DScene& S, const Rec& h, const Ray& r) {
+        V3 wo = unit(-r.d);
+        return c * ratio * vertex_ratio(S, me, h, wo, wo, r.wavelength);
+    }
+    RT_D void shadow(ShadowReq& q, int, const DScene& S, const Rec& h, const Ray& r, bool vdep) {
+        q.contrib *= ratio;
+        if (vdep) q.contrib *= vertex_ratio(S, me, h, unit(-r.d), unit(q.d), r.wavelength);
+    }
+    RT_D void scatter(const DScene& S, const Rec& h, const Ray& r, V3 d) {
+        ratio *= vertex_ratio(S, me, h, unit(-r.d), unit(d), r.wavelength);
+    }
+    RT_D void begin_vertex() {}
+    RT_D void resolved(int, V3) {}
+    RT_D void end_vertex() {}
+};
+
+struct ReplayHook {
+    const int8_t* fmap;
+    float* g;             // per-block LDS accumulators, one per parameter
+    V3 delta;             // dLoss / d(sample radiance)
+    V3 rem;               // radiance of this sample not yet collected
+    Score vs, pend;
+    bool vs_ok, pend_ok;
+    bool vdep[3];
+    RT_D void add(const Score& sc, V3 c) {
+        for (int k = 0; k < sc.n; k++) {
+            float v = delta.x * c.x * sc.v[k].x + delta.y * c.y * sc.v[k].y + delta.z * c.z * sc.v[k].z;
+            if (v != 0.0f && isfinite(v)) atomicAdd(g + sc.p[k], v);
+        }
+    }
+    RT_D V3 emit(V3 c) { rem -= c; return c; }
+    RT_D V3 emit_vertex(V3 c, const DScene& S, const Rec& h, const Ray& r) {
+        rem -= c;
+        Score sc;
+        V3 wo = unit(-r.d);
+        vertex_score(S, fmap, h, wo, wo, r.wavelength, sc);
+        add(sc, c);
+        return c;
+    }
+    RT_D void shadow(ShadowReq& q, int k, const DScene& S, const Rec& h, const Ray& r, bool vd) {
+        vdep[k] = vd;
+        if (vd && !vs_ok) { vertex_score(S, fmap, h, unit(-r.d), unit(q.d), r.wavelength, vs); vs_ok = true; }
+    }
+    RT_D void scatter(const DScene& S, const Rec& h, const Ray& r, V3 d) {
+        vertex_score(S, fmap, h, unit(-r.d), unit(d), r.wavelength, pend);
+        pend_ok = true;
+    }
+    RT_D void begin_vertex() { vs_ok = false; pend_ok = false; vdep[0] = vdep[1] = vdep[2] = false; }
+    RT_D void resolved(int k, V3 c) {
+        rem -= c;
+        if (vdep[k] && vs_ok) add(vs, c);
+    }
+    // radiance still to come arrives through this vertex's sampled direction
+    RT_D void end_vertex() { if (pend_ok) add(pend, rem); }
+};
+
+struct PlainHook : NoDiff {
+    RT_D void begin_vertex() {}
+    RT_D void resolved(int, V3) {}
+    RT_D void end_vertex() {}
+};
+
+// ray_color with hooks: the megakernel loop of ray_color (src/renderer.rs:371-585).
+template <class DH>
+RT_D V3 ray_color_t(const DScene& S, Ray ray, int max_depth, float bg_scale, bool use_photons, Rng& rng, Stack& st, DH& dh) {
+    PathState ps;
+    path_init(ps, ray);
+    V3 color = mk(0, 0, 0);
+    ShadowReq sh[3];
+    for (int depth = 0; depth < max_depth; depth++) {
+        HitId id = trace_closest(S, ps.ray, 0.001f, FINF, st);
+        int nsh = 0;
+        dh.begin_vertex();
+        bool cont = bounce_t(S, ps, id, depth, bg_scale, use_photons, rng, color, sh, nsh, st, dh);
+        for (int k = 0; k < nsh; k++) {
+            float tr = trace_transmittance(S, make_ray(sh[k].o, sh[k].d, sh[k].time), 0.001f, sh[k].t_max, st, false);
+            if (tr > 0.0f) { V3 c = sh[k].contrib * tr; color += c; dh.resolved(k, c); }
+        }
+        dh.end_vertex();
+        if (!cont) break;
+    }
+    return color;
+}
+
+}  // namespace rt

@@ -27,6 +27,7 @@ RT_HD V3 operator*(V3 a, float t) { return mk(a.x * t, a.y * t, a.z * t); }
 RT_HD V3 operator*(float t, V3 a) { return mk(a.x * t, a.y * t, a.z * t); }
 RT_HD V3 operator/(V3 a, float t) { return mk(a.x / t, a.y / t, a.z / t); }
 RT_HD V3& operator+=(V3& a, V3 b) { a.x += b.x; a.y += b.y; a.z += b.z; return a; }
+RT_HD V3& operator-=(V3& a, V3 b) { a.x -= b.x; a.y -= b.y; a.z -= b.z; return a; }
 RT_HD V3& operator*=(V3& a, V3 b) { a.x *= b.x; a.y *= b.y; a.z *= b.z; return a; }
 RT_HD V3& operator*=(V3& a, float t) { a.x *= t; a.y *= t; a.z *= t; return a; }
 RT_HD V3& operator/=(V3& a, float t) { a.x /= t; a.y /= t; a.z /= t; return a; }

@@ -652,6 +652,7 @@ RT_D Rec finalize(const DScene& S, const Ray& wr, const HitId& id, float t_min) 
     h.prim_id = 0;
     if (ob.shape == SH_MESH && id.prim >= 0) h.prim_id = S.tri_tag[ob.data_first + id.prim];
     else if (ob.shape == SH_BOX && id.prim >= 0) h.prim_id = (int32_t)S.quads[ob.data_first + id.prim].pad[0];
+    else if (ob.shape == SH_CONVEX && id.prim >= 0) h.prim_id = id.prim;    // bounding half-space (facet) index
     return h;
 }
 

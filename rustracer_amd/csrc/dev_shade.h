@@ -11,7 +11,7 @@
 namespace rt {
 
 enum { M_LAMBERTIAN = 1, M_DIELECTRIC = 2, M_SPECTRAL_DIELECTRIC = 3, M_PBR = 4, M_DIFFUSE_LIGHT = 5,
-       M_SPECTRAL_METAL = 6, M_ROUGH_DIELECTRIC = 7, M_PHASE = 100 };
+       M_SPECTRAL_METAL = 6, M_ROUGH_DIELECTRIC = 7, M_ROUGH_SPECTRAL = 8, M_PHASE = 100 };
 enum { BGK_SOLID = 0, BGK_GRADIENT = 1, BGK_PHYSICAL = 2, BGK_ENV = 3, BGK_ENV_SPLIT = 4 };
 
 __constant__ float c_cie_x[RT_CIE_N];
@@ -360,12 +360,12 @@ RT_D float smetal_spec_pdf(const DMaterial& m, const Ray& r, const Rec& h, V3 wi
 }
 
 // Rough GGX dielectric (extension): VNDF microfacet, exact Fresnel, weight G1(wi).
-RT_D bool rough_dielectric_scatter(const DMaterial& m, const Ray& r, const Rec& h, Rng& rng, Scatter& sr) {
-    float alpha = fmx(m.p[1] * m.p[1], 1e-4f);
+// eta = IOR on the incident side over the other side; returns the new
+// direction and whether it reflected.
+RT_D bool rough_dielectric_core(float alpha, float eta, const Ray& r, const Rec& h, Rng& rng, V3& wi, bool& refl, float& g1) {
     V3 n = h.n, wo = unit(-r.d);
     float cos_o = dot(wo, n);
     if (cos_o <= 0.0f) return false;
-    float eta = h.front ? 1.0f / m.p[0] : m.p[0];
     V3 t, b; vonb(n, t, b);
     float xi1 = rng_f(rng), xi2 = rng_f(rng);
     V3 wo_ts = mk(dot(wo, t), dot(wo, b), cos_o);
@@ -373,15 +373,44 @@ RT_D bool rough_dielectric_scatter(const DMaterial& m, const Ray& r, const Rec& 
     V3 mm = unit(t * mts.x + b * mts.y + n * mts.z);
     float com = fmn(dot(wo, mm), 1.0f);
     float fr = fresnel_dielectric(com, eta);
-    V3 wi;
-    bool refl = fr > rng_f(rng);
+    refl = fr > rng_f(rng);
     if (!refl) { V3 out; if (refract(-wo, mm, eta, out)) wi = unit(out); else refl = true; }
     if (refl) wi = unit(reflect(-wo, mm));
     float ci = dot(wi, n);
     if (refl ? ci <= 0.0f : ci >= 0.0f) return false;
-    float g1 = smith_g1(fabsf(ci), alpha);
+    g1 = smith_g1(fabsf(ci), alpha);
+    return true;
+}
+RT_D bool rough_dielectric_scatter(const DMaterial& m, const Ray& r, const Rec& h, Rng& rng, Scatter& sr) {
+    float alpha = fmx(m.p[1] * m.p[1], 1e-4f);
+    float eta = h.front ? 1.0f / m.p[0] : m.p[0];
+    V3 wi; bool refl; float g1;
+    if (!rough_dielectric_core(alpha, eta, r, h, rng, wi, refl, g1)) return false;
     sr.att = mk(g1, g1, g1);
     sr.ray = scatter_from(h.p, wi, r); sr.skip_pdf = true;
+    return true;
+}
+
+// Rough dispersive dielectric (extension): n(l) = n_d + s(l) - s(589.3 nm)
+// with s the Sellmeier law of p1-6; GGX alpha per primitive group; the fire
+// tint of the spectral dielectric on the first refraction.
+static constexpr float RSD_LAMBDA_D = 589.3f;
+RT_HD int rsd_alpha_field(int prim) { return prim <= 0 ? 7 : 8 + (prim - 1) % 16; }
+RT_D float rsd_ior(const DMaterial& m, float lambda) {
+    return m.p[0] + (sellmeier(lambda, m.p + 1, m.p + 4) - sellmeier(RSD_LAMBDA_D, m.p + 1, m.p + 4));
+}
+RT_D float rsd_alpha(const DMaterial& m, const Rec& h) { return fclamp(m.p[rsd_alpha_field(h.prim_id)], 1e-4f, 1.0f); }
+RT_D bool rsd_scatter(const DScene& S, const DMaterial& m, const Ray& r, const Rec& h, Rng& rng, Scatter& sr) {
+    float ior = rsd_ior(m, r.wavelength);
+    float eta = h.front ? 1.0f / ior : ior;
+    V3 wi; bool refl; float g1;
+    if (!rough_dielectric_core(rsd_alpha(m, h), eta, r, h, rng, wi, refl, g1)) return false;
+    V3 att = mk(g1, g1, g1);
+    bool tinted = false;
+    if (!r.spectral && !refl) { att *= spectral_fire_rgb(S, r.wavelength); tinted = true; }
+    sr.ray = scatter_from(h.p, wi, r);
+    if (tinted) sr.ray.spectral = true;
+    sr.att = att; sr.skip_pdf = true;
     return true;
 }
 
@@ -441,6 +470,7 @@ RT_D bool mat_scatter(const DScene& S, const Rec& h, const Ray& r, Rng& rng, Sca
         case M_PBR: return pbr_scatter(S, m, r, h, rng, sr);
         case M_SPECTRAL_METAL: return smetal_scatter(S, m, r, h, rng, sr);
         case M_ROUGH_DIELECTRIC: return rough_dielectric_scatter(m, r, h, rng, sr);
+        case M_ROUGH_SPECTRAL: return rsd_scatter(S, m, r, h, rng, sr);
         default: return false;   // DiffuseLight
     }
 }
@@ -742,8 +772,9 @@ RT_D void path_init(PathState& ps, const Ray& r) {
 // Shade the hit (or miss) `id` of ps.ray at bounce `depth`.  Adds direct
 // emission / background to `color`, appends up to three shadow requests and
 // advances the path.  Returns false when the path ends.
-RT_D bool bounce(const DScene& S, PathState& ps, const HitId& id, int depth, float bg_scale, bool use_photons, Rng& rng,
-                 V3& color, ShadowReq* sh, int& nsh, Stack& st) {
+template <class DH>
+RT_D bool bounce_t(const DScene& S, PathState& ps, const HitId& id, int depth, float bg_scale, bool use_photons, Rng& rng,
+                   V3& color, ShadowReq* sh, int& nsh, Stack& st, DH& dh) {
     Ray& ray = ps.ray;
     const bool lights_empty = S.num_lights == 0;
     const bool sun = physical_sun(S);
@@ -754,12 +785,12 @@ RT_D bool bounce(const DScene& S, PathState& ps, const HitId& id, int depth, flo
         float ep = env_pdf_value(S, ray.d);
         float ew = (sp + ep == 0.0f) ? 1.0f : (ps.prev_specular ? ps.prev_sun_w : ps.prev_mis);
         V3 bg = (depth == 0 || !env) ? bg_eval(S, ray.d) : env_eval(S, ray.d);
-        color += ps.thr * bg * bg_scale * ew;
+        color += dh.emit(ps.thr * bg * bg_scale * ew);
         return false;
     }
     Rec h = finalize(S, ray, id, 0.001f);
     float ew = (ps.prev_specular || lights_empty) ? 1.0f : ps.prev_mis;
-    color += ps.thr * mat_emitted(S, h) * ew;
+    color += dh.emit(ps.thr * mat_emitted(S, h) * ew);
     Scatter sr;
     if (!mat_scatter(S, h, ray, rng, sr)) return false;
     if (sr.skip_pdf) {
@@ -773,11 +804,12 @@ RT_D bool bounce(const DScene& S, PathState& ps, const HitId& id, int depth, flo
                 float pm = mat_spec_pdf(S, h, ray, ss);
                 V3 sc = bg_eval(S, ss);
                 float md = p_sun * p_sun + pm * pm;
-                if (md > 0.0f) { sh[nsh].o = h.p; sh[nsh].d = ss; sh[nsh].t_max = FINF; sh[nsh].time = ray.time; sh[nsh].contrib = ps.thr * bc * sc * p_sun / md; nsh++; }
+                if (md > 0.0f) { sh[nsh].o = h.p; sh[nsh].d = ss; sh[nsh].t_max = FINF; sh[nsh].time = ray.time; sh[nsh].contrib = ps.thr * bc * sc * p_sun / md; dh.shadow(sh[nsh], nsh, S, h, ray, false); nsh++; }
             }
             float pmb = mat_spec_pdf(S, h, ray, sr.ray.d);
             if (pmb > 0.0f) ps.prev_sun_w = (pmb * pmb) / fmx(pmb * pmb + p_sun * p_sun, 1e-8f);
         }
+        dh.scatter(S, h, ray, sr.ray.d);
         ps.thr *= sr.att;
         ray = sr.ray;
         ps.prev_specular = true;
@@ -785,7 +817,7 @@ RT_D bool bounce(const DScene& S, PathState& ps, const HitId& id, int depth, flo
     } else {
         if (use_photons) {
             V3 irr = photon_irradiance(S, h.p, h.n);
-            if (irr.x > 0.0f || irr.y > 0.0f || irr.z > 0.0f) color += ps.thr * mat_albedo_hint(S, h) * irr;
+            if (irr.x > 0.0f || irr.y > 0.0f || irr.z > 0.0f) color += dh.emit_vertex(ps.thr * mat_albedo_hint(S, h) * irr, S, h, ray);
         }
         if (!lights_empty) {
             V3 ldir = lights_generate(S, h.p, rng, ray.time);
@@ -800,6 +832,7 @@ RT_D bool bounce(const DScene& S, PathState& ps, const HitId& id, int depth, flo
                 if (md > 0.0f && brdf > 0.0f) {
                     sh[nsh].o = h.p; sh[nsh].d = ldir; sh[nsh].t_max = lrec.t * (1.0f - 1e-4f); sh[nsh].time = ray.time;
                     sh[nsh].contrib = ps.thr * sr.att * brdf * l_pdf * mat_emitted(S, lrec) / md;
+                    dh.shadow(sh[nsh], nsh, S, h, ray, true);
                     nsh++;
                 }
             }
@@ -815,6 +848,7 @@ RT_D bool bounce(const DScene& S, PathState& ps, const HitId& id, int depth, flo
                 float md = spdf * spdf + brdf * brdf + ap * ap;
                 sh[nsh].o = h.p; sh[nsh].d = ss; sh[nsh].t_max = FINF; sh[nsh].time = ray.time;
                 sh[nsh].contrib = ps.thr * sr.att * brdf * spdf * sc / md;
+                dh.shadow(sh[nsh], nsh, S, h, ray, true);
                 nsh++;
             }
         }
@@ -829,6 +863,7 @@ RT_D bool bounce(const DScene& S, PathState& ps, const HitId& id, int depth, flo
                     float md = epdf * epdf + brdf * brdf + ap * ap;
                     sh[nsh].o = h.p; sh[nsh].d = ed; sh[nsh].t_max = FINF; sh[nsh].time = ray.time;
                     sh[nsh].contrib = ps.thr * sr.att * brdf * epdf * ec / md;
+                    dh.shadow(sh[nsh], nsh, S, h, ray, true);
                     nsh++;
                 }
             }
@@ -849,6 +884,7 @@ RT_D bool bounce(const DScene& S, PathState& ps, const HitId& id, int depth, flo
         float e = env_pdf_value(S, ind);
         float nee = a * a + s * s + e * e;
         ps.prev_mis = (pdf_val * pdf_val) / fmx(pdf_val * pdf_val + nee, 1e-8f);
+        dh.scatter(S, h, ray, sc.d);
         ps.thr *= sr.att * (scat_pdf / pdf_val);
         ray = sc;
         ps.prev_specular = false;
@@ -860,6 +896,19 @@ RT_D bool bounce(const DScene& S, PathState& ps, const HitId& id, int depth, flo
         ps.thr /= p;
     }
     return true;
+}
+
+// Hooks of bounce_t for the plain renderers: no derivative bookkeeping.
+struct NoDiff {
+    RT_D V3 emit(V3 c) { return c; }
+    RT_D V3 emit_vertex(V3 c, const DScene&, const Rec&, const Ray&) { return c; }
+    RT_D void shadow(ShadowReq&, int, const DScene&, const Rec&, const Ray&, bool) {}
+    RT_D void scatter(const DScene&, const Rec&, const Ray&, V3) {}
+};
+RT_D bool bounce(const DScene& S, PathState& ps, const HitId& id, int depth, float bg_scale, bool use_photons, Rng& rng,
+                 V3& color, ShadowReq* sh, int& nsh, Stack& st) {
+    NoDiff nd;
+    return bounce_t(S, ps, id, depth, bg_scale, use_photons, rng, color, sh, nsh, st, nd);
 }
 
 // Megakernel form of ray_color: trace, shade, resolve shadow rays inline.

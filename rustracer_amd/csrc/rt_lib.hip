@@ -4,6 +4,7 @@
 // the plain per-pixel megakernel, photon tracing and gather, the E_ss table)
 // and the host code that owns contexts, uploads scenes and launches.
 #include "dev_shade.h"
+#include "dev_adjoint.h"
 #include "bvh_build.h"
 #include "../../include/rt_api.h"
 
@@ -95,6 +96,66 @@ __global__ __launch_bounds__(BLOCK) void k_render(DScene S, DCamera cam, DParams
         atomicAdd(P.counters + 5, (unsigned long long)st.s_prims);
         atomicAdd(P.counters + 6, (unsigned long long)st.s_rays);
     }
+}
+
+// Detached-sampling forward render: the k_render frame with BSDF values
+// evaluated on the materials `me` while every decision follows the scene's.
+__global__ __launch_bounds__(BLOCK) void k_detached(DScene S, DCamera cam, DParams P, const DMaterial* __restrict__ me,
+                                                    float* __restrict__ out, uint32_t spp) {
+    Stack st; init_stack(st);
+    uint32_t i = blockIdx.x * BLOCK + threadIdx.x;
+    uint32_t npix = P.width * P.height;
+    if (i >= npix) return;
+    uint32_t row = i / P.width, col = i - row * P.width;
+    V3 sum = mk(0, 0, 0);
+    for (uint32_t k = 0; k < spp; k++) {
+        uint32_t sidx = P.sample_idx + k;
+        Rng rng = rng_seed(pixel_seed(i, sidx, P.seed));
+        Ray r = pixel_ray(cam, P, row, col, sidx, rng);
+        DetachedHook dh;
+        dh.me = me; dh.ratio = mk(1, 1, 1);
+        sum += ray_color_t(S, r, (int)P.max_depth, P.bg_scale, P.use_photons != 0, rng, st, dh);
+    }
+    out[3 * i] += sum.x; out[3 * i + 1] += sum.y; out[3 * i + 2] += sum.z;
+}
+
+// Path-replay adjoint: per sample, the radiance L from its stream, then the
+// same path again distributing L over the vertices' dlog f (see
+// dev_adjoint.h).  Gradients gather in LDS per block, then one global atomic
+// per parameter and block.
+#define RT_MAX_PARAMS 64
+__global__ __launch_bounds__(BLOCK) void k_backward(DScene S, DCamera cam, DParams P, const int8_t* __restrict__ fmap,
+                                                    uint32_t nparams, const float* __restrict__ dimg, float inv_spp,
+                                                    uint32_t spp, float* __restrict__ grad) {
+    __shared__ float g[RT_MAX_PARAMS];
+    for (uint32_t t = threadIdx.x; t < nparams; t += BLOCK) g[t] = 0.0f;
+    __syncthreads();
+    Stack st; init_stack(st);
+    uint32_t i = blockIdx.x * BLOCK + threadIdx.x;
+    uint32_t npix = P.width * P.height;
+    if (i < npix) {
+        uint32_t row = i / P.width, col = i - row * P.width;
+        V3 delta = mk(dimg[3 * i], dimg[3 * i + 1], dimg[3 * i + 2]) * inv_spp;
+        if (delta.x != 0.0f || delta.y != 0.0f || delta.z != 0.0f) {
+            for (uint32_t k = 0; k < spp; k++) {
+                uint32_t sidx = P.sample_idx + k;
+                uint64_t seed = pixel_seed(i, sidx, P.seed);
+                Rng rng = rng_seed(seed);
+                Ray r = pixel_ray(cam, P, row, col, sidx, rng);
+                PlainHook ph;
+                V3 L = ray_color_t(S, r, (int)P.max_depth, P.bg_scale, P.use_photons != 0, rng, st, ph);
+                if (!isfinite(L.x) || !isfinite(L.y) || !isfinite(L.z)) continue;
+                rng = rng_seed(seed);
+                r = pixel_ray(cam, P, row, col, sidx, rng);
+                ReplayHook rh;
+                rh.fmap = fmap; rh.g = g; rh.delta = delta; rh.rem = L;
+                ray_color_t(S, r, (int)P.max_depth, P.bg_scale, P.use_photons != 0, rng, st, rh);
+            }
+        }
+    }
+    __syncthreads();
+    for (uint32_t t = threadIdx.x; t < nparams; t += BLOCK)
+        if (g[t] != 0.0f) atomicAdd(grad + t, g[t]);
 }
 
 // E_ss table (src/material.rs:206-243): one thread per (mu, alpha) texel.
@@ -496,6 +557,7 @@ struct rt_scene {
     std::vector<void*> allocs;
     uint64_t stats[8] = {0};
     std::vector<float4> photons_host;
+    std::vector<DMaterial> mats_host;     // the materials as on the device (rt_scene_set_params edits both)
     float world_min[3] = {0, 0, 0}, world_max[3] = {0, 0, 0};
     ~rt_scene() { for (void* p : allocs) (void)hipFree(p); }
 };
@@ -820,7 +882,7 @@ extern "C" int rt_scene_upload(rt_ctx* ctx, const rt_scene_desc* d, rt_scene** o
     Flat F;
     for (uint32_t i = 0; i < d->num_materials; i++) {
         uint32_t k = d->materials[i].kind;
-        if (k < RT_MAT_LAMBERTIAN || k > RT_MAT_ROUGH_DIELECTRIC) return fail(ctx, RT_ERR_UNSUPPORTED, "unsupported material kind " + std::to_string(k));
+        if (k < RT_MAT_LAMBERTIAN || k > RT_MAT_ROUGH_SPECTRAL) return fail(ctx, RT_ERR_UNSUPPORTED, "unsupported material kind " + std::to_string(k));
     }
     F.objs_input.resize(d->num_objects);
     F.obj_boxes.resize(d->num_objects);
@@ -904,6 +966,7 @@ extern "C" int rt_scene_upload(rt_ctx* ctx, const rt_scene_desc* d, rt_scene** o
         for (int k = 0; k < 24; k++) mats[i].p[k] = d->materials[i].p[k];
     }
     if ((rc = upload_vec(s.get(), mats, &S.mats))) return rc;
+    s->mats_host = mats;
     // background
     const rt_background& b = d->background;
     S.bg_kind = b.kind; S.bg_visible = b.visible_kind;
@@ -1258,6 +1321,147 @@ extern "C" int rt_render_batch(rt_ctx* ctx, const rt_batch_view* views, uint32_t
         HIPCHECK(ctx, hipMemcpy(views[v].accum_rgb, src, bytes, kind));
     }
     return RT_OK;
+}
+
+// ---- differentiable material parameters ----------------------------------------------------
+// Field -> parameter table (24 int8 per material, -1 = constant) after
+// checking that every mapped field is differentiable.
+static int param_map(rt_ctx* ctx, const rt_scene* s, const rt_param* prm, uint32_t n, std::vector<int8_t>& fmap) {
+    if (!prm || n == 0) return fail(ctx, RT_ERR_INVALID, "no parameters");
+    if (n > RT_MAX_PARAMS) return fail(ctx, RT_ERR_INVALID, "at most " + std::to_string(RT_MAX_PARAMS) + " parameters");
+    fmap.assign(24 * s->mats_host.size(), (int8_t)-1);
+    for (uint32_t i = 0; i < n; i++) {
+        const rt_param& q = prm[i];
+        std::string tag = "parameter " + std::to_string(i) + ": ";
+        if (q.material >= s->mats_host.size()) return fail(ctx, RT_ERR_INVALID, tag + "material out of range");
+        if (q.count == 0 || q.field >= 24 || q.count > 24 - q.field) return fail(ctx, RT_ERR_INVALID, tag + "field range outside p[0..24)");
+        const DMaterial& m = s->mats_host[q.material];
+        for (uint32_t f = q.field; f < q.field + q.count; f++) {
+            bool ok = (m.kind == RT_MAT_LAMBERTIAN && m.tex == 0 && f < 3) ||
+                      (m.kind == RT_MAT_ROUGH_SPECTRAL && (f <= 3 || f >= 7));
+            if (!ok) return fail(ctx, RT_ERR_UNSUPPORTED, tag + "field " + std::to_string(f) + " of material kind " +
+                                                         std::to_string(m.kind) + " is not differentiable");
+            int8_t& slot = fmap[24 * (size_t)q.material + f];
+            if (slot >= 0) return fail(ctx, RT_ERR_INVALID, tag + "field " + std::to_string(f) + " already mapped");
+            slot = (int8_t)i;
+        }
+    }
+    return RT_OK;
+}
+
+static int render_args(rt_ctx* ctx, rt_scene* s, const rt_camera* cam, const rt_sample_params* p, uint32_t spp) {
+    if (!cam || !p || spp == 0) return fail(ctx, RT_ERR_INVALID, "missing camera/params or spp = 0");
+    if (p->width == 0 || p->height == 0 || p->max_depth == 0) return fail(ctx, RT_ERR_INVALID, "empty frame");
+    if ((uint64_t)p->width * p->height > 0xffffffffull / 4) return fail(ctx, RT_ERR_INVALID, "image too large");
+    if (p->use_photons && s->S.num_photons == 0) return fail(ctx, RT_ERR_INVALID, "use_photons without a photon map");
+    return RT_OK;
+}
+
+extern "C" int rt_scene_set_params(rt_ctx* ctx, rt_scene* s, const rt_param* prm, uint32_t n, const float* values) {
+    if (!ctx || !s || !values) return RT_ERR_INVALID;
+    std::vector<int8_t> fmap;
+    int rc;
+    if ((rc = param_map(ctx, s, prm, n, fmap))) return rc;
+    for (uint32_t i = 0; i < n; i++)
+        if (!std::isfinite(values[i])) return fail(ctx, RT_ERR_INVALID, "parameter " + std::to_string(i) + " is not finite");
+    HIPCHECK(ctx, hipSetDevice(ctx->device));
+    for (uint32_t i = 0; i < n; i++)
+        for (uint32_t f = prm[i].field; f < prm[i].field + prm[i].count; f++) s->mats_host[prm[i].material].p[f] = values[i];
+    HIPCHECK(ctx, hipStreamSynchronize(ctx->stream));
+    HIPCHECK(ctx, hipMemcpy((void*)s->S.mats, s->mats_host.data(), sizeof(DMaterial) * s->mats_host.size(), hipMemcpyHostToDevice));
+    return RT_OK;
+}
+
+extern "C" int rt_scene_get_params(rt_ctx* ctx, rt_scene* s, const rt_param* prm, uint32_t n, float* values) {
+    if (!ctx || !s || !values) return RT_ERR_INVALID;
+    std::vector<int8_t> fmap;
+    int rc;
+    if ((rc = param_map(ctx, s, prm, n, fmap))) return rc;
+    for (uint32_t i = 0; i < n; i++) values[i] = s->mats_host[prm[i].material].p[prm[i].field];
+    return RT_OK;
+}
+
+extern "C" int rt_render_detached(rt_ctx* ctx, rt_scene* s, const rt_camera* cam, const rt_sample_params* p, uint32_t spp,
+                                  const rt_param* prm, uint32_t n, const float* eval_values, float* out) {
+    if (!ctx || !s || !out || !eval_values) return RT_ERR_INVALID;
+    int rc;
+    if ((rc = render_args(ctx, s, cam, p, spp))) return rc;
+    std::vector<int8_t> fmap;
+    if ((rc = param_map(ctx, s, prm, n, fmap))) return rc;
+    HIPCHECK(ctx, hipSetDevice(ctx->device));
+    std::vector<DMaterial> me = s->mats_host;
+    for (uint32_t i = 0; i < n; i++)
+        for (uint32_t f = prm[i].field; f < prm[i].field + prm[i].count; f++) me[prm[i].material].p[f] = eval_values[i];
+    void* dme = nullptr;
+    HIPCHECK(ctx, hipMalloc(&dme, sizeof(DMaterial) * me.size()));
+    std::unique_ptr<void, void (*)(void*)> guard(dme, [](void* q) { (void)hipFree(q); });
+    HIPCHECK(ctx, hipMemcpy(dme, me.data(), sizeof(DMaterial) * me.size(), hipMemcpyHostToDevice));
+    uint32_t npix = p->width * p->height;
+    Staged rout;
+    if ((rc = stage_out(ctx, out, (size_t)npix * 12, rout, false))) return rc;
+    DCamera C; DParams P;
+    to_dev(cam, p, C, P);
+    P.counters = nullptr;
+    HIPCHECK(ctx, hipMemsetAsync(rout.dev, 0, (size_t)npix * 12, ctx->stream));
+    if (ctx->timing) HIPCHECK(ctx, hipEventRecord(ctx->ev0, ctx->stream));
+    const uint32_t chunk = 4;
+    for (uint32_t k = 0; k < spp; k += chunk) {
+        DParams Q = P;
+        Q.sample_idx = P.sample_idx + k;
+        hipLaunchKernelGGL(k_detached, dim3((npix + BLOCK - 1) / BLOCK), dim3(BLOCK), 0, ctx->stream, s->S, C, Q,
+                           (const DMaterial*)dme, (float*)rout.dev, std::min(chunk, spp - k));
+        HIPCHECK(ctx, hipGetLastError());
+    }
+    if (ctx->timing) {
+        HIPCHECK(ctx, hipEventRecord(ctx->ev1, ctx->stream));
+        HIPCHECK(ctx, hipEventSynchronize(ctx->ev1));
+        float ms = 0.0f;
+        HIPCHECK(ctx, hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+        ctx->kernel_ms += ms;
+        ctx->launches += (spp + chunk - 1) / chunk;
+    }
+    return finish_out(ctx, out, (size_t)npix * 12, rout);
+}
+
+extern "C" int rt_backward(rt_ctx* ctx, rt_scene* s, const rt_camera* cam, const rt_sample_params* p, uint32_t spp,
+                           const rt_param* prm, uint32_t n, const float* dimg, float* grad) {
+    if (!ctx || !s || !dimg || !grad) return RT_ERR_INVALID;
+    int rc;
+    if ((rc = render_args(ctx, s, cam, p, spp))) return rc;
+    std::vector<int8_t> fmap;
+    if ((rc = param_map(ctx, s, prm, n, fmap))) return rc;
+    HIPCHECK(ctx, hipSetDevice(ctx->device));
+    uint32_t npix = p->width * p->height;
+    Staged din, gout;
+    if ((rc = stage_in(ctx, dimg, (size_t)npix * 12, din))) return rc;
+    if ((rc = stage_out(ctx, grad, (size_t)n * 4, gout, false))) return rc;
+    void* dmap = nullptr;
+    HIPCHECK(ctx, hipMalloc(&dmap, fmap.size()));
+    std::unique_ptr<void, void (*)(void*)> guard(dmap, [](void* q) { (void)hipFree(q); });
+    HIPCHECK(ctx, hipMemcpy(dmap, fmap.data(), fmap.size(), hipMemcpyHostToDevice));
+    HIPCHECK(ctx, hipMemsetAsync(gout.dev, 0, (size_t)n * 4, ctx->stream));
+    DCamera C; DParams P;
+    to_dev(cam, p, C, P);
+    P.counters = nullptr;
+    if (ctx->timing) HIPCHECK(ctx, hipEventRecord(ctx->ev0, ctx->stream));
+    const uint32_t chunk = 2;
+    for (uint32_t k = 0; k < spp; k += chunk) {
+        DParams Q = P;
+        Q.sample_idx = P.sample_idx + k;
+        hipLaunchKernelGGL(k_backward, dim3((npix + BLOCK - 1) / BLOCK), dim3(BLOCK), 0, ctx->stream, s->S, C, Q,
+                           (const int8_t*)dmap, n, (const float*)din.dev, 1.0f / (float)spp, std::min(chunk, spp - k),
+                           (float*)gout.dev);
+        HIPCHECK(ctx, hipGetLastError());
+    }
+    if (ctx->timing) {
+        HIPCHECK(ctx, hipEventRecord(ctx->ev1, ctx->stream));
+        HIPCHECK(ctx, hipEventSynchronize(ctx->ev1));
+        float ms = 0.0f;
+        HIPCHECK(ctx, hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+        ctx->kernel_ms += ms;
+        ctx->launches += (spp + chunk - 1) / chunk;
+    }
+    return finish_out(ctx, grad, (size_t)n * 4, gout);
 }
 
 extern "C" {

@@ -176,6 +176,56 @@ class Renderer:
         self._check(self.L.rt_render_batch(self.ctx, arr, len(views), C.byref(o)), "rt_render_batch")
         return [v["out"] for v in views]
 
+    # ---------------------------------------------------------------- differentiable parameters
+    @staticmethod
+    def _params(params):
+        """params: list of (material index, field, count) -> rt_param array."""
+        arr = (F.rt_param * len(params))()
+        for i, (m, f, c) in enumerate(params):
+            arr[i].material, arr[i].field, arr[i].count = int(m), int(f), int(c)
+        return arr
+
+    def set_params(self, params, values, scene=None):
+        vals = np.ascontiguousarray(values, dtype=np.float32)
+        assert len(vals) == len(params)
+        self._check(self.L.rt_scene_set_params(self.ctx, self.scene if scene is None else scene, self._params(params),
+                                               len(params), _ptr(vals)), "rt_scene_set_params")
+
+    def get_params(self, params, scene=None):
+        out = np.zeros(len(params), np.float32)
+        self._check(self.L.rt_scene_get_params(self.ctx, self.scene if scene is None else scene, self._params(params),
+                                               len(params), _ptr(out)), "rt_scene_get_params")
+        return out
+
+    def render_detached(self, out, camera, width, height, spp, params, eval_values, sample_idx=0, strata=0,
+                        bg_scale=1.0, photon_map=False, max_depth=50, seed=0, scene=None):
+        """The frame of render_accumulate with decisions at the scene's current
+        parameters and BSDF values at `eval_values` (finite differences with
+        common random numbers)."""
+        p = self.params(width, height, sample_idx, strata, bg_scale, max_depth, None, photon_map, seed)
+        cam = camera.to_c()
+        vals = np.ascontiguousarray(eval_values, dtype=np.float32)
+        self._check(self.L.rt_render_detached(self.ctx, self.scene if scene is None else scene, C.byref(cam), C.byref(p),
+                                              spp, self._params(params), len(params), _ptr(vals), _ptr(out)),
+                    "rt_render_detached")
+        return out
+
+    def backward(self, dloss_dimage, camera, width, height, spp, params, sample_idx=0, strata=0, bg_scale=1.0,
+                 photon_map=False, max_depth=50, seed=0, scene=None, grad=None):
+        """Path-replay adjoint: dLoss/dtheta for dloss_dimage = dLoss/d(mean
+        image) (H x W x 3, host array or device tensor); returns grad (float32
+        numpy array, or fills the given device tensor)."""
+        p = self.params(width, height, sample_idx, strata, bg_scale, max_depth, None, photon_map, seed)
+        cam = camera.to_c()
+        if hasattr(dloss_dimage, "data_ptr"):
+            d = dloss_dimage
+        else:
+            d = np.ascontiguousarray(dloss_dimage, dtype=np.float32)
+        g = np.zeros(len(params), np.float32) if grad is None else grad
+        self._check(self.L.rt_backward(self.ctx, self.scene if scene is None else scene, C.byref(cam), C.byref(p), spp,
+                                       self._params(params), len(params), _ptr(d), _ptr(g)), "rt_backward")
+        return g
+
     # ---------------------------------------------------------------- photons
     def photon_build_from_quad(self, origin, u, v, color, num_photons, gather_radius, scene=None):
         """PhotonMap::build_from_quad (src/photon.rs:97-135)."""

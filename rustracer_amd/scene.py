@@ -168,6 +168,27 @@ class RoughDielectric(Material):
         return [self.ior, self.roughness]
 
 
+class RoughSpectralDielectric(Material):
+    """Rough dispersive dielectric (extension): n(l) = n_d + s(l) - s(589.3 nm)
+    with s the Sellmeier law (B, C in um^2); GGX alpha `alpha_table` on
+    primitive 0 (a Diamond's table facet) and `alpha_groups[(k-1) % 16]` on
+    primitive k >= 1; the spectral dielectric's fire tint on the first
+    refraction.  Differentiable in n_d, B and every alpha."""
+    kind = F.MAT_ROUGH_SPECTRAL
+    FIELD_ND, FIELD_B1, FIELD_ALPHA_TABLE, FIELD_ALPHA_GROUP0 = 0, 1, 7, 8
+
+    def __init__(self, nd=2.417, sellmeier=(0.3306, 4.3356, 0.0, 0.175 ** 2, 0.106 ** 2, 0.0), alpha_table=0.02,
+                 alpha_groups=None):
+        self.nd = float(nd)
+        self.sellmeier = [float(x) for x in sellmeier]
+        self.alpha_table = float(alpha_table)
+        self.alpha_groups = [float(x) for x in (alpha_groups if alpha_groups is not None else [alpha_table] * 16)]
+        assert len(self.alpha_groups) == 16
+
+    def params(self):
+        return [self.nd] + self.sellmeier + [self.alpha_table] + self.alpha_groups
+
+
 # ------------------------------------------------------------------- shapes
 class Hittable:
     shape = 0
@@ -598,7 +619,14 @@ class Scene:
         d.background = self.background.to_c()
         keep["bg_env"] = self.background.env
         self._desc = (d, keep)
+        self._mat_index = dict(mat_index)
         return d
+
+    def material_index(self, mat):
+        """Index of `mat` in the descriptor's material table (for rt_param)."""
+        if not hasattr(self, "_mat_index"):
+            self.descriptor()
+        return self._mat_index[id(mat)]
 
 
 def _face_normals(v):

@@ -215,6 +215,47 @@ def gem_caustics(width=1280, height=720, num_photons=2_000_000, dispersion="cauc
                             num_photons=num_photons, gather_radius=0.05)
 
 
+ADJOINT_PARAM_NAMES = (["n_d", "sellmeier_B1", "alpha_table", "floor_albedo"] +
+                       [f"alpha_group{k}" for k in range(16)])
+
+
+def gem_adjoint(width=512, height=512, seed=4, rel_sigma=0.05):
+    """Adjoint material gradients on the gem scene.
+
+    The gem of gem_caustics() with a rough dispersive dielectric
+    (RoughSpectralDielectric: n_d 2.417, Peter's diamond Sellmeier law with
+    B1 0.3306, table alpha 0.02, 16 facet-group alphas drawn uniform in
+    [0.005, 0.05] with `seed`) on the diffuse 0.8 floor, same light and camera,
+    64 spp, depth 24, no photon map (caustics by path tracing through the rough
+    facets, so every term of the image is differentiated).  20 parameters:
+    n_d, B1, table alpha, grey floor albedo, 16 group alphas.  Returns the scene
+    at the target parameters plus cfg["params"] (rt_param triples),
+    cfg["theta_target"] and cfg["theta_start"] (the target perturbed by
+    Gaussian noise of `rel_sigma` relative, seeded).
+    """
+    from .scene import Camera, Diamond, DiffuseLight, Lambertian, Quad, RoughSpectralDielectric, Scene, Background
+    g = np.random.default_rng(seed)
+    groups = g.uniform(0.005, 0.05, size=16).astype(np.float32)
+    gem_mat = RoughSpectralDielectric(nd=2.417, alpha_table=0.02, alpha_groups=list(groups))
+    floor_mat = Lambertian((0.8, 0.8, 0.8))
+    gem = Diamond((0, 0, 0), 0.5, gem_mat, table_pct=57.0, crown_angle=34.5, pavilion_angle=40.75,
+                  girdle_facets=56, girdle_pct=0.0).rotated((0, 12, 0)).translated((0, 0.55, 0))
+    light = DiffuseLight((50.0, 50.0, 50.0))
+    lq = dict(q=(-0.15, 2.2, -0.15), u=(0.3, 0, 0), v=(0, 0, 0.3))
+    objs = [Quad((-4, 0, -4), (8, 0, 0), (0, 0, 8), floor_mat), gem, Quad(lq["q"], lq["u"], lq["v"], light)]
+    lights = [Quad(lq["q"], lq["u"], lq["v"], light)]
+    scene = Scene(objs, lights, Background.solid((0, 0, 0)), name="gem_adjoint")
+    cam = Camera((0.0, 1.6, -2.6), (0.0, 0.35, 0.0), (0, 1, 0), 35.0, width / height)
+    mg, mf = scene.material_index(gem_mat), scene.material_index(floor_mat)
+    R = RoughSpectralDielectric
+    params = [(mg, R.FIELD_ND, 1), (mg, R.FIELD_B1, 1), (mg, R.FIELD_ALPHA_TABLE, 1), (mf, 0, 3)]
+    params += [(mg, R.FIELD_ALPHA_GROUP0 + k, 1) for k in range(16)]
+    target = np.array([2.417, 0.3306, 0.02, 0.8] + list(groups), dtype=np.float32)
+    start = (target * (1.0 + rel_sigma * g.standard_normal(target.shape))).astype(np.float32)
+    return scene, cam, dict(width=width, height=height, spp=64, max_depth=24, params=params, theta_target=target,
+                            theta_start=start, names=list(ADJOINT_PARAM_NAMES))
+
+
 def perlin_grid(n=128, octaves=4, frequency=3.0, seed=2):
     """n^3 float32 density grid in [0,1]: `octaves`-octave Perlin turbulence
     over the unit cube, normalised by its maximum (x fastest, then y, z)."""

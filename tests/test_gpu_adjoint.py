@@ -1,0 +1,105 @@
+"""Path-replay adjoint on the device (rt_backward, rt_render_detached).
+
+* the rough spectral gem renders like the checker (same streams, 3 sigma);
+* the detached renderer at theta = theta_s is the plain frame;
+* the adjoint equals the checker's host gradient (forward score accumulation,
+  numeric dlog f) on the same per-pixel streams;
+* the adjoint equals central finite differences of the detached renderer
+  with common random numbers (h = 1e-3 relative) within 2 %.
+"""
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from rustracer_amd import Renderer, scenes
+
+pytestmark = pytest.mark.gpu
+
+
+def setup(r, W, H):
+    sc, cam, cfg = scenes.gem_adjoint(W, H)
+    r.upload(sc)
+    return sc, cam, cfg
+
+
+def test_rough_spectral_gem_matches_checker(gpu_renderer):
+    W, H = 32, 24
+    sc, cam, cfg = setup(gpu_renderer, W, H)
+    orc = O.OracleScene(sc)
+    g = np.zeros((4, H, W, 3), np.float32)
+    c = np.zeros((4, H, W, 3), np.float32)
+    for k in range(4):
+        gpu_renderer.render_tiles(g[k], None, k, 0, W, H, cam, max_depth=24, seed=1)
+        c[k] = orc.render_sample(cam, Renderer.params(W, H, sample_idx=k, max_depth=24, seed=1), mode=1)
+    close = np.isclose(g, c, rtol=1e-3, atol=1e-4).all(axis=-1)
+    assert close.mean() > 0.95, close.mean()
+    K = 128
+    g = np.zeros((K, H, W, 3), np.float32)
+    c = np.zeros((K, H, W, 3), np.float32)
+    for k in range(K):
+        gpu_renderer.render_tiles(g[k], None, k, 0, W, H, cam, max_depth=24, seed=2)
+        c[k] = orc.render_sample(cam, Renderer.params(W, H, sample_idx=k, max_depth=24, seed=2), mode=0)
+    ma, mb = g.mean(0), c.mean(0)
+    se = np.sqrt(g.var(0, ddof=1) / K + c.var(0, ddof=1) / K)
+    z = np.where(se > 0, np.abs(ma - mb) / np.maximum(se, 1e-12), np.where(np.abs(ma - mb) < 1e-6, 0, np.inf))
+    assert (z > 3).mean() <= 0.02 and (z > 6).mean() <= 0.002, ((z > 3).mean(), (z > 6).mean())
+
+
+def test_detached_at_current_parameters_is_the_plain_frame(gpu_renderer):
+    W, H = 40, 32
+    sc, cam, cfg = setup(gpu_renderer, W, H)
+    a = np.zeros((H, W, 3), np.float32)
+    b = np.zeros((H, W, 3), np.float32)
+    gpu_renderer.render_accumulate(a, cam, W, H, 8, max_depth=24, seed=3)
+    gpu_renderer.render_detached(b, cam, W, H, 8, cfg["params"], cfg["theta_target"], max_depth=24, seed=3)
+    assert np.allclose(a, b, rtol=1e-5, atol=1e-6), np.abs(a - b).max()
+    # parameters round-trip through the scene
+    got = gpu_renderer.get_params(cfg["params"])
+    assert np.array_equal(got, cfg["theta_target"])
+    gpu_renderer.set_params(cfg["params"], cfg["theta_start"])
+    assert np.array_equal(gpu_renderer.get_params(cfg["params"]), cfg["theta_start"])
+
+
+def test_adjoint_matches_host_replay(gpu_renderer):
+    W, H, spp = 24, 24, 8
+    sc, cam, cfg = setup(gpu_renderer, W, H)
+    P = cfg["params"]
+    w = np.random.default_rng(5).standard_normal((H, W, 3)).astype(np.float32)
+    g = gpu_renderer.backward(w, cam, W, H, spp, P, max_depth=24, seed=4)
+    orc = O.OracleScene(sc)
+    h = orc.backward(cam, Renderer.params(W, H, max_depth=24, seed=4), spp, P, w, mode=1)
+    scale = np.abs(h).max()
+    bad = np.abs(g - h) > 2e-2 * np.abs(h) + 2e-3 * scale
+    assert not bad.any(), np.c_[np.arange(len(P)), g, h][bad]
+
+
+def test_adjoint_matches_finite_differences_with_common_random_numbers(gpu_renderer):
+    W, H, spp = 64, 64, 16
+    sc, cam, cfg = setup(gpu_renderer, W, H)
+    P = cfg["params"]
+    theta = cfg["theta_start"]
+    gpu_renderer.set_params(P, theta)
+    target = np.zeros((H, W, 3), np.float32)
+    gpu_renderer.render_detached(target, cam, W, H, spp, P, cfg["theta_target"], max_depth=24, seed=99)
+    target /= spp
+    img = np.zeros((H, W, 3), np.float32)
+    gpu_renderer.render_accumulate(img, cam, W, H, spp, max_depth=24, seed=7)
+    img /= spp
+    dl = (2.0 * (img - target) / img.size).astype(np.float32)      # L = mean squared error
+    g = gpu_renderer.backward(dl, cam, W, H, spp, P, max_depth=24, seed=7)
+
+    def loss(th):
+        out = np.zeros((H, W, 3), np.float32)
+        gpu_renderer.render_detached(out, cam, W, H, spp, P, th, max_depth=24, seed=7)
+        return float((((out / spp).astype(np.float64) - target) ** 2).mean())
+    fd = np.zeros(len(P))
+    for j in range(len(P)):
+        h = 1e-3 * abs(float(theta[j]))
+        tp, tm = theta.astype(np.float64).copy(), theta.astype(np.float64).copy()
+        tp[j] += h
+        tm[j] -= h
+        fd[j] = (loss(tp) - loss(tm)) / (float(np.float32(tp[j])) - float(np.float32(tm[j])))
+    scale = np.abs(fd).max()
+    bad = np.abs(g - fd) > 2e-2 * np.abs(fd) + 1e-3 * scale
+    assert not bad.any(), np.c_[np.arange(len(P)), g, fd][bad]
+    assert np.count_nonzero(np.abs(fd) > 1e-2 * scale) >= 10

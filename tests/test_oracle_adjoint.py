@@ -1,0 +1,102 @@
+"""Pins of the adjoint checker in oracle/ (CPU only).
+
+* log f of the rough spectral dielectric against closed forms at normal
+  incidence, and against its own sampler: the integral of f |cos| over the
+  sphere (quadrature of the double-precision formula) equals the sampler's
+  mean weight G1(wi) (Monte Carlo over the float sampler);
+* the checker's gradient (forward score accumulation, numeric dlog f) equals
+  central finite differences of its detached renderer with common random
+  numbers (step 1e-3 relative, as the adjoint's own finite-difference check).
+"""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from rustracer_amd import Renderer, scenes
+from rustracer_amd import _ffi as F
+
+
+def fresnel_normal(k):
+    return ((1.0 - k) / (1.0 + k)) ** 2
+
+
+@pytest.mark.parametrize("k,alpha", [(1.5, 0.1), (2.417, 0.02), (1 / 2.417, 0.3)])
+def test_rsd_logf_normal_incidence_closed_forms(k, alpha):
+    n = [0.0, 0.0, 1.0]
+    D0 = 1.0 / (np.pi * alpha * alpha)     # GGX at the pole
+    lf = O.rsd_logf(n, n, n, k, alpha, True)
+    assert lf == pytest.approx(np.log(fresnel_normal(k) * D0 / 4.0), rel=1e-9)
+    lt = O.rsd_logf(n, [0.0, 0.0, -1.0], n, k, alpha, False)
+    # h = n, i.h = 1, o.h = -1: f = k^2 (1 - F) D / (1 - k)^2
+    assert lt == pytest.approx(np.log(k * k * (1.0 - fresnel_normal(k)) * D0 / (1.0 - k) ** 2), rel=1e-9)
+
+
+def sphere_quadrature(fn, nt=900, nph=720):
+    t = (np.arange(nt) + 0.5) * np.pi / nt
+    p = (np.arange(nph) + 0.5) * 2 * np.pi / nph
+    tot = 0.0
+    for ti in t:
+        st, ct = np.sin(ti), np.cos(ti)
+        for pj in p[::1]:
+            o = np.array([st * np.cos(pj), st * np.sin(pj), ct])
+            tot += fn(o) * st
+    return tot * (np.pi / nt) * (2 * np.pi / nph)
+
+
+@pytest.mark.parametrize("front,theta_deg", [(True, 40.0), (False, 25.0)])
+def test_rsd_logf_integrates_to_sampler_mean_weight(front, theta_deg):
+    nd, alpha = 1.8, 0.35
+    m = F.rt_material()
+    m.kind = F.MAT_ROUGH_SPECTRAL
+    vals = [nd, 0.0, 0.0, 0.0, 0.01, 0.01, 0.01, alpha] + [alpha] * 16
+    for i, v in enumerate(vals):
+        m.p[i] = v
+    th = np.radians(theta_deg)
+    wo = np.array([np.sin(th), 0.0, np.cos(th)])           # toward the viewer, normal +z
+    ray = np.array([-wo[0], -wo[1], -wo[2], 0, 0, 0, 550.0], np.float32)
+    ray[:3] = 0.0
+    ray[3:6] = -wo
+    rec = np.array([0, 0, 0, 0, 0, 1, 1.0], np.float32)
+    n = 400_000
+    out = np.zeros((n, 8), np.float32)
+    O.lib().rtc_scatter(C.byref(m), O._p(ray), 1, O._p(rec), 1 if front else 0, 12345, n, O._p(out))
+    mc = out[:, 0].astype(np.float64).mean()
+    se = out[:, 0].astype(np.float64).std() / np.sqrt(n)
+    k = nd if front else 1.0 / nd
+    nrm = np.array([0.0, 0.0, 1.0])
+
+    def f_cos(o):
+        lf = O.rsd_logf(wo, o, nrm, k, alpha, o[2] > 0)
+        return 0.0 if lf is None else np.exp(lf) * abs(o[2])
+    quad = sphere_quadrature(f_cos, nt=360, nph=180)
+    assert abs(quad - mc) < 4 * se + 5e-3 * mc, (quad, mc, se)
+
+
+def test_checker_gradient_matches_detached_finite_differences():
+    W = H = 16
+    sc, cam, cfg = scenes.gem_adjoint(W, H)
+    orc = O.OracleScene(sc)
+    P = cfg["params"]
+    sp = Renderer.params(W, H, max_depth=24, seed=1)
+    spp = 8
+    img0 = orc.render_accumulate(cam, sp, spp, mode=1)
+    det = orc.render_detached(cam, sp, spp, P, cfg["theta_target"], mode=1)
+    assert np.array_equal(img0, det)          # ratio 1 everywhere at theta = theta_s
+    w = np.random.default_rng(0).standard_normal((H, W, 3)).astype(np.float32)
+    g = orc.backward(cam, sp, spp, P, w, mode=1)
+    th = cfg["theta_target"].astype(np.float64)
+    fd = np.zeros(len(P))
+    for j in range(len(P)):
+        h = 1e-3 * abs(th[j])
+        tp, tm = th.copy(), th.copy()
+        tp[j] += h
+        tm[j] -= h
+        Ip = orc.render_detached(cam, sp, spp, P, tp, mode=1).astype(np.float64) / spp
+        Im = orc.render_detached(cam, sp, spp, P, tm, mode=1).astype(np.float64) / spp
+        fd[j] = ((Ip - Im) * w).sum() / (float(np.float32(tp[j])) - float(np.float32(tm[j])))
+    scale = np.abs(fd).max()
+    bad = np.abs(g - fd) > 2e-2 * np.abs(fd) + 1e-3 * scale
+    assert not bad.any(), np.c_[np.arange(len(P)), g, fd][bad]
+    assert np.count_nonzero(np.abs(fd) > 1e-3 * scale) >= 15
