@@ -9,9 +9,10 @@ the plain megakernel is timed on the same frames beside it.
 Default workload: the QBVH-bound mesh scene (a 1,310,720-triangle displaced
 icosphere in rough GGX glass under a 2048x1024 HDR environment, 1280x720,
 64 spp, depth 12).  The gem-with-caustics scene (1280x720, 256 spp, depth 24,
-2M photons) and the uneven batch (36 gem spin stills of mixed resolution plus
-a heterogeneous-volume glass box, one launch) are measured after it and
-reported under "also".
+2M photons), the uneven batch (36 gem spin stills of mixed resolution plus
+a heterogeneous-volume glass box, one launch) and the adjoint (gradient
+evaluations/s of a 20-parameter L2 image loss on a 512x512 rough gem, with a
+finite-difference check) are measured after it and reported under "also".
 
 Beside the throughput, on the same line:
   roofline     — algorithmic HBM bytes of the closest-hit traversal kernel
@@ -301,6 +302,136 @@ def measure_batch(args, with_cpu):
             "cpu_baseline": cpu}
 
 
+def measure_adjoint(args, with_cpu):
+    """Adjoint material gradients on the 512x512 gem: one gradient evaluation
+    = the primal frame at 64 spp (wavefront renderer) + dLoss/dI of the L2
+    image loss against a target rendered at the known parameters + the
+    path-replay backward over the 20 parameters (rt_backward).  Untimed
+    beside it: central finite differences with common random numbers
+    (h = 1e-3 relative) of the same loss through the detached renderer."""
+    import numpy as np
+    import torch
+    from rustracer_amd import Renderer, scenes
+
+    sc, cam, cfg = scenes.gem_adjoint(512, 512)
+    W, H, spp, depth, P = cfg["width"], cfg["height"], cfg["spp"], cfg["max_depth"], cfg["params"]
+    r = Renderer(0).upload(sc)
+    opts = dict(capacity=1 << 23, grid=8192, sort=False, use_graph=True)
+    target = torch.zeros((H, W, 3), dtype=torch.float32, device="cuda")
+    r.render_wavefront(target, cam, W, H, spp, max_depth=depth, seed=99, **opts)
+    target /= spp
+    theta = cfg["theta_start"]
+    r.set_params(P, theta)
+    img = torch.zeros_like(target)
+    grad = torch.zeros(len(P), dtype=torch.float32, device="cuda")
+    n = float(img.numel())
+
+    def step(k):
+        r.render_wavefront(img, cam, W, H, spp, max_depth=depth, seed=k, **opts)
+        dl = (2.0 / n) * (img / spp - target)
+        r.backward(dl, cam, W, H, spp, P, max_depth=depth, seed=k, grad=grad)
+
+    for k in range(args.warmup):
+        step(k)
+    torch.cuda.synchronize()
+    r.set_timing(True)
+    t0 = time.perf_counter()
+    for k in range(args.steps):
+        step(1000 + k)
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t0
+    kms, nl = r.timing()
+    r.set_timing(False)
+    rate = args.steps / el
+    g = grad.cpu().numpy().astype(np.float64)
+    assert np.isfinite(g).all(), "non-finite gradient"
+    log(f"[adjoint] {rate:.2f} gradient evaluations/s ({el * 1e3 / args.steps:.1f} ms each)")
+    # backward alone, HIP events on the library stream
+    img.zero_()
+    r.render_wavefront(img, cam, W, H, spp, max_depth=depth, seed=7, **opts)
+    dl = (2.0 / n) * (img / spp - target)
+    torch.cuda.synchronize()
+    r.set_timing(True)
+    g7 = r.backward(dl, cam, W, H, spp, P, max_depth=depth, seed=7).astype(np.float64)
+    bms, bn = r.timing()
+    r.set_timing(False)
+    # finite differences of the same loss, same streams (seed 7)
+    tgt = target.double()
+
+    def loss(th):
+        out = torch.zeros_like(img)
+        r.render_detached(out, cam, W, H, spp, P, th, max_depth=depth, seed=7)
+        return float(((out.double() / spp - tgt) ** 2).mean())
+
+    fd = np.zeros(len(P))
+    t0 = time.time()
+    for j in range(len(P)):
+        h = 1e-3 * abs(float(theta[j]))
+        tp, tm = theta.astype(np.float64).copy(), theta.astype(np.float64).copy()
+        tp[j] += h
+        tm[j] -= h
+        fd[j] = (loss(tp) - loss(tm)) / (float(np.float32(tp[j])) - float(np.float32(tm[j])))
+    fd_s = time.time() - t0
+    scale = np.abs(fd).max()
+    rel = np.abs(g7 - fd) / np.maximum(np.abs(fd), 1e-30)
+    within = np.abs(g7 - fd) <= 2e-2 * np.abs(fd) + 1e-3 * scale
+    log(f"[adjoint] finite differences: {int(within.sum())}/{len(P)} within 2% (+1e-3 of the largest), "
+        f"median relative error {float(np.median(rel)):.2e} ({fd_s:.1f}s)")
+    # traversal work of the backward: two traces of every path (primal + replay)
+    r.set_counting(True)
+    r.render_accumulate(img, cam, W, H, 4, max_depth=depth, seed=7)
+    torch.cuda.synchronize()
+    cnt = r.counters()
+    r.set_counting(False)
+    pb = PRIM_BYTES["gem"]
+    bps = ((cnt["node_tests"] + cnt["shadow_node_tests"]) * 256.0 + (cnt["prim_tests"] + cnt["shadow_prim_tests"]) * pb) \
+        / max(cnt["samples"], 1)
+    bpl = 2.0 * bps * W * H * spp / max(bn, 1)
+    achieved = bpl / (bms / 1e3 / max(bn, 1)) / 1e9
+    cpu = None
+    if with_cpu:
+        cpu = cpu_adjoint_baseline(sc, cam, cfg, args.cpu_seconds)
+    r.close()
+    return {"value": rate, "unit": "gradient evaluations/s", "ms_per_step": el * 1e3 / args.steps,
+            "config": {"workload": "gem_adjoint_512x512x64_20_params", "width": W, "height": H, "spp": spp,
+                       "max_depth": depth, "params": len(P), "forward": "wavefront", "backward": "path replay",
+                       "backward_ms": bms, "fd_h_rel": 1e-3,
+                       "fd_within_2pct": int(within.sum()), "fd_rel_err": [float(x) for x in rel],
+                       "gradient": [float(x) for x in g7], "finite_difference": [float(x) for x in fd]},
+            "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                         "frac": achieved / HBM_PEAK_GBS, "kernel": "k_backward", "avg_launch_ms": bms / max(bn, 1),
+                         "bytes_per_launch": bpl},
+            "cpu_baseline": cpu}
+
+
+def cpu_adjoint_baseline(scene, cam, cfg, seconds):
+    """The oracle's host gradient (forward score accumulation) plus its primal
+    frame, all host cores, on passes of the 512x512 frame at 1 sample per
+    pixel until `seconds` are used; scaled to gradient evaluations of 64 spp."""
+    import numpy as np
+    from oracle import oracle as O
+    from rustracer_amd import Renderer
+    cores = O.threads()
+    orc = O.OracleScene(scene)
+    P = cfg["params"]
+    orc.set_params(P, cfg["theta_start"])
+    W, H, spp = cfg["width"], cfg["height"], cfg["spp"]
+    dl = np.full((H, W, 3), 1.0 / (W * H * 3), np.float32)
+    done = 0
+    t0 = time.perf_counter()
+    while True:
+        sp = Renderer.params(W, H, sample_idx=done, max_depth=cfg["max_depth"], seed=7)
+        orc.render_sample(cam, sp, mode=1, nthreads=cores)
+        orc.backward(cam, sp, 1, P, dl, mode=1, nthreads=cores)
+        done += 1
+        if time.perf_counter() - t0 >= seconds or done >= 64:
+            break
+    el = time.perf_counter() - t0
+    return {"value": done / el / spp, "unit": "gradient evaluations/s", "cores": cores, "kind": "port",
+            "sample": f"{done} passes of {W}x{H} at 1 sample/pixel (primal + forward-score gradient) of the {spp} "
+                      f"per evaluation, {el:.1f}s"}
+
+
 def measured_traffic(workload, opts):
     """HBM bytes per k_wf_extend dispatch from the committed rocprofv3 PMC
     summary (FETCH_SIZE + WRITE_SIZE, copy-calibrated) when it was measured
@@ -355,9 +486,10 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--config", default="mesh", choices=sorted(workloads()) + ["batch"])
-    ap.add_argument("--also", default="gem,batch",
-                    help="comma-separated extra workloads ('' for none): " + ", ".join(sorted(workloads())) + ", batch")
+    ap.add_argument("--config", default="mesh", choices=sorted(workloads()) + ["batch", "adjoint"])
+    ap.add_argument("--also", default="gem,batch,adjoint",
+                    help="comma-separated extra workloads ('' for none): " + ", ".join(sorted(workloads())) +
+                         ", batch, adjoint")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-tune", action="store_true")
@@ -381,18 +513,25 @@ def main():
         log(f"calibration copy: {args.calib_copy_mb} MiB read + {args.calib_copy_mb} MiB written")
     if args.config == "batch":
         main_res = measure_batch(args, not args.no_cpu)
+    elif args.config == "adjoint":
+        main_res = measure_adjoint(args, not args.no_cpu)
     else:
         main_res = measure(args.config, args, with_cpu=not args.no_cpu)
     also = {}
     for name in [a for a in args.also.split(",") if a and a != args.config]:
-        m = measure_batch(args, not args.no_cpu) if name == "batch" else measure(name, args, with_cpu=not args.no_cpu)
-        also[m["config"]["workload"]] = {"value": m["value"], "unit": "samples/s", "config": m["config"],
+        if name == "batch":
+            m = measure_batch(args, not args.no_cpu)
+        elif name == "adjoint":
+            m = measure_adjoint(args, not args.no_cpu)
+        else:
+            m = measure(name, args, with_cpu=not args.no_cpu)
+        also[m["config"]["workload"]] = {"value": m["value"], "unit": m.get("unit", "samples/s"), "config": m["config"],
                                           "roofline": {k: m["roofline"][k] for k in ("achieved", "frac", "kernel", "avg_launch_ms")},
                                           "cpu_baseline": m["cpu_baseline"]}
     line = {
-        "metric": "GPU samples/s (wavefront vs plain) at fixed RMSE",
+        "metric": "GPU samples/s (wavefront vs plain) at fixed RMSE; adjoint grad evals/s",
         "value": main_res["value"],
-        "unit": "samples/s",
+        "unit": main_res.get("unit", "samples/s"),
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
