@@ -444,7 +444,9 @@ def measured_traffic(workload, opts):
         return None, "no PMC summary"
     if not rec:
         return None, "no PMC summary for this workload"
-    if any(opts.get(k) != v for k, v in rec["wavefront_opts"].items()):
+    # graph replay changes launch overhead, not what a dispatch reads or writes
+    keys = [k for k in rec["wavefront_opts"] if k != "use_graph"]
+    if any(opts.get(k) != rec["wavefront_opts"][k] for k in keys):
         return None, f"PMC summary measured with {rec['wavefront_opts']}, not {opts}"
     return rec["traffic_bytes_per_dispatch"], rec["source"]
 

@@ -178,7 +178,13 @@ typedef struct {
     uint32_t     env_w, env_h;  /* ENVMAP*: equirect float RGB, row-major, row 0 = +Y pole */
     const float *env_rgb;
     float        env_rot_y;     /* radians                                   */
+    uint32_t     env_sampling;  /* ENVMAP*: RT_ENV_CDF (the reference's row/column CDF
+                                   searches, src/renderer.rs:126-154) or RT_ENV_ALIAS
+                                   (Walker/Vose alias table over texels: same density,
+                                   same four draws, O(1) instead of two binary searches) */
 } rt_background;
+
+enum { RT_ENV_CDF = 0, RT_ENV_ALIAS = 1 };
 
 typedef struct {
     const rt_object   *objects;    uint32_t num_objects;   /* world, input order */

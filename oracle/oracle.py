@@ -53,6 +53,8 @@ def lib():
         L.rtc_render_detached.argtypes = [vp, vp, vp, C.c_uint32, vp, C.c_uint32, vp, vp, C.c_int, C.c_int]
         L.rtc_backward.argtypes = [vp, vp, vp, C.c_uint32, vp, C.c_uint32, vp, vp, C.c_int, C.c_int]
         L.rtc_rsd_logf.argtypes = [vp, vp, vp, C.c_double, C.c_double, C.c_int, C.POINTER(C.c_double)]
+        L.rtc_env_alias.argtypes = [vp, C.c_uint32, C.c_uint32, vp, vp]
+        L.rtc_env_sample_mode.argtypes = [vp, C.c_uint32, C.c_uint32, C.c_float, C.c_int, C.c_uint64, C.c_uint32, vp]
         _lib = L
     return _lib
 
@@ -256,3 +258,23 @@ def rsd_logf(i, o, n, k, alpha, refl):
     out = C.c_double()
     ok = L.rtc_rsd_logf(_p(a[0]), _p(a[1]), _p(a[2]), float(k), float(alpha), 1 if refl else 0, C.byref(out))
     return out.value if ok else None
+
+
+def env_alias(rgb):
+    """(acceptance probability, alias) per texel of an H x W x 3 image."""
+    rgb = np.ascontiguousarray(rgb, dtype=np.float32)
+    h, w = rgb.shape[:2]
+    prob = np.zeros(h * w, np.float32)
+    alias = np.zeros(h * w, np.uint32)
+    if lib().rtc_env_alias(_p(rgb), w, h, _p(prob), _p(alias)) != 0:
+        raise ValueError("env map has no energy")
+    return prob, alias
+
+
+def env_sample_mode(rgb, mode, seed, n, rot=0.0):
+    rgb = np.ascontiguousarray(rgb, dtype=np.float32)
+    h, w = rgb.shape[:2]
+    out = np.zeros((n, 4), np.float32)
+    if lib().rtc_env_sample_mode(_p(rgb), w, h, rot, mode, seed, n, _p(out)) != 0:
+        raise ValueError("env map has no energy")
+    return out

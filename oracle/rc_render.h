@@ -135,6 +135,44 @@ struct EnvMap {
     std::vector<float> img;   // w*h*3, row-major
     std::vector<float> marginal, cond;
     float inv_total = 0.0f, rot = 0.0f;
+    // alias-table sampling (extension): acceptance probability and alias per
+    // texel, built by Vose's method in double over the same weights
+    bool use_alias = false;
+    std::vector<float> aprob;
+    std::vector<uint32_t> alias;
+    bool init_alias() {
+        size_t n = (size_t)w * h;
+        std::vector<double> q(n);
+        double total = 0.0;
+        for (uint32_t j = 0; j < h; j++) {
+            float sin_j = std::sin(PI * ((float)j + 0.5f) / (float)h);
+            for (uint32_t i = 0; i < w; i++) {
+                const float* px = &img[((size_t)j * w + i) * 3];
+                float wt = Color(px[0], px[1], px[2]).luminance() * sin_j;
+                q[(size_t)j * w + i] = wt > 0.0f ? (double)wt : 0.0;
+                total += q[(size_t)j * w + i];
+            }
+        }
+        if (!(total > 0.0)) return false;
+        aprob.assign(n, 1.0f);
+        alias.resize(n);
+        std::vector<uint32_t> small, large;
+        for (size_t t = 0; t < n; t++) {
+            q[t] = q[t] * (double)n / total;
+            alias[t] = (uint32_t)t;
+            if (q[t] < 1.0) small.push_back((uint32_t)t); else large.push_back((uint32_t)t);
+        }
+        size_t si = 0, li = 0;
+        while (si < small.size() && li < large.size()) {
+            uint32_t a = small[si++], g = large[li];
+            aprob[a] = (float)q[a];
+            alias[a] = g;
+            q[g] = (q[g] + q[a]) - 1.0;
+            if (q[g] < 1.0) { li++; small.push_back(g); }
+        }
+        use_alias = true;
+        return true;
+    }
     void init(const float* rgb, uint32_t w_, uint32_t h_, float rot_y) {
         w = w_; h = h_; rot = rot_y;
         img.assign(rgb, rgb + (size_t)w * h * 3);
@@ -192,8 +230,18 @@ struct EnvMap {
         return lum * (float)((size_t)w * h) * inv_total / (2.0f * PI * PI);
     }
     Vec3 sample_dir(SmallRng& rng, float& pdf) const {
-        size_t j = std::min<size_t>(partition_point(marginal.data(), h, rng.gen()), h - 1);
-        size_t i = std::min<size_t>(partition_point(&cond[j * w], w, rng.gen()), w - 1);
+        size_t i, j;
+        if (use_alias) {
+            size_t n = (size_t)w * h;
+            size_t t = (size_t)(((unsigned __int128)rng.next_u64() * (unsigned __int128)n) >> 64);
+            if (t >= n) t = n - 1;
+            if (rng.gen() >= aprob[t]) t = alias[t];
+            j = t / w;
+            i = t - j * w;
+        } else {
+            j = std::min<size_t>(partition_point(marginal.data(), h, rng.gen()), h - 1);
+            i = std::min<size_t>(partition_point(&cond[j * w], w, rng.gen()), w - 1);
+        }
         float u = ((float)i + rng.gen()) / (float)w;
         float v = ((float)j + rng.gen()) / (float)h;
         float phi = 2.0f * PI * (u - 0.5f);

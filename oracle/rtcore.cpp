@@ -195,6 +195,7 @@ bool build_scene(OScene& S, const rt_scene_desc& d) {
     if (b.kind == RT_BG_ENVMAP || b.kind == RT_BG_ENVMAP_SPLIT) {
         if (!b.env_rgb || !b.env_w || !b.env_h) { S.err = "env map missing"; return false; }
         S.bg.env.init(b.env_rgb, b.env_w, b.env_h, b.env_rot_y);
+        if (b.env_sampling == RT_ENV_ALIAS && !S.bg.env.init_alias()) { S.err = "env map has no energy"; return false; }
     }
     return true;
 }
@@ -569,6 +570,25 @@ int rtc_backward(void* s, const rt_camera* cam, const rt_sample_params* p, uint3
 // log f of the rough spectral dielectric for explicit directions (tests).
 int rtc_rsd_logf(const float* i, const float* o, const float* nrm, double k, double alpha, int refl, double* lf) {
     return logf_rsd(D3{i[0], i[1], i[2]}, D3{o[0], o[1], o[2]}, D3{nrm[0], nrm[1], nrm[2]}, k, alpha, refl != 0, *lf) ? 1 : 0;
+}
+
+// Alias table of an env image (tests: implied probabilities).
+int rtc_env_alias(const float* rgb, uint32_t w, uint32_t h, float* prob, uint32_t* alias) {
+    EnvMap e; e.init(rgb, w, h, 0.0f);
+    if (!e.init_alias()) return -1;
+    for (size_t t = 0; t < (size_t)w * h; t++) { prob[t] = e.aprob[t]; alias[t] = e.alias[t]; }
+    return 0;
+}
+// Env sampling through a given sampler (0 CDF, 1 alias): n draws of dir xyz + pdf.
+int rtc_env_sample_mode(const float* rgb, uint32_t w, uint32_t h, float rot, int mode, uint64_t seed, uint32_t n, float* out) {
+    EnvMap e; e.init(rgb, w, h, rot);
+    if (mode == 1 && !e.init_alias()) return -1;
+    SmallRng rng = SmallRng::seed_from_u64(seed);
+    for (uint32_t i = 0; i < n; i++) {
+        float pdf; Vec3 d = e.sample_dir(rng, pdf);
+        out[i * 4] = d.x; out[i * 4 + 1] = d.y; out[i * 4 + 2] = d.z; out[i * 4 + 3] = pdf;
+    }
+    return 0;
 }
 
 }  // extern "C"

@@ -45,11 +45,14 @@ class rt_material(C.Structure):
     _fields_ = [("kind", C.c_uint32), ("tex", C.c_uint32), ("p", C.c_float * 24)]
 
 
+ENV_CDF, ENV_ALIAS = 0, 1
+
+
 class rt_background(C.Structure):
     _fields_ = [("kind", C.c_uint32), ("visible_kind", C.c_uint32), ("color", C.c_float * 3),
                 ("top", C.c_float * 3), ("bottom", C.c_float * 3), ("sun_dir", C.c_float * 3),
                 ("turbidity", C.c_float), ("env_w", C.c_uint32), ("env_h", C.c_uint32),
-                ("env_rgb", F32P), ("env_rot_y", C.c_float)]
+                ("env_rgb", F32P), ("env_rot_y", C.c_float), ("env_sampling", C.c_uint32)]
 
 
 class rt_scene_desc(C.Structure):

@@ -63,6 +63,8 @@ struct DScene {
     uint32_t bg_kind, bg_visible;
     float bg_color[3], bg_top[3], bg_bottom[3], sun_dir[3], turbidity;
     uint32_t env_w, env_h; float env_rot, env_inv_total;
+    uint32_t env_sampling;                 // 0 row/column CDF, 1 alias table
+    const float* env_aprob; const uint32_t* env_alias;   // alias table: acceptance probability, alias texel
     const float* env_rgb; const float* env_marg; const float* env_cond;
     const float* elut;
     float norms[3], fire_norms[3];

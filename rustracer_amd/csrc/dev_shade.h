@@ -580,8 +580,22 @@ RT_D float env_pdf_at(const DScene& S, uint32_t i, uint32_t j) {
     return l * (float)((size_t)S.env_w * S.env_h) * S.env_inv_total / (2.0f * PI * PI);
 }
 RT_D V3 env_sample_dir(const DScene& S, Rng& rng, float& pdf) {
-    uint32_t j = min(partition_point(S.env_marg, S.env_h, rng_f(rng)), S.env_h - 1);
-    uint32_t i = min(partition_point(S.env_cond + (size_t)j * S.env_w, S.env_w, rng_f(rng)), S.env_w - 1);
+    uint32_t i, j;
+    if (S.env_sampling == 1) {
+        // alias table over the W*H texels (Walker 1977, Vose 1991): the first
+        // draw picks a column of the table (64-bit multiply-high, no modulo
+        // bias), the second accepts it or takes its alias.  Same density and
+        // the same number of draws as the two CDF searches.
+        uint32_t n = S.env_w * S.env_h;
+        uint32_t t = (uint32_t)__umul64hi(rng_u64(rng), (uint64_t)n);
+        if (t >= n) t = n - 1;
+        if (rng_f(rng) >= S.env_aprob[t]) t = S.env_alias[t];
+        j = t / S.env_w;
+        i = t - j * S.env_w;
+    } else {
+        j = min(partition_point(S.env_marg, S.env_h, rng_f(rng)), S.env_h - 1);
+        i = min(partition_point(S.env_cond + (size_t)j * S.env_w, S.env_w, rng_f(rng)), S.env_w - 1);
+    }
     float u = ((float)i + rng_f(rng)) / (float)S.env_w;
     float v = ((float)j + rng_f(rng)) / (float)S.env_h;
     float phi = 2.0f * PI * (u - 0.5f);

@@ -565,6 +565,43 @@ struct rt_scene {
 static int fail(rt_ctx* c, int code, const std::string& msg) { if (c) c->err = msg; return code; }
 #define HIPCHECK(ctx, call) do { hipError_t e_ = (call); if (e_ != hipSuccess) return fail(ctx, RT_ERR_DEVICE, std::string(#call) + ": " + hipGetErrorString(e_)); } while (0)
 
+// Alias table over env texels with weights lum * sin(theta_row) (the density
+// of EnvMapData, src/renderer.rs:32-74): Vose's O(n) construction in double,
+// small and large worklists taken in index order, so the table is a pure
+// function of the image.
+static bool env_alias_table(const float* rgb, uint32_t w, uint32_t h, std::vector<float>& prob, std::vector<uint32_t>& alias) {
+    size_t n = (size_t)w * h;
+    std::vector<double> q(n);
+    double total = 0.0;
+    for (uint32_t j = 0; j < h; j++) {
+        float sj = std::sin(PI * ((float)j + 0.5f) / (float)h);
+        for (uint32_t i = 0; i < w; i++) {
+            const float* p = rgb + ((size_t)j * w + i) * 3;
+            float wt = lum(mk(p[0], p[1], p[2])) * sj;
+            q[(size_t)j * w + i] = wt > 0.0f ? (double)wt : 0.0;
+            total += q[(size_t)j * w + i];
+        }
+    }
+    if (!(total > 0.0)) return false;
+    prob.assign(n, 1.0f);
+    alias.resize(n);
+    std::vector<uint32_t> small, large;
+    for (size_t t = 0; t < n; t++) {
+        q[t] = q[t] * (double)n / total;
+        alias[t] = (uint32_t)t;
+        (q[t] < 1.0 ? small : large).push_back((uint32_t)t);
+    }
+    size_t si = 0, li = 0;
+    while (si < small.size() && li < large.size()) {
+        uint32_t a = small[si++], g = large[li];
+        prob[a] = (float)q[a];
+        alias[a] = g;
+        q[g] = (q[g] + q[a]) - 1.0;
+        if (q[g] < 1.0) { li++; small.push_back(g); }
+    }
+    return true;
+}
+
 static bool is_device_ptr(const void* p) {
     if (!p) return false;
     hipPointerAttribute_t a;
@@ -999,6 +1036,14 @@ extern "C" int rt_scene_upload(rt_ctx* ctx, const rt_scene_desc* d, rt_scene** o
         if ((rc = upload(s.get(), b.env_rgb, (size_t)w * h * 3, &S.env_rgb))) return rc;
         if ((rc = upload_vec(s.get(), cond, &S.env_cond))) return rc;
         if ((rc = upload_vec(s.get(), marg, &S.env_marg))) return rc;
+        S.env_sampling = b.env_sampling == RT_ENV_ALIAS ? 1u : 0u;
+        if (b.env_sampling == RT_ENV_ALIAS) {
+            std::vector<float> aprob;
+            std::vector<uint32_t> alias;
+            if (!env_alias_table(b.env_rgb, w, h, aprob, alias)) return fail(ctx, RT_ERR_INVALID, "env map has no energy");
+            if ((rc = upload_vec(s.get(), aprob, &S.env_aprob))) return rc;
+            if ((rc = upload_vec(s.get(), alias, &S.env_alias))) return rc;
+        } else if (b.env_sampling != RT_ENV_CDF) return fail(ctx, RT_ERR_INVALID, "env_sampling");
     }
     S.elut = ctx->elut;
     for (int k = 0; k < 3; k++) { S.norms[k] = ctx->norms[k]; S.fire_norms[k] = ctx->fire_norms[k]; }

@@ -437,12 +437,14 @@ class Background:
     """Background::Solid / Gradient / Physical / EnvMap / EnvMapSplit."""
 
     def __init__(self, kind=F.BG_SOLID, color=(0, 0, 0), top=(1, 1, 1), bottom=(0, 0, 0), sun_dir=(0, 1, 0),
-                 turbidity=3.0, env=None, env_rot_y_deg=0.0, visible=None):
+                 turbidity=3.0, env=None, env_rot_y_deg=0.0, visible=None, sampling="cdf"):
         self.kind, self.color, self.top, self.bottom = kind, v3(color), v3(top), v3(bottom)
         self.sun_dir, self.turbidity = v3(sun_dir), float(turbidity)
         self.env = None if env is None else np.ascontiguousarray(env, dtype=np.float32)
         self.env_rot = float(f32(env_rot_y_deg) * f32(math.pi / 180))
         self.visible = visible
+        assert sampling in ("cdf", "alias")
+        self.sampling = sampling
 
     @staticmethod
     def solid(c):
@@ -461,10 +463,12 @@ class Background:
         return Background(F.BG_PHYSICAL, sun_dir=d, turbidity=turbidity)
 
     @staticmethod
-    def envmap(image_hw3, rotate_y_deg=0.0, visible=None):
-        """visible: None or Background.solid / Background.gradient (EnvMapSplit)."""
+    def envmap(image_hw3, rotate_y_deg=0.0, visible=None, sampling="cdf"):
+        """visible: None or Background.solid / Background.gradient (EnvMapSplit).
+        sampling: "cdf" (the reference's row/column CDF searches) or "alias"
+        (alias table over texels, same density; extension)."""
         kind = F.BG_ENVMAP if visible is None else F.BG_ENVMAP_SPLIT
-        return Background(kind, env=image_hw3, env_rot_y_deg=rotate_y_deg, visible=visible)
+        return Background(kind, env=image_hw3, env_rot_y_deg=rotate_y_deg, visible=visible, sampling=sampling)
 
     def to_c(self):
         b = F.rt_background()
@@ -477,6 +481,7 @@ class Background:
             b.bottom[k] = float(src.bottom[k])
             b.sun_dir[k] = float(self.sun_dir[k])
         b.turbidity = self.turbidity
+        b.env_sampling = F.ENV_ALIAS if self.sampling == "alias" else F.ENV_CDF
         if self.env is not None:
             b.env_h, b.env_w = self.env.shape[0], self.env.shape[1]
             b.env_rgb = self.env.ctypes.data_as(F.F32P)

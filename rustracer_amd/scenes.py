@@ -169,7 +169,8 @@ def mesh_envmap(width=1280, height=720, level=8):
     (frequency 4, permutation seed 1), smooth vertex normals, rough GGX
     dielectric (IOR 1.5, roughness 0.15).  A 20x20 diffuse 0.8 ground quad at
     y = -1.1.  Lighting: sky_envmap() 2048x1024 (sun lobe sigma 1 deg, peak
-    5e4, elevation 35 deg, azimuth 30 deg), importance sampled.  Camera at
+    5e4, elevation 35 deg, azimuth 30 deg), importance sampled through an
+    alias table over its texels.  Camera at
     (0, 0.6, -3.6) looking at (0, -0.1, 0), vfov 40.  Render: 64 spp, max
     depth 12.
     """
@@ -177,7 +178,7 @@ def mesh_envmap(width=1280, height=720, level=8):
     v, n = displaced_icosphere(level)
     objs = [Mesh(v, RoughDielectric(1.5, 0.15), normals=n),
             Quad((-10, -1.1, -10), (20, 0, 0), (0, 0, 20), Lambertian((0.8, 0.8, 0.8)))]
-    scene = Scene(objs, [], Background.envmap(sky_envmap()), name=f"mesh_envmap_L{level}")
+    scene = Scene(objs, [], Background.envmap(sky_envmap(), sampling="alias"), name=f"mesh_envmap_L{level}")
     cam = Camera((0.0, 0.6, -3.6), (0.0, -0.1, 0.0), (0, 1, 0), 40.0, width / height)
     return scene, cam, dict(width=width, height=height, spp=64, max_depth=12)
 
@@ -280,7 +281,7 @@ def volume_glass_box(width=640, height=480, grid_n=128):
     GridMedium(inner, perlin_grid(grid_n), 8.0, (0.9, 0.9, 0.9), 0.6)
     objs = [Quad((-10, -0.001, -10), (20, 0, 0), (0, 0, 20), Lambertian((0.8, 0.8, 0.8))),
             Box((-0.6, 0.0, -0.6), (0.6, 1.2, 0.6), Dielectric(1.33)), inner]
-    scene = Scene(objs, [], Background.envmap(sky_envmap()), name="volume_glass_box")
+    scene = Scene(objs, [], Background.envmap(sky_envmap(), sampling="alias"), name="volume_glass_box")
     cam = Camera((0.0, 1.3, -3.0), (0.0, 0.55, 0.0), (0, 1, 0), 35.0, width / height)
     return scene, cam, dict(width=width, height=height, spp=64, max_depth=24)
 

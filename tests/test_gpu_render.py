@@ -97,19 +97,20 @@ def test_device_buffer_output_matches_host(gpu_renderer):
     assert np.array_equal(dev.cpu().numpy(), host)
 
 
-def small_env_scene(W=48, H=48):
+def small_env_scene(W=48, H=48, sampling="cdf"):
     from rustracer_amd import scene as S
     v, n = scenes.displaced_icosphere(3, amplitude=0.05)
     env = scenes.sky_envmap(128, 64, sun_sigma_deg=4.0, sun_peak=200.0)
     sc = S.Scene([S.Mesh(v, S.RoughDielectric(1.5, 0.15), normals=n),
                   S.Quad((-10, -1.1, -10), (20, 0, 0), (0, 0, 20), S.Lambertian((0.8, 0.8, 0.8)))],
-                 [], S.Background.envmap(env), name="small_env")
+                 [], S.Background.envmap(env, sampling=sampling), name="small_env")
     cam = S.Camera((0.0, 0.6, -3.6), (0.0, -0.1, 0.0), (0, 1, 0), 40.0, W / H)
     return sc, cam
 
 
-def test_env_mesh_rough_dielectric_same_streams_and_three_sigma(gpu_renderer):
-    sc, cam = small_env_scene()
+@pytest.mark.parametrize("sampling", ["cdf", "alias"])
+def test_env_mesh_rough_dielectric_same_streams_and_three_sigma(gpu_renderer, sampling):
+    sc, cam = small_env_scene(sampling=sampling)
     gpu_renderer.upload(sc)
     orc = O.OracleScene(sc)
     g = passes_gpu(gpu_renderer, cam, 48, 48, 4, 12)
