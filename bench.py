@@ -326,10 +326,12 @@ def measure_adjoint(args, with_cpu):
     grad = torch.zeros(len(P), dtype=torch.float32, device="cuda")
     n = float(img.numel())
 
+    bopts = dict(capacity=opts["capacity"], grid=opts["grid"], sort=opts["sort"])
+
     def step(k):
         r.render_wavefront(img, cam, W, H, spp, max_depth=depth, seed=k, **opts)
         dl = (2.0 / n) * (img / spp - target)
-        r.backward(dl, cam, W, H, spp, P, max_depth=depth, seed=k, grad=grad)
+        r.backward(dl, cam, W, H, spp, P, max_depth=depth, seed=k, grad=grad, wavefront=True, **bopts)
 
     for k in range(args.warmup):
         step(k)
@@ -352,9 +354,14 @@ def measure_adjoint(args, with_cpu):
     dl = (2.0 / n) * (img / spp - target)
     torch.cuda.synchronize()
     r.set_timing(True)
-    g7 = r.backward(dl, cam, W, H, spp, P, max_depth=depth, seed=7).astype(np.float64)
+    g7 = r.backward(dl, cam, W, H, spp, P, max_depth=depth, seed=7, wavefront=True, **bopts).astype(np.float64)
     bms, bn = r.timing()
     r.set_timing(False)
+    r.set_timing(True)
+    g7m = r.backward(dl, cam, W, H, spp, P, max_depth=depth, seed=7).astype(np.float64)
+    mms, mn = r.timing()
+    r.set_timing(False)
+    log(f"[adjoint] backward: wavefront {bms:.1f} ms, megakernel {mms:.1f} ms")
     # finite differences of the same loss, same streams (seed 7)
     tgt = target.double()
 
@@ -386,6 +393,7 @@ def measure_adjoint(args, with_cpu):
     pb = PRIM_BYTES["gem"]
     bps = ((cnt["node_tests"] + cnt["shadow_node_tests"]) * 256.0 + (cnt["prim_tests"] + cnt["shadow_prim_tests"]) * pb) \
         / max(cnt["samples"], 1)
+    # one "launch" = one chunk of paths through the primal and replay wavefronts
     bpl = 2.0 * bps * W * H * spp / max(bn, 1)
     achieved = bpl / (bms / 1e3 / max(bn, 1)) / 1e9
     cpu = None
@@ -394,12 +402,15 @@ def measure_adjoint(args, with_cpu):
     r.close()
     return {"value": rate, "unit": "gradient evaluations/s", "ms_per_step": el * 1e3 / args.steps,
             "config": {"workload": "gem_adjoint_512x512x64_20_params", "width": W, "height": H, "spp": spp,
-                       "max_depth": depth, "params": len(P), "forward": "wavefront", "backward": "path replay",
-                       "backward_ms": bms, "fd_h_rel": 1e-3,
+                       "max_depth": depth, "params": len(P), "forward": "wavefront",
+                       "backward": "path replay, wavefront", "backward_ms": bms, "backward_megakernel_ms": mms,
+                       "wavefront_vs_megakernel_grad_max_rel": float(np.abs(g7 - g7m).max() / max(np.abs(g7m).max(), 1e-30)),
+                       "fd_h_rel": 1e-3,
                        "fd_within_2pct": int(within.sum()), "fd_rel_err": [float(x) for x in rel],
                        "gradient": [float(x) for x in g7], "finite_difference": [float(x) for x in fd]},
             "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                         "frac": achieved / HBM_PEAK_GBS, "kernel": "k_backward", "avg_launch_ms": bms / max(bn, 1),
+                         "frac": achieved / HBM_PEAK_GBS, "kernel": "rt_backward_wavefront (primal + replay stages)",
+                         "avg_launch_ms": bms / max(bn, 1),
                          "bytes_per_launch": bpl},
             "cpu_baseline": cpu}
 

@@ -377,6 +377,16 @@ int  rt_backward(rt_ctx *ctx, rt_scene *scene, const rt_camera *cam,
                  const rt_param *dparams, uint32_t n, const float *dloss_dimage,
                  float *grad);
 
+/* rt_backward as wavefronts: per chunk of paths, the primal wavefront writes
+ * each sample's radiance, then the replay wavefront (the same extend / shade
+ * / shadow stages with per-path adjoint state) distributes it.  Same gradient
+ * as rt_backward up to the order of float additions; opts as for
+ * rt_render_wavefront (may be NULL).                                          */
+int  rt_backward_wavefront(rt_ctx *ctx, rt_scene *scene, const rt_camera *cam,
+                           const rt_sample_params *params, uint32_t spp,
+                           const rt_param *dparams, uint32_t n, const float *dloss_dimage,
+                           float *grad, const rt_wavefront_opts *opts);
+
 /* Trace photons and build the kd-tree on the device: PhotonMap::build /
  * build_from_quad (src/photon.rs:41-135, 165-191).  The map stays attached to
  * the scene; rt_sample_params.use_photons enables the gather.                */

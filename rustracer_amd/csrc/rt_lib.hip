@@ -272,7 +272,8 @@ struct __align__(16) WfPath {
     float sun_w; uint32_t flags; uint32_t pixel; uint32_t pad0;   // flags: 1 spectral, 2 prev_specular
     uint64_t rng[4];
     float hit_t; int32_t hit_slot; int32_t hit_prim; uint32_t key;
-    uint32_t pad1[4];
+    uint32_t sid;        // sample index within the chunk (per-sample radiance of the adjoint's primal pass)
+    uint32_t pad1[3];
 };
 static_assert(sizeof(WfPath) == 128, "path record is one 128-B line");
 
@@ -287,6 +288,7 @@ struct WfQueues {
     WfShadow* shadow;     // 3 * cap
     uint32_t* counters;   // 64: 32 per-bounce counters, then the chunk position
     float* fb;
+    float* lsamp;         // non-null: contributions go to the path's sample (3 floats per sid), not to fb
     uint32_t sort;        // 0: one bucket (no material sort)
 };
 
@@ -397,7 +399,7 @@ __global__ __launch_bounds__(BLOCK) void k_wf_generate(DCamera cam0, DParams P0,
     path_init(ps, r);
     WfPath w;
     wf_store(w, ps, rng);
-    w.pixel = fb_off + pixel; w.pad0 = view; w.hit_t = 0.0f; w.hit_slot = -1; w.hit_prim = -1; w.key = 0;
+    w.pixel = fb_off + pixel; w.pad0 = view; w.hit_t = 0.0f; w.hit_slot = -1; w.hit_prim = -1; w.key = 0; w.sid = i;
     Q.paths[i] = w;
     Q.active[i] = i;
 }
@@ -447,9 +449,10 @@ __device__ __forceinline__ bool shade_one(const DScene& S, const DParams& P, WfQ
     V3 color = mk(0, 0, 0);
     bool cont = bounce(S, ps, id, depth, P.bg_scale, P.use_photons != 0, rng, color, sh, nsh, st) && depth + 1 < (int)P.max_depth;
     if (color.x != 0.0f || color.y != 0.0f || color.z != 0.0f) {
-        atomicAdd(Q.fb + 3 * (size_t)w.pixel, color.x);
-        atomicAdd(Q.fb + 3 * (size_t)w.pixel + 1, color.y);
-        atomicAdd(Q.fb + 3 * (size_t)w.pixel + 2, color.z);
+        float* dst = Q.lsamp ? Q.lsamp + 3 * (size_t)w.sid : Q.fb + 3 * (size_t)w.pixel;
+        atomicAdd(dst, color.x);
+        atomicAdd(dst + 1, color.y);
+        atomicAdd(dst + 2, color.z);
     }
     if (cont) wf_store(w, ps, rng);
     return cont;
@@ -469,7 +472,7 @@ __global__ __launch_bounds__(BLOCK) void k_wf_shade(DScene S, DParams P, WfQueue
         bool cont = false;
         int nsh = 0;
         ShadowReq sh[3];
-        uint32_t pid = 0, pixel = 0, view = 0;
+        uint32_t pid = 0, pixel = 0, view = 0, sid = 0;
         if (valid) {
             int b = 0;
             while (i >= off[b + 1]) b++;
@@ -477,6 +480,7 @@ __global__ __launch_bounds__(BLOCK) void k_wf_shade(DScene S, DParams P, WfQueue
             WfPath& w = Q.paths[pid];
             pixel = w.pixel;
             view = w.pad0;
+            sid = w.sid;
             if (BATCH) cont = shade_one(B.scenes[view], B.views[view].P, Q, w, depth, sh, nsh, st);
             else cont = shade_one(S, P, Q, w, depth, sh, nsh, st);
         }
@@ -490,7 +494,7 @@ __global__ __launch_bounds__(BLOCK) void k_wf_shade(DScene S, DParams P, WfQueue
                 q.o_tmax = make_float4(sh[k].o.x, sh[k].o.y, sh[k].o.z, sh[k].t_max);
                 q.d_time = make_float4(sh[k].d.x, sh[k].d.y, sh[k].d.z, sh[k].time);
                 q.c_pixel = make_float4(sh[k].contrib.x, sh[k].contrib.y, sh[k].contrib.z, __uint_as_float(pixel));
-                q.meta = make_uint4(view, 0u, 0u, 0u);
+                q.meta = make_uint4(view, sid, 0u, 0u);
                 Q.shadow[s] = q;
             }
         }
@@ -508,14 +512,203 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(OCC > 0 ?
         float tr = BATCH ? trace_transmittance(B.scenes[q.meta.x], r, 0.001f, q.o_tmax.w, st, false)
                          : trace_transmittance(S, r, 0.001f, q.o_tmax.w, st, false);
         if (tr > 0.0f) {
-            size_t px = 3 * (size_t)__float_as_uint(q.c_pixel.w);
-            atomicAdd(Q.fb + px, q.c_pixel.x * tr);
-            atomicAdd(Q.fb + px + 1, q.c_pixel.y * tr);
-            atomicAdd(Q.fb + px + 2, q.c_pixel.z * tr);
+            float* dst = Q.lsamp ? Q.lsamp + 3 * (size_t)q.meta.y : Q.fb + 3 * (size_t)__float_as_uint(q.c_pixel.w);
+            atomicAdd(dst, q.c_pixel.x * tr);
+            atomicAdd(dst + 1, q.c_pixel.y * tr);
+            atomicAdd(dst + 2, q.c_pixel.z * tr);
         }
     }
 }
 
+
+// ---- wavefront path replay (adjoint) ---------------------------------------------------------
+// Per path slot, beside its WfPath: the radiance still to be collected, the
+// loss gradient of its sample, the pending score of the last scattered vertex
+// (applied when the next vertex is shaded, after that vertex's shadow rays
+// have been subtracted) and the vertex score of the current vertex's NEE.
+struct __align__(16) WfAdj {
+    float4 rem;                       // xyz
+    float4 delta;                     // xyz
+    int32_t pn, vn;
+    int8_t pp[RT_SCORE_MAX], vp[3];
+    int8_t pad0[7];
+    float pv[3 * RT_SCORE_MAX];
+    float vv[9];
+    float pad1[3];
+};
+static_assert(sizeof(WfAdj) == 176, "WfAdj layout");
+
+struct WfAdjQ {
+    WfAdj* adj;
+    const float* lsamp;               // primal radiance per sample of the chunk
+    const float* dimg;                // dLoss/dI per pixel
+    const int8_t* fmap;
+    float inv_spp;
+    uint32_t nparams;
+    float* grad;
+};
+
+__global__ __launch_bounds__(BLOCK) void k_wfb_init(WfQueues Q, WfAdjQ A, uint32_t n) {
+    uint32_t i = blockIdx.x * BLOCK + threadIdx.x;
+    if (i >= n) return;
+    const WfPath& w = Q.paths[i];
+    V3 L = mk(A.lsamp[3 * i], A.lsamp[3 * i + 1], A.lsamp[3 * i + 2]);
+    bool ok = isfinite(L.x) && isfinite(L.y) && isfinite(L.z);
+    size_t px = 3 * (size_t)w.pixel;
+    WfAdj a;
+    a.rem = ok ? make_float4(L.x, L.y, L.z, 0.0f) : make_float4(0, 0, 0, 0);
+    a.delta = ok ? make_float4(A.dimg[px] * A.inv_spp, A.dimg[px + 1] * A.inv_spp, A.dimg[px + 2] * A.inv_spp, 0.0f)
+                 : make_float4(0, 0, 0, 0);
+    a.pn = 0; a.vn = 0;
+    A.adj[i] = a;
+}
+
+struct WfReplayHook {
+    const int8_t* fmap;
+    float* g;
+    V3 delta, rem;
+    Score pend;
+    bool pend_ok;
+    Score vs;
+    bool vs_ok;
+    bool vdep[3];
+    RT_D void add(const Score& sc, V3 c) {
+        for (int k = 0; k < sc.n; k++) {
+            float v = delta.x * c.x * sc.v[k].x + delta.y * c.y * sc.v[k].y + delta.z * c.z * sc.v[k].z;
+            if (v != 0.0f && isfinite(v)) atomicAdd(g + sc.p[k], v);
+        }
+    }
+    RT_D V3 emit(V3 c) { rem -= c; return c; }
+    RT_D V3 emit_vertex(V3 c, const DScene& S, const Rec& h, const Ray& r) {
+        rem -= c;
+        Score sc;
+        V3 wo = unit(-r.d);
+        vertex_score(S, fmap, h, wo, wo, r.wavelength, sc);
+        add(sc, c);
+        return c;
+    }
+    RT_D void shadow(ShadowReq& q, int k, const DScene& S, const Rec& h, const Ray& r, bool vd) {
+        vdep[k] = vd;
+        if (vd && !vs_ok) { vertex_score(S, fmap, h, unit(-r.d), unit(q.d), r.wavelength, vs); vs_ok = true; }
+    }
+    RT_D void scatter(const DScene& S, const Rec& h, const Ray& r, V3 d) {
+        vertex_score(S, fmap, h, unit(-r.d), unit(d), r.wavelength, pend);
+        pend_ok = true;
+    }
+};
+
+__global__ __launch_bounds__(BLOCK) void k_wfb_shade(DScene S, DParams P, WfQueues Q, WfAdjQ A, int depth) {
+    __shared__ float g[RT_MAX_PARAMS];
+    for (uint32_t t = threadIdx.x; t < A.nparams; t += BLOCK) g[t] = 0.0f;
+    __syncthreads();
+    Stack st; init_stack(st);
+    uint32_t off[WF_NB + 1];
+    off[0] = 0;
+    for (int b = 0; b < WF_NB; b++) off[b + 1] = off[b] + Q.counters[WF_CNT_BUCKET + b];
+    uint32_t total = off[WF_NB];
+    uint32_t stride = gridDim.x * BLOCK;
+    for (uint32_t base = blockIdx.x * BLOCK; base < total; base += stride) {
+        uint32_t i = base + threadIdx.x;
+        bool valid = i < total;
+        bool cont = false;
+        int nsh = 0;
+        ShadowReq sh[3];
+        bool vdep[3] = {false, false, false};
+        uint32_t pid = 0;
+        if (valid) {
+            int b = 0;
+            while (i >= off[b + 1]) b++;
+            pid = Q.buckets[(size_t)b * Q.cap + (i - off[b])];
+            WfPath& w = Q.paths[pid];
+            WfAdj& a = A.adj[pid];
+            WfReplayHook rh;
+            rh.fmap = A.fmap; rh.g = g;
+            rh.delta = mk(a.delta.x, a.delta.y, a.delta.z);
+            rh.rem = mk(a.rem.x, a.rem.y, a.rem.z);
+            rh.pend_ok = false; rh.vs_ok = false;
+            rh.vdep[0] = rh.vdep[1] = rh.vdep[2] = false;
+            // what is left now arrived through the previous vertex's sampled direction
+            if (a.pn > 0) {
+                Score pv;
+                pv.n = a.pn;
+                for (int k = 0; k < a.pn; k++) { pv.p[k] = a.pp[k]; pv.v[k] = mk(a.pv[3 * k], a.pv[3 * k + 1], a.pv[3 * k + 2]); }
+                rh.add(pv, rh.rem);
+            }
+            if (depth < (int)P.max_depth) {
+                PathState ps; Rng rng;
+                wf_load(w, ps, rng);
+                HitId id; id.t = w.hit_t; id.slot = w.hit_slot; id.prim = w.hit_prim;
+                V3 color = mk(0, 0, 0);
+                cont = bounce_t(S, ps, id, depth, P.bg_scale, P.use_photons != 0, rng, color, sh, nsh, st, rh) &&
+                       depth + 1 < (int)P.max_depth;
+                if (cont) wf_store(w, ps, rng);
+            }
+            a.rem = make_float4(rh.rem.x, rh.rem.y, rh.rem.z, 0.0f);
+            a.pn = (cont && rh.pend_ok) ? rh.pend.n : 0;
+            for (int k = 0; k < a.pn; k++) {
+                a.pp[k] = (int8_t)rh.pend.p[k];
+                a.pv[3 * k] = rh.pend.v[k].x; a.pv[3 * k + 1] = rh.pend.v[k].y; a.pv[3 * k + 2] = rh.pend.v[k].z;
+            }
+            a.vn = rh.vs_ok ? min(rh.vs.n, 3) : 0;
+            for (int k = 0; k < a.vn; k++) {
+                a.vp[k] = (int8_t)rh.vs.p[k];
+                a.vv[3 * k] = rh.vs.v[k].x; a.vv[3 * k + 1] = rh.vs.v[k].y; a.vv[3 * k + 2] = rh.vs.v[k].z;
+            }
+            for (int k = 0; k < 3; k++) vdep[k] = rh.vdep[k];
+        }
+        uint32_t nslot = wave_append(Q.counters + WF_CNT_NEXT, 0u, cont);
+        if (cont) Q.next[nslot] = pid;
+        for (int k = 0; k < 3; k++) {
+            bool has = valid && k < nsh;
+            uint32_t s = wave_append(Q.counters + WF_CNT_SHADOW, 0u, has);
+            if (has) {
+                WfShadow q;
+                q.o_tmax = make_float4(sh[k].o.x, sh[k].o.y, sh[k].o.z, sh[k].t_max);
+                q.d_time = make_float4(sh[k].d.x, sh[k].d.y, sh[k].d.z, sh[k].time);
+                q.c_pixel = make_float4(sh[k].contrib.x, sh[k].contrib.y, sh[k].contrib.z, 0.0f);
+                q.meta = make_uint4(0u, pid, vdep[k] ? 1u : 0u, 0u);
+                Q.shadow[s] = q;
+            }
+        }
+    }
+    __syncthreads();
+    for (uint32_t t = threadIdx.x; t < A.nparams; t += BLOCK)
+        if (g[t] != 0.0f) atomicAdd(A.grad + t, g[t]);
+}
+
+// Shadow rays of the replay: the collected radiance leaves the path's
+// remainder; a vertex-dependent one (its value carries that vertex's BSDF)
+// adds its own score term.
+__global__ __launch_bounds__(BLOCK) void k_wfb_shadow(DScene S, WfQueues Q, WfAdjQ A) {
+    __shared__ float g[RT_MAX_PARAMS];
+    for (uint32_t t = threadIdx.x; t < A.nparams; t += BLOCK) g[t] = 0.0f;
+    __syncthreads();
+    Stack st; init_stack(st);
+    uint32_t count = Q.counters[WF_CNT_SHADOW];
+    uint32_t stride = gridDim.x * BLOCK;
+    for (uint32_t i = blockIdx.x * BLOCK + threadIdx.x; i < count; i += stride) {
+        WfShadow q = Q.shadow[i];
+        Ray r = make_ray(mk(q.o_tmax.x, q.o_tmax.y, q.o_tmax.z), mk(q.d_time.x, q.d_time.y, q.d_time.z), q.d_time.w);
+        float tr = trace_transmittance(S, r, 0.001f, q.o_tmax.w, st, false);
+        if (tr > 0.0f) {
+            WfAdj& a = A.adj[q.meta.y];
+            V3 c = mk(q.c_pixel.x, q.c_pixel.y, q.c_pixel.z) * tr;
+            atomicAdd(&a.rem.x, -c.x);
+            atomicAdd(&a.rem.y, -c.y);
+            atomicAdd(&a.rem.z, -c.z);
+            if (q.meta.z) {
+                V3 d = mk(a.delta.x, a.delta.y, a.delta.z);
+                for (int k = 0; k < a.vn; k++) {
+                    float v = d.x * c.x * a.vv[3 * k] + d.y * c.y * a.vv[3 * k + 1] + d.z * c.z * a.vv[3 * k + 2];
+                    if (v != 0.0f && isfinite(v)) atomicAdd(g + a.vp[k], v);
+                }
+            }
+        }
+    }
+    __syncthreads();
+    for (uint32_t t = threadIdx.x; t < A.nparams; t += BLOCK)
+        if (g[t] != 0.0f) atomicAdd(A.grad + t, g[t]);
+}
 
 // ============================================================================ host side
 namespace {
@@ -1186,6 +1379,9 @@ static int wf_buffers(rt_ctx* ctx, uint32_t cap, WfQueues& Q) {
     Q.shadow = (WfShadow*)m; m += b_shadow;
     Q.counters = (uint32_t*)m;
     Q.cap = cap;
+    Q.fb = nullptr;
+    Q.lsamp = nullptr;        // only the adjoint's primal pass routes contributions per sample
+    Q.sort = 1;
     ctx->wf_cap = cap;
     return RT_OK;
 }
@@ -1505,6 +1701,82 @@ extern "C" int rt_backward(rt_ctx* ctx, rt_scene* s, const rt_camera* cam, const
         HIPCHECK(ctx, hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
         ctx->kernel_ms += ms;
         ctx->launches += (spp + chunk - 1) / chunk;
+    }
+    return finish_out(ctx, grad, (size_t)n * 4, gout);
+}
+
+extern "C" int rt_backward_wavefront(rt_ctx* ctx, rt_scene* s, const rt_camera* cam, const rt_sample_params* p, uint32_t spp,
+                                     const rt_param* prm, uint32_t n, const float* dimg, float* grad,
+                                     const rt_wavefront_opts* opts) {
+    if (!ctx || !s || !dimg || !grad) return RT_ERR_INVALID;
+    int rc;
+    if ((rc = render_args(ctx, s, cam, p, spp))) return rc;
+    std::vector<int8_t> fmap;
+    if ((rc = param_map(ctx, s, prm, n, fmap))) return rc;
+    HIPCHECK(ctx, hipSetDevice(ctx->device));
+    uint32_t npix = p->width * p->height;
+    uint64_t total = (uint64_t)npix * spp;
+    WfOpts o;
+    if ((rc = wf_opts(ctx, opts, total, o))) return rc;
+    WfQueues Q;
+    if ((rc = wf_buffers(ctx, o.cap, Q))) return rc;
+    Q.sort = o.sort;
+    Staged din, gout;
+    if ((rc = stage_in(ctx, dimg, (size_t)npix * 12, din))) return rc;
+    if ((rc = stage_out(ctx, grad, (size_t)n * 4, gout, false))) return rc;
+    // side buffers: per-slot adjoint state, per-sample primal radiance, field map
+    size_t b_adj = (size_t)o.cap * sizeof(WfAdj), b_l = ((size_t)o.cap * 12 + 255) & ~(size_t)255;
+    char* side = nullptr;
+    HIPCHECK(ctx, hipMalloc(&side, b_adj + b_l + fmap.size()));
+    std::unique_ptr<void, void (*)(void*)> guard(side, [](void* q) { (void)hipFree(q); });
+    HIPCHECK(ctx, hipMemcpy(side + b_adj + b_l, fmap.data(), fmap.size(), hipMemcpyHostToDevice));
+    WfAdjQ A;
+    A.adj = (WfAdj*)side;
+    A.lsamp = (const float*)(side + b_adj);
+    A.fmap = (const int8_t*)(side + b_adj + b_l);
+    A.dimg = (const float*)din.dev;
+    A.inv_spp = 1.0f / (float)spp;
+    A.nparams = n;
+    A.grad = (float*)gout.dev;
+    HIPCHECK(ctx, hipMemsetAsync(gout.dev, 0, (size_t)n * 4, ctx->stream));
+    DCamera C; DParams P;
+    to_dev(cam, p, C, P);
+    P.counters = nullptr;
+    hipStream_t st = ctx->stream;
+    WfBatch none = {nullptr, nullptr, 0};
+    if (ctx->timing) HIPCHECK(ctx, hipEventRecord(ctx->ev0, st));
+    uint32_t depth = P.max_depth;
+    uint64_t chunks = 0;
+    for (uint64_t first = 0; first < total; first += o.cap, chunks++) {
+        uint32_t m = (uint32_t)std::min<uint64_t>(o.cap, total - first);
+        // primal: radiance of every sample of the chunk
+        HIPCHECK(ctx, hipMemsetAsync(side + b_adj, 0, (size_t)m * 12, st));
+        WfQueues Qp = Q;
+        Qp.lsamp = (float*)(side + b_adj);
+        wf_chunk<false>(ctx, s->S, C, P, Qp, first, m, o.grid, o.occ, depth, none);
+        // replay from the same streams
+        hipLaunchKernelGGL((k_wf_generate<false>), dim3((m + BLOCK - 1) / BLOCK), dim3(BLOCK), 0, st, C, P, Q, first, m, none);
+        hipLaunchKernelGGL(k_wfb_init, dim3((m + BLOCK - 1) / BLOCK), dim3(BLOCK), 0, st, Q, A, m);
+        hipLaunchKernelGGL(k_wf_reset, dim3(1), dim3(32), 0, st, Q.counters, m);
+        WfQueues QA = Q, QB = Q;
+        QB.active = Q.next; QB.next = Q.active;
+        for (uint32_t d = 0; d < depth; d++) {
+            const WfQueues& Qd = (d & 1) ? QB : QA;
+            launch_extend<false>(o.occ, o.grid, st, s->S, Qd, none);
+            hipLaunchKernelGGL(k_wfb_shade, dim3(o.grid), dim3(BLOCK), 0, st, s->S, P, Qd, A, (int)d);
+            hipLaunchKernelGGL(k_wfb_shadow, dim3(o.grid), dim3(BLOCK), 0, st, s->S, Qd, A);
+            hipLaunchKernelGGL(k_wf_advance, dim3(1), dim3(32), 0, st, Qd.counters);
+        }
+        HIPCHECK(ctx, hipGetLastError());
+    }
+    ctx->ext_used = 0;
+    if (ctx->timing) {
+        HIPCHECK(ctx, hipEventRecord(ctx->ev1, st));
+        HIPCHECK(ctx, hipEventSynchronize(ctx->ev1));
+        float ms = 0.0f;
+        HIPCHECK(ctx, hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+        ctx->kernel_ms += ms;
+        ctx->launches += chunks;
     }
     return finish_out(ctx, grad, (size_t)n * 4, gout);
 }

@@ -211,10 +211,12 @@ class Renderer:
         return out
 
     def backward(self, dloss_dimage, camera, width, height, spp, params, sample_idx=0, strata=0, bg_scale=1.0,
-                 photon_map=False, max_depth=50, seed=0, scene=None, grad=None):
+                 photon_map=False, max_depth=50, seed=0, scene=None, grad=None, wavefront=False, capacity=0, grid=0,
+                 sort=True, occupancy=0):
         """Path-replay adjoint: dLoss/dtheta for dloss_dimage = dLoss/d(mean
         image) (H x W x 3, host array or device tensor); returns grad (float32
-        numpy array, or fills the given device tensor)."""
+        numpy array, or fills the given device tensor).  wavefront=True runs
+        rt_backward_wavefront with the given launch options."""
         p = self.params(width, height, sample_idx, strata, bg_scale, max_depth, None, photon_map, seed)
         cam = camera.to_c()
         if hasattr(dloss_dimage, "data_ptr"):
@@ -222,8 +224,14 @@ class Renderer:
         else:
             d = np.ascontiguousarray(dloss_dimage, dtype=np.float32)
         g = np.zeros(len(params), np.float32) if grad is None else grad
-        self._check(self.L.rt_backward(self.ctx, self.scene if scene is None else scene, C.byref(cam), C.byref(p), spp,
-                                       self._params(params), len(params), _ptr(d), _ptr(g)), "rt_backward")
+        sc = self.scene if scene is None else scene
+        if wavefront:
+            o = F.rt_wavefront_opts(capacity=capacity, grid=grid, sort=1 if sort else 0, use_graph=0, occupancy=occupancy)
+            self._check(self.L.rt_backward_wavefront(self.ctx, sc, C.byref(cam), C.byref(p), spp, self._params(params),
+                                                     len(params), _ptr(d), _ptr(g), C.byref(o)), "rt_backward_wavefront")
+        else:
+            self._check(self.L.rt_backward(self.ctx, sc, C.byref(cam), C.byref(p), spp, self._params(params), len(params),
+                                           _ptr(d), _ptr(g)), "rt_backward")
         return g
 
     # ---------------------------------------------------------------- photons

@@ -103,3 +103,19 @@ def test_adjoint_matches_finite_differences_with_common_random_numbers(gpu_rende
     bad = np.abs(g - fd) > 2e-2 * np.abs(fd) + 1e-3 * scale
     assert not bad.any(), np.c_[np.arange(len(P)), g, fd][bad]
     assert np.count_nonzero(np.abs(fd) > 1e-2 * scale) >= 10
+
+
+@pytest.mark.parametrize("cap,sort", [(4096, True), (1 << 20, False)])
+def test_wavefront_adjoint_equals_megakernel_adjoint(gpu_renderer, cap, sort):
+    """The replay as wavefronts (primal radiance per sample, then the replay
+    stages) gives the megakernel replay's gradient; capacity 4096 splits the
+    frame into two chunks."""
+    W, H, spp = 24, 24, 8
+    sc, cam, cfg = setup(gpu_renderer, W, H)
+    P = cfg["params"]
+    gpu_renderer.set_params(P, cfg["theta_start"])
+    w = np.random.default_rng(6).standard_normal((H, W, 3)).astype(np.float32)
+    a = gpu_renderer.backward(w, cam, W, H, spp, P, max_depth=24, seed=8)
+    b = gpu_renderer.backward(w, cam, W, H, spp, P, max_depth=24, seed=8, wavefront=True, capacity=cap, sort=sort)
+    scale = np.abs(a).max()
+    assert np.allclose(a, b, rtol=2e-3, atol=2e-4 * scale), np.c_[a, b]
