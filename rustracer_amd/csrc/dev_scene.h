@@ -69,7 +69,10 @@ struct DScene {
     const float* elut;
     float norms[3], fire_norms[3];
     // photon map (kd order, 3 float4 per photon: pos+axis, power, normal)
-    const float4* photons; uint32_t num_photons; float ph_r2, ph_norm;
+    // photon map: left-balanced kd-tree in heap order, structure of arrays
+    // (position + split axis in w, power, normal), 16 B per array per photon
+    const float4* ph_pos; const float4* ph_pow; const float4* ph_nrm;
+    uint32_t num_photons; float ph_r2, ph_norm;
     uint32_t stack_bound;
 };
 

@@ -720,40 +720,46 @@ RT_D V3 lights_generate(const DScene& S, V3 origin, Rng& rng, float time) {
 }
 
 // ---- photon gather (kd_gather, src/photon.rs:195-228), iterative --------------------------
+// PhotonMap::irradiance (src/photon.rs:141-228): Epanechnikov-weighted sum of
+// the photons within the gather radius whose normal faces the query normal,
+// times 2/(pi^2 R^2).  Range search of the heap-ordered kd-tree without a
+// stack: bit d of `pend` says the far child of the depth-d ancestor still
+// needs a visit; when a descent leaves the tree, the deepest pending ancestor
+// is recomputed from the index ((i-1)/2 per level) and its far child entered.
 RT_D V3 photon_irradiance(const DScene& S, V3 pos, V3 nrm) {
-    if (S.num_photons == 0) return mk(0, 0, 0);
-    uint32_t stk_lo[48], stk_n[48];
-    int sp = 0;
-    stk_lo[0] = 0; stk_n[0] = S.num_photons; sp = 1;
-    V3 acc = mk(0, 0, 0);
+    uint32_t n = S.num_photons;
+    if (n == 0) return mk(0, 0, 0);
     float r2 = S.ph_r2;
-    while (sp > 0) {
-        sp--;
-        uint32_t lo = stk_lo[sp], n = stk_n[sp];
-        if (n == 0) continue;
-        uint32_t mid = n / 2;
-        const float4* ph = S.photons + 3 * (size_t)(lo + mid);
-        float4 P = ph[0];
-        float dx = P.x - pos.x, dy = P.y - pos.y, dz = P.z - pos.z;
-        float d2 = dx * dx + dy * dy + dz * dz;
-        if (d2 < r2) {
-            float4 N = ph[2];
-            float dt = N.x * nrm.x + N.y * nrm.y + N.z * nrm.z;
-            if (dt > 0.0f) { float4 W = ph[1]; acc += mk(W.x, W.y, W.z) * (1.0f - d2 / r2); }
+    V3 acc = mk(0, 0, 0);
+    uint32_t node = 0, pend = 0;
+    int depth = 0;
+    for (;;) {
+        if (node < n) {
+            float4 P = S.ph_pos[node];
+            float dx = P.x - pos.x, dy = P.y - pos.y, dz = P.z - pos.z;
+            float d2 = dx * dx + dy * dy + dz * dz;
+            if (d2 < r2) {
+                float4 N = S.ph_nrm[node];
+                float dt = N.x * nrm.x + N.y * nrm.y + N.z * nrm.z;
+                if (dt > 0.0f) { float4 W = S.ph_pow[node]; acc += mk(W.x, W.y, W.z) * (1.0f - d2 / r2); }
+            }
+            int axis = (int)P.w;
+            float sq = comp(pos, axis) - (axis == 0 ? P.x : (axis == 1 ? P.y : P.z));
+            if (sq * sq < r2) pend |= 1u << depth; else pend &= ~(1u << depth);
+            node = sq <= 0.0f ? 2 * node + 1 : 2 * node + 2;
+            depth++;
+            continue;
         }
+        if (pend == 0) break;
+        int d = 31 - __clz(pend);
+        pend &= ~(1u << d);
+        uint32_t anc = node;
+        for (int k = depth; k > d; k--) anc = (anc - 1) >> 1;
+        float4 P = S.ph_pos[anc];
         int axis = (int)P.w;
         float sq = comp(pos, axis) - (axis == 0 ? P.x : (axis == 1 ? P.y : P.z));
-        uint32_t llo = lo, ln = mid, rlo = lo + mid + 1, rn = n - mid - 1;
-        // recursion order: near side first, then (maybe) the far side; a LIFO
-        // stack pushes the far side first
-        bool far_ok = sq * sq < r2;
-        if (sq <= 0.0f) {
-            if (far_ok && sp < 48) { stk_lo[sp] = rlo; stk_n[sp] = rn; sp++; }
-            if (sp < 48) { stk_lo[sp] = llo; stk_n[sp] = ln; sp++; }
-        } else {
-            if (far_ok && sp < 48) { stk_lo[sp] = llo; stk_n[sp] = ln; sp++; }
-            if (sp < 48) { stk_lo[sp] = rlo; stk_n[sp] = rn; sp++; }
-        }
+        node = sq <= 0.0f ? 2 * anc + 2 : 2 * anc + 1;
+        depth = d + 1;
     }
     return acc * S.ph_norm;
 }

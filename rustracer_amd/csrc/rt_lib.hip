@@ -751,8 +751,9 @@ struct rt_scene {
     uint64_t stats[8] = {0};
     std::vector<float4> photons_host;
     std::vector<DMaterial> mats_host;     // the materials as on the device (rt_scene_set_params edits both)
+    void* ph_mem = nullptr;                // photon map (positions, powers, normals), replaced on rebuild
     float world_min[3] = {0, 0, 0}, world_max[3] = {0, 0, 0};
-    ~rt_scene() { for (void* p : allocs) (void)hipFree(p); }
+    ~rt_scene() { for (void* p : allocs) (void)hipFree(p); if (ph_mem) (void)hipFree(ph_mem); }
 };
 
 static int fail(rt_ctx* c, int code, const std::string& msg) { if (c) c->err = msg; return code; }
@@ -1783,23 +1784,36 @@ extern "C" int rt_backward_wavefront(rt_ctx* ctx, rt_scene* s, const rt_camera* 
 
 extern "C" {
 // ---- photon map --------------------------------------------------------------------------
-static void kd_build(float4* ph, size_t n) {
+// Left-balanced kd-tree in heap order (node i has children 2i+1, 2i+2, nodes
+// 0..n-1): split on the widest axis of the subtree's bounds at the index that
+// keeps the tree complete.  The gather (photon_irradiance) walks it without a
+// stack.  The reference's tree (src/photon.rs:165-191) splits at n/2 of each
+// range; both are exact range searches over the same photons.
+struct PhRec { float4 a, b, c; };
+static uint32_t lb_left_size(uint32_t n) {
+    if (n <= 1) return 0;
+    uint32_t m = 1;
+    while (m <= n / 2) m *= 2;                 // largest power of two <= n
+    uint32_t last = n - (m - 1);               // nodes on the partial last level
+    return (m / 2 - 1) + std::min(last, m / 2);
+}
+static void heap_build(PhRec* src, uint32_t n, uint32_t node, float4* pos, float4* pw, float4* nr) {
     if (n == 0) return;
     float mn[3] = {FINF, FINF, FINF}, mx[3] = {-FINF, -FINF, -FINF};
-    for (size_t i = 0; i < n; i++) {
-        float p[3] = {ph[3 * i].x, ph[3 * i].y, ph[3 * i].z};
+    for (uint32_t i = 0; i < n; i++) {
+        float p[3] = {src[i].a.x, src[i].a.y, src[i].a.z};
         for (int k = 0; k < 3; k++) { if (p[k] < mn[k]) mn[k] = p[k]; if (p[k] > mx[k]) mx[k] = p[k]; }
     }
     int axis = 0;
     for (int a = 1; a < 3; a++) if (!((mx[a] - mn[a]) < (mx[axis] - mn[axis]))) axis = a;
-    size_t mid = n / 2;
-    struct P3 { float4 a, b, c; };
-    P3* arr = reinterpret_cast<P3*>(ph);
-    auto key = [axis](const P3& p) { return axis == 0 ? p.a.x : (axis == 1 ? p.a.y : p.a.z); };
-    std::nth_element(arr, arr + mid, arr + n, [&](const P3& x, const P3& y) { return key(x) < key(y); });
-    arr[mid].a.w = (float)axis;
-    kd_build(ph, mid);
-    kd_build(ph + 3 * (mid + 1), n - mid - 1);
+    uint32_t L = lb_left_size(n);
+    auto key = [axis](const PhRec& p) { return axis == 0 ? p.a.x : (axis == 1 ? p.a.y : p.a.z); };
+    std::nth_element(src, src + L, src + n, [&](const PhRec& x, const PhRec& y) { return key(x) < key(y); });
+    pos[node] = make_float4(src[L].a.x, src[L].a.y, src[L].a.z, (float)axis);
+    pw[node] = src[L].b;
+    nr[node] = src[L].c;
+    heap_build(src, L, 2 * node + 1, pos, pw, nr);
+    heap_build(src + L + 1, n - L - 1, 2 * node + 2, pos, pw, nr);
 }
 
 int rt_photon_build(rt_ctx* ctx, rt_scene* s, const rt_photon_params* pp, uint32_t* num_stored) {
@@ -1870,11 +1884,18 @@ int rt_photon_build(rt_ctx* ctx, rt_scene* s, const rt_photon_params* pp, uint32
     std::vector<float4> kept;
     for (uint32_t i = 0; i < n; i++) if (ok[i]) { kept.push_back(all[3 * (size_t)i]); kept.push_back(all[3 * (size_t)i + 1]); kept.push_back(all[3 * (size_t)i + 2]); }
     size_t m = kept.size() / 3;
-    kd_build(kept.data(), m);
-    const float4* dph = nullptr;
-    int rc;
-    if ((rc = upload_vec(s, kept, &dph))) return rc;
-    s->S.photons = dph;
+    std::vector<PhRec> recs(m);
+    for (size_t i = 0; i < m; i++) recs[i] = PhRec{kept[3 * i], kept[3 * i + 1], kept[3 * i + 2]};
+    std::vector<float4> heap(3 * m + 3);
+    heap_build(recs.data(), (uint32_t)m, 0, heap.data(), heap.data() + m, heap.data() + 2 * m);
+    if (s->ph_mem) { (void)hipFree(s->ph_mem); s->ph_mem = nullptr; }
+    s->S.num_photons = 0;
+    HIPCHECK(ctx, hipMalloc(&s->ph_mem, heap.size() * sizeof(float4)));
+    HIPCHECK(ctx, hipMemcpy(s->ph_mem, heap.data(), heap.size() * sizeof(float4), hipMemcpyHostToDevice));
+    const float4* dph = (const float4*)s->ph_mem;
+    s->S.ph_pos = dph;
+    s->S.ph_pow = dph + m;
+    s->S.ph_nrm = dph + 2 * m;
     s->S.num_photons = (uint32_t)m;
     float r = pp->gather_radius;
     s->S.ph_r2 = r * r;
