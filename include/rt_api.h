@@ -55,8 +55,14 @@ typedef struct rt_scene rt_scene;
  *   CONVEX   half-spaces (n.xyz, d) x data_count in pool[data_first*4..];
  *            p0-2 center, p3 crown_h, p4 pavilion_h, p5 girdle_h,
  *            p6 polish, p7-9 bbox min, p10-12 bbox max                src/diamond.rs:203-274
- *   CYLINDER p0-2 center, p3 radius, p4 height                        src/cylinder.rs
- *   DISK     p0-2 center, p3-5 normal, p6 radius                      src/disk.rs
+ *            (also StepCut, whose hit is the same, src/step_cut.rs:388-405)
+ *   CYLINDER p0-2 center of the bottom cap, p3 radius, p4 height; y axis,
+ *            closed caps                                              src/cylinder.rs:124-215
+ *   DISK     p0-2 center, p3-5 normal (normalised on upload), p6 radius;
+ *            a valid NEE light                                        src/disk.rs:105-170
+ *   CONE     p0-2 center of the base, p3 radius, p4 height; apex on +y src/cone.rs:11-129
+ *   PLANE    p0-2 point, p3-5 normal (normalised on upload), p6 wave
+ *            amplitude, p7 wave scale (InfinitePlane)                 src/plane.rs:14-84
  */
 enum {
     RT_SHAPE_SPHERE   = 1,
@@ -65,7 +71,9 @@ enum {
     RT_SHAPE_BOX      = 4,
     RT_SHAPE_CONVEX   = 5,
     RT_SHAPE_CYLINDER = 6,
-    RT_SHAPE_DISK     = 7
+    RT_SHAPE_DISK     = 7,
+    RT_SHAPE_CONE     = 8,
+    RT_SHAPE_PLANE    = 9
 };
 
 /* ---- transforms ----------------------------------------------------------

@@ -212,6 +212,214 @@ struct Quad : Hittable {
     float light_area() const override { return material_is_emissive(mat) ? area : 0.0f; }
 };
 
+// ---- Cylinder (src/cylinder.rs:124-224) ----------------------------------------------
+static const float TAU_F = 6.28318530717958647692f;
+static inline float rs_fract(float x) { return x - std::trunc(x); }
+
+struct Cylinder : Hittable {
+    Point3 center; float radius = 1.0f, height = 1.0f;
+    const Material* mat = nullptr;
+    bool hit(const Ray& r, float t_min, float t_max, HitRecord& rec) const override {
+        Vec3 oc = r.origin - center;
+        float dx = r.direction.x, dz = r.direction.z, dy = r.direction.y;
+        float best_t = t_max;
+        bool found = false;
+        float a = dx * dx + dz * dz;
+        if (a > 1e-8f) {
+            float half_b = oc.x * dx + oc.z * dz;
+            float c = oc.x * oc.x + oc.z * oc.z - radius * radius;
+            float disc = half_b * half_b - a * c;
+            if (disc >= 0.0f) {
+                float sq = std::sqrt(disc);
+                float roots[2] = {(-half_b - sq) / a, (-half_b + sq) / a};
+                for (float root : roots) {
+                    if (root < t_min || root > best_t) continue;
+                    Point3 p = r.at(root);
+                    float ly = p.y - center.y;
+                    if (ly < 0.0f || ly > height) continue;
+                    Vec3 outward((p.x - center.x) / radius, 0.0f, (p.z - center.z) / radius);
+                    rec = HitRecord::make(p, root, mat, r, outward);
+                    float phi = std::atan2(p.z - center.z, p.x - center.x);
+                    rec.u = rs_fract(phi / TAU_F + 0.5f);
+                    rec.v = ly / height;
+                    best_t = root; found = true;
+                }
+            }
+        }
+        if (std::fabs(dy) > 1e-8f) {
+            const float caps[2][2] = {{0.0f, -1.0f}, {height, 1.0f}};
+            for (int k = 0; k < 2; k++) {
+                float t = (caps[k][0] - oc.y) / dy;
+                if (t < t_min || t > best_t) continue;
+                Point3 p = r.at(t);
+                float lx = p.x - center.x, lz = p.z - center.z;
+                if (lx * lx + lz * lz > radius * radius) continue;
+                rec = HitRecord::make(p, t, mat, r, Vec3(0.0f, caps[k][1], 0.0f));
+                rec.u = (lx / radius + 1.0f) * 0.5f;
+                rec.v = (lz / radius + 1.0f) * 0.5f;
+                best_t = t; found = true;
+            }
+        }
+        if (found) rec.prim_id = 0;
+        return found;
+    }
+    bool bounding_box(Aabb& out) const override {
+        Vec3 rv(radius, 0.0f, radius);
+        out = Aabb(center - rv, center + rv + Vec3(0.0f, height, 0.0f));
+        return true;
+    }
+};
+
+// ---- Cone (src/cone.rs:11-129) ---------------------------------------------------------
+struct Cone : Hittable {
+    Point3 center; float radius = 1.0f, height = 1.0f;
+    const Material* mat = nullptr;
+    bool hit(const Ray& r, float t_min, float t_max, HitRecord& rec) const override {
+        Vec3 oc = r.origin - center;
+        float h = height;
+        float k = (radius / h) * (radius / h);
+        float dx = r.direction.x, dy = r.direction.y, dz = r.direction.z;
+        float hy = h - oc.y;
+        float best_t = t_max;
+        bool found = false;
+        float a = dx * dx + dz * dz - k * dy * dy;
+        float half_b = oc.x * dx + oc.z * dz + k * hy * dy;
+        float c = oc.x * oc.x + oc.z * oc.z - k * hy * hy;
+        if (std::fabs(a) > 1e-8f) {
+            float disc = half_b * half_b - a * c;
+            if (disc >= 0.0f) {
+                float sq = std::sqrt(disc);
+                float roots[2] = {(-half_b - sq) / a, (-half_b + sq) / a};
+                for (float root : roots) {
+                    if (root < t_min || root > best_t) continue;
+                    Point3 p = r.at(root);
+                    float ly = p.y - center.y;
+                    if (ly < 0.0f || ly > h) continue;
+                    float lx = p.x - center.x, lz = p.z - center.z;
+                    Vec3 grad(lx, k * (h - ly), lz);
+                    Vec3 outward = grad.length_squared() > 1e-12f ? grad.unit() : Vec3(0.0f, 1.0f, 0.0f);
+                    rec = HitRecord::make(p, root, mat, r, outward);
+                    float phi = std::atan2(lz, lx);
+                    rec.u = rs_fract(phi / TAU_F + 0.5f);
+                    rec.v = ly / h;
+                    best_t = root; found = true;
+                }
+            }
+        }
+        if (std::fabs(dy) > 1e-8f) {
+            float t = -oc.y / dy;
+            if (t >= t_min && t <= best_t) {
+                Point3 p = r.at(t);
+                float lx = p.x - center.x, lz = p.z - center.z;
+                if (lx * lx + lz * lz <= radius * radius) {
+                    rec = HitRecord::make(p, t, mat, r, Vec3(0.0f, -1.0f, 0.0f));
+                    rec.u = (lx / radius + 1.0f) * 0.5f;
+                    rec.v = (lz / radius + 1.0f) * 0.5f;
+                    found = true;
+                }
+            }
+        }
+        if (found) rec.prim_id = 0;
+        return found;
+    }
+    bool bounding_box(Aabb& out) const override {
+        Vec3 rv(radius, 0.0f, radius);
+        out = Aabb(center - rv, center + rv + Vec3(0.0f, height, 0.0f));
+        return true;
+    }
+};
+
+// ---- Disk (src/disk.rs:103-182) ----------------------------------------------------------
+struct Disk : Hittable {
+    Point3 center; Vec3 normal; float radius = 1.0f;
+    const Material* mat = nullptr;
+    Vec3 u_axis, v_axis; float area = 0.0f;
+    void init(const Point3& c, const Vec3& n, float r, const Material* m) {
+        center = c; normal = n.unit(); radius = r; mat = m;
+        vonb(normal, u_axis, v_axis);
+        area = PI * radius * radius;
+    }
+    bool hit(const Ray& r, float t_min, float t_max, HitRecord& rec) const override {
+        float denom = normal.dot(r.direction);
+        if (!(std::fabs(denom) >= 1e-8f)) return false;
+        float t = normal.dot(center - r.origin) / denom;
+        if (t < t_min || t > t_max) return false;
+        Point3 p = r.at(t);
+        Vec3 off = p - center;
+        if (off.length_squared() > radius * radius) return false;
+        rec = HitRecord::make(p, t, mat, r, normal);
+        rec.u = (off.dot(u_axis) / radius + 1.0f) * 0.5f;
+        rec.v = (off.dot(v_axis) / radius + 1.0f) * 0.5f;
+        rec.prim_id = 0;
+        return true;
+    }
+    bool bounding_box(Aabb& out) const override {
+        Vec3 ext(radius * std::sqrt(rmax(1.0f - normal.x * normal.x, 0.0f)), radius * std::sqrt(rmax(1.0f - normal.y * normal.y, 0.0f)),
+                 radius * std::sqrt(rmax(1.0f - normal.z * normal.z, 0.0f)));
+        out = Aabb(center - ext, center + ext).pad();
+        return true;
+    }
+    float light_area() const override { return material_is_emissive(mat) ? area : 0.0f; }
+    float pdf_value(const Point3& origin, const Vec3& dir, float) const override {
+        Ray r = Ray::make(origin, dir);
+        HitRecord rec;
+        if (!hit(r, 0.001f, INF, rec)) return 0.0f;
+        float dist2 = (rec.p - origin).length_squared();
+        float cos_theta = std::fabs(dir.unit().dot(rec.normal));
+        if (cos_theta < 1e-8f || area <= 0.0f) return 0.0f;
+        return dist2 / (cos_theta * area);
+    }
+    Vec3 pdf_generate(const Point3& origin, SmallRng& rng, float) const override {
+        float rr = radius * std::sqrt(rng.gen());
+        float theta = 2.0f * PI * rng.gen();
+        Point3 p = center + rr * (std::cos(theta) * u_axis + std::sin(theta) * v_axis);
+        return (p - origin).unit();
+    }
+};
+
+// ---- InfinitePlane (src/plane.rs:14-107) -----------------------------------------------
+struct InfinitePlane : Hittable {
+    Point3 point; Vec3 normal; float wave_amplitude = 0.0f, wave_scale = 1.0f;
+    const Material* mat = nullptr;
+    Vec3 u_axis, v_axis;
+    void init(const Point3& p, const Vec3& n, float amp, float sc, const Material* m) {
+        point = p; normal = n.unit(); wave_amplitude = amp; wave_scale = sc; mat = m;
+        vonb(normal, u_axis, v_axis);
+    }
+    bool hit(const Ray& r, float t_min, float t_max, HitRecord& rec) const override {
+        float denom = normal.dot(r.direction);
+        if (!(std::fabs(denom) >= 1e-8f)) return false;
+        float t = normal.dot(point - r.origin) / denom;
+        if (t < t_min || t > t_max) return false;
+        Point3 p = r.at(t);
+        Vec3 off = p - point;
+        Vec3 sn = normal;
+        if (wave_amplitude > 1e-6f) {
+            float u = off.dot(u_axis), v = off.dot(v_axis);
+            float s = wave_scale, a = wave_amplitude;
+            float phi1 = s * u + 0.7f * s * v;
+            float phi2 = 1.3f * s * v - 0.5f * s * u;
+            float dh_du = a * s * std::cos(phi1) - 0.25f * a * s * std::cos(phi2);
+            float dh_dv = 0.7f * a * s * std::cos(phi1) + 0.65f * a * s * std::cos(phi2);
+            sn = (normal - dh_du * u_axis - dh_dv * v_axis).unit();
+        }
+        rec = HitRecord::make(p, t, mat, r, sn);
+        rec.u = rem_euclid(off.dot(u_axis), 1.0f);
+        rec.v = rem_euclid(off.dot(v_axis), 1.0f);
+        rec.prim_id = 0;
+        return true;
+    }
+    bool bounding_box(Aabb& out) const override {
+        const float B = 1e5f, E = 1e-3f;
+        float n[3] = {normal.x, normal.y, normal.z}, p[3] = {point.x, point.y, point.z}, lo[3], hi[3];
+        for (int k = 0; k < 3; k++) {
+            if (std::fabs(n[k]) > 0.999f) { lo[k] = p[k] - E; hi[k] = p[k] + E; } else { lo[k] = -B; hi[k] = B; }
+        }
+        out = Aabb(Point3(lo[0], lo[1], lo[2]), Point3(hi[0], hi[1], hi[2]));
+        return true;
+    }
+};
+
 // ---- Triangle ------------------------------------------------------------------
 struct Triangle : Hittable {
     Point3 v0, v1, v2; Vec3 n0, n1, n2, t0, t1, t2;

@@ -126,6 +126,26 @@ HPtr make_object(OScene& S, const rt_scene_desc& d, const rt_object& o) {
             c->bbox = Aabb(Point3(p[7], p[8], p[9]), Point3(p[10], p[11], p[12]));
             h = c; break;
         }
+        case RT_SHAPE_CYLINDER: {
+            auto c = std::make_shared<Cylinder>();
+            c->center = Point3(p[0], p[1], p[2]); c->radius = p[3]; c->height = p[4]; c->mat = mat;
+            h = c; break;
+        }
+        case RT_SHAPE_CONE: {
+            auto c = std::make_shared<Cone>();
+            c->center = Point3(p[0], p[1], p[2]); c->radius = p[3]; c->height = p[4]; c->mat = mat;
+            h = c; break;
+        }
+        case RT_SHAPE_DISK: {
+            auto dk = std::make_shared<Disk>();
+            dk->init(Point3(p[0], p[1], p[2]), Vec3(p[3], p[4], p[5]), p[6], mat);
+            h = dk; break;
+        }
+        case RT_SHAPE_PLANE: {
+            auto pl = std::make_shared<InfinitePlane>();
+            pl->init(Point3(p[0], p[1], p[2]), Vec3(p[3], p[4], p[5]), p[6], p[7], mat);
+            h = pl; break;
+        }
         default: S.err = "unsupported shape kind " + std::to_string(o.shape); return nullptr;
     }
     if (o.medium) {

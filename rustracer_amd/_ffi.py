@@ -15,7 +15,8 @@ RT_ERR_UNSUPPORTED = -2
 RT_ERR_DEVICE = -3
 RT_ERR_NOMEM = -4
 
-SHAPE_SPHERE, SHAPE_QUAD, SHAPE_MESH, SHAPE_BOX, SHAPE_CONVEX, SHAPE_CYLINDER, SHAPE_DISK = 1, 2, 3, 4, 5, 6, 7
+(SHAPE_SPHERE, SHAPE_QUAD, SHAPE_MESH, SHAPE_BOX, SHAPE_CONVEX, SHAPE_CYLINDER, SHAPE_DISK, SHAPE_CONE,
+ SHAPE_PLANE) = 1, 2, 3, 4, 5, 6, 7, 8, 9
 XFORM_TRANSLATE, XFORM_ROTATE, XFORM_SCALE = 1, 2, 3
 MEDIUM_CONSTANT, MEDIUM_GRID = 1, 2
 (MAT_LAMBERTIAN, MAT_DIELECTRIC, MAT_SPECTRAL_DIELECTRIC, MAT_PBR, MAT_DIFFUSE_LIGHT,

@@ -21,7 +21,8 @@
 
 namespace rt {
 
-enum { SH_SPHERE = 1, SH_QUAD = 2, SH_MESH = 3, SH_BOX = 4, SH_CONVEX = 5 };
+enum { SH_SPHERE = 1, SH_QUAD = 2, SH_MESH = 3, SH_BOX = 4, SH_CONVEX = 5, SH_CYLINDER = 6, SH_DISK = 7, SH_CONE = 8,
+       SH_PLANE = 9 };
 enum { XF_TRANSLATE = 1, XF_ROTATE = 2, XF_SCALE = 3 };
 enum { MED_NONE = 0, MED_CONSTANT = 1, MED_GRID = 2 };
 
@@ -259,6 +260,124 @@ RT_D void push_sorted(Stack& st, uint32_t mask, const float tn[8], const int32_t
     for (int i = 0; i < n; i++) st_push(st, cc[i]);
 }
 
+// ---- cylinder / cone / disk / infinite plane ----------------------------------------------
+// Object-space inline parameters as uploaded: CYLINDER and CONE p0-2 base
+// centre, p3 radius, p4 height; DISK p0-2 centre, p3-5 unit normal, p6 radius,
+// p7-9 / p10-12 the tangent frame of Vec3::onb, p13 area; PLANE p0-2 point,
+// p3-5 unit normal, p6 wave amplitude, p7 wave scale, p8-10 / p11-13 frame.
+// Closed [t_min, t_max]; `kind` names the surface hit (0 lateral / disk /
+// plane, 1 bottom cap or cone base, 2 top cap).
+static constexpr float TAU_F = 6.28318530717958647692f;
+
+// Cylinder::hit (src/cylinder.rs:133-185)
+RT_D bool cylinder_test(const float* p, const Ray& r, float t_min, float t_max, float& t_out, int& kind) {
+    V3 c = mk(p[0], p[1], p[2]);
+    float rad = p[3], hgt = p[4];
+    V3 oc = r.o - c;
+    float dx = r.d.x, dz = r.d.z, dy = r.d.y;
+    float best_t = t_max;
+    kind = -1;
+    float a = dx * dx + dz * dz;
+    if (a > 1e-8f) {
+        float half_b = oc.x * dx + oc.z * dz;
+        float cc = oc.x * oc.x + oc.z * oc.z - rad * rad;
+        float disc = half_b * half_b - a * cc;
+        if (disc >= 0.0f) {
+            float sq = sqrtf(disc);
+            float roots[2] = {(-half_b - sq) / a, (-half_b + sq) / a};
+            for (int k = 0; k < 2; k++) {
+                float root = roots[k];
+                if (root < t_min || root > best_t) continue;
+                float ly = (r.o.y + r.d.y * root) - c.y;
+                if (ly < 0.0f || ly > hgt) continue;
+                best_t = root; kind = 0;
+            }
+        }
+    }
+    if (fabsf(dy) > 1e-8f) {
+        for (int k = 0; k < 2; k++) {
+            float cap_y = k == 0 ? 0.0f : hgt;
+            float t = (cap_y - oc.y) / dy;
+            if (t < t_min || t > best_t) continue;
+            float lx = (r.o.x + r.d.x * t) - c.x, lz = (r.o.z + r.d.z * t) - c.z;
+            if (lx * lx + lz * lz > rad * rad) continue;
+            best_t = t; kind = 1 + k;
+        }
+    }
+    t_out = best_t;
+    return kind >= 0;
+}
+
+// Cone::hit (src/cone.rs:19-82): lateral x^2 + z^2 = k (h - y)^2, base disk at y = 0.
+RT_D bool cone_test(const float* p, const Ray& r, float t_min, float t_max, float& t_out, int& kind) {
+    V3 c = mk(p[0], p[1], p[2]);
+    float rad = p[3], h = p[4];
+    V3 oc = r.o - c;
+    float k = (rad / h) * (rad / h);
+    float dx = r.d.x, dy = r.d.y, dz = r.d.z;
+    float hy = h - oc.y;
+    float best_t = t_max;
+    kind = -1;
+    float a = dx * dx + dz * dz - k * dy * dy;
+    float half_b = oc.x * dx + oc.z * dz + k * hy * dy;
+    float cc = oc.x * oc.x + oc.z * oc.z - k * hy * hy;
+    if (fabsf(a) > 1e-8f) {
+        float disc = half_b * half_b - a * cc;
+        if (disc >= 0.0f) {
+            float sq = sqrtf(disc);
+            float roots[2] = {(-half_b - sq) / a, (-half_b + sq) / a};
+            for (int j = 0; j < 2; j++) {
+                float root = roots[j];
+                if (root < t_min || root > best_t) continue;
+                float ly = (r.o.y + r.d.y * root) - c.y;
+                if (ly < 0.0f || ly > h) continue;
+                best_t = root; kind = 0;
+            }
+        }
+    }
+    if (fabsf(dy) > 1e-8f) {
+        float t = -oc.y / dy;
+        if (t >= t_min && t <= best_t) {
+            float lx = (r.o.x + r.d.x * t) - c.x, lz = (r.o.z + r.d.z * t) - c.z;
+            if (lx * lx + lz * lz <= rad * rad) { best_t = t; kind = 1; }
+        }
+    }
+    t_out = best_t;
+    return kind >= 0;
+}
+
+// Disk::hit (src/disk.rs:147-160)
+RT_D bool disk_test(const float* p, const Ray& r, float t_min, float t_max, float& t) {
+    V3 n = mk(p[3], p[4], p[5]);
+    float denom = dot(n, r.d);
+    if (!(fabsf(denom) >= 1e-8f)) return false;
+    t = dot(n, mk(p[0], p[1], p[2]) - r.o) / denom;
+    if (t < t_min || t > t_max) return false;
+    V3 off = at(r, t) - mk(p[0], p[1], p[2]);
+    return !(len2(off) > p[6] * p[6]);
+}
+
+// InfinitePlane::hit (src/plane.rs:40-46)
+RT_D bool plane_test(const float* p, const Ray& r, float t_min, float t_max, float& t) {
+    V3 n = mk(p[3], p[4], p[5]);
+    float denom = dot(n, r.d);
+    if (!(fabsf(denom) >= 1e-8f)) return false;
+    t = dot(n, mk(p[0], p[1], p[2]) - r.o) / denom;
+    return !(t < t_min || t > t_max);
+}
+
+// Closest-hit test of an analytic shape with inline parameters.
+RT_D bool analytic_test(const DObject& ob, const Ray& r, float t_min, float t_max, float& t, int& kind) {
+    kind = 0;
+    switch (ob.shape) {
+        case SH_CYLINDER: return cylinder_test(ob.p, r, t_min, t_max, t, kind);
+        case SH_CONE: return cone_test(ob.p, r, t_min, t_max, t, kind);
+        case SH_DISK: return disk_test(ob.p, r, t_min, t_max, t);
+        default: return plane_test(ob.p, r, t_min, t_max, t);
+    }
+}
+RT_HD bool is_analytic(uint32_t shape) { return shape >= SH_CYLINDER && shape <= SH_PLANE; }
+
 // ---- media --------------------------------------------------------------------------
 RT_D uint64_t fbits(float f) { return (uint64_t)(uint32_t)__float_as_uint(f); }
 RT_D Rng medium_rng(const Ray& r) {
@@ -406,6 +525,11 @@ RT_D void object_closest(const DScene& S, const DObject& ob, int32_t slot, const
             if (convex_test(S.planes, ob.data_first, ob.data_count, r, t_min, closest, t, idx)) { closest = t; best.t = t; best.slot = slot; best.prim = idx; }
             break;
         }
+        case SH_CYLINDER: case SH_CONE: case SH_DISK: case SH_PLANE: {
+            float t; int kind;
+            if (analytic_test(ob, r, t_min, closest, t, kind)) { closest = t; best.t = t; best.slot = slot; best.prim = kind; }
+            break;
+        }
         default: {
             int32_t prim = -1; bool any = false;
             nested_closest(S, ob, r, t_min, closest, prim, any, st);
@@ -472,6 +596,7 @@ RT_D float object_transmittance(const DScene& S, const DObject& ob, const Ray& w
             case SH_SPHERE: { float t; h = sphere_test(ob.p, r, t_min, t_max, t); break; }
             case SH_QUAD: { float t, a, b; h = quad_test(S.quads[ob.data_first], r, t_min, t_max, t, a, b); break; }
             case SH_CONVEX: { float t; int idx; h = convex_test(S.planes, ob.data_first, ob.data_count, r, t_min, t_max, t, idx); break; }
+            case SH_CYLINDER: case SH_CONE: case SH_DISK: case SH_PLANE: { float t; int kind; h = analytic_test(ob, r, t_min, t_max, t, kind); break; }
             default: h = nested_any(S, ob, r, t_min, t_max, st);
         }
         tr = h ? 0.0f : 1.0f;
@@ -592,6 +717,65 @@ RT_D Rec object_record(const DScene& S, const DObject& ob, const Ray& r, const H
             h.u = u; h.v = v;
             break;
         }
+        case SH_CYLINDER: case SH_CONE: {
+            // Cylinder / Cone records (src/cylinder.rs:154-183, src/cone.rs:44-78)
+            V3 c = mk(ob.p[0], ob.p[1], ob.p[2]);
+            float rad = ob.p[3], hgt = ob.p[4];
+            V3 p = at(r, id.t);
+            float lx = p.x - c.x, lz = p.z - c.z;
+            if (id.prim == 0) {
+                float ly = p.y - c.y;
+                V3 outward;
+                if (ob.shape == SH_CYLINDER) outward = mk((p.x - c.x) / rad, 0.0f, (p.z - c.z) / rad);
+                else {
+                    float k = (rad / hgt) * (rad / hgt);
+                    V3 grad = mk(lx, k * (hgt - ly), lz);
+                    outward = len2(grad) > 1e-12f ? unit(grad) : mk(0.0f, 1.0f, 0.0f);
+                }
+                h = make_rec(p, id.t, r, outward);
+                float phi = atan2f(lz, lx);
+                float uu = phi / TAU_F + 0.5f;
+                h.u = uu - truncf(uu);
+                h.v = ly / hgt;
+            } else {
+                float ny = (id.prim == 1) ? -1.0f : 1.0f;
+                h = make_rec(p, id.t, r, mk(0.0f, ny, 0.0f));
+                h.u = (lx / rad + 1.0f) * 0.5f;
+                h.v = (lz / rad + 1.0f) * 0.5f;
+            }
+            break;
+        }
+        case SH_DISK: {
+            // Disk::hit record (src/disk.rs:154-158)
+            V3 p = at(r, id.t);
+            V3 off = p - mk(ob.p[0], ob.p[1], ob.p[2]);
+            h = make_rec(p, id.t, r, mk(ob.p[3], ob.p[4], ob.p[5]));
+            h.u = (dot(off, mk(ob.p[7], ob.p[8], ob.p[9])) / ob.p[6] + 1.0f) * 0.5f;
+            h.v = (dot(off, mk(ob.p[10], ob.p[11], ob.p[12])) / ob.p[6] + 1.0f) * 0.5f;
+            break;
+        }
+        case SH_PLANE: {
+            // InfinitePlane::hit record with the optional ripple normal (src/plane.rs:47-70)
+            V3 n = mk(ob.p[3], ob.p[4], ob.p[5]);
+            V3 ua = mk(ob.p[8], ob.p[9], ob.p[10]), va = mk(ob.p[11], ob.p[12], ob.p[13]);
+            V3 p = at(r, id.t);
+            V3 off = p - mk(ob.p[0], ob.p[1], ob.p[2]);
+            V3 sn = n;
+            float amp = ob.p[6];
+            if (amp > 1e-6f) {
+                float u = dot(off, ua), v = dot(off, va);
+                float sc = ob.p[7];
+                float phi1 = sc * u + 0.7f * sc * v;
+                float phi2 = 1.3f * sc * v - 0.5f * sc * u;
+                float dh_du = amp * sc * cosf(phi1) - 0.25f * amp * sc * cosf(phi2);
+                float dh_dv = 0.7f * amp * sc * cosf(phi1) + 0.65f * amp * sc * cosf(phi2);
+                sn = unit((n - dh_du * ua) - dh_dv * va);
+            }
+            h = make_rec(p, id.t, r, sn);
+            h.u = rem1(dot(off, ua));
+            h.v = rem1(dot(off, va));
+            break;
+        }
         default: {  // mesh triangle
             uint32_t i = ob.data_first + (uint32_t)id.prim;
             float t, b1, b2;
@@ -669,6 +853,7 @@ RT_D bool light_hit(const DScene& S, const DObject& ob, const Ray& wr, float t_m
         case SH_SPHERE: if (!sphere_test(ob.p, r, t_min, t_max, t)) return false; break;
         case SH_QUAD: { float a, b; if (!quad_test(S.quads[ob.data_first], r, t_min, t_max, t, a, b)) return false; break; }
         case SH_CONVEX: { int idx; if (!convex_test(S.planes, ob.data_first, ob.data_count, r, t_min, t_max, t, idx)) return false; id.prim = idx; break; }
+        case SH_DISK: if (!disk_test(ob.p, r, t_min, t_max, t)) return false; break;
         default: return false;
     }
     if (want_rec) {

@@ -673,6 +673,15 @@ RT_D float light_pdf_value(const DScene& S, const DObject& ob, V3 origin, V3 dir
         float ctm = sqrtf(1.0f - r2 / d2);
         return 1.0f / (2.0f * PI * (1.0f - ctm));
     }
+    if (ob.shape == SH_DISK) {
+        // Disk::pdf_value (src/disk.rs:130-137)
+        float t;
+        if (!disk_test(ob.p, r, 0.001f, FINF, t)) return 0.0f;
+        float dist2 = len2(at(r, t) - o);
+        float ct = fabsf(dot(unit(d), mk(ob.p[3], ob.p[4], ob.p[5])));
+        if (ct < 1e-8f || ob.p[13] <= 0.0f) return 0.0f;
+        return dist2 / (ct * ob.p[13]);
+    }
     return 0.0f;
 }
 RT_D float lights_pdf_value(const DScene& S, V3 origin, V3 dir, float time, Stack& st) {
@@ -706,6 +715,12 @@ RT_D V3 light_generate(const DScene& S, const DObject& ob, V3 origin, Rng& rng, 
             float st = sqrtf(fmx(1.0f - ct * ct, 0.0f));
             d = onb_local(onb_from_w(unit(tc)), mk(cosf(phi) * st, sinf(phi) * st, ct));
         }
+    } else if (ob.shape == SH_DISK) {
+        // Disk::pdf_generate (src/disk.rs:139-145): r = R sqrt(u), theta = 2 pi v
+        float rr = ob.p[6] * sqrtf(rng_f(rng));
+        float th = 2.0f * PI * rng_f(rng);
+        V3 pt = mk(ob.p[0], ob.p[1], ob.p[2]) + rr * (cosf(th) * mk(ob.p[7], ob.p[8], ob.p[9]) + sinf(th) * mk(ob.p[10], ob.p[11], ob.p[12]));
+        d = unit(pt - o);
     } else d = mk(1, 0, 0);
     for (uint32_t k = 0; k < ob.xf_count; k++) {
         const DXform& x = S.xforms[ob.xf_first + k];

@@ -1046,6 +1046,56 @@ bool flatten_object(Flat& F, const rt_scene_desc& d, const rt_object& o, DObject
             for (int k = 0; k < 3; k++) { bb.mn[k] = o.p[7 + k]; bb.mx[k] = o.p[10 + k]; }
             break;
         }
+        case RT_SHAPE_CYLINDER: case RT_SHAPE_CONE: {
+            if (!(o.p[3] > 0.0f) || !(o.p[4] > 0.0f)) { F.err = "cylinder / cone radius and height must be positive"; return false; }
+            if (is_light) { F.err = "cylinder / cone lights are not supported (no pdf in the reference)"; return false; }
+            // bounding_box (src/cylinder.rs:217-223, src/cone.rs:124-129)
+            for (int k = 0; k < 3; k++) {
+                float rk = k == 1 ? 0.0f : o.p[3];
+                bb.mn[k] = o.p[k] - rk;
+                bb.mx[k] = (o.p[k] + rk) + (k == 1 ? o.p[4] : 0.0f);
+            }
+            break;
+        }
+        case RT_SHAPE_DISK: case RT_SHAPE_PLANE: {
+            float nx = o.p[3], ny = o.p[4], nz = o.p[5];
+            float l2 = nx * nx + ny * ny + nz * nz;
+            if (!(l2 > 1e-12f)) { F.err = "disk / plane normal must be nonzero"; return false; }
+            // Vec3::unit and Vec3::onb (src/vec3.rs:30-42)
+            float l = std::sqrt(l2);
+            V3 n = mk(nx / l, ny / l, nz / l);
+            V3 up = std::fabs(n.y) < 0.9f ? mk(0.0f, 1.0f, 0.0f) : mk(1.0f, 0.0f, 0.0f);
+            V3 c = cross(n, up);
+            float cl = std::sqrt(len2(c));
+            V3 u = mk(c.x / cl, c.y / cl, c.z / cl);
+            V3 v = cross(n, u);
+            D.p[3] = n.x; D.p[4] = n.y; D.p[5] = n.z;
+            if (o.shape == RT_SHAPE_DISK) {
+                if (!(o.p[6] > 0.0f)) { F.err = "disk radius must be positive"; return false; }
+                float rad = o.p[6];
+                D.p[7] = u.x; D.p[8] = u.y; D.p[9] = u.z; D.p[10] = v.x; D.p[11] = v.y; D.p[12] = v.z;
+                D.p[13] = PI * rad * rad;
+                // bounding_box: extent R sqrt(1 - n_i^2), padded (src/disk.rs:172-181)
+                float nn[3] = {n.x, n.y, n.z};
+                Box b;
+                for (int k = 0; k < 3; k++) {
+                    float e = rad * std::sqrt(std::fmax(1.0f - nn[k] * nn[k], 0.0f));
+                    b.mn[k] = o.p[k] - e; b.mx[k] = o.p[k] + e;
+                }
+                bb = rtb::box_pad(b);
+            } else {
+                if (is_light) { F.err = "plane lights are not supported"; return false; }
+                D.p[6] = o.p[6]; D.p[7] = o.p[7];
+                D.p[8] = u.x; D.p[9] = u.y; D.p[10] = u.z; D.p[11] = v.x; D.p[12] = v.y; D.p[13] = v.z;
+                // bounding_box (src/plane.rs:86-98): thin slab across the normal axis, 1e5 elsewhere
+                float nn[3] = {n.x, n.y, n.z};
+                for (int k = 0; k < 3; k++) {
+                    if (std::fabs(nn[k]) > 0.999f) { bb.mn[k] = o.p[k] - 1e-3f; bb.mx[k] = o.p[k] + 1e-3f; }
+                    else { bb.mn[k] = -1e5f; bb.mx[k] = 1e5f; }
+                }
+            }
+            break;
+        }
         default:
             F.err = "unsupported shape kind " + std::to_string(o.shape);
             return false;

@@ -272,6 +272,61 @@ class Quad(Hittable):
         return list(self.q) + list(self.u) + list(self.v)
 
 
+class Cylinder(Hittable):
+    """Cylinder (src/cylinder.rs:124-130): y axis, closed caps, `center` the
+    centre of the bottom cap."""
+    shape = F.SHAPE_CYLINDER
+
+    def __init__(self, center, radius, height, mat):
+        super().__init__(mat)
+        self.center, self.radius, self.height = v3(center), float(radius), float(height)
+
+    def inline(self):
+        return list(self.center) + [self.radius, self.height]
+
+
+class Cone(Hittable):
+    """Cone (src/cone.rs:11-16): base disk at `center`, apex `height` above."""
+    shape = F.SHAPE_CONE
+
+    def __init__(self, center, radius, height, mat):
+        super().__init__(mat)
+        self.center, self.radius, self.height = v3(center), float(radius), float(height)
+
+    def inline(self):
+        return list(self.center) + [self.radius, self.height]
+
+
+class Disk(Hittable):
+    """Disk::new (src/disk.rs:115-124); the normal is normalised on upload."""
+    shape = F.SHAPE_DISK
+
+    def __init__(self, center, normal, radius, mat):
+        super().__init__(mat)
+        self.center, self.normal, self.radius = v3(center), v3(normal), float(radius)
+        if not float(np.dot(self.normal, self.normal)) > 1e-12:
+            raise ValueError("Disk::new: zero normal would give NaN hits")
+
+    def inline(self):
+        return list(self.center) + list(self.normal) + [self.radius]
+
+
+class InfinitePlane(Hittable):
+    """InfinitePlane::new (src/plane.rs:24-37): point, normal (normalised on
+    upload), optional ripple (wave amplitude, wave scale)."""
+    shape = F.SHAPE_PLANE
+
+    def __init__(self, point, normal, mat, wave_amplitude=0.0, wave_scale=1.0):
+        super().__init__(mat)
+        self.point, self.normal = v3(point), v3(normal)
+        self.wave_amplitude, self.wave_scale = float(wave_amplitude), float(wave_scale)
+        if not float(np.dot(self.normal, self.normal)) > 1e-12:
+            raise ValueError("InfinitePlane::new: zero normal would give NaN hits")
+
+    def inline(self):
+        return list(self.point) + list(self.normal) + [self.wave_amplitude, self.wave_scale]
+
+
 class Box(Hittable):
     """make_box (src/quad.rs:201-215): six quads in a nested BVH."""
     shape = F.SHAPE_BOX

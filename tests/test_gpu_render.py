@@ -273,3 +273,27 @@ def test_batch_rejects_bad_views(gpu_renderer):
         r.render_batch([dict(scene=r.scene, camera=cam, width=16, height=16, spp=0, out=out)])
     with pytest.raises(RtError):
         r.render_batch([dict(scene=r.scene, camera=cam, width=16, height=16, spp=2, out=out, photon_map=True)])
+
+
+def test_disk_light_and_analytic_shapes_same_streams_and_three_sigma(gpu_renderer):
+    """NEE through a disk light (Disk::pdf_value / pdf_generate) over a floor,
+    a cylinder, a cone and a rippled plane."""
+    from rustracer_amd import scene as S
+    light = S.DiffuseLight((6.0, 6.0, 6.0))
+    w = S.Lambertian((0.7, 0.6, 0.5))
+    objs = [S.InfinitePlane((0, 0, 0), (0, 1, 0), S.Lambertian((0.8, 0.8, 0.8)), wave_amplitude=0.03, wave_scale=5.0),
+            S.Cylinder((-0.5, 0, 0.2), 0.25, 0.6, w), S.Cone((0.4, 0, 0.1), 0.3, 0.7, S.Metal((0.9, 0.8, 0.7), 0.2)),
+            S.Disk((0, 1.5, 0), (0, -1, 0), 0.4, light)]
+    sc = S.Scene(objs, [S.Disk((0, 1.5, 0), (0, -1, 0), 0.4, light)], S.Background.solid((0.05, 0.05, 0.08)))
+    cam = S.Camera((0.0, 0.9, -2.5), (0.0, 0.3, 0.0), (0, 1, 0), 40.0, 1.0)
+    gpu_renderer.upload(sc)
+    orc = O.OracleScene(sc)
+    g = passes_gpu(gpu_renderer, cam, 40, 40, 4, 8)
+    c = passes_cpu(orc, cam, 40, 40, 4, 8, mode=1)
+    close = np.isclose(g, c, rtol=1e-3, atol=1e-5).all(axis=-1)
+    assert close.mean() > 0.95, close.mean()
+    K = 96
+    g = passes_gpu(gpu_renderer, cam, 32, 32, K, 8, seed=2)
+    c = passes_cpu(orc, cam, 32, 32, K, 8, mode=0)
+    z = three_sigma(g, c)
+    assert (z > 3).mean() <= 0.02 and (z > 6).mean() <= 0.002, ((z > 3).mean(), (z > 6).mean())

@@ -77,3 +77,30 @@ def test_empty_batch(gpu_renderer):
     scene, _, _ = scenes.box_room()
     gpu_renderer.upload(scene)
     assert len(gpu_renderer.trace_closest(random_rays(0, 0))) == 0
+
+
+def analytic_scene():
+    """Cylinders, cones, disks and infinite planes (one rippled), bare and
+    under scale / rotation / translation."""
+    w = S.Lambertian((0.7, 0.7, 0.7))
+    objs = [
+        S.InfinitePlane((0, -1.2, 0), (0, 1, 0), w),
+        S.InfinitePlane((0, 0, 3.0), (0.1, 0.2, -1.0), w, wave_amplitude=0.05, wave_scale=4.0),
+        S.Cylinder((0.5, -1.0, 0.2), 0.3, 0.9, w),
+        S.Cylinder((0, 0, 0), 0.2, 0.6, w).rotated((30, 0, 20)).translated((-0.8, 0.2, -0.3)),
+        S.Cone((-0.3, -1.0, 0.8), 0.35, 0.8, w),
+        S.Cone((0, 0, 0), 0.25, 0.5, w).scaled(1.3).rotated((0, 0, 90)).translated((0.9, 0.6, 0.9)),
+        S.Disk((0, 1.3, 0), (0, -1, 0), 0.4, S.DiffuseLight((4, 4, 4))),
+        S.Disk((0, 0, 0), (1, 1, 0), 0.3, w).translated((-0.6, -0.5, 1.0)),
+    ]
+    return S.Scene(objs, [], S.Background.solid((0, 0, 0)), name="analytic")
+
+
+def test_cylinder_cone_disk_plane_bit_exact(gpu_renderer):
+    hg = run_scene(gpu_renderer, analytic_scene(), random_rays(200000, 12, lo=(-2, -1.1, -2), hi=(2, 2, 2)))
+    ids = set(np.unique(hg["obj_id"]))
+    assert set(range(8)) <= ids, ids
+    # lateral surfaces and caps of the capped shapes are both hit
+    for obj in (2, 4):
+        ny = np.abs(hg["n"][hg["obj_id"] == obj, 1])
+        assert (ny > 0.99).any() and (ny < 0.9).any()
