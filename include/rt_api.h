@@ -98,18 +98,26 @@ typedef struct {
  * ignored).  medium = index + 1 into rt_scene_desc.media.
  *   CONSTANT  ConstantMedium (src/volume.rs:78-156)
  *   GRID      delta tracking through a dense float density grid spanning the
- *             boundary's bounding box (extension: the reference evaluates
- *             Perlin turbulence per lookup, src/volume.rs:160-263)
+ *             boundary's bounding box (extension of NOISE: a table lookup in
+ *             place of the per-lookup turbulence)
+ *   NOISE     NoiseMedium (src/volume.rs:160-263): delta tracking with
+ *             density max(turb(p * noise_scale, 7) / 2 - threshold, 0) * density
+ *             (Perlin, src/perlin.rs:35-72); the Perlin tables are given (the
+ *             reference draws them from an unseeded thread_rng)
  */
-enum { RT_MEDIUM_CONSTANT = 1, RT_MEDIUM_GRID = 2 };
+enum { RT_MEDIUM_CONSTANT = 1, RT_MEDIUM_GRID = 2, RT_MEDIUM_NOISE = 3 };
 
 typedef struct {
     uint32_t     kind;
-    float        density;      /* sigma_t (CONSTANT) or majorant scale (GRID) */
+    float        density;      /* sigma_t (CONSTANT) or majorant scale (GRID, NOISE) */
     float        albedo[3];
     float        g;            /* Henyey-Greenstein asymmetry                 */
     uint32_t     grid_n[3];    /* GRID only                                   */
     const float *grid;         /* GRID only: grid_n[0]*grid_n[1]*grid_n[2], x fastest, values in [0,1] */
+    float        noise_scale;  /* NOISE: feature frequency                    */
+    float        threshold;    /* NOISE: clipped to [0, 0.99]                 */
+    const float *perlin;       /* NOISE: 1536 floats: ranvec[256][3] (unit vectors),
+                                  then perm_x, perm_y, perm_z (256 each, integers 0..255) */
 } rt_medium;
 
 typedef struct {

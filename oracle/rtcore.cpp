@@ -164,6 +164,19 @@ HPtr make_object(OScene& S, const rt_scene_desc& d, const rt_object& o) {
             gm->grid.assign(m.grid, m.grid + (size_t)gm->nx * gm->ny * gm->nz);
             h->bounding_box(gm->box);
             h = gm;
+        } else if (m.kind == RT_MEDIUM_NOISE) {
+            if (!m.perlin) { S.err = "noise medium without Perlin tables"; return nullptr; }
+            auto nm = std::make_shared<NoiseMedium>();
+            nm->boundary = h; nm->density_scale = rmax(m.density, FLT_EPSILON);
+            nm->phase.albedo = Color(m.albedo[0], m.albedo[1], m.albedo[2]); nm->phase.g = m.g;
+            nm->noise_scale = m.noise_scale;
+            nm->threshold = rclamp(m.threshold, 0.0f, 0.99f);
+            nm->ranvec.assign(m.perlin, m.perlin + 768);
+            for (int k = 0; k < 256; k++) {
+                nm->perm_x[k] = (int)m.perlin[768 + k]; nm->perm_y[k] = (int)m.perlin[1024 + k]; nm->perm_z[k] = (int)m.perlin[1280 + k];
+            }
+            h->bounding_box(nm->box);
+            h = nm;
         } else { S.err = "medium kind"; return nullptr; }
     }
     for (uint32_t k = 0; k < o.xform_count; k++) {

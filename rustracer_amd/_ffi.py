@@ -18,7 +18,7 @@ RT_ERR_NOMEM = -4
 (SHAPE_SPHERE, SHAPE_QUAD, SHAPE_MESH, SHAPE_BOX, SHAPE_CONVEX, SHAPE_CYLINDER, SHAPE_DISK, SHAPE_CONE,
  SHAPE_PLANE) = 1, 2, 3, 4, 5, 6, 7, 8, 9
 XFORM_TRANSLATE, XFORM_ROTATE, XFORM_SCALE = 1, 2, 3
-MEDIUM_CONSTANT, MEDIUM_GRID = 1, 2
+MEDIUM_CONSTANT, MEDIUM_GRID, MEDIUM_NOISE = 1, 2, 3
 (MAT_LAMBERTIAN, MAT_DIELECTRIC, MAT_SPECTRAL_DIELECTRIC, MAT_PBR, MAT_DIFFUSE_LIGHT,
  MAT_SPECTRAL_METAL, MAT_ROUGH_DIELECTRIC, MAT_ROUGH_SPECTRAL) = 1, 2, 3, 4, 5, 6, 7, 8
 BG_SOLID, BG_GRADIENT, BG_PHYSICAL, BG_ENVMAP, BG_ENVMAP_SPLIT = 0, 1, 2, 3, 4
@@ -33,7 +33,8 @@ class rt_xform(C.Structure):
 
 class rt_medium(C.Structure):
     _fields_ = [("kind", C.c_uint32), ("density", C.c_float), ("albedo", C.c_float * 3),
-                ("g", C.c_float), ("grid_n", C.c_uint32 * 3), ("grid", F32P)]
+                ("g", C.c_float), ("grid_n", C.c_uint32 * 3), ("grid", F32P), ("noise_scale", C.c_float),
+                ("threshold", C.c_float), ("perlin", F32P)]
 
 
 class rt_object(C.Structure):

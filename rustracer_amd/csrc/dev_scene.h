@@ -24,7 +24,7 @@ namespace rt {
 enum { SH_SPHERE = 1, SH_QUAD = 2, SH_MESH = 3, SH_BOX = 4, SH_CONVEX = 5, SH_CYLINDER = 6, SH_DISK = 7, SH_CONE = 8,
        SH_PLANE = 9 };
 enum { XF_TRANSLATE = 1, XF_ROTATE = 2, XF_SCALE = 3 };
-enum { MED_NONE = 0, MED_CONSTANT = 1, MED_GRID = 2 };
+enum { MED_NONE = 0, MED_CONSTANT = 1, MED_GRID = 2, MED_NOISE = 3 };
 
 struct DXform { uint32_t kind; float v[9]; };
 
@@ -43,8 +43,9 @@ struct DQuad { float q[3], u[3], v[3], n[3], w[3], d, area; float pad[3]; };
 
 struct DMedium {
     uint32_t kind; float density; float albedo[3]; float g;
-    uint32_t nx, ny, nz; uint32_t grid_off;
+    uint32_t nx, ny, nz; uint32_t grid_off;     // GRID: dims and offset; NOISE: offset of the Perlin tables
     float bmin[3], bmax[3];
+    float noise_scale, threshold;               // NOISE
 };
 
 struct DMaterial { uint32_t kind, tex; float p[24]; };
@@ -400,6 +401,45 @@ RT_D bool boundary_t(const DScene& S, const DObject& ob, const Ray& r, float t_m
     return found;
 }
 
+// Perlin::noise / turb (src/perlin.rs:35-72) over tables at S.grids + off:
+// ranvec[256][3], then perm_x, perm_y, perm_z stored as exact floats.
+RT_D float perlin_noise(const float* tab, V3 p) {
+    float fx = floorf(p.x), fy = floorf(p.y), fz = floorf(p.z);
+    float u = p.x - fx, v = p.y - fy, w = p.z - fz;
+    u = u * u * (3.0f - 2.0f * u);
+    v = v * v * (3.0f - 2.0f * v);
+    w = w * w * (3.0f - 2.0f * w);
+    uint32_t i = (uint32_t)f2i(fx), j = (uint32_t)f2i(fy), k = (uint32_t)f2i(fz);
+    const float* px = tab + 768; const float* py = tab + 1024; const float* pz = tab + 1280;
+    float accum = 0.0f;
+    for (uint32_t di = 0; di < 2; di++)
+        for (uint32_t dj = 0; dj < 2; dj++)
+            for (uint32_t dk = 0; dk < 2; dk++) {
+                uint32_t idx = (uint32_t)px[(i + di) & 255u] ^ (uint32_t)py[(j + dj) & 255u] ^ (uint32_t)pz[(k + dk) & 255u];
+                V3 c = mk(tab[3 * idx], tab[3 * idx + 1], tab[3 * idx + 2]);
+                float fi = (float)di, fj = (float)dj, fk = (float)dk;
+                V3 wt = mk(u - fi, v - fj, w - fk);
+                accum += (fi * u + (1.0f - fi) * (1.0f - u)) * (fj * v + (1.0f - fj) * (1.0f - v)) *
+                         (fk * w + (1.0f - fk) * (1.0f - w)) * dot(c, wt);
+            }
+    return accum;
+}
+RT_D float perlin_turb(const float* tab, V3 p, int depth) {
+    float accum = 0.0f, weight = 1.0f;
+    V3 q = p;
+    for (int d = 0; d < depth; d++) {
+        accum += weight * perlin_noise(tab, q);
+        weight *= 0.5f;
+        q = mk(q.x * 2.0f, q.y * 2.0f, q.z * 2.0f);
+    }
+    return fabsf(accum);
+}
+// NoiseMedium::density_at (src/volume.rs:190-193)
+RT_D float noise_density(const DScene& S, const DMedium& m, V3 p) {
+    float n = perlin_turb(S.grids + m.grid_off, p * m.noise_scale, 7) * 0.5f;
+    return fmx(n - m.threshold, 0.0f) * m.density;
+}
+
 RT_D float grid_density(const DScene& S, const DMedium& m, V3 p) {
     float fx = (p.x - m.bmin[0]) / (m.bmax[0] - m.bmin[0]) * (float)m.nx - 0.5f;
     float fy = (p.y - m.bmin[1]) / (m.bmax[1] - m.bmin[1]) * (float)m.ny - 0.5f;
@@ -445,7 +485,7 @@ RT_D bool medium_hit(const DScene& S, const DObject& ob, const Ray& r, float t_m
     for (int it = 0; it < 1 << 20; it++) {
         t -= inv_maj * logf(fmx(rng_f(rng), 1.1920929e-7f));
         if (t >= t2) return false;
-        float ld = grid_density(S, m, at(r, t));
+        float ld = m.kind == MED_NOISE ? noise_density(S, m, at(r, t)) : grid_density(S, m, at(r, t));
         if (rng_f(rng) < ld / m.density) { t_out = t; return true; }
     }
     return false;
@@ -466,7 +506,7 @@ RT_D float medium_transmittance(const DScene& S, const DObject& ob, const Ray& r
     float sum = 0.0f;
     for (int i = 0; i < K; i++) {
         float t = t1 + seg_t * ((float)i + 0.5f) / (float)K;
-        sum += grid_density(S, m, at(r, t));
+        sum += m.kind == MED_NOISE ? noise_density(S, m, at(r, t)) : grid_density(S, m, at(r, t));
     }
     return expf(-sum / (float)K * l);
 }

@@ -1116,6 +1116,18 @@ bool flatten_object(Flat& F, const rt_scene_desc& d, const rt_object& o, DObject
             M.nx = m.grid_n[0]; M.ny = m.grid_n[1]; M.nz = m.grid_n[2];
             M.grid_off = (uint32_t)F.grids.size();
             F.grids.insert(F.grids.end(), m.grid, m.grid + (size_t)M.nx * M.ny * M.nz);
+        } else if (m.kind == RT_MEDIUM_NOISE) {
+            if (!m.perlin) { F.err = "noise medium without Perlin tables"; return false; }
+            for (int k = 768; k < 1536; k++)
+                if (!(m.perlin[k] >= 0.0f && m.perlin[k] <= 255.0f && m.perlin[k] == std::floor(m.perlin[k]))) {
+                    F.err = "noise medium permutation entries must be integers in 0..255"; return false;
+                }
+            // NoiseMedium::new (src/volume.rs:170-188)
+            M.density = std::fmax(m.density, 1.1920929e-7f);
+            M.threshold = std::fmin(std::fmax(m.threshold, 0.0f), 0.99f);
+            M.noise_scale = m.noise_scale;
+            M.grid_off = (uint32_t)F.grids.size();
+            F.grids.insert(F.grids.end(), m.perlin, m.perlin + 1536);
         } else { F.err = "medium kind"; return false; }
         F.media.push_back(M);
         D.medium = (uint32_t)F.media.size();

@@ -487,6 +487,28 @@ class GridMedium:
         boundary.medium_grid = grid
 
 
+class NoiseMedium:
+    """NoiseMedium::new (src/volume.rs:170-188): delta tracking with density
+    max(turb(p * noise_scale, 7) / 2 - threshold, 0) * density.  The reference
+    draws its Perlin tables (256 unit vectors, three permutations) from an
+    unseeded thread_rng; here they come from `seed` (numpy), so a scene is
+    reproducible and the checker sees the same tables."""
+
+    def __init__(self, boundary, color, density, noise_scale=1.0, threshold=0.0, g=0.0, seed=0):
+        self.boundary = boundary
+        rng = np.random.default_rng(seed)
+        v = rng.uniform(-1.0, 1.0, size=(256, 3)).astype(np.float32)
+        v = v / np.sqrt((v * v).sum(1, keepdims=True))
+        perms = [rng.permutation(256).astype(np.float32) for _ in range(3)]
+        boundary.medium = F.rt_medium(kind=F.MEDIUM_NOISE, density=float(density), g=float(g),
+                                      noise_scale=float(noise_scale), threshold=float(threshold))
+        for k in range(3):
+            boundary.medium.albedo[k] = float(v3(color)[k])
+        boundary.medium_grid = None
+        boundary.medium_perlin = np.ascontiguousarray(np.concatenate([v.astype(np.float32).ravel()] + perms),
+                                                      dtype=np.float32)
+
+
 # --------------------------------------------------------------- background
 class Background:
     """Background::Solid / Gradient / Physical / EnvMap / EnvMapSplit."""
@@ -645,6 +667,10 @@ class Scene:
                 if g is not None:
                     grids.append(g)
                     m.grid = g.ctypes.data_as(F.F32P)
+                pt = getattr(o, "medium_perlin", None)
+                if pt is not None:
+                    grids.append(pt)
+                    m.perlin = pt.ctypes.data_as(F.F32P)
                 media.append(m)
                 r.medium = len(media)
             return r

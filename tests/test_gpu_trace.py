@@ -104,3 +104,23 @@ def test_cylinder_cone_disk_plane_bit_exact(gpu_renderer):
     for obj in (2, 4):
         ny = np.abs(hg["n"][hg["obj_id"] == obj, 1])
         assert (ny > 0.99).any() and (ny < 0.9).any()
+
+
+def test_noise_medium_events_and_transmittance(gpu_renderer):
+    """NoiseMedium: the tracking stream is seeded from the ray's bits, so the
+    GPU and the checker draw the same free paths; an event's t differs only by
+    the ulps of log, which can move a rare accept / reject decision."""
+    b = S.Sphere((0, 0.2, 0), 0.9, None)
+    S.NoiseMedium(b, (0.9, 0.9, 0.9), 5.0, noise_scale=2.5, threshold=0.05, g=0.3, seed=3)
+    w = S.Lambertian((0.6, 0.6, 0.6))
+    scene = S.Scene([S.Quad((-3, -1, -3), (6, 0, 0), (0, 0, 6), w), b], [])
+    rays = random_rays(100000, 21, lo=(-1.5, -0.9, -1.5), hi=(1.5, 1.5, 1.5))
+    gpu_renderer.upload(scene)
+    orc = O.OracleScene(scene)
+    hg, hc = gpu_renderer.trace_closest(rays), orc.trace_closest(rays)
+    same = hg["obj_id"] == hc["obj_id"]
+    assert same.mean() > 0.995, same.mean()
+    both = same & (hc["obj_id"] == 1)
+    assert both.sum() > 2000
+    assert np.allclose(hg["t"][both], hc["t"][both], rtol=1e-3, atol=1e-4)
+    assert np.allclose(gpu_renderer.transmittance(rays), orc.transmittance(rays), rtol=1e-4, atol=1e-6)

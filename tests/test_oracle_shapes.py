@@ -170,3 +170,37 @@ def test_plane_distance_closed_form_and_ripple_normal():
         h = shoot(w, o, (0, -1, 0))
         n = np.array(h["n"])
         assert abs(np.linalg.norm(n) - 1) < 1e-5 and n[1] > 0.7 and abs(h["t"] - 1.0) < 1e-6
+
+
+# ---------------------------------------------------------------- NoiseMedium (src/volume.rs:160-263)
+def noise_sphere(threshold=0.1, density=4.0):
+    b = S.Sphere((0, 0, 0), 1.0, None)
+    S.NoiseMedium(b, (0.9, 0.9, 0.9), density, noise_scale=2.0, threshold=threshold, g=0.3, seed=5)
+    return O.OracleScene(S.Scene([b], []))
+
+
+def test_noise_medium_tracking_matches_quadrature_transmittance():
+    """Delta tracking passes a ray without an event with probability
+    exp(-optical depth); the shadow-ray quadrature estimates that depth.  Over
+    many rays the pass fraction equals the mean quadrature transmittance."""
+    orc = noise_sphere()
+    g = np.random.default_rng(7)
+    n = 40000
+    o = np.c_[g.uniform(-0.8, 0.8, n), g.uniform(-0.8, 0.8, n), np.full(n, -3.0)].astype(np.float32)
+    d = np.tile(np.array([[0, 0, 1]], np.float32), (n, 1))
+    r = rays_from(o, d, 0.001, np.inf)
+    passed = (orc.trace_closest(r)["obj_id"] < 0).mean()
+    tr = orc.transmittance(r).mean()
+    se = math.sqrt(tr * (1 - tr) / n)
+    assert 0.05 < tr < 0.95
+    assert abs(passed - tr) < 4 * se + 0.01, (passed, tr)
+
+
+def test_noise_medium_above_threshold_is_empty():
+    orc = noise_sphere(threshold=0.99)
+    g = np.random.default_rng(8)
+    n = 2000
+    o = np.c_[g.uniform(-0.8, 0.8, n), g.uniform(-0.8, 0.8, n), np.full(n, -3.0)].astype(np.float32)
+    r = rays_from(o, np.tile(np.array([[0, 0, 1]], np.float32), (n, 1)), 0.001, np.inf)
+    assert (orc.trace_closest(r)["obj_id"] < 0).all()
+    assert np.all(orc.transmittance(r) == 1.0)
