@@ -12,7 +12,9 @@ icosphere in rough GGX glass under a 2048x1024 HDR environment, 1280x720,
 2M photons), the uneven batch (36 gem spin stills of mixed resolution plus
 a heterogeneous-volume glass box, one launch) and the adjoint (gradient
 evaluations/s of a 20-parameter L2 image loss on a 512x512 rough gem, with a
-finite-difference check) are measured after it and reported under "also".
+finite-difference check) are measured after it and reported under "also",
+with the RMSE of 64-spp gem frames against a 4096-spp host image ("at fixed
+RMSE": GPU and host estimators agree, so samples/s compare at equal spp).
 
 Beside the throughput, on the same line:
   roofline     — algorithmic HBM bytes of the closest-hit traversal kernel
@@ -443,6 +445,80 @@ def cpu_adjoint_baseline(scene, cam, cfg, seconds):
                       f"per evaluation, {el:.1f}s"}
 
 
+def measure_rmse(args, with_cpu=True):
+    """"At fixed RMSE": the gem with caustics at 320x180, depth 24, its 2M-photon
+    map built once on the host (the checker) and imported into the device so
+    every renderer gathers from the same photons.  A 4096-spp image of the
+    checker (reference row streams) is the target; the RMSE (linear HDR, mean
+    radiance per pixel) of 64-spp frames of the plain megakernel, the
+    wavefront renderer and the checker itself (disjoint sample indices) are
+    compared.  Equal RMSE at equal spp makes samples/s the figure of merit."""
+    import numpy as np
+    import torch
+    from oracle import oracle as O
+    from rustracer_amd import Renderer, scenes
+    from rustracer_amd import _ffi as F
+
+    W, H, spp, ref_spp = 320, 180, 64, 4096
+    sc, cam, cfg = scenes.gem_caustics(W, H)
+    cores = O.threads()
+    orc = O.OracleScene(sc)
+    lq = cfg["light_quad"]
+    pp = F.rt_photon_params()
+    pp.emitter = F.EMIT_QUAD
+    for k in range(3):
+        pp.origin[k], pp.u[k], pp.v[k], pp.color[k] = lq["q"][k], lq["u"][k], lq["v"][k], cfg["light_color"][k]
+    pp.num_photons, pp.gather_radius = cfg["num_photons"], cfg["gather_radius"]
+    t0 = time.time()
+    nph = orc.photon_build(pp, nthreads=cores)
+    photons = orc.photon_dump(nph)
+    p = Renderer.params(W, H, sample_idx=0, max_depth=cfg["max_depth"], use_photons=True)
+    ref = orc.render_accumulate(cam, p, ref_spp, mode=0, nthreads=cores).astype(np.float64) / ref_spp
+    ref_s = time.time() - t0
+    log(f"[rmse] reference: {ref_spp} spp on {cores} host threads in {ref_s:.1f}s, {nph} photons")
+
+    def mse(img):
+        return float(((img.astype(np.float64) / spp - ref) ** 2).mean())
+
+    # the gem's caustic noise is heavy-tailed: one 64-spp frame's RMSE varies
+    # by ~10 % between independent draws, so mean squared errors over K
+    # independent frames are compared, with their standard errors
+    K = 16
+    r = Renderer(0).upload(sc)
+    r.photon_import(photons, cfg["gather_radius"])
+    a = torch.zeros((H, W, 3), dtype=torch.float32, device="cuda")
+    m_wf, m_plain = [], []
+    for k in range(K):
+        r.render_wavefront(a, cam, W, H, spp, max_depth=cfg["max_depth"], photon_map=True, seed=11 + k)
+        m_wf.append(mse(a.cpu().numpy()))
+    for k in range(2):
+        r.render_accumulate(a, cam, W, H, spp, max_depth=cfg["max_depth"], photon_map=True, seed=11 + k)
+        m_plain.append(mse(a.cpu().numpy()))
+    r.close()
+    m_host = []
+    for k in range(K):
+        pc = Renderer.params(W, H, sample_idx=100000 + 1000 * k, max_depth=cfg["max_depth"], use_photons=True)
+        m_host.append(mse(orc.render_accumulate(cam, pc, spp, mode=0, nthreads=cores)))
+    mw, mh = np.array(m_wf), np.array(m_host)
+    e_wf, e_host = float(np.sqrt(mw.mean())), float(np.sqrt(mh.mean()))
+    ratio = e_wf / e_host
+    # delta method on the ratio of square roots of two independent means
+    se = 0.5 * ratio * float(np.sqrt(mw.var(ddof=1) / K / mw.mean() ** 2 + mh.var(ddof=1) / K / mh.mean() ** 2))
+    log(f"[rmse] {spp} spp vs {ref_spp}-spp target over {K} frames each: wavefront {e_wf:.5f}, host {e_host:.5f} "
+        f"(ratio {ratio:.4f} +- {se:.4f}); plain == wavefront per frame: {np.allclose(m_plain, m_wf[:2], rtol=1e-4)}")
+    return {"value": ratio, "unit": "RMSE ratio (wavefront / host) at equal spp", "ms_per_step": None,
+            "config": {"workload": "gem_caustics_320x180_rmse", "width": W, "height": H, "spp": spp,
+                       "reference_spp": ref_spp, "frames": K, "photons": nph,
+                       "rmse": {"wavefront": e_wf, "host": e_host,
+                                "plain_first_frames": [float(np.sqrt(x)) for x in m_plain]},
+                       "rmse_per_frame": {"wavefront": [float(np.sqrt(x)) for x in m_wf],
+                                          "host": [float(np.sqrt(x)) for x in m_host]},
+                       "ratio_se": se, "within_2pct": bool(abs(ratio - 1.0) <= 0.02),
+                       "consistent_2se": bool(abs(ratio - 1.0) <= 2 * se), "reference_seconds": round(ref_s, 1)},
+            "roofline": {"achieved": None, "frac": None, "kernel": None, "avg_launch_ms": None},
+            "cpu_baseline": None}
+
+
 def measured_traffic(workload, opts):
     """HBM bytes per k_wf_extend dispatch from the committed rocprofv3 PMC
     summary (FETCH_SIZE + WRITE_SIZE, copy-calibrated) when it was measured
@@ -499,10 +575,10 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--config", default="mesh", choices=sorted(workloads()) + ["batch", "adjoint"])
-    ap.add_argument("--also", default="gem,batch,adjoint",
+    ap.add_argument("--config", default="mesh", choices=sorted(workloads()) + ["batch", "adjoint", "rmse"])
+    ap.add_argument("--also", default="gem,batch,adjoint,rmse",
                     help="comma-separated extra workloads ('' for none): " + ", ".join(sorted(workloads())) +
-                         ", batch, adjoint")
+                         ", batch, adjoint, rmse")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-tune", action="store_true")
@@ -528,6 +604,8 @@ def main():
         main_res = measure_batch(args, not args.no_cpu)
     elif args.config == "adjoint":
         main_res = measure_adjoint(args, not args.no_cpu)
+    elif args.config == "rmse":
+        main_res = measure_rmse(args)
     else:
         main_res = measure(args.config, args, with_cpu=not args.no_cpu)
     also = {}
@@ -536,6 +614,8 @@ def main():
             m = measure_batch(args, not args.no_cpu)
         elif name == "adjoint":
             m = measure_adjoint(args, not args.no_cpu)
+        elif name == "rmse":
+            m = measure_rmse(args)
         else:
             m = measure(name, args, with_cpu=not args.no_cpu)
         also[m["config"]["workload"]] = {"value": m["value"], "unit": m.get("unit", "samples/s"), "config": m["config"],

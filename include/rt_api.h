@@ -408,6 +408,11 @@ int  rt_backward_wavefront(rt_ctx *ctx, rt_scene *scene, const rt_camera *cam,
  * the scene; rt_sample_params.use_photons enables the gather.                */
 int  rt_photon_build(rt_ctx *ctx, rt_scene *scene, const rt_photon_params *pp,
                      uint32_t *num_stored);
+/* Attach a photon map built elsewhere (e.g. by the host's own
+ * rebuild_caustics): n photons of 9 floats (position, power, normal), the
+ * layout rt_photon_export writes; the device orders them for its gather.    */
+int  rt_photon_import(rt_ctx *ctx, rt_scene *scene, const float *photons, uint32_t n,
+                      float gather_radius);
 /* Irradiance queries against the attached map: PhotonMap::irradiance
  * (src/photon.rs:148-157).  pos_nrm: n * 6 floats (p.xyz, n.xyz).            */
 int  rt_photon_irradiance(rt_ctx *ctx, rt_scene *scene, const float *pos_nrm,

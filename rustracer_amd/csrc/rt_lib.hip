@@ -1878,6 +1878,29 @@ static void heap_build(PhRec* src, uint32_t n, uint32_t node, float4* pos, float
     heap_build(src + L + 1, n - L - 1, 2 * node + 2, pos, pw, nr);
 }
 
+// Heap-order the photons (position, power, normal per photon), replace the
+// scene's map on the device and keep the unordered set for export.
+static int install_photons(rt_ctx* ctx, rt_scene* s, std::vector<float4>& kept, float r) {
+    size_t m = kept.size() / 3;
+    std::vector<PhRec> recs(m);
+    for (size_t i = 0; i < m; i++) recs[i] = PhRec{kept[3 * i], kept[3 * i + 1], kept[3 * i + 2]};
+    std::vector<float4> heap(3 * m + 3);
+    heap_build(recs.data(), (uint32_t)m, 0, heap.data(), heap.data() + m, heap.data() + 2 * m);
+    if (s->ph_mem) { (void)hipFree(s->ph_mem); s->ph_mem = nullptr; }
+    s->S.num_photons = 0;
+    HIPCHECK(ctx, hipMalloc(&s->ph_mem, heap.size() * sizeof(float4)));
+    HIPCHECK(ctx, hipMemcpy(s->ph_mem, heap.data(), heap.size() * sizeof(float4), hipMemcpyHostToDevice));
+    const float4* dph = (const float4*)s->ph_mem;
+    s->S.ph_pos = dph;
+    s->S.ph_pow = dph + m;
+    s->S.ph_nrm = dph + 2 * m;
+    s->S.num_photons = (uint32_t)m;
+    s->S.ph_r2 = r * r;
+    s->S.ph_norm = 2.0f / (PI * PI * r * r);
+    s->photons_host.swap(kept);
+    return RT_OK;
+}
+
 int rt_photon_build(rt_ctx* ctx, rt_scene* s, const rt_photon_params* pp, uint32_t* num_stored) {
     if (!ctx || !s || !pp) return RT_ERR_INVALID;
     if (pp->num_photons == 0) return fail(ctx, RT_ERR_INVALID, "num_photons = 0");
@@ -1945,26 +1968,26 @@ int rt_photon_build(rt_ctx* ctx, rt_scene* s, const rt_photon_params* pp, uint32
     HIPCHECK(ctx, hipMemcpy(ok.data(), d_ok, ok.size() * sizeof(uint32_t), hipMemcpyDeviceToHost));
     std::vector<float4> kept;
     for (uint32_t i = 0; i < n; i++) if (ok[i]) { kept.push_back(all[3 * (size_t)i]); kept.push_back(all[3 * (size_t)i + 1]); kept.push_back(all[3 * (size_t)i + 2]); }
-    size_t m = kept.size() / 3;
-    std::vector<PhRec> recs(m);
-    for (size_t i = 0; i < m; i++) recs[i] = PhRec{kept[3 * i], kept[3 * i + 1], kept[3 * i + 2]};
-    std::vector<float4> heap(3 * m + 3);
-    heap_build(recs.data(), (uint32_t)m, 0, heap.data(), heap.data() + m, heap.data() + 2 * m);
-    if (s->ph_mem) { (void)hipFree(s->ph_mem); s->ph_mem = nullptr; }
-    s->S.num_photons = 0;
-    HIPCHECK(ctx, hipMalloc(&s->ph_mem, heap.size() * sizeof(float4)));
-    HIPCHECK(ctx, hipMemcpy(s->ph_mem, heap.data(), heap.size() * sizeof(float4), hipMemcpyHostToDevice));
-    const float4* dph = (const float4*)s->ph_mem;
-    s->S.ph_pos = dph;
-    s->S.ph_pow = dph + m;
-    s->S.ph_nrm = dph + 2 * m;
-    s->S.num_photons = (uint32_t)m;
-    float r = pp->gather_radius;
-    s->S.ph_r2 = r * r;
-    s->S.ph_norm = 2.0f / (PI * PI * r * r);
-    s->photons_host.swap(kept);
-    if (num_stored) *num_stored = (uint32_t)m;
+    int rc;
+    if ((rc = install_photons(ctx, s, kept, pp->gather_radius))) return rc;
+    if (num_stored) *num_stored = (uint32_t)(s->photons_host.size() / 3);
     return RT_OK;
+}
+
+int rt_photon_import(rt_ctx* ctx, rt_scene* s, const float* photons, uint32_t n, float gather_radius) {
+    if (!ctx || !s || (!photons && n)) return RT_ERR_INVALID;
+    if (!(gather_radius > 0.0f)) return fail(ctx, RT_ERR_INVALID, "gather_radius must be positive");
+    HIPCHECK(ctx, hipSetDevice(ctx->device));
+    std::vector<float4> kept((size_t)n * 3);
+    for (size_t i = 0; i < n; i++) {
+        const float* q = photons + 9 * i;
+        for (int k = 0; k < 9; k++)
+            if (!std::isfinite(q[k])) return fail(ctx, RT_ERR_INVALID, "photon " + std::to_string(i) + " is not finite");
+        kept[3 * i] = make_float4(q[0], q[1], q[2], 0.0f);
+        kept[3 * i + 1] = make_float4(q[3], q[4], q[5], 0.0f);
+        kept[3 * i + 2] = make_float4(q[6], q[7], q[8], 0.0f);
+    }
+    return install_photons(ctx, s, kept, gather_radius);
 }
 
 int rt_photon_irradiance(rt_ctx* ctx, rt_scene* s, const float* pn, uint32_t n, float* irr) {

@@ -260,6 +260,15 @@ class Renderer:
         self.num_photons = n.value
         return n.value
 
+    def photon_import(self, photons, gather_radius, scene=None):
+        """Attach a host-built photon map: (n, 9) float32 rows of position,
+        power, normal (the layout photon_export returns)."""
+        ph = np.ascontiguousarray(photons, dtype=np.float32).reshape(-1, 9)
+        self._check(self.L.rt_photon_import(self.ctx, self.scene if scene is None else scene, _ptr(ph), len(ph),
+                                            float(gather_radius)), "rt_photon_import")
+        self.num_photons = len(ph)
+        return len(ph)
+
     def photon_export(self):
         n = C.c_uint32()
         self._check(self.L.rt_photon_export(self.ctx, self.scene, None, 0, C.byref(n)), "rt_photon_export")

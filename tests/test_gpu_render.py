@@ -314,3 +314,33 @@ def test_noise_medium_three_sigma(gpu_renderer):
     assert np.isfinite(g).all()
     z = three_sigma(g, c)
     assert (z > 3).mean() <= 0.02 and (z > 6).mean() <= 0.002, ((z > 3).mean(), (z > 6).mean())
+
+
+def test_imported_host_photon_map_gives_the_checkers_frame(gpu_renderer):
+    """With the checker's own photons imported, the photon gather and the
+    same-stream frames match the checker's (the map is no longer a separate
+    random draw)."""
+    from rustracer_amd import _ffi as F
+    sc, cam, cfg = gem_small(32, 32, 100000)
+    gpu_renderer.upload(sc)
+    orc = O.OracleScene(sc)
+    lq = cfg["light_quad"]
+    pp = F.rt_photon_params()
+    pp.emitter = F.EMIT_QUAD
+    for k in range(3):
+        pp.origin[k], pp.u[k], pp.v[k], pp.color[k] = lq["q"][k], lq["u"][k], lq["v"][k], cfg["light_color"][k]
+    pp.num_photons, pp.gather_radius = 100000, cfg["gather_radius"]
+    n = orc.photon_build(pp)
+    ph = orc.photon_dump(n)
+    assert gpu_renderer.photon_import(ph, cfg["gather_radius"]) == n
+    back = gpu_renderer.photon_export()
+    assert np.array_equal(np.sort(back, axis=0), np.sort(ph, axis=0))
+    g = np.random.default_rng(5)
+    q = np.zeros((500, 6), np.float32)
+    q[:, [0, 2]] = g.uniform(-0.6, 0.6, size=(500, 2))
+    q[:, 4] = 1.0
+    assert np.allclose(gpu_renderer.photon_irradiance(q), orc.photon_irradiance(q), rtol=1e-4, atol=1e-6)
+    gp = passes_gpu(gpu_renderer, cam, 32, 32, 4, 24, seed=4, photons=True)
+    cp = passes_cpu(orc, cam, 32, 32, 4, 24, mode=1, seed=4, photons=True)
+    close = np.isclose(gp, cp, rtol=1e-3, atol=1e-4).all(axis=-1)
+    assert close.mean() > 0.95, close.mean()
