@@ -149,6 +149,12 @@ typedef struct {
  *                                                  src/material.rs:427-554
  *   ROUGH_DIELECTRIC p0 ior, p1 roughness (GGX transmission; extension, no
  *               counterpart in the reference)
+ *   BLACKBODY p0 temperature (K), p1 intensity; emits spectral_to_rgb(l) x
+ *               normalised Planck on spectral paths, its white-point average
+ *               otherwise                             src/material.rs:27-83
+ *   SSS         p0-2 albedo, p3-5 sigma_a, p6 ior, p7 density, p8 g
+ *               (dielectric boundary with HG scattering inside)
+ *                                                  src/material.rs:576-631
  *   ROUGH_SPECTRAL p0 n_d (IOR at 589.3 nm), p1-3 Sellmeier B1..B3,
  *               p4-6 Sellmeier C1..C3 (um^2): n(l) = n_d + s(l) - s(589.3 nm);
  *               p7 GGX alpha of primitive 0 (the table facet of a Diamond),
@@ -165,7 +171,9 @@ enum {
     RT_MAT_DIFFUSE_LIGHT       = 5,
     RT_MAT_SPECTRAL_METAL      = 6,
     RT_MAT_ROUGH_DIELECTRIC    = 7,
-    RT_MAT_ROUGH_SPECTRAL      = 8
+    RT_MAT_ROUGH_SPECTRAL      = 8,
+    RT_MAT_BLACKBODY           = 9,
+    RT_MAT_SSS                 = 10
 };
 
 typedef struct {

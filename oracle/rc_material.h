@@ -275,6 +275,36 @@ struct DiffuseLight : Material {
     bool is_emissive() const override { return emit.luminance() > 0.0f; }
 };
 
+// BlackbodyLight (src/material.rs:27-83)
+static inline float planck_raw(float lambda_nm, float temp_k) {
+    const float HC_OVER_K = 14388000.0f;
+    float x = HC_OVER_K / (lambda_nm * temp_k);
+    float l2 = lambda_nm * lambda_nm, l4 = l2 * l2;
+    return 1.0f / ((lambda_nm * l4) * rmax(std::exp(x) - 1.0f, 1e-30f));
+}
+struct BlackbodyLight : Material {
+    float temp_k = 6500.0f, intensity = 1.0f, norm = 1.0f;
+    Color avg_color;
+    void init() {
+        const int N = 65;
+        float raw[N], sum = 0.0f;
+        for (int i = 0; i < N; i++) { raw[i] = planck_raw(380.0f + (float)i * 5.0f, temp_k); sum += raw[i]; }
+        float mean = sum / (float)N;
+        norm = 1.0f / rmax(mean, 1e-30f);
+        Color acc;
+        for (int i = 0; i < N; i++) acc += spectral_to_rgb(380.0f + (float)i * 5.0f) * (raw[i] * norm);
+        avg_color = acc / (float)N * intensity;
+    }
+    bool scatter(const Ray&, const HitRecord&, SmallRng&, ScatterRecord&) const override { return false; }
+    Color emitted(float, float, const Point3&) const override { return avg_color; }
+    bool is_emissive() const override { return intensity > 0.0f; }
+    Color emitted_at(float, float, const Point3&, float lambda, bool spectral_weighted) const override {
+        if (spectral_weighted) { float sc = planck_raw(lambda, temp_k) * norm * intensity; return Color(sc, sc, sc); }
+        return avg_color;
+    }
+    Color albedo_hint(float, float, const Point3&) const override { return avg_color; }
+};
+
 struct Lambertian : Material {
     Texture tex;
     bool scatter(const Ray& r_in, const HitRecord& rec, SmallRng&, ScatterRecord& sr) const override {
@@ -627,6 +657,40 @@ struct RoughSpectralDielectric : Material {
 };
 
 // Henyey-Greenstein phase "material" of a medium (src/volume.rs:24-62).
+static inline Vec3 hg_sample(const Vec3& v_in, float g, SmallRng& rng);
+
+// SSSMaterial (src/material.rs:576-631)
+struct SSSMaterial : Material {
+    Color albedo, sigma_a; float ior = 1.3f, density = 1.0f, g = 0.0f;
+    static Color beer(const Color& sa, float t) { return Color(std::exp(-sa.x * t), std::exp(-sa.y * t), std::exp(-sa.z * t)); }
+    bool scatter(const Ray& r_in, const HitRecord& rec, SmallRng& rng, ScatterRecord& sr) const override {
+        Vec3 unit = r_in.direction.unit();
+        sr.skip_pdf = true;
+        if (!rec.front_face && density > 0.0f) {
+            float path_length = (rec.p - r_in.origin).length();
+            float t_scat = -std::log(rmax(rng.gen(), 1e-9f)) / density;
+            if (t_scat < path_length) {
+                Point3 p_scat = r_in.origin + unit * t_scat;
+                Vec3 nd = hg_sample(unit, g, rng);
+                sr.attenuation = albedo * beer(sigma_a, t_scat);
+                sr.ray = Ray::scatter_from(p_scat, nd, r_in);
+                return true;
+            }
+            bool refl;
+            Vec3 d = dielectric_boundary(ior, r_in, rec, rng, refl);
+            sr.attenuation = beer(sigma_a, path_length);
+            sr.ray = Ray::scatter_from(rec.p, d, r_in);
+            return true;
+        }
+        bool refl;
+        Vec3 d = dielectric_boundary(ior, r_in, rec, rng, refl);
+        sr.attenuation = Color(1, 1, 1);
+        sr.ray = Ray::scatter_from(rec.p, d, r_in);
+        return true;
+    }
+    Color albedo_hint(float, float, const Point3&) const override { return albedo; }
+};
+
 static inline Vec3 hg_sample(const Vec3& v_in, float g, SmallRng& rng) {
     float xi1 = rng.gen(), xi2 = rng.gen();
     float cos_theta;

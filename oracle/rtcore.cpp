@@ -58,6 +58,13 @@ Material* make_material(const rt_material& m) {
         case RT_MAT_DIFFUSE_LIGHT: { auto* d = new DiffuseLight(); d->emit = Color(m.p[0], m.p[1], m.p[2]); return d; }
         case RT_MAT_SPECTRAL_METAL: { auto* s = new SpectralMetal(); s->variant = (int)m.p[0]; s->roughness = m.p[1]; s->init(); return s; }
         case RT_MAT_ROUGH_DIELECTRIC: { auto* r = new RoughDielectric(); r->ior = m.p[0]; r->roughness = m.p[1]; return r; }
+        case RT_MAT_BLACKBODY: { auto* b = new BlackbodyLight(); b->temp_k = m.p[0]; b->intensity = m.p[1]; b->init(); return b; }
+        case RT_MAT_SSS: {
+            auto* q = new SSSMaterial();
+            q->albedo = Color(m.p[0], m.p[1], m.p[2]); q->sigma_a = Color(m.p[3], m.p[4], m.p[5]);
+            q->ior = m.p[6]; q->density = m.p[7]; q->g = m.p[8];
+            return q;
+        }
         case RT_MAT_ROUGH_SPECTRAL: {
             auto* r = new RoughSpectralDielectric();
             r->nd = m.p[0];
@@ -444,6 +451,11 @@ int rtc_eval(const char* fn, const float* a, float* out) {
     if (f == "metal_ior") { metal_ior((int)a[0], a[1], out[0], out[1]); return 0; }
     if (f == "spectral_to_rgb") { Color c = spectral_to_rgb(a[0]); out[0] = c.x; out[1] = c.y; out[2] = c.z; return 0; }
     if (f == "spectral_fire_rgb") { Color c = spectral_fire_rgb(a[0]); out[0] = c.x; out[1] = c.y; out[2] = c.z; return 0; }
+    if (f == "planck_raw") { out[0] = planck_raw(a[0], a[1]); return 0; }
+    if (f == "blackbody") {
+        BlackbodyLight b; b.temp_k = a[0]; b.intensity = a[1]; b.init();
+        out[0] = b.avg_color.x; out[1] = b.avg_color.y; out[2] = b.avg_color.z; out[3] = b.norm; return 0;
+    }
     if (f == "vndf") {
         Vec3 m = vndf_sample_aniso(Vec3(a[0], a[1], a[2]), a[3], a[4], a[5], a[6]);
         out[0] = m.x; out[1] = m.y; out[2] = m.z; return 0;

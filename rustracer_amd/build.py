@@ -3,6 +3,14 @@
 The whole library is one HIP translation unit compiled with hipcc for
 MI355X.  -ffp-contract=off keeps every f32 expression unfused, like the
 reference's Rust code; packed-FP32 instruction selection is disabled.
+
+-fno-slp-vectorize keeps every float scalar.  With packed FP32 arithmetic off,
+the SLP vectorizer's <2 x float> values buy nothing, and the gfx950 backend
+miscompiled one of them: a hit position merged from x/y lanes of a 64-bit
+register pair across a divergent dielectric reflect/refract branch kept the y
+lane only on the refract side, so reflected rays left an SSS sphere from a
+garbage origin (NaN radiance).  Scalar code merges each component after the
+branches reconverge.
 """
 import os
 import subprocess
@@ -13,7 +21,7 @@ OUT = os.path.join(HERE, "librtamd.so")
 
 HIPCC_FLAGS = [
     "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",
-    "-ffp-contract=off", "-fno-fast-math",
+    "-ffp-contract=off", "-fno-fast-math", "-fno-slp-vectorize",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-Wno-unused-function", "-Wno-unused-result",
 ]

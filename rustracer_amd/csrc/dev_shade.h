@@ -11,7 +11,7 @@
 namespace rt {
 
 enum { M_LAMBERTIAN = 1, M_DIELECTRIC = 2, M_SPECTRAL_DIELECTRIC = 3, M_PBR = 4, M_DIFFUSE_LIGHT = 5,
-       M_SPECTRAL_METAL = 6, M_ROUGH_DIELECTRIC = 7, M_ROUGH_SPECTRAL = 8, M_PHASE = 100 };
+       M_SPECTRAL_METAL = 6, M_ROUGH_DIELECTRIC = 7, M_ROUGH_SPECTRAL = 8, M_BLACKBODY = 9, M_SSS = 10, M_PHASE = 100 };
 enum { BGK_SOLID = 0, BGK_GRADIENT = 1, BGK_PHYSICAL = 2, BGK_ENV = 3, BGK_ENV_SPLIT = 4 };
 
 __constant__ float c_cie_x[RT_CIE_N];
@@ -431,6 +431,38 @@ RT_D V3 hg_sample(V3 vin, float g, Rng& rng) {
     return u * (st * cosf(phi)) + v * (st * sinf(phi)) + w * ct;
 }
 
+// SSSMaterial::scatter (src/material.rs:594-628): inside, an exponential
+// free flight before the exit surface scatters with HG and Beer-Lambert up to
+// that point; otherwise the dielectric boundary with absorption over the
+// traversed segment (none on entry).
+RT_D V3 beer_lambert(const float* sa, float t) { return mk(expf(-sa[0] * t), expf(-sa[1] * t), expf(-sa[2] * t)); }
+RT_D bool sss_scatter(const DMaterial& m, const Ray& r, const Rec& h, Rng& rng, Scatter& sr) {
+    V3 u = unit(r.d);
+    float density = m.p[7];
+    sr.skip_pdf = true;
+    if (!h.front && density > 0.0f) {
+        float path_length = vlen(h.p - r.o);
+        float t_scat = -logf(fmx(rng_f(rng), 1e-9f)) / density;
+        if (t_scat < path_length) {
+            V3 ps = r.o + u * t_scat;
+            V3 nd = hg_sample(u, m.p[8], rng);
+            sr.att = mk(m.p[0], m.p[1], m.p[2]) * beer_lambert(m.p + 3, t_scat);
+            sr.ray = scatter_from(ps, nd, r);
+            return true;
+        }
+        bool refl;
+        V3 d = dielectric_boundary(m.p[6], r, h, rng, refl);
+        sr.att = beer_lambert(m.p + 3, path_length);
+        sr.ray = scatter_from(h.p, d, r);
+        return true;
+    }
+    bool refl;
+    V3 d = dielectric_boundary(m.p[6], r, h, rng, refl);
+    sr.att = mk(1, 1, 1);
+    sr.ray = scatter_from(h.p, d, r);
+    return true;
+}
+
 // Material::scatter dispatch.  Medium events scatter with the HG phase function.
 RT_D bool mat_scatter(const DScene& S, const Rec& h, const Ray& r, Rng& rng, Scatter& sr) {
     if (h.medium) {
@@ -471,13 +503,28 @@ RT_D bool mat_scatter(const DScene& S, const Rec& h, const Ray& r, Rng& rng, Sca
         case M_SPECTRAL_METAL: return smetal_scatter(S, m, r, h, rng, sr);
         case M_ROUGH_DIELECTRIC: return rough_dielectric_scatter(m, r, h, rng, sr);
         case M_ROUGH_SPECTRAL: return rsd_scatter(S, m, r, h, rng, sr);
+        case M_SSS: return sss_scatter(m, r, h, rng, sr);
         default: return false;   // DiffuseLight
     }
 }
-RT_D V3 mat_emitted(const DScene& S, const Rec& h) {
+// planck_raw (src/spectrum.rs:204-208): relative Planck radiance
+RT_HD float planck_raw(float lambda_nm, float temp_k) {
+    const float HC_OVER_K = 14388000.0f;
+    float x = HC_OVER_K / (lambda_nm * temp_k);
+    float l2 = lambda_nm * lambda_nm, l4 = l2 * l2;
+    return 1.0f / ((lambda_nm * l4) * fmaxf(expf(x) - 1.0f, 1e-30f));
+}
+// Material::emitted_at for the ray that reached the emitter (its hero
+// wavelength and whether its throughput already carries the spectral weight).
+RT_D V3 mat_emitted(const DScene& S, const Rec& h, const Ray& r) {
     if (h.medium) return mk(0, 0, 0);
     const DMaterial& m = S.mats[h.mat];
     if (m.kind == M_DIFFUSE_LIGHT) return mk(m.p[0], m.p[1], m.p[2]);
+    if (m.kind == M_BLACKBODY) {
+        // BlackbodyLight::emitted_at (src/material.rs:69-80)
+        if (r.spectral) { float sc = planck_raw(r.wavelength, m.p[0]) * m.p[2] * m.p[1]; return mk(sc, sc, sc); }
+        return mk(m.p[3], m.p[4], m.p[5]);
+    }
     if (m.kind == M_PBR) return mk(m.p[9], m.p[10], m.p[11]) * m.p[12];
     return mk(0, 0, 0);
 }
@@ -488,6 +535,8 @@ RT_D V3 mat_albedo_hint(const DScene& S, const Rec& h) {
     if (m.kind == M_LAMBERTIAN) return tex_value(m, h.p);
     if (m.kind == M_PBR) return mk(m.p[0], m.p[1], m.p[2]);
     if (m.kind == M_DIFFUSE_LIGHT) return mk(m.p[0], m.p[1], m.p[2]);
+    if (m.kind == M_BLACKBODY) return mk(m.p[3], m.p[4], m.p[5]);
+    if (m.kind == M_SSS) return mk(m.p[0], m.p[1], m.p[2]);
     return mk(1, 1, 1);
 }
 RT_D bool mat_receives_caustics(const DScene& S, const Rec& h) {
@@ -825,7 +874,7 @@ RT_D bool bounce_t(const DScene& S, PathState& ps, const HitId& id, int depth, f
     }
     Rec h = finalize(S, ray, id, 0.001f);
     float ew = (ps.prev_specular || lights_empty) ? 1.0f : ps.prev_mis;
-    color += dh.emit(ps.thr * mat_emitted(S, h) * ew);
+    color += dh.emit(ps.thr * mat_emitted(S, h, ray) * ew);
     Scatter sr;
     if (!mat_scatter(S, h, ray, rng, sr)) return false;
     if (sr.skip_pdf) {
@@ -866,7 +915,7 @@ RT_D bool bounce_t(const DScene& S, PathState& ps, const HitId& id, int depth, f
                 float md = l_pdf * l_pdf + brdf * brdf + sp * sp + ep * ep;
                 if (md > 0.0f && brdf > 0.0f) {
                     sh[nsh].o = h.p; sh[nsh].d = ldir; sh[nsh].t_max = lrec.t * (1.0f - 1e-4f); sh[nsh].time = ray.time;
-                    sh[nsh].contrib = ps.thr * sr.att * brdf * l_pdf * mat_emitted(S, lrec) / md;
+                    sh[nsh].contrib = ps.thr * sr.att * brdf * l_pdf * mat_emitted(S, lrec, ray) / md;
                     dh.shadow(sh[nsh], nsh, S, h, ray, true);
                     nsh++;
                 }

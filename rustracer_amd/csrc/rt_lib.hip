@@ -1175,7 +1175,7 @@ extern "C" int rt_scene_upload(rt_ctx* ctx, const rt_scene_desc* d, rt_scene** o
     Flat F;
     for (uint32_t i = 0; i < d->num_materials; i++) {
         uint32_t k = d->materials[i].kind;
-        if (k < RT_MAT_LAMBERTIAN || k > RT_MAT_ROUGH_SPECTRAL) return fail(ctx, RT_ERR_UNSUPPORTED, "unsupported material kind " + std::to_string(k));
+        if (k < RT_MAT_LAMBERTIAN || k > RT_MAT_SSS) return fail(ctx, RT_ERR_UNSUPPORTED, "unsupported material kind " + std::to_string(k));
     }
     F.objs_input.resize(d->num_objects);
     F.obj_boxes.resize(d->num_objects);
@@ -1257,6 +1257,30 @@ extern "C" int rt_scene_upload(rt_ctx* ctx, const rt_scene_desc* d, rt_scene** o
     for (uint32_t i = 0; i < d->num_materials; i++) {
         mats[i].kind = d->materials[i].kind; mats[i].tex = d->materials[i].tex;
         for (int k = 0; k < 24; k++) mats[i].p[k] = d->materials[i].p[k];
+        if (mats[i].kind == RT_MAT_BLACKBODY) {
+            // BlackbodyLight::new (src/material.rs:49-61): normalising mean of
+            // Planck over the 65 CIE samples and the white-point average colour
+            float T = mats[i].p[0], intensity = mats[i].p[1];
+            float raw[RT_CIE_N], sum = 0.0f;
+            for (int j = 0; j < RT_CIE_N; j++) { raw[j] = planck_raw(380.0f + (float)j * 5.0f, T); sum += raw[j]; }
+            float mean = sum / (float)RT_CIE_N;
+            float norm = 1.0f / std::fmax(mean, 1e-30f);
+            float acc[3] = {0, 0, 0};
+            for (int j = 0; j < RT_CIE_N; j++) {
+                // spectral_to_rgb(380 + 5 j): the table lerp of cie_at, on the host
+                float t = std::fmin(std::fmax(((380.0f + (float)j * 5.0f) - RT_CIE_START) / RT_CIE_STEP, 0.0f), (float)(RT_CIE_N - 1));
+                int i0 = std::min((int)t, RT_CIE_N - 2);
+                float fr = t - (float)i0;
+                float x = RT_CIE_X[i0] + fr * (RT_CIE_X[i0 + 1] - RT_CIE_X[i0]);
+                float y = RT_CIE_Y[i0] + fr * (RT_CIE_Y[i0 + 1] - RT_CIE_Y[i0]);
+                float z = RT_CIE_Z[i0] + fr * (RT_CIE_Z[i0 + 1] - RT_CIE_Z[i0]);
+                float c[3]; xyz_srgb(x, y, z, c);
+                float w = raw[j] * norm;
+                for (int a = 0; a < 3; a++) acc[a] += (c[a] * ctx->norms[a]) * w;
+            }
+            mats[i].p[2] = norm;
+            for (int a = 0; a < 3; a++) mats[i].p[3 + a] = acc[a] / (float)RT_CIE_N * intensity;
+        }
     }
     if ((rc = upload_vec(s.get(), mats, &S.mats))) return rc;
     s->mats_host = mats;

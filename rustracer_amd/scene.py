@@ -168,6 +168,32 @@ class RoughDielectric(Material):
         return [self.ior, self.roughness]
 
 
+class BlackbodyLight(Material):
+    """BlackbodyLight::new (src/material.rs:49-61): Planck emitter at
+    `temp_k`, scaled by `intensity`."""
+    kind = F.MAT_BLACKBODY
+
+    def __init__(self, temp_k, intensity=1.0):
+        self.temp_k, self.intensity = float(temp_k), float(intensity)
+
+    def params(self):
+        return [self.temp_k, self.intensity]
+
+
+class SSSMaterial(Material):
+    """SSSMaterial (src/material.rs:576-631): dielectric boundary of `ior`
+    around a scattering interior (exponential free flight of `density`, HG
+    phase `g`, single-scatter `albedo`, Beer-Lambert `sigma_a`)."""
+    kind = F.MAT_SSS
+
+    def __init__(self, albedo, sigma_a, ior=1.3, density=1.0, g=0.0):
+        self.albedo, self.sigma_a = v3(albedo), v3(sigma_a)
+        self.ior, self.density, self.g = float(ior), float(density), float(g)
+
+    def params(self):
+        return list(self.albedo) + list(self.sigma_a) + [self.ior, self.density, self.g]
+
+
 class RoughSpectralDielectric(Material):
     """Rough dispersive dielectric (extension): n(l) = n_d + s(l) - s(589.3 nm)
     with s the Sellmeier law (B, C in um^2); GGX alpha `alpha_table` on

@@ -344,3 +344,28 @@ def test_imported_host_photon_map_gives_the_checkers_frame(gpu_renderer):
     cp = passes_cpu(orc, cam, 32, 32, 4, 24, mode=1, seed=4, photons=True)
     close = np.isclose(gp, cp, rtol=1e-3, atol=1e-4).all(axis=-1)
     assert close.mean() > 0.95, close.mean()
+
+
+def test_blackbody_light_and_sss_same_streams_and_three_sigma(gpu_renderer):
+    """A blackbody area light (spectral paths through a dispersive sphere see
+    its Planck weight, others its white-point average) over an SSS sphere."""
+    from rustracer_amd import scene as S
+    bb = S.BlackbodyLight(3200.0, 8.0)
+    lq = S.Quad((-0.3, 1.6, -0.3), (0.6, 0, 0), (0, 0, 0.6), bb)
+    objs = [S.Quad((-4, 0, -4), (8, 0, 0), (0, 0, 8), S.Lambertian((0.8, 0.8, 0.8))),
+            S.Sphere((-0.45, 0.35, 0.1), 0.35, S.SSSMaterial((0.95, 0.8, 0.7), (0.3, 0.8, 1.2), 1.35, 3.0, 0.2)),
+            S.Sphere((0.45, 0.35, 0.0), 0.35, S.SpectralDielectric(1.6, 0.02)), lq]
+    sc = S.Scene(objs, [S.Quad((-0.3, 1.6, -0.3), (0.6, 0, 0), (0, 0, 0.6), bb)], S.Background.solid((0.02, 0.02, 0.03)))
+    cam = S.Camera((0.0, 1.0, -2.6), (0.0, 0.4, 0.0), (0, 1, 0), 40.0, 1.0)
+    gpu_renderer.upload(sc)
+    orc = O.OracleScene(sc)
+    g = passes_gpu(gpu_renderer, cam, 40, 40, 4, 16)
+    c = passes_cpu(orc, cam, 40, 40, 4, 16, mode=1)
+    close = np.isclose(g, c, rtol=1e-3, atol=1e-5).all(axis=-1)
+    assert close.mean() > 0.95, close.mean()
+    K = 96
+    g = passes_gpu(gpu_renderer, cam, 32, 32, K, 16, seed=2)
+    c = passes_cpu(orc, cam, 32, 32, K, 16, mode=0)
+    assert np.isfinite(g).all()
+    z = three_sigma(g, c)
+    assert (z > 3).mean() <= 0.02 and (z > 6).mean() <= 0.002, ((z > 3).mean(), (z > 6).mean())

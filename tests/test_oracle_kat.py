@@ -466,3 +466,61 @@ def test_env_pdf_integrates_to_one():
 def test_lambertian_scatter_is_diffuse_placeholder():
     out = O.scatter(S.Lambertian((0.5, 0.25, 1.0)), (0, 0, -1), (0, 0, 1), 550.0, (0, 0, 0), (0, 0, -1), n=1)
     assert out[0, 6] == 0 and np.allclose(out[0, :3], [0.5, 0.25, 1.0])
+
+
+# ---------------------------------------------------------------- BlackbodyLight (src/material.rs:27-83, src/spectrum.rs:204-208)
+def test_planck_reference_kats_and_closed_form():
+    p = lambda lam, T: O.eval_fn("planck_raw", [lam, T], nout=1)
+    # planck_hotter_blackbody_is_bluer, planck_monotone_in_temperature (src/spectrum.rs:362-376)
+    assert p(450, 6500) / p(650, 6500) > p(450, 3000) / p(650, 3000)
+    for lam in (400.0, 550.0, 700.0):
+        assert p(lam, 6000) > p(lam, 3000) and p(lam, 10000) > p(lam, 6000)
+    for lam, T in [(380.0, 2700.0), (555.0, 5500.0), (700.0, 10000.0)]:
+        x = 14388000.0 / (lam * T)
+        assert p(lam, T) == pytest.approx(1.0 / (lam ** 5 * math.expm1(x)), rel=2e-5)
+
+
+def test_blackbody_white_point_and_spectral_mean():
+    """avg_color is the mean over the 65 CIE samples of spectral_to_rgb times
+    the normalised Planck weight: 6500 K is near neutral, 3000 K warm; the
+    normalised weights average to 1 over the samples."""
+    def bb(T, I=1.0):
+        return O.eval_fn("blackbody", [T, I], nout=4)
+    c65 = np.array(bb(6500.0)[:3])
+    c30 = np.array(bb(3000.0)[:3])
+    assert c30[0] > c30[2] * 2 and abs(c65[0] / c65[2] - 1) < 0.35
+    lam = 380.0 + 5.0 * np.arange(65)
+    T = 4200.0
+    out = bb(T, 2.5)
+    w = np.array([O.eval_fn("planck_raw", [l, T], nout=1) for l in lam]) * out[3]
+    assert w.mean() == pytest.approx(1.0, rel=1e-5)
+    rgb = np.array([O.eval_fn("spectral_to_rgb", [l], nout=3) for l in lam])
+    assert np.allclose((rgb * w[:, None]).mean(0) * 2.5, out[:3], rtol=1e-5)
+
+
+# ---------------------------------------------------------------- SSSMaterial (src/material.rs:576-631)
+def test_sss_free_flight_and_exit_absorption():
+    """From inside (back face at distance L along the ray), a scatter event
+    happens with probability 1 - exp(-density L), weighted albedo *
+    exp(-sigma_a t); otherwise the boundary is crossed with weight
+    exp(-sigma_a L)."""
+    albedo, sa, dens, L = (0.9, 0.7, 0.5), (0.2, 0.4, 0.8), 1.3, 0.8
+    m = S.SSSMaterial(albedo, sa, ior=1.4, density=dens, g=0.3)
+    out = O.scatter(m, (0, 0, -L), (0, 0, 1), 550.0, (0, 0, 0), (0, 0, -1), t=L, front_face=False, n=200000, seed=3)
+    assert (out[:, 7] == 1).all() and (out[:, 6] == 1).all()
+    exitw = np.exp(-np.array(sa) * L)
+    is_exit = np.isclose(out[:, :3], exitw, rtol=1e-5).all(1)
+    p_scat = 1 - math.exp(-dens * L)
+    frac = 1 - is_exit.mean()
+    assert abs(frac - p_scat) < 4 * math.sqrt(p_scat * (1 - p_scat) / len(out))
+    sc = out[~is_exit]
+    t = -np.log(sc[:, 0] / albedo[0]) / sa[0]                 # recover the free-flight length
+    assert (t > 0).all() and (t < L).all()
+    assert np.allclose(sc[:, 1], albedo[1] * np.exp(-sa[1] * t), rtol=1e-4)
+
+
+def test_sss_without_density_is_the_dielectric():
+    d = O.scatter(S.Dielectric(1.4), (0, 1, -1), (0, -0.6, 0.8), 550.0, (0, 0, 0), (0, 1, 0), n=64, seed=9)
+    s = O.scatter(S.SSSMaterial((1, 1, 1), (0, 0, 0), ior=1.4, density=0.0), (0, 1, -1), (0, -0.6, 0.8), 550.0,
+                  (0, 0, 0), (0, 1, 0), n=64, seed=9)
+    assert np.array_equal(d, s)
