@@ -134,8 +134,7 @@ typedef struct {
 
 /* ---- materials -----------------------------------------------------------
  * Parameter layout of rt_material.p[] per kind:
- *   LAMBERTIAN  p0-2 albedo; tex: 0 solid, 1 checker (p3 scale, p4-6 odd)
- *                                                  src/material.rs:85-102, src/texture.rs:26-35
+ *   LAMBERTIAN  albedo texture (below)           src/material.rs:85-102
  *   DIELECTRIC  p0 ior                             src/material.rs:323-332
  *   SPECTRAL_DIELECTRIC p0 cauchy_b, p1 cauchy_c, p2-4 absorption,
  *               p5 model (0 Cauchy, 1 Sellmeier), p6-8 Sellmeier B1..B3,
@@ -144,7 +143,7 @@ typedef struct {
  *               p6 anisotropy_angle, p7 clearcoat, p8 clearcoat_roughness,
  *               p9-11 emission, p12 emission_strength, p13 sheen,
  *               p14 sheen_tint                      src/material.rs:695-1050
- *   DIFFUSE_LIGHT p0-2 emit                         src/material.rs:14-25
+ *   DIFFUSE_LIGHT emission texture (below)          src/material.rs:14-25
  *   SPECTRAL_METAL p0 variant (0 gold, 1 copper, 2 silver), p1 roughness
  *                                                  src/material.rs:427-554
  *   ROUGH_DIELECTRIC p0 ior, p1 roughness (GGX transmission; extension, no
@@ -162,7 +161,21 @@ typedef struct {
  *               p[8 + (k-1) % 16]).  Rough dispersive dielectric with the
  *               fire tint of SPECTRAL_DIELECTRIC on the first refraction;
  *               extension (the differentiable gem of the adjoint path).
+ *
+ * Texture of LAMBERTIAN (albedo) and DIFFUSE_LIGHT (emission), selected by
+ * rt_material.tex (Texture::value, src/texture.rs:26-56):
+ *   RT_TEX_SOLID   p0-2 colour
+ *   RT_TEX_CHECKER p0-2 even, p3 scale, p4-6 odd: parity of the floors of
+ *                  scale * p
+ *   RT_TEX_IMAGE   p7 index into rt_scene_desc.images: nearest texel at
+ *                  (clamp(u), 1 - clamp(v)), sRGB-decoded through the 256-entry
+ *                  IEC 61966-2-1 table (src/output.rs:88-98)
+ *   RT_TEX_NOISE   p3 scale, p8 Perlin table index: grey
+ *                  0.5 (1 + sin(scale p.z + 10 turb(p, 7)))
+ *   RT_TEX_MOTTLE  p0-2 lo, p3 scale, p4-6 hi, p8 Perlin table index:
+ *                  lerp(lo, hi, clamp(2 turb(scale p, 7), 0, 1))
  */
+enum { RT_TEX_SOLID = 0, RT_TEX_CHECKER = 1, RT_TEX_IMAGE = 2, RT_TEX_NOISE = 3, RT_TEX_MOTTLE = 4 };
 enum {
     RT_MAT_LAMBERTIAN          = 1,
     RT_MAT_DIELECTRIC          = 2,
@@ -210,6 +223,13 @@ typedef struct {
 
 enum { RT_ENV_CDF = 0, RT_ENV_ALIAS = 1 };
 
+/* An 8-bit sRGB image (image::RgbImage): width x height texels of 3 bytes,
+ * row-major, row 0 at the top (v = 1). */
+typedef struct {
+    uint32_t       width, height;
+    const uint8_t *rgb;
+} rt_image;
+
 typedef struct {
     const rt_object   *objects;    uint32_t num_objects;   /* world, input order */
     const rt_object   *lights;     uint32_t num_lights;    /* NEE light list     */
@@ -223,6 +243,11 @@ typedef struct {
     const float *tri_v; const float *tri_n; const float *tri_t; const float *tri_uv;
     uint32_t     num_tris;
     rt_background background;
+    /* textures: images for RT_TEX_IMAGE; Perlin tables for RT_TEX_NOISE /
+     * RT_TEX_MOTTLE, num_perlin blocks of 1536 floats laid out as rt_medium.perlin
+     * (the host's Perlin::new draws, src/perlin.rs:11-33) */
+    const rt_image *images;     uint32_t num_images;
+    const float    *perlin;     uint32_t num_perlin;
 } rt_scene_desc;
 
 /* ---- rays and hits (parity surface) --------------------------------------- */

@@ -55,6 +55,8 @@ def lib():
         L.rtc_rsd_logf.argtypes = [vp, vp, vp, C.c_double, C.c_double, C.c_int, C.POINTER(C.c_double)]
         L.rtc_env_alias.argtypes = [vp, C.c_uint32, C.c_uint32, vp, vp]
         L.rtc_env_sample_mode.argtypes = [vp, C.c_uint32, C.c_uint32, C.c_float, C.c_int, C.c_uint64, C.c_uint32, vp]
+        L.rtc_texture_eval.argtypes = [vp, vp, vp, C.c_uint32, vp]
+        L.rtc_perlin.argtypes = [vp, vp, C.c_uint32, C.c_int, vp, vp]
         _lib = L
     return _lib
 
@@ -278,3 +280,30 @@ def env_sample_mode(rgb, mode, seed, n, rot=0.0):
     if lib().rtc_env_sample_mode(_p(rgb), w, h, rot, mode, seed, n, _p(out)) != 0:
         raise ValueError("env map has no energy")
     return out
+
+
+def texture_eval(material, uvp, scene=None):
+    """Texture::value of `material` (a Lambertian or DiffuseLight from
+    rustracer_amd.scene) at rows (u, v, x, y, z); `scene` supplies its image /
+    Perlin tables (default: a one-material scene built around it)."""
+    from rustracer_amd import scene as S
+    if scene is None:
+        scene = S.Scene([S.Sphere((0, 0, 0), 1.0, material)])
+    d = scene.descriptor()
+    mi = scene.material_index(material)
+    uvp = np.ascontiguousarray(uvp, dtype=np.float32).reshape(-1, 5)
+    out = np.zeros((uvp.shape[0], 3), np.float32)
+    if lib().rtc_texture_eval(C.byref(d.materials[mi]), C.byref(d), _p(uvp), uvp.shape[0], _p(out)) != 0:
+        raise ValueError("bad texture")
+    return out
+
+
+def perlin(tables, pts, depth=7):
+    """(Perlin::noise, Perlin::turb(depth)) at rows of `pts`."""
+    t = np.ascontiguousarray(tables, dtype=np.float32)
+    pts = np.ascontiguousarray(pts, dtype=np.float32).reshape(-1, 3)
+    noise = np.zeros(pts.shape[0], np.float32)
+    turb = np.zeros(pts.shape[0], np.float32)
+    if lib().rtc_perlin(_p(t), _p(pts), pts.shape[0], int(depth), _p(noise), _p(turb)) != 0:
+        raise ValueError("bad Perlin tables")
+    return noise, turb

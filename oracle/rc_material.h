@@ -3,7 +3,8 @@
 // Appearance layer restated from rustracer:
 //   spectral_to_rgb / spectral_fire_rgb   src/spectrum.rs:64-193 (FIRE_DESAT fixed at 0.5)
 //   fresnel_conductor, metal n,k, cauchy  src/spectrum.rs:213-277
-//   Lambertian (solid / checker texture)  src/material.rs:85-102, src/texture.rs:26-35
+//   Lambertian, Texture (solid, checker,  src/material.rs:85-102, src/texture.rs:26-56,
+//     image, noise, mottle), Perlin         src/perlin.rs:35-88, src/output.rs:88-98
 //   fresnel_dielectric / boundary          src/material.rs:106-149
 //   GGX helpers, E_ss LUT, VNDF            src/material.rs:151-321
 //   Dielectric, SpectralDielectric         src/material.rs:323-403
@@ -257,22 +258,105 @@ struct Material {
 };
 inline bool material_is_emissive(const Material* m) { return m && m->is_emissive(); }
 
+// Perlin::noise / turb (src/perlin.rs:35-88) over given tables: ranvec
+// (256 unit vectors) and three permutations of 0..255.
+struct Perlin {
+    float ranvec[256][3];
+    int perm_x[256], perm_y[256], perm_z[256];
+    // tables as the descriptor holds them: ranvec, then perm_x/y/z as floats
+    void load(const float* t) {
+        for (int k = 0; k < 256; k++) {
+            for (int a = 0; a < 3; a++) ranvec[k][a] = t[3 * k + a];
+            perm_x[k] = (int)t[768 + k]; perm_y[k] = (int)t[1024 + k]; perm_z[k] = (int)t[1280 + k];
+        }
+    }
+    float noise(const Point3& p) const {
+        float u = p.x - std::floor(p.x), v = p.y - std::floor(p.y), w = p.z - std::floor(p.z);
+        u = u * u * (3.0f - 2.0f * u);
+        v = v * v * (3.0f - 2.0f * v);
+        w = w * w * (3.0f - 2.0f * w);
+        int32_t i = f2i32(std::floor(p.x)), j = f2i32(std::floor(p.y)), k = f2i32(std::floor(p.z));
+        float accum = 0.0f;
+        for (int di = 0; di < 2; di++)
+            for (int dj = 0; dj < 2; dj++)
+                for (int dk = 0; dk < 2; dk++) {
+                    uint32_t ii = ((uint32_t)i + (uint32_t)di) & 255u, jj = ((uint32_t)j + (uint32_t)dj) & 255u,
+                             kk = ((uint32_t)k + (uint32_t)dk) & 255u;
+                    int idx = perm_x[ii] ^ perm_y[jj] ^ perm_z[kk];
+                    Vec3 c(ranvec[idx][0], ranvec[idx][1], ranvec[idx][2]);
+                    float fi = (float)di, fj = (float)dj, fk = (float)dk;
+                    Vec3 weight(u - fi, v - fj, w - fk);
+                    accum += (fi * u + (1.0f - fi) * (1.0f - u)) * (fj * v + (1.0f - fj) * (1.0f - v)) *
+                             (fk * w + (1.0f - fk) * (1.0f - w)) * c.dot(weight);
+                }
+        return accum;
+    }
+    float turb(const Point3& p, int depth) const {
+        float accum = 0.0f, weight = 1.0f;
+        Point3 q = p;
+        for (int d = 0; d < depth; d++) {
+            accum += weight * noise(q);
+            weight *= 0.5f;
+            q = Point3(q.x * 2.0f, q.y * 2.0f, q.z * 2.0f);
+        }
+        return std::fabs(accum);
+    }
+};
+
+// SRGB_DECODE_LUT (src/output.rs:88-95)
+static inline const float* srgb_decode_lut() {
+    static const std::vector<float> lut = [] {
+        std::vector<float> t(256);
+        for (int c = 0; c < 256; c++) {
+            float s = (float)c / 255.0f;
+            t[c] = s <= 0.04045f ? s / 12.92f : std::pow((s + 0.055f) / 1.055f, 2.4f);
+        }
+        return t;
+    }();
+    return lut.data();
+}
+
+struct RgbImage { uint32_t w = 0, h = 0; std::vector<uint8_t> rgb; };
+
+// Texture::value (src/texture.rs:26-56): kind 0 solid, 1 checker, 2 image,
+// 3 noise, 4 mottle (even = lo, odd = hi).
 struct Texture {
-    int kind = 0;  // 0 solid, 1 checker
+    int kind = 0;
     Color even, odd; float scale = 1.0f;
-    Color value(float, float, const Point3& p) const {
-        if (kind == 0) return even;
-        int32_t x = f2i32(std::floor(scale * p.x)), y = f2i32(std::floor(scale * p.y)), z = f2i32(std::floor(scale * p.z));
-        return ((x + y + z) % 2 == 0) ? even : odd;
+    std::shared_ptr<const RgbImage> img;
+    std::shared_ptr<const Perlin> perlin;
+    Color value(float u, float v, const Point3& p) const {
+        switch (kind) {
+            case 1: {
+                int32_t x = f2i32(std::floor(scale * p.x)), y = f2i32(std::floor(scale * p.y)), z = f2i32(std::floor(scale * p.z));
+                return ((x + y + z) % 2 == 0) ? even : odd;
+            }
+            case 2: {
+                float uc = rclamp(u, 0.0f, 1.0f), vc = 1.0f - rclamp(v, 0.0f, 1.0f);
+                uint32_t x = std::min(f2u32(uc * (float)img->w), img->w - 1), y = std::min(f2u32(vc * (float)img->h), img->h - 1);
+                const uint8_t* px = &img->rgb[3 * ((size_t)y * img->w + x)];
+                const float* lut = srgb_decode_lut();
+                return Color(lut[px[0]], lut[px[1]], lut[px[2]]);
+            }
+            case 3: {
+                float t = 1.0f + std::sin(scale * p.z + 10.0f * perlin->turb(p, 7));
+                return Color(0.5f, 0.5f, 0.5f) * t;
+            }
+            case 4: {
+                float m = rclamp(2.0f * perlin->turb(Point3(p.x * scale, p.y * scale, p.z * scale), 7), 0.0f, 1.0f);
+                return even * (1.0f - m) + odd * m;
+            }
+            default: return even;
+        }
     }
 };
 
 struct DiffuseLight : Material {
-    Color emit;
+    Texture emit;
     bool scatter(const Ray&, const HitRecord&, SmallRng&, ScatterRecord&) const override { return false; }
-    Color emitted(float, float, const Point3&) const override { return emit; }
-    Color albedo_hint(float, float, const Point3&) const override { return emit; }
-    bool is_emissive() const override { return emit.luminance() > 0.0f; }
+    Color emitted(float u, float v, const Point3& p) const override { return emit.value(u, v, p); }
+    Color albedo_hint(float u, float v, const Point3& p) const override { return emitted(u, v, p); }
+    bool is_emissive() const override { return emit.value(0.5f, 0.5f, Point3(0, 0, 0)).luminance() > 0.0f; }
 };
 
 // BlackbodyLight (src/material.rs:27-83)

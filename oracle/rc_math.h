@@ -40,6 +40,11 @@ static inline uint64_t f2usize(float x) {
     if (x >= 18446744073709551616.0f) return UINT64_MAX;
     return (uint64_t)x;
 }
+static inline uint32_t f2u32(float x) {
+    if (!(x > 0.0f)) return 0;
+    if (x >= 4294967296.0f) return UINT32_MAX;
+    return (uint32_t)x;
+}
 static inline int32_t f2i32(float x) {
     if (x != x) return 0;
     if (x <= -2147483648.0f) return INT32_MIN;

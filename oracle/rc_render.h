@@ -132,42 +132,10 @@ struct GridMedium : Hittable {
 // GridMedium with density max(turb(p * noise_scale, 7) / 2 - threshold, 0) *
 // density_scale, Perlin::noise / turb as src/perlin.rs:35-72 over given tables.
 struct NoiseMedium : GridMedium {
-    std::vector<float> ranvec;          // 256 x 3
-    int perm_x[256], perm_y[256], perm_z[256];
+    Perlin perlin;
     float noise_scale = 1.0f, threshold = 0.0f;
-    float noise(const Point3& p) const {
-        float u = p.x - std::floor(p.x), v = p.y - std::floor(p.y), w = p.z - std::floor(p.z);
-        u = u * u * (3.0f - 2.0f * u);
-        v = v * v * (3.0f - 2.0f * v);
-        w = w * w * (3.0f - 2.0f * w);
-        int32_t i = f2i32(std::floor(p.x)), j = f2i32(std::floor(p.y)), k = f2i32(std::floor(p.z));
-        float accum = 0.0f;
-        for (int di = 0; di < 2; di++)
-            for (int dj = 0; dj < 2; dj++)
-                for (int dk = 0; dk < 2; dk++) {
-                    uint32_t ii = ((uint32_t)i + (uint32_t)di) & 255u, jj = ((uint32_t)j + (uint32_t)dj) & 255u,
-                             kk = ((uint32_t)k + (uint32_t)dk) & 255u;
-                    int idx = perm_x[ii] ^ perm_y[jj] ^ perm_z[kk];
-                    Vec3 c(ranvec[3 * idx], ranvec[3 * idx + 1], ranvec[3 * idx + 2]);
-                    float fi = (float)di, fj = (float)dj, fk = (float)dk;
-                    Vec3 weight(u - fi, v - fj, w - fk);
-                    accum += (fi * u + (1.0f - fi) * (1.0f - u)) * (fj * v + (1.0f - fj) * (1.0f - v)) *
-                             (fk * w + (1.0f - fk) * (1.0f - w)) * c.dot(weight);
-                }
-        return accum;
-    }
-    float turb(const Point3& p, int depth) const {
-        float accum = 0.0f, weight = 1.0f;
-        Point3 q = p;
-        for (int d = 0; d < depth; d++) {
-            accum += weight * noise(q);
-            weight *= 0.5f;
-            q = Point3(q.x * 2.0f, q.y * 2.0f, q.z * 2.0f);
-        }
-        return std::fabs(accum);
-    }
     float density_at(const Point3& p) const override {
-        float n = turb(p * noise_scale, 7) * 0.5f;
+        float n = perlin.turb(p * noise_scale, 7) * 0.5f;
         return rmax(n - threshold, 0.0f) * density_scale;
     }
 };

@@ -29,14 +29,48 @@ struct OScene {
     std::string err;
 };
 
-Material* make_material(const rt_material& m) {
+static bool perlin_perms_ok(const float* t) {
+    for (int k = 768; k < 1536; k++)
+        if (!(t[k] >= 0.0f && t[k] <= 255.0f && t[k] == std::floor(t[k]))) return false;
+    return true;
+}
+
+// Texture of a LAMBERTIAN / DIFFUSE_LIGHT entry; images and Perlin tables
+// come from the descriptor (null: solid and checker only).
+static bool make_texture(const rt_material& m, const rt_scene_desc* d, Texture& t, std::string& err) {
+    t.kind = (int)m.tex;
+    t.even = Color(m.p[0], m.p[1], m.p[2]);
+    t.scale = m.p[3];
+    t.odd = Color(m.p[4], m.p[5], m.p[6]);
+    if (m.tex == RT_TEX_IMAGE) {
+        float f = m.p[7];
+        if (!d || !d->images || !(f >= 0.0f && f < (float)d->num_images && f == std::floor(f))) { err = "image index"; return false; }
+        const rt_image& im = d->images[(uint32_t)f];
+        if (!im.rgb || !im.width || !im.height) { err = "empty image"; return false; }
+        auto img = std::make_shared<RgbImage>();
+        img->w = im.width; img->h = im.height;
+        img->rgb.assign(im.rgb, im.rgb + (size_t)im.width * im.height * 3);
+        t.img = img;
+    } else if (m.tex == RT_TEX_NOISE || m.tex == RT_TEX_MOTTLE) {
+        float f = m.p[8];
+        if (!d || !d->perlin || !(f >= 0.0f && f < (float)d->num_perlin && f == std::floor(f))) { err = "Perlin table index"; return false; }
+        const float* tab = d->perlin + (size_t)1536 * (uint32_t)f;
+        if (!perlin_perms_ok(tab)) { err = "Perlin permutation entries must be integers in 0..255"; return false; }
+        auto pn = std::make_shared<Perlin>();
+        pn->load(tab);
+        t.perlin = pn;
+    } else if (m.tex > RT_TEX_MOTTLE) { err = "unsupported texture kind"; return false; }
+    return true;
+}
+
+Material* make_material(const rt_material& m, const rt_scene_desc* d, std::string& err) {
+    if (m.tex != RT_TEX_SOLID && m.kind != RT_MAT_LAMBERTIAN && m.kind != RT_MAT_DIFFUSE_LIGHT) {
+        err = "only LAMBERTIAN and DIFFUSE_LIGHT take a texture"; return nullptr;
+    }
     switch (m.kind) {
         case RT_MAT_LAMBERTIAN: {
             auto* l = new Lambertian();
-            l->tex.kind = (int)m.tex;
-            l->tex.even = Color(m.p[0], m.p[1], m.p[2]);
-            l->tex.scale = m.p[3];
-            l->tex.odd = Color(m.p[4], m.p[5], m.p[6]);
+            if (!make_texture(m, d, l->tex, err)) { delete l; return nullptr; }
             return l;
         }
         case RT_MAT_DIELECTRIC: { auto* d = new Dielectric(); d->ir = m.p[0]; return d; }
@@ -55,7 +89,11 @@ Material* make_material(const rt_material& m) {
             p->sheen = m.p[13]; p->sheen_tint = m.p[14];
             return p;
         }
-        case RT_MAT_DIFFUSE_LIGHT: { auto* d = new DiffuseLight(); d->emit = Color(m.p[0], m.p[1], m.p[2]); return d; }
+        case RT_MAT_DIFFUSE_LIGHT: {
+            auto* l = new DiffuseLight();
+            if (!make_texture(m, d, l->emit, err)) { delete l; return nullptr; }
+            return l;
+        }
         case RT_MAT_SPECTRAL_METAL: { auto* s = new SpectralMetal(); s->variant = (int)m.p[0]; s->roughness = m.p[1]; s->init(); return s; }
         case RT_MAT_ROUGH_DIELECTRIC: { auto* r = new RoughDielectric(); r->ior = m.p[0]; r->roughness = m.p[1]; return r; }
         case RT_MAT_BLACKBODY: { auto* b = new BlackbodyLight(); b->temp_k = m.p[0]; b->intensity = m.p[1]; b->init(); return b; }
@@ -178,10 +216,8 @@ HPtr make_object(OScene& S, const rt_scene_desc& d, const rt_object& o) {
             nm->phase.albedo = Color(m.albedo[0], m.albedo[1], m.albedo[2]); nm->phase.g = m.g;
             nm->noise_scale = m.noise_scale;
             nm->threshold = rclamp(m.threshold, 0.0f, 0.99f);
-            nm->ranvec.assign(m.perlin, m.perlin + 768);
-            for (int k = 0; k < 256; k++) {
-                nm->perm_x[k] = (int)m.perlin[768 + k]; nm->perm_y[k] = (int)m.perlin[1024 + k]; nm->perm_z[k] = (int)m.perlin[1280 + k];
-            }
+            if (!perlin_perms_ok(m.perlin)) { S.err = "noise medium permutation entries must be integers in 0..255"; return nullptr; }
+            nm->perlin.load(m.perlin);
             h->bounding_box(nm->box);
             h = nm;
         } else { S.err = "medium kind"; return nullptr; }
@@ -204,8 +240,9 @@ HPtr make_object(OScene& S, const rt_scene_desc& d, const rt_object& o) {
 
 bool build_scene(OScene& S, const rt_scene_desc& d) {
     for (uint32_t i = 0; i < d.num_materials; i++) {
-        Material* m = make_material(d.materials[i]);
-        if (!m) { S.err = "unsupported material kind"; return false; }
+        std::string err = "unsupported material kind";
+        Material* m = make_material(d.materials[i], &d, err);
+        if (!m) { S.err = "material " + std::to_string(i) + ": " + err; return false; }
         m->id = (int)i;
         S.mats.emplace_back(m);
         S.desc_mats.push_back(d.materials[i]);
@@ -434,10 +471,38 @@ void rtc_rng_range(uint64_t seed, float lo, float hi, uint32_t n, float* out) {
     for (uint32_t i = 0; i < n; i++) out[i] = r.gen_range(lo, hi);
 }
 
+// Texture::value of a LAMBERTIAN / DIFFUSE_LIGHT entry (images and Perlin
+// tables from `d`) at n points (u, v, x, y, z): n x 3 colours.
+int rtc_texture_eval(const rt_material* m, const rt_scene_desc* d, const float* uvp, uint32_t n, float* out) {
+    Texture t;
+    std::string err;
+    if (!make_texture(*m, d, t, err)) return -1;
+    for (uint32_t i = 0; i < n; i++) {
+        const float* q = uvp + (size_t)i * 5;
+        Color c = t.value(q[0], q[1], Point3(q[2], q[3], q[4]));
+        out[3 * i] = c.x; out[3 * i + 1] = c.y; out[3 * i + 2] = c.z;
+    }
+    return 0;
+}
+
+// Perlin::noise and Perlin::turb(depth) over 1536-float tables at n points.
+int rtc_perlin(const float* tables, const float* pts, uint32_t n, int depth, float* noise, float* turb) {
+    if (!perlin_perms_ok(tables)) return -1;
+    Perlin P;
+    P.load(tables);
+    for (uint32_t i = 0; i < n; i++) {
+        Point3 p(pts[3 * i], pts[3 * i + 1], pts[3 * i + 2]);
+        noise[i] = P.noise(p);
+        turb[i] = P.turb(p, depth);
+    }
+    return 0;
+}
+
 // Scalar functions by name: returns 0 on success.
 int rtc_eval(const char* fn, const float* a, float* out) {
     std::string f(fn);
     if (f == "ggx_ndf") { out[0] = ggx_ndf(a[0], a[1]); return 0; }
+    if (f == "srgb_decode") { int c = (int)a[0]; if (c < 0 || c > 255) return -1; out[0] = srgb_decode_lut()[c]; return 0; }
     if (f == "smith_g1") { out[0] = smith_g1(a[0], a[1]); return 0; }
     if (f == "smith_lambda") { out[0] = smith_lambda(a[0], a[1]); return 0; }
     if (f == "smith_g2") { out[0] = smith_g2_hc(a[0], a[1]); return 0; }
@@ -482,7 +547,8 @@ void rtc_e_lut(float* out) {
 // out: n x 8 = attenuation rgb, direction xyz, skip_pdf, ok.
 int rtc_scatter(const rt_material* m, const float* ray, int spectral_weighted, const float* rec, int front_face,
                 uint64_t seed, uint32_t n, float* out) {
-    std::unique_ptr<Material> mat(make_material(*m));
+    std::string err;
+    std::unique_ptr<Material> mat(make_material(*m, nullptr, err));
     if (!mat) return -1;
     Ray r = Ray::make(Point3(ray[0], ray[1], ray[2]), Vec3(ray[3], ray[4], ray[5]));
     r.wavelength = ray[6]; r.spectral_weighted = spectral_weighted != 0;

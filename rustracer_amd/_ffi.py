@@ -19,6 +19,7 @@ RT_ERR_NOMEM = -4
  SHAPE_PLANE) = 1, 2, 3, 4, 5, 6, 7, 8, 9
 XFORM_TRANSLATE, XFORM_ROTATE, XFORM_SCALE = 1, 2, 3
 MEDIUM_CONSTANT, MEDIUM_GRID, MEDIUM_NOISE = 1, 2, 3
+TEX_SOLID, TEX_CHECKER, TEX_IMAGE, TEX_NOISE, TEX_MOTTLE = 0, 1, 2, 3, 4
 (MAT_LAMBERTIAN, MAT_DIELECTRIC, MAT_SPECTRAL_DIELECTRIC, MAT_PBR, MAT_DIFFUSE_LIGHT,
  MAT_SPECTRAL_METAL, MAT_ROUGH_DIELECTRIC, MAT_ROUGH_SPECTRAL, MAT_BLACKBODY, MAT_SSS) = 1, 2, 3, 4, 5, 6, 7, 8, 9, 10
 BG_SOLID, BG_GRADIENT, BG_PHYSICAL, BG_ENVMAP, BG_ENVMAP_SPLIT = 0, 1, 2, 3, 4
@@ -57,6 +58,10 @@ class rt_background(C.Structure):
                 ("env_rgb", F32P), ("env_rot_y", C.c_float), ("env_sampling", C.c_uint32)]
 
 
+class rt_image(C.Structure):
+    _fields_ = [("width", C.c_uint32), ("height", C.c_uint32), ("rgb", C.POINTER(C.c_uint8))]
+
+
 class rt_scene_desc(C.Structure):
     _fields_ = [("objects", C.POINTER(rt_object)), ("num_objects", C.c_uint32),
                 ("lights", C.POINTER(rt_object)), ("num_lights", C.c_uint32),
@@ -65,7 +70,9 @@ class rt_scene_desc(C.Structure):
                 ("media", C.POINTER(rt_medium)), ("num_media", C.c_uint32),
                 ("pool", F32P), ("pool_len", C.c_uint32),
                 ("tri_v", F32P), ("tri_n", F32P), ("tri_t", F32P), ("tri_uv", F32P),
-                ("num_tris", C.c_uint32), ("background", rt_background)]
+                ("num_tris", C.c_uint32), ("background", rt_background),
+                ("images", C.POINTER(rt_image)), ("num_images", C.c_uint32),
+                ("perlin", F32P), ("num_perlin", C.c_uint32)]
 
 
 class rt_ray(C.Structure):

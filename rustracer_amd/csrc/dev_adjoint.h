@@ -192,7 +192,7 @@ RT_D V3 vertex_ratio(const DScene& S, const DMaterial* me, const Rec& h, V3 wo, 
     const DMaterial& ms = S.mats[h.mat];
     const DMaterial& mv = me[h.mat];
     if (ms.kind == M_LAMBERTIAN) {
-        V3 a = tex_value(ms, h.p), b = tex_value(mv, h.p);
+        V3 a = tex_value(S, ms, h.u, h.v, h.p), b = tex_value(S, mv, h.u, h.v, h.p);
         return mk(a.x != 0.0f ? b.x / a.x : 1.0f, a.y != 0.0f ? b.y / a.y : 1.0f, a.z != 0.0f ? b.z / a.z : 1.0f);
     }
     if (ms.kind == M_ROUGH_SPECTRAL) {

@@ -48,7 +48,10 @@ struct DMedium {
     float noise_scale, threshold;               // NOISE
 };
 
-struct DMaterial { uint32_t kind, tex; float p[24]; };
+// toff: RT_TEX_IMAGE byte offset of the image in DScene::tex_pix (tw x th
+// texels), RT_TEX_NOISE / RT_TEX_MOTTLE float offset of the Perlin tables in
+// DScene::grids.
+struct DMaterial { uint32_t kind, tex; float p[24]; uint32_t toff, tw, th, pad; };
 
 struct DScene {
     const float4* nodes;
@@ -61,6 +64,7 @@ struct DScene {
     const float4* planes;
     const DMaterial* mats;
     const DMedium* media; const float* grids;
+    const uint8_t* tex_pix; const float* srgb_lut;   // image texels (sRGB bytes) and their 256-entry decode table
     // background
     uint32_t bg_kind, bg_visible;
     float bg_color[3], bg_top[3], bg_bottom[3], sun_dir[3], turbidity;

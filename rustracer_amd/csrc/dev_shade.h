@@ -158,11 +158,31 @@ RT_D float energy_comp(const DScene& S, float mu, float al) { return fmn(1.0f / 
 
 struct Scatter { V3 att; Ray ray; bool skip_pdf; };
 
-RT_D V3 tex_value(const DMaterial& m, V3 p) {
-    V3 even = mk(m.p[0], m.p[1], m.p[2]);
-    if (m.tex == 0) return even;
-    int32_t x = f2i(floorf(m.p[3] * p.x)), y = f2i(floorf(m.p[3] * p.y)), z = f2i(floorf(m.p[3] * p.z));
-    return ((x + y + z) % 2 == 0) ? even : mk(m.p[4], m.p[5], m.p[6]);
+// Texture::value (src/texture.rs:26-56).
+RT_D V3 tex_value(const DScene& S, const DMaterial& m, float u, float v, V3 p) {
+    V3 a = mk(m.p[0], m.p[1], m.p[2]);
+    switch (m.tex) {
+        case 1: {
+            int32_t x = f2i(floorf(m.p[3] * p.x)), y = f2i(floorf(m.p[3] * p.y)), z = f2i(floorf(m.p[3] * p.z));
+            return ((x + y + z) % 2 == 0) ? a : mk(m.p[4], m.p[5], m.p[6]);
+        }
+        case 2: {
+            float uc = fclamp(u, 0.0f, 1.0f), vc = 1.0f - fclamp(v, 0.0f, 1.0f);
+            uint32_t x = min(f2u(uc * (float)m.tw), m.tw - 1), y = min(f2u(vc * (float)m.th), m.th - 1);
+            const uint8_t* px = S.tex_pix + m.toff + 3 * ((size_t)y * m.tw + x);
+            return mk(S.srgb_lut[px[0]], S.srgb_lut[px[1]], S.srgb_lut[px[2]]);
+        }
+        case 3: {
+            float t = 1.0f + sinf(m.p[3] * p.z + 10.0f * perlin_turb(S.grids + m.toff, p, 7));
+            return mk(0.5f * t, 0.5f * t, 0.5f * t);
+        }
+        case 4: {
+            float sc = m.p[3];
+            float w = fclamp(2.0f * perlin_turb(S.grids + m.toff, mk(p.x * sc, p.y * sc, p.z * sc), 7), 0.0f, 1.0f);
+            return a * (1.0f - w) + mk(m.p[4], m.p[5], m.p[6]) * w;
+        }
+        default: return a;
+    }
 }
 
 // PbrMaterial parameter block: p0-2 albedo, p3 roughness, p4 metallic,
@@ -475,7 +495,7 @@ RT_D bool mat_scatter(const DScene& S, const Rec& h, const Ray& r, Rng& rng, Sca
     const DMaterial& m = S.mats[h.mat];
     switch (m.kind) {
         case M_LAMBERTIAN:
-            sr.att = tex_value(m, h.p); sr.ray = scatter_from(h.p, h.n, r); sr.skip_pdf = false;
+            sr.att = tex_value(S, m, h.u, h.v, h.p); sr.ray = scatter_from(h.p, h.n, r); sr.skip_pdf = false;
             return true;
         case M_DIELECTRIC: {
             bool refl;
@@ -519,7 +539,7 @@ RT_HD float planck_raw(float lambda_nm, float temp_k) {
 RT_D V3 mat_emitted(const DScene& S, const Rec& h, const Ray& r) {
     if (h.medium) return mk(0, 0, 0);
     const DMaterial& m = S.mats[h.mat];
-    if (m.kind == M_DIFFUSE_LIGHT) return mk(m.p[0], m.p[1], m.p[2]);
+    if (m.kind == M_DIFFUSE_LIGHT) return tex_value(S, m, h.u, h.v, h.p);
     if (m.kind == M_BLACKBODY) {
         // BlackbodyLight::emitted_at (src/material.rs:69-80)
         if (r.spectral) { float sc = planck_raw(r.wavelength, m.p[0]) * m.p[2] * m.p[1]; return mk(sc, sc, sc); }
@@ -532,9 +552,9 @@ RT_D float mat_scattering_pdf(const Rec& h, V3 dir) { return fmx(dot(h.n, unit(d
 RT_D V3 mat_albedo_hint(const DScene& S, const Rec& h) {
     if (h.medium) return mk(1, 1, 1);
     const DMaterial& m = S.mats[h.mat];
-    if (m.kind == M_LAMBERTIAN) return tex_value(m, h.p);
+    if (m.kind == M_LAMBERTIAN) return tex_value(S, m, h.u, h.v, h.p);
     if (m.kind == M_PBR) return mk(m.p[0], m.p[1], m.p[2]);
-    if (m.kind == M_DIFFUSE_LIGHT) return mk(m.p[0], m.p[1], m.p[2]);
+    if (m.kind == M_DIFFUSE_LIGHT) return tex_value(S, m, h.u, h.v, h.p);
     if (m.kind == M_BLACKBODY) return mk(m.p[3], m.p[4], m.p[5]);
     if (m.kind == M_SSS) return mk(m.p[0], m.p[1], m.p[2]);
     return mk(1, 1, 1);

@@ -965,6 +965,14 @@ struct Flat {
     std::string err;
 };
 
+// Perlin tables (ranvec[256][3], perm_x, perm_y, perm_z as floats): the
+// permutations index the tables, so they must be integers in 0..255.
+static bool perlin_perms_ok(const float* t) {
+    for (int k = 768; k < 1536; k++)
+        if (!(t[k] >= 0.0f && t[k] <= 255.0f && t[k] == std::floor(t[k]))) return false;
+    return true;
+}
+
 bool flatten_object(Flat& F, const rt_scene_desc& d, const rt_object& o, DObject& D, Box& bb, uint32_t& extra, bool is_light) {
     memset(&D, 0, sizeof D);
     D.shape = o.shape;
@@ -1118,10 +1126,7 @@ bool flatten_object(Flat& F, const rt_scene_desc& d, const rt_object& o, DObject
             F.grids.insert(F.grids.end(), m.grid, m.grid + (size_t)M.nx * M.ny * M.nz);
         } else if (m.kind == RT_MEDIUM_NOISE) {
             if (!m.perlin) { F.err = "noise medium without Perlin tables"; return false; }
-            for (int k = 768; k < 1536; k++)
-                if (!(m.perlin[k] >= 0.0f && m.perlin[k] <= 255.0f && m.perlin[k] == std::floor(m.perlin[k]))) {
-                    F.err = "noise medium permutation entries must be integers in 0..255"; return false;
-                }
+            if (!perlin_perms_ok(m.perlin)) { F.err = "noise medium permutation entries must be integers in 0..255"; return false; }
             // NoiseMedium::new (src/volume.rs:170-188)
             M.density = std::fmax(m.density, 1.1920929e-7f);
             M.threshold = std::fmin(std::fmax(m.threshold, 0.0f), 0.99f);
@@ -1252,11 +1257,45 @@ extern "C" int rt_scene_upload(rt_ctx* ctx, const rt_scene_desc* d, rt_scene** o
     if ((rc = upload_vec(s.get(), F.quads, &S.quads))) return rc;
     if ((rc = upload_vec(s.get(), F.planes, &S.planes))) return rc;
     if ((rc = upload_vec(s.get(), F.media, &S.media))) return rc;
-    if ((rc = upload_vec(s.get(), F.grids, &S.grids))) return rc;
     std::vector<DMaterial> mats(d->num_materials);
+    std::vector<uint8_t> texels;
+    std::vector<int64_t> image_off(d->num_images, -1), perlin_off(d->num_perlin, -1);
     for (uint32_t i = 0; i < d->num_materials; i++) {
-        mats[i].kind = d->materials[i].kind; mats[i].tex = d->materials[i].tex;
-        for (int k = 0; k < 24; k++) mats[i].p[k] = d->materials[i].p[k];
+        const rt_material& dm = d->materials[i];
+        memset(&mats[i], 0, sizeof(DMaterial));
+        mats[i].kind = dm.kind; mats[i].tex = dm.tex;
+        for (int k = 0; k < 24; k++) mats[i].p[k] = dm.p[k];
+        if (dm.tex != RT_TEX_SOLID) {
+            std::string tag = "material " + std::to_string(i) + ": ";
+            if (dm.kind != RT_MAT_LAMBERTIAN && dm.kind != RT_MAT_DIFFUSE_LIGHT)
+                return fail(ctx, RT_ERR_INVALID, tag + "only LAMBERTIAN and DIFFUSE_LIGHT take a texture");
+            if (dm.tex > RT_TEX_MOTTLE) return fail(ctx, RT_ERR_UNSUPPORTED, tag + "unsupported texture kind " + std::to_string(dm.tex));
+            if (dm.tex == RT_TEX_IMAGE) {
+                float f = dm.p[7];
+                if (!d->images || !(f >= 0.0f && f < (float)d->num_images && f == std::floor(f))) return fail(ctx, RT_ERR_INVALID, tag + "image index");
+                uint32_t k = (uint32_t)f;
+                const rt_image& im = d->images[k];
+                if (!im.rgb || !im.width || !im.height) return fail(ctx, RT_ERR_INVALID, tag + "empty image");
+                size_t bytes = (size_t)im.width * im.height * 3;
+                if (image_off[k] < 0) {
+                    if (texels.size() + bytes > 0xffffffffull) return fail(ctx, RT_ERR_UNSUPPORTED, "texture images exceed 4 GiB");
+                    image_off[k] = (int64_t)texels.size();
+                    texels.insert(texels.end(), im.rgb, im.rgb + bytes);
+                }
+                mats[i].toff = (uint32_t)image_off[k]; mats[i].tw = im.width; mats[i].th = im.height;
+            } else if (dm.tex == RT_TEX_NOISE || dm.tex == RT_TEX_MOTTLE) {
+                float f = dm.p[8];
+                if (!d->perlin || !(f >= 0.0f && f < (float)d->num_perlin && f == std::floor(f))) return fail(ctx, RT_ERR_INVALID, tag + "Perlin table index");
+                uint32_t k = (uint32_t)f;
+                const float* t = d->perlin + (size_t)1536 * k;
+                if (!perlin_perms_ok(t)) return fail(ctx, RT_ERR_INVALID, tag + "Perlin permutation entries must be integers in 0..255");
+                if (perlin_off[k] < 0) {
+                    perlin_off[k] = (int64_t)F.grids.size();
+                    F.grids.insert(F.grids.end(), t, t + 1536);
+                }
+                mats[i].toff = (uint32_t)perlin_off[k];
+            }
+        }
         if (mats[i].kind == RT_MAT_BLACKBODY) {
             // BlackbodyLight::new (src/material.rs:49-61): normalising mean of
             // Planck over the 65 CIE samples and the white-point average colour
@@ -1281,6 +1320,17 @@ extern "C" int rt_scene_upload(rt_ctx* ctx, const rt_scene_desc* d, rt_scene** o
             mats[i].p[2] = norm;
             for (int a = 0; a < 3; a++) mats[i].p[3 + a] = acc[a] / (float)RT_CIE_N * intensity;
         }
+    }
+    if ((rc = upload_vec(s.get(), F.grids, &S.grids))) return rc;
+    if ((rc = upload_vec(s.get(), texels, &S.tex_pix))) return rc;
+    {
+        // SRGB_DECODE_LUT (src/output.rs:88-95), built with the host's powf
+        std::vector<float> lut(256);
+        for (int c = 0; c < 256; c++) {
+            float v = (float)c / 255.0f;
+            lut[c] = v <= 0.04045f ? v / 12.92f : std::pow((v + 0.055f) / 1.055f, 2.4f);
+        }
+        if ((rc = upload_vec(s.get(), lut, &S.srgb_lut))) return rc;
     }
     if ((rc = upload_vec(s.get(), mats, &S.mats))) return rc;
     s->mats_host = mats;

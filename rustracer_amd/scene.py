@@ -32,45 +32,135 @@ def cross(a, b):
     return np.array([a[1] * b[2] - a[2] * b[1], a[2] * b[0] - a[0] * b[2], a[0] * b[1] - a[1] * b[0]], dtype=np.float32)
 
 
+# ---------------------------------------------------------------- textures
+def perlin_tables(seed):
+    """Perlin::new (src/perlin.rs:11-33) with numpy draws: 256 unit vectors
+    from the cube [-1, 1]^3, then three permutations of 0..255; returned as
+    the 1536 floats the descriptor takes (ranvec, perm_x, perm_y, perm_z).
+    The reference draws from thread_rng (NoiseMedium) or StdRng(0x5EED)
+    (Mottle); a host binding passes its own Perlin's tables instead."""
+    rng = np.random.default_rng(seed)
+    v = rng.uniform(-1.0, 1.0, size=(256, 3)).astype(np.float32)
+    v = v / np.sqrt((v * v).sum(1, keepdims=True))
+    perms = [rng.permutation(256).astype(np.float32) for _ in range(3)]
+    return np.ascontiguousarray(np.concatenate([v.astype(np.float32).ravel()] + perms), dtype=np.float32)
+
+
+class Texture:
+    """Texture (src/texture.rs:7-18): `tex` kind and the rt_material.p slots
+    it fills; `image` / `perlin` are filled with descriptor indices."""
+    tex = F.TEX_SOLID
+    image = None
+    perlin = None
+
+    def slots(self, image_index=0, perlin_index=0):
+        return {}
+
+
+class SolidTexture(Texture):
+    def __init__(self, color):
+        self.color = v3(color)
+
+    def slots(self, image_index=0, perlin_index=0):
+        return {0: self.color[0], 1: self.color[1], 2: self.color[2]}
+
+
+class CheckerTexture(Texture):
+    """Texture::Checker: parity of floor(scale * p) (src/texture.rs:29-34)."""
+    tex = F.TEX_CHECKER
+
+    def __init__(self, even, odd, scale):
+        self.even, self.odd, self.scale = v3(even), v3(odd), float(scale)
+
+    def slots(self, image_index=0, perlin_index=0):
+        e, o = self.even, self.odd
+        return {0: e[0], 1: e[1], 2: e[2], 3: self.scale, 4: o[0], 5: o[1], 6: o[2]}
+
+
+class ImageTexture(Texture):
+    """Texture::Image (src/texture.rs:35-44): nearest texel of an 8-bit sRGB
+    image (H x W x 3, row 0 at the top), decoded to linear."""
+    tex = F.TEX_IMAGE
+
+    def __init__(self, rgb):
+        a = np.ascontiguousarray(rgb, dtype=np.uint8)
+        if a.ndim != 3 or a.shape[2] != 3 or a.shape[0] == 0 or a.shape[1] == 0:
+            raise ValueError("image texture must be H x W x 3 uint8")
+        self.image = a
+
+    def slots(self, image_index=0, perlin_index=0):
+        return {7: float(image_index)}
+
+
+class NoiseTexture(Texture):
+    """Texture::Noise (src/texture.rs:45-48): grey
+    0.5 (1 + sin(scale p.z + 10 turb(p, 7)))."""
+    tex = F.TEX_NOISE
+
+    def __init__(self, scale, seed=0, tables=None):
+        self.scale = float(scale)
+        self.perlin = perlin_tables(seed) if tables is None else np.ascontiguousarray(tables, dtype=np.float32)
+
+    def slots(self, image_index=0, perlin_index=0):
+        return {3: self.scale, 8: float(perlin_index)}
+
+
+class MottleTexture(Texture):
+    """Texture::Mottle (src/texture.rs:49-52): lerp(lo, hi,
+    clamp(2 turb(scale p, 7), 0, 1))."""
+    tex = F.TEX_MOTTLE
+
+    def __init__(self, scale, lo, hi, seed=0, tables=None):
+        self.scale, self.lo, self.hi = float(scale), v3(lo), v3(hi)
+        self.perlin = perlin_tables(seed) if tables is None else np.ascontiguousarray(tables, dtype=np.float32)
+
+    def slots(self, image_index=0, perlin_index=0):
+        lo, hi = self.lo, self.hi
+        return {0: lo[0], 1: lo[1], 2: lo[2], 3: self.scale, 4: hi[0], 5: hi[1], 6: hi[2], 8: float(perlin_index)}
+
+
+def as_texture(t):
+    return t if isinstance(t, Texture) else SolidTexture(t)
+
+
 # ---------------------------------------------------------------- materials
 class Material:
     kind = 0
     tex = 0
+    texture = None
 
     def params(self):
         return []
 
-    def to_c(self):
+    def to_c(self, image_index=0, perlin_index=0):
         m = F.rt_material()
         m.kind = self.kind
-        m.tex = self.tex
+        m.tex = self.texture.tex if self.texture is not None else self.tex
         for i, v in enumerate(self.params()):
             m.p[i] = float(v)
+        if self.texture is not None:
+            for i, v in self.texture.slots(image_index, perlin_index).items():
+                m.p[i] = float(v)
         return m
 
 
 class Lambertian(Material):
-    """src/material.rs:85-102 with a Solid texture."""
+    """src/material.rs:85-102: albedo from a texture (a colour is a Solid one)."""
     kind = F.MAT_LAMBERTIAN
 
     def __init__(self, color):
-        self.color = v3(color)
+        self.texture = as_texture(color)
 
-    def params(self):
-        return list(self.color)
+    @property
+    def color(self):
+        return getattr(self.texture, "color", None)
 
 
 class Checker(Lambertian):
     """Lambertian over Texture::Checker (src/texture.rs:28-33, src/scene_file.rs:1146-1147)."""
-    tex = 1
 
     def __init__(self, even, odd, scale):
-        self.color = v3(even)
-        self.odd = v3(odd)
-        self.scale = float(scale)
-
-    def params(self):
-        return list(self.color) + [self.scale] + list(self.odd)
+        self.texture = CheckerTexture(even, odd, scale)
 
 
 class Dielectric(Material):
@@ -135,14 +225,15 @@ def Metal(color, fuzz):
 
 
 class DiffuseLight(Material):
-    """src/material.rs:14-25."""
+    """src/material.rs:14-25: emission from a texture (a colour is a Solid one)."""
     kind = F.MAT_DIFFUSE_LIGHT
 
     def __init__(self, emit):
-        self.emit = v3(emit)
+        self.texture = as_texture(emit)
 
-    def params(self):
-        return list(self.emit)
+    @property
+    def emit(self):
+        return getattr(self.texture, "color", None)
 
 
 class SpectralMetal(Material):
@@ -522,17 +613,12 @@ class NoiseMedium:
 
     def __init__(self, boundary, color, density, noise_scale=1.0, threshold=0.0, g=0.0, seed=0):
         self.boundary = boundary
-        rng = np.random.default_rng(seed)
-        v = rng.uniform(-1.0, 1.0, size=(256, 3)).astype(np.float32)
-        v = v / np.sqrt((v * v).sum(1, keepdims=True))
-        perms = [rng.permutation(256).astype(np.float32) for _ in range(3)]
         boundary.medium = F.rt_medium(kind=F.MEDIUM_NOISE, density=float(density), g=float(g),
                                       noise_scale=float(noise_scale), threshold=float(threshold))
         for k in range(3):
             boundary.medium.albedo[k] = float(v3(color)[k])
         boundary.medium_grid = None
-        boundary.medium_perlin = np.ascontiguousarray(np.concatenate([v.astype(np.float32).ravel()] + perms),
-                                                      dtype=np.float32)
+        boundary.medium_perlin = perlin_tables(seed)
 
 
 # --------------------------------------------------------------- background
@@ -709,7 +795,31 @@ class Scene:
         d.objects, d.num_objects = C.cast(keep["objs"], C.POINTER(F.rt_object)), len(objs)
         keep["lights"] = (F.rt_object * max(len(lights), 1))(*lights)
         d.lights, d.num_lights = C.cast(keep["lights"], C.POINTER(F.rt_object)), len(lights)
-        cm = [m.to_c() for m in mats]
+        images, image_index, perlins, perlin_index = [], {}, [], {}
+        cm = []
+        for m in mats:
+            t = m.texture
+            ii = pi = 0
+            if t is not None and t.image is not None:
+                if id(t) not in image_index:
+                    image_index[id(t)] = len(images)
+                    images.append(t.image)
+                ii = image_index[id(t)]
+            if t is not None and t.perlin is not None:
+                if id(t) not in perlin_index:
+                    perlin_index[id(t)] = len(perlins)
+                    perlins.append(t.perlin)
+                pi = perlin_index[id(t)]
+            cm.append(m.to_c(ii, pi))
+        if images:
+            ims = [F.rt_image(width=a.shape[1], height=a.shape[0], rgb=a.ctypes.data_as(C.POINTER(C.c_uint8))) for a in images]
+            keep["images"] = (F.rt_image * len(ims))(*ims)
+            keep["image_data"] = images
+            d.images, d.num_images = C.cast(keep["images"], C.POINTER(F.rt_image)), len(ims)
+        if perlins:
+            PT = np.ascontiguousarray(np.concatenate(perlins), dtype=np.float32)
+            keep["perlin"] = PT
+            d.perlin, d.num_perlin = PT.ctypes.data_as(F.F32P), len(perlins)
         keep["mats"] = (F.rt_material * max(len(cm), 1))(*cm)
         d.materials, d.num_materials = C.cast(keep["mats"], C.POINTER(F.rt_material)), len(cm)
         keep["xf"] = (F.rt_xform * max(len(xforms), 1))(*xforms)

@@ -369,3 +369,42 @@ def test_blackbody_light_and_sss_same_streams_and_three_sigma(gpu_renderer):
     assert np.isfinite(g).all()
     z = three_sigma(g, c)
     assert (z > 3).mean() <= 0.02 and (z > 6).mean() <= 0.002, ((z > 3).mean(), (z > 6).mean())
+
+
+def test_textures_same_streams_and_three_sigma(gpu_renderer):
+    """Image, noise, mottle and checker albedo textures and a checker-textured
+    area light (emission and NEE both read the light's (u, v, p))."""
+    from rustracer_amd import scene as S
+    img = ((np.arange(6 * 8 * 3).reshape(6, 8, 3) * 37) % 256).astype(np.uint8)
+    emit = S.DiffuseLight(S.CheckerTexture((6.0, 5.0, 4.0), (2.0, 3.0, 6.0), 5.0))
+    lq = S.Quad((-0.4, 1.8, -0.4), (0.8, 0, 0), (0, 0, 0.8), emit)
+    objs = [S.Quad((-4, 0, -4), (8, 0, 0), (0, 0, 8), S.Lambertian(S.MottleTexture(2.0, (0.3, 0.3, 0.32), (0.7, 0.6, 0.5), seed=4))),
+            S.Quad((-1.5, 0, 1.2), (3, 0, 0), (0, 2, 0), S.Lambertian(S.ImageTexture(img))),
+            S.Sphere((-0.5, 0.4, 0.2), 0.4, S.Lambertian(S.NoiseTexture(3.0, seed=6))),
+            S.Sphere((0.5, 0.35, 0.0), 0.35, S.Checker((0.9, 0.9, 0.9), (0.1, 0.2, 0.6), 6.0)), lq]
+    sc = S.Scene(objs, [lq], S.Background.solid((0.05, 0.05, 0.06)))
+    cam = S.Camera((0.0, 1.0, -2.6), (0.0, 0.6, 0.0), (0, 1, 0), 45.0, 1.0)
+    gpu_renderer.upload(sc)
+    orc = O.OracleScene(sc)
+    g = passes_gpu(gpu_renderer, cam, 40, 40, 4, 8)
+    c = passes_cpu(orc, cam, 40, 40, 4, 8, mode=1)
+    close = np.isclose(g, c, rtol=1e-3, atol=1e-5).all(axis=-1)
+    assert close.mean() > 0.95, close.mean()
+    K = 96
+    g = passes_gpu(gpu_renderer, cam, 32, 32, K, 8, seed=2)
+    c = passes_cpu(orc, cam, 32, 32, K, 8, mode=0)
+    assert np.isfinite(g).all()
+    z = three_sigma(g, c)
+    assert (z > 3).mean() <= 0.02 and (z > 6).mean() <= 0.002, ((z > 3).mean(), (z > 6).mean())
+
+
+def test_texture_upload_rejections(gpu_renderer):
+    from rustracer_amd import scene as S
+    bad = S.Dielectric(1.5)
+    bad.texture = S.NoiseTexture(1.0)
+    with pytest.raises(RuntimeError):
+        gpu_renderer.upload(S.Scene([S.Sphere((0, 0, 0), 1.0, bad)]))
+    t = S.perlin_tables(0)
+    t[800] = 300.0
+    with pytest.raises(RuntimeError):
+        gpu_renderer.upload(S.Scene([S.Sphere((0, 0, 0), 1.0, S.Lambertian(S.NoiseTexture(1.0, tables=t)))]))
