@@ -46,7 +46,7 @@ def workloads():
     return {
         "box": lambda: scenes.box_room(256, 256),
         "mesh": lambda: scenes.mesh_envmap(1280, 720, level=8),
-        "gem": lambda: scenes.gem_caustics(1280, 720),
+        "gem": lambda: scenes.gem_caustics(1280, 720, **scenes.GEM_BENCH),
     }
 
 
@@ -76,7 +76,7 @@ def measure(name, args, with_cpu):
         lq = cfg["light_quad"]
         t0 = time.time()
         stored = r.photon_build_from_quad(lq["q"], lq["u"], lq["v"], cfg["light_color"], cfg["num_photons"],
-                                          cfg["gather_radius"])
+                                          cfg["gather_radius"], knn=cfg.get("knn", 0))
         ph = {"traced": cfg["num_photons"], "stored": stored, "build_s": round(time.time() - t0, 3)}
     acc = torch.zeros((H, W, 3), dtype=torch.float32, device="cuda")
     log(f"[{name}] scene {build_s:.1f}s upload {upload_s:.1f}s, {stats['nodes']} nodes, {stats['triangles']} tris")
@@ -212,7 +212,7 @@ def measure_batch(args, with_cpu):
         if use_ph:
             lq = cfg["light_quad"]
             photons += r.photon_build_from_quad(lq["q"], lq["u"], lq["v"], cfg["light_color"], cfg["num_photons"],
-                                                cfg["gather_radius"], scene=h)
+                                                cfg["gather_radius"], scene=h, knn=cfg.get("knn", 0))
         W, H = cfg["width"], cfg["height"]
         views.append(dict(scene=h, camera=cam, width=W, height=H, spp=cfg["spp"], max_depth=cfg["max_depth"],
                           photon_map=use_ph, strata=int(cfg["spp"] ** 0.5),
@@ -460,7 +460,7 @@ def measure_rmse(args, with_cpu=True):
     from rustracer_amd import _ffi as F
 
     W, H, spp, ref_spp = 320, 180, 64, 4096
-    sc, cam, cfg = scenes.gem_caustics(W, H)
+    sc, cam, cfg = scenes.gem_caustics(W, H, **scenes.GEM_BENCH)
     cores = O.threads()
     orc = O.OracleScene(sc)
     lq = cfg["light_quad"]
@@ -468,7 +468,7 @@ def measure_rmse(args, with_cpu=True):
     pp.emitter = F.EMIT_QUAD
     for k in range(3):
         pp.origin[k], pp.u[k], pp.v[k], pp.color[k] = lq["q"][k], lq["u"][k], lq["v"][k], cfg["light_color"][k]
-    pp.num_photons, pp.gather_radius = cfg["num_photons"], cfg["gather_radius"]
+    pp.num_photons, pp.gather_radius, pp.knn = cfg["num_photons"], cfg["gather_radius"], cfg["knn"]
     t0 = time.time()
     nph = orc.photon_build(pp, nthreads=cores)
     photons = orc.photon_dump(nph)
@@ -486,6 +486,7 @@ def measure_rmse(args, with_cpu=True):
     K = 16
     r = Renderer(0).upload(sc)
     r.photon_import(photons, cfg["gather_radius"])
+    r.photon_set_gather(cfg["gather_radius"], cfg["knn"])
     a = torch.zeros((H, W, 3), dtype=torch.float32, device="cuda")
     m_wf, m_plain = [], []
     for k in range(K):
@@ -553,7 +554,7 @@ def cpu_baseline(scene, cam, cfg, seconds, photons=False):
         pp.emitter = F.EMIT_QUAD
         for k in range(3):
             pp.origin[k], pp.u[k], pp.v[k], pp.color[k] = lq["q"][k], lq["u"][k], lq["v"][k], cfg["light_color"][k]
-        pp.num_photons, pp.gather_radius = cfg["num_photons"], cfg["gather_radius"]
+        pp.num_photons, pp.gather_radius, pp.knn = cfg["num_photons"], cfg["gather_radius"], cfg.get("knn", 0)
         orc.photon_build(pp, nthreads=cores)
     W, H, spp = cfg["width"], cfg["height"], cfg["spp"]
     strata = int(spp ** 0.5)

@@ -295,7 +295,10 @@ typedef struct {
 
 /* ---- wavefront renderer options ----------------------------------------------
  * capacity: paths in flight per chunk (0: 4M); grid: blocks of the persistent
- * stage kernels (0: 2048); sort: 1 files paths by material before shading;
+ * stage kernels (0: 2048); sort: 0 shades paths in extension order, 1 files
+ * them by material kind first, 2 splits them into two shade classes (misses,
+ * media and the specular materials; everything else), each shaded by a kernel
+ * compiled for its materials only;
  * use_graph: 1 captures one chunk's launches in a hipGraph and replays it;
  * occupancy: register budget of the traversal stages — 0 compiler's choice,
  * 4 or 5: at least that many waves per SIMD (fewer VGPRs, some spills).      */
@@ -317,8 +320,16 @@ typedef struct {
     float    sun_dir[3];       /* SUN_DISK                                       */
     uint32_t num_photons;
     float    gather_radius;
-    uint32_t knn;              /* 0 = fixed-radius gather (reference); k > 0 = k-NN extension */
+    uint32_t knn;              /* 0: fixed-radius gather (PhotonMap::irradiance,
+                                  src/photon.rs:148-157); 1..RT_KNN_MAX: k-NN
+                                  gather (extension): the squared radius is the
+                                  k-th smallest squared distance among the
+                                  hemisphere's photons closer than gather_radius
+                                  (gather_radius when fewer), normalised by
+                                  2 / (pi^2 R^2)                               */
 } rt_photon_params;
+
+#define RT_KNN_MAX 64
 
 /* ---- context ---------------------------------------------------------------- */
 int         rt_ctx_create(int device, rt_ctx **out);
@@ -446,6 +457,10 @@ int  rt_photon_build(rt_ctx *ctx, rt_scene *scene, const rt_photon_params *pp,
  * layout rt_photon_export writes; the device orders them for its gather.    */
 int  rt_photon_import(rt_ctx *ctx, rt_scene *scene, const float *photons, uint32_t n,
                       float gather_radius);
+/* Change the gather of the attached map: fixed radius (knn 0) or k-NN with
+ * gather_radius as the search bound (knn 1..RT_KNN_MAX; see rt_photon_params).
+ * No counterpart in the reference, which gathers in a fixed radius.          */
+int  rt_photon_set_gather(rt_ctx *ctx, rt_scene *scene, float gather_radius, uint32_t knn);
 /* Irradiance queries against the attached map: PhotonMap::irradiance
  * (src/photon.rs:148-157).  pos_nrm: n * 6 floats (p.xyz, n.xyz).            */
 int  rt_photon_irradiance(rt_ctx *ctx, rt_scene *scene, const float *pos_nrm,

@@ -56,6 +56,7 @@ def lib():
         L.rtc_env_alias.argtypes = [vp, C.c_uint32, C.c_uint32, vp, vp]
         L.rtc_env_sample_mode.argtypes = [vp, C.c_uint32, C.c_uint32, C.c_float, C.c_int, C.c_uint64, C.c_uint32, vp]
         L.rtc_texture_eval.argtypes = [vp, vp, vp, C.c_uint32, vp]
+        L.rtc_photon_set_gather.argtypes = [vp, C.c_float, C.c_uint32]
         L.rtc_perlin.argtypes = [vp, vp, C.c_uint32, C.c_int, vp, vp]
         _lib = L
     return _lib
@@ -166,6 +167,10 @@ class OracleScene:
         n = C.c_uint32()
         self.L.rtc_photon_build(self.h, C.byref(pp), nthreads or threads(), C.byref(n))
         return n.value
+
+    def photon_set_gather(self, radius, knn=0):
+        if self.L.rtc_photon_set_gather(self.h, float(radius), int(knn)) != 0:
+            raise ValueError("bad gather parameters")
 
     def photon_dump(self, cap):
         out = np.zeros((cap, 9), dtype=np.float32)

@@ -428,19 +428,69 @@ static inline void kd_gather(const KdPhoton* nodes, size_t n, const float pos[3]
     }
 }
 
+// k-NN gather (extension, no reference counterpart): the same Epanechnikov
+// estimate with the squared radius R2 = the k-th smallest squared distance
+// among the photons on the query's hemisphere closer than the gather radius,
+// and the normalisation 2 / (pi^2 R2).  With fewer than k such photons (or a
+// zero k-th distance) it is the fixed-radius gather.  Photons at exactly R2
+// weigh 0, so ties at the k-th distance do not matter.
+// k smallest squared distances (candidates: d2 < bound, positive normal dot)
+// by the usual kd-tree search with a max-heap; bound shrinks to the heap top.
+static inline void kd_knn(const KdPhoton* nodes, size_t n, const float pos[3], const float nrm[3], size_t k,
+                          std::vector<float>& heap, float& bound) {
+    if (n == 0) return;
+    size_t mid = n / 2;
+    const KdPhoton& node = nodes[mid];
+    float dx = node.pos[0] - pos[0], dy = node.pos[1] - pos[1], dz = node.pos[2] - pos[2];
+    float d2 = dx * dx + dy * dy + dz * dz;
+    if (d2 < bound && node.nor[0] * nrm[0] + node.nor[1] * nrm[1] + node.nor[2] * nrm[2] > 0.0f) {
+        if (heap.size() == k) { std::pop_heap(heap.begin(), heap.end()); heap.pop_back(); }
+        heap.push_back(d2);
+        std::push_heap(heap.begin(), heap.end());
+        if (heap.size() == k) bound = heap.front();
+    }
+    int axis = node.axis;
+    float sq = pos[axis] - node.pos[axis];
+    const KdPhoton* lo = nodes; size_t nlo = mid;
+    const KdPhoton* hi = nodes + mid + 1; size_t nhi = n - mid - 1;
+    if (sq <= 0.0f) {
+        kd_knn(lo, nlo, pos, nrm, k, heap, bound);
+        if (sq * sq < bound) kd_knn(hi, nhi, pos, nrm, k, heap, bound);
+    } else {
+        kd_knn(hi, nhi, pos, nrm, k, heap, bound);
+        if (sq * sq < bound) kd_knn(lo, nlo, pos, nrm, k, heap, bound);
+    }
+}
+
 struct PhotonMap {
     std::vector<KdPhoton> nodes; float gather_r2 = 0, norm = 0;
+    uint32_t knn = 0;
     void finish(float radius) {
         kd_build(nodes.data(), nodes.size());
+        set_radius(radius);
+    }
+    void set_radius(float radius) {
         gather_r2 = radius * radius;
         norm = 2.0f / (PI * PI * radius * radius);
+    }
+    float knn_r2(const Point3& pos, const Vec3& normal) const {
+        float p[3] = {pos.x, pos.y, pos.z}, n[3] = {normal.x, normal.y, normal.z};
+        std::vector<float> heap;
+        heap.reserve(knn);
+        float bound = gather_r2;
+        kd_knn(nodes.data(), nodes.size(), p, n, knn, heap, bound);
+        if (heap.size() < knn) return gather_r2;
+        float r2 = heap.front();
+        return r2 > 0.0f ? r2 : gather_r2;
     }
     Color irradiance(const Point3& pos, const Vec3& normal) const {
         if (nodes.empty()) return Color();
         float p[3] = {pos.x, pos.y, pos.z}, n[3] = {normal.x, normal.y, normal.z};
+        float r2 = knn ? knn_r2(pos, normal) : gather_r2;
+        float nm = r2 == gather_r2 ? norm : 2.0f / (PI * PI * r2);
         Color acc;
-        kd_gather(nodes.data(), nodes.size(), p, gather_r2, n, acc);
-        return acc * norm;
+        kd_gather(nodes.data(), nodes.size(), p, r2, n, acc);
+        return acc * nm;
     }
 };
 

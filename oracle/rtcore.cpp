@@ -400,6 +400,7 @@ int rtc_render_accumulate(void* s, const rt_camera* cam, const rt_sample_params*
 
 int rtc_photon_build(void* s, const rt_photon_params* pp, int threads, uint32_t* stored) {
     OScene& S = *(OScene*)s;
+    if (pp->knn > RT_KNN_MAX) return -1;
     if (pp->emitter == RT_EMIT_QUAD) {
         photon_build_from_quad(S.photons, S.world, Point3(pp->origin[0], pp->origin[1], pp->origin[2]),
                                Vec3(pp->u[0], pp->u[1], pp->u[2]), Vec3(pp->v[0], pp->v[1], pp->v[2]),
@@ -409,8 +410,18 @@ int rtc_photon_build(void* s, const rt_photon_params* pp, int threads, uint32_t*
         photon_build_sun(S.photons, S.world, have, bb, Vec3(pp->sun_dir[0], pp->sun_dir[1], pp->sun_dir[2]),
                          Color(pp->color[0], pp->color[1], pp->color[2]), pp->num_photons, pp->gather_radius, threads);
     }
+    S.photons.knn = pp->knn;
     S.have_photons = true;
     if (stored) *stored = (uint32_t)S.photons.nodes.size();
+    return 0;
+}
+
+// Gather parameters of the built map: radius and k (0: fixed radius).
+int rtc_photon_set_gather(void* s, float radius, uint32_t knn) {
+    OScene& S = *(OScene*)s;
+    if (knn > RT_KNN_MAX || !(radius > 0.0f)) return -1;
+    S.photons.set_radius(radius);
+    S.photons.knn = knn;
     return 0;
 }
 

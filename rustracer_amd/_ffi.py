@@ -158,6 +158,7 @@ SIGNATURES = {
                                         C.POINTER(rt_param), C.c_uint32, VP, VP, C.POINTER(rt_wavefront_opts)]),
     "rt_photon_build": (C.c_int, [VP, VP, C.POINTER(rt_photon_params), C.POINTER(C.c_uint32)]),
     "rt_photon_import": (C.c_int, [VP, VP, VP, C.c_uint32, C.c_float]),
+    "rt_photon_set_gather": (C.c_int, [VP, VP, C.c_float, C.c_uint32]),
     "rt_photon_irradiance": (C.c_int, [VP, VP, VP, C.c_uint32, VP]),
     "rt_photon_export": (C.c_int, [VP, VP, VP, C.c_uint32, C.POINTER(C.c_uint32)]),
     "rt_ctx_set_timing": (C.c_int, [VP, C.c_int]),

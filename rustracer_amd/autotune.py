@@ -56,7 +56,7 @@ def tune_batch(r, views, probe_spp=4, log=None):
 def tune(r, run, log=None):
     """Coordinate descent over the launch options; `run(opts, seed)` renders
     one probe frame on the renderer's stream."""
-    best = dict(capacity=1 << 22, grid=2048, sort=True, use_graph=True, occupancy=0)
+    best = dict(capacity=1 << 22, grid=2048, sort=1, use_graph=True, occupancy=0)
     table = []
 
     def measure(o):
@@ -68,7 +68,7 @@ def tune(r, run, log=None):
 
     t0 = time.time()
     best_ms = measure(best)
-    for key, values in (("occupancy", [0, 4, 5]), ("capacity", CAPACITIES), ("grid", GRIDS), ("sort", [True, False]),
+    for key, values in (("occupancy", [0, 4, 5]), ("capacity", CAPACITIES), ("grid", GRIDS), ("sort", [0, 1, 2]),
                         ("use_graph", [True, False])):
         for v in values:
             if v == best[key]:

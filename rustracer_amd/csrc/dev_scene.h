@@ -79,6 +79,7 @@ struct DScene {
     // (position + split axis in w, power, normal), 16 B per array per photon
     const float4* ph_pos; const float4* ph_pow; const float4* ph_nrm;
     uint32_t num_photons; float ph_r2, ph_norm;
+    uint32_t ph_knn;                       // 0: fixed-radius gather; k: k-NN gather bounded by ph_r2
     uint32_t stack_bound;
 };
 
@@ -92,6 +93,7 @@ struct Stack {
     uint32_t stride;
     int32_t priv[RT_PRIV_STACK];
     int32_t sp;
+    float* heap;       // k-NN photon gather: this lane's LDS column (stride) of RT_KNN_MAX floats, or null
     uint32_t n_nodes, n_prims, n_rays;   // traversal work, for the counting pass
     uint32_t s_nodes, s_prims, s_rays;   // the same for shadow (transmittance) queries
 };

@@ -483,7 +483,16 @@ RT_D bool sss_scatter(const DMaterial& m, const Ray& r, const Rec& h, Rng& rng, 
     return true;
 }
 
+// Material kinds whose scatter is a delta / sampled-lobe event (skip_pdf):
+// the wavefront's specular shade class.  MASK (a set of 1 << kind) lets a
+// caller that knows the hit's class compile only those materials.
+#define RT_MASK_ALL 0xffffffffu
+#define RT_MASK_SPECULAR ((1u << M_DIELECTRIC) | (1u << M_SPECTRAL_DIELECTRIC) | (1u << M_ROUGH_DIELECTRIC) | \
+                          (1u << M_ROUGH_SPECTRAL) | (1u << M_SSS))
+#define RT_IN(MASK, K) (((MASK) >> (K)) & 1u)
+
 // Material::scatter dispatch.  Medium events scatter with the HG phase function.
+template <uint32_t MASK = RT_MASK_ALL>
 RT_D bool mat_scatter(const DScene& S, const Rec& h, const Ray& r, Rng& rng, Scatter& sr) {
     if (h.medium) {
         const DMedium& md = S.media[h.medium - 1];
@@ -495,15 +504,18 @@ RT_D bool mat_scatter(const DScene& S, const Rec& h, const Ray& r, Rng& rng, Sca
     const DMaterial& m = S.mats[h.mat];
     switch (m.kind) {
         case M_LAMBERTIAN:
+            if (!RT_IN(MASK, M_LAMBERTIAN)) return false;
             sr.att = tex_value(S, m, h.u, h.v, h.p); sr.ray = scatter_from(h.p, h.n, r); sr.skip_pdf = false;
             return true;
         case M_DIELECTRIC: {
+            if (!RT_IN(MASK, M_DIELECTRIC)) return false;
             bool refl;
             V3 d = dielectric_boundary(m.p[0], r, h, rng, refl);
             sr.att = mk(1, 1, 1); sr.ray = scatter_from(h.p, d, r); sr.skip_pdf = true;
             return true;
         }
         case M_SPECTRAL_DIELECTRIC: {
+            if (!RT_IN(MASK, M_SPECTRAL_DIELECTRIC)) return false;
             float ior = m.p[5] != 0.0f ? sellmeier(r.wavelength, m.p + 6, m.p + 9) : cauchy(r.wavelength, m.p[0], m.p[1]);
             bool refl;
             V3 d = dielectric_boundary(ior, r, h, rng, refl);
@@ -519,11 +531,11 @@ RT_D bool mat_scatter(const DScene& S, const Rec& h, const Ray& r, Rng& rng, Sca
             sr.att = att; sr.skip_pdf = true;
             return true;
         }
-        case M_PBR: return pbr_scatter(S, m, r, h, rng, sr);
-        case M_SPECTRAL_METAL: return smetal_scatter(S, m, r, h, rng, sr);
-        case M_ROUGH_DIELECTRIC: return rough_dielectric_scatter(m, r, h, rng, sr);
-        case M_ROUGH_SPECTRAL: return rsd_scatter(S, m, r, h, rng, sr);
-        case M_SSS: return sss_scatter(m, r, h, rng, sr);
+        case M_PBR: return RT_IN(MASK, M_PBR) && pbr_scatter(S, m, r, h, rng, sr);
+        case M_SPECTRAL_METAL: return RT_IN(MASK, M_SPECTRAL_METAL) && smetal_scatter(S, m, r, h, rng, sr);
+        case M_ROUGH_DIELECTRIC: return RT_IN(MASK, M_ROUGH_DIELECTRIC) && rough_dielectric_scatter(m, r, h, rng, sr);
+        case M_ROUGH_SPECTRAL: return RT_IN(MASK, M_ROUGH_SPECTRAL) && rsd_scatter(S, m, r, h, rng, sr);
+        case M_SSS: return RT_IN(MASK, M_SSS) && sss_scatter(m, r, h, rng, sr);
         default: return false;   // DiffuseLight
     }
 }
@@ -564,18 +576,20 @@ RT_D bool mat_receives_caustics(const DScene& S, const Rec& h) {
     const DMaterial& m = S.mats[h.mat];
     return m.kind == M_LAMBERTIAN || (m.kind == M_PBR && m.p[4] < 0.999f);
 }
+template <uint32_t MASK = RT_MASK_ALL>
 RT_D V3 mat_spec_brdf_cos(const DScene& S, const Rec& h, const Ray& r, V3 wi) {
     if (h.medium) return mk(0, 0, 0);
     const DMaterial& m = S.mats[h.mat];
-    if (m.kind == M_PBR) return pbr_brdf_cos(S, m, r, h, wi);
-    if (m.kind == M_SPECTRAL_METAL) return smetal_brdf_cos(S, m, r, h, wi);
+    if (RT_IN(MASK, M_PBR) && m.kind == M_PBR) return pbr_brdf_cos(S, m, r, h, wi);
+    if (RT_IN(MASK, M_SPECTRAL_METAL) && m.kind == M_SPECTRAL_METAL) return smetal_brdf_cos(S, m, r, h, wi);
     return mk(0, 0, 0);
 }
+template <uint32_t MASK = RT_MASK_ALL>
 RT_D float mat_spec_pdf(const DScene& S, const Rec& h, const Ray& r, V3 wi) {
     if (h.medium) return 0.0f;
     const DMaterial& m = S.mats[h.mat];
-    if (m.kind == M_PBR) return pbr_spec_pdf(m, r, h, wi);
-    if (m.kind == M_SPECTRAL_METAL) return smetal_spec_pdf(m, r, h, wi);
+    if (RT_IN(MASK, M_PBR) && m.kind == M_PBR) return pbr_spec_pdf(m, r, h, wi);
+    if (RT_IN(MASK, M_SPECTRAL_METAL) && m.kind == M_SPECTRAL_METAL) return smetal_spec_pdf(m, r, h, wi);
     return 0.0f;
 }
 
@@ -810,10 +824,8 @@ RT_D V3 lights_generate(const DScene& S, V3 origin, Rng& rng, float time) {
 // stack: bit d of `pend` says the far child of the depth-d ancestor still
 // needs a visit; when a descent leaves the tree, the deepest pending ancestor
 // is recomputed from the index ((i-1)/2 per level) and its far child entered.
-RT_D V3 photon_irradiance(const DScene& S, V3 pos, V3 nrm) {
+RT_D V3 photon_gather(const DScene& S, V3 pos, V3 nrm, float r2) {
     uint32_t n = S.num_photons;
-    if (n == 0) return mk(0, 0, 0);
-    float r2 = S.ph_r2;
     V3 acc = mk(0, 0, 0);
     uint32_t node = 0, pend = 0;
     int depth = 0;
@@ -845,7 +857,91 @@ RT_D V3 photon_irradiance(const DScene& S, V3 pos, V3 nrm) {
         node = sq <= 0.0f ? 2 * anc + 2 : 2 * anc + 1;
         depth = d + 1;
     }
-    return acc * S.ph_norm;
+    return acc;
+}
+
+// k-NN gather radius (extension): the k-th smallest squared distance among the
+// photons on the query's hemisphere closer than the gather radius, kept in a
+// max-heap of k floats in this lane's LDS column.  The walk is the gather's,
+// with the pruning radius shrinking to the heap's root once k are held; a
+// far child queued under a larger radius is re-tested when it is popped.
+// Fewer than k photons (or a zero k-th distance): the gather radius.
+RT_D float photon_knn_r2(const DScene& S, V3 pos, V3 nrm, float* heap, uint32_t stride) {
+    uint32_t n = S.num_photons, k = S.ph_knn;
+    float bound = S.ph_r2;
+    uint32_t cnt = 0;
+    uint32_t node = 0, pend = 0;
+    int depth = 0;
+    bool skip = false;     // the popped far child was pruned by the shrunken radius
+    for (;;) {
+        if (node < n && !skip) {
+            float4 P = S.ph_pos[node];
+            float dx = P.x - pos.x, dy = P.y - pos.y, dz = P.z - pos.z;
+            float d2 = dx * dx + dy * dy + dz * dz;
+            if (d2 < bound) {
+                float4 N = S.ph_nrm[node];
+                if (N.x * nrm.x + N.y * nrm.y + N.z * nrm.z > 0.0f) {
+                    if (cnt < k) {                       // push, sift up
+                        uint32_t i = cnt++;
+                        while (i > 0) {
+                            uint32_t p = (i - 1) >> 1;
+                            float hp = heap[p * stride];
+                            if (hp >= d2) break;
+                            heap[i * stride] = hp;
+                            i = p;
+                        }
+                        heap[i * stride] = d2;
+                    } else {                             // replace the root, sift down
+                        uint32_t i = 0;
+                        for (;;) {
+                            uint32_t l = 2 * i + 1;
+                            if (l >= k) break;
+                            uint32_t c = l;
+                            float hc = heap[l * stride];
+                            if (l + 1 < k) { float hr = heap[(l + 1) * stride]; if (hr > hc) { c = l + 1; hc = hr; } }
+                            if (hc <= d2) break;
+                            heap[i * stride] = hc;
+                            i = c;
+                        }
+                        heap[i * stride] = d2;
+                    }
+                    if (cnt == k) bound = heap[0];
+                }
+            }
+            int axis = (int)P.w;
+            float sq = comp(pos, axis) - (axis == 0 ? P.x : (axis == 1 ? P.y : P.z));
+            if (sq * sq < bound) pend |= 1u << depth; else pend &= ~(1u << depth);
+            node = sq <= 0.0f ? 2 * node + 1 : 2 * node + 2;
+            depth++;
+            continue;
+        }
+        skip = false;
+        if (pend == 0) break;
+        int d = 31 - __clz(pend);
+        pend &= ~(1u << d);
+        uint32_t anc = node;
+        for (int j = depth; j > d; j--) anc = (anc - 1) >> 1;
+        float4 P = S.ph_pos[anc];
+        int axis = (int)P.w;
+        float sq = comp(pos, axis) - (axis == 0 ? P.x : (axis == 1 ? P.y : P.z));
+        node = sq <= 0.0f ? 2 * anc + 2 : 2 * anc + 1;
+        depth = d + 1;
+        skip = !(sq * sq < bound);
+    }
+    if (cnt < k) return S.ph_r2;
+    float r2 = heap[0];
+    return r2 > 0.0f ? r2 : S.ph_r2;
+}
+
+// PhotonMap::irradiance (src/photon.rs:148-157); with a k-NN map the radius
+// comes from photon_knn_r2 (here, through st.heap) or from `knn_r2` (the
+// wavefront's k_wf_knn stage computed it for this hit).
+RT_D V3 photon_irradiance(const DScene& S, V3 pos, V3 nrm, float knn_r2, const Stack& st) {
+    if (S.num_photons == 0) return mk(0, 0, 0);
+    float r2 = S.ph_r2;
+    if (S.ph_knn) r2 = st.heap ? photon_knn_r2(S, pos, nrm, st.heap, st.stride) : knn_r2;
+    float norm = r2 == S.ph_r2 ? S.ph_norm : 2.0f / (PI * PI * r2);
+    return photon_gather(S, pos, nrm, r2) * norm;
 }
 
 // ---- one bounce of ray_color (src/renderer.rs:371-585) ---------------------------------------
@@ -859,6 +955,7 @@ struct PathState {
     V3 thr;
     float prev_mis, prev_sun_w;
     bool prev_specular;
+    float knn_r2;        // k-NN gather radius of the current hit when computed ahead (wavefront)
 };
 
 // A deferred shadow query: `contrib` is added (times the transmittance) when
@@ -870,13 +967,13 @@ struct ShadowReq {
 };
 
 RT_D void path_init(PathState& ps, const Ray& r) {
-    ps.ray = r; ps.thr = mk(1, 1, 1); ps.prev_specular = true; ps.prev_mis = 1.0f; ps.prev_sun_w = 1.0f;
+    ps.ray = r; ps.thr = mk(1, 1, 1); ps.prev_specular = true; ps.prev_mis = 1.0f; ps.prev_sun_w = 1.0f; ps.knn_r2 = 0.0f;
 }
 
 // Shade the hit (or miss) `id` of ps.ray at bounce `depth`.  Adds direct
 // emission / background to `color`, appends up to three shadow requests and
 // advances the path.  Returns false when the path ends.
-template <class DH>
+template <class DH, uint32_t MASK = RT_MASK_ALL>
 RT_D bool bounce_t(const DScene& S, PathState& ps, const HitId& id, int depth, float bg_scale, bool use_photons, Rng& rng,
                    V3& color, ShadowReq* sh, int& nsh, Stack& st, DH& dh) {
     Ray& ray = ps.ray;
@@ -896,21 +993,22 @@ RT_D bool bounce_t(const DScene& S, PathState& ps, const HitId& id, int depth, f
     float ew = (ps.prev_specular || lights_empty) ? 1.0f : ps.prev_mis;
     color += dh.emit(ps.thr * mat_emitted(S, h, ray) * ew);
     Scatter sr;
-    if (!mat_scatter(S, h, ray, rng, sr)) return false;
-    if (sr.skip_pdf) {
+    if (!mat_scatter<MASK>(S, h, ray, rng, sr)) return false;
+    constexpr bool SPEC_ONLY = (MASK & ~RT_MASK_SPECULAR) == 0u;   // every scatter here is skip_pdf
+    if (SPEC_ONLY || sr.skip_pdf) {
         ps.prev_sun_w = 1.0f;
         if (sun) {
             float p_sun = 1.0f / (2.0f * PI * (1.0f - COS_SUN_MAX));
             V3 su = unit(mk(S.sun_dir[0], S.sun_dir[1], S.sun_dir[2]));
             V3 ss = sample_sun_cone(su, COS_SUN_MAX, rng);
-            V3 bc = mat_spec_brdf_cos(S, h, ray, ss);
+            V3 bc = mat_spec_brdf_cos<MASK>(S, h, ray, ss);
             if (bc.x > 0.0f || bc.y > 0.0f || bc.z > 0.0f) {
-                float pm = mat_spec_pdf(S, h, ray, ss);
+                float pm = mat_spec_pdf<MASK>(S, h, ray, ss);
                 V3 sc = bg_eval(S, ss);
                 float md = p_sun * p_sun + pm * pm;
                 if (md > 0.0f) { sh[nsh].o = h.p; sh[nsh].d = ss; sh[nsh].t_max = FINF; sh[nsh].time = ray.time; sh[nsh].contrib = ps.thr * bc * sc * p_sun / md; dh.shadow(sh[nsh], nsh, S, h, ray, false); nsh++; }
             }
-            float pmb = mat_spec_pdf(S, h, ray, sr.ray.d);
+            float pmb = mat_spec_pdf<MASK>(S, h, ray, sr.ray.d);
             if (pmb > 0.0f) ps.prev_sun_w = (pmb * pmb) / fmx(pmb * pmb + p_sun * p_sun, 1e-8f);
         }
         dh.scatter(S, h, ray, sr.ray.d);
@@ -920,7 +1018,7 @@ RT_D bool bounce_t(const DScene& S, PathState& ps, const HitId& id, int depth, f
         ps.prev_mis = 1.0f;
     } else {
         if (use_photons) {
-            V3 irr = photon_irradiance(S, h.p, h.n);
+            V3 irr = photon_irradiance(S, h.p, h.n, ps.knn_r2, st);
             if (irr.x > 0.0f || irr.y > 0.0f || irr.z > 0.0f) color += dh.emit_vertex(ps.thr * mat_albedo_hint(S, h) * irr, S, h, ray);
         }
         if (!lights_empty) {
@@ -1009,10 +1107,11 @@ struct NoDiff {
     RT_D void shadow(ShadowReq&, int, const DScene&, const Rec&, const Ray&, bool) {}
     RT_D void scatter(const DScene&, const Rec&, const Ray&, V3) {}
 };
+template <uint32_t MASK = RT_MASK_ALL>
 RT_D bool bounce(const DScene& S, PathState& ps, const HitId& id, int depth, float bg_scale, bool use_photons, Rng& rng,
                  V3& color, ShadowReq* sh, int& nsh, Stack& st) {
     NoDiff nd;
-    return bounce_t(S, ps, id, depth, bg_scale, use_photons, rng, color, sh, nsh, st, nd);
+    return bounce_t<NoDiff, MASK>(S, ps, id, depth, bg_scale, use_photons, rng, color, sh, nsh, st, nd);
 }
 
 // Megakernel form of ray_color: trace, shade, resolve shadow rays inline.

@@ -27,9 +27,17 @@ struct LdsStack {
         return s + threadIdx.x;
     }
 };
+// The k-NN photon gather's max-heap: RT_KNN_MAX floats per lane (32 KB per
+// block), allocated only by the kernel variants that search k-NN radii.
+struct LdsHeap {
+    __device__ static float* column() {
+        __shared__ float h[RT_KNN_MAX * BLOCK];
+        return h + threadIdx.x;
+    }
+};
 
 __device__ __forceinline__ void init_stack(Stack& st) {
-    st.lds = LdsStack::column(); st.stride = BLOCK; st.sp = 0;
+    st.lds = LdsStack::column(); st.stride = BLOCK; st.sp = 0; st.heap = nullptr;
     st.n_nodes = 0; st.n_prims = 0; st.n_rays = 0;
     st.s_nodes = 0; st.s_prims = 0; st.s_rays = 0;
 }
@@ -68,9 +76,11 @@ __global__ __launch_bounds__(BLOCK) void k_trace_tr(DScene S, const rt_ray* __re
 
 // Plain megakernel: one thread per pixel, `spp` consecutive sample passes,
 // radiance summed in registers and added to (or stored into) `out`.
+template <bool KNN>
 __global__ __launch_bounds__(BLOCK) void k_render(DScene S, DCamera cam, DParams P, const uint8_t* __restrict__ conv,
                                                   float* __restrict__ out, uint32_t spp, int accumulate) {
     Stack st; init_stack(st);
+    if (KNN) st.heap = LdsHeap::column();
     uint32_t i = blockIdx.x * BLOCK + threadIdx.x;
     uint32_t npix = P.width * P.height;
     if (i >= npix) return;
@@ -242,10 +252,13 @@ __global__ __launch_bounds__(BLOCK) void k_photon_trace(DScene S, DEmitter E, ui
     ok[i] = stored;
 }
 
+template <bool KNN>
 __global__ __launch_bounds__(BLOCK) void k_photon_irr(DScene S, const float* __restrict__ pn, uint32_t n, float* __restrict__ out) {
     uint32_t i = blockIdx.x * BLOCK + threadIdx.x;
     if (i >= n) return;
-    V3 c = photon_irradiance(S, mk(pn[6 * i], pn[6 * i + 1], pn[6 * i + 2]), mk(pn[6 * i + 3], pn[6 * i + 4], pn[6 * i + 5]));
+    Stack st;
+    st.heap = KNN ? LdsHeap::column() : nullptr; st.stride = BLOCK;
+    V3 c = photon_irradiance(S, mk(pn[6 * i], pn[6 * i + 1], pn[6 * i + 2]), mk(pn[6 * i + 3], pn[6 * i + 4], pn[6 * i + 5]), 0.0f, st);
     out[3 * i] = c.x; out[3 * i + 1] = c.y; out[3 * i + 2] = c.z;
 }
 
@@ -258,7 +271,8 @@ __global__ __launch_bounds__(BLOCK) void k_photon_irr(DScene S, const float* __r
 // queue length from device memory, so the host never synchronises inside a
 // frame.  Contributions go straight into the frame accumulator with float
 // atomics.
-#define WF_NB 10          // sort keys: 0 miss, 1..7 material kind, 8 medium event, 9 spare
+#define WF_NB 12          // sort keys (sort 1): 0 miss, 1..10 material kind, 11 medium event;
+                          // (sort 2): 0 miss / specular / medium, 1 every other material
 #define WF_CNT_ACTIVE 0   // counters[0]: paths to extend this bounce
 #define WF_CNT_NEXT 1     // counters[1]: survivors written by shade
 #define WF_CNT_SHADOW 2   // counters[2]: shadow requests
@@ -289,7 +303,7 @@ struct WfQueues {
     uint32_t* counters;   // 64: 32 per-bounce counters, then the chunk position
     float* fb;
     float* lsamp;         // non-null: contributions go to the path's sample (3 floats per sid), not to fb
-    uint32_t sort;        // 0: one bucket (no material sort)
+    uint32_t sort;        // 0: one bucket; 1: a bucket per material kind; 2: two shade classes
 };
 
 __device__ __forceinline__ uint32_t lane_id() { return __lane_id(); }
@@ -320,6 +334,7 @@ __device__ __forceinline__ void wf_load(const WfPath& w, PathState& ps, Rng& rng
     ps.prev_mis = w.thr_mis.w;
     ps.prev_sun_w = w.sun_w;
     ps.prev_specular = (w.flags & 2u) != 0;
+    ps.knn_r2 = __uint_as_float(w.pad1[0]);
     rng.s0 = w.rng[0]; rng.s1 = w.rng[1]; rng.s2 = w.rng[2]; rng.s3 = w.rng[3];
 }
 __device__ __forceinline__ void wf_store(WfPath& w, const PathState& ps, const Rng& rng) {
@@ -404,19 +419,23 @@ __global__ __launch_bounds__(BLOCK) void k_wf_generate(DCamera cam0, DParams P0,
     Q.active[i] = i;
 }
 
-__device__ __forceinline__ uint32_t sort_key(const DScene& S, const HitId& id) {
+// Bucket of a hit: per material kind (sort 1) or per shade class (sort 2:
+// misses, media and the specular materials share k_wf_shade<..., 1>).
+__device__ __forceinline__ uint32_t sort_key(const DScene& S, const HitId& id, uint32_t sort) {
+    if (sort == 0) return 0u;
     if (id.slot < 0) return 0u;
-    if (id.prim == -2) return 8u;
+    if (id.prim == -2) return sort == 1 ? 11u : 0u;
     uint32_t k = S.mats[S.objs[id.slot].material].kind;
-    return k < 8u ? k : 9u;
+    if (sort == 1) return k < 11u ? k : 0u;
+    return RT_IN(RT_MASK_SPECULAR, k) ? 0u : 1u;
 }
 
-__device__ __forceinline__ uint32_t extend_one(const DScene& S, WfPath& w, bool sort, Stack& st) {
+__device__ __forceinline__ uint32_t extend_one(const DScene& S, WfPath& w, uint32_t sort, Stack& st) {
     float4 o = w.o_time, d = w.d_wl;
     Ray r = make_ray(mk(o.x, o.y, o.z), mk(d.x, d.y, d.z), o.w);
     HitId id = trace_closest(S, r, 0.001f, FINF, st);
     w.hit_t = id.t; w.hit_slot = id.slot; w.hit_prim = id.prim;
-    return sort ? sort_key(S, id) : 0u;
+    return sort_key(S, id, sort);
 }
 
 // OCC = minimum waves per SIMD the register allocator must allow (0: its own choice).
@@ -432,14 +451,15 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(OCC > 0 ?
         if (valid) {
             pid = Q.active[i];
             WfPath& w = Q.paths[pid];
-            if (BATCH) key = extend_one(B.scenes[w.pad0], w, Q.sort != 0, st);
-            else key = extend_one(S, w, Q.sort != 0, st);
+            if (BATCH) key = extend_one(B.scenes[w.pad0], w, Q.sort, st);
+            else key = extend_one(S, w, Q.sort, st);
         }
         uint32_t slot = wave_append(Q.counters + WF_CNT_BUCKET, key, valid);
         if (valid) Q.buckets[(size_t)key * Q.cap + slot] = pid;
     }
 }
 
+template <uint32_t MASK>
 __device__ __forceinline__ bool shade_one(const DScene& S, const DParams& P, WfQueues& Q, WfPath& w, int depth,
                                           ShadowReq* sh, int& nsh, Stack& st) {
     if (depth >= (int)P.max_depth) return false;
@@ -447,7 +467,7 @@ __device__ __forceinline__ bool shade_one(const DScene& S, const DParams& P, WfQ
     wf_load(w, ps, rng);
     HitId id; id.t = w.hit_t; id.slot = w.hit_slot; id.prim = w.hit_prim;
     V3 color = mk(0, 0, 0);
-    bool cont = bounce(S, ps, id, depth, P.bg_scale, P.use_photons != 0, rng, color, sh, nsh, st) && depth + 1 < (int)P.max_depth;
+    bool cont = bounce<MASK>(S, ps, id, depth, P.bg_scale, P.use_photons != 0, rng, color, sh, nsh, st) && depth + 1 < (int)P.max_depth;
     if (color.x != 0.0f || color.y != 0.0f || color.z != 0.0f) {
         float* dst = Q.lsamp ? Q.lsamp + 3 * (size_t)w.sid : Q.fb + 3 * (size_t)w.pixel;
         atomicAdd(dst, color.x);
@@ -458,13 +478,19 @@ __device__ __forceinline__ bool shade_one(const DScene& S, const DParams& P, WfQ
     return cont;
 }
 
-template <bool BATCH>
+// CLASS 0: every bucket with every material; CLASS 1: bucket 0 of the class
+// split (misses, media, specular materials); CLASS 2: bucket 1 (the rest).
+// A class kernel compiles only its materials, so it needs fewer registers.
+template <bool BATCH, int CLASS>
 __global__ __launch_bounds__(BLOCK) void k_wf_shade(DScene S, DParams P, WfQueues Q, int depth, WfBatch B) {
+    constexpr uint32_t MASK = CLASS == 1 ? RT_MASK_SPECULAR : CLASS == 2 ? (RT_MASK_ALL & ~RT_MASK_SPECULAR) : RT_MASK_ALL;
+    constexpr int B0 = CLASS == 2 ? 1 : 0;
+    constexpr int NB = CLASS == 0 ? WF_NB : 1;
     Stack st; init_stack(st);
-    uint32_t off[WF_NB + 1];
+    uint32_t off[NB + 1];
     off[0] = 0;
-    for (int b = 0; b < WF_NB; b++) off[b + 1] = off[b] + Q.counters[WF_CNT_BUCKET + b];
-    uint32_t total = off[WF_NB];
+    for (int b = 0; b < NB; b++) off[b + 1] = off[b] + Q.counters[WF_CNT_BUCKET + B0 + b];
+    uint32_t total = off[NB];
     uint32_t stride = gridDim.x * BLOCK;
     for (uint32_t base = blockIdx.x * BLOCK; base < total; base += stride) {
         uint32_t i = base + threadIdx.x;
@@ -476,13 +502,13 @@ __global__ __launch_bounds__(BLOCK) void k_wf_shade(DScene S, DParams P, WfQueue
         if (valid) {
             int b = 0;
             while (i >= off[b + 1]) b++;
-            pid = Q.buckets[(size_t)b * Q.cap + (i - off[b])];
+            pid = Q.buckets[(size_t)(B0 + b) * Q.cap + (i - off[b])];
             WfPath& w = Q.paths[pid];
             pixel = w.pixel;
             view = w.pad0;
             sid = w.sid;
-            if (BATCH) cont = shade_one(B.scenes[view], B.views[view].P, Q, w, depth, sh, nsh, st);
-            else cont = shade_one(S, P, Q, w, depth, sh, nsh, st);
+            if (BATCH) cont = shade_one<MASK>(B.scenes[view], B.views[view].P, Q, w, depth, sh, nsh, st);
+            else cont = shade_one<MASK>(S, P, Q, w, depth, sh, nsh, st);
         }
         uint32_t nslot = wave_append(Q.counters + WF_CNT_NEXT, 0u, cont);
         if (cont) Q.next[nslot] = pid;
@@ -498,6 +524,33 @@ __global__ __launch_bounds__(BLOCK) void k_wf_shade(DScene S, DParams P, WfQueue
                 Q.shadow[s] = q;
             }
         }
+    }
+}
+
+// k-NN photon maps: between extend and shade, the gather radius of every
+// hit whose material can take the diffuse branch (Lambertian, PBR), into the
+// path record, so the shade kernel keeps its occupancy (the heap needs 32 KB
+// of LDS per block).  Paths whose hit does not gather are left as they are.
+template <bool BATCH>
+__global__ __launch_bounds__(BLOCK) void k_wf_knn(DScene S0, DParams P0, WfQueues Q, WfBatch B) {
+    float* heap = LdsHeap::column();
+    uint32_t total = 0;
+    for (int b = 0; b < WF_NB; b++) total += Q.counters[WF_CNT_BUCKET + b];
+    uint32_t stride = gridDim.x * BLOCK;
+    for (uint32_t i = blockIdx.x * BLOCK + threadIdx.x; i < total; i += stride) {
+        uint32_t b = 0, base = 0;
+        while (i >= base + Q.counters[WF_CNT_BUCKET + b]) { base += Q.counters[WF_CNT_BUCKET + b]; b++; }
+        uint32_t pid = Q.buckets[(size_t)b * Q.cap + (i - base)];
+        WfPath& w = Q.paths[pid];
+        const DScene& S = BATCH ? B.scenes[w.pad0] : S0;
+        bool photons = BATCH ? B.views[w.pad0].P.use_photons != 0 : P0.use_photons != 0;
+        if (!photons || S.ph_knn == 0 || S.num_photons == 0 || w.hit_slot < 0 || w.hit_prim == -2) continue;
+        uint32_t kind = S.mats[S.objs[w.hit_slot].material].kind;
+        if (kind != M_LAMBERTIAN && kind != M_PBR) continue;
+        Ray r = make_ray(mk(w.o_time.x, w.o_time.y, w.o_time.z), mk(w.d_wl.x, w.d_wl.y, w.d_wl.z), w.o_time.w);
+        HitId id; id.t = w.hit_t; id.slot = w.hit_slot; id.prim = w.hit_prim;
+        Rec h = finalize(S, r, id, 0.001f);
+        w.pad1[0] = __float_as_uint(photon_knn_r2(S, h.p, h.n, heap, BLOCK));
     }
 }
 
@@ -1451,6 +1504,7 @@ static int render_common(rt_ctx* ctx, rt_scene* s, const rt_camera* cam, const r
     DCamera C; DParams P;
     to_dev(cam, p, C, P);
     P.counters = ctx->counters;
+    const bool knn = p->use_photons && s->S.ph_knn;
     dim3 grid((npix + BLOCK - 1) / BLOCK);
     if (ctx->timing) HIPCHECK(ctx, hipEventRecord(ctx->ev0, ctx->stream));
     uint32_t nlaunch = 1;
@@ -1462,12 +1516,14 @@ static int render_common(rt_ctx* ctx, rt_scene* s, const rt_camera* cam, const r
             DParams Q = P;
             Q.sample_idx = P.sample_idx + k;
             uint32_t m = std::min(chunk, spp - k);
-            hipLaunchKernelGGL(k_render, grid, dim3(BLOCK), 0, ctx->stream, s->S, C, Q, (const uint8_t*)cin.dev, (float*)rout.dev, m, 1);
+            if (knn) hipLaunchKernelGGL(k_render<true>, grid, dim3(BLOCK), 0, ctx->stream, s->S, C, Q, (const uint8_t*)cin.dev, (float*)rout.dev, m, 1);
+            else hipLaunchKernelGGL(k_render<false>, grid, dim3(BLOCK), 0, ctx->stream, s->S, C, Q, (const uint8_t*)cin.dev, (float*)rout.dev, m, 1);
             HIPCHECK(ctx, hipGetLastError());
         }
         nlaunch = (spp + chunk - 1) / chunk;
     } else {
-        hipLaunchKernelGGL(k_render, grid, dim3(BLOCK), 0, ctx->stream, s->S, C, P, (const uint8_t*)cin.dev, (float*)rout.dev, 1u, 0);
+        if (knn) hipLaunchKernelGGL(k_render<true>, grid, dim3(BLOCK), 0, ctx->stream, s->S, C, P, (const uint8_t*)cin.dev, (float*)rout.dev, 1u, 0);
+        else hipLaunchKernelGGL(k_render<false>, grid, dim3(BLOCK), 0, ctx->stream, s->S, C, P, (const uint8_t*)cin.dev, (float*)rout.dev, 1u, 0);
         HIPCHECK(ctx, hipGetLastError());
     }
     if (ctx->timing) {
@@ -1542,7 +1598,7 @@ static void launch_shadow(int occ, uint32_t grid, hipStream_t st, const DScene& 
 // from the counters, for graph replays) and run `depth` bounces.
 template <bool BT>
 static void wf_chunk(rt_ctx* ctx, const DScene& S, const DCamera& C, const DParams& P, const WfQueues& Q0,
-                     uint64_t first, uint32_t n, uint32_t grid, int occ, uint32_t depth, const WfBatch& Bt) {
+                     uint64_t first, uint32_t n, uint32_t grid, int occ, uint32_t depth, const WfBatch& Bt, bool knn) {
     hipStream_t st = ctx->stream;
     hipLaunchKernelGGL((k_wf_generate<BT>), dim3((n + BLOCK - 1) / BLOCK), dim3(BLOCK), 0, st, C, P, Q0, first, n, Bt);
     hipLaunchKernelGGL(k_wf_reset, dim3(1), dim3(32), 0, st, Q0.counters, n);
@@ -1557,7 +1613,13 @@ static void wf_chunk(rt_ctx* ctx, const DScene& S, const DCamera& C, const DPara
         if (ev) (void)hipEventRecord(ctx->ext_ev[ctx->ext_used], st);
         launch_extend<BT>(occ, grid, st, S, Q, Bt);
         if (ev) { (void)hipEventRecord(ctx->ext_ev[ctx->ext_used + 1], st); ctx->ext_used += 2; }
-        hipLaunchKernelGGL((k_wf_shade<BT>), dim3(grid), dim3(BLOCK), 0, st, S, P, Q, (int)d, Bt);
+        if (knn) hipLaunchKernelGGL((k_wf_knn<BT>), dim3(grid), dim3(BLOCK), 0, st, S, P, Q, Bt);
+        if (Q.sort == 2) {
+            hipLaunchKernelGGL((k_wf_shade<BT, 1>), dim3(grid), dim3(BLOCK), 0, st, S, P, Q, (int)d, Bt);
+            hipLaunchKernelGGL((k_wf_shade<BT, 2>), dim3(grid), dim3(BLOCK), 0, st, S, P, Q, (int)d, Bt);
+        } else {
+            hipLaunchKernelGGL((k_wf_shade<BT, 0>), dim3(grid), dim3(BLOCK), 0, st, S, P, Q, (int)d, Bt);
+        }
         launch_shadow<BT>(occ, grid, st, S, Q, Bt);
         hipLaunchKernelGGL(k_wf_advance, dim3(1), dim3(32), 0, st, Q.counters);
     }
@@ -1572,6 +1634,7 @@ static int wf_opts(rt_ctx* ctx, const rt_wavefront_opts* opts, uint64_t total, W
     o.use_graph = opts ? opts->use_graph : 0u;
     o.occ = opts ? (int)opts->occupancy : 0;
     if (o.cap < BLOCK || o.cap > (1u << 26)) return fail(ctx, RT_ERR_INVALID, "wavefront capacity out of range");
+    if (o.sort > 2) return fail(ctx, RT_ERR_INVALID, "wavefront sort must be 0, 1 or 2");
     if (o.grid == 0 || o.grid > (1u << 20)) return fail(ctx, RT_ERR_INVALID, "wavefront grid out of range");
     if (o.cap > total) o.cap = (uint32_t)((total + BLOCK - 1) / BLOCK * BLOCK);
     return RT_OK;
@@ -1580,7 +1643,7 @@ static int wf_opts(rt_ctx* ctx, const rt_wavefront_opts* opts, uint64_t total, W
 // Every chunk of the global path range through every bounce.  fb must be zeroed.
 template <bool BT>
 static int wf_run(rt_ctx* ctx, const DScene& S, const DCamera& C, const DParams& P, WfQueues& Q, const WfOpts& o,
-                  uint64_t total, uint32_t depth, const WfBatch& Bt) {
+                  uint64_t total, uint32_t depth, const WfBatch& Bt, bool knn) {
     uint32_t cap = o.cap;
     uint64_t full = total / cap, rem = total % cap;
     if (ctx->timing) HIPCHECK(ctx, hipEventRecord(ctx->ev0, ctx->stream));
@@ -1589,7 +1652,7 @@ static int wf_run(rt_ctx* ctx, const DScene& S, const DCamera& C, const DParams&
         hipGraphExec_t exec = nullptr;
         hipLaunchKernelGGL(k_wf_set_first, dim3(1), dim3(32), 0, ctx->stream, Q.counters, (uint64_t)0);
         HIPCHECK(ctx, hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
-        wf_chunk<BT>(ctx, S, C, P, Q, ~0ull, cap, o.grid, o.occ, depth, Bt);
+        wf_chunk<BT>(ctx, S, C, P, Q, ~0ull, cap, o.grid, o.occ, depth, Bt, knn);
         hipLaunchKernelGGL(k_wf_bump_first, dim3(1), dim3(32), 0, ctx->stream, Q.counters, cap);
         HIPCHECK(ctx, hipStreamEndCapture(ctx->stream, &graph));
         HIPCHECK(ctx, hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
@@ -1597,9 +1660,9 @@ static int wf_run(rt_ctx* ctx, const DScene& S, const DCamera& C, const DParams&
         HIPCHECK(ctx, hipGraphExecDestroy(exec));
         HIPCHECK(ctx, hipGraphDestroy(graph));
     } else {
-        for (uint64_t c = 0; c < full; c++) wf_chunk<BT>(ctx, S, C, P, Q, c * (uint64_t)cap, cap, o.grid, o.occ, depth, Bt);
+        for (uint64_t c = 0; c < full; c++) wf_chunk<BT>(ctx, S, C, P, Q, c * (uint64_t)cap, cap, o.grid, o.occ, depth, Bt, knn);
     }
-    if (rem) wf_chunk<BT>(ctx, S, C, P, Q, full * (uint64_t)cap, (uint32_t)rem, o.grid, o.occ, depth, Bt);
+    if (rem) wf_chunk<BT>(ctx, S, C, P, Q, full * (uint64_t)cap, (uint32_t)rem, o.grid, o.occ, depth, Bt, knn);
     HIPCHECK(ctx, hipGetLastError());
     if (ctx->timing) {
         HIPCHECK(ctx, hipEventRecord(ctx->ev1, ctx->stream));
@@ -1641,7 +1704,7 @@ extern "C" int rt_render_wavefront(rt_ctx* ctx, rt_scene* s, const rt_camera* ca
     P.counters = nullptr;
     HIPCHECK(ctx, hipMemsetAsync(rout.dev, 0, (size_t)npix * 12, ctx->stream));
     WfBatch none = {nullptr, nullptr, 0};
-    if ((rc = wf_run<false>(ctx, s->S, C, P, Q, o, total, P.max_depth, none))) return rc;
+    if ((rc = wf_run<false>(ctx, s->S, C, P, Q, o, total, P.max_depth, none, P.use_photons && s->S.ph_knn))) return rc;
     return finish_out(ctx, out, (size_t)npix * 12, rout);
 }
 
@@ -1652,6 +1715,7 @@ extern "C" int rt_render_batch(rt_ctx* ctx, const rt_batch_view* views, uint32_t
     std::vector<DScene> ds(nviews);
     uint64_t total = 0, fb_pix = 0;
     uint32_t depth = 0;
+    bool knn = false;
     for (uint32_t v = 0; v < nviews; v++) {
         const rt_batch_view& b = views[v];
         if (!b.scene || !b.accum_rgb || b.spp == 0) return fail(ctx, RT_ERR_INVALID, "batch view " + std::to_string(v) + ": missing scene/output/spp");
@@ -1670,6 +1734,7 @@ extern "C" int rt_render_batch(rt_ctx* ctx, const rt_batch_view* views, uint32_t
         if (fb_pix > 0xffffffffull / 4) return fail(ctx, RT_ERR_INVALID, "batch frames too large");
         depth = std::max(depth, p.max_depth);
         ds[v] = b.scene->S;
+        knn = knn || (p.use_photons && ds[v].ph_knn);
     }
     WfOpts o;
     int rc;
@@ -1690,7 +1755,7 @@ extern "C" int rt_render_batch(rt_ctx* ctx, const rt_batch_view* views, uint32_t
     WfBatch Bt = {(const DScene*)(side + bv), (const DView*)side, nviews};
     DCamera C = dv[0].cam;
     DParams P = dv[0].P;
-    if ((rc = wf_run<true>(ctx, ds[0], C, P, Q, o, total, depth, Bt))) return rc;
+    if ((rc = wf_run<true>(ctx, ds[0], C, P, Q, o, total, depth, Bt, knn))) return rc;
     HIPCHECK(ctx, hipStreamSynchronize(ctx->stream));
     for (uint32_t v = 0; v < nviews; v++) {
         const float* src = Q.fb + 3 * (size_t)dv[v].fb_off;
@@ -1732,6 +1797,7 @@ static int render_args(rt_ctx* ctx, rt_scene* s, const rt_camera* cam, const rt_
     if (p->width == 0 || p->height == 0 || p->max_depth == 0) return fail(ctx, RT_ERR_INVALID, "empty frame");
     if ((uint64_t)p->width * p->height > 0xffffffffull / 4) return fail(ctx, RT_ERR_INVALID, "image too large");
     if (p->use_photons && s->S.num_photons == 0) return fail(ctx, RT_ERR_INVALID, "use_photons without a photon map");
+    if (p->use_photons && s->S.ph_knn) return fail(ctx, RT_ERR_UNSUPPORTED, "the adjoint gathers photons in a fixed radius only");
     return RT_OK;
 }
 
@@ -1890,7 +1956,7 @@ extern "C" int rt_backward_wavefront(rt_ctx* ctx, rt_scene* s, const rt_camera* 
         HIPCHECK(ctx, hipMemsetAsync(side + b_adj, 0, (size_t)m * 12, st));
         WfQueues Qp = Q;
         Qp.lsamp = (float*)(side + b_adj);
-        wf_chunk<false>(ctx, s->S, C, P, Qp, first, m, o.grid, o.occ, depth, none);
+        wf_chunk<false>(ctx, s->S, C, P, Qp, first, m, o.grid, o.occ, depth, none, false);   // no k-NN maps here (render_args)
         // replay from the same streams
         hipLaunchKernelGGL((k_wf_generate<false>), dim3((m + BLOCK - 1) / BLOCK), dim3(BLOCK), 0, st, C, P, Q, first, m, none);
         hipLaunchKernelGGL(k_wfb_init, dim3((m + BLOCK - 1) / BLOCK), dim3(BLOCK), 0, st, Q, A, m);
@@ -1971,6 +2037,7 @@ static int install_photons(rt_ctx* ctx, rt_scene* s, std::vector<float4>& kept, 
     s->S.num_photons = (uint32_t)m;
     s->S.ph_r2 = r * r;
     s->S.ph_norm = 2.0f / (PI * PI * r * r);
+    s->S.ph_knn = 0;
     s->photons_host.swap(kept);
     return RT_OK;
 }
@@ -1978,7 +2045,8 @@ static int install_photons(rt_ctx* ctx, rt_scene* s, std::vector<float4>& kept, 
 int rt_photon_build(rt_ctx* ctx, rt_scene* s, const rt_photon_params* pp, uint32_t* num_stored) {
     if (!ctx || !s || !pp) return RT_ERR_INVALID;
     if (pp->num_photons == 0) return fail(ctx, RT_ERR_INVALID, "num_photons = 0");
-    if (pp->knn) return fail(ctx, RT_ERR_UNSUPPORTED, "k-NN gather is not built yet");
+    if (pp->knn > RT_KNN_MAX) return fail(ctx, RT_ERR_UNSUPPORTED, "knn above RT_KNN_MAX");
+    if (!(pp->gather_radius > 0.0f)) return fail(ctx, RT_ERR_INVALID, "gather_radius must be positive");
     HIPCHECK(ctx, hipSetDevice(ctx->device));
     DEmitter E;
     memset(&E, 0, sizeof E);
@@ -2044,7 +2112,18 @@ int rt_photon_build(rt_ctx* ctx, rt_scene* s, const rt_photon_params* pp, uint32
     for (uint32_t i = 0; i < n; i++) if (ok[i]) { kept.push_back(all[3 * (size_t)i]); kept.push_back(all[3 * (size_t)i + 1]); kept.push_back(all[3 * (size_t)i + 2]); }
     int rc;
     if ((rc = install_photons(ctx, s, kept, pp->gather_radius))) return rc;
+    s->S.ph_knn = pp->knn;
     if (num_stored) *num_stored = (uint32_t)(s->photons_host.size() / 3);
+    return RT_OK;
+}
+
+int rt_photon_set_gather(rt_ctx* ctx, rt_scene* s, float gather_radius, uint32_t knn) {
+    if (!ctx || !s) return RT_ERR_INVALID;
+    if (!(gather_radius > 0.0f)) return fail(ctx, RT_ERR_INVALID, "gather_radius must be positive");
+    if (knn > RT_KNN_MAX) return fail(ctx, RT_ERR_UNSUPPORTED, "knn above RT_KNN_MAX");
+    s->S.ph_r2 = gather_radius * gather_radius;
+    s->S.ph_norm = 2.0f / (PI * PI * gather_radius * gather_radius);
+    s->S.ph_knn = knn;
     return RT_OK;
 }
 
@@ -2072,7 +2151,8 @@ int rt_photon_irradiance(rt_ctx* ctx, rt_scene* s, const float* pn, uint32_t n, 
     int rc;
     if ((rc = stage_in(ctx, pn, (size_t)n * 24, pin))) return rc;
     if ((rc = stage_out(ctx, irr, (size_t)n * 12, pout, false))) return rc;
-    hipLaunchKernelGGL(k_photon_irr, dim3((n + BLOCK - 1) / BLOCK), dim3(BLOCK), 0, ctx->stream, s->S, (const float*)pin.dev, n, (float*)pout.dev);
+    if (s->S.ph_knn) hipLaunchKernelGGL(k_photon_irr<true>, dim3((n + BLOCK - 1) / BLOCK), dim3(BLOCK), 0, ctx->stream, s->S, (const float*)pin.dev, n, (float*)pout.dev);
+    else hipLaunchKernelGGL(k_photon_irr<false>, dim3((n + BLOCK - 1) / BLOCK), dim3(BLOCK), 0, ctx->stream, s->S, (const float*)pin.dev, n, (float*)pout.dev);
     HIPCHECK(ctx, hipGetLastError());
     return finish_out(ctx, irr, (size_t)n * 12, pout);
 }

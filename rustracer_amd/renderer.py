@@ -154,7 +154,7 @@ class Renderer:
         """Same frame as render_accumulate, through the wavefront stage kernels."""
         p = self.params(width, height, sample_idx, strata, bg_scale, max_depth, None, photon_map, seed)
         cam = camera.to_c()
-        o = F.rt_wavefront_opts(capacity=capacity, grid=grid, sort=1 if sort else 0, use_graph=1 if use_graph else 0,
+        o = F.rt_wavefront_opts(capacity=capacity, grid=grid, sort=int(sort), use_graph=1 if use_graph else 0,
                                 occupancy=occupancy)
         self._check(self.L.rt_render_wavefront(self.ctx, self.scene if scene is None else scene, C.byref(cam), C.byref(p), spp, _ptr(out), C.byref(o)),
                     "rt_render_wavefront")
@@ -171,7 +171,7 @@ class Renderer:
                                         v.get("max_depth", 50), None, v.get("photon_map", False), v.get("seed", 0))
             arr[i].spp = v["spp"]
             arr[i].accum_rgb = _ptr(v["out"])
-        o = F.rt_wavefront_opts(capacity=capacity, grid=grid, sort=1 if sort else 0, use_graph=1 if use_graph else 0,
+        o = F.rt_wavefront_opts(capacity=capacity, grid=grid, sort=int(sort), use_graph=1 if use_graph else 0,
                                 occupancy=occupancy)
         self._check(self.L.rt_render_batch(self.ctx, arr, len(views), C.byref(o)), "rt_render_batch")
         return [v["out"] for v in views]
@@ -226,7 +226,7 @@ class Renderer:
         g = np.zeros(len(params), np.float32) if grad is None else grad
         sc = self.scene if scene is None else scene
         if wavefront:
-            o = F.rt_wavefront_opts(capacity=capacity, grid=grid, sort=1 if sort else 0, use_graph=0, occupancy=occupancy)
+            o = F.rt_wavefront_opts(capacity=capacity, grid=grid, sort=int(sort), use_graph=0, occupancy=occupancy)
             self._check(self.L.rt_backward_wavefront(self.ctx, sc, C.byref(cam), C.byref(p), spp, self._params(params),
                                                      len(params), _ptr(d), _ptr(g), C.byref(o)), "rt_backward_wavefront")
         else:
@@ -235,26 +235,27 @@ class Renderer:
         return g
 
     # ---------------------------------------------------------------- photons
-    def photon_build_from_quad(self, origin, u, v, color, num_photons, gather_radius, scene=None):
-        """PhotonMap::build_from_quad (src/photon.rs:97-135)."""
+    def photon_build_from_quad(self, origin, u, v, color, num_photons, gather_radius, scene=None, knn=0):
+        """PhotonMap::build_from_quad (src/photon.rs:97-135); knn > 0 gathers
+        the k nearest photons within gather_radius (extension)."""
         pp = F.rt_photon_params()
         pp.emitter = F.EMIT_QUAD
         for k in range(3):
             pp.origin[k], pp.u[k], pp.v[k], pp.color[k] = float(origin[k]), float(u[k]), float(v[k]), float(color[k])
-        pp.num_photons, pp.gather_radius = num_photons, gather_radius
+        pp.num_photons, pp.gather_radius, pp.knn = num_photons, gather_radius, knn
         n = C.c_uint32()
         self._check(self.L.rt_photon_build(self.ctx, scene if scene is not None else self.scene, C.byref(pp), C.byref(n)),
                     "rt_photon_build")
         self.num_photons = n.value
         return n.value
 
-    def photon_build_sun(self, sun_dir, sun_color, num_photons, gather_radius):
+    def photon_build_sun(self, sun_dir, sun_color, num_photons, gather_radius, knn=0):
         """PhotonMap::build (src/photon.rs:41-94): directional disk toward `sun_dir`."""
         pp = F.rt_photon_params()
         pp.emitter = F.EMIT_SUN_DISK
         for k in range(3):
             pp.sun_dir[k], pp.color[k] = float(sun_dir[k]), float(sun_color[k])
-        pp.num_photons, pp.gather_radius = num_photons, gather_radius
+        pp.num_photons, pp.gather_radius, pp.knn = num_photons, gather_radius, knn
         n = C.c_uint32()
         self._check(self.L.rt_photon_build(self.ctx, self.scene, C.byref(pp), C.byref(n)), "rt_photon_build")
         self.num_photons = n.value
@@ -268,6 +269,11 @@ class Renderer:
                                             float(gather_radius)), "rt_photon_import")
         self.num_photons = len(ph)
         return len(ph)
+
+    def photon_set_gather(self, gather_radius, knn=0, scene=None):
+        """Fixed-radius (knn 0) or k-NN gather for the attached photon map."""
+        self._check(self.L.rt_photon_set_gather(self.ctx, self.scene if scene is None else scene, float(gather_radius),
+                                                int(knn)), "rt_photon_set_gather")
 
     def photon_export(self):
         n = C.c_uint32()

@@ -183,7 +183,8 @@ def mesh_envmap(width=1280, height=720, level=8):
     return scene, cam, dict(width=width, height=height, spp=64, max_depth=12)
 
 
-def gem_caustics(width=1280, height=720, num_photons=2_000_000, dispersion="cauchy", spin_deg=0.0):
+def gem_caustics(width=1280, height=720, num_photons=2_000_000, dispersion="cauchy", spin_deg=0.0, knn=0,
+                 gather_radius=0.05):
     """Dispersive gem with caustics.
 
     A round brilliant (Diamond, table 57 %, crown 34.5 deg, pavilion 40.75 deg,
@@ -194,8 +195,9 @@ def gem_caustics(width=1280, height=720, num_photons=2_000_000, dispersion="cauc
     Peter's Sellmeier law (B 0.3306, 4.3356; C 0.175^2, 0.106^2 um^2).  Floor:
     diffuse 0.8 quad 8x8 at y = 0.  Light: a 0.3x0.3 DiffuseLight quad of
     radiance 50 at y = 2.2 above the stone, facing down, in world and light
-    list.  Caustic photon map from that quad (build_from_quad), gather radius
-    0.05.  Camera at (0, 1.6, -2.6) looking at (0, 0.35, 0), vfov 35, black
+    list.  Caustic photon map from that quad (build_from_quad), fixed gather
+    radius 0.05, or with knn > 0 the k nearest photons within gather_radius.
+    Camera at (0, 1.6, -2.6) looking at (0, 0.35, 0), vfov 35, black
     background.  Render: 256 spp, max depth 24.
     """
     from .scene import Background, Camera, Diamond, DiffuseLight, Lambertian, Quad, Scene, SpectralDielectric
@@ -213,7 +215,12 @@ def gem_caustics(width=1280, height=720, num_photons=2_000_000, dispersion="cauc
     scene = Scene(objs, lights, Background.solid((0, 0, 0)), name="gem_caustics" if not spin_deg else f"gem_spin{spin_deg:g}")
     cam = Camera((0.0, 1.6, -2.6), (0.0, 0.35, 0.0), (0, 1, 0), 35.0, width / height)
     return scene, cam, dict(width=width, height=height, spp=256, max_depth=24, light_quad=lq, light_color=(50.0, 50.0, 50.0),
-                            num_photons=num_photons, gather_radius=0.05)
+                            num_photons=num_photons, gather_radius=gather_radius, knn=knn)
+
+
+# The gem of the headline caustic workload: Sellmeier diamond, k = 64 nearest
+# photons within the reference's default gather radius 0.15 (src/scene_file.rs:499).
+GEM_BENCH = dict(dispersion="sellmeier", knn=64, gather_radius=0.15)
 
 
 ADJOINT_PARAM_NAMES = (["n_d", "sellmeier_B1", "alpha_table", "floor_albedo"] +
@@ -300,7 +307,7 @@ def spin_batch(n_stills=36, step_deg=10.0, spp=64, seed=3, num_photons=2_000_000
     views = []
     for k in range(n_stills):
         W, H = SPIN_RESOLUTIONS[picks[k]]
-        sc, cam, cfg = gem_caustics(W, H, num_photons=num_photons, spin_deg=k * step_deg)
+        sc, cam, cfg = gem_caustics(W, H, num_photons=num_photons, spin_deg=k * step_deg, **GEM_BENCH)
         cfg["spp"] = spp
         views.append((sc, cam, cfg))
     if with_volume:

@@ -105,7 +105,7 @@ def test_adjoint_matches_finite_differences_with_common_random_numbers(gpu_rende
     assert np.count_nonzero(np.abs(fd) > 1e-2 * scale) >= 10
 
 
-@pytest.mark.parametrize("cap,sort", [(4096, True), (1 << 20, False)])
+@pytest.mark.parametrize("cap,sort", [(4096, 1), (1 << 20, 0), (8192, 2)])
 def test_wavefront_adjoint_equals_megakernel_adjoint(gpu_renderer, cap, sort):
     """The replay as wavefronts (primal radiance per sample, then the replay
     stages) gives the megakernel replay's gradient; capacity 4096 splits the

@@ -185,8 +185,8 @@ def test_gem_with_photons_three_sigma(gpu_renderer):
     assert (z > 3).mean() <= 0.02 and (z > 6).mean() <= 0.002, ((z > 3).mean(), (z > 6).mean())
 
 
-@pytest.mark.parametrize("sort,graph,cap,occ", [(True, False, 0, 0), (False, False, 4096, 4), (True, True, 8192, 5),
-                                                 (True, True, 2048, 0)])
+@pytest.mark.parametrize("sort,graph,cap,occ", [(1, False, 0, 0), (0, False, 4096, 4), (1, True, 8192, 5),
+                                                 (1, True, 2048, 0), (2, False, 4096, 0), (2, True, 2048, 4)])
 def test_wavefront_equals_megakernel(gpu_renderer, sort, graph, cap, occ):
     """Same per-pixel streams, same bounce code: the wavefront frame equals the
     megakernel frame up to the order of float atomic additions.  Capacity 2048
@@ -201,7 +201,8 @@ def test_wavefront_equals_megakernel(gpu_renderer, sort, graph, cap, occ):
     assert np.allclose(a, b, rtol=1e-4, atol=1e-5), np.abs(a - b).max()
 
 
-def test_wavefront_gem_with_photons_equals_megakernel(gpu_renderer):
+@pytest.mark.parametrize("sort", [0, 2])
+def test_wavefront_gem_with_photons_equals_megakernel(gpu_renderer, sort):
     sc, cam, cfg = gem_small(40, 32, 50000)
     gpu_renderer.upload(sc)
     lq = cfg["light_quad"]
@@ -209,7 +210,7 @@ def test_wavefront_gem_with_photons_equals_megakernel(gpu_renderer):
     a = np.zeros((32, 40, 3), np.float32)
     b = np.zeros((32, 40, 3), np.float32)
     gpu_renderer.render_accumulate(a, cam, 40, 32, 16, max_depth=24, seed=6, photon_map=True)
-    gpu_renderer.render_wavefront(b, cam, 40, 32, 16, max_depth=24, seed=6, photon_map=True, capacity=8192)
+    gpu_renderer.render_wavefront(b, cam, 40, 32, 16, max_depth=24, seed=6, photon_map=True, capacity=8192, sort=sort)
     assert np.allclose(a, b, rtol=1e-4, atol=1e-5), np.abs(a - b).max()
 
 
@@ -408,3 +409,50 @@ def test_texture_upload_rejections(gpu_renderer):
     t[800] = 300.0
     with pytest.raises(RuntimeError):
         gpu_renderer.upload(S.Scene([S.Sphere((0, 0, 0), 1.0, S.Lambertian(S.NoiseTexture(1.0, tables=t)))]))
+
+
+def test_knn_photon_gather_matches_checker_and_megakernel(gpu_renderer):
+    """k-NN gather (extension) over the checker's imported photons: the
+    irradiance queries, the megakernel frame (LDS heap in the kernel) and the
+    wavefront frame (radii from the k_wf_knn stage) against the checker."""
+    from rustracer_amd import _ffi as F
+    sc, cam, cfg = gem_small(32, 32, 200000)
+    gpu_renderer.upload(sc)
+    orc = O.OracleScene(sc)
+    lq = cfg["light_quad"]
+    pp = F.rt_photon_params()
+    pp.emitter = F.EMIT_QUAD
+    for k in range(3):
+        pp.origin[k], pp.u[k], pp.v[k], pp.color[k] = lq["q"][k], lq["u"][k], lq["v"][k], cfg["light_color"][k]
+    pp.num_photons, pp.gather_radius = 200000, cfg["gather_radius"]
+    n = orc.photon_build(pp)
+    ph = orc.photon_dump(n)
+    gpu_renderer.photon_import(ph, cfg["gather_radius"])
+    g = np.random.default_rng(6)
+    q = np.zeros((600, 6), np.float32)
+    near = ph[g.integers(0, n, 300), :3]
+    q[:300, [0, 2]] = near[:, [0, 2]] + g.normal(0, 0.05, size=(300, 2))
+    q[300:, [0, 2]] = g.uniform(-1.0, 1.0, size=(300, 2))
+    q[:, 4] = 1.0
+    for k in (1, 8, 64):
+        gpu_renderer.photon_set_gather(0.25, k)
+        orc.photon_set_gather(0.25, k)
+        a, b = gpu_renderer.photon_irradiance(q), orc.photon_irradiance(q)
+        assert np.allclose(a, b, rtol=1e-4, atol=1e-6), (k, np.abs(a - b).max())
+        assert k == 1 or (a.sum(1) > 0).mean() > 0.3
+    gpu_renderer.photon_set_gather(0.25, 64)
+    orc.photon_set_gather(0.25, 64)
+    gp = passes_gpu(gpu_renderer, cam, 32, 32, 4, 24, seed=4, photons=True)
+    cp = passes_cpu(orc, cam, 32, 32, 4, 24, mode=1, seed=4, photons=True)
+    close = np.isclose(gp, cp, rtol=1e-3, atol=1e-4).all(axis=-1)
+    assert close.mean() > 0.95, close.mean()
+    a = np.zeros((32, 32, 3), np.float32)
+    b = np.zeros((32, 32, 3), np.float32)
+    gpu_renderer.render_accumulate(a, cam, 32, 32, 16, max_depth=24, seed=7, photon_map=True)
+    for sort in (0, 1, 2):
+        b[:] = 0
+        gpu_renderer.render_wavefront(b, cam, 32, 32, 16, max_depth=24, seed=7, photon_map=True, capacity=4096, sort=sort)
+        assert np.allclose(a, b, rtol=1e-4, atol=1e-5), (sort, np.abs(a - b).max())
+    from rustracer_amd.renderer import RtError
+    with pytest.raises(RtError, match="fixed radius"):
+        gpu_renderer.backward(np.zeros((32, 32, 3), np.float32), cam, 32, 32, 4, [(0, 0, 1)], max_depth=8, photon_map=True)
