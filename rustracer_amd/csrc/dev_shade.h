@@ -824,8 +824,13 @@ RT_D V3 lights_generate(const DScene& S, V3 origin, Rng& rng, float time) {
 // stack: bit d of `pend` says the far child of the depth-d ancestor still
 // needs a visit; when a descent leaves the tree, the deepest pending ancestor
 // is recomputed from the index ((i-1)/2 per level) and its far child entered.
-RT_D V3 photon_gather(const DScene& S, V3 pos, V3 nrm, float r2) {
+// The Epanechnikov sum of kd_gather (src/photon.rs:195-228) over the
+// heap-ordered tree, before normalisation; `count` (optional) receives the
+// number of photons that passed the radius and hemisphere tests.
+template <bool SUM>
+RT_D V3 photon_walk(const DScene& S, V3 pos, V3 nrm, float r2, uint32_t& count) {
     uint32_t n = S.num_photons;
+    uint32_t cnt = 0;
     V3 acc = mk(0, 0, 0);
     uint32_t node = 0, pend = 0;
     int depth = 0;
@@ -837,7 +842,10 @@ RT_D V3 photon_gather(const DScene& S, V3 pos, V3 nrm, float r2) {
             if (d2 < r2) {
                 float4 N = S.ph_nrm[node];
                 float dt = N.x * nrm.x + N.y * nrm.y + N.z * nrm.z;
-                if (dt > 0.0f) { float4 W = S.ph_pow[node]; acc += mk(W.x, W.y, W.z) * (1.0f - d2 / r2); }
+                if (dt > 0.0f) {
+                    if (SUM) { float4 W = S.ph_pow[node]; acc += mk(W.x, W.y, W.z) * (1.0f - d2 / r2); }
+                    cnt++;
+                }
             }
             int axis = (int)P.w;
             float sq = comp(pos, axis) - (axis == 0 ? P.x : (axis == 1 ? P.y : P.z));
@@ -857,19 +865,25 @@ RT_D V3 photon_gather(const DScene& S, V3 pos, V3 nrm, float r2) {
         node = sq <= 0.0f ? 2 * anc + 2 : 2 * anc + 1;
         depth = d + 1;
     }
+    count = cnt;
     return acc;
 }
+RT_D V3 photon_gather(const DScene& S, V3 pos, V3 nrm, float r2) { uint32_t c; return photon_walk<true>(S, pos, nrm, r2, c); }
 
 // k-NN gather radius (extension): the k-th smallest squared distance among the
 // photons on the query's hemisphere closer than the gather radius, kept in a
 // max-heap of k floats in this lane's LDS column.  The walk is the gather's,
 // with the pruning radius shrinking to the heap's root once k are held; a
 // far child queued under a larger radius is re-tested when it is popped.
-// Fewer than k photons (or a zero k-th distance): the gather radius.
-RT_D float photon_knn_r2(const DScene& S, V3 pos, V3 nrm, float* heap, uint32_t stride) {
+// Fewer than k photons (or a zero k-th distance): the gather radius.  Until
+// k are held the walk is exactly the gather's at that radius, so `acc`
+// carries its sum and `count` < k says it is the answer.
+RT_D float photon_knn_r2(const DScene& S, V3 pos, V3 nrm, float* heap, uint32_t stride, float bound0, V3& acc,
+                         uint32_t& count) {
     uint32_t n = S.num_photons, k = S.ph_knn;
-    float bound = S.ph_r2;
+    float bound = bound0;
     uint32_t cnt = 0;
+    acc = mk(0, 0, 0);
     uint32_t node = 0, pend = 0;
     int depth = 0;
     bool skip = false;     // the popped far child was pruned by the shrunken radius
@@ -881,6 +895,7 @@ RT_D float photon_knn_r2(const DScene& S, V3 pos, V3 nrm, float* heap, uint32_t 
             if (d2 < bound) {
                 float4 N = S.ph_nrm[node];
                 if (N.x * nrm.x + N.y * nrm.y + N.z * nrm.z > 0.0f) {
+                    if (cnt < k) { float4 W = S.ph_pow[node]; acc += mk(W.x, W.y, W.z) * (1.0f - d2 / bound); }
                     if (cnt < k) {                       // push, sift up
                         uint32_t i = cnt++;
                         while (i > 0) {
@@ -928,20 +943,30 @@ RT_D float photon_knn_r2(const DScene& S, V3 pos, V3 nrm, float* heap, uint32_t 
         depth = d + 1;
         skip = !(sq * sq < bound);
     }
+    count = cnt;
     if (cnt < k) return S.ph_r2;
     float r2 = heap[0];
     return r2 > 0.0f ? r2 : S.ph_r2;
 }
 
-// PhotonMap::irradiance (src/photon.rs:148-157); with a k-NN map the radius
-// comes from photon_knn_r2 (here, through st.heap) or from `knn_r2` (the
-// wavefront's k_wf_knn stage computed it for this hit).
-RT_D V3 photon_irradiance(const DScene& S, V3 pos, V3 nrm, float knn_r2, const Stack& st) {
-    if (S.num_photons == 0) return mk(0, 0, 0);
-    float r2 = S.ph_r2;
-    if (S.ph_knn) r2 = st.heap ? photon_knn_r2(S, pos, nrm, st.heap, st.stride) : knn_r2;
+// k-NN irradiance: the heap search's radius, then the gather at that radius
+// (with fewer than k photons the search already summed the gather).
+RT_D V3 photon_knn_irradiance(const DScene& S, V3 pos, V3 nrm, float* heap, uint32_t stride) {
+    V3 acc;
+    uint32_t cnt;
+    float r2 = photon_knn_r2(S, pos, nrm, heap, stride, S.ph_r2, acc, cnt);
+    if (cnt < S.ph_knn) return acc * S.ph_norm;
     float norm = r2 == S.ph_r2 ? S.ph_norm : 2.0f / (PI * PI * r2);
     return photon_gather(S, pos, nrm, r2) * norm;
+}
+
+// PhotonMap::irradiance (src/photon.rs:148-157).  A k-NN map is gathered here
+// when the caller has an LDS heap (st.heap), else `knn_irr` holds the value
+// the wavefront's k_wf_knn stage computed for this hit.
+RT_D V3 photon_irradiance(const DScene& S, V3 pos, V3 nrm, V3 knn_irr, const Stack& st) {
+    if (S.num_photons == 0) return mk(0, 0, 0);
+    if (S.ph_knn) return st.heap ? photon_knn_irradiance(S, pos, nrm, st.heap, st.stride) : knn_irr;
+    return photon_gather(S, pos, nrm, S.ph_r2) * S.ph_norm;
 }
 
 // ---- one bounce of ray_color (src/renderer.rs:371-585) ---------------------------------------
@@ -955,7 +980,7 @@ struct PathState {
     V3 thr;
     float prev_mis, prev_sun_w;
     bool prev_specular;
-    float knn_r2;        // k-NN gather radius of the current hit when computed ahead (wavefront)
+    V3 knn_irr;          // k-NN photon irradiance of the current hit when computed ahead (wavefront)
 };
 
 // A deferred shadow query: `contrib` is added (times the transmittance) when
@@ -967,7 +992,7 @@ struct ShadowReq {
 };
 
 RT_D void path_init(PathState& ps, const Ray& r) {
-    ps.ray = r; ps.thr = mk(1, 1, 1); ps.prev_specular = true; ps.prev_mis = 1.0f; ps.prev_sun_w = 1.0f; ps.knn_r2 = 0.0f;
+    ps.ray = r; ps.thr = mk(1, 1, 1); ps.prev_specular = true; ps.prev_mis = 1.0f; ps.prev_sun_w = 1.0f; ps.knn_irr = mk(0, 0, 0);
 }
 
 // Shade the hit (or miss) `id` of ps.ray at bounce `depth`.  Adds direct
@@ -1018,7 +1043,7 @@ RT_D bool bounce_t(const DScene& S, PathState& ps, const HitId& id, int depth, f
         ps.prev_mis = 1.0f;
     } else {
         if (use_photons) {
-            V3 irr = photon_irradiance(S, h.p, h.n, ps.knn_r2, st);
+            V3 irr = photon_irradiance(S, h.p, h.n, ps.knn_irr, st);
             if (irr.x > 0.0f || irr.y > 0.0f || irr.z > 0.0f) color += dh.emit_vertex(ps.thr * mat_albedo_hint(S, h) * irr, S, h, ray);
         }
         if (!lights_empty) {

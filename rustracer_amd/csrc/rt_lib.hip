@@ -258,7 +258,7 @@ __global__ __launch_bounds__(BLOCK) void k_photon_irr(DScene S, const float* __r
     if (i >= n) return;
     Stack st;
     st.heap = KNN ? LdsHeap::column() : nullptr; st.stride = BLOCK;
-    V3 c = photon_irradiance(S, mk(pn[6 * i], pn[6 * i + 1], pn[6 * i + 2]), mk(pn[6 * i + 3], pn[6 * i + 4], pn[6 * i + 5]), 0.0f, st);
+    V3 c = photon_irradiance(S, mk(pn[6 * i], pn[6 * i + 1], pn[6 * i + 2]), mk(pn[6 * i + 3], pn[6 * i + 4], pn[6 * i + 5]), mk(0, 0, 0), st);
     out[3 * i] = c.x; out[3 * i + 1] = c.y; out[3 * i + 2] = c.z;
 }
 
@@ -334,7 +334,7 @@ __device__ __forceinline__ void wf_load(const WfPath& w, PathState& ps, Rng& rng
     ps.prev_mis = w.thr_mis.w;
     ps.prev_sun_w = w.sun_w;
     ps.prev_specular = (w.flags & 2u) != 0;
-    ps.knn_r2 = __uint_as_float(w.pad1[0]);
+    ps.knn_irr = mk(__uint_as_float(w.pad1[0]), __uint_as_float(w.pad1[1]), __uint_as_float(w.pad1[2]));
     rng.s0 = w.rng[0]; rng.s1 = w.rng[1]; rng.s2 = w.rng[2]; rng.s3 = w.rng[3];
 }
 __device__ __forceinline__ void wf_store(WfPath& w, const PathState& ps, const Rng& rng) {
@@ -527,7 +527,7 @@ __global__ __launch_bounds__(BLOCK) void k_wf_shade(DScene S, DParams P, WfQueue
     }
 }
 
-// k-NN photon maps: between extend and shade, the gather radius of every
+// k-NN photon maps: between extend and shade, the photon irradiance of every
 // hit whose material can take the diffuse branch (Lambertian, PBR), into the
 // path record, so the shade kernel keeps its occupancy (the heap needs 32 KB
 // of LDS per block).  Paths whose hit does not gather are left as they are.
@@ -550,7 +550,8 @@ __global__ __launch_bounds__(BLOCK) void k_wf_knn(DScene S0, DParams P0, WfQueue
         Ray r = make_ray(mk(w.o_time.x, w.o_time.y, w.o_time.z), mk(w.d_wl.x, w.d_wl.y, w.d_wl.z), w.o_time.w);
         HitId id; id.t = w.hit_t; id.slot = w.hit_slot; id.prim = w.hit_prim;
         Rec h = finalize(S, r, id, 0.001f);
-        w.pad1[0] = __float_as_uint(photon_knn_r2(S, h.p, h.n, heap, BLOCK));
+        V3 e = photon_knn_irradiance(S, h.p, h.n, heap, BLOCK);
+        w.pad1[0] = __float_as_uint(e.x); w.pad1[1] = __float_as_uint(e.y); w.pad1[2] = __float_as_uint(e.z);
     }
 }
 
