@@ -154,6 +154,12 @@ typedef struct {
  *   SSS         p0-2 albedo, p3-5 sigma_a, p6 ior, p7 density, p8 g
  *               (dielectric boundary with HG scattering inside)
  *                                                  src/material.rs:576-631
+ *   PEARL       p0-2 base_color, p3 ior, p4 film_thickness (nm),
+ *               p5 orient_strength, p6 film_scale, p7 luster_roughness: GGX
+ *               luster tinted by thin-film interference over a Lambertian body;
+ *               the nacre glow follows the Physical background's sun when there
+ *               is one (the renderer's pearl sun), the view angle otherwise
+ *                                                  src/material.rs:1052-1287
  *   ROUGH_SPECTRAL p0 n_d (IOR at 589.3 nm), p1-3 Sellmeier B1..B3,
  *               p4-6 Sellmeier C1..C3 (um^2): n(l) = n_d + s(l) - s(589.3 nm);
  *               p7 GGX alpha of primitive 0 (the table facet of a Diamond),
@@ -186,7 +192,8 @@ enum {
     RT_MAT_ROUGH_DIELECTRIC    = 7,
     RT_MAT_ROUGH_SPECTRAL      = 8,
     RT_MAT_BLACKBODY           = 9,
-    RT_MAT_SSS                 = 10
+    RT_MAT_SSS                 = 10,
+    RT_MAT_PEARL               = 11
 };
 
 typedef struct {

@@ -743,6 +743,127 @@ struct RoughSpectralDielectric : Material {
 // Henyey-Greenstein phase "material" of a medium (src/volume.rs:24-62).
 static inline Vec3 hg_sample(const Vec3& v_in, float g, SmallRng& rng);
 
+// Thin-film colour (src/material.rs:1091-1136): the 256-entry table of the
+// 65-wavelength interference integral over optical path difference, read
+// with linear interpolation, blended toward grey at grazing angles.
+static const int NACRE_N = 256;
+static const float NACRE_OPD_MAX = 2500.0f;
+static inline const std::vector<Color>& nacre_lut() {
+    static const std::vector<Color> lut = [] {
+        std::vector<Color> t(NACRE_N);
+        for (int i = 0; i < NACRE_N; i++) {
+            float opd = (float)i / (float)(NACRE_N - 1) * NACRE_OPD_MAX;
+            Color c;
+            float lambda = 380.0f;
+            while (lambda <= 700.01f) {
+                float irid = 0.5f * (1.0f + std::cos(2.0f * PI * opd / lambda));
+                c += spectral_to_rgb(lambda) * irid;
+                lambda += 5.0f;
+            }
+            t[i] = c / 65.0f;
+        }
+        return t;
+    }();
+    return lut;
+}
+static inline Color nacre_color(float cos_theta, float film_ior, float thickness) {
+    float sin_sq = rmax(1.0f - cos_theta * cos_theta, 0.0f);
+    float cos_t = std::sqrt(rmax(1.0f - sin_sq / (film_ior * film_ior), 0.0f));
+    float opd = 2.0f * film_ior * thickness * cos_t;
+    const std::vector<Color>& lut = nacre_lut();
+    float t = rclamp(opd / NACRE_OPD_MAX, 0.0f, 1.0f) * (float)(NACRE_N - 1);
+    size_t lo = f2usize(t);
+    size_t hi = std::min<size_t>(lo + 1, NACRE_N - 1);
+    Color raw = lut[lo] * (1.0f - (t - (float)lo)) + lut[hi] * (t - (float)lo);
+    float blend = std::pow(cos_theta, 0.5f);
+    return raw * blend + Color(0.5f, 0.5f, 0.5f) * (1.0f - blend);
+}
+// oil_noise (src/material.rs:1138-1147)
+static inline float oil_noise(const Point3& p) {
+    float a = std::sin(p.x * 1.618f + p.z);
+    float b = std::sin(p.z * 1.618f - p.x);
+    float c = std::sin(p.x * 0.618f + p.y + p.z);
+    return (a + b + c) * (1.0f / 3.0f);
+}
+
+// The renderer's pearl sun (src/material.rs:1052-1089): set from a Physical
+// background for the frame, absent otherwise and while photons are traced.
+struct PearlSun { bool active = false; Vec3 dir = Vec3(0, 1, 0); };
+
+// PearlMaterial (src/material.rs:1149-1287)
+struct PearlMaterial : Material {
+    Color base_color = Color(0.98f, 0.93f, 0.88f);
+    float ior = 1.56f, film_thickness = 450.0f, orient_strength = 0.30f, film_scale = 3.0f, luster_roughness = 0.05f;
+    const PearlSun* sun = nullptr;
+    float varied(const Point3& p) const { return film_thickness + oil_noise(p * film_scale) * 150.0f; }
+    bool scatter(const Ray& r_in, const HitRecord& rec, SmallRng& rng, ScatterRecord& sr) const override {
+        Vec3 n = rec.normal;
+        Vec3 wo = (-r_in.direction).unit();
+        float cos_theta = rclamp(wo.dot(n), 0.0f, 1.0f);
+        float th = varied(rec.p);
+        float f = rmax(schlick(cos_theta, 1.0f / ior), 0.10f);
+        float cos_illumin = (sun && sun->active) ? rclamp(n.dot(sun->dir), 0.0f, 1.0f) : cos_theta;
+        if (rng.gen() < f) {
+            Vec3 t, b; vonb(n, t, b);
+            float al = rmax(luster_roughness * luster_roughness, 1e-4f);
+            float xi1 = rng.gen(), xi2 = rng.gen();
+            Vec3 wo_ts(wo.dot(t), wo.dot(b), cos_theta);
+            Vec3 m_ts = vndf_sample_aniso(wo_ts, al, al, xi1, xi2);
+            Vec3 m = (t * m_ts.x + b * m_ts.y + n * m_ts.z).unit();
+            float vo_h = rmax(wo.dot(m), 0.0f);
+            Vec3 wi = (m * (2.0f * vo_h) - wo).unit();
+            float cos_i = wi.dot(n);
+            if (cos_i <= 0.0f) return false;
+            float f_h = schlick(vo_h, 1.0f / ior);
+            float g1_wi = smith_g1(cos_i, al);
+            Color orient = nacre_color(vo_h, ior, th);
+            sr.attenuation = orient * (f_h * g1_wi / f);
+            sr.ray = Ray::scatter_from(rec.p, wi, r_in);
+            sr.skip_pdf = true;
+            return true;
+        }
+        Color orient = nacre_color(cos_illumin, ior, th);
+        float s = orient_strength;
+        Color tinted = base_color * (orient * s + Color(1, 1, 1) * (1.0f - s));
+        sr.attenuation = tinted * (1.0f / (1.0f - f));
+        sr.ray = Ray::scatter_from(rec.p, n, r_in);
+        sr.skip_pdf = false;
+        return true;
+    }
+    float scattering_pdf(const Ray&, const HitRecord& rec, const Ray& scattered) const override {
+        return rmax(rec.normal.dot(scattered.direction.unit()) / PI, 0.0f);
+    }
+    Color specular_brdf_cos(const Ray& r_in, const HitRecord& rec, const Vec3& wi) const override {
+        Vec3 n = rec.normal;
+        Vec3 wo = (-r_in.direction).unit();
+        float cos_o = rclamp(wo.dot(n), 0.0f, 1.0f);
+        float cos_i = wi.dot(n);
+        if (cos_i <= 0.0f) return Color();
+        Vec3 h = (wo + wi).unit();
+        float wo_h = rmax(wo.dot(h), 0.0f), h_n = rmax(h.dot(n), 0.0f);
+        float al = rmax(luster_roughness * luster_roughness, 1e-4f);
+        float d = ggx_ndf(h_n, al);
+        float g1_o = smith_g1(cos_o, al), g1_i = smith_g1(cos_i, al);
+        float f_h = schlick(wo_h, 1.0f / ior);
+        Color orient = nacre_color(wo_h, ior, varied(rec.p));
+        return orient * (d * f_h * g1_o * g1_i / (4.0f * cos_o));
+    }
+    float specular_sampling_pdf(const Ray& r_in, const HitRecord& rec, const Vec3& wi) const override {
+        Vec3 n = rec.normal;
+        Vec3 wo = (-r_in.direction).unit();
+        float cos_o = rclamp(wo.dot(n), 0.0f, 1.0f);
+        float cos_i = wi.dot(n);
+        if (cos_i <= 0.0f) return 0.0f;
+        Vec3 h = (wo + wi).unit();
+        float h_n = rmax(h.dot(n), 0.0f);
+        float f = rmax(schlick(cos_o, 1.0f / ior), 0.10f);
+        float al = rmax(luster_roughness * luster_roughness, 1e-4f);
+        return f * ggx_ndf(h_n, al) * smith_g1(cos_o, al) / (4.0f * cos_o);
+    }
+    Color albedo_hint(float, float, const Point3&) const override { return base_color; }
+    bool can_receive_caustics() const override { return true; }
+};
+
 // SSSMaterial (src/material.rs:576-631)
 struct SSSMaterial : Material {
     Color albedo, sigma_a; float ior = 1.3f, density = 1.0f, g = 0.0f;

@@ -18,6 +18,7 @@ using namespace rc;
 namespace {
 
 struct OScene {
+    PearlSun pearl_sun;                      // set from a Physical background (render_tiles, src/renderer.rs:676-679)
     std::vector<std::unique_ptr<Material>> mats;
     std::vector<rt_material> desc_mats;     // the descriptor's table, kept for the adjoint checker
     std::vector<HPtr> world_objs;
@@ -97,6 +98,12 @@ Material* make_material(const rt_material& m, const rt_scene_desc* d, std::strin
         case RT_MAT_SPECTRAL_METAL: { auto* s = new SpectralMetal(); s->variant = (int)m.p[0]; s->roughness = m.p[1]; s->init(); return s; }
         case RT_MAT_ROUGH_DIELECTRIC: { auto* r = new RoughDielectric(); r->ior = m.p[0]; r->roughness = m.p[1]; return r; }
         case RT_MAT_BLACKBODY: { auto* b = new BlackbodyLight(); b->temp_k = m.p[0]; b->intensity = m.p[1]; b->init(); return b; }
+        case RT_MAT_PEARL: {
+            auto* q = new PearlMaterial();
+            q->base_color = Color(m.p[0], m.p[1], m.p[2]); q->ior = m.p[3]; q->film_thickness = m.p[4];
+            q->orient_strength = m.p[5]; q->film_scale = m.p[6]; q->luster_roughness = m.p[7];
+            return q;
+        }
         case RT_MAT_SSS: {
             auto* q = new SSSMaterial();
             q->albedo = Color(m.p[0], m.p[1], m.p[2]); q->sigma_a = Color(m.p[3], m.p[4], m.p[5]);
@@ -264,6 +271,9 @@ bool build_scene(OScene& S, const rt_scene_desc& d) {
     }
     const rt_background& b = d.background;
     S.bg.kind = (int)b.kind; S.bg.visible_kind = (int)b.visible_kind;
+    S.pearl_sun.active = b.kind == RT_BG_PHYSICAL;
+    S.pearl_sun.dir = Vec3(b.sun_dir[0], b.sun_dir[1], b.sun_dir[2]);
+    for (auto& m : S.mats) if (auto* q = dynamic_cast<PearlMaterial*>(m.get())) q->sun = &S.pearl_sun;
     S.bg.color = Color(b.color[0], b.color[1], b.color[2]);
     S.bg.top = Color(b.top[0], b.top[1], b.top[2]);
     S.bg.bottom = Color(b.bottom[0], b.bottom[1], b.bottom[2]);
@@ -401,6 +411,8 @@ int rtc_render_accumulate(void* s, const rt_camera* cam, const rt_sample_params*
 int rtc_photon_build(void* s, const rt_photon_params* pp, int threads, uint32_t* stored) {
     OScene& S = *(OScene*)s;
     if (pp->knn > RT_KNN_MAX) return -1;
+    bool sun_was = S.pearl_sun.active;
+    S.pearl_sun.active = false;              // photon tracing sees no pearl sun
     if (pp->emitter == RT_EMIT_QUAD) {
         photon_build_from_quad(S.photons, S.world, Point3(pp->origin[0], pp->origin[1], pp->origin[2]),
                                Vec3(pp->u[0], pp->u[1], pp->u[2]), Vec3(pp->v[0], pp->v[1], pp->v[2]),
@@ -411,6 +423,7 @@ int rtc_photon_build(void* s, const rt_photon_params* pp, int threads, uint32_t*
                          Color(pp->color[0], pp->color[1], pp->color[2]), pp->num_photons, pp->gather_radius, threads);
     }
     S.photons.knn = pp->knn;
+    S.pearl_sun.active = sun_was;
     S.have_photons = true;
     if (stored) *stored = (uint32_t)S.photons.nodes.size();
     return 0;
@@ -513,6 +526,8 @@ int rtc_perlin(const float* tables, const float* pts, uint32_t n, int depth, flo
 int rtc_eval(const char* fn, const float* a, float* out) {
     std::string f(fn);
     if (f == "ggx_ndf") { out[0] = ggx_ndf(a[0], a[1]); return 0; }
+    if (f == "nacre_color") { Color c = nacre_color(a[0], a[1], a[2]); out[0] = c.x; out[1] = c.y; out[2] = c.z; return 0; }
+    if (f == "oil_noise") { out[0] = oil_noise(Point3(a[0], a[1], a[2])); return 0; }
     if (f == "srgb_decode") { int c = (int)a[0]; if (c < 0 || c > 255) return -1; out[0] = srgb_decode_lut()[c]; return 0; }
     if (f == "smith_g1") { out[0] = smith_g1(a[0], a[1]); return 0; }
     if (f == "smith_lambda") { out[0] = smith_lambda(a[0], a[1]); return 0; }

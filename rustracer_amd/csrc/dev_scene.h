@@ -65,6 +65,8 @@ struct DScene {
     const DMaterial* mats;
     const DMedium* media; const float* grids;
     const uint8_t* tex_pix; const float* srgb_lut;   // image texels (sRGB bytes) and their 256-entry decode table
+    const float* nacre;                    // thin-film colour over optical path difference: 256 RGB entries
+    float pearl_sun[3]; uint32_t pearl_sun_on;      // the renderer's pearl sun (a Physical background's)
     // background
     uint32_t bg_kind, bg_visible;
     float bg_color[3], bg_top[3], bg_bottom[3], sun_dir[3], turbidity;

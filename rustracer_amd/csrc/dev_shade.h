@@ -11,7 +11,9 @@
 namespace rt {
 
 enum { M_LAMBERTIAN = 1, M_DIELECTRIC = 2, M_SPECTRAL_DIELECTRIC = 3, M_PBR = 4, M_DIFFUSE_LIGHT = 5,
-       M_SPECTRAL_METAL = 6, M_ROUGH_DIELECTRIC = 7, M_ROUGH_SPECTRAL = 8, M_BLACKBODY = 9, M_SSS = 10, M_PHASE = 100 };
+       M_SPECTRAL_METAL = 6, M_ROUGH_DIELECTRIC = 7, M_ROUGH_SPECTRAL = 8, M_BLACKBODY = 9, M_SSS = 10, M_PEARL = 11,
+       M_PHASE = 100 };
+#define M_KINDS 11
 enum { BGK_SOLID = 0, BGK_GRADIENT = 1, BGK_PHYSICAL = 2, BGK_ENV = 3, BGK_ENV_SPLIT = 4 };
 
 __constant__ float c_cie_x[RT_CIE_N];
@@ -491,6 +493,85 @@ RT_D bool sss_scatter(const DMaterial& m, const Ray& r, const Rec& h, Rng& rng, 
                           (1u << M_ROUGH_SPECTRAL) | (1u << M_SSS))
 #define RT_IN(MASK, K) (((MASK) >> (K)) & 1u)
 
+// nacre_color (src/material.rs:1119-1136): the thin-film table (built on the
+// host like NACRE_LUT, src/material.rs:1091-1117) at the optical path
+// difference, blended toward grey at grazing angles.
+RT_D V3 nacre_color(const DScene& S, float cos_theta, float film_ior, float th) {
+    float sin_sq = fmx(1.0f - cos_theta * cos_theta, 0.0f);
+    float cos_t = sqrtf(fmx(1.0f - sin_sq / (film_ior * film_ior), 0.0f));
+    float opd = 2.0f * film_ior * th * cos_t;
+    float t = fclamp(opd / 2500.0f, 0.0f, 1.0f) * 255.0f;
+    uint32_t lo = f2u(t), hi = min(lo + 1u, 255u);
+    V3 a = mk(S.nacre[3 * lo], S.nacre[3 * lo + 1], S.nacre[3 * lo + 2]);
+    V3 b = mk(S.nacre[3 * hi], S.nacre[3 * hi + 1], S.nacre[3 * hi + 2]);
+    V3 raw = a * (1.0f - (t - (float)lo)) + b * (t - (float)lo);
+    float blend = powf(cos_theta, 0.5f);
+    return raw * blend + mk(0.5f, 0.5f, 0.5f) * (1.0f - blend);
+}
+// oil_noise (src/material.rs:1138-1147)
+RT_D float oil_noise(V3 p) {
+    float a = sinf(p.x * 1.618f + p.z);
+    float b = sinf(p.z * 1.618f - p.x);
+    float c = sinf(p.x * 0.618f + p.y + p.z);
+    return (a + b + c) * (1.0f / 3.0f);
+}
+// PearlMaterial (src/material.rs:1166-1285): p0-2 base colour, p3 ior,
+// p4 film thickness, p5 orient strength, p6 film scale, p7 luster roughness.
+RT_D float pearl_thickness(const DMaterial& m, V3 p) { return m.p[4] + oil_noise(p * m.p[6]) * 150.0f; }
+RT_D bool pearl_scatter(const DScene& S, const DMaterial& m, const Ray& r, const Rec& h, Rng& rng, Scatter& sr) {
+    V3 n = h.n, wo = unit(-r.d);
+    float cos_theta = fclamp(dot(wo, n), 0.0f, 1.0f);
+    float th = pearl_thickness(m, h.p);
+    float ior = m.p[3];
+    float f = fmx(schlick(cos_theta, 1.0f / ior), 0.10f);
+    float cos_il = S.pearl_sun_on ? fclamp(dot(n, mk(S.pearl_sun[0], S.pearl_sun[1], S.pearl_sun[2])), 0.0f, 1.0f) : cos_theta;
+    if (rng_f(rng) < f) {
+        V3 t, b; vonb(n, t, b);
+        float al = fmx(m.p[7] * m.p[7], 1e-4f);
+        float xi1 = rng_f(rng), xi2 = rng_f(rng);
+        V3 wo_ts = mk(dot(wo, t), dot(wo, b), cos_theta);
+        V3 mts = vndf_sample(wo_ts, al, al, xi1, xi2);
+        V3 mm = unit(t * mts.x + b * mts.y + n * mts.z);
+        float voh = fmx(dot(wo, mm), 0.0f);
+        V3 wi = unit(2.0f * voh * mm - wo);
+        float cos_i = dot(wi, n);
+        if (cos_i <= 0.0f) return false;
+        float fh = schlick(voh, 1.0f / ior);
+        float g1 = smith_g1(cos_i, al);
+        sr.att = nacre_color(S, voh, ior, th) * (fh * g1 / f);
+        sr.ray = scatter_from(h.p, wi, r); sr.skip_pdf = true;
+        return true;
+    }
+    V3 orient = nacre_color(S, cos_il, ior, th);
+    float s = m.p[5];
+    V3 tinted = mk(m.p[0], m.p[1], m.p[2]) * (orient * s + mk(1, 1, 1) * (1.0f - s));
+    sr.att = tinted * (1.0f / (1.0f - f));
+    sr.ray = scatter_from(h.p, n, r); sr.skip_pdf = false;
+    return true;
+}
+RT_D V3 pearl_brdf_cos(const DScene& S, const DMaterial& m, const Ray& r, const Rec& h, V3 wi) {
+    V3 n = h.n, wo = unit(-r.d);
+    float cos_o = fclamp(dot(wo, n), 0.0f, 1.0f), cos_i = dot(wi, n);
+    if (cos_i <= 0.0f) return mk(0, 0, 0);
+    V3 hv = unit(wo + wi);
+    float woh = fmx(dot(wo, hv), 0.0f), hn = fmx(dot(hv, n), 0.0f);
+    float al = fmx(m.p[7] * m.p[7], 1e-4f);
+    float d = ggx_ndf(hn, al);
+    float g1o = smith_g1(cos_o, al), g1i = smith_g1(cos_i, al);
+    float fh = schlick(woh, 1.0f / m.p[3]);
+    return nacre_color(S, woh, m.p[3], pearl_thickness(m, h.p)) * (d * fh * g1o * g1i / (4.0f * cos_o));
+}
+RT_D float pearl_spec_pdf(const DMaterial& m, const Ray& r, const Rec& h, V3 wi) {
+    V3 n = h.n, wo = unit(-r.d);
+    float cos_o = fclamp(dot(wo, n), 0.0f, 1.0f), cos_i = dot(wi, n);
+    if (cos_i <= 0.0f) return 0.0f;
+    V3 hv = unit(wo + wi);
+    float hn = fmx(dot(hv, n), 0.0f);
+    float f = fmx(schlick(cos_o, 1.0f / m.p[3]), 0.10f);
+    float al = fmx(m.p[7] * m.p[7], 1e-4f);
+    return f * ggx_ndf(hn, al) * smith_g1(cos_o, al) / (4.0f * cos_o);
+}
+
 // Material::scatter dispatch.  Medium events scatter with the HG phase function.
 template <uint32_t MASK = RT_MASK_ALL>
 RT_D bool mat_scatter(const DScene& S, const Rec& h, const Ray& r, Rng& rng, Scatter& sr) {
@@ -536,6 +617,7 @@ RT_D bool mat_scatter(const DScene& S, const Rec& h, const Ray& r, Rng& rng, Sca
         case M_ROUGH_DIELECTRIC: return RT_IN(MASK, M_ROUGH_DIELECTRIC) && rough_dielectric_scatter(m, r, h, rng, sr);
         case M_ROUGH_SPECTRAL: return RT_IN(MASK, M_ROUGH_SPECTRAL) && rsd_scatter(S, m, r, h, rng, sr);
         case M_SSS: return RT_IN(MASK, M_SSS) && sss_scatter(m, r, h, rng, sr);
+        case M_PEARL: return RT_IN(MASK, M_PEARL) && pearl_scatter(S, m, r, h, rng, sr);
         default: return false;   // DiffuseLight
     }
 }
@@ -568,13 +650,13 @@ RT_D V3 mat_albedo_hint(const DScene& S, const Rec& h) {
     if (m.kind == M_PBR) return mk(m.p[0], m.p[1], m.p[2]);
     if (m.kind == M_DIFFUSE_LIGHT) return tex_value(S, m, h.u, h.v, h.p);
     if (m.kind == M_BLACKBODY) return mk(m.p[3], m.p[4], m.p[5]);
-    if (m.kind == M_SSS) return mk(m.p[0], m.p[1], m.p[2]);
+    if (m.kind == M_SSS || m.kind == M_PEARL) return mk(m.p[0], m.p[1], m.p[2]);
     return mk(1, 1, 1);
 }
 RT_D bool mat_receives_caustics(const DScene& S, const Rec& h) {
     if (h.medium) return false;
     const DMaterial& m = S.mats[h.mat];
-    return m.kind == M_LAMBERTIAN || (m.kind == M_PBR && m.p[4] < 0.999f);
+    return m.kind == M_LAMBERTIAN || m.kind == M_PEARL || (m.kind == M_PBR && m.p[4] < 0.999f);
 }
 template <uint32_t MASK = RT_MASK_ALL>
 RT_D V3 mat_spec_brdf_cos(const DScene& S, const Rec& h, const Ray& r, V3 wi) {
@@ -582,6 +664,7 @@ RT_D V3 mat_spec_brdf_cos(const DScene& S, const Rec& h, const Ray& r, V3 wi) {
     const DMaterial& m = S.mats[h.mat];
     if (RT_IN(MASK, M_PBR) && m.kind == M_PBR) return pbr_brdf_cos(S, m, r, h, wi);
     if (RT_IN(MASK, M_SPECTRAL_METAL) && m.kind == M_SPECTRAL_METAL) return smetal_brdf_cos(S, m, r, h, wi);
+    if (RT_IN(MASK, M_PEARL) && m.kind == M_PEARL) return pearl_brdf_cos(S, m, r, h, wi);
     return mk(0, 0, 0);
 }
 template <uint32_t MASK = RT_MASK_ALL>
@@ -590,6 +673,7 @@ RT_D float mat_spec_pdf(const DScene& S, const Rec& h, const Ray& r, V3 wi) {
     const DMaterial& m = S.mats[h.mat];
     if (RT_IN(MASK, M_PBR) && m.kind == M_PBR) return pbr_spec_pdf(m, r, h, wi);
     if (RT_IN(MASK, M_SPECTRAL_METAL) && m.kind == M_SPECTRAL_METAL) return smetal_spec_pdf(m, r, h, wi);
+    if (RT_IN(MASK, M_PEARL) && m.kind == M_PEARL) return pearl_spec_pdf(m, r, h, wi);
     return 0.0f;
 }
 

@@ -271,7 +271,7 @@ __global__ __launch_bounds__(BLOCK) void k_photon_irr(DScene S, const float* __r
 // queue length from device memory, so the host never synchronises inside a
 // frame.  Contributions go straight into the frame accumulator with float
 // atomics.
-#define WF_NB 12          // sort keys (sort 1): 0 miss, 1..10 material kind, 11 medium event;
+#define WF_NB (M_KINDS + 2)   // sort keys (sort 1): 0 miss, 1..M_KINDS material kind, M_KINDS + 1 medium event;
                           // (sort 2): 0 miss / specular / medium, 1 every other material
 #define WF_CNT_ACTIVE 0   // counters[0]: paths to extend this bounce
 #define WF_CNT_NEXT 1     // counters[1]: survivors written by shade
@@ -424,9 +424,9 @@ __global__ __launch_bounds__(BLOCK) void k_wf_generate(DCamera cam0, DParams P0,
 __device__ __forceinline__ uint32_t sort_key(const DScene& S, const HitId& id, uint32_t sort) {
     if (sort == 0) return 0u;
     if (id.slot < 0) return 0u;
-    if (id.prim == -2) return sort == 1 ? 11u : 0u;
+    if (id.prim == -2) return sort == 1 ? (uint32_t)M_KINDS + 1u : 0u;
     uint32_t k = S.mats[S.objs[id.slot].material].kind;
-    if (sort == 1) return k < 11u ? k : 0u;
+    if (sort == 1) return k <= (uint32_t)M_KINDS ? k : 0u;
     return RT_IN(RT_MASK_SPECULAR, k) ? 0u : 1u;
 }
 
@@ -546,7 +546,7 @@ __global__ __launch_bounds__(BLOCK) void k_wf_knn(DScene S0, DParams P0, WfQueue
         bool photons = BATCH ? B.views[w.pad0].P.use_photons != 0 : P0.use_photons != 0;
         if (!photons || S.ph_knn == 0 || S.num_photons == 0 || w.hit_slot < 0 || w.hit_prim == -2) continue;
         uint32_t kind = S.mats[S.objs[w.hit_slot].material].kind;
-        if (kind != M_LAMBERTIAN && kind != M_PBR) continue;
+        if (kind != M_LAMBERTIAN && kind != M_PBR && kind != M_PEARL) continue;
         Ray r = make_ray(mk(w.o_time.x, w.o_time.y, w.o_time.z), mk(w.d_wl.x, w.d_wl.y, w.d_wl.z), w.o_time.w);
         HitId id; id.t = w.hit_t; id.slot = w.hit_slot; id.prim = w.hit_prim;
         Rec h = finalize(S, r, id, 0.001f);
@@ -1019,6 +1019,38 @@ struct Flat {
     std::string err;
 };
 
+// spectral_to_rgb (src/spectrum.rs:104-114) on the host: the CIE table lerp,
+// XYZ -> linear sRGB clamped, times the per-channel normalisation.
+static void host_spectral_rgb(const float norms[3], float lambda, float out[3]) {
+    float t = std::fmin(std::fmax((lambda - RT_CIE_START) / RT_CIE_STEP, 0.0f), (float)(RT_CIE_N - 1));
+    int i0 = std::min((int)t, RT_CIE_N - 2);
+    float fr = t - (float)i0;
+    float x = RT_CIE_X[i0] + fr * (RT_CIE_X[i0 + 1] - RT_CIE_X[i0]);
+    float y = RT_CIE_Y[i0] + fr * (RT_CIE_Y[i0 + 1] - RT_CIE_Y[i0]);
+    float z = RT_CIE_Z[i0] + fr * (RT_CIE_Z[i0 + 1] - RT_CIE_Z[i0]);
+    float c[3]; xyz_srgb(x, y, z, c);
+    for (int a = 0; a < 3; a++) out[a] = c[a] * norms[a];
+}
+
+// NACRE_LUT (src/material.rs:1091-1117): the thin-film interference integral
+// over 65 wavelengths at 256 optical path differences in [0, 2500] nm.
+static std::vector<float> nacre_table(const float norms[3]) {
+    std::vector<float> lut(256 * 3);
+    for (int i = 0; i < 256; i++) {
+        float opd = (float)i / 255.0f * 2500.0f;
+        float c[3] = {0, 0, 0};
+        float lambda = 380.0f;
+        while (lambda <= 700.01f) {
+            float irid = 0.5f * (1.0f + std::cos(2.0f * PI * opd / lambda));
+            float s[3]; host_spectral_rgb(norms, lambda, s);
+            for (int a = 0; a < 3; a++) c[a] += s[a] * irid;
+            lambda += 5.0f;
+        }
+        for (int a = 0; a < 3; a++) lut[3 * i + a] = c[a] / 65.0f;
+    }
+    return lut;
+}
+
 // Perlin tables (ranvec[256][3], perm_x, perm_y, perm_z as floats): the
 // permutations index the tables, so they must be integers in 0..255.
 static bool perlin_perms_ok(const float* t) {
@@ -1234,7 +1266,7 @@ extern "C" int rt_scene_upload(rt_ctx* ctx, const rt_scene_desc* d, rt_scene** o
     Flat F;
     for (uint32_t i = 0; i < d->num_materials; i++) {
         uint32_t k = d->materials[i].kind;
-        if (k < RT_MAT_LAMBERTIAN || k > RT_MAT_SSS) return fail(ctx, RT_ERR_UNSUPPORTED, "unsupported material kind " + std::to_string(k));
+        if (k < RT_MAT_LAMBERTIAN || k > RT_MAT_PEARL) return fail(ctx, RT_ERR_UNSUPPORTED, "unsupported material kind " + std::to_string(k));
     }
     F.objs_input.resize(d->num_objects);
     F.obj_boxes.resize(d->num_objects);
@@ -1360,16 +1392,9 @@ extern "C" int rt_scene_upload(rt_ctx* ctx, const rt_scene_desc* d, rt_scene** o
             float norm = 1.0f / std::fmax(mean, 1e-30f);
             float acc[3] = {0, 0, 0};
             for (int j = 0; j < RT_CIE_N; j++) {
-                // spectral_to_rgb(380 + 5 j): the table lerp of cie_at, on the host
-                float t = std::fmin(std::fmax(((380.0f + (float)j * 5.0f) - RT_CIE_START) / RT_CIE_STEP, 0.0f), (float)(RT_CIE_N - 1));
-                int i0 = std::min((int)t, RT_CIE_N - 2);
-                float fr = t - (float)i0;
-                float x = RT_CIE_X[i0] + fr * (RT_CIE_X[i0 + 1] - RT_CIE_X[i0]);
-                float y = RT_CIE_Y[i0] + fr * (RT_CIE_Y[i0 + 1] - RT_CIE_Y[i0]);
-                float z = RT_CIE_Z[i0] + fr * (RT_CIE_Z[i0 + 1] - RT_CIE_Z[i0]);
-                float c[3]; xyz_srgb(x, y, z, c);
+                float c[3]; host_spectral_rgb(ctx->norms, 380.0f + (float)j * 5.0f, c);
                 float w = raw[j] * norm;
-                for (int a = 0; a < 3; a++) acc[a] += (c[a] * ctx->norms[a]) * w;
+                for (int a = 0; a < 3; a++) acc[a] += c[a] * w;
             }
             mats[i].p[2] = norm;
             for (int a = 0; a < 3; a++) mats[i].p[3 + a] = acc[a] / (float)RT_CIE_N * intensity;
@@ -1386,6 +1411,7 @@ extern "C" int rt_scene_upload(rt_ctx* ctx, const rt_scene_desc* d, rt_scene** o
         }
         if ((rc = upload_vec(s.get(), lut, &S.srgb_lut))) return rc;
     }
+    if ((rc = upload_vec(s.get(), nacre_table(ctx->norms), &S.nacre))) return rc;
     if ((rc = upload_vec(s.get(), mats, &S.mats))) return rc;
     s->mats_host = mats;
     // background
@@ -1395,6 +1421,9 @@ extern "C" int rt_scene_upload(rt_ctx* ctx, const rt_scene_desc* d, rt_scene** o
         S.bg_color[k] = b.color[k]; S.bg_top[k] = b.top[k]; S.bg_bottom[k] = b.bottom[k]; S.sun_dir[k] = b.sun_dir[k];
     }
     S.turbidity = b.turbidity;
+    // render_tiles sets the pearl sun from a Physical background (src/renderer.rs:676-679)
+    S.pearl_sun_on = b.kind == RT_BG_PHYSICAL ? 1u : 0u;
+    for (int k = 0; k < 3; k++) S.pearl_sun[k] = b.sun_dir[k];
     if (b.kind == RT_BG_ENVMAP || b.kind == RT_BG_ENVMAP_SPLIT) {
         if (!b.env_rgb || !b.env_w || !b.env_h) return fail(ctx, RT_ERR_INVALID, "env map missing");
         uint32_t w = b.env_w, h = b.env_h;
@@ -2102,6 +2131,7 @@ int rt_photon_build(rt_ctx* ctx, rt_scene* s, const rt_photon_params* pp, uint32
     std::unique_ptr<void, void (*)(void*)> g2(d_ok, [](void* p) { (void)hipFree(p); });
     DScene S = s->S;
     S.num_photons = 0;
+    S.pearl_sun_on = 0;       // photon tracing sees no pearl sun (src/material.rs:1052-1059)
     hipLaunchKernelGGL(k_photon_trace, dim3((n + BLOCK - 1) / BLOCK), dim3(BLOCK), 0, ctx->stream, S, E, n, d_out, d_ok);
     HIPCHECK(ctx, hipGetLastError());
     HIPCHECK(ctx, hipStreamSynchronize(ctx->stream));

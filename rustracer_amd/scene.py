@@ -285,6 +285,23 @@ class SSSMaterial(Material):
         return list(self.albedo) + list(self.sigma_a) + [self.ior, self.density, self.g]
 
 
+class PearlMaterial(Material):
+    """PearlMaterial (src/material.rs:1149-1287): GGX luster tinted by
+    thin-film interference over a Lambertian body; defaults of the scene
+    file (src/scene_file.rs:460-465)."""
+    kind = F.MAT_PEARL
+
+    def __init__(self, base_color=(0.98, 0.93, 0.88), ior=1.56, film_thickness=450.0, orient_strength=0.30,
+                 film_scale=3.0, luster_roughness=0.05):
+        self.base_color = v3(base_color)
+        self.ior, self.film_thickness, self.orient_strength = float(ior), float(film_thickness), float(orient_strength)
+        self.film_scale, self.luster_roughness = float(film_scale), float(luster_roughness)
+
+    def params(self):
+        return list(self.base_color) + [self.ior, self.film_thickness, self.orient_strength, self.film_scale,
+                                        self.luster_roughness]
+
+
 class RoughSpectralDielectric(Material):
     """Rough dispersive dielectric (extension): n(l) = n_d + s(l) - s(589.3 nm)
     with s the Sellmeier law (B, C in um^2); GGX alpha `alpha_table` on

@@ -456,3 +456,36 @@ def test_knn_photon_gather_matches_checker_and_megakernel(gpu_renderer):
     from rustracer_amd.renderer import RtError
     with pytest.raises(RtError, match="fixed radius"):
         gpu_renderer.backward(np.zeros((32, 32, 3), np.float32), cam, 32, 32, 4, [(0, 0, 1)], max_depth=8, photon_map=True)
+
+
+@pytest.mark.parametrize("sky", [True, False])
+def test_pearl_same_streams_and_three_sigma(gpu_renderer, sky):
+    """PearlMaterial: under a Physical sky the body glow follows the pearl
+    sun (and the luster gets sun NEE); under a quad light it follows the view."""
+    from rustracer_amd import scene as S
+    pearl = S.PearlMaterial(film_scale=4.0)
+    objs = [S.Quad((-4, 0, -4), (8, 0, 0), (0, 0, 8), S.Lambertian((0.6, 0.6, 0.6))),
+            S.Sphere((0.0, 0.4, 0.0), 0.4, pearl),
+            S.Sphere((0.7, 0.25, -0.3), 0.25, S.PearlMaterial(base_color=(0.2, 0.2, 0.25), luster_roughness=0.2))]
+    lights = []
+    if sky:
+        bg = S.Background.physical(30.0, 35.0, 3.0)
+    else:
+        lq = S.Quad((-0.3, 1.8, -0.3), (0.6, 0, 0), (0, 0, 0.6), S.DiffuseLight((8.0, 8.0, 8.0)))
+        objs.append(lq)
+        lights = [lq]
+        bg = S.Background.solid((0.05, 0.05, 0.06))
+    sc = S.Scene(objs, lights, bg)
+    cam = S.Camera((0.0, 1.0, -2.4), (0.0, 0.35, 0.0), (0, 1, 0), 40.0, 1.0)
+    gpu_renderer.upload(sc)
+    orc = O.OracleScene(sc)
+    g = passes_gpu(gpu_renderer, cam, 40, 40, 4, 12)
+    c = passes_cpu(orc, cam, 40, 40, 4, 12, mode=1)
+    close = np.isclose(g, c, rtol=1e-3, atol=1e-4).all(axis=-1)
+    assert close.mean() > 0.95, close.mean()
+    K = 96
+    g = passes_gpu(gpu_renderer, cam, 32, 32, K, 12, seed=2)
+    c = passes_cpu(orc, cam, 32, 32, K, 12, mode=0)
+    assert np.isfinite(g).all()
+    z = three_sigma(g, c)
+    assert (z > 3).mean() <= 0.02 and (z > 6).mean() <= 0.002, ((z > 3).mean(), (z > 6).mean())
