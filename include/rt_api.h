@@ -142,7 +142,12 @@ typedef struct {
  *   PBR         p0-2 albedo, p3 roughness, p4 metallic, p5 anisotropy,
  *               p6 anisotropy_angle, p7 clearcoat, p8 clearcoat_roughness,
  *               p9-11 emission, p12 emission_strength, p13 sheen,
- *               p14 sheen_tint                      src/material.rs:695-1050
+ *               p14 sheen_tint, p15 film_thickness (nm; 0 = no thin-film tint of
+ *               the clearcoat), p16 film_ior, p17-21 image maps as index + 1
+ *               into rt_scene_desc.images (0 = none): albedo (sRGB), roughness,
+ *               metallic, ambient occlusion (linear, red channel), tangent-space
+ *               normal (applies where the hit has a tangent: meshes); bilinear
+ *               fetches                             src/material.rs:632-1050
  *   DIFFUSE_LIGHT emission texture (below)          src/material.rs:14-25
  *   SPECTRAL_METAL p0 variant (0 gold, 1 copper, 2 silver), p1 roughness
  *                                                  src/material.rs:427-554

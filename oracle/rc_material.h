@@ -519,11 +519,122 @@ struct SpectralMetal : Material {
     Color albedo_hint(float, float, const Point3&) const override { return avg_color; }
 };
 
+// Thin-film colour (src/material.rs:1091-1136): the 256-entry table of the
+// 65-wavelength interference integral over optical path difference, read
+// with linear interpolation, blended toward grey at grazing angles.
+static const int NACRE_N = 256;
+static const float NACRE_OPD_MAX = 2500.0f;
+static inline const std::vector<Color>& nacre_lut() {
+    static const std::vector<Color> lut = [] {
+        std::vector<Color> t(NACRE_N);
+        for (int i = 0; i < NACRE_N; i++) {
+            float opd = (float)i / (float)(NACRE_N - 1) * NACRE_OPD_MAX;
+            Color c;
+            float lambda = 380.0f;
+            while (lambda <= 700.01f) {
+                float irid = 0.5f * (1.0f + std::cos(2.0f * PI * opd / lambda));
+                c += spectral_to_rgb(lambda) * irid;
+                lambda += 5.0f;
+            }
+            t[i] = c / 65.0f;
+        }
+        return t;
+    }();
+    return lut;
+}
+static inline Color nacre_color(float cos_theta, float film_ior, float thickness) {
+    float sin_sq = rmax(1.0f - cos_theta * cos_theta, 0.0f);
+    float cos_t = std::sqrt(rmax(1.0f - sin_sq / (film_ior * film_ior), 0.0f));
+    float opd = 2.0f * film_ior * thickness * cos_t;
+    const std::vector<Color>& lut = nacre_lut();
+    float t = rclamp(opd / NACRE_OPD_MAX, 0.0f, 1.0f) * (float)(NACRE_N - 1);
+    size_t lo = f2usize(t);
+    size_t hi = std::min<size_t>(lo + 1, NACRE_N - 1);
+    Color raw = lut[lo] * (1.0f - (t - (float)lo)) + lut[hi] * (t - (float)lo);
+    float blend = std::pow(cos_theta, 0.5f);
+    return raw * blend + Color(0.5f, 0.5f, 0.5f) * (1.0f - blend);
+}
+// oil_noise (src/material.rs:1138-1147)
+static inline float oil_noise(const Point3& p) {
+    float a = std::sin(p.x * 1.618f + p.z);
+    float b = std::sin(p.z * 1.618f - p.x);
+    float c = std::sin(p.x * 0.618f + p.y + p.z);
+    return (a + b + c) * (1.0f / 3.0f);
+}
+
+// Texture-map fetches of PbrMaterial (src/material.rs:632-671): the 2x2
+// bilinear neighbourhood with texel centres at half-integers, edges clamped.
+static inline void bilinear_coords(const RgbImage& img, float u, float v, uint32_t& x0, uint32_t& y0, uint32_t& x1,
+                                   uint32_t& y1, float& tx, float& ty) {
+    u = rclamp(u, 0.0f, 1.0f);
+    v = 1.0f - rclamp(v, 0.0f, 1.0f);
+    float fx = rclamp(u * (float)img.w - 0.5f, 0.0f, (float)(img.w - 1));
+    float fy = rclamp(v * (float)img.h - 0.5f, 0.0f, (float)(img.h - 1));
+    x0 = f2u32(fx); y0 = f2u32(fy);
+    x1 = std::min(x0 + 1, img.w - 1); y1 = std::min(y0 + 1, img.h - 1);
+    tx = fx - (float)x0; ty = fy - (float)y0;
+}
+static inline const uint8_t* texel(const RgbImage& img, uint32_t x, uint32_t y) { return &img.rgb[3 * ((size_t)y * img.w + x)]; }
+static inline Color sample_srgb(const RgbImage& img, float u, float v) {
+    uint32_t x0, y0, x1, y1; float tx, ty;
+    bilinear_coords(img, u, v, x0, y0, x1, y1, tx, ty);
+    const float* lut = srgb_decode_lut();
+    auto px = [&](uint32_t x, uint32_t y) { const uint8_t* q = texel(img, x, y); return Color(lut[q[0]], lut[q[1]], lut[q[2]]); };
+    Color c0 = px(x0, y0) * (1.0f - tx) + px(x1, y0) * tx;
+    Color c1 = px(x0, y1) * (1.0f - tx) + px(x1, y1) * tx;
+    return c0 * (1.0f - ty) + c1 * ty;
+}
+static inline float sample_linear(const RgbImage& img, float u, float v) {
+    uint32_t x0, y0, x1, y1; float tx, ty;
+    bilinear_coords(img, u, v, x0, y0, x1, y1, tx, ty);
+    auto px = [&](uint32_t x, uint32_t y) { return (float)texel(img, x, y)[0] / 255.0f; };
+    float v0 = px(x0, y0) * (1.0f - tx) + px(x1, y0) * tx;
+    float v1 = px(x0, y1) * (1.0f - tx) + px(x1, y1) * tx;
+    return v0 * (1.0f - ty) + v1 * ty;
+}
+static inline Color sample_rgb(const RgbImage& img, float u, float v) {
+    uint32_t x0, y0, x1, y1; float tx, ty;
+    bilinear_coords(img, u, v, x0, y0, x1, y1, tx, ty);
+    auto px = [&](uint32_t x, uint32_t y) { const uint8_t* q = texel(img, x, y); return Color((float)q[0] / 255.0f, (float)q[1] / 255.0f, (float)q[2] / 255.0f); };
+    Color c0 = px(x0, y0) * (1.0f - tx) + px(x1, y0) * tx;
+    Color c1 = px(x0, y1) * (1.0f - tx) + px(x1, y1) * tx;
+    return c0 * (1.0f - ty) + c1 * ty;
+}
+struct PbrParams { Vec3 normal; Color albedo; float roughness, metallic; };
+
 struct PbrMaterial : Material {
     Color albedo = Color(0.8f, 0.8f, 0.8f);
     float roughness = 0.5f, metallic = 0.0f, anisotropy = 0.0f, anisotropy_angle = 0.0f;
     float clearcoat = 0.0f, clearcoat_roughness = 0.03f, sheen = 0.0f, sheen_tint = 0.5f;
     Color emission; float emission_strength = 0.0f;
+    float film_thickness = 0.0f, film_ior = 1.5f;
+    std::shared_ptr<const RgbImage> albedo_tex, roughness_tex, metallic_tex, ao_tex, normal_tex;
+
+    // tbn_normal / albedo_at / roughness_at / metallic_at (src/material.rs:755-782)
+    Vec3 tbn_normal(const HitRecord& rec) const {
+        if (normal_tex && rec.tangent.length_squared() > 1e-6f) {
+            Color raw = sample_rgb(*normal_tex, rec.u, rec.v);
+            Vec3 ts(raw.x * 2.0f - 1.0f, raw.y * 2.0f - 1.0f, raw.z * 2.0f - 1.0f);
+            Vec3 ng = rec.normal, t = rec.tangent;
+            Vec3 b = ng.cross(t);
+            Vec3 pert = t * ts.x + b * ts.y + ng * ts.z;
+            return pert.length_squared() > 1e-12f ? pert.unit() : rec.normal;
+        }
+        return rec.normal;
+    }
+    Color albedo_at(float u, float v) const {
+        Color c = albedo_tex ? sample_srgb(*albedo_tex, u, v) : albedo;
+        return ao_tex ? c * sample_linear(*ao_tex, u, v) : c;
+    }
+    PbrParams params(const HitRecord& rec) const {
+        PbrParams p;
+        p.normal = tbn_normal(rec);
+        p.albedo = albedo_at(rec.u, rec.v);
+        p.roughness = roughness_tex ? sample_linear(*roughness_tex, rec.u, rec.v) : roughness;
+        p.metallic = metallic_tex ? sample_linear(*metallic_tex, rec.u, rec.v) : metallic;
+        return p;
+    }
+    Color film_color(float cos_h) const { return film_thickness > 0.0f ? nacre_color(cos_h, film_ior, film_thickness) : Color(1, 1, 1); }
 
     void frame(const Vec3& n, Vec3& t, Vec3& b) const {
         Vec3 t0, b0; vonb(n, t0, b0);
@@ -534,7 +645,10 @@ struct PbrMaterial : Material {
     float aspect() const { return std::sqrt(rmax(1.0f - 0.9f * rclamp(anisotropy, 0.0f, 1.0f), 0.001f)); }
 
     bool scatter(const Ray& r_in, const HitRecord& rec, SmallRng& rng, ScatterRecord& sr) const override {
-        Vec3 n = rec.normal;
+        const PbrParams pp = params(rec);
+        const Color albedo = pp.albedo;
+        const float roughness = pp.roughness, metallic = pp.metallic;
+        Vec3 n = pp.normal;
         Vec3 wo = (-r_in.direction).unit();
         float cos_o = wo.dot(n);
         if (cos_o <= 0.0f) return false;
@@ -557,7 +671,7 @@ struct PbrMaterial : Material {
             float f_coat = schlick(vo_h, 1.0f / 1.5f) * clearcoat;
             float lam_o = smith_lambda(cos_o, alpha_coat), lam_i = smith_lambda(cos_i, alpha_coat);
             float w = ((1.0f + lam_o) / (1.0f + lam_o + lam_i)) * specular_energy_comp(cos_o, alpha_coat);
-            sr.attenuation = Color(1, 1, 1) * (f_coat * w / p_coat);
+            sr.attenuation = film_color(vo_h) * (f_coat * w / p_coat);
             sr.ray = Ray::scatter_from(rec.p, wi, r_in);
             sr.skip_pdf = true;
             return true;
@@ -600,11 +714,15 @@ struct PbrMaterial : Material {
     Color emitted(float, float, const Point3&) const override { return emission * emission_strength; }
     bool is_emissive() const override { return emission_strength > 0.0f && emission.luminance() > 0.0f; }
     float scattering_pdf(const Ray&, const HitRecord& rec, const Ray& scattered) const override {
-        float cosine = rec.normal.dot(scattered.direction.unit());
+        float cosine = params(rec).normal.dot(scattered.direction.unit());
         return rmax(cosine / PI, 0.0f);
     }
+    Vec3 shading_normal(const HitRecord& rec) const override { return params(rec).normal; }
     Color specular_brdf_cos(const Ray& r_in, const HitRecord& rec, const Vec3& wi) const override {
-        Vec3 n = rec.normal, wo = (-r_in.direction).unit();
+        const PbrParams pp = params(rec);
+        const Color albedo = pp.albedo;
+        const float roughness = pp.roughness, metallic = pp.metallic;
+        Vec3 n = pp.normal, wo = (-r_in.direction).unit();
         float cos_o = wo.dot(n), cos_i = wi.dot(n);
         if (cos_o <= 0.0f || cos_i <= 0.0f) return Color();
         Vec3 h = (wo + wi).unit();
@@ -616,7 +734,7 @@ struct PbrMaterial : Material {
             float d = ggx_ndf(h_n, ac);
             float g2 = smith_g2_hc(smith_lambda(cos_o, ac), smith_lambda(cos_i, ac));
             float f_c = schlick(wo_h, 1.0f / 1.5f) * clearcoat;
-            result += Color(1, 1, 1) * (d * f_c * g2 / (4.0f * cos_o) * specular_energy_comp(cos_o, ac));
+            result += film_color(wo_h) * (d * f_c * g2 / (4.0f * cos_o) * specular_energy_comp(cos_o, ac));
         }
         float alpha = rmax(roughness * roughness, 1e-4f);
         float asp = aspect(); float ax = alpha / asp, ay = alpha * asp;
@@ -629,7 +747,10 @@ struct PbrMaterial : Material {
         return result;
     }
     float specular_sampling_pdf(const Ray& r_in, const HitRecord& rec, const Vec3& wi) const override {
-        Vec3 n = rec.normal, wo = (-r_in.direction).unit();
+        const PbrParams pp = params(rec);
+        const Color albedo = pp.albedo;
+        const float roughness = pp.roughness, metallic = pp.metallic;
+        Vec3 n = pp.normal, wo = (-r_in.direction).unit();
         float cos_o = wo.dot(n), cos_i = wi.dot(n);
         if (cos_o <= 0.0f || cos_i <= 0.0f) return 0.0f;
         Vec3 h = (wo + wi).unit();
@@ -654,7 +775,7 @@ struct PbrMaterial : Material {
         }
         return pdf;
     }
-    Color albedo_hint(float, float, const Point3&) const override { return albedo; }
+    Color albedo_hint(float u, float v, const Point3&) const override { return albedo_at(u, v); }
     bool can_receive_caustics() const override { return metallic < 0.999f; }
 };
 
@@ -742,49 +863,6 @@ struct RoughSpectralDielectric : Material {
 
 // Henyey-Greenstein phase "material" of a medium (src/volume.rs:24-62).
 static inline Vec3 hg_sample(const Vec3& v_in, float g, SmallRng& rng);
-
-// Thin-film colour (src/material.rs:1091-1136): the 256-entry table of the
-// 65-wavelength interference integral over optical path difference, read
-// with linear interpolation, blended toward grey at grazing angles.
-static const int NACRE_N = 256;
-static const float NACRE_OPD_MAX = 2500.0f;
-static inline const std::vector<Color>& nacre_lut() {
-    static const std::vector<Color> lut = [] {
-        std::vector<Color> t(NACRE_N);
-        for (int i = 0; i < NACRE_N; i++) {
-            float opd = (float)i / (float)(NACRE_N - 1) * NACRE_OPD_MAX;
-            Color c;
-            float lambda = 380.0f;
-            while (lambda <= 700.01f) {
-                float irid = 0.5f * (1.0f + std::cos(2.0f * PI * opd / lambda));
-                c += spectral_to_rgb(lambda) * irid;
-                lambda += 5.0f;
-            }
-            t[i] = c / 65.0f;
-        }
-        return t;
-    }();
-    return lut;
-}
-static inline Color nacre_color(float cos_theta, float film_ior, float thickness) {
-    float sin_sq = rmax(1.0f - cos_theta * cos_theta, 0.0f);
-    float cos_t = std::sqrt(rmax(1.0f - sin_sq / (film_ior * film_ior), 0.0f));
-    float opd = 2.0f * film_ior * thickness * cos_t;
-    const std::vector<Color>& lut = nacre_lut();
-    float t = rclamp(opd / NACRE_OPD_MAX, 0.0f, 1.0f) * (float)(NACRE_N - 1);
-    size_t lo = f2usize(t);
-    size_t hi = std::min<size_t>(lo + 1, NACRE_N - 1);
-    Color raw = lut[lo] * (1.0f - (t - (float)lo)) + lut[hi] * (t - (float)lo);
-    float blend = std::pow(cos_theta, 0.5f);
-    return raw * blend + Color(0.5f, 0.5f, 0.5f) * (1.0f - blend);
-}
-// oil_noise (src/material.rs:1138-1147)
-static inline float oil_noise(const Point3& p) {
-    float a = std::sin(p.x * 1.618f + p.z);
-    float b = std::sin(p.z * 1.618f - p.x);
-    float c = std::sin(p.x * 0.618f + p.y + p.z);
-    return (a + b + c) * (1.0f / 3.0f);
-}
 
 // The renderer's pearl sun (src/material.rs:1052-1089): set from a Physical
 // background for the frame, absent otherwise and while photons are traced.

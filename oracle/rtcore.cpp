@@ -88,6 +88,19 @@ Material* make_material(const rt_material& m, const rt_scene_desc* d, std::strin
             p->anisotropy = m.p[5]; p->anisotropy_angle = m.p[6]; p->clearcoat = m.p[7]; p->clearcoat_roughness = m.p[8];
             p->emission = Color(m.p[9], m.p[10], m.p[11]); p->emission_strength = m.p[12];
             p->sheen = m.p[13]; p->sheen_tint = m.p[14];
+            p->film_thickness = m.p[15]; p->film_ior = m.p[16];
+            std::shared_ptr<const RgbImage>* maps[5] = {&p->albedo_tex, &p->roughness_tex, &p->metallic_tex, &p->ao_tex, &p->normal_tex};
+            for (int k = 0; k < 5; k++) {
+                float f = m.p[17 + k];
+                if (f == 0.0f) continue;
+                if (!d || !d->images || !(f >= 1.0f && f <= (float)d->num_images && f == std::floor(f))) { err = "PBR map image index"; delete p; return nullptr; }
+                const rt_image& im = d->images[(uint32_t)f - 1];
+                if (!im.rgb || !im.width || !im.height) { err = "empty image"; delete p; return nullptr; }
+                auto img = std::make_shared<RgbImage>();
+                img->w = im.width; img->h = im.height;
+                img->rgb.assign(im.rgb, im.rgb + (size_t)im.width * im.height * 3);
+                *maps[k] = img;
+            }
             return p;
         }
         case RT_MAT_DIFFUSE_LIGHT: {
@@ -505,6 +518,26 @@ int rtc_texture_eval(const rt_material* m, const rt_scene_desc* d, const float* 
         const float* q = uvp + (size_t)i * 5;
         Color c = t.value(q[0], q[1], Point3(q[2], q[3], q[4]));
         out[3 * i] = c.x; out[3 * i + 1] = c.y; out[3 * i + 2] = c.z;
+    }
+    return 0;
+}
+
+// PbrMaterial::params for n hit records (u, v, normal xyz, tangent xyz):
+// n x 8 = shading normal, albedo (maps and ambient occlusion), roughness, metallic.
+int rtc_pbr_params(const rt_material* m, const rt_scene_desc* d, const float* recs, uint32_t n, float* out) {
+    std::string err;
+    std::unique_ptr<Material> mat(make_material(*m, d, err));
+    auto* p = dynamic_cast<PbrMaterial*>(mat.get());
+    if (!p) return -1;
+    for (uint32_t i = 0; i < n; i++) {
+        const float* q = recs + (size_t)i * 8;
+        HitRecord h;
+        h.u = q[0]; h.v = q[1];
+        h.normal = Vec3(q[2], q[3], q[4]); h.tangent = Vec3(q[5], q[6], q[7]);
+        PbrParams pp = p->params(h);
+        float* o = out + (size_t)i * 8;
+        o[0] = pp.normal.x; o[1] = pp.normal.y; o[2] = pp.normal.z;
+        o[3] = pp.albedo.x; o[4] = pp.albedo.y; o[5] = pp.albedo.z; o[6] = pp.roughness; o[7] = pp.metallic;
     }
     return 0;
 }

@@ -51,7 +51,9 @@ struct DMedium {
 // toff: RT_TEX_IMAGE byte offset of the image in DScene::tex_pix (tw x th
 // texels), RT_TEX_NOISE / RT_TEX_MOTTLE float offset of the Perlin tables in
 // DScene::grids.
-struct DMaterial { uint32_t kind, tex; float p[24]; uint32_t toff, tw, th, pad; };
+// map_*: the PBR texture maps (albedo, roughness, metallic, ambient
+// occlusion, normal) as byte offset / width / height in tex_pix, width 0 = none.
+struct DMaterial { uint32_t kind, tex; float p[24]; uint32_t toff, tw, th, pad; uint32_t map_off[5], map_w[5], map_h[5], pad2; };
 
 struct DScene {
     const float4* nodes;

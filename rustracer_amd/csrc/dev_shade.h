@@ -187,9 +187,103 @@ RT_D V3 tex_value(const DScene& S, const DMaterial& m, float u, float v, V3 p) {
     }
 }
 
+// nacre_color (src/material.rs:1119-1136): the thin-film table (built on the
+// host like NACRE_LUT, src/material.rs:1091-1117) at the optical path
+// difference, blended toward grey at grazing angles.
+RT_D V3 nacre_color(const DScene& S, float cos_theta, float film_ior, float th) {
+    float sin_sq = fmx(1.0f - cos_theta * cos_theta, 0.0f);
+    float cos_t = sqrtf(fmx(1.0f - sin_sq / (film_ior * film_ior), 0.0f));
+    float opd = 2.0f * film_ior * th * cos_t;
+    float t = fclamp(opd / 2500.0f, 0.0f, 1.0f) * 255.0f;
+    uint32_t lo = f2u(t), hi = min(lo + 1u, 255u);
+    V3 a = mk(S.nacre[3 * lo], S.nacre[3 * lo + 1], S.nacre[3 * lo + 2]);
+    V3 b = mk(S.nacre[3 * hi], S.nacre[3 * hi + 1], S.nacre[3 * hi + 2]);
+    V3 raw = a * (1.0f - (t - (float)lo)) + b * (t - (float)lo);
+    float blend = powf(cos_theta, 0.5f);
+    return raw * blend + mk(0.5f, 0.5f, 0.5f) * (1.0f - blend);
+}
+// oil_noise (src/material.rs:1138-1147)
+RT_D float oil_noise(V3 p) {
+    float a = sinf(p.x * 1.618f + p.z);
+    float b = sinf(p.z * 1.618f - p.x);
+    float c = sinf(p.x * 0.618f + p.y + p.z);
+    return (a + b + c) * (1.0f / 3.0f);
+}
+// Texture-map fetches of PbrMaterial (src/material.rs:632-671): the 2x2
+// bilinear neighbourhood, texel centres at half-integers, edges clamped.
+struct Bilin { uint32_t i00, i10, i01, i11; float tx, ty; };
+RT_D Bilin bilinear_at(uint32_t off, uint32_t w, uint32_t h, float u, float v) {
+    u = fclamp(u, 0.0f, 1.0f);
+    v = 1.0f - fclamp(v, 0.0f, 1.0f);
+    float fx = fclamp(u * (float)w - 0.5f, 0.0f, (float)(w - 1));
+    float fy = fclamp(v * (float)h - 0.5f, 0.0f, (float)(h - 1));
+    uint32_t x0 = f2u(fx), y0 = f2u(fy);
+    uint32_t x1 = min(x0 + 1u, w - 1u), y1 = min(y0 + 1u, h - 1u);
+    Bilin b;
+    b.i00 = off + 3u * (y0 * w + x0); b.i10 = off + 3u * (y0 * w + x1);
+    b.i01 = off + 3u * (y1 * w + x0); b.i11 = off + 3u * (y1 * w + x1);
+    b.tx = fx - (float)x0; b.ty = fy - (float)y0;
+    return b;
+}
+RT_D V3 texel_srgb(const DScene& S, uint32_t i) { return mk(S.srgb_lut[S.tex_pix[i]], S.srgb_lut[S.tex_pix[i + 1]], S.srgb_lut[S.tex_pix[i + 2]]); }
+RT_D V3 texel_rgb(const DScene& S, uint32_t i) {
+    return mk((float)S.tex_pix[i] / 255.0f, (float)S.tex_pix[i + 1] / 255.0f, (float)S.tex_pix[i + 2] / 255.0f);
+}
+RT_D V3 sample_srgb(const DScene& S, const DMaterial& m, int k, float u, float v) {
+    Bilin b = bilinear_at(m.map_off[k], m.map_w[k], m.map_h[k], u, v);
+    V3 c0 = texel_srgb(S, b.i00) * (1.0f - b.tx) + texel_srgb(S, b.i10) * b.tx;
+    V3 c1 = texel_srgb(S, b.i01) * (1.0f - b.tx) + texel_srgb(S, b.i11) * b.tx;
+    return c0 * (1.0f - b.ty) + c1 * b.ty;
+}
+RT_D V3 sample_rgb(const DScene& S, const DMaterial& m, int k, float u, float v) {
+    Bilin b = bilinear_at(m.map_off[k], m.map_w[k], m.map_h[k], u, v);
+    V3 c0 = texel_rgb(S, b.i00) * (1.0f - b.tx) + texel_rgb(S, b.i10) * b.tx;
+    V3 c1 = texel_rgb(S, b.i01) * (1.0f - b.tx) + texel_rgb(S, b.i11) * b.tx;
+    return c0 * (1.0f - b.ty) + c1 * b.ty;
+}
+RT_D float sample_linear(const DScene& S, const DMaterial& m, int k, float u, float v) {
+    Bilin b = bilinear_at(m.map_off[k], m.map_w[k], m.map_h[k], u, v);
+    float v0 = (float)S.tex_pix[b.i00] / 255.0f * (1.0f - b.tx) + (float)S.tex_pix[b.i10] / 255.0f * b.tx;
+    float v1 = (float)S.tex_pix[b.i01] / 255.0f * (1.0f - b.tx) + (float)S.tex_pix[b.i11] / 255.0f * b.tx;
+    return v0 * (1.0f - b.ty) + v1 * b.ty;
+}
+
 // PbrMaterial parameter block: p0-2 albedo, p3 roughness, p4 metallic,
 // p5 anisotropy, p6 angle, p7 clearcoat, p8 coat roughness, p9-11 emission,
-// p12 strength, p13 sheen, p14 sheen_tint.
+// p12 strength, p13 sheen, p14 sheen_tint, p15 film thickness, p16 film ior;
+// maps (DMaterial::map_*, width 0 = none): 0 albedo, 1 roughness,
+// 2 metallic, 3 ambient occlusion, 4 tangent-space normal.
+enum { MAP_ALBEDO = 0, MAP_ROUGH = 1, MAP_METAL = 2, MAP_AO = 3, MAP_NORMAL = 4 };
+struct PbrP { V3 n, albedo; float rough, metal; };
+// PbrMaterial::tbn_normal (src/material.rs:755-769): normal maps need the
+// hit's tangent (meshes)
+RT_D V3 pbr_normal(const DScene& S, const DMaterial& m, const Rec& h) {
+    if (m.map_w[MAP_NORMAL] && len2(h.tangent) > 1e-6f) {
+        V3 raw = sample_rgb(S, m, MAP_NORMAL, h.u, h.v);
+        V3 ts = mk(raw.x * 2.0f - 1.0f, raw.y * 2.0f - 1.0f, raw.z * 2.0f - 1.0f);
+        V3 ng = h.n, t = h.tangent;
+        V3 b = cross(ng, t);
+        V3 pert = t * ts.x + b * ts.y + ng * ts.z;
+        return len2(pert) > 1e-12f ? unit(pert) : h.n;
+    }
+    return h.n;
+}
+RT_D V3 pbr_albedo(const DScene& S, const DMaterial& m, float u, float v) {
+    V3 c = m.map_w[MAP_ALBEDO] ? sample_srgb(S, m, MAP_ALBEDO, u, v) : mk(m.p[0], m.p[1], m.p[2]);
+    return m.map_w[MAP_AO] ? c * sample_linear(S, m, MAP_AO, u, v) : c;
+}
+// PbrMaterial::params (src/material.rs:771-782)
+RT_D PbrP pbr_params(const DScene& S, const DMaterial& m, const Rec& h) {
+    PbrP q;
+    q.n = pbr_normal(S, m, h);
+    q.albedo = pbr_albedo(S, m, h.u, h.v);
+    q.rough = m.map_w[MAP_ROUGH] ? sample_linear(S, m, MAP_ROUGH, h.u, h.v) : m.p[3];
+    q.metal = m.map_w[MAP_METAL] ? sample_linear(S, m, MAP_METAL, h.u, h.v) : m.p[4];
+    return q;
+}
+RT_D V3 pbr_film(const DScene& S, const DMaterial& m, float cos_h) {
+    return m.p[15] > 0.0f ? nacre_color(S, cos_h, m.p[16], m.p[15]) : mk(1, 1, 1);
+}
 RT_D void pbr_frame(const DMaterial& m, V3 n, V3& t, V3& b) {
     V3 t0, b0; vonb(n, t0, b0);
     float ca = cosf(m.p[6]), sa = sinf(m.p[6]);
@@ -198,13 +292,13 @@ RT_D void pbr_frame(const DMaterial& m, V3 n, V3& t, V3& b) {
 }
 RT_D float pbr_aspect(const DMaterial& m) { return sqrtf(fmx(1.0f - 0.9f * fclamp(m.p[5], 0.0f, 1.0f), 0.001f)); }
 
-RT_D bool pbr_scatter(const DScene& S, const DMaterial& m, const Ray& r, const Rec& h, Rng& rng, Scatter& sr) {
-    V3 n = h.n;
+RT_D bool pbr_scatter(const DScene& S, const DMaterial& m, const PbrP& q, const Ray& r, const Rec& h, Rng& rng, Scatter& sr) {
+    V3 n = q.n;
     V3 wo = unit(-r.d);
     float cos_o = dot(wo, n);
     if (cos_o <= 0.0f) return false;
-    V3 albedo = mk(m.p[0], m.p[1], m.p[2]);
-    float rough = m.p[3], metallic = m.p[4], clearcoat = m.p[7];
+    V3 albedo = q.albedo;
+    float rough = q.rough, metallic = q.metal, clearcoat = m.p[7];
     float alpha = fmx(rough * rough, 1e-4f);
     V3 f0 = mk(0.04f, 0.04f, 0.04f) * (1.0f - metallic) + albedo * metallic;
     float p_coat = clearcoat > 1e-3f ? fmx(schlick(cos_o, 1.0f / 1.5f) * clearcoat, 0.12f * clearcoat) : 0.0f;
@@ -224,7 +318,7 @@ RT_D bool pbr_scatter(const DScene& S, const DMaterial& m, const Ray& r, const R
         float fc = schlick(voh, 1.0f / 1.5f) * clearcoat;
         float lo = smith_lambda(cos_o, ac), li = smith_lambda(cos_i, ac);
         float w = ((1.0f + lo) / (1.0f + lo + li)) * energy_comp(S, cos_o, ac);
-        sr.att = mk(1, 1, 1) * (fc * w / p_coat);
+        sr.att = pbr_film(S, m, voh) * (fc * w / p_coat);
         sr.ray = scatter_from(h.p, wi, r); sr.skip_pdf = true;
         return true;
     }
@@ -263,14 +357,14 @@ RT_D bool pbr_scatter(const DScene& S, const DMaterial& m, const Ray& r, const R
     return true;
 }
 
-RT_D V3 pbr_brdf_cos(const DScene& S, const DMaterial& m, const Ray& r, const Rec& h, V3 wi) {
-    V3 n = h.n, wo = unit(-r.d);
+RT_D V3 pbr_brdf_cos(const DScene& S, const DMaterial& m, const PbrP& q, const Ray& r, const Rec& h, V3 wi) {
+    V3 n = q.n, wo = unit(-r.d);
     float cos_o = dot(wo, n), cos_i = dot(wi, n);
     if (cos_o <= 0.0f || cos_i <= 0.0f) return mk(0, 0, 0);
     V3 hv = unit(wo + wi);
     float woh = fmx(dot(wo, hv), 0.0f), hn = fmx(dot(hv, n), 0.0f);
-    V3 albedo = mk(m.p[0], m.p[1], m.p[2]);
-    float metallic = m.p[4], clearcoat = m.p[7];
+    V3 albedo = q.albedo;
+    float metallic = q.metal, clearcoat = m.p[7];
     V3 f0 = mk(0.04f, 0.04f, 0.04f) * (1.0f - metallic) + albedo * metallic;
     V3 res = mk(0, 0, 0);
     if (clearcoat > 1e-3f) {
@@ -278,9 +372,9 @@ RT_D V3 pbr_brdf_cos(const DScene& S, const DMaterial& m, const Ray& r, const Re
         float d = ggx_ndf(hn, ac);
         float g2 = 1.0f / (1.0f + smith_lambda(cos_o, ac) + smith_lambda(cos_i, ac));
         float fc = schlick(woh, 1.0f / 1.5f) * clearcoat;
-        res += mk(1, 1, 1) * (d * fc * g2 / (4.0f * cos_o) * energy_comp(S, cos_o, ac));
+        res += pbr_film(S, m, woh) * (d * fc * g2 / (4.0f * cos_o) * energy_comp(S, cos_o, ac));
     }
-    float alpha = fmx(m.p[3] * m.p[3], 1e-4f);
+    float alpha = fmx(q.rough * q.rough, 1e-4f);
     float asp = pbr_aspect(m); float ax = alpha / asp, ay = alpha * asp;
     V3 t, b; pbr_frame(m, n, t, b);
     V3 hts = mk(dot(hv, t), dot(hv, b), hn), wots = mk(dot(wo, t), dot(wo, b), cos_o), wits = mk(dot(wi, t), dot(wi, b), cos_i);
@@ -291,14 +385,14 @@ RT_D V3 pbr_brdf_cos(const DScene& S, const DMaterial& m, const Ray& r, const Re
     return res;
 }
 
-RT_D float pbr_spec_pdf(const DMaterial& m, const Ray& r, const Rec& h, V3 wi) {
-    V3 n = h.n, wo = unit(-r.d);
+RT_D float pbr_spec_pdf(const DMaterial& m, const PbrP& q, const Ray& r, const Rec& h, V3 wi) {
+    V3 n = q.n, wo = unit(-r.d);
     float cos_o = dot(wo, n), cos_i = dot(wi, n);
     if (cos_o <= 0.0f || cos_i <= 0.0f) return 0.0f;
     V3 hv = unit(wo + wi);
     float hn = fmx(dot(hv, n), 0.0f);
-    V3 albedo = mk(m.p[0], m.p[1], m.p[2]);
-    float metallic = m.p[4], clearcoat = m.p[7];
+    V3 albedo = q.albedo;
+    float metallic = q.metal, clearcoat = m.p[7];
     V3 f0 = mk(0.04f, 0.04f, 0.04f) * (1.0f - metallic) + albedo * metallic;
     float p_coat = clearcoat > 1e-3f ? fmx(schlick(cos_o, 1.0f / 1.5f) * clearcoat, 0.12f * clearcoat) : 0.0f;
     V3 fa = schlick_color(cos_o, f0);
@@ -309,7 +403,7 @@ RT_D float pbr_spec_pdf(const DMaterial& m, const Ray& r, const Rec& h, V3 wi) {
         pdf += p_coat * ggx_ndf(hn, ac) * smith_g1(cos_o, ac) / (4.0f * cos_o);
     }
     if ((1.0f - p_coat) * p_spec > 1e-6f) {
-        float alpha = fmx(m.p[3] * m.p[3], 1e-4f);
+        float alpha = fmx(q.rough * q.rough, 1e-4f);
         float asp = pbr_aspect(m); float ax = alpha / asp, ay = alpha * asp;
         V3 t, b; pbr_frame(m, n, t, b);
         V3 hts = mk(dot(hv, t), dot(hv, b), hn), wots = mk(dot(wo, t), dot(wo, b), cos_o);
@@ -493,28 +587,6 @@ RT_D bool sss_scatter(const DMaterial& m, const Ray& r, const Rec& h, Rng& rng, 
                           (1u << M_ROUGH_SPECTRAL) | (1u << M_SSS))
 #define RT_IN(MASK, K) (((MASK) >> (K)) & 1u)
 
-// nacre_color (src/material.rs:1119-1136): the thin-film table (built on the
-// host like NACRE_LUT, src/material.rs:1091-1117) at the optical path
-// difference, blended toward grey at grazing angles.
-RT_D V3 nacre_color(const DScene& S, float cos_theta, float film_ior, float th) {
-    float sin_sq = fmx(1.0f - cos_theta * cos_theta, 0.0f);
-    float cos_t = sqrtf(fmx(1.0f - sin_sq / (film_ior * film_ior), 0.0f));
-    float opd = 2.0f * film_ior * th * cos_t;
-    float t = fclamp(opd / 2500.0f, 0.0f, 1.0f) * 255.0f;
-    uint32_t lo = f2u(t), hi = min(lo + 1u, 255u);
-    V3 a = mk(S.nacre[3 * lo], S.nacre[3 * lo + 1], S.nacre[3 * lo + 2]);
-    V3 b = mk(S.nacre[3 * hi], S.nacre[3 * hi + 1], S.nacre[3 * hi + 2]);
-    V3 raw = a * (1.0f - (t - (float)lo)) + b * (t - (float)lo);
-    float blend = powf(cos_theta, 0.5f);
-    return raw * blend + mk(0.5f, 0.5f, 0.5f) * (1.0f - blend);
-}
-// oil_noise (src/material.rs:1138-1147)
-RT_D float oil_noise(V3 p) {
-    float a = sinf(p.x * 1.618f + p.z);
-    float b = sinf(p.z * 1.618f - p.x);
-    float c = sinf(p.x * 0.618f + p.y + p.z);
-    return (a + b + c) * (1.0f / 3.0f);
-}
 // PearlMaterial (src/material.rs:1166-1285): p0-2 base colour, p3 ior,
 // p4 film thickness, p5 orient strength, p6 film scale, p7 luster roughness.
 RT_D float pearl_thickness(const DMaterial& m, V3 p) { return m.p[4] + oil_noise(p * m.p[6]) * 150.0f; }
@@ -612,7 +684,7 @@ RT_D bool mat_scatter(const DScene& S, const Rec& h, const Ray& r, Rng& rng, Sca
             sr.att = att; sr.skip_pdf = true;
             return true;
         }
-        case M_PBR: return RT_IN(MASK, M_PBR) && pbr_scatter(S, m, r, h, rng, sr);
+        case M_PBR: return RT_IN(MASK, M_PBR) && pbr_scatter(S, m, pbr_params(S, m, h), r, h, rng, sr);
         case M_SPECTRAL_METAL: return RT_IN(MASK, M_SPECTRAL_METAL) && smetal_scatter(S, m, r, h, rng, sr);
         case M_ROUGH_DIELECTRIC: return RT_IN(MASK, M_ROUGH_DIELECTRIC) && rough_dielectric_scatter(m, r, h, rng, sr);
         case M_ROUGH_SPECTRAL: return RT_IN(MASK, M_ROUGH_SPECTRAL) && rsd_scatter(S, m, r, h, rng, sr);
@@ -643,11 +715,20 @@ RT_D V3 mat_emitted(const DScene& S, const Rec& h, const Ray& r) {
     return mk(0, 0, 0);
 }
 RT_D float mat_scattering_pdf(const Rec& h, V3 dir) { return fmx(dot(h.n, unit(dir)) / PI, 0.0f); }
+// Material::shading_normal (src/hittable.rs:58-60): the PBR normal map's
+// perturbed normal, the hit normal otherwise; the diffuse branch samples and
+// evaluates its cosine lobe about it.
+RT_D V3 mat_shading_normal(const DScene& S, const Rec& h) {
+    if (h.medium) return h.n;
+    const DMaterial& m = S.mats[h.mat];
+    return m.kind == M_PBR ? pbr_normal(S, m, h) : h.n;
+}
+RT_D float scattering_pdf_n(V3 n, V3 dir) { return fmx(dot(n, unit(dir)) / PI, 0.0f); }
 RT_D V3 mat_albedo_hint(const DScene& S, const Rec& h) {
     if (h.medium) return mk(1, 1, 1);
     const DMaterial& m = S.mats[h.mat];
     if (m.kind == M_LAMBERTIAN) return tex_value(S, m, h.u, h.v, h.p);
-    if (m.kind == M_PBR) return mk(m.p[0], m.p[1], m.p[2]);
+    if (m.kind == M_PBR) return pbr_albedo(S, m, h.u, h.v);
     if (m.kind == M_DIFFUSE_LIGHT) return tex_value(S, m, h.u, h.v, h.p);
     if (m.kind == M_BLACKBODY) return mk(m.p[3], m.p[4], m.p[5]);
     if (m.kind == M_SSS || m.kind == M_PEARL) return mk(m.p[0], m.p[1], m.p[2]);
@@ -662,7 +743,7 @@ template <uint32_t MASK = RT_MASK_ALL>
 RT_D V3 mat_spec_brdf_cos(const DScene& S, const Rec& h, const Ray& r, V3 wi) {
     if (h.medium) return mk(0, 0, 0);
     const DMaterial& m = S.mats[h.mat];
-    if (RT_IN(MASK, M_PBR) && m.kind == M_PBR) return pbr_brdf_cos(S, m, r, h, wi);
+    if (RT_IN(MASK, M_PBR) && m.kind == M_PBR) return pbr_brdf_cos(S, m, pbr_params(S, m, h), r, h, wi);
     if (RT_IN(MASK, M_SPECTRAL_METAL) && m.kind == M_SPECTRAL_METAL) return smetal_brdf_cos(S, m, r, h, wi);
     if (RT_IN(MASK, M_PEARL) && m.kind == M_PEARL) return pearl_brdf_cos(S, m, r, h, wi);
     return mk(0, 0, 0);
@@ -671,7 +752,7 @@ template <uint32_t MASK = RT_MASK_ALL>
 RT_D float mat_spec_pdf(const DScene& S, const Rec& h, const Ray& r, V3 wi) {
     if (h.medium) return 0.0f;
     const DMaterial& m = S.mats[h.mat];
-    if (RT_IN(MASK, M_PBR) && m.kind == M_PBR) return pbr_spec_pdf(m, r, h, wi);
+    if (RT_IN(MASK, M_PBR) && m.kind == M_PBR) return pbr_spec_pdf(m, pbr_params(S, m, h), r, h, wi);
     if (RT_IN(MASK, M_SPECTRAL_METAL) && m.kind == M_SPECTRAL_METAL) return smetal_spec_pdf(m, r, h, wi);
     if (RT_IN(MASK, M_PEARL) && m.kind == M_PEARL) return pearl_spec_pdf(m, r, h, wi);
     return 0.0f;
@@ -1126,6 +1207,7 @@ RT_D bool bounce_t(const DScene& S, PathState& ps, const HitId& id, int depth, f
         ps.prev_specular = true;
         ps.prev_mis = 1.0f;
     } else {
+        const V3 sn = mat_shading_normal(S, h);
         if (use_photons) {
             V3 irr = photon_irradiance(S, h.p, h.n, ps.knn_irr, st);
             if (irr.x > 0.0f || irr.y > 0.0f || irr.z > 0.0f) color += dh.emit_vertex(ps.thr * mat_albedo_hint(S, h) * irr, S, h, ray);
@@ -1136,7 +1218,7 @@ RT_D bool bounce_t(const DScene& S, PathState& ps, const HitId& id, int depth, f
             Rec lrec;
             if (lights_closest(S, shadow, 0.001f, FINF, lrec)) {
                 float l_pdf = lights_pdf_value(S, h.p, ldir, ray.time, st);
-                float brdf = mat_scattering_pdf(h, shadow.d);
+                float brdf = scattering_pdf_n(sn, shadow.d);
                 float sp = sun_pdf_value(S, ldir);
                 float ep = env_pdf_value(S, ldir);
                 float md = l_pdf * l_pdf + brdf * brdf + sp * sp + ep * ep;
@@ -1151,7 +1233,7 @@ RT_D bool bounce_t(const DScene& S, PathState& ps, const HitId& id, int depth, f
         if (sun) {
             V3 su = unit(mk(S.sun_dir[0], S.sun_dir[1], S.sun_dir[2]));
             V3 ss = sample_sun_cone(su, COS_SUN_MAX, rng);
-            float brdf = mat_scattering_pdf(h, ss);
+            float brdf = scattering_pdf_n(sn, ss);
             if (brdf > 0.0f) {
                 float spdf = 1.0f / (2.0f * PI * (1.0f - COS_SUN_MAX));
                 V3 sc = bg_eval(S, ss);
@@ -1167,7 +1249,7 @@ RT_D bool bounce_t(const DScene& S, PathState& ps, const HitId& id, int depth, f
             float epdf;
             V3 ed = env_sample_dir(S, rng, epdf);
             if (epdf > 0.0f) {
-                float brdf = mat_scattering_pdf(h, ed);
+                float brdf = scattering_pdf_n(sn, ed);
                 if (brdf > 0.0f) {
                     float ap = lights_empty ? 0.0f : lights_pdf_value(S, h.p, ed, ray.time, st);
                     V3 ec = env_eval(S, ed);
@@ -1179,7 +1261,7 @@ RT_D bool bounce_t(const DScene& S, PathState& ps, const HitId& id, int depth, f
                 }
             }
         }
-        Onb uvw = onb_from_w(h.n);
+        Onb uvw = onb_from_w(sn);
         float r1 = rng_f(rng), r2 = rng_f(rng);
         float phi = 2.0f * PI * r1;
         float sr2 = sqrtf(r2);
@@ -1187,7 +1269,7 @@ RT_D bool bounce_t(const DScene& S, PathState& ps, const HitId& id, int depth, f
         float pdf_val = fmx(dot(unit(ind), uvw.w) / PI, 0.0f);
         if (pdf_val <= 0.0f) return false;
         Ray sc = scatter_from(h.p, ind, ray);
-        float scat_pdf = mat_scattering_pdf(h, sc.d);
+        float scat_pdf = scattering_pdf_n(sn, sc.d);
         if (scat_pdf <= 0.0f) return false;
         if (dot(ind, h.n) <= 0.0f) return false;
         float a = lights_empty ? 0.0f : lights_pdf_value(S, h.p, ind, ray.time, st);

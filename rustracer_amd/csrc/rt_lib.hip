@@ -1351,6 +1351,30 @@ extern "C" int rt_scene_upload(rt_ctx* ctx, const rt_scene_desc* d, rt_scene** o
         memset(&mats[i], 0, sizeof(DMaterial));
         mats[i].kind = dm.kind; mats[i].tex = dm.tex;
         for (int k = 0; k < 24; k++) mats[i].p[k] = dm.p[k];
+        // an image of the descriptor in the texel pool (each image once)
+        auto place_image = [&](uint32_t k, uint32_t& off, uint32_t& w, uint32_t& h) -> int {
+            const rt_image& im = d->images[k];
+            if (!im.rgb || !im.width || !im.height) return fail(ctx, RT_ERR_INVALID, "material " + std::to_string(i) + ": empty image");
+            size_t bytes = (size_t)im.width * im.height * 3;
+            if (image_off[k] < 0) {
+                if (texels.size() + bytes > 0xffffffffull) return fail(ctx, RT_ERR_UNSUPPORTED, "texture images exceed 4 GiB");
+                image_off[k] = (int64_t)texels.size();
+                texels.insert(texels.end(), im.rgb, im.rgb + bytes);
+            }
+            off = (uint32_t)image_off[k]; w = im.width; h = im.height;
+            return RT_OK;
+        };
+        if (dm.kind == RT_MAT_PBR) {
+            for (int k = 0; k < 5; k++) {
+                float f = dm.p[17 + k];
+                if (f == 0.0f) continue;
+                if (!d->images || !(f >= 1.0f && f <= (float)d->num_images && f == std::floor(f)))
+                    return fail(ctx, RT_ERR_INVALID, "material " + std::to_string(i) + ": PBR map image index");
+                int rc2 = place_image((uint32_t)f - 1, mats[i].map_off[k], mats[i].map_w[k], mats[i].map_h[k]);
+                if (rc2) return rc2;
+            }
+            if (dm.p[15] > 0.0f && !(dm.p[16] > 0.0f)) return fail(ctx, RT_ERR_INVALID, "material " + std::to_string(i) + ": film_ior must be positive");
+        }
         if (dm.tex != RT_TEX_SOLID) {
             std::string tag = "material " + std::to_string(i) + ": ";
             if (dm.kind != RT_MAT_LAMBERTIAN && dm.kind != RT_MAT_DIFFUSE_LIGHT)
@@ -1359,16 +1383,8 @@ extern "C" int rt_scene_upload(rt_ctx* ctx, const rt_scene_desc* d, rt_scene** o
             if (dm.tex == RT_TEX_IMAGE) {
                 float f = dm.p[7];
                 if (!d->images || !(f >= 0.0f && f < (float)d->num_images && f == std::floor(f))) return fail(ctx, RT_ERR_INVALID, tag + "image index");
-                uint32_t k = (uint32_t)f;
-                const rt_image& im = d->images[k];
-                if (!im.rgb || !im.width || !im.height) return fail(ctx, RT_ERR_INVALID, tag + "empty image");
-                size_t bytes = (size_t)im.width * im.height * 3;
-                if (image_off[k] < 0) {
-                    if (texels.size() + bytes > 0xffffffffull) return fail(ctx, RT_ERR_UNSUPPORTED, "texture images exceed 4 GiB");
-                    image_off[k] = (int64_t)texels.size();
-                    texels.insert(texels.end(), im.rgb, im.rgb + bytes);
-                }
-                mats[i].toff = (uint32_t)image_off[k]; mats[i].tw = im.width; mats[i].th = im.height;
+                int rc2 = place_image((uint32_t)f, mats[i].toff, mats[i].tw, mats[i].th);
+                if (rc2) return rc2;
             } else if (dm.tex == RT_TEX_NOISE || dm.tex == RT_TEX_MOTTLE) {
                 float f = dm.p[8];
                 if (!d->perlin || !(f >= 0.0f && f < (float)d->num_perlin && f == std::floor(f))) return fail(ctx, RT_ERR_INVALID, tag + "Perlin table index");

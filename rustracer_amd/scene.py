@@ -199,24 +199,33 @@ class SpectralDielectric(Material):
 
 
 class PbrMaterial(Material):
-    """src/material.rs:695-1050 (scalar parameters; texture maps and thin film
-    are not carried over the ABI)."""
+    """PbrMaterial (src/material.rs:695-1050) with its thin-film clearcoat tint
+    and optional image maps (H x W x 3 uint8, row 0 at the top): albedo (sRGB),
+    roughness / metallic / ambient occlusion (linear, red channel) and a
+    tangent-space normal map (used on meshes, which carry tangents)."""
     kind = F.MAT_PBR
+    MAP_NAMES = ("albedo_map", "roughness_map", "metallic_map", "ao_map", "normal_map")
 
     def __init__(self, albedo=(0.8, 0.8, 0.8), roughness=0.5, metallic=0.0, anisotropy=0.0,
                  anisotropy_angle=0.0, clearcoat=0.0, clearcoat_roughness=0.03, emission=(0, 0, 0),
-                 emission_strength=0.0, sheen=0.0, sheen_tint=0.5):
+                 emission_strength=0.0, sheen=0.0, sheen_tint=0.5, film_thickness=0.0, film_ior=1.5,
+                 albedo_map=None, roughness_map=None, metallic_map=None, ao_map=None, normal_map=None):
         self.albedo = v3(albedo)
         self.roughness, self.metallic = float(roughness), float(metallic)
         self.anisotropy, self.anisotropy_angle = float(anisotropy), float(anisotropy_angle)
         self.clearcoat, self.clearcoat_roughness = float(clearcoat), float(clearcoat_roughness)
         self.emission, self.emission_strength = v3(emission), float(emission_strength)
         self.sheen, self.sheen_tint = float(sheen), float(sheen_tint)
+        self.film_thickness, self.film_ior = float(film_thickness), float(film_ior)
+        self.maps = [None if a is None else ImageTexture(a).image
+                     for a in (albedo_map, roughness_map, metallic_map, ao_map, normal_map)]
+        self.map_index = [0] * 5      # descriptor image index + 1, set by Scene.descriptor
 
     def params(self):
         return (list(self.albedo) + [self.roughness, self.metallic, self.anisotropy, self.anisotropy_angle,
                                      self.clearcoat, self.clearcoat_roughness] + list(self.emission)
-                + [self.emission_strength, self.sheen, self.sheen_tint])
+                + [self.emission_strength, self.sheen, self.sheen_tint, self.film_thickness, self.film_ior]
+                + [float(k) for k in self.map_index])
 
 
 def Metal(color, fuzz):
@@ -815,6 +824,12 @@ class Scene:
         images, image_index, perlins, perlin_index = [], {}, [], {}
         cm = []
         for m in mats:
+            for k, img in enumerate(getattr(m, "maps", ())):
+                if img is not None:
+                    if id(img) not in image_index:
+                        image_index[id(img)] = len(images)
+                        images.append(img)
+                    m.map_index[k] = image_index[id(img)] + 1
             t = m.texture
             ii = pi = 0
             if t is not None and t.image is not None:

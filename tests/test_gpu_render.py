@@ -489,3 +489,44 @@ def test_pearl_same_streams_and_three_sigma(gpu_renderer, sky):
     assert np.isfinite(g).all()
     z = three_sigma(g, c)
     assert (z > 3).mean() <= 0.02 and (z > 6).mean() <= 0.002, ((z > 3).mean(), (z > 6).mean())
+
+
+def test_pbr_maps_and_film_same_streams_and_three_sigma(gpu_renderer):
+    """PbrMaterial maps (albedo, roughness, metallic, ambient occlusion,
+    normal on a tangent-carrying mesh) and a thin-film clearcoat."""
+    from rustracer_amd import scene as S
+    g = np.random.default_rng(11)
+    alb = g.integers(0, 256, size=(8, 8, 3), dtype=np.uint8)
+    rough = g.integers(20, 240, size=(8, 8, 3), dtype=np.uint8)
+    metal = np.zeros((4, 4, 3), np.uint8)
+    metal[::2, ::2] = 255
+    ao = g.integers(120, 256, size=(4, 4, 3), dtype=np.uint8)
+    nmap = np.zeros((6, 6, 3), np.uint8)
+    nmap[..., 0] = g.integers(90, 170, size=(6, 6))
+    nmap[..., 1] = g.integers(90, 170, size=(6, 6))
+    nmap[..., 2] = 230
+    wall = S.PbrMaterial(albedo_map=alb, roughness_map=rough, metallic_map=metal, ao_map=ao, normal_map=nmap)
+    # a wall of two triangles with uvs and tangents along +x
+    v = np.array([[[-1.2, 0, 0.9], [1.2, 0, 0.9], [1.2, 1.6, 0.9]], [[-1.2, 0, 0.9], [1.2, 1.6, 0.9], [-1.2, 1.6, 0.9]]], np.float32)
+    uv = np.array([[[0, 0], [1, 0], [1, 1]], [[0, 0], [1, 1], [0, 1]]], np.float32)
+    tang = np.tile(np.array([1, 0, 0], np.float32), (2, 3, 1))
+    nrm = np.tile(np.array([0, 0, -1], np.float32), (2, 3, 1))
+    objs = [S.Quad((-4, 0, -4), (8, 0, 0), (0, 0, 8), S.PbrMaterial(albedo_map=alb, roughness=0.6)),
+            S.Mesh(v, wall, normals=nrm, tangents=tang, uvs=uv),
+            S.Sphere((0.0, 0.35, 0.0), 0.35, S.PbrMaterial(albedo=(0.1, 0.1, 0.12), roughness=0.4, clearcoat=1.0,
+                                                            clearcoat_roughness=0.05, film_thickness=520.0, film_ior=1.45))]
+    lq = S.Quad((-0.3, 1.8, -0.6), (0.6, 0, 0), (0, 0, 0.6), S.DiffuseLight((10.0, 10.0, 10.0)))
+    sc = S.Scene(objs + [lq], [lq], S.Background.gradient((0.3, 0.4, 0.6), (0.9, 0.9, 0.9)))
+    cam = S.Camera((0.0, 0.9, -2.4), (0.0, 0.5, 0.5), (0, 1, 0), 45.0, 1.0)
+    gpu_renderer.upload(sc)
+    orc = O.OracleScene(sc)
+    gp = passes_gpu(gpu_renderer, cam, 40, 40, 4, 12)
+    cp = passes_cpu(orc, cam, 40, 40, 4, 12, mode=1)
+    close = np.isclose(gp, cp, rtol=1e-3, atol=1e-4).all(axis=-1)
+    assert close.mean() > 0.95, close.mean()
+    K = 96
+    gp = passes_gpu(gpu_renderer, cam, 32, 32, K, 12, seed=2)
+    cp = passes_cpu(orc, cam, 32, 32, K, 12, mode=0)
+    assert np.isfinite(gp).all()
+    z = three_sigma(gp, cp)
+    assert (z > 3).mean() <= 0.02 and (z > 6).mean() <= 0.002, ((z > 3).mean(), (z > 6).mean())
