@@ -413,12 +413,15 @@ int  rt_render_batch(rt_ctx *ctx, const rt_batch_view *views, uint32_t nviews,
  * material `material` to theta[i] (count 3 on a LAMBERTIAN albedo makes one
  * grey albedo parameter).  Differentiable fields: LAMBERTIAN (solid, tex 0)
  * p0-2; ROUGH_SPECTRAL p0 (n_d), p1-3 (Sellmeier B), p7-23 (GGX alpha).
- * Others return RT_ERR_UNSUPPORTED.                                           */
+ * material = RT_PARAM_MEDIUM | k addresses rt_scene_desc.media[k] instead:
+ * field 0 (count 1) is a CONSTANT medium's density sigma_t (its boundary
+ * objects must carry no transform).  Others return RT_ERR_UNSUPPORTED.      */
 typedef struct {
     uint32_t material;
     uint32_t field;
     uint32_t count;
 } rt_param;
+#define RT_PARAM_MEDIUM 0x40000000u
 
 /* Write theta into the scene's materials (device and host copies); the
  * geometry, BVH and photon map are untouched.                                */

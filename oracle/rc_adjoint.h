@@ -91,15 +91,31 @@ static inline bool rsd_vertex_logf(const float* p, const HitRecord& rec, const V
     return logf_rsd(d3(wo), d3(wi), d3(rec.normal), k, rsd_alpha_d(p, rec.prim_id), refl, lf);
 }
 
-// Parameter map: fmap[24 * material + field] = parameter index or -1.
+// Constant media of one descriptor entry, for the sigma_t parameters: the
+// medium objects built from it, and whether any of them sits under a transform.
+struct MediumSet { std::vector<const ConstantMedium*> objs; bool transformed = false; bool constant = true; };
+
+// Parameter map: fmap[24 * material + field] = parameter index or -1;
+// mpar[k] = the parameter of media[k]'s density or -1.
 struct ParamMap {
     std::vector<int8_t> fmap;
+    std::vector<int> mpar;
     int n = 0;
-    bool build(const std::vector<rt_material>& mats, const rt_param* prm, uint32_t np) {
+    bool build(const std::vector<rt_material>& mats, const rt_param* prm, uint32_t np,
+               const std::vector<MediumSet>* media = nullptr) {
         if (!prm || np == 0 || np > 64) return false;
         fmap.assign(24 * mats.size(), (int8_t)-1);
+        mpar.assign(media ? media->size() : 0, -1);
         for (uint32_t i = 0; i < np; i++) {
             const rt_param& q = prm[i];
+            if (q.material & RT_PARAM_MEDIUM) {
+                uint32_t k = q.material & ~RT_PARAM_MEDIUM;
+                if (!media || k >= media->size() || q.field != 0 || q.count != 1 || mpar[k] >= 0) return false;
+                const MediumSet& ms = (*media)[k];
+                if (ms.objs.empty() || ms.transformed || !ms.constant) return false;
+                mpar[k] = (int)i;
+                continue;
+            }
             if (q.material >= mats.size() || q.count == 0 || q.field >= 24 || q.count > 24 - q.field) return false;
             const rt_material& m = mats[q.material];
             for (uint32_t f = q.field; f < q.field + q.count; f++) {
@@ -174,23 +190,47 @@ static inline Color vertex_ratio(const std::vector<rt_material>& ms, const std::
     return Color(1, 1, 1);
 }
 
-// Forward score accumulation for one sample.
+// A parameterised constant medium object: its density at theta_s (ds) and
+// at theta (de), and its parameter index.
+struct MedParam { const ConstantMedium* cm; int param; float ds, de; };
+// In-medium length of the segment [t_min, t_end] (the medium's own interval
+// logic), and whether the segment ended in one of its scattering events.
+static inline float med_overlap(const MedParam& m, const Ray& r, float t_min, float t_end) {
+    float len;
+    return inside_length(*m.cm->boundary, r, t_min, t_end, len) ? len : 0.0f;
+}
+static inline std::vector<MedParam> med_params(const ParamMap& pm, const std::vector<MediumSet>& media, const float* de) {
+    std::vector<MedParam> out;
+    for (size_t k = 0; k < pm.mpar.size(); k++)
+        if (pm.mpar[k] >= 0)
+            for (const ConstantMedium* cm : media[k].objs) out.push_back(MedParam{cm, pm.mpar[k], cm->density, de ? de[pm.mpar[k]] : cm->density});
+    return out;
+}
+
+// Forward score accumulation for one sample.  A traced segment adds
+// dlog p / dsigma = -L (+ 1 / sigma when it ends in that medium's event) to
+// every later term; a shadow ray adds -L to its own term.
 struct ScoreHook {
     const std::vector<rt_material>* mats;
     const ParamMap* pm;
+    const std::vector<MedParam>* med = nullptr;
     Color delta;
-    std::vector<Color> prefix, tmp;
+    std::vector<Color> prefix, tmp, shadow;
+    bool shadow_on = false;
     double* grad;
-    void init(const std::vector<rt_material>* m, const ParamMap* p, Color d, double* g) {
-        mats = m; pm = p; delta = d; grad = g;
-        prefix.assign(p->n, Color()); tmp.assign(p->n, Color());
+    void init(const std::vector<rt_material>* m, const ParamMap* p, Color d, double* g, const std::vector<MedParam>* md) {
+        mats = m; pm = p; delta = d; grad = g; med = md;
+        prefix.assign(p->n, Color()); tmp.assign(p->n, Color()); shadow.assign(p->n, Color());
+        shadow_on = false;
     }
     void acc(const Color& c, const std::vector<Color>* extra) {
         for (int j = 0; j < pm->n; j++) {
             Color s = prefix[j];
             if (extra) s += (*extra)[j];
+            if (shadow_on) s += shadow[j];
             grad[j] += (double)delta.x * c.x * s.x + (double)delta.y * c.y * s.y + (double)delta.z * c.z * s.z;
         }
+        shadow_on = false;
     }
     Color add(const Color& c) { acc(c, nullptr); return c; }
     Color add_v(const Color& c, const HitRecord& rec, const Ray& ray, const Vec3& wi) {
@@ -199,21 +239,56 @@ struct ScoreHook {
         return c;
     }
     void scatter(const HitRecord& rec, const Ray& ray, const Vec3& wi) {
+        shadow_on = false;
         vertex_dlogf(*mats, *pm, rec, ray, wi, tmp);
         for (int j = 0; j < pm->n; j++) prefix[j] += tmp[j];
     }
+    void segment(const Ray& r, bool hit, const HitRecord& rec) {
+        shadow_on = false;
+        if (!med) return;
+        for (const MedParam& m : *med) {
+            float s = -med_overlap(m, r, 0.001f, hit ? rec.t : INF);
+            if (hit && rec.mat == &m.cm->phase) s += 1.0f / m.ds;
+            prefix[m.param] += Color(s, s, s);
+        }
+    }
+    void shadow_seg(const Ray& r, float t_max) {
+        for (auto& c : shadow) c = Color();
+        shadow_on = med && !med->empty();
+        if (!shadow_on) return;
+        for (const MedParam& m : *med) { float s = -med_overlap(m, r, 0.001f, t_max); shadow[m.param] += Color(s, s, s); }
+    }
 };
 
-// Detached estimator: decisions at the scene's parameters, BSDF values at `me`.
+// Detached estimator: decisions at the scene's parameters, BSDF values and
+// medium densities at theta.
 struct RatioHook {
     const std::vector<rt_material>* ms;
     const std::vector<rt_material>* me;
+    const std::vector<MedParam>* med = nullptr;
     Color ratio = Color(1, 1, 1);
-    Color add(const Color& c) { return c * ratio; }
+    float shadow = 1.0f;
+    Color add(const Color& c) { Color v = c * ratio * shadow; shadow = 1.0f; return v; }
     Color add_v(const Color& c, const HitRecord& rec, const Ray& ray, const Vec3& wi) {
-        return c * ratio * vertex_ratio(*ms, *me, rec, ray, wi);
+        Color v = c * ratio * vertex_ratio(*ms, *me, rec, ray, wi) * shadow;
+        shadow = 1.0f;
+        return v;
     }
-    void scatter(const HitRecord& rec, const Ray& ray, const Vec3& wi) { ratio *= vertex_ratio(*ms, *me, rec, ray, wi); }
+    void scatter(const HitRecord& rec, const Ray& ray, const Vec3& wi) { shadow = 1.0f; ratio *= vertex_ratio(*ms, *me, rec, ray, wi); }
+    void segment(const Ray& r, bool hit, const HitRecord& rec) {
+        shadow = 1.0f;
+        if (!med) return;
+        for (const MedParam& m : *med) {
+            float f = std::exp(-(m.de - m.ds) * med_overlap(m, r, 0.001f, hit ? rec.t : INF));
+            if (hit && rec.mat == &m.cm->phase) f *= m.de / m.ds;
+            ratio *= f;
+        }
+    }
+    void shadow_seg(const Ray& r, float t_max) {
+        shadow = 1.0f;
+        if (!med) return;
+        for (const MedParam& m : *med) shadow *= std::exp(-(m.de - m.ds) * med_overlap(m, r, 0.001f, t_max));
+    }
 };
 
 }  // namespace rc

@@ -604,6 +604,8 @@ struct NoHook {
     Color add(const Color& c) { return c; }
     Color add_v(const Color& c, const HitRecord&, const Ray&, const Vec3&) { return c; }
     void scatter(const HitRecord&, const Ray&, const Vec3&) {}
+    void segment(const Ray&, bool, const HitRecord&) {}     // a traced path segment (to the hit, or a miss)
+    void shadow_seg(const Ray&, float) {}                    // the shadow ray of the next contribution
 };
 
 template <class H>
@@ -617,7 +619,9 @@ static inline Color ray_color_h(const Ray& r0, const SceneView& S, float bg_scal
     float prev_mis_w_brdf = 1.0f, prev_spec_sun_weight = 1.0f;
     for (int depth = 0; depth < max_depth; depth++) {
         HitRecord rec;
-        if (!world.hit(ray, 0.001f, INF, rec)) {
+        bool hit = world.hit(ray, 0.001f, INF, rec);
+        hk.segment(ray, hit, rec);
+        if (!hit) {
             float sun_pdf = sun_pdf_value(ray.direction, background);
             float env_pdf = env_pdf_value(ray.direction, background);
             float emit_w = (sun_pdf + env_pdf == 0.0f) ? 1.0f : (prev_specular ? prev_spec_sun_weight : prev_mis_w_brdf);
@@ -637,6 +641,7 @@ static inline Color ray_color_h(const Ray& r0, const SceneView& S, float bg_scal
                 Vec3 sun_sample = sample_sun_cone(sun_u, COS_SUN_MAX, rng);
                 Ray sun_shadow = Ray::at_time(rec.p, sun_sample, ray.time);
                 float tr = world.transmittance(sun_shadow, 0.001f, INF);
+                hk.shadow_seg(sun_shadow, INF);
                 if (tr > 0.0f) {
                     Color brdf_cos = rec.mat->specular_brdf_cos(ray, rec, sun_sample);
                     if (brdf_cos.x > 0.0f || brdf_cos.y > 0.0f || brdf_cos.z > 0.0f) {
@@ -669,6 +674,7 @@ static inline Color ray_color_h(const Ray& r0, const SceneView& S, float bg_scal
                 HitRecord lrec;
                 if (lights.hit(shadow, 0.001f, INF, lrec)) {
                     float tr = world.transmittance(shadow, 0.001f, lrec.t * (1.0f - 1e-4f));
+                    hk.shadow_seg(shadow, lrec.t * (1.0f - 1e-4f));
                     if (tr > 0.0f) {
                         float l_pdf = lights.pdf_value(rec.p, light_dir, ray.time);
                         float brdf = rec.mat->scattering_pdf(ray, rec, shadow);
@@ -687,6 +693,7 @@ static inline Color ray_color_h(const Ray& r0, const SceneView& S, float bg_scal
                 Vec3 sun_sample = sample_sun_cone(sun_u, COS_SUN_MAX, rng);
                 Ray sun_shadow = Ray::at_time(rec.p, sun_sample, ray.time);
                 float tr = world.transmittance(sun_shadow, 0.001f, INF);
+                hk.shadow_seg(sun_shadow, INF);
                 if (tr > 0.0f) {
                     float brdf = rec.mat->scattering_pdf(ray, rec, sun_shadow);
                     if (brdf > 0.0f) {
@@ -704,6 +711,7 @@ static inline Color ray_color_h(const Ray& r0, const SceneView& S, float bg_scal
                 if (env_pdf > 0.0f) {
                     Ray shadow = Ray::at_time(rec.p, env_dir, ray.time);
                     float tr = world.transmittance(shadow, 0.001f, INF);
+                    hk.shadow_seg(shadow, INF);
                     if (tr > 0.0f) {
                         float brdf = rec.mat->scattering_pdf(ray, rec, shadow);
                         if (brdf > 0.0f) {

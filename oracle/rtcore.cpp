@@ -19,6 +19,7 @@ namespace {
 
 struct OScene {
     PearlSun pearl_sun;                      // set from a Physical background (render_tiles, src/renderer.rs:676-679)
+    std::vector<MediumSet> media;            // per descriptor medium: its constant-medium objects (sigma_t parameters)
     std::vector<std::unique_ptr<Material>> mats;
     std::vector<rt_material> desc_mats;     // the descriptor's table, kept for the adjoint checker
     std::vector<HPtr> world_objs;
@@ -216,10 +217,15 @@ HPtr make_object(OScene& S, const rt_scene_desc& d, const rt_object& o) {
     if (o.medium) {
         if (o.medium > d.num_media) { S.err = "medium index"; return nullptr; }
         const rt_medium& m = d.media[o.medium - 1];
+        if (S.media.size() < d.num_media) S.media.resize(d.num_media);
+        MediumSet& mset = S.media[o.medium - 1];
+        if (o.xform_count) mset.transformed = true;
+        if (m.kind != RT_MEDIUM_CONSTANT) mset.constant = false;
         if (m.kind == RT_MEDIUM_CONSTANT) {
             auto cm = std::make_shared<ConstantMedium>();
             cm->boundary = h; cm->density = m.density; cm->neg_inv_density = -1.0f / m.density;
             cm->phase.albedo = Color(m.albedo[0], m.albedo[1], m.albedo[2]); cm->phase.g = m.g;
+            mset.objs.push_back(cm.get());
             h = cm;
         } else if (m.kind == RT_MEDIUM_GRID) {
             auto gm = std::make_shared<GridMedium>();
@@ -653,9 +659,17 @@ int rtc_env_eval(const float* rgb, uint32_t w, uint32_t h, float rot, const floa
 int rtc_set_params(void* s, const rt_param* prm, uint32_t n, const float* vals) {
     OScene& S = *(OScene*)s;
     ParamMap pm;
-    if (!pm.build(S.desc_mats, prm, n)) return -1;
-    for (uint32_t i = 0; i < n; i++)
+    if (!pm.build(S.desc_mats, prm, n, &S.media)) return -1;
+    for (uint32_t i = 0; i < n; i++) {
+        if (prm[i].material & RT_PARAM_MEDIUM) {
+            for (const ConstantMedium* cm : S.media[prm[i].material & ~RT_PARAM_MEDIUM].objs) {
+                auto* w = const_cast<ConstantMedium*>(cm);
+                w->density = vals[i]; w->neg_inv_density = -1.0f / vals[i];
+            }
+            continue;
+        }
         for (uint32_t f = prm[i].field; f < prm[i].field + prm[i].count; f++) S.desc_mats[prm[i].material].p[f] = vals[i];
+    }
     for (size_t i = 0; i < S.mats.size(); i++) {
         const rt_material& m = S.desc_mats[i];
         if (auto* l = dynamic_cast<Lambertian*>(S.mats[i].get())) l->tex.even = Color(m.p[0], m.p[1], m.p[2]);
@@ -679,10 +693,13 @@ int rtc_render_detached(void* s, const rt_camera* cam, const rt_sample_params* p
                         uint32_t n, const float* eval_vals, float* accum, int mode, int threads) {
     OScene& S = *(OScene*)s;
     ParamMap pm;
-    if (!pm.build(S.desc_mats, prm, n)) return -1;
+    if (!pm.build(S.desc_mats, prm, n, &S.media)) return -1;
     std::vector<rt_material> me = S.desc_mats;
-    for (uint32_t i = 0; i < n; i++)
+    for (uint32_t i = 0; i < n; i++) {
+        if (prm[i].material & RT_PARAM_MEDIUM) continue;
         for (uint32_t f = prm[i].field; f < prm[i].field + prm[i].count; f++) me[prm[i].material].p[f] = eval_vals[i];
+    }
+    const std::vector<MedParam> mp = med_params(pm, S.media, eval_vals);
     SceneView V = view(S, p->use_photons != 0);
     Camera C = to_cam(*cam);
     SampleParams base = to_params(*p);
@@ -695,7 +712,7 @@ int rtc_render_detached(void* s, const rt_camera* cam, const rt_sample_params* p
             for (uint32_t col = 0; col < P.width; col++) {
                 size_t i = row * P.width + col;
                 if (mode == 1) rng = SmallRng::seed_from_u64(row_or_pixel_seed(1, (uint32_t)row, (uint32_t)i, P));
-                RatioHook hk; hk.ms = &S.desc_mats; hk.me = &me;
+                RatioHook hk; hk.ms = &S.desc_mats; hk.me = &me; hk.med = &mp;
                 Color c = render_pixel_h(V, C, P, (uint32_t)row, col, rng, hk);
                 accum[3 * i] += c.x; accum[3 * i + 1] += c.y; accum[3 * i + 2] += c.z;
             }
@@ -710,7 +727,8 @@ int rtc_backward(void* s, const rt_camera* cam, const rt_sample_params* p, uint3
                  const float* dimg, double* grad, int mode, int threads) {
     OScene& S = *(OScene*)s;
     ParamMap pm;
-    if (!pm.build(S.desc_mats, prm, n)) return -1;
+    if (!pm.build(S.desc_mats, prm, n, &S.media)) return -1;
+    const std::vector<MedParam> mp = med_params(pm, S.media, nullptr);
     SceneView V = view(S, p->use_photons != 0);
     Camera C = to_cam(*cam);
     SampleParams base = to_params(*p);
@@ -724,7 +742,7 @@ int rtc_backward(void* s, const rt_camera* cam, const rt_sample_params* p, uint3
                 size_t i = row * P.width + col;
                 if (mode == 1) rng = SmallRng::seed_from_u64(row_or_pixel_seed(1, (uint32_t)row, (uint32_t)i, P));
                 Color d(dimg[3 * i] / (float)spp, dimg[3 * i + 1] / (float)spp, dimg[3 * i + 2] / (float)spp);
-                hk.init(&S.desc_mats, &pm, d, rows[row].data());
+                hk.init(&S.desc_mats, &pm, d, rows[row].data(), &mp);
                 render_pixel_h(V, C, P, (uint32_t)row, col, rng, hk);
             }
         });

@@ -20,6 +20,7 @@ RT_ERR_NOMEM = -4
 XFORM_TRANSLATE, XFORM_ROTATE, XFORM_SCALE = 1, 2, 3
 MEDIUM_CONSTANT, MEDIUM_GRID, MEDIUM_NOISE = 1, 2, 3
 TEX_SOLID, TEX_CHECKER, TEX_IMAGE, TEX_NOISE, TEX_MOTTLE = 0, 1, 2, 3, 4
+PARAM_MEDIUM = 0x40000000    # rt_param.material | k: the density of rt_scene_desc.media[k]
 (MAT_LAMBERTIAN, MAT_DIELECTRIC, MAT_SPECTRAL_DIELECTRIC, MAT_PBR, MAT_DIFFUSE_LIGHT,
  MAT_SPECTRAL_METAL, MAT_ROUGH_DIELECTRIC, MAT_ROUGH_SPECTRAL, MAT_BLACKBODY, MAT_SSS, MAT_PEARL) = \
     1, 2, 3, 4, 5, 6, 7, 8, 9, 10, 11

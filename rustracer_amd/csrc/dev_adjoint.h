@@ -209,10 +209,42 @@ RT_D V3 vertex_ratio(const DScene& S, const DMaterial* me, const Rec& h, V3 wo, 
     return mk(1, 1, 1);
 }
 
+// ---- constant-medium densities ---------------------------------------------------------------
+// A parameterised medium object: its leaf slot, parameter index, and density
+// at theta_s (ds, the scene's) and at theta (de).  A traced segment has log
+// density -sigma L (no event before its end, L the in-medium length) plus
+// log sigma when it ends in that medium's event; a shadow ray's
+// transmittance is exp(-sigma L).
+struct MedPar { int32_t slot, param; float ds, de; };
+RT_D float medium_overlap(const DScene& S, const DObject& ob, const Ray& wr, float t_min, float t_end) {
+    Ray r = to_object(S, ob, wr);
+    float a, b;
+    if (!boundary_t(S, ob, r, -FINF, FINF, a)) return 0.0f;
+    if (!boundary_t(S, ob, r, a + 0.0001f, FINF, b)) return 0.0f;
+    float t1 = fmx(fmx(a, 0.0f), t_min), t2 = fmn(b, t_end);
+    return t1 < t2 ? (t2 - t1) * vlen(r.d) : 0.0f;
+}
+RT_D float segment_length(const DScene& S, const MedPar& m, const Ray& r, const HitId& id) {
+    return medium_overlap(S, S.objs[m.slot], r, 0.001f, id.slot < 0 ? FINF : id.t);
+}
+RT_D bool segment_event(const MedPar& m, const HitId& id) { return id.prim == -2 && id.slot == m.slot; }
+RT_D float shadow_length(const DScene& S, const MedPar& m, const ShadowReq& q) {
+    return medium_overlap(S, S.objs[m.slot], make_ray(q.o, q.d, q.time), 0.001f, q.t_max);
+}
+
 // ---- hooks -----------------------------------------------------------------------------------
 struct DetachedHook {
     const DMaterial* me;
+    const MedPar* mp;
+    int nmp;
     V3 ratio;
+    RT_D void segment(const DScene& S, const Ray& r, const HitId& id) {
+        for (int k = 0; k < nmp; k++) {
+            float f = expf(-(mp[k].de - mp[k].ds) * segment_length(S, mp[k], r, id));
+            if (segment_event(mp[k], id)) f *= mp[k].de / mp[k].ds;
+            ratio *= f;
+        }
+    }
     RT_D V3 emit(V3 c) { return c * ratio; }
     RT_D V3 emit_vertex(V3 c, const DScene& S, const Rec& h, const Ray& r) {
         V3 wo = unit(-r.d);
@@ -221,17 +253,20 @@ struct DetachedHook {
     RT_D void shadow(ShadowReq& q, int, const DScene& S, const Rec& h, const Ray& r, bool vdep) {
         q.contrib *= ratio;
         if (vdep) q.contrib *= vertex_ratio(S, me, h, unit(-r.d), unit(q.d), r.wavelength);
+        for (int k = 0; k < nmp; k++) q.contrib *= expf(-(mp[k].de - mp[k].ds) * shadow_length(S, mp[k], q));
     }
     RT_D void scatter(const DScene& S, const Rec& h, const Ray& r, V3 d) {
         ratio *= vertex_ratio(S, me, h, unit(-r.d), unit(d), r.wavelength);
     }
     RT_D void begin_vertex() {}
-    RT_D void resolved(int, V3) {}
+    RT_D void resolved(int, V3, const ShadowReq&, const DScene&) {}
     RT_D void end_vertex() {}
 };
 
 struct ReplayHook {
     const int8_t* fmap;
+    const MedPar* mp;
+    int nmp;
     float* g;             // per-block LDS accumulators, one per parameter
     V3 delta;             // dLoss / d(sample radiance)
     V3 rem;               // radiance of this sample not yet collected
@@ -242,6 +277,18 @@ struct ReplayHook {
         for (int k = 0; k < sc.n; k++) {
             float v = delta.x * c.x * sc.v[k].x + delta.y * c.y * sc.v[k].y + delta.z * c.z * sc.v[k].z;
             if (v != 0.0f && isfinite(v)) atomicAdd(g + sc.p[k], v);
+        }
+    }
+    RT_D void add1(int p, float s, V3 c) {
+        float v = (delta.x * c.x + delta.y * c.y + delta.z * c.z) * s;
+        if (v != 0.0f && isfinite(v)) atomicAdd(g + p, v);
+    }
+    // the radiance still to come travelled this segment
+    RT_D void segment(const DScene& S, const Ray& r, const HitId& id) {
+        for (int k = 0; k < nmp; k++) {
+            float s = -segment_length(S, mp[k], r, id);
+            if (segment_event(mp[k], id)) s += 1.0f / mp[k].ds;
+            add1(mp[k].param, s, rem);
         }
     }
     RT_D V3 emit(V3 c) { rem -= c; return c; }
@@ -262,9 +309,10 @@ struct ReplayHook {
         pend_ok = true;
     }
     RT_D void begin_vertex() { vs_ok = false; pend_ok = false; vdep[0] = vdep[1] = vdep[2] = false; }
-    RT_D void resolved(int k, V3 c) {
+    RT_D void resolved(int k, V3 c, const ShadowReq& q, const DScene& S) {
         rem -= c;
         if (vdep[k] && vs_ok) add(vs, c);
+        for (int j = 0; j < nmp; j++) add1(mp[j].param, -shadow_length(S, mp[j], q), c);
     }
     // radiance still to come arrives through this vertex's sampled direction
     RT_D void end_vertex() { if (pend_ok) add(pend, rem); }
@@ -272,7 +320,7 @@ struct ReplayHook {
 
 struct PlainHook : NoDiff {
     RT_D void begin_vertex() {}
-    RT_D void resolved(int, V3) {}
+    RT_D void resolved(int, V3, const ShadowReq&, const DScene&) {}
     RT_D void end_vertex() {}
 };
 
@@ -290,7 +338,7 @@ RT_D V3 ray_color_t(const DScene& S, Ray ray, int max_depth, float bg_scale, boo
         bool cont = bounce_t(S, ps, id, depth, bg_scale, use_photons, rng, color, sh, nsh, st, dh);
         for (int k = 0; k < nsh; k++) {
             float tr = trace_transmittance(S, make_ray(sh[k].o, sh[k].d, sh[k].time), 0.001f, sh[k].t_max, st, false);
-            if (tr > 0.0f) { V3 c = sh[k].contrib * tr; color += c; dh.resolved(k, c); }
+            if (tr > 0.0f) { V3 c = sh[k].contrib * tr; color += c; dh.resolved(k, c, sh[k], S); }
         }
         dh.end_vertex();
         if (!cont) break;

@@ -1167,6 +1167,7 @@ template <class DH, uint32_t MASK = RT_MASK_ALL>
 RT_D bool bounce_t(const DScene& S, PathState& ps, const HitId& id, int depth, float bg_scale, bool use_photons, Rng& rng,
                    V3& color, ShadowReq* sh, int& nsh, Stack& st, DH& dh) {
     Ray& ray = ps.ray;
+    dh.segment(S, ray, id);
     const bool lights_empty = S.num_lights == 0;
     const bool sun = physical_sun(S);
     const bool env = has_env(S);
@@ -1293,6 +1294,7 @@ RT_D bool bounce_t(const DScene& S, PathState& ps, const HitId& id, int depth, f
 
 // Hooks of bounce_t for the plain renderers: no derivative bookkeeping.
 struct NoDiff {
+    RT_D void segment(const DScene&, const Ray&, const HitId&) {}
     RT_D V3 emit(V3 c) { return c; }
     RT_D V3 emit_vertex(V3 c, const DScene&, const Rec&, const Ray&) { return c; }
     RT_D void shadow(ShadowReq&, int, const DScene&, const Rec&, const Ray&, bool) {}

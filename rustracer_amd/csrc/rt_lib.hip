@@ -111,7 +111,7 @@ __global__ __launch_bounds__(BLOCK) void k_render(DScene S, DCamera cam, DParams
 // Detached-sampling forward render: the k_render frame with BSDF values
 // evaluated on the materials `me` while every decision follows the scene's.
 __global__ __launch_bounds__(BLOCK) void k_detached(DScene S, DCamera cam, DParams P, const DMaterial* __restrict__ me,
-                                                    float* __restrict__ out, uint32_t spp) {
+                                                    const MedPar* __restrict__ mp, int nmp, float* __restrict__ out, uint32_t spp) {
     Stack st; init_stack(st);
     uint32_t i = blockIdx.x * BLOCK + threadIdx.x;
     uint32_t npix = P.width * P.height;
@@ -123,7 +123,7 @@ __global__ __launch_bounds__(BLOCK) void k_detached(DScene S, DCamera cam, DPara
         Rng rng = rng_seed(pixel_seed(i, sidx, P.seed));
         Ray r = pixel_ray(cam, P, row, col, sidx, rng);
         DetachedHook dh;
-        dh.me = me; dh.ratio = mk(1, 1, 1);
+        dh.me = me; dh.mp = mp; dh.nmp = nmp; dh.ratio = mk(1, 1, 1);
         sum += ray_color_t(S, r, (int)P.max_depth, P.bg_scale, P.use_photons != 0, rng, st, dh);
     }
     out[3 * i] += sum.x; out[3 * i + 1] += sum.y; out[3 * i + 2] += sum.z;
@@ -135,6 +135,7 @@ __global__ __launch_bounds__(BLOCK) void k_detached(DScene S, DCamera cam, DPara
 // per parameter and block.
 #define RT_MAX_PARAMS 64
 __global__ __launch_bounds__(BLOCK) void k_backward(DScene S, DCamera cam, DParams P, const int8_t* __restrict__ fmap,
+                                                    const MedPar* __restrict__ mp, int nmp,
                                                     uint32_t nparams, const float* __restrict__ dimg, float inv_spp,
                                                     uint32_t spp, float* __restrict__ grad) {
     __shared__ float g[RT_MAX_PARAMS];
@@ -158,7 +159,7 @@ __global__ __launch_bounds__(BLOCK) void k_backward(DScene S, DCamera cam, DPara
                 rng = rng_seed(seed);
                 r = pixel_ray(cam, P, row, col, sidx, rng);
                 ReplayHook rh;
-                rh.fmap = fmap; rh.g = g; rh.delta = delta; rh.rem = L;
+                rh.fmap = fmap; rh.mp = mp; rh.nmp = nmp; rh.g = g; rh.delta = delta; rh.rem = L;
                 ray_color_t(S, r, (int)P.max_depth, P.bg_scale, P.use_photons != 0, rng, st, rh);
             }
         }
@@ -433,6 +434,7 @@ __device__ __forceinline__ uint32_t sort_key(const DScene& S, const HitId& id, u
 __device__ __forceinline__ uint32_t extend_one(const DScene& S, WfPath& w, uint32_t sort, Stack& st) {
     float4 o = w.o_time, d = w.d_wl;
     Ray r = make_ray(mk(o.x, o.y, o.z), mk(d.x, d.y, d.z), o.w);
+    r.wavelength = d.w; r.spectral = (w.flags & 1u) != 0;   // media seed their free-flight stream from the ray, wavelength included
     HitId id = trace_closest(S, r, 0.001f, FINF, st);
     w.hit_t = id.t; w.hit_slot = id.slot; w.hit_prim = id.prim;
     return sort_key(S, id, sort);
@@ -548,6 +550,7 @@ __global__ __launch_bounds__(BLOCK) void k_wf_knn(DScene S0, DParams P0, WfQueue
         uint32_t kind = S.mats[S.objs[w.hit_slot].material].kind;
         if (kind != M_LAMBERTIAN && kind != M_PBR && kind != M_PEARL) continue;
         Ray r = make_ray(mk(w.o_time.x, w.o_time.y, w.o_time.z), mk(w.d_wl.x, w.d_wl.y, w.d_wl.z), w.o_time.w);
+        r.wavelength = w.d_wl.w; r.spectral = (w.flags & 1u) != 0;
         HitId id; id.t = w.hit_t; id.slot = w.hit_slot; id.prim = w.hit_prim;
         Rec h = finalize(S, r, id, 0.001f);
         V3 e = photon_knn_irradiance(S, h.p, h.n, heap, BLOCK);
@@ -597,6 +600,8 @@ struct WfAdjQ {
     const float* lsamp;               // primal radiance per sample of the chunk
     const float* dimg;                // dLoss/dI per pixel
     const int8_t* fmap;
+    const MedPar* mp;                 // parameterised media
+    int nmp;
     float inv_spp;
     uint32_t nparams;
     float* grad;
@@ -619,6 +624,8 @@ __global__ __launch_bounds__(BLOCK) void k_wfb_init(WfQueues Q, WfAdjQ A, uint32
 
 struct WfReplayHook {
     const int8_t* fmap;
+    const MedPar* mp;
+    int nmp;
     float* g;
     V3 delta, rem;
     Score pend;
@@ -630,6 +637,15 @@ struct WfReplayHook {
         for (int k = 0; k < sc.n; k++) {
             float v = delta.x * c.x * sc.v[k].x + delta.y * c.y * sc.v[k].y + delta.z * c.z * sc.v[k].z;
             if (v != 0.0f && isfinite(v)) atomicAdd(g + sc.p[k], v);
+        }
+    }
+    // the radiance still to come travelled this segment (medium densities)
+    RT_D void segment(const DScene& S, const Ray& r, const HitId& id) {
+        for (int k = 0; k < nmp; k++) {
+            float s = -segment_length(S, mp[k], r, id);
+            if (segment_event(mp[k], id)) s += 1.0f / mp[k].ds;
+            float v = (delta.x * rem.x + delta.y * rem.y + delta.z * rem.z) * s;
+            if (v != 0.0f && isfinite(v)) atomicAdd(g + mp[k].param, v);
         }
     }
     RT_D V3 emit(V3 c) { rem -= c; return c; }
@@ -676,7 +692,7 @@ __global__ __launch_bounds__(BLOCK) void k_wfb_shade(DScene S, DParams P, WfQueu
             WfPath& w = Q.paths[pid];
             WfAdj& a = A.adj[pid];
             WfReplayHook rh;
-            rh.fmap = A.fmap; rh.g = g;
+            rh.fmap = A.fmap; rh.mp = A.mp; rh.nmp = A.nmp; rh.g = g;
             rh.delta = mk(a.delta.x, a.delta.y, a.delta.z);
             rh.rem = mk(a.rem.x, a.rem.y, a.rem.z);
             rh.pend_ok = false; rh.vs_ok = false;
@@ -757,6 +773,15 @@ __global__ __launch_bounds__(BLOCK) void k_wfb_shadow(DScene S, WfQueues Q, WfAd
                     if (v != 0.0f && isfinite(v)) atomicAdd(g + a.vp[k], v);
                 }
             }
+            if (A.nmp) {
+                ShadowReq sq;
+                sq.o = r.o; sq.d = r.d; sq.t_max = q.o_tmax.w; sq.time = q.d_time.w;
+                float dc = a.delta.x * c.x + a.delta.y * c.y + a.delta.z * c.z;
+                for (int k = 0; k < A.nmp; k++) {
+                    float v = -shadow_length(S, A.mp[k], sq) * dc;
+                    if (v != 0.0f && isfinite(v)) atomicAdd(g + A.mp[k].param, v);
+                }
+            }
         }
     }
     __syncthreads();
@@ -805,6 +830,11 @@ struct rt_scene {
     uint64_t stats[8] = {0};
     std::vector<float4> photons_host;
     std::vector<DMaterial> mats_host;     // the materials as on the device (rt_scene_set_params edits both)
+    std::vector<DMedium> media_host;      // the media as on the device (rt_scene_set_params edits densities)
+    std::vector<uint32_t> media_src;      // per device medium: descriptor medium index
+    std::vector<uint8_t> media_xf;        // per device medium: boundary under transforms
+    std::vector<int32_t> media_slot;      // per device medium: leaf slot of its object
+    uint32_t desc_media = 0;
     void* ph_mem = nullptr;                // photon map (positions, powers, normals), replaced on rebuild
     float world_min[3] = {0, 0, 0}, world_max[3] = {0, 0, 0};
     ~rt_scene() { for (void* p : allocs) (void)hipFree(p); if (ph_mem) (void)hipFree(ph_mem); }
@@ -1015,6 +1045,8 @@ struct Flat {
     std::vector<DQuad> quads;
     std::vector<float4> planes;
     std::vector<DMedium> media;
+    std::vector<uint32_t> media_src;      // per device medium: its rt_scene_desc.media index
+    std::vector<uint8_t> media_xf;        // per device medium: 1 when its boundary carries transforms
     std::vector<float> grids;
     std::string err;
 };
@@ -1221,6 +1253,8 @@ bool flatten_object(Flat& F, const rt_scene_desc& d, const rt_object& o, DObject
             F.grids.insert(F.grids.end(), m.perlin, m.perlin + 1536);
         } else { F.err = "medium kind"; return false; }
         F.media.push_back(M);
+        F.media_src.push_back(o.medium - 1);
+        F.media_xf.push_back(o.xform_count ? 1 : 0);
         D.medium = (uint32_t)F.media.size();
     }
     D.xf_first = (uint32_t)F.xforms.size();
@@ -1343,6 +1377,10 @@ extern "C" int rt_scene_upload(rt_ctx* ctx, const rt_scene_desc* d, rt_scene** o
     if ((rc = upload_vec(s.get(), F.quads, &S.quads))) return rc;
     if ((rc = upload_vec(s.get(), F.planes, &S.planes))) return rc;
     if ((rc = upload_vec(s.get(), F.media, &S.media))) return rc;
+    s->media_host = F.media; s->media_src = F.media_src; s->media_xf = F.media_xf; s->desc_media = d->num_media;
+    s->media_slot.assign(F.media.size(), -1);
+    for (size_t k = 0; k < objs_leaf.size(); k++)
+        if (objs_leaf[k].medium) s->media_slot[objs_leaf[k].medium - 1] = (int32_t)k;
     std::vector<DMaterial> mats(d->num_materials);
     std::vector<uint8_t> texels;
     std::vector<int64_t> image_off(d->num_images, -1), perlin_off(d->num_perlin, -1);
@@ -1815,13 +1853,38 @@ extern "C" int rt_render_batch(rt_ctx* ctx, const rt_batch_view* views, uint32_t
 // ---- differentiable material parameters ----------------------------------------------------
 // Field -> parameter table (24 int8 per material, -1 = constant) after
 // checking that every mapped field is differentiable.
-static int param_map(rt_ctx* ctx, const rt_scene* s, const rt_param* prm, uint32_t n, std::vector<int8_t>& fmap) {
+// fmap[24 * material + field] = parameter or -1; mpar: one entry per medium
+// object of each density parameter (de: the evaluation values, or the scene's).
+static int param_map(rt_ctx* ctx, const rt_scene* s, const rt_param* prm, uint32_t n, std::vector<int8_t>& fmap,
+                     std::vector<MedPar>* mpar = nullptr, const float* de = nullptr) {
     if (!prm || n == 0) return fail(ctx, RT_ERR_INVALID, "no parameters");
     if (n > RT_MAX_PARAMS) return fail(ctx, RT_ERR_INVALID, "at most " + std::to_string(RT_MAX_PARAMS) + " parameters");
     fmap.assign(24 * s->mats_host.size(), (int8_t)-1);
+    if (mpar) mpar->clear();
+    std::vector<int> seen(s->desc_media, -1);
     for (uint32_t i = 0; i < n; i++) {
         const rt_param& q = prm[i];
         std::string tag = "parameter " + std::to_string(i) + ": ";
+        if (q.material & RT_PARAM_MEDIUM) {
+            uint32_t k = q.material & ~RT_PARAM_MEDIUM;
+            if (k >= s->desc_media) return fail(ctx, RT_ERR_INVALID, tag + "medium out of range");
+            if (q.field != 0 || q.count != 1) return fail(ctx, RT_ERR_UNSUPPORTED, tag + "a medium has one parameter: field 0, count 1");
+            if (seen[k] >= 0) return fail(ctx, RT_ERR_INVALID, tag + "medium already mapped");
+            seen[k] = (int)i;
+            int found = 0;
+            for (size_t j = 0; j < s->media_host.size(); j++) {
+                if (s->media_src[j] != k) continue;
+                if (s->media_host[j].kind != RT_MEDIUM_CONSTANT) return fail(ctx, RT_ERR_UNSUPPORTED, tag + "only CONSTANT media are differentiable");
+                if (s->media_xf[j]) return fail(ctx, RT_ERR_UNSUPPORTED, tag + "the medium's boundary carries a transform");
+                found++;
+                if (mpar) {
+                    float d0 = s->media_host[j].density;
+                    mpar->push_back(MedPar{s->media_slot[j], (int32_t)i, d0, de ? de[i] : d0});
+                }
+            }
+            if (!found) return fail(ctx, RT_ERR_INVALID, tag + "no object uses that medium");
+            continue;
+        }
         if (q.material >= s->mats_host.size()) return fail(ctx, RT_ERR_INVALID, tag + "material out of range");
         if (q.count == 0 || q.field >= 24 || q.count > 24 - q.field) return fail(ctx, RT_ERR_INVALID, tag + "field range outside p[0..24)");
         const DMaterial& m = s->mats_host[q.material];
@@ -1855,10 +1918,20 @@ extern "C" int rt_scene_set_params(rt_ctx* ctx, rt_scene* s, const rt_param* prm
     for (uint32_t i = 0; i < n; i++)
         if (!std::isfinite(values[i])) return fail(ctx, RT_ERR_INVALID, "parameter " + std::to_string(i) + " is not finite");
     HIPCHECK(ctx, hipSetDevice(ctx->device));
-    for (uint32_t i = 0; i < n; i++)
+    bool media = false;
+    for (uint32_t i = 0; i < n; i++) {
+        if (prm[i].material & RT_PARAM_MEDIUM) {
+            if (!(values[i] > 0.0f)) return fail(ctx, RT_ERR_INVALID, "parameter " + std::to_string(i) + ": a density must be positive");
+            for (size_t j = 0; j < s->media_host.size(); j++)
+                if (s->media_src[j] == (prm[i].material & ~RT_PARAM_MEDIUM)) s->media_host[j].density = values[i];
+            media = true;
+            continue;
+        }
         for (uint32_t f = prm[i].field; f < prm[i].field + prm[i].count; f++) s->mats_host[prm[i].material].p[f] = values[i];
+    }
     HIPCHECK(ctx, hipStreamSynchronize(ctx->stream));
     HIPCHECK(ctx, hipMemcpy((void*)s->S.mats, s->mats_host.data(), sizeof(DMaterial) * s->mats_host.size(), hipMemcpyHostToDevice));
+    if (media) HIPCHECK(ctx, hipMemcpy((void*)s->S.media, s->media_host.data(), sizeof(DMedium) * s->media_host.size(), hipMemcpyHostToDevice));
     return RT_OK;
 }
 
@@ -1867,7 +1940,14 @@ extern "C" int rt_scene_get_params(rt_ctx* ctx, rt_scene* s, const rt_param* prm
     std::vector<int8_t> fmap;
     int rc;
     if ((rc = param_map(ctx, s, prm, n, fmap))) return rc;
-    for (uint32_t i = 0; i < n; i++) values[i] = s->mats_host[prm[i].material].p[prm[i].field];
+    for (uint32_t i = 0; i < n; i++) {
+        if (prm[i].material & RT_PARAM_MEDIUM) {
+            for (size_t j = 0; j < s->media_host.size(); j++)
+                if (s->media_src[j] == (prm[i].material & ~RT_PARAM_MEDIUM)) { values[i] = s->media_host[j].density; break; }
+            continue;
+        }
+        values[i] = s->mats_host[prm[i].material].p[prm[i].field];
+    }
     return RT_OK;
 }
 
@@ -1877,15 +1957,22 @@ extern "C" int rt_render_detached(rt_ctx* ctx, rt_scene* s, const rt_camera* cam
     int rc;
     if ((rc = render_args(ctx, s, cam, p, spp))) return rc;
     std::vector<int8_t> fmap;
-    if ((rc = param_map(ctx, s, prm, n, fmap))) return rc;
+    std::vector<MedPar> mpar;
+    if ((rc = param_map(ctx, s, prm, n, fmap, &mpar, eval_values))) return rc;
+    for (const MedPar& m : mpar) if (!(m.de > 0.0f)) return fail(ctx, RT_ERR_INVALID, "a density must be positive");
     HIPCHECK(ctx, hipSetDevice(ctx->device));
     std::vector<DMaterial> me = s->mats_host;
-    for (uint32_t i = 0; i < n; i++)
+    for (uint32_t i = 0; i < n; i++) {
+        if (prm[i].material & RT_PARAM_MEDIUM) continue;
         for (uint32_t f = prm[i].field; f < prm[i].field + prm[i].count; f++) me[prm[i].material].p[f] = eval_values[i];
+    }
     void* dme = nullptr;
-    HIPCHECK(ctx, hipMalloc(&dme, sizeof(DMaterial) * me.size()));
+    size_t bme = (sizeof(DMaterial) * me.size() + 255) & ~(size_t)255;
+    HIPCHECK(ctx, hipMalloc(&dme, bme + sizeof(MedPar) * (mpar.size() + 1)));
     std::unique_ptr<void, void (*)(void*)> guard(dme, [](void* q) { (void)hipFree(q); });
     HIPCHECK(ctx, hipMemcpy(dme, me.data(), sizeof(DMaterial) * me.size(), hipMemcpyHostToDevice));
+    if (!mpar.empty()) HIPCHECK(ctx, hipMemcpy((char*)dme + bme, mpar.data(), sizeof(MedPar) * mpar.size(), hipMemcpyHostToDevice));
+    const MedPar* dmp = (const MedPar*)((char*)dme + bme);
     uint32_t npix = p->width * p->height;
     Staged rout;
     if ((rc = stage_out(ctx, out, (size_t)npix * 12, rout, false))) return rc;
@@ -1899,7 +1986,7 @@ extern "C" int rt_render_detached(rt_ctx* ctx, rt_scene* s, const rt_camera* cam
         DParams Q = P;
         Q.sample_idx = P.sample_idx + k;
         hipLaunchKernelGGL(k_detached, dim3((npix + BLOCK - 1) / BLOCK), dim3(BLOCK), 0, ctx->stream, s->S, C, Q,
-                           (const DMaterial*)dme, (float*)rout.dev, std::min(chunk, spp - k));
+                           (const DMaterial*)dme, dmp, (int)mpar.size(), (float*)rout.dev, std::min(chunk, spp - k));
         HIPCHECK(ctx, hipGetLastError());
     }
     if (ctx->timing) {
@@ -1919,16 +2006,20 @@ extern "C" int rt_backward(rt_ctx* ctx, rt_scene* s, const rt_camera* cam, const
     int rc;
     if ((rc = render_args(ctx, s, cam, p, spp))) return rc;
     std::vector<int8_t> fmap;
-    if ((rc = param_map(ctx, s, prm, n, fmap))) return rc;
+    std::vector<MedPar> mpar;
+    if ((rc = param_map(ctx, s, prm, n, fmap, &mpar))) return rc;
     HIPCHECK(ctx, hipSetDevice(ctx->device));
     uint32_t npix = p->width * p->height;
     Staged din, gout;
     if ((rc = stage_in(ctx, dimg, (size_t)npix * 12, din))) return rc;
     if ((rc = stage_out(ctx, grad, (size_t)n * 4, gout, false))) return rc;
     void* dmap = nullptr;
-    HIPCHECK(ctx, hipMalloc(&dmap, fmap.size()));
+    size_t bmap = (fmap.size() + 255) & ~(size_t)255;
+    HIPCHECK(ctx, hipMalloc(&dmap, bmap + sizeof(MedPar) * (mpar.size() + 1)));
     std::unique_ptr<void, void (*)(void*)> guard(dmap, [](void* q) { (void)hipFree(q); });
     HIPCHECK(ctx, hipMemcpy(dmap, fmap.data(), fmap.size(), hipMemcpyHostToDevice));
+    if (!mpar.empty()) HIPCHECK(ctx, hipMemcpy((char*)dmap + bmap, mpar.data(), sizeof(MedPar) * mpar.size(), hipMemcpyHostToDevice));
+    const MedPar* dmp = (const MedPar*)((char*)dmap + bmap);
     HIPCHECK(ctx, hipMemsetAsync(gout.dev, 0, (size_t)n * 4, ctx->stream));
     DCamera C; DParams P;
     to_dev(cam, p, C, P);
@@ -1939,8 +2030,8 @@ extern "C" int rt_backward(rt_ctx* ctx, rt_scene* s, const rt_camera* cam, const
         DParams Q = P;
         Q.sample_idx = P.sample_idx + k;
         hipLaunchKernelGGL(k_backward, dim3((npix + BLOCK - 1) / BLOCK), dim3(BLOCK), 0, ctx->stream, s->S, C, Q,
-                           (const int8_t*)dmap, n, (const float*)din.dev, 1.0f / (float)spp, std::min(chunk, spp - k),
-                           (float*)gout.dev);
+                           (const int8_t*)dmap, dmp, (int)mpar.size(), n, (const float*)din.dev, 1.0f / (float)spp,
+                           std::min(chunk, spp - k), (float*)gout.dev);
         HIPCHECK(ctx, hipGetLastError());
     }
     if (ctx->timing) {
@@ -1961,7 +2052,8 @@ extern "C" int rt_backward_wavefront(rt_ctx* ctx, rt_scene* s, const rt_camera* 
     int rc;
     if ((rc = render_args(ctx, s, cam, p, spp))) return rc;
     std::vector<int8_t> fmap;
-    if ((rc = param_map(ctx, s, prm, n, fmap))) return rc;
+    std::vector<MedPar> mpar;
+    if ((rc = param_map(ctx, s, prm, n, fmap, &mpar))) return rc;
     HIPCHECK(ctx, hipSetDevice(ctx->device));
     uint32_t npix = p->width * p->height;
     uint64_t total = (uint64_t)npix * spp;
@@ -1975,14 +2067,18 @@ extern "C" int rt_backward_wavefront(rt_ctx* ctx, rt_scene* s, const rt_camera* 
     if ((rc = stage_out(ctx, grad, (size_t)n * 4, gout, false))) return rc;
     // side buffers: per-slot adjoint state, per-sample primal radiance, field map
     size_t b_adj = (size_t)o.cap * sizeof(WfAdj), b_l = ((size_t)o.cap * 12 + 255) & ~(size_t)255;
+    size_t b_map = (fmap.size() + 255) & ~(size_t)255;
     char* side = nullptr;
-    HIPCHECK(ctx, hipMalloc(&side, b_adj + b_l + fmap.size()));
+    HIPCHECK(ctx, hipMalloc(&side, b_adj + b_l + b_map + sizeof(MedPar) * (mpar.size() + 1)));
     std::unique_ptr<void, void (*)(void*)> guard(side, [](void* q) { (void)hipFree(q); });
     HIPCHECK(ctx, hipMemcpy(side + b_adj + b_l, fmap.data(), fmap.size(), hipMemcpyHostToDevice));
+    if (!mpar.empty()) HIPCHECK(ctx, hipMemcpy(side + b_adj + b_l + b_map, mpar.data(), sizeof(MedPar) * mpar.size(), hipMemcpyHostToDevice));
     WfAdjQ A;
     A.adj = (WfAdj*)side;
     A.lsamp = (const float*)(side + b_adj);
     A.fmap = (const int8_t*)(side + b_adj + b_l);
+    A.mp = (const MedPar*)(side + b_adj + b_l + b_map);
+    A.nmp = (int)mpar.size();
     A.dimg = (const float*)din.dev;
     A.inv_spp = 1.0f / (float)spp;
     A.nparams = n;

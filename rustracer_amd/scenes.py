@@ -227,6 +227,27 @@ ADJOINT_PARAM_NAMES = (["n_d", "sellmeier_B1", "alpha_table", "floor_albedo"] +
                        [f"alpha_group{k}" for k in range(16)])
 
 
+def fog_adjoint(width=64, height=64):
+    """A constant-density fog ball (sigma_t 2.0, albedo 0.8, HG g 0.3) over a
+    grey Lambertian floor, beside a diffuse sphere, under a quad light and a
+    sky gradient: camera rays, bounces, shadow rays and misses all cross the
+    fog.  cfg["params"]: the fog's sigma_t (rt_param with the medium flag) and
+    the floor's grey albedo; cfg["theta"] their values."""
+    from .scene import Background, Camera, ConstantMedium, DiffuseLight, Lambertian, Quad, Scene, Sphere
+    from . import _ffi as F
+    floor_mat = Lambertian((0.7, 0.7, 0.7))
+    fog = Sphere((0.0, 0.55, 0.0), 0.5, None)
+    ConstantMedium(fog, 2.0, (0.8, 0.8, 0.8), g=0.3)
+    lq = Quad((-0.35, 1.9, -0.35), (0.7, 0, 0), (0, 0, 0.7), DiffuseLight((6.0, 6.0, 6.0)))
+    objs = [Quad((-4, 0, -4), (8, 0, 0), (0, 0, 8), floor_mat), fog,
+            Sphere((0.75, 0.3, 0.4), 0.3, Lambertian((0.6, 0.3, 0.2))), lq]
+    scene = Scene(objs, [lq], Background.gradient((0.4, 0.5, 0.7), (0.9, 0.9, 0.9)), name="fog_adjoint")
+    cam = Camera((0.0, 1.0, -2.4), (0.0, 0.45, 0.0), (0, 1, 0), 40.0, width / height)
+    params = [(F.PARAM_MEDIUM | 0, 0, 1), (scene.material_index(floor_mat), 0, 3)]
+    return scene, cam, dict(width=width, height=height, spp=16, max_depth=12, params=params,
+                            theta=np.array([2.0, 0.7], np.float32))
+
+
 def gem_adjoint(width=512, height=512, seed=4, rel_sigma=0.05):
     """Adjoint material gradients on the gem scene.
 

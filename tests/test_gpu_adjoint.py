@@ -5,7 +5,9 @@
 * the adjoint equals the checker's host gradient (forward score accumulation,
   numeric dlog f) on the same per-pixel streams;
 * the adjoint equals central finite differences of the detached renderer
-  with common random numbers (h = 1e-3 relative) within 2 %.
+  with common random numbers (h = 1e-3 relative) within 2 %, material
+  parameters and a constant medium's sigma_t alike;
+* the wavefront replay gives the megakernel replay's gradient.
 """
 import numpy as np
 import pytest
@@ -119,3 +121,49 @@ def test_wavefront_adjoint_equals_megakernel_adjoint(gpu_renderer, cap, sort):
     b = gpu_renderer.backward(w, cam, W, H, spp, P, max_depth=24, seed=8, wavefront=True, capacity=cap, sort=sort)
     scale = np.abs(a).max()
     assert np.allclose(a, b, rtol=2e-3, atol=2e-4 * scale), np.c_[a, b]
+
+
+def test_medium_sigma_adjoint_matches_fd_and_wavefront(gpu_renderer):
+    """A constant medium's sigma_t (and a floor albedo beside it): the device
+    replay (megakernel and wavefront) equals central differences of the
+    device's detached renderer with common random numbers (h = 1e-3 relative)
+    within 2 %.  (The checker's own score gradient equals its own differences
+    in the CPU suite; the two do not share streams here, because a medium's
+    free-flight stream is seeded from the ray's bits and an ulp of difference
+    at the boundary reseeds the rest of the path.)"""
+    W, H, spp = 24, 24, 16
+    sc, cam, cfg = scenes.fog_adjoint(W, H)
+    gpu_renderer.upload(sc)
+    P, theta = cfg["params"], cfg["theta"]
+    D = cfg["max_depth"]
+    assert np.array_equal(gpu_renderer.get_params(P), theta)
+    a = np.zeros((H, W, 3), np.float32)
+    b = np.zeros((H, W, 3), np.float32)
+    gpu_renderer.render_accumulate(a, cam, W, H, spp, max_depth=D, seed=3)
+    gpu_renderer.render_detached(b, cam, W, H, spp, P, theta, max_depth=D, seed=3)
+    assert np.allclose(a, b, rtol=1e-5, atol=1e-6), np.abs(a - b).max()
+    w = np.random.default_rng(7).standard_normal((H, W, 3)).astype(np.float32)
+    g = gpu_renderer.backward(w, cam, W, H, spp, P, max_depth=D, seed=4)
+    for cap, sort in ((4096, 2), (1 << 20, 1)):
+        gw = gpu_renderer.backward(w, cam, W, H, spp, P, max_depth=D, seed=4, wavefront=True, capacity=cap, sort=sort)
+        assert np.allclose(gw, g, rtol=1e-3, atol=1e-4 * np.abs(g).max()), np.c_[gw, g]
+
+    def img(th):
+        out = np.zeros((H, W, 3), np.float32)
+        gpu_renderer.render_detached(out, cam, W, H, spp, P, th, max_depth=D, seed=4)
+        return out.astype(np.float64) / spp
+    fd = np.zeros(len(P))
+    for j in range(len(P)):
+        hh = 1e-3 * abs(float(theta[j]))
+        tp, tm = theta.astype(np.float64).copy(), theta.astype(np.float64).copy()
+        tp[j] += hh
+        tm[j] -= hh
+        fd[j] = ((img(tp) - img(tm)) * w).sum() / (float(np.float32(tp[j])) - float(np.float32(tm[j])))
+    assert np.all(np.abs(fd) > 1e-3)
+    assert np.allclose(g, fd, rtol=2e-2), np.c_[g, fd]
+    # the density is a scene parameter: set, read back, and it changes the frame
+    gpu_renderer.set_params(P, [2.5, 0.7])
+    assert np.allclose(gpu_renderer.get_params(P), [2.5, 0.7])
+    c = np.zeros((H, W, 3), np.float32)
+    gpu_renderer.render_accumulate(c, cam, W, H, spp, max_depth=D, seed=3)
+    assert not np.allclose(a, c)

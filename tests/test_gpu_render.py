@@ -214,6 +214,25 @@ def test_wavefront_gem_with_photons_equals_megakernel(gpu_renderer, sort):
     assert np.allclose(a, b, rtol=1e-4, atol=1e-5), np.abs(a - b).max()
 
 
+@pytest.mark.parametrize("which,sort", [("fog", 1), ("fog", 2), ("volume", 0)])
+def test_wavefront_with_media_equals_megakernel(gpu_renderer, which, sort):
+    """Media draw their free-flight distance from a stream seeded by the ray
+    (origin, direction, wavelength), so the wavefront's extend stage must
+    rebuild the ray whole to take the megakernel's medium events."""
+    if which == "fog":
+        sc, cam, cfg = scenes.fog_adjoint(32, 24)
+        D = cfg["max_depth"]
+    else:
+        sc, cam, cfg = scenes.volume_glass_box(32, 24, grid_n=16)
+        D = 24
+    gpu_renderer.upload(sc)
+    a = np.zeros((24, 32, 3), np.float32)
+    b = np.zeros((24, 32, 3), np.float32)
+    gpu_renderer.render_accumulate(a, cam, 32, 24, 8, max_depth=D, seed=4)
+    gpu_renderer.render_wavefront(b, cam, 32, 24, 8, max_depth=D, seed=4, capacity=4096, sort=sort)
+    assert np.allclose(a, b, rtol=1e-4, atol=1e-5), np.abs(a - b).max()
+
+
 def test_volume_glass_box_same_streams_and_three_sigma(gpu_renderer):
     """Dielectric box around a delta-tracked grid medium under the HDR sky."""
     sc, cam, cfg = scenes.volume_glass_box(32, 24, grid_n=16)
