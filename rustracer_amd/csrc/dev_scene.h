@@ -78,12 +78,15 @@ struct DScene {
     const float* env_rgb; const float* env_marg; const float* env_cond;
     const float* elut;
     float norms[3], fire_norms[3];
-    // photon map (kd order, 3 float4 per photon: pos+axis, power, normal)
     // photon map: left-balanced kd-tree in heap order, structure of arrays
     // (position + split axis in w, power, normal), 16 B per array per photon
     const float4* ph_pos; const float4* ph_pow; const float4* ph_nrm;
     uint32_t num_photons; float ph_r2, ph_norm;
     uint32_t ph_knn;                       // 0: fixed-radius gather; k: k-NN gather bounded by ph_r2
+    // the same photons in the k-NN bucket tree: leaf order records {pos, 0}
+    // {normal, 0} and powers, heap-order planes {value, axis bits}, leaf offsets
+    const float4* pk_rec; const float4* pk_pow; const float2* pk_split; const uint32_t* pk_leaf;
+    uint32_t pk_depth;
     uint32_t stack_bound;
 };
 

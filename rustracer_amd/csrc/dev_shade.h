@@ -988,7 +988,8 @@ RT_D V3 lights_generate(const DScene& S, V3 origin, Rng& rng, float time) {
 // times 2/(pi^2 R^2).  Range search of the heap-ordered kd-tree without a
 // stack: bit d of `pend` says the far child of the depth-d ancestor still
 // needs a visit; when a descent leaves the tree, the deepest pending ancestor
-// is recomputed from the index ((i-1)/2 per level) and its far child entered.
+// is recomputed from the index (in heap order the ancestor d levels up of
+// node i is ((i + 1) >> d) - 1) and its far child entered.
 // The Epanechnikov sum of kd_gather (src/photon.rs:195-228) over the
 // heap-ordered tree, before normalisation; `count` (optional) receives the
 // number of photons that passed the radius and hemisphere tests.
@@ -1002,10 +1003,10 @@ RT_D V3 photon_walk(const DScene& S, V3 pos, V3 nrm, float r2, uint32_t& count) 
     for (;;) {
         if (node < n) {
             float4 P = S.ph_pos[node];
+            float4 N = S.ph_nrm[node];
             float dx = P.x - pos.x, dy = P.y - pos.y, dz = P.z - pos.z;
             float d2 = dx * dx + dy * dy + dz * dz;
             if (d2 < r2) {
-                float4 N = S.ph_nrm[node];
                 float dt = N.x * nrm.x + N.y * nrm.y + N.z * nrm.z;
                 if (dt > 0.0f) {
                     if (SUM) { float4 W = S.ph_pow[node]; acc += mk(W.x, W.y, W.z) * (1.0f - d2 / r2); }
@@ -1022,8 +1023,7 @@ RT_D V3 photon_walk(const DScene& S, V3 pos, V3 nrm, float r2, uint32_t& count) 
         if (pend == 0) break;
         int d = 31 - __clz(pend);
         pend &= ~(1u << d);
-        uint32_t anc = node;
-        for (int k = depth; k > d; k--) anc = (anc - 1) >> 1;
+        uint32_t anc = ((node + 1) >> (depth - d)) - 1;      // heap order: the ancestor's index + 1 is a prefix of node + 1
         float4 P = S.ph_pos[anc];
         int axis = (int)P.w;
         float sq = comp(pos, axis) - (axis == 0 ? P.x : (axis == 1 ? P.y : P.z));
@@ -1035,81 +1035,116 @@ RT_D V3 photon_walk(const DScene& S, V3 pos, V3 nrm, float r2, uint32_t& count) 
 }
 RT_D V3 photon_gather(const DScene& S, V3 pos, V3 nrm, float r2) { uint32_t c; return photon_walk<true>(S, pos, nrm, r2, c); }
 
-// k-NN gather radius (extension): the k-th smallest squared distance among the
-// photons on the query's hemisphere closer than the gather radius, kept in a
-// max-heap of k floats in this lane's LDS column.  The walk is the gather's,
-// with the pruning radius shrinking to the heap's root once k are held; a
-// far child queued under a larger radius is re-tested when it is popped.
-// Fewer than k photons (or a zero k-th distance): the gather radius.  Until
-// k are held the walk is exactly the gather's at that radius, so `acc`
-// carries its sum and `count` < k says it is the answer.
-RT_D float photon_knn_r2(const DScene& S, V3 pos, V3 nrm, float* heap, uint32_t stride, float bound0, V3& acc,
-                         uint32_t& count) {
-    uint32_t n = S.num_photons, k = S.ph_knn;
-    float bound = bound0;
-    uint32_t cnt = 0;
-    acc = mk(0, 0, 0);
-    uint32_t node = 0, pend = 0;
-    int depth = 0;
-    bool skip = false;     // the popped far child was pruned by the shrunken radius
+// k-NN gather (extension) over the bucket tree: a complete binary tree of
+// pk_depth levels of splitting planes (heap order, {plane, axis bits}) whose
+// leaves hold at most RT_PK_LEAF photons stored contiguously (position and
+// normal in one 32-B record, power apart).  A leaf's photons load
+// independently of one another, so a lane has them in flight together rather
+// than one dependent load per photon as in the photon-per-node tree; the
+// plane array is small and stays in cache.
+#ifndef RT_PK_LEAF
+#define RT_PK_LEAF 8
+#endif
+
+// Push d2 into this lane's max-heap of k floats (its LDS column, `stride`
+// apart); once k are held, replace the root when d2 is smaller.
+RT_D void knn_heap_add(float* heap, uint32_t stride, uint32_t k, uint32_t& cnt, float d2) {
+    if (cnt < k) {                               // push, sift up
+        uint32_t i = cnt++;
+        while (i > 0) {
+            uint32_t p = (i - 1) >> 1;
+            float hp = heap[p * stride];
+            if (hp >= d2) break;
+            heap[i * stride] = hp;
+            i = p;
+        }
+        heap[i * stride] = d2;
+        return;
+    }
+    uint32_t i = 0;                              // replace the root, sift down
     for (;;) {
-        if (node < n && !skip) {
-            float4 P = S.ph_pos[node];
-            float dx = P.x - pos.x, dy = P.y - pos.y, dz = P.z - pos.z;
-            float d2 = dx * dx + dy * dy + dz * dz;
-            if (d2 < bound) {
-                float4 N = S.ph_nrm[node];
-                if (N.x * nrm.x + N.y * nrm.y + N.z * nrm.z > 0.0f) {
-                    if (cnt < k) { float4 W = S.ph_pow[node]; acc += mk(W.x, W.y, W.z) * (1.0f - d2 / bound); }
-                    if (cnt < k) {                       // push, sift up
-                        uint32_t i = cnt++;
-                        while (i > 0) {
-                            uint32_t p = (i - 1) >> 1;
-                            float hp = heap[p * stride];
-                            if (hp >= d2) break;
-                            heap[i * stride] = hp;
-                            i = p;
-                        }
-                        heap[i * stride] = d2;
-                    } else {                             // replace the root, sift down
-                        uint32_t i = 0;
-                        for (;;) {
-                            uint32_t l = 2 * i + 1;
-                            if (l >= k) break;
-                            uint32_t c = l;
-                            float hc = heap[l * stride];
-                            if (l + 1 < k) { float hr = heap[(l + 1) * stride]; if (hr > hc) { c = l + 1; hc = hr; } }
-                            if (hc <= d2) break;
-                            heap[i * stride] = hc;
-                            i = c;
-                        }
-                        heap[i * stride] = d2;
-                    }
-                    if (cnt == k) bound = heap[0];
-                }
-            }
-            int axis = (int)P.w;
-            float sq = comp(pos, axis) - (axis == 0 ? P.x : (axis == 1 ? P.y : P.z));
+        uint32_t l = 2 * i + 1;
+        if (l >= k) break;
+        uint32_t c = l;
+        float hc = heap[l * stride];
+        if (l + 1 < k) { float hr = heap[(l + 1) * stride]; if (hr > hc) { c = l + 1; hc = hr; } }
+        if (hc <= d2) break;
+        heap[i * stride] = hc;
+        i = c;
+    }
+    heap[i * stride] = d2;
+}
+
+// Range walk of the bucket tree around `pos` over the photons on the query's
+// hemisphere.  KNN = false: the Epanechnikov sum within r2 (the gather of
+// src/photon.rs:195-228 in another order).  KNN = true: additionally keep the
+// k smallest squared distances in the heap, the pruning radius shrinking to
+// the heap's root once k are held (a far child queued under a larger radius
+// is re-tested when it is popped); the sum then covers only the first k
+// photons found, which is the whole gather when fewer than k lie within r2.
+// Stackless: bit d of `pend` marks the far child of the depth-d plane as
+// still to visit; the ancestor d levels up of node i is ((i + 1) >> d) - 1.
+template <bool KNN>
+RT_D V3 pk_walk(const DScene& S, V3 pos, V3 nrm, float r2, float* heap, uint32_t stride, uint32_t& count, float& bound_out) {
+    const uint32_t D = S.pk_depth, k = S.ph_knn;
+    const uint32_t first_leaf = (1u << D) - 1u;
+    float bound = r2;
+    uint32_t cnt = 0;
+    V3 acc = mk(0, 0, 0);
+    uint32_t node = 0, pend = 0, depth = 0;
+    for (;;) {
+        while (depth < D) {                                  // descend to the near leaf
+            float2 sp = S.pk_split[node];
+            float sq = comp(pos, __float_as_int(sp.y)) - sp.x;
             if (sq * sq < bound) pend |= 1u << depth; else pend &= ~(1u << depth);
             node = sq <= 0.0f ? 2 * node + 1 : 2 * node + 2;
             depth++;
-            continue;
         }
-        skip = false;
-        if (pend == 0) break;
-        int d = 31 - __clz(pend);
-        pend &= ~(1u << d);
-        uint32_t anc = node;
-        for (int j = depth; j > d; j--) anc = (anc - 1) >> 1;
-        float4 P = S.ph_pos[anc];
-        int axis = (int)P.w;
-        float sq = comp(pos, axis) - (axis == 0 ? P.x : (axis == 1 ? P.y : P.z));
-        node = sq <= 0.0f ? 2 * anc + 2 : 2 * anc + 1;
-        depth = d + 1;
-        skip = !(sq * sq < bound);
+        uint32_t leaf = node - first_leaf;
+        uint32_t b = S.pk_leaf[leaf], e = S.pk_leaf[leaf + 1];
+        for (uint32_t i = b; i < e; i++) {
+            float4 P = S.pk_rec[2 * i], N = S.pk_rec[2 * i + 1];
+            float dx = P.x - pos.x, dy = P.y - pos.y, dz = P.z - pos.z;
+            float d2 = dx * dx + dy * dy + dz * dz;
+            if (d2 < bound && N.x * nrm.x + N.y * nrm.y + N.z * nrm.z > 0.0f) {
+                if (!KNN || cnt < k) { float4 W = S.pk_pow[i]; acc += mk(W.x, W.y, W.z) * (1.0f - d2 / bound); }
+                if (KNN) {
+                    knn_heap_add(heap, stride, k, cnt, d2);
+                    if (cnt == k) bound = heap[0];
+                } else {
+                    cnt++;
+                }
+            }
+        }
+        bool more = false;                                   // pop the deepest pending far child
+        while (pend) {
+            uint32_t d = 31u - (uint32_t)__clz(pend);
+            pend &= ~(1u << d);
+            uint32_t anc = ((node + 1) >> (depth - d)) - 1;
+            float2 sp = S.pk_split[anc];
+            float sq = comp(pos, __float_as_int(sp.y)) - sp.x;
+            if (KNN && !(sq * sq < bound)) continue;
+            node = sq <= 0.0f ? 2 * anc + 2 : 2 * anc + 1;
+            depth = d + 1;
+            more = true;
+            break;
+        }
+        if (!more) break;
     }
     count = cnt;
-    if (cnt < k) return S.ph_r2;
+    bound_out = bound;
+    return acc;
+}
+
+// k-NN gather radius: the k-th smallest squared distance among the photons on
+// the query's hemisphere closer than the gather radius.  Fewer than k (or a
+// zero k-th distance): the gather radius, and `acc` / `count` < k carry that
+// gather's sum.
+RT_D float photon_knn_r2(const DScene& S, V3 pos, V3 nrm, float* heap, uint32_t stride, float bound0, V3& acc,
+                         uint32_t& count) {
+    float bound;
+    acc = pk_walk<true>(S, pos, nrm, bound0, heap, stride, count, bound);
+    if (count < S.ph_knn) return S.ph_r2;
     float r2 = heap[0];
     return r2 > 0.0f ? r2 : S.ph_r2;
 }
@@ -1122,7 +1157,8 @@ RT_D V3 photon_knn_irradiance(const DScene& S, V3 pos, V3 nrm, float* heap, uint
     float r2 = photon_knn_r2(S, pos, nrm, heap, stride, S.ph_r2, acc, cnt);
     if (cnt < S.ph_knn) return acc * S.ph_norm;
     float norm = r2 == S.ph_r2 ? S.ph_norm : 2.0f / (PI * PI * r2);
-    return photon_gather(S, pos, nrm, r2) * norm;
+    float unused;
+    return pk_walk<false>(S, pos, nrm, r2, nullptr, 0, cnt, unused) * norm;
 }
 
 // PhotonMap::irradiance (src/photon.rs:148-157).  A k-NN map is gathered here

@@ -440,6 +440,14 @@ __device__ __forceinline__ uint32_t extend_one(const DScene& S, WfPath& w, uint3
     return sort_key(S, id, sort);
 }
 
+// Whether the path's new hit takes a k-NN photon estimate in shade: a surface
+// (not a medium event) whose material can take the diffuse branch.
+__device__ __forceinline__ bool knn_gathers(const DScene& S, const WfPath& w) {
+    if (S.ph_knn == 0 || S.num_photons == 0 || w.hit_slot < 0 || w.hit_prim == -2) return false;
+    uint32_t kind = S.mats[S.objs[w.hit_slot].material].kind;
+    return kind == M_LAMBERTIAN || kind == M_PBR || kind == M_PEARL;
+}
+
 // OCC = minimum waves per SIMD the register allocator must allow (0: its own choice).
 template <int OCC, bool BATCH>
 __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(OCC > 0 ? OCC : 1, 8))) void k_wf_extend(DScene S, WfQueues Q, WfBatch B) {
@@ -530,9 +538,10 @@ __global__ __launch_bounds__(BLOCK) void k_wf_shade(DScene S, DParams P, WfQueue
 }
 
 // k-NN photon maps: between extend and shade, the photon irradiance of every
-// hit whose material can take the diffuse branch (Lambertian, PBR), into the
-// path record, so the shade kernel keeps its occupancy (the heap needs 32 KB
-// of LDS per block).  Paths whose hit does not gather are left as they are.
+// hit whose material can take the diffuse branch (Lambertian, PBR, Pearl),
+// into the path record, so the shade kernel keeps its occupancy (the heap
+// needs 32 KB of LDS per block).  Paths whose hit does not gather are left as
+// they are.
 template <bool BATCH>
 __global__ __launch_bounds__(BLOCK) void k_wf_knn(DScene S0, DParams P0, WfQueues Q, WfBatch B) {
     float* heap = LdsHeap::column();
@@ -546,9 +555,7 @@ __global__ __launch_bounds__(BLOCK) void k_wf_knn(DScene S0, DParams P0, WfQueue
         WfPath& w = Q.paths[pid];
         const DScene& S = BATCH ? B.scenes[w.pad0] : S0;
         bool photons = BATCH ? B.views[w.pad0].P.use_photons != 0 : P0.use_photons != 0;
-        if (!photons || S.ph_knn == 0 || S.num_photons == 0 || w.hit_slot < 0 || w.hit_prim == -2) continue;
-        uint32_t kind = S.mats[S.objs[w.hit_slot].material].kind;
-        if (kind != M_LAMBERTIAN && kind != M_PBR && kind != M_PEARL) continue;
+        if (!photons || !knn_gathers(S, w)) continue;
         Ray r = make_ray(mk(w.o_time.x, w.o_time.y, w.o_time.z), mk(w.d_wl.x, w.d_wl.y, w.d_wl.z), w.o_time.w);
         r.wavelength = w.d_wl.w; r.spectral = (w.flags & 1u) != 0;
         HitId id; id.t = w.hit_t; id.slot = w.hit_slot; id.prim = w.hit_prim;
@@ -1663,8 +1670,6 @@ static int wf_buffers(rt_ctx* ctx, uint32_t cap, WfQueues& Q) {
     return RT_OK;
 }
 
-// Launch one chunk: generate `n` paths at frame position `first` (~0: read
-// from the counters, for graph replays) and run every bounce.
 template <bool BT>
 static void launch_extend(int occ, uint32_t grid, hipStream_t st, const DScene& S, const WfQueues& Q, const WfBatch& B) {
     if (occ >= 5) hipLaunchKernelGGL((k_wf_extend<5, BT>), dim3(grid), dim3(BLOCK), 0, st, S, Q, B);
@@ -2160,22 +2165,79 @@ static void heap_build(PhRec* src, uint32_t n, uint32_t node, float4* pos, float
     heap_build(src + L + 1, n - L - 1, 2 * node + 2, pos, pw, nr);
 }
 
-// Heap-order the photons (position, power, normal per photon), replace the
-// scene's map on the device and keep the unordered set for export.
+// k-NN bucket tree: depth-D complete tree of planes over the photon range;
+// each plane splits its range at the middle on the widest axis of the range's
+// bounds (photons left of the middle have a coordinate <= the plane, the rest
+// >=), and the 2^D leaves keep at most RT_PK_LEAF photons each, in place.
+static void pk_build(PhRec* src, uint32_t n, uint32_t base, uint32_t node, uint32_t depth, uint32_t D,
+                     float2* split, uint32_t* leaf_off) {
+    if (depth == D) { leaf_off[node - ((1u << D) - 1u)] = base; return; }
+    float mn[3] = {FINF, FINF, FINF}, mx[3] = {-FINF, -FINF, -FINF};
+    for (uint32_t i = 0; i < n; i++) {
+        float p[3] = {src[i].a.x, src[i].a.y, src[i].a.z};
+        for (int k = 0; k < 3; k++) { if (p[k] < mn[k]) mn[k] = p[k]; if (p[k] > mx[k]) mx[k] = p[k]; }
+    }
+    int axis = 0;
+    for (int a = 1; a < 3; a++) if (!((mx[a] - mn[a]) < (mx[axis] - mn[axis]))) axis = a;
+    uint32_t mid = n / 2;
+    auto key = [axis](const PhRec& p) { return axis == 0 ? p.a.x : (axis == 1 ? p.a.y : p.a.z); };
+    float plane = 0.0f;
+    if (n) {
+        std::nth_element(src, src + mid, src + n, [&](const PhRec& x, const PhRec& y) { return key(x) < key(y); });
+        plane = key(src[mid]);
+    }
+    int32_t ab = axis;
+    float af;
+    memcpy(&af, &ab, 4);
+    split[node] = make_float2(plane, af);
+    pk_build(src, mid, base, 2 * node + 1, depth + 1, D, split, leaf_off);
+    pk_build(src + mid, n - mid, base + mid, 2 * node + 2, depth + 1, D, split, leaf_off);
+}
+
+// Heap-order the photons (position, power, normal per photon), lay the same
+// photons out as the k-NN bucket tree, replace the scene's map on the device
+// and keep the unordered set for export.
 static int install_photons(rt_ctx* ctx, rt_scene* s, std::vector<float4>& kept, float r) {
     size_t m = kept.size() / 3;
     std::vector<PhRec> recs(m);
     for (size_t i = 0; i < m; i++) recs[i] = PhRec{kept[3 * i], kept[3 * i + 1], kept[3 * i + 2]};
-    std::vector<float4> heap(3 * m + 3);
-    heap_build(recs.data(), (uint32_t)m, 0, heap.data(), heap.data() + m, heap.data() + 2 * m);
+    uint32_t D = 0;
+    while ((size_t)RT_PK_LEAF << D < m) D++;
+    if (D > 30) return fail(ctx, RT_ERR_UNSUPPORTED, "photon map too large");
+    size_t nsplit = ((size_t)1 << D) - 1, nleaf = (size_t)1 << D;
+    // one block: heap-order pos/pow/nrm (3m), bucket records (2m) and powers (m), planes, leaf offsets
+    size_t f4 = 6 * m + 3;
+    size_t bytes = f4 * sizeof(float4) + ((nsplit * sizeof(float2) + 15) & ~(size_t)15) + (nleaf + 1) * sizeof(uint32_t);
+    std::vector<char> blob(bytes);
+    float4* heap = (float4*)blob.data();
+    {
+        std::vector<PhRec> tmp(recs);
+        heap_build(tmp.data(), (uint32_t)m, 0, heap, heap + m, heap + 2 * m);
+    }
+    float4* pk_rec = heap + 3 * m;
+    float4* pk_pow = heap + 5 * m;
+    float2* pk_split = (float2*)(blob.data() + f4 * sizeof(float4));
+    uint32_t* pk_leaf = (uint32_t*)(blob.data() + f4 * sizeof(float4) + ((nsplit * sizeof(float2) + 15) & ~(size_t)15));
+    pk_build(recs.data(), (uint32_t)m, 0, 0, 0, D, pk_split, pk_leaf);
+    pk_leaf[nleaf] = (uint32_t)m;
+    for (size_t i = 0; i < m; i++) {
+        pk_rec[2 * i] = make_float4(recs[i].a.x, recs[i].a.y, recs[i].a.z, 0.0f);
+        pk_rec[2 * i + 1] = make_float4(recs[i].c.x, recs[i].c.y, recs[i].c.z, 0.0f);
+        pk_pow[i] = recs[i].b;
+    }
     if (s->ph_mem) { (void)hipFree(s->ph_mem); s->ph_mem = nullptr; }
     s->S.num_photons = 0;
-    HIPCHECK(ctx, hipMalloc(&s->ph_mem, heap.size() * sizeof(float4)));
-    HIPCHECK(ctx, hipMemcpy(s->ph_mem, heap.data(), heap.size() * sizeof(float4), hipMemcpyHostToDevice));
+    HIPCHECK(ctx, hipMalloc(&s->ph_mem, bytes));
+    HIPCHECK(ctx, hipMemcpy(s->ph_mem, blob.data(), bytes, hipMemcpyHostToDevice));
     const float4* dph = (const float4*)s->ph_mem;
     s->S.ph_pos = dph;
     s->S.ph_pow = dph + m;
     s->S.ph_nrm = dph + 2 * m;
+    s->S.pk_rec = dph + 3 * m;
+    s->S.pk_pow = dph + 5 * m;
+    s->S.pk_split = (const float2*)((const char*)s->ph_mem + f4 * sizeof(float4));
+    s->S.pk_leaf = (const uint32_t*)((const char*)s->ph_mem + f4 * sizeof(float4) + ((nsplit * sizeof(float2) + 15) & ~(size_t)15));
+    s->S.pk_depth = D;
     s->S.num_photons = (uint32_t)m;
     s->S.ph_r2 = r * r;
     s->S.ph_norm = 2.0f / (PI * PI * r * r);
