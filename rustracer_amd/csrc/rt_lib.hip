@@ -388,6 +388,22 @@ struct WfBatch {
     uint32_t nviews;
 };
 
+// Path j of a sample pass covers pixel tile_order_pixel(j): bands of 8 rows,
+// each walked in 8-column groups, row-major inside a group, so a wave's 64
+// camera rays leave an 8 x 8 block of the image (coherent traversal and
+// photon lookups) instead of a 64-pixel row.  A bijection of [0, W * H); the
+// pixel's random stream depends on the pixel, not on j.
+__device__ __forceinline__ uint32_t tile_order_pixel(uint32_t j, uint32_t W, uint32_t H) {
+    uint32_t band = j / (8u * W);
+    uint32_t hb = min(8u, H - 8u * band);
+    uint32_t k = j - band * 8u * W;
+    uint32_t grp = k / (8u * hb);
+    uint32_t gw = min(8u, W - 8u * grp);
+    uint32_t local = k - grp * 8u * hb;
+    uint32_t row = local / gw;
+    return (band * 8u + row) * W + grp * 8u + (local - row * gw);
+}
+
 template <bool BATCH>
 __global__ __launch_bounds__(BLOCK) void k_wf_generate(DCamera cam0, DParams P0, WfQueues Q, uint64_t first, uint32_t n, WfBatch B) {
     uint32_t i = blockIdx.x * BLOCK + threadIdx.x;
@@ -406,7 +422,7 @@ __global__ __launch_bounds__(BLOCK) void k_wf_generate(DCamera cam0, DParams P0,
         g -= V.first;
     }
     uint32_t npix = P.width * P.height;
-    uint32_t pixel = (uint32_t)(g % npix);
+    uint32_t pixel = tile_order_pixel((uint32_t)(g % npix), P.width, P.height);
     uint32_t sidx = P.sample_idx + (uint32_t)(g / npix);
     uint32_t row = pixel / P.width, col = pixel - row * P.width;
     Rng rng = rng_seed(pixel_seed(pixel, sidx, P.seed));
