@@ -8,7 +8,7 @@ library; `Renderer()` does, and raises if it is missing or no GPU is present.
 """
 from .scene import (Background, Box, Camera, Checker, ConstantMedium, Convex, Diamond, Dielectric,  # noqa: F401
                     DiffuseLight, GridMedium, Lambertian, Mesh, Metal, PbrMaterial, Quad, RoughDielectric,
-                    Scene, SpectralDielectric, SpectralMetal, Sphere)
+                    Scene, SpectralDielectric, SpectralMetal, Sphere, StepCut)
 from .renderer import Renderer, RtError  # noqa: F401
 
 __all__ = ["Renderer", "RtError", "Scene", "Camera", "Background"]

@@ -603,6 +603,193 @@ def Diamond(center, radius, mat, table_pct=53.0, crown_angle=34.5, pavilion_angl
                   bbox=(np.asarray(bb[0], np.float32), np.asarray(bb[1], np.float32)))
 
 
+def step_outline(kind, a, b, corner_pct=12.0, segs=4, pear=False, points=None):
+    """StepOutline::vertices (src/step_cut.rs:37-108): the girdle outline in
+    XZ relative to the centre, from the semi-extents a (X) and b (Z), sorted
+    by angle about the vertex centroid (convex outlines come out CCW).
+    kind: "rect" (45-degree corner chamfers of corner_pct % of min(a, b); 0:
+    sharp), "rounded_rect" (quarter arcs of `segs` segments), "triangle",
+    "hexagon", "pointed_oval" (|cos|^p pinched ellipse, 2 * max(segs, 4)
+    vertices; `pear` keeps the -X end round) or "polygon" (`points`)."""
+    a, b = f32(a), f32(b)
+    mn = min(a, b)
+    if kind == "rect":
+        c = min(f32(corner_pct) / f32(100.0) * mn, f32(0.45) * mn)
+        if c <= f32(1e-6):
+            v = [(a, b), (a, -b), (-a, -b), (-a, b)]
+        else:
+            v = [(a - c, b), (a, b - c), (a, -(b - c)), (a - c, -b),
+                 (-(a - c), -b), (-a, -(b - c)), (-a, b - c), (-(a - c), b)]
+    elif kind == "rounded_rect":
+        r = min(f32(corner_pct) / f32(100.0) * mn, f32(0.45) * mn)
+        sg = max(int(segs), 1)
+        v = []
+        for sx in (f32(1.0), f32(-1.0)):
+            for sz in (f32(1.0), f32(-1.0)):
+                cx, cz = sx * (a - r), sz * (b - r)
+                for j in range(sg + 1):
+                    ang = f32(j) / f32(sg) * f32(math.pi / 2)
+                    v.append((cx + sx * r * f32(math.cos(ang)), cz + sz * r * f32(math.sin(ang))))
+    elif kind == "triangle":
+        v = []
+        for k in range(3):
+            th = f32(math.pi / 2) + f32(k) * f32(2.0) * f32(math.pi) / f32(3.0)
+            v.append((a * f32(math.cos(th)), b * f32(math.sin(th))))
+    elif kind == "hexagon":
+        v = []
+        for k in range(6):
+            th = f32(k) * f32(math.pi) / f32(3.0)
+            v.append((a * f32(math.cos(th)), b * f32(math.sin(th))))
+    elif kind == "pointed_oval":
+        n = max(int(segs), 4) * 2
+        v = []
+        for i in range(n):
+            th = f32(2.0) * f32(math.pi) * f32(i) / f32(n)
+            c, sn = f32(math.cos(th)), f32(math.sin(th))
+            pw = (f32(0.6) if c >= 0 else f32(1.0)) if pear else f32(0.65)
+            sgn = f32(0.0) if c == 0 else f32(math.copysign(1.0, c))
+            v.append((a * sgn * f32(abs(c) ** pw), b * sn))
+    elif kind == "polygon":
+        v = [(f32(x), f32(z)) for x, z in points]
+    else:
+        raise ValueError(f"unknown step outline {kind!r}")
+    cx = f32(sum(float(q[0]) for q in v) / len(v))
+    cz = f32(sum(float(q[1]) for q in v) / len(v))
+    return sorted(v, key=lambda q: math.atan2(float(q[1] - cz), float(q[0] - cx)))
+
+
+def _check_outline(v, scale):
+    """validate_outline (src/step_cut.rs:192-232): drop repeated vertices
+    (wrap-around included), then require three or more vertices, no short
+    edge, non-zero area and a convex turn at every vertex."""
+    eps = 1e-6 * float(scale)
+    out = []
+    for q in v:
+        if out and math.hypot(float(q[0] - out[-1][0]), float(q[1] - out[-1][1])) < eps:
+            continue
+        out.append(q)
+    if len(out) > 1 and math.hypot(float(out[0][0] - out[-1][0]), float(out[0][1] - out[-1][1])) < eps:
+        out.pop()
+    n = len(out)
+    if n < 3:
+        raise ValueError("StepCut: outline needs at least 3 distinct vertices")
+    for i in range(n):
+        p, q = out[i], out[(i + 1) % n]
+        if math.hypot(float(p[0] - q[0]), float(p[1] - q[1])) < eps:
+            raise ValueError("StepCut: outline has a zero-length edge")
+    area2 = sum(float(out[i][0]) * float(out[(i + 1) % n][1]) - float(out[(i + 1) % n][0]) * float(out[i][1])
+                for i in range(n))
+    if abs(area2) < eps * eps:
+        raise ValueError("StepCut: outline has zero area")
+    for i in range(n):
+        p, q, r = out[i], out[(i + 1) % n], out[(i + 2) % n]
+        turn = float(q[0] - p[0]) * float(r[1] - q[1]) - float(q[1] - p[1]) * float(r[0] - q[0])
+        if turn * area2 < 0:
+            raise ValueError("StepCut: polygon outline must be convex")
+    return out
+
+
+def StepCut(center, half_a, half_b, mat, table_pct=65.0, crown_angle=35.0, pavilion_angle=42.0,
+            outline=("rect", {"corner_pct": 12.0}), crown_steps=3, pavilion_steps=3, culet_pct=8.0,
+            step_spread=0.25, chevrons=0, girdle_pct=0.0, polish=0.0):
+    """Step-family cut (emerald, princess, cushion, trillion, marquise ...) as
+    half-spaces, after StepCut::new (src/step_cut.rs:236-384); its hit is the
+    convex hit with the same cylindrical uv as Diamond (src/step_cut.rs:
+    388-405).  Per outline edge (outward normal m, offset h): an optional
+    vertical girdle band face, `crown_steps` crown rows and `pavilion_steps`
+    pavilion rows of equal rises whose runs follow row angles spread from
+    (1 + step_spread) x to (1 - step_spread) x the nominal tangent, scaled so
+    the runs add up to the table (culet) inset; `chevrons` diagonal rows per
+    outline vertex toward table and culet; then the table and culet caps.
+    The box is exact from the outline and the cap heights.  `outline` is
+    (kind, keywords of step_outline)."""
+    a, b = f32(half_a), f32(half_b)
+    if not (a > 0 and b > 0 and math.isfinite(a) and math.isfinite(b)):
+        raise ValueError("StepCut: half extents must be positive and finite")
+    c = v3(center)
+    deg = f32(math.pi) / f32(180.0)
+    k = min(max(f32(table_pct) / f32(100.0), f32(0.05)), f32(0.95))
+    kc = min(max(f32(culet_pct) / f32(100.0), f32(0.0)), f32(0.5))
+    s_in = b * (f32(1.0) - k)
+    sp_in = b * (f32(1.0) - kc)
+    tan_c = f32(math.tan(float(f32(crown_angle) * deg)))
+    tan_p = f32(math.tan(float(f32(pavilion_angle) * deg)))
+    crown_h = s_in * tan_c
+    pavilion_h = sp_in * tan_p
+    girdle_h = min(max(f32(girdle_pct) / f32(100.0) * f32(2.0) * b, f32(0.0)), f32(0.3) * b)
+    if not (math.isfinite(girdle_pct) and 0.0 <= girdle_pct < 100.0):
+        raise ValueError(f"StepCut: girdle_pct must be in [0, 100), got {girdle_pct}")
+    if not (crown_h > 0 and pavilion_h > 0):
+        raise ValueError(f"StepCut: crown/pavilion angles must be in (0, 90) degrees, got {crown_angle} / {pavilion_angle}")
+    kind, kw = outline if isinstance(outline, tuple) else (outline, {})
+    verts = _check_outline(step_outline(kind, a, b, **kw), max(a, b))
+    nv = len(verts)
+    gx = f32(sum(float(q[0]) for q in verts) / nv)
+    gz = f32(sum(float(q[1]) for q in verts) / nv)
+
+    def flat_unit(x, z):
+        ln = f32(math.sqrt(float(x * x + z * z)))
+        return (x / ln, z / ln) if ln > 1e-12 else (f32(0.0), f32(0.0))
+    edges = []
+    for i in range(nv):
+        p, q = verts[i], verts[(i + 1) % nv]
+        mx, mz = flat_unit(q[1] - p[1], -(q[0] - p[0]))
+        if mx * (p[0] - gx) + mz * (p[1] - gz) < 0:
+            mx, mz = -mx, -mz
+        edges.append((mx, mz, mx * p[0] + mz * p[1]))
+    cs, ps = max(int(crown_steps), 1), max(int(pavilion_steps), 1)
+    spread = min(max(f32(step_spread), f32(0.0)), f32(0.9))
+
+    def factor(i, rows):
+        return f32(1.0) if rows <= 1 else f32(1.0) + spread * (f32(1.0) - f32(2.0) * f32(i) / f32(rows - 1))
+    dv, dvp = crown_h / f32(cs), pavilion_h / f32(ps)
+    norm_c = s_in / f32(sum(float(dv / (tan_c * factor(i, cs))) for i in range(cs)))
+    norm_p = sp_in / f32(sum(float(dvp / (tan_p * factor(i, ps))) for i in range(ps)))
+    planes = []
+
+    def add(n, pt):
+        n = np.asarray(n, np.float32)
+        planes.append([n[0], n[1], n[2], f32(np.dot(n, np.asarray(pt, np.float32)))])
+    for mx, mz, h in edges:
+        if girdle_h > 0:
+            planes.append([mx, f32(0.0), mz, mx * c[0] + mz * c[2] + h])
+        for rows, tanv, dvv, nrm, sgn in ((cs, tan_c, dv, norm_c, f32(1.0)), (ps, tan_p, dvp, norm_p, f32(-1.0))):
+            dist, y = h, sgn * f32(0.5) * girdle_h
+            for i in range(rows):
+                dh = dvv / (tanv * factor(i, rows)) * nrm
+                add(unit([mx * dvv, sgn * dh, mz * dvv]), [c[0] + mx * dist, c[1] + y, c[2] + mz * dist])
+                dist = dist - dh
+                y = y + sgn * dvv
+    for i in range(nv):
+        v, vp, vn = verts[i], verts[(i + nv - 1) % nv], verts[(i + 1) % nv]
+        u1, u2 = flat_unit(vp[0] - v[0], vp[1] - v[1]), flat_unit(vn[0] - v[0], vn[1] - v[1])
+        bis = (u1[0] + u2[0], u1[1] + u2[1])
+        if bis[0] * bis[0] + bis[1] * bis[1] < 1e-12:
+            continue
+        c3 = np.array([c[0] + v[0], c[1], c[2] + v[1]], np.float32)
+        d3 = np.array([-bis[1], 0.0, bis[0]], np.float32)
+        tc = np.array([c[0] + v[0] * k, c[1] + f32(0.5) * girdle_h + crown_h, c[2] + v[1] * k], np.float32)
+        cc = np.array([c[0] + v[0] * kc, c[1] - f32(0.5) * girdle_h - pavilion_h, c[2] + v[1] * kc], np.float32)
+        for j in range(int(chevrons)):
+            fj = f32(j + 1) / f32(int(chevrons))
+            for pt in (c3 + (tc - c3) * fj, c3 + (cc - c3) * fj):
+                n = unit(cross(d3, pt - c3))
+                if np.dot(n, c) > np.dot(n, c3):
+                    n = -n
+                add(n, c3)
+    planes.append([f32(0.0), f32(1.0), f32(0.0), c[1] + f32(0.5) * girdle_h + crown_h])
+    planes.append([f32(0.0), f32(-1.0), f32(0.0), -(c[1] - f32(0.5) * girdle_h - pavilion_h)])
+    top = f32(0.5) * girdle_h + s_in * tan_c
+    bottom = f32(0.5) * girdle_h + sp_in * tan_p
+    xs = [q[0] for q in verts]
+    zs = [q[1] for q in verts]
+    bb = (np.array([c[0] + min(xs), c[1] - bottom, c[2] + min(zs)], np.float32),
+          np.array([c[0] + max(xs), c[1] + top, c[2] + max(zs)], np.float32))
+    pol = min(max(float(polish), 0.0), 0.25) if math.isfinite(polish) else 0.0
+    return Convex(np.array(planes, np.float32), mat, center=c, crown_h=crown_h, pavilion_h=pavilion_h,
+                  girdle_h=girdle_h, polish=pol, bbox=bb)
+
+
 class ConstantMedium:
     """src/volume.rs:78-156: wraps a boundary shape (its material is ignored)."""
 

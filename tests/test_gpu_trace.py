@@ -124,3 +124,23 @@ def test_noise_medium_events_and_transmittance(gpu_renderer):
     assert both.sum() > 2000
     assert np.allclose(hg["t"][both], hc["t"][both], rtol=1e-3, atol=1e-4)
     assert np.allclose(gpu_renderer.transmittance(rays), orc.transmittance(rays), rtol=1e-4, atol=1e-6)
+
+
+def test_step_cuts_bit_exact(gpu_renderer):
+    """StepCut family (src/step_cut.rs) as convex half-spaces: emerald with a
+    girdle band, princess with chevrons, cushion, trillion, marquise with
+    polish, under transforms."""
+    w = S.Lambertian((0.7, 0.7, 0.7))
+    g = S.Dielectric(1.6)
+    objs = [
+        S.Quad((-3, -1, -3), (6, 0, 0), (0, 0, 6), w),
+        S.StepCut((-1.2, 0.0, -1.0), 0.6, 0.4, g, girdle_pct=3.0),
+        S.StepCut((0.0, 0.0, -1.0), 0.45, 0.45, g, outline=("rect", {"corner_pct": 0.0}), chevrons=2).rotated((0, 30, 0)),
+        S.StepCut((1.2, 0.0, -1.0), 0.5, 0.4, g, outline=("rounded_rect", {"corner_pct": 30.0, "segs": 4})),
+        S.StepCut((-1.0, 0.0, 0.8), 0.5, 0.5, g, outline="triangle", table_pct=55.0),
+        S.StepCut((0.6, 0.2, 0.8), 0.7, 0.3, g, outline=("pointed_oval", {"segs": 12, "pear": True}), polish=0.05,
+                  step_spread=0.4).rotated((20, 0, 10)),
+    ]
+    sc = S.Scene(objs, [], S.Background.solid((0, 0, 0)), name="step_cuts")
+    hg = run_scene(gpu_renderer, sc, random_rays(200000, 21, lo=(-2, -1.0, -2), hi=(2, 1.5, 2)))
+    assert set(range(6)) <= set(np.unique(hg["obj_id"]))
