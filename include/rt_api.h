@@ -343,6 +343,33 @@ typedef struct {
 
 #define RT_KNN_MAX 64
 
+/* ---- per-pixel accumulation (src/main.rs:124-133, 603-677, 696-718) --------
+ * The firefly clamp and adaptive-sampling settings of the reference's render
+ * loop.  Defaults there: firefly_clamp 8, firefly_min_samples 4,
+ * firefly_abs_floor 5 (FireflyConfig::default), min_samples 128, threshold
+ * 0.05 (ADAPTIVE_THRESHOLD).                                                 */
+typedef struct {
+    float    firefly_clamp;        /* 0 disables the clamp                      */
+    uint32_t firefly_min_samples;  /* clamp once a pixel holds this many samples */
+    float    firefly_abs_floor;    /* threshold = max(clamp * mean luminance, floor) */
+    uint32_t adaptive;             /* 1: converged pixels stop being rendered   */
+    uint32_t min_samples;          /* adaptive: mark_converged from this many passes */
+    float    convergence_pct;      /* adaptive: stop once this % of pixels converged */
+    float    threshold;            /* relative standard error of every non-dark channel */
+    uint32_t reserved;
+} rt_accum_opts;
+
+/* The caller-owned per-pixel state of one image (width*height pixels): the
+ * accumulator (3 floats per pixel), the count of finite samples, the Welford
+ * M2 per channel (3 floats) and the convergence flags.  Host or device
+ * memory; the four may not mix host and device pointers.                     */
+typedef struct {
+    float    *accum;
+    uint32_t *samples;
+    float    *var_m2;
+    uint8_t  *converged;
+} rt_pixel_state;
+
 /* ---- context ---------------------------------------------------------------- */
 int         rt_ctx_create(int device, rt_ctx **out);
 void        rt_ctx_destroy(rt_ctx *ctx);
@@ -484,6 +511,37 @@ int  rt_photon_irradiance(rt_ctx *ctx, rt_scene *scene, const float *pos_nrm,
  * (position, power, normal); *count receives the map size.                   */
 int  rt_photon_export(rt_ctx *ctx, rt_scene *scene, float *out, uint32_t cap,
                       uint32_t *count);
+
+/* accumulate_sample (src/main.rs:603-644): add one sample pass (scratch_rgb,
+ * npix*3 floats) into the state.  Per pixel: converged pixels and samples
+ * with a non-finite channel are skipped; once the pixel holds
+ * firefly_min_samples samples a sample whose luminance exceeds
+ * max(firefly_clamp * mean luminance, firefly_abs_floor) is scaled down to
+ * it; then the sum, the count and the per-channel Welford M2 are updated.    */
+int  rt_accumulate_sample(rt_ctx *ctx, const float *scratch_rgb, uint32_t npix,
+                          const rt_accum_opts *opts, const rt_pixel_state *state);
+
+/* mark_converged (src/main.rs:646-677): flag every pixel with samples whose
+ * non-dark channels (mean >= 1e-4) all have a relative standard error below
+ * opts->threshold; *n_converged receives the number of flagged pixels.       */
+int  rt_mark_converged(rt_ctx *ctx, uint32_t npix, const rt_accum_opts *opts,
+                       const rt_pixel_state *state, uint32_t *n_converged);
+
+/* The reference's render loop (src/main.rs:463-506) on the device: for
+ * s = 0 .. samples_max-1 one pass of sample index params->sample_idx + s
+ * (render_tiles, converged pixels skipped when adaptive), accumulated as
+ * rt_accumulate_sample does, then, when adaptive and s + 1 >= min_samples,
+ * rt_mark_converged; the loop stops after a pass when adaptive and the
+ * converged count reaches convergence_pct % of the pixels.  Render,
+ * accumulation and the convergence test run fused per pixel, one launch per
+ * pass, with no host round trip: the stop test is read by the next launch.
+ * *samples_done receives the passes run, *n_converged the flagged pixels
+ * (either may be NULL).  The state is read and updated in place, so a
+ * later call continues it.                                                  */
+int  rt_render_progressive(rt_ctx *ctx, rt_scene *scene, const rt_camera *cam,
+                           const rt_sample_params *params, uint32_t samples_max,
+                           const rt_accum_opts *opts, const rt_pixel_state *state,
+                           uint32_t *samples_done, uint32_t *n_converged);
 
 /* ---- measurement (no counterpart in the reference: its --bench mode times
  * render_tiles with Instant, src/main.rs:68-117) ------------------------------ */

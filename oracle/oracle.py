@@ -59,6 +59,8 @@ def lib():
         L.rtc_photon_set_gather.argtypes = [vp, C.c_float, C.c_uint32]
         L.rtc_pbr_params.argtypes = [vp, vp, vp, C.c_uint32, vp]
         L.rtc_perlin.argtypes = [vp, vp, C.c_uint32, C.c_int, vp, vp]
+        L.rtc_accumulate_sample.argtypes = [vp, C.c_uint32, C.c_float, C.c_uint32, C.c_float, vp, vp, vp, vp]
+        L.rtc_mark_converged.argtypes = [C.c_uint32, C.c_float, vp, vp, vp, vp]
         _lib = L
     return _lib
 
@@ -328,3 +330,19 @@ def pbr_params(material, recs):
     if lib().rtc_pbr_params(C.byref(d.materials[mi]), C.byref(d), _p(recs), recs.shape[0], _p(out)) != 0:
         raise ValueError("not a PBR material")
     return out
+
+
+def accumulate_sample(scratch, state, firefly_clamp=8.0, firefly_min_samples=4, firefly_abs_floor=5.0):
+    """accumulate_sample (src/main.rs:603-644) into a state dict of numpy
+    arrays (accum (n,3) f32, samples (n,) u32, var_m2 (n,3) f32, converged
+    (n,) u8), in place."""
+    sc = np.ascontiguousarray(scratch, dtype=np.float32).reshape(-1, 3)
+    lib().rtc_accumulate_sample(_p(sc), len(sc), firefly_clamp, firefly_min_samples, firefly_abs_floor,
+                                _p(state["accum"]), _p(state["samples"]), _p(state["var_m2"]), _p(state["converged"]))
+    return state
+
+
+def mark_converged(state, threshold=0.05):
+    """mark_converged (src/main.rs:646-677) in place; returns the count."""
+    return lib().rtc_mark_converged(len(state["converged"]), threshold, _p(state["accum"]), _p(state["samples"]),
+                                    _p(state["var_m2"]), _p(state["converged"]))

@@ -779,4 +779,57 @@ int rtc_env_sample_mode(const float* rgb, uint32_t w, uint32_t h, float rot, int
     return 0;
 }
 
+// ---- per-pixel accumulation (src/main.rs:597-677) -------------------------------
+// accumulate_sample: per pixel, skip converged pixels and non-finite samples;
+// firefly clamp on luminance once old_n >= min_samples; sum, count and
+// per-channel Welford M2.  Color / f32 divides each component.
+static inline float acc_lum(float x, float y, float z) { return 0.2126f * x + 0.7152f * y + 0.0722f * z; }
+
+int rtc_accumulate_sample(const float* sc, uint32_t npix, float ff_clamp, uint32_t ff_min, float ff_floor,
+                          float* acc, uint32_t* ns, float* m2, const uint8_t* conv) {
+    for (uint32_t i = 0; i < npix; i++) {
+        if (conv[i]) continue;
+        float s[3] = {sc[3 * i], sc[3 * i + 1], sc[3 * i + 2]};
+        if (!std::isfinite(s[0]) || !std::isfinite(s[1]) || !std::isfinite(s[2])) continue;
+        float* a = acc + 3 * (size_t)i;
+        uint32_t old_n = ns[i];
+        float om[3] = {0.0f, 0.0f, 0.0f};
+        if (old_n > 0) for (int c = 0; c < 3; c++) om[c] = a[c] / (float)old_n;
+        float old_lum = acc_lum(om[0], om[1], om[2]);
+        if (ff_clamp > 0.0f && old_n >= ff_min) {
+            float l = acc_lum(s[0], s[1], s[2]);
+            float th = std::fmax(ff_clamp * old_lum, ff_floor);
+            if (l > th) { float f = th / l; for (int c = 0; c < 3; c++) s[c] = s[c] * f; }
+        }
+        for (int c = 0; c < 3; c++) a[c] = a[c] + s[c];
+        ns[i] = old_n + 1;
+        for (int c = 0; c < 3; c++) {
+            float nm = a[c] / (float)ns[i];
+            m2[3 * (size_t)i + c] = m2[3 * (size_t)i + c] + (s[c] - om[c]) * (s[c] - nm);
+        }
+    }
+    return 0;
+}
+
+// mark_converged: returns the number of converged pixels after the update.
+int rtc_mark_converged(uint32_t npix, float threshold, const float* acc, const uint32_t* ns, const float* m2, uint8_t* conv) {
+    int total = 0;
+    for (uint32_t i = 0; i < npix; i++) {
+        if (!conv[i] && ns[i] > 0) {
+            uint32_t n = ns[i];
+            bool ok = true;
+            for (int c = 0; c < 3 && ok; c++) {
+                float m = acc[3 * (size_t)i + c] / (float)(n > 1 ? n : 1);
+                if (m < 1e-4f) continue;
+                float var = m2[3 * (size_t)i + c] / (float)(n - 1 > 1 ? n - 1 : 1);
+                float se = std::sqrt(var / (float)n);
+                if (se / m >= threshold) ok = false;
+            }
+            if (ok) conv[i] = 1;
+        }
+        total += conv[i] ? 1 : 0;
+    }
+    return total;
+}
+
 }  // extern "C"

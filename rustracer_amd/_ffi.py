@@ -105,6 +105,16 @@ class rt_wavefront_opts(C.Structure):
                 ("occupancy", C.c_uint32)]
 
 
+class rt_accum_opts(C.Structure):
+    _fields_ = [("firefly_clamp", C.c_float), ("firefly_min_samples", C.c_uint32), ("firefly_abs_floor", C.c_float),
+                ("adaptive", C.c_uint32), ("min_samples", C.c_uint32), ("convergence_pct", C.c_float),
+                ("threshold", C.c_float), ("reserved", C.c_uint32)]
+
+
+class rt_pixel_state(C.Structure):
+    _fields_ = [("accum", C.c_void_p), ("samples", C.c_void_p), ("var_m2", C.c_void_p), ("converged", C.c_void_p)]
+
+
 class rt_batch_view(C.Structure):
     _fields_ = [("scene", C.c_void_p), ("camera", rt_camera), ("params", rt_sample_params), ("spp", C.c_uint32),
                 ("accum_rgb", C.c_void_p)]
@@ -163,6 +173,12 @@ SIGNATURES = {
     "rt_photon_set_gather": (C.c_int, [VP, VP, C.c_float, C.c_uint32]),
     "rt_photon_irradiance": (C.c_int, [VP, VP, VP, C.c_uint32, VP]),
     "rt_photon_export": (C.c_int, [VP, VP, VP, C.c_uint32, C.POINTER(C.c_uint32)]),
+    "rt_accumulate_sample": (C.c_int, [VP, VP, C.c_uint32, C.POINTER(rt_accum_opts), C.POINTER(rt_pixel_state)]),
+    "rt_mark_converged": (C.c_int, [VP, C.c_uint32, C.POINTER(rt_accum_opts), C.POINTER(rt_pixel_state),
+                                    C.POINTER(C.c_uint32)]),
+    "rt_render_progressive": (C.c_int, [VP, VP, C.POINTER(rt_camera), C.POINTER(rt_sample_params), C.c_uint32,
+                                        C.POINTER(rt_accum_opts), C.POINTER(rt_pixel_state), C.POINTER(C.c_uint32),
+                                        C.POINTER(C.c_uint32)]),
     "rt_ctx_set_timing": (C.c_int, [VP, C.c_int]),
     "rt_ctx_get_timing": (C.c_int, [VP, C.POINTER(C.c_double), C.POINTER(C.c_uint64)]),
     "rt_ctx_set_counting": (C.c_int, [VP, C.c_int]),

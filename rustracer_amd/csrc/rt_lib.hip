@@ -5,6 +5,7 @@
 // and the host code that owns contexts, uploads scenes and launches.
 #include "dev_shade.h"
 #include "dev_adjoint.h"
+#include "dev_accum.h"
 #include "bvh_build.h"
 #include "../../include/rt_api.h"
 
@@ -106,6 +107,67 @@ __global__ __launch_bounds__(BLOCK) void k_render(DScene S, DCamera cam, DParams
         atomicAdd(P.counters + 5, (unsigned long long)st.s_prims);
         atomicAdd(P.counters + 6, (unsigned long long)st.s_rays);
     }
+}
+
+// accumulate_sample (src/main.rs:603-644), one thread per pixel.
+__global__ __launch_bounds__(BLOCK) void k_accumulate(const float* __restrict__ sc, uint32_t npix, rt_accum_opts o,
+                                                      float* acc, uint32_t* ns, float* m2, const uint8_t* conv) {
+    uint32_t i = blockIdx.x * BLOCK + threadIdx.x;
+    if (i >= npix || conv[i]) return;
+    accumulate_pixel(acc + 3 * (size_t)i, ns + i, m2 + 3 * (size_t)i, mk(sc[3 * (size_t)i], sc[3 * (size_t)i + 1], sc[3 * (size_t)i + 2]), o);
+}
+
+// mark_converged (src/main.rs:646-677): flag, then count every flagged pixel.
+__global__ __launch_bounds__(BLOCK) void k_mark_converged(uint32_t npix, float threshold, const float* acc, const uint32_t* ns,
+                                                          const float* m2, uint8_t* conv, uint32_t* count) {
+    uint32_t i = blockIdx.x * BLOCK + threadIdx.x;
+    bool c = false;
+    if (i < npix) {
+        c = conv[i] != 0;
+        if (!c && pixel_converged(acc + 3 * (size_t)i, ns[i], m2 + 3 * (size_t)i, threshold)) { conv[i] = 1; c = true; }
+    }
+    unsigned long long b = __ballot(c);
+    if (__lane_id() == 0 && b) atomicAdd(count, (uint32_t)__popcll(b));
+}
+
+// Flagged pixels of a state handed to rt_render_progressive.
+__global__ __launch_bounds__(BLOCK) void k_count_flags(uint32_t npix, const uint8_t* conv, uint32_t* count) {
+    uint32_t i = blockIdx.x * BLOCK + threadIdx.x;
+    unsigned long long b = __ballot(i < npix && conv[i] != 0);
+    if (__lane_id() == 0 && b) atomicAdd(count, (uint32_t)__popcll(b));
+}
+
+// One pass of the reference's render loop (src/main.rs:463-506) fused per
+// pixel: render_tiles' sample, accumulate_sample, and (adaptive, from
+// min_samples passes on) mark_converged.  ctl[0] = stopped (set between
+// passes by k_progressive_step), ctl[1] = converged pixels, ctl[2] = passes run.
+template <bool KNN>
+__global__ __launch_bounds__(BLOCK) void k_progressive(DScene S, DCamera cam, DParams P, rt_accum_opts o, uint32_t pass,
+                                                       float* acc, uint32_t* ns, float* m2, uint8_t* conv, uint32_t* ctl) {
+    if (ctl[0]) return;
+    Stack st; init_stack(st);
+    if (KNN) st.heap = LdsHeap::column();
+    uint32_t i = blockIdx.x * BLOCK + threadIdx.x;
+    uint32_t npix = P.width * P.height;
+    if (i >= npix || conv[i]) return;
+    uint32_t row = i / P.width, col = i - row * P.width;
+    Rng rng = rng_seed(pixel_seed(i, P.sample_idx, P.seed));
+    Ray r = pixel_ray(cam, P, row, col, P.sample_idx, rng);
+    V3 c = ray_color(S, r, (int)P.max_depth, P.bg_scale, P.use_photons != 0, rng, st);
+    float* a = acc + 3 * (size_t)i;
+    float* v = m2 + 3 * (size_t)i;
+    accumulate_pixel(a, ns + i, v, c, o);
+    if (o.adaptive && pass + 1 >= o.min_samples && pixel_converged(a, ns[i], v, o.threshold)) {
+        conv[i] = 1;
+        atomicAdd(ctl + 1, 1u);
+    }
+}
+// After pass `pass`: count it, and stop when adaptive and the converged count
+// reaches convergence_pct % of the pixels (the reference's target_reached).
+__global__ void k_progressive_step(uint32_t* ctl, uint32_t pass, uint32_t npix, rt_accum_opts o) {
+    if (threadIdx.x != 0 || ctl[0]) return;
+    ctl[2] = pass + 1;
+    if (o.adaptive && (float)ctl[1] >= (float)npix * o.convergence_pct / 100.0f) ctl[0] = 1;
 }
 
 // Detached-sampling forward render: the k_render frame with BSDF values
@@ -844,6 +906,7 @@ struct rt_ctx {
     size_t ext_used = 0;
     double extend_ms = 0.0;
     uint64_t extend_launches = 0;
+    uint32_t* prog_ctl = nullptr;             // rt_render_progressive: stopped, converged count, passes run
 };
 
 struct rt_scene {
@@ -1002,6 +1065,7 @@ void rt_ctx_destroy(rt_ctx* ctx) {
     if (ctx->elut) (void)hipFree(ctx->elut);
     if (ctx->counters) (void)hipFree(ctx->counters);
     if (ctx->wf_mem) (void)hipFree(ctx->wf_mem);
+    if (ctx->prog_ctl) (void)hipFree(ctx->prog_ctl);
     for (hipEvent_t e : ctx->ext_ev) (void)hipEventDestroy(e);
     if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
     if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
@@ -1650,6 +1714,126 @@ int rt_render_sample(rt_ctx* ctx, rt_scene* s, const rt_camera* cam, const rt_sa
 
 int rt_render_accumulate(rt_ctx* ctx, rt_scene* s, const rt_camera* cam, const rt_sample_params* p, uint32_t spp, float* out) {
     return render_common(ctx, s, cam, p, nullptr, out, spp, true);
+}
+
+// The four arrays of a pixel state on the device (staged when the caller
+// passed host memory, copied back by state_finish).
+struct StateDev {
+    Staged acc, ns, m2, conv;
+    size_t npix = 0;
+};
+static int state_stage(rt_ctx* ctx, const rt_pixel_state* st, uint32_t npix, StateDev& d) {
+    if (!st || !st->accum || !st->samples || !st->var_m2 || !st->converged) return fail(ctx, RT_ERR_INVALID, "pixel state with a null array");
+    bool dev = is_device_ptr(st->accum);
+    if (is_device_ptr(st->samples) != dev || is_device_ptr(st->var_m2) != dev || is_device_ptr(st->converged) != dev)
+        return fail(ctx, RT_ERR_INVALID, "pixel state mixes host and device arrays");
+    d.npix = npix;
+    int rc;
+    if ((rc = stage_out(ctx, st->accum, (size_t)npix * 12, d.acc, true))) return rc;
+    if ((rc = stage_out(ctx, st->samples, (size_t)npix * 4, d.ns, true))) return rc;
+    if ((rc = stage_out(ctx, st->var_m2, (size_t)npix * 12, d.m2, true))) return rc;
+    if ((rc = stage_out(ctx, st->converged, (size_t)npix, d.conv, true))) return rc;
+    return RT_OK;
+}
+static int state_finish(rt_ctx* ctx, const rt_pixel_state* st, StateDev& d) {
+    int rc;
+    if ((rc = finish_out(ctx, st->accum, d.npix * 12, d.acc))) return rc;
+    if ((rc = finish_out(ctx, st->samples, d.npix * 4, d.ns))) return rc;
+    if ((rc = finish_out(ctx, st->var_m2, d.npix * 12, d.m2))) return rc;
+    return finish_out(ctx, st->converged, d.npix, d.conv);
+}
+static int check_accum_opts(rt_ctx* ctx, const rt_accum_opts* o) {
+    if (!o) return fail(ctx, RT_ERR_INVALID, "null accumulation options");
+    if (!std::isfinite(o->firefly_clamp) || !std::isfinite(o->firefly_abs_floor) || !std::isfinite(o->threshold) ||
+        !std::isfinite(o->convergence_pct))
+        return fail(ctx, RT_ERR_INVALID, "accumulation options must be finite");
+    return RT_OK;
+}
+static int ctl_buffer(rt_ctx* ctx) {
+    if (!ctx->prog_ctl) HIPCHECK(ctx, hipMalloc((void**)&ctx->prog_ctl, 4 * sizeof(uint32_t)));
+    return RT_OK;
+}
+
+int rt_accumulate_sample(rt_ctx* ctx, const float* sc, uint32_t npix, const rt_accum_opts* o, const rt_pixel_state* st) {
+    if (!ctx || (!sc && npix)) return RT_ERR_INVALID;
+    int rc;
+    if ((rc = check_accum_opts(ctx, o))) return rc;
+    if (npix == 0) return RT_OK;
+    HIPCHECK(ctx, hipSetDevice(ctx->device));
+    Staged in;
+    StateDev d;
+    if ((rc = stage_in(ctx, sc, (size_t)npix * 12, in))) return rc;
+    if ((rc = state_stage(ctx, st, npix, d))) return rc;
+    hipLaunchKernelGGL(k_accumulate, dim3((npix + BLOCK - 1) / BLOCK), dim3(BLOCK), 0, ctx->stream, (const float*)in.dev, npix, *o,
+                       (float*)d.acc.dev, (uint32_t*)d.ns.dev, (float*)d.m2.dev, (const uint8_t*)d.conv.dev);
+    HIPCHECK(ctx, hipGetLastError());
+    return state_finish(ctx, st, d);
+}
+
+int rt_mark_converged(rt_ctx* ctx, uint32_t npix, const rt_accum_opts* o, const rt_pixel_state* st, uint32_t* n_converged) {
+    if (!ctx) return RT_ERR_INVALID;
+    int rc;
+    if ((rc = check_accum_opts(ctx, o))) return rc;
+    if (npix == 0) { if (n_converged) *n_converged = 0; return RT_OK; }
+    HIPCHECK(ctx, hipSetDevice(ctx->device));
+    StateDev d;
+    if ((rc = state_stage(ctx, st, npix, d))) return rc;
+    if ((rc = ctl_buffer(ctx))) return rc;
+    HIPCHECK(ctx, hipMemsetAsync(ctx->prog_ctl, 0, 4 * sizeof(uint32_t), ctx->stream));
+    hipLaunchKernelGGL(k_mark_converged, dim3((npix + BLOCK - 1) / BLOCK), dim3(BLOCK), 0, ctx->stream, npix, o->threshold,
+                       (const float*)d.acc.dev, (const uint32_t*)d.ns.dev, (const float*)d.m2.dev, (uint8_t*)d.conv.dev, ctx->prog_ctl);
+    HIPCHECK(ctx, hipGetLastError());
+    uint32_t cnt = 0;
+    HIPCHECK(ctx, hipMemcpyAsync(&cnt, ctx->prog_ctl, 4, hipMemcpyDeviceToHost, ctx->stream));
+    if ((rc = state_finish(ctx, st, d))) return rc;
+    if (n_converged) *n_converged = cnt;
+    return RT_OK;
+}
+
+int rt_render_progressive(rt_ctx* ctx, rt_scene* s, const rt_camera* cam, const rt_sample_params* p, uint32_t samples_max,
+                          const rt_accum_opts* o, const rt_pixel_state* st, uint32_t* samples_done, uint32_t* n_converged) {
+    if (!ctx || !s || !cam || !p) return RT_ERR_INVALID;
+    int rc;
+    if ((rc = check_accum_opts(ctx, o))) return rc;
+    if (p->width == 0 || p->height == 0) return fail(ctx, RT_ERR_INVALID, "empty image");
+    if ((uint64_t)p->width * p->height > 0xffffffffull / 12) return fail(ctx, RT_ERR_INVALID, "image too large");
+    if (p->use_photons && s->S.num_photons == 0 && s->photons_host.empty()) return fail(ctx, RT_ERR_INVALID, "use_photons without a photon map");
+    HIPCHECK(ctx, hipSetDevice(ctx->device));
+    uint32_t npix = p->width * p->height;
+    StateDev d;
+    if ((rc = state_stage(ctx, st, npix, d))) return rc;
+    if ((rc = ctl_buffer(ctx))) return rc;
+    DCamera C; DParams P;
+    to_dev(cam, p, C, P);
+    P.counters = nullptr;
+    const bool knn = p->use_photons && s->S.ph_knn;
+    dim3 grid((npix + BLOCK - 1) / BLOCK);
+    HIPCHECK(ctx, hipMemsetAsync(ctx->prog_ctl, 0, 4 * sizeof(uint32_t), ctx->stream));
+    hipLaunchKernelGGL(k_count_flags, grid, dim3(BLOCK), 0, ctx->stream, npix, (const uint8_t*)d.conv.dev, ctx->prog_ctl + 1);
+    if (ctx->timing) HIPCHECK(ctx, hipEventRecord(ctx->ev0, ctx->stream));
+    float* acc = (float*)d.acc.dev; uint32_t* ns = (uint32_t*)d.ns.dev; float* m2 = (float*)d.m2.dev; uint8_t* conv = (uint8_t*)d.conv.dev;
+    for (uint32_t k = 0; k < samples_max; k++) {
+        DParams Q = P;
+        Q.sample_idx = P.sample_idx + k;
+        if (knn) hipLaunchKernelGGL(k_progressive<true>, grid, dim3(BLOCK), 0, ctx->stream, s->S, C, Q, *o, k, acc, ns, m2, conv, ctx->prog_ctl);
+        else hipLaunchKernelGGL(k_progressive<false>, grid, dim3(BLOCK), 0, ctx->stream, s->S, C, Q, *o, k, acc, ns, m2, conv, ctx->prog_ctl);
+        hipLaunchKernelGGL(k_progressive_step, dim3(1), dim3(64), 0, ctx->stream, ctx->prog_ctl, k, npix, *o);
+        HIPCHECK(ctx, hipGetLastError());
+    }
+    if (ctx->timing) {
+        HIPCHECK(ctx, hipEventRecord(ctx->ev1, ctx->stream));
+        HIPCHECK(ctx, hipEventSynchronize(ctx->ev1));
+        float ms = 0.0f;
+        HIPCHECK(ctx, hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+        ctx->kernel_ms += ms;
+        ctx->launches += 2ull * samples_max;
+    }
+    uint32_t ctl[4] = {0, 0, 0, 0};
+    HIPCHECK(ctx, hipMemcpyAsync(ctl, ctx->prog_ctl, sizeof ctl, hipMemcpyDeviceToHost, ctx->stream));
+    if ((rc = state_finish(ctx, st, d))) return rc;
+    if (samples_done) *samples_done = ctl[2];
+    if (n_converged) *n_converged = ctl[1];
+    return RT_OK;
 }
 
 

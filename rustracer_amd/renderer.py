@@ -148,6 +148,60 @@ class Renderer:
                     "rt_render_accumulate")
         return out
 
+    @staticmethod
+    def accum_opts(firefly_clamp=8.0, firefly_min_samples=4, firefly_abs_floor=5.0, adaptive=False,
+                   min_samples=128, convergence_pct=99.0, threshold=0.05):
+        """rt_accum_opts; the defaults are the reference's (FireflyConfig::default,
+        src/main.rs:124-133, 176, 696-718)."""
+        return F.rt_accum_opts(firefly_clamp=firefly_clamp, firefly_min_samples=firefly_min_samples,
+                               firefly_abs_floor=firefly_abs_floor, adaptive=1 if adaptive else 0,
+                               min_samples=min_samples, convergence_pct=convergence_pct, threshold=threshold)
+
+    @staticmethod
+    def pixel_state(state):
+        """rt_pixel_state over a dict of arrays (numpy or torch, host or
+        device): accum (n,3) float32, samples (n,) uint32 (torch: int32),
+        var_m2 (n,3) float32, converged (n,) uint8."""
+        return F.rt_pixel_state(accum=_ptr(state["accum"]), samples=_ptr(state["samples"]),
+                                var_m2=_ptr(state["var_m2"]), converged=_ptr(state["converged"]))
+
+    @staticmethod
+    def new_pixel_state(npix):
+        """A fresh host state for npix pixels (numpy)."""
+        return dict(accum=np.zeros((npix, 3), np.float32), samples=np.zeros(npix, np.uint32),
+                    var_m2=np.zeros((npix, 3), np.float32), converged=np.zeros(npix, np.uint8))
+
+    def accumulate_sample(self, scratch, state, opts=None):
+        """accumulate_sample (src/main.rs:603-644) of one pass into `state`."""
+        o = opts if opts is not None else self.accum_opts()
+        st = self.pixel_state(state)
+        npix = int(np.prod(scratch.shape[:-1]))
+        self._check(self.L.rt_accumulate_sample(self.ctx, _ptr(scratch), npix, C.byref(o), C.byref(st)),
+                    "rt_accumulate_sample")
+        return state
+
+    def mark_converged(self, state, opts=None):
+        """mark_converged (src/main.rs:646-677); returns the converged count."""
+        o = opts if opts is not None else self.accum_opts()
+        st = self.pixel_state(state)
+        n = C.c_uint32()
+        self._check(self.L.rt_mark_converged(self.ctx, len(state["converged"]), C.byref(o), C.byref(st), C.byref(n)),
+                    "rt_mark_converged")
+        return n.value
+
+    def render_progressive(self, state, camera, width, height, samples_max, opts=None, sample_idx=0, strata=0,
+                           bg_scale=1.0, photon_map=False, max_depth=50, seed=0):
+        """The reference's render loop (src/main.rs:463-506) on the device;
+        returns (passes run, converged pixels)."""
+        o = opts if opts is not None else self.accum_opts()
+        p = self.params(width, height, sample_idx, strata, bg_scale, max_depth, None, photon_map, seed)
+        cam = camera.to_c()
+        st = self.pixel_state(state)
+        done, nconv = C.c_uint32(), C.c_uint32()
+        self._check(self.L.rt_render_progressive(self.ctx, self.scene, C.byref(cam), C.byref(p), samples_max, C.byref(o),
+                                                 C.byref(st), C.byref(done), C.byref(nconv)), "rt_render_progressive")
+        return done.value, nconv.value
+
     def render_wavefront(self, out, camera, width, height, spp, sample_idx=0, strata=0, bg_scale=1.0,
                          photon_map=False, max_depth=50, seed=0, capacity=0, grid=0, sort=True, use_graph=False,
                          occupancy=0, scene=None):

@@ -37,7 +37,7 @@ def test_api_version_without_device():
 
 STRUCTS = ["rt_xform", "rt_medium", "rt_object", "rt_material", "rt_background", "rt_scene_desc",
            "rt_ray", "rt_hit", "rt_camera", "rt_sample_params", "rt_photon_params", "rt_wavefront_opts", "rt_batch_view",
-           "rt_param", "rt_image"]
+           "rt_param", "rt_image", "rt_accum_opts", "rt_pixel_state"]
 
 
 def test_struct_layouts_match_c_compiler():

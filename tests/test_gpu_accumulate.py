@@ -1,0 +1,119 @@
+"""Per-pixel accumulation on the device (rt_accumulate_sample,
+rt_mark_converged, rt_render_progressive) against the checker's
+accumulate_sample / mark_converged (src/main.rs:603-677): bit-exact state
+(sums, counts, Welford M2, flags), host and device buffers, and the fused
+render loop (src/main.rs:463-506) against the same loop run pass by pass —
+rt_render_sample for the radiance, the checker for the bookkeeping."""
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from rustracer_amd import Renderer, scenes
+
+pytestmark = pytest.mark.gpu
+
+
+def fresh(n):
+    return Renderer.new_pixel_state(n)
+
+
+def same(a, b):
+    for k in ("accum", "samples", "var_m2", "converged"):
+        x, y = np.asarray(a[k]), np.asarray(b[k])
+        assert np.array_equal(x.view(np.uint8), y.view(np.uint8)), (k, np.argwhere(x != y)[:5])
+
+
+def passes(n, k, seed):
+    g = np.random.default_rng(seed)
+    x = g.gamma(0.7, 1.0, size=(k, n, 3)).astype(np.float32)
+    spikes = g.random((k, n)) < 0.03
+    x[spikes] *= 500.0
+    x[g.random((k, n, 3)) < 0.01] = np.nan
+    x[g.random((k, n, 3)) < 0.005] = np.inf
+    return x
+
+
+@pytest.mark.parametrize("clamp", [8.0, 0.0])
+def test_accumulate_and_mark_bit_exact_host_buffers(gpu_renderer, clamp):
+    n, k = 5000, 12
+    x = passes(n, k, 3)
+    d, h = fresh(n), fresh(n)
+    d["converged"][::97] = 1
+    h["converged"][::97] = 1
+    o = Renderer.accum_opts(firefly_clamp=clamp)
+    for j in range(k):
+        gpu_renderer.accumulate_sample(x[j], d, o)
+        O.accumulate_sample(x[j], h, firefly_clamp=clamp)
+        same(d, h)
+    o = Renderer.accum_opts(threshold=0.3)
+    assert gpu_renderer.mark_converged(d, o) == O.mark_converged(h, 0.3)
+    same(d, h)
+    assert 0 < d["converged"].mean() < 1
+
+
+def test_accumulate_device_buffers(gpu_renderer):
+    torch = pytest.importorskip("torch")
+    n, k = 4096, 6
+    x = passes(n, k, 4)
+    h = fresh(n)
+    dev = dict(accum=torch.zeros((n, 3), dtype=torch.float32, device="cuda"),
+               samples=torch.zeros(n, dtype=torch.int32, device="cuda"),
+               var_m2=torch.zeros((n, 3), dtype=torch.float32, device="cuda"),
+               converged=torch.zeros(n, dtype=torch.uint8, device="cuda"))
+    for j in range(k):
+        gpu_renderer.accumulate_sample(torch.from_numpy(x[j]).cuda(), dev)
+        O.accumulate_sample(x[j], h)
+    back = {key: v.cpu().numpy() for key, v in dev.items()}
+    back["samples"] = back["samples"].view(np.uint32)
+    same(back, h)
+
+
+def loop_reference(r, cam, W, H, samples_max, D, seed, adaptive, min_samples, pct, thr):
+    """src/main.rs:463-506 pass by pass: device radiance, checker bookkeeping."""
+    st = fresh(W * H)
+    n_conv = 0
+    s = 0
+    while s < samples_max:
+        sc = np.zeros((H, W, 3), np.float32)
+        r.render_tiles(sc, st["converged"] if adaptive else None, s, 0, W, H, cam, max_depth=D, seed=seed)
+        O.accumulate_sample(sc, st)
+        s += 1
+        if adaptive and s >= min_samples:
+            n_conv = O.mark_converged(st, thr)
+        if adaptive and np.float32(n_conv) >= np.float32(W * H) * np.float32(pct) / np.float32(100.0):
+            break
+    return st, s, n_conv
+
+
+@pytest.mark.parametrize("adaptive", [False, True])
+def test_progressive_equals_the_pass_by_pass_loop(gpu_renderer, adaptive):
+    sc, cam, cfg = scenes.box_room(40, 32)
+    gpu_renderer.upload(sc)
+    W, H, D = 40, 32, cfg["max_depth"]
+    kw = dict(adaptive=adaptive, min_samples=6, convergence_pct=40.0, threshold=0.25)
+    ref, s_ref, c_ref = loop_reference(gpu_renderer, cam, W, H, 40, D, 5, adaptive, 6, 40.0, 0.25)
+    st = fresh(W * H)
+    done, nconv = gpu_renderer.render_progressive(st, cam, W, H, 40, Renderer.accum_opts(**kw), max_depth=D, seed=5)
+    assert (done, nconv) == (s_ref, c_ref)
+    same(st, ref)
+    if adaptive:
+        assert 6 <= done < 40 and nconv >= 0.4 * W * H
+    else:
+        assert done == 40 and (st["samples"] > 0).all()
+
+
+def test_progressive_continues_a_state(gpu_renderer):
+    sc, cam, cfg = scenes.box_room(24, 24)
+    gpu_renderer.upload(sc)
+    a, b = fresh(576), fresh(576)
+    gpu_renderer.render_progressive(a, cam, 24, 24, 9, max_depth=cfg["max_depth"], seed=2)
+    gpu_renderer.render_progressive(b, cam, 24, 24, 4, max_depth=cfg["max_depth"], seed=2)
+    gpu_renderer.render_progressive(b, cam, 24, 24, 5, sample_idx=4, max_depth=cfg["max_depth"], seed=2)
+    same(a, b)
+
+
+def test_accumulate_rejects_bad_options(gpu_renderer):
+    from rustracer_amd import RtError
+    st = fresh(4)
+    with pytest.raises(RtError):
+        gpu_renderer.accumulate_sample(np.zeros((4, 3), np.float32), st, Renderer.accum_opts(firefly_clamp=float("nan")))
