@@ -543,6 +543,24 @@ int  rt_render_progressive(rt_ctx *ctx, rt_scene *scene, const rt_camera *cam,
                            const rt_accum_opts *opts, const rt_pixel_state *state,
                            uint32_t *samples_done, uint32_t *n_converged);
 
+/* The saved image (src/main.rs:571-582, src/output.rs:103-190): per pixel the
+ * mean radiance accum / max(n, 1) (n = samples[i], or spp when samples is
+ * NULL), then apply_optics (vignette 1 - vignette * r^2, barrel distortion
+ * sampling c + (p - c) * (1 - distortion * r^2) bilinearly, r = 1 at the
+ * corners; both 0: untouched), then tone_map with scale = exposure (ACES or
+ * AgX "Punchy"), the IEC sRGB encode and the 8-bit store
+ * (min(encode, 0.999) * 256, truncated).  rgb8: width*height*3 bytes, row 0 =
+ * image top.                                                                */
+enum { RT_TONE_ACES = 0, RT_TONE_AGX = 1 };
+typedef struct {
+    float    exposure;
+    uint32_t tonemapper;           /* RT_TONE_ACES / RT_TONE_AGX */
+    float    vignette;
+    float    distortion;
+} rt_resolve_opts;
+int  rt_resolve_rgb8(rt_ctx *ctx, const float *accum, const uint32_t *samples, uint32_t spp,
+                     uint32_t width, uint32_t height, const rt_resolve_opts *opts, uint8_t *rgb8);
+
 /* ---- measurement (no counterpart in the reference: its --bench mode times
  * render_tiles with Instant, src/main.rs:68-117) ------------------------------ */
 /* Bracket every render call's kernels with HIP events on the context's stream;

@@ -61,6 +61,10 @@ def lib():
         L.rtc_perlin.argtypes = [vp, vp, C.c_uint32, C.c_int, vp, vp]
         L.rtc_accumulate_sample.argtypes = [vp, C.c_uint32, C.c_float, C.c_uint32, C.c_float, vp, vp, vp, vp]
         L.rtc_mark_converged.argtypes = [C.c_uint32, C.c_float, vp, vp, vp, vp]
+        L.rtc_apply_optics.argtypes = [vp, C.c_uint32, C.c_uint32, C.c_float, C.c_float, vp]
+        L.rtc_tone_eval.argtypes = [C.c_int, vp, C.c_uint32, vp]
+        L.rtc_resolve_rgb8.argtypes = [vp, vp, C.c_uint32, C.c_uint32, C.c_uint32, C.c_float, C.c_int, C.c_float,
+                                       C.c_float, vp]
         _lib = L
     return _lib
 
@@ -346,3 +350,33 @@ def mark_converged(state, threshold=0.05):
     """mark_converged (src/main.rs:646-677) in place; returns the count."""
     return lib().rtc_mark_converged(len(state["converged"]), threshold, _p(state["accum"]), _p(state["samples"]),
                                     _p(state["var_m2"]), _p(state["converged"]))
+
+
+def resolve_rgb8(accum, samples, spp, width, height, exposure=1.0, agx=False, vignette=0.0, distortion=0.0):
+    """The saved image (src/main.rs:571-582, src/output.rs:11-186): mean,
+    optics, tone map, sRGB, 8 bits; (height, width, 3) uint8."""
+    a = np.ascontiguousarray(accum, dtype=np.float32).reshape(-1, 3)
+    n = None if samples is None else np.ascontiguousarray(samples, dtype=np.uint32).reshape(-1)
+    out = np.zeros((height, width, 3), np.uint8)
+    lib().rtc_resolve_rgb8(_p(a), None if n is None else _p(n), spp, width, height, exposure, 1 if agx else 0,
+                           vignette, distortion, _p(out))
+    return out
+
+
+def apply_optics(img, vignette, distortion):
+    """apply_optics (src/output.rs:150-186) of an (h, w, 3) float32 image."""
+    a = np.ascontiguousarray(img, dtype=np.float32)
+    out = np.zeros_like(a)
+    lib().rtc_apply_optics(_p(a), a.shape[1], a.shape[0], vignette, distortion, _p(out))
+    return out
+
+
+def tone_eval(which, x):
+    """which: "aces" / "agx" ((n, 3) -> (n, 3)), "byte" (8-bit sRGB store of
+    each value), "aces_curve", "agx_curve" (elementwise)."""
+    code = {"aces": 0, "agx": 1, "byte": 2, "aces_curve": 3, "agx_curve": 4}[which]
+    a = np.ascontiguousarray(x, dtype=np.float32)
+    n = len(a) if code < 2 else a.size
+    out = np.zeros_like(a)
+    lib().rtc_tone_eval(code, _p(a), n, _p(out))
+    return out

@@ -832,4 +832,99 @@ int rtc_mark_converged(uint32_t npix, float threshold, const float* acc, const u
     return total;
 }
 
+// ---- the saved image (src/main.rs:571-582, src/output.rs:11-186) -----------------
+static inline float o_aces_curve(float x) { x = std::fmax(x, 0.0f); return (x * (2.51f * x + 0.03f)) / (x * (2.43f * x + 0.59f) + 0.14f); }
+static inline void o_aces(const float c[3], float out[3]) {
+    float r = c[0] * 0.59719f + c[1] * 0.35458f + c[2] * 0.04823f;
+    float g = c[0] * 0.07600f + c[1] * 0.90834f + c[2] * 0.01566f;
+    float b = c[0] * 0.02840f + c[1] * 0.13383f + c[2] * 0.83777f;
+    r = o_aces_curve(r); g = o_aces_curve(g); b = o_aces_curve(b);
+    out[0] = r * 1.60475f + g * -0.53108f + b * -0.07367f;
+    out[1] = r * -0.10208f + g * 1.10813f + b * -0.00605f;
+    out[2] = r * -0.00327f + g * -0.07276f + b * 1.07602f;
+}
+static inline float o_clamp(float x, float lo, float hi) { if (x < lo) x = lo; if (x > hi) x = hi; return x; }
+static inline float o_agx_curve(float x) {
+    float x2 = x * x, x4 = x2 * x2;
+    return 15.5f * x4 * x2 - 40.14f * x4 * x + 31.96f * x4 - 6.868f * x2 * x + 0.4298f * x2 + 0.1191f * x - 0.00232f;
+}
+static inline void o_agx(const float c[3], float out[3]) {
+    float v[3] = {c[0] * 0.842479062253094f + c[1] * 0.0784335999999992f + c[2] * 0.0792237451477643f,
+                  c[0] * 0.0423282422610123f + c[1] * 0.878468636469772f + c[2] * 0.0791661274605434f,
+                  c[0] * 0.0423756549057051f + c[1] * 0.0784336f + c[2] * 0.879142973793104f};
+    for (int k = 0; k < 3; k++) v[k] = o_agx_curve(o_clamp((std::log2(std::fmax(v[k], 1e-10f)) - -12.47393f) / 16.5f, 0.0f, 1.0f));
+    float luma = acc_lum(v[0], v[1], v[2]);
+    for (int k = 0; k < 3; k++) v[k] = luma + 1.4f * (std::pow(v[k], 1.35f) - luma);
+    out[0] = v[0] * 1.19687900512017f + v[1] * -0.0980208811401368f + v[2] * -0.0990297440797205f;
+    out[1] = v[0] * -0.0528968517574562f + v[1] * 1.15190312990417f + v[2] * -0.0989611768448433f;
+    out[2] = v[0] * -0.0529716355144234f + v[1] * -0.0980434501171241f + v[2] * 1.15107567775604f;
+}
+static inline uint8_t o_byte(float x) {
+    x = std::fmax(x, 0.0f);
+    float e = x <= 0.0031308f ? 12.92f * x : 1.055f * std::pow(x, 1.0f / 2.4f) - 0.055f;
+    float v = o_clamp(e, 0.0f, 0.999f) * 256.0f;
+    if (!(v > 0.0f)) return 0;
+    return v >= 255.0f ? 255 : (uint8_t)v;
+}
+// apply_optics (src/output.rs:150-186) on a linear image of w*h RGB floats.
+int rtc_apply_optics(const float* raw, uint32_t w, uint32_t h, float vignette, float distortion, float* img) {
+    size_t npix = (size_t)w * h;
+    if (vignette == 0.0f && distortion == 0.0f) { std::copy(raw, raw + 3 * npix, img); return 0; }
+    float cx = ((float)w - 1.0f) * 0.5f, cy = ((float)h - 1.0f) * 0.5f;
+    float rc = std::fmax(std::sqrt(cx * cx + cy * cy), 1e-6f);
+    for (uint32_t y = 0; y < h; y++)
+        for (uint32_t x = 0; x < w; x++) {
+            float dx = ((float)x - cx) / rc, dy = ((float)y - cy) / rc;
+            float r2 = dx * dx + dy * dy;
+            float s = 1.0f - distortion * r2;
+            float sx = o_clamp(cx + dx * rc * s, 0.0f, (float)w - 1.0f);
+            float sy = o_clamp(cy + dy * rc * s, 0.0f, (float)h - 1.0f);
+            size_t x0 = (size_t)sx, y0 = (size_t)sy;
+            size_t x1 = std::min<size_t>(x0 + 1, w - 1), y1 = std::min<size_t>(y0 + 1, h - 1);
+            float fx = sx - (float)x0, fy = sy - (float)y0;
+            float vig = std::fmax(1.0f - vignette * r2, 0.0f);
+            for (int k = 0; k < 3; k++) {
+                float c00 = raw[3 * (y0 * w + x0) + k], c10 = raw[3 * (y0 * w + x1) + k];
+                float c01 = raw[3 * (y1 * w + x0) + k], c11 = raw[3 * (y1 * w + x1) + k];
+                float c = (c00 * (1.0f - fx) + c10 * fx) * (1.0f - fy) + (c01 * (1.0f - fx) + c11 * fx) * fy;
+                img[3 * ((size_t)y * w + x) + k] = c * vig;
+            }
+        }
+    return 0;
+}
+
+// Tone-mapping pieces for tests: which 0 aces_tonemap, 1 agx_tonemap (3 -> 3
+// floats per item), 2 the 8-bit sRGB store of one channel, 3 aces_curve,
+// 4 agx_curve (1 -> 1).
+int rtc_tone_eval(int which, const float* in, uint32_t n, float* out) {
+    for (uint32_t i = 0; i < n; i++) {
+        if (which == 0) o_aces(in + 3 * i, out + 3 * i);
+        else if (which == 1) o_agx(in + 3 * i, out + 3 * i);
+        else if (which == 2) out[i] = (float)o_byte(in[i]);
+        else if (which == 3) out[i] = o_aces_curve(in[i]);
+        else if (which == 4) out[i] = o_agx_curve(in[i]);
+        else return -1;
+    }
+    return 0;
+}
+
+int rtc_resolve_rgb8(const float* acc, const uint32_t* ns, uint32_t spp, uint32_t w, uint32_t h, float exposure, int agx,
+                     float vignette, float distortion, uint8_t* out) {
+    size_t npix = (size_t)w * h;
+    std::vector<float> raw(3 * npix), img(3 * npix);
+    for (size_t i = 0; i < npix; i++) {
+        uint32_t n0 = ns ? ns[i] : spp;
+        float n = (float)(n0 > 1 ? n0 : 1);
+        for (int k = 0; k < 3; k++) raw[3 * i + k] = acc[3 * i + k] / n;
+    }
+    rtc_apply_optics(raw.data(), w, h, vignette, distortion, img.data());
+    for (size_t i = 0; i < npix; i++) {
+        float c[3] = {img[3 * i] * exposure, img[3 * i + 1] * exposure, img[3 * i + 2] * exposure};
+        float m[3];
+        if (agx) o_agx(c, m); else o_aces(c, m);
+        for (int k = 0; k < 3; k++) out[3 * i + k] = o_byte(m[k]);
+    }
+    return 0;
+}
+
 }  // extern "C"

@@ -115,6 +115,13 @@ class rt_pixel_state(C.Structure):
     _fields_ = [("accum", C.c_void_p), ("samples", C.c_void_p), ("var_m2", C.c_void_p), ("converged", C.c_void_p)]
 
 
+class rt_resolve_opts(C.Structure):
+    _fields_ = [("exposure", C.c_float), ("tonemapper", C.c_uint32), ("vignette", C.c_float), ("distortion", C.c_float)]
+
+
+TONE_ACES, TONE_AGX = 0, 1
+
+
 class rt_batch_view(C.Structure):
     _fields_ = [("scene", C.c_void_p), ("camera", rt_camera), ("params", rt_sample_params), ("spp", C.c_uint32),
                 ("accum_rgb", C.c_void_p)]
@@ -179,6 +186,7 @@ SIGNATURES = {
     "rt_render_progressive": (C.c_int, [VP, VP, C.POINTER(rt_camera), C.POINTER(rt_sample_params), C.c_uint32,
                                         C.POINTER(rt_accum_opts), C.POINTER(rt_pixel_state), C.POINTER(C.c_uint32),
                                         C.POINTER(C.c_uint32)]),
+    "rt_resolve_rgb8": (C.c_int, [VP, VP, VP, C.c_uint32, C.c_uint32, C.c_uint32, C.POINTER(rt_resolve_opts), VP]),
     "rt_ctx_set_timing": (C.c_int, [VP, C.c_int]),
     "rt_ctx_get_timing": (C.c_int, [VP, C.POINTER(C.c_double), C.POINTER(C.c_uint64)]),
     "rt_ctx_set_counting": (C.c_int, [VP, C.c_int]),

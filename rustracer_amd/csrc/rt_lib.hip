@@ -137,6 +137,43 @@ __global__ __launch_bounds__(BLOCK) void k_count_flags(uint32_t npix, const uint
     if (__lane_id() == 0 && b) atomicAdd(count, (uint32_t)__popcll(b));
 }
 
+// Mean radiance of pixel j: accum / max(n, 1) per component (src/main.rs:574-577).
+__device__ __forceinline__ V3 resolve_raw(const float* acc, const uint32_t* ns, uint32_t spp, uint32_t j) {
+    float n = (float)max(ns ? ns[j] : spp, 1u);
+    return mk(acc[3 * (size_t)j] / n, acc[3 * (size_t)j + 1] / n, acc[3 * (size_t)j + 2] / n);
+}
+// rt_resolve_rgb8: apply_optics (src/output.rs:150-186) reading the means on
+// the fly, then tone_map (src/output.rs:103-111); one thread per pixel.
+__global__ __launch_bounds__(BLOCK) void k_resolve(const float* __restrict__ acc, const uint32_t* __restrict__ ns, uint32_t spp,
+                                                   uint32_t w, uint32_t h, rt_resolve_opts o, uint8_t* __restrict__ out) {
+    uint32_t i = blockIdx.x * BLOCK + threadIdx.x;
+    if (i >= w * h) return;
+    uint32_t y = i / w, x = i - y * w;
+    V3 c;
+    if (o.vignette == 0.0f && o.distortion == 0.0f) {
+        c = resolve_raw(acc, ns, spp, i);
+    } else {
+        float cx = ((float)w - 1.0f) * 0.5f, cy = ((float)h - 1.0f) * 0.5f;
+        float rc = fmaxf(sqrtf(cx * cx + cy * cy), 1e-6f);
+        float dx = ((float)x - cx) / rc, dy = ((float)y - cy) / rc;
+        float r2 = dx * dx + dy * dy;
+        float s = 1.0f - o.distortion * r2;
+        float sx = fclamp(cx + dx * rc * s, 0.0f, (float)w - 1.0f);
+        float sy = fclamp(cy + dy * rc * s, 0.0f, (float)h - 1.0f);
+        uint32_t x0 = f2u(sx), y0 = f2u(sy);
+        uint32_t x1 = min(x0 + 1, w - 1), y1 = min(y0 + 1, h - 1);
+        float fx = sx - (float)x0, fy = sy - (float)y0;
+        V3 c00 = resolve_raw(acc, ns, spp, y0 * w + x0), c10 = resolve_raw(acc, ns, spp, y0 * w + x1);
+        V3 c01 = resolve_raw(acc, ns, spp, y1 * w + x0), c11 = resolve_raw(acc, ns, spp, y1 * w + x1);
+        c = (c00 * (1.0f - fx) + c10 * fx) * (1.0f - fy) + (c01 * (1.0f - fx) + c11 * fx) * fy;
+        c = c * fmaxf(1.0f - o.vignette * r2, 0.0f);
+    }
+    V3 m = o.tonemapper == RT_TONE_AGX ? agx_tonemap(c * o.exposure) : aces_tonemap(c * o.exposure);
+    out[3 * (size_t)i] = srgb_byte(m.x);
+    out[3 * (size_t)i + 1] = srgb_byte(m.y);
+    out[3 * (size_t)i + 2] = srgb_byte(m.z);
+}
+
 // One pass of the reference's render loop (src/main.rs:463-506) fused per
 // pixel: render_tiles' sample, accumulate_sample, and (adaptive, from
 // min_samples passes on) mark_converged.  ctl[0] = stopped (set between
@@ -1788,6 +1825,28 @@ int rt_mark_converged(rt_ctx* ctx, uint32_t npix, const rt_accum_opts* o, const 
     if ((rc = state_finish(ctx, st, d))) return rc;
     if (n_converged) *n_converged = cnt;
     return RT_OK;
+}
+
+int rt_resolve_rgb8(rt_ctx* ctx, const float* acc, const uint32_t* ns, uint32_t spp, uint32_t w, uint32_t h,
+                    const rt_resolve_opts* o, uint8_t* out) {
+    if (!ctx || !o) return RT_ERR_INVALID;
+    if ((uint64_t)w * h > 0xffffffffull / 12) return fail(ctx, RT_ERR_INVALID, "image too large");
+    if (o->tonemapper != RT_TONE_ACES && o->tonemapper != RT_TONE_AGX) return fail(ctx, RT_ERR_INVALID, "unknown tone mapper");
+    if (!std::isfinite(o->exposure) || !std::isfinite(o->vignette) || !std::isfinite(o->distortion))
+        return fail(ctx, RT_ERR_INVALID, "resolve options must be finite");
+    uint32_t npix = w * h;
+    if (npix == 0) return RT_OK;
+    if (!acc || !out) return RT_ERR_INVALID;
+    HIPCHECK(ctx, hipSetDevice(ctx->device));
+    Staged a, n, r;
+    int rc;
+    if ((rc = stage_in(ctx, acc, (size_t)npix * 12, a))) return rc;
+    if ((rc = stage_in(ctx, ns, ns ? (size_t)npix * 4 : 0, n))) return rc;
+    if ((rc = stage_out(ctx, out, (size_t)npix * 3, r, false))) return rc;
+    hipLaunchKernelGGL(k_resolve, dim3((npix + BLOCK - 1) / BLOCK), dim3(BLOCK), 0, ctx->stream, (const float*)a.dev,
+                       (const uint32_t*)n.dev, spp, w, h, *o, (uint8_t*)r.dev);
+    HIPCHECK(ctx, hipGetLastError());
+    return finish_out(ctx, out, (size_t)npix * 3, r);
 }
 
 int rt_render_progressive(rt_ctx* ctx, rt_scene* s, const rt_camera* cam, const rt_sample_params* p, uint32_t samples_max,

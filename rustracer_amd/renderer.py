@@ -202,6 +202,19 @@ class Renderer:
                                                  C.byref(st), C.byref(done), C.byref(nconv)), "rt_render_progressive")
         return done.value, nconv.value
 
+    def resolve_rgb8(self, accum, width, height, samples=None, spp=1, exposure=1.0, tonemapper="aces",
+                     vignette=0.0, distortion=0.0, out=None):
+        """The saved 8-bit image (src/main.rs:571-582, src/output.rs): mean
+        radiance (accum / samples, or / spp), lens optics, ACES or AgX tone
+        map, sRGB encode; returns (height, width, 3) uint8."""
+        o = F.rt_resolve_opts(exposure=exposure, tonemapper=F.TONE_AGX if tonemapper == "agx" else F.TONE_ACES,
+                              vignette=vignette, distortion=distortion)
+        if out is None:
+            out = np.zeros((height, width, 3), np.uint8)
+        self._check(self.L.rt_resolve_rgb8(self.ctx, _ptr(accum), _ptr(samples), spp, width, height, C.byref(o), _ptr(out)),
+                    "rt_resolve_rgb8")
+        return out
+
     def render_wavefront(self, out, camera, width, height, spp, sample_idx=0, strata=0, bg_scale=1.0,
                          photon_map=False, max_depth=50, seed=0, capacity=0, grid=0, sort=True, use_graph=False,
                          occupancy=0, scene=None):

@@ -18,7 +18,7 @@ HEADER = os.path.join(ROOT, "include", "rt_api.h")
 def declared_functions():
     text = open(HEADER).read()
     text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
-    return sorted(set(re.findall(r"\b(rt_[a-z_]+)\s*\(", text)))
+    return sorted(set(re.findall(r"\b(rt_[a-z0-9_]+)\s*\(", text)))
 
 
 def test_header_functions_are_all_in_the_ctypes_table():
@@ -37,7 +37,7 @@ def test_api_version_without_device():
 
 STRUCTS = ["rt_xform", "rt_medium", "rt_object", "rt_material", "rt_background", "rt_scene_desc",
            "rt_ray", "rt_hit", "rt_camera", "rt_sample_params", "rt_photon_params", "rt_wavefront_opts", "rt_batch_view",
-           "rt_param", "rt_image", "rt_accum_opts", "rt_pixel_state"]
+           "rt_param", "rt_image", "rt_accum_opts", "rt_pixel_state", "rt_resolve_opts"]
 
 
 def test_struct_layouts_match_c_compiler():

@@ -117,3 +117,32 @@ def test_accumulate_rejects_bad_options(gpu_renderer):
     st = fresh(4)
     with pytest.raises(RtError):
         gpu_renderer.accumulate_sample(np.zeros((4, 3), np.float32), st, Renderer.accum_opts(firefly_clamp=float("nan")))
+
+
+@pytest.mark.parametrize("tm,vig,dist", [("aces", 0.0, 0.0), ("agx", 0.0, 0.0), ("aces", 0.3, 0.1), ("agx", 0.25, -0.05)])
+def test_resolve_rgb8_matches_checker(gpu_renderer, tm, vig, dist):
+    """Bytes equal the checker's except where a last-ulp difference between
+    the device's and glibc's powf / log2f moves a value across a byte
+    boundary: at most 1 apart, on at most 1e-4 of the channels."""
+    H, W = 72, 96
+    g = np.random.default_rng(8)
+    acc = (g.lognormal(-1.0, 2.0, size=(H, W, 3)) * 4).astype(np.float32)
+    n = g.integers(0, 40, size=H * W).astype(np.uint32)
+    for samples, spp in ((n, 0), (None, 16)):
+        d = gpu_renderer.resolve_rgb8(acc, W, H, samples=samples, spp=spp, exposure=1.5, tonemapper=tm,
+                                      vignette=vig, distortion=dist)
+        c = O.resolve_rgb8(acc, samples, spp, W, H, exposure=1.5, agx=tm == "agx", vignette=vig, distortion=dist)
+        diff = np.abs(d.astype(np.int32) - c.astype(np.int32))
+        assert diff.max() <= 1 and (diff > 0).mean() <= 1e-4, (diff.max(), (diff > 0).mean())
+        assert 20 < d.mean() < 240
+
+
+def test_progressive_then_resolve(gpu_renderer):
+    sc, cam, cfg = scenes.box_room(48, 40)
+    gpu_renderer.upload(sc)
+    st = fresh(48 * 40)
+    gpu_renderer.render_progressive(st, cam, 48, 40, 16, max_depth=cfg["max_depth"], seed=3)
+    d = gpu_renderer.resolve_rgb8(st["accum"], 48, 40, samples=st["samples"], tonemapper="agx")
+    c = O.resolve_rgb8(st["accum"], st["samples"], 0, 48, 40, agx=True)
+    diff = np.abs(d.astype(np.int32) - c.astype(np.int32))
+    assert diff.max() <= 1 and (diff > 0).mean() <= 1e-3

@@ -1,0 +1,80 @@
+"""The checker's image output (src/output.rs): the reference's optics tests
+(src/output.rs:191-233) restated, and closed forms for the tone curves and
+the 8-bit sRGB store (Narkowicz's ACES fit at 0, 1 and infinity, the AgX
+polynomial at its ends, the white point through the ACES matrices, the IEC
+61966-2-1 encode)."""
+import math
+
+import numpy as np
+
+from oracle import oracle as O
+
+
+def test_optics_identity_when_off():
+    img = np.zeros((4, 4, 3), np.float32)
+    img[..., 0] = np.arange(16).reshape(4, 4)
+    out = O.apply_optics(img, 0.0, 0.0)
+    assert np.abs(out - img).max() < 1e-7
+
+
+def test_vignette_darkens_corners_not_center():
+    out = O.apply_optics(np.ones((4, 4, 3), np.float32), 0.4, 0.0)
+    assert 0.9 < out[1, 1, 0] < 1.0
+    assert abs(out[0, 0, 0] - 0.6) < 1e-6
+
+
+def test_barrel_distortion_pulls_corners_inward():
+    img = np.zeros((9, 9, 3), np.float32)
+    img[..., 0] = np.arange(9)[None, :]
+    out = O.apply_optics(img, 0.0, 0.2)
+    assert out[0, 0, 0] > img[0, 0, 0]
+    assert abs(out[4, 4, 0] - img[4, 4, 0]) < 1e-6
+
+
+def test_aces_curve_closed_forms():
+    c = O.tone_eval("aces_curve", np.array([0.0, 1.0, 1e6, -3.0], np.float32))
+    assert c[0] == 0.0 and c[3] == 0.0
+    assert abs(c[1] - 2.54 / 3.16) < 1e-6
+    assert abs(c[2] - 2.51 / 2.43) < 1e-5
+
+
+def test_aces_maps_white_through_unit_row_sums():
+    out = O.tone_eval("aces", np.ones((1, 3), np.float32))[0]
+    assert np.allclose(out, 2.54 / 3.16, atol=2e-5)
+
+
+def test_agx_curve_ends_and_monotone_grey_ramp():
+    c = O.tone_eval("agx_curve", np.array([0.0, 1.0], np.float32))
+    assert abs(c[0] + 0.00232) < 1e-7
+    assert abs(c[1] - (15.5 - 40.14 + 31.96 - 6.868 + 0.4298 + 0.1191 - 0.00232)) < 1e-5
+    g = np.geomspace(1e-3, 1e3, 200).astype(np.float32)
+    out = O.tone_eval("agx", np.repeat(g[:, None], 3, axis=1))
+    assert (np.diff(out[:, 1]) >= 0).all() and out[0, 1] < 0.05 and out[-1, 1] > 0.9
+    # below about 2^-12.1 the curve goes negative and powf(x, 1.35) is NaN, which
+    # the 8-bit store turns into 0 (f32::max(NaN, 0) = 0), as in the reference
+    dark = O.tone_eval("agx", np.full((1, 3), 1e-6, np.float32))
+    assert np.isnan(dark).all() and (O.tone_eval("byte", dark.ravel()) == 0).all()
+
+
+def test_srgb_byte_follows_the_iec_curve():
+    x = np.linspace(-0.5, 1.5, 20001).astype(np.float32)
+    b = O.tone_eval("byte", x)
+    xd = np.maximum(x.astype(np.float64), 0.0)
+    enc = np.where(xd <= 0.0031308, 12.92 * xd, 1.055 * xd ** (1 / 2.4) - 0.055)
+    v = np.minimum(enc, 0.999) * 256.0
+    want = np.floor(v)
+    near = np.abs(v - np.round(v)) < 1e-3                 # float32 rounding may land either side
+    assert np.array_equal(b[~near], want[~near])
+    assert np.abs(b - want).max() <= 1
+    assert O.tone_eval("byte", np.array([np.nan], np.float32))[0] == 0
+
+
+def test_resolve_divides_by_per_pixel_counts_or_spp():
+    acc = np.full((2, 2, 3), 0.5, np.float32)
+    acc[0, 0] = 2.0
+    n = np.array([4, 1, 0, 1], np.uint32)
+    a = O.resolve_rgb8(acc, n, 0, 2, 2)
+    b = O.resolve_rgb8(acc / np.array([4, 1, 1, 1], np.float32).reshape(2, 2, 1), None, 1, 2, 2)
+    assert np.array_equal(a, b)
+    c = O.resolve_rgb8(acc, None, 4, 2, 2, exposure=4.0)
+    assert np.array_equal(c, O.resolve_rgb8(acc, None, 1, 2, 2))
