@@ -528,16 +528,17 @@ int  rt_mark_converged(rt_ctx *ctx, uint32_t npix, const rt_accum_opts *opts,
                        const rt_pixel_state *state, uint32_t *n_converged);
 
 /* The reference's render loop (src/main.rs:463-506) on the device: for
- * s = 0 .. samples_max-1 one pass of sample index params->sample_idx + s
+ * s = 0 .. samples_max-1 one pass of sample index k = params->sample_idx + s
  * (render_tiles, converged pixels skipped when adaptive), accumulated as
- * rt_accumulate_sample does, then, when adaptive and s + 1 >= min_samples,
- * rt_mark_converged; the loop stops after a pass when adaptive and the
- * converged count reaches convergence_pct % of the pixels.  Render,
- * accumulation and the convergence test run fused per pixel, one launch per
- * pass, with no host round trip: the stop test is read by the next launch.
- * *samples_done receives the passes run, *n_converged the flagged pixels
- * (either may be NULL).  The state is read and updated in place, so a
- * later call continues it.                                                  */
+ * rt_accumulate_sample does, then, when adaptive and k + 1 >= min_samples,
+ * rt_mark_converged (k + 1 is the frame's pass count, so a call continuing a
+ * frame at sample_idx = k marks as the reference's loop does at pass k); the
+ * loop stops after a pass when adaptive and the converged count reaches
+ * convergence_pct % of the pixels.  Render, accumulation and the convergence
+ * test run fused per pixel, one launch per pass, with no host round trip: the
+ * stop test is read by the next launch.  *samples_done receives the passes
+ * run, *n_converged the flagged pixels (either may be NULL).  The state is
+ * read and updated in place, so a later call continues it.                  */
 int  rt_render_progressive(rt_ctx *ctx, rt_scene *scene, const rt_camera *cam,
                            const rt_sample_params *params, uint32_t samples_max,
                            const rt_accum_opts *opts, const rt_pixel_state *state,

@@ -175,8 +175,8 @@ __global__ __launch_bounds__(BLOCK) void k_resolve(const float* __restrict__ acc
 }
 
 // One pass of the reference's render loop (src/main.rs:463-506) fused per
-// pixel: render_tiles' sample, accumulate_sample, and (adaptive, from
-// min_samples passes on) mark_converged.  ctl[0] = stopped (set between
+// pixel: render_tiles' sample, accumulate_sample, and (adaptive, once the
+// frame's pass count sample_idx + 1 reaches min_samples) mark_converged.  ctl[0] = stopped (set between
 // passes by k_progressive_step), ctl[1] = converged pixels, ctl[2] = passes run.
 template <bool KNN>
 __global__ __launch_bounds__(BLOCK) void k_progressive(DScene S, DCamera cam, DParams P, rt_accum_opts o, uint32_t pass,
@@ -194,7 +194,7 @@ __global__ __launch_bounds__(BLOCK) void k_progressive(DScene S, DCamera cam, DP
     float* a = acc + 3 * (size_t)i;
     float* v = m2 + 3 * (size_t)i;
     accumulate_pixel(a, ns + i, v, c, o);
-    if (o.adaptive && pass + 1 >= o.min_samples && pixel_converged(a, ns[i], v, o.threshold)) {
+    if (o.adaptive && P.sample_idx + 1 >= o.min_samples && pixel_converged(a, ns[i], v, o.threshold)) {
         conv[i] = 1;
         atomicAdd(ctl + 1, 1u);
     }

@@ -102,14 +102,20 @@ def test_progressive_equals_the_pass_by_pass_loop(gpu_renderer, adaptive):
         assert done == 40 and (st["samples"] > 0).all()
 
 
-def test_progressive_continues_a_state(gpu_renderer):
+@pytest.mark.parametrize("adaptive", [False, True])
+def test_progressive_continues_a_state(gpu_renderer, adaptive):
+    """A frame rendered in two calls (the second at sample_idx 4) equals one
+    call; adaptive marking counts passes from the frame's start."""
     sc, cam, cfg = scenes.box_room(24, 24)
     gpu_renderer.upload(sc)
+    o = Renderer.accum_opts(adaptive=adaptive, min_samples=3, convergence_pct=100.0, threshold=0.3)
     a, b = fresh(576), fresh(576)
-    gpu_renderer.render_progressive(a, cam, 24, 24, 9, max_depth=cfg["max_depth"], seed=2)
-    gpu_renderer.render_progressive(b, cam, 24, 24, 4, max_depth=cfg["max_depth"], seed=2)
-    gpu_renderer.render_progressive(b, cam, 24, 24, 5, sample_idx=4, max_depth=cfg["max_depth"], seed=2)
+    gpu_renderer.render_progressive(a, cam, 24, 24, 9, o, max_depth=cfg["max_depth"], seed=2)
+    gpu_renderer.render_progressive(b, cam, 24, 24, 4, o, max_depth=cfg["max_depth"], seed=2)
+    gpu_renderer.render_progressive(b, cam, 24, 24, 5, o, sample_idx=4, max_depth=cfg["max_depth"], seed=2)
     same(a, b)
+    if adaptive:
+        assert 0 < a["converged"].mean() < 1
 
 
 def test_accumulate_rejects_bad_options(gpu_renderer):
