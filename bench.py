@@ -89,10 +89,12 @@ def measure(name, args, with_cpu):
     def step_plain(k):
         r.render_accumulate(acc, cam, W, H, spp, strata=strata, max_depth=depth, seed=k, photon_map=photons)
 
-    def step_wf(k, graph=None):
+    def step_wf(k, graph=None, streams=None):
         o = dict(tuned) if tuned else {}
         if graph is not None:
             o["use_graph"] = graph
+        if streams is not None:
+            o["streams"] = streams
         r.render_wavefront(acc, cam, W, H, spp, strata=strata, max_depth=depth, seed=k, photon_map=photons, **o)
 
     def timed(step):
@@ -151,11 +153,13 @@ def measure(name, args, with_cpu):
                          "bytes_per_launch": (closest_bps + shadow_bps) * spl,
                          "achieved": (closest_bps + shadow_bps) * spl / (kms / 1e3 / max(nl, 1)) / 1e9}
     if "wavefront" in res:
-        # per-launch HIP events around k_wf_extend: the non-graph launch path
+        # per-launch HIP events around k_wf_extend: the non-graph launch path on
+        # one stream (with two, a launch's events would also time the other
+        # stream's kernels sharing the GPU)
         r.set_timing(True)
         nfr = max(1, args.steps)
         for k in range(nfr):
-            step_wf(2000 + k, graph=False)
+            step_wf(2000 + k, graph=False, streams=1)
         torch.cuda.synchronize()
         ems, en = r.stage_timing()
         r.set_timing(False)

@@ -313,13 +313,17 @@ typedef struct {
  * compiled for its materials only;
  * use_graph: 1 captures one chunk's launches in a hipGraph and replays it;
  * occupancy: register budget of the traversal stages — 0 compiler's choice,
- * 4 or 5: at least that many waves per SIMD (fewer VGPRs, some spills).      */
+ * 4 or 5: at least that many waves per SIMD (fewer VGPRs, some spills);
+ * streams: 2 runs consecutive chunks on two HIP streams with queues of their
+ * own, so a chunk's last bounces (few paths, latency-bound) overlap the next
+ * chunk's first ones; 0 or 1: one stream.                                    */
 typedef struct {
     uint32_t capacity;
     uint32_t grid;
     uint32_t sort;
     uint32_t use_graph;
     uint32_t occupancy;
+    uint32_t streams;
 } rt_wavefront_opts;
 
 /* ---- caustic photon map (src/photon.rs, src/scene.rs:35-74) --------------- */

@@ -102,7 +102,7 @@ class rt_sample_params(C.Structure):
 
 class rt_wavefront_opts(C.Structure):
     _fields_ = [("capacity", C.c_uint32), ("grid", C.c_uint32), ("sort", C.c_uint32), ("use_graph", C.c_uint32),
-                ("occupancy", C.c_uint32)]
+                ("occupancy", C.c_uint32), ("streams", C.c_uint32)]
 
 
 class rt_accum_opts(C.Structure):

@@ -2,7 +2,7 @@
 
 Picks the register budget (occupancy target) of the traversal stages, the
 path-queue capacity, the persistent grid of the stage kernels, the material
-sort and hipGraph replay by coordinate descent on a short frame of
+sort, hipGraph replay and one or two chunk streams by coordinate descent on a short frame of
 the actual workload (same scene, camera, depth and photon setting), timing
 each candidate with HIP events on the library's stream.  The search is cheap
 (each probe renders a few passes) and the result is returned as the kwargs of
@@ -56,7 +56,7 @@ def tune_batch(r, views, probe_spp=4, log=None):
 def tune(r, run, log=None):
     """Coordinate descent over the launch options; `run(opts, seed)` renders
     one probe frame on the renderer's stream."""
-    best = dict(capacity=1 << 22, grid=2048, sort=1, use_graph=True, occupancy=0)
+    best = dict(capacity=1 << 22, grid=2048, sort=1, use_graph=True, occupancy=0, streams=1)
     table = []
 
     def measure(o):
@@ -69,7 +69,7 @@ def tune(r, run, log=None):
     t0 = time.time()
     best_ms = measure(best)
     for key, values in (("occupancy", [0, 4, 5]), ("capacity", CAPACITIES), ("grid", GRIDS), ("sort", [0, 1, 2]),
-                        ("use_graph", [True, False])):
+                        ("use_graph", [True, False]), ("streams", [1, 2])):
         for v in values:
             if v == best[key]:
                 continue

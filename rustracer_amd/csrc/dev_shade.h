@@ -1082,10 +1082,13 @@ RT_D void knn_heap_add(float* heap, uint32_t stride, uint32_t k, uint32_t& cnt, 
 // the heap's root once k are held (a far child queued under a larger radius
 // is re-tested when it is popped); the sum then covers only the first k
 // photons found, which is the whole gather when fewer than k lie within r2.
+// KNN = false with stop > 0: give up once `stop` photons are counted (the
+// caller then knows the query holds at least that many).
 // Stackless: bit d of `pend` marks the far child of the depth-d plane as
 // still to visit; the ancestor d levels up of node i is ((i + 1) >> d) - 1.
 template <bool KNN>
-RT_D V3 pk_walk(const DScene& S, V3 pos, V3 nrm, float r2, float* heap, uint32_t stride, uint32_t& count, float& bound_out) {
+RT_D V3 pk_walk(const DScene& S, V3 pos, V3 nrm, float r2, float* heap, uint32_t stride, uint32_t& count, float& bound_out,
+                uint32_t stop = 0) {
     const uint32_t D = S.pk_depth, k = S.ph_knn;
     const uint32_t first_leaf = (1u << D) - 1u;
     float bound = r2;
@@ -1113,6 +1116,7 @@ RT_D V3 pk_walk(const DScene& S, V3 pos, V3 nrm, float r2, float* heap, uint32_t
                     if (cnt == k) bound = heap[0];
                 } else {
                     cnt++;
+                    if (stop && cnt >= stop) { count = cnt; bound_out = bound; return acc; }
                 }
             }
         }
@@ -1147,6 +1151,19 @@ RT_D float photon_knn_r2(const DScene& S, V3 pos, V3 nrm, float* heap, uint32_t 
     if (count < S.ph_knn) return S.ph_r2;
     float r2 = heap[0];
     return r2 > 0.0f ? r2 : S.ph_r2;
+}
+
+// First pass of the wavefront's k-NN stage, without the heap: the gather
+// within ph_r2, abandoned once k photons are counted.  True (irradiance in
+// `e`) when fewer than k lie within the radius: the walk and its sum are the
+// heap search's, whose radius cannot shrink before k are held.
+RT_D bool photon_knn_sparse(const DScene& S, V3 pos, V3 nrm, V3& e) {
+    uint32_t cnt;
+    float unused;
+    V3 acc = pk_walk<false>(S, pos, nrm, S.ph_r2, nullptr, 0, cnt, unused, S.ph_knn);
+    if (cnt >= S.ph_knn) return false;
+    e = acc * S.ph_norm;
+    return true;
 }
 
 // k-NN irradiance: the heap search's radius, then the gather at that radius

@@ -376,6 +376,7 @@ __global__ __launch_bounds__(BLOCK) void k_photon_irr(DScene S, const float* __r
 #define WF_CNT_ACTIVE 0   // counters[0]: paths to extend this bounce
 #define WF_CNT_NEXT 1     // counters[1]: survivors written by shade
 #define WF_CNT_SHADOW 2   // counters[2]: shadow requests
+#define WF_CNT_DENSE 3    // counters[3]: k-NN queries holding at least k photons (second k-NN pass)
 #define WF_CNT_BUCKET 8   // counters[8..8+WF_NB): bucket sizes
 #define WF_CNT_FIRST 32   // counters[32..33]: 64-bit frame position of a graph-replayed chunk
 
@@ -404,6 +405,7 @@ struct WfQueues {
     float* fb;
     float* lsamp;         // non-null: contributions go to the path's sample (3 floats per sid), not to fb
     uint32_t sort;        // 0: one bucket; 1: a bucket per material kind; 2: two shade classes
+    uint32_t* dense;      // cap: paths whose k-NN query needs the heap search
 };
 
 __device__ __forceinline__ uint32_t lane_id() { return __lane_id(); }
@@ -654,29 +656,62 @@ __global__ __launch_bounds__(BLOCK) void k_wf_shade(DScene S, DParams P, WfQueue
 
 // k-NN photon maps: between extend and shade, the photon irradiance of every
 // hit whose material can take the diffuse branch (Lambertian, PBR, Pearl),
-// into the path record, so the shade kernel keeps its occupancy (the heap
-// needs 32 KB of LDS per block).  Paths whose hit does not gather are left as
+// into the path record, so the shade kernel keeps its occupancy.  Two
+// passes: k_wf_knn (no LDS, full occupancy) finishes every query with fewer
+// than k photons within the gather radius and queues the rest; k_wf_knn_dense
+// runs the heap search (32 KB of LDS per block) on those only, so its lanes
+// all do the same kind of work.  Paths whose hit does not gather are left as
 // they are.
 template <bool BATCH>
+__device__ __forceinline__ bool knn_query(const DScene& S, const WfPath& w, V3& pos, V3& nrm) {
+    Ray r = make_ray(mk(w.o_time.x, w.o_time.y, w.o_time.z), mk(w.d_wl.x, w.d_wl.y, w.d_wl.z), w.o_time.w);
+    r.wavelength = w.d_wl.w; r.spectral = (w.flags & 1u) != 0;
+    HitId id; id.t = w.hit_t; id.slot = w.hit_slot; id.prim = w.hit_prim;
+    Rec h = finalize(S, r, id, 0.001f);
+    pos = h.p; nrm = h.n;
+    return true;
+}
+__device__ __forceinline__ void knn_store(WfPath& w, V3 e) {
+    w.pad1[0] = __float_as_uint(e.x); w.pad1[1] = __float_as_uint(e.y); w.pad1[2] = __float_as_uint(e.z);
+}
+template <bool BATCH>
 __global__ __launch_bounds__(BLOCK) void k_wf_knn(DScene S0, DParams P0, WfQueues Q, WfBatch B) {
-    float* heap = LdsHeap::column();
     uint32_t total = 0;
     for (int b = 0; b < WF_NB; b++) total += Q.counters[WF_CNT_BUCKET + b];
     uint32_t stride = gridDim.x * BLOCK;
+    for (uint32_t base0 = blockIdx.x * BLOCK; base0 < total; base0 += stride) {
+        uint32_t i = base0 + threadIdx.x;
+        bool dense = false;
+        uint32_t pid = 0;
+        if (i < total) {
+            uint32_t b = 0, base = 0;
+            while (i >= base + Q.counters[WF_CNT_BUCKET + b]) { base += Q.counters[WF_CNT_BUCKET + b]; b++; }
+            pid = Q.buckets[(size_t)b * Q.cap + (i - base)];
+            WfPath& w = Q.paths[pid];
+            const DScene& S = BATCH ? B.scenes[w.pad0] : S0;
+            bool photons = BATCH ? B.views[w.pad0].P.use_photons != 0 : P0.use_photons != 0;
+            if (photons && knn_gathers(S, w)) {
+                V3 pos, nrm, e;
+                knn_query<BATCH>(S, w, pos, nrm);
+                if (photon_knn_sparse(S, pos, nrm, e)) knn_store(w, e);
+                else dense = true;
+            }
+        }
+        uint32_t slot = wave_append(Q.counters + WF_CNT_DENSE, 0u, dense);
+        if (dense) Q.dense[slot] = pid;
+    }
+}
+template <bool BATCH>
+__global__ __launch_bounds__(BLOCK) void k_wf_knn_dense(DScene S0, WfQueues Q, WfBatch B) {
+    float* heap = LdsHeap::column();
+    uint32_t total = Q.counters[WF_CNT_DENSE];
+    uint32_t stride = gridDim.x * BLOCK;
     for (uint32_t i = blockIdx.x * BLOCK + threadIdx.x; i < total; i += stride) {
-        uint32_t b = 0, base = 0;
-        while (i >= base + Q.counters[WF_CNT_BUCKET + b]) { base += Q.counters[WF_CNT_BUCKET + b]; b++; }
-        uint32_t pid = Q.buckets[(size_t)b * Q.cap + (i - base)];
-        WfPath& w = Q.paths[pid];
+        WfPath& w = Q.paths[Q.dense[i]];
         const DScene& S = BATCH ? B.scenes[w.pad0] : S0;
-        bool photons = BATCH ? B.views[w.pad0].P.use_photons != 0 : P0.use_photons != 0;
-        if (!photons || !knn_gathers(S, w)) continue;
-        Ray r = make_ray(mk(w.o_time.x, w.o_time.y, w.o_time.z), mk(w.d_wl.x, w.d_wl.y, w.d_wl.z), w.o_time.w);
-        r.wavelength = w.d_wl.w; r.spectral = (w.flags & 1u) != 0;
-        HitId id; id.t = w.hit_t; id.slot = w.hit_slot; id.prim = w.hit_prim;
-        Rec h = finalize(S, r, id, 0.001f);
-        V3 e = photon_knn_irradiance(S, h.p, h.n, heap, BLOCK);
-        w.pad1[0] = __float_as_uint(e.x); w.pad1[1] = __float_as_uint(e.y); w.pad1[2] = __float_as_uint(e.z);
+        V3 pos, nrm;
+        knn_query<BATCH>(S, w, pos, nrm);
+        knn_store(w, photon_knn_irradiance(S, pos, nrm, heap, BLOCK));
     }
 }
 
@@ -944,6 +979,11 @@ struct rt_ctx {
     double extend_ms = 0.0;
     uint64_t extend_launches = 0;
     uint32_t* prog_ctl = nullptr;             // rt_render_progressive: stopped, converged count, passes run
+    // second wavefront stream and its queues (rt_wavefront_opts.streams = 2)
+    hipStream_t stream2 = nullptr;
+    void* wf_mem2 = nullptr;
+    size_t wf_bytes2 = 0;
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
 };
 
 struct rt_scene {
@@ -1103,6 +1143,10 @@ void rt_ctx_destroy(rt_ctx* ctx) {
     if (ctx->counters) (void)hipFree(ctx->counters);
     if (ctx->wf_mem) (void)hipFree(ctx->wf_mem);
     if (ctx->prog_ctl) (void)hipFree(ctx->prog_ctl);
+    if (ctx->wf_mem2) (void)hipFree(ctx->wf_mem2);
+    if (ctx->ev_fork) (void)hipEventDestroy(ctx->ev_fork);
+    if (ctx->ev_join) (void)hipEventDestroy(ctx->ev_join);
+    if (ctx->stream2) (void)hipStreamDestroy(ctx->stream2);
     for (hipEvent_t e : ctx->ext_ev) (void)hipEventDestroy(e);
     if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
     if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
@@ -1899,25 +1943,28 @@ int rt_render_progressive(rt_ctx* ctx, rt_scene* s, const rt_camera* cam, const 
 }  // extern "C"
 
 // ---- wavefront renderer --------------------------------------------------------------------------
-static int wf_buffers(rt_ctx* ctx, uint32_t cap, WfQueues& Q) {
+static int wf_buffers(rt_ctx* ctx, uint32_t cap, WfQueues& Q, int slot = 0) {
     auto al = [](size_t x) { return (x + 255) & ~(size_t)255; };
     size_t b_paths = al((size_t)cap * sizeof(WfPath));
     size_t b_q = al((size_t)cap * 4);
     size_t b_buckets = al((size_t)WF_NB * cap * 4);
     size_t b_shadow = al((size_t)3 * cap * sizeof(WfShadow));
     size_t b_cnt = 256;
-    size_t total = b_paths + 2 * b_q + b_buckets + b_shadow + b_cnt;
-    if (ctx->wf_bytes < total) {
-        if (ctx->wf_mem) HIPCHECK(ctx, hipFree(ctx->wf_mem));
-        ctx->wf_mem = nullptr; ctx->wf_bytes = 0;
-        hipError_t e = hipMalloc(&ctx->wf_mem, total);
+    size_t total = b_paths + 3 * b_q + b_buckets + b_shadow + b_cnt;
+    void*& mem = slot ? ctx->wf_mem2 : ctx->wf_mem;
+    size_t& bytes = slot ? ctx->wf_bytes2 : ctx->wf_bytes;
+    if (bytes < total) {
+        if (mem) HIPCHECK(ctx, hipFree(mem));
+        mem = nullptr; bytes = 0;
+        hipError_t e = hipMalloc(&mem, total);
         if (e != hipSuccess) return fail(ctx, RT_ERR_NOMEM, std::string("wavefront buffers: ") + hipGetErrorString(e));
-        ctx->wf_bytes = total;
+        bytes = total;
     }
-    char* m = (char*)ctx->wf_mem;
+    char* m = (char*)mem;
     Q.paths = (WfPath*)m; m += b_paths;
     Q.active = (uint32_t*)m; m += b_q;
     Q.next = (uint32_t*)m; m += b_q;
+    Q.dense = (uint32_t*)m; m += b_q;
     Q.buckets = (uint32_t*)m; m += b_buckets;
     Q.shadow = (WfShadow*)m; m += b_shadow;
     Q.counters = (uint32_t*)m;
@@ -1945,9 +1992,8 @@ static void launch_shadow(int occ, uint32_t grid, hipStream_t st, const DScene& 
 // Launch one chunk: generate `n` paths at global position `first` (~0: read
 // from the counters, for graph replays) and run `depth` bounces.
 template <bool BT>
-static void wf_chunk(rt_ctx* ctx, const DScene& S, const DCamera& C, const DParams& P, const WfQueues& Q0,
+static void wf_chunk(rt_ctx* ctx, hipStream_t st, const DScene& S, const DCamera& C, const DParams& P, const WfQueues& Q0,
                      uint64_t first, uint32_t n, uint32_t grid, int occ, uint32_t depth, const WfBatch& Bt, bool knn) {
-    hipStream_t st = ctx->stream;
     hipLaunchKernelGGL((k_wf_generate<BT>), dim3((n + BLOCK - 1) / BLOCK), dim3(BLOCK), 0, st, C, P, Q0, first, n, Bt);
     hipLaunchKernelGGL(k_wf_reset, dim3(1), dim3(32), 0, st, Q0.counters, n);
     WfQueues A = Q0, B = Q0;
@@ -1961,7 +2007,10 @@ static void wf_chunk(rt_ctx* ctx, const DScene& S, const DCamera& C, const DPara
         if (ev) (void)hipEventRecord(ctx->ext_ev[ctx->ext_used], st);
         launch_extend<BT>(occ, grid, st, S, Q, Bt);
         if (ev) { (void)hipEventRecord(ctx->ext_ev[ctx->ext_used + 1], st); ctx->ext_used += 2; }
-        if (knn) hipLaunchKernelGGL((k_wf_knn<BT>), dim3(grid), dim3(BLOCK), 0, st, S, P, Q, Bt);
+        if (knn) {
+            hipLaunchKernelGGL((k_wf_knn<BT>), dim3(grid), dim3(BLOCK), 0, st, S, P, Q, Bt);
+            hipLaunchKernelGGL((k_wf_knn_dense<BT>), dim3(grid), dim3(BLOCK), 0, st, S, Q, Bt);
+        }
         if (Q.sort == 2) {
             hipLaunchKernelGGL((k_wf_shade<BT, 1>), dim3(grid), dim3(BLOCK), 0, st, S, P, Q, (int)d, Bt);
             hipLaunchKernelGGL((k_wf_shade<BT, 2>), dim3(grid), dim3(BLOCK), 0, st, S, P, Q, (int)d, Bt);
@@ -1973,7 +2022,7 @@ static void wf_chunk(rt_ctx* ctx, const DScene& S, const DCamera& C, const DPara
     }
 }
 
-struct WfOpts { uint32_t cap, grid, sort, use_graph; int occ; };
+struct WfOpts { uint32_t cap, grid, sort, use_graph, streams; int occ; };
 
 static int wf_opts(rt_ctx* ctx, const rt_wavefront_opts* opts, uint64_t total, WfOpts& o) {
     o.cap = opts && opts->capacity ? opts->capacity : (1u << 22);
@@ -1981,6 +2030,8 @@ static int wf_opts(rt_ctx* ctx, const rt_wavefront_opts* opts, uint64_t total, W
     o.sort = opts ? opts->sort : 1u;
     o.use_graph = opts ? opts->use_graph : 0u;
     o.occ = opts ? (int)opts->occupancy : 0;
+    o.streams = opts && opts->streams == 2 ? 2u : 1u;
+    if (opts && opts->streams > 2) return fail(ctx, RT_ERR_INVALID, "wavefront streams must be 0, 1 or 2");
     if (o.cap < BLOCK || o.cap > (1u << 26)) return fail(ctx, RT_ERR_INVALID, "wavefront capacity out of range");
     if (o.sort > 2) return fail(ctx, RT_ERR_INVALID, "wavefront sort must be 0, 1 or 2");
     if (o.grid == 0 || o.grid > (1u << 20)) return fail(ctx, RT_ERR_INVALID, "wavefront grid out of range");
@@ -1988,32 +2039,66 @@ static int wf_opts(rt_ctx* ctx, const rt_wavefront_opts* opts, uint64_t total, W
     return RT_OK;
 }
 
-// Every chunk of the global path range through every bounce.  fb must be zeroed.
+// The second stream and its events, created on first use.
+static int wf_stream2(rt_ctx* ctx) {
+    if (!ctx->stream2) HIPCHECK(ctx, hipStreamCreate(&ctx->stream2));
+    if (!ctx->ev_fork) HIPCHECK(ctx, hipEventCreateWithFlags(&ctx->ev_fork, hipEventDisableTiming));
+    if (!ctx->ev_join) HIPCHECK(ctx, hipEventCreateWithFlags(&ctx->ev_join, hipEventDisableTiming));
+    return RT_OK;
+}
+
+// Every chunk of the global path range through every bounce.  fb must be
+// zeroed.  With Q2 (two streams), chunk c runs on stream c % 2 with that
+// stream's queues; both add into the same accumulator; the second stream
+// starts after the first's earlier work and the first waits for it at the end.
 template <bool BT>
 static int wf_run(rt_ctx* ctx, const DScene& S, const DCamera& C, const DParams& P, WfQueues& Q, const WfOpts& o,
-                  uint64_t total, uint32_t depth, const WfBatch& Bt, bool knn) {
+                  uint64_t total, uint32_t depth, const WfBatch& Bt, bool knn, WfQueues* Q2 = nullptr) {
     uint32_t cap = o.cap;
     uint64_t full = total / cap, rem = total % cap;
-    if (ctx->timing) HIPCHECK(ctx, hipEventRecord(ctx->ev0, ctx->stream));
-    if (o.use_graph && full > 0) {
-        hipGraph_t graph = nullptr;
-        hipGraphExec_t exec = nullptr;
-        hipLaunchKernelGGL(k_wf_set_first, dim3(1), dim3(32), 0, ctx->stream, Q.counters, (uint64_t)0);
-        HIPCHECK(ctx, hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
-        wf_chunk<BT>(ctx, S, C, P, Q, ~0ull, cap, o.grid, o.occ, depth, Bt, knn);
-        hipLaunchKernelGGL(k_wf_bump_first, dim3(1), dim3(32), 0, ctx->stream, Q.counters, cap);
-        HIPCHECK(ctx, hipStreamEndCapture(ctx->stream, &graph));
-        HIPCHECK(ctx, hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
-        for (uint64_t c = 0; c < full; c++) HIPCHECK(ctx, hipGraphLaunch(exec, ctx->stream));
-        HIPCHECK(ctx, hipGraphExecDestroy(exec));
-        HIPCHECK(ctx, hipGraphDestroy(graph));
-    } else {
-        for (uint64_t c = 0; c < full; c++) wf_chunk<BT>(ctx, S, C, P, Q, c * (uint64_t)cap, cap, o.grid, o.occ, depth, Bt, knn);
+    const bool two = Q2 && full + (rem ? 1 : 0) >= 2;
+    hipStream_t s1 = ctx->stream, s2 = two ? ctx->stream2 : ctx->stream;
+    auto lane_of = [&](uint64_t c) { return two && (c & 1); };
+    if (ctx->timing) HIPCHECK(ctx, hipEventRecord(ctx->ev0, s1));
+    if (two) {
+        HIPCHECK(ctx, hipEventRecord(ctx->ev_fork, s1));
+        HIPCHECK(ctx, hipStreamWaitEvent(s2, ctx->ev_fork, 0));
     }
-    if (rem) wf_chunk<BT>(ctx, S, C, P, Q, full * (uint64_t)cap, (uint32_t)rem, o.grid, o.occ, depth, Bt, knn);
+    if (o.use_graph && full > 0) {
+        const bool g2 = two && full >= 2;
+        hipGraph_t graph[2] = {nullptr, nullptr};
+        hipGraphExec_t exec[2] = {nullptr, nullptr};
+        for (int k = 0; k < (g2 ? 2 : 1); k++) {
+            hipStream_t sk = k ? s2 : s1;
+            WfQueues& Qk = k ? *Q2 : Q;
+            hipLaunchKernelGGL(k_wf_set_first, dim3(1), dim3(32), 0, sk, Qk.counters, (uint64_t)k * cap);
+            HIPCHECK(ctx, hipStreamBeginCapture(sk, hipStreamCaptureModeThreadLocal));
+            wf_chunk<BT>(ctx, sk, S, C, P, Qk, ~0ull, cap, o.grid, o.occ, depth, Bt, knn);
+            hipLaunchKernelGGL(k_wf_bump_first, dim3(1), dim3(32), 0, sk, Qk.counters, g2 ? 2 * cap : cap);
+            HIPCHECK(ctx, hipStreamEndCapture(sk, &graph[k]));
+            HIPCHECK(ctx, hipGraphInstantiate(&exec[k], graph[k], nullptr, nullptr, 0));
+        }
+        for (uint64_t c = 0; c < full; c++) {
+            int k = g2 && (c & 1) ? 1 : 0;
+            HIPCHECK(ctx, hipGraphLaunch(exec[k], k ? s2 : s1));
+        }
+        for (int k = 0; k < 2; k++) {
+            if (exec[k]) HIPCHECK(ctx, hipGraphExecDestroy(exec[k]));
+            if (graph[k]) HIPCHECK(ctx, hipGraphDestroy(graph[k]));
+        }
+    } else {
+        for (uint64_t c = 0; c < full; c++)
+            wf_chunk<BT>(ctx, lane_of(c) ? s2 : s1, S, C, P, lane_of(c) ? *Q2 : Q, c * (uint64_t)cap, cap, o.grid, o.occ, depth, Bt, knn);
+    }
+    if (rem) wf_chunk<BT>(ctx, lane_of(full) ? s2 : s1, S, C, P, lane_of(full) ? *Q2 : Q, full * (uint64_t)cap, (uint32_t)rem,
+                          o.grid, o.occ, depth, Bt, knn);
     HIPCHECK(ctx, hipGetLastError());
+    if (two) {
+        HIPCHECK(ctx, hipEventRecord(ctx->ev_join, s2));
+        HIPCHECK(ctx, hipStreamWaitEvent(s1, ctx->ev_join, 0));
+    }
     if (ctx->timing) {
-        HIPCHECK(ctx, hipEventRecord(ctx->ev1, ctx->stream));
+        HIPCHECK(ctx, hipEventRecord(ctx->ev1, s1));
         HIPCHECK(ctx, hipEventSynchronize(ctx->ev1));
         float ms = 0.0f;
         HIPCHECK(ctx, hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
@@ -2025,6 +2110,19 @@ static int wf_run(rt_ctx* ctx, const DScene& S, const DCamera& C, const DParams&
         }
     }
     ctx->ext_used = 0;
+    return RT_OK;
+}
+
+// Queues of the second stream when the options ask for two.
+static int wf_second(rt_ctx* ctx, const WfOpts& o, const WfQueues& Q, WfQueues& Q2, WfQueues*& out) {
+    out = nullptr;
+    if (o.streams != 2) return RT_OK;
+    int rc;
+    if ((rc = wf_stream2(ctx))) return rc;
+    if ((rc = wf_buffers(ctx, o.cap, Q2, 1))) return rc;
+    Q2.sort = Q.sort;
+    Q2.fb = Q.fb;
+    out = &Q2;
     return RT_OK;
 }
 
@@ -2052,7 +2150,9 @@ extern "C" int rt_render_wavefront(rt_ctx* ctx, rt_scene* s, const rt_camera* ca
     P.counters = nullptr;
     HIPCHECK(ctx, hipMemsetAsync(rout.dev, 0, (size_t)npix * 12, ctx->stream));
     WfBatch none = {nullptr, nullptr, 0};
-    if ((rc = wf_run<false>(ctx, s->S, C, P, Q, o, total, P.max_depth, none, P.use_photons && s->S.ph_knn))) return rc;
+    WfQueues Q2, *q2;
+    if ((rc = wf_second(ctx, o, Q, Q2, q2))) return rc;
+    if ((rc = wf_run<false>(ctx, s->S, C, P, Q, o, total, P.max_depth, none, P.use_photons && s->S.ph_knn, q2))) return rc;
     return finish_out(ctx, out, (size_t)npix * 12, rout);
 }
 
@@ -2103,7 +2203,9 @@ extern "C" int rt_render_batch(rt_ctx* ctx, const rt_batch_view* views, uint32_t
     WfBatch Bt = {(const DScene*)(side + bv), (const DView*)side, nviews};
     DCamera C = dv[0].cam;
     DParams P = dv[0].P;
-    if ((rc = wf_run<true>(ctx, ds[0], C, P, Q, o, total, depth, Bt, knn))) return rc;
+    WfQueues Q2, *q2;
+    if ((rc = wf_second(ctx, o, Q, Q2, q2))) return rc;
+    if ((rc = wf_run<true>(ctx, ds[0], C, P, Q, o, total, depth, Bt, knn, q2))) return rc;
     HIPCHECK(ctx, hipStreamSynchronize(ctx->stream));
     for (uint32_t v = 0; v < nviews; v++) {
         const float* src = Q.fb + 3 * (size_t)dv[v].fb_off;
@@ -2362,7 +2464,7 @@ extern "C" int rt_backward_wavefront(rt_ctx* ctx, rt_scene* s, const rt_camera* 
         HIPCHECK(ctx, hipMemsetAsync(side + b_adj, 0, (size_t)m * 12, st));
         WfQueues Qp = Q;
         Qp.lsamp = (float*)(side + b_adj);
-        wf_chunk<false>(ctx, s->S, C, P, Qp, first, m, o.grid, o.occ, depth, none, false);   // no k-NN maps here (render_args)
+        wf_chunk<false>(ctx, st, s->S, C, P, Qp, first, m, o.grid, o.occ, depth, none, false);   // no k-NN maps here (render_args)
         // replay from the same streams
         hipLaunchKernelGGL((k_wf_generate<false>), dim3((m + BLOCK - 1) / BLOCK), dim3(BLOCK), 0, st, C, P, Q, first, m, none);
         hipLaunchKernelGGL(k_wfb_init, dim3((m + BLOCK - 1) / BLOCK), dim3(BLOCK), 0, st, Q, A, m);

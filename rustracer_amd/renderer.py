@@ -217,17 +217,17 @@ class Renderer:
 
     def render_wavefront(self, out, camera, width, height, spp, sample_idx=0, strata=0, bg_scale=1.0,
                          photon_map=False, max_depth=50, seed=0, capacity=0, grid=0, sort=True, use_graph=False,
-                         occupancy=0, scene=None):
+                         occupancy=0, streams=1, scene=None):
         """Same frame as render_accumulate, through the wavefront stage kernels."""
         p = self.params(width, height, sample_idx, strata, bg_scale, max_depth, None, photon_map, seed)
         cam = camera.to_c()
         o = F.rt_wavefront_opts(capacity=capacity, grid=grid, sort=int(sort), use_graph=1 if use_graph else 0,
-                                occupancy=occupancy)
+                                occupancy=occupancy, streams=streams)
         self._check(self.L.rt_render_wavefront(self.ctx, self.scene if scene is None else scene, C.byref(cam), C.byref(p), spp, _ptr(out), C.byref(o)),
                     "rt_render_wavefront")
         return out
 
-    def render_batch(self, views, capacity=0, grid=0, sort=True, use_graph=False, occupancy=0):
+    def render_batch(self, views, capacity=0, grid=0, sort=True, use_graph=False, occupancy=0, streams=1):
         """views: list of dicts with keys scene (handle), camera, width, height,
         spp, out, and optionally max_depth, photon_map, seed, bg_scale."""
         arr = (F.rt_batch_view * len(views))()
@@ -239,7 +239,7 @@ class Renderer:
             arr[i].spp = v["spp"]
             arr[i].accum_rgb = _ptr(v["out"])
         o = F.rt_wavefront_opts(capacity=capacity, grid=grid, sort=int(sort), use_graph=1 if use_graph else 0,
-                                occupancy=occupancy)
+                                occupancy=occupancy, streams=streams)
         self._check(self.L.rt_render_batch(self.ctx, arr, len(views), C.byref(o)), "rt_render_batch")
         return [v["out"] for v in views]
 

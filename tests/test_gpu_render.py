@@ -185,19 +185,22 @@ def test_gem_with_photons_three_sigma(gpu_renderer):
     assert (z > 3).mean() <= 0.02 and (z > 6).mean() <= 0.002, ((z > 3).mean(), (z > 6).mean())
 
 
-@pytest.mark.parametrize("sort,graph,cap,occ", [(1, False, 0, 0), (0, False, 4096, 4), (1, True, 8192, 5),
-                                                 (1, True, 2048, 0), (2, False, 4096, 0), (2, True, 2048, 4)])
-def test_wavefront_equals_megakernel(gpu_renderer, sort, graph, cap, occ):
+@pytest.mark.parametrize("sort,graph,cap,occ,streams", [(1, False, 0, 0, 1), (0, False, 4096, 4, 1), (1, True, 8192, 5, 1),
+                                                         (1, True, 2048, 0, 1), (2, False, 4096, 0, 1), (2, True, 2048, 4, 1),
+                                                         (1, False, 2048, 0, 2), (2, True, 2048, 0, 2), (1, True, 4096, 0, 2)])
+def test_wavefront_equals_megakernel(gpu_renderer, sort, graph, cap, occ, streams):
     """Same per-pixel streams, same bounce code: the wavefront frame equals the
     megakernel frame up to the order of float atomic additions.  Capacity 2048
-    with the graph replays one captured chunk seven times."""
+    with the graph replays one captured chunk seven times; with two streams,
+    two captured chunks alternate (four and three replays) and a remainder
+    chunk follows on the second stream at capacity 4096."""
     scene, cam, cfg = scenes.box_room(48, 40)
     gpu_renderer.upload(scene)
     a = np.zeros((40, 48, 3), np.float32)
     b = np.zeros((40, 48, 3), np.float32)
     gpu_renderer.render_accumulate(a, cam, 48, 40, 8, max_depth=cfg["max_depth"], seed=5)
     gpu_renderer.render_wavefront(b, cam, 48, 40, 8, max_depth=cfg["max_depth"], seed=5, sort=sort, use_graph=graph,
-                                  capacity=cap, occupancy=occ)
+                                  capacity=cap, occupancy=occ, streams=streams)
     assert np.allclose(a, b, rtol=1e-4, atol=1e-5), np.abs(a - b).max()
 
 
@@ -252,8 +255,8 @@ def test_volume_glass_box_same_streams_and_three_sigma(gpu_renderer):
     assert (z > 3).mean() <= 0.02 and (z > 6).mean() <= 0.002, ((z > 3).mean(), (z > 6).mean())
 
 
-@pytest.mark.parametrize("graph,cap", [(False, 4096), (True, 2048)])
-def test_batch_equals_individual_wavefront_renders(gpu_renderer, graph, cap):
+@pytest.mark.parametrize("graph,cap,streams", [(False, 4096, 1), (True, 2048, 1), (True, 2048, 2), (False, 2048, 2)])
+def test_batch_equals_individual_wavefront_renders(gpu_renderer, graph, cap, streams):
     """One batched launch over views of uneven resolution, spp, depth, seed and
     scene (gem spin stills with their own photon maps, plus the volume box)
     gives each view the frame its own wavefront render gives."""
@@ -271,7 +274,7 @@ def test_batch_equals_individual_wavefront_renders(gpu_renderer, graph, cap):
         views.append(dict(scene=r.upload_extra(sc), camera=cam, width=32, height=24, spp=5, max_depth=12, seed=11))
         for v in views:
             v["out"] = np.zeros((v["height"], v["width"], 3), np.float32)
-        r.render_batch(views, capacity=cap, use_graph=graph)
+        r.render_batch(views, capacity=cap, use_graph=graph, streams=streams)
         for v in views:
             ref = np.zeros_like(v["out"])
             r.render_wavefront(ref, v["camera"], v["width"], v["height"], v["spp"], max_depth=v["max_depth"],
