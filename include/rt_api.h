@@ -314,9 +314,11 @@ typedef struct {
  * use_graph: 1 captures one chunk's launches in a hipGraph and replays it;
  * occupancy: register budget of the traversal stages — 0 compiler's choice,
  * 4 or 5: at least that many waves per SIMD (fewer VGPRs, some spills);
- * streams: 2 runs consecutive chunks on two HIP streams with queues of their
- * own, so a chunk's last bounces (few paths, latency-bound) overlap the next
- * chunk's first ones; 0 or 1: one stream.                                    */
+ * streams: n in 2..RT_WF_MAX_STREAMS runs consecutive chunks round-robin on
+ * n HIP streams, each with queues of its own, so a chunk's last bounces (few
+ * paths, latency-bound) overlap the next chunks' first ones; 0 or 1: one
+ * stream.                                                                    */
+#define RT_WF_MAX_STREAMS 4
 typedef struct {
     uint32_t capacity;
     uint32_t grid;

@@ -69,7 +69,7 @@ def tune(r, run, log=None):
     t0 = time.time()
     best_ms = measure(best)
     for key, values in (("occupancy", [0, 4, 5]), ("capacity", CAPACITIES), ("grid", GRIDS), ("sort", [0, 1, 2]),
-                        ("use_graph", [True, False]), ("streams", [1, 2])):
+                        ("use_graph", [True, False]), ("streams", [1, 2, 3, 4])):
         for v in values:
             if v == best[key]:
                 continue

@@ -979,11 +979,11 @@ struct rt_ctx {
     double extend_ms = 0.0;
     uint64_t extend_launches = 0;
     uint32_t* prog_ctl = nullptr;             // rt_render_progressive: stopped, converged count, passes run
-    // second wavefront stream and its queues (rt_wavefront_opts.streams = 2)
-    hipStream_t stream2 = nullptr;
-    void* wf_mem2 = nullptr;
-    size_t wf_bytes2 = 0;
-    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+    // extra wavefront streams and their queues (rt_wavefront_opts.streams > 1)
+    hipStream_t xstream[RT_WF_MAX_STREAMS - 1] = {};
+    void* xwf_mem[RT_WF_MAX_STREAMS - 1] = {};
+    size_t xwf_bytes[RT_WF_MAX_STREAMS - 1] = {};
+    hipEvent_t ev_fork = nullptr, ev_join[RT_WF_MAX_STREAMS - 1] = {};
 };
 
 struct rt_scene {
@@ -1143,10 +1143,12 @@ void rt_ctx_destroy(rt_ctx* ctx) {
     if (ctx->counters) (void)hipFree(ctx->counters);
     if (ctx->wf_mem) (void)hipFree(ctx->wf_mem);
     if (ctx->prog_ctl) (void)hipFree(ctx->prog_ctl);
-    if (ctx->wf_mem2) (void)hipFree(ctx->wf_mem2);
+    for (int k = 0; k < RT_WF_MAX_STREAMS - 1; k++) {
+        if (ctx->xwf_mem[k]) (void)hipFree(ctx->xwf_mem[k]);
+        if (ctx->ev_join[k]) (void)hipEventDestroy(ctx->ev_join[k]);
+        if (ctx->xstream[k]) (void)hipStreamDestroy(ctx->xstream[k]);
+    }
     if (ctx->ev_fork) (void)hipEventDestroy(ctx->ev_fork);
-    if (ctx->ev_join) (void)hipEventDestroy(ctx->ev_join);
-    if (ctx->stream2) (void)hipStreamDestroy(ctx->stream2);
     for (hipEvent_t e : ctx->ext_ev) (void)hipEventDestroy(e);
     if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
     if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
@@ -1951,8 +1953,8 @@ static int wf_buffers(rt_ctx* ctx, uint32_t cap, WfQueues& Q, int slot = 0) {
     size_t b_shadow = al((size_t)3 * cap * sizeof(WfShadow));
     size_t b_cnt = 256;
     size_t total = b_paths + 3 * b_q + b_buckets + b_shadow + b_cnt;
-    void*& mem = slot ? ctx->wf_mem2 : ctx->wf_mem;
-    size_t& bytes = slot ? ctx->wf_bytes2 : ctx->wf_bytes;
+    void*& mem = slot ? ctx->xwf_mem[slot - 1] : ctx->wf_mem;
+    size_t& bytes = slot ? ctx->xwf_bytes[slot - 1] : ctx->wf_bytes;
     if (bytes < total) {
         if (mem) HIPCHECK(ctx, hipFree(mem));
         mem = nullptr; bytes = 0;
@@ -2030,8 +2032,8 @@ static int wf_opts(rt_ctx* ctx, const rt_wavefront_opts* opts, uint64_t total, W
     o.sort = opts ? opts->sort : 1u;
     o.use_graph = opts ? opts->use_graph : 0u;
     o.occ = opts ? (int)opts->occupancy : 0;
-    o.streams = opts && opts->streams == 2 ? 2u : 1u;
-    if (opts && opts->streams > 2) return fail(ctx, RT_ERR_INVALID, "wavefront streams must be 0, 1 or 2");
+    o.streams = opts && opts->streams > 1 ? opts->streams : 1u;
+    if (o.streams > RT_WF_MAX_STREAMS) return fail(ctx, RT_ERR_INVALID, "wavefront streams above RT_WF_MAX_STREAMS");
     if (o.cap < BLOCK || o.cap > (1u << 26)) return fail(ctx, RT_ERR_INVALID, "wavefront capacity out of range");
     if (o.sort > 2) return fail(ctx, RT_ERR_INVALID, "wavefront sort must be 0, 1 or 2");
     if (o.grid == 0 || o.grid > (1u << 20)) return fail(ctx, RT_ERR_INVALID, "wavefront grid out of range");
@@ -2039,71 +2041,75 @@ static int wf_opts(rt_ctx* ctx, const rt_wavefront_opts* opts, uint64_t total, W
     return RT_OK;
 }
 
-// The second stream and its events, created on first use.
-static int wf_stream2(rt_ctx* ctx) {
-    if (!ctx->stream2) HIPCHECK(ctx, hipStreamCreate(&ctx->stream2));
+// The extra streams and their events, created on first use.
+static int wf_streams(rt_ctx* ctx, uint32_t n) {
     if (!ctx->ev_fork) HIPCHECK(ctx, hipEventCreateWithFlags(&ctx->ev_fork, hipEventDisableTiming));
-    if (!ctx->ev_join) HIPCHECK(ctx, hipEventCreateWithFlags(&ctx->ev_join, hipEventDisableTiming));
+    for (uint32_t k = 0; k + 1 < n; k++) {
+        if (!ctx->xstream[k]) HIPCHECK(ctx, hipStreamCreate(&ctx->xstream[k]));
+        if (!ctx->ev_join[k]) HIPCHECK(ctx, hipEventCreateWithFlags(&ctx->ev_join[k], hipEventDisableTiming));
+    }
     return RT_OK;
 }
 
 // Every chunk of the global path range through every bounce.  fb must be
-// zeroed.  With Q2 (two streams), chunk c runs on stream c % 2 with that
-// stream's queues; both add into the same accumulator; the second stream
-// starts after the first's earlier work and the first waits for it at the end.
+// zeroed.  With L lanes (Qs[0..L), L > 1: more streams), chunk c runs on lane
+// c % L: that lane's stream and queues; all add into the same accumulator.
+// The extra streams start after the context stream's earlier work, and the
+// context stream waits for all of them at the end.
 template <bool BT>
-static int wf_run(rt_ctx* ctx, const DScene& S, const DCamera& C, const DParams& P, WfQueues& Q, const WfOpts& o,
-                  uint64_t total, uint32_t depth, const WfBatch& Bt, bool knn, WfQueues* Q2 = nullptr) {
+static int wf_run(rt_ctx* ctx, const DScene& S, const DCamera& C, const DParams& P, WfQueues* Qs, uint32_t lanes,
+                  const WfOpts& o, uint64_t total, uint32_t depth, const WfBatch& Bt, bool knn) {
     uint32_t cap = o.cap;
     uint64_t full = total / cap, rem = total % cap;
-    const bool two = Q2 && full + (rem ? 1 : 0) >= 2;
-    hipStream_t s1 = ctx->stream, s2 = two ? ctx->stream2 : ctx->stream;
-    auto lane_of = [&](uint64_t c) { return two && (c & 1); };
-    if (ctx->timing) HIPCHECK(ctx, hipEventRecord(ctx->ev0, s1));
-    if (two) {
-        HIPCHECK(ctx, hipEventRecord(ctx->ev_fork, s1));
-        HIPCHECK(ctx, hipStreamWaitEvent(s2, ctx->ev_fork, 0));
+    uint64_t nchunks = full + (rem ? 1 : 0);
+    const uint32_t L = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(lanes, nchunks));
+    auto stream_of = [&](uint32_t k) { return k ? ctx->xstream[k - 1] : ctx->stream; };
+    hipStream_t s0 = ctx->stream;
+    if (ctx->timing) HIPCHECK(ctx, hipEventRecord(ctx->ev0, s0));
+    if (L > 1) {
+        HIPCHECK(ctx, hipEventRecord(ctx->ev_fork, s0));
+        for (uint32_t k = 1; k < L; k++) HIPCHECK(ctx, hipStreamWaitEvent(stream_of(k), ctx->ev_fork, 0));
     }
     if (o.use_graph && full > 0) {
-        const bool g2 = two && full >= 2;
-        hipGraph_t graph[2] = {nullptr, nullptr};
-        hipGraphExec_t exec[2] = {nullptr, nullptr};
-        for (int k = 0; k < (g2 ? 2 : 1); k++) {
-            hipStream_t sk = k ? s2 : s1;
-            WfQueues& Qk = k ? *Q2 : Q;
-            hipLaunchKernelGGL(k_wf_set_first, dim3(1), dim3(32), 0, sk, Qk.counters, (uint64_t)k * cap);
+        const uint32_t G = (uint32_t)std::min<uint64_t>(L, full);     // lanes that replay a captured chunk
+        hipGraph_t graph[RT_WF_MAX_STREAMS] = {};
+        hipGraphExec_t exec[RT_WF_MAX_STREAMS] = {};
+        for (uint32_t k = 0; k < G; k++) {
+            hipStream_t sk = stream_of(k);
+            hipLaunchKernelGGL(k_wf_set_first, dim3(1), dim3(32), 0, sk, Qs[k].counters, (uint64_t)k * cap);
             HIPCHECK(ctx, hipStreamBeginCapture(sk, hipStreamCaptureModeThreadLocal));
-            wf_chunk<BT>(ctx, sk, S, C, P, Qk, ~0ull, cap, o.grid, o.occ, depth, Bt, knn);
-            hipLaunchKernelGGL(k_wf_bump_first, dim3(1), dim3(32), 0, sk, Qk.counters, g2 ? 2 * cap : cap);
+            wf_chunk<BT>(ctx, sk, S, C, P, Qs[k], ~0ull, cap, o.grid, o.occ, depth, Bt, knn);
+            hipLaunchKernelGGL(k_wf_bump_first, dim3(1), dim3(32), 0, sk, Qs[k].counters, G * cap);
             HIPCHECK(ctx, hipStreamEndCapture(sk, &graph[k]));
             HIPCHECK(ctx, hipGraphInstantiate(&exec[k], graph[k], nullptr, nullptr, 0));
         }
-        for (uint64_t c = 0; c < full; c++) {
-            int k = g2 && (c & 1) ? 1 : 0;
-            HIPCHECK(ctx, hipGraphLaunch(exec[k], k ? s2 : s1));
-        }
-        for (int k = 0; k < 2; k++) {
-            if (exec[k]) HIPCHECK(ctx, hipGraphExecDestroy(exec[k]));
-            if (graph[k]) HIPCHECK(ctx, hipGraphDestroy(graph[k]));
+        for (uint64_t c = 0; c < full; c++) HIPCHECK(ctx, hipGraphLaunch(exec[c % G], stream_of((uint32_t)(c % G))));
+        for (uint32_t k = 0; k < G; k++) {
+            HIPCHECK(ctx, hipGraphExecDestroy(exec[k]));
+            HIPCHECK(ctx, hipGraphDestroy(graph[k]));
         }
     } else {
-        for (uint64_t c = 0; c < full; c++)
-            wf_chunk<BT>(ctx, lane_of(c) ? s2 : s1, S, C, P, lane_of(c) ? *Q2 : Q, c * (uint64_t)cap, cap, o.grid, o.occ, depth, Bt, knn);
+        for (uint64_t c = 0; c < full; c++) {
+            uint32_t k = (uint32_t)(c % L);
+            wf_chunk<BT>(ctx, stream_of(k), S, C, P, Qs[k], c * (uint64_t)cap, cap, o.grid, o.occ, depth, Bt, knn);
+        }
     }
-    if (rem) wf_chunk<BT>(ctx, lane_of(full) ? s2 : s1, S, C, P, lane_of(full) ? *Q2 : Q, full * (uint64_t)cap, (uint32_t)rem,
-                          o.grid, o.occ, depth, Bt, knn);
+    if (rem) {
+        uint32_t k = (uint32_t)(full % L);
+        wf_chunk<BT>(ctx, stream_of(k), S, C, P, Qs[k], full * (uint64_t)cap, (uint32_t)rem, o.grid, o.occ, depth, Bt, knn);
+    }
     HIPCHECK(ctx, hipGetLastError());
-    if (two) {
-        HIPCHECK(ctx, hipEventRecord(ctx->ev_join, s2));
-        HIPCHECK(ctx, hipStreamWaitEvent(s1, ctx->ev_join, 0));
+    for (uint32_t k = 1; k < L; k++) {
+        HIPCHECK(ctx, hipEventRecord(ctx->ev_join[k - 1], stream_of(k)));
+        HIPCHECK(ctx, hipStreamWaitEvent(s0, ctx->ev_join[k - 1], 0));
     }
     if (ctx->timing) {
-        HIPCHECK(ctx, hipEventRecord(ctx->ev1, s1));
+        HIPCHECK(ctx, hipEventRecord(ctx->ev1, s0));
         HIPCHECK(ctx, hipEventSynchronize(ctx->ev1));
         float ms = 0.0f;
         HIPCHECK(ctx, hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
         ctx->kernel_ms += ms;
-        ctx->launches += full + (rem ? 1 : 0);   // one "launch" = one chunk of paths through every bounce
+        ctx->launches += nchunks;   // one "launch" = one chunk of paths through every bounce
         for (size_t k = 0; k + 1 < ctx->ext_used; k += 2) {
             float e = 0.0f;
             if (hipEventElapsedTime(&e, ctx->ext_ev[k], ctx->ext_ev[k + 1]) == hipSuccess) { ctx->extend_ms += e; ctx->extend_launches++; }
@@ -2113,16 +2119,16 @@ static int wf_run(rt_ctx* ctx, const DScene& S, const DCamera& C, const DParams&
     return RT_OK;
 }
 
-// Queues of the second stream when the options ask for two.
-static int wf_second(rt_ctx* ctx, const WfOpts& o, const WfQueues& Q, WfQueues& Q2, WfQueues*& out) {
-    out = nullptr;
-    if (o.streams != 2) return RT_OK;
+// Queues of every lane the options ask for (Qs[0] already set up).
+static int wf_lanes(rt_ctx* ctx, const WfOpts& o, WfQueues* Qs) {
+    if (o.streams <= 1) return RT_OK;
     int rc;
-    if ((rc = wf_stream2(ctx))) return rc;
-    if ((rc = wf_buffers(ctx, o.cap, Q2, 1))) return rc;
-    Q2.sort = Q.sort;
-    Q2.fb = Q.fb;
-    out = &Q2;
+    if ((rc = wf_streams(ctx, o.streams))) return rc;
+    for (uint32_t k = 1; k < o.streams; k++) {
+        if ((rc = wf_buffers(ctx, o.cap, Qs[k], (int)k))) return rc;
+        Qs[k].sort = Qs[0].sort;
+        Qs[k].fb = Qs[0].fb;
+    }
     return RT_OK;
 }
 
@@ -2150,9 +2156,10 @@ extern "C" int rt_render_wavefront(rt_ctx* ctx, rt_scene* s, const rt_camera* ca
     P.counters = nullptr;
     HIPCHECK(ctx, hipMemsetAsync(rout.dev, 0, (size_t)npix * 12, ctx->stream));
     WfBatch none = {nullptr, nullptr, 0};
-    WfQueues Q2, *q2;
-    if ((rc = wf_second(ctx, o, Q, Q2, q2))) return rc;
-    if ((rc = wf_run<false>(ctx, s->S, C, P, Q, o, total, P.max_depth, none, P.use_photons && s->S.ph_knn, q2))) return rc;
+    WfQueues Qs[RT_WF_MAX_STREAMS];
+    Qs[0] = Q;
+    if ((rc = wf_lanes(ctx, o, Qs))) return rc;
+    if ((rc = wf_run<false>(ctx, s->S, C, P, Qs, o.streams, o, total, P.max_depth, none, P.use_photons && s->S.ph_knn))) return rc;
     return finish_out(ctx, out, (size_t)npix * 12, rout);
 }
 
@@ -2203,9 +2210,10 @@ extern "C" int rt_render_batch(rt_ctx* ctx, const rt_batch_view* views, uint32_t
     WfBatch Bt = {(const DScene*)(side + bv), (const DView*)side, nviews};
     DCamera C = dv[0].cam;
     DParams P = dv[0].P;
-    WfQueues Q2, *q2;
-    if ((rc = wf_second(ctx, o, Q, Q2, q2))) return rc;
-    if ((rc = wf_run<true>(ctx, ds[0], C, P, Q, o, total, depth, Bt, knn, q2))) return rc;
+    WfQueues Qs[RT_WF_MAX_STREAMS];
+    Qs[0] = Q;
+    if ((rc = wf_lanes(ctx, o, Qs))) return rc;
+    if ((rc = wf_run<true>(ctx, ds[0], C, P, Qs, o.streams, o, total, depth, Bt, knn))) return rc;
     HIPCHECK(ctx, hipStreamSynchronize(ctx->stream));
     for (uint32_t v = 0; v < nviews; v++) {
         const float* src = Q.fb + 3 * (size_t)dv[v].fb_off;

@@ -187,13 +187,14 @@ def test_gem_with_photons_three_sigma(gpu_renderer):
 
 @pytest.mark.parametrize("sort,graph,cap,occ,streams", [(1, False, 0, 0, 1), (0, False, 4096, 4, 1), (1, True, 8192, 5, 1),
                                                          (1, True, 2048, 0, 1), (2, False, 4096, 0, 1), (2, True, 2048, 4, 1),
-                                                         (1, False, 2048, 0, 2), (2, True, 2048, 0, 2), (1, True, 4096, 0, 2)])
+                                                         (1, False, 2048, 0, 2), (2, True, 2048, 0, 2), (1, True, 4096, 0, 2),
+                                                         (1, True, 2048, 0, 3), (0, False, 2048, 0, 4), (1, True, 4096, 0, 4)])
 def test_wavefront_equals_megakernel(gpu_renderer, sort, graph, cap, occ, streams):
     """Same per-pixel streams, same bounce code: the wavefront frame equals the
     megakernel frame up to the order of float atomic additions.  Capacity 2048
-    with the graph replays one captured chunk seven times; with two streams,
-    two captured chunks alternate (four and three replays) and a remainder
-    chunk follows on the second stream at capacity 4096."""
+    with the graph replays one captured chunk seven times; with n streams the
+    chunks go round-robin over n captured chunks (fewer when there are fewer
+    full chunks than streams) and the remainder follows on the next lane."""
     scene, cam, cfg = scenes.box_room(48, 40)
     gpu_renderer.upload(scene)
     a = np.zeros((40, 48, 3), np.float32)
@@ -255,7 +256,8 @@ def test_volume_glass_box_same_streams_and_three_sigma(gpu_renderer):
     assert (z > 3).mean() <= 0.02 and (z > 6).mean() <= 0.002, ((z > 3).mean(), (z > 6).mean())
 
 
-@pytest.mark.parametrize("graph,cap,streams", [(False, 4096, 1), (True, 2048, 1), (True, 2048, 2), (False, 2048, 2)])
+@pytest.mark.parametrize("graph,cap,streams", [(False, 4096, 1), (True, 2048, 1), (True, 2048, 2), (False, 2048, 2),
+                                               (True, 2048, 4)])
 def test_batch_equals_individual_wavefront_renders(gpu_renderer, graph, cap, streams):
     """One batched launch over views of uneven resolution, spp, depth, seed and
     scene (gem spin stills with their own photon maps, plus the volume box)
