@@ -332,12 +332,28 @@ def measure_adjoint(args, with_cpu):
     grad = torch.zeros(len(P), dtype=torch.float32, device="cuda")
     n = float(img.numel())
 
-    bopts = dict(capacity=opts["capacity"], grid=opts["grid"], sort=opts["sort"])
+    def step_with(o, k):
+        r.render_wavefront(img, cam, W, H, spp, max_depth=depth, seed=k, **o)
+        dl = (2.0 / n) * (img / spp - target)
+        r.backward(dl, cam, W, H, spp, P, max_depth=depth, seed=k, grad=grad, wavefront=True,
+                   capacity=o["capacity"], grid=o["grid"], sort=o["sort"], streams=o["streams"])
+
+    # chunk capacity x streams: a few probes of one evaluation each
+    probes = []
+    for cap, ns in ((1 << 23, 1), (1 << 22, 2), (1 << 22, 3), (1 << 21, 3), (1 << 21, 4)):
+        o = dict(opts, capacity=cap, streams=ns)
+        step_with(o, 500)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        step_with(o, 501)
+        torch.cuda.synchronize()
+        probes.append((time.perf_counter() - t0, o))
+    opts = min(probes, key=lambda x: x[0])[1]
+    bopts = dict(capacity=opts["capacity"], grid=opts["grid"], sort=opts["sort"], streams=opts["streams"])
+    log(f"[adjoint] options {opts} ({', '.join(f'{t * 1e3:.0f}' for t, _ in probes)} ms per probe)")
 
     def step(k):
-        r.render_wavefront(img, cam, W, H, spp, max_depth=depth, seed=k, **opts)
-        dl = (2.0 / n) * (img / spp - target)
-        r.backward(dl, cam, W, H, spp, P, max_depth=depth, seed=k, grad=grad, wavefront=True, **bopts)
+        step_with(opts, k)
 
     for k in range(args.warmup):
         step(k)
@@ -408,7 +424,7 @@ def measure_adjoint(args, with_cpu):
     r.close()
     return {"value": rate, "unit": "gradient evaluations/s", "ms_per_step": el * 1e3 / args.steps,
             "config": {"workload": "gem_adjoint_512x512x64_20_params", "width": W, "height": H, "spp": spp,
-                       "max_depth": depth, "params": len(P), "forward": "wavefront",
+                       "max_depth": depth, "params": len(P), "forward": "wavefront", "wavefront_opts": opts,
                        "backward": "path replay, wavefront", "backward_ms": bms, "backward_megakernel_ms": mms,
                        "wavefront_vs_megakernel_grad_max_rel": float(np.abs(g7 - g7m).max() / max(np.abs(g7m).max(), 1e-30)),
                        "fd_h_rel": 1e-3,

@@ -2439,46 +2439,65 @@ extern "C" int rt_backward_wavefront(rt_ctx* ctx, rt_scene* s, const rt_camera* 
     Staged din, gout;
     if ((rc = stage_in(ctx, dimg, (size_t)npix * 12, din))) return rc;
     if ((rc = stage_out(ctx, grad, (size_t)n * 4, gout, false))) return rc;
-    // side buffers: per-slot adjoint state, per-sample primal radiance, field map
+    WfQueues Qs[RT_WF_MAX_STREAMS];
+    Qs[0] = Q;
+    if ((rc = wf_lanes(ctx, o, Qs))) return rc;
+    const uint64_t nchunks = (total + o.cap - 1) / o.cap;
+    const uint32_t L = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(o.streams, nchunks));
+    // side buffers: per lane the per-slot adjoint state and per-sample primal
+    // radiance; shared: the field map and the medium parameters
     size_t b_adj = (size_t)o.cap * sizeof(WfAdj), b_l = ((size_t)o.cap * 12 + 255) & ~(size_t)255;
+    size_t b_lane = b_adj + b_l;
     size_t b_map = (fmap.size() + 255) & ~(size_t)255;
     char* side = nullptr;
-    HIPCHECK(ctx, hipMalloc(&side, b_adj + b_l + b_map + sizeof(MedPar) * (mpar.size() + 1)));
+    HIPCHECK(ctx, hipMalloc(&side, L * b_lane + b_map + sizeof(MedPar) * (mpar.size() + 1)));
     std::unique_ptr<void, void (*)(void*)> guard(side, [](void* q) { (void)hipFree(q); });
-    HIPCHECK(ctx, hipMemcpy(side + b_adj + b_l, fmap.data(), fmap.size(), hipMemcpyHostToDevice));
-    if (!mpar.empty()) HIPCHECK(ctx, hipMemcpy(side + b_adj + b_l + b_map, mpar.data(), sizeof(MedPar) * mpar.size(), hipMemcpyHostToDevice));
-    WfAdjQ A;
-    A.adj = (WfAdj*)side;
-    A.lsamp = (const float*)(side + b_adj);
-    A.fmap = (const int8_t*)(side + b_adj + b_l);
-    A.mp = (const MedPar*)(side + b_adj + b_l + b_map);
-    A.nmp = (int)mpar.size();
-    A.dimg = (const float*)din.dev;
-    A.inv_spp = 1.0f / (float)spp;
-    A.nparams = n;
-    A.grad = (float*)gout.dev;
+    char* shared = side + L * b_lane;
+    HIPCHECK(ctx, hipMemcpy(shared, fmap.data(), fmap.size(), hipMemcpyHostToDevice));
+    if (!mpar.empty()) HIPCHECK(ctx, hipMemcpy(shared + b_map, mpar.data(), sizeof(MedPar) * mpar.size(), hipMemcpyHostToDevice));
+    WfAdjQ As[RT_WF_MAX_STREAMS];
+    for (uint32_t k = 0; k < L; k++) {
+        WfAdjQ& A = As[k];
+        A.adj = (WfAdj*)(side + k * b_lane);
+        A.lsamp = (const float*)(side + k * b_lane + b_adj);
+        A.fmap = (const int8_t*)shared;
+        A.mp = (const MedPar*)(shared + b_map);
+        A.nmp = (int)mpar.size();
+        A.dimg = (const float*)din.dev;
+        A.inv_spp = 1.0f / (float)spp;
+        A.nparams = n;
+        A.grad = (float*)gout.dev;
+    }
     HIPCHECK(ctx, hipMemsetAsync(gout.dev, 0, (size_t)n * 4, ctx->stream));
     DCamera C; DParams P;
     to_dev(cam, p, C, P);
     P.counters = nullptr;
-    hipStream_t st = ctx->stream;
+    auto stream_of = [&](uint32_t k) { return k ? ctx->xstream[k - 1] : ctx->stream; };
     WfBatch none = {nullptr, nullptr, 0};
-    if (ctx->timing) HIPCHECK(ctx, hipEventRecord(ctx->ev0, st));
+    if (ctx->timing) HIPCHECK(ctx, hipEventRecord(ctx->ev0, ctx->stream));
+    if (L > 1) {
+        HIPCHECK(ctx, hipEventRecord(ctx->ev_fork, ctx->stream));
+        for (uint32_t k = 1; k < L; k++) HIPCHECK(ctx, hipStreamWaitEvent(stream_of(k), ctx->ev_fork, 0));
+    }
     uint32_t depth = P.max_depth;
     uint64_t chunks = 0;
     for (uint64_t first = 0; first < total; first += o.cap, chunks++) {
+        const uint32_t k = (uint32_t)(chunks % L);
+        hipStream_t st = stream_of(k);
+        const WfQueues& Qk = Qs[k];
+        const WfAdjQ& A = As[k];
         uint32_t m = (uint32_t)std::min<uint64_t>(o.cap, total - first);
         // primal: radiance of every sample of the chunk
-        HIPCHECK(ctx, hipMemsetAsync(side + b_adj, 0, (size_t)m * 12, st));
-        WfQueues Qp = Q;
-        Qp.lsamp = (float*)(side + b_adj);
+        HIPCHECK(ctx, hipMemsetAsync((void*)A.lsamp, 0, (size_t)m * 12, st));
+        WfQueues Qp = Qk;
+        Qp.lsamp = (float*)A.lsamp;
         wf_chunk<false>(ctx, st, s->S, C, P, Qp, first, m, o.grid, o.occ, depth, none, false);   // no k-NN maps here (render_args)
         // replay from the same streams
-        hipLaunchKernelGGL((k_wf_generate<false>), dim3((m + BLOCK - 1) / BLOCK), dim3(BLOCK), 0, st, C, P, Q, first, m, none);
-        hipLaunchKernelGGL(k_wfb_init, dim3((m + BLOCK - 1) / BLOCK), dim3(BLOCK), 0, st, Q, A, m);
-        hipLaunchKernelGGL(k_wf_reset, dim3(1), dim3(32), 0, st, Q.counters, m);
-        WfQueues QA = Q, QB = Q;
-        QB.active = Q.next; QB.next = Q.active;
+        hipLaunchKernelGGL((k_wf_generate<false>), dim3((m + BLOCK - 1) / BLOCK), dim3(BLOCK), 0, st, C, P, Qk, first, m, none);
+        hipLaunchKernelGGL(k_wfb_init, dim3((m + BLOCK - 1) / BLOCK), dim3(BLOCK), 0, st, Qk, A, m);
+        hipLaunchKernelGGL(k_wf_reset, dim3(1), dim3(32), 0, st, Qk.counters, m);
+        WfQueues QA = Qk, QB = Qk;
+        QB.active = Qk.next; QB.next = Qk.active;
         for (uint32_t d = 0; d < depth; d++) {
             const WfQueues& Qd = (d & 1) ? QB : QA;
             launch_extend<false>(o.occ, o.grid, st, s->S, Qd, none);
@@ -2488,9 +2507,13 @@ extern "C" int rt_backward_wavefront(rt_ctx* ctx, rt_scene* s, const rt_camera* 
         }
         HIPCHECK(ctx, hipGetLastError());
     }
+    for (uint32_t k = 1; k < L; k++) {
+        HIPCHECK(ctx, hipEventRecord(ctx->ev_join[k - 1], stream_of(k)));
+        HIPCHECK(ctx, hipStreamWaitEvent(ctx->stream, ctx->ev_join[k - 1], 0));
+    }
     ctx->ext_used = 0;
     if (ctx->timing) {
-        HIPCHECK(ctx, hipEventRecord(ctx->ev1, st));
+        HIPCHECK(ctx, hipEventRecord(ctx->ev1, ctx->stream));
         HIPCHECK(ctx, hipEventSynchronize(ctx->ev1));
         float ms = 0.0f;
         HIPCHECK(ctx, hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));

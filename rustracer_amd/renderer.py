@@ -279,7 +279,7 @@ class Renderer:
 
     def backward(self, dloss_dimage, camera, width, height, spp, params, sample_idx=0, strata=0, bg_scale=1.0,
                  photon_map=False, max_depth=50, seed=0, scene=None, grad=None, wavefront=False, capacity=0, grid=0,
-                 sort=True, occupancy=0):
+                 sort=True, occupancy=0, streams=1):
         """Path-replay adjoint: dLoss/dtheta for dloss_dimage = dLoss/d(mean
         image) (H x W x 3, host array or device tensor); returns grad (float32
         numpy array, or fills the given device tensor).  wavefront=True runs
@@ -293,7 +293,8 @@ class Renderer:
         g = np.zeros(len(params), np.float32) if grad is None else grad
         sc = self.scene if scene is None else scene
         if wavefront:
-            o = F.rt_wavefront_opts(capacity=capacity, grid=grid, sort=int(sort), use_graph=0, occupancy=occupancy)
+            o = F.rt_wavefront_opts(capacity=capacity, grid=grid, sort=int(sort), use_graph=0, occupancy=occupancy,
+                                    streams=streams)
             self._check(self.L.rt_backward_wavefront(self.ctx, sc, C.byref(cam), C.byref(p), spp, self._params(params),
                                                      len(params), _ptr(d), _ptr(g), C.byref(o)), "rt_backward_wavefront")
         else:

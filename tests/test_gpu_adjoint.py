@@ -107,18 +107,19 @@ def test_adjoint_matches_finite_differences_with_common_random_numbers(gpu_rende
     assert np.count_nonzero(np.abs(fd) > 1e-2 * scale) >= 10
 
 
-@pytest.mark.parametrize("cap,sort", [(4096, 1), (1 << 20, 0), (8192, 2)])
-def test_wavefront_adjoint_equals_megakernel_adjoint(gpu_renderer, cap, sort):
+@pytest.mark.parametrize("cap,sort,streams", [(4096, 1, 1), (1 << 20, 0, 1), (8192, 2, 1), (1024, 1, 3), (2048, 0, 4)])
+def test_wavefront_adjoint_equals_megakernel_adjoint(gpu_renderer, cap, sort, streams):
     """The replay as wavefronts (primal radiance per sample, then the replay
     stages) gives the megakernel replay's gradient; capacity 4096 splits the
-    frame into two chunks."""
+    frame into two chunks, capacity 1024 into nine over three streams."""
     W, H, spp = 24, 24, 8
     sc, cam, cfg = setup(gpu_renderer, W, H)
     P = cfg["params"]
     gpu_renderer.set_params(P, cfg["theta_start"])
     w = np.random.default_rng(6).standard_normal((H, W, 3)).astype(np.float32)
     a = gpu_renderer.backward(w, cam, W, H, spp, P, max_depth=24, seed=8)
-    b = gpu_renderer.backward(w, cam, W, H, spp, P, max_depth=24, seed=8, wavefront=True, capacity=cap, sort=sort)
+    b = gpu_renderer.backward(w, cam, W, H, spp, P, max_depth=24, seed=8, wavefront=True, capacity=cap, sort=sort,
+                              streams=streams)
     scale = np.abs(a).max()
     assert np.allclose(a, b, rtol=2e-3, atol=2e-4 * scale), np.c_[a, b]
 
