@@ -550,6 +550,14 @@ int  rt_render_progressive(rt_ctx *ctx, rt_scene *scene, const rt_camera *cam,
                            const rt_accum_opts *opts, const rt_pixel_state *state,
                            uint32_t *samples_done, uint32_t *n_converged);
 
+/* render_aux_pass (src/renderer.rs:597-645): the denoiser's auxiliary
+ * buffers, width*height*3 floats each, row 0 = image top.  One ray per pixel
+ * through the pixel centre; a hit stores the material's albedo hint clamped
+ * to [0, 1] and the hit's normal, a miss the background colour and a zero
+ * normal.                                                                    */
+int  rt_render_aux(rt_ctx *ctx, rt_scene *scene, const rt_camera *cam, uint32_t width, uint32_t height,
+                   float *albedo, float *normal);
+
 /* The saved image (src/main.rs:571-582, src/output.rs:103-190): per pixel the
  * mean radiance accum / max(n, 1) (n = samples[i], or spp when samples is
  * NULL), then apply_optics (vignette 1 - vignette * r^2, barrel distortion

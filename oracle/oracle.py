@@ -61,6 +61,7 @@ def lib():
         L.rtc_perlin.argtypes = [vp, vp, C.c_uint32, C.c_int, vp, vp]
         L.rtc_accumulate_sample.argtypes = [vp, C.c_uint32, C.c_float, C.c_uint32, C.c_float, vp, vp, vp, vp]
         L.rtc_mark_converged.argtypes = [C.c_uint32, C.c_float, vp, vp, vp, vp]
+        L.rtc_render_aux.argtypes = [vp, vp, C.c_uint32, C.c_uint32, vp, vp]
         L.rtc_apply_optics.argtypes = [vp, C.c_uint32, C.c_uint32, C.c_float, C.c_float, vp]
         L.rtc_tone_eval.argtypes = [C.c_int, vp, C.c_uint32, vp]
         L.rtc_resolve_rgb8.argtypes = [vp, vp, C.c_uint32, C.c_uint32, C.c_uint32, C.c_float, C.c_int, C.c_float,
@@ -189,6 +190,14 @@ class OracleScene:
         out = np.zeros((len(pos_nrm), 3), dtype=np.float32)
         self.L.rtc_photon_irradiance(self.h, _p(pos_nrm), len(pos_nrm), _p(out), nthreads or threads())
         return out
+
+    def render_aux(self, camera, width, height):
+        """render_aux_pass (src/renderer.rs:597-645): (albedo, normal)."""
+        a = np.zeros((height, width, 3), np.float32)
+        n = np.zeros((height, width, 3), np.float32)
+        cam = camera.to_c()
+        self.L.rtc_render_aux(self.h, C.byref(cam), width, height, _p(a), _p(n))
+        return a, n
 
     def world_nodes(self):
         n = self.L.rtc_world_nodes(self.h, None, None, None, 0)

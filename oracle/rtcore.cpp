@@ -927,4 +927,32 @@ int rtc_resolve_rgb8(const float* acc, const uint32_t* ns, uint32_t spp, uint32_
     return 0;
 }
 
+// render_aux_pass (src/renderer.rs:597-645).
+int rtc_render_aux(void* s, const rt_camera* cam, uint32_t w, uint32_t h, float* alb, float* nrm) {
+    OScene& S = *(OScene*)s;
+    Camera c = to_cam(*cam);
+    float wd = (float)std::max<uint32_t>(w - 1, 1), hd = (float)std::max<uint32_t>(h - 1, 1);
+    for (uint32_t i = 0; i < w * h; i++) {
+        uint32_t row = i / w, col = i % w;
+        SmallRng rng = SmallRng::seed_from_u64((uint64_t)i ^ 0x9E3779B97F4A7C15ull);
+        float u = ((float)col + 0.5f) / wd;
+        float v = ((float)(h - 1 - row) + 0.5f) / hd;
+        Ray ray = c.get_ray(u, v, rng);
+        HitRecord rec;
+        float a[3], n[3];
+        if (!S.world.hit(ray, 0.001f, INF, rec)) {
+            Color bg = S.bg.eval(ray.direction);
+            a[0] = bg.x; a[1] = bg.y; a[2] = bg.z;
+            n[0] = n[1] = n[2] = 0.0f;
+        } else {
+            Color al = rec.mat->albedo_hint(rec.u, rec.v, rec.p);
+            float cl[3] = {al.x, al.y, al.z};
+            for (int k = 0; k < 3; k++) { float x = cl[k]; if (x < 0.0f) x = 0.0f; if (x > 1.0f) x = 1.0f; a[k] = x; }
+            n[0] = rec.normal.x; n[1] = rec.normal.y; n[2] = rec.normal.z;
+        }
+        for (int k = 0; k < 3; k++) { alb[3 * (size_t)i + k] = a[k]; nrm[3 * (size_t)i + k] = n[k]; }
+    }
+    return 0;
+}
+
 }  // extern "C"

@@ -732,6 +732,18 @@ RT_D V3 mat_albedo_hint(const DScene& S, const Rec& h) {
     if (m.kind == M_DIFFUSE_LIGHT) return tex_value(S, m, h.u, h.v, h.p);
     if (m.kind == M_BLACKBODY) return mk(m.p[3], m.p[4], m.p[5]);
     if (m.kind == M_SSS || m.kind == M_PEARL) return mk(m.p[0], m.p[1], m.p[2]);
+    if (m.kind == M_SPECTRAL_METAL) {
+        // SpectralMetal::new's avg_color (src/material.rs:435-443): the mean of
+        // spectral_to_rgb(lambda) * F0(lambda) over the 13 table wavelengths
+        V3 acc = mk(0, 0, 0);
+        for (int i = 0; i < 13; i++) {
+            float lam = 380.0f + (float)i * 25.0f;
+            float n, k;
+            metal_nk((int)m.p[0], lam, n, k);
+            acc += spectral_to_rgb(S, lam) * fresnel_conductor(1.0f, n, k);
+        }
+        return acc / 13.0f;
+    }
     return mk(1, 1, 1);
 }
 RT_D bool mat_receives_caustics(const DScene& S, const Rec& h) {

@@ -207,6 +207,35 @@ __global__ void k_progressive_step(uint32_t* ctl, uint32_t pass, uint32_t npix, 
     if (o.adaptive && (float)ctl[1] >= (float)npix * o.convergence_pct / 100.0f) ctl[0] = 1;
 }
 
+// render_aux_pass (src/renderer.rs:597-645): per pixel one ray through the
+// pixel centre (its own stream, seeded i ^ 0x9E3779B97F4A7C15, for the lens
+// and time draws); a hit gives the material's albedo hint clamped to [0, 1]
+// and the hit normal, a miss the background colour and a zero normal.
+__global__ __launch_bounds__(BLOCK) void k_aux(DScene S, DCamera cam, uint32_t w, uint32_t h, float* __restrict__ alb,
+                                               float* __restrict__ nrm) {
+    Stack st; init_stack(st);
+    uint32_t i = blockIdx.x * BLOCK + threadIdx.x;
+    if (i >= w * h) return;
+    uint32_t row = i / w, col = i - row * w;
+    Rng rng = rng_seed((uint64_t)i ^ 0x9E3779B97F4A7C15ull);
+    float u = ((float)col + 0.5f) / (float)max(w - 1, 1u);
+    float v = ((float)(h - 1 - row) + 0.5f) / (float)max(h - 1, 1u);
+    Ray r = camera_ray(cam, u, v, rng);
+    HitId id = trace_closest(S, r, 0.001f, FINF, st);
+    V3 a, n;
+    if (id.slot < 0) {
+        a = bg_eval(S, r.d);
+        n = mk(0, 0, 0);
+    } else {
+        Rec hr = finalize(S, r, id, 0.001f);
+        V3 c = mat_albedo_hint(S, hr);
+        a = mk(fclamp(c.x, 0.0f, 1.0f), fclamp(c.y, 0.0f, 1.0f), fclamp(c.z, 0.0f, 1.0f));
+        n = hr.n;
+    }
+    alb[3 * (size_t)i] = a.x; alb[3 * (size_t)i + 1] = a.y; alb[3 * (size_t)i + 2] = a.z;
+    nrm[3 * (size_t)i] = n.x; nrm[3 * (size_t)i + 1] = n.y; nrm[3 * (size_t)i + 2] = n.z;
+}
+
 // Detached-sampling forward render: the k_render frame with BSDF values
 // evaluated on the materials `me` while every decision follows the scene's.
 __global__ __launch_bounds__(BLOCK) void k_detached(DScene S, DCamera cam, DParams P, const DMaterial* __restrict__ me,
@@ -1871,6 +1900,27 @@ int rt_mark_converged(rt_ctx* ctx, uint32_t npix, const rt_accum_opts* o, const 
     if ((rc = state_finish(ctx, st, d))) return rc;
     if (n_converged) *n_converged = cnt;
     return RT_OK;
+}
+
+int rt_render_aux(rt_ctx* ctx, rt_scene* s, const rt_camera* cam, uint32_t w, uint32_t h, float* albedo, float* normal) {
+    if (!ctx || !s || !cam || !albedo || !normal) return RT_ERR_INVALID;
+    if (w == 0 || h == 0) return fail(ctx, RT_ERR_INVALID, "empty image");
+    if ((uint64_t)w * h > 0xffffffffull / 12) return fail(ctx, RT_ERR_INVALID, "image too large");
+    HIPCHECK(ctx, hipSetDevice(ctx->device));
+    uint32_t npix = w * h;
+    Staged a, n;
+    int rc;
+    if ((rc = stage_out(ctx, albedo, (size_t)npix * 12, a, false))) return rc;
+    if ((rc = stage_out(ctx, normal, (size_t)npix * 12, n, false))) return rc;
+    rt_sample_params p;
+    memset(&p, 0, sizeof p);
+    DCamera C; DParams P;
+    to_dev(cam, &p, C, P);
+    hipLaunchKernelGGL(k_aux, dim3((npix + BLOCK - 1) / BLOCK), dim3(BLOCK), 0, ctx->stream, s->S, C, w, h,
+                       (float*)a.dev, (float*)n.dev);
+    HIPCHECK(ctx, hipGetLastError());
+    if ((rc = finish_out(ctx, albedo, (size_t)npix * 12, a))) return rc;
+    return finish_out(ctx, normal, (size_t)npix * 12, n);
 }
 
 int rt_resolve_rgb8(rt_ctx* ctx, const float* acc, const uint32_t* ns, uint32_t spp, uint32_t w, uint32_t h,

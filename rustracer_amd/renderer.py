@@ -202,6 +202,19 @@ class Renderer:
                                                  C.byref(st), C.byref(done), C.byref(nconv)), "rt_render_progressive")
         return done.value, nconv.value
 
+    def render_aux(self, camera, width, height, albedo=None, normal=None):
+        """render_aux_pass (src/renderer.rs:597-645): first-hit albedo and
+        normal buffers for a denoiser; returns (albedo, normal), each
+        (height, width, 3) float32 (host arrays unless given)."""
+        if albedo is None:
+            albedo = np.zeros((height, width, 3), np.float32)
+        if normal is None:
+            normal = np.zeros((height, width, 3), np.float32)
+        cam = camera.to_c()
+        self._check(self.L.rt_render_aux(self.ctx, self.scene, C.byref(cam), width, height, _ptr(albedo), _ptr(normal)),
+                    "rt_render_aux")
+        return albedo, normal
+
     def resolve_rgb8(self, accum, width, height, samples=None, spp=1, exposure=1.0, tonemapper="aces",
                      vignette=0.0, distortion=0.0, out=None):
         """The saved 8-bit image (src/main.rs:571-582, src/output.rs): mean

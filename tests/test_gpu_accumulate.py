@@ -152,3 +152,34 @@ def test_progressive_then_resolve(gpu_renderer):
     c = O.resolve_rgb8(st["accum"], st["samples"], 0, 48, 40, agx=True)
     diff = np.abs(d.astype(np.int32) - c.astype(np.int32))
     assert diff.max() <= 1 and (diff > 0).mean() <= 1e-3
+
+
+def test_aux_pass_matches_checker(gpu_renderer):
+    """render_aux_pass (src/renderer.rs:597-645) over every albedo hint: a
+    checker floor, Lambertian, spectral metal (mean spectral F0), PBR,
+    dielectric (white), SSS, Pearl, an area light and a blackbody emitter, a
+    fog ball (medium events: white), under a gradient sky."""
+    from rustracer_amd import scene as S
+    objs = [
+        S.Quad((-4, 0, -4), (8, 0, 0), (0, 0, 8), S.Checker((0.1, 0.1, 0.1), (0.9, 0.8, 0.7), 0.5)),
+        S.Sphere((-1.6, 0.4, 0.0), 0.4, S.Lambertian((0.8, 0.3, 0.1))),
+        S.Sphere((-0.8, 0.4, 0.6), 0.4, S.SpectralMetal("copper", 0.1)),
+        S.Sphere((0.0, 0.4, 0.0), 0.4, S.PbrMaterial((0.2, 0.6, 0.9), 0.4, 0.3)),
+        S.Sphere((0.8, 0.4, 0.6), 0.4, S.Dielectric(1.5)),
+        S.Sphere((1.6, 0.4, 0.0), 0.4, S.SSSMaterial((0.9, 0.7, 0.6), (0.5, 1.0, 2.0))),
+        S.Sphere((0.4, 0.3, -0.8), 0.3, S.PearlMaterial()),
+        S.Sphere((-0.4, 1.3, 0.4), 0.25, S.BlackbodyLight(4500.0, 3.0)),
+        S.Quad((-0.5, 2.2, -0.5), (1, 0, 0), (0, 0, 1), S.DiffuseLight((4.0, 4.0, 4.0))),
+    ]
+    fog = S.Sphere((1.2, 1.2, -0.6), 0.35, None)
+    S.ConstantMedium(fog, 3.0, (0.9, 0.9, 0.9))
+    objs.append(fog)
+    sc = S.Scene(objs, [objs[-2]], S.Background.gradient((0.4, 0.5, 0.9), (1.0, 1.0, 1.0)))
+    cam = S.Camera((0.0, 1.6, -4.0), (0.0, 0.5, 0.0), (0, 1, 0), 45.0, 1.5)
+    gpu_renderer.upload(sc)
+    W, H = 96, 64
+    ga, gn = gpu_renderer.render_aux(cam, W, H)
+    ca, cn = O.OracleScene(sc).render_aux(cam, W, H)
+    assert np.allclose(ga, ca, rtol=1e-5, atol=1e-6), np.abs(ga - ca).max()
+    assert np.allclose(gn, cn, rtol=1e-5, atol=1e-6), np.abs(gn - cn).max()
+    assert len(np.unique(ga.reshape(-1, 3).round(3), axis=0)) > 8
