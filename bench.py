@@ -559,6 +559,92 @@ def measured_traffic(workload, opts):
     return rec["traffic_bytes_per_dispatch"], rec["source"]
 
 
+def measure_output(args, with_cpu=True):
+    """The render loop's per-pixel work and the saved image on the device, at
+    1280x720 with every buffer in HBM: accumulate_sample (12 B sample + 29 B
+    state read, 28 B written per pixel), mark_converged (29 B read + 1 B
+    written), the 8-bit resolve with optics (4 x 12 B + 4 x 4 B read through
+    the bilinear taps, 3 B written), the AOV pass (one closest-hit ray per
+    pixel on the 1.31M-triangle mesh) and the render loop on the mesh with the
+    firefly clamp on: fused per pixel (rt_render_progressive, 16 passes) and
+    through the wavefront (rt_render_progressive_wavefront, 64 passes).  Kernel time from HIP events on
+    the library stream; beside it the checker's accumulate_sample on the host."""
+    import numpy as np
+    import torch
+    from rustracer_amd import Renderer, scenes
+    W, H = 1280, 720
+    n = W * H
+    r = Renderer(0)
+    g = torch.Generator(device="cuda").manual_seed(1)
+    sc_buf = torch.rand((n, 3), generator=g, device="cuda") * 2.0
+    st = dict(accum=torch.zeros((n, 3), device="cuda"), samples=torch.zeros(n, dtype=torch.int32, device="cuda"),
+              var_m2=torch.zeros((n, 3), device="cuda"), converged=torch.zeros(n, dtype=torch.uint8, device="cuda"))
+    o = Renderer.accum_opts(adaptive=True, threshold=0.05)
+    rgb8 = torch.zeros((H, W, 3), dtype=torch.uint8, device="cuda")
+
+    def timed(fn, reps):
+        """Kernel ms per call: the library's HIP events on its own stream."""
+        fn()
+        torch.cuda.synchronize()
+        r.set_timing(True)
+        for _ in range(reps):
+            fn()
+        ms, _ = r.timing()
+        r.set_timing(False)
+        return ms / reps
+
+    t_acc = timed(lambda: r.accumulate_sample(sc_buf, st, o), 20)
+    t_mark = timed(lambda: r.mark_converged(st, o), 20)
+    t_res = timed(lambda: r.resolve_rgb8(st["accum"], W, H, samples=st["samples"], tonemapper="agx", vignette=0.2,
+                                         distortion=0.05, out=rgb8), 20)
+    scene, cam, cfg = scenes.mesh_envmap(W, H, level=8)
+    r.upload(scene)
+    alb = torch.zeros((H, W, 3), device="cuda")
+    nrm = torch.zeros((H, W, 3), device="cuda")
+    t_aux = timed(lambda: r.render_aux(cam, W, H, alb, nrm), 5)
+    st2 = dict(accum=torch.zeros((n, 3), device="cuda"), samples=torch.zeros(n, dtype=torch.int32, device="cuda"),
+               var_m2=torch.zeros((n, 3), device="cuda"), converged=torch.zeros(n, dtype=torch.uint8, device="cuda"))
+    passes = 16
+    t_prog = timed(lambda: r.render_progressive(st2, cam, W, H, passes, Renderer.accum_opts(), max_depth=cfg["max_depth"]), 1)
+    wopts = dict(capacity=1 << 23, grid=8192, sort=1, occupancy=4, streams=3)
+    wpasses = 64
+    t_progw = timed(lambda: r.render_progressive(st2, cam, W, H, wpasses, Renderer.accum_opts(), max_depth=cfg["max_depth"],
+                                                 wavefront=True, **wopts), 1)
+    gbs = lambda b, ms: b / (ms * 1e-3) / 1e9
+    b_acc, b_mark, b_res = n * (12 + 29 + 28), n * (29 + 1), n * (4 * 12 + 4 * 4 + 3)
+    log(f"[output] accumulate {t_acc:.3f} ms ({gbs(b_acc, t_acc):.0f} GB/s), mark {t_mark:.3f} ms "
+        f"({gbs(b_mark, t_mark):.0f} GB/s), resolve {t_res:.3f} ms ({gbs(b_res, t_res):.0f} GB/s), aux {t_aux:.2f} ms, "
+        f"progressive {n * passes / (t_prog * 1e-3) / 1e6:.1f} M samples/s (megakernel), "
+        f"{n * wpasses / (t_progw * 1e-3) / 1e6:.1f} M samples/s (wavefront)")
+    cpu = None
+    if with_cpu:
+        from oracle import oracle as O
+        hs = {k: (v.cpu().numpy().view(np.uint32) if k == "samples" else v.cpu().numpy()) for k, v in st.items()}
+        x = sc_buf.cpu().numpy()
+        reps, t0 = 0, time.perf_counter()
+        while time.perf_counter() - t0 < 3.0:
+            O.accumulate_sample(x, hs)
+            reps += 1
+        el = time.perf_counter() - t0
+        cpu = {"value": n * reps / el, "unit": "pixels/s", "cores": 1, "kind": "port",
+               "sample": f"{reps} accumulate_sample passes over 1280x720 on one thread, {el:.1f}s"}
+    r.close()
+    return {"value": n / (t_acc * 1e-3), "unit": "pixels/s (accumulate_sample)", "ms_per_step": t_acc,
+            "config": {"workload": "render_loop_output_1280x720", "width": W, "height": H,
+                       "accumulate_ms": t_acc, "mark_converged_ms": t_mark, "resolve_rgb8_ms": t_res,
+                       "aux_pass_ms": t_aux, "aux_rays_per_s": n / (t_aux * 1e-3),
+                       "progressive_samples_per_s": n * passes / (t_prog * 1e-3), "progressive_passes": passes,
+                       "progressive_wavefront_samples_per_s": n * wpasses / (t_progw * 1e-3),
+                       "progressive_wavefront_passes": wpasses, "progressive_wavefront_opts": wopts,
+                       "bytes_per_pixel": {"accumulate": 69, "mark_converged": 30, "resolve_rgb8": 67}},
+            "roofline": {"bound": "hbm", "achieved": gbs(b_acc, t_acc), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                         "frac": gbs(b_acc, t_acc) / HBM_PEAK_GBS, "kernel": "k_accumulate",
+                         "avg_launch_ms": t_acc, "bytes_per_launch": b_acc,
+                         "others": {"k_mark_converged": {"achieved": gbs(b_mark, t_mark)},
+                                    "k_resolve": {"achieved": gbs(b_res, t_res)}}},
+            "cpu_baseline": cpu}
+
+
 def cpu_baseline(scene, cam, cfg, seconds, photons=False):
     """The oracle's render_tiles with the reference's row streams, all host
     cores, passes of the same frame until `seconds` of wall time are used
@@ -596,10 +682,10 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--config", default="mesh", choices=sorted(workloads()) + ["batch", "adjoint", "rmse"])
-    ap.add_argument("--also", default="gem,batch,adjoint,rmse",
+    ap.add_argument("--config", default="mesh", choices=sorted(workloads()) + ["batch", "adjoint", "rmse", "output"])
+    ap.add_argument("--also", default="gem,batch,adjoint,rmse,output",
                     help="comma-separated extra workloads ('' for none): " + ", ".join(sorted(workloads())) +
-                         ", batch, adjoint, rmse")
+                         ", batch, adjoint, rmse, output")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-tune", action="store_true")
@@ -627,6 +713,8 @@ def main():
         main_res = measure_adjoint(args, not args.no_cpu)
     elif args.config == "rmse":
         main_res = measure_rmse(args)
+    elif args.config == "output":
+        main_res = measure_output(args, not args.no_cpu)
     else:
         main_res = measure(args.config, args, with_cpu=not args.no_cpu)
     also = {}
@@ -637,6 +725,8 @@ def main():
             m = measure_adjoint(args, not args.no_cpu)
         elif name == "rmse":
             m = measure_rmse(args)
+        elif name == "output":
+            m = measure_output(args, not args.no_cpu)
         else:
             m = measure(name, args, with_cpu=not args.no_cpu)
         also[m["config"]["workload"]] = {"value": m["value"], "unit": m.get("unit", "samples/s"), "config": m["config"],

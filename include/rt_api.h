@@ -550,6 +550,19 @@ int  rt_render_progressive(rt_ctx *ctx, rt_scene *scene, const rt_camera *cam,
                            const rt_accum_opts *opts, const rt_pixel_state *state,
                            uint32_t *samples_done, uint32_t *n_converged);
 
+/* rt_render_progressive's loop with the passes rendered by the wavefront
+ * renderer (opts as rt_render_wavefront's; NULL: its defaults): chunks of
+ * whole passes (capacity / pixels passes each, one when adaptive) render
+ * their per-sample radiance, then fold into the state pass by pass in order.
+ * Same per-pixel streams as the megakernel loop; each sample's radiance sums
+ * its contributions in the wavefront's stage order, so states agree to float
+ * rounding.  No counterpart in the reference.                               */
+int  rt_render_progressive_wavefront(rt_ctx *ctx, rt_scene *scene, const rt_camera *cam,
+                                     const rt_sample_params *params, uint32_t samples_max,
+                                     const rt_accum_opts *opts, const rt_pixel_state *state,
+                                     const rt_wavefront_opts *wf_opts, uint32_t *samples_done,
+                                     uint32_t *n_converged);
+
 /* render_aux_pass (src/renderer.rs:597-645): the denoiser's auxiliary
  * buffers, width*height*3 floats each, row 0 = image top.  One ray per pixel
  * through the pixel centre; a hit stores the material's albedo hint clamped
@@ -578,8 +591,9 @@ int  rt_resolve_rgb8(rt_ctx *ctx, const float *accum, const uint32_t *samples, u
 
 /* ---- measurement (no counterpart in the reference: its --bench mode times
  * render_tiles with Instant, src/main.rs:68-117) ------------------------------ */
-/* Bracket every render call's kernels with HIP events on the context's stream;
- * enabling resets the totals.                                                 */
+/* Bracket every render call's kernels (and those of rt_accumulate_sample,
+ * rt_mark_converged, rt_resolve_rgb8 and rt_render_aux) with HIP events on
+ * the context's stream; enabling resets the totals.                           */
 int  rt_ctx_set_timing(rt_ctx *ctx, int enable);
 /* Summed kernel time (ms) and kernel launch count since timing was enabled. */
 int  rt_ctx_get_timing(rt_ctx *ctx, double *kernel_ms, uint64_t *launches);

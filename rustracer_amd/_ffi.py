@@ -188,6 +188,10 @@ SIGNATURES = {
                                         C.POINTER(C.c_uint32)]),
     "rt_resolve_rgb8": (C.c_int, [VP, VP, VP, C.c_uint32, C.c_uint32, C.c_uint32, C.POINTER(rt_resolve_opts), VP]),
     "rt_render_aux": (C.c_int, [VP, VP, C.POINTER(rt_camera), C.c_uint32, C.c_uint32, VP, VP]),
+    "rt_render_progressive_wavefront": (C.c_int, [VP, VP, C.POINTER(rt_camera), C.POINTER(rt_sample_params), C.c_uint32,
+                                                  C.POINTER(rt_accum_opts), C.POINTER(rt_pixel_state),
+                                                  C.POINTER(rt_wavefront_opts), C.POINTER(C.c_uint32),
+                                                  C.POINTER(C.c_uint32)]),
     "rt_ctx_set_timing": (C.c_int, [VP, C.c_int]),
     "rt_ctx_get_timing": (C.c_int, [VP, C.POINTER(C.c_double), C.POINTER(C.c_uint64)]),
     "rt_ctx_set_counting": (C.c_int, [VP, C.c_int]),

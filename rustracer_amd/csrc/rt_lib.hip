@@ -117,24 +117,31 @@ __global__ __launch_bounds__(BLOCK) void k_accumulate(const float* __restrict__ 
     accumulate_pixel(acc + 3 * (size_t)i, ns + i, m2 + 3 * (size_t)i, mk(sc[3 * (size_t)i], sc[3 * (size_t)i + 1], sc[3 * (size_t)i + 2]), o);
 }
 
+// Add a per-lane count into *count with one atomic per wave (every lane of
+// the wave calls it).
+__device__ __forceinline__ void wave_count(uint32_t local, uint32_t* count) {
+    for (int off = 32; off > 0; off >>= 1) local += __shfl_xor(local, off);
+    if (__lane_id() == 0 && local) atomicAdd(count, local);
+}
+#define RT_COUNT_GRID 1024u   // blocks of the grid-stride counting kernels: few atomics on one counter
+
 // mark_converged (src/main.rs:646-677): flag, then count every flagged pixel.
 __global__ __launch_bounds__(BLOCK) void k_mark_converged(uint32_t npix, float threshold, const float* acc, const uint32_t* ns,
                                                           const float* m2, uint8_t* conv, uint32_t* count) {
-    uint32_t i = blockIdx.x * BLOCK + threadIdx.x;
-    bool c = false;
-    if (i < npix) {
-        c = conv[i] != 0;
+    uint32_t local = 0;
+    for (uint32_t i = blockIdx.x * BLOCK + threadIdx.x; i < npix; i += gridDim.x * BLOCK) {
+        bool c = conv[i] != 0;
         if (!c && pixel_converged(acc + 3 * (size_t)i, ns[i], m2 + 3 * (size_t)i, threshold)) { conv[i] = 1; c = true; }
+        local += c ? 1u : 0u;
     }
-    unsigned long long b = __ballot(c);
-    if (__lane_id() == 0 && b) atomicAdd(count, (uint32_t)__popcll(b));
+    wave_count(local, count);
 }
 
 // Flagged pixels of a state handed to rt_render_progressive.
 __global__ __launch_bounds__(BLOCK) void k_count_flags(uint32_t npix, const uint8_t* conv, uint32_t* count) {
-    uint32_t i = blockIdx.x * BLOCK + threadIdx.x;
-    unsigned long long b = __ballot(i < npix && conv[i] != 0);
-    if (__lane_id() == 0 && b) atomicAdd(count, (uint32_t)__popcll(b));
+    uint32_t local = 0;
+    for (uint32_t i = blockIdx.x * BLOCK + threadIdx.x; i < npix; i += gridDim.x * BLOCK) local += conv[i] != 0 ? 1u : 0u;
+    wave_count(local, count);
 }
 
 // Mean radiance of pixel j: accum / max(n, 1) per component (src/main.rs:574-577).
@@ -186,18 +193,21 @@ __global__ __launch_bounds__(BLOCK) void k_progressive(DScene S, DCamera cam, DP
     if (KNN) st.heap = LdsHeap::column();
     uint32_t i = blockIdx.x * BLOCK + threadIdx.x;
     uint32_t npix = P.width * P.height;
-    if (i >= npix || conv[i]) return;
-    uint32_t row = i / P.width, col = i - row * P.width;
-    Rng rng = rng_seed(pixel_seed(i, P.sample_idx, P.seed));
-    Ray r = pixel_ray(cam, P, row, col, P.sample_idx, rng);
-    V3 c = ray_color(S, r, (int)P.max_depth, P.bg_scale, P.use_photons != 0, rng, st);
-    float* a = acc + 3 * (size_t)i;
-    float* v = m2 + 3 * (size_t)i;
-    accumulate_pixel(a, ns + i, v, c, o);
-    if (o.adaptive && P.sample_idx + 1 >= o.min_samples && pixel_converged(a, ns[i], v, o.threshold)) {
-        conv[i] = 1;
-        atomicAdd(ctl + 1, 1u);
+    bool newly = false;
+    if (i < npix && !conv[i]) {
+        uint32_t row = i / P.width, col = i - row * P.width;
+        Rng rng = rng_seed(pixel_seed(i, P.sample_idx, P.seed));
+        Ray r = pixel_ray(cam, P, row, col, P.sample_idx, rng);
+        V3 c = ray_color(S, r, (int)P.max_depth, P.bg_scale, P.use_photons != 0, rng, st);
+        float* a = acc + 3 * (size_t)i;
+        float* v = m2 + 3 * (size_t)i;
+        accumulate_pixel(a, ns + i, v, c, o);
+        if (o.adaptive && P.sample_idx + 1 >= o.min_samples && pixel_converged(a, ns[i], v, o.threshold)) {
+            conv[i] = 1;
+            newly = true;
+        }
     }
+    wave_count(newly ? 1u : 0u, ctl + 1);
 }
 // After pass `pass`: count it, and stop when adaptive and the converged count
 // reaches convergence_pct % of the pixels (the reference's target_reached).
@@ -435,6 +445,7 @@ struct WfQueues {
     float* lsamp;         // non-null: contributions go to the path's sample (3 floats per sid), not to fb
     uint32_t sort;        // 0: one bucket; 1: a bucket per material kind; 2: two shade classes
     uint32_t* dense;      // cap: paths whose k-NN query needs the heap search
+    const uint32_t* stop; // non-null and nonzero at chunk start: the chunk renders no path (a stopped render loop)
 };
 
 __device__ __forceinline__ uint32_t lane_id() { return __lane_id(); }
@@ -478,8 +489,9 @@ __device__ __forceinline__ void wf_store(WfPath& w, const PathState& ps, const R
 }
 
 // Zero the per-bounce counters (a kernel, not a memset, so graph replays are safe).
-__global__ void k_wf_reset(uint32_t* counters, uint32_t active) {
+__global__ void k_wf_reset(uint32_t* counters, uint32_t active, const uint32_t* stop) {
     uint32_t i = threadIdx.x;
+    if (stop && *stop) active = 0;
     if (i < 32) counters[i] = (i == WF_CNT_ACTIVE) ? active : 0u;
 }
 // After shade: survivors become the active set, everything else restarts at 0.
@@ -742,6 +754,33 @@ __global__ __launch_bounds__(BLOCK) void k_wf_knn_dense(DScene S0, WfQueues Q, W
         knn_query<BATCH>(S, w, pos, nrm);
         knn_store(w, photon_knn_irradiance(S, pos, nrm, heap, BLOCK));
     }
+}
+
+// The wavefront render loop's bookkeeping: K passes of per-sample radiance
+// (lsamp, sample id k * npix + j for path j of pass k, which covered pixel
+// tile_order_pixel(j)) into the pixel state, pass by pass per pixel as
+// accumulate_sample does, with mark_converged after each pass when adaptive
+// (K is then 1, so the stop test runs between passes).
+__global__ __launch_bounds__(BLOCK) void k_accum_passes(const float* __restrict__ lsamp, uint32_t K, uint32_t w, uint32_t h,
+                                                        uint32_t pass0, rt_accum_opts o, float* acc, uint32_t* ns, float* m2,
+                                                        uint8_t* conv, uint32_t* ctl) {
+    if (ctl[0]) return;
+    uint32_t npix = w * h;
+    uint32_t local = 0;
+    for (uint32_t j = blockIdx.x * BLOCK + threadIdx.x; j < npix; j += gridDim.x * BLOCK) {
+        uint32_t px = tile_order_pixel(j, w, h);
+        float* a = acc + 3 * (size_t)px;
+        float* v = m2 + 3 * (size_t)px;
+        for (uint32_t k = 0; k < K && !conv[px]; k++) {
+            const float* c = lsamp + 3 * ((size_t)k * npix + j);
+            accumulate_pixel(a, ns + px, v, mk(c[0], c[1], c[2]), o);
+            if (o.adaptive && pass0 + k + 1 >= o.min_samples && pixel_converged(a, ns[px], v, o.threshold)) {
+                conv[px] = 1;
+                local++;
+            }
+        }
+    }
+    wave_count(local, ctl + 1);
 }
 
 template <int OCC, bool BATCH>
@@ -1828,6 +1867,22 @@ int rt_render_accumulate(rt_ctx* ctx, rt_scene* s, const rt_camera* cam, const r
     return render_common(ctx, s, cam, p, nullptr, out, spp, true);
 }
 
+// Timing of one kernel launch on the context stream (rt_ctx_set_timing).
+static int time_begin(rt_ctx* ctx) {
+    if (ctx->timing) HIPCHECK(ctx, hipEventRecord(ctx->ev0, ctx->stream));
+    return RT_OK;
+}
+static int time_end(rt_ctx* ctx) {
+    if (!ctx->timing) return RT_OK;
+    HIPCHECK(ctx, hipEventRecord(ctx->ev1, ctx->stream));
+    HIPCHECK(ctx, hipEventSynchronize(ctx->ev1));
+    float ms = 0.0f;
+    HIPCHECK(ctx, hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+    ctx->kernel_ms += ms;
+    ctx->launches += 1;
+    return RT_OK;
+}
+
 // The four arrays of a pixel state on the device (staged when the caller
 // passed host memory, copied back by state_finish).
 struct StateDev {
@@ -1876,9 +1931,11 @@ int rt_accumulate_sample(rt_ctx* ctx, const float* sc, uint32_t npix, const rt_a
     StateDev d;
     if ((rc = stage_in(ctx, sc, (size_t)npix * 12, in))) return rc;
     if ((rc = state_stage(ctx, st, npix, d))) return rc;
+    if ((rc = time_begin(ctx))) return rc;
     hipLaunchKernelGGL(k_accumulate, dim3((npix + BLOCK - 1) / BLOCK), dim3(BLOCK), 0, ctx->stream, (const float*)in.dev, npix, *o,
                        (float*)d.acc.dev, (uint32_t*)d.ns.dev, (float*)d.m2.dev, (const uint8_t*)d.conv.dev);
     HIPCHECK(ctx, hipGetLastError());
+    if ((rc = time_end(ctx))) return rc;
     return state_finish(ctx, st, d);
 }
 
@@ -1892,9 +1949,11 @@ int rt_mark_converged(rt_ctx* ctx, uint32_t npix, const rt_accum_opts* o, const 
     if ((rc = state_stage(ctx, st, npix, d))) return rc;
     if ((rc = ctl_buffer(ctx))) return rc;
     HIPCHECK(ctx, hipMemsetAsync(ctx->prog_ctl, 0, 4 * sizeof(uint32_t), ctx->stream));
-    hipLaunchKernelGGL(k_mark_converged, dim3((npix + BLOCK - 1) / BLOCK), dim3(BLOCK), 0, ctx->stream, npix, o->threshold,
+    if ((rc = time_begin(ctx))) return rc;
+    hipLaunchKernelGGL(k_mark_converged, dim3(std::min((npix + BLOCK - 1) / BLOCK, RT_COUNT_GRID)), dim3(BLOCK), 0, ctx->stream, npix, o->threshold,
                        (const float*)d.acc.dev, (const uint32_t*)d.ns.dev, (const float*)d.m2.dev, (uint8_t*)d.conv.dev, ctx->prog_ctl);
     HIPCHECK(ctx, hipGetLastError());
+    if ((rc = time_end(ctx))) return rc;
     uint32_t cnt = 0;
     HIPCHECK(ctx, hipMemcpyAsync(&cnt, ctx->prog_ctl, 4, hipMemcpyDeviceToHost, ctx->stream));
     if ((rc = state_finish(ctx, st, d))) return rc;
@@ -1916,9 +1975,11 @@ int rt_render_aux(rt_ctx* ctx, rt_scene* s, const rt_camera* cam, uint32_t w, ui
     memset(&p, 0, sizeof p);
     DCamera C; DParams P;
     to_dev(cam, &p, C, P);
+    if ((rc = time_begin(ctx))) return rc;
     hipLaunchKernelGGL(k_aux, dim3((npix + BLOCK - 1) / BLOCK), dim3(BLOCK), 0, ctx->stream, s->S, C, w, h,
                        (float*)a.dev, (float*)n.dev);
     HIPCHECK(ctx, hipGetLastError());
+    if ((rc = time_end(ctx))) return rc;
     if ((rc = finish_out(ctx, albedo, (size_t)npix * 12, a))) return rc;
     return finish_out(ctx, normal, (size_t)npix * 12, n);
 }
@@ -1939,9 +2000,11 @@ int rt_resolve_rgb8(rt_ctx* ctx, const float* acc, const uint32_t* ns, uint32_t 
     if ((rc = stage_in(ctx, acc, (size_t)npix * 12, a))) return rc;
     if ((rc = stage_in(ctx, ns, ns ? (size_t)npix * 4 : 0, n))) return rc;
     if ((rc = stage_out(ctx, out, (size_t)npix * 3, r, false))) return rc;
+    if ((rc = time_begin(ctx))) return rc;
     hipLaunchKernelGGL(k_resolve, dim3((npix + BLOCK - 1) / BLOCK), dim3(BLOCK), 0, ctx->stream, (const float*)a.dev,
                        (const uint32_t*)n.dev, spp, w, h, *o, (uint8_t*)r.dev);
     HIPCHECK(ctx, hipGetLastError());
+    if ((rc = time_end(ctx))) return rc;
     return finish_out(ctx, out, (size_t)npix * 3, r);
 }
 
@@ -1964,7 +2027,8 @@ int rt_render_progressive(rt_ctx* ctx, rt_scene* s, const rt_camera* cam, const 
     const bool knn = p->use_photons && s->S.ph_knn;
     dim3 grid((npix + BLOCK - 1) / BLOCK);
     HIPCHECK(ctx, hipMemsetAsync(ctx->prog_ctl, 0, 4 * sizeof(uint32_t), ctx->stream));
-    hipLaunchKernelGGL(k_count_flags, grid, dim3(BLOCK), 0, ctx->stream, npix, (const uint8_t*)d.conv.dev, ctx->prog_ctl + 1);
+    hipLaunchKernelGGL(k_count_flags, dim3(std::min((npix + BLOCK - 1) / BLOCK, RT_COUNT_GRID)), dim3(BLOCK), 0, ctx->stream, npix,
+                       (const uint8_t*)d.conv.dev, ctx->prog_ctl + 1);
     if (ctx->timing) HIPCHECK(ctx, hipEventRecord(ctx->ev0, ctx->stream));
     float* acc = (float*)d.acc.dev; uint32_t* ns = (uint32_t*)d.ns.dev; float* m2 = (float*)d.m2.dev; uint8_t* conv = (uint8_t*)d.conv.dev;
     for (uint32_t k = 0; k < samples_max; k++) {
@@ -2022,7 +2086,8 @@ static int wf_buffers(rt_ctx* ctx, uint32_t cap, WfQueues& Q, int slot = 0) {
     Q.counters = (uint32_t*)m;
     Q.cap = cap;
     Q.fb = nullptr;
-    Q.lsamp = nullptr;        // only the adjoint's primal pass routes contributions per sample
+    Q.lsamp = nullptr;        // per-sample contributions: the adjoint's primal pass, the wavefront render loop
+    Q.stop = nullptr;
     Q.sort = 1;
     ctx->wf_cap = cap;
     return RT_OK;
@@ -2047,7 +2112,7 @@ template <bool BT>
 static void wf_chunk(rt_ctx* ctx, hipStream_t st, const DScene& S, const DCamera& C, const DParams& P, const WfQueues& Q0,
                      uint64_t first, uint32_t n, uint32_t grid, int occ, uint32_t depth, const WfBatch& Bt, bool knn) {
     hipLaunchKernelGGL((k_wf_generate<BT>), dim3((n + BLOCK - 1) / BLOCK), dim3(BLOCK), 0, st, C, P, Q0, first, n, Bt);
-    hipLaunchKernelGGL(k_wf_reset, dim3(1), dim3(32), 0, st, Q0.counters, n);
+    hipLaunchKernelGGL(k_wf_reset, dim3(1), dim3(32), 0, st, Q0.counters, n, Q0.stop);
     WfQueues A = Q0, B = Q0;
     B.active = Q0.next; B.next = Q0.active;
     for (uint32_t d = 0; d < depth; d++) {
@@ -2211,6 +2276,103 @@ extern "C" int rt_render_wavefront(rt_ctx* ctx, rt_scene* s, const rt_camera* ca
     if ((rc = wf_lanes(ctx, o, Qs))) return rc;
     if ((rc = wf_run<false>(ctx, s->S, C, P, Qs, o.streams, o, total, P.max_depth, none, P.use_photons && s->S.ph_knn))) return rc;
     return finish_out(ctx, out, (size_t)npix * 12, rout);
+}
+
+// rt_render_progressive through the wavefront: chunks of K whole passes
+// (K = capacity / pixels, 1 when adaptive) render their per-sample radiance
+// into a buffer of the chunk's lane; k_accum_passes then folds the passes
+// into the pixel state in order.  Chunks go round-robin over the streams;
+// each chunk's bookkeeping waits for the previous chunk's (an event per
+// lane), its rendering does not.
+extern "C" int rt_render_progressive_wavefront(rt_ctx* ctx, rt_scene* s, const rt_camera* cam, const rt_sample_params* p,
+                                               uint32_t samples_max, const rt_accum_opts* ao, const rt_pixel_state* state,
+                                               const rt_wavefront_opts* opts, uint32_t* samples_done, uint32_t* n_converged) {
+    if (!ctx || !s || !cam || !p) return RT_ERR_INVALID;
+    int rc;
+    if ((rc = check_accum_opts(ctx, ao))) return rc;
+    if (p->width == 0 || p->height == 0) return fail(ctx, RT_ERR_INVALID, "empty image");
+    if (p->max_depth == 0) return fail(ctx, RT_ERR_INVALID, "max_depth = 0");
+    if ((uint64_t)p->width * p->height > 0xffffffffull / 12) return fail(ctx, RT_ERR_INVALID, "image too large");
+    if (p->use_photons && s->S.num_photons == 0) return fail(ctx, RT_ERR_INVALID, "use_photons without a photon map");
+    HIPCHECK(ctx, hipSetDevice(ctx->device));
+    uint32_t npix = p->width * p->height;
+    StateDev d;
+    if ((rc = state_stage(ctx, state, npix, d))) return rc;
+    if ((rc = ctl_buffer(ctx))) return rc;
+    WfOpts o;
+    if ((rc = wf_opts(ctx, opts, (uint64_t)npix * std::max(samples_max, 1u), o))) return rc;
+    uint32_t K = ao->adaptive ? 1u : std::max(1u, o.cap / npix);
+    if (o.cap < npix) o.cap = (npix + BLOCK - 1) / BLOCK * BLOCK;   // a chunk holds at least one whole pass
+    o.cap = std::max(o.cap, K * npix);
+    WfQueues Qs[RT_WF_MAX_STREAMS];
+    if ((rc = wf_buffers(ctx, o.cap, Qs[0]))) return rc;
+    Qs[0].sort = o.sort;
+    if ((rc = wf_lanes(ctx, o, Qs))) return rc;
+    const uint32_t nchunks = (samples_max + K - 1) / K;
+    const uint32_t L = std::max(1u, std::min(o.streams, nchunks));
+    size_t b_l = ((size_t)K * npix * 12 + 255) & ~(size_t)255;
+    char* side = nullptr;
+    HIPCHECK(ctx, hipMalloc(&side, std::max<size_t>(L * b_l, 256)));
+    std::unique_ptr<void, void (*)(void*)> guard(side, [](void* q) { (void)hipFree(q); });
+    std::vector<hipEvent_t> done_ev(L, nullptr);
+    for (uint32_t k = 0; k < L; k++) HIPCHECK(ctx, hipEventCreateWithFlags(&done_ev[k], hipEventDisableTiming));
+    struct EvGuard { std::vector<hipEvent_t>& v; ~EvGuard() { for (hipEvent_t e : v) if (e) (void)hipEventDestroy(e); } } eg{done_ev};
+    DCamera C; DParams P;
+    to_dev(cam, p, C, P);
+    P.counters = nullptr;
+    const bool knn = p->use_photons && s->S.ph_knn;
+    auto stream_of = [&](uint32_t k) { return k ? ctx->xstream[k - 1] : ctx->stream; };
+    float* acc = (float*)d.acc.dev; uint32_t* ns = (uint32_t*)d.ns.dev; float* m2 = (float*)d.m2.dev; uint8_t* conv = (uint8_t*)d.conv.dev;
+    HIPCHECK(ctx, hipMemsetAsync(ctx->prog_ctl, 0, 4 * sizeof(uint32_t), ctx->stream));
+    hipLaunchKernelGGL(k_count_flags, dim3(std::min((npix + BLOCK - 1) / BLOCK, RT_COUNT_GRID)), dim3(BLOCK), 0, ctx->stream, npix,
+                       (const uint8_t*)conv, ctx->prog_ctl + 1);
+    if (ctx->timing) HIPCHECK(ctx, hipEventRecord(ctx->ev0, ctx->stream));
+    if (L > 1) {
+        HIPCHECK(ctx, hipEventRecord(ctx->ev_fork, ctx->stream));
+        for (uint32_t k = 1; k < L; k++) HIPCHECK(ctx, hipStreamWaitEvent(stream_of(k), ctx->ev_fork, 0));
+    }
+    WfBatch none = {nullptr, nullptr, 0};
+    uint32_t grid_acc = std::min((npix + BLOCK - 1) / BLOCK, 4096u);
+    for (uint32_t c = 0; c < nchunks; c++) {
+        const uint32_t k = c % L;
+        hipStream_t st = stream_of(k);
+        uint32_t pass0 = c * K, kk = std::min(K, samples_max - pass0);
+        float* lsamp = (float*)(side + k * b_l);
+        HIPCHECK(ctx, hipMemsetAsync(lsamp, 0, (size_t)kk * npix * 12, st));
+        WfQueues Qc = Qs[k];
+        Qc.fb = nullptr;
+        Qc.lsamp = lsamp;
+        Qc.stop = ctx->prog_ctl;
+        DParams Pc = P;
+        Pc.sample_idx = P.sample_idx + pass0;
+        wf_chunk<false>(ctx, st, s->S, C, Pc, Qc, 0, kk * npix, o.grid, o.occ, Pc.max_depth, none, knn);
+        if (c > 0 && L > 1) HIPCHECK(ctx, hipStreamWaitEvent(st, done_ev[(c - 1) % L], 0));   // passes in order
+        hipLaunchKernelGGL(k_accum_passes, dim3(grid_acc), dim3(BLOCK), 0, st, (const float*)lsamp, kk, p->width, p->height,
+                           P.sample_idx + pass0, *ao, acc, ns, m2, conv, ctx->prog_ctl);
+        for (uint32_t q = 0; q < kk; q++)
+            hipLaunchKernelGGL(k_progressive_step, dim3(1), dim3(64), 0, st, ctx->prog_ctl, pass0 + q, npix, *ao);
+        HIPCHECK(ctx, hipEventRecord(done_ev[k], st));
+        HIPCHECK(ctx, hipGetLastError());
+    }
+    for (uint32_t k = 1; k < L; k++) {
+        HIPCHECK(ctx, hipEventRecord(ctx->ev_join[k - 1], stream_of(k)));
+        HIPCHECK(ctx, hipStreamWaitEvent(ctx->stream, ctx->ev_join[k - 1], 0));
+    }
+    ctx->ext_used = 0;
+    if (ctx->timing) {
+        HIPCHECK(ctx, hipEventRecord(ctx->ev1, ctx->stream));
+        HIPCHECK(ctx, hipEventSynchronize(ctx->ev1));
+        float ms = 0.0f;
+        HIPCHECK(ctx, hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+        ctx->kernel_ms += ms;
+        ctx->launches += nchunks;
+    }
+    uint32_t ctl[4] = {0, 0, 0, 0};
+    HIPCHECK(ctx, hipMemcpyAsync(ctl, ctx->prog_ctl, sizeof ctl, hipMemcpyDeviceToHost, ctx->stream));
+    if ((rc = state_finish(ctx, state, d))) return rc;
+    if (samples_done) *samples_done = ctl[2];
+    if (n_converged) *n_converged = ctl[1];
+    return RT_OK;
 }
 
 extern "C" int rt_render_batch(rt_ctx* ctx, const rt_batch_view* views, uint32_t nviews, const rt_wavefront_opts* opts) {
@@ -2545,7 +2707,7 @@ extern "C" int rt_backward_wavefront(rt_ctx* ctx, rt_scene* s, const rt_camera* 
         // replay from the same streams
         hipLaunchKernelGGL((k_wf_generate<false>), dim3((m + BLOCK - 1) / BLOCK), dim3(BLOCK), 0, st, C, P, Qk, first, m, none);
         hipLaunchKernelGGL(k_wfb_init, dim3((m + BLOCK - 1) / BLOCK), dim3(BLOCK), 0, st, Qk, A, m);
-        hipLaunchKernelGGL(k_wf_reset, dim3(1), dim3(32), 0, st, Qk.counters, m);
+        hipLaunchKernelGGL(k_wf_reset, dim3(1), dim3(32), 0, st, Qk.counters, m, (const uint32_t*)nullptr);
         WfQueues QA = Qk, QB = Qk;
         QB.active = Qk.next; QB.next = Qk.active;
         for (uint32_t d = 0; d < depth; d++) {

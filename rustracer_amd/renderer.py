@@ -190,16 +190,25 @@ class Renderer:
         return n.value
 
     def render_progressive(self, state, camera, width, height, samples_max, opts=None, sample_idx=0, strata=0,
-                           bg_scale=1.0, photon_map=False, max_depth=50, seed=0):
+                           bg_scale=1.0, photon_map=False, max_depth=50, seed=0, wavefront=False, capacity=0, grid=0,
+                           sort=1, use_graph=False, occupancy=0, streams=1):
         """The reference's render loop (src/main.rs:463-506) on the device;
-        returns (passes run, converged pixels)."""
+        returns (passes run, converged pixels).  wavefront=True renders the
+        passes with the wavefront stages (rt_render_progressive_wavefront)."""
         o = opts if opts is not None else self.accum_opts()
         p = self.params(width, height, sample_idx, strata, bg_scale, max_depth, None, photon_map, seed)
         cam = camera.to_c()
         st = self.pixel_state(state)
         done, nconv = C.c_uint32(), C.c_uint32()
-        self._check(self.L.rt_render_progressive(self.ctx, self.scene, C.byref(cam), C.byref(p), samples_max, C.byref(o),
-                                                 C.byref(st), C.byref(done), C.byref(nconv)), "rt_render_progressive")
+        if wavefront:
+            w = F.rt_wavefront_opts(capacity=capacity, grid=grid, sort=int(sort), use_graph=1 if use_graph else 0,
+                                    occupancy=occupancy, streams=streams)
+            self._check(self.L.rt_render_progressive_wavefront(self.ctx, self.scene, C.byref(cam), C.byref(p), samples_max,
+                                                               C.byref(o), C.byref(st), C.byref(w), C.byref(done),
+                                                               C.byref(nconv)), "rt_render_progressive_wavefront")
+        else:
+            self._check(self.L.rt_render_progressive(self.ctx, self.scene, C.byref(cam), C.byref(p), samples_max, C.byref(o),
+                                                     C.byref(st), C.byref(done), C.byref(nconv)), "rt_render_progressive")
         return done.value, nconv.value
 
     def render_aux(self, camera, width, height, albedo=None, normal=None):

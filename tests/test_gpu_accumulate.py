@@ -183,3 +183,43 @@ def test_aux_pass_matches_checker(gpu_renderer):
     assert np.allclose(ga, ca, rtol=1e-5, atol=1e-6), np.abs(ga - ca).max()
     assert np.allclose(gn, cn, rtol=1e-5, atol=1e-6), np.abs(gn - cn).max()
     assert len(np.unique(ga.reshape(-1, 3).round(3), axis=0)) > 8
+
+
+def loop_wavefront(r, cam, W, H, samples_max, D, seed, adaptive, min_samples, pct, thr):
+    """The same loop with each pass's radiance from a one-sample wavefront
+    frame (its per-sample sums are the wavefront render loop's)."""
+    st = fresh(W * H)
+    n_conv, s = 0, 0
+    while s < samples_max:
+        sc = np.zeros((H, W, 3), np.float32)
+        r.render_wavefront(sc, cam, W, H, 1, sample_idx=s, max_depth=D, seed=seed, capacity=1 << 16)
+        O.accumulate_sample(sc, st)
+        s += 1
+        if adaptive and s >= min_samples:
+            n_conv = O.mark_converged(st, thr)
+        if adaptive and np.float32(n_conv) >= np.float32(W * H) * np.float32(pct) / np.float32(100.0):
+            break
+    return st, s, n_conv
+
+
+@pytest.mark.parametrize("adaptive,cap,streams", [(True, 0, 1), (False, 4 * 1280, 1), (False, 3 * 1280, 3), (True, 0, 2)])
+def test_progressive_wavefront_equals_the_pass_by_pass_loop(gpu_renderer, adaptive, cap, streams):
+    """Passes rendered by the wavefront (several whole passes per chunk when
+    not adaptive, chunks over several streams) fold into the same state as
+    the loop run pass by pass over one-sample wavefront frames."""
+    sc, cam, cfg = scenes.box_room(40, 32)
+    gpu_renderer.upload(sc)
+    W, H, D = 40, 32, cfg["max_depth"]
+    ref, s_ref, c_ref = loop_wavefront(gpu_renderer, cam, W, H, 30, D, 5, adaptive, 6, 40.0, 0.25)
+    st = fresh(W * H)
+    o = Renderer.accum_opts(adaptive=adaptive, min_samples=6, convergence_pct=40.0, threshold=0.25)
+    done, nconv = gpu_renderer.render_progressive(st, cam, W, H, 30, o, max_depth=D, seed=5, wavefront=True,
+                                                  capacity=cap, streams=streams)
+    assert (done, nconv) == (s_ref, c_ref)
+    same(st, ref)
+    # and the megakernel loop agrees to float rounding
+    mk = fresh(W * H)
+    gpu_renderer.render_progressive(mk, cam, W, H, 30, o, max_depth=D, seed=5)
+    if not adaptive:
+        assert np.allclose(mk["accum"], st["accum"], rtol=1e-5, atol=1e-5)
+        assert np.array_equal(mk["samples"], st["samples"])
