@@ -554,3 +554,21 @@ def test_pbr_maps_and_film_same_streams_and_three_sigma(gpu_renderer):
     assert np.isfinite(gp).all()
     z = three_sigma(gp, cp)
     assert (z > 3).mean() <= 0.02 and (z > 6).mean() <= 0.002, ((z > 3).mean(), (z > 6).mean())
+
+
+def test_toml_scene_renders_like_the_checker(gpu_renderer, tmp_path):
+    """A scene file of the reference's TOML format with every shape, material,
+    light and volume kind (rustracer_amd.scene_file) uploads and renders on
+    the device like the checker on the same per-pixel streams (media reseed
+    their tracking stream from the ray's bits, so a few pixels diverge)."""
+    from rustracer_amd import scene_file as SF
+    from tests.test_scene_file import write_scene
+    sc, camera, info = SF.load(write_scene(tmp_path))
+    cam = camera(48 / 32)
+    gpu_renderer.upload(sc)
+    orc = O.OracleScene(sc)
+    g = passes_gpu(gpu_renderer, cam, 48, 32, 2, 12, seed=3)
+    c = passes_cpu(orc, cam, 48, 32, 2, 12, mode=1, seed=3)
+    assert np.isfinite(g).all()
+    close = np.isclose(g, c, rtol=1e-3, atol=1e-4).all(axis=-1)
+    assert close.mean() > 0.85, close.mean()
