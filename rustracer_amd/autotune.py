@@ -29,8 +29,9 @@ def tune_wavefront(r, out, cam, W, H, spp, depth, photons=False, probe_spp=None,
     info) where info["table"] lists every probe as (opts, ms per probe frame)."""
     npix = W * H
     if probe_spp is None:
-        # enough paths for two chunks of the largest capacity, at most the real spp
-        probe_spp = max(1, min(spp, (2 * CAPACITIES[-1] + npix - 1) // npix))
+        # enough paths for four chunks of the largest capacity (so one to four
+        # chunk streams differ in the probe), at most the real spp
+        probe_spp = max(1, min(spp, (4 * CAPACITIES[-1] + npix - 1) // npix))
 
     def run(o, seed):
         r.render_wavefront(out, cam, W, H, probe_spp, max_depth=depth, photon_map=photons, seed=seed, **o)
