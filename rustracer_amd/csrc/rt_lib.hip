@@ -418,6 +418,7 @@ __global__ __launch_bounds__(BLOCK) void k_photon_irr(DScene S, const float* __r
 #define WF_CNT_DENSE 3    // counters[3]: k-NN queries holding at least k photons (second k-NN pass)
 #define WF_CNT_BUCKET 8   // counters[8..8+WF_NB): bucket sizes
 #define WF_CNT_FIRST 32   // counters[32..33]: 64-bit frame position of a graph-replayed chunk
+#define WF_CNT_GEN 34     // counters[34]: paths generated when flagged pixels are skipped
 
 struct __align__(16) WfPath {
     float4 o_time;       // ray origin, time
@@ -446,6 +447,7 @@ struct WfQueues {
     uint32_t sort;        // 0: one bucket; 1: a bucket per material kind; 2: two shade classes
     uint32_t* dense;      // cap: paths whose k-NN query needs the heap search
     const uint32_t* stop; // non-null and nonzero at chunk start: the chunk renders no path (a stopped render loop)
+    const uint8_t* conv;  // non-null: flagged pixels get no path (the render loop's converged pixels)
 };
 
 __device__ __forceinline__ uint32_t lane_id() { return __lane_id(); }
@@ -489,10 +491,14 @@ __device__ __forceinline__ void wf_store(WfPath& w, const PathState& ps, const R
 }
 
 // Zero the per-bounce counters (a kernel, not a memset, so graph replays are safe).
-__global__ void k_wf_reset(uint32_t* counters, uint32_t active, const uint32_t* stop) {
+__global__ void k_wf_reset(uint32_t* counters, uint32_t active, const uint32_t* stop, int generated) {
     uint32_t i = threadIdx.x;
+    if (generated) active = counters[WF_CNT_GEN];
     if (stop && *stop) active = 0;
     if (i < 32) counters[i] = (i == WF_CNT_ACTIVE) ? active : 0u;
+}
+__global__ void k_wf_gen_zero(uint32_t* counters) {
+    if (threadIdx.x == 0) counters[WF_CNT_GEN] = 0u;
 }
 // After shade: survivors become the active set, everything else restarts at 0.
 __global__ void k_wf_advance(uint32_t* counters) {
@@ -575,7 +581,13 @@ __global__ __launch_bounds__(BLOCK) void k_wf_generate(DCamera cam0, DParams P0,
     wf_store(w, ps, rng);
     w.pixel = fb_off + pixel; w.pad0 = view; w.hit_t = 0.0f; w.hit_slot = -1; w.hit_prim = -1; w.key = 0; w.sid = i;
     Q.paths[i] = w;
-    Q.active[i] = i;
+    if (Q.conv) {                         // compact: only unflagged pixels enter the active set
+        bool live = Q.conv[pixel] == 0;
+        uint32_t slot = wave_append(Q.counters + WF_CNT_GEN, 0u, live);
+        if (live) Q.active[slot] = i;
+    } else {
+        Q.active[i] = i;
+    }
 }
 
 // Bucket of a hit: per material kind (sort 1) or per shade class (sort 2:
@@ -2088,6 +2100,7 @@ static int wf_buffers(rt_ctx* ctx, uint32_t cap, WfQueues& Q, int slot = 0) {
     Q.fb = nullptr;
     Q.lsamp = nullptr;        // per-sample contributions: the adjoint's primal pass, the wavefront render loop
     Q.stop = nullptr;
+    Q.conv = nullptr;
     Q.sort = 1;
     ctx->wf_cap = cap;
     return RT_OK;
@@ -2111,8 +2124,9 @@ static void launch_shadow(int occ, uint32_t grid, hipStream_t st, const DScene& 
 template <bool BT>
 static void wf_chunk(rt_ctx* ctx, hipStream_t st, const DScene& S, const DCamera& C, const DParams& P, const WfQueues& Q0,
                      uint64_t first, uint32_t n, uint32_t grid, int occ, uint32_t depth, const WfBatch& Bt, bool knn) {
+    if (Q0.conv) hipLaunchKernelGGL(k_wf_gen_zero, dim3(1), dim3(64), 0, st, Q0.counters);
     hipLaunchKernelGGL((k_wf_generate<BT>), dim3((n + BLOCK - 1) / BLOCK), dim3(BLOCK), 0, st, C, P, Q0, first, n, Bt);
-    hipLaunchKernelGGL(k_wf_reset, dim3(1), dim3(32), 0, st, Q0.counters, n, Q0.stop);
+    hipLaunchKernelGGL(k_wf_reset, dim3(1), dim3(32), 0, st, Q0.counters, n, Q0.stop, Q0.conv ? 1 : 0);
     WfQueues A = Q0, B = Q0;
     B.active = Q0.next; B.next = Q0.active;
     for (uint32_t d = 0; d < depth; d++) {
@@ -2343,6 +2357,7 @@ extern "C" int rt_render_progressive_wavefront(rt_ctx* ctx, rt_scene* s, const r
         Qc.fb = nullptr;
         Qc.lsamp = lsamp;
         Qc.stop = ctx->prog_ctl;
+        Qc.conv = conv;                   // converged pixels: no path (their samples are skipped anyway)
         DParams Pc = P;
         Pc.sample_idx = P.sample_idx + pass0;
         wf_chunk<false>(ctx, st, s->S, C, Pc, Qc, 0, kk * npix, o.grid, o.occ, Pc.max_depth, none, knn);
@@ -2707,7 +2722,7 @@ extern "C" int rt_backward_wavefront(rt_ctx* ctx, rt_scene* s, const rt_camera* 
         // replay from the same streams
         hipLaunchKernelGGL((k_wf_generate<false>), dim3((m + BLOCK - 1) / BLOCK), dim3(BLOCK), 0, st, C, P, Qk, first, m, none);
         hipLaunchKernelGGL(k_wfb_init, dim3((m + BLOCK - 1) / BLOCK), dim3(BLOCK), 0, st, Qk, A, m);
-        hipLaunchKernelGGL(k_wf_reset, dim3(1), dim3(32), 0, st, Qk.counters, m, (const uint32_t*)nullptr);
+        hipLaunchKernelGGL(k_wf_reset, dim3(1), dim3(32), 0, st, Qk.counters, m, (const uint32_t*)nullptr, 0);
         WfQueues QA = Qk, QB = Qk;
         QB.active = Qk.next; QB.next = Qk.active;
         for (uint32_t d = 0; d < depth; d++) {
