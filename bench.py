@@ -576,7 +576,14 @@ def measure_output(args, with_cpu=True):
     n = W * H
     r = Renderer(0)
     g = torch.Generator(device="cuda").manual_seed(1)
-    sc_buf = torch.rand((n, 3), generator=g, device="cuda") * 2.0
+    # four different passes in turn: the per-pixel variance stays well above
+    # the convergence threshold, so every call does every pixel's work
+    sc_bufs = [torch.rand((n, 3), generator=g, device="cuda") * 2.0 for _ in range(4)]
+    turn = [0]
+
+    def next_pass():
+        turn[0] += 1
+        return sc_bufs[turn[0] % 4]
     st = dict(accum=torch.zeros((n, 3), device="cuda"), samples=torch.zeros(n, dtype=torch.int32, device="cuda"),
               var_m2=torch.zeros((n, 3), device="cuda"), converged=torch.zeros(n, dtype=torch.uint8, device="cuda"))
     o = Renderer.accum_opts(adaptive=True, threshold=0.05)
@@ -593,10 +600,12 @@ def measure_output(args, with_cpu=True):
         r.set_timing(False)
         return ms / reps
 
-    t_acc = timed(lambda: r.accumulate_sample(sc_buf, st, o), 20)
+    t_acc = timed(lambda: r.accumulate_sample(next_pass(), st, o), 20)
     t_mark = timed(lambda: r.mark_converged(st, o), 20)
     t_res = timed(lambda: r.resolve_rgb8(st["accum"], W, H, samples=st["samples"], tonemapper="agx", vignette=0.2,
                                          distortion=0.05, out=rgb8), 20)
+    converged = float(st["converged"].float().mean())
+    assert converged < 0.01, f"timing state converged ({converged}): the bookkeeping kernels would skip work"
     scene, cam, cfg = scenes.mesh_envmap(W, H, level=8)
     r.upload(scene)
     alb = torch.zeros((H, W, 3), device="cuda")
@@ -620,7 +629,7 @@ def measure_output(args, with_cpu=True):
     if with_cpu:
         from oracle import oracle as O
         hs = {k: (v.cpu().numpy().view(np.uint32) if k == "samples" else v.cpu().numpy()) for k, v in st.items()}
-        x = sc_buf.cpu().numpy()
+        x = sc_bufs[1].cpu().numpy()
         reps, t0 = 0, time.perf_counter()
         while time.perf_counter() - t0 < 3.0:
             O.accumulate_sample(x, hs)
