@@ -261,17 +261,29 @@ RT_D uint32_t node_test(const float4* node, const Ray& r, float t_min, float t_m
 // f32::total_cmp key
 RT_D int32_t tkey(float f) { int32_t i = __float_as_int(f); return i ^ (int32_t)(((uint32_t)(i >> 31)) >> 1); }
 
-// Push hit children far-to-near (stable for equal keys) so the nearest pops first.
+// Push hit children far-to-near (stable for equal keys) so the nearest pops
+// first.  Each child's slot is its rank in that order (children with a
+// larger key, or an equal key and a lower index, go below it), so the sort
+// needs no dynamically indexed private array (no scratch traffic).
 RT_D void push_sorted(Stack& st, uint32_t mask, const float tn[8], const int32_t ch[8]) {
-    int32_t kk[8]; int32_t cc[8]; int n = 0;
+    int32_t key[8];
 #pragma unroll
-    for (int c = 0; c < 8; c++) if (mask & (1u << c)) { kk[n] = tkey(tn[c]); cc[n] = ch[c]; n++; }
-    for (int i = 1; i < n; i++) {
-        int32_t k = kk[i], v = cc[i]; int j = i - 1;
-        while (j >= 0 && k > kk[j]) { kk[j + 1] = kk[j]; cc[j + 1] = cc[j]; j--; }
-        kk[j + 1] = k; cc[j + 1] = v;
+    for (int c = 0; c < 8; c++) key[c] = tkey(tn[c]);
+#pragma unroll
+    for (int c = 0; c < 8; c++) {
+        if (!(mask & (1u << c))) continue;
+        int32_t rank = 0;
+#pragma unroll
+        for (int d = 0; d < 8; d++) {
+            if (d == c) continue;
+            bool before = (mask & (1u << d)) && (key[d] > key[c] || (key[d] == key[c] && d < c));
+            rank += before ? 1 : 0;
+        }
+        int32_t at = st.sp + rank;
+        if (at < RT_LDS_STACK) st.lds[at * st.stride] = ch[c];
+        else st.priv[at - RT_LDS_STACK] = ch[c];
     }
-    for (int i = 0; i < n; i++) st_push(st, cc[i]);
+    st.sp += __popc(mask);
 }
 
 // ---- cylinder / cone / disk / infinite plane ----------------------------------------------
