@@ -1095,10 +1095,11 @@ RT_D void knn_heap_add(float* heap, uint32_t stride, uint32_t k, uint32_t& cnt, 
 // is re-tested when it is popped); the sum then covers only the first k
 // photons found, which is the whole gather when fewer than k lie within r2.
 // KNN = false with stop > 0: give up once `stop` photons are counted (the
-// caller then knows the query holds at least that many).
+// caller then knows the query holds at least that many).  SUM = false (with
+// KNN): no sum, for callers that already know k photons lie within r2.
 // Stackless: bit d of `pend` marks the far child of the depth-d plane as
 // still to visit; the ancestor d levels up of node i is ((i + 1) >> d) - 1.
-template <bool KNN>
+template <bool KNN, bool SUM = true>
 RT_D V3 pk_walk(const DScene& S, V3 pos, V3 nrm, float r2, float* heap, uint32_t stride, uint32_t& count, float& bound_out,
                 uint32_t stop = 0) {
     const uint32_t D = S.pk_depth, k = S.ph_knn;
@@ -1122,7 +1123,7 @@ RT_D V3 pk_walk(const DScene& S, V3 pos, V3 nrm, float r2, float* heap, uint32_t
             float dx = P.x - pos.x, dy = P.y - pos.y, dz = P.z - pos.z;
             float d2 = dx * dx + dy * dy + dz * dz;
             if (d2 < bound && N.x * nrm.x + N.y * nrm.y + N.z * nrm.z > 0.0f) {
-                if (!KNN || cnt < k) { float4 W = S.pk_pow[i]; acc += mk(W.x, W.y, W.z) * (1.0f - d2 / bound); }
+                if (!KNN || (SUM && cnt < k)) { float4 W = S.pk_pow[i]; acc += mk(W.x, W.y, W.z) * (1.0f - d2 / bound); }
                 if (KNN) {
                     knn_heap_add(heap, stride, k, cnt, d2);
                     if (cnt == k) bound = heap[0];
@@ -1187,6 +1188,22 @@ RT_D V3 photon_knn_irradiance(const DScene& S, V3 pos, V3 nrm, float* heap, uint
     if (cnt < S.ph_knn) return acc * S.ph_norm;
     float norm = r2 == S.ph_r2 ? S.ph_norm : 2.0f / (PI * PI * r2);
     float unused;
+    return pk_walk<false>(S, pos, nrm, r2, nullptr, 0, cnt, unused) * norm;
+}
+
+// Second pass of the wavefront's k-NN stage, for a query that the first pass
+// found to hold at least k photons within ph_r2: the heap search without the
+// sum of its first k photons (never used then), then the gather at the k-th
+// distance.  Equals photon_knn_irradiance; should the search hold fewer than
+// k after all, the gather within ph_r2 is that function's sum (same walk,
+// same order, the radius never shrinking).
+RT_D V3 photon_knn_dense(const DScene& S, V3 pos, V3 nrm, float* heap, uint32_t stride) {
+    uint32_t cnt;
+    float unused;
+    pk_walk<true, false>(S, pos, nrm, S.ph_r2, heap, stride, cnt, unused);
+    if (cnt < S.ph_knn) return pk_walk<false>(S, pos, nrm, S.ph_r2, nullptr, 0, cnt, unused) * S.ph_norm;
+    float r2 = heap[0] > 0.0f ? heap[0] : S.ph_r2;
+    float norm = r2 == S.ph_r2 ? S.ph_norm : 2.0f / (PI * PI * r2);
     return pk_walk<false>(S, pos, nrm, r2, nullptr, 0, cnt, unused) * norm;
 }
 

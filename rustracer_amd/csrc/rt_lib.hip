@@ -764,7 +764,7 @@ __global__ __launch_bounds__(BLOCK) void k_wf_knn_dense(DScene S0, WfQueues Q, W
         const DScene& S = BATCH ? B.scenes[w.pad0] : S0;
         V3 pos, nrm;
         knn_query<BATCH>(S, w, pos, nrm);
-        knn_store(w, photon_knn_irradiance(S, pos, nrm, heap, BLOCK));
+        knn_store(w, photon_knn_dense(S, pos, nrm, heap, BLOCK));
     }
 }
 
