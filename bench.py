@@ -552,11 +552,12 @@ def measured_traffic(workload, opts):
         return None, "no PMC summary"
     if not rec:
         return None, "no PMC summary for this workload"
-    # graph replay changes launch overhead, not what a dispatch reads or writes
-    keys = [k for k in rec["wavefront_opts"] if k != "use_graph"]
-    if any(opts.get(k) != rec["wavefront_opts"][k] for k in keys):
-        return None, f"PMC summary measured with {rec['wavefront_opts']}, not {opts}"
-    return rec["traffic_bytes_per_dispatch"], rec["source"]
+    # one record per set of launch options that changes what a dispatch reads
+    # or writes (graph replay and chunk streams change neither)
+    for r in rec if isinstance(rec, list) else [rec]:
+        if all(opts.get(k) == v for k, v in r["wavefront_opts"].items() if k not in ("use_graph", "streams")):
+            return r["traffic_bytes_per_dispatch"], r["source"]
+    return None, f"no PMC summary measured with {opts}"
 
 
 def measure_output(args, with_cpu=True):
