@@ -16,10 +16,12 @@ enum { M_LAMBERTIAN = 1, M_DIELECTRIC = 2, M_SPECTRAL_DIELECTRIC = 3, M_PBR = 4,
 #define M_KINDS 11
 enum { BGK_SOLID = 0, BGK_GRADIENT = 1, BGK_PHYSICAL = 2, BGK_ENV = 3, BGK_ENV_SPLIT = 4 };
 
-__constant__ float c_cie_x[RT_CIE_N];
-__constant__ float c_cie_y[RT_CIE_N];
-__constant__ float c_cie_z[RT_CIE_N];
-__constant__ float c_metal[6][RT_METAL_N];
+// Initialised in place: every kernel translation unit holds its own copy.
+__constant__ float c_cie_x[RT_CIE_N] = {RT_CIE_X_DATA};
+__constant__ float c_cie_y[RT_CIE_N] = {RT_CIE_Y_DATA};
+__constant__ float c_cie_z[RT_CIE_N] = {RT_CIE_Z_DATA};
+__constant__ float c_metal[6][RT_METAL_N] = {{RT_GOLD_N_DATA}, {RT_GOLD_K_DATA}, {RT_COPPER_N_DATA},
+                                             {RT_COPPER_K_DATA}, {RT_SILVER_N_DATA}, {RT_SILVER_K_DATA}};
 
 // ---- spectrum (src/spectrum.rs:64-277) ------------------------------------------
 RT_HD void xyz_srgb(float x, float y, float z, float o[3]) {

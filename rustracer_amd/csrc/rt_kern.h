@@ -1,0 +1,214 @@
+// Shared by the kernel translation units and the host code of librtamd:
+// launch geometry, the per-lane LDS traversal stack and k-NN heap, and the
+// device-resident layouts of the wavefront renderer (path records, queues,
+// shadow requests, batch views, adjoint replay records).
+#pragma once
+#include "dev_shade.h"
+#include "dev_adjoint.h"
+#include "dev_accum.h"
+#include "../../include/rt_api.h"
+
+using namespace rt;
+
+#define BLOCK 128
+
+struct LdsStack {
+    __device__ static int32_t* column() {
+        __shared__ int32_t s[RT_LDS_STACK * BLOCK];
+        return s + threadIdx.x;
+    }
+};
+// The k-NN photon gather's max-heap: RT_KNN_MAX floats per lane (32 KB per
+// block), allocated only by the kernel variants that search k-NN radii.
+struct LdsHeap {
+    __device__ static float* column() {
+        __shared__ float h[RT_KNN_MAX * BLOCK];
+        return h + threadIdx.x;
+    }
+};
+
+__device__ __forceinline__ void init_stack(Stack& st) {
+    st.lds = LdsStack::column(); st.stride = BLOCK; st.sp = 0; st.heap = nullptr;
+    st.n_nodes = 0; st.n_prims = 0; st.n_rays = 0;
+    st.s_nodes = 0; st.s_prims = 0; st.s_rays = 0;
+}
+
+__device__ __forceinline__ Ray ray_from_c(const rt_ray& q) {
+    return make_ray(mk(q.origin[0], q.origin[1], q.origin[2]), mk(q.dir[0], q.dir[1], q.dir[2]), q.time);
+}
+
+// Add a per-lane count into *count with one atomic per wave (every lane of
+// the wave calls it).
+__device__ __forceinline__ void wave_count(uint32_t local, uint32_t* count) {
+    for (int off = 32; off > 0; off >>= 1) local += __shfl_xor(local, off);
+    if (__lane_id() == 0 && local) atomicAdd(count, local);
+}
+#define RT_COUNT_GRID 1024u   // blocks of the grid-stride counting kernels: few atomics on one counter
+
+#define RT_MAX_PARAMS 64
+
+// Photon emission + tracing (PhotonMap::build_from_quad / build, src/photon.rs:41-135, 232-277).
+struct DEmitter {
+    uint32_t kind;
+    float o[3], u[3], v[3], n[3], ta[3], ba[3];
+    float power[3];
+    float lum_clamp;
+    float disk_c[3], disk_r, t[3], b[3], sun_down[3];
+};
+
+// ============================================================================ wavefront
+// Path records live in HBM (128 B each, one cache line); queues hold path ids.
+// Per bounce: k_wf_extend traces every active path and files its id under the
+// material key of the hit (the sort), k_wf_shade runs one bounce in key order
+// and appends survivors and shadow requests, k_wf_shadow resolves the shadow
+// requests.  All three are persistent grid-stride kernels that read their
+// queue length from device memory, so the host never synchronises inside a
+// frame.  Contributions go straight into the frame accumulator with float
+// atomics.
+#define WF_NB (M_KINDS + 2)   // sort keys (sort 1): 0 miss, 1..M_KINDS material kind, M_KINDS + 1 medium event;
+                          // (sort 2): 0 miss / specular / medium, 1 every other material
+#define WF_CNT_ACTIVE 0   // counters[0]: paths to extend this bounce
+#define WF_CNT_NEXT 1     // counters[1]: survivors written by shade
+#define WF_CNT_SHADOW 2   // counters[2]: shadow requests
+#define WF_CNT_DENSE 3    // counters[3]: k-NN queries holding at least k photons (second k-NN pass)
+#define WF_CNT_BUCKET 8   // counters[8..8+WF_NB): bucket sizes
+#define WF_CNT_FIRST 32   // counters[32..33]: 64-bit frame position of a graph-replayed chunk
+#define WF_CNT_GEN 34     // counters[34]: paths generated when flagged pixels are skipped
+
+struct __align__(16) WfPath {
+    float4 o_time;       // ray origin, time
+    float4 d_wl;         // ray direction, wavelength
+    float4 thr_mis;      // throughput, prev_mis
+    float sun_w; uint32_t flags; uint32_t pixel; uint32_t pad0;   // flags: 1 spectral, 2 prev_specular
+    uint64_t rng[4];
+    float hit_t; int32_t hit_slot; int32_t hit_prim; uint32_t key;
+    uint32_t sid;        // sample index within the chunk (per-sample radiance of the adjoint's primal pass)
+    uint32_t pad1[3];
+};
+static_assert(sizeof(WfPath) == 128, "path record is one 128-B line");
+
+struct __align__(16) WfShadow { float4 o_tmax; float4 d_time; float4 c_pixel; uint4 meta; };   // meta.x = view
+
+struct WfQueues {
+    WfPath* paths;
+    uint32_t cap;
+    uint32_t* active;     // cap
+    uint32_t* next;       // cap
+    uint32_t* buckets;    // WF_NB * cap
+    WfShadow* shadow;     // 3 * cap
+    uint32_t* counters;   // 64: 32 per-bounce counters, then the chunk position
+    float* fb;
+    float* lsamp;         // non-null: contributions go to the path's sample (3 floats per sid), not to fb
+    uint32_t sort;        // 0: one bucket; 1: a bucket per material kind; 2: two shade classes
+    uint32_t* dense;      // cap: paths whose k-NN query needs the heap search
+    const uint32_t* stop; // non-null and nonzero at chunk start: the chunk renders no path (a stopped render loop)
+    const uint8_t* conv;  // non-null: flagged pixels get no path (the render loop's converged pixels)
+};
+
+__device__ __forceinline__ uint32_t lane_id() { return __lane_id(); }
+
+// Wave-aggregated append: one atomic per distinct key in the wave.
+__device__ __forceinline__ uint32_t wave_append(uint32_t* counters, uint32_t key, bool valid) {
+    uint32_t slot = 0;
+    uint32_t lane = lane_id();
+    unsigned long long pending = __ballot(valid);
+    while (pending) {
+        int leader = __ffsll(pending) - 1;
+        uint32_t k = __shfl(key, leader);
+        unsigned long long m = __ballot(valid && key == k);
+        uint32_t base = 0;
+        if ((int)lane == leader) base = atomicAdd(&counters[k], (uint32_t)__popcll(m));
+        base = __shfl(base, leader);
+        if (valid && key == k) slot = base + (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
+        pending &= ~m;
+    }
+    return slot;
+}
+
+__device__ __forceinline__ void wf_load(const WfPath& w, PathState& ps, Rng& rng) {
+    ps.ray = make_ray(mk(w.o_time.x, w.o_time.y, w.o_time.z), mk(w.d_wl.x, w.d_wl.y, w.d_wl.z), w.o_time.w);
+    ps.ray.wavelength = w.d_wl.w;
+    ps.ray.spectral = (w.flags & 1u) != 0;
+    ps.thr = mk(w.thr_mis.x, w.thr_mis.y, w.thr_mis.z);
+    ps.prev_mis = w.thr_mis.w;
+    ps.prev_sun_w = w.sun_w;
+    ps.prev_specular = (w.flags & 2u) != 0;
+    ps.knn_irr = mk(__uint_as_float(w.pad1[0]), __uint_as_float(w.pad1[1]), __uint_as_float(w.pad1[2]));
+    rng.s0 = w.rng[0]; rng.s1 = w.rng[1]; rng.s2 = w.rng[2]; rng.s3 = w.rng[3];
+}
+__device__ __forceinline__ void wf_store(WfPath& w, const PathState& ps, const Rng& rng) {
+    w.o_time = make_float4(ps.ray.o.x, ps.ray.o.y, ps.ray.o.z, ps.ray.time);
+    w.d_wl = make_float4(ps.ray.d.x, ps.ray.d.y, ps.ray.d.z, ps.ray.wavelength);
+    w.thr_mis = make_float4(ps.thr.x, ps.thr.y, ps.thr.z, ps.prev_mis);
+    w.sun_w = ps.prev_sun_w;
+    w.flags = (ps.ray.spectral ? 1u : 0u) | (ps.prev_specular ? 2u : 0u);
+    w.rng[0] = rng.s0; w.rng[1] = rng.s1; w.rng[2] = rng.s2; w.rng[3] = rng.s3;
+}
+
+
+// One render of a batch: its camera and pass parameters, its slice of the
+// global path index space and of the concatenated frame accumulator.
+struct DView {
+    DCamera cam;
+    DParams P;
+    uint64_t first, count;    // global path range [first, first + count)
+    uint32_t fb_off, npix;
+};
+struct WfBatch {
+    const DScene* scenes;     // per view
+    const DView* views;
+    uint32_t nviews;
+};
+
+// Path j of a sample pass covers pixel tile_order_pixel(j): bands of 8 rows,
+// each walked in 8-column groups, row-major inside a group, so a wave's 64
+// camera rays leave an 8 x 8 block of the image (coherent traversal and
+// photon lookups) instead of a 64-pixel row.  A bijection of [0, W * H); the
+// pixel's random stream depends on the pixel, not on j.
+__device__ __forceinline__ uint32_t tile_order_pixel(uint32_t j, uint32_t W, uint32_t H) {
+    uint32_t band = j / (8u * W);
+    uint32_t hb = min(8u, H - 8u * band);
+    uint32_t k = j - band * 8u * W;
+    uint32_t grp = k / (8u * hb);
+    uint32_t gw = min(8u, W - 8u * grp);
+    uint32_t local = k - grp * 8u * hb;
+    uint32_t row = local / gw;
+    return (band * 8u + row) * W + grp * 8u + (local - row * gw);
+}
+
+// Whether the path's new hit takes a k-NN photon estimate in shade: a surface
+// (not a medium event) whose material can take the diffuse branch.
+__device__ __forceinline__ bool knn_gathers(const DScene& S, const WfPath& w) {
+    if (S.ph_knn == 0 || S.num_photons == 0 || w.hit_slot < 0 || w.hit_prim == -2) return false;
+    uint32_t kind = S.mats[S.objs[w.hit_slot].material].kind;
+    return kind == M_LAMBERTIAN || kind == M_PBR || kind == M_PEARL;
+}
+
+// ---- wavefront path replay (adjoint) ---------------------------------------------------------
+// Per path slot, beside its WfPath: the radiance still to be collected, the
+// loss gradient of its sample, the pending score of the last scattered vertex
+// (applied when the next vertex is shaded, after that vertex's shadow rays
+// have been subtracted) and the vertex score of the current vertex's NEE.
+struct __align__(16) WfAdj {
+    float4 rem;                       // xyz
+    float4 delta;                     // xyz
+    int32_t pn, vn;
+    int8_t pp[RT_SCORE_MAX], vp[3];
+    int8_t pad0[7];
+    float pv[3 * RT_SCORE_MAX];
+    float vv[9];
+    float pad1[3];
+};
+static_assert(sizeof(WfAdj) == 176, "WfAdj layout");
+
+struct WfAdjQ {
+    WfAdj* adj;
+    const float* lsamp;               // primal radiance per sample of the chunk
+    const float* dimg;                // dLoss/dI per pixel
+    const int8_t* fmap;
+    const MedPar* mp;                 // parameterised media
+    int nmp;
+    float inv_spp;
+    uint32_t nparams;
+    float* grad;
+};

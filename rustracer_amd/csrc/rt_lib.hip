@@ -1,13 +1,11 @@
 // librtamd: the C ABI of include/rt_api.h on MI355X (gfx950).
 //
-// One translation unit: the HIP kernels of the radiance path (trace batches,
-// the plain per-pixel megakernel, photon tracing and gather, the E_ss table)
-// and the host code that owns contexts, uploads scenes and launches.
-#include "dev_shade.h"
-#include "dev_adjoint.h"
-#include "dev_accum.h"
+// Host code: contexts, scene upload (flattening, BVH build, tables), photon
+// map construction, and the launches of the kernels, which live in the k_*.hip
+// translation units (declared in rt_kdecl.h) so they compile in parallel.
+#include "rt_kern.h"
+#include "rt_kdecl.h"
 #include "bvh_build.h"
-#include "../../include/rt_api.h"
 
 #include <string>
 #include <vector>
@@ -19,1012 +17,6 @@
 
 using namespace rt;
 
-#define BLOCK 128
-
-// ============================================================================ kernels
-struct LdsStack {
-    __device__ static int32_t* column() {
-        __shared__ int32_t s[RT_LDS_STACK * BLOCK];
-        return s + threadIdx.x;
-    }
-};
-// The k-NN photon gather's max-heap: RT_KNN_MAX floats per lane (32 KB per
-// block), allocated only by the kernel variants that search k-NN radii.
-struct LdsHeap {
-    __device__ static float* column() {
-        __shared__ float h[RT_KNN_MAX * BLOCK];
-        return h + threadIdx.x;
-    }
-};
-
-__device__ __forceinline__ void init_stack(Stack& st) {
-    st.lds = LdsStack::column(); st.stride = BLOCK; st.sp = 0; st.heap = nullptr;
-    st.n_nodes = 0; st.n_prims = 0; st.n_rays = 0;
-    st.s_nodes = 0; st.s_prims = 0; st.s_rays = 0;
-}
-
-__device__ __forceinline__ Ray ray_from_c(const rt_ray& q) {
-    return make_ray(mk(q.origin[0], q.origin[1], q.origin[2]), mk(q.dir[0], q.dir[1], q.dir[2]), q.time);
-}
-
-__global__ __launch_bounds__(BLOCK) void k_trace_closest(DScene S, const rt_ray* __restrict__ rays, uint32_t n, rt_hit* __restrict__ hits) {
-    Stack st; init_stack(st);
-    uint32_t i = blockIdx.x * BLOCK + threadIdx.x;
-    if (i >= n) return;
-    rt_ray q = rays[i];
-    Ray r = ray_from_c(q);
-    HitId id = trace_closest(S, r, q.t_min, q.t_max, st);
-    rt_hit h;
-    memset(&h, 0, sizeof h);
-    if (id.slot >= 0) {
-        Rec rec = finalize(S, r, id, q.t_min);
-        h.t = rec.t; h.p[0] = rec.p.x; h.p[1] = rec.p.y; h.p[2] = rec.p.z;
-        h.n[0] = rec.n.x; h.n[1] = rec.n.y; h.n[2] = rec.n.z;
-        h.u = rec.u; h.v = rec.v; h.obj_id = rec.obj_id; h.prim_id = rec.prim_id; h.front_face = rec.front ? 1u : 0u;
-    } else { h.obj_id = -1; h.prim_id = -1; }
-    hits[i] = h;
-}
-
-__global__ __launch_bounds__(BLOCK) void k_trace_tr(DScene S, const rt_ray* __restrict__ rays, uint32_t n, float* __restrict__ tr, uint8_t* __restrict__ occ) {
-    Stack st; init_stack(st);
-    uint32_t i = blockIdx.x * BLOCK + threadIdx.x;
-    if (i >= n) return;
-    rt_ray q = rays[i];
-    Ray r = ray_from_c(q);
-    if (occ) occ[i] = trace_transmittance(S, r, q.t_min, q.t_max, st, true) <= 0.0f ? 1 : 0;
-    else tr[i] = trace_transmittance(S, r, q.t_min, q.t_max, st, false);
-}
-
-// Plain megakernel: one thread per pixel, `spp` consecutive sample passes,
-// radiance summed in registers and added to (or stored into) `out`.
-template <bool KNN>
-__global__ __launch_bounds__(BLOCK) void k_render(DScene S, DCamera cam, DParams P, const uint8_t* __restrict__ conv,
-                                                  float* __restrict__ out, uint32_t spp, int accumulate) {
-    Stack st; init_stack(st);
-    if (KNN) st.heap = LdsHeap::column();
-    uint32_t i = blockIdx.x * BLOCK + threadIdx.x;
-    uint32_t npix = P.width * P.height;
-    if (i >= npix) return;
-    float sx = 0.0f, sy = 0.0f, sz = 0.0f;
-    if (!(conv && conv[i])) {
-        uint32_t row = i / P.width, col = i - row * P.width;
-        for (uint32_t k = 0; k < spp; k++) {
-            uint32_t sidx = P.sample_idx + k;
-            Rng rng = rng_seed(pixel_seed(i, sidx, P.seed));
-            Ray r = pixel_ray(cam, P, row, col, sidx, rng);
-            V3 c = ray_color(S, r, (int)P.max_depth, P.bg_scale, P.use_photons != 0, rng, st);
-            sx += c.x; sy += c.y; sz += c.z;
-        }
-    }
-    if (accumulate) { out[3 * i] += sx; out[3 * i + 1] += sy; out[3 * i + 2] += sz; }
-    else { out[3 * i] = sx; out[3 * i + 1] = sy; out[3 * i + 2] = sz; }
-    if (P.counters) {
-        atomicAdd(P.counters + 0, (unsigned long long)st.n_nodes);
-        atomicAdd(P.counters + 1, (unsigned long long)st.n_prims);
-        atomicAdd(P.counters + 2, (unsigned long long)st.n_rays);
-        atomicAdd(P.counters + 3, (unsigned long long)((conv && conv[i]) ? 0u : spp));
-        atomicAdd(P.counters + 4, (unsigned long long)st.s_nodes);
-        atomicAdd(P.counters + 5, (unsigned long long)st.s_prims);
-        atomicAdd(P.counters + 6, (unsigned long long)st.s_rays);
-    }
-}
-
-// accumulate_sample (src/main.rs:603-644), one thread per pixel.
-__global__ __launch_bounds__(BLOCK) void k_accumulate(const float* __restrict__ sc, uint32_t npix, rt_accum_opts o,
-                                                      float* acc, uint32_t* ns, float* m2, const uint8_t* conv) {
-    uint32_t i = blockIdx.x * BLOCK + threadIdx.x;
-    if (i >= npix || conv[i]) return;
-    accumulate_pixel(acc + 3 * (size_t)i, ns + i, m2 + 3 * (size_t)i, mk(sc[3 * (size_t)i], sc[3 * (size_t)i + 1], sc[3 * (size_t)i + 2]), o);
-}
-
-// Add a per-lane count into *count with one atomic per wave (every lane of
-// the wave calls it).
-__device__ __forceinline__ void wave_count(uint32_t local, uint32_t* count) {
-    for (int off = 32; off > 0; off >>= 1) local += __shfl_xor(local, off);
-    if (__lane_id() == 0 && local) atomicAdd(count, local);
-}
-#define RT_COUNT_GRID 1024u   // blocks of the grid-stride counting kernels: few atomics on one counter
-
-// mark_converged (src/main.rs:646-677): flag, then count every flagged pixel.
-__global__ __launch_bounds__(BLOCK) void k_mark_converged(uint32_t npix, float threshold, const float* acc, const uint32_t* ns,
-                                                          const float* m2, uint8_t* conv, uint32_t* count) {
-    uint32_t local = 0;
-    for (uint32_t i = blockIdx.x * BLOCK + threadIdx.x; i < npix; i += gridDim.x * BLOCK) {
-        bool c = conv[i] != 0;
-        if (!c && pixel_converged(acc + 3 * (size_t)i, ns[i], m2 + 3 * (size_t)i, threshold)) { conv[i] = 1; c = true; }
-        local += c ? 1u : 0u;
-    }
-    wave_count(local, count);
-}
-
-// Flagged pixels of a state handed to rt_render_progressive.
-__global__ __launch_bounds__(BLOCK) void k_count_flags(uint32_t npix, const uint8_t* conv, uint32_t* count) {
-    uint32_t local = 0;
-    for (uint32_t i = blockIdx.x * BLOCK + threadIdx.x; i < npix; i += gridDim.x * BLOCK) local += conv[i] != 0 ? 1u : 0u;
-    wave_count(local, count);
-}
-
-// Mean radiance of pixel j: accum / max(n, 1) per component (src/main.rs:574-577).
-__device__ __forceinline__ V3 resolve_raw(const float* acc, const uint32_t* ns, uint32_t spp, uint32_t j) {
-    float n = (float)max(ns ? ns[j] : spp, 1u);
-    return mk(acc[3 * (size_t)j] / n, acc[3 * (size_t)j + 1] / n, acc[3 * (size_t)j + 2] / n);
-}
-// rt_resolve_rgb8: apply_optics (src/output.rs:150-186) reading the means on
-// the fly, then tone_map (src/output.rs:103-111); one thread per pixel.
-__global__ __launch_bounds__(BLOCK) void k_resolve(const float* __restrict__ acc, const uint32_t* __restrict__ ns, uint32_t spp,
-                                                   uint32_t w, uint32_t h, rt_resolve_opts o, uint8_t* __restrict__ out) {
-    uint32_t i = blockIdx.x * BLOCK + threadIdx.x;
-    if (i >= w * h) return;
-    uint32_t y = i / w, x = i - y * w;
-    V3 c;
-    if (o.vignette == 0.0f && o.distortion == 0.0f) {
-        c = resolve_raw(acc, ns, spp, i);
-    } else {
-        float cx = ((float)w - 1.0f) * 0.5f, cy = ((float)h - 1.0f) * 0.5f;
-        float rc = fmaxf(sqrtf(cx * cx + cy * cy), 1e-6f);
-        float dx = ((float)x - cx) / rc, dy = ((float)y - cy) / rc;
-        float r2 = dx * dx + dy * dy;
-        float s = 1.0f - o.distortion * r2;
-        float sx = fclamp(cx + dx * rc * s, 0.0f, (float)w - 1.0f);
-        float sy = fclamp(cy + dy * rc * s, 0.0f, (float)h - 1.0f);
-        uint32_t x0 = f2u(sx), y0 = f2u(sy);
-        uint32_t x1 = min(x0 + 1, w - 1), y1 = min(y0 + 1, h - 1);
-        float fx = sx - (float)x0, fy = sy - (float)y0;
-        V3 c00 = resolve_raw(acc, ns, spp, y0 * w + x0), c10 = resolve_raw(acc, ns, spp, y0 * w + x1);
-        V3 c01 = resolve_raw(acc, ns, spp, y1 * w + x0), c11 = resolve_raw(acc, ns, spp, y1 * w + x1);
-        c = (c00 * (1.0f - fx) + c10 * fx) * (1.0f - fy) + (c01 * (1.0f - fx) + c11 * fx) * fy;
-        c = c * fmaxf(1.0f - o.vignette * r2, 0.0f);
-    }
-    V3 m = o.tonemapper == RT_TONE_AGX ? agx_tonemap(c * o.exposure) : aces_tonemap(c * o.exposure);
-    out[3 * (size_t)i] = srgb_byte(m.x);
-    out[3 * (size_t)i + 1] = srgb_byte(m.y);
-    out[3 * (size_t)i + 2] = srgb_byte(m.z);
-}
-
-// One pass of the reference's render loop (src/main.rs:463-506) fused per
-// pixel: render_tiles' sample, accumulate_sample, and (adaptive, once the
-// frame's pass count sample_idx + 1 reaches min_samples) mark_converged.  ctl[0] = stopped (set between
-// passes by k_progressive_step), ctl[1] = converged pixels, ctl[2] = passes run.
-template <bool KNN>
-__global__ __launch_bounds__(BLOCK) void k_progressive(DScene S, DCamera cam, DParams P, rt_accum_opts o, uint32_t pass,
-                                                       float* acc, uint32_t* ns, float* m2, uint8_t* conv, uint32_t* ctl) {
-    if (ctl[0]) return;
-    Stack st; init_stack(st);
-    if (KNN) st.heap = LdsHeap::column();
-    uint32_t i = blockIdx.x * BLOCK + threadIdx.x;
-    uint32_t npix = P.width * P.height;
-    bool newly = false;
-    if (i < npix && !conv[i]) {
-        uint32_t row = i / P.width, col = i - row * P.width;
-        Rng rng = rng_seed(pixel_seed(i, P.sample_idx, P.seed));
-        Ray r = pixel_ray(cam, P, row, col, P.sample_idx, rng);
-        V3 c = ray_color(S, r, (int)P.max_depth, P.bg_scale, P.use_photons != 0, rng, st);
-        float* a = acc + 3 * (size_t)i;
-        float* v = m2 + 3 * (size_t)i;
-        accumulate_pixel(a, ns + i, v, c, o);
-        if (o.adaptive && P.sample_idx + 1 >= o.min_samples && pixel_converged(a, ns[i], v, o.threshold)) {
-            conv[i] = 1;
-            newly = true;
-        }
-    }
-    wave_count(newly ? 1u : 0u, ctl + 1);
-}
-// After pass `pass`: count it, and stop when adaptive and the converged count
-// reaches convergence_pct % of the pixels (the reference's target_reached).
-__global__ void k_progressive_step(uint32_t* ctl, uint32_t pass, uint32_t npix, rt_accum_opts o) {
-    if (threadIdx.x != 0 || ctl[0]) return;
-    ctl[2] = pass + 1;
-    if (o.adaptive && (float)ctl[1] >= (float)npix * o.convergence_pct / 100.0f) ctl[0] = 1;
-}
-
-// render_aux_pass (src/renderer.rs:597-645): per pixel one ray through the
-// pixel centre (its own stream, seeded i ^ 0x9E3779B97F4A7C15, for the lens
-// and time draws); a hit gives the material's albedo hint clamped to [0, 1]
-// and the hit normal, a miss the background colour and a zero normal.
-__global__ __launch_bounds__(BLOCK) void k_aux(DScene S, DCamera cam, uint32_t w, uint32_t h, float* __restrict__ alb,
-                                               float* __restrict__ nrm) {
-    Stack st; init_stack(st);
-    uint32_t i = blockIdx.x * BLOCK + threadIdx.x;
-    if (i >= w * h) return;
-    uint32_t row = i / w, col = i - row * w;
-    Rng rng = rng_seed((uint64_t)i ^ 0x9E3779B97F4A7C15ull);
-    float u = ((float)col + 0.5f) / (float)max(w - 1, 1u);
-    float v = ((float)(h - 1 - row) + 0.5f) / (float)max(h - 1, 1u);
-    Ray r = camera_ray(cam, u, v, rng);
-    HitId id = trace_closest(S, r, 0.001f, FINF, st);
-    V3 a, n;
-    if (id.slot < 0) {
-        a = bg_eval(S, r.d);
-        n = mk(0, 0, 0);
-    } else {
-        Rec hr = finalize(S, r, id, 0.001f);
-        V3 c = mat_albedo_hint(S, hr);
-        a = mk(fclamp(c.x, 0.0f, 1.0f), fclamp(c.y, 0.0f, 1.0f), fclamp(c.z, 0.0f, 1.0f));
-        n = hr.n;
-    }
-    alb[3 * (size_t)i] = a.x; alb[3 * (size_t)i + 1] = a.y; alb[3 * (size_t)i + 2] = a.z;
-    nrm[3 * (size_t)i] = n.x; nrm[3 * (size_t)i + 1] = n.y; nrm[3 * (size_t)i + 2] = n.z;
-}
-
-// Detached-sampling forward render: the k_render frame with BSDF values
-// evaluated on the materials `me` while every decision follows the scene's.
-__global__ __launch_bounds__(BLOCK) void k_detached(DScene S, DCamera cam, DParams P, const DMaterial* __restrict__ me,
-                                                    const MedPar* __restrict__ mp, int nmp, float* __restrict__ out, uint32_t spp) {
-    Stack st; init_stack(st);
-    uint32_t i = blockIdx.x * BLOCK + threadIdx.x;
-    uint32_t npix = P.width * P.height;
-    if (i >= npix) return;
-    uint32_t row = i / P.width, col = i - row * P.width;
-    V3 sum = mk(0, 0, 0);
-    for (uint32_t k = 0; k < spp; k++) {
-        uint32_t sidx = P.sample_idx + k;
-        Rng rng = rng_seed(pixel_seed(i, sidx, P.seed));
-        Ray r = pixel_ray(cam, P, row, col, sidx, rng);
-        DetachedHook dh;
-        dh.me = me; dh.mp = mp; dh.nmp = nmp; dh.ratio = mk(1, 1, 1);
-        sum += ray_color_t(S, r, (int)P.max_depth, P.bg_scale, P.use_photons != 0, rng, st, dh);
-    }
-    out[3 * i] += sum.x; out[3 * i + 1] += sum.y; out[3 * i + 2] += sum.z;
-}
-
-// Path-replay adjoint: per sample, the radiance L from its stream, then the
-// same path again distributing L over the vertices' dlog f (see
-// dev_adjoint.h).  Gradients gather in LDS per block, then one global atomic
-// per parameter and block.
-#define RT_MAX_PARAMS 64
-__global__ __launch_bounds__(BLOCK) void k_backward(DScene S, DCamera cam, DParams P, const int8_t* __restrict__ fmap,
-                                                    const MedPar* __restrict__ mp, int nmp,
-                                                    uint32_t nparams, const float* __restrict__ dimg, float inv_spp,
-                                                    uint32_t spp, float* __restrict__ grad) {
-    __shared__ float g[RT_MAX_PARAMS];
-    for (uint32_t t = threadIdx.x; t < nparams; t += BLOCK) g[t] = 0.0f;
-    __syncthreads();
-    Stack st; init_stack(st);
-    uint32_t i = blockIdx.x * BLOCK + threadIdx.x;
-    uint32_t npix = P.width * P.height;
-    if (i < npix) {
-        uint32_t row = i / P.width, col = i - row * P.width;
-        V3 delta = mk(dimg[3 * i], dimg[3 * i + 1], dimg[3 * i + 2]) * inv_spp;
-        if (delta.x != 0.0f || delta.y != 0.0f || delta.z != 0.0f) {
-            for (uint32_t k = 0; k < spp; k++) {
-                uint32_t sidx = P.sample_idx + k;
-                uint64_t seed = pixel_seed(i, sidx, P.seed);
-                Rng rng = rng_seed(seed);
-                Ray r = pixel_ray(cam, P, row, col, sidx, rng);
-                PlainHook ph;
-                V3 L = ray_color_t(S, r, (int)P.max_depth, P.bg_scale, P.use_photons != 0, rng, st, ph);
-                if (!isfinite(L.x) || !isfinite(L.y) || !isfinite(L.z)) continue;
-                rng = rng_seed(seed);
-                r = pixel_ray(cam, P, row, col, sidx, rng);
-                ReplayHook rh;
-                rh.fmap = fmap; rh.mp = mp; rh.nmp = nmp; rh.g = g; rh.delta = delta; rh.rem = L;
-                ray_color_t(S, r, (int)P.max_depth, P.bg_scale, P.use_photons != 0, rng, st, rh);
-            }
-        }
-    }
-    __syncthreads();
-    for (uint32_t t = threadIdx.x; t < nparams; t += BLOCK)
-        if (g[t] != 0.0f) atomicAdd(grad + t, g[t]);
-}
-
-// E_ss table (src/material.rs:206-243): one thread per (mu, alpha) texel.
-__global__ void k_elut(float* out) {
-    uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= 64 * 64) return;
-    uint32_t i = t / 64, j = t % 64;
-    float mu = ((float)i + 0.5f) / 64.0f;
-    float alpha = fmaxf(((float)j + 0.5f) / 64.0f, 1e-3f);
-    Rng rng = rng_seed((uint64_t)(i * 64 + j) + 0x9E3779B9ULL);
-    V3 wo = mk(0.0f, 0.0f, mu);
-    float lo = smith_lambda(mu, alpha);
-    double sum = 0.0;
-    for (int k = 0; k < 64; k++)
-        for (int l = 0; l < 64; l++) {
-            float xi1 = ((float)k + rng_f(rng)) / 64.0f;
-            float xi2 = ((float)l + rng_f(rng)) / 64.0f;
-            V3 m = vndf_sample(wo, alpha, alpha, xi1, xi2);
-            float voh = fmaxf(dot(wo, m), 0.0f);
-            V3 wi = 2.0f * voh * m - wo;
-            if (wi.z <= 0.0f) continue;
-            float li = smith_lambda(wi.z, alpha);
-            sum += (double)((1.0f + lo) / (1.0f + lo + li));
-        }
-    out[t] = (float)(sum / 4096.0);
-}
-
-// Photon emission + tracing (PhotonMap::build_from_quad / build, src/photon.rs:41-135, 232-277).
-struct DEmitter {
-    uint32_t kind;
-    float o[3], u[3], v[3], n[3], ta[3], ba[3];
-    float power[3];
-    float lum_clamp;
-    float disk_c[3], disk_r, t[3], b[3], sun_down[3];
-};
-
-__global__ __launch_bounds__(BLOCK) void k_photon_trace(DScene S, DEmitter E, uint32_t n, float4* __restrict__ out, uint32_t* __restrict__ ok) {
-    Stack st; init_stack(st);
-    uint32_t i = blockIdx.x * BLOCK + threadIdx.x;
-    if (i >= n) return;
-    Rng rng = rng_seed(((uint64_t)i * 6364136223846793005ULL) ^ 0x9E3779B97F4A7C15ULL);
-    V3 pos, dir;
-    if (E.kind == 0) {
-        float s = rng_f(rng), t = rng_f(rng);
-        pos = mk(E.o[0], E.o[1], E.o[2]) + mk(E.u[0], E.u[1], E.u[2]) * s + mk(E.v[0], E.v[1], E.v[2]) * t;
-        float ct = sqrtf(rng_f(rng));
-        float sn = sqrtf(1.0f - ct * ct);
-        float phi = 2.0f * PI * rng_f(rng);
-        dir = sn * cosf(phi) * mk(E.ta[0], E.ta[1], E.ta[2]) + sn * sinf(phi) * mk(E.ba[0], E.ba[1], E.ba[2]) + ct * mk(E.n[0], E.n[1], E.n[2]);
-    } else {
-        float r = E.disk_r * sqrtf(rng_f(rng));
-        float phi = 2.0f * PI * rng_f(rng);
-        pos = mk(E.disk_c[0], E.disk_c[1], E.disk_c[2]) + mk(E.t[0], E.t[1], E.t[2]) * (r * cosf(phi)) + mk(E.b[0], E.b[1], E.b[2]) * (r * sinf(phi));
-        dir = mk(E.sun_down[0], E.sun_down[1], E.sun_down[2]);
-    }
-    V3 power = mk(E.power[0], E.power[1], E.power[2]);
-    uint32_t spec_depth = 0;
-    float wl = rng_range(rng, 380.0f, 700.0f);
-    uint32_t stored = 0;
-    for (int d = 0; d < 12; d++) {
-        Ray ray = make_ray(pos, dir, 0.0f);
-        ray.wavelength = wl;
-        HitId id = trace_closest(S, ray, 0.001f, FINF, st);
-        if (id.slot < 0) break;
-        Rec h = finalize(S, ray, id, 0.001f);
-        Scatter sr;
-        if (!mat_scatter(S, h, ray, rng, sr)) break;
-        if (sr.skip_pdf) {
-            power *= sr.att;
-            pos = sr.ray.o; dir = sr.ray.d;
-            spec_depth++;
-        } else {
-            if (spec_depth > 0 && mat_receives_caustics(S, h)) {
-                float l = lum(power);
-                if (l > E.lum_clamp) power *= E.lum_clamp / l;
-                out[3 * (size_t)i] = make_float4(h.p.x, h.p.y, h.p.z, 0.0f);
-                out[3 * (size_t)i + 1] = make_float4(power.x, power.y, power.z, 0.0f);
-                out[3 * (size_t)i + 2] = make_float4(h.n.x, h.n.y, h.n.z, 0.0f);
-                stored = 1;
-            }
-            break;
-        }
-    }
-    ok[i] = stored;
-}
-
-template <bool KNN>
-__global__ __launch_bounds__(BLOCK) void k_photon_irr(DScene S, const float* __restrict__ pn, uint32_t n, float* __restrict__ out) {
-    uint32_t i = blockIdx.x * BLOCK + threadIdx.x;
-    if (i >= n) return;
-    Stack st;
-    st.heap = KNN ? LdsHeap::column() : nullptr; st.stride = BLOCK;
-    V3 c = photon_irradiance(S, mk(pn[6 * i], pn[6 * i + 1], pn[6 * i + 2]), mk(pn[6 * i + 3], pn[6 * i + 4], pn[6 * i + 5]), mk(0, 0, 0), st);
-    out[3 * i] = c.x; out[3 * i + 1] = c.y; out[3 * i + 2] = c.z;
-}
-
-// ============================================================================ wavefront
-// Path records live in HBM (128 B each, one cache line); queues hold path ids.
-// Per bounce: k_wf_extend traces every active path and files its id under the
-// material key of the hit (the sort), k_wf_shade runs one bounce in key order
-// and appends survivors and shadow requests, k_wf_shadow resolves the shadow
-// requests.  All three are persistent grid-stride kernels that read their
-// queue length from device memory, so the host never synchronises inside a
-// frame.  Contributions go straight into the frame accumulator with float
-// atomics.
-#define WF_NB (M_KINDS + 2)   // sort keys (sort 1): 0 miss, 1..M_KINDS material kind, M_KINDS + 1 medium event;
-                          // (sort 2): 0 miss / specular / medium, 1 every other material
-#define WF_CNT_ACTIVE 0   // counters[0]: paths to extend this bounce
-#define WF_CNT_NEXT 1     // counters[1]: survivors written by shade
-#define WF_CNT_SHADOW 2   // counters[2]: shadow requests
-#define WF_CNT_DENSE 3    // counters[3]: k-NN queries holding at least k photons (second k-NN pass)
-#define WF_CNT_BUCKET 8   // counters[8..8+WF_NB): bucket sizes
-#define WF_CNT_FIRST 32   // counters[32..33]: 64-bit frame position of a graph-replayed chunk
-#define WF_CNT_GEN 34     // counters[34]: paths generated when flagged pixels are skipped
-
-struct __align__(16) WfPath {
-    float4 o_time;       // ray origin, time
-    float4 d_wl;         // ray direction, wavelength
-    float4 thr_mis;      // throughput, prev_mis
-    float sun_w; uint32_t flags; uint32_t pixel; uint32_t pad0;   // flags: 1 spectral, 2 prev_specular
-    uint64_t rng[4];
-    float hit_t; int32_t hit_slot; int32_t hit_prim; uint32_t key;
-    uint32_t sid;        // sample index within the chunk (per-sample radiance of the adjoint's primal pass)
-    uint32_t pad1[3];
-};
-static_assert(sizeof(WfPath) == 128, "path record is one 128-B line");
-
-struct __align__(16) WfShadow { float4 o_tmax; float4 d_time; float4 c_pixel; uint4 meta; };   // meta.x = view
-
-struct WfQueues {
-    WfPath* paths;
-    uint32_t cap;
-    uint32_t* active;     // cap
-    uint32_t* next;       // cap
-    uint32_t* buckets;    // WF_NB * cap
-    WfShadow* shadow;     // 3 * cap
-    uint32_t* counters;   // 64: 32 per-bounce counters, then the chunk position
-    float* fb;
-    float* lsamp;         // non-null: contributions go to the path's sample (3 floats per sid), not to fb
-    uint32_t sort;        // 0: one bucket; 1: a bucket per material kind; 2: two shade classes
-    uint32_t* dense;      // cap: paths whose k-NN query needs the heap search
-    const uint32_t* stop; // non-null and nonzero at chunk start: the chunk renders no path (a stopped render loop)
-    const uint8_t* conv;  // non-null: flagged pixels get no path (the render loop's converged pixels)
-};
-
-__device__ __forceinline__ uint32_t lane_id() { return __lane_id(); }
-
-// Wave-aggregated append: one atomic per distinct key in the wave.
-__device__ __forceinline__ uint32_t wave_append(uint32_t* counters, uint32_t key, bool valid) {
-    uint32_t slot = 0;
-    uint32_t lane = lane_id();
-    unsigned long long pending = __ballot(valid);
-    while (pending) {
-        int leader = __ffsll(pending) - 1;
-        uint32_t k = __shfl(key, leader);
-        unsigned long long m = __ballot(valid && key == k);
-        uint32_t base = 0;
-        if ((int)lane == leader) base = atomicAdd(&counters[k], (uint32_t)__popcll(m));
-        base = __shfl(base, leader);
-        if (valid && key == k) slot = base + (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
-        pending &= ~m;
-    }
-    return slot;
-}
-
-__device__ __forceinline__ void wf_load(const WfPath& w, PathState& ps, Rng& rng) {
-    ps.ray = make_ray(mk(w.o_time.x, w.o_time.y, w.o_time.z), mk(w.d_wl.x, w.d_wl.y, w.d_wl.z), w.o_time.w);
-    ps.ray.wavelength = w.d_wl.w;
-    ps.ray.spectral = (w.flags & 1u) != 0;
-    ps.thr = mk(w.thr_mis.x, w.thr_mis.y, w.thr_mis.z);
-    ps.prev_mis = w.thr_mis.w;
-    ps.prev_sun_w = w.sun_w;
-    ps.prev_specular = (w.flags & 2u) != 0;
-    ps.knn_irr = mk(__uint_as_float(w.pad1[0]), __uint_as_float(w.pad1[1]), __uint_as_float(w.pad1[2]));
-    rng.s0 = w.rng[0]; rng.s1 = w.rng[1]; rng.s2 = w.rng[2]; rng.s3 = w.rng[3];
-}
-__device__ __forceinline__ void wf_store(WfPath& w, const PathState& ps, const Rng& rng) {
-    w.o_time = make_float4(ps.ray.o.x, ps.ray.o.y, ps.ray.o.z, ps.ray.time);
-    w.d_wl = make_float4(ps.ray.d.x, ps.ray.d.y, ps.ray.d.z, ps.ray.wavelength);
-    w.thr_mis = make_float4(ps.thr.x, ps.thr.y, ps.thr.z, ps.prev_mis);
-    w.sun_w = ps.prev_sun_w;
-    w.flags = (ps.ray.spectral ? 1u : 0u) | (ps.prev_specular ? 2u : 0u);
-    w.rng[0] = rng.s0; w.rng[1] = rng.s1; w.rng[2] = rng.s2; w.rng[3] = rng.s3;
-}
-
-// Zero the per-bounce counters (a kernel, not a memset, so graph replays are safe).
-__global__ void k_wf_reset(uint32_t* counters, uint32_t active, const uint32_t* stop, int generated) {
-    uint32_t i = threadIdx.x;
-    if (generated) active = counters[WF_CNT_GEN];
-    if (stop && *stop) active = 0;
-    if (i < 32) counters[i] = (i == WF_CNT_ACTIVE) ? active : 0u;
-}
-__global__ void k_wf_gen_zero(uint32_t* counters) {
-    if (threadIdx.x == 0) counters[WF_CNT_GEN] = 0u;
-}
-// After shade: survivors become the active set, everything else restarts at 0.
-__global__ void k_wf_advance(uint32_t* counters) {
-    uint32_t i = threadIdx.x;
-    __shared__ uint32_t nxt;
-    if (i == 0) nxt = counters[WF_CNT_NEXT];
-    __syncthreads();
-    if (i < 32) counters[i] = (i == WF_CNT_ACTIVE) ? nxt : 0u;
-}
-
-// counters[WF_CNT_FIRST..+1] hold the frame position of a graph-replayed
-// chunk; they lie past the 32 per-bounce counters that reset/advance clear.
-__global__ void k_wf_set_first(uint32_t* counters, uint64_t first) {
-    if (threadIdx.x == 0) { counters[WF_CNT_FIRST] = (uint32_t)first; counters[WF_CNT_FIRST + 1] = (uint32_t)(first >> 32); }
-}
-__global__ void k_wf_bump_first(uint32_t* counters, uint32_t n) {
-    if (threadIdx.x == 0) {
-        uint64_t f = ((uint64_t)counters[WF_CNT_FIRST + 1] << 32) | counters[WF_CNT_FIRST];
-        f += n;
-        counters[WF_CNT_FIRST] = (uint32_t)f; counters[WF_CNT_FIRST + 1] = (uint32_t)(f >> 32);
-    }
-}
-
-// One render of a batch: its camera and pass parameters, its slice of the
-// global path index space and of the concatenated frame accumulator.
-struct DView {
-    DCamera cam;
-    DParams P;
-    uint64_t first, count;    // global path range [first, first + count)
-    uint32_t fb_off, npix;
-};
-struct WfBatch {
-    const DScene* scenes;     // per view
-    const DView* views;
-    uint32_t nviews;
-};
-
-// Path j of a sample pass covers pixel tile_order_pixel(j): bands of 8 rows,
-// each walked in 8-column groups, row-major inside a group, so a wave's 64
-// camera rays leave an 8 x 8 block of the image (coherent traversal and
-// photon lookups) instead of a 64-pixel row.  A bijection of [0, W * H); the
-// pixel's random stream depends on the pixel, not on j.
-__device__ __forceinline__ uint32_t tile_order_pixel(uint32_t j, uint32_t W, uint32_t H) {
-    uint32_t band = j / (8u * W);
-    uint32_t hb = min(8u, H - 8u * band);
-    uint32_t k = j - band * 8u * W;
-    uint32_t grp = k / (8u * hb);
-    uint32_t gw = min(8u, W - 8u * grp);
-    uint32_t local = k - grp * 8u * hb;
-    uint32_t row = local / gw;
-    return (band * 8u + row) * W + grp * 8u + (local - row * gw);
-}
-
-template <bool BATCH>
-__global__ __launch_bounds__(BLOCK) void k_wf_generate(DCamera cam0, DParams P0, WfQueues Q, uint64_t first, uint32_t n, WfBatch B) {
-    uint32_t i = blockIdx.x * BLOCK + threadIdx.x;
-    if (i >= n) return;
-    if (first == ~0ull) first = ((uint64_t)Q.counters[WF_CNT_FIRST + 1] << 32) | Q.counters[WF_CNT_FIRST];
-    uint64_t g = first + i;
-    uint32_t view = 0, fb_off = 0;
-    DCamera cam = cam0;
-    DParams P = P0;
-    if (BATCH) {
-        uint32_t lo = 0, hi = B.nviews;            // last view with views[v].first <= g
-        while (hi - lo > 1) { uint32_t mid = (lo + hi) / 2; if (B.views[mid].first <= g) lo = mid; else hi = mid; }
-        view = lo;
-        const DView& V = B.views[view];
-        cam = V.cam; P = V.P; fb_off = V.fb_off;
-        g -= V.first;
-    }
-    uint32_t npix = P.width * P.height;
-    uint32_t pixel = tile_order_pixel((uint32_t)(g % npix), P.width, P.height);
-    uint32_t sidx = P.sample_idx + (uint32_t)(g / npix);
-    uint32_t row = pixel / P.width, col = pixel - row * P.width;
-    Rng rng = rng_seed(pixel_seed(pixel, sidx, P.seed));
-    Ray r = pixel_ray(cam, P, row, col, sidx, rng);
-    PathState ps;
-    path_init(ps, r);
-    WfPath w;
-    wf_store(w, ps, rng);
-    w.pixel = fb_off + pixel; w.pad0 = view; w.hit_t = 0.0f; w.hit_slot = -1; w.hit_prim = -1; w.key = 0; w.sid = i;
-    Q.paths[i] = w;
-    if (Q.conv) {                         // compact: only unflagged pixels enter the active set
-        bool live = Q.conv[pixel] == 0;
-        uint32_t slot = wave_append(Q.counters + WF_CNT_GEN, 0u, live);
-        if (live) Q.active[slot] = i;
-    } else {
-        Q.active[i] = i;
-    }
-}
-
-// Bucket of a hit: per material kind (sort 1) or per shade class (sort 2:
-// misses, media and the specular materials share k_wf_shade<..., 1>).
-__device__ __forceinline__ uint32_t sort_key(const DScene& S, const HitId& id, uint32_t sort) {
-    if (sort == 0) return 0u;
-    if (id.slot < 0) return 0u;
-    if (id.prim == -2) return sort == 1 ? (uint32_t)M_KINDS + 1u : 0u;
-    uint32_t k = S.mats[S.objs[id.slot].material].kind;
-    if (sort == 1) return k <= (uint32_t)M_KINDS ? k : 0u;
-    return RT_IN(RT_MASK_SPECULAR, k) ? 0u : 1u;
-}
-
-__device__ __forceinline__ uint32_t extend_one(const DScene& S, WfPath& w, uint32_t sort, Stack& st) {
-    float4 o = w.o_time, d = w.d_wl;
-    Ray r = make_ray(mk(o.x, o.y, o.z), mk(d.x, d.y, d.z), o.w);
-    r.wavelength = d.w; r.spectral = (w.flags & 1u) != 0;   // media seed their free-flight stream from the ray, wavelength included
-    HitId id = trace_closest(S, r, 0.001f, FINF, st);
-    w.hit_t = id.t; w.hit_slot = id.slot; w.hit_prim = id.prim;
-    return sort_key(S, id, sort);
-}
-
-// Whether the path's new hit takes a k-NN photon estimate in shade: a surface
-// (not a medium event) whose material can take the diffuse branch.
-__device__ __forceinline__ bool knn_gathers(const DScene& S, const WfPath& w) {
-    if (S.ph_knn == 0 || S.num_photons == 0 || w.hit_slot < 0 || w.hit_prim == -2) return false;
-    uint32_t kind = S.mats[S.objs[w.hit_slot].material].kind;
-    return kind == M_LAMBERTIAN || kind == M_PBR || kind == M_PEARL;
-}
-
-// OCC = minimum waves per SIMD the register allocator must allow (0: its own choice).
-template <int OCC, bool BATCH>
-__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(OCC > 0 ? OCC : 1, 8))) void k_wf_extend(DScene S, WfQueues Q, WfBatch B) {
-    Stack st; init_stack(st);
-    uint32_t count = Q.counters[WF_CNT_ACTIVE];
-    uint32_t stride = gridDim.x * BLOCK;
-    for (uint32_t base = blockIdx.x * BLOCK; base < count; base += stride) {
-        uint32_t i = base + threadIdx.x;
-        bool valid = i < count;
-        uint32_t pid = 0, key = 0;
-        if (valid) {
-            pid = Q.active[i];
-            WfPath& w = Q.paths[pid];
-            if (BATCH) key = extend_one(B.scenes[w.pad0], w, Q.sort, st);
-            else key = extend_one(S, w, Q.sort, st);
-        }
-        uint32_t slot = wave_append(Q.counters + WF_CNT_BUCKET, key, valid);
-        if (valid) Q.buckets[(size_t)key * Q.cap + slot] = pid;
-    }
-}
-
-template <uint32_t MASK>
-__device__ __forceinline__ bool shade_one(const DScene& S, const DParams& P, WfQueues& Q, WfPath& w, int depth,
-                                          ShadowReq* sh, int& nsh, Stack& st) {
-    if (depth >= (int)P.max_depth) return false;
-    PathState ps; Rng rng;
-    wf_load(w, ps, rng);
-    HitId id; id.t = w.hit_t; id.slot = w.hit_slot; id.prim = w.hit_prim;
-    V3 color = mk(0, 0, 0);
-    bool cont = bounce<MASK>(S, ps, id, depth, P.bg_scale, P.use_photons != 0, rng, color, sh, nsh, st) && depth + 1 < (int)P.max_depth;
-    if (color.x != 0.0f || color.y != 0.0f || color.z != 0.0f) {
-        float* dst = Q.lsamp ? Q.lsamp + 3 * (size_t)w.sid : Q.fb + 3 * (size_t)w.pixel;
-        atomicAdd(dst, color.x);
-        atomicAdd(dst + 1, color.y);
-        atomicAdd(dst + 2, color.z);
-    }
-    if (cont) wf_store(w, ps, rng);
-    return cont;
-}
-
-// CLASS 0: every bucket with every material; CLASS 1: bucket 0 of the class
-// split (misses, media, specular materials); CLASS 2: bucket 1 (the rest).
-// A class kernel compiles only its materials, so it needs fewer registers.
-template <bool BATCH, int CLASS>
-__global__ __launch_bounds__(BLOCK) void k_wf_shade(DScene S, DParams P, WfQueues Q, int depth, WfBatch B) {
-    constexpr uint32_t MASK = CLASS == 1 ? RT_MASK_SPECULAR : CLASS == 2 ? (RT_MASK_ALL & ~RT_MASK_SPECULAR) : RT_MASK_ALL;
-    constexpr int B0 = CLASS == 2 ? 1 : 0;
-    constexpr int NB = CLASS == 0 ? WF_NB : 1;
-    Stack st; init_stack(st);
-    uint32_t off[NB + 1];
-    off[0] = 0;
-    for (int b = 0; b < NB; b++) off[b + 1] = off[b] + Q.counters[WF_CNT_BUCKET + B0 + b];
-    uint32_t total = off[NB];
-    uint32_t stride = gridDim.x * BLOCK;
-    for (uint32_t base = blockIdx.x * BLOCK; base < total; base += stride) {
-        uint32_t i = base + threadIdx.x;
-        bool valid = i < total;
-        bool cont = false;
-        int nsh = 0;
-        ShadowReq sh[3];
-        uint32_t pid = 0, pixel = 0, view = 0, sid = 0;
-        if (valid) {
-            int b = 0;
-            while (i >= off[b + 1]) b++;
-            pid = Q.buckets[(size_t)(B0 + b) * Q.cap + (i - off[b])];
-            WfPath& w = Q.paths[pid];
-            pixel = w.pixel;
-            view = w.pad0;
-            sid = w.sid;
-            if (BATCH) cont = shade_one<MASK>(B.scenes[view], B.views[view].P, Q, w, depth, sh, nsh, st);
-            else cont = shade_one<MASK>(S, P, Q, w, depth, sh, nsh, st);
-        }
-        uint32_t nslot = wave_append(Q.counters + WF_CNT_NEXT, 0u, cont);
-        if (cont) Q.next[nslot] = pid;
-        for (int k = 0; k < 3; k++) {
-            bool has = valid && k < nsh;
-            uint32_t s = wave_append(Q.counters + WF_CNT_SHADOW, 0u, has);
-            if (has) {
-                WfShadow q;
-                q.o_tmax = make_float4(sh[k].o.x, sh[k].o.y, sh[k].o.z, sh[k].t_max);
-                q.d_time = make_float4(sh[k].d.x, sh[k].d.y, sh[k].d.z, sh[k].time);
-                q.c_pixel = make_float4(sh[k].contrib.x, sh[k].contrib.y, sh[k].contrib.z, __uint_as_float(pixel));
-                q.meta = make_uint4(view, sid, 0u, 0u);
-                Q.shadow[s] = q;
-            }
-        }
-    }
-}
-
-// k-NN photon maps: between extend and shade, the photon irradiance of every
-// hit whose material can take the diffuse branch (Lambertian, PBR, Pearl),
-// into the path record, so the shade kernel keeps its occupancy.  Two
-// passes: k_wf_knn (no LDS, full occupancy) finishes every query with fewer
-// than k photons within the gather radius and queues the rest; k_wf_knn_dense
-// runs the heap search (32 KB of LDS per block) on those only, so its lanes
-// all do the same kind of work.  Paths whose hit does not gather are left as
-// they are.
-template <bool BATCH>
-__device__ __forceinline__ bool knn_query(const DScene& S, const WfPath& w, V3& pos, V3& nrm) {
-    Ray r = make_ray(mk(w.o_time.x, w.o_time.y, w.o_time.z), mk(w.d_wl.x, w.d_wl.y, w.d_wl.z), w.o_time.w);
-    r.wavelength = w.d_wl.w; r.spectral = (w.flags & 1u) != 0;
-    HitId id; id.t = w.hit_t; id.slot = w.hit_slot; id.prim = w.hit_prim;
-    Rec h = finalize(S, r, id, 0.001f);
-    pos = h.p; nrm = h.n;
-    return true;
-}
-__device__ __forceinline__ void knn_store(WfPath& w, V3 e) {
-    w.pad1[0] = __float_as_uint(e.x); w.pad1[1] = __float_as_uint(e.y); w.pad1[2] = __float_as_uint(e.z);
-}
-template <bool BATCH>
-__global__ __launch_bounds__(BLOCK) void k_wf_knn(DScene S0, DParams P0, WfQueues Q, WfBatch B) {
-    uint32_t total = 0;
-    for (int b = 0; b < WF_NB; b++) total += Q.counters[WF_CNT_BUCKET + b];
-    uint32_t stride = gridDim.x * BLOCK;
-    for (uint32_t base0 = blockIdx.x * BLOCK; base0 < total; base0 += stride) {
-        uint32_t i = base0 + threadIdx.x;
-        bool dense = false;
-        uint32_t pid = 0;
-        if (i < total) {
-            uint32_t b = 0, base = 0;
-            while (i >= base + Q.counters[WF_CNT_BUCKET + b]) { base += Q.counters[WF_CNT_BUCKET + b]; b++; }
-            pid = Q.buckets[(size_t)b * Q.cap + (i - base)];
-            WfPath& w = Q.paths[pid];
-            const DScene& S = BATCH ? B.scenes[w.pad0] : S0;
-            bool photons = BATCH ? B.views[w.pad0].P.use_photons != 0 : P0.use_photons != 0;
-            if (photons && knn_gathers(S, w)) {
-                V3 pos, nrm, e;
-                knn_query<BATCH>(S, w, pos, nrm);
-                if (photon_knn_sparse(S, pos, nrm, e)) knn_store(w, e);
-                else dense = true;
-            }
-        }
-        uint32_t slot = wave_append(Q.counters + WF_CNT_DENSE, 0u, dense);
-        if (dense) Q.dense[slot] = pid;
-    }
-}
-template <bool BATCH>
-__global__ __launch_bounds__(BLOCK) void k_wf_knn_dense(DScene S0, WfQueues Q, WfBatch B) {
-    float* heap = LdsHeap::column();
-    uint32_t total = Q.counters[WF_CNT_DENSE];
-    uint32_t stride = gridDim.x * BLOCK;
-    for (uint32_t i = blockIdx.x * BLOCK + threadIdx.x; i < total; i += stride) {
-        WfPath& w = Q.paths[Q.dense[i]];
-        const DScene& S = BATCH ? B.scenes[w.pad0] : S0;
-        V3 pos, nrm;
-        knn_query<BATCH>(S, w, pos, nrm);
-        knn_store(w, photon_knn_dense(S, pos, nrm, heap, BLOCK));
-    }
-}
-
-// The wavefront render loop's bookkeeping: K passes of per-sample radiance
-// (lsamp, sample id k * npix + j for path j of pass k, which covered pixel
-// tile_order_pixel(j)) into the pixel state, pass by pass per pixel as
-// accumulate_sample does, with mark_converged after each pass when adaptive
-// (K is then 1, so the stop test runs between passes).
-__global__ __launch_bounds__(BLOCK) void k_accum_passes(const float* __restrict__ lsamp, uint32_t K, uint32_t w, uint32_t h,
-                                                        uint32_t pass0, rt_accum_opts o, float* acc, uint32_t* ns, float* m2,
-                                                        uint8_t* conv, uint32_t* ctl) {
-    if (ctl[0]) return;
-    uint32_t npix = w * h;
-    uint32_t local = 0;
-    for (uint32_t j = blockIdx.x * BLOCK + threadIdx.x; j < npix; j += gridDim.x * BLOCK) {
-        uint32_t px = tile_order_pixel(j, w, h);
-        float* a = acc + 3 * (size_t)px;
-        float* v = m2 + 3 * (size_t)px;
-        for (uint32_t k = 0; k < K && !conv[px]; k++) {
-            const float* c = lsamp + 3 * ((size_t)k * npix + j);
-            accumulate_pixel(a, ns + px, v, mk(c[0], c[1], c[2]), o);
-            if (o.adaptive && pass0 + k + 1 >= o.min_samples && pixel_converged(a, ns[px], v, o.threshold)) {
-                conv[px] = 1;
-                local++;
-            }
-        }
-    }
-    wave_count(local, ctl + 1);
-}
-
-template <int OCC, bool BATCH>
-__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(OCC > 0 ? OCC : 1, 8))) void k_wf_shadow(DScene S, WfQueues Q, WfBatch B) {
-    Stack st; init_stack(st);
-    uint32_t count = Q.counters[WF_CNT_SHADOW];
-    uint32_t stride = gridDim.x * BLOCK;
-    for (uint32_t i = blockIdx.x * BLOCK + threadIdx.x; i < count; i += stride) {
-        WfShadow q = Q.shadow[i];
-        Ray r = make_ray(mk(q.o_tmax.x, q.o_tmax.y, q.o_tmax.z), mk(q.d_time.x, q.d_time.y, q.d_time.z), q.d_time.w);
-        float tr = BATCH ? trace_transmittance(B.scenes[q.meta.x], r, 0.001f, q.o_tmax.w, st, false)
-                         : trace_transmittance(S, r, 0.001f, q.o_tmax.w, st, false);
-        if (tr > 0.0f) {
-            float* dst = Q.lsamp ? Q.lsamp + 3 * (size_t)q.meta.y : Q.fb + 3 * (size_t)__float_as_uint(q.c_pixel.w);
-            atomicAdd(dst, q.c_pixel.x * tr);
-            atomicAdd(dst + 1, q.c_pixel.y * tr);
-            atomicAdd(dst + 2, q.c_pixel.z * tr);
-        }
-    }
-}
-
-
-// ---- wavefront path replay (adjoint) ---------------------------------------------------------
-// Per path slot, beside its WfPath: the radiance still to be collected, the
-// loss gradient of its sample, the pending score of the last scattered vertex
-// (applied when the next vertex is shaded, after that vertex's shadow rays
-// have been subtracted) and the vertex score of the current vertex's NEE.
-struct __align__(16) WfAdj {
-    float4 rem;                       // xyz
-    float4 delta;                     // xyz
-    int32_t pn, vn;
-    int8_t pp[RT_SCORE_MAX], vp[3];
-    int8_t pad0[7];
-    float pv[3 * RT_SCORE_MAX];
-    float vv[9];
-    float pad1[3];
-};
-static_assert(sizeof(WfAdj) == 176, "WfAdj layout");
-
-struct WfAdjQ {
-    WfAdj* adj;
-    const float* lsamp;               // primal radiance per sample of the chunk
-    const float* dimg;                // dLoss/dI per pixel
-    const int8_t* fmap;
-    const MedPar* mp;                 // parameterised media
-    int nmp;
-    float inv_spp;
-    uint32_t nparams;
-    float* grad;
-};
-
-__global__ __launch_bounds__(BLOCK) void k_wfb_init(WfQueues Q, WfAdjQ A, uint32_t n) {
-    uint32_t i = blockIdx.x * BLOCK + threadIdx.x;
-    if (i >= n) return;
-    const WfPath& w = Q.paths[i];
-    V3 L = mk(A.lsamp[3 * i], A.lsamp[3 * i + 1], A.lsamp[3 * i + 2]);
-    bool ok = isfinite(L.x) && isfinite(L.y) && isfinite(L.z);
-    size_t px = 3 * (size_t)w.pixel;
-    WfAdj a;
-    a.rem = ok ? make_float4(L.x, L.y, L.z, 0.0f) : make_float4(0, 0, 0, 0);
-    a.delta = ok ? make_float4(A.dimg[px] * A.inv_spp, A.dimg[px + 1] * A.inv_spp, A.dimg[px + 2] * A.inv_spp, 0.0f)
-                 : make_float4(0, 0, 0, 0);
-    a.pn = 0; a.vn = 0;
-    A.adj[i] = a;
-}
-
-struct WfReplayHook {
-    const int8_t* fmap;
-    const MedPar* mp;
-    int nmp;
-    float* g;
-    V3 delta, rem;
-    Score pend;
-    bool pend_ok;
-    Score vs;
-    bool vs_ok;
-    bool vdep[3];
-    RT_D void add(const Score& sc, V3 c) {
-        for (int k = 0; k < sc.n; k++) {
-            float v = delta.x * c.x * sc.v[k].x + delta.y * c.y * sc.v[k].y + delta.z * c.z * sc.v[k].z;
-            if (v != 0.0f && isfinite(v)) atomicAdd(g + sc.p[k], v);
-        }
-    }
-    // the radiance still to come travelled this segment (medium densities)
-    RT_D void segment(const DScene& S, const Ray& r, const HitId& id) {
-        for (int k = 0; k < nmp; k++) {
-            float s = -segment_length(S, mp[k], r, id);
-            if (segment_event(mp[k], id)) s += 1.0f / mp[k].ds;
-            float v = (delta.x * rem.x + delta.y * rem.y + delta.z * rem.z) * s;
-            if (v != 0.0f && isfinite(v)) atomicAdd(g + mp[k].param, v);
-        }
-    }
-    RT_D V3 emit(V3 c) { rem -= c; return c; }
-    RT_D V3 emit_vertex(V3 c, const DScene& S, const Rec& h, const Ray& r) {
-        rem -= c;
-        Score sc;
-        V3 wo = unit(-r.d);
-        vertex_score(S, fmap, h, wo, wo, r.wavelength, sc);
-        add(sc, c);
-        return c;
-    }
-    RT_D void shadow(ShadowReq& q, int k, const DScene& S, const Rec& h, const Ray& r, bool vd) {
-        vdep[k] = vd;
-        if (vd && !vs_ok) { vertex_score(S, fmap, h, unit(-r.d), unit(q.d), r.wavelength, vs); vs_ok = true; }
-    }
-    RT_D void scatter(const DScene& S, const Rec& h, const Ray& r, V3 d) {
-        vertex_score(S, fmap, h, unit(-r.d), unit(d), r.wavelength, pend);
-        pend_ok = true;
-    }
-};
-
-__global__ __launch_bounds__(BLOCK) void k_wfb_shade(DScene S, DParams P, WfQueues Q, WfAdjQ A, int depth) {
-    __shared__ float g[RT_MAX_PARAMS];
-    for (uint32_t t = threadIdx.x; t < A.nparams; t += BLOCK) g[t] = 0.0f;
-    __syncthreads();
-    Stack st; init_stack(st);
-    uint32_t off[WF_NB + 1];
-    off[0] = 0;
-    for (int b = 0; b < WF_NB; b++) off[b + 1] = off[b] + Q.counters[WF_CNT_BUCKET + b];
-    uint32_t total = off[WF_NB];
-    uint32_t stride = gridDim.x * BLOCK;
-    for (uint32_t base = blockIdx.x * BLOCK; base < total; base += stride) {
-        uint32_t i = base + threadIdx.x;
-        bool valid = i < total;
-        bool cont = false;
-        int nsh = 0;
-        ShadowReq sh[3];
-        bool vdep[3] = {false, false, false};
-        uint32_t pid = 0;
-        if (valid) {
-            int b = 0;
-            while (i >= off[b + 1]) b++;
-            pid = Q.buckets[(size_t)b * Q.cap + (i - off[b])];
-            WfPath& w = Q.paths[pid];
-            WfAdj& a = A.adj[pid];
-            WfReplayHook rh;
-            rh.fmap = A.fmap; rh.mp = A.mp; rh.nmp = A.nmp; rh.g = g;
-            rh.delta = mk(a.delta.x, a.delta.y, a.delta.z);
-            rh.rem = mk(a.rem.x, a.rem.y, a.rem.z);
-            rh.pend_ok = false; rh.vs_ok = false;
-            rh.vdep[0] = rh.vdep[1] = rh.vdep[2] = false;
-            // what is left now arrived through the previous vertex's sampled direction
-            if (a.pn > 0) {
-                Score pv;
-                pv.n = a.pn;
-                for (int k = 0; k < a.pn; k++) { pv.p[k] = a.pp[k]; pv.v[k] = mk(a.pv[3 * k], a.pv[3 * k + 1], a.pv[3 * k + 2]); }
-                rh.add(pv, rh.rem);
-            }
-            if (depth < (int)P.max_depth) {
-                PathState ps; Rng rng;
-                wf_load(w, ps, rng);
-                HitId id; id.t = w.hit_t; id.slot = w.hit_slot; id.prim = w.hit_prim;
-                V3 color = mk(0, 0, 0);
-                cont = bounce_t(S, ps, id, depth, P.bg_scale, P.use_photons != 0, rng, color, sh, nsh, st, rh) &&
-                       depth + 1 < (int)P.max_depth;
-                if (cont) wf_store(w, ps, rng);
-            }
-            a.rem = make_float4(rh.rem.x, rh.rem.y, rh.rem.z, 0.0f);
-            a.pn = (cont && rh.pend_ok) ? rh.pend.n : 0;
-            for (int k = 0; k < a.pn; k++) {
-                a.pp[k] = (int8_t)rh.pend.p[k];
-                a.pv[3 * k] = rh.pend.v[k].x; a.pv[3 * k + 1] = rh.pend.v[k].y; a.pv[3 * k + 2] = rh.pend.v[k].z;
-            }
-            a.vn = rh.vs_ok ? min(rh.vs.n, 3) : 0;
-            for (int k = 0; k < a.vn; k++) {
-                a.vp[k] = (int8_t)rh.vs.p[k];
-                a.vv[3 * k] = rh.vs.v[k].x; a.vv[3 * k + 1] = rh.vs.v[k].y; a.vv[3 * k + 2] = rh.vs.v[k].z;
-            }
-            for (int k = 0; k < 3; k++) vdep[k] = rh.vdep[k];
-        }
-        uint32_t nslot = wave_append(Q.counters + WF_CNT_NEXT, 0u, cont);
-        if (cont) Q.next[nslot] = pid;
-        for (int k = 0; k < 3; k++) {
-            bool has = valid && k < nsh;
-            uint32_t s = wave_append(Q.counters + WF_CNT_SHADOW, 0u, has);
-            if (has) {
-                WfShadow q;
-                q.o_tmax = make_float4(sh[k].o.x, sh[k].o.y, sh[k].o.z, sh[k].t_max);
-                q.d_time = make_float4(sh[k].d.x, sh[k].d.y, sh[k].d.z, sh[k].time);
-                q.c_pixel = make_float4(sh[k].contrib.x, sh[k].contrib.y, sh[k].contrib.z, 0.0f);
-                q.meta = make_uint4(0u, pid, vdep[k] ? 1u : 0u, 0u);
-                Q.shadow[s] = q;
-            }
-        }
-    }
-    __syncthreads();
-    for (uint32_t t = threadIdx.x; t < A.nparams; t += BLOCK)
-        if (g[t] != 0.0f) atomicAdd(A.grad + t, g[t]);
-}
-
-// Shadow rays of the replay: the collected radiance leaves the path's
-// remainder; a vertex-dependent one (its value carries that vertex's BSDF)
-// adds its own score term.
-__global__ __launch_bounds__(BLOCK) void k_wfb_shadow(DScene S, WfQueues Q, WfAdjQ A) {
-    __shared__ float g[RT_MAX_PARAMS];
-    for (uint32_t t = threadIdx.x; t < A.nparams; t += BLOCK) g[t] = 0.0f;
-    __syncthreads();
-    Stack st; init_stack(st);
-    uint32_t count = Q.counters[WF_CNT_SHADOW];
-    uint32_t stride = gridDim.x * BLOCK;
-    for (uint32_t i = blockIdx.x * BLOCK + threadIdx.x; i < count; i += stride) {
-        WfShadow q = Q.shadow[i];
-        Ray r = make_ray(mk(q.o_tmax.x, q.o_tmax.y, q.o_tmax.z), mk(q.d_time.x, q.d_time.y, q.d_time.z), q.d_time.w);
-        float tr = trace_transmittance(S, r, 0.001f, q.o_tmax.w, st, false);
-        if (tr > 0.0f) {
-            WfAdj& a = A.adj[q.meta.y];
-            V3 c = mk(q.c_pixel.x, q.c_pixel.y, q.c_pixel.z) * tr;
-            atomicAdd(&a.rem.x, -c.x);
-            atomicAdd(&a.rem.y, -c.y);
-            atomicAdd(&a.rem.z, -c.z);
-            if (q.meta.z) {
-                V3 d = mk(a.delta.x, a.delta.y, a.delta.z);
-                for (int k = 0; k < a.vn; k++) {
-                    float v = d.x * c.x * a.vv[3 * k] + d.y * c.y * a.vv[3 * k + 1] + d.z * c.z * a.vv[3 * k + 2];
-                    if (v != 0.0f && isfinite(v)) atomicAdd(g + a.vp[k], v);
-                }
-            }
-            if (A.nmp) {
-                ShadowReq sq;
-                sq.o = r.o; sq.d = r.d; sq.t_max = q.o_tmax.w; sq.time = q.d_time.w;
-                float dc = a.delta.x * c.x + a.delta.y * c.y + a.delta.z * c.z;
-                for (int k = 0; k < A.nmp; k++) {
-                    float v = -shadow_length(S, A.mp[k], sq) * dc;
-                    if (v != 0.0f && isfinite(v)) atomicAdd(g + A.mp[k].param, v);
-                }
-            }
-        }
-    }
-    __syncthreads();
-    for (uint32_t t = threadIdx.x; t < A.nparams; t += BLOCK)
-        if (g[t] != 0.0f) atomicAdd(A.grad + t, g[t]);
-}
 
 // ============================================================================ host side
 namespace {
@@ -1199,14 +191,6 @@ int rt_ctx_create(int device, rt_ctx** out) {
     c->device = device;
     HIPCHECK(c.get(), hipSetDevice(device));
     HIPCHECK(c.get(), hipStreamCreate(&c->stream));
-    HIPCHECK(c.get(), hipMemcpyToSymbol(HIP_SYMBOL(c_cie_x), RT_CIE_X, sizeof(RT_CIE_X)));
-    HIPCHECK(c.get(), hipMemcpyToSymbol(HIP_SYMBOL(c_cie_y), RT_CIE_Y, sizeof(RT_CIE_Y)));
-    HIPCHECK(c.get(), hipMemcpyToSymbol(HIP_SYMBOL(c_cie_z), RT_CIE_Z, sizeof(RT_CIE_Z)));
-    float metal[6][RT_METAL_N];
-    memcpy(metal[0], RT_GOLD_N, sizeof RT_GOLD_N); memcpy(metal[1], RT_GOLD_K, sizeof RT_GOLD_K);
-    memcpy(metal[2], RT_COPPER_N, sizeof RT_COPPER_N); memcpy(metal[3], RT_COPPER_K, sizeof RT_COPPER_K);
-    memcpy(metal[4], RT_SILVER_N, sizeof RT_SILVER_N); memcpy(metal[5], RT_SILVER_K, sizeof RT_SILVER_K);
-    HIPCHECK(c.get(), hipMemcpyToSymbol(HIP_SYMBOL(c_metal), metal, sizeof metal));
     HIPCHECK(c.get(), hipMalloc(&c->elut, 64 * 64 * sizeof(float)));
     hipLaunchKernelGGL(k_elut, dim3(64 * 64 / 64), dim3(64), 0, c->stream, c->elut);
     HIPCHECK(c.get(), hipGetLastError());
@@ -3047,3 +2031,4 @@ int rt_ctx_get_counters(rt_ctx* ctx, uint64_t out[8]) {
 }
 
 }  // extern "C"
+
