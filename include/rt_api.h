@@ -329,11 +329,23 @@ typedef struct {
 } rt_wavefront_opts;
 
 /* ---- caustic photon map (src/photon.rs, src/scene.rs:35-74) --------------- */
-enum { RT_EMIT_QUAD = 0, RT_EMIT_SUN_DISK = 1 };
+/* RT_EMIT_QUAD: PhotonMap::build_from_quad (src/photon.rs:97-135).
+ * RT_EMIT_SUN_DISK: PhotonMap::build (src/photon.rs:41-94): a disk facing
+ *   sun_dir, sized to the world's photon_bounds (finite geometry only: an
+ *   infinite plane has none, src/plane.rs:102-106; transforms forward theirs),
+ *   or a 20-unit disk 30 units up sun_dir when no finite geometry exists.
+ * RT_EMIT_SCENE: SceneData::rebuild_caustics (src/scene.rs:35-74): the scene's
+ *   background picks the emitter.  Physical: SUN_DISK toward its sun_dir with
+ *   colour pi * background(power_dir), power_dir = unit(sun_dir + 0.04 *
+ *   perp); env map (EnvMap / EnvMapSplit): SUN_DISK toward the normalised sum
+ *   of (0, 1, 0) and eight CDF direction draws of SmallRng(0xC0FFEE), colour
+ *   pi * env(dir); otherwise QUAD with origin / u / v / color (the scene's
+ *   caustic_quad, its first light).  rt_photon_emitter reports the choice.   */
+enum { RT_EMIT_QUAD = 0, RT_EMIT_SUN_DISK = 1, RT_EMIT_SCENE = 2 };
 
 typedef struct {
-    uint32_t emitter;          /* RT_EMIT_QUAD: build_from_quad; SUN_DISK: build */
-    float    origin[3], u[3], v[3];   /* QUAD                                      */
+    uint32_t emitter;          /* RT_EMIT_*                                      */
+    float    origin[3], u[3], v[3];   /* QUAD (and SCENE's fallback quad)         */
     float    color[3];         /* QUAD: light colour; SUN_DISK: sun_color        */
     float    sun_dir[3];       /* SUN_DISK                                       */
     uint32_t num_photons;
@@ -500,6 +512,17 @@ int  rt_backward_wavefront(rt_ctx *ctx, rt_scene *scene, const rt_camera *cam,
  * the scene; rt_sample_params.use_photons enables the gather.                */
 int  rt_photon_build(rt_ctx *ctx, rt_scene *scene, const rt_photon_params *pp,
                      uint32_t *num_stored);
+/* The emitter rt_photon_build would use for pp: RT_EMIT_SCENE resolved into
+ * the SUN_DISK (sun_dir, color) or QUAD rebuild_caustics picks
+ * (src/scene.rs:35-74); other kinds are copied.  RT_ERR_INVALID when
+ * RT_EMIT_SCENE finds neither a sun, an env map nor a quad (u x v = 0): the
+ * reference then builds no map.                                              */
+int  rt_photon_emitter(rt_ctx *ctx, rt_scene *scene, const rt_photon_params *pp,
+                       rt_photon_params *out);
+/* The world's photon_bounds (src/bvh.rs:206-208, src/hittable.rs:199-211):
+ * min[3], max[3] into out6; *has = 0 when only proxy geometry (infinite
+ * planes) exists.  Host-side data of the uploaded scene.                     */
+int  rt_scene_photon_bounds(rt_ctx *ctx, rt_scene *scene, float *out6, int *has);
 /* Attach a photon map built elsewhere (e.g. by the host's own
  * rebuild_caustics): n photons of 9 floats (position, power, normal), the
  * layout rt_photon_export writes; the device orders them for its gather.    */

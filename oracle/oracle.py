@@ -37,6 +37,8 @@ def lib():
         L.rtc_render_sample.argtypes = [vp, vp, vp, vp, vp, C.c_int, C.c_int]
         L.rtc_render_accumulate.argtypes = [vp, vp, vp, C.c_uint32, vp, C.c_int, C.c_int]
         L.rtc_photon_build.argtypes = [vp, vp, C.c_int, C.POINTER(C.c_uint32)]
+        L.rtc_photon_emitter.argtypes = [vp, vp, vp]
+        L.rtc_photon_bounds.argtypes = [vp, vp]
         L.rtc_photon_dump.argtypes = [vp, vp, C.c_uint32]
         L.rtc_photon_irradiance.argtypes = [vp, vp, C.c_uint32, vp, C.c_int]
         L.rtc_world_nodes.argtypes = [vp, vp, vp, vp, C.c_uint32]
@@ -175,6 +177,21 @@ class OracleScene:
         n = C.c_uint32()
         self.L.rtc_photon_build(self.h, C.byref(pp), nthreads or threads(), C.byref(n))
         return n.value
+
+    def photon_emitter(self, pp):
+        """rebuild_caustics' emitter for pp (RT_EMIT_SCENE resolved), or None."""
+        from rustracer_amd import _ffi as F
+        out = F.rt_photon_params()
+        if self.L.rtc_photon_emitter(self.h, C.byref(pp), C.byref(out)) != 0:
+            return None
+        return out
+
+    def photon_bounds(self):
+        """The world's photon_bounds as (min, max), or None."""
+        out = np.zeros(6, np.float32)
+        if not self.L.rtc_photon_bounds(self.h, _p(out)):
+            return None
+        return out[:3], out[3:]
 
     def photon_set_gather(self, radius, knn=0):
         if self.L.rtc_photon_set_gather(self.h, float(radius), int(knn)) != 0:

@@ -46,6 +46,9 @@ struct Hittable {
     virtual ~Hittable() {}
     virtual bool hit(const Ray& r, float t_min, float t_max, HitRecord& rec) const = 0;
     virtual bool bounding_box(Aabb& out) const = 0;
+    // Hittable::photon_bounds (src/hittable.rs:105-109): the bounds that size
+    // the caustic photon disk; proxy geometry (the infinite plane) has none.
+    virtual bool photon_bounds(Aabb& out) const { return bounding_box(out); }
     virtual bool any_hit(const Ray& r, float t_min, float t_max) const {
         HitRecord tmp;
         return hit(r, t_min, t_max, tmp);
@@ -417,7 +420,7 @@ struct InfinitePlane : Hittable {
         }
         out = Aabb(Point3(lo[0], lo[1], lo[2]), Point3(hi[0], hi[1], hi[2]));
         return true;
-    }
+    }    bool photon_bounds(Aabb&) const override { return false; }   // src/plane.rs:102-106
 };
 
 // ---- Triangle ------------------------------------------------------------------
@@ -620,6 +623,17 @@ struct HittableList : Hittable {
         }
         return true;
     }
+    // src/hittable.rs:199-211: union of the children that have photon bounds
+    bool photon_bounds(Aabb& out) const override {
+        bool have = false;
+        for (auto& o : objects) {
+            Aabb b;
+            if (!o->photon_bounds(b)) continue;
+            out = have ? Aabb::surrounding(out, b) : b;
+            have = true;
+        }
+        return have;
+    }
     float pdf_value(const Point3& origin, const Vec3& dir, float time) const override {
         if (objects.empty()) return 0.0f;
         float weight = 1.0f / (float)objects.size();
@@ -782,6 +796,13 @@ struct BvhTree : Hittable {
     }
 
     void from_list(const std::vector<HPtr>& objs) {
+        has_photon_bbox = false;
+        for (auto& o : objs) {
+            Aabb b;
+            if (!o->photon_bounds(b)) continue;
+            photon_bbox = has_photon_bbox ? Aabb::surrounding(photon_bbox, b) : b;
+            has_photon_bbox = true;
+        }
         std::vector<Aabb> boxes(objs.size());
         for (size_t i = 0; i < objs.size(); i++) { Aabb b; if (objs[i]->bounding_box(b)) boxes[i] = b; }
         int32_t root = build(objs, boxes, bbox);
@@ -919,6 +940,10 @@ struct BvhTree : Hittable {
         return objects[idx]->pdf_generate(origin, rng, time);
     }
     bool bounding_box(Aabb& out) const override { if (nodes.empty()) return false; out = bbox; return true; }
+    // src/bvh.rs:206-208, 604: union of the objects' photon bounds, in input order
+    bool has_photon_bbox = false;
+    Aabb photon_bbox;
+    bool photon_bounds(Aabb& out) const override { if (!has_photon_bbox) return false; out = photon_bbox; return true; }
 };
 
 // ---- transforms ------------------------------------------------------------------
@@ -941,6 +966,10 @@ struct UniformScale : Hittable {
         Aabb b; if (!object->bounding_box(b)) return false;
         out = Aabb(factor * b.min, factor * b.max); return true;
     }
+    bool photon_bounds(Aabb& out) const override {      // src/transform.rs:76-80
+        Aabb b; if (!object->photon_bounds(b)) return false;
+        out = Aabb(factor * b.min, factor * b.max); return true;
+    }
 };
 
 struct Translate : Hittable {
@@ -960,10 +989,15 @@ struct Translate : Hittable {
         Aabb b; if (!object->bounding_box(b)) return false;
         out = Aabb(b.min + offset, b.max + offset); return true;
     }
+    bool photon_bounds(Aabb& out) const override {      // src/transform.rs:131-135
+        Aabb b; if (!object->photon_bounds(b)) return false;
+        out = Aabb(b.min + offset, b.max + offset); return true;
+    }
 };
 
 struct Rotate : Hittable {
     HPtr object; float m[3][3]; Aabb bbox; bool has_bbox = false;
+    Aabb photon_bbox; bool has_photon_bbox = false;      // src/transform.rs:146-148, 200
     Vec3 apply(const Vec3& v) const {
         return Vec3(m[0][0] * v.x + m[0][1] * v.y + m[0][2] * v.z,
                     m[1][0] * v.x + m[1][1] * v.y + m[1][2] * v.z,
@@ -974,10 +1008,7 @@ struct Rotate : Hittable {
                     m[0][1] * v.x + m[1][1] * v.y + m[2][1] * v.z,
                     m[0][2] * v.x + m[1][2] * v.y + m[2][2] * v.z);
     }
-    void init_bbox() {
-        Aabb bb;
-        has_bbox = object->bounding_box(bb);
-        if (!has_bbox) return;
+    Aabb rotated_aabb(const Aabb& bb) const {
         Point3 mn(INF, INF, INF), mx(-INF, -INF, -INF);
         for (int i = 0; i < 2; i++) for (int j = 0; j < 2; j++) for (int k = 0; k < 2; k++) {
             Vec3 c(i == 0 ? bb.min.x : bb.max.x, j == 0 ? bb.min.y : bb.max.y, k == 0 ? bb.min.z : bb.max.z);
@@ -986,7 +1017,14 @@ struct Rotate : Hittable {
             mn.y = rmin(mn.y, r.y); mx.y = rmax(mx.y, r.y);
             mn.z = rmin(mn.z, r.z); mx.z = rmax(mx.z, r.z);
         }
-        bbox = Aabb(mn, mx);
+        return Aabb(mn, mx);
+    }
+    void init_bbox() {
+        Aabb bb;
+        has_bbox = object->bounding_box(bb);
+        if (has_bbox) bbox = rotated_aabb(bb);
+        has_photon_bbox = object->photon_bounds(bb);
+        if (has_photon_bbox) photon_bbox = rotated_aabb(bb);
     }
     Ray rotated(const Ray& r) const { return Ray::scatter_from(apply_t(r.origin), apply_t(r.direction), r); }
     bool hit(const Ray& r, float t_min, float t_max, HitRecord& rec) const override {
@@ -1000,6 +1038,7 @@ struct Rotate : Hittable {
     float pdf_value(const Point3& o, const Vec3& d, float time) const override { return object->pdf_value(apply_t(o), apply_t(d), time); }
     Vec3 pdf_generate(const Point3& o, SmallRng& rng, float time) const override { return apply(object->pdf_generate(apply_t(o), rng, time)); }
     bool bounding_box(Aabb& out) const override { if (!has_bbox) return false; out = bbox; return true; }
+    bool photon_bounds(Aabb& out) const override { if (!has_photon_bbox) return false; out = photon_bbox; return true; }
 };
 
 }  // namespace rc

@@ -62,6 +62,7 @@ struct ConstantMedium : Hittable {
         return 1.0f;
     }
     bool bounding_box(Aabb& out) const override { return boundary->bounding_box(out); }
+    bool photon_bounds(Aabb& out) const override { return boundary->photon_bounds(out); }   // src/volume.rs:153-155, 260-262
 };
 
 // NoiseMedium with the per-lookup Perlin turbulence replaced by a trilinear
@@ -126,6 +127,7 @@ struct GridMedium : Hittable {
         return std::exp(-sum / (float)K * l);
     }
     bool bounding_box(Aabb& out) const override { return boundary->bounding_box(out); }
+    bool photon_bounds(Aabb& out) const override { return boundary->photon_bounds(out); }   // src/volume.rs:153-155, 260-262
 };
 
 // NoiseMedium (src/volume.rs:160-263): the same tracking and quadrature as
@@ -242,9 +244,11 @@ struct EnvMap {
         float lum = Color(p[0], p[1], p[2]).luminance();
         return lum * (float)((size_t)w * h) * inv_total / (2.0f * PI * PI);
     }
-    Vec3 sample_dir(SmallRng& rng, float& pdf) const {
+    // cdf: the two CDF searches even when the alias table samples the map
+    // (rebuild_caustics draws through EnvMapData::sample_dir, src/scene.rs:58-62)
+    Vec3 sample_dir(SmallRng& rng, float& pdf, bool cdf = false) const {
         size_t i, j;
-        if (use_alias) {
+        if (use_alias && !cdf) {
             size_t n = (size_t)w * h;
             size_t t = (size_t)(((unsigned __int128)rng.next_u64() * (unsigned __int128)n) >> 64);
             if (t >= n) t = n - 1;

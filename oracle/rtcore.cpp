@@ -427,9 +427,58 @@ int rtc_render_accumulate(void* s, const rt_camera* cam, const rt_sample_params*
     return 0;
 }
 
-int rtc_photon_build(void* s, const rt_photon_params* pp, int threads, uint32_t* stored) {
+// SceneData::rebuild_caustics (src/scene.rs:35-74) for RT_EMIT_SCENE: a
+// Physical background emits from its sun with the colour of the sky just off
+// the sun axis; an env map from the normalised sum of (0, 1, 0) and eight
+// luminance-sampled directions of SmallRng(0xC0FFEE) with the map's colour
+// there; otherwise the caustic quad in pp.  Other kinds are copied.
+// Returns -1 when RT_EMIT_SCENE finds no emitter (no quad given).
+int rtc_photon_emitter(void* s, const rt_photon_params* pp, rt_photon_params* out) {
     OScene& S = *(OScene*)s;
-    if (pp->knn > RT_KNN_MAX) return -1;
+    *out = *pp;
+    if (pp->emitter != RT_EMIT_SCENE) return 0;
+    Vec3 dir; Color c;
+    if (S.bg.kind == RT_BG_PHYSICAL) {
+        Vec3 sun_dir = S.bg.sun_dir;
+        Vec3 perp = std::fabs(sun_dir.x) < 0.9f ? Vec3(0.0f, -sun_dir.z, sun_dir.y).unit()
+                                                 : Vec3(-sun_dir.z, 0.0f, sun_dir.x).unit();
+        Vec3 power_dir = (sun_dir + perp * 0.04f).unit();
+        dir = sun_dir;
+        c = S.bg.eval(power_dir) * PI;
+    } else if (S.bg.has_env()) {
+        SmallRng rng = SmallRng::seed_from_u64(0xC0FFEEull);
+        dir = Vec3(0.0f, 1.0f, 0.0f);
+        for (int i = 0; i < 8; i++) { float pdf; dir = dir + S.bg.env.sample_dir(rng, pdf, true); }
+        dir = dir.unit();
+        c = S.bg.env.sample(dir) * PI;
+    } else {
+        Vec3 u(pp->u[0], pp->u[1], pp->u[2]), v(pp->v[0], pp->v[1], pp->v[2]);
+        if (!(u.cross(v).length() > 0.0f)) return -1;
+        out->emitter = RT_EMIT_QUAD;
+        return 0;
+    }
+    out->emitter = RT_EMIT_SUN_DISK;
+    out->sun_dir[0] = dir.x; out->sun_dir[1] = dir.y; out->sun_dir[2] = dir.z;
+    out->color[0] = c.x; out->color[1] = c.y; out->color[2] = c.z;
+    return 0;
+}
+
+// The world's photon_bounds (src/bvh.rs:206-208): out[0..6) = min, max;
+// returns 0 when only proxy geometry (infinite planes) is present.
+int rtc_photon_bounds(void* s, float* out) {
+    OScene& S = *(OScene*)s;
+    Aabb bb;
+    if (!S.world.photon_bounds(bb)) return 0;
+    out[0] = bb.min.x; out[1] = bb.min.y; out[2] = bb.min.z; out[3] = bb.max.x; out[4] = bb.max.y; out[5] = bb.max.z;
+    return 1;
+}
+
+int rtc_photon_build(void* s, const rt_photon_params* pp_in, int threads, uint32_t* stored) {
+    OScene& S = *(OScene*)s;
+    if (pp_in->knn > RT_KNN_MAX) return -1;
+    rt_photon_params res;
+    if (rtc_photon_emitter(s, pp_in, &res) != 0) return -1;
+    const rt_photon_params* pp = &res;
     bool sun_was = S.pearl_sun.active;
     S.pearl_sun.active = false;              // photon tracing sees no pearl sun
     if (pp->emitter == RT_EMIT_QUAD) {
@@ -437,7 +486,7 @@ int rtc_photon_build(void* s, const rt_photon_params* pp, int threads, uint32_t*
                                Vec3(pp->u[0], pp->u[1], pp->u[2]), Vec3(pp->v[0], pp->v[1], pp->v[2]),
                                Color(pp->color[0], pp->color[1], pp->color[2]), pp->num_photons, pp->gather_radius, threads);
     } else {
-        Aabb bb; bool have = S.world.bounding_box(bb);
+        Aabb bb; bool have = S.world.photon_bounds(bb);   // PhotonMap::build sizes its disk from photon_bounds (src/photon.rs:61)
         photon_build_sun(S.photons, S.world, have, bb, Vec3(pp->sun_dir[0], pp->sun_dir[1], pp->sun_dir[2]),
                          Color(pp->color[0], pp->color[1], pp->color[2]), pp->num_photons, pp->gather_radius, threads);
     }

@@ -25,7 +25,7 @@ PARAM_MEDIUM = 0x40000000    # rt_param.material | k: the density of rt_scene_de
  MAT_SPECTRAL_METAL, MAT_ROUGH_DIELECTRIC, MAT_ROUGH_SPECTRAL, MAT_BLACKBODY, MAT_SSS, MAT_PEARL) = \
     1, 2, 3, 4, 5, 6, 7, 8, 9, 10, 11
 BG_SOLID, BG_GRADIENT, BG_PHYSICAL, BG_ENVMAP, BG_ENVMAP_SPLIT = 0, 1, 2, 3, 4
-EMIT_QUAD, EMIT_SUN_DISK = 0, 1
+EMIT_QUAD, EMIT_SUN_DISK, EMIT_SCENE = 0, 1, 2
 
 F32P = C.POINTER(C.c_float)
 
@@ -176,6 +176,8 @@ SIGNATURES = {
     "rt_backward_wavefront": (C.c_int, [VP, VP, C.POINTER(rt_camera), C.POINTER(rt_sample_params), C.c_uint32,
                                         C.POINTER(rt_param), C.c_uint32, VP, VP, C.POINTER(rt_wavefront_opts)]),
     "rt_photon_build": (C.c_int, [VP, VP, C.POINTER(rt_photon_params), C.POINTER(C.c_uint32)]),
+    "rt_photon_emitter": (C.c_int, [VP, VP, C.POINTER(rt_photon_params), C.POINTER(rt_photon_params)]),
+    "rt_scene_photon_bounds": (C.c_int, [VP, VP, VP, C.POINTER(C.c_int)]),
     "rt_photon_import": (C.c_int, [VP, VP, VP, C.c_uint32, C.c_float]),
     "rt_photon_set_gather": (C.c_int, [VP, VP, C.c_float, C.c_uint32]),
     "rt_photon_irradiance": (C.c_int, [VP, VP, VP, C.c_uint32, VP]),

@@ -841,9 +841,11 @@ RT_D float env_pdf_at(const DScene& S, uint32_t i, uint32_t j) {
     float l = lum(mk(p[0], p[1], p[2]));
     return l * (float)((size_t)S.env_w * S.env_h) * S.env_inv_total / (2.0f * PI * PI);
 }
-RT_D V3 env_sample_dir(const DScene& S, Rng& rng, float& pdf) {
+// cdf: the two CDF searches whatever the scene's sampler (rebuild_caustics
+// draws through EnvMapData::sample_dir, src/scene.rs:58-62).
+RT_D V3 env_sample_dir(const DScene& S, Rng& rng, float& pdf, bool cdf = false) {
     uint32_t i, j;
-    if (S.env_sampling == 1) {
+    if (S.env_sampling == 1 && !cdf) {
         // alias table over the W*H texels (Walker 1977, Vose 1991): the first
         // draw picks a column of the table (64-bit multiply-high, no modulo
         // bias), the second accepts it or takes its alias.  Same density and

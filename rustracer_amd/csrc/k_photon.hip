@@ -50,3 +50,33 @@ __global__ __launch_bounds__(BLOCK) void k_photon_trace(DScene S, DEmitter E, ui
     }
     ok[i] = stored;
 }
+
+// The direction and colour of SceneData::rebuild_caustics' emitter
+// (src/scene.rs:35-74), one thread: out[0..3) direction, out[3..6) colour,
+// out[6] = 1 Physical sun, 2 env map, 0 neither (the caller's quad).
+__global__ void k_caustic_emitter(DScene S, float* out) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    V3 dir = mk(0, 0, 0), c = mk(0, 0, 0);
+    float kind = 0.0f;
+    if (S.bg_kind == BGK_PHYSICAL) {
+        V3 sd = mk(S.sun_dir[0], S.sun_dir[1], S.sun_dir[2]);
+        V3 perp = fabsf(sd.x) < 0.9f ? unit(mk(0.0f, -sd.z, sd.y)) : unit(mk(-sd.z, 0.0f, sd.x));
+        V3 pd = unit(sd + perp * 0.04f);
+        dir = sd;
+        c = bg_eval(S, pd) * PI;
+        kind = 1.0f;
+    } else if (has_env(S)) {
+        Rng rng = rng_seed(0xC0FFEEull);
+        dir = mk(0.0f, 1.0f, 0.0f);
+        for (int k = 0; k < 8; k++) {
+            float pdf;
+            dir = dir + env_sample_dir(S, rng, pdf, true);
+        }
+        dir = unit(dir);
+        c = env_sample(S, dir) * PI;
+        kind = 2.0f;
+    }
+    out[0] = dir.x; out[1] = dir.y; out[2] = dir.z;
+    out[3] = c.x; out[4] = c.y; out[5] = c.z;
+    out[6] = kind;
+}

@@ -36,6 +36,7 @@ __global__ void k_backward(DScene S, DCamera cam, DParams P, const int8_t* fmap,
 
 // k_photon.hip
 __global__ void k_photon_trace(DScene S, DEmitter E, uint32_t n, float4* out, uint32_t* ok);
+__global__ void k_caustic_emitter(DScene S, float* out);
 
 // k_wf_trace.hip, k_wf_knn.hip, k_wf_shade_*.hip
 template <bool BATCH> __global__ void k_wf_generate(DCamera cam0, DParams P0, WfQueues Q, uint64_t first, uint32_t n, WfBatch B);

@@ -72,6 +72,10 @@ struct rt_scene {
     uint32_t desc_media = 0;
     void* ph_mem = nullptr;                // photon map (positions, powers, normals), replaced on rebuild
     float world_min[3] = {0, 0, 0}, world_max[3] = {0, 0, 0};
+    // photon_bounds of the world (src/bvh.rs:206-208): the union of the
+    // objects' boxes without the infinite planes' proxies
+    bool have_pb = false;
+    float pb_min[3] = {0, 0, 0}, pb_max[3] = {0, 0, 0};
     ~rt_scene() { for (void* p : allocs) (void)hipFree(p); if (ph_mem) (void)hipFree(ph_mem); }
 };
 
@@ -750,6 +754,15 @@ extern "C" int rt_scene_upload(rt_ctx* ctx, const rt_scene_desc* d, rt_scene** o
     for (int k = 0; k < 3; k++) { S.norms[k] = ctx->norms[k]; S.fire_norms[k] = ctx->fire_norms[k]; }
     S.stack_bound = bound;
     for (int k = 0; k < 3; k++) { s->world_min[k] = wb.mn[k]; s->world_max[k] = wb.mx[k]; }
+    for (uint32_t i = 0; i < d->num_objects; i++) {
+        if (d->objects[i].shape == RT_SHAPE_PLANE) continue;        // InfinitePlane::photon_bounds = None (src/plane.rs:102-106)
+        const rtb::Box& b = F.obj_boxes[i];
+        for (int k = 0; k < 3; k++) {
+            s->pb_min[k] = s->have_pb ? std::fmin(s->pb_min[k], b.mn[k]) : b.mn[k];
+            s->pb_max[k] = s->have_pb ? std::fmax(s->pb_max[k], b.mx[k]) : b.mx[k];
+        }
+        s->have_pb = true;
+    }
     size_t node_count = F.nodes.size();
     s->stats[0] = node_count;                 // nodes
     s->stats[1] = F.tri_tag.size();           // triangles
@@ -1849,8 +1862,47 @@ static int install_photons(rt_ctx* ctx, rt_scene* s, std::vector<float4>& kept, 
     return RT_OK;
 }
 
-int rt_photon_build(rt_ctx* ctx, rt_scene* s, const rt_photon_params* pp, uint32_t* num_stored) {
-    if (!ctx || !s || !pp) return RT_ERR_INVALID;
+int rt_scene_photon_bounds(rt_ctx* ctx, rt_scene* s, float* out6, int* has) {
+    if (!ctx || !s || !out6 || !has) return RT_ERR_INVALID;
+    *has = s->have_pb ? 1 : 0;
+    for (int k = 0; k < 3; k++) { out6[k] = s->pb_min[k]; out6[3 + k] = s->pb_max[k]; }
+    return RT_OK;
+}
+
+// RT_EMIT_SCENE -> the emitter SceneData::rebuild_caustics picks
+// (src/scene.rs:35-74); the sky / env evaluation and the eight env draws run
+// on the device (k_caustic_emitter).
+int rt_photon_emitter(rt_ctx* ctx, rt_scene* s, const rt_photon_params* pp, rt_photon_params* out) {
+    if (!ctx || !s || !pp || !out) return RT_ERR_INVALID;
+    *out = *pp;
+    if (pp->emitter == RT_EMIT_QUAD || pp->emitter == RT_EMIT_SUN_DISK) return RT_OK;
+    if (pp->emitter != RT_EMIT_SCENE) return fail(ctx, RT_ERR_INVALID, "unknown photon emitter kind");
+    HIPCHECK(ctx, hipSetDevice(ctx->device));
+    float* d = nullptr;
+    HIPCHECK(ctx, hipMalloc(&d, 8 * sizeof(float)));
+    std::unique_ptr<void, void (*)(void*)> g(d, [](void* p) { (void)hipFree(p); });
+    hipLaunchKernelGGL(k_caustic_emitter, dim3(1), dim3(64), 0, ctx->stream, s->S, d);
+    HIPCHECK(ctx, hipGetLastError());
+    float h[8];
+    HIPCHECK(ctx, hipMemcpyAsync(h, d, 7 * sizeof(float), hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHECK(ctx, hipStreamSynchronize(ctx->stream));
+    if (h[6] == 0.0f) {
+        V3 u = mk(pp->u[0], pp->u[1], pp->u[2]), v = mk(pp->v[0], pp->v[1], pp->v[2]);
+        if (!(len2(cross(u, v)) > 0.0f)) return fail(ctx, RT_ERR_INVALID, "no caustic emitter: no sun, no env map and no quad");
+        out->emitter = RT_EMIT_QUAD;
+        return RT_OK;
+    }
+    out->emitter = RT_EMIT_SUN_DISK;
+    for (int k = 0; k < 3; k++) { out->sun_dir[k] = h[k]; out->color[k] = h[3 + k]; }
+    return RT_OK;
+}
+
+int rt_photon_build(rt_ctx* ctx, rt_scene* s, const rt_photon_params* pp_in, uint32_t* num_stored) {
+    if (!ctx || !s || !pp_in) return RT_ERR_INVALID;
+    rt_photon_params res;
+    int rc0 = rt_photon_emitter(ctx, s, pp_in, &res);
+    if (rc0) return rc0;
+    const rt_photon_params* pp = &res;
     if (pp->num_photons == 0) return fail(ctx, RT_ERR_INVALID, "num_photons = 0");
     if (pp->knn > RT_KNN_MAX) return fail(ctx, RT_ERR_UNSUPPORTED, "knn above RT_KNN_MAX");
     if (!(pp->gather_radius > 0.0f)) return fail(ctx, RT_ERR_INVALID, "gather_radius must be positive");
@@ -1875,8 +1927,9 @@ int rt_photon_build(rt_ctx* ctx, rt_scene* s, const rt_photon_params* pp, uint32
         E.lum_clamp = 15.0f;
     } else {
         // PhotonMap::build (src/photon.rs:41-94): a disk facing the sun, sized
-        // to the bounding sphere of the scene bounds and pushed out along the
-        // sun direction.
+        // to the bounding sphere of the world's photon bounds and pushed out
+        // along the sun direction; without finite geometry, the 20-unit disk
+        // 30 units up the sun direction.
         V3 sd = mk(pp->sun_dir[0], pp->sun_dir[1], pp->sun_dir[2]);
         V3 sdu = sd / std::sqrt(len2(sd));
         V3 down = -sd;
@@ -1887,11 +1940,18 @@ int rt_photon_build(rt_ctx* ctx, rt_scene* s, const rt_photon_params* pp, uint32
         V3 bb = cross(down, tt);
         const float CLIP = 1.0e3f, DISK_R = 20.0f;
         auto clip = [&](float v) { return v < -CLIP ? -CLIP : (v > CLIP ? CLIP : v); };
-        V3 mn = mk(clip(s->world_min[0]), clip(s->world_min[1]), clip(s->world_min[2]));
-        V3 mx = mk(clip(s->world_max[0]), clip(s->world_max[1]), clip(s->world_max[2]));
-        V3 c = (mn + mx) * 0.5f;
-        float r = std::fmax(std::sqrt(len2(mx - mn)) * 0.5f, 1.0e-3f);
-        V3 dc = c + sdu * r;
+        V3 dc;
+        float r;
+        if (s->have_pb) {
+            V3 mn = mk(clip(s->pb_min[0]), clip(s->pb_min[1]), clip(s->pb_min[2]));
+            V3 mx = mk(clip(s->pb_max[0]), clip(s->pb_max[1]), clip(s->pb_max[2]));
+            V3 c = (mn + mx) * 0.5f;
+            r = std::fmax(std::sqrt(len2(mx - mn)) * 0.5f, 1.0e-3f);
+            dc = c + sdu * r;
+        } else {
+            dc = sdu * 30.0f;
+            r = DISK_R;
+        }
         float disk_area = PI * r * r;
         float f = disk_area / ((float)n * PI);
         E.kind = 1;

@@ -351,6 +351,60 @@ class Renderer:
         self.num_photons = n.value
         return n.value
 
+    def photon_build(self, pp, scene=None):
+        """rt_photon_build with a filled rt_photon_params."""
+        n = C.c_uint32()
+        self._check(self.L.rt_photon_build(self.ctx, self.scene if scene is None else scene, C.byref(pp), C.byref(n)),
+                    "rt_photon_build")
+        self.num_photons = n.value
+        return n.value
+
+    @staticmethod
+    def caustic_params(info, knn=0):
+        """rt_photon_params for SceneData::rebuild_caustics (src/scene.rs:35-74):
+        RT_EMIT_SCENE with the scene's caustic quad (its first light) as the
+        fallback emitter.  info: the caustics settings of a loaded scene
+        (num_photons, gather_radius, light_quad)."""
+        pp = F.rt_photon_params()
+        pp.emitter = F.EMIT_SCENE
+        lq = info.get("light_quad")
+        if lq is not None:
+            for k in range(3):
+                pp.origin[k], pp.u[k], pp.v[k], pp.color[k] = (float(lq["q"][k]), float(lq["u"][k]), float(lq["v"][k]),
+                                                               float(lq["color"][k]))
+        pp.num_photons, pp.gather_radius, pp.knn = int(info["num_photons"]), float(info["gather_radius"]), int(knn)
+        return pp
+
+    def photon_emitter(self, pp, scene=None):
+        """The emitter rt_photon_build uses for pp (RT_EMIT_SCENE resolved)."""
+        out = F.rt_photon_params()
+        self._check(self.L.rt_photon_emitter(self.ctx, self.scene if scene is None else scene, C.byref(pp), C.byref(out)),
+                    "rt_photon_emitter")
+        return out
+
+    def rebuild_caustics(self, info, knn=0, scene=None):
+        """SceneData::rebuild_caustics (src/scene.rs:35-74): nothing when the
+        scene's caustics are off or no emitter exists (no sun, no env map, no
+        light quad); otherwise the map of the emitter the background picks.
+        Returns the number of stored photons (0 when nothing was built)."""
+        if not info.get("caustics"):
+            return 0
+        sc = self.scene if scene is None else scene
+        pp = self.caustic_params(info, knn)
+        try:
+            self.photon_emitter(pp, sc)
+        except RtError:
+            return 0
+        return self.photon_build(pp, sc)
+
+    def photon_bounds(self, scene=None):
+        """The uploaded world's photon_bounds: (min, max) or None."""
+        out = np.zeros(6, np.float32)
+        has = C.c_int()
+        self._check(self.L.rt_scene_photon_bounds(self.ctx, self.scene if scene is None else scene, _ptr(out), C.byref(has)),
+                    "rt_scene_photon_bounds")
+        return (out[:3], out[3:]) if has.value else None
+
     def photon_import(self, photons, gather_radius, scene=None):
         """Attach a host-built photon map: (n, 9) float32 rows of position,
         power, normal (the layout photon_export returns)."""
