@@ -433,6 +433,22 @@ int  rt_render_wavefront(rt_ctx *ctx, rt_scene *scene, const rt_camera *cam,
                          const rt_sample_params *params, uint32_t spp,
                          float *accum_rgb, const rt_wavefront_opts *opts);
 
+/* rt_render_wavefront for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  No counterpart in the reference.  The path queues live in a
+ * fixed device array of RT_NOALLOC_CAPACITY paths (48 MiB) that every GPU
+ * the library runs on holds until the process ends; the call allocates and
+ * frees nothing (no hipMalloc, hipFree, streams, graphs or events) and clears
+ * with kernels, not hipMemsetAsync.  accum_rgb must be device memory.  opts
+ * (may be NULL): capacity 0 = RT_NOALLOC_CAPACITY, larger is RT_ERR_INVALID;
+ * streams must be <= 1 and use_graph 0.  Same frame as rt_render_wavefront
+ * with the same options.  Calls that overlap on one device share the array
+ * and race; each call leaves its queues in it.                               */
+#define RT_NOALLOC_CAPACITY (1u << 17)
+int  rt_render_wavefront_noalloc(rt_ctx *ctx, rt_scene *scene, const rt_camera *cam,
+                                 const rt_sample_params *params, uint32_t spp,
+                                 float *accum_rgb, const rt_wavefront_opts *opts);
+
 /* One render of a batch: the scene it draws (each view may differ, e.g. the
  * same gem spun to another angle), its camera and pass parameters (width,
  * height, max_depth, bg_scale, use_photons, seed, sample_idx, strata), its

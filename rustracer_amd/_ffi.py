@@ -26,6 +26,7 @@ PARAM_MEDIUM = 0x40000000    # rt_param.material | k: the density of rt_scene_de
     1, 2, 3, 4, 5, 6, 7, 8, 9, 10, 11
 BG_SOLID, BG_GRADIENT, BG_PHYSICAL, BG_ENVMAP, BG_ENVMAP_SPLIT = 0, 1, 2, 3, 4
 EMIT_QUAD, EMIT_SUN_DISK, EMIT_SCENE = 0, 1, 2
+NOALLOC_CAPACITY = 1 << 17
 
 F32P = C.POINTER(C.c_float)
 
@@ -166,6 +167,8 @@ SIGNATURES = {
     "rt_render_accumulate": (C.c_int, [VP, VP, C.POINTER(rt_camera), C.POINTER(rt_sample_params), C.c_uint32, VP]),
     "rt_render_wavefront": (C.c_int, [VP, VP, C.POINTER(rt_camera), C.POINTER(rt_sample_params), C.c_uint32, VP,
                                       C.POINTER(rt_wavefront_opts)]),
+    "rt_render_wavefront_noalloc": (C.c_int, [VP, VP, C.POINTER(rt_camera), C.POINTER(rt_sample_params), C.c_uint32,
+                                              VP, C.POINTER(rt_wavefront_opts)]),
     "rt_render_batch": (C.c_int, [VP, C.POINTER(rt_batch_view), C.c_uint32, C.POINTER(rt_wavefront_opts)]),
     "rt_scene_set_params": (C.c_int, [VP, VP, C.POINTER(rt_param), C.c_uint32, VP]),
     "rt_scene_get_params": (C.c_int, [VP, VP, C.POINTER(rt_param), C.c_uint32, VP]),

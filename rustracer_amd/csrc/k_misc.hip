@@ -228,3 +228,9 @@ __global__ __launch_bounds__(BLOCK) void k_photon_irr(DScene S, const float* __r
 
 template __global__ void k_photon_irr<true>(DScene, const float*, uint32_t, float*);
 template __global__ void k_photon_irr<false>(DScene, const float*, uint32_t, float*);
+
+// Fill n words with v (grid-stride): clears buffers without hipMemsetAsync,
+// so a caller may capture the launch in a graph.
+__global__ __launch_bounds__(BLOCK) void k_fill_u32(uint32_t* p, uint64_t n, uint32_t v) {
+    for (uint64_t i = (uint64_t)blockIdx.x * BLOCK + threadIdx.x; i < n; i += (uint64_t)gridDim.x * BLOCK) p[i] = v;
+}

@@ -23,6 +23,7 @@ __global__ void k_wf_bump_first(uint32_t* counters, uint32_t n);
 __global__ void k_accum_passes(const float* lsamp, uint32_t K, uint32_t w, uint32_t h, uint32_t pass0, rt_accum_opts o, float* acc,
                                uint32_t* ns, float* m2, uint8_t* conv, uint32_t* ctl);
 template <bool KNN> __global__ void k_photon_irr(DScene S, const float* pn, uint32_t n, float* out);
+__global__ void k_fill_u32(uint32_t* p, uint64_t n, uint32_t v);
 
 // k_render.hip, k_progressive.hip
 template <bool KNN> __global__ void k_render(DScene S, DCamera cam, DParams P, const uint8_t* conv, float* out, uint32_t spp, int accumulate);

@@ -14,6 +14,7 @@
 #include <memory>
 #include <algorithm>
 #include <cmath>
+#include <mutex>
 
 using namespace rt;
 
@@ -56,6 +57,9 @@ struct rt_ctx {
     void* xwf_mem[RT_WF_MAX_STREAMS - 1] = {};
     size_t xwf_bytes[RT_WF_MAX_STREAMS - 1] = {};
     hipEvent_t ev_fork = nullptr, ev_join[RT_WF_MAX_STREAMS - 1] = {};
+    // this device's copy of the file-scope `scratch` array (rt_render_wavefront_noalloc)
+    void* scratch = nullptr;
+    bool no_alloc = false;                    // inside rt_render_wavefront_noalloc: no event creation
 };
 
 struct rt_scene {
@@ -179,6 +183,53 @@ static void spectral_norms(float norms[3], float fire[3]) {
     for (int k = 0; k < 3; k++) { norms[k] = (float)RT_CIE_N / std::fmax(s[k], 1e-6f); fire[k] = (float)RT_CIE_N / std::fmax(f[k], 1e-6f); }
 }
 
+// ---- wavefront buffer layout ------------------------------------------------------------------
+// One block per lane: path records, the active / next / dense queues, the
+// per-key buckets, the shadow requests, then 256 B of counters.
+constexpr size_t wf_al(size_t x) { return (x + 255) & ~(size_t)255; }
+constexpr size_t wf_layout_bytes(uint32_t cap) {
+    return wf_al((size_t)cap * sizeof(WfPath)) + 3 * wf_al((size_t)cap * 4) + wf_al((size_t)WF_NB * cap * 4) +
+           wf_al((size_t)3 * cap * sizeof(WfShadow)) + 256;
+}
+static void wf_layout(char* m, uint32_t cap, WfQueues& Q) {
+    Q.paths = (WfPath*)m; m += wf_al((size_t)cap * sizeof(WfPath));
+    Q.active = (uint32_t*)m; m += wf_al((size_t)cap * 4);
+    Q.next = (uint32_t*)m; m += wf_al((size_t)cap * 4);
+    Q.dense = (uint32_t*)m; m += wf_al((size_t)cap * 4);
+    Q.buckets = (uint32_t*)m; m += wf_al((size_t)WF_NB * cap * 4);
+    Q.shadow = (WfShadow*)m; m += wf_al((size_t)3 * cap * sizeof(WfShadow));
+    Q.counters = (uint32_t*)m;
+    Q.cap = cap;
+    Q.fb = nullptr;
+    Q.lsamp = nullptr;        // per-sample contributions: the adjoint's primal pass, the wavefront render loop
+    Q.stop = nullptr;
+    Q.conv = nullptr;
+    Q.sort = 1;
+}
+
+// The no-allocation wavefront's queues: a file-scope device array of one
+// lane's buffers at RT_NOALLOC_CAPACITY paths (48 MiB), held by every GPU the
+// library runs on until the process ends.  Host code never uses the name
+// itself as a device address: rt_ctx_create looks up this device's copy once
+// (hipGetSymbolAddress, with the device current) and keeps it in ctx->scratch.
+static_assert(RT_NOALLOC_CAPACITY % BLOCK == 0, "capacity is whole blocks");
+static __device__ __align__(256) char scratch[wf_layout_bytes(RT_NOALLOC_CAPACITY)];
+
+static int scratch_lookup(rt_ctx* c) {
+    static std::mutex mu;
+    static void* per_device[256] = {};
+    std::lock_guard<std::mutex> lock(mu);
+    if (c->device >= 256) return RT_OK;                 // no copy: the no-allocation entry point refuses
+    if (!per_device[c->device]) {
+        void* ptr = nullptr;
+        hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return fail(c, RT_ERR_DEVICE, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+        per_device[c->device] = ptr;
+    }
+    c->scratch = per_device[c->device];
+    return RT_OK;
+}
+
 extern "C" {
 
 int rt_api_version(void) { return RT_API_VERSION; }
@@ -195,6 +246,7 @@ int rt_ctx_create(int device, rt_ctx** out) {
     c->device = device;
     HIPCHECK(c.get(), hipSetDevice(device));
     HIPCHECK(c.get(), hipStreamCreate(&c->stream));
+    if (int rc = scratch_lookup(c.get())) return rc;
     HIPCHECK(c.get(), hipMalloc(&c->elut, 64 * 64 * sizeof(float)));
     hipLaunchKernelGGL(k_elut, dim3(64 * 64 / 64), dim3(64), 0, c->stream, c->elut);
     HIPCHECK(c.get(), hipGetLastError());
@@ -1069,13 +1121,7 @@ int rt_render_progressive(rt_ctx* ctx, rt_scene* s, const rt_camera* cam, const 
 
 // ---- wavefront renderer --------------------------------------------------------------------------
 static int wf_buffers(rt_ctx* ctx, uint32_t cap, WfQueues& Q, int slot = 0) {
-    auto al = [](size_t x) { return (x + 255) & ~(size_t)255; };
-    size_t b_paths = al((size_t)cap * sizeof(WfPath));
-    size_t b_q = al((size_t)cap * 4);
-    size_t b_buckets = al((size_t)WF_NB * cap * 4);
-    size_t b_shadow = al((size_t)3 * cap * sizeof(WfShadow));
-    size_t b_cnt = 256;
-    size_t total = b_paths + 3 * b_q + b_buckets + b_shadow + b_cnt;
+    size_t total = wf_layout_bytes(cap);
     void*& mem = slot ? ctx->xwf_mem[slot - 1] : ctx->wf_mem;
     size_t& bytes = slot ? ctx->xwf_bytes[slot - 1] : ctx->wf_bytes;
     if (bytes < total) {
@@ -1085,20 +1131,7 @@ static int wf_buffers(rt_ctx* ctx, uint32_t cap, WfQueues& Q, int slot = 0) {
         if (e != hipSuccess) return fail(ctx, RT_ERR_NOMEM, std::string("wavefront buffers: ") + hipGetErrorString(e));
         bytes = total;
     }
-    char* m = (char*)mem;
-    Q.paths = (WfPath*)m; m += b_paths;
-    Q.active = (uint32_t*)m; m += b_q;
-    Q.next = (uint32_t*)m; m += b_q;
-    Q.dense = (uint32_t*)m; m += b_q;
-    Q.buckets = (uint32_t*)m; m += b_buckets;
-    Q.shadow = (WfShadow*)m; m += b_shadow;
-    Q.counters = (uint32_t*)m;
-    Q.cap = cap;
-    Q.fb = nullptr;
-    Q.lsamp = nullptr;        // per-sample contributions: the adjoint's primal pass, the wavefront render loop
-    Q.stop = nullptr;
-    Q.conv = nullptr;
-    Q.sort = 1;
+    wf_layout((char*)mem, cap, Q);
     ctx->wf_cap = cap;
     return RT_OK;
 }
@@ -1128,7 +1161,7 @@ static void wf_chunk(rt_ctx* ctx, hipStream_t st, const DScene& S, const DCamera
     B.active = Q0.next; B.next = Q0.active;
     for (uint32_t d = 0; d < depth; d++) {
         const WfQueues& Q = (d & 1) ? B : A;
-        bool ev = ctx->timing && first != ~0ull;
+        bool ev = ctx->timing && !ctx->no_alloc && first != ~0ull;
         if (ev) {
             while (ctx->ext_ev.size() < ctx->ext_used + 2) { hipEvent_t e; if (hipEventCreate(&e) != hipSuccess) { ev = false; break; } ctx->ext_ev.push_back(e); }
         }
@@ -1287,6 +1320,51 @@ extern "C" int rt_render_wavefront(rt_ctx* ctx, rt_scene* s, const rt_camera* ca
     if ((rc = wf_lanes(ctx, o, Qs))) return rc;
     if ((rc = wf_run<false>(ctx, s->S, C, P, Qs, o.streams, o, total, P.max_depth, none, P.use_photons && s->S.ph_knn))) return rc;
     return finish_out(ctx, out, (size_t)npix * 12, rout);
+}
+
+// rt_render_wavefront without any allocation: one lane of queues in this
+// device's `scratch` array, the frame zeroed by a fill kernel, no streams,
+// graphs or per-launch events created.  Calls overlapping on one device share
+// the array and race.
+extern "C" int rt_render_wavefront_noalloc(rt_ctx* ctx, rt_scene* s, const rt_camera* cam, const rt_sample_params* p,
+                                           uint32_t spp, float* out, const rt_wavefront_opts* opts) {
+    if (!ctx || !s || !cam || !p || !out) return RT_ERR_INVALID;
+    if (p->width == 0 || p->height == 0 || spp == 0) return fail(ctx, RT_ERR_INVALID, "empty frame");
+    if (p->max_depth == 0) return fail(ctx, RT_ERR_INVALID, "max_depth = 0");
+    if (p->use_photons && s->S.num_photons == 0) return fail(ctx, RT_ERR_INVALID, "use_photons without a photon map");
+    if ((uint64_t)p->width * p->height > 0xffffffffull / 4) return fail(ctx, RT_ERR_INVALID, "image too large");
+    if (!ctx->scratch) return fail(ctx, RT_ERR_UNSUPPORTED, "no scratch array on this device");
+    HIPCHECK(ctx, hipSetDevice(ctx->device));
+    if (!is_device_ptr(out)) return fail(ctx, RT_ERR_INVALID, "rt_render_wavefront_noalloc: out must be device memory");
+    uint32_t npix = p->width * p->height;
+    uint64_t total = (uint64_t)npix * spp;
+    rt_wavefront_opts oo = opts ? *opts : rt_wavefront_opts{0, 0, 1, 0, 0, 1};
+    if (oo.capacity == 0) oo.capacity = RT_NOALLOC_CAPACITY;
+    if (oo.capacity > RT_NOALLOC_CAPACITY)
+        return fail(ctx, RT_ERR_INVALID, "capacity " + std::to_string(oo.capacity) + " exceeds the no-allocation scratch (" +
+                                             std::to_string(RT_NOALLOC_CAPACITY) + " paths)");
+    if (oo.streams > 1 || oo.use_graph) return fail(ctx, RT_ERR_INVALID, "rt_render_wavefront_noalloc runs one stream without graphs");
+    WfOpts o;
+    int rc;
+    if ((rc = wf_opts(ctx, &oo, total, o))) return rc;
+    WfQueues Q;
+    wf_layout((char*)ctx->scratch, o.cap, Q);
+    Q.sort = o.sort;
+    Q.fb = out;
+    DCamera C; DParams P;
+    to_dev(cam, p, C, P);
+    P.counters = nullptr;
+    hipLaunchKernelGGL(k_fill_u32, dim3(std::min((npix * 3 + BLOCK - 1) / BLOCK, 4096u)), dim3(BLOCK), 0, ctx->stream,
+                       (uint32_t*)out, (uint64_t)npix * 3, 0u);
+    hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(BLOCK), 0, ctx->stream, Q.counters, (uint64_t)64, 0u);
+    HIPCHECK(ctx, hipGetLastError());
+    WfBatch none = {nullptr, nullptr, 0};
+    ctx->no_alloc = true;
+    rc = wf_run<false>(ctx, s->S, C, P, &Q, 1, o, total, P.max_depth, none, P.use_photons && s->S.ph_knn);
+    ctx->no_alloc = false;
+    if (rc) return rc;
+    HIPCHECK(ctx, hipStreamSynchronize(ctx->stream));
+    return RT_OK;
 }
 
 // rt_render_progressive through the wavefront: chunks of K whole passes

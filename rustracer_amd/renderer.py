@@ -249,6 +249,19 @@ class Renderer:
                     "rt_render_wavefront")
         return out
 
+    def render_wavefront_noalloc(self, out, camera, width, height, spp, sample_idx=0, strata=0, bg_scale=1.0,
+                                 photon_map=False, max_depth=50, seed=0, capacity=0, grid=0, sort=True, occupancy=0,
+                                 scene=None):
+        """render_wavefront without device allocations (rt_render_wavefront_noalloc):
+        out must be device memory (e.g. a torch cuda tensor); capacity at most
+        NOALLOC_CAPACITY paths, one stream, no graphs."""
+        p = self.params(width, height, sample_idx, strata, bg_scale, max_depth, None, photon_map, seed)
+        cam = camera.to_c()
+        o = F.rt_wavefront_opts(capacity=capacity, grid=grid, sort=int(sort), use_graph=0, occupancy=occupancy, streams=1)
+        self._check(self.L.rt_render_wavefront_noalloc(self.ctx, self.scene if scene is None else scene, C.byref(cam),
+                                                       C.byref(p), spp, _ptr(out), C.byref(o)), "rt_render_wavefront_noalloc")
+        return out
+
     def render_batch(self, views, capacity=0, grid=0, sort=True, use_graph=False, occupancy=0, streams=1):
         """views: list of dicts with keys scene (handle), camera, width, height,
         spp, out, and optionally max_depth, photon_map, seed, bg_scale."""
