@@ -3,6 +3,7 @@
 // device-resident layouts of the wavefront renderer (path records, queues,
 // shadow requests, batch views, adjoint replay records).
 #pragma once
+#include <type_traits>
 #include "dev_shade.h"
 #include "dev_adjoint.h"
 #include "dev_accum.h"
@@ -89,21 +90,24 @@ static_assert(sizeof(WfPath) == 128, "path record is one 128-B line");
 
 struct __align__(16) WfShadow { float4 o_tmax; float4 d_time; float4 c_pixel; uint4 meta; };   // meta.x = view
 
+// Every field has a default, so a WfQueues is never handed to a kernel with
+// an unset pointer (a forward stage once wrote through an uninitialised one).
 struct WfQueues {
-    WfPath* paths;
-    uint32_t cap;
-    uint32_t* active;     // cap
-    uint32_t* next;       // cap
-    uint32_t* buckets;    // WF_NB * cap
-    WfShadow* shadow;     // 3 * cap
-    uint32_t* counters;   // 64: 32 per-bounce counters, then the chunk position
-    float* fb;
-    float* lsamp;         // non-null: contributions go to the path's sample (3 floats per sid), not to fb
-    uint32_t sort;        // 0: one bucket; 1: a bucket per material kind; 2: two shade classes
-    uint32_t* dense;      // cap: paths whose k-NN query needs the heap search
-    const uint32_t* stop; // non-null and nonzero at chunk start: the chunk renders no path (a stopped render loop)
-    const uint8_t* conv;  // non-null: flagged pixels get no path (the render loop's converged pixels)
+    WfPath* paths = nullptr;
+    uint32_t cap = 0;
+    uint32_t* active = nullptr;     // cap
+    uint32_t* next = nullptr;       // cap
+    uint32_t* buckets = nullptr;    // WF_NB * cap
+    WfShadow* shadow = nullptr;     // 3 * cap
+    uint32_t* counters = nullptr;   // 64: 32 per-bounce counters, then the chunk position
+    float* fb = nullptr;
+    float* lsamp = nullptr;         // non-null: contributions go to the path's sample (3 floats per sid), not to fb
+    uint32_t sort = 1;              // 0: one bucket; 1: a bucket per material kind; 2: two shade classes
+    uint32_t* dense = nullptr;      // cap: paths whose k-NN query needs the heap search
+    const uint32_t* stop = nullptr; // non-null and nonzero at chunk start: the chunk renders no path (a stopped render loop)
+    const uint8_t* conv = nullptr;  // non-null: flagged pixels get no path (the render loop's converged pixels)
 };
+static_assert(std::is_trivially_copyable<WfQueues>::value, "kernel argument");
 
 __device__ __forceinline__ uint32_t lane_id() { return __lane_id(); }
 

@@ -13,6 +13,8 @@ import numpy as np
 import pytest
 
 from oracle import oracle as O
+
+import imgstats as IS
 from rustracer_amd import Renderer, scenes
 
 pytestmark = pytest.mark.gpu
@@ -35,16 +37,14 @@ def test_rough_spectral_gem_matches_checker(gpu_renderer):
         c[k] = orc.render_sample(cam, Renderer.params(W, H, sample_idx=k, max_depth=24, seed=1), mode=1)
     close = np.isclose(g, c, rtol=1e-3, atol=1e-4).all(axis=-1)
     assert close.mean() > 0.95, close.mean()
-    K = 128
-    g = np.zeros((K, H, W, 3), np.float32)
-    c = np.zeros((K, H, W, 3), np.float32)
-    for k in range(K):
-        gpu_renderer.render_tiles(g[k], None, k, 0, W, H, cam, max_depth=24, seed=2)
-        c[k] = orc.render_sample(cam, Renderer.params(W, H, sample_idx=k, max_depth=24, seed=2), mode=0)
-    ma, mb = g.mean(0), c.mean(0)
-    se = np.sqrt(g.var(0, ddof=1) / K + c.var(0, ddof=1) / K)
-    z = np.where(se > 0, np.abs(ma - mb) / np.maximum(se, 1e-12), np.where(np.abs(ma - mb) < 1e-6, 0, np.inf))
-    assert (z > 3).mean() <= 0.02 and (z > 6).mean() <= 0.002, ((z > 3).mean(), (z > 6).mean())
+    def gpu(first, spp):
+        out = np.zeros((H, W, 3), np.float32)
+        gpu_renderer.render_accumulate(out, cam, W, H, spp, sample_idx=first, max_depth=24, seed=2)
+        return out
+
+    def cpu(first, spp):
+        return orc.render_accumulate(cam, Renderer.params(W, H, sample_idx=first, max_depth=24, seed=3), spp, mode=0)
+    IS.gate(IS.frames(gpu, 128, 32), IS.frames(cpu, 128, 32))
 
 
 def test_detached_at_current_parameters_is_the_plain_frame(gpu_renderer):
