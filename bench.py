@@ -83,7 +83,7 @@ def measure(name, args, with_cpu):
 
     tuned, tune_info = (json.loads(args.wf_opts) if args.wf_opts else {}), None
     if not args.no_tune and not args.wf_opts:
-        tuned, tune_info = tune_wavefront(r, acc, cam, W, H, spp, depth, photons)
+        tuned, tune_info = tune_wavefront(r, acc, cam, W, H, spp, depth, photons, knn=bool(cfg.get("knn", 0)))
         log(f"[{name}] autotuned wavefront: {tuned} ({tune_info['seconds']}s)")
 
     def step_plain(k):

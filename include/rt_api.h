@@ -33,7 +33,7 @@
 extern "C" {
 #endif
 
-#define RT_API_VERSION 1
+#define RT_API_VERSION 2
 
 /* ---- status codes ------------------------------------------------------- */
 #define RT_OK               0
@@ -317,7 +317,12 @@ typedef struct {
  * streams: n in 2..RT_WF_MAX_STREAMS runs consecutive chunks round-robin on
  * n HIP streams, each with queues of its own, so a chunk's last bounces (few
  * paths, latency-bound) overlap the next chunks' first ones; 0 or 1: one
- * stream.                                                                    */
+ * stream;
+ * block: lanes per block of the traversal stages (closest hit, shadow): 64,
+ * 128 or 256 (0: 128);
+ * knn_wave_max: k-NN gathers that still need the nearest-photon search after
+ * the first pass are searched one query per lane (each with a heap in LDS)
+ * when more than this many wait, else one query per wave (0: 65536).        */
 #define RT_WF_MAX_STREAMS 4
 typedef struct {
     uint32_t capacity;
@@ -326,6 +331,8 @@ typedef struct {
     uint32_t use_graph;
     uint32_t occupancy;
     uint32_t streams;
+    uint32_t block;
+    uint32_t knn_wave_max;
 } rt_wavefront_opts;
 
 /* ---- caustic photon map (src/photon.rs, src/scene.rs:35-74) --------------- */

@@ -25,6 +25,7 @@ PARAM_MEDIUM = 0x40000000    # rt_param.material | k: the density of rt_scene_de
  MAT_SPECTRAL_METAL, MAT_ROUGH_DIELECTRIC, MAT_ROUGH_SPECTRAL, MAT_BLACKBODY, MAT_SSS, MAT_PEARL) = \
     1, 2, 3, 4, 5, 6, 7, 8, 9, 10, 11
 BG_SOLID, BG_GRADIENT, BG_PHYSICAL, BG_ENVMAP, BG_ENVMAP_SPLIT = 0, 1, 2, 3, 4
+API_VERSION = 2          # include/rt_api.h RT_API_VERSION: the layouts below
 EMIT_QUAD, EMIT_SUN_DISK, EMIT_SCENE = 0, 1, 2
 NOALLOC_CAPACITY = 1 << 17
 
@@ -103,7 +104,7 @@ class rt_sample_params(C.Structure):
 
 class rt_wavefront_opts(C.Structure):
     _fields_ = [("capacity", C.c_uint32), ("grid", C.c_uint32), ("sort", C.c_uint32), ("use_graph", C.c_uint32),
-                ("occupancy", C.c_uint32), ("streams", C.c_uint32)]
+                ("occupancy", C.c_uint32), ("streams", C.c_uint32), ("block", C.c_uint32), ("knn_wave_max", C.c_uint32)]
 
 
 class rt_accum_opts(C.Structure):
@@ -227,5 +228,7 @@ def lib():
             fn = getattr(L, name)
             fn.restype = res
             fn.argtypes = args
+        if L.rt_api_version() != API_VERSION:
+            raise ImportError(f"{LIB_PATH} has ABI version {L.rt_api_version()}, these bindings {API_VERSION}: rebuild")
         _lib = L
     return _lib

@@ -1,8 +1,10 @@
 """Autotuner for the wavefront renderer's launch parameters.
 
-Picks the register budget (occupancy target) of the traversal stages, the
-path-queue capacity, the persistent grid of the stage kernels, the material
-sort, hipGraph replay and one or two chunk streams by coordinate descent on a short frame of
+Picks the register budget (occupancy target) and block size of the traversal
+stages, the path-queue capacity, the persistent grid of the stage kernels, the
+material sort, hipGraph replay, one to four chunk streams and, with k-NN
+photon maps, the dense-query count up to which the nearest-photon search runs
+one query per wave, by coordinate descent on a short frame of
 the actual workload (same scene, camera, depth and photon setting), timing
 each candidate with HIP events on the library's stream.  The search is cheap
 (each probe renders a few passes) and the result is returned as the kwargs of
@@ -12,6 +14,8 @@ import time
 
 CAPACITIES = [1 << 20, 1 << 21, 1 << 22, 1 << 23]
 GRIDS = [1024, 2048, 4096, 8192]
+BLOCKS = [64, 128, 256]
+KNN_WAVE_MAX = [16384, 65536, 262144]
 
 
 def _probe(r, run, opts, repeats=2):
@@ -24,7 +28,7 @@ def _probe(r, run, opts, repeats=2):
     return ms / repeats
 
 
-def tune_wavefront(r, out, cam, W, H, spp, depth, photons=False, probe_spp=None, log=None):
+def tune_wavefront(r, out, cam, W, H, spp, depth, photons=False, probe_spp=None, log=None, knn=False):
     """Tunes `Renderer.render_wavefront` on one frame.  Returns (best_opts,
     info) where info["table"] lists every probe as (opts, ms per probe frame)."""
     npix = W * H
@@ -35,12 +39,12 @@ def tune_wavefront(r, out, cam, W, H, spp, depth, photons=False, probe_spp=None,
 
     def run(o, seed):
         r.render_wavefront(out, cam, W, H, probe_spp, max_depth=depth, photon_map=photons, seed=seed, **o)
-    best, info = tune(r, run, log)
+    best, info = tune(r, run, log, knn=photons and knn)
     info["probe_spp"] = probe_spp
     return best, info
 
 
-def tune_batch(r, views, probe_spp=4, log=None):
+def tune_batch(r, views, probe_spp=4, log=None, knn=True):
     """Tunes `Renderer.render_batch` on copies of the views with at most
     `probe_spp` samples per pixel each."""
     probe = [dict(v, spp=min(v["spp"], probe_spp)) for v in views]
@@ -49,15 +53,17 @@ def tune_batch(r, views, probe_spp=4, log=None):
         for v in probe:
             v["seed"] = seed
         r.render_batch(probe, **o)
-    best, info = tune(r, run, log)
+    best, info = tune(r, run, log, knn=knn and any(v.get("photon_map") for v in views))
     info["probe_spp"] = probe_spp
     return best, info
 
 
-def tune(r, run, log=None):
+def tune(r, run, log=None, knn=False):
     """Coordinate descent over the launch options; `run(opts, seed)` renders
     one probe frame on the renderer's stream."""
-    best = dict(capacity=1 << 22, grid=2048, sort=1, use_graph=True, occupancy=0, streams=1)
+    best = dict(capacity=1 << 22, grid=2048, sort=1, use_graph=True, occupancy=0, streams=1, block=128)
+    if knn:
+        best["knn_wave_max"] = 65536
     table = []
 
     def measure(o):
@@ -69,8 +75,11 @@ def tune(r, run, log=None):
 
     t0 = time.time()
     best_ms = measure(best)
-    for key, values in (("occupancy", [0, 4, 5]), ("capacity", CAPACITIES), ("grid", GRIDS), ("sort", [0, 1, 2]),
-                        ("use_graph", [True, False]), ("streams", [1, 2, 3, 4])):
+    axes = [("occupancy", [0, 4, 5]), ("block", BLOCKS), ("capacity", CAPACITIES), ("grid", GRIDS), ("sort", [0, 1, 2]),
+            ("use_graph", [True, False]), ("streams", [1, 2, 3, 4])]
+    if knn:
+        axes.append(("knn_wave_max", KNN_WAVE_MAX))
+    for key, values in axes:
         for v in values:
             if v == best[key]:
                 continue

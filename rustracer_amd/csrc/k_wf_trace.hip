@@ -62,12 +62,12 @@ __device__ __forceinline__ uint32_t extend_one(const DScene& S, WfPath& w, uint3
 
 
 // OCC = minimum waves per SIMD the register allocator must allow (0: its own choice).
-template <int OCC, bool BATCH>
-__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(OCC > 0 ? OCC : 1, 8))) void k_wf_extend(DScene S, WfQueues Q, WfBatch B) {
-    Stack st; init_stack(st);
+template <int OCC, bool BATCH, int BS>
+__global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(OCC > 0 ? OCC : 1, 8))) void k_wf_extend(DScene S, WfQueues Q, WfBatch B) {
+    Stack st; init_stack_bs<BS>(st);
     uint32_t count = Q.counters[WF_CNT_ACTIVE];
-    uint32_t stride = gridDim.x * BLOCK;
-    for (uint32_t base = blockIdx.x * BLOCK; base < count; base += stride) {
+    uint32_t stride = gridDim.x * BS;
+    for (uint32_t base = blockIdx.x * BS; base < count; base += stride) {
         uint32_t i = base + threadIdx.x;
         bool valid = i < count;
         uint32_t pid = 0, key = 0;
@@ -83,12 +83,12 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(OCC > 0 ?
 }
 
 
-template <int OCC, bool BATCH>
-__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(OCC > 0 ? OCC : 1, 8))) void k_wf_shadow(DScene S, WfQueues Q, WfBatch B) {
-    Stack st; init_stack(st);
+template <int OCC, bool BATCH, int BS>
+__global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(OCC > 0 ? OCC : 1, 8))) void k_wf_shadow(DScene S, WfQueues Q, WfBatch B) {
+    Stack st; init_stack_bs<BS>(st);
     uint32_t count = Q.counters[WF_CNT_SHADOW];
-    uint32_t stride = gridDim.x * BLOCK;
-    for (uint32_t i = blockIdx.x * BLOCK + threadIdx.x; i < count; i += stride) {
+    uint32_t stride = gridDim.x * BS;
+    for (uint32_t i = blockIdx.x * BS + threadIdx.x; i < count; i += stride) {
         WfShadow q = Q.shadow[i];
         Ray r = make_ray(mk(q.o_tmax.x, q.o_tmax.y, q.o_tmax.z), mk(q.d_time.x, q.d_time.y, q.d_time.z), q.d_time.w);
         float tr = BATCH ? trace_transmittance(B.scenes[q.meta.x], r, 0.001f, q.o_tmax.w, st, false)
@@ -102,17 +102,41 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(OCC > 0 ?
     }
 }
 
-template __global__ void k_wf_extend<0, true>(DScene, WfQueues, WfBatch);
-template __global__ void k_wf_extend<0, false>(DScene, WfQueues, WfBatch);
-template __global__ void k_wf_extend<4, true>(DScene, WfQueues, WfBatch);
-template __global__ void k_wf_extend<4, false>(DScene, WfQueues, WfBatch);
-template __global__ void k_wf_extend<5, true>(DScene, WfQueues, WfBatch);
-template __global__ void k_wf_extend<5, false>(DScene, WfQueues, WfBatch);
-template __global__ void k_wf_shadow<0, true>(DScene, WfQueues, WfBatch);
-template __global__ void k_wf_shadow<0, false>(DScene, WfQueues, WfBatch);
-template __global__ void k_wf_shadow<5, true>(DScene, WfQueues, WfBatch);
-template __global__ void k_wf_shadow<5, false>(DScene, WfQueues, WfBatch);
-template __global__ void k_wf_shadow<6, true>(DScene, WfQueues, WfBatch);
-template __global__ void k_wf_shadow<6, false>(DScene, WfQueues, WfBatch);
 template __global__ void k_wf_generate<true>(DCamera, DParams, WfQueues, uint64_t, uint32_t, WfBatch);
 template __global__ void k_wf_generate<false>(DCamera, DParams, WfQueues, uint64_t, uint32_t, WfBatch);
+template __global__ void k_wf_extend<0, true, 64>(DScene, WfQueues, WfBatch);
+template __global__ void k_wf_extend<0, true, 128>(DScene, WfQueues, WfBatch);
+template __global__ void k_wf_extend<0, true, 256>(DScene, WfQueues, WfBatch);
+template __global__ void k_wf_extend<0, false, 64>(DScene, WfQueues, WfBatch);
+template __global__ void k_wf_extend<0, false, 128>(DScene, WfQueues, WfBatch);
+template __global__ void k_wf_extend<0, false, 256>(DScene, WfQueues, WfBatch);
+template __global__ void k_wf_extend<4, true, 64>(DScene, WfQueues, WfBatch);
+template __global__ void k_wf_extend<4, true, 128>(DScene, WfQueues, WfBatch);
+template __global__ void k_wf_extend<4, true, 256>(DScene, WfQueues, WfBatch);
+template __global__ void k_wf_extend<4, false, 64>(DScene, WfQueues, WfBatch);
+template __global__ void k_wf_extend<4, false, 128>(DScene, WfQueues, WfBatch);
+template __global__ void k_wf_extend<4, false, 256>(DScene, WfQueues, WfBatch);
+template __global__ void k_wf_extend<5, true, 64>(DScene, WfQueues, WfBatch);
+template __global__ void k_wf_extend<5, true, 128>(DScene, WfQueues, WfBatch);
+template __global__ void k_wf_extend<5, true, 256>(DScene, WfQueues, WfBatch);
+template __global__ void k_wf_extend<5, false, 64>(DScene, WfQueues, WfBatch);
+template __global__ void k_wf_extend<5, false, 128>(DScene, WfQueues, WfBatch);
+template __global__ void k_wf_extend<5, false, 256>(DScene, WfQueues, WfBatch);
+template __global__ void k_wf_shadow<0, true, 64>(DScene, WfQueues, WfBatch);
+template __global__ void k_wf_shadow<0, true, 128>(DScene, WfQueues, WfBatch);
+template __global__ void k_wf_shadow<0, true, 256>(DScene, WfQueues, WfBatch);
+template __global__ void k_wf_shadow<0, false, 64>(DScene, WfQueues, WfBatch);
+template __global__ void k_wf_shadow<0, false, 128>(DScene, WfQueues, WfBatch);
+template __global__ void k_wf_shadow<0, false, 256>(DScene, WfQueues, WfBatch);
+template __global__ void k_wf_shadow<5, true, 64>(DScene, WfQueues, WfBatch);
+template __global__ void k_wf_shadow<5, true, 128>(DScene, WfQueues, WfBatch);
+template __global__ void k_wf_shadow<5, true, 256>(DScene, WfQueues, WfBatch);
+template __global__ void k_wf_shadow<5, false, 64>(DScene, WfQueues, WfBatch);
+template __global__ void k_wf_shadow<5, false, 128>(DScene, WfQueues, WfBatch);
+template __global__ void k_wf_shadow<5, false, 256>(DScene, WfQueues, WfBatch);
+template __global__ void k_wf_shadow<6, true, 64>(DScene, WfQueues, WfBatch);
+template __global__ void k_wf_shadow<6, true, 128>(DScene, WfQueues, WfBatch);
+template __global__ void k_wf_shadow<6, true, 256>(DScene, WfQueues, WfBatch);
+template __global__ void k_wf_shadow<6, false, 64>(DScene, WfQueues, WfBatch);
+template __global__ void k_wf_shadow<6, false, 128>(DScene, WfQueues, WfBatch);
+template __global__ void k_wf_shadow<6, false, 256>(DScene, WfQueues, WfBatch);

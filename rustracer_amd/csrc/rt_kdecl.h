@@ -41,10 +41,11 @@ __global__ void k_caustic_emitter(DScene S, float* out);
 
 // k_wf_trace.hip, k_wf_knn.hip, k_wf_shade_*.hip
 template <bool BATCH> __global__ void k_wf_generate(DCamera cam0, DParams P0, WfQueues Q, uint64_t first, uint32_t n, WfBatch B);
-template <int OCC, bool BATCH> __global__ void k_wf_extend(DScene S, WfQueues Q, WfBatch B);
-template <int OCC, bool BATCH> __global__ void k_wf_shadow(DScene S, WfQueues Q, WfBatch B);
+template <int OCC, bool BATCH, int BS> __global__ void k_wf_extend(DScene S, WfQueues Q, WfBatch B);
+template <int OCC, bool BATCH, int BS> __global__ void k_wf_shadow(DScene S, WfQueues Q, WfBatch B);
 template <bool BATCH> __global__ void k_wf_knn(DScene S0, DParams P0, WfQueues Q, WfBatch B);
 template <bool BATCH> __global__ void k_wf_knn_dense(DScene S0, WfQueues Q, WfBatch B);
+template <bool BATCH> __global__ void k_wf_knn_wave(DScene S0, WfQueues Q, WfBatch B);
 template <bool BATCH, int CLASS> __global__ void k_wf_shade(DScene S, DParams P, WfQueues Q, int depth, WfBatch B);
 
 // k_wfb.hip

@@ -33,6 +33,21 @@ __device__ __forceinline__ void init_stack(Stack& st) {
     st.n_nodes = 0; st.n_prims = 0; st.n_rays = 0;
     st.s_nodes = 0; st.s_prims = 0; st.s_rays = 0;
 }
+// The traversal stages of the wavefront are compiled for blocks of 64, 128
+// and 256 lanes (the autotuner picks one); their LDS stack is sized to it.
+template <int BS>
+struct LdsStackT {
+    __device__ static int32_t* column() {
+        __shared__ int32_t s[RT_LDS_STACK * BS];
+        return s + threadIdx.x;
+    }
+};
+template <int BS>
+__device__ __forceinline__ void init_stack_bs(Stack& st) {
+    st.lds = LdsStackT<BS>::column(); st.stride = BS; st.sp = 0; st.heap = nullptr;
+    st.n_nodes = 0; st.n_prims = 0; st.n_rays = 0;
+    st.s_nodes = 0; st.s_prims = 0; st.s_rays = 0;
+}
 
 __device__ __forceinline__ Ray ray_from_c(const rt_ray& q) {
     return make_ray(mk(q.origin[0], q.origin[1], q.origin[2]), mk(q.dir[0], q.dir[1], q.dir[2]), q.time);
@@ -106,6 +121,10 @@ struct WfQueues {
     uint32_t* dense = nullptr;      // cap: paths whose k-NN query needs the heap search
     const uint32_t* stop = nullptr; // non-null and nonzero at chunk start: the chunk renders no path (a stopped render loop)
     const uint8_t* conv = nullptr;  // non-null: flagged pixels get no path (the render loop's converged pixels)
+    // dense k-NN queries at most this many: one query per wave, else one per
+    // lane (on the 2M-photon gem at 1280 x 720 the crossover lies between
+    // 64k and 256k queries; camera-ray bounces exceed it, later ones do not)
+    uint32_t knn_wave_max = 65536;
 };
 static_assert(std::is_trivially_copyable<WfQueues>::value, "kernel argument");
 

@@ -1136,24 +1136,36 @@ static int wf_buffers(rt_ctx* ctx, uint32_t cap, WfQueues& Q, int slot = 0) {
     return RT_OK;
 }
 
-template <bool BT>
-static void launch_extend(int occ, uint32_t grid, hipStream_t st, const DScene& S, const WfQueues& Q, const WfBatch& B) {
-    if (occ >= 5) hipLaunchKernelGGL((k_wf_extend<5, BT>), dim3(grid), dim3(BLOCK), 0, st, S, Q, B);
-    else if (occ == 4) hipLaunchKernelGGL((k_wf_extend<4, BT>), dim3(grid), dim3(BLOCK), 0, st, S, Q, B);
-    else hipLaunchKernelGGL((k_wf_extend<0, BT>), dim3(grid), dim3(BLOCK), 0, st, S, Q, B);
+// The traversal stages at the chosen register budget and block size; grid is
+// in blocks of 128 lanes, so the launch keeps the same lanes in flight.
+template <bool BT, int BS>
+static void launch_trace_bs(bool shadow, int occ, uint32_t grid, hipStream_t st, const DScene& S, const WfQueues& Q,
+                            const WfBatch& B) {
+    dim3 g(std::max(1u, (uint32_t)((uint64_t)grid * BLOCK / BS))), b(BS);
+    if (!shadow) {
+        if (occ >= 5) hipLaunchKernelGGL((k_wf_extend<5, BT, BS>), g, b, 0, st, S, Q, B);
+        else if (occ == 4) hipLaunchKernelGGL((k_wf_extend<4, BT, BS>), g, b, 0, st, S, Q, B);
+        else hipLaunchKernelGGL((k_wf_extend<0, BT, BS>), g, b, 0, st, S, Q, B);
+    } else {
+        if (occ >= 5) hipLaunchKernelGGL((k_wf_shadow<6, BT, BS>), g, b, 0, st, S, Q, B);
+        else if (occ == 4) hipLaunchKernelGGL((k_wf_shadow<5, BT, BS>), g, b, 0, st, S, Q, B);
+        else hipLaunchKernelGGL((k_wf_shadow<0, BT, BS>), g, b, 0, st, S, Q, B);
+    }
 }
 template <bool BT>
-static void launch_shadow(int occ, uint32_t grid, hipStream_t st, const DScene& S, const WfQueues& Q, const WfBatch& B) {
-    if (occ >= 5) hipLaunchKernelGGL((k_wf_shadow<6, BT>), dim3(grid), dim3(BLOCK), 0, st, S, Q, B);
-    else if (occ == 4) hipLaunchKernelGGL((k_wf_shadow<5, BT>), dim3(grid), dim3(BLOCK), 0, st, S, Q, B);
-    else hipLaunchKernelGGL((k_wf_shadow<0, BT>), dim3(grid), dim3(BLOCK), 0, st, S, Q, B);
+static void launch_trace(bool shadow, int occ, uint32_t block, uint32_t grid, hipStream_t st, const DScene& S,
+                         const WfQueues& Q, const WfBatch& B) {
+    if (block == 64) launch_trace_bs<BT, 64>(shadow, occ, grid, st, S, Q, B);
+    else if (block == 256) launch_trace_bs<BT, 256>(shadow, occ, grid, st, S, Q, B);
+    else launch_trace_bs<BT, 128>(shadow, occ, grid, st, S, Q, B);
 }
 
 // Launch one chunk: generate `n` paths at global position `first` (~0: read
 // from the counters, for graph replays) and run `depth` bounces.
 template <bool BT>
 static void wf_chunk(rt_ctx* ctx, hipStream_t st, const DScene& S, const DCamera& C, const DParams& P, const WfQueues& Q0,
-                     uint64_t first, uint32_t n, uint32_t grid, int occ, uint32_t depth, const WfBatch& Bt, bool knn) {
+                     uint64_t first, uint32_t n, uint32_t grid, int occ, uint32_t block, uint32_t depth, const WfBatch& Bt,
+                     bool knn) {
     if (Q0.conv) hipLaunchKernelGGL(k_wf_gen_zero, dim3(1), dim3(64), 0, st, Q0.counters);
     hipLaunchKernelGGL((k_wf_generate<BT>), dim3((n + BLOCK - 1) / BLOCK), dim3(BLOCK), 0, st, C, P, Q0, first, n, Bt);
     hipLaunchKernelGGL(k_wf_reset, dim3(1), dim3(32), 0, st, Q0.counters, n, Q0.stop, Q0.conv ? 1 : 0);
@@ -1166,11 +1178,12 @@ static void wf_chunk(rt_ctx* ctx, hipStream_t st, const DScene& S, const DCamera
             while (ctx->ext_ev.size() < ctx->ext_used + 2) { hipEvent_t e; if (hipEventCreate(&e) != hipSuccess) { ev = false; break; } ctx->ext_ev.push_back(e); }
         }
         if (ev) (void)hipEventRecord(ctx->ext_ev[ctx->ext_used], st);
-        launch_extend<BT>(occ, grid, st, S, Q, Bt);
+        launch_trace<BT>(false, occ, block, grid, st, S, Q, Bt);
         if (ev) { (void)hipEventRecord(ctx->ext_ev[ctx->ext_used + 1], st); ctx->ext_used += 2; }
         if (knn) {
             hipLaunchKernelGGL((k_wf_knn<BT>), dim3(grid), dim3(BLOCK), 0, st, S, P, Q, Bt);
             hipLaunchKernelGGL((k_wf_knn_dense<BT>), dim3(grid), dim3(BLOCK), 0, st, S, Q, Bt);
+            hipLaunchKernelGGL((k_wf_knn_wave<BT>), dim3(grid), dim3(BLOCK), 0, st, S, Q, Bt);
         }
         if (Q.sort == 2) {
             hipLaunchKernelGGL((k_wf_shade<BT, 1>), dim3(grid), dim3(BLOCK), 0, st, S, P, Q, (int)d, Bt);
@@ -1178,12 +1191,12 @@ static void wf_chunk(rt_ctx* ctx, hipStream_t st, const DScene& S, const DCamera
         } else {
             hipLaunchKernelGGL((k_wf_shade<BT, 0>), dim3(grid), dim3(BLOCK), 0, st, S, P, Q, (int)d, Bt);
         }
-        launch_shadow<BT>(occ, grid, st, S, Q, Bt);
+        launch_trace<BT>(true, occ, block, grid, st, S, Q, Bt);
         hipLaunchKernelGGL(k_wf_advance, dim3(1), dim3(32), 0, st, Q.counters);
     }
 }
 
-struct WfOpts { uint32_t cap, grid, sort, use_graph, streams; int occ; };
+struct WfOpts { uint32_t cap, grid, sort, use_graph, streams, block, knn_wave_max; int occ; };
 
 static int wf_opts(rt_ctx* ctx, const rt_wavefront_opts* opts, uint64_t total, WfOpts& o) {
     o.cap = opts && opts->capacity ? opts->capacity : (1u << 22);
@@ -1192,6 +1205,9 @@ static int wf_opts(rt_ctx* ctx, const rt_wavefront_opts* opts, uint64_t total, W
     o.use_graph = opts ? opts->use_graph : 0u;
     o.occ = opts ? (int)opts->occupancy : 0;
     o.streams = opts && opts->streams > 1 ? opts->streams : 1u;
+    o.block = opts && opts->block ? opts->block : (uint32_t)BLOCK;
+    o.knn_wave_max = opts && opts->knn_wave_max ? opts->knn_wave_max : 65536u;
+    if (o.block != 64 && o.block != 128 && o.block != 256) return fail(ctx, RT_ERR_INVALID, "wavefront block must be 64, 128 or 256");
     if (o.streams > RT_WF_MAX_STREAMS) return fail(ctx, RT_ERR_INVALID, "wavefront streams above RT_WF_MAX_STREAMS");
     if (o.cap < BLOCK || o.cap > (1u << 26)) return fail(ctx, RT_ERR_INVALID, "wavefront capacity out of range");
     if (o.sort > 2) return fail(ctx, RT_ERR_INVALID, "wavefront sort must be 0, 1 or 2");
@@ -1237,7 +1253,7 @@ static int wf_run(rt_ctx* ctx, const DScene& S, const DCamera& C, const DParams&
             hipStream_t sk = stream_of(k);
             hipLaunchKernelGGL(k_wf_set_first, dim3(1), dim3(32), 0, sk, Qs[k].counters, (uint64_t)k * cap);
             HIPCHECK(ctx, hipStreamBeginCapture(sk, hipStreamCaptureModeThreadLocal));
-            wf_chunk<BT>(ctx, sk, S, C, P, Qs[k], ~0ull, cap, o.grid, o.occ, depth, Bt, knn);
+            wf_chunk<BT>(ctx, sk, S, C, P, Qs[k], ~0ull, cap, o.grid, o.occ, o.block, depth, Bt, knn);
             hipLaunchKernelGGL(k_wf_bump_first, dim3(1), dim3(32), 0, sk, Qs[k].counters, G * cap);
             HIPCHECK(ctx, hipStreamEndCapture(sk, &graph[k]));
             HIPCHECK(ctx, hipGraphInstantiate(&exec[k], graph[k], nullptr, nullptr, 0));
@@ -1250,12 +1266,12 @@ static int wf_run(rt_ctx* ctx, const DScene& S, const DCamera& C, const DParams&
     } else {
         for (uint64_t c = 0; c < full; c++) {
             uint32_t k = (uint32_t)(c % L);
-            wf_chunk<BT>(ctx, stream_of(k), S, C, P, Qs[k], c * (uint64_t)cap, cap, o.grid, o.occ, depth, Bt, knn);
+            wf_chunk<BT>(ctx, stream_of(k), S, C, P, Qs[k], c * (uint64_t)cap, cap, o.grid, o.occ, o.block, depth, Bt, knn);
         }
     }
     if (rem) {
         uint32_t k = (uint32_t)(full % L);
-        wf_chunk<BT>(ctx, stream_of(k), S, C, P, Qs[k], full * (uint64_t)cap, (uint32_t)rem, o.grid, o.occ, depth, Bt, knn);
+        wf_chunk<BT>(ctx, stream_of(k), S, C, P, Qs[k], full * (uint64_t)cap, (uint32_t)rem, o.grid, o.occ, o.block, depth, Bt, knn);
     }
     HIPCHECK(ctx, hipGetLastError());
     for (uint32_t k = 1; k < L; k++) {
@@ -1287,6 +1303,7 @@ static int wf_lanes(rt_ctx* ctx, const WfOpts& o, WfQueues* Qs) {
         if ((rc = wf_buffers(ctx, o.cap, Qs[k], (int)k))) return rc;
         Qs[k].sort = Qs[0].sort;
         Qs[k].fb = Qs[0].fb;
+        Qs[k].knn_wave_max = Qs[0].knn_wave_max;
     }
     return RT_OK;
 }
@@ -1307,6 +1324,7 @@ extern "C" int rt_render_wavefront(rt_ctx* ctx, rt_scene* s, const rt_camera* ca
     WfQueues Q;
     if ((rc = wf_buffers(ctx, o.cap, Q))) return rc;
     Q.sort = o.sort;
+    Q.knn_wave_max = o.knn_wave_max;
     Staged rout;
     if ((rc = stage_out(ctx, out, (size_t)npix * 12, rout, false))) return rc;
     Q.fb = (float*)rout.dev;
@@ -1338,7 +1356,7 @@ extern "C" int rt_render_wavefront_noalloc(rt_ctx* ctx, rt_scene* s, const rt_ca
     if (!is_device_ptr(out)) return fail(ctx, RT_ERR_INVALID, "rt_render_wavefront_noalloc: out must be device memory");
     uint32_t npix = p->width * p->height;
     uint64_t total = (uint64_t)npix * spp;
-    rt_wavefront_opts oo = opts ? *opts : rt_wavefront_opts{0, 0, 1, 0, 0, 1};
+    rt_wavefront_opts oo = opts ? *opts : rt_wavefront_opts{0, 0, 1, 0, 0, 1, 0, 0};
     if (oo.capacity == 0) oo.capacity = RT_NOALLOC_CAPACITY;
     if (oo.capacity > RT_NOALLOC_CAPACITY)
         return fail(ctx, RT_ERR_INVALID, "capacity " + std::to_string(oo.capacity) + " exceeds the no-allocation scratch (" +
@@ -1350,6 +1368,7 @@ extern "C" int rt_render_wavefront_noalloc(rt_ctx* ctx, rt_scene* s, const rt_ca
     WfQueues Q;
     wf_layout((char*)ctx->scratch, o.cap, Q);
     Q.sort = o.sort;
+    Q.knn_wave_max = o.knn_wave_max;
     Q.fb = out;
     DCamera C; DParams P;
     to_dev(cam, p, C, P);
@@ -1396,6 +1415,7 @@ extern "C" int rt_render_progressive_wavefront(rt_ctx* ctx, rt_scene* s, const r
     WfQueues Qs[RT_WF_MAX_STREAMS];
     if ((rc = wf_buffers(ctx, o.cap, Qs[0]))) return rc;
     Qs[0].sort = o.sort;
+    Qs[0].knn_wave_max = o.knn_wave_max;
     if ((rc = wf_lanes(ctx, o, Qs))) return rc;
     const uint32_t nchunks = (samples_max + K - 1) / K;
     const uint32_t L = std::max(1u, std::min(o.streams, nchunks));
@@ -1435,7 +1455,7 @@ extern "C" int rt_render_progressive_wavefront(rt_ctx* ctx, rt_scene* s, const r
         Qc.conv = conv;                   // converged pixels: no path (their samples are skipped anyway)
         DParams Pc = P;
         Pc.sample_idx = P.sample_idx + pass0;
-        wf_chunk<false>(ctx, st, s->S, C, Pc, Qc, 0, kk * npix, o.grid, o.occ, Pc.max_depth, none, knn);
+        wf_chunk<false>(ctx, st, s->S, C, Pc, Qc, 0, kk * npix, o.grid, o.occ, o.block, Pc.max_depth, none, knn);
         if (c > 0 && L > 1) HIPCHECK(ctx, hipStreamWaitEvent(st, done_ev[(c - 1) % L], 0));   // passes in order
         hipLaunchKernelGGL(k_accum_passes, dim3(grid_acc), dim3(BLOCK), 0, st, (const float*)lsamp, kk, p->width, p->height,
                            P.sample_idx + pass0, *ao, acc, ns, m2, conv, ctx->prog_ctl);
@@ -1499,6 +1519,7 @@ extern "C" int rt_render_batch(rt_ctx* ctx, const rt_batch_view* views, uint32_t
     WfQueues Q;
     if ((rc = wf_buffers(ctx, o.cap, Q))) return rc;
     Q.sort = o.sort;
+    Q.knn_wave_max = o.knn_wave_max;
     // batch side data: views, scenes, concatenated accumulator
     size_t bv = (sizeof(DView) * nviews + 255) & ~(size_t)255, bs = (sizeof(DScene) * nviews + 255) & ~(size_t)255;
     size_t bf = (size_t)fb_pix * 12;
@@ -1738,6 +1759,7 @@ extern "C" int rt_backward_wavefront(rt_ctx* ctx, rt_scene* s, const rt_camera* 
     WfQueues Q;
     if ((rc = wf_buffers(ctx, o.cap, Q))) return rc;
     Q.sort = o.sort;
+    Q.knn_wave_max = o.knn_wave_max;
     Staged din, gout;
     if ((rc = stage_in(ctx, dimg, (size_t)npix * 12, din))) return rc;
     if ((rc = stage_out(ctx, grad, (size_t)n * 4, gout, false))) return rc;
@@ -1793,7 +1815,7 @@ extern "C" int rt_backward_wavefront(rt_ctx* ctx, rt_scene* s, const rt_camera* 
         HIPCHECK(ctx, hipMemsetAsync((void*)A.lsamp, 0, (size_t)m * 12, st));
         WfQueues Qp = Qk;
         Qp.lsamp = (float*)A.lsamp;
-        wf_chunk<false>(ctx, st, s->S, C, P, Qp, first, m, o.grid, o.occ, depth, none, false);   // no k-NN maps here (render_args)
+        wf_chunk<false>(ctx, st, s->S, C, P, Qp, first, m, o.grid, o.occ, o.block, depth, none, false);   // no k-NN maps here (render_args)
         // replay from the same streams
         hipLaunchKernelGGL((k_wf_generate<false>), dim3((m + BLOCK - 1) / BLOCK), dim3(BLOCK), 0, st, C, P, Qk, first, m, none);
         hipLaunchKernelGGL(k_wfb_init, dim3((m + BLOCK - 1) / BLOCK), dim3(BLOCK), 0, st, Qk, A, m);
@@ -1802,7 +1824,7 @@ extern "C" int rt_backward_wavefront(rt_ctx* ctx, rt_scene* s, const rt_camera* 
         QB.active = Qk.next; QB.next = Qk.active;
         for (uint32_t d = 0; d < depth; d++) {
             const WfQueues& Qd = (d & 1) ? QB : QA;
-            launch_extend<false>(o.occ, o.grid, st, s->S, Qd, none);
+            launch_trace<false>(false, o.occ, o.block, o.grid, st, s->S, Qd, none);
             hipLaunchKernelGGL(k_wfb_shade, dim3(o.grid), dim3(BLOCK), 0, st, s->S, P, Qd, A, (int)d);
             hipLaunchKernelGGL(k_wfb_shadow, dim3(o.grid), dim3(BLOCK), 0, st, s->S, Qd, A);
             hipLaunchKernelGGL(k_wf_advance, dim3(1), dim3(32), 0, st, Qd.counters);

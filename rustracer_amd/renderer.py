@@ -191,7 +191,7 @@ class Renderer:
 
     def render_progressive(self, state, camera, width, height, samples_max, opts=None, sample_idx=0, strata=0,
                            bg_scale=1.0, photon_map=False, max_depth=50, seed=0, wavefront=False, capacity=0, grid=0,
-                           sort=1, use_graph=False, occupancy=0, streams=1):
+                           sort=1, use_graph=False, occupancy=0, streams=1, block=0, knn_wave_max=0):
         """The reference's render loop (src/main.rs:463-506) on the device;
         returns (passes run, converged pixels).  wavefront=True renders the
         passes with the wavefront stages (rt_render_progressive_wavefront)."""
@@ -202,7 +202,7 @@ class Renderer:
         done, nconv = C.c_uint32(), C.c_uint32()
         if wavefront:
             w = F.rt_wavefront_opts(capacity=capacity, grid=grid, sort=int(sort), use_graph=1 if use_graph else 0,
-                                    occupancy=occupancy, streams=streams)
+                                    occupancy=occupancy, streams=streams, block=block, knn_wave_max=knn_wave_max)
             self._check(self.L.rt_render_progressive_wavefront(self.ctx, self.scene, C.byref(cam), C.byref(p), samples_max,
                                                                C.byref(o), C.byref(st), C.byref(w), C.byref(done),
                                                                C.byref(nconv)), "rt_render_progressive_wavefront")
@@ -239,30 +239,32 @@ class Renderer:
 
     def render_wavefront(self, out, camera, width, height, spp, sample_idx=0, strata=0, bg_scale=1.0,
                          photon_map=False, max_depth=50, seed=0, capacity=0, grid=0, sort=True, use_graph=False,
-                         occupancy=0, streams=1, scene=None):
+                         occupancy=0, streams=1, scene=None, block=0, knn_wave_max=0):
         """Same frame as render_accumulate, through the wavefront stage kernels."""
         p = self.params(width, height, sample_idx, strata, bg_scale, max_depth, None, photon_map, seed)
         cam = camera.to_c()
         o = F.rt_wavefront_opts(capacity=capacity, grid=grid, sort=int(sort), use_graph=1 if use_graph else 0,
-                                occupancy=occupancy, streams=streams)
+                                occupancy=occupancy, streams=streams, block=block, knn_wave_max=knn_wave_max)
         self._check(self.L.rt_render_wavefront(self.ctx, self.scene if scene is None else scene, C.byref(cam), C.byref(p), spp, _ptr(out), C.byref(o)),
                     "rt_render_wavefront")
         return out
 
     def render_wavefront_noalloc(self, out, camera, width, height, spp, sample_idx=0, strata=0, bg_scale=1.0,
                                  photon_map=False, max_depth=50, seed=0, capacity=0, grid=0, sort=True, occupancy=0,
-                                 scene=None):
+                                 scene=None, block=0, knn_wave_max=0):
         """render_wavefront without device allocations (rt_render_wavefront_noalloc):
         out must be device memory (e.g. a torch cuda tensor); capacity at most
         NOALLOC_CAPACITY paths, one stream, no graphs."""
         p = self.params(width, height, sample_idx, strata, bg_scale, max_depth, None, photon_map, seed)
         cam = camera.to_c()
-        o = F.rt_wavefront_opts(capacity=capacity, grid=grid, sort=int(sort), use_graph=0, occupancy=occupancy, streams=1)
+        o = F.rt_wavefront_opts(capacity=capacity, grid=grid, sort=int(sort), use_graph=0, occupancy=occupancy, streams=1,
+                                block=block, knn_wave_max=knn_wave_max)
         self._check(self.L.rt_render_wavefront_noalloc(self.ctx, self.scene if scene is None else scene, C.byref(cam),
                                                        C.byref(p), spp, _ptr(out), C.byref(o)), "rt_render_wavefront_noalloc")
         return out
 
-    def render_batch(self, views, capacity=0, grid=0, sort=True, use_graph=False, occupancy=0, streams=1):
+    def render_batch(self, views, capacity=0, grid=0, sort=True, use_graph=False, occupancy=0, streams=1, block=0,
+                     knn_wave_max=0):
         """views: list of dicts with keys scene (handle), camera, width, height,
         spp, out, and optionally max_depth, photon_map, seed, bg_scale."""
         arr = (F.rt_batch_view * len(views))()
@@ -274,7 +276,7 @@ class Renderer:
             arr[i].spp = v["spp"]
             arr[i].accum_rgb = _ptr(v["out"])
         o = F.rt_wavefront_opts(capacity=capacity, grid=grid, sort=int(sort), use_graph=1 if use_graph else 0,
-                                occupancy=occupancy, streams=streams)
+                                occupancy=occupancy, streams=streams, block=block, knn_wave_max=knn_wave_max)
         self._check(self.L.rt_render_batch(self.ctx, arr, len(views), C.byref(o)), "rt_render_batch")
         return [v["out"] for v in views]
 
@@ -314,7 +316,7 @@ class Renderer:
 
     def backward(self, dloss_dimage, camera, width, height, spp, params, sample_idx=0, strata=0, bg_scale=1.0,
                  photon_map=False, max_depth=50, seed=0, scene=None, grad=None, wavefront=False, capacity=0, grid=0,
-                 sort=True, occupancy=0, streams=1):
+                 sort=True, occupancy=0, streams=1, block=0):
         """Path-replay adjoint: dLoss/dtheta for dloss_dimage = dLoss/d(mean
         image) (H x W x 3, host array or device tensor); returns grad (float32
         numpy array, or fills the given device tensor).  wavefront=True runs
@@ -329,7 +331,7 @@ class Renderer:
         sc = self.scene if scene is None else scene
         if wavefront:
             o = F.rt_wavefront_opts(capacity=capacity, grid=grid, sort=int(sort), use_graph=0, occupancy=occupancy,
-                                    streams=streams)
+                                    streams=streams, block=block)
             self._check(self.L.rt_backward_wavefront(self.ctx, sc, C.byref(cam), C.byref(p), spp, self._params(params),
                                                      len(params), _ptr(d), _ptr(g), C.byref(o)), "rt_backward_wavefront")
         else:

@@ -219,11 +219,11 @@ def test_gem_with_photons_three_sigma(gpu_renderer):
     three_sigma_gate(gpu_renderer, orc, cam, 32, 32, 24, seed=3, photons=True)
 
 
-@pytest.mark.parametrize("sort,graph,cap,occ,streams", [(1, False, 0, 0, 1), (0, False, 4096, 4, 1), (1, True, 8192, 5, 1),
-                                                         (1, True, 2048, 0, 1), (2, False, 4096, 0, 1), (2, True, 2048, 4, 1),
-                                                         (1, False, 2048, 0, 2), (2, True, 2048, 0, 2), (1, True, 4096, 0, 2),
-                                                         (1, True, 2048, 0, 3), (0, False, 2048, 0, 4), (1, True, 4096, 0, 4)])
-def test_wavefront_equals_megakernel(gpu_renderer, sort, graph, cap, occ, streams):
+@pytest.mark.parametrize("sort,graph,cap,occ,streams,block", [
+    (1, False, 0, 0, 1, 0), (0, False, 4096, 4, 1, 64), (1, True, 8192, 5, 1, 256), (1, True, 2048, 0, 1, 0),
+    (2, False, 4096, 0, 1, 0), (2, True, 2048, 4, 1, 64), (1, False, 2048, 0, 2, 256), (2, True, 2048, 0, 2, 0),
+    (1, True, 4096, 0, 2, 0), (1, True, 2048, 0, 3, 64), (0, False, 2048, 0, 4, 0), (1, True, 4096, 5, 4, 256)])
+def test_wavefront_equals_megakernel(gpu_renderer, sort, graph, cap, occ, streams, block):
     """Same per-pixel streams, same bounce code: the wavefront frame equals the
     megakernel frame up to the order of float atomic additions.  Capacity 2048
     with the graph replays one captured chunk seven times; with n streams the
@@ -235,7 +235,24 @@ def test_wavefront_equals_megakernel(gpu_renderer, sort, graph, cap, occ, stream
     b = np.zeros((40, 48, 3), np.float32)
     gpu_renderer.render_accumulate(a, cam, 48, 40, 8, max_depth=cfg["max_depth"], seed=5)
     gpu_renderer.render_wavefront(b, cam, 48, 40, 8, max_depth=cfg["max_depth"], seed=5, sort=sort, use_graph=graph,
-                                  capacity=cap, occupancy=occ, streams=streams)
+                                  capacity=cap, occupancy=occ, streams=streams, block=block)
+    assert np.allclose(a, b, rtol=1e-4, atol=1e-5), np.abs(a - b).max()
+
+
+@pytest.mark.parametrize("knn_wave_max", [1, 1 << 30])
+def test_wavefront_knn_search_regimes_equal_megakernel(gpu_renderer, knn_wave_max):
+    """The dense k-NN queries searched one per lane (heap in LDS) and one per
+    wave (sorted set across the lanes) give the megakernel's frame."""
+    sc, cam, cfg = scenes.gem_caustics(40, 32, num_photons=100_000, knn=64, gather_radius=0.15)
+    gpu_renderer.upload(sc)
+    lq = cfg["light_quad"]
+    gpu_renderer.photon_build_from_quad(lq["q"], lq["u"], lq["v"], cfg["light_color"], 100_000, 0.15, knn=64)
+    a = np.zeros((32, 40, 3), np.float32)
+    b = np.zeros((32, 40, 3), np.float32)
+    gpu_renderer.render_accumulate(a, cam, 40, 32, 8, max_depth=24, seed=6, photon_map=True)
+    gpu_renderer.render_wavefront(b, cam, 40, 32, 8, max_depth=24, seed=6, photon_map=True, capacity=4096,
+                                  knn_wave_max=knn_wave_max, block=64)
+    assert (a.sum(-1) > 0).mean() > 0.3
     assert np.allclose(a, b, rtol=1e-4, atol=1e-5), np.abs(a - b).max()
 
 
