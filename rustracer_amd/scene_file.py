@@ -14,11 +14,17 @@ an optional visible solid or gradient backdrop); every [[objects]] shape
 mesh from OBJ, constant / noise volume sphere / box) with scale / rotate /
 rotate_y / spin / translate; every material (lambertian, checker, noise,
 image, metal, dielectric, spectral_dielectric, diffuse_light, pbr,
-textured_pbr, spectral_metal, pearl, sss); [[lights]] quads (temp_k ->
-blackbody); [caustics].  Not covered: [[gems]] grading-report ingestion
-(ValueError) and camera animation ([animation] is only flagged in `info`).
+textured_pbr, spectral_metal, pearl, sss); [[gems]] grading reports
+(`build_gem`, src/scene_file.rs:793-897: brilliant and every step-family cut
+preset, diamond / sapphire / ruby / emerald and the checker / opaque debug
+materials, colour-grade tint) with the `--spin` override of `load_with`
+(src/scene_file.rs:625-648); [[lights]] quads (temp_k -> blackbody);
+[caustics].  Emissive objects join the light list as the reference decides
+(light_area() > 0, i.e. Material::is_emissive, src/scene_file.rs:1040-1045).
+Not covered: camera animation ([animation] is only flagged in `info`).
 Images load with Pillow.
 """
+import math
 import os
 
 import numpy as np
@@ -253,15 +259,184 @@ def _background(b, base):
     raise ValueError(f"unknown background type {kind!r}")
 
 
-def load(path):
-    """Scene, camera(aspect) and info of a reference TOML scene file."""
+def _lum(c):
+    """Color::luminance (src/vec3.rs:33), in f32."""
+    return f32(f32(f32(0.2126) * f32(c[0])) + f32(f32(0.7152) * f32(c[1]))) + f32(f32(0.0722) * f32(c[2]))
+
+
+def _srgb_decode(v):
+    x = f32(v) / f32(255.0)
+    return x / f32(12.92) if x <= f32(0.04045) else f32(((x + f32(0.055)) / f32(1.055)) ** f32(2.4))
+
+
+def is_emissive(mat):
+    """Material::is_emissive (src/material.rs:22-24, 67, 926-928): a diffuse
+    light whose texture at (u, v) = (0.5, 0.5), p = 0 has positive luminance, a
+    blackbody light of positive intensity, a PBR material with positive
+    emission strength and emission luminance; nothing else."""
+    if isinstance(mat, S.BlackbodyLight):
+        return mat.intensity > 0.0
+    if isinstance(mat, S.PbrMaterial):
+        return mat.emission_strength > 0.0 and _lum(mat.emission) > 0.0
+    if isinstance(mat, S.DiffuseLight):
+        t = mat.texture
+        if isinstance(t, S.SolidTexture):
+            c = t.color
+        elif isinstance(t, S.CheckerTexture):
+            c = t.even                                     # floor(scale * 0) sums to 0: even
+        elif isinstance(t, S.ImageTexture):
+            h, w = t.image.shape[:2]
+            x = min(int(f32(0.5) * f32(w)), w - 1)
+            y = min(int(f32(0.5) * f32(h)), h - 1)
+            c = [_srgb_decode(k) for k in t.image[y, x]]
+        elif isinstance(t, S.MottleTexture):
+            c = t.lo                                      # turb(0) = 0: the low colour
+        else:                                             # noise: 0.5 (1 + sin 0) at p = 0
+            c = (0.5, 0.5, 0.5)
+        return _lum(c) > 0.0
+    return False
+
+
+# Gem material presets (src/scene_file.rs:681-697): Cauchy B, C (um^2) and a
+# body-colour Beer-Lambert sigma per scene unit at scale_mm = 0.001.
+GEM_PRESETS = {
+    "diamond": (2.351, 0.0230, (0.0, 0.0, 0.0)),
+    "sapphire": (1.741, 0.0094, (650.0, 320.0, 25.0)),
+    "ruby": (1.741, 0.0094, (25.0, 550.0, 450.0)),
+    "emerald": (1.556, 0.0073, (480.0, 25.0, 380.0)),
+}
+
+# Step-family cut presets (src/scene_file.rs:741-768): outline, culet %,
+# crown steps, pavilion steps, step spread, default chevron rows.
+STEP_CUT_PRESETS = {
+    "step": (("rect", {"corner_pct": 12.0}), 8.0, 3, 3, 0.25, 0),
+    "princess": (("rect", {"corner_pct": 0.0}), 0.0, 4, 4, 0.15, 3),
+    "asscher": (("rect", {"corner_pct": 22.0}), 8.0, 3, 3, 0.20, 0),
+    "radiant": (("rect", {"corner_pct": 12.0}), 0.0, 4, 4, 0.15, 2),
+    "baguette": (("rect", {"corner_pct": 4.0}), 8.0, 2, 2, 0.10, 0),
+    "cushion": (("rounded_rect", {"corner_pct": 30.0, "segs": 4}), 8.0, 3, 3, 0.15, 0),
+    "trillion": (("triangle", {}), 0.0, 4, 4, 0.15, 2),
+    "marquise": (("pointed_oval", {"segs": 24, "pear": False}), 0.0, 4, 4, 0.15, 2),
+    "pear": (("pointed_oval", {"segs": 24, "pear": True}), 0.0, 4, 4, 0.15, 2),
+}
+
+_DEG = f32(57.2957795130823208767981548141051703)     # f32::to_degrees
+
+
+def gem_absorption(g, preset_abs):
+    """gem_absorption (src/scene_file.rs:717-735): preset body colour plus the
+    D..Z grade tint, rescaled by 0.001 / scale_mm, plus an explicit
+    `absorption` (per scene unit, not rescaled)."""
+    grade = str(g.get("color_grade", "D")).strip().upper()[:1]
+    sigma = {"D": 0.0, "E": 0.0, "F": 0.0, "G": 50.0, "H": 80.0, "I": 120.0, "J": 170.0, "K": 230.0, "L": 300.0,
+             "M": 300.0}.get(grade, 400.0)
+    tint = [f32(c) * f32(sigma) for c in (1.0, 0.75, 0.35)]
+    fix = f32(0.001) / f32(g.get("scale_mm", 0.001))
+    extra = g.get("absorption") or (0.0, 0.0, 0.0)
+    return tuple(float(f32(f32(f32(a) + t) * fix) + f32(e)) for a, t, e in zip(preset_abs, tint, extra))
+
+
+def build_gem(g, where="gem"):
+    """build_gem (src/scene_file.rs:793-897): a [[gems]] grading report ->
+    a Diamond (cut "brilliant") or StepCut (step-family presets) with its
+    spectral-dielectric preset, placed by scale, Euler rotation about the
+    gem centre (rotate + rotate_y + spin) and translate."""
+    name = f"{where} '{g.get('report', '')}'"
+    scale_mm = f32(g.get("scale_mm", 0.001))
+    if g.get("measurements_mm") is not None:
+        m = [f32(x) for x in g["measurements_mm"]]
+        if m[0] <= 0 or m[1] <= 0:
+            raise ValueError(f"{name}: measurements_mm must be positive")
+        l, w = (m[0], m[1]) if m[0] >= m[1] else (m[1], m[0])
+        oval = l / w if (l - w) / w > f32(0.02) else f32(1.0)
+        w_mm = w
+    elif g.get("carat") is not None:
+        ct = f32(g["carat"])
+        if ct <= 0:
+            raise ValueError(f"{name}: carat must be positive")
+        w_mm, oval = f32((ct / f32(0.00355)) ** (1.0 / 3.0)), f32(1.0)
+    else:
+        raise ValueError(f"{name}: provide either carat or measurements_mm")
+    if scale_mm <= 0:
+        raise ValueError(f"{name}: scale_mm must be positive")
+    radius = f32(0.5) * w_mm * scale_mm
+    table_pct = f32(g.get("table_pct", 53.0))
+    if not (0.0 < table_pct < 100.0):
+        raise ValueError(f"{name}: table_pct must be in (0, 100), got {table_pct}")
+    girdle_pct = f32(g.get("girdle_pct", 2.0))
+    if not (math.isfinite(girdle_pct) and 0.0 <= girdle_pct < 100.0):
+        raise ValueError(f"{name}: girdle_pct must be in [0, 100), got {girdle_pct}")
+    table_r = (table_pct / f32(100.0)) * radius
+    crown = f32(g.get("crown_angle", 34.5))
+    if g.get("crown_pct") is not None:
+        crown = f32(math.atan(f32(f32(f32(g["crown_pct"]) / f32(100.0)) * w_mm * scale_mm) / (radius - table_r))) * _DEG
+    pav = f32(g.get("pavilion_angle", 40.75))
+    if g.get("pavilion_pct") is not None:
+        pav = f32(math.atan(f32(f32(f32(g["pavilion_pct"]) / f32(100.0)) * w_mm * scale_mm) / radius)) * _DEG
+    if not (0.0 < crown < 90.0) or not (0.0 < pav < 90.0):
+        raise ValueError(f"{name}: crown / pavilion angle must be in (0, 90) deg, got {crown}, {pav}")
+    finish = str(g.get("girdle_finish") or "").lower()
+    if finish not in ("", "faceted", "polished", "bruted"):
+        raise ValueError(f"{name}: girdle_finish must be faceted|polished|bruted, got {finish!r}")
+    facets = g.get("girdle_facets") or (64 if oval > f32(1.05) else 16)
+    polish = float(g.get("polish", 0.02))
+    mname = g.get("material") or "diamond"
+    if mname == "checker":
+        mat = S.Checker((0.9, 0.9, 0.9), (0.05, 0.05, 0.05), 12.0)
+    elif mname == "opaque":
+        mat = S.Lambertian((0.8, 0.8, 0.8))
+    else:
+        if mname not in GEM_PRESETS:
+            raise ValueError(f"{name}: unknown gem material '{mname}' (expected diamond, sapphire, ruby or emerald)")
+        b, c, pabs = GEM_PRESETS[mname]
+        if g.get("cauchy_b") is not None:
+            b = g["cauchy_b"]
+        if g.get("cauchy_c") is not None:
+            c = g["cauchy_c"]
+        mat = S.SpectralDielectric(b, c, gem_absorption(g, pabs))
+    center = _p3(g["center"])
+    cut = g.get("cut") or "brilliant"
+    if cut == "brilliant":
+        h = S.Diamond(center, float(radius), mat, table_pct=float(table_pct), crown_angle=float(crown),
+                      pavilion_angle=float(pav), oval_ratio=float(oval), girdle_facets=int(facets),
+                      girdle_pct=float(girdle_pct), faceted=finish == "faceted", polish=polish)
+    else:
+        if cut not in STEP_CUT_PRESETS:
+            raise ValueError(f"{name}: unknown gem cut '{cut}' (expected brilliant, step, princess, asscher, radiant, "
+                             "baguette, cushion, trillion, marquise or pear)")
+        outline, culet, cs, ps, spread, chev = STEP_CUT_PRESETS[cut]
+        chev = g.get("chevrons") if g.get("chevrons") is not None else chev
+        h = S.StepCut(center, float(radius * oval), float(radius), mat, table_pct=float(table_pct),
+                      crown_angle=float(crown), pavilion_angle=float(pav), outline=outline, crown_steps=cs,
+                      pavilion_steps=ps, culet_pct=culet, step_spread=spread, chevrons=int(chev),
+                      girdle_pct=float(girdle_pct), polish=polish)
+    if g.get("scale") is not None:
+        h = h.scaled(g["scale"])
+    ang = [float(f32(a)) for a in (g.get("rotate") or [0.0, 0.0, 0.0])]
+    ang[1] = float(f32(f32(ang[1]) + f32(g.get("rotate_y") or 0.0)) + f32(g.get("spin") or 0.0))
+    if any(a != 0.0 for a in ang):
+        h = h.rotated(tuple(ang), pivot=center)
+    if g.get("translate") is not None:
+        h = h.translated(_p3(g["translate"]))
+    return h
+
+
+def _light_shape(h):
+    """NEE light sampling exists on the device for quads, spheres and disks."""
+    return isinstance(h, (S.Quad, S.Sphere, S.Disk))
+
+
+def load(path, gem_spin=None):
+    """Scene, camera(aspect) and info of a reference TOML scene file;
+    `gem_spin` overrides every [[gems]] spin (load_with, src/scene_file.rs:
+    625-648)."""
     import tomli
     with open(path, "rb") as fh:
         doc = tomli.load(fh)
     base = os.path.dirname(os.path.abspath(path))
-    if doc.get("gems"):
-        raise ValueError("[[gems]] (grading-report ingestion) is not supported: describe the stone as a "
-                         "diamond or step_cut object")
+    if gem_spin is not None:
+        for gm in doc.get("gems", []):
+            gm["spin"] = gem_spin
     objects, lights = [], []
     for i, o in enumerate(doc.get("objects", [])):
         mat = _material(o["material"], base) if o.get("material") else None
@@ -275,8 +450,13 @@ def load(path):
         if o.get("translate") is not None:
             h = h.translated(_p3(o["translate"]))
         objects.append(h)
-        if isinstance(h.mat, (S.DiffuseLight, S.BlackbodyLight)) and isinstance(h, (S.Quad, S.Sphere, S.Disk)):
-            lights.append(h)                     # emissive NEE-sampleable objects join the light list
+        if h.mat is not None and is_emissive(h.mat):     # light_area() > 0 joins the NEE list
+            if not _light_shape(h):
+                raise ValueError(f"objects[{i}]: an emissive {type(h).__name__} would be an NEE light, which the "
+                                 "device samples for quads, spheres and disks only")
+            lights.append(h)
+    for i, gm in enumerate(doc.get("gems", [])):
+        objects.append(build_gem(gm, f"gems[{i}]"))
     for lc in doc.get("lights", []):
         mat = (S.BlackbodyLight(lc["temp_k"], lc.get("intensity", 1.0)) if lc.get("temp_k") is not None
                else S.DiffuseLight(_p3(lc["color"])))

@@ -245,3 +245,44 @@ def test_progressive_wavefront_against_the_oracle_loop(gpu_renderer, adaptive):
     assert close.mean() > 0.95, close.mean()
     if adaptive:
         assert np.mean(dev["converged"] == host["converged"]) > 0.97
+
+
+def test_reference_bench_scene_on_the_device(gpu_renderer):
+    """The reference's --bench scene (scenes/bench.toml, a fixture here):
+    [[gems]] diamond and princess sapphire, the OBJ mesh, the blackbody panel
+    and the dusk sun.  rebuild_caustics on the device takes the sun branch and
+    stores the oracle's photon set (src/photon.rs:324-343's bounds: more than
+    1,000 finite photons, light under the diamond); with the oracle's photons
+    imported the frame matches the oracle on the same streams and within the
+    3-sigma gate, megakernel and wavefront, at the reference's depth 50."""
+    import os
+    from rustracer_amd import scene_file as SF
+    r = gpu_renderer
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "bench_scene", "bench.toml")
+    sc, camera, info = SF.load(path)
+    r.upload(sc)
+    orc = O.OracleScene(sc)
+    n = r.rebuild_caustics(info)
+    nc = orc.photon_build(Renderer.caustic_params(info))
+    assert n > 1_000 and abs(n - nc) <= 0.02 * nc + 5, (n, nc)
+    ph, pc = r.photon_export(), orc.photon_dump(nc)
+    assert np.isfinite(ph).all()
+    a = np.sort(ph[:, 0] * 1e3 + ph[:, 2])
+    b = np.sort(pc[:, 0] * 1e3 + pc[:, 2])
+    m = min(len(a), len(b))
+    assert np.mean(np.isclose(a[:m], b[:m], atol=1e-3)) > 0.95
+    irr = r.photon_irradiance(np.array([[0.0, 0.0, 2.5, 0.0, 1.0, 0.0]], np.float32))
+    assert np.isfinite(irr).all() and irr[0, 0] > 0.0
+    r.photon_import(pc, info["gather_radius"])
+    W, H, D = 48, 32, 50
+    cam = camera(W / H)
+    g = np.zeros((H, W, 3), np.float32)
+    r.render_accumulate(g, cam, W, H, 2, max_depth=D, seed=1, photon_map=True)
+    c = orc.render_accumulate(cam, Renderer.params(W, H, max_depth=D, seed=1, use_photons=True), 2, mode=1)
+    assert same_stream_fraction(g, c) > 0.9
+    w = np.zeros((H, W, 3), np.float32)
+    r.render_wavefront(w, cam, W, H, 2, max_depth=D, seed=1, photon_map=True, capacity=4096)
+    assert np.allclose(w, g, rtol=1e-4, atol=1e-4), np.abs(w - g).max()
+    ref = orc_frames(orc, cam, W, H, D, seed=9, photons=True)
+    IS.gate(dev_frames(r, cam, W, H, D, seed=2, photons=True), ref)
+    IS.gate(dev_frames(r, cam, W, H, D, seed=3, photons=True, wavefront=True, capacity=16384, streams=2), ref)

@@ -167,3 +167,25 @@ def test_rebuild_caustics_quad_fallback_and_none():
         pp.origin[k], pp.u[k], pp.v[k], pp.color[k] = q.q[k], q.u[k], q.v[k], 15.0
     e = orc.photon_emitter(pp)
     assert e.emitter == F.EMIT_QUAD and list(e.origin) == list(pp.origin)
+
+
+def test_gem_scene_photon_map_is_populated_and_finite():
+    """src/photon.rs:324-343 on the reference's own bench scene (a fixture of
+    tests/golden): rebuild_caustics under the dusk sun stores more than 1,000
+    finite photons, and the floor right under the diamond (girdle centre
+    (0, 1.30, 2.5)) receives positive irradiance."""
+    import os
+    from rustracer_amd import Renderer
+    from rustracer_amd import scene_file as SF
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "bench_scene", "bench.toml")
+    sc, _, info = SF.load(path)
+    orc = O.OracleScene(sc)
+    pp = Renderer.caustic_params(info)
+    e = orc.photon_emitter(pp)
+    assert e.emitter == F.EMIT_SUN_DISK                # Physical background: the sun branch
+    n = orc.photon_build(pp)
+    assert n > 1_000, n
+    ph = orc.photon_dump(n)
+    assert np.isfinite(ph).all()
+    irr = orc.photon_irradiance(np.array([[0.0, 0.0, 2.5, 0.0, 1.0, 0.0]], np.float32))
+    assert np.isfinite(irr).all() and irr[0, 0] > 0.0, irr

@@ -120,10 +120,140 @@ def test_focus_distance_defaults_to_the_look_distance(tmp_path):
     assert abs(np.linalg.norm(c.horizontal) - focus * 2 * np.tan(np.radians(22.5))) < 1e-4
 
 
-def test_gems_are_refused(tmp_path):
-    p = tmp_path / "g.toml"
-    p.write_text('[camera]\nlook_from = [0, 0, -1]\nlook_at = [0, 0, 0]\n[[gems]]\nreport = "x"\ncarat = 1.0\n')
-    with pytest.raises(ValueError, match="gems"):
+BENCH_TOML = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "bench_scene", "bench.toml")
+
+
+def gem(text):
+    import tomli
+    return tomli.loads(text)["gems"][0]
+
+
+def test_gem_defaults_to_tolkowsky_cut():
+    """src/scene_file.rs:1441-1450: the report's defaults are table 53 %,
+    crown 34.5 deg, pavilion 40.75 deg (the built stone carries them)."""
+    h = SF.build_gem(gem("[[gems]]\ncarat = 0.5\ncenter = [0,0,0]\n"))
+    want = S.Diamond((0, 0, 0), float(np.float32(0.5) * np.float32((0.5 / 0.00355) ** (1 / 3)) * np.float32(0.001)),
+                     None, table_pct=53.0, crown_angle=34.5, pavilion_angle=40.75, girdle_pct=2.0, polish=0.02)
+    assert np.allclose(h.planes, want.planes, rtol=1e-5, atol=1e-9)
+
+
+def test_gem_material_presets_have_distinct_ior_and_absorption():
+    """src/scene_file.rs:1452-1467."""
+    d, s_, r, e = (SF.GEM_PRESETS[k] for k in ("diamond", "sapphire", "ruby", "emerald"))
+    assert abs(d[0] - 2.351) < 1e-6 and sum(x * x for x in d[2]) == 0.0
+    nf = float(O.eval_fn("cauchy", [486.1, d[0], d[1]], nout=1))
+    nc = float(O.eval_fn("cauchy", [656.3, d[0], d[1]], nout=1))
+    assert abs(nf - nc - 0.044) < 0.002
+    assert abs(s_[0] - 1.741) < 1e-6 and abs(e[0] - 1.556) < 1e-6
+    assert s_[2][0] > 100 and s_[2][2] < 100 and r[2][1] > 100 and r[2][0] < 100
+    assert e[2][1] < 100 and e[2][0] > 100 and e[2][2] > 100
+    with pytest.raises(ValueError, match="material"):
+        SF.build_gem(gem('[[gems]]\ncarat = 1.0\nmaterial = "topaz"\ncenter = [0,0,0]\n'))
+
+
+def _extent(h):
+    lo, hi = h.bbox
+    return float(hi[0] - lo[0]), float(hi[2] - lo[2])
+
+
+def test_gem_step_cut_uses_measurements_as_rectangle():
+    """src/scene_file.rs:1470-1482."""
+    w, d = _extent(SF.build_gem(gem('[[gems]]\nmeasurements_mm = [9.0, 6.0, 3.9]\ncut = "step"\nmaterial = "emerald"\n'
+                                    'center = [0,0,0]\n')))
+    assert abs(w - 0.009) < 1e-6 and abs(d - 0.006) < 1e-6
+
+
+@pytest.mark.parametrize("extra", ['material = "sapphire"\n', "chevrons = 2\n"])
+def test_gem_princess_cut_is_square(extra):
+    """src/scene_file.rs:1484-1508."""
+    w, d = _extent(SF.build_gem(gem('[[gems]]\nmeasurements_mm = [7.0, 7.0, 4.2]\ncut = "princess"\n' + extra +
+                                    'center = [0,0,0]\n')))
+    assert abs(w - 0.007) < 1e-6 and abs(d - 0.007) < 1e-6
+
+
+def test_gem_trillion_and_marquise_build():
+    """src/scene_file.rs:1510-1531."""
+    w, d = _extent(SF.build_gem(gem('[[gems]]\nmeasurements_mm = [7.0, 7.0, 4.2]\ncut = "trillion"\ncenter = [0,0,0]\n')))
+    a = 0.0035
+    assert abs(w - 2.0 * 0.866 * a) < 1e-5 and abs(d - 1.5 * a) < 1e-5
+    w, d = _extent(SF.build_gem(gem('[[gems]]\nmeasurements_mm = [10.0, 4.0, 3.0]\ncut = "marquise"\ncenter = [0,0,0]\n')))
+    assert abs(w - 0.010) < 1e-6 and abs(d - 0.004) < 1e-6
+
+
+def test_gem_unknown_cut_or_material_errors():
+    """src/scene_file.rs:1533-1548."""
+    with pytest.raises(ValueError, match="cut"):
+        SF.build_gem(gem('[[gems]]\ncarat = 1.0\ncut = "heart"\ncenter = [0,0,0]\n'))
+    with pytest.raises(ValueError, match="material"):
+        SF.build_gem(gem('[[gems]]\ncarat = 1.0\nmaterial = "topaz"\ncenter = [0,0,0]\n'))
+
+
+def test_gem_crown_and_pavilion_percentages_and_placement():
+    """IGI percentages become facet angles through the round-space geometry
+    (src/scene_file.rs:818-828): crown height = crown_pct of the minor axis
+    over the crown's run, pavilion depth = pavilion_pct over the radius; the
+    stone spins about its own centre and keeps its size."""
+    g = gem('[[gems]]\nmeasurements_mm = [9.0, 9.0, 5.6]\ntable_pct = 57.0\ncrown_pct = 14.5\npavilion_pct = 43.0\n'
+            'center = [0.0, 1.3, 2.5]\nscale_mm = 0.3\n')
+    h = SF.build_gem(g)
+    r = 0.5 * 9.0 * 0.3
+    lo, hi = h.bbox
+    # table at crown height above the girdle band, culet at pavilion depth below
+    girdle = 0.02 * 2 * r
+    assert abs((hi[1] - 1.3) - (0.5 * girdle + 0.145 * 9.0 * 0.3)) < 1e-4
+    assert abs((1.3 - lo[1]) - (0.5 * girdle + 0.43 * 9.0 * 0.3)) < 1e-4
+    g["spin"] = 30.0
+    sp = SF.build_gem(g)
+    assert len(sp.xforms) == 3 and np.allclose(sp.xforms[0][1], [0.0, -1.3, -2.5])
+
+
+def test_gem_grade_tint_and_scale_rescale():
+    """gem_absorption (src/scene_file.rs:717-735): D..F clear, G..Z a
+    yellow-brown tint, both rescaled by 0.001 / scale_mm; an explicit
+    absorption adds unscaled."""
+    assert SF.gem_absorption({"color_grade": "E", "scale_mm": 0.001}, (0, 0, 0)) == (0.0, 0.0, 0.0)
+    a = SF.gem_absorption({"color_grade": "K", "scale_mm": 0.01, "absorption": [1.0, 2.0, 3.0]}, (650.0, 320.0, 25.0))
+    want = [(650.0 + 230.0) * 0.1 + 1.0, (320.0 + 172.5) * 0.1 + 2.0, (25.0 + 80.5) * 0.1 + 3.0]
+    assert np.allclose(a, want, rtol=1e-6)
+
+
+def test_the_reference_bench_scene_loads():
+    """scenes/bench.toml (the reference's --bench scene, kept as a fixture
+    with its OBJ): 11 room quads, the basketball mesh, the brilliant diamond
+    and the princess sapphire, the blackbody ceiling panel; caustics on."""
+    sc, camera, info = SF.load(BENCH_TOML)
+    kinds = [type(o).__name__ for o in sc.objects]
+    assert kinds.count("Quad") == 12 and kinds.count("Mesh") == 1 and kinds.count("Convex") == 2
+    assert len(sc.lights) == 1 and isinstance(sc.lights[0].mat, S.BlackbodyLight)
+    assert info["caustics"] and info["num_photons"] == 500_000 and info["gather_radius"] == 0.5
+    sapph = sc.objects[kinds.index("Convex") + 1]
+    assert isinstance(sapph.mat, S.SpectralDielectric) and abs(sapph.mat.cauchy_b - 1.741) < 1e-6
+    # the sapphire preset's body colour rescaled to scale_mm 0.3 (grade D: no tint)
+    assert np.allclose(sapph.mat.absorption, np.array([650.0, 320.0, 25.0]) * (0.001 / 0.3), rtol=1e-5)
+    O.OracleScene(sc)
+    # load_with's spin override turns every stone about its own centre
+    sc2, _, _ = SF.load(BENCH_TOML, gem_spin=20.0)
+    d2 = [o for o in sc2.objects if type(o).__name__ == "Convex"]
+    assert all(len(o.xforms) == 3 for o in d2)
+
+
+def test_emissive_objects_join_the_light_list(tmp_path):
+    """Objects with light_area() > 0 become NEE lights (src/scene_file.rs:
+    1040-1045): an emissive PBR sphere joins, a zero-colour diffuse light does
+    not; an emissive mesh would need mesh light sampling and is refused."""
+    base = '[camera]\nlook_from = [0, 0, -3]\nlook_at = [0, 0, 0]\n'
+    p = tmp_path / "e.toml"
+    p.write_text(base + '[[objects]]\nshape = { type = "sphere", center = [0, 0, 0], radius = 0.5 }\n'
+                 'material = { type = "pbr", albedo = [0.5, 0.5, 0.5], emission = [1.0, 0.5, 0.2], emission_strength = 3.0 }\n'
+                 '[[objects]]\nshape = { type = "quad", corner = [-1, 1, -1], u = [2, 0, 0], v = [0, 0, 2] }\n'
+                 'material = { type = "diffuse_light", color = [0.0, 0.0, 0.0] }\n')
+    sc, _, _ = SF.load(str(p))
+    assert len(sc.lights) == 1 and isinstance(sc.lights[0].mat, S.PbrMaterial)
+    obj = tmp_path / "t.obj"
+    obj.write_text("v 0 0 0\nv 1 0 0\nv 0 1 0\nf 1 2 3\n")
+    p.write_text(base + f'[[objects]]\nshape = {{ type = "mesh", path = "{obj}" }}\n'
+                 'material = { type = "diffuse_light", color = [4.0, 4.0, 4.0] }\n')
+    with pytest.raises(ValueError, match="NEE light"):
         SF.load(str(p))
 
 
