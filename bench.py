@@ -41,12 +41,27 @@ HBM_PEAK_GBS = 8000.0
 PRIM_BYTES = {"box": 80.0, "mesh": 48.0, "gem": 16.0 * 57}
 
 
+BENCH_TOML = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tests", "golden", "bench_scene", "bench.toml")
+
+
+def reference_bench_scene():
+    """The reference's own --bench workload (run_bench, src/main.rs:84-117):
+    scenes/bench.toml (room, OBJ mesh, [[gems]] diamond and sapphire, dusk
+    sun, blackbody panel) at 1200 x 800, 128 samples, depth 50 (ray_color's
+    MAX_DEPTH), the photon map of rebuild_caustics."""
+    from rustracer_amd import scene_file
+    scene, camera, info = scene_file.load(BENCH_TOML)
+    W, H = 1200, 800
+    return scene, camera(W / H), dict(width=W, height=H, spp=128, max_depth=50, caustics=info)
+
+
 def workloads():
     from rustracer_amd import scenes
     return {
         "box": lambda: scenes.box_room(256, 256),
         "mesh": lambda: scenes.mesh_envmap(1280, 720, level=8),
         "gem": lambda: scenes.gem_caustics(1280, 720, **scenes.GEM_BENCH),
+        "bench_toml": reference_bench_scene,
     }
 
 
@@ -70,9 +85,15 @@ def measure(name, args, with_cpu):
     r.upload(scene)
     upload_s = time.time() - t0
     stats = r.stats()
-    photons = "num_photons" in cfg
+    photons = "num_photons" in cfg or "caustics" in cfg
     ph = None
-    if photons:
+    if "caustics" in cfg:
+        t0 = time.time()
+        stored = r.rebuild_caustics(cfg["caustics"])
+        photons = stored > 0
+        ph = {"traced": cfg["caustics"]["num_photons"], "stored": stored, "build_s": round(time.time() - t0, 3),
+              "emitter": "rebuild_caustics"}
+    elif photons:
         lq = cfg["light_quad"]
         t0 = time.time()
         stored = r.photon_build_from_quad(lq["q"], lq["u"], lq["v"], cfg["light_color"], cfg["num_photons"],
@@ -664,7 +685,9 @@ def cpu_baseline(scene, cam, cfg, seconds, photons=False):
     from rustracer_amd import _ffi as F
     cores = O.threads()
     orc = O.OracleScene(scene)
-    if photons:
+    if photons and "caustics" in cfg:
+        orc.photon_build(Renderer.caustic_params(cfg["caustics"]), nthreads=cores)
+    elif photons:
         lq = cfg["light_quad"]
         pp = F.rt_photon_params()
         pp.emitter = F.EMIT_QUAD
@@ -693,7 +716,7 @@ def main():
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--config", default="mesh", choices=sorted(workloads()) + ["batch", "adjoint", "rmse", "output"])
-    ap.add_argument("--also", default="gem,batch,adjoint,rmse,output",
+    ap.add_argument("--also", default="gem,batch,adjoint,rmse,output,bench_toml",
                     help="comma-separated extra workloads ('' for none): " + ", ".join(sorted(workloads())) +
                          ", batch, adjoint, rmse, output")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
