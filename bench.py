@@ -487,21 +487,28 @@ def cpu_adjoint_baseline(scene, cam, cfg, seconds):
 
 
 def measure_rmse(args, with_cpu=True):
-    """"At fixed RMSE": the gem with caustics at 320x180, depth 24, its 2M-photon
-    map built once on the host (the checker) and imported into the device so
-    every renderer gathers from the same photons.  A 4096-spp image of the
-    checker (reference row streams) is the target; the RMSE (linear HDR, mean
-    radiance per pixel) of 64-spp frames of the plain megakernel, the
-    wavefront renderer and the checker itself (disjoint sample indices) are
-    compared.  Equal RMSE at equal spp makes samples/s the figure of merit."""
+    """"At fixed RMSE" on the headline gem (Sellmeier, k = 64 within 0.15) at
+    640 x 360, depth 24.  Target: a 4096-spp image of the oracle (reference
+    row streams, its own 2M-photon map).  The device renders with photons it
+    traced itself (rt_photon_build).  RMSE is linear HDR over mean radiance.
+    Two comparisons of 64-spp frames:
+      * paired: frame k of the wavefront renderer against the oracle's frame
+        on the same per-pixel streams (mode 1, same seed).  Both estimators
+        draw the same numbers, so the frame-to-frame spread of the caustics
+        cancels and the ratio of mean squared errors is pinned to a fraction
+        of a percent;
+      * independent: the oracle's own row streams at disjoint sample indices,
+        the reference's estimator as the reference draws it.
+    Equal RMSE at equal spp makes samples/s the figure of merit."""
     import numpy as np
     import torch
     from oracle import oracle as O
     from rustracer_amd import Renderer, scenes
     from rustracer_amd import _ffi as F
 
-    W, H, spp, ref_spp = 320, 180, 64, 4096
+    W, H, spp, ref_spp, K = 640, 360, 64, 4096, 8
     sc, cam, cfg = scenes.gem_caustics(W, H, **scenes.GEM_BENCH)
+    D = cfg["max_depth"]
     cores = O.threads()
     orc = O.OracleScene(sc)
     lq = cfg["light_quad"]
@@ -512,51 +519,52 @@ def measure_rmse(args, with_cpu=True):
     pp.num_photons, pp.gather_radius, pp.knn = cfg["num_photons"], cfg["gather_radius"], cfg["knn"]
     t0 = time.time()
     nph = orc.photon_build(pp, nthreads=cores)
-    photons = orc.photon_dump(nph)
-    p = Renderer.params(W, H, sample_idx=0, max_depth=cfg["max_depth"], use_photons=True)
+    p = Renderer.params(W, H, sample_idx=0, max_depth=D, use_photons=True)
     ref = orc.render_accumulate(cam, p, ref_spp, mode=0, nthreads=cores).astype(np.float64) / ref_spp
     ref_s = time.time() - t0
-    log(f"[rmse] reference: {ref_spp} spp on {cores} host threads in {ref_s:.1f}s, {nph} photons")
+    log(f"[rmse] reference: {ref_spp} spp at {W}x{H} on {cores} host threads in {ref_s:.1f}s, {nph} photons")
 
     def mse(img):
-        return float(((img.astype(np.float64) / spp - ref) ** 2).mean())
+        return float(((np.asarray(img, np.float64) / spp - ref) ** 2).mean())
 
-    # the gem's caustic noise is heavy-tailed: one 64-spp frame's RMSE varies
-    # by ~10 % between independent draws, so mean squared errors over K
-    # independent frames are compared, with their standard errors
-    K = 16
     r = Renderer(0).upload(sc)
-    r.photon_import(photons, cfg["gather_radius"])
-    r.photon_set_gather(cfg["gather_radius"], cfg["knn"])
+    nd = r.photon_build(pp)                          # the device traces its own map
     a = torch.zeros((H, W, 3), dtype=torch.float32, device="cuda")
-    m_wf, m_plain = [], []
+    m_wf, m_pair, m_ind = [], [], []
     for k in range(K):
-        r.render_wavefront(a, cam, W, H, spp, max_depth=cfg["max_depth"], photon_map=True, seed=11 + k)
+        r.render_wavefront(a, cam, W, H, spp, max_depth=D, photon_map=True, seed=11 + k)
         m_wf.append(mse(a.cpu().numpy()))
-    for k in range(2):
-        r.render_accumulate(a, cam, W, H, spp, max_depth=cfg["max_depth"], photon_map=True, seed=11 + k)
-        m_plain.append(mse(a.cpu().numpy()))
+    r.render_accumulate(a, cam, W, H, spp, max_depth=D, photon_map=True, seed=11)
+    m_plain = mse(a.cpu().numpy())
     r.close()
-    m_host = []
     for k in range(K):
-        pc = Renderer.params(W, H, sample_idx=100000 + 1000 * k, max_depth=cfg["max_depth"], use_photons=True)
-        m_host.append(mse(orc.render_accumulate(cam, pc, spp, mode=0, nthreads=cores)))
-    mw, mh = np.array(m_wf), np.array(m_host)
-    e_wf, e_host = float(np.sqrt(mw.mean())), float(np.sqrt(mh.mean()))
-    ratio = e_wf / e_host
-    # delta method on the ratio of square roots of two independent means
-    se = 0.5 * ratio * float(np.sqrt(mw.var(ddof=1) / K / mw.mean() ** 2 + mh.var(ddof=1) / K / mh.mean() ** 2))
-    log(f"[rmse] {spp} spp vs {ref_spp}-spp target over {K} frames each: wavefront {e_wf:.5f}, host {e_host:.5f} "
-        f"(ratio {ratio:.4f} +- {se:.4f}); plain == wavefront per frame: {np.allclose(m_plain, m_wf[:2], rtol=1e-4)}")
-    return {"value": ratio, "unit": "RMSE ratio (wavefront / host) at equal spp", "ms_per_step": None,
-            "config": {"workload": "gem_caustics_320x180_rmse", "width": W, "height": H, "spp": spp,
-                       "reference_spp": ref_spp, "frames": K, "photons": nph,
-                       "rmse": {"wavefront": e_wf, "host": e_host,
-                                "plain_first_frames": [float(np.sqrt(x)) for x in m_plain]},
-                       "rmse_per_frame": {"wavefront": [float(np.sqrt(x)) for x in m_wf],
-                                          "host": [float(np.sqrt(x)) for x in m_host]},
-                       "ratio_se": se, "within_2pct": bool(abs(ratio - 1.0) <= 0.02),
-                       "consistent_2se": bool(abs(ratio - 1.0) <= 2 * se), "reference_seconds": round(ref_s, 1)},
+        pc = Renderer.params(W, H, max_depth=D, use_photons=True, seed=11 + k)
+        m_pair.append(mse(orc.render_accumulate(cam, pc, spp, mode=1, nthreads=cores)))
+        pc = Renderer.params(W, H, sample_idx=100000 + 1000 * k, max_depth=D, use_photons=True)
+        m_ind.append(mse(orc.render_accumulate(cam, pc, spp, mode=0, nthreads=cores)))
+    mw, mp, mi = np.array(m_wf), np.array(m_pair), np.array(m_ind)
+    e_wf, e_pair, e_ind = (float(np.sqrt(x.mean())) for x in (mw, mp, mi))
+    ratio = e_wf / e_pair
+    # paired: delta method on sqrt(mean(mw) / mean(mp)) with the per-frame differences
+    d = mw / mp.mean() - mp / mp.mean()
+    se = 0.5 * float(np.sqrt(d.var(ddof=1) / K))
+    ratio_ind = e_wf / e_ind
+    se_ind = 0.5 * ratio_ind * float(np.sqrt(mw.var(ddof=1) / K / mw.mean() ** 2 + mi.var(ddof=1) / K / mi.mean() ** 2))
+    log(f"[rmse] {spp} spp vs the {ref_spp}-spp target, {K} frames each: wavefront {e_wf:.5f}, oracle on the same "
+        f"streams {e_pair:.5f} (ratio {ratio:.4f} +- {se:.4f}), oracle row streams {e_ind:.5f} (ratio {ratio_ind:.4f} "
+        f"+- {se_ind:.4f}); device photons {nd}, oracle photons {nph}")
+    return {"value": ratio, "unit": "RMSE ratio (wavefront / oracle, same streams) at equal spp", "ms_per_step": None,
+            "config": {"workload": "gem_caustics_640x360_rmse", "width": W, "height": H, "spp": spp,
+                       "reference_spp": ref_spp, "frames": K, "photons": {"device": nd, "oracle": nph},
+                       "rmse": {"wavefront": e_wf, "oracle_same_streams": e_pair, "oracle_row_streams": e_ind,
+                                "plain_first_frame": float(np.sqrt(m_plain))},
+                       "rmse_per_frame": {"wavefront": [float(np.sqrt(x)) for x in mw],
+                                          "oracle_same_streams": [float(np.sqrt(x)) for x in mp],
+                                          "oracle_row_streams": [float(np.sqrt(x)) for x in mi]},
+                       "ratio_se": se, "within_2pct": bool(abs(ratio - 1.0) <= 0.02 and se <= 0.007),
+                       "ratio_row_streams": ratio_ind, "ratio_row_streams_se": se_ind,
+                       "row_streams_consistent_2se": bool(abs(ratio_ind - 1.0) <= 2 * se_ind),
+                       "reference_seconds": round(ref_s, 1)},
             "roofline": {"achieved": None, "frac": None, "kernel": None, "avg_launch_ms": None},
             "cpu_baseline": None}
 
