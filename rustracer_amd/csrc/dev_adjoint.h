@@ -23,39 +23,41 @@
 //
 // Differentiable fields: Lambertian (solid) albedo p0-2 (dlog f_c/da_c =
 // 1/a_c); rough spectral dielectric n_d, Sellmeier B1-3 and GGX alpha, with
-// dlog f/dn and dlog f/dalpha from forward-mode dual numbers through the
+// dlog f/dn and dlog f/dalpha from forward-mode dual numbers in double through the
 // Walter et al. (2007) microfacet BSDF.
 #pragma once
 
 namespace rt {
 
 // ---- forward-mode dual numbers -------------------------------------------------------------
-struct Dl {
-    float v, d;
-    RT_HD Dl() : v(0.0f), d(0.0f) {}
-    RT_HD Dl(float a, float b = 0.0f) : v(a), d(b) {}
+// Value and derivative in R (float or double); scalar operands are float.
+template <class R> struct DlT {
+    R v, d;
+    RT_HD DlT() : v(0), d(0) {}
+    RT_HD DlT(R a, R b = 0) : v(a), d(b) {}
 };
-RT_HD Dl operator+(Dl a, Dl b) { return Dl(a.v + b.v, a.d + b.d); }
-RT_HD Dl operator-(Dl a, Dl b) { return Dl(a.v - b.v, a.d - b.d); }
-RT_HD Dl operator*(Dl a, Dl b) { return Dl(a.v * b.v, a.d * b.v + a.v * b.d); }
-RT_HD Dl operator/(Dl a, Dl b) { return Dl(a.v / b.v, (a.d * b.v - a.v * b.d) / (b.v * b.v)); }
-RT_HD Dl operator-(Dl a) { return Dl(-a.v, -a.d); }
-RT_HD Dl operator+(Dl a, float b) { return Dl(a.v + b, a.d); }
-RT_HD Dl operator+(float a, Dl b) { return Dl(a + b.v, b.d); }
-RT_HD Dl operator-(Dl a, float b) { return Dl(a.v - b, a.d); }
-RT_HD Dl operator-(float a, Dl b) { return Dl(a - b.v, -b.d); }
-RT_HD Dl operator*(Dl a, float b) { return Dl(a.v * b, a.d * b); }
-RT_HD Dl operator*(float a, Dl b) { return Dl(a * b.v, a * b.d); }
-RT_HD Dl operator/(Dl a, float b) { return Dl(a.v / b, a.d / b); }
-RT_HD Dl operator/(float a, Dl b) { return Dl(a / b.v, -a * b.d / (b.v * b.v)); }
+template <class R> RT_HD DlT<R> operator+(DlT<R> a, DlT<R> b) { return DlT<R>(a.v + b.v, a.d + b.d); }
+template <class R> RT_HD DlT<R> operator-(DlT<R> a, DlT<R> b) { return DlT<R>(a.v - b.v, a.d - b.d); }
+template <class R> RT_HD DlT<R> operator*(DlT<R> a, DlT<R> b) { return DlT<R>(a.v * b.v, a.d * b.v + a.v * b.d); }
+template <class R> RT_HD DlT<R> operator/(DlT<R> a, DlT<R> b) { return DlT<R>(a.v / b.v, (a.d * b.v - a.v * b.d) / (b.v * b.v)); }
+template <class R> RT_HD DlT<R> operator-(DlT<R> a) { return DlT<R>(-a.v, -a.d); }
+template <class R> RT_HD DlT<R> operator+(DlT<R> a, float b) { return DlT<R>(a.v + (R)b, a.d); }
+template <class R> RT_HD DlT<R> operator+(float a, DlT<R> b) { return DlT<R>((R)a + b.v, b.d); }
+template <class R> RT_HD DlT<R> operator-(DlT<R> a, float b) { return DlT<R>(a.v - (R)b, a.d); }
+template <class R> RT_HD DlT<R> operator-(float a, DlT<R> b) { return DlT<R>((R)a - b.v, -b.d); }
+template <class R> RT_HD DlT<R> operator*(DlT<R> a, float b) { return DlT<R>(a.v * (R)b, a.d * (R)b); }
+template <class R> RT_HD DlT<R> operator*(float a, DlT<R> b) { return DlT<R>((R)a * b.v, (R)a * b.d); }
+template <class R> RT_HD DlT<R> operator/(DlT<R> a, float b) { return DlT<R>(a.v / (R)b, a.d / (R)b); }
+template <class R> RT_HD DlT<R> operator/(float a, DlT<R> b) { return DlT<R>((R)a / b.v, -(R)a * b.d / (b.v * b.v)); }
+template <class R> RT_HD R val(DlT<R> a) { return a.v; }
+template <class R> RT_HD DlT<R> xsqrt(DlT<R> a) { R s = sqrt(a.v); return DlT<R>(s, s > 0 ? a.d * (R)0.5 / s : (R)0); }
+template <class R> RT_HD DlT<R> xlog(DlT<R> a) { return DlT<R>(log(a.v), a.d / a.v); }
+template <class R> RT_HD DlT<R> xabs(DlT<R> a) { return a.v < 0 ? -a : a; }
+using Dd = DlT<double>;
 RT_HD float val(float a) { return a; }
-RT_HD float val(Dl a) { return a.v; }
 RT_HD float xsqrt(float a) { return sqrtf(a); }
-RT_HD Dl xsqrt(Dl a) { float s = sqrtf(a.v); return Dl(s, s > 0.0f ? a.d * 0.5f / s : 0.0f); }
 RT_HD float xlog(float a) { return logf(a); }
-RT_HD Dl xlog(Dl a) { return Dl(logf(a.v), a.d / a.v); }
 RT_HD float xabs(float a) { return fabsf(a); }
-RT_HD Dl xabs(Dl a) { return a.v < 0.0f ? -a : a; }
 RT_HD double val(double a) { return a; }
 RT_HD double xsqrt(double a) { return sqrt(a); }
 RT_HD double xlog(double a) { return log(a); }
@@ -63,7 +65,7 @@ RT_HD double xabs(double a) { return fabs(a); }
 
 // ---- Walter et al. 2007 rough dielectric, log of the BSDF value ---------------------------
 // The same formulas as the sampler (fresnel_dielectric, smith_g1, ggx_ndf),
-// over T = float or Dl.
+// over T = double (the detached ratio) or DlT<double> (the scores).
 template <class T> RT_HD T t_fresnel(T ci, T eta) {
     T s2i = T(1.0f) - ci * ci;
     if (val(s2i) < 0.0f) s2i = T(0.0f);
@@ -88,10 +90,16 @@ RT_HD bool rsd_logf(V3 i, V3 o, V3 n, T k, T al, bool refl, T& lf) {
     T eta = T(1.0f) / k;
     T G = t_g1(ain, al) * t_g1(aon, al);
     if (refl) {
-        V3 h = unit(i + o);
-        if (!(dot(i, h) * cin > 0.0f)) return false;
-        T F = t_fresnel(T(fminf(dot(i, h), 1.0f)), eta);
-        T D = t_ggx(T(dot(h, n)), al);
+        // half vector in T's precision: at alpha ~ 0.005, 1 - (h.n)^2 is a
+        // few hundred float ulps
+        T hx = T(i.x) + o.x, hy = T(i.y) + o.y, hz = T(i.z) + o.z;
+        T hl = xsqrt(hx * hx + hy * hy + hz * hz);
+        if (!(val(hl) > 0.0f)) return false;
+        T ih = (hx * i.x + hy * i.y + hz * i.z) / hl;
+        if (!(val(ih) * cin > 0.0f)) return false;
+        if (val(ih) > 1.0f) ih = T(1.0f);
+        T F = t_fresnel(ih, eta);
+        T D = t_ggx((hx * n.x + hy * n.y + hz * n.z) / hl, al);
         if (!(val(F) > 0.0f) || !(val(D) > 0.0f) || !(val(G) > 0.0f)) return false;
         lf = xlog(F) + xlog(G) + xlog(D) - logf(4.0f * ain * aon);
         return true;
@@ -127,6 +135,16 @@ RT_HD float rsd_dn_dB(const float* p, float lambda, int j) {
     return term(lambda) - term(RSD_LAMBDA_D);
 }
 
+// n(lambda) in double: the detached ratio compares two parameter sets whose
+// n differ by a few float ulps for a 1e-3 step of a Sellmeier coefficient.
+RT_D double rsd_ior_d(const DMaterial& m, float lambda) {
+    auto sell = [&](double lam) {
+        double l = lam * 1e-3, l2 = l * l;
+        return sqrt(1.0 + m.p[1] * l2 / (l2 - m.p[4]) + m.p[2] * l2 / (l2 - m.p[5]) + m.p[3] * l2 / (l2 - m.p[6]));
+    };
+    return (double)m.p[0] + (sell(lambda) - sell(RSD_LAMBDA_D));
+}
+
 // ---- per-vertex scores ---------------------------------------------------------------------
 #define RT_SCORE_MAX 6
 struct Score {
@@ -152,35 +170,32 @@ RT_D void vertex_score(const DScene& S, const int8_t* fmap, const Rec& h, V3 wo,
         if (fm[1] >= 0) score_add(sc, fm[1], mk(0.0f, 1.0f / m.p[1], 0.0f));
         if (fm[2] >= 0) score_add(sc, fm[2], mk(0.0f, 0.0f, 1.0f / m.p[2]));
     } else if (m.kind == M_ROUGH_SPECTRAL) {
+        // dual numbers in double: dlog D / dalpha = 2 / alpha - 4 alpha c^2 / (1 + c^2 (alpha^2 - 1))
+        // is a difference of two terms of order 1 / alpha whose sum over a
+        // lobe nearly vanishes (D stays normalised), and in float the
+        // cancelling denominator biases that small sum by tenths of a percent
         int fa = rsd_alpha_field(h.prim_id);
-        float nl = rsd_ior(m, lambda);
-        float k = h.front ? nl : 1.0f / nl, dk = h.front ? 1.0f : -1.0f / (nl * nl);
-        float al = fclamp(m.p[fa], 1e-4f, 1.0f);
+        double nl = rsd_ior_d(m, lambda);
+        double k = h.front ? nl : 1.0 / nl, dk = h.front ? 1.0 : -1.0 / (nl * nl);
+        double al = (double)fclamp(m.p[fa], 1e-4f, 1.0f);
         bool refl = dot(wi, h.n) > 0.0f;
         if (fm[0] >= 0 || fm[1] >= 0 || fm[2] >= 0 || fm[3] >= 0) {
-            Dl lf;
-            if (rsd_logf<Dl>(wo, wi, h.n, Dl(k, dk), Dl(al, 0.0f), refl, lf)) {
-                float gn = lf.d;
+            Dd lf;
+            if (rsd_logf<Dd>(wo, wi, h.n, Dd(k, dk), Dd(al, 0.0), refl, lf)) {
+                float gn = (float)lf.d;
                 if (fm[0] >= 0) score_add(sc, fm[0], mk(gn, gn, gn));
                 for (int j = 0; j < 3; j++)
                     if (fm[1 + j] >= 0) { float g = gn * rsd_dn_dB(m.p, lambda, j); score_add(sc, fm[1 + j], mk(g, g, g)); }
             }
         }
         if (fm[fa] >= 0 && m.p[fa] > 1e-4f && m.p[fa] < 1.0f) {
-            Dl lf;
-            if (rsd_logf<Dl>(wo, wi, h.n, Dl(k, 0.0f), Dl(al, 1.0f), refl, lf)) score_add(sc, fm[fa], mk(lf.d, lf.d, lf.d));
+            Dd lf;
+            if (rsd_logf<Dd>(wo, wi, h.n, Dd(k, 0.0), Dd(al, 1.0), refl, lf)) {
+                float ga = (float)lf.d;
+                score_add(sc, fm[fa], mk(ga, ga, ga));
+            }
         }
     }
-}
-
-// n(lambda) in double: the detached ratio compares two parameter sets whose
-// n differ by a few float ulps for a 1e-3 step of a Sellmeier coefficient.
-RT_D double rsd_ior_d(const DMaterial& m, float lambda) {
-    auto sell = [&](double lam) {
-        double l = lam * 1e-3, l2 = l * l;
-        return sqrt(1.0 + m.p[1] * l2 / (l2 - m.p[4]) + m.p[2] * l2 / (l2 - m.p[5]) + m.p[3] * l2 / (l2 - m.p[6]));
-    };
-    return (double)m.p[0] + (sell(lambda) - sell(RSD_LAMBDA_D));
 }
 
 // f_theta / f_theta_s at a vertex: `me` holds the evaluation materials.  The

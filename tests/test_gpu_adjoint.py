@@ -1,12 +1,16 @@
 """Path-replay adjoint on the device (rt_backward, rt_render_detached).
 
 * the rough spectral gem renders like the checker (same streams, 3 sigma);
-* the detached renderer at theta = theta_s is the plain frame;
+* the detached renderer at theta = theta_s is the plain frame; away from
+  theta_s it estimates the image at theta (3 sigma against the plain frame
+  rendered after set_params(theta)), and equals the oracle's detached
+  renderer sample by sample;
 * the adjoint equals the checker's host gradient (forward score accumulation,
   numeric dlog f) on the same per-pixel streams;
-* the adjoint equals central finite differences of the detached renderer
-  with common random numbers (h = 1e-3 relative) within 2 %, material
-  parameters and a constant medium's sigma_t alike;
+* the adjoint equals finite differences of the detached renderer with common
+  random numbers within 2 %, no absolute slack: Richardson-extrapolated
+  central differences (h = 1e-3 and 5e-4 relative) for the gem's 20
+  parameters, central differences (h = 1e-3) for a constant medium's sigma_t;
 * the wavefront replay gives the megakernel replay's gradient.
 """
 import numpy as np
@@ -47,6 +51,70 @@ def test_rough_spectral_gem_matches_checker(gpu_renderer):
     IS.gate(IS.frames(gpu, 128, 32), IS.frames(cpu, 128, 32))
 
 
+def detached_thetas(cfg):
+    """Parameter vectors away from theta_s whose BSDF ratios f_theta / f_theta_s
+    stay bounded: the 16 group alphas, the table alpha and the floor albedo 5 %
+    off (a GGX lobe's ratio is bounded by (alpha / alpha_s)^+-2), and every
+    parameter 0.5 % off.  A 5 % change of n_d or B1 is not one of them: the
+    refracted half vector then moves by several lobe widths of the 0.005
+    alpha facets, the ratio's distribution is heavy-tailed (the largest
+    frame-mean difference grows with the frame count), and the oracle's own
+    detached renderer fails the normal-theory gate against its plain frame
+    just as the device's does; that case is checked sample by sample against
+    the oracle instead."""
+    ts, tt = cfg["theta_target"], cfg["theta_start"]
+    rough = ts.copy()
+    rough[2:] = tt[2:]
+    return {"alphas_albedo_5pct": rough, "all_0p5pct": (ts + 0.1 * (tt - ts)).astype(np.float32)}
+
+
+@pytest.mark.parametrize("which", ["alphas_albedo_5pct", "all_0p5pct"])
+def test_detached_away_from_the_scene_parameters_renders_that_image(gpu_renderer, which):
+    """render_detached(theta) with decisions at theta_s estimates the image at
+    theta: its per-pixel means agree within the 3-sigma gate with
+    render_accumulate after set_params(theta)."""
+    W, H = 32, 24
+    sc, cam, cfg = setup(gpu_renderer, W, H)
+    P = cfg["params"]
+    th_s, th = cfg["theta_target"], detached_thetas(cfg)[which]
+
+    def detached(first, spp):
+        out = np.zeros((H, W, 3), np.float32)
+        gpu_renderer.render_detached(out, cam, W, H, spp, P, th, sample_idx=first, max_depth=24, seed=2)
+        return out
+
+    def plain(first, spp):
+        out = np.zeros((H, W, 3), np.float32)
+        gpu_renderer.render_accumulate(out, cam, W, H, spp, sample_idx=first, max_depth=24, seed=3)
+        return out
+    gpu_renderer.set_params(P, th_s)
+    a = IS.frames(detached, 128, 32)
+    gpu_renderer.set_params(P, th)
+    b = IS.frames(plain, 128, 32)
+    gpu_renderer.set_params(P, th_s)
+    assert np.abs(a.mean - b.mean).max() > 0 and (b.mean > 0).mean() > 0.3
+    IS.gate(a, b)
+
+
+def test_detached_far_from_the_scene_parameters_matches_the_oracle_per_sample(gpu_renderer):
+    """Every parameter 5 % off (n_d and B1 included): the device's detached
+    frame equals the oracle's detached frame on the same per-pixel streams,
+    sample by sample."""
+    W, H = 32, 24
+    sc, cam, cfg = setup(gpu_renderer, W, H)
+    P = cfg["params"]
+    orc = O.OracleScene(sc)
+    g = np.zeros((4, H, W, 3), np.float32)
+    c = np.zeros((4, H, W, 3), np.float32)
+    for k in range(4):
+        gpu_renderer.render_detached(g[k], cam, W, H, 1, P, cfg["theta_start"], sample_idx=k, max_depth=24, seed=5)
+        c[k] = orc.render_detached(cam, Renderer.params(W, H, sample_idx=k, max_depth=24, seed=5), 1, P,
+                                   cfg["theta_start"], mode=1)
+    assert (g.sum(axis=0) > 0).mean() > 0.3
+    close = np.isclose(g, c, rtol=1e-3, atol=1e-4).all(axis=-1)
+    assert close.mean() > 0.95, close.mean()
+
+
 def test_detached_at_current_parameters_is_the_plain_frame(gpu_renderer):
     W, H = 40, 32
     sc, cam, cfg = setup(gpu_renderer, W, H)
@@ -62,12 +130,16 @@ def test_detached_at_current_parameters_is_the_plain_frame(gpu_renderer):
     assert np.array_equal(gpu_renderer.get_params(cfg["params"]), cfg["theta_start"])
 
 
-def test_adjoint_matches_host_replay(gpu_renderer):
+@pytest.mark.parametrize("wavefront", [False, True])
+def test_adjoint_matches_host_replay(gpu_renderer, wavefront):
+    """Megakernel and wavefront replay against the oracle's host gradient
+    (forward score accumulation, a different algorithm) on the same streams."""
     W, H, spp = 24, 24, 8
     sc, cam, cfg = setup(gpu_renderer, W, H)
     P = cfg["params"]
     w = np.random.default_rng(5).standard_normal((H, W, 3)).astype(np.float32)
-    g = gpu_renderer.backward(w, cam, W, H, spp, P, max_depth=24, seed=4)
+    kw = dict(wavefront=True, capacity=2048, streams=2) if wavefront else {}
+    g = gpu_renderer.backward(w, cam, W, H, spp, P, max_depth=24, seed=4, **kw)
     orc = O.OracleScene(sc)
     h = orc.backward(cam, Renderer.params(W, H, max_depth=24, seed=4), spp, P, w, mode=1)
     scale = np.abs(h).max()
@@ -94,17 +166,19 @@ def test_adjoint_matches_finite_differences_with_common_random_numbers(gpu_rende
         out = np.zeros((H, W, 3), np.float32)
         gpu_renderer.render_detached(out, cam, W, H, spp, P, th, max_depth=24, seed=7)
         return float((((out / spp).astype(np.float64) - target) ** 2).mean())
-    fd = np.zeros(len(P))
-    for j in range(len(P)):
-        h = 1e-3 * abs(float(theta[j]))
+    def central(j, hrel):
+        h = hrel * abs(float(theta[j]))
         tp, tm = theta.astype(np.float64).copy(), theta.astype(np.float64).copy()
         tp[j] += h
         tm[j] -= h
-        fd[j] = (loss(tp) - loss(tm)) / (float(np.float32(tp[j])) - float(np.float32(tm[j])))
-    scale = np.abs(fd).max()
-    bad = np.abs(g - fd) > 2e-2 * np.abs(fd) + 1e-3 * scale
+        return (loss(tp) - loss(tm)) / (float(np.float32(tp[j])) - float(np.float32(tm[j])))
+    # Richardson over h = 1e-3 and 5e-4 relative cancels the central
+    # difference's O(h^2) term (the nearly specular facets curve the BSDF
+    # ratio strongly in n_d); no absolute slack
+    fd = np.array([(4.0 * central(j, 5e-4) - central(j, 1e-3)) / 3.0 for j in range(len(P))])
+    bad = np.abs(g - fd) > 2e-2 * np.abs(fd)
     assert not bad.any(), np.c_[np.arange(len(P)), g, fd][bad]
-    assert np.count_nonzero(np.abs(fd) > 1e-2 * scale) >= 10
+    assert np.count_nonzero(np.abs(fd) > 1e-2 * np.abs(fd).max()) >= 10
 
 
 @pytest.mark.parametrize("cap,sort,streams", [(4096, 1, 1), (1 << 20, 0, 1), (8192, 2, 1), (1024, 1, 3), (2048, 0, 4)])
