@@ -139,10 +139,11 @@ class Renderer:
         return scratch
 
     def render_accumulate(self, out, camera, width, height, spp, sample_idx=0, strata=0, bg_scale=1.0,
-                          photon_map=False, max_depth=50, seed=0, scene=None):
+                          photon_map=False, max_depth=50, seed=0, scene=None, frame_time=None):
         """Sum of `spp` passes (sample_idx .. sample_idx+spp-1) into `out`;
-        `scene` is a handle from upload_extra (default: the current scene)."""
-        p = self.params(width, height, sample_idx, strata, bg_scale, max_depth, None, photon_map, seed)
+        `scene` is a handle from upload_extra (default: the current scene);
+        frame_time = (t0, t1) draws each camera ray's time in [t0, t1)."""
+        p = self.params(width, height, sample_idx, strata, bg_scale, max_depth, frame_time, photon_map, seed)
         cam = camera.to_c()
         self._check(self.L.rt_render_accumulate(self.ctx, self.scene if scene is None else scene, C.byref(cam), C.byref(p), spp, _ptr(out)),
                     "rt_render_accumulate")
@@ -239,9 +240,9 @@ class Renderer:
 
     def render_wavefront(self, out, camera, width, height, spp, sample_idx=0, strata=0, bg_scale=1.0,
                          photon_map=False, max_depth=50, seed=0, capacity=0, grid=0, sort=True, use_graph=False,
-                         occupancy=0, streams=1, scene=None, block=0, knn_wave_max=0):
+                         occupancy=0, streams=1, scene=None, block=0, knn_wave_max=0, frame_time=None):
         """Same frame as render_accumulate, through the wavefront stage kernels."""
-        p = self.params(width, height, sample_idx, strata, bg_scale, max_depth, None, photon_map, seed)
+        p = self.params(width, height, sample_idx, strata, bg_scale, max_depth, frame_time, photon_map, seed)
         cam = camera.to_c()
         o = F.rt_wavefront_opts(capacity=capacity, grid=grid, sort=int(sort), use_graph=1 if use_graph else 0,
                                 occupancy=occupancy, streams=streams, block=block, knn_wave_max=knn_wave_max)
