@@ -33,7 +33,7 @@
 extern "C" {
 #endif
 
-#define RT_API_VERSION 2
+#define RT_API_VERSION 3
 
 /* ---- status codes ------------------------------------------------------- */
 #define RT_OK               0
@@ -303,6 +303,14 @@ typedef struct {
     float    frame_t0, frame_t1;
     uint32_t use_photons;    /* photon_map: Some(..) when a map was built     */
     uint64_t seed;           /* mixed into every per-pixel stream             */
+    uint32_t wavelengths;    /* 0: one hero wavelength uniform over [380, 700)
+                                nm, the reference's (src/renderer.rs:726);
+                                extension: 1 one wavelength over [380, 780)
+                                with the CIE table extended to 780 nm; 4 four
+                                hero wavelengths over [380, 780) (lambda_0 and
+                                lambda_0 + 100 k mod 400 nm) combined by the
+                                balance heuristic (spectral MIS); the adjoint
+                                entry points take 0 or 1                     */
 } rt_sample_params;
 
 /* ---- wavefront renderer options ----------------------------------------------
@@ -364,6 +372,11 @@ typedef struct {
                                   hemisphere's photons closer than gather_radius
                                   (gather_radius when fewer), normalised by
                                   2 / (pi^2 R^2)                               */
+    uint32_t wavelengths;      /* 0: photon wavelength over [380, 700), the
+                                  reference's (src/photon.rs:244); 1 or 4
+                                  (extension): over [380, 780) with the
+                                  extended CIE table, for renders with
+                                  wavelengths 1 or 4                           */
 } rt_photon_params;
 
 #define RT_KNN_MAX 64
@@ -443,7 +456,7 @@ int  rt_render_wavefront(rt_ctx *ctx, rt_scene *scene, const rt_camera *cam,
 /* rt_render_wavefront for callers that cannot allocate device memory (code
  * under HIP graph capture, tight serving loops, harnesses that forbid
  * hipMalloc).  No counterpart in the reference.  The path queues live in a
- * fixed device array of RT_NOALLOC_CAPACITY paths (48 MiB) that every GPU
+ * fixed device array of RT_NOALLOC_CAPACITY paths (52 MiB) that every GPU
  * the library runs on holds until the process ends; the call allocates and
  * frees nothing (no hipMalloc, hipFree, streams, graphs or events) and clears
  * with kernels, not hipMemsetAsync.  accum_rgb must be device memory.  opts

@@ -17,6 +17,7 @@
 #pragma once
 #include "rc_geom.h"
 #include "../rustracer_amd/csrc/spectral_data.h"
+#include "rc_walter.h"
 
 namespace rc {
 
@@ -28,20 +29,28 @@ static inline void xyz_to_srgb_clamped(float x, float y, float z, float o[3]) {
 }
 static const float FIRE_DESAT = 0.5f;
 
+// Per-channel normalisations of both tints: over the reference's table
+// (380-700 nm) and, for the wide wavelength modes (extension), over the table
+// extended to 780 nm, so that E_lambda[tint] = (1, 1, 1) over either range.
 struct SpectralTables {
     float norms[3], fire_norms[3];
-    SpectralTables() {
+    float norms_w[3], fire_norms_w[3];
+    static void sums(int n, const float* X, const float* Y, const float* Z, float nm[3], float fn[3]) {
         float s[3] = {0, 0, 0}, f[3] = {0, 0, 0};
-        for (int i = 0; i < RT_CIE_N; i++) {
-            float rgb[3]; xyz_to_srgb_clamped(RT_CIE_X[i], RT_CIE_Y[i], RT_CIE_Z[i], rgb);
+        for (int i = 0; i < n; i++) {
+            float rgb[3]; xyz_to_srgb_clamped(X[i], Y[i], Z[i], rgb);
             s[0] += rgb[0]; s[1] += rgb[1]; s[2] += rgb[2];
-            float c[3]; fire_color(RT_CIE_X[i], RT_CIE_Y[i], RT_CIE_Z[i], c);
+            float c[3]; fire_color(X[i], Y[i], Z[i], c);
             f[0] += c[0]; f[1] += c[1]; f[2] += c[2];
         }
         for (int k = 0; k < 3; k++) {
-            norms[k] = (float)RT_CIE_N / rmax(s[k], 1e-6f);
-            fire_norms[k] = (float)RT_CIE_N / rmax(f[k], 1e-6f);
+            nm[k] = (float)n / rmax(s[k], 1e-6f);
+            fn[k] = (float)n / rmax(f[k], 1e-6f);
         }
+    }
+    SpectralTables() {
+        sums(RT_CIE_N, RT_CIE_X, RT_CIE_Y, RT_CIE_Z, norms, fire_norms);
+        sums(RT_CIE_WIDE_N, RT_CIE_WX, RT_CIE_WY, RT_CIE_WZ, norms_w, fire_norms_w);
     }
     static void fire_color(float x, float y, float z, float c[3]) {
         float w = rmax(std::pow(y, 0.5f), 0.08f);
@@ -56,25 +65,33 @@ struct SpectralTables {
 };
 static const SpectralTables& spectral_tables() { static SpectralTables t; return t; }
 
-static inline void cie_interp(float lambda, float& x, float& y, float& z) {
-    float t = rclamp((lambda - RT_CIE_START) / RT_CIE_STEP, 0.0f, (float)(RT_CIE_N - 1));
-    size_t i = std::min<uint64_t>(f2usize(t), RT_CIE_N - 2);
+// wide: the table through 780 nm, else the reference's (clamped at 700 nm)
+static inline void cie_interp(float lambda, float& x, float& y, float& z, bool wide = false) {
+    const int n = wide ? RT_CIE_WIDE_N : RT_CIE_N;
+    float t = rclamp((lambda - RT_CIE_START) / RT_CIE_STEP, 0.0f, (float)(n - 1));
+    size_t i = std::min<uint64_t>(f2usize(t), (uint64_t)(n - 2));
     float frac = t - (float)i;
-    x = RT_CIE_X[i] + frac * (RT_CIE_X[i + 1] - RT_CIE_X[i]);
-    y = RT_CIE_Y[i] + frac * (RT_CIE_Y[i + 1] - RT_CIE_Y[i]);
-    z = RT_CIE_Z[i] + frac * (RT_CIE_Z[i + 1] - RT_CIE_Z[i]);
+    x = RT_CIE_WX[i] + frac * (RT_CIE_WX[i + 1] - RT_CIE_WX[i]);
+    y = RT_CIE_WY[i] + frac * (RT_CIE_WY[i + 1] - RT_CIE_WY[i]);
+    z = RT_CIE_WZ[i] + frac * (RT_CIE_WZ[i + 1] - RT_CIE_WZ[i]);
 }
-static inline Color spectral_to_rgb(float lambda) {
-    float x, y, z; cie_interp(lambda, x, y, z);
+static inline Color spectral_to_rgb(float lambda, bool wide = false) {
+    float x, y, z; cie_interp(lambda, x, y, z, wide);
     float rgb[3]; xyz_to_srgb_clamped(x, y, z, rgb);
-    const float* n = spectral_tables().norms;
+    const float* n = wide ? spectral_tables().norms_w : spectral_tables().norms;
     return Color(rgb[0] * n[0], rgb[1] * n[1], rgb[2] * n[2]);
 }
-static inline Color spectral_fire_rgb(float lambda) {
-    float x, y, z; cie_interp(lambda, x, y, z);
+static inline Color spectral_fire_rgb(float lambda, bool wide = false) {
+    float x, y, z; cie_interp(lambda, x, y, z, wide);
     float c[3]; SpectralTables::fire_color(x, y, z, c);
-    const float* n = spectral_tables().fire_norms;
+    const float* n = wide ? spectral_tables().fire_norms_w : spectral_tables().fire_norms;
     return Color(c[0] * n[0], c[1] * n[1], c[2] * n[2]);
+}
+// the four-wavelength rotation: lambda_k = lambda_0 + 100 k nm, wrapped into [380, 780)
+static inline float hero_lambda(float l0, int k) {
+    float t = (l0 - 380.0f) + 100.0f * (float)k;
+    if (t >= 400.0f) t -= 400.0f;
+    return 380.0f + t;
 }
 static inline float cauchy_ior(float lambda_nm, float b, float c) {
     float lam_um = lambda_nm * 1e-3f;
@@ -248,6 +265,8 @@ struct Material {
     virtual bool scatter(const Ray& r_in, const HitRecord& rec, SmallRng& rng, ScatterRecord& sr) const = 0;
     virtual Color emitted(float, float, const Point3&) const { return Color(); }
     virtual Color emitted_at(float u, float v, const Point3& p, float, bool) const { return emitted(u, v, p); }
+    // emitted_at for a ray of wavelength mode wl (Ray::wl); only a blackbody depends on it
+    virtual Color emitted_wl(float u, float v, const Point3& p, float lambda, bool sw, int) const { return emitted_at(u, v, p, lambda, sw); }
     virtual float scattering_pdf(const Ray&, const HitRecord&, const Ray&) const { return 0.0f; }
     virtual Color albedo_hint(float, float, const Point3&) const { return Color(1, 1, 1); }
     virtual bool can_receive_caustics() const { return false; }
@@ -255,6 +274,12 @@ struct Material {
     virtual Color specular_brdf_cos(const Ray&, const HitRecord&, const Vec3&) const { return Color(); }
     virtual float specular_sampling_pdf(const Ray&, const HitRecord&, const Vec3&) const { return 0.0f; }
     virtual Vec3 shading_normal(const HitRecord& rec) const { return rec.normal; }
+    // Four hero wavelengths (extension): multiply q[k] by f(lambda_k) / f(lambda_0)
+    // and r[k] by p(lambda_k) / p(lambda_0) for the direction wi this material
+    // sampled from r_in (k = 1..3 in slots 0..2); true when the material is spectral.
+    virtual bool hero_scatter(const Ray&, const HitRecord&, const Vec3&, float*, float*) const { return false; }
+    // f(lambda_k) / f(lambda_0) of specular_brdf_cos toward wi; false: no ratio (1).
+    virtual bool hero_brdf(const Ray&, const HitRecord&, const Vec3&, float*) const { return false; }
 };
 inline bool material_is_emissive(const Material* m) { return m && m->is_emissive(); }
 
@@ -367,17 +392,26 @@ static inline float planck_raw(float lambda_nm, float temp_k) {
     return 1.0f / ((lambda_nm * l4) * rmax(std::exp(x) - 1.0f, 1e-30f));
 }
 struct BlackbodyLight : Material {
-    float temp_k = 6500.0f, intensity = 1.0f, norm = 1.0f;
-    Color avg_color;
-    void init() {
-        const int N = 65;
-        float raw[N], sum = 0.0f;
-        for (int i = 0; i < N; i++) { raw[i] = planck_raw(380.0f + (float)i * 5.0f, temp_k); sum += raw[i]; }
-        float mean = sum / (float)N;
-        norm = 1.0f / rmax(mean, 1e-30f);
+    float temp_k = 6500.0f, intensity = 1.0f, norm = 1.0f, norm_w = 1.0f;
+    Color avg_color, avg_color_w;
+    // the normalising mean and average colour over n table samples: 65 for
+    // the reference (380-700 nm), 81 for the wide wavelength modes (extension)
+    void init_over(int n, bool wide, float& nm, Color& avg) const {
+        float raw[RT_CIE_WIDE_N], sum = 0.0f;
+        for (int i = 0; i < n; i++) { raw[i] = planck_raw(380.0f + (float)i * 5.0f, temp_k); sum += raw[i]; }
+        nm = 1.0f / rmax(sum / (float)n, 1e-30f);
         Color acc;
-        for (int i = 0; i < N; i++) acc += spectral_to_rgb(380.0f + (float)i * 5.0f) * (raw[i] * norm);
-        avg_color = acc / (float)N * intensity;
+        for (int i = 0; i < n; i++) acc += spectral_to_rgb(380.0f + (float)i * 5.0f, wide) * (raw[i] * nm);
+        avg = acc / (float)n * intensity;
+    }
+    void init() {
+        init_over(RT_CIE_N, false, norm, avg_color);
+        init_over(RT_CIE_WIDE_N, true, norm_w, avg_color_w);
+    }
+    Color emitted_wl(float u, float v, const Point3& p, float lambda, bool sw, int wl) const override {
+        if (wl == 0) return emitted_at(u, v, p, lambda, sw);
+        if (sw) { float sc = planck_raw(lambda, temp_k) * norm_w * intensity; return Color(sc, sc, sc); }
+        return avg_color_w;
     }
     bool scatter(const Ray&, const HitRecord&, SmallRng&, ScatterRecord&) const override { return false; }
     Color emitted(float, float, const Point3&) const override { return avg_color; }
@@ -427,7 +461,10 @@ struct SpectralDielectric : Material {
         Vec3 dir = dielectric_boundary(ior, r_in, rec, rng, reflected);
         Color att(1, 1, 1);
         bool tinted = false;
-        if (!r_in.spectral_weighted && !reflected) { att = spectral_fire_rgb(r_in.wavelength); tinted = true; }
+        if (!r_in.spectral_weighted && !reflected) {
+            if (r_in.wl != 2) att = spectral_fire_rgb(r_in.wavelength, r_in.wl != 0);
+            tinted = true;
+        }
         if (!rec.front_face) {
             float d = rec.t;
             att.x *= std::exp(-absorption.x * d);
@@ -435,9 +472,29 @@ struct SpectralDielectric : Material {
             att.z *= std::exp(-absorption.z * d);
         }
         sr.ray = Ray::scatter_from(rec.p, dir, r_in);
-        if (tinted) sr.ray.spectral_weighted = true;
+        if (tinted) { sr.ray.spectral_weighted = true; sr.ray.tint = 2; }
         sr.attenuation = att;
         sr.skip_pdf = true;
+        return true;
+    }
+    // a refraction is a delta at lambda_0's angle only; a reflection was
+    // chosen with probability 1 (total internal reflection) or F, the value
+    // over it being 1, so both ratios are the probability's
+    bool hero_scatter(const Ray& r_in, const HitRecord& rec, const Vec3& wi, float* q, float* r) const override {
+        if (wi.dot(rec.normal) <= 0.0f) { for (int k = 0; k < 3; k++) q[k] = r[k] = 0.0f; return true; }
+        Vec3 u = r_in.direction.unit();
+        float c = rmin((-u).dot(rec.normal), 1.0f);
+        float s = std::sqrt(1.0f - c * c);
+        auto p_refl = [&](float lam) {
+            float ior = ior_at(lam);
+            float ratio = rec.front_face ? 1.0f / ior : ior;
+            return ratio * s > 1.0f ? 1.0f : fresnel_dielectric(c, ratio);
+        };
+        float p0 = p_refl(r_in.wavelength);
+        for (int k = 0; k < 3; k++) {
+            float f = p0 > 0.0f ? p_refl(hero_lambda(r_in.wavelength, k + 1)) / p0 : 0.0f;
+            q[k] *= f; r[k] *= f;
+        }
         return true;
     }
 };
@@ -485,9 +542,10 @@ struct SpectralMetal : Material {
             weight = ((1.0f + lam_o) / (1.0f + lam_o + lam_i)) * specular_energy_comp(cos_o, alpha);
         }
         float fw = fresnel * weight;
-        sr.attenuation = r_in.spectral_weighted ? Color(fw, fw, fw) : spectral_to_rgb(lambda) * fw;
+        bool now = r_in.spectral_weighted || r_in.wl == 2;      // four wavelengths: tint per wavelength later
+        sr.attenuation = now ? Color(fw, fw, fw) : spectral_to_rgb(lambda, r_in.wl != 0) * fw;
         sr.ray = Ray::scatter_from(rec.p, dir, r_in);
-        if (!r_in.spectral_weighted) sr.ray.spectral_weighted = true;
+        if (!r_in.spectral_weighted) { sr.ray.spectral_weighted = true; sr.ray.tint = 1; }
         sr.skip_pdf = true;
         return true;
     }
@@ -504,7 +562,28 @@ struct SpectralMetal : Material {
         float in, ik; metal_ior(variant, r_in.wavelength, in, ik);
         float fr = fresnel_conductor(wo_h, in, ik);
         float val = d * fr * g2 / (4.0f * cos_o) * specular_energy_comp(cos_o, alpha);
-        return r_in.spectral_weighted ? Color(val, val, val) : spectral_to_rgb(r_in.wavelength) * val;
+        bool now = r_in.spectral_weighted || r_in.wl == 2;
+        return now ? Color(val, val, val) : spectral_to_rgb(r_in.wavelength, r_in.wl != 0) * val;
+    }
+    // only the conductor Fresnel factor at the half vector depends on lambda
+    void fresnel_ratio(const Ray& r_in, const Vec3& wi, float* b) const {
+        Vec3 wo = (-r_in.direction).unit();
+        float ch = rmax(wo.dot((wo + wi).unit()), 0.0f);
+        float n0, k0; metal_ior(variant, r_in.wavelength, n0, k0);
+        float f0 = fresnel_conductor(ch, n0, k0);
+        for (int k = 0; k < 3; k++) {
+            float n, kk; metal_ior(variant, hero_lambda(r_in.wavelength, k + 1), n, kk);
+            b[k] = f0 > 0.0f ? fresnel_conductor(ch, n, kk) / f0 : 0.0f;
+        }
+    }
+    bool hero_scatter(const Ray& r_in, const HitRecord&, const Vec3& wi, float* q, float*) const override {
+        float b[3]; fresnel_ratio(r_in, wi, b);
+        for (int k = 0; k < 3; k++) q[k] *= b[k];
+        return true;
+    }
+    bool hero_brdf(const Ray& r_in, const HitRecord&, const Vec3& wi, float* b) const override {
+        fresnel_ratio(r_in, wi, b);
+        return true;
     }
     float specular_sampling_pdf(const Ray& r_in, const HitRecord& rec, const Vec3& wi) const override {
         if (roughness == 0.0f) return 0.0f;
@@ -852,11 +931,45 @@ struct RoughSpectralDielectric : Material {
         float g1 = smith_g1(std::fabs(cos_i), al);
         Color att(g1, g1, g1);
         bool tinted = false;
-        if (!r_in.spectral_weighted && !refl) { att = att * spectral_fire_rgb(r_in.wavelength); tinted = true; }
+        if (!r_in.spectral_weighted && !refl) {
+            if (r_in.wl != 2) att = att * spectral_fire_rgb(r_in.wavelength, r_in.wl != 0);
+            tinted = true;
+        }
         sr.ray = Ray::scatter_from(rec.p, wi, r_in);
-        if (tinted) sr.ray.spectral_weighted = true;
+        if (tinted) { sr.ray.spectral_weighted = true; sr.ray.tint = 2; }
         sr.attenuation = att;
         sr.skip_pdf = true;
+        return true;
+    }
+    // reflection: the half vector is fixed, only F moves with lambda (value
+    // and density alike); transmission: the microfacet BSDF value and the
+    // sampler's density, both in double (rc_walter.h)
+    bool hero_scatter(const Ray& r_in, const HitRecord& rec, const Vec3& wi, float* q, float* r) const override {
+        Vec3 wo = (-r_in.direction).unit();
+        float l0 = r_in.wavelength;
+        if (wi.dot(rec.normal) > 0.0f) {
+            float c = rmin(wo.dot((wo + wi).unit()), 1.0f);
+            auto F = [&](float lam) { float n = ior_at(lam); return fresnel_dielectric(c, rec.front_face ? 1.0f / n : n); };
+            float f0 = F(l0);
+            for (int k = 0; k < 3; k++) {
+                float f = f0 > 0.0f ? F(hero_lambda(l0, k + 1)) / f0 : 0.0f;
+                q[k] *= f; r[k] *= f;
+            }
+            return true;
+        }
+        double a = (double)alpha_of(rec.prim_id);
+        auto logs = [&](float lam, double& lf, double& lp) {
+            double n = (double)ior_at(lam);
+            double k = rec.front_face ? n : 1.0 / n;
+            return logf_rsd(d3(wo), d3(wi), d3(rec.normal), k, a, false, lf) && logpdf_rsd_t(d3(wo), d3(wi), d3(rec.normal), k, a, lp);
+        };
+        double lf0, lp0;
+        bool ok0 = logs(l0, lf0, lp0);
+        for (int k = 0; k < 3; k++) {
+            double lf, lp;
+            if (ok0 && logs(hero_lambda(l0, k + 1), lf, lp)) { q[k] *= (float)std::exp(lf - lf0); r[k] *= (float)std::exp(lp - lp0); }
+            else q[k] = r[k] = 0.0f;
+        }
         return true;
     }
 };

@@ -121,11 +121,16 @@ struct Ray {
     float time;
     float wavelength;
     bool spectral_weighted;
+    // wavelength mode (extension): 0 the reference's [380, 700); 1 one
+    // wavelength over [380, 780); 2 four hero wavelengths over [380, 780),
+    // their tints applied per wavelength at each contribution (tint: 0 none,
+    // 1 the CMF colour, 2 the fire tint)
+    uint8_t wl = 0, tint = 0;
     Ray() : time(0), wavelength(550.0f), spectral_weighted(false) {}
     static Ray at_time(const Point3& o, const Vec3& d, float t) {
         Ray r; r.origin = o; r.direction = d;
         r.inv_dir = Vec3(1.0f / d.x, 1.0f / d.y, 1.0f / d.z);
-        r.time = t; r.wavelength = 550.0f; r.spectral_weighted = false;
+        r.time = t; r.wavelength = 550.0f; r.spectral_weighted = false; r.wl = 0; r.tint = 0;
         return r;
     }
     static Ray make(const Point3& o, const Vec3& d) { return at_time(o, d, 0.0f); }
@@ -133,6 +138,7 @@ struct Ray {
         Ray r; r.origin = o; r.direction = d;
         r.inv_dir = Vec3(1.0f / d.x, 1.0f / d.y, 1.0f / d.z);
         r.time = src.time; r.wavelength = src.wavelength; r.spectral_weighted = src.spectral_weighted;
+        r.wl = src.wl; r.tint = src.tint;
         return r;
     }
     Point3 at(float t) const { return origin + t * direction; }

@@ -498,12 +498,15 @@ struct PhotonMap {
     }
 };
 
-static inline bool trace_photon(const Hittable& world, Point3 pos, Vec3 dir, float lum_clamp, Color power, SmallRng& rng, KdPhoton& out) {
+// wide (extension): the photon's wavelength over [380, 780) with the extended table
+static inline bool trace_photon(const Hittable& world, Point3 pos, Vec3 dir, float lum_clamp, Color power, SmallRng& rng, KdPhoton& out,
+                                bool wide = false) {
     uint32_t spec_depth = 0;
-    float wavelength = rng.gen_range(380.0f, 700.0f);
+    float wavelength = rng.gen_range(380.0f, wide ? 780.0f : 700.0f);
     for (int i = 0; i < PHOTON_MAX_DEPTH; i++) {
         Ray ray = Ray::at_time(pos, dir, 0.0f);
         ray.wavelength = wavelength;
+        ray.wl = wide ? 1 : 0;
         HitRecord rec;
         if (!world.hit(ray, 0.001f, INF, rec)) return false;
         ScatterRecord sr;
@@ -540,7 +543,8 @@ static void parallel_for(size_t n, int threads, F f) {
 
 static inline uint64_t photon_seed(uint32_t i) { return ((uint64_t)i * 6364136223846793005ULL) ^ 0x9E3779B97F4A7C15ULL; }
 
-static inline void photon_build_from_quad(PhotonMap& pm, const Hittable& world, Point3 lo, Vec3 lu, Vec3 lv, Color lc, uint32_t num, float radius, int threads) {
+static inline void photon_build_from_quad(PhotonMap& pm, const Hittable& world, Point3 lo, Vec3 lu, Vec3 lv, Color lc, uint32_t num, float radius, int threads,
+                                          bool wide = false) {
     Vec3 cr = lu.cross(lv);
     float quad_area = cr.length();
     Vec3 normal = cr / quad_area;
@@ -555,14 +559,15 @@ static inline void photon_build_from_quad(PhotonMap& pm, const Hittable& world, 
         float sin_theta = std::sqrt(1.0f - cos_theta * cos_theta);
         float phi = 2.0f * PI * rng.gen();
         Vec3 dir = sin_theta * std::cos(phi) * ta + sin_theta * std::sin(phi) * ba + cos_theta * normal;
-        ok[i] = trace_photon(world, origin, dir, 15.0f, power, rng, tmp[i]);
+        ok[i] = trace_photon(world, origin, dir, 15.0f, power, rng, tmp[i], wide);
     });
     pm.nodes.clear();
     for (size_t i = 0; i < num; i++) if (ok[i]) pm.nodes.push_back(tmp[i]);
     pm.finish(radius);
 }
 
-static inline void photon_build_sun(PhotonMap& pm, const Hittable& world, bool have_bounds, Aabb bb, Vec3 sun_dir, Color sun_color, uint32_t num, float radius, int threads) {
+static inline void photon_build_sun(PhotonMap& pm, const Hittable& world, bool have_bounds, Aabb bb, Vec3 sun_dir, Color sun_color, uint32_t num, float radius, int threads,
+                                    bool wide = false) {
     Vec3 sun_down = (-sun_dir).unit();
     Vec3 up = std::fabs(sun_down.x) < 0.999f ? Vec3(1, 0, 0) : Vec3(0, 1, 0);
     Vec3 t = sun_down.cross(up).unit();
@@ -585,7 +590,7 @@ static inline void photon_build_sun(PhotonMap& pm, const Hittable& world, bool h
         float r = dr * std::sqrt(rng.gen());
         float phi = 2.0f * PI * rng.gen();
         Point3 origin = dc + t * (r * std::cos(phi)) + b * (r * std::sin(phi));
-        ok[i] = trace_photon(world, origin, sun_down, lum_clamp, power, rng, tmp[i]);
+        ok[i] = trace_photon(world, origin, sun_down, lum_clamp, power, rng, tmp[i], wide);
     });
     pm.nodes.clear();
     for (size_t i = 0; i < num; i++) if (ok[i]) pm.nodes.push_back(tmp[i]);
@@ -612,6 +617,50 @@ struct NoHook {
     void shadow_seg(const Ray&, float) {}                    // the shadow ray of the next contribution
 };
 
+// ---- four hero wavelengths (extension) ---------------------------------------
+// Hero wavelength spectral sampling (Wilkie, Nawaz, Droske, Weidlich and
+// Hanika, EGSR 2014) over the tinted-RGB estimator: the camera ray's lambda_0
+// is uniform over [380, 780) and hero_lambda(lambda_0, k) are its rotations.
+// Decisions and contributions are made at lambda_0 with no spectral tint;
+// each contribution is then multiplied by the balance-heuristic combination
+// of the four wavelengths,
+//     (T(l_0) E_0 + sum_k q_k b_k E_k T(l_k)) / (E_0 (1 + sum_k r_k)),
+// q_k and r_k the running products of the BSDF-value and density ratios of
+// the path's sampled vertices (Material::hero_scatter), b_k the next-event
+// connection's own BSDF ratio (Material::hero_brdf), E a blackbody emitter's
+// Planck radiance, T the tint of the path's first spectral event per
+// wavelength.  No spectral event: weight 1 (the one-wavelength estimator).
+struct Hero4 {
+    float q[3] = {1, 1, 1}, r[3] = {1, 1, 1};
+    bool on = false;
+    static Color tint(int kind, float lam) {
+        if (kind == 1) return spectral_to_rgb(lam, true);
+        if (kind == 2) return spectral_fire_rgb(lam, true);
+        return Color(1, 1, 1);
+    }
+    Color weight(const Ray& ray, int kind, const float* b = nullptr, float bb_temp = 0.0f) const {
+        if (ray.wl != 2) return Color(1, 1, 1);
+        if (!on && kind == 0 && !b && !(bb_temp > 0.0f)) return Color(1, 1, 1);
+        float l0 = ray.wavelength;
+        float e0 = bb_temp > 0.0f ? planck_raw(l0, bb_temp) : 1.0f;
+        Color num = tint(kind, l0);
+        float den = 1.0f;
+        for (int k = 0; k < 3; k++) {
+            float lk = hero_lambda(l0, k + 1);
+            float w = q[k] * (b ? b[k] : 1.0f) * (bb_temp > 0.0f ? planck_raw(lk, bb_temp) / e0 : 1.0f);
+            if (w != 0.0f) num += tint(kind, lk) * w;
+            den += r[k];
+        }
+        return num / den;
+    }
+};
+// A blackbody emitter reached on a tinted path emits per wavelength.
+static inline float hero_bb_temp(const HitRecord& rec, const Ray& ray) {
+    if (!ray.spectral_weighted) return 0.0f;
+    const BlackbodyLight* bb = dynamic_cast<const BlackbodyLight*>(rec.mat);
+    return bb ? bb->temp_k : 0.0f;
+}
+
 template <class H>
 static inline Color ray_color_h(const Ray& r0, const SceneView& S, float bg_scale, int max_depth, SmallRng& rng, H& hk) {
     const Hittable& world = *S.world;
@@ -621,20 +670,25 @@ static inline Color ray_color_h(const Ray& r0, const SceneView& S, float bg_scal
     Ray ray = r0;
     bool prev_specular = true;
     float prev_mis_w_brdf = 1.0f, prev_spec_sun_weight = 1.0f;
+    Hero4 h4;
     for (int depth = 0; depth < max_depth; depth++) {
         HitRecord rec;
         bool hit = world.hit(ray, 0.001f, INF, rec);
         hk.segment(ray, hit, rec);
+        const int ptint = ray.spectral_weighted ? ray.tint : 0;
         if (!hit) {
             float sun_pdf = sun_pdf_value(ray.direction, background);
             float env_pdf = env_pdf_value(ray.direction, background);
             float emit_w = (sun_pdf + env_pdf == 0.0f) ? 1.0f : (prev_specular ? prev_spec_sun_weight : prev_mis_w_brdf);
             Color bg = (depth == 0 || !background.has_env()) ? background.eval(ray.direction) : background.env_eval(ray.direction);
-            color += hk.add(throughput * bg * bg_scale * emit_w);
+            color += hk.add(throughput * bg * bg_scale * emit_w * h4.weight(ray, ptint));
             break;
         }
         float emit_w = (prev_specular || lights.objects.empty()) ? 1.0f : prev_mis_w_brdf;
-        color += hk.add(throughput * rec.mat->emitted_at(rec.u, rec.v, rec.p, ray.wavelength, ray.spectral_weighted) * emit_w);
+        Color em = rec.mat->emitted_wl(rec.u, rec.v, rec.p, ray.wavelength, ray.spectral_weighted, ray.wl);
+        if (em.x != 0.0f || em.y != 0.0f || em.z != 0.0f) em = em * h4.weight(ray, ptint, nullptr, hero_bb_temp(rec, ray));
+        color += hk.add(throughput * em * emit_w);
+        const Color w4 = h4.weight(ray, ptint);      // this vertex's next-event weight
         ScatterRecord sr;
         if (!rec.mat->scatter(ray, rec, rng, sr)) break;
         if (sr.skip_pdf) {
@@ -651,6 +705,9 @@ static inline Color ray_color_h(const Ray& r0, const SceneView& S, float bg_scal
                     if (brdf_cos.x > 0.0f || brdf_cos.y > 0.0f || brdf_cos.z > 0.0f) {
                         float p_mat = rec.mat->specular_sampling_pdf(ray, rec, sun_sample);
                         Color sun_color = background.eval(sun_sample);
+                        float b[3];
+                        bool spec = rec.mat->hero_brdf(ray, rec, sun_sample, b);
+                        sun_color = sun_color * h4.weight(ray, ray.spectral_weighted ? ray.tint : (spec ? 1 : 0), spec ? b : nullptr);
                         float mis_d = p_sun * p_sun + p_mat * p_mat;
                         if (mis_d > 0.0f) color += hk.add(throughput * tr * brdf_cos * sun_color * p_sun / mis_d);
                     }
@@ -660,6 +717,7 @@ static inline Color ray_color_h(const Ray& r0, const SceneView& S, float bg_scal
                     prev_spec_sun_weight = (p_mat_brdf * p_mat_brdf) / rmax(p_mat_brdf * p_mat_brdf + p_sun * p_sun, 1e-8f);
             }
             hk.scatter(rec, ray, sr.ray.direction);
+            if (ray.wl == 2 && rec.mat->hero_scatter(ray, rec, sr.ray.direction, h4.q, h4.r)) h4.on = true;
             throughput *= sr.attenuation;
             ray = sr.ray;
             prev_specular = true;
@@ -669,7 +727,7 @@ static inline Color ray_color_h(const Ray& r0, const SceneView& S, float bg_scal
                 Color irr = S.photons->irradiance(rec.p, rec.normal);
                 if (irr.x > 0.0f || irr.y > 0.0f || irr.z > 0.0f) {
                     Color alb = rec.mat->albedo_hint(rec.u, rec.v, rec.p);
-                    color += hk.add_v(throughput * alb * irr, rec, ray, -ray.direction);
+                    color += hk.add_v(throughput * alb * irr * w4, rec, ray, -ray.direction);
                 }
             }
             if (!lights.objects.empty()) {
@@ -686,7 +744,8 @@ static inline Color ray_color_h(const Ray& r0, const SceneView& S, float bg_scal
                         float env_p = env_pdf_value(light_dir, background);
                         float mis_d = l_pdf * l_pdf + brdf * brdf + sun_p * sun_p + env_p * env_p;
                         if (mis_d > 0.0f && brdf > 0.0f) {
-                            Color nee_emit = lrec.mat->emitted_at(lrec.u, lrec.v, lrec.p, ray.wavelength, ray.spectral_weighted);
+                            Color nee_emit = lrec.mat->emitted_wl(lrec.u, lrec.v, lrec.p, ray.wavelength, ray.spectral_weighted, ray.wl);
+                            nee_emit = nee_emit * h4.weight(ray, ptint, nullptr, hero_bb_temp(lrec, ray));
                             color += hk.add_v(throughput * tr * sr.attenuation * brdf * l_pdf * nee_emit / mis_d, rec, ray, light_dir);
                         }
                     }
@@ -705,7 +764,7 @@ static inline Color ray_color_h(const Ray& r0, const SceneView& S, float bg_scal
                         Color sun_color = background.eval(sun_sample);
                         float area_pdf = lights.objects.empty() ? 0.0f : lights.pdf_value(rec.p, sun_sample, ray.time);
                         float mis_d = sun_pdf * sun_pdf + brdf * brdf + area_pdf * area_pdf;
-                        color += hk.add_v(throughput * tr * sr.attenuation * brdf * sun_pdf * sun_color / mis_d, rec, ray, sun_sample);
+                        color += hk.add_v(throughput * tr * sr.attenuation * brdf * sun_pdf * sun_color * w4 / mis_d, rec, ray, sun_sample);
                     }
                 }
             }
@@ -722,7 +781,7 @@ static inline Color ray_color_h(const Ray& r0, const SceneView& S, float bg_scal
                             float area_pdf = lights.objects.empty() ? 0.0f : lights.pdf_value(rec.p, env_dir, ray.time);
                             Color env_color = background.env_eval(env_dir);
                             float mis_d = env_pdf * env_pdf + brdf * brdf + area_pdf * area_pdf;
-                            color += hk.add_v(throughput * tr * sr.attenuation * brdf * env_pdf * env_color / mis_d, rec, ray, env_dir);
+                            color += hk.add_v(throughput * tr * sr.attenuation * brdf * env_pdf * env_color * w4 / mis_d, rec, ray, env_dir);
                         }
                     }
                 }
@@ -749,7 +808,8 @@ static inline Color ray_color_h(const Ray& r0, const SceneView& S, float bg_scal
             prev_specular = false;
         }
         if (depth >= 2) {
-            float survive = rmax(rmax(throughput.x, throughput.y), throughput.z);
+            Color eff = throughput * h4.weight(ray, ray.spectral_weighted ? ray.tint : 0);
+            float survive = rmax(rmax(eff.x, eff.y), eff.z);
             float p = rmin(survive, 1.0f);
             if (survive <= 0.0f || rng.gen() >= p) break;
             throughput /= p;
@@ -768,6 +828,7 @@ struct SampleParams {
     float bg_scale;
     bool use_frame_time; float t0, t1;
     uint64_t seed;
+    uint8_t wl = 0;             // Ray::wl of the camera rays
 };
 
 // Per-pixel body of render_tiles (src/renderer.rs:693-727).
@@ -793,7 +854,8 @@ static inline Color render_pixel_h(const SceneView& S, const Camera& cam, const 
     float v = ((float)ray_y + vj) / h_denom;
     Ray cam_ray = cam.get_ray(u, v, rng);
     if (P.use_frame_time) cam_ray.time = P.t0 + rng.gen() * (P.t1 - P.t0);
-    cam_ray.wavelength = rng.gen_range(380.0f, 700.0f);
+    cam_ray.wavelength = rng.gen_range(380.0f, P.wl ? 780.0f : 700.0f);
+    cam_ray.wl = P.wl;
     return ray_color_h(cam_ray, S, P.bg_scale, (int)P.max_depth, rng, hk);
 }
 static inline Color render_pixel(const SceneView& S, const Camera& cam, const SampleParams& P, uint32_t row, uint32_t col, SmallRng& rng) {

@@ -326,6 +326,7 @@ SampleParams to_params(const rt_sample_params& p) {
     s.sample_idx = p.sample_idx; s.strata = p.strata; s.width = p.width; s.height = p.height;
     s.max_depth = p.max_depth; s.bg_scale = p.bg_scale; s.use_frame_time = p.use_frame_time != 0;
     s.t0 = p.frame_t0; s.t1 = p.frame_t1; s.seed = p.seed;
+    s.wl = p.wavelengths == 4 ? 2 : (p.wavelengths == 1 ? 1 : 0);
     return s;
 }
 
@@ -484,11 +485,13 @@ int rtc_photon_build(void* s, const rt_photon_params* pp_in, int threads, uint32
     if (pp->emitter == RT_EMIT_QUAD) {
         photon_build_from_quad(S.photons, S.world, Point3(pp->origin[0], pp->origin[1], pp->origin[2]),
                                Vec3(pp->u[0], pp->u[1], pp->u[2]), Vec3(pp->v[0], pp->v[1], pp->v[2]),
-                               Color(pp->color[0], pp->color[1], pp->color[2]), pp->num_photons, pp->gather_radius, threads);
+                               Color(pp->color[0], pp->color[1], pp->color[2]), pp->num_photons, pp->gather_radius, threads,
+                               pp->wavelengths != 0);
     } else {
         Aabb bb; bool have = S.world.photon_bounds(bb);   // PhotonMap::build sizes its disk from photon_bounds (src/photon.rs:61)
         photon_build_sun(S.photons, S.world, have, bb, Vec3(pp->sun_dir[0], pp->sun_dir[1], pp->sun_dir[2]),
-                         Color(pp->color[0], pp->color[1], pp->color[2]), pp->num_photons, pp->gather_radius, threads);
+                         Color(pp->color[0], pp->color[1], pp->color[2]), pp->num_photons, pp->gather_radius, threads,
+                         pp->wavelengths != 0);
     }
     S.photons.knn = pp->knn;
     S.pearl_sun.active = sun_was;
@@ -630,6 +633,13 @@ int rtc_eval(const char* fn, const float* a, float* out) {
     if (f == "metal_ior") { metal_ior((int)a[0], a[1], out[0], out[1]); return 0; }
     if (f == "spectral_to_rgb") { Color c = spectral_to_rgb(a[0]); out[0] = c.x; out[1] = c.y; out[2] = c.z; return 0; }
     if (f == "spectral_fire_rgb") { Color c = spectral_fire_rgb(a[0]); out[0] = c.x; out[1] = c.y; out[2] = c.z; return 0; }
+    if (f == "spectral_to_rgb_wide") { Color c = spectral_to_rgb(a[0], true); out[0] = c.x; out[1] = c.y; out[2] = c.z; return 0; }
+    if (f == "spectral_fire_rgb_wide") { Color c = spectral_fire_rgb(a[0], true); out[0] = c.x; out[1] = c.y; out[2] = c.z; return 0; }
+    if (f == "hero_lambda") { out[0] = hero_lambda(a[0], (int)a[1]); return 0; }
+    if (f == "blackbody_wide") {
+        BlackbodyLight b; b.temp_k = a[0]; b.intensity = a[1]; b.init();
+        out[0] = b.avg_color_w.x; out[1] = b.avg_color_w.y; out[2] = b.avg_color_w.z; out[3] = b.norm_w; return 0;
+    }
     if (f == "planck_raw") { out[0] = planck_raw(a[0], a[1]); return 0; }
     if (f == "blackbody") {
         BlackbodyLight b; b.temp_k = a[0]; b.intensity = a[1]; b.init();

@@ -25,7 +25,7 @@ PARAM_MEDIUM = 0x40000000    # rt_param.material | k: the density of rt_scene_de
  MAT_SPECTRAL_METAL, MAT_ROUGH_DIELECTRIC, MAT_ROUGH_SPECTRAL, MAT_BLACKBODY, MAT_SSS, MAT_PEARL) = \
     1, 2, 3, 4, 5, 6, 7, 8, 9, 10, 11
 BG_SOLID, BG_GRADIENT, BG_PHYSICAL, BG_ENVMAP, BG_ENVMAP_SPLIT = 0, 1, 2, 3, 4
-API_VERSION = 2          # include/rt_api.h RT_API_VERSION: the layouts below
+API_VERSION = 3          # include/rt_api.h RT_API_VERSION: the layouts below
 EMIT_QUAD, EMIT_SUN_DISK, EMIT_SCENE = 0, 1, 2
 NOALLOC_CAPACITY = 1 << 17
 
@@ -99,7 +99,7 @@ class rt_sample_params(C.Structure):
     _fields_ = [("sample_idx", C.c_uint32), ("strata", C.c_uint32), ("width", C.c_uint32),
                 ("height", C.c_uint32), ("bg_scale", C.c_float), ("max_depth", C.c_uint32),
                 ("use_frame_time", C.c_uint32), ("frame_t0", C.c_float), ("frame_t1", C.c_float),
-                ("use_photons", C.c_uint32), ("seed", C.c_uint64)]
+                ("use_photons", C.c_uint32), ("seed", C.c_uint64), ("wavelengths", C.c_uint32)]
 
 
 class rt_wavefront_opts(C.Structure):
@@ -136,7 +136,8 @@ class rt_param(C.Structure):
 class rt_photon_params(C.Structure):
     _fields_ = [("emitter", C.c_uint32), ("origin", C.c_float * 3), ("u", C.c_float * 3),
                 ("v", C.c_float * 3), ("color", C.c_float * 3), ("sun_dir", C.c_float * 3),
-                ("num_photons", C.c_uint32), ("gather_radius", C.c_float), ("knn", C.c_uint32)]
+                ("num_photons", C.c_uint32), ("gather_radius", C.c_float), ("knn", C.c_uint32),
+                ("wavelengths", C.c_uint32)]
 
 
 # Numpy dtypes with the same layout, for batches of rays / hits.

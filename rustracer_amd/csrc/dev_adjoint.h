@@ -54,77 +54,6 @@ template <class R> RT_HD DlT<R> xsqrt(DlT<R> a) { R s = sqrt(a.v); return DlT<R>
 template <class R> RT_HD DlT<R> xlog(DlT<R> a) { return DlT<R>(log(a.v), a.d / a.v); }
 template <class R> RT_HD DlT<R> xabs(DlT<R> a) { return a.v < 0 ? -a : a; }
 using Dd = DlT<double>;
-RT_HD float val(float a) { return a; }
-RT_HD float xsqrt(float a) { return sqrtf(a); }
-RT_HD float xlog(float a) { return logf(a); }
-RT_HD float xabs(float a) { return fabsf(a); }
-RT_HD double val(double a) { return a; }
-RT_HD double xsqrt(double a) { return sqrt(a); }
-RT_HD double xlog(double a) { return log(a); }
-RT_HD double xabs(double a) { return fabs(a); }
-
-// ---- Walter et al. 2007 rough dielectric, log of the BSDF value ---------------------------
-// The same formulas as the sampler (fresnel_dielectric, smith_g1, ggx_ndf),
-// over T = double (the detached ratio) or DlT<double> (the scores).
-template <class T> RT_HD T t_fresnel(T ci, T eta) {
-    T s2i = T(1.0f) - ci * ci;
-    if (val(s2i) < 0.0f) s2i = T(0.0f);
-    T s2t = eta * eta * s2i;
-    if (val(s2t) >= 1.0f) return T(1.0f);
-    T ct = xsqrt(T(1.0f) - s2t);
-    T rs = (eta * ci - ct) / (eta * ci + ct);
-    T rp = (eta * ct - ci) / (eta * ct + ci);
-    return (rs * rs + rp * rp) * 0.5f;
-}
-template <class T> RT_HD T t_g1(float c, T al) { T a2 = al * al; return (2.0f * c) / (c + xsqrt(a2 + (T(1.0f) - a2) * (c * c))); }
-template <class T> RT_HD T t_ggx(T ch, T al) { T a2 = al * al; T d = T(1.0f) + ch * ch * (a2 - 1.0f); return a2 / (PI * d * d); }
-
-// log f(i, o) for unit i (toward the previous vertex), o (the new direction),
-// shading normal n; k = IOR on o's side over IOR on i's side for a
-// transmission (the Fresnel eta is 1/k in both cases).  False where f = 0.
-template <class T>
-RT_HD bool rsd_logf(V3 i, V3 o, V3 n, T k, T al, bool refl, T& lf) {
-    float cin = dot(i, n), con = dot(o, n);
-    float ain = fabsf(cin), aon = fabsf(con);
-    if (!(ain > 0.0f) || !(aon > 0.0f)) return false;
-    T eta = T(1.0f) / k;
-    T G = t_g1(ain, al) * t_g1(aon, al);
-    if (refl) {
-        // half vector in T's precision: at alpha ~ 0.005, 1 - (h.n)^2 is a
-        // few hundred float ulps
-        T hx = T(i.x) + o.x, hy = T(i.y) + o.y, hz = T(i.z) + o.z;
-        T hl = xsqrt(hx * hx + hy * hy + hz * hz);
-        if (!(val(hl) > 0.0f)) return false;
-        T ih = (hx * i.x + hy * i.y + hz * i.z) / hl;
-        if (!(val(ih) * cin > 0.0f)) return false;
-        if (val(ih) > 1.0f) ih = T(1.0f);
-        T F = t_fresnel(ih, eta);
-        T D = t_ggx((hx * n.x + hy * n.y + hz * n.z) / hl, al);
-        if (!(val(F) > 0.0f) || !(val(D) > 0.0f) || !(val(G) > 0.0f)) return false;
-        lf = xlog(F) + xlog(G) + xlog(D) - logf(4.0f * ain * aon);
-        return true;
-    }
-    T hx = i.x + k * o.x, hy = i.y + k * o.y, hz = i.z + k * o.z;
-    T len = xsqrt(hx * hx + hy * hy + hz * hz);
-    if (!(val(len) > 0.0f)) return false;
-    hx = hx / len; hy = hy / len; hz = hz / len;
-    T hn = hx * n.x + hy * n.y + hz * n.z;
-    if (val(hn) < 0.0f) { hx = -hx; hy = -hy; hz = -hz; hn = -hn; }
-    T ih = hx * i.x + hy * i.y + hz * i.z;
-    T oh = hx * o.x + hy * o.y + hz * o.z;
-    // the microfacet must face both directions from their own sides
-    if (!(val(ih) * cin > 0.0f) || !(val(oh) * con > 0.0f)) return false;
-    T aih = xabs(ih);
-    if (val(aih) > 1.0f) aih = T(1.0f);
-    T tr = T(1.0f) - t_fresnel(aih, eta);
-    T den = ih + k * oh;
-    T D = t_ggx(hn, al);
-    if (!(val(tr) > 0.0f) || !(val(D) > 0.0f) || !(val(G) > 0.0f) || val(ih) == 0.0f || val(oh) == 0.0f || val(den) == 0.0f)
-        return false;
-    lf = xlog(tr) + xlog(G) + xlog(D) + xlog(aih) + xlog(xabs(oh)) + 2.0f * xlog(k) - 2.0f * xlog(xabs(den)) - logf(ain * aon);
-    return true;
-}
-
 // d n(lambda) / d B_j of the rough spectral dielectric's dispersion law.
 RT_HD float rsd_dn_dB(const float* p, float lambda, int j) {
     auto term = [&](float lam) {

@@ -81,19 +81,24 @@ RT_D Onb onb_from_w(V3 n) {
 }
 RT_D V3 onb_local(const Onb& o, V3 a) { return a.x * o.u + a.y * o.v + a.z * o.w; }
 
+// wl: 0 the reference's hero wavelength over [380, 700) nm; 1 one wavelength
+// over [380, 780) (the wide CIE table); 2 four hero wavelengths over
+// [380, 780), whose spectral tints are applied per wavelength when a
+// contribution is made (tint: 0 none yet, 1 CMF colour, 2 fire tint).
 struct Ray {
     V3 o, d, inv;
     float time, wavelength;
     bool spectral;
+    uint8_t wl, tint;
 };
 RT_D Ray make_ray(V3 o, V3 d, float time) {
     Ray r; r.o = o; r.d = d; r.inv = mk(1.0f / d.x, 1.0f / d.y, 1.0f / d.z);
-    r.time = time; r.wavelength = 550.0f; r.spectral = false;
+    r.time = time; r.wavelength = 550.0f; r.spectral = false; r.wl = 0; r.tint = 0;
     return r;
 }
 RT_D Ray scatter_from(V3 o, V3 d, const Ray& src) {
     Ray r = make_ray(o, d, src.time);
-    r.wavelength = src.wavelength; r.spectral = src.spectral;
+    r.wavelength = src.wavelength; r.spectral = src.spectral; r.wl = src.wl; r.tint = src.tint;
     return r;
 }
 RT_D V3 at(const Ray& r, float t) { return r.o + t * r.d; }

@@ -78,6 +78,7 @@ struct DScene {
     const float* env_rgb; const float* env_marg; const float* env_cond;
     const float* elut;
     float norms[3], fire_norms[3];
+    float norms_w[3], fire_norms_w[3];     // the same over the wide table (380-780 nm)
     // photon map: left-balanced kd-tree in heap order, structure of arrays
     // (position + split axis in w, power, normal), 16 B per array per photon
     const float4* ph_pos; const float4* ph_pow; const float4* ph_nrm;
@@ -216,7 +217,7 @@ RT_D Ray to_object(const DScene& S, const DObject& ob, const Ray& r) {
         else { float inv = 1.0f / x.v[0]; o = inv * o; d = inv * d; }
     }
     Ray q = make_ray(o, d, r.time);
-    q.wavelength = r.wavelength; q.spectral = r.spectral;
+    q.wavelength = r.wavelength; q.spectral = r.spectral; q.wl = r.wl; q.tint = r.tint;
     return q;
 }
 

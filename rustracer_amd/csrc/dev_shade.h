@@ -7,6 +7,7 @@
 #pragma once
 #include "dev_scene.h"
 #include "spectral_data.h"
+#include "dev_walter.h"
 
 namespace rt {
 
@@ -17,9 +18,9 @@ enum { M_LAMBERTIAN = 1, M_DIELECTRIC = 2, M_SPECTRAL_DIELECTRIC = 3, M_PBR = 4,
 enum { BGK_SOLID = 0, BGK_GRADIENT = 1, BGK_PHYSICAL = 2, BGK_ENV = 3, BGK_ENV_SPLIT = 4 };
 
 // Initialised in place: every kernel translation unit holds its own copy.
-__constant__ float c_cie_x[RT_CIE_N] = {RT_CIE_X_DATA};
-__constant__ float c_cie_y[RT_CIE_N] = {RT_CIE_Y_DATA};
-__constant__ float c_cie_z[RT_CIE_N] = {RT_CIE_Z_DATA};
+__constant__ float c_cie_x[RT_CIE_WIDE_N] = {RT_CIE_X_DATA, RT_CIE_X_TAIL_DATA};
+__constant__ float c_cie_y[RT_CIE_WIDE_N] = {RT_CIE_Y_DATA, RT_CIE_Y_TAIL_DATA};
+__constant__ float c_cie_z[RT_CIE_WIDE_N] = {RT_CIE_Z_DATA, RT_CIE_Z_TAIL_DATA};
 __constant__ float c_metal[6][RT_METAL_N] = {{RT_GOLD_N_DATA}, {RT_GOLD_K_DATA}, {RT_COPPER_N_DATA},
                                              {RT_COPPER_K_DATA}, {RT_SILVER_N_DATA}, {RT_SILVER_K_DATA}};
 
@@ -29,18 +30,22 @@ RT_HD void xyz_srgb(float x, float y, float z, float o[3]) {
     o[1] = fmaxf(-0.9689f * x + 1.8758f * y + 0.0415f * z, 0.0f);
     o[2] = fmaxf(0.0557f * x - 0.2040f * y + 1.0570f * z, 0.0f);
 }
-RT_D void cie_at(float lambda, float& x, float& y, float& z) {
-    float t = fclamp((lambda - RT_CIE_START) / RT_CIE_STEP, 0.0f, (float)(RT_CIE_N - 1));
-    uint32_t i = min(f2u(t), (uint32_t)(RT_CIE_N - 2));
+// wide: the table through 780 nm (the wide wavelength modes), else the
+// reference's, which stops (and clamps) at 700 nm
+RT_D void cie_at(float lambda, float& x, float& y, float& z, bool wide = false) {
+    const int n = wide ? RT_CIE_WIDE_N : RT_CIE_N;
+    float t = fclamp((lambda - RT_CIE_START) / RT_CIE_STEP, 0.0f, (float)(n - 1));
+    uint32_t i = min(f2u(t), (uint32_t)(n - 2));
     float fr = t - (float)i;
     x = c_cie_x[i] + fr * (c_cie_x[i + 1] - c_cie_x[i]);
     y = c_cie_y[i] + fr * (c_cie_y[i + 1] - c_cie_y[i]);
     z = c_cie_z[i] + fr * (c_cie_z[i + 1] - c_cie_z[i]);
 }
-RT_D V3 spectral_to_rgb(const DScene& S, float lambda) {
-    float x, y, z; cie_at(lambda, x, y, z);
+RT_D V3 spectral_to_rgb(const DScene& S, float lambda, bool wide = false) {
+    float x, y, z; cie_at(lambda, x, y, z, wide);
     float c[3]; xyz_srgb(x, y, z, c);
-    return mk(c[0] * S.norms[0], c[1] * S.norms[1], c[2] * S.norms[2]);
+    const float* nm = wide ? S.norms_w : S.norms;
+    return mk(c[0] * nm[0], c[1] * nm[1], c[2] * nm[2]);
 }
 RT_HD void fire_color(float x, float y, float z, float c[3]) {
     float w = fmaxf(powf(y, 0.5f), 0.08f);
@@ -50,10 +55,11 @@ RT_HD void fire_color(float x, float y, float z, float c[3]) {
     float l = 0.2126f * c[0] + 0.7152f * c[1] + 0.0722f * c[2];
     for (int k = 0; k < 3; k++) c[k] = c[k] * (1.0f - d) + l * d;
 }
-RT_D V3 spectral_fire_rgb(const DScene& S, float lambda) {
-    float x, y, z; cie_at(lambda, x, y, z);
+RT_D V3 spectral_fire_rgb(const DScene& S, float lambda, bool wide = false) {
+    float x, y, z; cie_at(lambda, x, y, z, wide);
     float c[3]; fire_color(x, y, z, c);
-    return mk(c[0] * S.fire_norms[0], c[1] * S.fire_norms[1], c[2] * S.fire_norms[2]);
+    const float* nm = wide ? S.fire_norms_w : S.fire_norms;
+    return mk(c[0] * nm[0], c[1] * nm[1], c[2] * nm[2]);
 }
 RT_D void metal_nk(int variant, float lambda, float& n, float& k) {
     const float* ns = c_metal[variant * 2];
@@ -443,9 +449,10 @@ RT_D bool smetal_scatter(const DScene& S, const DMaterial& m, const Ray& r, cons
         w = ((1.0f + lo) / (1.0f + lo + li)) * energy_comp(S, cos_o, alpha);
     }
     float fw = fr * w;
-    sr.att = r.spectral ? mk(fw, fw, fw) : spectral_to_rgb(S, r.wavelength) * fw;
+    // four hero wavelengths: the tint is applied per wavelength when a contribution is made
+    sr.att = (r.spectral || r.wl == 2) ? mk(fw, fw, fw) : spectral_to_rgb(S, r.wavelength, r.wl != 0) * fw;
     sr.ray = scatter_from(h.p, dir, r);
-    if (!r.spectral) sr.ray.spectral = true;
+    if (!r.spectral) { sr.ray.spectral = true; sr.ray.tint = 1; }
     sr.skip_pdf = true;
     return true;
 }
@@ -463,7 +470,7 @@ RT_D V3 smetal_brdf_cos(const DScene& S, const DMaterial& m, const Ray& r, const
     float in, ik; metal_nk((int)m.p[0], r.wavelength, in, ik);
     float fr = fresnel_conductor(woh, in, ik);
     float val = d * fr * g2 / (4.0f * cos_o) * energy_comp(S, cos_o, alpha);
-    return r.spectral ? mk(val, val, val) : spectral_to_rgb(S, r.wavelength) * val;
+    return (r.spectral || r.wl == 2) ? mk(val, val, val) : spectral_to_rgb(S, r.wavelength, r.wl != 0) * val;
 }
 RT_D float smetal_spec_pdf(const DMaterial& m, const Ray& r, const Rec& h, V3 wi) {
     float rough = m.p[1];
@@ -525,9 +532,9 @@ RT_D bool rsd_scatter(const DScene& S, const DMaterial& m, const Ray& r, const R
     if (!rough_dielectric_core(rsd_alpha(m, h), eta, r, h, rng, wi, refl, g1)) return false;
     V3 att = mk(g1, g1, g1);
     bool tinted = false;
-    if (!r.spectral && !refl) { att *= spectral_fire_rgb(S, r.wavelength); tinted = true; }
+    if (!r.spectral && !refl) { if (r.wl != 2) att *= spectral_fire_rgb(S, r.wavelength, r.wl != 0); tinted = true; }
     sr.ray = scatter_from(h.p, wi, r);
-    if (tinted) sr.ray.spectral = true;
+    if (tinted) { sr.ray.spectral = true; sr.ray.tint = 2; }
     sr.att = att; sr.skip_pdf = true;
     return true;
 }
@@ -646,6 +653,11 @@ RT_D float pearl_spec_pdf(const DMaterial& m, const Ray& r, const Rec& h, V3 wi)
     return f * ggx_ndf(hn, al) * smith_g1(cos_o, al) / (4.0f * cos_o);
 }
 
+// SpectralDielectric's IOR at lambda: the Sellmeier law when p5 != 0 (extension), else Cauchy.
+RT_D float sdiel_ior(const DMaterial& m, float lambda) {
+    return m.p[5] != 0.0f ? sellmeier(lambda, m.p + 6, m.p + 9) : cauchy(lambda, m.p[0], m.p[1]);
+}
+
 // Material::scatter dispatch.  Medium events scatter with the HG phase function.
 template <uint32_t MASK = RT_MASK_ALL>
 RT_D bool mat_scatter(const DScene& S, const Rec& h, const Ray& r, Rng& rng, Scatter& sr) {
@@ -671,18 +683,18 @@ RT_D bool mat_scatter(const DScene& S, const Rec& h, const Ray& r, Rng& rng, Sca
         }
         case M_SPECTRAL_DIELECTRIC: {
             if (!RT_IN(MASK, M_SPECTRAL_DIELECTRIC)) return false;
-            float ior = m.p[5] != 0.0f ? sellmeier(r.wavelength, m.p + 6, m.p + 9) : cauchy(r.wavelength, m.p[0], m.p[1]);
+            float ior = sdiel_ior(m, r.wavelength);
             bool refl;
             V3 d = dielectric_boundary(ior, r, h, rng, refl);
             V3 att = mk(1, 1, 1);
             bool tinted = false;
-            if (!r.spectral && !refl) { att = spectral_fire_rgb(S, r.wavelength); tinted = true; }
+            if (!r.spectral && !refl) { if (r.wl != 2) att = spectral_fire_rgb(S, r.wavelength, r.wl != 0); tinted = true; }
             if (!h.front) {
                 float dd = h.t;
                 att.x *= expf(-m.p[2] * dd); att.y *= expf(-m.p[3] * dd); att.z *= expf(-m.p[4] * dd);
             }
             sr.ray = scatter_from(h.p, d, r);
-            if (tinted) sr.ray.spectral = true;
+            if (tinted) { sr.ray.spectral = true; sr.ray.tint = 2; }
             sr.att = att; sr.skip_pdf = true;
             return true;
         }
@@ -710,8 +722,9 @@ RT_D V3 mat_emitted(const DScene& S, const Rec& h, const Ray& r) {
     if (m.kind == M_DIFFUSE_LIGHT) return tex_value(S, m, h.u, h.v, h.p);
     if (m.kind == M_BLACKBODY) {
         // BlackbodyLight::emitted_at (src/material.rs:69-80)
-        if (r.spectral) { float sc = planck_raw(r.wavelength, m.p[0]) * m.p[2] * m.p[1]; return mk(sc, sc, sc); }
-        return mk(m.p[3], m.p[4], m.p[5]);
+        // the wide wavelength modes normalise over the table extended to 780 nm (p6-9)
+        if (r.spectral) { float sc = planck_raw(r.wavelength, m.p[0]) * (r.wl != 0 ? m.p[6] : m.p[2]) * m.p[1]; return mk(sc, sc, sc); }
+        return r.wl != 0 ? mk(m.p[7], m.p[8], m.p[9]) : mk(m.p[3], m.p[4], m.p[5]);
     }
     if (m.kind == M_PBR) return mk(m.p[9], m.p[10], m.p[11]) * m.p[12];
     return mk(0, 0, 0);
@@ -1246,14 +1259,152 @@ RT_D void path_init(PathState& ps, const Ray& r) {
     ps.ray = r; ps.thr = mk(1, 1, 1); ps.prev_specular = true; ps.prev_mis = 1.0f; ps.prev_sun_w = 1.0f; ps.knn_irr = mk(0, 0, 0);
 }
 
+// ---- four hero wavelengths (extension) -------------------------------------------------
+// Hero wavelength spectral sampling (Wilkie, Nawaz, Droske, Weidlich and
+// Hanika, EGSR 2014): a camera path carries lambda_0 uniform over [380, 780)
+// and the rotations lambda_k = lambda_0 + 100 k nm (mod the range), k = 1..3.
+// Every decision is drawn at lambda_0 and every contribution is computed at
+// lambda_0 without a spectral tint; it is then weighted by the balance
+// heuristic over the four rotations,
+//     sum_j (f_j / f_0) E_j T(lambda_j)  /  sum_j (p_j / p_0),
+// where f_j / f_0 and p_j / p_0 are the products over the path's vertices of
+// the BSDF-value and sampling-density ratios (q and r below; 1 at lambda_0),
+// E_j / E_0 a blackbody emitter's ratio, and T the tint of the path's first
+// spectral event (the CMF colour of a spectral metal, the fire tint of a
+// dispersive refraction) evaluated per wavelength.  A dispersive specular
+// refraction has density 0 at the other wavelengths: q and r drop to 0 there
+// and the path goes on as a one-wavelength path.  Paths without a spectral
+// event get weight 1, so they are the one-wavelength estimator's.  Next-event
+// contributions are weighted on the path up to their vertex with the
+// connection's own BSDF ratio; the next-event / BSDF MIS weights use
+// lambda_0's densities, which at every spectral material that samples the
+// sun (the spectral metal) do not depend on the wavelength.
+struct Spec4 {
+    float q[3], r[3];
+    bool on;
+};
+RT_D void spec4_init(Spec4& s) {
+    for (int k = 0; k < 3; k++) { s.q[k] = 1.0f; s.r[k] = 1.0f; }
+    s.on = false;
+}
+RT_D float hero_lambda(float l0, int k) {
+    float t = (l0 - 380.0f) + 100.0f * (float)k;
+    if (t >= 400.0f) t -= 400.0f;
+    return 380.0f + t;
+}
+RT_D V3 spec4_tint(const DScene& S, int kind, float lam) {
+    if (kind == 1) return spectral_to_rgb(S, lam, true);
+    if (kind == 2) return spectral_fire_rgb(S, lam, true);
+    return mk(1, 1, 1);
+}
+// Weight of a contribution made at lambda_0 (see above): kind is the tint,
+// b (may be null) the connection's own BSDF ratios f_k / f_0, bb_temp > 0 a
+// blackbody emitter's temperature (its radiance ratio per wavelength).
+// Paths of the other modes (a batch may mix them) get 1.
+RT_D V3 spec4_weight(const DScene& S, const Ray& ray, const Spec4& s, int kind, const float* b, float bb_temp) {
+    if (ray.wl != 2) return mk(1, 1, 1);
+    if (!s.on && kind == 0 && b == nullptr && !(bb_temp > 0.0f)) return mk(1, 1, 1);
+    float l0 = ray.wavelength;
+    float e0 = bb_temp > 0.0f ? planck_raw(l0, bb_temp) : 1.0f;
+    V3 num = spec4_tint(S, kind, l0);
+    float den = 1.0f;
+    for (int k = 0; k < 3; k++) {
+        float lk = hero_lambda(l0, k + 1);
+        float w = s.q[k] * (b ? b[k] : 1.0f);
+        if (bb_temp > 0.0f) w *= planck_raw(lk, bb_temp) / e0;
+        if (w != 0.0f) num += spec4_tint(S, kind, lk) * w;
+        den += s.r[k];
+    }
+    return num / den;
+}
+// A spectral metal's BSDF ratio f_k / f_0 between wo and wi: the conductor
+// Fresnel factor at the half vector (the rest does not depend on lambda).
+RT_D void spec4_metal_ratio(const DMaterial& m, V3 wo, V3 wi, float l0, float b[3]) {
+    float ch = fmx(dot(wo, unit(wo + wi)), 0.0f);
+    float n0, k0; metal_nk((int)m.p[0], l0, n0, k0);
+    float f0 = fresnel_conductor(ch, n0, k0);
+    for (int k = 0; k < 3; k++) {
+        float nk, kk; metal_nk((int)m.p[0], hero_lambda(l0, k + 1), nk, kk);
+        b[k] = f0 > 0.0f ? fresnel_conductor(ch, nk, kk) / f0 : 0.0f;
+    }
+}
+// q and r after the scatter from `ray` at h toward wi (spectral materials).
+RT_D void spec4_scatter(const DScene& S, const Rec& h, const Ray& ray, V3 wi, Spec4& s) {
+    if (h.medium || ray.wl != 2) return;
+    const DMaterial& m = S.mats[h.mat];
+    V3 wo = unit(-ray.d);
+    float l0 = ray.wavelength;
+    if (m.kind == M_SPECTRAL_METAL) {
+        float b[3];
+        spec4_metal_ratio(m, wo, wi, l0, b);
+        for (int k = 0; k < 3; k++) s.q[k] *= b[k];
+        s.on = true;
+    } else if (m.kind == M_SPECTRAL_DIELECTRIC) {
+        s.on = true;
+        if (dot(wi, h.n) <= 0.0f) {                       // refraction: a delta at lambda_0 only
+            for (int k = 0; k < 3; k++) { s.q[k] = 0.0f; s.r[k] = 0.0f; }
+            return;
+        }
+        // reflection: probability 1 under total internal reflection, else F
+        // (dielectric_boundary); the BSDF value over it is 1
+        float c = fmn(dot(wo, h.n), 1.0f);
+        float sn = sqrtf(1.0f - c * c);
+        auto prefl = [&](float lam) {
+            float ior = sdiel_ior(m, lam);
+            float ratio = h.front ? 1.0f / ior : ior;
+            return ratio * sn > 1.0f ? 1.0f : fresnel_dielectric(c, ratio);
+        };
+        float p0 = prefl(l0);
+        for (int k = 0; k < 3; k++) {
+            float f = p0 > 0.0f ? prefl(hero_lambda(l0, k + 1)) / p0 : 0.0f;
+            s.q[k] *= f; s.r[k] *= f;
+        }
+    } else if (m.kind == M_ROUGH_SPECTRAL) {
+        s.on = true;
+        double al = (double)rsd_alpha(m, h);
+        double n0 = (double)rsd_ior(m, l0);
+        if (dot(wi, h.n) > 0.0f) {                        // reflection: only F depends on lambda
+            float com = fmn(dot(wo, unit(wo + wi)), 1.0f);
+            float f0 = fresnel_dielectric(com, h.front ? 1.0f / (float)n0 : (float)n0);
+            for (int k = 0; k < 3; k++) {
+                float nk = rsd_ior(m, hero_lambda(l0, k + 1));
+                float f = f0 > 0.0f ? fresnel_dielectric(com, h.front ? 1.0f / nk : nk) / f0 : 0.0f;
+                s.q[k] *= f; s.r[k] *= f;
+            }
+            return;
+        }
+        double lf0, lp0;
+        double k0 = h.front ? n0 : 1.0 / n0;
+        bool ok0 = rsd_logf<double>(wo, wi, h.n, k0, al, false, lf0) && rsd_logpdf_t<double>(wo, wi, h.n, k0, al, lp0);
+        for (int k = 0; k < 3; k++) {
+            double nk = (double)rsd_ior(m, hero_lambda(l0, k + 1));
+            double kk = h.front ? nk : 1.0 / nk;
+            double lf, lp;
+            if (ok0 && rsd_logf<double>(wo, wi, h.n, kk, al, false, lf) && rsd_logpdf_t<double>(wo, wi, h.n, kk, al, lp)) {
+                s.q[k] *= (float)exp(lf - lf0);
+                s.r[k] *= (float)exp(lp - lp0);
+            } else {
+                s.q[k] = 0.0f; s.r[k] = 0.0f;
+            }
+        }
+    }
+}
+RT_D float spec4_bb_temp(const DScene& S, const Rec& h, const Ray& ray) {
+    if (h.medium || !ray.spectral) return 0.0f;
+    const DMaterial& m = S.mats[h.mat];
+    return m.kind == M_BLACKBODY ? m.p[0] : 0.0f;
+}
+
 // Shade the hit (or miss) `id` of ps.ray at bounce `depth`.  Adds direct
 // emission / background to `color`, appends up to three shadow requests and
-// advances the path.  Returns false when the path ends.
-template <class DH, uint32_t MASK = RT_MASK_ALL>
+// advances the path.  Returns false when the path ends.  H4: the path carries
+// four hero wavelengths, their ratios in *s4.
+template <class DH, uint32_t MASK = RT_MASK_ALL, bool H4 = false>
 RT_D bool bounce_t(const DScene& S, PathState& ps, const HitId& id, int depth, float bg_scale, bool use_photons, Rng& rng,
-                   V3& color, ShadowReq* sh, int& nsh, Stack& st, DH& dh) {
+                   V3& color, ShadowReq* sh, int& nsh, Stack& st, DH& dh, Spec4* s4 = nullptr) {
     Ray& ray = ps.ray;
     dh.segment(S, ray, id);
+    const int ptint = ray.spectral ? ray.tint : 0;    // the tint the path carries into this vertex
     const bool lights_empty = S.num_lights == 0;
     const bool sun = physical_sun(S);
     const bool env = has_env(S);
@@ -1263,12 +1414,17 @@ RT_D bool bounce_t(const DScene& S, PathState& ps, const HitId& id, int depth, f
         float ep = env_pdf_value(S, ray.d);
         float ew = (sp + ep == 0.0f) ? 1.0f : (ps.prev_specular ? ps.prev_sun_w : ps.prev_mis);
         V3 bg = (depth == 0 || !env) ? bg_eval(S, ray.d) : env_eval(S, ray.d);
+        if constexpr (H4) bg *= spec4_weight(S, ray, *s4, ptint, nullptr, 0.0f);
         color += dh.emit(ps.thr * bg * bg_scale * ew);
         return false;
     }
     Rec h = finalize(S, ray, id, 0.001f);
     float ew = (ps.prev_specular || lights_empty) ? 1.0f : ps.prev_mis;
-    color += dh.emit(ps.thr * mat_emitted(S, h, ray) * ew);
+    V3 em = mat_emitted(S, h, ray);
+    if constexpr (H4) if (em.x != 0.0f || em.y != 0.0f || em.z != 0.0f) em *= spec4_weight(S, ray, *s4, ptint, nullptr, spec4_bb_temp(S, h, ray));
+    color += dh.emit(ps.thr * em * ew);
+    V3 w4 = mk(1, 1, 1);                                // this vertex's next-event weight (four wavelengths)
+    if constexpr (H4) w4 = spec4_weight(S, ray, *s4, ptint, nullptr, 0.0f);
     Scatter sr;
     if (!mat_scatter<MASK>(S, h, ray, rng, sr)) return false;
     constexpr bool SPEC_ONLY = (MASK & ~RT_MASK_SPECULAR) == 0u;   // every scatter here is skip_pdf
@@ -1282,6 +1438,15 @@ RT_D bool bounce_t(const DScene& S, PathState& ps, const HitId& id, int depth, f
             if (bc.x > 0.0f || bc.y > 0.0f || bc.z > 0.0f) {
                 float pm = mat_spec_pdf<MASK>(S, h, ray, ss);
                 V3 sc = bg_eval(S, ss);
+                if constexpr (H4) {
+                    if (!h.medium && S.mats[h.mat].kind == M_SPECTRAL_METAL) {
+                        float b[3];
+                        spec4_metal_ratio(S.mats[h.mat], unit(-ray.d), ss, ray.wavelength, b);
+                        sc *= spec4_weight(S, ray, *s4, ray.spectral ? ray.tint : 1, b, 0.0f);
+                    } else {
+                        sc *= w4;
+                    }
+                }
                 float md = p_sun * p_sun + pm * pm;
                 if (md > 0.0f) { sh[nsh].o = h.p; sh[nsh].d = ss; sh[nsh].t_max = FINF; sh[nsh].time = ray.time; sh[nsh].contrib = ps.thr * bc * sc * p_sun / md; dh.shadow(sh[nsh], nsh, S, h, ray, false); nsh++; }
             }
@@ -1289,6 +1454,7 @@ RT_D bool bounce_t(const DScene& S, PathState& ps, const HitId& id, int depth, f
             if (pmb > 0.0f) ps.prev_sun_w = (pmb * pmb) / fmx(pmb * pmb + p_sun * p_sun, 1e-8f);
         }
         dh.scatter(S, h, ray, sr.ray.d);
+        if constexpr (H4) spec4_scatter(S, h, ray, sr.ray.d, *s4);
         ps.thr *= sr.att;
         ray = sr.ray;
         ps.prev_specular = true;
@@ -1297,7 +1463,7 @@ RT_D bool bounce_t(const DScene& S, PathState& ps, const HitId& id, int depth, f
         const V3 sn = mat_shading_normal(S, h);
         if (use_photons) {
             V3 irr = photon_irradiance(S, h.p, h.n, ps.knn_irr, st);
-            if (irr.x > 0.0f || irr.y > 0.0f || irr.z > 0.0f) color += dh.emit_vertex(ps.thr * mat_albedo_hint(S, h) * irr, S, h, ray);
+            if (irr.x > 0.0f || irr.y > 0.0f || irr.z > 0.0f) color += dh.emit_vertex(ps.thr * mat_albedo_hint(S, h) * irr * w4, S, h, ray);
         }
         if (!lights_empty) {
             V3 ldir = lights_generate(S, h.p, rng, ray.time);
@@ -1311,7 +1477,9 @@ RT_D bool bounce_t(const DScene& S, PathState& ps, const HitId& id, int depth, f
                 float md = l_pdf * l_pdf + brdf * brdf + sp * sp + ep * ep;
                 if (md > 0.0f && brdf > 0.0f) {
                     sh[nsh].o = h.p; sh[nsh].d = ldir; sh[nsh].t_max = lrec.t * (1.0f - 1e-4f); sh[nsh].time = ray.time;
-                    sh[nsh].contrib = ps.thr * sr.att * brdf * l_pdf * mat_emitted(S, lrec, ray) / md;
+                    V3 le = mat_emitted(S, lrec, ray);
+                    if constexpr (H4) le *= spec4_weight(S, ray, *s4, ptint, nullptr, spec4_bb_temp(S, lrec, ray));
+                    sh[nsh].contrib = ps.thr * sr.att * brdf * l_pdf * le / md;
                     dh.shadow(sh[nsh], nsh, S, h, ray, true);
                     nsh++;
                 }
@@ -1327,7 +1495,7 @@ RT_D bool bounce_t(const DScene& S, PathState& ps, const HitId& id, int depth, f
                 float ap = lights_empty ? 0.0f : lights_pdf_value(S, h.p, ss, ray.time, st);
                 float md = spdf * spdf + brdf * brdf + ap * ap;
                 sh[nsh].o = h.p; sh[nsh].d = ss; sh[nsh].t_max = FINF; sh[nsh].time = ray.time;
-                sh[nsh].contrib = ps.thr * sr.att * brdf * spdf * sc / md;
+                sh[nsh].contrib = ps.thr * sr.att * brdf * spdf * sc * w4 / md;
                 dh.shadow(sh[nsh], nsh, S, h, ray, true);
                 nsh++;
             }
@@ -1342,7 +1510,7 @@ RT_D bool bounce_t(const DScene& S, PathState& ps, const HitId& id, int depth, f
                     V3 ec = env_eval(S, ed);
                     float md = epdf * epdf + brdf * brdf + ap * ap;
                     sh[nsh].o = h.p; sh[nsh].d = ed; sh[nsh].t_max = FINF; sh[nsh].time = ray.time;
-                    sh[nsh].contrib = ps.thr * sr.att * brdf * epdf * ec / md;
+                    sh[nsh].contrib = ps.thr * sr.att * brdf * epdf * ec * w4 / md;
                     dh.shadow(sh[nsh], nsh, S, h, ray, true);
                     nsh++;
                 }
@@ -1370,7 +1538,9 @@ RT_D bool bounce_t(const DScene& S, PathState& ps, const HitId& id, int depth, f
         ps.prev_specular = false;
     }
     if (depth >= 2) {
-        float survive = fmx(fmx(ps.thr.x, ps.thr.y), ps.thr.z);
+        V3 eff = ps.thr;                                 // four wavelengths: with the path's spectral weight
+        if constexpr (H4) eff *= spec4_weight(S, ray, *s4, ray.spectral ? ray.tint : 0, nullptr, 0.0f);
+        float survive = fmx(fmx(eff.x, eff.y), eff.z);
         float p = fmn(survive, 1.0f);
         if (survive <= 0.0f || rng_f(rng) >= p) return false;
         ps.thr /= p;
@@ -1386,23 +1556,27 @@ struct NoDiff {
     RT_D void shadow(ShadowReq&, int, const DScene&, const Rec&, const Ray&, bool) {}
     RT_D void scatter(const DScene&, const Rec&, const Ray&, V3) {}
 };
-template <uint32_t MASK = RT_MASK_ALL>
+template <uint32_t MASK = RT_MASK_ALL, bool H4 = false>
 RT_D bool bounce(const DScene& S, PathState& ps, const HitId& id, int depth, float bg_scale, bool use_photons, Rng& rng,
-                 V3& color, ShadowReq* sh, int& nsh, Stack& st) {
+                 V3& color, ShadowReq* sh, int& nsh, Stack& st, Spec4* s4 = nullptr) {
     NoDiff nd;
-    return bounce_t<NoDiff, MASK>(S, ps, id, depth, bg_scale, use_photons, rng, color, sh, nsh, st, nd);
+    return bounce_t<NoDiff, MASK, H4>(S, ps, id, depth, bg_scale, use_photons, rng, color, sh, nsh, st, nd, s4);
 }
 
 // Megakernel form of ray_color: trace, shade, resolve shadow rays inline.
+// H4: the ray carries four hero wavelengths (wl 2).
+template <bool H4 = false>
 RT_D V3 ray_color(const DScene& S, Ray ray, int max_depth, float bg_scale, bool use_photons, Rng& rng, Stack& st) {
     PathState ps;
     path_init(ps, ray);
+    Spec4 s4;
+    spec4_init(s4);
     V3 color = mk(0, 0, 0);
     ShadowReq sh[3];
     for (int depth = 0; depth < max_depth; depth++) {
         HitId id = trace_closest(S, ps.ray, 0.001f, FINF, st);
         int nsh = 0;
-        bool cont = bounce(S, ps, id, depth, bg_scale, use_photons, rng, color, sh, nsh, st);
+        bool cont = bounce<RT_MASK_ALL, H4>(S, ps, id, depth, bg_scale, use_photons, rng, color, sh, nsh, st, &s4);
         for (int k = 0; k < nsh; k++) {
             float tr = trace_transmittance(S, make_ray(sh[k].o, sh[k].d, sh[k].time), 0.001f, sh[k].t_max, st, false);
             if (tr > 0.0f) color += sh[k].contrib * tr;
@@ -1435,6 +1609,7 @@ RT_D Ray camera_ray(const DCamera& c, float s, float t, Rng& rng) {
 
 struct DParams {
     uint32_t sample_idx, strata, width, height, max_depth, use_frame_time, use_photons;
+    uint32_t wl;                    // Ray::wl of the camera rays: 0 reference, 1 wide, 2 four hero wavelengths
     float bg_scale, t0, t1;
     uint64_t seed;
     unsigned long long* counters;   // counting pass: [0] node tests, [1] primitive tests, [2] rays, [3] samples
@@ -1459,7 +1634,8 @@ RT_D Ray pixel_ray(const DCamera& cam, const DParams& P, uint32_t row, uint32_t 
     float v = ((float)ray_y + vj) / h_denom;
     Ray r = camera_ray(cam, u, v, rng);
     if (P.use_frame_time) r.time = P.t0 + rng_f(rng) * (P.t1 - P.t0);
-    r.wavelength = rng_range(rng, 380.0f, 700.0f);
+    r.wavelength = rng_range(rng, 380.0f, P.wl != 0 ? 780.0f : 700.0f);
+    r.wl = (uint8_t)P.wl;
     return r;
 }
 

@@ -22,11 +22,12 @@ __global__ __launch_bounds__(BLOCK) void k_photon_trace(DScene S, DEmitter E, ui
     }
     V3 power = mk(E.power[0], E.power[1], E.power[2]);
     uint32_t spec_depth = 0;
-    float wl = rng_range(rng, 380.0f, 700.0f);
+    float wl = rng_range(rng, 380.0f, E.wide ? 780.0f : 700.0f);
     uint32_t stored = 0;
     for (int d = 0; d < 12; d++) {
         Ray ray = make_ray(pos, dir, 0.0f);
         ray.wavelength = wl;
+        ray.wl = E.wide ? 1 : 0;
         HitId id = trace_closest(S, ray, 0.001f, FINF, st);
         if (id.slot < 0) break;
         Rec h = finalize(S, ray, id, 0.001f);

@@ -2,8 +2,10 @@
 #include "rt_kern.h"
 
 // Plain megakernel: one thread per pixel, `spp` consecutive sample passes,
-// radiance summed in registers and added to (or stored into) `out`.
-template <bool KNN>
+// radiance summed in registers and added to (or stored into) `out`.  H4:
+// four hero wavelengths per camera ray (an instantiation of its own, so the
+// reference path keeps its register budget).
+template <bool KNN, bool H4>
 __global__ __launch_bounds__(BLOCK) void k_render(DScene S, DCamera cam, DParams P, const uint8_t* __restrict__ conv,
                                                   float* __restrict__ out, uint32_t spp, int accumulate) {
     Stack st; init_stack(st);
@@ -18,7 +20,7 @@ __global__ __launch_bounds__(BLOCK) void k_render(DScene S, DCamera cam, DParams
             uint32_t sidx = P.sample_idx + k;
             Rng rng = rng_seed(pixel_seed(i, sidx, P.seed));
             Ray r = pixel_ray(cam, P, row, col, sidx, rng);
-            V3 c = ray_color(S, r, (int)P.max_depth, P.bg_scale, P.use_photons != 0, rng, st);
+            V3 c = ray_color<H4>(S, r, (int)P.max_depth, P.bg_scale, P.use_photons != 0, rng, st);
             sx += c.x; sy += c.y; sz += c.z;
         }
     }
@@ -35,5 +37,7 @@ __global__ __launch_bounds__(BLOCK) void k_render(DScene S, DCamera cam, DParams
     }
 }
 
-template __global__ void k_render<true>(DScene, DCamera, DParams, const uint8_t*, float*, uint32_t, int);
-template __global__ void k_render<false>(DScene, DCamera, DParams, const uint8_t*, float*, uint32_t, int);
+template __global__ void k_render<true, false>(DScene, DCamera, DParams, const uint8_t*, float*, uint32_t, int);
+template __global__ void k_render<false, false>(DScene, DCamera, DParams, const uint8_t*, float*, uint32_t, int);
+template __global__ void k_render<true, true>(DScene, DCamera, DParams, const uint8_t*, float*, uint32_t, int);
+template __global__ void k_render<false, true>(DScene, DCamera, DParams, const uint8_t*, float*, uint32_t, int);

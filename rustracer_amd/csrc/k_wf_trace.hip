@@ -28,6 +28,7 @@ __global__ __launch_bounds__(BLOCK) void k_wf_generate(DCamera cam0, DParams P0,
     path_init(ps, r);
     WfPath w;
     wf_store(w, ps, rng);
+    if (Q.h4) { Spec4 s4; spec4_init(s4); spec4_store(Q, i, s4); }
     w.pixel = fb_off + pixel; w.pad0 = view; w.hit_t = 0.0f; w.hit_slot = -1; w.hit_prim = -1; w.key = 0; w.sid = i;
     Q.paths[i] = w;
     if (Q.conv) {                         // compact: only unflagged pixels enter the active set

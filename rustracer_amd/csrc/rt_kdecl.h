@@ -26,8 +26,8 @@ template <bool KNN> __global__ void k_photon_irr(DScene S, const float* pn, uint
 __global__ void k_fill_u32(uint32_t* p, uint64_t n, uint32_t v);
 
 // k_render.hip, k_progressive.hip
-template <bool KNN> __global__ void k_render(DScene S, DCamera cam, DParams P, const uint8_t* conv, float* out, uint32_t spp, int accumulate);
-template <bool KNN> __global__ void k_progressive(DScene S, DCamera cam, DParams P, rt_accum_opts o, uint32_t pass, float* acc,
+template <bool KNN, bool H4> __global__ void k_render(DScene S, DCamera cam, DParams P, const uint8_t* conv, float* out, uint32_t spp, int accumulate);
+template <bool KNN, bool H4> __global__ void k_progressive(DScene S, DCamera cam, DParams P, rt_accum_opts o, uint32_t pass, float* acc,
                                                   uint32_t* ns, float* m2, uint8_t* conv, uint32_t* ctl);
 
 // k_adjoint.hip
@@ -46,7 +46,7 @@ template <int OCC, bool BATCH, int BS> __global__ void k_wf_shadow(DScene S, WfQ
 template <bool BATCH> __global__ void k_wf_knn(DScene S0, DParams P0, WfQueues Q, WfBatch B);
 template <bool BATCH> __global__ void k_wf_knn_dense(DScene S0, WfQueues Q, WfBatch B);
 template <bool BATCH> __global__ void k_wf_knn_wave(DScene S0, WfQueues Q, WfBatch B);
-template <bool BATCH, int CLASS> __global__ void k_wf_shade(DScene S, DParams P, WfQueues Q, int depth, WfBatch B);
+template <bool BATCH, int CLASS, bool H4> __global__ void k_wf_shade(DScene S, DParams P, WfQueues Q, int depth, WfBatch B);
 
 // k_wfb.hip
 __global__ void k_wfb_init(WfQueues Q, WfAdjQ A, uint32_t n);

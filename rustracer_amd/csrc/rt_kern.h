@@ -70,6 +70,7 @@ struct DEmitter {
     float power[3];
     float lum_clamp;
     float disk_c[3], disk_r, t[3], b[3], sun_down[3];
+    uint32_t wide;             // photon wavelengths over [380, 780) (Ray::wl 1), else [380, 700)
 };
 
 // ============================================================================ wavefront
@@ -95,7 +96,7 @@ struct __align__(16) WfPath {
     float4 o_time;       // ray origin, time
     float4 d_wl;         // ray direction, wavelength
     float4 thr_mis;      // throughput, prev_mis
-    float sun_w; uint32_t flags; uint32_t pixel; uint32_t pad0;   // flags: 1 spectral, 2 prev_specular
+    float sun_w; uint32_t flags; uint32_t pixel; uint32_t pad0;   // flags: 1 spectral, 2 prev_specular, bits 2-3 Ray::wl, 4-5 Ray::tint
     uint64_t rng[4];
     float hit_t; int32_t hit_slot; int32_t hit_prim; uint32_t key;
     uint32_t sid;        // sample index within the chunk (per-sample radiance of the adjoint's primal pass)
@@ -125,6 +126,10 @@ struct WfQueues {
     // lane (on the 2M-photon gem at 1280 x 720 the crossover lies between
     // 64k and 256k queries; camera-ray bounces exceed it, later ones do not)
     uint32_t knn_wave_max = 65536;
+    // four hero wavelengths: 2 per path, {q0, q1, q2, r0} {r1, r2, on, 0} (Spec4),
+    // read and written by the shade stage's four-wavelength kernels (h4 != 0)
+    float4* spec = nullptr;
+    uint32_t h4 = 0;
 };
 static_assert(std::is_trivially_copyable<WfQueues>::value, "kernel argument");
 
@@ -152,6 +157,8 @@ __device__ __forceinline__ void wf_load(const WfPath& w, PathState& ps, Rng& rng
     ps.ray = make_ray(mk(w.o_time.x, w.o_time.y, w.o_time.z), mk(w.d_wl.x, w.d_wl.y, w.d_wl.z), w.o_time.w);
     ps.ray.wavelength = w.d_wl.w;
     ps.ray.spectral = (w.flags & 1u) != 0;
+    ps.ray.wl = (uint8_t)((w.flags >> 2) & 3u);
+    ps.ray.tint = (uint8_t)((w.flags >> 4) & 3u);
     ps.thr = mk(w.thr_mis.x, w.thr_mis.y, w.thr_mis.z);
     ps.prev_mis = w.thr_mis.w;
     ps.prev_sun_w = w.sun_w;
@@ -159,12 +166,20 @@ __device__ __forceinline__ void wf_load(const WfPath& w, PathState& ps, Rng& rng
     ps.knn_irr = mk(__uint_as_float(w.pad1[0]), __uint_as_float(w.pad1[1]), __uint_as_float(w.pad1[2]));
     rng.s0 = w.rng[0]; rng.s1 = w.rng[1]; rng.s2 = w.rng[2]; rng.s3 = w.rng[3];
 }
+__device__ __forceinline__ void spec4_load(const WfQueues& Q, uint32_t pid, Spec4& s) {
+    float4 a = Q.spec[2 * (size_t)pid], b = Q.spec[2 * (size_t)pid + 1];
+    s.q[0] = a.x; s.q[1] = a.y; s.q[2] = a.z; s.r[0] = a.w; s.r[1] = b.x; s.r[2] = b.y; s.on = b.z != 0.0f;
+}
+__device__ __forceinline__ void spec4_store(const WfQueues& Q, uint32_t pid, const Spec4& s) {
+    Q.spec[2 * (size_t)pid] = make_float4(s.q[0], s.q[1], s.q[2], s.r[0]);
+    Q.spec[2 * (size_t)pid + 1] = make_float4(s.r[1], s.r[2], s.on ? 1.0f : 0.0f, 0.0f);
+}
 __device__ __forceinline__ void wf_store(WfPath& w, const PathState& ps, const Rng& rng) {
     w.o_time = make_float4(ps.ray.o.x, ps.ray.o.y, ps.ray.o.z, ps.ray.time);
     w.d_wl = make_float4(ps.ray.d.x, ps.ray.d.y, ps.ray.d.z, ps.ray.wavelength);
     w.thr_mis = make_float4(ps.thr.x, ps.thr.y, ps.thr.z, ps.prev_mis);
     w.sun_w = ps.prev_sun_w;
-    w.flags = (ps.ray.spectral ? 1u : 0u) | (ps.prev_specular ? 2u : 0u);
+    w.flags = (ps.ray.spectral ? 1u : 0u) | (ps.prev_specular ? 2u : 0u) | ((uint32_t)ps.ray.wl << 2) | ((uint32_t)ps.ray.tint << 4);
     w.rng[0] = rng.s0; w.rng[1] = rng.s1; w.rng[2] = rng.s2; w.rng[3] = rng.s3;
 }
 

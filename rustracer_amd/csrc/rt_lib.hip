@@ -36,6 +36,7 @@ struct rt_ctx {
     std::string err;
     float* elut = nullptr;
     float norms[3], fire_norms[3];
+    float norms_w[3], fire_norms_w[3];     // over the wide table (380-780 nm)
     // measurement
     bool timing = false;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
@@ -172,15 +173,17 @@ static int finish_out(rt_ctx* c, void* p, size_t bytes, Staged& s) {
 }
 
 // ---- spectral normalisation (src/spectrum.rs:80-95, 162-178), host f32 ----------
-static void spectral_norms(float norms[3], float fire[3]) {
+// n = RT_CIE_N: the reference's table (380-700 nm); n = RT_CIE_WIDE_N: the
+// same normalisation over 380-780 nm for the wide wavelength modes.
+static void spectral_norms(float norms[3], float fire[3], int n = RT_CIE_N) {
     float s[3] = {0, 0, 0}, f[3] = {0, 0, 0};
-    for (int i = 0; i < RT_CIE_N; i++) {
-        float rgb[3]; xyz_srgb(RT_CIE_X[i], RT_CIE_Y[i], RT_CIE_Z[i], rgb);
+    for (int i = 0; i < n; i++) {
+        float rgb[3]; xyz_srgb(RT_CIE_WX[i], RT_CIE_WY[i], RT_CIE_WZ[i], rgb);
         s[0] += rgb[0]; s[1] += rgb[1]; s[2] += rgb[2];
-        float c[3]; fire_color(RT_CIE_X[i], RT_CIE_Y[i], RT_CIE_Z[i], c);
+        float c[3]; fire_color(RT_CIE_WX[i], RT_CIE_WY[i], RT_CIE_WZ[i], c);
         f[0] += c[0]; f[1] += c[1]; f[2] += c[2];
     }
-    for (int k = 0; k < 3; k++) { norms[k] = (float)RT_CIE_N / std::fmax(s[k], 1e-6f); fire[k] = (float)RT_CIE_N / std::fmax(f[k], 1e-6f); }
+    for (int k = 0; k < 3; k++) { norms[k] = (float)n / std::fmax(s[k], 1e-6f); fire[k] = (float)n / std::fmax(f[k], 1e-6f); }
 }
 
 // ---- wavefront buffer layout ------------------------------------------------------------------
@@ -189,7 +192,7 @@ static void spectral_norms(float norms[3], float fire[3]) {
 constexpr size_t wf_al(size_t x) { return (x + 255) & ~(size_t)255; }
 constexpr size_t wf_layout_bytes(uint32_t cap) {
     return wf_al((size_t)cap * sizeof(WfPath)) + 3 * wf_al((size_t)cap * 4) + wf_al((size_t)WF_NB * cap * 4) +
-           wf_al((size_t)3 * cap * sizeof(WfShadow)) + 256;
+           wf_al((size_t)3 * cap * sizeof(WfShadow)) + wf_al((size_t)2 * cap * sizeof(float4)) + 256;
 }
 static void wf_layout(char* m, uint32_t cap, WfQueues& Q) {
     Q.paths = (WfPath*)m; m += wf_al((size_t)cap * sizeof(WfPath));
@@ -198,6 +201,7 @@ static void wf_layout(char* m, uint32_t cap, WfQueues& Q) {
     Q.dense = (uint32_t*)m; m += wf_al((size_t)cap * 4);
     Q.buckets = (uint32_t*)m; m += wf_al((size_t)WF_NB * cap * 4);
     Q.shadow = (WfShadow*)m; m += wf_al((size_t)3 * cap * sizeof(WfShadow));
+    Q.spec = (float4*)m; m += wf_al((size_t)2 * cap * sizeof(float4));
     Q.counters = (uint32_t*)m;
     Q.cap = cap;
     Q.fb = nullptr;
@@ -208,7 +212,7 @@ static void wf_layout(char* m, uint32_t cap, WfQueues& Q) {
 }
 
 // The no-allocation wavefront's queues: a file-scope device array of one
-// lane's buffers at RT_NOALLOC_CAPACITY paths (48 MiB), held by every GPU the
+// lane's buffers at RT_NOALLOC_CAPACITY paths (52 MiB), held by every GPU the
 // library runs on until the process ends.  Host code never uses the name
 // itself as a device address: rt_ctx_create looks up this device's copy once
 // (hipGetSymbolAddress, with the device current) and keeps it in ctx->scratch.
@@ -252,6 +256,7 @@ int rt_ctx_create(int device, rt_ctx** out) {
     HIPCHECK(c.get(), hipGetLastError());
     HIPCHECK(c.get(), hipStreamSynchronize(c->stream));
     spectral_norms(c->norms, c->fire_norms);
+    spectral_norms(c->norms_w, c->fire_norms_w, RT_CIE_WIDE_N);
     *out = c.release();
     return RT_OK;
 }
@@ -343,13 +348,13 @@ struct Flat {
 
 // spectral_to_rgb (src/spectrum.rs:104-114) on the host: the CIE table lerp,
 // XYZ -> linear sRGB clamped, times the per-channel normalisation.
-static void host_spectral_rgb(const float norms[3], float lambda, float out[3]) {
-    float t = std::fmin(std::fmax((lambda - RT_CIE_START) / RT_CIE_STEP, 0.0f), (float)(RT_CIE_N - 1));
-    int i0 = std::min((int)t, RT_CIE_N - 2);
+static void host_spectral_rgb(const float norms[3], float lambda, float out[3], int n = RT_CIE_N) {
+    float t = std::fmin(std::fmax((lambda - RT_CIE_START) / RT_CIE_STEP, 0.0f), (float)(n - 1));
+    int i0 = std::min((int)t, n - 2);
     float fr = t - (float)i0;
-    float x = RT_CIE_X[i0] + fr * (RT_CIE_X[i0 + 1] - RT_CIE_X[i0]);
-    float y = RT_CIE_Y[i0] + fr * (RT_CIE_Y[i0 + 1] - RT_CIE_Y[i0]);
-    float z = RT_CIE_Z[i0] + fr * (RT_CIE_Z[i0 + 1] - RT_CIE_Z[i0]);
+    float x = RT_CIE_WX[i0] + fr * (RT_CIE_WX[i0 + 1] - RT_CIE_WX[i0]);
+    float y = RT_CIE_WY[i0] + fr * (RT_CIE_WY[i0 + 1] - RT_CIE_WY[i0]);
+    float z = RT_CIE_WZ[i0] + fr * (RT_CIE_WZ[i0 + 1] - RT_CIE_WZ[i0]);
     float c[3]; xyz_srgb(x, y, z, c);
     for (int a = 0; a < 3; a++) out[a] = c[a] * norms[a];
 }
@@ -742,6 +747,18 @@ extern "C" int rt_scene_upload(rt_ctx* ctx, const rt_scene_desc* d, rt_scene** o
             }
             mats[i].p[2] = norm;
             for (int a = 0; a < 3; a++) mats[i].p[3 + a] = acc[a] / (float)RT_CIE_N * intensity;
+            // the same over the table extended to 780 nm (the wide wavelength
+            // modes): p6 normalising mean, p7-9 average colour
+            float raw_w[RT_CIE_WIDE_N], sum_w = 0.0f;
+            for (int j = 0; j < RT_CIE_WIDE_N; j++) { raw_w[j] = planck_raw(380.0f + (float)j * 5.0f, T); sum_w += raw_w[j]; }
+            float norm_w = 1.0f / std::fmax(sum_w / (float)RT_CIE_WIDE_N, 1e-30f);
+            float acc_w[3] = {0, 0, 0};
+            for (int j = 0; j < RT_CIE_WIDE_N; j++) {
+                float c[3]; host_spectral_rgb(ctx->norms_w, 380.0f + (float)j * 5.0f, c, RT_CIE_WIDE_N);
+                for (int a = 0; a < 3; a++) acc_w[a] += c[a] * raw_w[j] * norm_w;
+            }
+            mats[i].p[6] = norm_w;
+            for (int a = 0; a < 3; a++) mats[i].p[7 + a] = acc_w[a] / (float)RT_CIE_WIDE_N * intensity;
         }
     }
     if ((rc = upload_vec(s.get(), F.grids, &S.grids))) return rc;
@@ -803,7 +820,10 @@ extern "C" int rt_scene_upload(rt_ctx* ctx, const rt_scene_desc* d, rt_scene** o
         } else if (b.env_sampling != RT_ENV_CDF) return fail(ctx, RT_ERR_INVALID, "env_sampling");
     }
     S.elut = ctx->elut;
-    for (int k = 0; k < 3; k++) { S.norms[k] = ctx->norms[k]; S.fire_norms[k] = ctx->fire_norms[k]; }
+    for (int k = 0; k < 3; k++) {
+        S.norms[k] = ctx->norms[k]; S.fire_norms[k] = ctx->fire_norms[k];
+        S.norms_w[k] = ctx->norms_w[k]; S.fire_norms_w[k] = ctx->fire_norms_w[k];
+    }
     S.stack_bound = bound;
     for (int k = 0; k < 3; k++) { s->world_min[k] = wb.mn[k]; s->world_max[k] = wb.mx[k]; }
     for (uint32_t i = 0; i < d->num_objects; i++) {
@@ -861,6 +881,12 @@ int rt_trace_closest(rt_ctx* ctx, rt_scene* s, const rt_ray* rays, uint32_t n, r
 int rt_trace_any(rt_ctx* ctx, rt_scene* s, const rt_ray* rays, uint32_t n, uint8_t* occ) { return trace_common(ctx, s, rays, n, occ, 1); }
 int rt_transmittance(rt_ctx* ctx, rt_scene* s, const rt_ray* rays, uint32_t n, float* tr) { return trace_common(ctx, s, rays, n, tr, 2); }
 
+// rt_sample_params.wavelengths -> Ray::wl: 0 the reference's one wavelength
+// over [380, 700), 1 one over [380, 780), 4 four hero wavelengths over
+// [380, 780) (wl 2); -1 for any other value.
+static int wl_mode(const rt_sample_params* p) {
+    return p->wavelengths == 0 ? 0 : p->wavelengths == 1 ? 1 : p->wavelengths == 4 ? 2 : -1;
+}
 static void to_dev(const rt_camera* c, const rt_sample_params* p, DCamera& C, DParams& P) {
     for (int k = 0; k < 3; k++) {
         C.origin[k] = c->origin[k]; C.lower_left[k] = c->lower_left[k]; C.horizontal[k] = c->horizontal[k];
@@ -870,11 +896,20 @@ static void to_dev(const rt_camera* c, const rt_sample_params* p, DCamera& C, DP
     P.sample_idx = p->sample_idx; P.strata = p->strata; P.width = p->width; P.height = p->height;
     P.max_depth = p->max_depth; P.use_frame_time = p->use_frame_time; P.use_photons = p->use_photons;
     P.bg_scale = p->bg_scale; P.t0 = p->frame_t0; P.t1 = p->frame_t1; P.seed = p->seed;
+    P.wl = (uint32_t)std::max(wl_mode(p), 0);
+}
+static void launch_render(bool knn, bool h4, dim3 grid, hipStream_t st, const DScene& S, const DCamera& C, const DParams& Q,
+                          const uint8_t* conv, float* out, uint32_t m, int acc) {
+    if (knn && h4) hipLaunchKernelGGL((k_render<true, true>), grid, dim3(BLOCK), 0, st, S, C, Q, conv, out, m, acc);
+    else if (knn) hipLaunchKernelGGL((k_render<true, false>), grid, dim3(BLOCK), 0, st, S, C, Q, conv, out, m, acc);
+    else if (h4) hipLaunchKernelGGL((k_render<false, true>), grid, dim3(BLOCK), 0, st, S, C, Q, conv, out, m, acc);
+    else hipLaunchKernelGGL((k_render<false, false>), grid, dim3(BLOCK), 0, st, S, C, Q, conv, out, m, acc);
 }
 
 static int render_common(rt_ctx* ctx, rt_scene* s, const rt_camera* cam, const rt_sample_params* p, const uint8_t* conv,
                          float* out, uint32_t spp, bool accumulate) {
     if (!ctx || !s || !cam || !p || !out) return RT_ERR_INVALID;
+    if (wl_mode(p) < 0) return fail(ctx, RT_ERR_INVALID, "wavelengths must be 0, 1 or 4");
     if (p->width == 0 || p->height == 0) return fail(ctx, RT_ERR_INVALID, "empty image");
     if ((uint64_t)p->width * p->height > 0xffffffffull / 4) return fail(ctx, RT_ERR_INVALID, "image too large");
     if (p->use_photons && s->S.num_photons == 0 && s->photons_host.empty()) return fail(ctx, RT_ERR_INVALID, "use_photons without a photon map");
@@ -899,14 +934,12 @@ static int render_common(rt_ctx* ctx, rt_scene* s, const rt_camera* cam, const r
             DParams Q = P;
             Q.sample_idx = P.sample_idx + k;
             uint32_t m = std::min(chunk, spp - k);
-            if (knn) hipLaunchKernelGGL(k_render<true>, grid, dim3(BLOCK), 0, ctx->stream, s->S, C, Q, (const uint8_t*)cin.dev, (float*)rout.dev, m, 1);
-            else hipLaunchKernelGGL(k_render<false>, grid, dim3(BLOCK), 0, ctx->stream, s->S, C, Q, (const uint8_t*)cin.dev, (float*)rout.dev, m, 1);
+            launch_render(knn, P.wl == 2, grid, ctx->stream, s->S, C, Q, (const uint8_t*)cin.dev, (float*)rout.dev, m, 1);
             HIPCHECK(ctx, hipGetLastError());
         }
         nlaunch = (spp + chunk - 1) / chunk;
     } else {
-        if (knn) hipLaunchKernelGGL(k_render<true>, grid, dim3(BLOCK), 0, ctx->stream, s->S, C, P, (const uint8_t*)cin.dev, (float*)rout.dev, 1u, 0);
-        else hipLaunchKernelGGL(k_render<false>, grid, dim3(BLOCK), 0, ctx->stream, s->S, C, P, (const uint8_t*)cin.dev, (float*)rout.dev, 1u, 0);
+        launch_render(knn, P.wl == 2, grid, ctx->stream, s->S, C, P, (const uint8_t*)cin.dev, (float*)rout.dev, 1u, 0);
         HIPCHECK(ctx, hipGetLastError());
     }
     if (ctx->timing) {
@@ -1072,6 +1105,7 @@ int rt_resolve_rgb8(rt_ctx* ctx, const float* acc, const uint32_t* ns, uint32_t 
 int rt_render_progressive(rt_ctx* ctx, rt_scene* s, const rt_camera* cam, const rt_sample_params* p, uint32_t samples_max,
                           const rt_accum_opts* o, const rt_pixel_state* st, uint32_t* samples_done, uint32_t* n_converged) {
     if (!ctx || !s || !cam || !p) return RT_ERR_INVALID;
+    if (wl_mode(p) < 0) return fail(ctx, RT_ERR_INVALID, "wavelengths must be 0, 1 or 4");
     int rc;
     if ((rc = check_accum_opts(ctx, o))) return rc;
     if (p->width == 0 || p->height == 0) return fail(ctx, RT_ERR_INVALID, "empty image");
@@ -1095,8 +1129,10 @@ int rt_render_progressive(rt_ctx* ctx, rt_scene* s, const rt_camera* cam, const 
     for (uint32_t k = 0; k < samples_max; k++) {
         DParams Q = P;
         Q.sample_idx = P.sample_idx + k;
-        if (knn) hipLaunchKernelGGL(k_progressive<true>, grid, dim3(BLOCK), 0, ctx->stream, s->S, C, Q, *o, k, acc, ns, m2, conv, ctx->prog_ctl);
-        else hipLaunchKernelGGL(k_progressive<false>, grid, dim3(BLOCK), 0, ctx->stream, s->S, C, Q, *o, k, acc, ns, m2, conv, ctx->prog_ctl);
+        if (knn && Q.wl == 2) hipLaunchKernelGGL((k_progressive<true, true>), grid, dim3(BLOCK), 0, ctx->stream, s->S, C, Q, *o, k, acc, ns, m2, conv, ctx->prog_ctl);
+        else if (knn) hipLaunchKernelGGL((k_progressive<true, false>), grid, dim3(BLOCK), 0, ctx->stream, s->S, C, Q, *o, k, acc, ns, m2, conv, ctx->prog_ctl);
+        else if (Q.wl == 2) hipLaunchKernelGGL((k_progressive<false, true>), grid, dim3(BLOCK), 0, ctx->stream, s->S, C, Q, *o, k, acc, ns, m2, conv, ctx->prog_ctl);
+        else hipLaunchKernelGGL((k_progressive<false, false>), grid, dim3(BLOCK), 0, ctx->stream, s->S, C, Q, *o, k, acc, ns, m2, conv, ctx->prog_ctl);
         hipLaunchKernelGGL(k_progressive_step, dim3(1), dim3(64), 0, ctx->stream, ctx->prog_ctl, k, npix, *o);
         HIPCHECK(ctx, hipGetLastError());
     }
@@ -1185,11 +1221,16 @@ static void wf_chunk(rt_ctx* ctx, hipStream_t st, const DScene& S, const DCamera
             hipLaunchKernelGGL((k_wf_knn_dense<BT>), dim3(grid), dim3(BLOCK), 0, st, S, Q, Bt);
             hipLaunchKernelGGL((k_wf_knn_wave<BT>), dim3(grid), dim3(BLOCK), 0, st, S, Q, Bt);
         }
-        if (Q.sort == 2) {
-            hipLaunchKernelGGL((k_wf_shade<BT, 1>), dim3(grid), dim3(BLOCK), 0, st, S, P, Q, (int)d, Bt);
-            hipLaunchKernelGGL((k_wf_shade<BT, 2>), dim3(grid), dim3(BLOCK), 0, st, S, P, Q, (int)d, Bt);
+        if (Q.sort == 2 && Q.h4) {
+            hipLaunchKernelGGL((k_wf_shade<BT, 1, true>), dim3(grid), dim3(BLOCK), 0, st, S, P, Q, (int)d, Bt);
+            hipLaunchKernelGGL((k_wf_shade<BT, 2, true>), dim3(grid), dim3(BLOCK), 0, st, S, P, Q, (int)d, Bt);
+        } else if (Q.sort == 2) {
+            hipLaunchKernelGGL((k_wf_shade<BT, 1, false>), dim3(grid), dim3(BLOCK), 0, st, S, P, Q, (int)d, Bt);
+            hipLaunchKernelGGL((k_wf_shade<BT, 2, false>), dim3(grid), dim3(BLOCK), 0, st, S, P, Q, (int)d, Bt);
+        } else if (Q.h4) {
+            hipLaunchKernelGGL((k_wf_shade<BT, 0, true>), dim3(grid), dim3(BLOCK), 0, st, S, P, Q, (int)d, Bt);
         } else {
-            hipLaunchKernelGGL((k_wf_shade<BT, 0>), dim3(grid), dim3(BLOCK), 0, st, S, P, Q, (int)d, Bt);
+            hipLaunchKernelGGL((k_wf_shade<BT, 0, false>), dim3(grid), dim3(BLOCK), 0, st, S, P, Q, (int)d, Bt);
         }
         launch_trace<BT>(true, occ, block, grid, st, S, Q, Bt);
         hipLaunchKernelGGL(k_wf_advance, dim3(1), dim3(32), 0, st, Q.counters);
@@ -1304,6 +1345,7 @@ static int wf_lanes(rt_ctx* ctx, const WfOpts& o, WfQueues* Qs) {
         Qs[k].sort = Qs[0].sort;
         Qs[k].fb = Qs[0].fb;
         Qs[k].knn_wave_max = Qs[0].knn_wave_max;
+        Qs[k].h4 = Qs[0].h4;
     }
     return RT_OK;
 }
@@ -1311,6 +1353,7 @@ static int wf_lanes(rt_ctx* ctx, const WfOpts& o, WfQueues* Qs) {
 extern "C" int rt_render_wavefront(rt_ctx* ctx, rt_scene* s, const rt_camera* cam, const rt_sample_params* p, uint32_t spp,
                                    float* out, const rt_wavefront_opts* opts) {
     if (!ctx || !s || !cam || !p || !out) return RT_ERR_INVALID;
+    if (wl_mode(p) < 0) return fail(ctx, RT_ERR_INVALID, "wavelengths must be 0, 1 or 4");
     if (p->width == 0 || p->height == 0 || spp == 0) return fail(ctx, RT_ERR_INVALID, "empty frame");
     if (p->max_depth == 0) return fail(ctx, RT_ERR_INVALID, "max_depth = 0");
     if (p->use_photons && s->S.num_photons == 0) return fail(ctx, RT_ERR_INVALID, "use_photons without a photon map");
@@ -1331,6 +1374,7 @@ extern "C" int rt_render_wavefront(rt_ctx* ctx, rt_scene* s, const rt_camera* ca
     DCamera C; DParams P;
     to_dev(cam, p, C, P);
     P.counters = nullptr;
+    Q.h4 = P.wl == 2;
     HIPCHECK(ctx, hipMemsetAsync(rout.dev, 0, (size_t)npix * 12, ctx->stream));
     WfBatch none = {nullptr, nullptr, 0};
     WfQueues Qs[RT_WF_MAX_STREAMS];
@@ -1347,6 +1391,7 @@ extern "C" int rt_render_wavefront(rt_ctx* ctx, rt_scene* s, const rt_camera* ca
 extern "C" int rt_render_wavefront_noalloc(rt_ctx* ctx, rt_scene* s, const rt_camera* cam, const rt_sample_params* p,
                                            uint32_t spp, float* out, const rt_wavefront_opts* opts) {
     if (!ctx || !s || !cam || !p || !out) return RT_ERR_INVALID;
+    if (wl_mode(p) < 0) return fail(ctx, RT_ERR_INVALID, "wavelengths must be 0, 1 or 4");
     if (p->width == 0 || p->height == 0 || spp == 0) return fail(ctx, RT_ERR_INVALID, "empty frame");
     if (p->max_depth == 0) return fail(ctx, RT_ERR_INVALID, "max_depth = 0");
     if (p->use_photons && s->S.num_photons == 0) return fail(ctx, RT_ERR_INVALID, "use_photons without a photon map");
@@ -1373,6 +1418,7 @@ extern "C" int rt_render_wavefront_noalloc(rt_ctx* ctx, rt_scene* s, const rt_ca
     DCamera C; DParams P;
     to_dev(cam, p, C, P);
     P.counters = nullptr;
+    Q.h4 = P.wl == 2;
     hipLaunchKernelGGL(k_fill_u32, dim3(std::min((npix * 3 + BLOCK - 1) / BLOCK, 4096u)), dim3(BLOCK), 0, ctx->stream,
                        (uint32_t*)out, (uint64_t)npix * 3, 0u);
     hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(BLOCK), 0, ctx->stream, Q.counters, (uint64_t)64, 0u);
@@ -1396,6 +1442,7 @@ extern "C" int rt_render_progressive_wavefront(rt_ctx* ctx, rt_scene* s, const r
                                                uint32_t samples_max, const rt_accum_opts* ao, const rt_pixel_state* state,
                                                const rt_wavefront_opts* opts, uint32_t* samples_done, uint32_t* n_converged) {
     if (!ctx || !s || !cam || !p) return RT_ERR_INVALID;
+    if (wl_mode(p) < 0) return fail(ctx, RT_ERR_INVALID, "wavelengths must be 0, 1 or 4");
     int rc;
     if ((rc = check_accum_opts(ctx, ao))) return rc;
     if (p->width == 0 || p->height == 0) return fail(ctx, RT_ERR_INVALID, "empty image");
@@ -1416,6 +1463,7 @@ extern "C" int rt_render_progressive_wavefront(rt_ctx* ctx, rt_scene* s, const r
     if ((rc = wf_buffers(ctx, o.cap, Qs[0]))) return rc;
     Qs[0].sort = o.sort;
     Qs[0].knn_wave_max = o.knn_wave_max;
+    Qs[0].h4 = wl_mode(p) == 2;
     if ((rc = wf_lanes(ctx, o, Qs))) return rc;
     const uint32_t nchunks = (samples_max + K - 1) / K;
     const uint32_t L = std::max(1u, std::min(o.streams, nchunks));
@@ -1492,13 +1540,15 @@ extern "C" int rt_render_batch(rt_ctx* ctx, const rt_batch_view* views, uint32_t
     std::vector<DScene> ds(nviews);
     uint64_t total = 0, fb_pix = 0;
     uint32_t depth = 0;
-    bool knn = false;
+    bool knn = false, h4 = false;
     for (uint32_t v = 0; v < nviews; v++) {
         const rt_batch_view& b = views[v];
         if (!b.scene || !b.accum_rgb || b.spp == 0) return fail(ctx, RT_ERR_INVALID, "batch view " + std::to_string(v) + ": missing scene/output/spp");
         if (b.scene->ctx != ctx) return fail(ctx, RT_ERR_INVALID, "batch view " + std::to_string(v) + ": scene of another context");
         const rt_sample_params& p = b.params;
         if (p.width == 0 || p.height == 0 || p.max_depth == 0) return fail(ctx, RT_ERR_INVALID, "batch view " + std::to_string(v) + ": empty frame");
+        if (wl_mode(&p) < 0) return fail(ctx, RT_ERR_INVALID, "batch view " + std::to_string(v) + ": wavelengths must be 0, 1 or 4");
+        h4 = h4 || wl_mode(&p) == 2;
         if (p.use_photons && b.scene->S.num_photons == 0) return fail(ctx, RT_ERR_INVALID, "batch view " + std::to_string(v) + ": no photon map");
         to_dev(&b.camera, &p, dv[v].cam, dv[v].P);
         dv[v].P.counters = nullptr;
@@ -1520,6 +1570,7 @@ extern "C" int rt_render_batch(rt_ctx* ctx, const rt_batch_view* views, uint32_t
     if ((rc = wf_buffers(ctx, o.cap, Q))) return rc;
     Q.sort = o.sort;
     Q.knn_wave_max = o.knn_wave_max;
+    Q.h4 = h4;
     // batch side data: views, scenes, concatenated accumulator
     size_t bv = (sizeof(DView) * nviews + 255) & ~(size_t)255, bs = (sizeof(DScene) * nviews + 255) & ~(size_t)255;
     size_t bf = (size_t)fb_pix * 12;
@@ -1604,6 +1655,8 @@ static int render_args(rt_ctx* ctx, rt_scene* s, const rt_camera* cam, const rt_
     if ((uint64_t)p->width * p->height > 0xffffffffull / 4) return fail(ctx, RT_ERR_INVALID, "image too large");
     if (p->use_photons && s->S.num_photons == 0) return fail(ctx, RT_ERR_INVALID, "use_photons without a photon map");
     if (p->use_photons && s->S.ph_knn) return fail(ctx, RT_ERR_UNSUPPORTED, "the adjoint gathers photons in a fixed radius only");
+    if (wl_mode(p) < 0) return fail(ctx, RT_ERR_INVALID, "wavelengths must be 0, 1 or 4");
+    if (wl_mode(p) == 2) return fail(ctx, RT_ERR_UNSUPPORTED, "the adjoint differentiates one wavelength per path (wavelengths 0 or 1)");
     return RT_OK;
 }
 
@@ -2005,10 +2058,12 @@ int rt_photon_build(rt_ctx* ctx, rt_scene* s, const rt_photon_params* pp_in, uin
     const rt_photon_params* pp = &res;
     if (pp->num_photons == 0) return fail(ctx, RT_ERR_INVALID, "num_photons = 0");
     if (pp->knn > RT_KNN_MAX) return fail(ctx, RT_ERR_UNSUPPORTED, "knn above RT_KNN_MAX");
+    if (pp->wavelengths != 0 && pp->wavelengths != 1 && pp->wavelengths != 4) return fail(ctx, RT_ERR_INVALID, "wavelengths must be 0, 1 or 4");
     if (!(pp->gather_radius > 0.0f)) return fail(ctx, RT_ERR_INVALID, "gather_radius must be positive");
     HIPCHECK(ctx, hipSetDevice(ctx->device));
     DEmitter E;
     memset(&E, 0, sizeof E);
+    E.wide = pp->wavelengths != 0 ? 1u : 0u;
     uint32_t n = pp->num_photons;
     if (pp->emitter == RT_EMIT_QUAD) {
         V3 lu = mk(pp->u[0], pp->u[1], pp->u[2]), lv = mk(pp->v[0], pp->v[1], pp->v[2]);

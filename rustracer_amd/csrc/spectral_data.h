@@ -1,6 +1,6 @@
 /*
  * CIE 1931 2-degree colour-matching functions, 380-700 nm in 5 nm steps
- * (65 samples), and Johnson & Christy (1972) complex IOR of gold, copper and
+ * (65 samples) plus their 705-780 nm rows, and Johnson & Christy (1972) complex IOR of gold, copper and
  * silver, 380-680 nm in 25 nm steps.  These are published data tables; the
  * values are the ones rustracer uses (src/spectrum.rs:13-62 and 218-225) so
  * that spectral weights agree with the reference.  Data only, no code.
@@ -57,6 +57,27 @@ static const float RT_CIE_Y[65] = {RT_CIE_Y_DATA};
     0.000000f, 0.000000f, 0.000000f, 0.000000f, 0.000000f, \
     0.000000f, 0.000000f, 0.000000f, 0.000000f, 0.000000f
 static const float RT_CIE_Z[65] = {RT_CIE_Z_DATA};
+
+/* The same observer's 705-780 nm rows (16 samples; z-bar is 0 there), for the
+ * wide wavelength range [380, 780) of the four-wavelength mode (extension:
+ * the reference's table stops at 700 nm).  CIE 1931 standard tabulation; its
+ * chromaticity over these rows is the constant (0.734690, 0.265310). */
+#define RT_CIE_WIDE_N 81
+#define RT_CIE_X_TAIL_DATA \
+    0.008110916f, 0.005790346f, 0.004109457f, 0.002899327f, 0.002049190f, \
+    0.001439971f, 0.000999949f, 0.000690079f, 0.000476021f, 0.000332301f, \
+    0.000234826f, 0.000166151f, 0.000117413f, 0.0000830753f, 0.0000587065f, \
+    0.0000415099f
+#define RT_CIE_Y_TAIL_DATA \
+    0.002929000f, 0.002091000f, 0.001484000f, 0.001047000f, 0.000740000f, \
+    0.000520000f, 0.000361100f, 0.000249200f, 0.000171900f, 0.000120000f, \
+    0.000084800f, 0.000060000f, 0.000042400f, 0.0000300000f, 0.0000212000f, \
+    0.0000149900f
+#define RT_CIE_Z_TAIL_DATA \
+    0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f
+static const float RT_CIE_WX[81] = {RT_CIE_X_DATA, RT_CIE_X_TAIL_DATA};
+static const float RT_CIE_WY[81] = {RT_CIE_Y_DATA, RT_CIE_Y_TAIL_DATA};
+static const float RT_CIE_WZ[81] = {RT_CIE_Z_DATA, RT_CIE_Z_TAIL_DATA};
 #define RT_GOLD_N_DATA 1.69f, 1.65f, 1.54f, 1.33f, 0.97f, 0.54f, 0.30f, 0.23f, 0.20f, 0.19f, 0.18f, 0.16f, 0.16f
 static const float RT_GOLD_N[13] = {RT_GOLD_N_DATA};
 #define RT_GOLD_K_DATA 1.91f, 1.94f, 1.95f, 1.92f, 1.80f, 1.93f, 2.41f, 3.00f, 3.56f, 4.07f, 4.56f, 4.99f, 5.41f

@@ -117,7 +117,10 @@ class Renderer:
     # ---------------------------------------------------------------- render
     @staticmethod
     def params(width, height, sample_idx=0, strata=0, bg_scale=1.0, max_depth=50, frame_time=None,
-               use_photons=False, seed=0):
+               use_photons=False, seed=0, wavelengths=0):
+        """rt_sample_params; wavelengths: 0 the reference's one hero wavelength
+        over [380, 700) nm, 1 one over [380, 780), 4 four hero wavelengths over
+        [380, 780) (extension)."""
         p = F.rt_sample_params()
         p.sample_idx, p.strata, p.width, p.height = sample_idx, strata, width, height
         p.bg_scale, p.max_depth = bg_scale, max_depth
@@ -125,13 +128,14 @@ class Renderer:
             p.use_frame_time, p.frame_t0, p.frame_t1 = 1, frame_time[0], frame_time[1]
         p.use_photons = 1 if use_photons else 0
         p.seed = seed
+        p.wavelengths = wavelengths
         return p
 
     def render_tiles(self, scratch, converged, sample_idx, strata, width, height, camera, bg_scale=1.0,
-                     photon_map=False, frame_time=None, max_depth=50, seed=0):
+                     photon_map=False, frame_time=None, max_depth=50, seed=0, wavelengths=0):
         """render_tiles: one pass into `scratch` (H*W*3 float32, host array or
         device tensor)."""
-        p = self.params(width, height, sample_idx, strata, bg_scale, max_depth, frame_time, photon_map, seed)
+        p = self.params(width, height, sample_idx, strata, bg_scale, max_depth, frame_time, photon_map, seed, wavelengths)
         cam = camera.to_c()
         conv = None if converged is None else np.ascontiguousarray(converged, dtype=np.uint8)
         self._check(self.L.rt_render_sample(self.ctx, self.scene, C.byref(cam), C.byref(p), _ptr(conv), _ptr(scratch)),
@@ -139,11 +143,11 @@ class Renderer:
         return scratch
 
     def render_accumulate(self, out, camera, width, height, spp, sample_idx=0, strata=0, bg_scale=1.0,
-                          photon_map=False, max_depth=50, seed=0, scene=None, frame_time=None):
+                          photon_map=False, max_depth=50, seed=0, scene=None, frame_time=None, wavelengths=0):
         """Sum of `spp` passes (sample_idx .. sample_idx+spp-1) into `out`;
         `scene` is a handle from upload_extra (default: the current scene);
         frame_time = (t0, t1) draws each camera ray's time in [t0, t1)."""
-        p = self.params(width, height, sample_idx, strata, bg_scale, max_depth, frame_time, photon_map, seed)
+        p = self.params(width, height, sample_idx, strata, bg_scale, max_depth, frame_time, photon_map, seed, wavelengths)
         cam = camera.to_c()
         self._check(self.L.rt_render_accumulate(self.ctx, self.scene if scene is None else scene, C.byref(cam), C.byref(p), spp, _ptr(out)),
                     "rt_render_accumulate")
@@ -192,12 +196,12 @@ class Renderer:
 
     def render_progressive(self, state, camera, width, height, samples_max, opts=None, sample_idx=0, strata=0,
                            bg_scale=1.0, photon_map=False, max_depth=50, seed=0, wavefront=False, capacity=0, grid=0,
-                           sort=1, use_graph=False, occupancy=0, streams=1, block=0, knn_wave_max=0):
+                           sort=1, use_graph=False, occupancy=0, streams=1, block=0, knn_wave_max=0, wavelengths=0):
         """The reference's render loop (src/main.rs:463-506) on the device;
         returns (passes run, converged pixels).  wavefront=True renders the
         passes with the wavefront stages (rt_render_progressive_wavefront)."""
         o = opts if opts is not None else self.accum_opts()
-        p = self.params(width, height, sample_idx, strata, bg_scale, max_depth, None, photon_map, seed)
+        p = self.params(width, height, sample_idx, strata, bg_scale, max_depth, None, photon_map, seed, wavelengths)
         cam = camera.to_c()
         st = self.pixel_state(state)
         done, nconv = C.c_uint32(), C.c_uint32()
@@ -240,9 +244,9 @@ class Renderer:
 
     def render_wavefront(self, out, camera, width, height, spp, sample_idx=0, strata=0, bg_scale=1.0,
                          photon_map=False, max_depth=50, seed=0, capacity=0, grid=0, sort=True, use_graph=False,
-                         occupancy=0, streams=1, scene=None, block=0, knn_wave_max=0, frame_time=None):
+                         occupancy=0, streams=1, scene=None, block=0, knn_wave_max=0, frame_time=None, wavelengths=0):
         """Same frame as render_accumulate, through the wavefront stage kernels."""
-        p = self.params(width, height, sample_idx, strata, bg_scale, max_depth, frame_time, photon_map, seed)
+        p = self.params(width, height, sample_idx, strata, bg_scale, max_depth, frame_time, photon_map, seed, wavelengths)
         cam = camera.to_c()
         o = F.rt_wavefront_opts(capacity=capacity, grid=grid, sort=int(sort), use_graph=1 if use_graph else 0,
                                 occupancy=occupancy, streams=streams, block=block, knn_wave_max=knn_wave_max)
@@ -252,11 +256,11 @@ class Renderer:
 
     def render_wavefront_noalloc(self, out, camera, width, height, spp, sample_idx=0, strata=0, bg_scale=1.0,
                                  photon_map=False, max_depth=50, seed=0, capacity=0, grid=0, sort=True, occupancy=0,
-                                 scene=None, block=0, knn_wave_max=0):
+                                 scene=None, block=0, knn_wave_max=0, wavelengths=0):
         """render_wavefront without device allocations (rt_render_wavefront_noalloc):
         out must be device memory (e.g. a torch cuda tensor); capacity at most
         NOALLOC_CAPACITY paths, one stream, no graphs."""
-        p = self.params(width, height, sample_idx, strata, bg_scale, max_depth, None, photon_map, seed)
+        p = self.params(width, height, sample_idx, strata, bg_scale, max_depth, None, photon_map, seed, wavelengths)
         cam = camera.to_c()
         o = F.rt_wavefront_opts(capacity=capacity, grid=grid, sort=int(sort), use_graph=0, occupancy=occupancy, streams=1,
                                 block=block, knn_wave_max=knn_wave_max)
@@ -267,13 +271,14 @@ class Renderer:
     def render_batch(self, views, capacity=0, grid=0, sort=True, use_graph=False, occupancy=0, streams=1, block=0,
                      knn_wave_max=0):
         """views: list of dicts with keys scene (handle), camera, width, height,
-        spp, out, and optionally max_depth, photon_map, seed, bg_scale."""
+        spp, out, and optionally max_depth, photon_map, seed, bg_scale, wavelengths."""
         arr = (F.rt_batch_view * len(views))()
         for i, v in enumerate(views):
             arr[i].scene = v["scene"]
             arr[i].camera = v["camera"].to_c()
             arr[i].params = self.params(v["width"], v["height"], 0, v.get("strata", 0), v.get("bg_scale", 1.0),
-                                        v.get("max_depth", 50), None, v.get("photon_map", False), v.get("seed", 0))
+                                        v.get("max_depth", 50), None, v.get("photon_map", False), v.get("seed", 0),
+                                        v.get("wavelengths", 0))
             arr[i].spp = v["spp"]
             arr[i].accum_rgb = _ptr(v["out"])
         o = F.rt_wavefront_opts(capacity=capacity, grid=grid, sort=int(sort), use_graph=1 if use_graph else 0,
@@ -303,11 +308,11 @@ class Renderer:
         return out
 
     def render_detached(self, out, camera, width, height, spp, params, eval_values, sample_idx=0, strata=0,
-                        bg_scale=1.0, photon_map=False, max_depth=50, seed=0, scene=None):
+                        bg_scale=1.0, photon_map=False, max_depth=50, seed=0, scene=None, wavelengths=0):
         """The frame of render_accumulate with decisions at the scene's current
         parameters and BSDF values at `eval_values` (finite differences with
-        common random numbers)."""
-        p = self.params(width, height, sample_idx, strata, bg_scale, max_depth, None, photon_map, seed)
+        common random numbers); wavelengths 0 or 1."""
+        p = self.params(width, height, sample_idx, strata, bg_scale, max_depth, None, photon_map, seed, wavelengths)
         cam = camera.to_c()
         vals = np.ascontiguousarray(eval_values, dtype=np.float32)
         self._check(self.L.rt_render_detached(self.ctx, self.scene if scene is None else scene, C.byref(cam), C.byref(p),
@@ -317,12 +322,12 @@ class Renderer:
 
     def backward(self, dloss_dimage, camera, width, height, spp, params, sample_idx=0, strata=0, bg_scale=1.0,
                  photon_map=False, max_depth=50, seed=0, scene=None, grad=None, wavefront=False, capacity=0, grid=0,
-                 sort=True, occupancy=0, streams=1, block=0):
+                 sort=True, occupancy=0, streams=1, block=0, wavelengths=0):
         """Path-replay adjoint: dLoss/dtheta for dloss_dimage = dLoss/d(mean
         image) (H x W x 3, host array or device tensor); returns grad (float32
         numpy array, or fills the given device tensor).  wavefront=True runs
-        rt_backward_wavefront with the given launch options."""
-        p = self.params(width, height, sample_idx, strata, bg_scale, max_depth, None, photon_map, seed)
+        rt_backward_wavefront with the given launch options; wavelengths 0 or 1."""
+        p = self.params(width, height, sample_idx, strata, bg_scale, max_depth, None, photon_map, seed, wavelengths)
         cam = camera.to_c()
         if hasattr(dloss_dimage, "data_ptr"):
             d = dloss_dimage
@@ -341,27 +346,28 @@ class Renderer:
         return g
 
     # ---------------------------------------------------------------- photons
-    def photon_build_from_quad(self, origin, u, v, color, num_photons, gather_radius, scene=None, knn=0):
+    def photon_build_from_quad(self, origin, u, v, color, num_photons, gather_radius, scene=None, knn=0, wavelengths=0):
         """PhotonMap::build_from_quad (src/photon.rs:97-135); knn > 0 gathers
-        the k nearest photons within gather_radius (extension)."""
+        the k nearest photons within gather_radius (extension); wavelengths 1
+        or 4 draws the photon wavelengths over [380, 780) (extension)."""
         pp = F.rt_photon_params()
         pp.emitter = F.EMIT_QUAD
         for k in range(3):
             pp.origin[k], pp.u[k], pp.v[k], pp.color[k] = float(origin[k]), float(u[k]), float(v[k]), float(color[k])
-        pp.num_photons, pp.gather_radius, pp.knn = num_photons, gather_radius, knn
+        pp.num_photons, pp.gather_radius, pp.knn, pp.wavelengths = num_photons, gather_radius, knn, wavelengths
         n = C.c_uint32()
         self._check(self.L.rt_photon_build(self.ctx, scene if scene is not None else self.scene, C.byref(pp), C.byref(n)),
                     "rt_photon_build")
         self.num_photons = n.value
         return n.value
 
-    def photon_build_sun(self, sun_dir, sun_color, num_photons, gather_radius, knn=0):
+    def photon_build_sun(self, sun_dir, sun_color, num_photons, gather_radius, knn=0, wavelengths=0):
         """PhotonMap::build (src/photon.rs:41-94): directional disk toward `sun_dir`."""
         pp = F.rt_photon_params()
         pp.emitter = F.EMIT_SUN_DISK
         for k in range(3):
             pp.sun_dir[k], pp.color[k] = float(sun_dir[k]), float(sun_color[k])
-        pp.num_photons, pp.gather_radius, pp.knn = num_photons, gather_radius, knn
+        pp.num_photons, pp.gather_radius, pp.knn, pp.wavelengths = num_photons, gather_radius, knn, wavelengths
         n = C.c_uint32()
         self._check(self.L.rt_photon_build(self.ctx, self.scene, C.byref(pp), C.byref(n)), "rt_photon_build")
         self.num_photons = n.value
@@ -376,7 +382,7 @@ class Renderer:
         return n.value
 
     @staticmethod
-    def caustic_params(info, knn=0):
+    def caustic_params(info, knn=0, wavelengths=0):
         """rt_photon_params for SceneData::rebuild_caustics (src/scene.rs:35-74):
         RT_EMIT_SCENE with the scene's caustic quad (its first light) as the
         fallback emitter.  info: the caustics settings of a loaded scene
@@ -389,6 +395,7 @@ class Renderer:
                 pp.origin[k], pp.u[k], pp.v[k], pp.color[k] = (float(lq["q"][k]), float(lq["u"][k]), float(lq["v"][k]),
                                                                float(lq["color"][k]))
         pp.num_photons, pp.gather_radius, pp.knn = int(info["num_photons"]), float(info["gather_radius"]), int(knn)
+        pp.wavelengths = int(wavelengths)
         return pp
 
     def photon_emitter(self, pp, scene=None):
@@ -398,7 +405,7 @@ class Renderer:
                     "rt_photon_emitter")
         return out
 
-    def rebuild_caustics(self, info, knn=0, scene=None):
+    def rebuild_caustics(self, info, knn=0, scene=None, wavelengths=0):
         """SceneData::rebuild_caustics (src/scene.rs:35-74): nothing when the
         scene's caustics are off or no emitter exists (no sun, no env map, no
         light quad); otherwise the map of the emitter the background picks.
@@ -406,7 +413,7 @@ class Renderer:
         if not info.get("caustics"):
             return 0
         sc = self.scene if scene is None else scene
-        pp = self.caustic_params(info, knn)
+        pp = self.caustic_params(info, knn, wavelengths)
         try:
             self.photon_emitter(pp, sc)
         except RtError:

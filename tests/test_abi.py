@@ -32,7 +32,7 @@ def test_library_exports_every_declared_symbol():
 
 
 def test_api_version_without_device():
-    assert F.lib().rt_api_version() == F.API_VERSION == 2
+    assert F.lib().rt_api_version() == F.API_VERSION == 3
 
 
 STRUCTS = ["rt_xform", "rt_medium", "rt_object", "rt_material", "rt_background", "rt_scene_desc",
