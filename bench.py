@@ -55,12 +55,22 @@ def reference_bench_scene():
     return scene, camera(W / H), dict(width=W, height=H, spp=128, max_depth=50, caustics=info)
 
 
+def gem_workload():
+    """The dispersive-gem caustic workload as configured: Sellmeier diamond,
+    2M-photon k = 64 map, 1280x720, 256 spp, depth 24, four hero wavelengths
+    over [380, 780) nm (camera paths and photons alike)."""
+    from rustracer_amd import scenes
+    sc, cam, cfg = scenes.gem_caustics(1280, 720, **scenes.GEM_BENCH)
+    cfg["wavelengths"] = 4
+    return sc, cam, cfg
+
+
 def workloads():
     from rustracer_amd import scenes
     return {
         "box": lambda: scenes.box_room(256, 256),
         "mesh": lambda: scenes.mesh_envmap(1280, 720, level=8),
-        "gem": lambda: scenes.gem_caustics(1280, 720, **scenes.GEM_BENCH),
+        "gem": gem_workload,
         "bench_toml": reference_bench_scene,
     }
 
@@ -79,6 +89,7 @@ def measure(name, args, with_cpu):
     scene, cam, cfg = workloads()[name]()
     build_s = time.time() - t0
     W, H, spp, depth = cfg["width"], cfg["height"], cfg["spp"], cfg["max_depth"]
+    wl = cfg.get("wavelengths", 0)
     strata = int(spp ** 0.5)
     r = Renderer(0)
     t0 = time.time()
@@ -89,7 +100,7 @@ def measure(name, args, with_cpu):
     ph = None
     if "caustics" in cfg:
         t0 = time.time()
-        stored = r.rebuild_caustics(cfg["caustics"])
+        stored = r.rebuild_caustics(cfg["caustics"], wavelengths=wl)
         photons = stored > 0
         ph = {"traced": cfg["caustics"]["num_photons"], "stored": stored, "build_s": round(time.time() - t0, 3),
               "emitter": "rebuild_caustics"}
@@ -97,26 +108,29 @@ def measure(name, args, with_cpu):
         lq = cfg["light_quad"]
         t0 = time.time()
         stored = r.photon_build_from_quad(lq["q"], lq["u"], lq["v"], cfg["light_color"], cfg["num_photons"],
-                                          cfg["gather_radius"], knn=cfg.get("knn", 0))
+                                          cfg["gather_radius"], knn=cfg.get("knn", 0), wavelengths=wl)
         ph = {"traced": cfg["num_photons"], "stored": stored, "build_s": round(time.time() - t0, 3)}
     acc = torch.zeros((H, W, 3), dtype=torch.float32, device="cuda")
     log(f"[{name}] scene {build_s:.1f}s upload {upload_s:.1f}s, {stats['nodes']} nodes, {stats['triangles']} tris")
 
     tuned, tune_info = (json.loads(args.wf_opts) if args.wf_opts else {}), None
     if not args.no_tune and not args.wf_opts:
-        tuned, tune_info = tune_wavefront(r, acc, cam, W, H, spp, depth, photons, knn=bool(cfg.get("knn", 0)))
+        tuned, tune_info = tune_wavefront(r, acc, cam, W, H, spp, depth, photons, knn=bool(cfg.get("knn", 0)),
+                                          wavelengths=wl)
         log(f"[{name}] autotuned wavefront: {tuned} ({tune_info['seconds']}s)")
 
     def step_plain(k):
-        r.render_accumulate(acc, cam, W, H, spp, strata=strata, max_depth=depth, seed=k, photon_map=photons)
+        r.render_accumulate(acc, cam, W, H, spp, strata=strata, max_depth=depth, seed=k, photon_map=photons,
+                            wavelengths=wl)
 
-    def step_wf(k, graph=None, streams=None):
+    def step_wf(k, graph=None, streams=None, wavelengths=None):
         o = dict(tuned) if tuned else {}
         if graph is not None:
             o["use_graph"] = graph
         if streams is not None:
             o["streams"] = streams
-        r.render_wavefront(acc, cam, W, H, spp, strata=strata, max_depth=depth, seed=k, photon_map=photons, **o)
+        r.render_wavefront(acc, cam, W, H, spp, strata=strata, max_depth=depth, seed=k, photon_map=photons,
+                           wavelengths=wl if wavelengths is None else wavelengths, **o)
 
     def timed(step):
         for k in range(args.warmup):
@@ -188,6 +202,18 @@ def measure(name, args, with_cpu):
         roof["wavefront"] = {"kernel": "k_wf_extend", "avg_launch_ms": ems / max(en, 1), "bytes_per_launch": bpl,
                              "achieved": bpl / (ems / 1e3 / max(en, 1)) / 1e9, "launches": en}
     rh = roof[head]
+    one_wl = None
+    if wl and "wavefront" in res:
+        # the reference's single wavelength over [380, 700) on the same map, beside it
+        for k in range(max(1, args.warmup)):
+            step_wf(3000 + k, wavelengths=0)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for k in range(args.steps):
+            step_wf(3100 + k, wavelengths=0)
+        torch.cuda.synchronize()
+        one_wl = samples * args.steps / (time.perf_counter() - t0)
+        log(f"[{name}] wavefront, one wavelength over [380, 700): {one_wl / 1e6:.1f} M samples/s")
 
     cpu = cpu_baseline(scene, cam, cfg, args.cpu_seconds, photons) if with_cpu else None
     r.close()
@@ -199,6 +225,9 @@ def measure(name, args, with_cpu):
         "value": rates[head], "ms_per_step": el * 1e3 / args.steps,
         "config": {"workload": scene.name, "width": W, "height": H, "spp": spp, "max_depth": depth,
                    "renderer": head, "samples_per_s": rates,
+                   "wavelengths": ("four hero wavelengths over [380, 780) nm" if wl == 4 else
+                                   "one over [380, 780) nm" if wl == 1 else "one hero wavelength over [380, 700) nm"),
+                   "wavefront_samples_per_s_one_wavelength_380_700": one_wl,
                    "wavefront_over_plain": (rates["wavefront"] / rates["plain"]) if len(rates) == 2 else None,
                    "frame_max_rel_diff_wavefront_vs_plain": same_frame,
                    "wavefront_opts": tuned,
@@ -220,8 +249,10 @@ def measure_batch(args, with_cpu):
     """Uneven batch: 36 gem spin stills (10 degree steps, resolutions drawn
     from 256x256 / 512x384 / 960x540, 64 spp, depth 24, 2M-photon caustic map
     each) plus the 640x480 volume glass box (64 spp, depth 24), all in one
-    rt_render_batch launch per step.  The same views rendered one
-    rt_render_wavefront call after another are timed beside it."""
+    rt_render_batch launch per step.  The gem stills draw four hero
+    wavelengths over [380, 780) nm (the gem workload's mode, photons alike).
+    The same views rendered one rt_render_wavefront call after another are
+    timed beside it."""
     import numpy as np
     import torch
     from rustracer_amd import Renderer, scenes
@@ -237,10 +268,10 @@ def measure_batch(args, with_cpu):
         if use_ph:
             lq = cfg["light_quad"]
             photons += r.photon_build_from_quad(lq["q"], lq["u"], lq["v"], cfg["light_color"], cfg["num_photons"],
-                                                cfg["gather_radius"], scene=h, knn=cfg.get("knn", 0))
+                                                cfg["gather_radius"], scene=h, knn=cfg.get("knn", 0), wavelengths=4)
         W, H = cfg["width"], cfg["height"]
         views.append(dict(scene=h, camera=cam, width=W, height=H, spp=cfg["spp"], max_depth=cfg["max_depth"],
-                          photon_map=use_ph, strata=int(cfg["spp"] ** 0.5),
+                          photon_map=use_ph, strata=int(cfg["spp"] ** 0.5), wavelengths=4 if use_ph else 0,
                           out=torch.zeros((H, W, 3), dtype=torch.float32, device="cuda")))
     setup_s = time.time() - t0
     samples = sum(v["width"] * v["height"] * v["spp"] for v in views)
@@ -259,7 +290,8 @@ def measure_batch(args, with_cpu):
         o = {key: val for key, val in tuned.items()}
         for v in views:
             r.render_wavefront(v["out"], v["camera"], v["width"], v["height"], v["spp"], strata=v["strata"],
-                               max_depth=v["max_depth"], photon_map=v["photon_map"], seed=k, scene=v["scene"], **o)
+                               max_depth=v["max_depth"], photon_map=v["photon_map"], seed=k, scene=v["scene"],
+                               wavelengths=v["wavelengths"], **o)
 
     def timed(step):
         for k in range(args.warmup):
@@ -557,7 +589,9 @@ def measure_rmse(args, with_cpu=True):
         of a percent;
       * independent: the oracle's own row streams at disjoint sample indices,
         the reference's estimator as the reference draws it.
-    Equal RMSE at equal spp makes samples/s the figure of merit."""
+    Equal RMSE at equal spp makes samples/s the figure of merit.  Every
+    render (target included) draws four hero wavelengths over [380, 780) nm,
+    the mode the gem workload is timed in."""
     import numpy as np
     import torch
     from oracle import oracle as O
@@ -575,9 +609,11 @@ def measure_rmse(args, with_cpu=True):
     for k in range(3):
         pp.origin[k], pp.u[k], pp.v[k], pp.color[k] = lq["q"][k], lq["u"][k], lq["v"][k], cfg["light_color"][k]
     pp.num_photons, pp.gather_radius, pp.knn = cfg["num_photons"], cfg["gather_radius"], cfg["knn"]
+    WL = 4
+    pp.wavelengths = WL
     t0 = time.time()
     nph = orc.photon_build(pp, nthreads=cores)
-    p = Renderer.params(W, H, sample_idx=0, max_depth=D, use_photons=True)
+    p = Renderer.params(W, H, sample_idx=0, max_depth=D, use_photons=True, wavelengths=WL)
     ref = orc.render_accumulate(cam, p, ref_spp, mode=0, nthreads=cores).astype(np.float64) / ref_spp
     ref_s = time.time() - t0
     log(f"[rmse] reference: {ref_spp} spp at {W}x{H} on {cores} host threads in {ref_s:.1f}s, {nph} photons")
@@ -590,15 +626,15 @@ def measure_rmse(args, with_cpu=True):
     a = torch.zeros((H, W, 3), dtype=torch.float32, device="cuda")
     m_wf, m_pair, m_ind = [], [], []
     for k in range(K):
-        r.render_wavefront(a, cam, W, H, spp, max_depth=D, photon_map=True, seed=11 + k)
+        r.render_wavefront(a, cam, W, H, spp, max_depth=D, photon_map=True, seed=11 + k, wavelengths=WL)
         m_wf.append(mse(a.cpu().numpy()))
-    r.render_accumulate(a, cam, W, H, spp, max_depth=D, photon_map=True, seed=11)
+    r.render_accumulate(a, cam, W, H, spp, max_depth=D, photon_map=True, seed=11, wavelengths=WL)
     m_plain = mse(a.cpu().numpy())
     r.close()
     for k in range(K):
-        pc = Renderer.params(W, H, max_depth=D, use_photons=True, seed=11 + k)
+        pc = Renderer.params(W, H, max_depth=D, use_photons=True, seed=11 + k, wavelengths=WL)
         m_pair.append(mse(orc.render_accumulate(cam, pc, spp, mode=1, nthreads=cores)))
-        pc = Renderer.params(W, H, sample_idx=100000 + 1000 * k, max_depth=D, use_photons=True)
+        pc = Renderer.params(W, H, sample_idx=100000 + 1000 * k, max_depth=D, use_photons=True, wavelengths=WL)
         m_ind.append(mse(orc.render_accumulate(cam, pc, spp, mode=0, nthreads=cores)))
     mw, mp, mi = np.array(m_wf), np.array(m_pair), np.array(m_ind)
     e_wf, e_pair, e_ind = (float(np.sqrt(x.mean())) for x in (mw, mp, mi))
@@ -614,6 +650,7 @@ def measure_rmse(args, with_cpu=True):
     return {"value": ratio, "unit": "RMSE ratio (wavefront / oracle, same streams) at equal spp", "ms_per_step": None,
             "config": {"workload": "gem_caustics_640x360_rmse", "width": W, "height": H, "spp": spp,
                        "reference_spp": ref_spp, "frames": K, "photons": {"device": nd, "oracle": nph},
+                       "wavelengths": "four hero wavelengths over [380, 780) nm",
                        "rmse": {"wavefront": e_wf, "oracle_same_streams": e_pair, "oracle_row_streams": e_ind,
                                 "plain_first_frame": float(np.sqrt(m_plain))},
                        "rmse_per_frame": {"wavefront": [float(np.sqrt(x)) for x in mw],
@@ -760,13 +797,15 @@ def cpu_baseline(scene, cam, cfg, seconds, photons=False):
         for k in range(3):
             pp.origin[k], pp.u[k], pp.v[k], pp.color[k] = lq["q"][k], lq["u"][k], lq["v"][k], cfg["light_color"][k]
         pp.num_photons, pp.gather_radius, pp.knn = cfg["num_photons"], cfg["gather_radius"], cfg.get("knn", 0)
+        pp.wavelengths = cfg.get("wavelengths", 0)
         orc.photon_build(pp, nthreads=cores)
     W, H, spp = cfg["width"], cfg["height"], cfg["spp"]
     strata = int(spp ** 0.5)
     done = 0
     t0 = time.perf_counter()
     while True:
-        p = Renderer.params(W, H, sample_idx=done, strata=strata, max_depth=cfg["max_depth"], use_photons=photons)
+        p = Renderer.params(W, H, sample_idx=done, strata=strata, max_depth=cfg["max_depth"], use_photons=photons,
+                            wavelengths=cfg.get("wavelengths", 0))
         orc.render_sample(cam, p, mode=0, nthreads=cores)
         done += 1
         if time.perf_counter() - t0 >= seconds or done >= 100000:

@@ -28,7 +28,7 @@ def _probe(r, run, opts, repeats=2):
     return ms / repeats
 
 
-def tune_wavefront(r, out, cam, W, H, spp, depth, photons=False, probe_spp=None, log=None, knn=False):
+def tune_wavefront(r, out, cam, W, H, spp, depth, photons=False, probe_spp=None, log=None, knn=False, wavelengths=0):
     """Tunes `Renderer.render_wavefront` on one frame.  Returns (best_opts,
     info) where info["table"] lists every probe as (opts, ms per probe frame)."""
     npix = W * H
@@ -38,7 +38,8 @@ def tune_wavefront(r, out, cam, W, H, spp, depth, photons=False, probe_spp=None,
         probe_spp = max(1, min(spp, (4 * CAPACITIES[-1] + npix - 1) // npix))
 
     def run(o, seed):
-        r.render_wavefront(out, cam, W, H, probe_spp, max_depth=depth, photon_map=photons, seed=seed, **o)
+        r.render_wavefront(out, cam, W, H, probe_spp, max_depth=depth, photon_map=photons, seed=seed,
+                           wavelengths=wavelengths, **o)
     best, info = tune(r, run, log, knn=photons and knn)
     info["probe_spp"] = probe_spp
     return best, info
