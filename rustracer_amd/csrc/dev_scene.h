@@ -96,12 +96,21 @@ struct DScene {
 #define RT_LDS_STACK 32
 #endif
 #define RT_PRIV_STACK 96
+// Pointers into LDS: per-lane stack and heap accesses through them compile to
+// ds_read / ds_write, not to flat accesses that wait on every memory counter.
+typedef __attribute__((address_space(3))) int32_t lds_i32;
+typedef __attribute__((address_space(3))) float lds_f32;
+// The stack's fields are all scalars and pointers, so the compiler keeps the
+// struct in registers; the overflow array is a separate private array of the
+// kernel (dynamically indexed, hence in scratch), touched only past
+// RT_LDS_STACK entries.
 struct Stack {
-    int32_t* lds;      // base of this lane's column, stride = blockDim.x
+    lds_i32* lds;      // base of this lane's column, stride = blockDim.x
     uint32_t stride;
-    int32_t priv[RT_PRIV_STACK];
+    int32_t* priv;     // this lane's overflow: RT_PRIV_STACK entries
     int32_t sp;
-    float* heap;       // k-NN photon gather: this lane's LDS column (stride) of RT_KNN_MAX floats, or null
+    lds_f32* heap;     // k-NN photon gather: this lane's LDS column (stride) of RT_KNN_MAX floats
+    bool has_heap;     // heap is set (an LDS pointer may be 0, so its value cannot say)
     uint32_t n_nodes, n_prims, n_rays;   // traversal work, for the counting pass
     uint32_t s_nodes, s_prims, s_rays;   // the same for shadow (transmittance) queries
 };

@@ -1077,7 +1077,7 @@ RT_D V3 photon_gather(const DScene& S, V3 pos, V3 nrm, float r2) { uint32_t c; r
 
 // Push d2 into this lane's max-heap of k floats (its LDS column, `stride`
 // apart); once k are held, replace the root when d2 is smaller.
-RT_D void knn_heap_add(float* heap, uint32_t stride, uint32_t k, uint32_t& cnt, float d2) {
+RT_D void knn_heap_add(lds_f32* heap, uint32_t stride, uint32_t k, uint32_t& cnt, float d2) {
     if (cnt < k) {                               // push, sift up
         uint32_t i = cnt++;
         while (i > 0) {
@@ -1117,7 +1117,7 @@ RT_D void knn_heap_add(float* heap, uint32_t stride, uint32_t k, uint32_t& cnt, 
 // Stackless: bit d of `pend` marks the far child of the depth-d plane as
 // still to visit; the ancestor d levels up of node i is ((i + 1) >> d) - 1.
 template <bool KNN, bool SUM = true>
-RT_D V3 pk_walk(const DScene& S, V3 pos, V3 nrm, float r2, float* heap, uint32_t stride, uint32_t& count, float& bound_out,
+RT_D V3 pk_walk(const DScene& S, V3 pos, V3 nrm, float r2, lds_f32* heap, uint32_t stride, uint32_t& count, float& bound_out,
                 uint32_t stop = 0) {
     const uint32_t D = S.pk_depth, k = S.ph_knn;
     const uint32_t first_leaf = (1u << D) - 1u;
@@ -1174,7 +1174,7 @@ RT_D V3 pk_walk(const DScene& S, V3 pos, V3 nrm, float r2, float* heap, uint32_t
 // the query's hemisphere closer than the gather radius.  Fewer than k (or a
 // zero k-th distance): the gather radius, and `acc` / `count` < k carry that
 // gather's sum.
-RT_D float photon_knn_r2(const DScene& S, V3 pos, V3 nrm, float* heap, uint32_t stride, float bound0, V3& acc,
+RT_D float photon_knn_r2(const DScene& S, V3 pos, V3 nrm, lds_f32* heap, uint32_t stride, float bound0, V3& acc,
                          uint32_t& count) {
     float bound;
     acc = pk_walk<true>(S, pos, nrm, bound0, heap, stride, count, bound);
@@ -1198,7 +1198,7 @@ RT_D bool photon_knn_sparse(const DScene& S, V3 pos, V3 nrm, V3& e) {
 
 // k-NN irradiance: the heap search's radius, then the gather at that radius
 // (with fewer than k photons the search already summed the gather).
-RT_D V3 photon_knn_irradiance(const DScene& S, V3 pos, V3 nrm, float* heap, uint32_t stride) {
+RT_D V3 photon_knn_irradiance(const DScene& S, V3 pos, V3 nrm, lds_f32* heap, uint32_t stride) {
     V3 acc;
     uint32_t cnt;
     float r2 = photon_knn_r2(S, pos, nrm, heap, stride, S.ph_r2, acc, cnt);
@@ -1214,7 +1214,7 @@ RT_D V3 photon_knn_irradiance(const DScene& S, V3 pos, V3 nrm, float* heap, uint
 // distance.  Equals photon_knn_irradiance; should the search hold fewer than
 // k after all, the gather within ph_r2 is that function's sum (same walk,
 // same order, the radius never shrinking).
-RT_D V3 photon_knn_dense(const DScene& S, V3 pos, V3 nrm, float* heap, uint32_t stride) {
+RT_D V3 photon_knn_dense(const DScene& S, V3 pos, V3 nrm, lds_f32* heap, uint32_t stride) {
     uint32_t cnt;
     float unused;
     pk_walk<true, false>(S, pos, nrm, S.ph_r2, heap, stride, cnt, unused);
@@ -1225,11 +1225,11 @@ RT_D V3 photon_knn_dense(const DScene& S, V3 pos, V3 nrm, float* heap, uint32_t 
 }
 
 // PhotonMap::irradiance (src/photon.rs:148-157).  A k-NN map is gathered here
-// when the caller has an LDS heap (st.heap), else `knn_irr` holds the value
+// when the caller has an LDS heap (st.has_heap), else `knn_irr` holds the value
 // the wavefront's k_wf_knn stage computed for this hit.
 RT_D V3 photon_irradiance(const DScene& S, V3 pos, V3 nrm, V3 knn_irr, const Stack& st) {
     if (S.num_photons == 0) return mk(0, 0, 0);
-    if (S.ph_knn) return st.heap ? photon_knn_irradiance(S, pos, nrm, st.heap, st.stride) : knn_irr;
+    if (S.ph_knn) return st.has_heap ? photon_knn_irradiance(S, pos, nrm, st.heap, st.stride) : knn_irr;
     return photon_gather(S, pos, nrm, S.ph_r2) * S.ph_norm;
 }
 

@@ -12,7 +12,7 @@ __global__ __launch_bounds__(BLOCK) void k_backward(DScene S, DCamera cam, DPara
     __shared__ float g[RT_MAX_PARAMS];
     for (uint32_t t = threadIdx.x; t < nparams; t += BLOCK) g[t] = 0.0f;
     __syncthreads();
-    Stack st; init_stack(st);
+    RT_STACK(st);
     uint32_t i = blockIdx.x * BLOCK + threadIdx.x;
     uint32_t npix = P.width * P.height;
     if (i < npix) {

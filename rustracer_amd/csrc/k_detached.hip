@@ -5,7 +5,7 @@
 // evaluated on the materials `me` while every decision follows the scene's.
 __global__ __launch_bounds__(BLOCK) void k_detached(DScene S, DCamera cam, DParams P, const DMaterial* __restrict__ me,
                                                     const MedPar* __restrict__ mp, int nmp, float* __restrict__ out, uint32_t spp) {
-    Stack st; init_stack(st);
+    RT_STACK(st);
     uint32_t i = blockIdx.x * BLOCK + threadIdx.x;
     uint32_t npix = P.width * P.height;
     if (i >= npix) return;

@@ -4,7 +4,7 @@
 #include "rt_kern.h"
 
 __global__ __launch_bounds__(BLOCK) void k_trace_closest(DScene S, const rt_ray* __restrict__ rays, uint32_t n, rt_hit* __restrict__ hits) {
-    Stack st; init_stack(st);
+    RT_STACK(st);
     uint32_t i = blockIdx.x * BLOCK + threadIdx.x;
     if (i >= n) return;
     rt_ray q = rays[i];
@@ -22,7 +22,7 @@ __global__ __launch_bounds__(BLOCK) void k_trace_closest(DScene S, const rt_ray*
 }
 
 __global__ __launch_bounds__(BLOCK) void k_trace_tr(DScene S, const rt_ray* __restrict__ rays, uint32_t n, float* __restrict__ tr, uint8_t* __restrict__ occ) {
-    Stack st; init_stack(st);
+    RT_STACK(st);
     uint32_t i = blockIdx.x * BLOCK + threadIdx.x;
     if (i >= n) return;
     rt_ray q = rays[i];
@@ -109,7 +109,7 @@ __global__ void k_progressive_step(uint32_t* ctl, uint32_t pass, uint32_t npix, 
 // and the hit normal, a miss the background colour and a zero normal.
 __global__ __launch_bounds__(BLOCK) void k_aux(DScene S, DCamera cam, uint32_t w, uint32_t h, float* __restrict__ alb,
                                                float* __restrict__ nrm) {
-    Stack st; init_stack(st);
+    RT_STACK(st);
     uint32_t i = blockIdx.x * BLOCK + threadIdx.x;
     if (i >= w * h) return;
     uint32_t row = i / w, col = i - row * w;
@@ -221,7 +221,7 @@ __global__ __launch_bounds__(BLOCK) void k_photon_irr(DScene S, const float* __r
     uint32_t i = blockIdx.x * BLOCK + threadIdx.x;
     if (i >= n) return;
     Stack st;
-    st.heap = KNN ? LdsHeap::column() : nullptr; st.stride = BLOCK;
+    st.heap = KNN ? LdsHeap::column() : nullptr; st.has_heap = KNN; st.stride = BLOCK;
     V3 c = photon_irradiance(S, mk(pn[6 * i], pn[6 * i + 1], pn[6 * i + 2]), mk(pn[6 * i + 3], pn[6 * i + 4], pn[6 * i + 5]), mk(0, 0, 0), st);
     out[3 * i] = c.x; out[3 * i + 1] = c.y; out[3 * i + 2] = c.z;
 }

@@ -2,7 +2,7 @@
 #include "rt_kern.h"
 
 __global__ __launch_bounds__(BLOCK) void k_photon_trace(DScene S, DEmitter E, uint32_t n, float4* __restrict__ out, uint32_t* __restrict__ ok) {
-    Stack st; init_stack(st);
+    RT_STACK(st);
     uint32_t i = blockIdx.x * BLOCK + threadIdx.x;
     if (i >= n) return;
     Rng rng = rng_seed(((uint64_t)i * 6364136223846793005ULL) ^ 0x9E3779B97F4A7C15ULL);

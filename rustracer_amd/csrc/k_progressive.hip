@@ -10,8 +10,8 @@ template <bool KNN, bool H4>
 __global__ __launch_bounds__(BLOCK) void k_progressive(DScene S, DCamera cam, DParams P, rt_accum_opts o, uint32_t pass,
                                                        float* acc, uint32_t* ns, float* m2, uint8_t* conv, uint32_t* ctl) {
     if (ctl[0]) return;
-    Stack st; init_stack(st);
-    if (KNN) st.heap = LdsHeap::column();
+    RT_STACK(st);
+    if (KNN) { st.heap = LdsHeap::column(); st.has_heap = true; }
     uint32_t i = blockIdx.x * BLOCK + threadIdx.x;
     uint32_t npix = P.width * P.height;
     bool newly = false;

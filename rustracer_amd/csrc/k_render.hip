@@ -8,8 +8,8 @@
 template <bool KNN, bool H4>
 __global__ __launch_bounds__(BLOCK) void k_render(DScene S, DCamera cam, DParams P, const uint8_t* __restrict__ conv,
                                                   float* __restrict__ out, uint32_t spp, int accumulate) {
-    Stack st; init_stack(st);
-    if (KNN) st.heap = LdsHeap::column();
+    RT_STACK(st);
+    if (KNN) { st.heap = LdsHeap::column(); st.has_heap = true; }
     uint32_t i = blockIdx.x * BLOCK + threadIdx.x;
     uint32_t npix = P.width * P.height;
     if (i >= npix) return;

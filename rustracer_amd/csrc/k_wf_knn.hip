@@ -165,7 +165,7 @@ template <bool BATCH>
 __global__ __launch_bounds__(BLOCK) void k_wf_knn_dense(DScene S0, WfQueues Q, WfBatch B) {
     const uint32_t total = Q.counters[WF_CNT_DENSE];
     if (total <= Q.knn_wave_max) return;
-    float* heap = LdsHeap::column();
+    lds_f32* heap = LdsHeap::column();
     uint32_t stride = gridDim.x * BLOCK;
     for (uint32_t i = blockIdx.x * BLOCK + threadIdx.x; i < total; i += stride) {
         WfPath& w = Q.paths[Q.dense[i]];

@@ -65,7 +65,7 @@ __device__ __forceinline__ uint32_t extend_one(const DScene& S, WfPath& w, uint3
 // OCC = minimum waves per SIMD the register allocator must allow (0: its own choice).
 template <int OCC, bool BATCH, int BS>
 __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(OCC > 0 ? OCC : 1, 8))) void k_wf_extend(DScene S, WfQueues Q, WfBatch B) {
-    Stack st; init_stack_bs<BS>(st);
+    RT_STACK_BS(st, BS);
     uint32_t count = Q.counters[WF_CNT_ACTIVE];
     uint32_t stride = gridDim.x * BS;
     for (uint32_t base = blockIdx.x * BS; base < count; base += stride) {
@@ -86,7 +86,7 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(OCC > 0 ? OC
 
 template <int OCC, bool BATCH, int BS>
 __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(OCC > 0 ? OCC : 1, 8))) void k_wf_shadow(DScene S, WfQueues Q, WfBatch B) {
-    Stack st; init_stack_bs<BS>(st);
+    RT_STACK_BS(st, BS);
     uint32_t count = Q.counters[WF_CNT_SHADOW];
     uint32_t stride = gridDim.x * BS;
     for (uint32_t i = blockIdx.x * BS + threadIdx.x; i < count; i += stride) {

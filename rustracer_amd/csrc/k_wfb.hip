@@ -65,7 +65,7 @@ __global__ __launch_bounds__(BLOCK) void k_wfb_shade(DScene S, DParams P, WfQueu
     __shared__ float g[RT_MAX_PARAMS];
     for (uint32_t t = threadIdx.x; t < A.nparams; t += BLOCK) g[t] = 0.0f;
     __syncthreads();
-    Stack st; init_stack(st);
+    RT_STACK(st);
     uint32_t off[WF_NB + 1];
     off[0] = 0;
     for (int b = 0; b < WF_NB; b++) off[b + 1] = off[b] + Q.counters[WF_CNT_BUCKET + b];
@@ -147,7 +147,7 @@ __global__ __launch_bounds__(BLOCK) void k_wfb_shadow(DScene S, WfQueues Q, WfAd
     __shared__ float g[RT_MAX_PARAMS];
     for (uint32_t t = threadIdx.x; t < A.nparams; t += BLOCK) g[t] = 0.0f;
     __syncthreads();
-    Stack st; init_stack(st);
+    RT_STACK(st);
     uint32_t count = Q.counters[WF_CNT_SHADOW];
     uint32_t stride = gridDim.x * BLOCK;
     for (uint32_t i = blockIdx.x * BLOCK + threadIdx.x; i < count; i += stride) {

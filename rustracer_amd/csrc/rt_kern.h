@@ -14,22 +14,27 @@ using namespace rt;
 #define BLOCK 128
 
 struct LdsStack {
-    __device__ static int32_t* column() {
+    __device__ static lds_i32* column() {
         __shared__ int32_t s[RT_LDS_STACK * BLOCK];
-        return s + threadIdx.x;
+        return (lds_i32*)(s + threadIdx.x);
     }
 };
 // The k-NN photon gather's max-heap: RT_KNN_MAX floats per lane (32 KB per
 // block), allocated only by the kernel variants that search k-NN radii.
 struct LdsHeap {
-    __device__ static float* column() {
+    __device__ static lds_f32* column() {
         __shared__ float h[RT_KNN_MAX * BLOCK];
-        return h + threadIdx.x;
+        return (lds_f32*)(h + threadIdx.x);
     }
 };
 
-__device__ __forceinline__ void init_stack(Stack& st) {
-    st.lds = LdsStack::column(); st.stride = BLOCK; st.sp = 0; st.heap = nullptr;
+// A traversal stack: the lane's LDS column plus its private overflow array,
+// declared beside the stack so the Stack itself stays in registers.
+#define RT_STACK(st) int32_t st##_priv[RT_PRIV_STACK]; Stack st; init_stack(st, st##_priv)
+#define RT_STACK_BS(st, BS) int32_t st##_priv[RT_PRIV_STACK]; Stack st; init_stack_bs<BS>(st, st##_priv)
+
+__device__ __forceinline__ void init_stack(Stack& st, int32_t* priv) {
+    st.lds = LdsStack::column(); st.stride = BLOCK; st.priv = priv; st.sp = 0; st.heap = nullptr; st.has_heap = false;
     st.n_nodes = 0; st.n_prims = 0; st.n_rays = 0;
     st.s_nodes = 0; st.s_prims = 0; st.s_rays = 0;
 }
@@ -37,14 +42,14 @@ __device__ __forceinline__ void init_stack(Stack& st) {
 // and 256 lanes (the autotuner picks one); their LDS stack is sized to it.
 template <int BS>
 struct LdsStackT {
-    __device__ static int32_t* column() {
+    __device__ static lds_i32* column() {
         __shared__ int32_t s[RT_LDS_STACK * BS];
-        return s + threadIdx.x;
+        return (lds_i32*)(s + threadIdx.x);
     }
 };
 template <int BS>
-__device__ __forceinline__ void init_stack_bs(Stack& st) {
-    st.lds = LdsStackT<BS>::column(); st.stride = BS; st.sp = 0; st.heap = nullptr;
+__device__ __forceinline__ void init_stack_bs(Stack& st, int32_t* priv) {
+    st.lds = LdsStackT<BS>::column(); st.stride = BS; st.priv = priv; st.sp = 0; st.heap = nullptr; st.has_heap = false;
     st.n_nodes = 0; st.n_prims = 0; st.n_rays = 0;
     st.s_nodes = 0; st.s_prims = 0; st.s_rays = 0;
 }
