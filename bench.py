@@ -79,6 +79,64 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
+# Path-queue bytes the wavefront stages move per ray (implementation overhead,
+# reported beside the algorithmic bytes): extend reads the queue entry (4 B)
+# and the ray of the 128-B path record (36 B) and writes the hit (12 B) and
+# the bucket entry (4 B); shade reads the bucket entry and the record and
+# writes the record and the next-queue entry; each shadow request is written
+# by shade and read by shadow (64 B).
+QUEUE_BYTES = {"extend": 4 + 36 + 12 + 4, "shade": 4 + 128 + 128 + 4, "shadow_request": 64}
+
+
+def stage_figures(st, cnt, samples):
+    """Per-stage figures of a timed wavefront pass (graphs off, one stream):
+    average ms per launch of each stage, the path-queue bytes it moves (per
+    sample and as GB/s over the stage's time) and, with a k-NN photon map,
+    the k-NN stage's algorithmic bytes (photon records read by the reference
+    algorithm's gathers, from the counting pass)."""
+    ns = max(cnt["samples"], 1)
+    rays, srays = cnt["rays"] / ns, cnt["shadow_rays"] / ns
+    qb = {"extend": rays * QUEUE_BYTES["extend"],
+          "shade": rays * QUEUE_BYTES["shade"] + srays * QUEUE_BYTES["shadow_request"],
+          "shadow": srays * QUEUE_BYTES["shadow_request"]}
+    out = {}
+    for k, (ms, n) in st.items():
+        if not n:
+            continue
+        f = {"avg_launch_ms": ms / n, "launches": n, "total_ms": ms}
+        if k in qb:
+            f["queue_bytes_per_sample"] = qb[k]
+            f["queue_gbs"] = qb[k] * samples / (ms / 1e3) / 1e9
+        if k == "knn":
+            pbs = cnt["photon_bytes"] / ns
+            bpl = pbs * samples / n
+            f.update({"photon_bytes_per_sample": pbs, "bytes_per_launch": bpl,
+                      "achieved_gbs": bpl / (ms / n / 1e3) / 1e9,
+                      "note": "photon map is cache-resident (L2 / MALL); latency-bound walks, not an HBM figure"})
+        out[k] = f
+    return out
+
+
+def tune_plain(r, step_plain, log_name):
+    """Register budget of the plain megakernel (0: the compiler's 256 VGPRs, 2:
+    two waves per SIMD), picked on one untimed frame each."""
+    import torch
+    best, best_ms = 0, None
+    for occ in (0, 2):
+        r.set_megakernel_occupancy(occ)
+        step_plain(50)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        step_plain(51)
+        torch.cuda.synchronize()
+        ms = (time.perf_counter() - t0) * 1e3
+        log(f"[{log_name}] plain megakernel, occupancy {occ}: {ms:.1f} ms per frame")
+        if best_ms is None or ms < best_ms:
+            best, best_ms = occ, ms
+    r.set_megakernel_occupancy(best)
+    return best
+
+
 def measure(name, args, with_cpu):
     import numpy as np
     import torch
@@ -161,7 +219,9 @@ def measure(name, args, with_cpu):
 
     samples = W * H * spp
     res = {}
+    plain_occ = None
     if args.renderer in ("both", "plain"):
+        plain_occ = tune_plain(r, step_plain, name)
         res["plain"] = timed(step_plain)
         log(f"[{name}] plain {samples * args.steps / res['plain'][0] / 1e6:.1f} M samples/s")
     if args.renderer in ("both", "wavefront"):
@@ -197,10 +257,11 @@ def measure(name, args, with_cpu):
             step_wf(2000 + k, graph=False, streams=1)
         torch.cuda.synchronize()
         ems, en = r.stage_timing()
+        stages = stage_figures(r.stage_times(), cnt, samples * nfr)
         r.set_timing(False)
         bpl = closest_bps * samples * nfr / max(en, 1)
         roof["wavefront"] = {"kernel": "k_wf_extend", "avg_launch_ms": ems / max(en, 1), "bytes_per_launch": bpl,
-                             "achieved": bpl / (ems / 1e3 / max(en, 1)) / 1e9, "launches": en}
+                             "achieved": bpl / (ems / 1e3 / max(en, 1)) / 1e9, "launches": en, "stages": stages}
     rh = roof[head]
     one_wl = None
     if wl and "wavefront" in res:
@@ -229,6 +290,7 @@ def measure(name, args, with_cpu):
                                    "one over [380, 780) nm" if wl == 1 else "one hero wavelength over [380, 700) nm"),
                    "wavefront_samples_per_s_one_wavelength_380_700": one_wl,
                    "wavefront_over_plain": (rates["wavefront"] / rates["plain"]) if len(rates) == 2 else None,
+                   "plain_megakernel_occupancy": plain_occ,
                    "frame_max_rel_diff_wavefront_vs_plain": same_frame,
                    "wavefront_opts": tuned,
                    "autotune": tune_info and {k: tune_info[k] for k in ("probe_spp", "probes", "seconds")},
@@ -240,6 +302,8 @@ def measure(name, args, with_cpu):
                      "closest_bytes_per_sample": closest_bps, "shadow_bytes_per_sample": shadow_bps,
                      "node_tests_per_sample": cnt["node_tests"] / ns, "prim_tests_per_sample": cnt["prim_tests"] / ns,
                      "rays_per_sample": cnt["rays"] / ns, "shadow_rays_per_sample": cnt["shadow_rays"] / ns,
+                     "photon_bytes_per_sample": cnt["photon_bytes"] / ns,
+                     "stages": roof.get("wavefront", {}).get("stages"),
                      "per_renderer": roof},
         "cpu_baseline": cpu,
     }
@@ -320,6 +384,7 @@ def measure_batch(args, with_cpu):
     r.render_batch(views, **tb)
     torch.cuda.synchronize()
     ems, en = r.stage_timing()
+    st_times = r.stage_times()
     r.set_timing(False)
     # traversal work per sample from a counting pass of the plain megakernel
     # on every view at 2 spp
@@ -337,6 +402,7 @@ def measure_batch(args, with_cpu):
     closest_bps = (cnt["node_tests"] * 256.0 + cnt["prim_tests"] * pb) / max(cnt["samples"], 1)
     bpl = closest_bps * samples / max(en, 1)
     achieved = bpl / (ems / 1e3 / max(en, 1)) / 1e9
+    stages = stage_figures(st_times, cnt, samples)
     cpu = None
     if with_cpu:
         sc, cam, cfg = items[-1]
@@ -357,7 +423,7 @@ def measure_batch(args, with_cpu):
                        "photons_stored": photons, "setup_s": round(setup_s, 2)},
             "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                          "frac": achieved / HBM_PEAK_GBS, "kernel": "k_wf_extend", "avg_launch_ms": ems / max(en, 1),
-                         "bytes_per_launch": bpl, "launches": en},
+                         "bytes_per_launch": bpl, "launches": en, "stages": stages},
             "cpu_baseline": cpu}
 
 
@@ -868,7 +934,8 @@ def main():
         else:
             m = measure(name, args, with_cpu=not args.no_cpu)
         also[m["config"]["workload"]] = {"value": m["value"], "unit": m.get("unit", "samples/s"), "config": m["config"],
-                                          "roofline": {k: m["roofline"][k] for k in ("achieved", "frac", "kernel", "avg_launch_ms")},
+                                          "roofline": {k: m["roofline"][k] for k in ("achieved", "frac", "kernel", "avg_launch_ms",
+                                                                                     "stages") if k in m["roofline"]},
                                           "cpu_baseline": m["cpu_baseline"]}
     line = {
         "metric": "GPU samples/s (wavefront vs plain) at fixed RMSE; adjoint grad evals/s",

@@ -661,11 +661,25 @@ int  rt_ctx_get_timing(rt_ctx *ctx, double *kernel_ms, uint64_t *launches);
 int  rt_ctx_set_counting(rt_ctx *ctx, int enable);
 /* Megakernel renders since counting was enabled: out[0] BVH node tests,
  * [1] primitive tests, [2] rays of closest-hit queries, [3] samples,
- * [4] node tests, [5] primitive tests, [6] rays of shadow queries, [7] 0.    */
+ * [4] node tests, [5] primitive tests, [6] rays of shadow queries, [7] bytes
+ * of photon-map records the photon gathers read (kd photon: 32 B position +
+ * normal, 16 B power when summed; bucket tree: 8 B per split plane, 32 B
+ * record per photon of a visited leaf, 16 B power when summed).            */
 int  rt_ctx_get_counters(rt_ctx *ctx, uint64_t out[8]);
 /* Wavefront renders with timing on and graphs off: summed duration (ms) and
  * count of the k_wf_extend launches (closest-hit traversal stage).           */
 int  rt_ctx_get_stage_timing(rt_ctx *ctx, double *extend_ms, uint64_t *extend_launches);
+/* The same for every stage of a bounce: [0] closest hit (k_wf_extend), [1]
+ * k-NN photon gather (k_wf_knn + k_wf_knn_dense + k_wf_knn_wave, bounces of
+ * renders with a k-NN photon map only), [2] shade (k_wf_shade), [3] shadow
+ * (k_wf_shadow): summed duration (ms) and number of timed bounces.           */
+int  rt_ctx_get_stage_times(rt_ctx *ctx, double ms[4], uint64_t launches[4]);
+/* Register budget of the plain megakernel (rt_render_sample,
+ * rt_render_accumulate, rt_render_tiles): 0 the compiler's choice (all 256
+ * VGPRs, one wave per SIMD), 2 at least two waves per SIMD (at most 128
+ * VGPRs, the rest spilled).  Same frames either way; a tuning knob, as the
+ * wavefront's occupancy option is.                                           */
+int  rt_ctx_set_megakernel_occupancy(rt_ctx *ctx, uint32_t occupancy);
 
 #ifdef __cplusplus
 }

@@ -204,6 +204,8 @@ SIGNATURES = {
     "rt_ctx_set_counting": (C.c_int, [VP, C.c_int]),
     "rt_ctx_get_counters": (C.c_int, [VP, C.POINTER(C.c_uint64)]),
     "rt_ctx_get_stage_timing": (C.c_int, [VP, C.POINTER(C.c_double), C.POINTER(C.c_uint64)]),
+    "rt_ctx_get_stage_times": (C.c_int, [VP, C.POINTER(C.c_double), C.POINTER(C.c_uint64)]),
+    "rt_ctx_set_megakernel_occupancy": (C.c_int, [VP, C.c_uint32]),
 }
 
 _lib = None

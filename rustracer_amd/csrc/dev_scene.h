@@ -113,6 +113,7 @@ struct Stack {
     bool has_heap;     // heap is set (an LDS pointer may be 0, so its value cannot say)
     uint32_t n_nodes, n_prims, n_rays;   // traversal work, for the counting pass
     uint32_t s_nodes, s_prims, s_rays;   // the same for shadow (transmittance) queries
+    uint32_t n_pbytes;                   // photon gathers: record bytes read
 };
 RT_D void st_push(Stack& s, int32_t v) {
     if (s.sp < RT_LDS_STACK) s.lds[s.sp * s.stride] = v;

@@ -1022,8 +1022,11 @@ RT_D V3 lights_generate(const DScene& S, V3 origin, Rng& rng, float time) {
 // The Epanechnikov sum of kd_gather (src/photon.rs:195-228) over the
 // heap-ordered tree, before normalisation; `count` (optional) receives the
 // number of photons that passed the radius and hemisphere tests.
+// nb (may be null): adds the bytes of the photon records the walk reads
+// (16-B position + split axis and 16-B normal per node visited, 16-B power
+// per photon summed), for the counting pass.
 template <bool SUM>
-RT_D V3 photon_walk(const DScene& S, V3 pos, V3 nrm, float r2, uint32_t& count) {
+RT_D V3 photon_walk(const DScene& S, V3 pos, V3 nrm, float r2, uint32_t& count, uint32_t* nb = nullptr) {
     uint32_t n = S.num_photons;
     uint32_t cnt = 0;
     V3 acc = mk(0, 0, 0);
@@ -1033,12 +1036,13 @@ RT_D V3 photon_walk(const DScene& S, V3 pos, V3 nrm, float r2, uint32_t& count) 
         if (node < n) {
             float4 P = S.ph_pos[node];
             float4 N = S.ph_nrm[node];
+            if (nb) *nb += 32u;
             float dx = P.x - pos.x, dy = P.y - pos.y, dz = P.z - pos.z;
             float d2 = dx * dx + dy * dy + dz * dz;
             if (d2 < r2) {
                 float dt = N.x * nrm.x + N.y * nrm.y + N.z * nrm.z;
                 if (dt > 0.0f) {
-                    if (SUM) { float4 W = S.ph_pow[node]; acc += mk(W.x, W.y, W.z) * (1.0f - d2 / r2); }
+                    if (SUM) { float4 W = S.ph_pow[node]; acc += mk(W.x, W.y, W.z) * (1.0f - d2 / r2); if (nb) *nb += 16u; }
                     cnt++;
                 }
             }
@@ -1062,7 +1066,10 @@ RT_D V3 photon_walk(const DScene& S, V3 pos, V3 nrm, float r2, uint32_t& count) 
     count = cnt;
     return acc;
 }
-RT_D V3 photon_gather(const DScene& S, V3 pos, V3 nrm, float r2) { uint32_t c; return photon_walk<true>(S, pos, nrm, r2, c); }
+RT_D V3 photon_gather(const DScene& S, V3 pos, V3 nrm, float r2, uint32_t* nb = nullptr) {
+    uint32_t c;
+    return photon_walk<true>(S, pos, nrm, r2, c, nb);
+}
 
 // k-NN gather (extension) over the bucket tree: a complete binary tree of
 // pk_depth levels of splitting planes (heap order, {plane, axis bits}) whose
@@ -1116,9 +1123,12 @@ RT_D void knn_heap_add(lds_f32* heap, uint32_t stride, uint32_t k, uint32_t& cnt
 // KNN): no sum, for callers that already know k photons lie within r2.
 // Stackless: bit d of `pend` marks the far child of the depth-d plane as
 // still to visit; the ancestor d levels up of node i is ((i + 1) >> d) - 1.
+// nb (may be null): adds the bytes the walk reads (8-B plane per split
+// visited, 32-B record per photon of a visited leaf, 16-B power per photon
+// summed), for the counting pass.
 template <bool KNN, bool SUM = true>
 RT_D V3 pk_walk(const DScene& S, V3 pos, V3 nrm, float r2, lds_f32* heap, uint32_t stride, uint32_t& count, float& bound_out,
-                uint32_t stop = 0) {
+                uint32_t stop = 0, uint32_t* nb = nullptr) {
     const uint32_t D = S.pk_depth, k = S.ph_knn;
     const uint32_t first_leaf = (1u << D) - 1u;
     float bound = r2;
@@ -1128,6 +1138,7 @@ RT_D V3 pk_walk(const DScene& S, V3 pos, V3 nrm, float r2, lds_f32* heap, uint32
     for (;;) {
         while (depth < D) {                                  // descend to the near leaf
             float2 sp = S.pk_split[node];
+            if (nb) *nb += 8u;
             float sq = comp(pos, __float_as_int(sp.y)) - sp.x;
             if (sq * sq < bound) pend |= 1u << depth; else pend &= ~(1u << depth);
             node = sq <= 0.0f ? 2 * node + 1 : 2 * node + 2;
@@ -1135,12 +1146,17 @@ RT_D V3 pk_walk(const DScene& S, V3 pos, V3 nrm, float r2, lds_f32* heap, uint32
         }
         uint32_t leaf = node - first_leaf;
         uint32_t b = S.pk_leaf[leaf], e = S.pk_leaf[leaf + 1];
+        if (nb) *nb += 32u * (e - b);
         for (uint32_t i = b; i < e; i++) {
             float4 P = S.pk_rec[2 * i], N = S.pk_rec[2 * i + 1];
             float dx = P.x - pos.x, dy = P.y - pos.y, dz = P.z - pos.z;
             float d2 = dx * dx + dy * dy + dz * dz;
             if (d2 < bound && N.x * nrm.x + N.y * nrm.y + N.z * nrm.z > 0.0f) {
-                if (!KNN || (SUM && cnt < k)) { float4 W = S.pk_pow[i]; acc += mk(W.x, W.y, W.z) * (1.0f - d2 / bound); }
+                if (!KNN || (SUM && cnt < k)) {
+                    float4 W = S.pk_pow[i];
+                    acc += mk(W.x, W.y, W.z) * (1.0f - d2 / bound);
+                    if (nb) *nb += 16u;
+                }
                 if (KNN) {
                     knn_heap_add(heap, stride, k, cnt, d2);
                     if (cnt == k) bound = heap[0];
@@ -1156,6 +1172,7 @@ RT_D V3 pk_walk(const DScene& S, V3 pos, V3 nrm, float r2, lds_f32* heap, uint32
             pend &= ~(1u << d);
             uint32_t anc = ((node + 1) >> (depth - d)) - 1;
             float2 sp = S.pk_split[anc];
+            if (nb) *nb += 8u;
             float sq = comp(pos, __float_as_int(sp.y)) - sp.x;
             if (KNN && !(sq * sq < bound)) continue;
             node = sq <= 0.0f ? 2 * anc + 2 : 2 * anc + 1;
@@ -1175,9 +1192,9 @@ RT_D V3 pk_walk(const DScene& S, V3 pos, V3 nrm, float r2, lds_f32* heap, uint32
 // zero k-th distance): the gather radius, and `acc` / `count` < k carry that
 // gather's sum.
 RT_D float photon_knn_r2(const DScene& S, V3 pos, V3 nrm, lds_f32* heap, uint32_t stride, float bound0, V3& acc,
-                         uint32_t& count) {
+                         uint32_t& count, uint32_t* nb = nullptr) {
     float bound;
-    acc = pk_walk<true>(S, pos, nrm, bound0, heap, stride, count, bound);
+    acc = pk_walk<true>(S, pos, nrm, bound0, heap, stride, count, bound, 0, nb);
     if (count < S.ph_knn) return S.ph_r2;
     float r2 = heap[0];
     return r2 > 0.0f ? r2 : S.ph_r2;
@@ -1198,14 +1215,14 @@ RT_D bool photon_knn_sparse(const DScene& S, V3 pos, V3 nrm, V3& e) {
 
 // k-NN irradiance: the heap search's radius, then the gather at that radius
 // (with fewer than k photons the search already summed the gather).
-RT_D V3 photon_knn_irradiance(const DScene& S, V3 pos, V3 nrm, lds_f32* heap, uint32_t stride) {
+RT_D V3 photon_knn_irradiance(const DScene& S, V3 pos, V3 nrm, lds_f32* heap, uint32_t stride, uint32_t* nb = nullptr) {
     V3 acc;
     uint32_t cnt;
-    float r2 = photon_knn_r2(S, pos, nrm, heap, stride, S.ph_r2, acc, cnt);
+    float r2 = photon_knn_r2(S, pos, nrm, heap, stride, S.ph_r2, acc, cnt, nb);
     if (cnt < S.ph_knn) return acc * S.ph_norm;
     float norm = r2 == S.ph_r2 ? S.ph_norm : 2.0f / (PI * PI * r2);
     float unused;
-    return pk_walk<false>(S, pos, nrm, r2, nullptr, 0, cnt, unused) * norm;
+    return pk_walk<false>(S, pos, nrm, r2, nullptr, 0, cnt, unused, 0, nb) * norm;
 }
 
 // Second pass of the wavefront's k-NN stage, for a query that the first pass
@@ -1227,10 +1244,11 @@ RT_D V3 photon_knn_dense(const DScene& S, V3 pos, V3 nrm, lds_f32* heap, uint32_
 // PhotonMap::irradiance (src/photon.rs:148-157).  A k-NN map is gathered here
 // when the caller has an LDS heap (st.has_heap), else `knn_irr` holds the value
 // the wavefront's k_wf_knn stage computed for this hit.
-RT_D V3 photon_irradiance(const DScene& S, V3 pos, V3 nrm, V3 knn_irr, const Stack& st) {
+// The gather's photon-record bytes go to st.n_pbytes (counting pass).
+RT_D V3 photon_irradiance(const DScene& S, V3 pos, V3 nrm, V3 knn_irr, Stack& st) {
     if (S.num_photons == 0) return mk(0, 0, 0);
-    if (S.ph_knn) return st.has_heap ? photon_knn_irradiance(S, pos, nrm, st.heap, st.stride) : knn_irr;
-    return photon_gather(S, pos, nrm, S.ph_r2) * S.ph_norm;
+    if (S.ph_knn) return st.has_heap ? photon_knn_irradiance(S, pos, nrm, st.heap, st.stride, &st.n_pbytes) : knn_irr;
+    return photon_gather(S, pos, nrm, S.ph_r2, &st.n_pbytes) * S.ph_norm;
 }
 
 // ---- one bounce of ray_color (src/renderer.rs:371-585) ---------------------------------------

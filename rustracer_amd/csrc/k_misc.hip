@@ -221,7 +221,7 @@ __global__ __launch_bounds__(BLOCK) void k_photon_irr(DScene S, const float* __r
     uint32_t i = blockIdx.x * BLOCK + threadIdx.x;
     if (i >= n) return;
     Stack st;
-    st.heap = KNN ? LdsHeap::column() : nullptr; st.has_heap = KNN; st.stride = BLOCK;
+    st.heap = KNN ? LdsHeap::column() : nullptr; st.has_heap = KNN; st.stride = BLOCK; st.n_pbytes = 0;
     V3 c = photon_irradiance(S, mk(pn[6 * i], pn[6 * i + 1], pn[6 * i + 2]), mk(pn[6 * i + 3], pn[6 * i + 4], pn[6 * i + 5]), mk(0, 0, 0), st);
     out[3 * i] = c.x; out[3 * i + 1] = c.y; out[3 * i + 2] = c.z;
 }

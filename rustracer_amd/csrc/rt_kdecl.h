@@ -25,8 +25,8 @@ __global__ void k_accum_passes(const float* lsamp, uint32_t K, uint32_t w, uint3
 template <bool KNN> __global__ void k_photon_irr(DScene S, const float* pn, uint32_t n, float* out);
 __global__ void k_fill_u32(uint32_t* p, uint64_t n, uint32_t v);
 
-// k_render.hip, k_progressive.hip
-template <bool KNN, bool H4> __global__ void k_render(DScene S, DCamera cam, DParams P, const uint8_t* conv, float* out, uint32_t spp, int accumulate);
+// k_render.hip, k_render_o2.hip (k_render.inc), k_progressive.hip
+template <int OCC, bool KNN, bool H4> __global__ void k_render(DScene S, DCamera cam, DParams P, const uint8_t* conv, float* out, uint32_t spp, int accumulate);
 template <bool KNN, bool H4> __global__ void k_progressive(DScene S, DCamera cam, DParams P, rt_accum_opts o, uint32_t pass, float* acc,
                                                   uint32_t* ns, float* m2, uint8_t* conv, uint32_t* ctl);
 

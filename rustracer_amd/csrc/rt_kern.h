@@ -37,6 +37,7 @@ __device__ __forceinline__ void init_stack(Stack& st, int32_t* priv) {
     st.lds = LdsStack::column(); st.stride = BLOCK; st.priv = priv; st.sp = 0; st.heap = nullptr; st.has_heap = false;
     st.n_nodes = 0; st.n_prims = 0; st.n_rays = 0;
     st.s_nodes = 0; st.s_prims = 0; st.s_rays = 0;
+    st.n_pbytes = 0;
 }
 // The traversal stages of the wavefront are compiled for blocks of 64, 128
 // and 256 lanes (the autotuner picks one); their LDS stack is sized to it.
@@ -52,6 +53,7 @@ __device__ __forceinline__ void init_stack_bs(Stack& st, int32_t* priv) {
     st.lds = LdsStackT<BS>::column(); st.stride = BS; st.priv = priv; st.sp = 0; st.heap = nullptr; st.has_heap = false;
     st.n_nodes = 0; st.n_prims = 0; st.n_rays = 0;
     st.s_nodes = 0; st.s_prims = 0; st.s_rays = 0;
+    st.n_pbytes = 0;
 }
 
 __device__ __forceinline__ Ray ray_from_c(const rt_ray& q) {

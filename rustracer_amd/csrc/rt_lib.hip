@@ -48,10 +48,11 @@ struct rt_ctx {
     size_t wf_bytes = 0;
     uint32_t wf_cap = 0;
     // per-launch events around k_wf_extend (timing on, graphs off)
-    std::vector<hipEvent_t> ext_ev;
+    std::vector<hipEvent_t> ext_ev;           // five per timed bounce (wf_chunk)
     size_t ext_used = 0;
-    double extend_ms = 0.0;
-    uint64_t extend_launches = 0;
+    std::vector<bool> ext_knn;                // per timed bounce: the k-NN stage ran
+    double stage_ms[4] = {0.0, 0.0, 0.0, 0.0};   // wavefront stages: extend, k-NN, shade, shadow
+    uint64_t stage_n[4] = {0, 0, 0, 0};
     uint32_t* prog_ctl = nullptr;             // rt_render_progressive: stopped, converged count, passes run
     // extra wavefront streams and their queues (rt_wavefront_opts.streams > 1)
     hipStream_t xstream[RT_WF_MAX_STREAMS - 1] = {};
@@ -61,6 +62,7 @@ struct rt_ctx {
     // this device's copy of the file-scope `scratch` array (rt_render_wavefront_noalloc)
     void* scratch = nullptr;
     bool no_alloc = false;                    // inside rt_render_wavefront_noalloc: no event creation
+    int mk_occ = 0;                           // register budget of the plain megakernel (rt_ctx_set_megakernel_occupancy)
 };
 
 struct rt_scene {
@@ -898,12 +900,20 @@ static void to_dev(const rt_camera* c, const rt_sample_params* p, DCamera& C, DP
     P.bg_scale = p->bg_scale; P.t0 = p->frame_t0; P.t1 = p->frame_t1; P.seed = p->seed;
     P.wl = (uint32_t)std::max(wl_mode(p), 0);
 }
-static void launch_render(bool knn, bool h4, dim3 grid, hipStream_t st, const DScene& S, const DCamera& C, const DParams& Q,
-                          const uint8_t* conv, float* out, uint32_t m, int acc) {
-    if (knn && h4) hipLaunchKernelGGL((k_render<true, true>), grid, dim3(BLOCK), 0, st, S, C, Q, conv, out, m, acc);
-    else if (knn) hipLaunchKernelGGL((k_render<true, false>), grid, dim3(BLOCK), 0, st, S, C, Q, conv, out, m, acc);
-    else if (h4) hipLaunchKernelGGL((k_render<false, true>), grid, dim3(BLOCK), 0, st, S, C, Q, conv, out, m, acc);
-    else hipLaunchKernelGGL((k_render<false, false>), grid, dim3(BLOCK), 0, st, S, C, Q, conv, out, m, acc);
+extern "C++" {
+template <int OCC>
+static void launch_render_occ(bool knn, bool h4, dim3 grid, hipStream_t st, const DScene& S, const DCamera& C,
+                              const DParams& Q, const uint8_t* conv, float* out, uint32_t m, int acc) {
+    if (knn && h4) hipLaunchKernelGGL((k_render<OCC, true, true>), grid, dim3(BLOCK), 0, st, S, C, Q, conv, out, m, acc);
+    else if (knn) hipLaunchKernelGGL((k_render<OCC, true, false>), grid, dim3(BLOCK), 0, st, S, C, Q, conv, out, m, acc);
+    else if (h4) hipLaunchKernelGGL((k_render<OCC, false, true>), grid, dim3(BLOCK), 0, st, S, C, Q, conv, out, m, acc);
+    else hipLaunchKernelGGL((k_render<OCC, false, false>), grid, dim3(BLOCK), 0, st, S, C, Q, conv, out, m, acc);
+}
+}
+static void launch_render(int occ, bool knn, bool h4, dim3 grid, hipStream_t st, const DScene& S, const DCamera& C,
+                          const DParams& Q, const uint8_t* conv, float* out, uint32_t m, int acc) {
+    if (occ == 2) launch_render_occ<2>(knn, h4, grid, st, S, C, Q, conv, out, m, acc);
+    else launch_render_occ<0>(knn, h4, grid, st, S, C, Q, conv, out, m, acc);
 }
 
 static int render_common(rt_ctx* ctx, rt_scene* s, const rt_camera* cam, const rt_sample_params* p, const uint8_t* conv,
@@ -934,12 +944,12 @@ static int render_common(rt_ctx* ctx, rt_scene* s, const rt_camera* cam, const r
             DParams Q = P;
             Q.sample_idx = P.sample_idx + k;
             uint32_t m = std::min(chunk, spp - k);
-            launch_render(knn, P.wl == 2, grid, ctx->stream, s->S, C, Q, (const uint8_t*)cin.dev, (float*)rout.dev, m, 1);
+            launch_render(ctx->mk_occ, knn, P.wl == 2, grid, ctx->stream, s->S, C, Q, (const uint8_t*)cin.dev, (float*)rout.dev, m, 1);
             HIPCHECK(ctx, hipGetLastError());
         }
         nlaunch = (spp + chunk - 1) / chunk;
     } else {
-        launch_render(knn, P.wl == 2, grid, ctx->stream, s->S, C, P, (const uint8_t*)cin.dev, (float*)rout.dev, 1u, 0);
+        launch_render(ctx->mk_occ, knn, P.wl == 2, grid, ctx->stream, s->S, C, P, (const uint8_t*)cin.dev, (float*)rout.dev, 1u, 0);
         HIPCHECK(ctx, hipGetLastError());
     }
     if (ctx->timing) {
@@ -1209,18 +1219,23 @@ static void wf_chunk(rt_ctx* ctx, hipStream_t st, const DScene& S, const DCamera
     B.active = Q0.next; B.next = Q0.active;
     for (uint32_t d = 0; d < depth; d++) {
         const WfQueues& Q = (d & 1) ? B : A;
+        // per-stage events (timing on, no graph): before extend, after extend,
+        // after the k-NN stage, after shade, after shadow
         bool ev = ctx->timing && !ctx->no_alloc && first != ~0ull;
         if (ev) {
-            while (ctx->ext_ev.size() < ctx->ext_used + 2) { hipEvent_t e; if (hipEventCreate(&e) != hipSuccess) { ev = false; break; } ctx->ext_ev.push_back(e); }
+            while (ctx->ext_ev.size() < ctx->ext_used + 5) { hipEvent_t e; if (hipEventCreate(&e) != hipSuccess) { ev = false; break; } ctx->ext_ev.push_back(e); }
         }
-        if (ev) (void)hipEventRecord(ctx->ext_ev[ctx->ext_used], st);
+        hipEvent_t* E = ev ? &ctx->ext_ev[ctx->ext_used] : nullptr;
+        if (ev) { ctx->ext_used += 5; ctx->ext_knn.push_back(knn); }
+        if (ev) (void)hipEventRecord(E[0], st);
         launch_trace<BT>(false, occ, block, grid, st, S, Q, Bt);
-        if (ev) { (void)hipEventRecord(ctx->ext_ev[ctx->ext_used + 1], st); ctx->ext_used += 2; }
+        if (ev) (void)hipEventRecord(E[1], st);
         if (knn) {
             hipLaunchKernelGGL((k_wf_knn<BT>), dim3(grid), dim3(BLOCK), 0, st, S, P, Q, Bt);
             hipLaunchKernelGGL((k_wf_knn_dense<BT>), dim3(grid), dim3(BLOCK), 0, st, S, Q, Bt);
             hipLaunchKernelGGL((k_wf_knn_wave<BT>), dim3(grid), dim3(BLOCK), 0, st, S, Q, Bt);
         }
+        if (ev) (void)hipEventRecord(E[2], st);
         if (Q.sort == 2 && Q.h4) {
             hipLaunchKernelGGL((k_wf_shade<BT, 1, true>), dim3(grid), dim3(BLOCK), 0, st, S, P, Q, (int)d, Bt);
             hipLaunchKernelGGL((k_wf_shade<BT, 2, true>), dim3(grid), dim3(BLOCK), 0, st, S, P, Q, (int)d, Bt);
@@ -1232,7 +1247,9 @@ static void wf_chunk(rt_ctx* ctx, hipStream_t st, const DScene& S, const DCamera
         } else {
             hipLaunchKernelGGL((k_wf_shade<BT, 0, false>), dim3(grid), dim3(BLOCK), 0, st, S, P, Q, (int)d, Bt);
         }
+        if (ev) (void)hipEventRecord(E[3], st);
         launch_trace<BT>(true, occ, block, grid, st, S, Q, Bt);
+        if (ev) (void)hipEventRecord(E[4], st);
         hipLaunchKernelGGL(k_wf_advance, dim3(1), dim3(32), 0, st, Q.counters);
     }
 }
@@ -1326,12 +1343,17 @@ static int wf_run(rt_ctx* ctx, const DScene& S, const DCamera& C, const DParams&
         HIPCHECK(ctx, hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
         ctx->kernel_ms += ms;
         ctx->launches += nchunks;   // one "launch" = one chunk of paths through every bounce
-        for (size_t k = 0; k + 1 < ctx->ext_used; k += 2) {
-            float e = 0.0f;
-            if (hipEventElapsedTime(&e, ctx->ext_ev[k], ctx->ext_ev[k + 1]) == hipSuccess) { ctx->extend_ms += e; ctx->extend_launches++; }
+        for (size_t g = 0; 5 * g + 4 < ctx->ext_used; g++) {
+            const hipEvent_t* E = &ctx->ext_ev[5 * g];
+            for (int k = 0; k < 4; k++) {
+                if (k == 1 && !ctx->ext_knn[g]) continue;
+                float e = 0.0f;
+                if (hipEventElapsedTime(&e, E[k], E[k + 1]) == hipSuccess) { ctx->stage_ms[k] += e; ctx->stage_n[k]++; }
+            }
         }
     }
     ctx->ext_used = 0;
+    ctx->ext_knn.clear();
     return RT_OK;
 }
 
@@ -1517,6 +1539,7 @@ extern "C" int rt_render_progressive_wavefront(rt_ctx* ctx, rt_scene* s, const r
         HIPCHECK(ctx, hipStreamWaitEvent(ctx->stream, ctx->ev_join[k - 1], 0));
     }
     ctx->ext_used = 0;
+    ctx->ext_knn.clear();
     if (ctx->timing) {
         HIPCHECK(ctx, hipEventRecord(ctx->ev1, ctx->stream));
         HIPCHECK(ctx, hipEventSynchronize(ctx->ev1));
@@ -1889,6 +1912,7 @@ extern "C" int rt_backward_wavefront(rt_ctx* ctx, rt_scene* s, const rt_camera* 
         HIPCHECK(ctx, hipStreamWaitEvent(ctx->stream, ctx->ev_join[k - 1], 0));
     }
     ctx->ext_used = 0;
+    ctx->ext_knn.clear();
     if (ctx->timing) {
         HIPCHECK(ctx, hipEventRecord(ctx->ev1, ctx->stream));
         HIPCHECK(ctx, hipEventSynchronize(ctx->ev1));
@@ -2206,15 +2230,19 @@ int rt_ctx_set_timing(rt_ctx* ctx, int enable) {
     ctx->timing = enable != 0;
     ctx->kernel_ms = 0.0;
     ctx->launches = 0;
-    ctx->extend_ms = 0.0;
-    ctx->extend_launches = 0;
+    for (int k = 0; k < 4; k++) { ctx->stage_ms[k] = 0.0; ctx->stage_n[k] = 0; }
     return RT_OK;
 }
 
 int rt_ctx_get_stage_timing(rt_ctx* ctx, double* extend_ms, uint64_t* extend_launches) {
     if (!ctx || !extend_ms || !extend_launches) return RT_ERR_INVALID;
-    *extend_ms = ctx->extend_ms;
-    *extend_launches = ctx->extend_launches;
+    *extend_ms = ctx->stage_ms[0];
+    *extend_launches = ctx->stage_n[0];
+    return RT_OK;
+}
+int rt_ctx_get_stage_times(rt_ctx* ctx, double ms[4], uint64_t launches[4]) {
+    if (!ctx || !ms || !launches) return RT_ERR_INVALID;
+    for (int k = 0; k < 4; k++) { ms[k] = ctx->stage_ms[k]; launches[k] = ctx->stage_n[k]; }
     return RT_OK;
 }
 
@@ -2225,6 +2253,12 @@ int rt_ctx_get_timing(rt_ctx* ctx, double* kernel_ms, uint64_t* launches) {
     return RT_OK;
 }
 
+int rt_ctx_set_megakernel_occupancy(rt_ctx* ctx, uint32_t occupancy) {
+    if (!ctx) return RT_ERR_INVALID;
+    if (occupancy != 0 && occupancy != 2) return fail(ctx, RT_ERR_INVALID, "megakernel occupancy must be 0 or 2");
+    ctx->mk_occ = (int)occupancy;
+    return RT_OK;
+}
 int rt_ctx_set_counting(rt_ctx* ctx, int enable) {
     if (!ctx) return RT_ERR_INVALID;
     HIPCHECK(ctx, hipSetDevice(ctx->device));

@@ -465,6 +465,11 @@ class Renderer:
         self._check(self.L.rt_ctx_get_timing(self.ctx, C.byref(ms), C.byref(n)), "rt_ctx_get_timing")
         return ms.value, n.value
 
+    def set_megakernel_occupancy(self, occupancy):
+        """Register budget of the plain megakernel: 0 (compiler's choice) or 2
+        (at least two waves per SIMD)."""
+        self._check(self.L.rt_ctx_set_megakernel_occupancy(self.ctx, int(occupancy)), "rt_ctx_set_megakernel_occupancy")
+
     def set_counting(self, enable=True):
         self._check(self.L.rt_ctx_set_counting(self.ctx, 1 if enable else 0), "rt_ctx_set_counting")
 
@@ -472,7 +477,15 @@ class Renderer:
         out = (C.c_uint64 * 8)()
         self._check(self.L.rt_ctx_get_counters(self.ctx, out), "rt_ctx_get_counters")
         return {"node_tests": out[0], "prim_tests": out[1], "rays": out[2], "samples": out[3],
-                "shadow_node_tests": out[4], "shadow_prim_tests": out[5], "shadow_rays": out[6]}
+                "shadow_node_tests": out[4], "shadow_prim_tests": out[5], "shadow_rays": out[6],
+                "photon_bytes": out[7]}
+
+    def stage_times(self):
+        """Per-stage wavefront timing since timing was enabled (graphs off):
+        {stage: (ms, timed bounces)} for extend, knn, shade, shadow."""
+        ms, n = (C.c_double * 4)(), (C.c_uint64 * 4)()
+        self._check(self.L.rt_ctx_get_stage_times(self.ctx, ms, n), "rt_ctx_get_stage_times")
+        return {k: (ms[i], n[i]) for i, k in enumerate(("extend", "knn", "shade", "shadow"))}
 
     def stage_timing(self):
         ms, n = C.c_double(), C.c_uint64()
