@@ -236,15 +236,28 @@ RT_D float mnps(float a, float b) { return a < b ? a : b; }
 RT_D float mxps(float a, float b) { return a > b ? a : b; }
 
 // Returns the hit mask; t_near[c] for each child.  Children read as ints.
-// Bounds are loaded and tested one axis at a time (4 x float4 live) so the
-// traversal kernels keep fewer VGPRs than with the whole 192-B box block live.
+// The node's 14 float4 loads are all issued before the slab tests (a
+// scheduling barrier keeps the compiler from spreading them over three
+// dependent rounds to save registers), so a node costs one memory round trip.
+#ifndef RT_NODE_PRELOAD
+#define RT_NODE_PRELOAD 1
+#endif
 RT_D uint32_t node_test(const float4* node, const Ray& r, float t_min, float t_max, float tn[8], int32_t ch[8]) {
     float t0[8], t1[8];
+#if RT_NODE_PRELOAD
+    float4 q[14];
+#pragma unroll
+    for (int i = 0; i < 14; i++) q[i] = node[i];
+    __builtin_amdgcn_sched_barrier(0);      // all 14 loads in flight before the slab tests
+#define RT_NODE_Q(i) q[i]
+#else
+#define RT_NODE_Q(i) node[i]
+#endif
 #pragma unroll
     for (int c = 0; c < 8; c++) { t0[c] = t_min; t1[c] = t_max; }
 #pragma unroll
     for (int a = 0; a < 3; a++) {
-        float4 l0 = node[2 * a], l1 = node[2 * a + 1], h0 = node[6 + 2 * a], h1 = node[7 + 2 * a];
+        float4 l0 = RT_NODE_Q(2 * a), l1 = RT_NODE_Q(2 * a + 1), h0 = RT_NODE_Q(6 + 2 * a), h1 = RT_NODE_Q(7 + 2 * a);
         float lo[8] = {l0.x, l0.y, l0.z, l0.w, l1.x, l1.y, l1.z, l1.w};
         float hi[8] = {h0.x, h0.y, h0.z, h0.w, h1.x, h1.y, h1.z, h1.w};
         float o = a == 0 ? r.o.x : (a == 1 ? r.o.y : r.o.z);
@@ -257,7 +270,8 @@ RT_D uint32_t node_test(const float4* node, const Ray& r, float t_min, float t_m
             t1[c] = mnps(t1[c], mxps(da, db));
         }
     }
-    float4 c0 = node[12], c1 = node[13];
+    float4 c0 = RT_NODE_Q(12), c1 = RT_NODE_Q(13);
+#undef RT_NODE_Q
     ch[0] = __float_as_int(c0.x); ch[1] = __float_as_int(c0.y); ch[2] = __float_as_int(c0.z); ch[3] = __float_as_int(c0.w);
     ch[4] = __float_as_int(c1.x); ch[5] = __float_as_int(c1.y); ch[6] = __float_as_int(c1.z); ch[7] = __float_as_int(c1.w);
     uint32_t mask = 0;
