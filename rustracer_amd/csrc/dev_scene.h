@@ -40,6 +40,7 @@ struct DObject {
 };
 
 struct DQuad { float q[3], u[3], v[3], n[3], w[3], d, area; float pad[3]; };
+static_assert(sizeof(DQuad) == 80, "quad record is five 16-B words");
 
 struct DMedium {
     uint32_t kind; float density; float albedo[3]; float g;
@@ -100,6 +101,10 @@ struct DScene {
 // ds_read / ds_write, not to flat accesses that wait on every memory counter.
 typedef __attribute__((address_space(3))) int32_t lds_i32;
 typedef __attribute__((address_space(3))) float lds_f32;
+// The overflow stack through a private (address-space-5) pointer: stores to
+// it cannot alias global memory, so loads of scene records stay hoisted out
+// of the traversal loops.
+typedef __attribute__((address_space(5))) int32_t priv_i32;
 // The stack's fields are all scalars and pointers, so the compiler keeps the
 // struct in registers; the overflow array is a separate private array of the
 // kernel (dynamically indexed, hence in scratch), touched only past
@@ -107,7 +112,7 @@ typedef __attribute__((address_space(3))) float lds_f32;
 struct Stack {
     lds_i32* lds;      // base of this lane's column, stride = blockDim.x
     uint32_t stride;
-    int32_t* priv;     // this lane's overflow: RT_PRIV_STACK entries
+    priv_i32* priv;    // this lane's overflow: RT_PRIV_STACK entries
     int32_t sp;
     lds_f32* heap;     // k-NN photon gather: this lane's LDS column (stride) of RT_KNN_MAX floats
     bool has_heap;     // heap is set (an LDS pointer may be 0, so its value cannot say)
@@ -146,7 +151,18 @@ RT_D bool sphere_test(const float* p, const Ray& r, float t_min, float t_max, fl
     return true;
 }
 
-RT_D bool quad_test(const DQuad& q, const Ray& r, float t_min, float t_max, float& t_out, float& al, float& be) {
+RT_D bool quad_test(const DQuad& qm, const Ray& r, float t_min, float t_max, float& t_out, float& al, float& be) {
+    // the 80-B record as five 16-B loads in flight together (quad arrays are
+    // 16-B aligned: device allocations are, and the record is 5 x 16 B)
+    const float4* p4 = reinterpret_cast<const float4*>(&qm);
+    float4 x0 = p4[0], x1 = p4[1], x2 = p4[2], x3 = p4[3], x4 = p4[4];
+    __builtin_amdgcn_sched_barrier(0);
+    DQuad q;
+    q.q[0] = x0.x; q.q[1] = x0.y; q.q[2] = x0.z; q.u[0] = x0.w;
+    q.u[1] = x1.x; q.u[2] = x1.y; q.v[0] = x1.z; q.v[1] = x1.w;
+    q.v[2] = x2.x; q.n[0] = x2.y; q.n[1] = x2.z; q.n[2] = x2.w;
+    q.w[0] = x3.x; q.w[1] = x3.y; q.w[2] = x3.z; q.d = x3.w;
+    q.area = x4.x;
     V3 n = mk(q.n[0], q.n[1], q.n[2]);
     float denom = dot(n, r.d);
     if (!(fabsf(denom) >= 1e-8f)) return false;
@@ -165,6 +181,7 @@ RT_D bool quad_test(const DQuad& q, const Ray& r, float t_min, float t_max, floa
 // Moller-Trumbore (src/triangle.rs:75-126) on precomputed v0, e1, e2.
 RT_D bool tri_test(const float4* tris, uint32_t i, const Ray& r, float t_min, float t_max, float& t_out, float& b1o, float& b2o) {
     float4 A = tris[3 * i], B = tris[3 * i + 1], Cc = tris[3 * i + 2];
+    __builtin_amdgcn_sched_barrier(0);      // the record's three loads in flight together
     V3 v0 = mk(A.x, A.y, A.z), e1 = mk(B.x, B.y, B.z), e2 = mk(Cc.x, Cc.y, Cc.z);
     V3 h = cross(r.d, e2);
     float a = dot(e1, h);
@@ -570,6 +587,10 @@ struct HitId {
 
 // Closest hit inside a nested tree (mesh or box); updates closest / prim.
 RT_D void nested_closest(const DScene& S, const DObject& ob, const Ray& r, float t_min, float& closest, int32_t& prim, bool& any, Stack& st) {
+    // the object's fields read once, before the loop (a load inside the leaf
+    // branch cannot be hoisted by the compiler: it is conditional)
+    const bool mesh = ob.shape == SH_MESH;
+    const uint32_t first = ob.data_first;
     int32_t base = st.sp;
     st_push(st, (int32_t)ob.root);
     while (st.sp > base) {
@@ -578,8 +599,8 @@ RT_D void nested_closest(const DScene& S, const DObject& ob, const Ray& r, float
             uint32_t k = (uint32_t)(-e - 1);
             float t, a, b;
             st.n_prims++;
-            bool h = ob.shape == SH_MESH ? tri_test(S.tris, ob.data_first + k, r, t_min, closest, t, a, b)
-                                          : quad_test(S.quads[ob.data_first + k], r, t_min, closest, t, a, b);
+            bool h = mesh ? tri_test(S.tris, first + k, r, t_min, closest, t, a, b)
+                          : quad_test(S.quads[first + k], r, t_min, closest, t, a, b);
             if (h) { closest = t; prim = (int32_t)k; any = true; }
             continue;
         }
@@ -652,6 +673,8 @@ RT_D HitId trace_closest(const DScene& S, const Ray& r, float t_min, float t_max
 
 // ---- occlusion / transmittance ---------------------------------------------------------
 RT_D bool nested_any(const DScene& S, const DObject& ob, const Ray& r, float t_min, float t_max, Stack& st) {
+    const bool mesh = ob.shape == SH_MESH;
+    const uint32_t first = ob.data_first;
     int32_t base = st.sp;
     st_push(st, (int32_t)ob.root);
     while (st.sp > base) {
@@ -660,8 +683,8 @@ RT_D bool nested_any(const DScene& S, const DObject& ob, const Ray& r, float t_m
             uint32_t k = (uint32_t)(-e - 1);
             float t, a, b;
             st.n_prims++;
-            bool h = ob.shape == SH_MESH ? tri_test(S.tris, ob.data_first + k, r, t_min, t_max, t, a, b)
-                                          : quad_test(S.quads[ob.data_first + k], r, t_min, t_max, t, a, b);
+            bool h = mesh ? tri_test(S.tris, first + k, r, t_min, t_max, t, a, b)
+                          : quad_test(S.quads[first + k], r, t_min, t_max, t, a, b);
             if (h) { st.sp = base; return true; }
             continue;
         }

@@ -34,7 +34,7 @@ struct LdsHeap {
 #define RT_STACK_BS(st, BS) int32_t st##_priv[RT_PRIV_STACK]; Stack st; init_stack_bs<BS>(st, st##_priv)
 
 __device__ __forceinline__ void init_stack(Stack& st, int32_t* priv) {
-    st.lds = LdsStack::column(); st.stride = BLOCK; st.priv = priv; st.sp = 0; st.heap = nullptr; st.has_heap = false;
+    st.lds = LdsStack::column(); st.stride = BLOCK; st.priv = (priv_i32*)priv; st.sp = 0; st.heap = nullptr; st.has_heap = false;
     st.n_nodes = 0; st.n_prims = 0; st.n_rays = 0;
     st.s_nodes = 0; st.s_prims = 0; st.s_rays = 0;
     st.n_pbytes = 0;
@@ -50,7 +50,7 @@ struct LdsStackT {
 };
 template <int BS>
 __device__ __forceinline__ void init_stack_bs(Stack& st, int32_t* priv) {
-    st.lds = LdsStackT<BS>::column(); st.stride = BS; st.priv = priv; st.sp = 0; st.heap = nullptr; st.has_heap = false;
+    st.lds = LdsStackT<BS>::column(); st.stride = BS; st.priv = (priv_i32*)priv; st.sp = 0; st.heap = nullptr; st.has_heap = false;
     st.n_nodes = 0; st.n_prims = 0; st.n_rays = 0;
     st.s_nodes = 0; st.s_prims = 0; st.s_rays = 0;
     st.n_pbytes = 0;
