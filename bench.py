@@ -280,7 +280,7 @@ def measure(name, args, with_cpu):
     r.close()
     traffic, traffic_note = None, None
     if head == "wavefront":
-        traffic, traffic_note = measured_traffic(scene.name, tuned)
+        traffic, traffic_note = measured_traffic(scene.name, tuned, rh["bytes_per_launch"])
     el = res[head][0]
     return {
         "value": rates[head], "ms_per_step": el * 1e3 / args.steps,
@@ -730,10 +730,14 @@ def measure_rmse(args, with_cpu=True):
             "cpu_baseline": None}
 
 
-def measured_traffic(workload, opts):
+def measured_traffic(workload, opts, bytes_per_launch):
     """HBM bytes per k_wf_extend dispatch from the committed rocprofv3 PMC
-    summary (FETCH_SIZE + WRITE_SIZE, copy-calibrated) when it was measured
-    with the same launch options; otherwise None and the reason."""
+    summary (FETCH_SIZE + WRITE_SIZE, copy-calibrated) of the same kernel
+    variant (occupancy target, block size, material sort) on this workload.
+    A dispatch's DRAM bytes scale with the rays it traces, so when the
+    record was measured at another queue capacity its measured ratio of DRAM
+    bytes to algorithmic bytes is applied to this launch's algorithmic bytes
+    (the note says so).  Otherwise None and the reason."""
     path = os.path.join(ROOT, "profiles", "traffic.json")
     try:
         with open(path) as f:
@@ -742,12 +746,17 @@ def measured_traffic(workload, opts):
         return None, "no PMC summary"
     if not rec:
         return None, "no PMC summary for this workload"
-    # one record per set of launch options that changes what a dispatch reads
-    # or writes (graph replay and chunk streams change neither)
+    # graph replay and chunk streams change neither what a dispatch reads nor writes
     for r in rec if isinstance(rec, list) else [rec]:
-        if all(opts.get(k) == v for k, v in r["wavefront_opts"].items() if k not in ("use_graph", "streams")):
-            return r["traffic_bytes_per_dispatch"], r["source"]
-    return None, f"no PMC summary measured with {opts}"
+        o = r["wavefront_opts"]
+        if all(opts.get(k, d) == o.get(k, d) for k, d in (("occupancy", 0), ("block", 128), ("sort", 1))):
+            if all(opts.get(k) == o.get(k) for k in ("capacity", "grid")):
+                return r["traffic_bytes_per_dispatch"], r["source"]
+            ratio = r["traffic_bytes_per_dispatch"] / r["algorithmic_bytes_per_dispatch"]
+            return ratio * bytes_per_launch, (f"{r['source']} (measured at capacity {o.get('capacity')}, grid "
+                                              f"{o.get('grid')}: DRAM / algorithmic bytes {ratio:.3f}, applied to "
+                                              f"this launch's algorithmic bytes)")
+    return None, f"no PMC summary measured for the kernel variant of {opts}"
 
 
 def measure_output(args, with_cpu=True):

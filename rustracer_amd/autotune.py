@@ -12,15 +12,21 @@ each candidate with HIP events on the library's stream.  The search is cheap
 """
 import time
 
-CAPACITIES = [1 << 20, 1 << 21, 1 << 22, 1 << 23]
-GRIDS = [1024, 2048, 4096, 8192]
+# up to 32 M paths per chunk (about 13 GB of queues per chunk stream): on
+# 288 GB the larger chunks cost nothing and leave fewer latency-bound tails
+CAPACITIES = [1 << 20, 1 << 21, 1 << 22, 1 << 23, 1 << 24, 1 << 25]
+GRIDS = [1024, 2048, 4096, 8192, 16384]
 BLOCKS = [64, 128, 256]
-KNN_WAVE_MAX = [16384, 65536, 262144]
+KNN_WAVE_MAX = [16384, 65536, 262144, 1 << 20]
 
 
 def _probe(r, run, opts, repeats=2):
-    run(opts, 99)                       # warm / allocate
     r.set_timing(True)
+    run(opts, 99)                       # warm / allocate
+    warm_ms, _ = r.timing()
+    if warm_ms > 500.0:                 # long probe frames: one timed run is enough
+        repeats = 1
+    r.set_timing(True)                  # (re-enabling resets the totals)
     for k in range(repeats):
         run(opts, 100 + k)
     ms, _ = r.timing()
@@ -33,9 +39,9 @@ def tune_wavefront(r, out, cam, W, H, spp, depth, photons=False, probe_spp=None,
     info) where info["table"] lists every probe as (opts, ms per probe frame)."""
     npix = W * H
     if probe_spp is None:
-        # enough paths for four chunks of the largest capacity (so one to four
-        # chunk streams differ in the probe), at most the real spp
-        probe_spp = max(1, min(spp, (4 * CAPACITIES[-1] + npix - 1) // npix))
+        # enough paths for two chunks of the largest capacity (eight of 8 M, so
+        # one to four chunk streams differ in the probe), at most the real spp
+        probe_spp = max(1, min(spp, (2 * CAPACITIES[-1] + npix - 1) // npix))
 
     def run(o, seed):
         r.render_wavefront(out, cam, W, H, probe_spp, max_depth=depth, photon_map=photons, seed=seed,

@@ -139,6 +139,17 @@ struct WfQueues {
     uint32_t h4 = 0;
 };
 static_assert(std::is_trivially_copyable<WfQueues>::value, "kernel argument");
+// A default WfQueues has every pointer null and every count zero, checked at
+// compile time in every unit that launches the stages.
+constexpr WfQueues kWfQueuesDefault{};
+static_assert(kWfQueuesDefault.paths == nullptr && kWfQueuesDefault.active == nullptr &&
+              kWfQueuesDefault.next == nullptr && kWfQueuesDefault.buckets == nullptr &&
+              kWfQueuesDefault.shadow == nullptr && kWfQueuesDefault.counters == nullptr &&
+              kWfQueuesDefault.fb == nullptr && kWfQueuesDefault.lsamp == nullptr &&
+              kWfQueuesDefault.dense == nullptr && kWfQueuesDefault.stop == nullptr &&
+              kWfQueuesDefault.conv == nullptr && kWfQueuesDefault.spec == nullptr &&
+              kWfQueuesDefault.cap == 0 && kWfQueuesDefault.h4 == 0,
+              "every WfQueues field has a null / zero default");
 
 __device__ __forceinline__ uint32_t lane_id() { return __lane_id(); }
 
