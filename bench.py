@@ -876,18 +876,40 @@ def cpu_baseline(scene, cam, cfg, seconds, photons=False):
         orc.photon_build(pp, nthreads=cores)
     W, H, spp = cfg["width"], cfg["height"], cfg["spp"]
     strata = int(spp ** 0.5)
+
+    def params(k):
+        return Renderer.params(W, H, sample_idx=k, strata=strata, max_depth=cfg["max_depth"], use_photons=photons,
+                               wavelengths=cfg.get("wavelengths", 0))
+    orc.render_sample(cam, params(spp - 1), mode=0, nthreads=cores)      # warm-up pass (untimed)
     done = 0
     t0 = time.perf_counter()
     while True:
-        p = Renderer.params(W, H, sample_idx=done, strata=strata, max_depth=cfg["max_depth"], use_photons=photons,
-                            wavelengths=cfg.get("wavelengths", 0))
-        orc.render_sample(cam, p, mode=0, nthreads=cores)
+        orc.render_sample(cam, params(done), mode=0, nthreads=cores)
         done += 1
         if time.perf_counter() - t0 >= seconds or done >= 100000:
             break
     el = time.perf_counter() - t0
+    # one thread, one pass: the scalar rate beside the all-core one
+    t1 = time.perf_counter()
+    orc.render_sample(cam, params(0), mode=0, nthreads=1)
+    one = W * H / (time.perf_counter() - t1)
     return {"value": W * H * done / el, "unit": "samples/s", "cores": cores, "kind": "port",
-            "sample": f"{done} full passes of {W}x{H} at 1 sample/pixel (of {spp}), {el:.1f}s"}
+            "sample": f"{done} full passes of {W}x{H} at 1 sample/pixel (of {spp}), {el:.1f}s, after one warm-up pass",
+            "one_thread": one, "host": host_cpu()}
+
+
+def host_cpu():
+    """The host CPU model and logical CPU count, for the CPU baseline."""
+    model = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {"model": model, "logical_cpus": os.cpu_count()}
 
 
 def main():
