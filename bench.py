@@ -466,8 +466,9 @@ def measure_adjoint(args, with_cpu):
 
     # chunk capacity x streams: a few probes of one evaluation each
     probes = []
-    for cap, ns in ((1 << 23, 1), (1 << 22, 2), (1 << 22, 3), (1 << 21, 3), (1 << 21, 4)):
-        o = dict(opts, capacity=cap, streams=ns)
+    for cap, ns, grid in ((1 << 24, 1, 16384), (1 << 23, 2, 16384), (1 << 23, 1, 8192), (1 << 22, 2, 8192),
+                          (1 << 22, 3, 8192), (1 << 21, 3, 8192), (1 << 21, 4, 8192)):
+        o = dict(opts, capacity=cap, streams=ns, grid=grid)
         step_with(o, 500)
         torch.cuda.synchronize()
         t0 = time.perf_counter()
