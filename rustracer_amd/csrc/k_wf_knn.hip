@@ -36,7 +36,7 @@ __global__ __launch_bounds__(BLOCK) void k_wf_knn(DScene S0, DParams P0, WfQueue
             while (i >= base + Q.counters[WF_CNT_BUCKET + b]) { base += Q.counters[WF_CNT_BUCKET + b]; b++; }
             pid = Q.buckets[(size_t)b * Q.cap + (i - base)];
             WfPath& w = Q.paths[pid];
-            const DScene& S = BATCH ? B.scenes[w.pad0] : S0;
+            const DScene S = BATCH ? view_scene(B, w.pad0) : S0;
             bool photons = BATCH ? B.views[w.pad0].P.use_photons != 0 : P0.use_photons != 0;
             if (photons && knn_gathers(S, w)) {
                 V3 pos, nrm, e;
@@ -169,7 +169,7 @@ __global__ __launch_bounds__(BLOCK) void k_wf_knn_dense(DScene S0, WfQueues Q, W
     uint32_t stride = gridDim.x * BLOCK;
     for (uint32_t i = blockIdx.x * BLOCK + threadIdx.x; i < total; i += stride) {
         WfPath& w = Q.paths[Q.dense[i]];
-        const DScene& S = BATCH ? B.scenes[w.pad0] : S0;
+        const DScene S = BATCH ? view_scene(B, w.pad0) : S0;
         V3 pos, nrm;
         knn_query<BATCH>(S, w, pos, nrm);
         knn_store(w, photon_knn_dense(S, pos, nrm, heap, BLOCK));
@@ -184,7 +184,7 @@ __global__ __launch_bounds__(BLOCK) void k_wf_knn_wave(DScene S0, WfQueues Q, Wf
     const uint32_t waves = gridDim.x * WPB;
     for (uint32_t q = blockIdx.x * WPB + threadIdx.x / 64; q < total; q += waves) {
         WfPath& w = Q.paths[Q.dense[q]];
-        const DScene& S = BATCH ? B.scenes[w.pad0] : S0;
+        const DScene S = BATCH ? view_scene(B, w.pad0) : S0;
         V3 pos, nrm;
         knn_query<BATCH>(S, w, pos, nrm);
         V3 e = knn_wave(S, pos, nrm, lane);

@@ -75,7 +75,7 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(OCC > 0 ? OC
         if (valid) {
             pid = Q.active[i];
             WfPath& w = Q.paths[pid];
-            if (BATCH) key = extend_one(B.scenes[w.pad0], w, Q.sort, st);
+            if (BATCH) key = extend_one(view_scene(B, w.pad0), w, Q.sort, st);
             else key = extend_one(S, w, Q.sort, st);
         }
         uint32_t slot = wave_append(Q.counters + WF_CNT_BUCKET, key, valid);
@@ -92,7 +92,7 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(OCC > 0 ? OC
     for (uint32_t i = blockIdx.x * BS + threadIdx.x; i < count; i += stride) {
         WfShadow q = Q.shadow[i];
         Ray r = make_ray(mk(q.o_tmax.x, q.o_tmax.y, q.o_tmax.z), mk(q.d_time.x, q.d_time.y, q.d_time.z), q.d_time.w);
-        float tr = BATCH ? trace_transmittance(B.scenes[q.meta.x], r, 0.001f, q.o_tmax.w, st, false)
+        float tr = BATCH ? trace_transmittance(view_scene(B, q.meta.x), r, 0.001f, q.o_tmax.w, st, false)
                          : trace_transmittance(S, r, 0.001f, q.o_tmax.w, st, false);
         if (tr > 0.0f) {
             float* dst = Q.lsamp ? Q.lsamp + 3 * (size_t)q.meta.y : Q.fb + 3 * (size_t)__float_as_uint(q.c_pixel.w);

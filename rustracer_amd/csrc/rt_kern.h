@@ -216,6 +216,32 @@ struct WfBatch {
     uint32_t nviews;
 };
 
+// A view's scene record with every pointer re-typed as global memory.  The
+// records sit in an array in device memory, so a pointer loaded from one is
+// generic and every access through it would be a flat access (which also
+// waits on the LDS counter); rebuilt from its integer value as an
+// address-space-1 pointer, the accesses compile to global loads.
+template <class T>
+__device__ __forceinline__ const T* as_global(const T* p) {
+    typedef const __attribute__((address_space(1))) T gT;
+    return (const T*)(gT*)(uintptr_t)p;
+}
+__device__ __forceinline__ DScene view_scene(const WfBatch& B, uint32_t view) {
+    DScene S = B.scenes[view];
+    S.nodes = as_global(S.nodes); S.objs = as_global(S.objs); S.lights = as_global(S.lights);
+    S.xforms = as_global(S.xforms); S.tris = as_global(S.tris); S.tri_tag = as_global(S.tri_tag);
+    S.tri_n = as_global(S.tri_n); S.tri_t = as_global(S.tri_t); S.tri_uv = as_global(S.tri_uv);
+    S.quads = as_global(S.quads); S.planes = as_global(S.planes); S.mats = as_global(S.mats);
+    S.media = as_global(S.media); S.grids = as_global(S.grids); S.tex_pix = as_global(S.tex_pix);
+    S.srgb_lut = as_global(S.srgb_lut); S.nacre = as_global(S.nacre);
+    S.env_aprob = as_global(S.env_aprob); S.env_alias = as_global(S.env_alias); S.env_rgb = as_global(S.env_rgb);
+    S.env_marg = as_global(S.env_marg); S.env_cond = as_global(S.env_cond); S.elut = as_global(S.elut);
+    S.ph_pos = as_global(S.ph_pos); S.ph_pow = as_global(S.ph_pow); S.ph_nrm = as_global(S.ph_nrm);
+    S.pk_rec = as_global(S.pk_rec); S.pk_pow = as_global(S.pk_pow); S.pk_split = as_global(S.pk_split);
+    S.pk_leaf = as_global(S.pk_leaf);
+    return S;
+}
+
 // Path j of a sample pass covers pixel tile_order_pixel(j): bands of 8 rows,
 // each walked in 8-column groups, row-major inside a group, so a wave's 64
 // camera rays leave an 8 x 8 block of the image (coherent traversal and
