@@ -272,6 +272,12 @@ RT_D uint32_t node_test(const float4* node, const Ray& r, float t_min, float t_m
 #endif
 #pragma unroll
     for (int c = 0; c < 8; c++) { t0[c] = t_min; t1[c] = t_max; }
+    // The slab bounds are NaN only when a direction component's inverse is
+    // infinite (0 x inf); with a finite inverse and t_min > 0 the hardware
+    // min / max give the compare-select results bit for bit (t_near >= t_min
+    // > 0, so no signed zero reaches t_near or the mask), in half the
+    // instructions.  Otherwise the reference's ordered compare-selects.
+    const bool fast = t_min > 0.0f && fabsf(r.inv.x) < FINF && fabsf(r.inv.y) < FINF && fabsf(r.inv.z) < FINF;
 #pragma unroll
     for (int a = 0; a < 3; a++) {
         float4 l0 = RT_NODE_Q(2 * a), l1 = RT_NODE_Q(2 * a + 1), h0 = RT_NODE_Q(6 + 2 * a), h1 = RT_NODE_Q(7 + 2 * a);
@@ -279,12 +285,22 @@ RT_D uint32_t node_test(const float4* node, const Ray& r, float t_min, float t_m
         float hi[8] = {h0.x, h0.y, h0.z, h0.w, h1.x, h1.y, h1.z, h1.w};
         float o = a == 0 ? r.o.x : (a == 1 ? r.o.y : r.o.z);
         float id = a == 0 ? r.inv.x : (a == 1 ? r.inv.y : r.inv.z);
+        if (fast) {
 #pragma unroll
-        for (int c = 0; c < 8; c++) {
-            float da = (lo[c] - o) * id;
-            float db = (hi[c] - o) * id;
-            t0[c] = mxps(t0[c], mnps(da, db));
-            t1[c] = mnps(t1[c], mxps(da, db));
+            for (int c = 0; c < 8; c++) {
+                float da = (lo[c] - o) * id;
+                float db = (hi[c] - o) * id;
+                t0[c] = fmaxf(t0[c], fminf(da, db));
+                t1[c] = fminf(t1[c], fmaxf(da, db));
+            }
+        } else {
+#pragma unroll
+            for (int c = 0; c < 8; c++) {
+                float da = (lo[c] - o) * id;
+                float db = (hi[c] - o) * id;
+                t0[c] = mxps(t0[c], mnps(da, db));
+                t1[c] = mnps(t1[c], mxps(da, db));
+            }
         }
     }
     float4 c0 = RT_NODE_Q(12), c1 = RT_NODE_Q(13);
