@@ -252,15 +252,19 @@ RT_D Ray to_object(const DScene& S, const DObject& ob, const Ray& r) {
 RT_D float mnps(float a, float b) { return a < b ? a : b; }
 RT_D float mxps(float a, float b) { return a > b ? a : b; }
 
-// Returns the hit mask; t_near[c] for each child.  Children read as ints.
-// The node's 14 float4 loads are all issued before the slab tests (a
-// scheduling barrier keeps the compiler from spreading them over three
-// dependent rounds to save registers), so a node costs one memory round trip.
+// f32::total_cmp key
+RT_D int32_t tkey(float f) { int32_t i = __float_as_int(f); return i ^ (int32_t)(((uint32_t)(i >> 31)) >> 1); }
+
+// Returns the hit mask; key[c] = the total_cmp key of child c's t_near.
+// Children read as ints.  The node's 14 float4 loads are all issued before
+// the slab tests (a scheduling barrier keeps the compiler from spreading them
+// over three dependent rounds to save registers), so a node costs one memory
+// round trip.
 #ifndef RT_NODE_PRELOAD
 #define RT_NODE_PRELOAD 1
 #endif
-RT_D uint32_t node_test(const float4* node, const Ray& r, float t_min, float t_max, float tn[8], int32_t ch[8]) {
-    float t0[8], t1[8];
+
+RT_D uint32_t node_test(const float4* node, const Ray& r, float t_min, float t_max, int32_t key[8], int32_t ch[8]) {
 #if RT_NODE_PRELOAD
     float4 q[14];
 #pragma unroll
@@ -270,78 +274,87 @@ RT_D uint32_t node_test(const float4* node, const Ray& r, float t_min, float t_m
 #else
 #define RT_NODE_Q(i) node[i]
 #endif
-#pragma unroll
-    for (int c = 0; c < 8; c++) { t0[c] = t_min; t1[c] = t_max; }
-    // The slab bounds are NaN only when a direction component's inverse is
-    // infinite (0 x inf); with a finite inverse and t_min > 0 the hardware
-    // min / max give the compare-select results bit for bit (t_near >= t_min
-    // > 0, so no signed zero reaches t_near or the mask), in half the
-    // instructions.  Otherwise the reference's ordered compare-selects.
-    const bool fast = t_min > 0.0f && fabsf(r.inv.x) < FINF && fabsf(r.inv.y) < FINF && fabsf(r.inv.z) < FINF;
+    float lo[3][8], hi[3][8];
 #pragma unroll
     for (int a = 0; a < 3; a++) {
         float4 l0 = RT_NODE_Q(2 * a), l1 = RT_NODE_Q(2 * a + 1), h0 = RT_NODE_Q(6 + 2 * a), h1 = RT_NODE_Q(7 + 2 * a);
-        float lo[8] = {l0.x, l0.y, l0.z, l0.w, l1.x, l1.y, l1.z, l1.w};
-        float hi[8] = {h0.x, h0.y, h0.z, h0.w, h1.x, h1.y, h1.z, h1.w};
-        float o = a == 0 ? r.o.x : (a == 1 ? r.o.y : r.o.z);
-        float id = a == 0 ? r.inv.x : (a == 1 ? r.inv.y : r.inv.z);
-        if (fast) {
-#pragma unroll
-            for (int c = 0; c < 8; c++) {
-                float da = (lo[c] - o) * id;
-                float db = (hi[c] - o) * id;
-                t0[c] = fmaxf(t0[c], fminf(da, db));
-                t1[c] = fminf(t1[c], fmaxf(da, db));
-            }
-        } else {
-#pragma unroll
-            for (int c = 0; c < 8; c++) {
-                float da = (lo[c] - o) * id;
-                float db = (hi[c] - o) * id;
-                t0[c] = mxps(t0[c], mnps(da, db));
-                t1[c] = mnps(t1[c], mxps(da, db));
-            }
-        }
+        lo[a][0] = l0.x; lo[a][1] = l0.y; lo[a][2] = l0.z; lo[a][3] = l0.w;
+        lo[a][4] = l1.x; lo[a][5] = l1.y; lo[a][6] = l1.z; lo[a][7] = l1.w;
+        hi[a][0] = h0.x; hi[a][1] = h0.y; hi[a][2] = h0.z; hi[a][3] = h0.w;
+        hi[a][4] = h1.x; hi[a][5] = h1.y; hi[a][6] = h1.z; hi[a][7] = h1.w;
     }
     float4 c0 = RT_NODE_Q(12), c1 = RT_NODE_Q(13);
 #undef RT_NODE_Q
     ch[0] = __float_as_int(c0.x); ch[1] = __float_as_int(c0.y); ch[2] = __float_as_int(c0.z); ch[3] = __float_as_int(c0.w);
     ch[4] = __float_as_int(c1.x); ch[5] = __float_as_int(c1.y); ch[6] = __float_as_int(c1.z); ch[7] = __float_as_int(c1.w);
+    const float o[3] = {r.o.x, r.o.y, r.o.z}, id[3] = {r.inv.x, r.inv.y, r.inv.z};
     uint32_t mask = 0;
+    // The slab bounds are NaN only when a direction component's inverse is
+    // infinite (0 x inf).  With all three inverses finite and t_min > 0 the
+    // hardware min / max (three-operand forms over the axes) give the ordered
+    // compare-selects' results bit for bit: t_near >= t_min > 0, so no signed
+    // zero reaches t_near or the mask, and t_near's bits are its total_cmp key.
+    // Otherwise the reference's compare-selects, axis by axis.
+    const bool fast = t_min > 0.0f && fabsf(r.inv.x) < FINF && fabsf(r.inv.y) < FINF && fabsf(r.inv.z) < FINF;
+    if (fast) {
 #pragma unroll
-    for (int c = 0; c < 8; c++) {
-        tn[c] = t0[c];
-        if (t0[c] <= t1[c] && ch[c] != INT32_MIN) mask |= 1u << c;
+        for (int c = 0; c < 8; c++) {
+            float d[3][2];
+#pragma unroll
+            for (int a = 0; a < 3; a++) { d[a][0] = (lo[a][c] - o[a]) * id[a]; d[a][1] = (hi[a][c] - o[a]) * id[a]; }
+            float tn = fmaxf(fmaxf(fmaxf(t_min, fminf(d[0][0], d[0][1])), fminf(d[1][0], d[1][1])), fminf(d[2][0], d[2][1]));
+            float tf = fminf(fminf(fminf(t_max, fmaxf(d[0][0], d[0][1])), fmaxf(d[1][0], d[1][1])), fmaxf(d[2][0], d[2][1]));
+            key[c] = __float_as_int(tn);
+            if (tn <= tf && ch[c] != INT32_MIN) mask |= 1u << c;
+        }
+    } else {
+#pragma unroll
+        for (int c = 0; c < 8; c++) {
+            float t0 = t_min, t1 = t_max;
+#pragma unroll
+            for (int a = 0; a < 3; a++) {
+                float da = (lo[a][c] - o[a]) * id[a];
+                float db = (hi[a][c] - o[a]) * id[a];
+                t0 = mxps(t0, mnps(da, db));
+                t1 = mnps(t1, mxps(da, db));
+            }
+            key[c] = tkey(t0);
+            if (t0 <= t1 && ch[c] != INT32_MIN) mask |= 1u << c;
+        }
     }
     return mask;
 }
 
-// f32::total_cmp key
-RT_D int32_t tkey(float f) { int32_t i = __float_as_int(f); return i ^ (int32_t)(((uint32_t)(i >> 31)) >> 1); }
-
 // Push hit children far-to-near (stable for equal keys) so the nearest pops
-// first.  Each child's slot is its rank in that order (children with a
-// larger key, or an equal key and a lower index, go below it), so the sort
-// needs no dynamically indexed private array (no scratch traffic).
-RT_D void push_sorted(Stack& st, uint32_t mask, const float tn[8], const int32_t ch[8]) {
-    int32_t key[8];
+// first.  Each child's slot is its rank in that order: the hit children with
+// a larger key, or an equal key and a lower index, go below it.  One integer
+// comparison per pair of slots decides both directions (for c < d, c goes
+// below d exactly when key[c] >= key[d]), and no private array is indexed
+// dynamically (no scratch traffic).
+RT_D void push_sorted(Stack& st, uint32_t mask, const int32_t key[8], const int32_t ch[8]) {
+    int32_t rank[8];
 #pragma unroll
-    for (int c = 0; c < 8; c++) key[c] = tkey(tn[c]);
+    for (int c = 0; c < 8; c++) rank[c] = 0;
+#pragma unroll
+    for (int c = 0; c < 8; c++) {
+        const bool hc = (mask >> c) & 1u;
+#pragma unroll
+        for (int d = c + 1; d < 8; d++) {
+            const bool hd = (mask >> d) & 1u;
+            const bool c_below = key[c] >= key[d];
+            rank[d] += (hc && c_below) ? 1 : 0;
+            rank[c] += (hd && !c_below) ? 1 : 0;
+        }
+    }
+    const int32_t sp = st.sp;
 #pragma unroll
     for (int c = 0; c < 8; c++) {
         if (!(mask & (1u << c))) continue;
-        int32_t rank = 0;
-#pragma unroll
-        for (int d = 0; d < 8; d++) {
-            if (d == c) continue;
-            bool before = (mask & (1u << d)) && (key[d] > key[c] || (key[d] == key[c] && d < c));
-            rank += before ? 1 : 0;
-        }
-        int32_t at = st.sp + rank;
+        int32_t at = sp + rank[c];
         if (at < RT_LDS_STACK) st.lds[at * st.stride] = ch[c];
         else st.priv[at - RT_LDS_STACK] = ch[c];
     }
-    st.sp += __popc(mask);
+    st.sp = sp + __popc(mask);
 }
 
 // ---- cylinder / cone / disk / infinite plane ----------------------------------------------
@@ -620,10 +633,10 @@ RT_D void nested_closest(const DScene& S, const DObject& ob, const Ray& r, float
             if (h) { closest = t; prim = (int32_t)k; any = true; }
             continue;
         }
-        float tn[8]; int32_t ch[8];
-        uint32_t mask = node_test(S.nodes + (size_t)e * 16, r, t_min, closest, tn, ch);
+        int32_t key[8]; int32_t ch[8];
+        uint32_t mask = node_test(S.nodes + (size_t)e * 16, r, t_min, closest, key, ch);
         st.n_nodes++;
-        if (mask) push_sorted(st, mask, tn, ch);
+        if (mask) push_sorted(st, mask, key, ch);
     }
 }
 
@@ -679,10 +692,10 @@ RT_D HitId trace_closest(const DScene& S, const Ray& r, float t_min, float t_max
             object_closest(S, S.objs[slot], slot, r, t_min, closest, best, st);
             continue;
         }
-        float tn[8]; int32_t ch[8];
-        uint32_t mask = node_test(S.nodes + (size_t)e * 16, r, t_min, closest, tn, ch);
+        int32_t key[8]; int32_t ch[8];
+        uint32_t mask = node_test(S.nodes + (size_t)e * 16, r, t_min, closest, key, ch);
         st.n_nodes++;
-        if (mask) push_sorted(st, mask, tn, ch);
+        if (mask) push_sorted(st, mask, key, ch);
     }
     return best;
 }
@@ -704,8 +717,8 @@ RT_D bool nested_any(const DScene& S, const DObject& ob, const Ray& r, float t_m
             if (h) { st.sp = base; return true; }
             continue;
         }
-        float tn[8]; int32_t ch[8];
-        uint32_t mask = node_test(S.nodes + (size_t)e * 16, r, t_min, t_max, tn, ch);
+        int32_t key[8]; int32_t ch[8];
+        uint32_t mask = node_test(S.nodes + (size_t)e * 16, r, t_min, t_max, key, ch);
         st.n_nodes++;
 #pragma unroll
         for (int c = 0; c < 8; c++) if (mask & (1u << c)) st_push(st, ch[c]);
@@ -760,8 +773,8 @@ RT_D float trace_transmittance_impl(const DScene& S, const Ray& r, float t_min, 
             if (tr <= 0.0f) return 0.0f;
             continue;
         }
-        float tn[8]; int32_t ch[8];
-        uint32_t mask = node_test(S.nodes + (size_t)e * 16, r, t_min, t_max, tn, ch);
+        int32_t key[8]; int32_t ch[8];
+        uint32_t mask = node_test(S.nodes + (size_t)e * 16, r, t_min, t_max, key, ch);
         st.n_nodes++;
 #pragma unroll
         for (int c = 0; c < 8; c++) if (mask & (1u << c)) st_push(st, ch[c]);
