@@ -113,16 +113,21 @@ struct Rng {
 };
 RT_HD Rng rng_seed(uint64_t state) {
     Rng r;
-    uint64_t sv[4];
-    for (int i = 0; i < 4; i++) {
-        state += 0x9e3779b97f4a7c15ULL;
-        uint64_t z = state;
-        z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ULL;
-        z = (z ^ (z >> 27)) * 0x94d049bb133111ebULL;
-        sv[i] = z ^ (z >> 31);
+    // an all-zero state is replaced by seed 0's (a loop, not a recursive
+    // call, so the function inlines into every kernel)
+    for (int pass = 0; pass < 2; pass++) {
+        uint64_t sv[4];
+        for (int i = 0; i < 4; i++) {
+            state += 0x9e3779b97f4a7c15ULL;
+            uint64_t z = state;
+            z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ULL;
+            z = (z ^ (z >> 27)) * 0x94d049bb133111ebULL;
+            sv[i] = z ^ (z >> 31);
+        }
+        r.s0 = sv[0]; r.s1 = sv[1]; r.s2 = sv[2]; r.s3 = sv[3];
+        if ((r.s0 | r.s1 | r.s2 | r.s3) != 0) break;
+        state = 0;
     }
-    r.s0 = sv[0]; r.s1 = sv[1]; r.s2 = sv[2]; r.s3 = sv[3];
-    if ((r.s0 | r.s1 | r.s2 | r.s3) == 0) return rng_seed(0);
     return r;
 }
 RT_HD uint64_t rng_u64(Rng& r) {

@@ -1413,20 +1413,30 @@ RT_D float spec4_bb_temp(const DScene& S, const Rec& h, const Ray& ray) {
     return m.kind == M_BLACKBODY ? m.p[0] : 0.0f;
 }
 
+// Where a vertex's shadow requests go.  ShadowList keeps them for the caller,
+// sh[0..n) in the order they are filed; a hook sees each request with its
+// filing index.  (The wavefront's shade stage files them straight into its
+// shadow queue instead, so no request stays live past the point it is made.)
+struct ShadowList {
+    ShadowReq* sh;
+    int& n;
+    RT_D int index() const { return n; }
+    RT_D void file(const ShadowReq& q) { sh[n++] = q; }
+};
+
 // Shade the hit (or miss) `id` of ps.ray at bounce `depth`.  Adds direct
-// emission / background to `color`, appends up to three shadow requests and
-// advances the path.  Returns false when the path ends.  H4: the path carries
-// four hero wavelengths, their ratios in *s4.
-template <class DH, uint32_t MASK = RT_MASK_ALL, bool H4 = false>
-RT_D bool bounce_t(const DScene& S, PathState& ps, const HitId& id, int depth, float bg_scale, bool use_photons, Rng& rng,
-                   V3& color, ShadowReq* sh, int& nsh, Stack& st, DH& dh, Spec4* s4 = nullptr) {
+// emission / background to `color`, files up to three shadow requests with
+// `sink` and advances the path.  Returns false when the path ends.  H4: the
+// path carries four hero wavelengths, their ratios in *s4.
+template <class DH, uint32_t MASK = RT_MASK_ALL, bool H4 = false, class SINK>
+RT_D bool bounce_s(const DScene& S, PathState& ps, const HitId& id, int depth, float bg_scale, bool use_photons, Rng& rng,
+                   V3& color, SINK& sink, Stack& st, DH& dh, Spec4* s4 = nullptr) {
     Ray& ray = ps.ray;
     dh.segment(S, ray, id);
     const int ptint = ray.spectral ? ray.tint : 0;    // the tint the path carries into this vertex
     const bool lights_empty = S.num_lights == 0;
     const bool sun = physical_sun(S);
     const bool env = has_env(S);
-    nsh = 0;
     if (id.slot < 0) {
         float sp = sun_pdf_value(S, ray.d);
         float ep = env_pdf_value(S, ray.d);
@@ -1466,7 +1476,12 @@ RT_D bool bounce_t(const DScene& S, PathState& ps, const HitId& id, int depth, f
                     }
                 }
                 float md = p_sun * p_sun + pm * pm;
-                if (md > 0.0f) { sh[nsh].o = h.p; sh[nsh].d = ss; sh[nsh].t_max = FINF; sh[nsh].time = ray.time; sh[nsh].contrib = ps.thr * bc * sc * p_sun / md; dh.shadow(sh[nsh], nsh, S, h, ray, false); nsh++; }
+                if (md > 0.0f) {
+                    ShadowReq q;
+                    q.o = h.p; q.d = ss; q.t_max = FINF; q.time = ray.time; q.contrib = ps.thr * bc * sc * p_sun / md;
+                    dh.shadow(q, sink.index(), S, h, ray, false);
+                    sink.file(q);
+                }
             }
             float pmb = mat_spec_pdf<MASK>(S, h, ray, sr.ray.d);
             if (pmb > 0.0f) ps.prev_sun_w = (pmb * pmb) / fmx(pmb * pmb + p_sun * p_sun, 1e-8f);
@@ -1494,12 +1509,13 @@ RT_D bool bounce_t(const DScene& S, PathState& ps, const HitId& id, int depth, f
                 float ep = env_pdf_value(S, ldir);
                 float md = l_pdf * l_pdf + brdf * brdf + sp * sp + ep * ep;
                 if (md > 0.0f && brdf > 0.0f) {
-                    sh[nsh].o = h.p; sh[nsh].d = ldir; sh[nsh].t_max = lrec.t * (1.0f - 1e-4f); sh[nsh].time = ray.time;
+                    ShadowReq q;
+                    q.o = h.p; q.d = ldir; q.t_max = lrec.t * (1.0f - 1e-4f); q.time = ray.time;
                     V3 le = mat_emitted(S, lrec, ray);
                     if constexpr (H4) le *= spec4_weight(S, ray, *s4, ptint, nullptr, spec4_bb_temp(S, lrec, ray));
-                    sh[nsh].contrib = ps.thr * sr.att * brdf * l_pdf * le / md;
-                    dh.shadow(sh[nsh], nsh, S, h, ray, true);
-                    nsh++;
+                    q.contrib = ps.thr * sr.att * brdf * l_pdf * le / md;
+                    dh.shadow(q, sink.index(), S, h, ray, true);
+                    sink.file(q);
                 }
             }
         }
@@ -1512,10 +1528,11 @@ RT_D bool bounce_t(const DScene& S, PathState& ps, const HitId& id, int depth, f
                 V3 sc = bg_eval(S, ss);
                 float ap = lights_empty ? 0.0f : lights_pdf_value(S, h.p, ss, ray.time, st);
                 float md = spdf * spdf + brdf * brdf + ap * ap;
-                sh[nsh].o = h.p; sh[nsh].d = ss; sh[nsh].t_max = FINF; sh[nsh].time = ray.time;
-                sh[nsh].contrib = ps.thr * sr.att * brdf * spdf * sc * w4 / md;
-                dh.shadow(sh[nsh], nsh, S, h, ray, true);
-                nsh++;
+                ShadowReq q;
+                q.o = h.p; q.d = ss; q.t_max = FINF; q.time = ray.time;
+                q.contrib = ps.thr * sr.att * brdf * spdf * sc * w4 / md;
+                dh.shadow(q, sink.index(), S, h, ray, true);
+                sink.file(q);
             }
         }
         if (env) {
@@ -1527,10 +1544,11 @@ RT_D bool bounce_t(const DScene& S, PathState& ps, const HitId& id, int depth, f
                     float ap = lights_empty ? 0.0f : lights_pdf_value(S, h.p, ed, ray.time, st);
                     V3 ec = env_eval(S, ed);
                     float md = epdf * epdf + brdf * brdf + ap * ap;
-                    sh[nsh].o = h.p; sh[nsh].d = ed; sh[nsh].t_max = FINF; sh[nsh].time = ray.time;
-                    sh[nsh].contrib = ps.thr * sr.att * brdf * epdf * ec * w4 / md;
-                    dh.shadow(sh[nsh], nsh, S, h, ray, true);
-                    nsh++;
+                    ShadowReq q;
+                    q.o = h.p; q.d = ed; q.t_max = FINF; q.time = ray.time;
+                    q.contrib = ps.thr * sr.att * brdf * epdf * ec * w4 / md;
+                    dh.shadow(q, sink.index(), S, h, ray, true);
+                    sink.file(q);
                 }
             }
         }
@@ -1574,6 +1592,14 @@ struct NoDiff {
     RT_D void shadow(ShadowReq&, int, const DScene&, const Rec&, const Ray&, bool) {}
     RT_D void scatter(const DScene&, const Rec&, const Ray&, V3) {}
 };
+// bounce_s with the requests kept in sh[0..nsh).
+template <class DH, uint32_t MASK = RT_MASK_ALL, bool H4 = false>
+RT_D bool bounce_t(const DScene& S, PathState& ps, const HitId& id, int depth, float bg_scale, bool use_photons, Rng& rng,
+                   V3& color, ShadowReq* sh, int& nsh, Stack& st, DH& dh, Spec4* s4 = nullptr) {
+    nsh = 0;
+    ShadowList sink{sh, nsh};
+    return bounce_s<DH, MASK, H4>(S, ps, id, depth, bg_scale, use_photons, rng, color, sink, st, dh, s4);
+}
 template <uint32_t MASK = RT_MASK_ALL, bool H4 = false>
 RT_D bool bounce(const DScene& S, PathState& ps, const HitId& id, int depth, float bg_scale, bool use_photons, Rng& rng,
                  V3& color, ShadowReq* sh, int& nsh, Stack& st, Spec4* s4 = nullptr) {
