@@ -748,15 +748,18 @@ def measured_traffic(workload, opts, bytes_per_launch):
     if not rec:
         return None, "no PMC summary for this workload"
     # graph replay and chunk streams change neither what a dispatch reads nor writes
-    for r in rec if isinstance(rec, list) else [rec]:
+    same = [r for r in (rec if isinstance(rec, list) else [rec])
+            if all(opts.get(k, d) == r["wavefront_opts"].get(k, d) for k, d in (("occupancy", 0), ("block", 128), ("sort", 1)))]
+    for r in same:
+        if all(opts.get(k) == r["wavefront_opts"].get(k) for k in ("capacity", "grid")):
+            return r["traffic_bytes_per_dispatch"], r["source"]
+    if same:
+        r = same[0]
         o = r["wavefront_opts"]
-        if all(opts.get(k, d) == o.get(k, d) for k, d in (("occupancy", 0), ("block", 128), ("sort", 1))):
-            if all(opts.get(k) == o.get(k) for k in ("capacity", "grid")):
-                return r["traffic_bytes_per_dispatch"], r["source"]
-            ratio = r["traffic_bytes_per_dispatch"] / r["algorithmic_bytes_per_dispatch"]
-            return ratio * bytes_per_launch, (f"{r['source']} (measured at capacity {o.get('capacity')}, grid "
-                                              f"{o.get('grid')}: DRAM / algorithmic bytes {ratio:.3f}, applied to "
-                                              f"this launch's algorithmic bytes)")
+        ratio = r["traffic_bytes_per_dispatch"] / r["algorithmic_bytes_per_dispatch"]
+        return ratio * bytes_per_launch, (f"{r['source']} (measured at capacity {o.get('capacity')}, grid "
+                                          f"{o.get('grid')}: DRAM / algorithmic bytes {ratio:.3f}, applied to "
+                                          f"this launch's algorithmic bytes)")
     return None, f"no PMC summary measured for the kernel variant of {opts}"
 
 
