@@ -1226,18 +1226,25 @@ RT_D V3 photon_knn_irradiance(const DScene& S, V3 pos, V3 nrm, lds_f32* heap, ui
 }
 
 // Second pass of the wavefront's k-NN stage, for a query that the first pass
-// found to hold at least k photons within ph_r2: the heap search without the
-// sum of its first k photons (never used then), then the gather at the k-th
-// distance.  Equals photon_knn_irradiance; should the search hold fewer than
-// k after all, the gather within ph_r2 is that function's sum (same walk,
-// same order, the radius never shrinking).
-RT_D V3 photon_knn_dense(const DScene& S, V3 pos, V3 nrm, lds_f32* heap, uint32_t stride) {
+// found to hold at least k photons within ph_r2, in two halves (two kernels:
+// the heap search needs the LDS heap, the gather does not).  The heap search
+// without the sum of its first k photons (never used then) gives the gather
+// radius: the k-th distance, or ph_r2 should the search hold fewer than k
+// after all or the k-th distance be 0 ...
+RT_D float photon_knn_radius(const DScene& S, V3 pos, V3 nrm, lds_f32* heap, uint32_t stride) {
     uint32_t cnt;
     float unused;
     pk_walk<true, false>(S, pos, nrm, S.ph_r2, heap, stride, cnt, unused);
-    if (cnt < S.ph_knn) return pk_walk<false>(S, pos, nrm, S.ph_r2, nullptr, 0, cnt, unused) * S.ph_norm;
-    float r2 = heap[0] > 0.0f ? heap[0] : S.ph_r2;
+    if (cnt < S.ph_knn) return S.ph_r2;
+    return heap[0] > 0.0f ? heap[0] : S.ph_r2;
+}
+// ... and the gather at that radius.  Together they equal
+// photon_knn_irradiance: with fewer than k photons the gather within ph_r2
+// is that function's sum (same walk, same order, the radius never shrinking).
+RT_D V3 photon_knn_gather(const DScene& S, V3 pos, V3 nrm, float r2) {
     float norm = r2 == S.ph_r2 ? S.ph_norm : 2.0f / (PI * PI * r2);
+    uint32_t cnt;
+    float unused;
     return pk_walk<false>(S, pos, nrm, r2, nullptr, 0, cnt, unused) * norm;
 }
 

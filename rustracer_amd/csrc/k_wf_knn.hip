@@ -9,7 +9,9 @@
 // searches those: one query per lane with a heap in LDS when they are many,
 // one query per wave when they are few (the walk's latency then spread over
 // the wave's lanes).  Paths whose hit does not gather are
-// left as they are.
+// left as they are.  The first pass leaves each dense query's point, normal
+// and path id in the shadow queue's memory, which is idle between extend and
+// shade (two float4 per dense slot); the heap search adds the radius it finds.
 template <bool BATCH>
 __device__ __forceinline__ bool knn_query(const DScene& S, const WfPath& w, V3& pos, V3& nrm) {
     Ray r = make_ray(mk(w.o_time.x, w.o_time.y, w.o_time.z), mk(w.d_wl.x, w.d_wl.y, w.d_wl.z), w.o_time.w);
@@ -31,6 +33,7 @@ __global__ __launch_bounds__(BLOCK) void k_wf_knn(DScene S0, DParams P0, WfQueue
         uint32_t i = base0 + threadIdx.x;
         bool dense = false;
         uint32_t pid = 0;
+        V3 pos = mk(0, 0, 0), nrm = mk(0, 0, 0);
         if (i < total) {
             uint32_t b = 0, base = 0;
             while (i >= base + Q.counters[WF_CNT_BUCKET + b]) { base += Q.counters[WF_CNT_BUCKET + b]; b++; }
@@ -39,14 +42,19 @@ __global__ __launch_bounds__(BLOCK) void k_wf_knn(DScene S0, DParams P0, WfQueue
             const DScene S = BATCH ? view_scene(B, w.pad0) : S0;
             bool photons = BATCH ? B.views[w.pad0].P.use_photons != 0 : P0.use_photons != 0;
             if (photons && knn_gathers(S, w)) {
-                V3 pos, nrm, e;
+                V3 e;
                 knn_query<BATCH>(S, w, pos, nrm);
                 if (photon_knn_sparse(S, pos, nrm, e)) knn_store(w, e);
                 else dense = true;
             }
         }
         uint32_t slot = wave_append(Q.counters + WF_CNT_DENSE, 0u, dense);
-        if (dense) Q.dense[slot] = pid;
+        if (dense) {
+            Q.dense[slot] = pid;
+            float4* stash = reinterpret_cast<float4*>(Q.shadow);
+            stash[2 * (size_t)slot] = make_float4(pos.x, pos.y, pos.z, __uint_as_float(pid));
+            stash[2 * (size_t)slot + 1] = make_float4(nrm.x, nrm.y, nrm.z, 0.0f);
+        }
     }
 }
 // ---- the dense pass: one query per wave ------------------------------------------------------
@@ -157,22 +165,39 @@ __device__ V3 knn_wave(const DScene& S, V3 pos, V3 nrm, uint32_t lane) {
     return mk(wave_sum(a.x), wave_sum(a.y), wave_sum(a.z)) * norm;
 }
 
-// The dense pass is two kernels launched back to back, each of which returns
-// at once unless the dense count is in its regime: k_wf_knn_dense takes one
-// query per lane with a heap in LDS when more than knn_wave_max queries wait,
-// k_wf_knn_wave one query per wave (no LDS, full occupancy) otherwise.
+// The dense pass is launched as three kernels back to back, each of which
+// returns at once unless the dense count is in its regime.  More than
+// knn_wave_max queries: k_wf_knn_dense runs each query's heap search (one
+// query per lane, the heap in LDS) and leaves the radius it finds in the
+// stash; k_wf_knn_gather then sums the gather at that radius with no LDS, at
+// full occupancy (photon_knn_radius and photon_knn_gather: the walks and
+// sums of a one-kernel search).  Otherwise k_wf_knn_wave takes one query per
+// wave.
 template <bool BATCH>
 __global__ __launch_bounds__(BLOCK) void k_wf_knn_dense(DScene S0, WfQueues Q, WfBatch B) {
     const uint32_t total = Q.counters[WF_CNT_DENSE];
     if (total <= Q.knn_wave_max) return;
     lds_f32* heap = LdsHeap::column();
+    float4* stash = reinterpret_cast<float4*>(Q.shadow);
     uint32_t stride = gridDim.x * BLOCK;
     for (uint32_t i = blockIdx.x * BLOCK + threadIdx.x; i < total; i += stride) {
-        WfPath& w = Q.paths[Q.dense[i]];
+        float4 a = stash[2 * (size_t)i], n = stash[2 * (size_t)i + 1];
+        const DScene S = BATCH ? view_scene(B, Q.paths[__float_as_uint(a.w)].pad0) : S0;
+        n.w = photon_knn_radius(S, mk(a.x, a.y, a.z), mk(n.x, n.y, n.z), heap, BLOCK);
+        stash[2 * (size_t)i + 1] = n;
+    }
+}
+template <bool BATCH>
+__global__ __launch_bounds__(BLOCK) void k_wf_knn_gather(DScene S0, WfQueues Q, WfBatch B) {
+    const uint32_t total = Q.counters[WF_CNT_DENSE];
+    if (total <= Q.knn_wave_max) return;
+    const float4* stash = reinterpret_cast<const float4*>(Q.shadow);
+    uint32_t stride = gridDim.x * BLOCK;
+    for (uint32_t i = blockIdx.x * BLOCK + threadIdx.x; i < total; i += stride) {
+        float4 a = stash[2 * (size_t)i], n = stash[2 * (size_t)i + 1];
+        WfPath& w = Q.paths[__float_as_uint(a.w)];
         const DScene S = BATCH ? view_scene(B, w.pad0) : S0;
-        V3 pos, nrm;
-        knn_query<BATCH>(S, w, pos, nrm);
-        knn_store(w, photon_knn_dense(S, pos, nrm, heap, BLOCK));
+        knn_store(w, photon_knn_gather(S, mk(a.x, a.y, a.z), mk(n.x, n.y, n.z), n.w));
     }
 }
 template <bool BATCH>
@@ -194,7 +219,9 @@ __global__ __launch_bounds__(BLOCK) void k_wf_knn_wave(DScene S0, WfQueues Q, Wf
 
 template __global__ void k_wf_knn<true>(DScene, DParams, WfQueues, WfBatch);
 template __global__ void k_wf_knn_dense<true>(DScene, WfQueues, WfBatch);
+template __global__ void k_wf_knn_gather<true>(DScene, WfQueues, WfBatch);
 template __global__ void k_wf_knn_wave<true>(DScene, WfQueues, WfBatch);
 template __global__ void k_wf_knn<false>(DScene, DParams, WfQueues, WfBatch);
 template __global__ void k_wf_knn_dense<false>(DScene, WfQueues, WfBatch);
+template __global__ void k_wf_knn_gather<false>(DScene, WfQueues, WfBatch);
 template __global__ void k_wf_knn_wave<false>(DScene, WfQueues, WfBatch);

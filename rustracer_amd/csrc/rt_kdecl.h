@@ -45,6 +45,7 @@ template <int OCC, bool BATCH, int BS> __global__ void k_wf_extend(DScene S, WfQ
 template <int OCC, bool BATCH, int BS> __global__ void k_wf_shadow(DScene S, WfQueues Q, WfBatch B);
 template <bool BATCH> __global__ void k_wf_knn(DScene S0, DParams P0, WfQueues Q, WfBatch B);
 template <bool BATCH> __global__ void k_wf_knn_dense(DScene S0, WfQueues Q, WfBatch B);
+template <bool BATCH> __global__ void k_wf_knn_gather(DScene S0, WfQueues Q, WfBatch B);
 template <bool BATCH> __global__ void k_wf_knn_wave(DScene S0, WfQueues Q, WfBatch B);
 template <bool BATCH, int CLASS, bool H4> __global__ void k_wf_shade(DScene S, DParams P, WfQueues Q, int depth, WfBatch B);
 

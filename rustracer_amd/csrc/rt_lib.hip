@@ -1233,6 +1233,7 @@ static void wf_chunk(rt_ctx* ctx, hipStream_t st, const DScene& S, const DCamera
         if (knn) {
             hipLaunchKernelGGL((k_wf_knn<BT>), dim3(grid), dim3(BLOCK), 0, st, S, P, Q, Bt);
             hipLaunchKernelGGL((k_wf_knn_dense<BT>), dim3(grid), dim3(BLOCK), 0, st, S, Q, Bt);
+            hipLaunchKernelGGL((k_wf_knn_gather<BT>), dim3(grid), dim3(BLOCK), 0, st, S, Q, Bt);
             hipLaunchKernelGGL((k_wf_knn_wave<BT>), dim3(grid), dim3(BLOCK), 0, st, S, Q, Bt);
         }
         if (ev) (void)hipEventRecord(E[2], st);
