@@ -1147,8 +1147,17 @@ RT_D V3 pk_walk(const DScene& S, V3 pos, V3 nrm, float r2, lds_f32* heap, uint32
         uint32_t leaf = node - first_leaf;
         uint32_t b = S.pk_leaf[leaf], e = S.pk_leaf[leaf + 1];
         if (nb) *nb += 32u * (e - b);
-        for (uint32_t i = b; i < e; i++) {
-            float4 P = S.pk_rec[2 * i], N = S.pk_rec[2 * i + 1];
+        // the leaf's records (at most RT_PK_LEAF) all requested before the
+        // first is used: one memory round trip per leaf, not per photon
+        float4 PL[RT_PK_LEAF], NL[RT_PK_LEAF];
+#pragma unroll
+        for (uint32_t j = 0; j < RT_PK_LEAF; j++)
+            if (b + j < e) { PL[j] = S.pk_rec[2 * (b + j)]; NL[j] = S.pk_rec[2 * (b + j) + 1]; }
+#pragma unroll
+        for (uint32_t j = 0; j < RT_PK_LEAF; j++) {
+            if (b + j >= e) break;
+            const uint32_t i = b + j;
+            float4 P = PL[j], N = NL[j];
             float dx = P.x - pos.x, dy = P.y - pos.y, dz = P.z - pos.z;
             float d2 = dx * dx + dy * dy + dz * dz;
             if (d2 < bound && N.x * nrm.x + N.y * nrm.y + N.z * nrm.z > 0.0f) {
