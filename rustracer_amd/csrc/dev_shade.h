@@ -1083,32 +1083,45 @@ RT_D V3 photon_gather(const DScene& S, V3 pos, V3 nrm, float r2, uint32_t* nb = 
 #endif
 
 // Push d2 into this lane's max-heap of k floats (its LDS column, `stride`
-// apart); once k are held, replace the root when d2 is smaller.
-RT_D void knn_heap_add(lds_f32* heap, uint32_t stride, uint32_t k, uint32_t& cnt, float d2) {
+// apart); once k are held, replace the root when d2 is smaller.  The heap is
+// 4-ary (children of i: 4i+1 .. 4i+4): three levels for k = 64, and a
+// level's child reads are independent, so a sift waits on half the LDS round
+// trips a binary heap would.  Returns the new root (the largest held), which
+// is the k-th smallest distance once k are held, whatever the heap's shape.
+RT_D float knn_heap_add(lds_f32* heap, uint32_t stride, uint32_t k, uint32_t& cnt, float d2) {
     if (cnt < k) {                               // push, sift up
         uint32_t i = cnt++;
+        float root = d2;
         while (i > 0) {
-            uint32_t p = (i - 1) >> 1;
+            uint32_t p = (i - 1) >> 2;
             float hp = heap[p * stride];
-            if (hp >= d2) break;
+            if (hp >= d2) { root = p == 0 ? hp : heap[0]; break; }
             heap[i * stride] = hp;
             i = p;
         }
         heap[i * stride] = d2;
-        return;
+        return i == 0 ? d2 : root;
     }
     uint32_t i = 0;                              // replace the root, sift down
+    float root = d2;
     for (;;) {
-        uint32_t l = 2 * i + 1;
-        if (l >= k) break;
-        uint32_t c = l;
-        float hc = heap[l * stride];
-        if (l + 1 < k) { float hr = heap[(l + 1) * stride]; if (hr > hc) { c = l + 1; hc = hr; } }
+        const uint32_t c0 = 4 * i + 1;
+        if (c0 >= k) break;
+        uint32_t c = c0;
+        float hc = heap[c0 * stride];
+        float h1 = c0 + 1 < k ? heap[(c0 + 1) * stride] : -FINF;
+        float h2 = c0 + 2 < k ? heap[(c0 + 2) * stride] : -FINF;
+        float h3 = c0 + 3 < k ? heap[(c0 + 3) * stride] : -FINF;
+        if (h1 > hc) { c = c0 + 1; hc = h1; }
+        if (h2 > hc) { c = c0 + 2; hc = h2; }
+        if (h3 > hc) { c = c0 + 3; hc = h3; }
         if (hc <= d2) break;
+        if (i == 0) root = hc;
         heap[i * stride] = hc;
         i = c;
     }
     heap[i * stride] = d2;
+    return root;
 }
 
 // Range walk of the bucket tree around `pos` over the photons on the query's
@@ -1167,8 +1180,8 @@ RT_D V3 pk_walk(const DScene& S, V3 pos, V3 nrm, float r2, lds_f32* heap, uint32
                     if (nb) *nb += 16u;
                 }
                 if (KNN) {
-                    knn_heap_add(heap, stride, k, cnt, d2);
-                    if (cnt == k) bound = heap[0];
+                    float root = knn_heap_add(heap, stride, k, cnt, d2);
+                    if (cnt == k) bound = root;
                 } else {
                     cnt++;
                     if (stop && cnt >= stop) { count = cnt; bound_out = bound; return acc; }
