@@ -259,51 +259,52 @@ RT_D int32_t tkey(float f) { int32_t i = __float_as_int(f); return i ^ (int32_t)
 // Children read as ints.  The node's 14 float4 loads are all issued before
 // the slab tests (a scheduling barrier keeps the compiler from spreading them
 // over three dependent rounds to save registers), so a node costs one memory
-// round trip.
-#ifndef RT_NODE_PRELOAD
-#define RT_NODE_PRELOAD 1
-#endif
-
+// round trip.  Each lane loads an axis's planes near side first: the upper
+// bounds when its ray's inverse direction component is negative.
 RT_D uint32_t node_test(const float4* node, const Ray& r, float t_min, float t_max, int32_t key[8], int32_t ch[8]) {
-#if RT_NODE_PRELOAD
+    const bool neg[3] = {r.inv.x < 0.0f, r.inv.y < 0.0f, r.inv.z < 0.0f};
     float4 q[14];
 #pragma unroll
-    for (int i = 0; i < 14; i++) q[i] = node[i];
+    for (int a = 0; a < 3; a++) {                      // planes of axis a: float4 2a, 2a+1 lower, 6+2a, 7+2a upper
+        const int s = neg[a] ? 6 : 0;
+        q[2 * a] = node[2 * a + s]; q[2 * a + 1] = node[2 * a + 1 + s];
+        q[6 + 2 * a] = node[6 + 2 * a - s]; q[7 + 2 * a] = node[7 + 2 * a - s];
+    }
+    q[12] = node[12]; q[13] = node[13];
     __builtin_amdgcn_sched_barrier(0);      // all 14 loads in flight before the slab tests
-#define RT_NODE_Q(i) q[i]
-#else
-#define RT_NODE_Q(i) node[i]
-#endif
-    float lo[3][8], hi[3][8];
+    float nr[3][8], fr[3][8];                 // near and far planes
 #pragma unroll
     for (int a = 0; a < 3; a++) {
-        float4 l0 = RT_NODE_Q(2 * a), l1 = RT_NODE_Q(2 * a + 1), h0 = RT_NODE_Q(6 + 2 * a), h1 = RT_NODE_Q(7 + 2 * a);
-        lo[a][0] = l0.x; lo[a][1] = l0.y; lo[a][2] = l0.z; lo[a][3] = l0.w;
-        lo[a][4] = l1.x; lo[a][5] = l1.y; lo[a][6] = l1.z; lo[a][7] = l1.w;
-        hi[a][0] = h0.x; hi[a][1] = h0.y; hi[a][2] = h0.z; hi[a][3] = h0.w;
-        hi[a][4] = h1.x; hi[a][5] = h1.y; hi[a][6] = h1.z; hi[a][7] = h1.w;
+        float4 n0 = q[2 * a], n1 = q[2 * a + 1], f0 = q[6 + 2 * a], f1 = q[7 + 2 * a];
+        nr[a][0] = n0.x; nr[a][1] = n0.y; nr[a][2] = n0.z; nr[a][3] = n0.w;
+        nr[a][4] = n1.x; nr[a][5] = n1.y; nr[a][6] = n1.z; nr[a][7] = n1.w;
+        fr[a][0] = f0.x; fr[a][1] = f0.y; fr[a][2] = f0.z; fr[a][3] = f0.w;
+        fr[a][4] = f1.x; fr[a][5] = f1.y; fr[a][6] = f1.z; fr[a][7] = f1.w;
     }
-    float4 c0 = RT_NODE_Q(12), c1 = RT_NODE_Q(13);
-#undef RT_NODE_Q
+    float4 c0 = q[12], c1 = q[13];
     ch[0] = __float_as_int(c0.x); ch[1] = __float_as_int(c0.y); ch[2] = __float_as_int(c0.z); ch[3] = __float_as_int(c0.w);
     ch[4] = __float_as_int(c1.x); ch[5] = __float_as_int(c1.y); ch[6] = __float_as_int(c1.z); ch[7] = __float_as_int(c1.w);
     const float o[3] = {r.o.x, r.o.y, r.o.z}, id[3] = {r.inv.x, r.inv.y, r.inv.z};
     uint32_t mask = 0;
     // The slab bounds are NaN only when a direction component's inverse is
-    // infinite (0 x inf).  With all three inverses finite and t_min > 0 the
-    // hardware min / max (three-operand forms over the axes) give the ordered
-    // compare-selects' results bit for bit: t_near >= t_min > 0, so no signed
-    // zero reaches t_near or the mask, and t_near's bits are its total_cmp key.
-    // Otherwise the reference's compare-selects, axis by axis.
+    // infinite (0 x inf).  With all three inverses finite, a child's box
+    // lo <= hi (an empty slot's box is a point) and t_min > 0: the near
+    // plane's distance is the smaller of the pair bit for bit (rounded
+    // subtraction, and multiplication by a number of fixed sign, keep the
+    // order), so the compare-selects of the reference reduce to the hardware
+    // max / min (three-operand forms) over the axes' near and far distances:
+    // t_near >= t_min > 0, so no signed zero reaches t_near or the mask, and
+    // t_near's bits are its total_cmp key.  Otherwise the reference's
+    // compare-selects on the (lower, upper) pairs, axis by axis.
     const bool fast = t_min > 0.0f && fabsf(r.inv.x) < FINF && fabsf(r.inv.y) < FINF && fabsf(r.inv.z) < FINF;
     if (fast) {
 #pragma unroll
         for (int c = 0; c < 8; c++) {
-            float d[3][2];
+            float dn[3], df[3];
 #pragma unroll
-            for (int a = 0; a < 3; a++) { d[a][0] = (lo[a][c] - o[a]) * id[a]; d[a][1] = (hi[a][c] - o[a]) * id[a]; }
-            float tn = fmaxf(fmaxf(fmaxf(t_min, fminf(d[0][0], d[0][1])), fminf(d[1][0], d[1][1])), fminf(d[2][0], d[2][1]));
-            float tf = fminf(fminf(fminf(t_max, fmaxf(d[0][0], d[0][1])), fmaxf(d[1][0], d[1][1])), fmaxf(d[2][0], d[2][1]));
+            for (int a = 0; a < 3; a++) { dn[a] = (nr[a][c] - o[a]) * id[a]; df[a] = (fr[a][c] - o[a]) * id[a]; }
+            float tn = fmaxf(fmaxf(fmaxf(t_min, dn[0]), dn[1]), dn[2]);
+            float tf = fminf(fminf(fminf(t_max, df[0]), df[1]), df[2]);
             key[c] = __float_as_int(tn);
             if (tn <= tf && ch[c] != INT32_MIN) mask |= 1u << c;
         }
@@ -313,8 +314,9 @@ RT_D uint32_t node_test(const float4* node, const Ray& r, float t_min, float t_m
             float t0 = t_min, t1 = t_max;
 #pragma unroll
             for (int a = 0; a < 3; a++) {
-                float da = (lo[a][c] - o[a]) * id[a];
-                float db = (hi[a][c] - o[a]) * id[a];
+                const float lo = neg[a] ? fr[a][c] : nr[a][c], hi = neg[a] ? nr[a][c] : fr[a][c];
+                float da = (lo - o[a]) * id[a];
+                float db = (hi - o[a]) * id[a];
                 t0 = mxps(t0, mnps(da, db));
                 t1 = mnps(t1, mxps(da, db));
             }
