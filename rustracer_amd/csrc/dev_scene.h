@@ -331,8 +331,9 @@ RT_D uint32_t node_test(const float4* node, const Ray& r, float t_min, float t_m
 // first.  Each child's slot is its rank in that order: the hit children with
 // a larger key, or an equal key and a lower index, go below it.  One integer
 // comparison per pair of slots decides both directions (for c < d, c goes
-// below d exactly when key[c] >= key[d]), and no private array is indexed
-// dynamically (no scratch traffic).
+// below d exactly when key[c] >= key[d]); the sort itself indexes no private
+// array dynamically (the stack's overflow past RT_LDS_STACK entries, a
+// private array, is written only on deep stacks).
 RT_D void push_sorted(Stack& st, uint32_t mask, const int32_t key[8], const int32_t ch[8]) {
     int32_t rank[8];
 #pragma unroll
