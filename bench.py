@@ -83,9 +83,9 @@ def log(msg):
 # reported beside the algorithmic bytes): extend reads the queue entry (4 B)
 # and the ray of the 128-B path record (36 B) and writes the hit (12 B) and
 # the bucket entry (4 B); shade reads the bucket entry and the record and
-# writes the record and the next-queue entry; each shadow request is written
-# by shade and read by shadow (64 B).
-QUEUE_BYTES = {"extend": 4 + 36 + 12 + 4, "shade": 4 + 128 + 128 + 4, "shadow_request": 64}
+# writes the record and the next-queue entry; each shadow request and its
+# 4-B entry in the request list are written by shade and read by shadow.
+QUEUE_BYTES = {"extend": 4 + 36 + 12 + 4, "shade": 4 + 128 + 128 + 4, "shadow_request": 64 + 4}
 
 
 def stage_figures(st, cnt, samples):

@@ -456,7 +456,7 @@ int  rt_render_wavefront(rt_ctx *ctx, rt_scene *scene, const rt_camera *cam,
 /* rt_render_wavefront for callers that cannot allocate device memory (code
  * under HIP graph capture, tight serving loops, harnesses that forbid
  * hipMalloc).  No counterpart in the reference.  The path queues live in a
- * fixed device array of RT_NOALLOC_CAPACITY paths (52 MiB) that every GPU
+ * fixed device array of RT_NOALLOC_CAPACITY paths (53.5 MiB) that every GPU
  * the library runs on holds until the process ends; the call allocates and
  * frees nothing (no hipMalloc, hipFree, streams, graphs or events) and clears
  * with kernels, not hipMemsetAsync.  accum_rgb must be device memory.  opts

@@ -90,7 +90,7 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(OCC > 0 ? OC
     uint32_t count = Q.counters[WF_CNT_SHADOW];
     uint32_t stride = gridDim.x * BS;
     for (uint32_t i = blockIdx.x * BS + threadIdx.x; i < count; i += stride) {
-        WfShadow q = Q.shadow[i];
+        WfShadow q = Q.shadow[Q.shadow_idx[i]];
         Ray r = make_ray(mk(q.o_tmax.x, q.o_tmax.y, q.o_tmax.z), mk(q.d_time.x, q.d_time.y, q.d_time.z), q.d_time.w);
         float tr = BATCH ? trace_transmittance(view_scene(B, q.meta.x), r, 0.001f, q.o_tmax.w, st, false)
                          : trace_transmittance(S, r, 0.001f, q.o_tmax.w, st, false);

@@ -92,9 +92,11 @@ struct DEmitter {
 #define WF_NB (M_KINDS + 2)   // sort keys (sort 1): 0 miss, 1..M_KINDS material kind, M_KINDS + 1 medium event;
                           // (sort 2): 0 miss / specular / medium, 1 every other material
 #define WF_CNT_ACTIVE 0   // counters[0]: paths to extend this bounce
-#define WF_CNT_NEXT 1     // counters[1]: survivors written by shade
-#define WF_CNT_SHADOW 2   // counters[2]: shadow requests
-#define WF_CNT_DENSE 3    // counters[3]: k-NN queries holding at least k photons (second k-NN pass)
+#define WF_CNT_DENSE 1    // counters[1]: k-NN queries holding at least k photons (second k-NN pass)
+#define WF_CNT_NEXT 2     // counters[2]: survivors written by shade
+#define WF_CNT_SHADOW 3   // counters[3]: shadow requests; with NEXT one 8-byte-aligned pair, so the
+                          // shade stage reserves both with one 64-bit atomic per wave
+static_assert(WF_CNT_NEXT % 2 == 0 && WF_CNT_SHADOW == WF_CNT_NEXT + 1, "NEXT / SHADOW pair");
 #define WF_CNT_BUCKET 8   // counters[8..8+WF_NB): bucket sizes
 #define WF_CNT_FIRST 32   // counters[32..33]: 64-bit frame position of a graph-replayed chunk
 #define WF_CNT_GEN 34     // counters[34]: paths generated when flagged pixels are skipped
@@ -121,7 +123,13 @@ struct WfQueues {
     uint32_t* active = nullptr;     // cap
     uint32_t* next = nullptr;       // cap
     uint32_t* buckets = nullptr;    // WF_NB * cap
-    WfShadow* shadow = nullptr;     // 3 * cap
+    // 3 * cap: the forward shade stage writes the k-th shadow request of the
+    // bounce's i-th path at k * cap + i (at most three: area light, sun,
+    // environment) and lists the filled slots in shadow_idx, which k_wf_shadow
+    // walks; the replay kernels (k_wfb_*) fill it densely; between extend and
+    // shade the k-NN stage keeps its dense queries here
+    WfShadow* shadow = nullptr;
+    uint32_t* shadow_idx = nullptr; // 3 * cap
     uint32_t* counters = nullptr;   // 64: 32 per-bounce counters, then the chunk position
     float* fb = nullptr;
     float* lsamp = nullptr;         // non-null: contributions go to the path's sample (3 floats per sid), not to fb
@@ -144,7 +152,8 @@ static_assert(std::is_trivially_copyable<WfQueues>::value, "kernel argument");
 constexpr WfQueues kWfQueuesDefault{};
 static_assert(kWfQueuesDefault.paths == nullptr && kWfQueuesDefault.active == nullptr &&
               kWfQueuesDefault.next == nullptr && kWfQueuesDefault.buckets == nullptr &&
-              kWfQueuesDefault.shadow == nullptr && kWfQueuesDefault.counters == nullptr &&
+              kWfQueuesDefault.shadow == nullptr && kWfQueuesDefault.shadow_idx == nullptr &&
+              kWfQueuesDefault.counters == nullptr &&
               kWfQueuesDefault.fb == nullptr && kWfQueuesDefault.lsamp == nullptr &&
               kWfQueuesDefault.dense == nullptr && kWfQueuesDefault.stop == nullptr &&
               kWfQueuesDefault.conv == nullptr && kWfQueuesDefault.spec == nullptr &&
@@ -169,6 +178,25 @@ __device__ __forceinline__ uint32_t wave_append(uint32_t* counters, uint32_t key
         pending &= ~m;
     }
     return slot;
+}
+
+// Reserve `a` (0 or 1) slots in the count at pair[0] and `b` (0 to 3) in the
+// count at pair[1] for every lane of the wave with one 64-bit atomic (pair
+// 8-byte aligned; the first count never reaches 2^32, so no carry crosses).
+// Returns each lane's first slot in the two lists.  Every lane of the wave
+// must call it (the prefix sums are over the wave).
+__device__ __forceinline__ uint2 wave_reserve2(uint32_t* pair, uint32_t a, uint32_t b) {
+    const uint32_t lane = lane_id();
+    const unsigned long long lt = (1ull << lane) - 1ull;
+    const unsigned long long ma = __ballot(a != 0u), m1 = __ballot((b & 1u) != 0u), m2 = __ballot((b & 2u) != 0u);
+    const uint32_t ta = (uint32_t)__popcll(ma), tb = (uint32_t)__popcll(m1) + 2u * (uint32_t)__popcll(m2);
+    const int leader = __ffsll(__ballot(true)) - 1;
+    unsigned long long base = 0;
+    if ((int)lane == leader && (ta | tb) != 0u)
+        base = atomicAdd(reinterpret_cast<unsigned long long*>(pair), ((unsigned long long)tb << 32) | ta);
+    base = __shfl(base, leader);
+    return make_uint2((uint32_t)base + (uint32_t)__popcll(ma & lt),
+                      (uint32_t)(base >> 32) + (uint32_t)__popcll(m1 & lt) + 2u * (uint32_t)__popcll(m2 & lt));
 }
 
 __device__ __forceinline__ void wf_load(const WfPath& w, PathState& ps, Rng& rng) {

@@ -190,11 +190,12 @@ static void spectral_norms(float norms[3], float fire[3], int n = RT_CIE_N) {
 
 // ---- wavefront buffer layout ------------------------------------------------------------------
 // One block per lane: path records, the active / next / dense queues, the
-// per-key buckets, the shadow requests, then 256 B of counters.
+// per-key buckets, the shadow requests and their index list, then 256 B of
+// counters.
 constexpr size_t wf_al(size_t x) { return (x + 255) & ~(size_t)255; }
 constexpr size_t wf_layout_bytes(uint32_t cap) {
     return wf_al((size_t)cap * sizeof(WfPath)) + 3 * wf_al((size_t)cap * 4) + wf_al((size_t)WF_NB * cap * 4) +
-           wf_al((size_t)3 * cap * sizeof(WfShadow)) + wf_al((size_t)2 * cap * sizeof(float4)) + 256;
+           wf_al((size_t)3 * cap * sizeof(WfShadow)) + wf_al((size_t)3 * cap * 4) + wf_al((size_t)2 * cap * sizeof(float4)) + 256;
 }
 static void wf_layout(char* m, uint32_t cap, WfQueues& Q) {
     Q.paths = (WfPath*)m; m += wf_al((size_t)cap * sizeof(WfPath));
@@ -203,6 +204,7 @@ static void wf_layout(char* m, uint32_t cap, WfQueues& Q) {
     Q.dense = (uint32_t*)m; m += wf_al((size_t)cap * 4);
     Q.buckets = (uint32_t*)m; m += wf_al((size_t)WF_NB * cap * 4);
     Q.shadow = (WfShadow*)m; m += wf_al((size_t)3 * cap * sizeof(WfShadow));
+    Q.shadow_idx = (uint32_t*)m; m += wf_al((size_t)3 * cap * 4);
     Q.spec = (float4*)m; m += wf_al((size_t)2 * cap * sizeof(float4));
     Q.counters = (uint32_t*)m;
     Q.cap = cap;
@@ -214,7 +216,7 @@ static void wf_layout(char* m, uint32_t cap, WfQueues& Q) {
 }
 
 // The no-allocation wavefront's queues: a file-scope device array of one
-// lane's buffers at RT_NOALLOC_CAPACITY paths (52 MiB), held by every GPU the
+// lane's buffers at RT_NOALLOC_CAPACITY paths (53.5 MiB), held by every GPU the
 // library runs on until the process ends.  Host code never uses the name
 // itself as a device address: rt_ctx_create looks up this device's copy once
 // (hipGetSymbolAddress, with the device current) and keeps it in ctx->scratch.
