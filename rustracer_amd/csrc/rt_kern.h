@@ -162,22 +162,26 @@ static_assert(kWfQueuesDefault.paths == nullptr && kWfQueuesDefault.active == nu
 
 __device__ __forceinline__ uint32_t lane_id() { return __lane_id(); }
 
-// Wave-aggregated append: one atomic per distinct key in the wave.
+// Wave-aggregated append: one atomic per distinct key in the wave, all of
+// them issued together (the lowest lane of each key's group adds the group's
+// count to that key's counter in one vector atomic), so the wave waits for
+// one round trip however many keys it holds.
 __device__ __forceinline__ uint32_t wave_append(uint32_t* counters, uint32_t key, bool valid) {
-    uint32_t slot = 0;
-    uint32_t lane = lane_id();
+    const uint32_t lane = lane_id();
+    unsigned long long same = 0;                 // the lanes holding this lane's key
     unsigned long long pending = __ballot(valid);
     while (pending) {
-        int leader = __ffsll(pending) - 1;
-        uint32_t k = __shfl(key, leader);
-        unsigned long long m = __ballot(valid && key == k);
-        uint32_t base = 0;
-        if ((int)lane == leader) base = atomicAdd(&counters[k], (uint32_t)__popcll(m));
-        base = __shfl(base, leader);
-        if (valid && key == k) slot = base + (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
+        const int leader = __ffsll(pending) - 1;
+        const uint32_t k = __shfl(key, leader);
+        const unsigned long long m = __ballot(valid && key == k);
+        if (valid && key == k) same = m;
         pending &= ~m;
     }
-    return slot;
+    const int first = valid ? __ffsll(same) - 1 : (int)lane;
+    uint32_t base = 0;
+    if (valid && (int)lane == first) base = atomicAdd(&counters[key], (uint32_t)__popcll(same));
+    base = __shfl(base, first);
+    return valid ? base + (uint32_t)__popcll(same & ((1ull << lane) - 1ull)) : 0u;
 }
 
 // Reserve `a` (0 or 1) slots in the count at pair[0] and `b` (0 to 3) in the
