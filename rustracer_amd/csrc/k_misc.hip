@@ -163,6 +163,7 @@ __global__ void k_wf_reset(uint32_t* counters, uint32_t active, const uint32_t* 
     if (generated) active = counters[WF_CNT_GEN];
     if (stop && *stop) active = 0;
     if (i < 32) counters[i] = (i == WF_CNT_ACTIVE) ? active : 0u;
+    if (i < WF_NB) counters[WF_CNT_BKT(i)] = 0u;
 }
 __global__ void k_wf_gen_zero(uint32_t* counters) {
     if (threadIdx.x == 0) counters[WF_CNT_GEN] = 0u;
@@ -174,6 +175,7 @@ __global__ void k_wf_advance(uint32_t* counters) {
     if (i == 0) nxt = counters[WF_CNT_NEXT];
     __syncthreads();
     if (i < 32) counters[i] = (i == WF_CNT_ACTIVE) ? nxt : 0u;
+    if (i < WF_NB) counters[WF_CNT_BKT(i)] = 0u;
 }
 
 // counters[WF_CNT_FIRST..+1] hold the frame position of a graph-replayed

@@ -78,7 +78,7 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(OCC > 0 ? OC
             if (BATCH) key = extend_one(view_scene(B, w.pad0), w, Q.sort, st);
             else key = extend_one(S, w, Q.sort, st);
         }
-        uint32_t slot = wave_append(Q.counters + WF_CNT_BUCKET, key, valid);
+        uint32_t slot = wave_append(Q.counters + WF_CNT_BKT(0), key, valid, WF_BKT_STRIDE);
         if (valid) Q.buckets[(size_t)key * Q.cap + slot] = pid;
     }
 }

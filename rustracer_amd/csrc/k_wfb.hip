@@ -68,7 +68,7 @@ __global__ __launch_bounds__(BLOCK) void k_wfb_shade(DScene S, DParams P, WfQueu
     RT_STACK(st);
     uint32_t off[WF_NB + 1];
     off[0] = 0;
-    for (int b = 0; b < WF_NB; b++) off[b + 1] = off[b] + Q.counters[WF_CNT_BUCKET + b];
+    for (int b = 0; b < WF_NB; b++) off[b + 1] = off[b] + Q.counters[WF_CNT_BKT(b)];
     uint32_t total = off[WF_NB];
     uint32_t stride = gridDim.x * BLOCK;
     for (uint32_t base = blockIdx.x * BLOCK; base < total; base += stride) {

@@ -97,7 +97,12 @@ struct DEmitter {
 #define WF_CNT_SHADOW 3   // counters[3]: shadow requests; with NEXT one 8-byte-aligned pair, so the
                           // shade stage reserves both with one 64-bit atomic per wave
 static_assert(WF_CNT_NEXT % 2 == 0 && WF_CNT_SHADOW == WF_CNT_NEXT + 1, "NEXT / SHADOW pair");
-#define WF_CNT_BUCKET 8   // counters[8..8+WF_NB): bucket sizes
+// bucket sizes: bucket b's counter at WF_CNT_BKT(b), 256 B apart, so the
+// atomics of one extend wave's buckets go to different L2 channels
+#define WF_BKT_STRIDE 64
+#define WF_CNT_BKT(b) (64 + (b) * WF_BKT_STRIDE)
+#define WF_CNT_WORDS 1024 // the counter block: 4 KB
+static_assert(WF_CNT_BKT(WF_NB) <= WF_CNT_WORDS, "bucket counters fit the block");
 #define WF_CNT_FIRST 32   // counters[32..33]: 64-bit frame position of a graph-replayed chunk
 #define WF_CNT_GEN 34     // counters[34]: paths generated when flagged pixels are skipped
 
@@ -130,7 +135,7 @@ struct WfQueues {
     // shade the k-NN stage keeps its dense queries here
     WfShadow* shadow = nullptr;
     uint32_t* shadow_idx = nullptr; // 3 * cap
-    uint32_t* counters = nullptr;   // 64: 32 per-bounce counters, then the chunk position
+    uint32_t* counters = nullptr;   // WF_CNT_WORDS: 32 per-bounce counters, the chunk position, the bucket sizes
     float* fb = nullptr;
     float* lsamp = nullptr;         // non-null: contributions go to the path's sample (3 floats per sid), not to fb
     uint32_t sort = 1;              // 0: one bucket; 1: a bucket per material kind; 2: two shade classes
@@ -166,7 +171,7 @@ __device__ __forceinline__ uint32_t lane_id() { return __lane_id(); }
 // them issued together (the lowest lane of each key's group adds the group's
 // count to that key's counter in one vector atomic), so the wave waits for
 // one round trip however many keys it holds.
-__device__ __forceinline__ uint32_t wave_append(uint32_t* counters, uint32_t key, bool valid) {
+__device__ __forceinline__ uint32_t wave_append(uint32_t* counters, uint32_t key, bool valid, uint32_t stride = 1) {
     const uint32_t lane = lane_id();
     unsigned long long same = 0;                 // the lanes holding this lane's key
     unsigned long long pending = __ballot(valid);
@@ -179,7 +184,7 @@ __device__ __forceinline__ uint32_t wave_append(uint32_t* counters, uint32_t key
     }
     const int first = valid ? __ffsll(same) - 1 : (int)lane;
     uint32_t base = 0;
-    if (valid && (int)lane == first) base = atomicAdd(&counters[key], (uint32_t)__popcll(same));
+    if (valid && (int)lane == first) base = atomicAdd(&counters[key * stride], (uint32_t)__popcll(same));
     base = __shfl(base, first);
     return valid ? base + (uint32_t)__popcll(same & ((1ull << lane) - 1ull)) : 0u;
 }

@@ -190,12 +190,12 @@ static void spectral_norms(float norms[3], float fire[3], int n = RT_CIE_N) {
 
 // ---- wavefront buffer layout ------------------------------------------------------------------
 // One block per lane: path records, the active / next / dense queues, the
-// per-key buckets, the shadow requests and their index list, then 256 B of
+// per-key buckets, the shadow requests and their index list, then 4 KB of
 // counters.
 constexpr size_t wf_al(size_t x) { return (x + 255) & ~(size_t)255; }
 constexpr size_t wf_layout_bytes(uint32_t cap) {
     return wf_al((size_t)cap * sizeof(WfPath)) + 3 * wf_al((size_t)cap * 4) + wf_al((size_t)WF_NB * cap * 4) +
-           wf_al((size_t)3 * cap * sizeof(WfShadow)) + wf_al((size_t)3 * cap * 4) + wf_al((size_t)2 * cap * sizeof(float4)) + 256;
+           wf_al((size_t)3 * cap * sizeof(WfShadow)) + wf_al((size_t)3 * cap * 4) + wf_al((size_t)2 * cap * sizeof(float4)) + 4 * WF_CNT_WORDS;
 }
 static void wf_layout(char* m, uint32_t cap, WfQueues& Q) {
     Q.paths = (WfPath*)m; m += wf_al((size_t)cap * sizeof(WfPath));
@@ -1446,7 +1446,7 @@ extern "C" int rt_render_wavefront_noalloc(rt_ctx* ctx, rt_scene* s, const rt_ca
     Q.h4 = P.wl == 2;
     hipLaunchKernelGGL(k_fill_u32, dim3(std::min((npix * 3 + BLOCK - 1) / BLOCK, 4096u)), dim3(BLOCK), 0, ctx->stream,
                        (uint32_t*)out, (uint64_t)npix * 3, 0u);
-    hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(BLOCK), 0, ctx->stream, Q.counters, (uint64_t)64, 0u);
+    hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(BLOCK), 0, ctx->stream, Q.counters, (uint64_t)WF_CNT_WORDS, 0u);
     HIPCHECK(ctx, hipGetLastError());
     WfBatch none = {nullptr, nullptr, 0};
     ctx->no_alloc = true;
