@@ -26,8 +26,10 @@ __device__ __forceinline__ void knn_store(WfPath& w, V3 e) {
 }
 template <bool BATCH>
 __global__ __launch_bounds__(BLOCK) void k_wf_knn(DScene S0, DParams P0, WfQueues Q, WfBatch B) {
-    uint32_t total = 0;
-    for (int b = 0; b < WF_NB; b++) total += Q.counters[WF_CNT_BKT(b)];
+    uint32_t off[WF_NB + 1];                 // bucket starts, read once
+    off[0] = 0;
+    for (int b = 0; b < WF_NB; b++) off[b + 1] = off[b] + Q.counters[WF_CNT_BKT(b)];
+    const uint32_t total = off[WF_NB];
     uint32_t stride = gridDim.x * BLOCK;
     for (uint32_t base0 = blockIdx.x * BLOCK; base0 < total; base0 += stride) {
         uint32_t i = base0 + threadIdx.x;
@@ -35,9 +37,9 @@ __global__ __launch_bounds__(BLOCK) void k_wf_knn(DScene S0, DParams P0, WfQueue
         uint32_t pid = 0;
         V3 pos = mk(0, 0, 0), nrm = mk(0, 0, 0);
         if (i < total) {
-            uint32_t b = 0, base = 0;
-            while (i >= base + Q.counters[WF_CNT_BKT(b)]) { base += Q.counters[WF_CNT_BKT(b)]; b++; }
-            pid = Q.buckets[(size_t)b * Q.cap + (i - base)];
+            uint32_t b = 0;
+            while (i >= off[b + 1]) b++;
+            pid = Q.buckets[(size_t)b * Q.cap + (i - off[b])];
             WfPath& w = Q.paths[pid];
             const DScene S = BATCH ? view_scene(B, w.pad0) : S0;
             bool photons = BATCH ? B.views[w.pad0].P.use_photons != 0 : P0.use_photons != 0;
