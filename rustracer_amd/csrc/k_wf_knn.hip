@@ -11,7 +11,8 @@
 // the wave's lanes).  Paths whose hit does not gather are
 // left as they are.  The first pass leaves each dense query's point, normal
 // and path id in the shadow queue's memory, which is idle between extend and
-// shade (two float4 per dense slot); the heap search adds the radius it finds.
+// shade (two float4 per dense slot: point and path id, normal and the path's
+// view); the heap search reads the view and puts the radius it finds in its place.
 template <bool BATCH>
 __device__ __forceinline__ bool knn_query(const DScene& S, const WfPath& w, V3& pos, V3& nrm) {
     Ray r = make_ray(mk(w.o_time.x, w.o_time.y, w.o_time.z), mk(w.d_wl.x, w.d_wl.y, w.d_wl.z), w.o_time.w);
@@ -34,14 +35,15 @@ __global__ __launch_bounds__(BLOCK) void k_wf_knn(DScene S0, DParams P0, WfQueue
     for (uint32_t base0 = blockIdx.x * BLOCK; base0 < total; base0 += stride) {
         uint32_t i = base0 + threadIdx.x;
         bool dense = false;
-        uint32_t pid = 0;
+        uint32_t pid = 0, view = 0;
         V3 pos = mk(0, 0, 0), nrm = mk(0, 0, 0);
         if (i < total) {
             uint32_t b = 0;
             while (i >= off[b + 1]) b++;
             pid = Q.buckets[(size_t)b * Q.cap + (i - off[b])];
             WfPath& w = Q.paths[pid];
-            const DScene S = BATCH ? view_scene(B, w.pad0) : S0;
+            view = w.pad0;
+            const DScene S = BATCH ? view_scene(B, view) : S0;
             bool photons = BATCH ? B.views[w.pad0].P.use_photons != 0 : P0.use_photons != 0;
             if (photons && knn_gathers(S, w)) {
                 V3 e;
@@ -55,7 +57,7 @@ __global__ __launch_bounds__(BLOCK) void k_wf_knn(DScene S0, DParams P0, WfQueue
             Q.dense[slot] = pid;
             float4* stash = reinterpret_cast<float4*>(Q.shadow);
             stash[2 * (size_t)slot] = make_float4(pos.x, pos.y, pos.z, __uint_as_float(pid));
-            stash[2 * (size_t)slot + 1] = make_float4(nrm.x, nrm.y, nrm.z, 0.0f);
+            stash[2 * (size_t)slot + 1] = make_float4(nrm.x, nrm.y, nrm.z, __uint_as_float(BATCH ? view : 0u));
         }
     }
 }
@@ -184,7 +186,7 @@ __global__ __launch_bounds__(BLOCK) void k_wf_knn_dense(DScene S0, WfQueues Q, W
     uint32_t stride = gridDim.x * BLOCK;
     for (uint32_t i = blockIdx.x * BLOCK + threadIdx.x; i < total; i += stride) {
         float4 a = stash[2 * (size_t)i], n = stash[2 * (size_t)i + 1];
-        const DScene S = BATCH ? view_scene(B, Q.paths[__float_as_uint(a.w)].pad0) : S0;
+        const DScene S = BATCH ? view_scene(B, __float_as_uint(n.w)) : S0;   // the view, until the radius replaces it
         n.w = photon_knn_radius(S, mk(a.x, a.y, a.z), mk(n.x, n.y, n.z), heap, BLOCK);
         stash[2 * (size_t)i + 1] = n;
     }
